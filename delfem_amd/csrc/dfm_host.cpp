@@ -1,0 +1,243 @@
+// dfm_host.cpp -- integer setup work done once per pattern on the host:
+// FE adjacency pattern, ILU(k) symbolic factorisation, greedy colouring,
+// level sets of the factor DAG and the sliced-ELL layout plans.
+#include <algorithm>
+#include <cstring>
+
+#include "dfm_internal.h"
+
+namespace dfm {
+
+// CElemAry::MakePattern_FEM (src/femfield/elem_ary.cpp:225-245, 675-892):
+// neighbours of each node through shared elements, self excluded, ascending.
+long fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned* conn,
+                 std::vector<unsigned>& rowptr, std::vector<unsigned>& col) {
+  std::vector<unsigned> eptr(nnode + 1, 0), elist((size_t)nelem * npe);
+  for (size_t t = 0; t < (size_t)nelem * npe; t++) eptr[conn[t] + 1]++;
+  for (unsigned i = 0; i < nnode; i++) eptr[i + 1] += eptr[i];
+  {
+    std::vector<unsigned> fill(eptr.begin(), eptr.end() - 1);
+    for (unsigned e = 0; e < nelem; e++)
+      for (unsigned k = 0; k < npe; k++) elist[fill[conn[(size_t)e * npe + k]]++] = e;
+  }
+  std::vector<int> mark(nnode, -1);
+  rowptr.assign(nnode + 1, 0);
+  col.clear();
+  col.reserve((size_t)nnode * 8);
+  for (unsigned i = 0; i < nnode; i++) {
+    const size_t r0 = col.size();
+    mark[i] = (int)i;
+    for (unsigned q = eptr[i]; q < eptr[i + 1]; q++) {
+      const unsigned* no = conn + (size_t)elist[q] * npe;
+      for (unsigned k = 0; k < npe; k++) {
+        const unsigned j = no[k];
+        if (mark[j] != (int)i) {
+          mark[j] = (int)i;
+          col.push_back(j);
+        }
+      }
+    }
+    std::sort(col.begin() + r0, col.end());
+    rowptr[i + 1] = (unsigned)col.size();
+  }
+  return (long)col.size();
+}
+
+// ILU(k) symbolic with DelFEM's level rule (src/matvec/matdiafrac_blkcrs.cpp:1249-1377):
+// new fill gets max(lev_ik, lev_kj)+1, an entry keeps the first level it was
+// given, k is taken from the growing sorted row in ascending order.
+int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
+                  int lev_fill, const std::vector<int>* order, std::vector<unsigned>& frow,
+                  std::vector<unsigned>& fcol, std::vector<int>& flev, std::vector<unsigned>& fdiaind) {
+  const unsigned ncrs0 = rowptr[nb];
+  std::vector<unsigned> prow(nb + 1), pcol(ncrs0);
+  if (order) {  // src/matvec/mat_blkcrs.cpp:493-553
+    std::vector<int> inv(nb);
+    for (unsigned i = 0; i < nb; i++) inv[(*order)[i]] = (int)i;
+    prow[0] = 0;
+    for (unsigned inew = 0; inew < nb; inew++) {
+      const unsigned iold = (unsigned)(*order)[inew];
+      const unsigned n0 = prow[inew], cnt = rowptr[iold + 1] - rowptr[iold];
+      for (unsigned t = 0; t < cnt; t++) pcol[n0 + t] = (unsigned)inv[col[rowptr[iold] + t]];
+      std::sort(pcol.begin() + n0, pcol.begin() + n0 + cnt);
+      prow[inew + 1] = n0 + cnt;
+    }
+  } else {
+    prow = rowptr;
+    pcol = col;
+    for (unsigned i = 0; i < nb; i++)
+      for (unsigned q = prow[i] + 1; q < prow[i + 1]; q++)
+        if (pcol[q] <= pcol[q - 1]) {
+          set_error("ILU pattern row %u is not strictly ascending", i);
+          return DFM_UNSORTED;
+        }
+  }
+  const bool unchanged = ncrs0 == 0 || (unsigned long long)ncrs0 == (unsigned long long)nb * (nb - 1) || lev_fill == 0;
+  fdiaind.assign(nb, 0);
+  if (unchanged) {
+    frow = prow;
+    fcol = pcol;
+    flev.assign(ncrs0, 0);
+    for (unsigned i = 0; i < nb; i++) {
+      fdiaind[i] = prow[i + 1];
+      for (unsigned q = prow[i]; q < prow[i + 1]; q++)
+        if (pcol[q] > i) {
+          fdiaind[i] = q;
+          break;
+        }
+    }
+    return DFM_OK;
+  }
+  struct Node {
+    unsigned c;
+    int lev, next;
+  };
+  std::vector<Node> list;
+  list.reserve(256);
+  frow.assign(nb + 1, 0);
+  fcol.clear();
+  flev.clear();
+  fcol.reserve((size_t)ncrs0 * 2);
+  flev.reserve((size_t)ncrs0 * 2);
+  for (unsigned i = 0; i < nb; i++) {
+    list.clear();
+    int head = -1;
+    for (unsigned q = prow[i]; q < prow[i + 1]; q++) list.push_back({pcol[q], 0, (int)list.size() + 1});
+    if (!list.empty()) {
+      list.back().next = -1;
+      head = 0;
+    }
+    for (int kc = head; kc != -1; kc = list[kc].next) {
+      const unsigned k = list[kc].c;
+      const int lik = list[kc].lev;
+      if (lev_fill != -1 && lik + 1 > lev_fill) continue;
+      if (k >= i) break;
+      int cur = kc;
+      for (unsigned q = fdiaind[k]; q < frow[k + 1]; q++) {
+        const int lkj = flev[q];
+        if (lev_fill != -1 && lkj + 1 > lev_fill) continue;
+        const unsigned j = fcol[q];
+        if (j == i) continue;
+        for (;;) {
+          const int nx = list[cur].next;
+          if (nx == -1 || list[nx].c >= j) break;
+          cur = nx;
+        }
+        const int nx = list[cur].next;
+        if (nx != -1 && list[nx].c == j) continue;
+        list.push_back({j, std::max(lik, lkj) + 1, nx});
+        list[cur].next = (int)list.size() - 1;
+        cur = (int)list.size() - 1;
+      }
+    }
+    const unsigned n0 = (unsigned)fcol.size();
+    fdiaind[i] = n0 + (unsigned)list.size();
+    for (int t = head; t != -1; t = list[t].next) {
+      if (list[t].c > i && fdiaind[i] == n0 + (unsigned)list.size()) fdiaind[i] = (unsigned)fcol.size();
+      fcol.push_back(list[t].c);
+      flev.push_back(list[t].lev);
+    }
+    frow[i + 1] = (unsigned)fcol.size();
+  }
+  return DFM_OK;
+}
+
+// Greedy colouring in natural order: each node takes the smallest colour not
+// used by an already coloured neighbour.
+int greedy_colour(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
+                  std::vector<int>& colour) {
+  colour.assign(nb, -1);
+  std::vector<int> stamp(nb + 2, -1);
+  int ncol = 0;
+  for (unsigned i = 0; i < nb; i++) {
+    for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+      const int c = colour[col[q]];
+      if (c >= 0) stamp[c] = (int)i;
+    }
+    int c = 0;
+    while (stamp[c] == (int)i) c++;
+    colour[i] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  return ncol;
+}
+
+void colour_order(unsigned nb, const std::vector<int>& colour, int ncolour, std::vector<int>& order) {
+  std::vector<unsigned> cnt(ncolour + 1, 0);
+  for (unsigned i = 0; i < nb; i++) cnt[colour[i] + 1]++;
+  for (int c = 0; c < ncolour; c++) cnt[c + 1] += cnt[c];
+  order.assign(nb, 0);
+  for (unsigned i = 0; i < nb; i++) order[cnt[colour[i]]++] = (int)i;
+}
+
+// Level sets of the factor DAG: lower -> longest chain through L(i),
+// upper -> longest chain through U(i) taken from the last row.
+int level_sets(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
+               const std::vector<unsigned>& fdiaind, bool lower, std::vector<int>& level) {
+  level.assign(nb, 0);
+  int nlev = 0;
+  if (lower) {
+    for (unsigned i = 0; i < nb; i++) {
+      int l = 0;
+      for (unsigned q = frow[i]; q < fdiaind[i]; q++) l = std::max(l, level[fcol[q]] + 1);
+      level[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  } else {
+    for (long ii = (long)nb - 1; ii >= 0; ii--) {
+      int l = 0;
+      for (unsigned q = fdiaind[ii]; q < frow[ii + 1]; q++) l = std::max(l, level[fcol[q]] + 1);
+      level[ii] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  }
+  return nlev;
+}
+
+void build_sell_plan(const std::vector<int>& rows_by_group, const std::vector<int>& group_ptr,
+                     const std::vector<unsigned>& beg, const std::vector<unsigned>& end,
+                     const std::vector<unsigned>& colv, const std::vector<long long>* srcv,
+                     SellPlan& plan) {
+  const int ngroup = (int)group_ptr.size() - 1;
+  plan.group_slice_ptr.assign(ngroup + 1, 0);
+  plan.slice_ptr.assign(1, 0);
+  plan.rows.clear();
+  int nslice = 0;
+  long nk = 0;
+  for (int g = 0; g < ngroup; g++) {
+    for (int r0 = group_ptr[g]; r0 < group_ptr[g + 1]; r0 += kSlice) {
+      int w = 0;
+      for (int l = 0; l < kSlice; l++) {
+        const int t = r0 + l;
+        if (t < group_ptr[g + 1]) {
+          const int row = rows_by_group[t];
+          plan.rows.push_back(row);
+          w = std::max(w, (int)(end[row] - beg[row]));
+        } else {
+          plan.rows.push_back(-1);
+        }
+      }
+      nk += w;
+      plan.slice_ptr.push_back((int)nk);
+      nslice++;
+    }
+    plan.group_slice_ptr[g + 1] = nslice;
+  }
+  plan.nslice = nslice;
+  plan.nk = nk;
+  plan.col.assign((size_t)nk * kSlice, -1);
+  plan.src.assign((size_t)nk * kSlice, -1);
+  for (int s = 0; s < nslice; s++) {
+    for (int l = 0; l < kSlice; l++) {
+      const int row = plan.rows[(size_t)s * kSlice + l];
+      if (row < 0) continue;
+      long k = plan.slice_ptr[s];
+      for (unsigned q = beg[row]; q < end[row]; q++, k++) {
+        plan.col[(size_t)k * kSlice + l] = (int)colv[q];
+        plan.src[(size_t)k * kSlice + l] = srcv ? (*srcv)[q] : (long long)q;
+      }
+    }
+  }
+}
+
+}  // namespace dfm

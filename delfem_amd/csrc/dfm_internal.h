@@ -1,0 +1,188 @@
+// dfm_internal.h -- internal structures of the delfem_amd HIP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "dfm.h"
+
+namespace dfm {
+
+void set_error(const char* fmt, ...);
+hipStream_t stream();
+int ensure_init();
+
+#define DFM_HIP(call)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      ::dfm::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return DFM_HIP_ERROR;                                                              \
+    }                                                                                    \
+  } while (0)
+
+#define DFM_CHECK(cond, code, ...)   \
+  do {                               \
+    if (!(cond)) {                   \
+      ::dfm::set_error(__VA_ARGS__); \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+#define DFM_TRY(expr)        \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != DFM_OK) return rc_; \
+  } while (0)
+
+// Owning device buffer (hipMalloc on the library's device).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count) {
+    if (count == n && p) return DFM_OK;
+    release();
+    if (count == 0) return DFM_OK;
+    DFM_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+    return DFM_OK;
+  }
+  int upload(const T* h, size_t count) {
+    DFM_TRY(alloc(count));
+    if (count) DFM_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+    return DFM_OK;
+  }
+  int upload(const std::vector<T>& v) { return upload(v.data(), v.size()); }
+  int download(T* h, size_t count) const {
+    if (count) DFM_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+    DFM_HIP(hipStreamSynchronize(stream()));
+    return DFM_OK;
+  }
+};
+
+constexpr int kSlice = 64;  // rows per SELL slice = one wavefront
+
+// ---------------------------------------------------------------------------
+// Host-side algorithms (dfm_host.cpp)
+long fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned* conn,
+                 std::vector<unsigned>& rowptr, std::vector<unsigned>& col);
+// returns DFM_OK or DFM_UNSORTED
+int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
+                  int lev_fill, const std::vector<int>* order, std::vector<unsigned>& frow,
+                  std::vector<unsigned>& fcol, std::vector<int>& flev, std::vector<unsigned>& fdiaind);
+int greedy_colour(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
+                  std::vector<int>& colour);
+void colour_order(unsigned nb, const std::vector<int>& colour, int ncolour, std::vector<int>& order);
+int level_sets(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
+               const std::vector<unsigned>& fdiaind, bool lower, std::vector<int>& level);
+
+// Sliced-ELL layout plan: rows grouped into "groups" (level sets, or one group
+// for a plain SpMV), each group cut into slices of kSlice rows.  For slice s
+// and block slot k (slice_ptr[s] <= k < slice_ptr[s+1]) lane l holds the
+// block whose source index is src[k*kSlice+l] (-1: padding).
+struct SellPlan {
+  int nslice = 0;
+  long nk = 0;                      // total block slots per lane
+  std::vector<int> slice_ptr;       // nslice+1
+  std::vector<int> rows;            // nslice*kSlice, -1 padding
+  std::vector<int> col;             // nk*kSlice, -1 padding
+  std::vector<long long> src;       // nk*kSlice, -1 padding
+  std::vector<int> group_slice_ptr; // ngroup+1
+};
+// rows_by_group: rows listed group after group; group_ptr: ngroup+1 offsets.
+// For each row, entries [beg(row), end(row)) of (colv, srcv) are laid out in order.
+void build_sell_plan(const std::vector<int>& rows_by_group, const std::vector<int>& group_ptr,
+                     const std::vector<unsigned>& beg, const std::vector<unsigned>& end,
+                     const std::vector<unsigned>& colv, const std::vector<long long>* srcv,
+                     SellPlan& plan);
+
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Sell {
+  int nslice = 0;
+  long nk = 0;
+  int bb = 0;
+  DevBuf<int> slice_ptr, rows, col;
+  DevBuf<T> val, dval;  // val: nk*bb*kSlice, dval: nslice*bb*kSlice
+  std::vector<int> group_slice_ptr;
+};
+
+// Deterministic reduction state shared by the Krylov kernels.
+struct KState {
+  double rz, pAp, rr, inv_rr0, tol2;
+  double alpha, beta, omega, r_r2, rz_new;
+  double d0, d1, d2;    // generic reduction outputs
+  double conv_rr;       // rr at convergence
+  int done, iter, conv_iter, pad;
+  unsigned tickets[8];
+};
+
+}  // namespace dfm
+
+struct dfm_bcrs_s {
+  unsigned nb = 0, b = 0;
+  std::vector<unsigned> rowptr, col;   // host pattern (CRS without diagonal)
+  dfm::DevBuf<int> d_rowptr, d_col;
+  dfm::DevBuf<double> d_val, d_dia;
+  unsigned long long version = 1;       // bumped whenever values change
+};
+
+struct dfm_asm_s {
+  dfm_bcrs_s* A = nullptr;
+  unsigned nnode = 0, nelem = 0, npe = 0;
+  dfm::DevBuf<unsigned> d_conn;
+  dfm::DevBuf<int> d_elsup_ptr, d_elsup;  // node -> elements (ascending)
+  dfm::DevBuf<int> d_slot;                 // per (node, element-slot, local j): CRS index, -1 diag, -2 dropped
+};
+
+struct dfm_ilu_s {
+  unsigned nb = 0, b = 0;
+  int lev_fill = 0, ordering = DFM_ORDER_NATURAL, ncolour = 0;
+  bool permuted = false;
+  std::vector<int> order, inv;          // order[new] = old
+  std::vector<unsigned> frow, fcol, fdiaind;
+  std::vector<int> flev;
+  int nlev_lower = 0, nlev_upper = 0;
+  // device factor in CRS (factor numbering)
+  dfm::DevBuf<int> d_frow, d_fcol, d_fdiaind, d_order, d_inv;
+  dfm::DevBuf<long long> d_fsrc;        // A block per factor entry, -1 for fill
+  dfm::DevBuf<double> d_fval, d_fdia;
+  dfm::DevBuf<int> d_rows_L;            // rows sorted by lower level
+  std::vector<int> lev_ptr_L;
+  dfm::DevBuf<int> d_nsing;
+  // triangular-solve layouts (value copies refreshed after each factorisation)
+  dfm::SellPlan planL, planU;
+  dfm::DevBuf<long long> d_srcL, d_srcU;
+  dfm::Sell<double> L64, U64;
+  dfm::Sell<float> L32, U32;
+  bool have32 = false;
+  // permuted copy of A for the Krylov SpMV
+  dfm::SellPlan planA;
+  dfm::DevBuf<long long> d_srcA;
+  dfm::Sell<double> A64;
+  dfm::Sell<float> A32;
+  unsigned long long a_version = 0;     // version of A mirrored in A64 / A32
+  bool a32_valid = false;
+  const dfm_bcrs_s* A_pattern_owner = nullptr;
+  bool factored = false;
+  // Krylov workspace (factor numbering)
+  dfm::DevBuf<double> w64;              // several vectors of n doubles
+  dfm::DevBuf<float> w32;
+  dfm::DevBuf<double> partials;
+  dfm::DevBuf<dfm::KState> state;
+  dfm::KState* h_state = nullptr;       // pinned mirror
+};

@@ -1,0 +1,912 @@
+// dfm_kernels.h -- gfx950 device kernels of delfem_amd.
+//
+// All floating-point expressions are written in the evaluation order of the
+// DelFEM reference routine they replace and the library is compiled with
+// -ffp-contract=off, so the FP64 kernels reproduce the reference bit for bit
+// when they see the same operands in the same order (natural ordering).
+// The kernels are HBM-bandwidth bound (0.1-0.25 flop/byte): the layouts keep
+// every wavefront's loads contiguous (sliced ELL with values stored
+// [slot][entry][lane]) and reductions are deterministic (fixed tree, last
+// workgroup sums the per-workgroup partials in index order).
+#pragma once
+
+#include "dfm_internal.h"
+
+namespace dfm {
+
+constexpr int kNT = 256;             // threads per workgroup
+constexpr int kWaves = kNT / 64;     // wavefronts (= slices) per workgroup
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Sum of v over the workgroup, valid in thread 0.
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kWaves; i++) s += sh[i];
+  __syncthreads();
+  return s;
+}
+
+__device__ inline void store_agent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double load_agent(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Each workgroup contributes ND partial sums; the workgroup that finishes last
+// (ticket == nparts-1) adds all partials in index order and returns true in
+// every thread with out[] valid in thread 0.  partials: ND*nparts doubles.
+template <int ND>
+__device__ inline bool finish_sums(double (&v)[ND], double* partials, int part, int nparts,
+                                   unsigned* ticket, double (&out)[ND]) {
+  __shared__ double sh[kWaves];
+  __shared__ int amlast;
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    const double s = block_sum(v[d], sh);
+    if (threadIdx.x == 0) store_agent(&partials[(size_t)d * nparts + part], s);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    amlast = (t == (unsigned)nparts - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!amlast) return false;
+  __threadfence();
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kNT) acc += load_agent(&partials[(size_t)d * nparts + i]);
+    out[d] = block_sum(acc, sh);
+  }
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Block row kernels in the reference's summation order.
+// acc[r] += sum_c A[r*B+c]*x[c], the B-term sum formed first (matdia_blkcrs.cpp:1027-1058)
+template <int B, typename T>
+__device__ inline void blk_madd(T* acc, const T* a, const T* x) {
+  if constexpr (B == 1) {
+    acc[0] += a[0] * x[0];
+  } else if constexpr (B == 2) {
+    acc[0] += a[0] * x[0] + a[1] * x[1];
+    acc[1] += a[2] * x[0] + a[3] * x[1];
+  } else {
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+      T s = a[r * B] * x[0];
+#pragma unroll
+      for (int c = 1; c < B; c++) s = s + a[r * B + c] * x[c];
+      acc[r] += s;
+    }
+  }
+}
+// acc[r] -= sum_c A[r*B+c]*x[c] (matdiafrac_blkcrs.cpp:447-449, 613-615)
+template <int B, typename T>
+__device__ inline void blk_msub(T* acc, const T* a, const T* x) {
+  if constexpr (B == 1) {
+    acc[0] -= a[0] * x[0];
+  } else {
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+      T s = a[r * B] * x[0];
+#pragma unroll
+      for (int c = 1; c < B; c++) s = s + a[r * B + c] * x[c];
+      acc[r] -= s;
+    }
+  }
+}
+// y[r] = sum_c D[r*B+c]*t[c] (matdiafrac_blkcrs.cpp:451-456)
+template <int B, typename T>
+__device__ inline void blk_mul(T* y, const T* d, const T* t) {
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    T s = d[r * B] * t[0];
+#pragma unroll
+    for (int c = 1; c < B; c++) s = s + d[r * B + c] * t[c];
+    y[r] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CRS MatVec on the canonical storage: y = beta*y + alpha*A*x
+// (CMatDia_BlkCrs::MatVec, src/matvec/matdia_blkcrs.cpp:921-1094).
+template <int B>
+static __global__ void __launch_bounds__(kNT) k_crs_matvec(int nb, const int* __restrict__ rowptr,
+                                                   const int* __restrict__ col,
+                                                   const double* __restrict__ val,
+                                                   const double* __restrict__ dia, double alpha,
+                                                   const double* __restrict__ x, double beta,
+                                                   double* __restrict__ y) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i >= nb) return;
+  constexpr int BB = B * B;
+  double acc[B];
+#pragma unroll
+  for (int r = 0; r < B; r++) acc[r] = y[(size_t)i * B + r] * beta;
+  const int q0 = rowptr[i], q1 = rowptr[i + 1];
+  for (int q = q0; q <= q1; q++) {
+    const double* A = q < q1 ? val + (size_t)q * BB : dia + (size_t)i * BB;
+    const double* xj = x + (size_t)(q < q1 ? col[q] : i) * B;
+    if constexpr (B == 1) {
+      acc[0] += alpha * A[0] * xj[0];
+    } else {
+#pragma unroll
+      for (int r = 0; r < B; r++) {
+        double s = A[r * B] * xj[0];
+#pragma unroll
+        for (int c = 1; c < B; c++) s = s + A[r * B + c] * xj[c];
+        acc[r] += alpha * s;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < B; r++) y[(size_t)i * B + r] = acc[r];
+}
+
+// Generic block size (b > 3): each product added separately (:1072-1085).
+static __global__ void __launch_bounds__(kNT) k_crs_matvec_gen(int nb, int b, const int* __restrict__ rowptr,
+                                                       const int* __restrict__ col,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ dia, double alpha,
+                                                       const double* __restrict__ x, double beta,
+                                                       double* __restrict__ y) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i >= nb) return;
+  const int bb = b * b;
+  for (int r = 0; r < b; r++) y[(size_t)i * b + r] *= beta;
+  const int q0 = rowptr[i], q1 = rowptr[i + 1];
+  for (int q = q0; q <= q1; q++) {
+    const double* A = q < q1 ? val + (size_t)q * bb : dia + (size_t)i * bb;
+    const double* xj = x + (size_t)(q < q1 ? col[q] : i) * b;
+    for (int r = 0; r < b; r++) {
+      double a = y[(size_t)i * b + r];
+      for (int c = 0; c < b; c++) a += alpha * A[r * b + c] * xj[c];
+      y[(size_t)i * b + r] = a;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dirichlet conditions (CMatDia_BlkCrs::SetBoundaryCondition, matdia_blkcrs.cpp:318-342)
+static __global__ void k_bc_rows(int nb, int b, const int* __restrict__ rowptr, double* __restrict__ val,
+                          double* __restrict__ dia, const int* __restrict__ flag) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i >= nb) return;
+  const int bb = b * b;
+  for (int d = 0; d < b; d++) {
+    if (flag[(size_t)i * b + d] == 0) continue;
+    double* D = dia + (size_t)i * bb;
+    for (int k = 0; k < b; k++) D[k * b + d] = 0.0;
+    for (int k = 0; k < b; k++) D[d * b + k] = 0.0;
+    D[d * b + d] = 1.0;
+    for (int q = rowptr[i]; q < rowptr[i + 1]; q++)
+      for (int k = 0; k < b; k++) val[(size_t)q * bb + d * b + k] = 0.0;
+  }
+}
+static __global__ void k_bc_cols(long ncrs, int b, const int* __restrict__ col, double* __restrict__ val,
+                          const int* __restrict__ flag) {
+  const long q = (long)blockIdx.x * kNT + threadIdx.x;
+  if (q >= ncrs) return;
+  const int j = col[q], bb = b * b;
+  for (int d = 0; d < b; d++) {
+    if (flag[(size_t)j * b + d] == 0) continue;
+    for (int k = 0; k < b; k++) val[(size_t)q * bb + k * b + d] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Element assembly, gathered by matrix row.  Node i visits its elements in
+// ascending element order and adds its row of each element matrix, so every
+// entry receives its element contributions in the order sequential Mearge
+// calls would add them (matdia_blkcrs.cpp:226-256); the residual likewise
+// (vector_blk.h:170-181).  No atomics, no colouring.
+__device__ inline int find_col(const int* __restrict__ col, int q0, int q1, int j) {
+  for (int q = q0; q < q1; q++)
+    if (col[q] == j) return q;
+  return -1;
+}
+
+// src/femeqn/ker_emat_tri.cpp:12-14, 36-69
+__device__ inline double d_tri_area(const double* p0, const double* p1, const double* p2) {
+  return 0.5 * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+__device__ inline void d_tri_dldx(double dl[3][2], const double* p0, const double* p1, const double* p2) {
+  const double area = d_tri_area(p0, p1, p2);
+  const double t = 0.5 / area;
+  dl[0][0] = t * (p1[1] - p2[1]);
+  dl[1][0] = t * (p2[1] - p0[1]);
+  dl[2][0] = t * (p0[1] - p1[1]);
+  dl[0][1] = t * (p2[0] - p1[0]);
+  dl[1][1] = t * (p0[0] - p2[0]);
+  dl[2][1] = t * (p1[0] - p0[0]);
+}
+// include/delfem/femeqn/ker_emat_tet.h:32-77
+__device__ inline double d_tet_volume(const double* p0, const double* p1, const double* p2, const double* p3) {
+  double v = (p1[0] - p0[0]) * ((p2[1] - p0[1]) * (p3[2] - p0[2]) - (p3[1] - p0[1]) * (p2[2] - p0[2])) -
+             (p1[1] - p0[1]) * ((p2[0] - p0[0]) * (p3[2] - p0[2]) - (p3[0] - p0[0]) * (p2[2] - p0[2])) +
+             (p1[2] - p0[2]) * ((p2[0] - p0[0]) * (p3[1] - p0[1]) - (p3[0] - p0[0]) * (p2[1] - p0[1]));
+  v *= 0.1666666666666;
+  return v;
+}
+__device__ inline void d_tet_dldx(double d[4][3], const double* p0, const double* p1, const double* p2,
+                                  const double* p3) {
+  const double vol = d_tet_volume(p0, p1, p2, p3);
+  const double t = 1.0 / (vol * 6.0);
+  d[0][0] = -t * ((p2[1] - p1[1]) * (p3[2] - p1[2]) - (p3[1] - p1[1]) * (p2[2] - p1[2]));
+  d[0][1] = +t * ((p2[0] - p1[0]) * (p3[2] - p1[2]) - (p3[0] - p1[0]) * (p2[2] - p1[2]));
+  d[0][2] = -t * ((p2[0] - p1[0]) * (p3[1] - p1[1]) - (p3[0] - p1[0]) * (p2[1] - p1[1]));
+  d[1][0] = +t * ((p3[1] - p2[1]) * (p0[2] - p2[2]) - (p0[1] - p2[1]) * (p3[2] - p2[2]));
+  d[1][1] = -t * ((p3[0] - p2[0]) * (p0[2] - p2[2]) - (p0[0] - p2[0]) * (p3[2] - p2[2]));
+  d[1][2] = +t * ((p3[0] - p2[0]) * (p0[1] - p2[1]) - (p0[0] - p2[0]) * (p3[1] - p2[1]));
+  d[2][0] = -t * ((p0[1] - p3[1]) * (p1[2] - p3[2]) - (p1[1] - p3[1]) * (p0[2] - p3[2]));
+  d[2][1] = +t * ((p0[0] - p3[0]) * (p1[2] - p3[2]) - (p1[0] - p3[0]) * (p0[2] - p3[2]));
+  d[2][2] = -t * ((p0[0] - p3[0]) * (p1[1] - p3[1]) - (p1[0] - p3[0]) * (p0[1] - p3[1]));
+  d[3][0] = +t * ((p1[1] - p0[1]) * (p2[2] - p0[2]) - (p2[1] - p0[1]) * (p1[2] - p0[2]));
+  d[3][1] = -t * ((p1[0] - p0[0]) * (p2[2] - p0[2]) - (p2[0] - p0[0]) * (p1[2] - p0[2]));
+  d[3][2] = +t * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+
+// AddLinearSystem_Poisson2D_P1 (src/femeqn/eqn_poisson.cpp:49-125), row of node i.
+static __global__ void __launch_bounds__(kNT) k_asm_poisson2d(int nnode, const unsigned* __restrict__ conn,
+                                                      const int* __restrict__ eptr, const int* __restrict__ elist,
+                                                      const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                      double* __restrict__ val, double* __restrict__ dia,
+                                                      const double* __restrict__ coord,
+                                                      const double* __restrict__ alpha, double source,
+                                                      const double* __restrict__ value, double* __restrict__ res) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i >= nnode) return;
+  const int q0 = rowptr[i], q1 = rowptr[i + 1];
+  double racc = res ? res[i] : 0.0;
+  double dacc = dia[i];
+  for (int t = eptr[i]; t < eptr[i + 1]; t++) {
+    const int e = elist[t];
+    unsigned no[3];
+    double p[3][2];
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      no[k] = conn[(size_t)e * 3 + k];
+      p[k][0] = coord[(size_t)no[k] * 2];
+      p[k][1] = coord[(size_t)no[k] * 2 + 1];
+      if ((int)no[k] == i) a = k;
+    }
+    const double area = d_tri_area(p[0], p[1], p[2]);
+    double dl[3][2];
+    d_tri_dldx(dl, p[0], p[1], p[2]);
+    double em[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) em[j] = alpha[e] * area * (dl[a][0] * dl[j][0] + dl[a][1] * dl[j][1]);
+    double er = source * area * 0.33333333333333333;
+#pragma unroll
+    for (int j = 0; j < 3; j++) er -= em[j] * value[no[j]];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (j == a) {
+        dacc += em[j];
+      } else {
+        const int q = find_col(col, q0, q1, (int)no[j]);
+        if (q >= 0) val[q] += em[j];
+      }
+    }
+    racc += er;
+  }
+  dia[i] = dacc;
+  if (res) res[i] = racc;
+}
+
+// AddLinSys_LinearSolid3D_Static_P1 (src/femeqn/eqn_linear_solid3d.cpp:241-334), row of node i.
+static __global__ void __launch_bounds__(kNT) k_asm_solid3d(int nnode, const unsigned* __restrict__ conn,
+                                                    const int* __restrict__ eptr, const int* __restrict__ elist,
+                                                    const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                    double* __restrict__ val, double* __restrict__ dia,
+                                                    const double* __restrict__ coord,
+                                                    const double* __restrict__ lambda,
+                                                    const double* __restrict__ myu, double rho, double gx,
+                                                    double gy, double gz, const double* __restrict__ disp,
+                                                    double* __restrict__ res) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i >= nnode) return;
+  const int q0 = rowptr[i], q1 = rowptr[i + 1];
+  double racc[3] = {0.0, 0.0, 0.0};
+  if (res)
+    for (int x = 0; x < 3; x++) racc[x] = res[(size_t)i * 3 + x];
+  for (int t = eptr[i]; t < eptr[i + 1]; t++) {
+    const int e = elist[t];
+    unsigned no[4];
+    double p[4][3];
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      no[k] = conn[(size_t)e * 4 + k];
+#pragma unroll
+      for (int d = 0; d < 3; d++) p[k][d] = coord[(size_t)no[k] * 3 + d];
+      if ((int)no[k] == i) a = k;
+    }
+    const double vol = d_tet_volume(p[0], p[1], p[2], p[3]);
+    double dl[4][3];
+    d_tet_dldx(dl, p[0], p[1], p[2], p[3]);
+    const double lam = lambda[e], mu = myu[e];
+    double er[3] = {vol * rho * gx * 0.25, vol * rho * gy * 0.25, vol * rho * gz * 0.25};
+    for (int c = 0; c < 4; c++) {
+      double E[9];
+#pragma unroll
+      for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 3; y++) E[x * 3 + y] = vol * (lam * dl[a][x] * dl[c][y] + mu * dl[c][x] * dl[a][y]);
+      const double dd = dl[a][0] * dl[c][0] + dl[a][1] * dl[c][1] + dl[a][2] * dl[c][2];
+      E[0] += vol * mu * dd;
+      E[4] += vol * mu * dd;
+      E[8] += vol * mu * dd;
+      const double* u = disp + (size_t)no[c] * 3;
+      er[0] -= E[0] * u[0] + E[1] * u[1] + E[2] * u[2];
+      er[1] -= E[3] * u[0] + E[4] * u[1] + E[5] * u[2];
+      er[2] -= E[6] * u[0] + E[7] * u[1] + E[8] * u[2];
+      double* out;
+      if (c == a) {
+        out = dia + (size_t)i * 9;
+      } else {
+        const int q = find_col(col, q0, q1, (int)no[c]);
+        if (q < 0) continue;
+        out = val + (size_t)q * 9;
+      }
+#pragma unroll
+      for (int s = 0; s < 9; s++) out[s] += E[s];
+    }
+#pragma unroll
+    for (int x = 0; x < 3; x++) racc[x] += er[x];
+  }
+  if (res)
+    for (int x = 0; x < 3; x++) res[(size_t)i * 3 + x] = racc[x];
+}
+
+// ---------------------------------------------------------------------------
+// ILU numeric factorisation (CMatDiaFrac_BlkCrs::SetValue_Initialize + DoILUDecomp,
+// include/delfem/matvec/matdiafrac_blkcrs.h:90-117, src/matvec/matdiafrac_blkcrs.cpp:769-1102).
+static __global__ void k_ilu_copy(long nf, int bb, const long long* __restrict__ fsrc, const double* __restrict__ aval,
+                           double* __restrict__ fval) {
+  const long t = (long)blockIdx.x * kNT + threadIdx.x;
+  if (t >= nf * bb) return;
+  const long q = t / bb;
+  const int e = (int)(t - q * bb);
+  const long long s = fsrc[q];
+  fval[t] = s >= 0 ? aval[(size_t)s * bb + e] : 0.0;
+}
+static __global__ void k_ilu_copy_dia(int nb, int bb, const int* __restrict__ order, const double* __restrict__ adia,
+                               double* __restrict__ fdia) {
+  const long t = (long)blockIdx.x * kNT + threadIdx.x;
+  if (t >= (long)nb * bb) return;
+  const long i = t / bb;
+  const int e = (int)(t - i * bb);
+  const long io = order ? order[i] : i;
+  fdia[t] = adia[(size_t)io * bb + e];
+}
+
+// One thread per row of a level of the lower DAG: IKJ update with the
+// finished rows k in L(i), then invert the pivot block and scale the U row.
+template <int B>
+static __global__ void __launch_bounds__(kNT) k_ilu_factor(int nrow, const int* __restrict__ rows,
+                                                   const int* __restrict__ frow, const int* __restrict__ fcol,
+                                                   const int* __restrict__ fdiaind, double* __restrict__ fval,
+                                                   double* __restrict__ fdia, int* __restrict__ nsing) {
+  const int t = blockIdx.x * kNT + threadIdx.x;
+  if (t >= nrow) return;
+  constexpr int BB = B * B;
+  const int i = rows[t];
+  const int r0 = frow[i], rd = fdiaind[i], r1 = frow[i + 1];
+  double* D = fdia + (size_t)i * BB;
+  for (int qk = r0; qk < rd; qk++) {
+    const int k = fcol[qk];
+    double L[BB];
+#pragma unroll
+    for (int s = 0; s < BB; s++) L[s] = fval[(size_t)qk * BB + s];
+    int p = qk + 1;
+    const int u0 = fdiaind[k], u1 = frow[k + 1];
+    for (int qj = u0; qj < u1; qj++) {
+      const int j = fcol[qj];
+      double* Aij;
+      if (j == i) {
+        Aij = D;
+      } else {
+        while (p < r1 && fcol[p] < j) p++;
+        if (p >= r1 || fcol[p] != j) continue;
+        Aij = fval + (size_t)p * BB;
+      }
+      const double* U = fval + (size_t)qj * BB;
+      double u[BB];
+#pragma unroll
+      for (int s = 0; s < BB; s++) u[s] = U[s];
+      if constexpr (B == 1) {
+        Aij[0] -= L[0] * u[0];
+      } else {
+#pragma unroll
+        for (int r = 0; r < B; r++)
+#pragma unroll
+          for (int c = 0; c < B; c++) {
+            double s = L[r * B] * u[c];
+#pragma unroll
+            for (int m = 1; m < B; m++) s = s + L[r * B + m] * u[m * B + c];
+            Aij[r * B + c] -= s;
+          }
+      }
+    }
+  }
+  double d[BB];
+#pragma unroll
+  for (int s = 0; s < BB; s++) d[s] = D[s];
+  bool singular = false;
+  if constexpr (B == 1) {
+    if (fabs(d[0]) > 1.0e-30) d[0] = 1.0 / d[0];
+    else singular = true;
+  } else if constexpr (B == 2) {
+    const double det = d[0] * d[3] - d[1] * d[2];
+    if (fabs(det) > 1.0e-30) {
+      const double id = 1.0 / det;
+      const double d0 = d[0];
+      d[0] = id * d[3];
+      d[1] = -id * d[1];
+      d[2] = -id * d[2];
+      d[3] = id * d0;
+    } else {
+      singular = true;
+    }
+  } else {
+    const double det = d[0] * d[4] * d[8] + d[3] * d[7] * d[2] + d[6] * d[1] * d[5] - d[0] * d[7] * d[5] -
+                       d[6] * d[4] * d[2] - d[3] * d[1] * d[8];
+    if (fabs(det) > 1.0e-30) {
+      double s[9];
+#pragma unroll
+      for (int z = 0; z < 9; z++) s[z] = d[z];
+      const double id = 1.0 / det;
+      d[0] = id * (s[4] * s[8] - s[5] * s[7]);
+      d[1] = id * (s[2] * s[7] - s[1] * s[8]);
+      d[2] = id * (s[1] * s[5] - s[2] * s[4]);
+      d[3] = id * (s[5] * s[6] - s[3] * s[8]);
+      d[4] = id * (s[0] * s[8] - s[2] * s[6]);
+      d[5] = id * (s[2] * s[3] - s[0] * s[5]);
+      d[6] = id * (s[3] * s[7] - s[4] * s[6]);
+      d[7] = id * (s[1] * s[6] - s[0] * s[7]);
+      d[8] = id * (s[0] * s[4] - s[1] * s[3]);
+    } else {
+      singular = true;
+    }
+  }
+  if (singular) atomicAdd(nsing, 1);
+#pragma unroll
+  for (int s = 0; s < BB; s++) D[s] = d[s];
+  for (int q = rd; q < r1; q++) {
+    double* U = fval + (size_t)q * BB;
+    double u[BB];
+#pragma unroll
+    for (int s = 0; s < BB; s++) u[s] = U[s];
+    if constexpr (B == 1) {
+      U[0] = u[0] * d[0];
+    } else {
+#pragma unroll
+      for (int r = 0; r < B; r++)
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+          double s = d[r * B] * u[c];
+#pragma unroll
+          for (int m = 1; m < B; m++) s = s + d[r * B + m] * u[m * B + c];
+          U[r * B + c] = s;
+        }
+    }
+  }
+}
+
+// Generic block size (b > 3, b <= 8): CalcSubMatPr / CalcInvMat / CalcMatPr
+// (matdiafrac_blkcrs.cpp:45-136).
+static __global__ void __launch_bounds__(kNT) k_ilu_factor_gen(int nrow, int b, const int* __restrict__ rows,
+                                                       const int* __restrict__ frow, const int* __restrict__ fcol,
+                                                       const int* __restrict__ fdiaind, double* __restrict__ fval,
+                                                       double* __restrict__ fdia, int* __restrict__ nsing) {
+  const int t = blockIdx.x * kNT + threadIdx.x;
+  if (t >= nrow) return;
+  const int bb = b * b;
+  const int i = rows[t];
+  const int r0 = frow[i], rd = fdiaind[i], r1 = frow[i + 1];
+  double* D = fdia + (size_t)i * bb;
+  for (int qk = r0; qk < rd; qk++) {
+    const int k = fcol[qk];
+    const double* L = fval + (size_t)qk * bb;
+    int p = qk + 1;
+    for (int qj = fdiaind[k]; qj < frow[k + 1]; qj++) {
+      const int j = fcol[qj];
+      double* Aij;
+      if (j == i) {
+        Aij = D;
+      } else {
+        while (p < r1 && fcol[p] < j) p++;
+        if (p >= r1 || fcol[p] != j) continue;
+        Aij = fval + (size_t)p * bb;
+      }
+      const double* U = fval + (size_t)qj * bb;
+      for (int r = 0; r < b; r++)
+        for (int c = 0; c < b; c++)
+          for (int m = 0; m < b; m++) Aij[r * b + c] -= L[r * b + m] * U[m * b + c];
+    }
+  }
+  bool singular = false;
+  for (int r = 0; r < b && !singular; r++) {
+    if (fabs(D[r * b + r]) < 1.0e-30) {
+      singular = true;
+      break;
+    }
+    double tt = 1.0 / D[r * b + r];
+    D[r * b + r] = 1.0;
+    for (int k = 0; k < b; k++) D[r * b + k] *= tt;
+    for (int j = 0; j < b; j++) {
+      if (j == r) continue;
+      tt = D[j * b + r];
+      D[j * b + r] = 0.0;
+      for (int k = 0; k < b; k++) D[j * b + k] -= tt * D[r * b + k];
+    }
+  }
+  if (singular) atomicAdd(nsing, 1);
+  double tmp[64];
+  for (int q = rd; q < r1; q++) {
+    double* U = fval + (size_t)q * bb;
+    for (int r = 0; r < b; r++)
+      for (int c = 0; c < b; c++) {
+        tmp[r * b + c] = 0.0;
+        for (int m = 0; m < b; m++) tmp[r * b + c] += D[r * b + m] * U[m * b + c];
+      }
+    for (int s = 0; s < bb; s++) U[s] = tmp[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sliced-ELL value refresh: to[(k*bb+e)*64+lane] = from[src*bb+e] (0 for padding).
+template <typename T>
+static __global__ void k_sell_gather(long nslot, int bb, const long long* __restrict__ src, const double* __restrict__ from,
+                              T* __restrict__ to) {
+  const long slot = (long)blockIdx.x * kNT + threadIdx.x;
+  if (slot >= nslot) return;
+  const long k = slot >> 6;
+  const int lane = (int)(slot & 63);
+  const long long s = src[slot];
+  for (int e = 0; e < bb; e++)
+    to[((size_t)k * bb + e) * 64 + lane] = s >= 0 ? (T)from[(size_t)s * bb + e] : (T)0;
+}
+// Diagonal blocks per slice: to[(s*bb+e)*64+lane] = from[map(row)*bb+e].
+template <typename T>
+static __global__ void k_sell_gather_dia(long nslot, int bb, const int* __restrict__ rows, const int* __restrict__ order,
+                                  const double* __restrict__ from, T* __restrict__ to) {
+  const long slot = (long)blockIdx.x * kNT + threadIdx.x;
+  if (slot >= nslot) return;
+  const long s = slot >> 6;
+  const int lane = (int)(slot & 63);
+  const int row = rows[slot];
+  const long src = row < 0 ? -1 : (order ? order[row] : row);
+  for (int e = 0; e < bb; e++) to[((size_t)s * bb + e) * 64 + lane] = src >= 0 ? (T)from[(size_t)src * bb + e] : (T)0;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV on the sliced-ELL copy of A (rows = slice*64+lane, diagonal last):
+//   y = A x            (res == nullptr)
+//   y = res - A x      (true residual form)
+// Fused reductions (all optional): d0 = <y,w0>, d1 = <y,y>, d2 = <u2,v2>.
+// BiCGSTAB omega = <s,AMs>/<AMs,AMs> uses w0 = s.
+// Reductions accumulate in double.  MODE selects the Krylov epilogue.
+enum { EPI_NONE = 0, EPI_PCG_PAP = 1, EPI_BICG_ALPHA = 2, EPI_BICG_OMEGA = 3, EPI_STORE = 4 };
+
+template <int B, typename T, int MODE>
+static __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __restrict__ sptr,
+                                             const int* __restrict__ col, const T* __restrict__ val,
+                                             const T* __restrict__ dval, const T* __restrict__ x,
+                                             T* __restrict__ y, const T* __restrict__ res,
+                                             const T* __restrict__ w0, const T* __restrict__ u2,
+                                             const T* __restrict__ v2, double* __restrict__ partials,
+                                             KState* __restrict__ st) {
+  // Krylov-internal products stop once the solver has converged; the
+  // true-residual form (EPI_STORE) always runs.
+  if (MODE != EPI_STORE && st && st->done) return;
+  constexpr int BB = B * B;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * kWaves + wave;
+  double dsum[3] = {0.0, 0.0, 0.0};
+  if (s < nslice) {
+    const int row = s * 64 + lane;
+    if (row < nb) {
+      T acc[B];
+#pragma unroll
+      for (int r = 0; r < B; r++) acc[r] = (T)0;
+      const int k0 = sptr[s], k1 = sptr[s + 1];
+      for (int k = k0; k < k1; k++) {
+        const int c = col[(size_t)k * 64 + lane];
+        if (c < 0) break;
+        T a[BB], xv[B];
+#pragma unroll
+        for (int e = 0; e < BB; e++) a[e] = val[((size_t)k * BB + e) * 64 + lane];
+#pragma unroll
+        for (int e = 0; e < B; e++) xv[e] = x[(size_t)c * B + e];
+        blk_madd<B, T>(acc, a, xv);
+      }
+      {
+        T a[BB], xv[B];
+#pragma unroll
+        for (int e = 0; e < BB; e++) a[e] = dval[((size_t)s * BB + e) * 64 + lane];
+#pragma unroll
+        for (int e = 0; e < B; e++) xv[e] = x[(size_t)row * B + e];
+        blk_madd<B, T>(acc, a, xv);
+      }
+#pragma unroll
+      for (int r = 0; r < B; r++) {
+        const size_t o = (size_t)row * B + r;
+        const T yv = res ? (T)(res[o] - acc[r]) : acc[r];
+        y[o] = yv;
+        if (w0) dsum[0] += (double)yv * (double)w0[o];
+        dsum[1] += (double)yv * (double)yv;
+        if (u2) dsum[2] += (double)u2[o] * (double)v2[o];
+      }
+    }
+  }
+  if constexpr (MODE != EPI_NONE) {
+    double out[3];
+    if (finish_sums<3>(dsum, partials, blockIdx.x, gridDim.x, &st->tickets[0], out) && threadIdx.x == 0) {
+      if constexpr (MODE == EPI_PCG_PAP) {
+        st->pAp = out[0];
+        st->alpha = st->rz / st->pAp;
+      } else if constexpr (MODE == EPI_BICG_ALPHA) {
+        st->r_r2 = out[2];
+        st->alpha = st->r_r2 / out[0];
+      } else if constexpr (MODE == EPI_BICG_OMEGA) {
+        st->omega = out[0] / out[1];
+      } else {
+        st->d0 = out[0];
+        st->d1 = out[1];
+        st->d2 = out[2];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves over one level set (ForwardSubstitution / BackwardSubstitution,
+// src/matvec/matdiafrac_blkcrs.cpp:332-663).  Lower: out_i = Dinv_i (rhs_i - sum L_ij out_j);
+// rhs may alias out.  Upper: v_i -= sum U_ij v_j in place; optionally the rows
+// finished here contribute <r,v> to a reduction spanning all level launches.
+template <int B, typename T>
+static __global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* __restrict__ rows,
+                                                   const int* __restrict__ sptr, const int* __restrict__ col,
+                                                   const T* __restrict__ val, const T* __restrict__ dval,
+                                                   const T* rhs, T* out, const KState* __restrict__ st) {
+  if (st && st->done) return;
+  constexpr int BB = B * B;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = s0 + blockIdx.x * kWaves + wave;
+  if (s >= s1) return;
+  const int row = rows[(size_t)s * 64 + lane];
+  if (row < 0) return;
+  T t[B];
+#pragma unroll
+  for (int r = 0; r < B; r++) t[r] = rhs[(size_t)row * B + r];
+  const int k0 = sptr[s], k1 = sptr[s + 1];
+  for (int k = k0; k < k1; k++) {
+    const int c = col[(size_t)k * 64 + lane];
+    if (c < 0) break;
+    T a[BB], v[B];
+#pragma unroll
+    for (int e = 0; e < BB; e++) a[e] = val[((size_t)k * BB + e) * 64 + lane];
+#pragma unroll
+    for (int e = 0; e < B; e++) v[e] = out[(size_t)c * B + e];
+    blk_msub<B, T>(t, a, v);
+  }
+  T d[BB], y[B];
+#pragma unroll
+  for (int e = 0; e < BB; e++) d[e] = dval[((size_t)s * BB + e) * 64 + lane];
+  blk_mul<B, T>(y, d, t);
+#pragma unroll
+  for (int r = 0; r < B; r++) out[(size_t)row * B + r] = y[r];
+}
+
+template <int B, typename T, bool DOT>
+static __global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const int* __restrict__ rows,
+                                                   const int* __restrict__ sptr, const int* __restrict__ col,
+                                                   const T* __restrict__ val, T* vec, const T* __restrict__ rvec,
+                                                   double* __restrict__ partials, int part0, int nparts,
+                                                   KState* __restrict__ st, int epi) {
+  if (st && st->done) return;
+  constexpr int BB = B * B;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = s0 + blockIdx.x * kWaves + wave;
+  double dsum[1] = {0.0};
+  if (s < s1) {
+    const int row = rows[(size_t)s * 64 + lane];
+    if (row >= 0) {
+      T t[B];
+#pragma unroll
+      for (int r = 0; r < B; r++) t[r] = vec[(size_t)row * B + r];
+      const int k0 = sptr[s], k1 = sptr[s + 1];
+      for (int k = k0; k < k1; k++) {
+        const int c = col[(size_t)k * 64 + lane];
+        if (c < 0) break;
+        T a[BB], v[B];
+#pragma unroll
+        for (int e = 0; e < BB; e++) a[e] = val[((size_t)k * BB + e) * 64 + lane];
+#pragma unroll
+        for (int e = 0; e < B; e++) v[e] = vec[(size_t)c * B + e];
+        blk_msub<B, T>(t, a, v);
+      }
+#pragma unroll
+      for (int r = 0; r < B; r++) {
+        vec[(size_t)row * B + r] = t[r];
+        if (DOT) dsum[0] += (double)rvec[(size_t)row * B + r] * (double)t[r];
+      }
+    }
+  }
+  if constexpr (DOT) {
+    double out[1];
+    if (finish_sums<1>(dsum, partials, part0 + blockIdx.x, nparts, &st->tickets[1], out) && threadIdx.x == 0) {
+      if (epi == 0) {            // PCG initial: rz
+        st->rz = out[0];
+      } else {                   // PCG iteration: beta = rz_new / rz
+        st->rz_new = out[0];
+        st->beta = out[0] / st->rz;
+        st->rz = out[0];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Krylov vector updates (LsSol::Solve_PCG / Solve_PBiCGSTAB, solver_ls_iter.cpp)
+// PCG: r += (-alpha) z ; x += alpha p ; rr = <r,r> ; convergence test (:140-154)
+template <typename T>
+static __global__ void __launch_bounds__(kNT) k_pcg_update_xr(long n, const T* __restrict__ z, const T* __restrict__ p,
+                                                      T* __restrict__ r, T* __restrict__ x,
+                                                      double* __restrict__ partials, KState* __restrict__ st) {
+  if (st->done) return;
+  const double alpha = st->alpha;
+  const T a = (T)alpha, ma = (T)(-alpha);
+  double dsum[1] = {0.0};
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
+    const T rv = r[i] + ma * z[i];
+    r[i] = rv;
+    x[i] = x[i] + a * p[i];
+    dsum[0] += (double)rv * (double)rv;
+  }
+  double out[1];
+  if (finish_sums<1>(dsum, partials, blockIdx.x, gridDim.x, &st->tickets[2], out) && threadIdx.x == 0) {
+    st->rr = out[0];
+    if (out[0] * st->inv_rr0 < st->tol2) {
+      st->done = 1;
+      st->conv_iter = st->iter;
+      st->conv_rr = out[0];
+    }
+    st->iter += 1;
+  }
+}
+// PCG: p = beta p + z (:163-164)
+template <typename T>
+static __global__ void __launch_bounds__(kNT) k_pcg_update_p(long n, const T* __restrict__ z, T* __restrict__ p,
+                                                     const KState* __restrict__ st) {
+  if (st->done) return;
+  const T beta = (T)st->beta;
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) p[i] = beta * p[i] + z[i];
+}
+// BiCGSTAB: s = r + (-alpha) AMp (:355-357)
+template <typename T>
+static __global__ void __launch_bounds__(kNT) k_bicg_s(long n, const T* __restrict__ r, const T* __restrict__ AMp,
+                                               T* __restrict__ s, const KState* __restrict__ st) {
+  if (st->done) return;
+  const T ma = (T)(-st->alpha);
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) s[i] = r[i] + ma * AMp[i];
+}
+// BiCGSTAB: x += alpha Mp; x += omega Ms; r = s + (-omega) AMs; <r,r>, <r,r2>;
+// convergence test and beta (:376-398)
+template <typename T>
+static __global__ void __launch_bounds__(kNT) k_bicg_xr(long n, const T* __restrict__ Mp, const T* __restrict__ Ms,
+                                                const T* __restrict__ s, const T* __restrict__ AMs,
+                                                const T* __restrict__ r2, T* __restrict__ x, T* __restrict__ r,
+                                                double* __restrict__ partials, KState* __restrict__ st) {
+  if (st->done) return;
+  const T a = (T)st->alpha, w = (T)st->omega, mw = (T)(-st->omega);
+  double dsum[2] = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
+    T xv = x[i] + a * Mp[i];
+    xv = xv + w * Ms[i];
+    x[i] = xv;
+    const T rv = s[i] + mw * AMs[i];
+    r[i] = rv;
+    dsum[0] += (double)rv * (double)rv;
+    dsum[1] += (double)rv * (double)r2[i];
+  }
+  double out[2];
+  if (finish_sums<2>(dsum, partials, blockIdx.x, gridDim.x, &st->tickets[3], out) && threadIdx.x == 0) {
+    st->rr = out[0];
+    if (out[0] * st->inv_rr0 < st->tol2) {
+      st->done = 1;
+      st->conv_iter = st->iter;
+      st->conv_rr = out[0];
+    } else {
+      st->beta = out[1] * st->alpha / (st->r_r2 * st->omega);
+    }
+    st->iter += 1;
+  }
+}
+// BiCGSTAB: p = beta p; p += r; p += (-beta*omega) AMp (:401-403)
+template <typename T>
+static __global__ void __launch_bounds__(kNT) k_bicg_p(long n, const T* __restrict__ r, const T* __restrict__ AMp,
+                                               T* __restrict__ p, const KState* __restrict__ st) {
+  if (st->done) return;
+  const T beta = (T)st->beta, mbw = (T)(-st->beta * st->omega);
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
+    T pv = beta * p[i];
+    pv = pv + r[i];
+    pv = pv + mbw * AMp[i];
+    p[i] = pv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plain vector kernels
+template <typename T>
+static __global__ void k_dot(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restrict__ partials,
+                      KState* __restrict__ st) {
+  double dsum[1] = {0.0};
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT)
+    dsum[0] += (double)x[i] * (double)y[i];
+  double out[1];
+  if (finish_sums<1>(dsum, partials, blockIdx.x, gridDim.x, &st->tickets[4], out) && threadIdx.x == 0)
+    st->d0 = out[0];
+}
+static __global__ void k_axpy(long n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] += alpha * x[i];
+}
+static __global__ void k_scal(long n, double alpha, double* __restrict__ x) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) x[i] *= alpha;
+}
+static __global__ void k_zero_bc(long n, const int* __restrict__ flag, double* __restrict__ x) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT)
+    if (flag[i] == 1) x[i] = 0.0;
+}
+// out[new] = in[order[new]] (OrderingVector_OldToNew, ordering_blk.cpp:151-166) with type conversion and scale
+template <typename TI, typename TO>
+static __global__ void k_permute(int nb, int b, const int* __restrict__ order, const TI* __restrict__ in,
+                          TO* __restrict__ out, double scale) {
+  const long t = (long)blockIdx.x * kNT + threadIdx.x;
+  if (t >= (long)nb * b) return;
+  const long i = t / b;
+  const int r = (int)(t - i * b);
+  const long src = order ? order[i] : i;
+  out[t] = (TO)(scale * (double)in[(size_t)src * b + r]);
+}
+// out[order[new]] = in[new] (OrderingVector_NewToOld, ordering_blk.cpp:136-149)
+template <typename T>
+static __global__ void k_unpermute(int nb, int b, const int* __restrict__ order, const T* __restrict__ in, T* __restrict__ out) {
+  const long t = (long)blockIdx.x * kNT + threadIdx.x;
+  if (t >= (long)nb * b) return;
+  const long i = t / b;
+  const int r = (int)(t - i * b);
+  const long dst = order ? order[i] : i;
+  out[(size_t)dst * b + r] = in[t];
+}
+// x64 += scale * (double)d32 (refinement update)
+static __global__ void k_refine_update(long n, double scale, const float* __restrict__ d, double* __restrict__ x) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT)
+    x[i] = x[i] + scale * (double)d[i];
+}
+template <typename T>
+static __global__ void k_fill(long n, T v, T* __restrict__ x) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) x[i] = v;
+}
+
+}  // namespace dfm

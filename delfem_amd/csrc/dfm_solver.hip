@@ -1,0 +1,499 @@
+// dfm_solver.hip -- device-resident Krylov solvers: LsSol::Solve_PCG and
+// LsSol::Solve_PBiCGSTAB (src/ls/solver_ls_iter.cpp:100-172, 291-408) with
+// the ILU preconditioner, plus the mixed-precision variant (FP32 matrix,
+// factor and inner Krylov; FP64 reductions, true residual and refinement).
+//
+// All vectors live in factor numbering; the right-hand side is permuted in
+// once and the solution and residual are permuted out once.  Scalars
+// (alpha, beta, omega, residual norms) never leave the device inside the
+// iteration: each kernel reads them from a KState and the host only polls the
+// convergence flag every check_every iterations.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "dfm_kernels.h"
+
+namespace dfm {
+
+static inline int grid_for(long n, int per = kNT) { return (int)std::max<long>(1, (n + per - 1) / per); }
+static inline int grid_vec(long n) { return (int)std::min<long>(std::max<long>(1, (n + kNT - 1) / kNT), 2048); }
+
+#define DFM_LAUNCH(kernel, grid, ...)                                                       \
+  do {                                                                                      \
+    if ((grid) > 0) {                                                                       \
+      hipLaunchKernelGGL((kernel), dim3(grid), dim3(kNT), 0, ::dfm::stream(), __VA_ARGS__); \
+      DFM_HIP(hipGetLastError());                                                           \
+    }                                                                                       \
+  } while (0)
+
+template <typename T>
+static int sell_alloc_like(const Sell<double>& src, Sell<T>& dst) {
+  dst.nslice = src.nslice;
+  dst.nk = src.nk;
+  dst.bb = src.bb;
+  dst.group_slice_ptr = src.group_slice_ptr;
+  DFM_TRY(dst.val.alloc(std::max<size_t>(1, (size_t)src.nk * src.bb * kSlice)));
+  DFM_TRY(dst.dval.alloc(std::max<size_t>(1, (size_t)src.nslice * src.bb * kSlice)));
+  return DFM_OK;
+}
+
+// Refresh the sliced-ELL value copies: factor (always) and A (when its
+// values changed since the last refresh); FP32 copies when requested.
+int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32) {
+  const int bb = (int)(M->b * M->b);
+  auto gather = [&](auto& S, const DevBuf<long long>& src, const double* from) -> int {
+    using TT = std::remove_pointer_t<decltype(S.val.p)>;
+    const long nslot = S.nk * kSlice;
+    DFM_LAUNCH(k_sell_gather<TT>, grid_for(nslot), nslot, bb, src.p, from, S.val.p);
+    return DFM_OK;
+  };
+  auto gather_dia = [&](auto& S, const int* rows, const int* order, const double* from) -> int {
+    using TT = std::remove_pointer_t<decltype(S.val.p)>;
+    const long nslot = (long)S.nslice * kSlice;
+    DFM_LAUNCH(k_sell_gather_dia<TT>, grid_for(nslot), nslot, bb, rows, order, from, S.dval.p);
+    return DFM_OK;
+  };
+  DFM_TRY(gather(M->L64, M->d_srcL, M->d_fval.p));
+  DFM_TRY(gather_dia(M->L64, M->L64.rows.p, nullptr, M->d_fdia.p));
+  DFM_TRY(gather(M->U64, M->d_srcU, M->d_fval.p));
+  if (want32) {
+    if (!M->have32) {
+      DFM_TRY(sell_alloc_like(M->L64, M->L32));
+      DFM_TRY(sell_alloc_like(M->U64, M->U32));
+      DFM_TRY(sell_alloc_like(M->A64, M->A32));
+      M->have32 = true;
+      M->a32_valid = false;
+    }
+    DFM_TRY(gather(M->L32, M->d_srcL, M->d_fval.p));
+    DFM_TRY(gather_dia(M->L32, M->L64.rows.p, nullptr, M->d_fdia.p));
+    DFM_TRY(gather(M->U32, M->d_srcU, M->d_fval.p));
+  }
+  if (A && (M->a_version != A->version || (want32 && !M->a32_valid))) {
+    const int* ord = M->permuted ? M->d_order.p : nullptr;
+    DFM_TRY(gather(M->A64, M->d_srcA, A->d_val.p));
+    DFM_TRY(gather_dia(M->A64, M->A64.rows.p, ord, A->d_dia.p));
+    if (want32) {
+      DFM_TRY(gather(M->A32, M->d_srcA, A->d_val.p));
+      DFM_TRY(gather_dia(M->A32, M->A64.rows.p, ord, A->d_dia.p));
+      M->a32_valid = true;
+    } else {
+      M->a32_valid = false;
+    }
+    M->a_version = A->version;
+  }
+  return DFM_OK;
+}
+
+// ---------------------------------------------------------------------------
+template <int B, typename T>
+static int sweep_lower(const Sell<T>& L, const Sell<double>& plan, const T* rhs, T* out, const KState* st) {
+  const int ng = (int)plan.group_slice_ptr.size() - 1;
+  for (int g = 0; g < ng; g++) {
+    const int s0 = plan.group_slice_ptr[g], s1 = plan.group_slice_ptr[g + 1];
+    DFM_LAUNCH((k_trsv_lower<B, T>), grid_for(s1 - s0, kWaves), s0, s1, plan.rows.p, plan.slice_ptr.p, plan.col.p,
+               L.val.p, L.dval.p, rhs, out, st);
+  }
+  return DFM_OK;
+}
+static int upper_nparts(const Sell<double>& plan) {
+  int n = 0;
+  const int ng = (int)plan.group_slice_ptr.size() - 1;
+  for (int g = 0; g < ng; g++) n += grid_for(plan.group_slice_ptr[g + 1] - plan.group_slice_ptr[g], kWaves);
+  return n;
+}
+template <int B, typename T, bool DOT>
+static int sweep_upper(const Sell<T>& U, const Sell<double>& plan, T* vec, const T* rvec, double* partials,
+                       KState* st, int epi) {
+  const int ng = (int)plan.group_slice_ptr.size() - 1;
+  const int nparts = DOT ? upper_nparts(plan) : 0;
+  int part0 = 0;
+  for (int g = 0; g < ng; g++) {
+    const int s0 = plan.group_slice_ptr[g], s1 = plan.group_slice_ptr[g + 1];
+    const int grid = grid_for(s1 - s0, kWaves);
+    DFM_LAUNCH((k_trsv_upper<B, T, DOT>), grid, s0, s1, plan.rows.p, plan.slice_ptr.p, plan.col.p, U.val.p, vec,
+               rvec, partials, part0, nparts, st, epi);
+    part0 += grid;
+  }
+  return DFM_OK;
+}
+
+int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper) {
+  switch (M->b) {
+    case 1:
+      if (lower) DFM_TRY((sweep_lower<1, double>(M->L64, M->L64, vec, vec, nullptr)));
+      if (upper) DFM_TRY((sweep_upper<1, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, 0)));
+      break;
+    case 2:
+      if (lower) DFM_TRY((sweep_lower<2, double>(M->L64, M->L64, vec, vec, nullptr)));
+      if (upper) DFM_TRY((sweep_upper<2, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, 0)));
+      break;
+    case 3:
+      if (lower) DFM_TRY((sweep_lower<3, double>(M->L64, M->L64, vec, vec, nullptr)));
+      if (upper) DFM_TRY((sweep_upper<3, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, 0)));
+      break;
+    default:
+      set_error("block size %u: the device substitutions support 1x1, 2x2 and 3x3 blocks", M->b);
+      return DFM_NOT_IMPLEMENTED;
+  }
+  return DFM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Krylov cores.  Plain views of one precision's operators.
+template <typename T>
+struct Ops {
+  const Sell<T>* A;
+  const Sell<T>* L;
+  const Sell<T>* U;
+  const Sell<double>* planA;
+  const Sell<double>* planL;
+  const Sell<double>* planU;
+  int nb;
+};
+
+template <typename T>
+static Ops<T> ops_of(dfm_ilu_s* M);
+template <>
+Ops<double> ops_of<double>(dfm_ilu_s* M) {
+  return {&M->A64, &M->L64, &M->U64, &M->A64, &M->L64, &M->U64, (int)M->nb};
+}
+template <>
+Ops<float> ops_of<float>(dfm_ilu_s* M) {
+  return {&M->A32, &M->L32, &M->U32, &M->A64, &M->L64, &M->U64, (int)M->nb};
+}
+
+template <int B, typename T, int MODE>
+static int spmv(const Ops<T>& o, const T* x, T* y, const T* res, const T* w0, const T* u2, const T* v2,
+                double* partials, KState* st) {
+  const int nslice = o.planA->nslice;
+  DFM_LAUNCH((k_spmv<B, T, MODE>), grid_for(nslice, kWaves), nslice, o.nb, o.planA->slice_ptr.p, o.planA->col.p,
+             o.A->val.p, o.A->dval.p, x, y, res, w0, u2, v2, partials, st);
+  return DFM_OK;
+}
+
+static int read_state(dfm_ilu_s* M) {
+  DFM_HIP(hipMemcpyAsync(M->h_state, M->state.p, sizeof(KState), hipMemcpyDeviceToHost, stream()));
+  DFM_HIP(hipStreamSynchronize(stream()));
+  return DFM_OK;
+}
+static int write_state(dfm_ilu_s* M) {
+  DFM_HIP(hipMemcpyAsync(M->state.p, M->h_state, sizeof(KState), hipMemcpyHostToDevice, stream()));
+  return DFM_OK;
+}
+template <typename T>
+static int device_dot(dfm_ilu_s* M, long n, const T* x, const T* y, double* out) {
+  DFM_LAUNCH(k_dot<T>, grid_vec(n), n, x, y, M->partials.p, M->state.p);
+  DFM_TRY(read_state(M));
+  *out = M->h_state->d0;
+  return DFM_OK;
+}
+
+struct KResult {
+  double conv;
+  unsigned iter;
+  bool converged;
+};
+
+// LsSol::Solve_PCG (solver_ls_iter.cpp:100-172).  r: in rhs, out final residual; x: out.
+template <int B, typename T>
+static int pcg_core(dfm_ilu_s* M, const Ops<T>& o, T* r, T* x, T* p, T* z, double tol, unsigned maxit,
+                    unsigned check_every, KResult& res) {
+  const long n = (long)o.nb * B;
+  KState* st = M->state.p;
+  double rr0 = 0.0;
+  DFM_TRY(device_dot<T>(M, n, r, r, &rr0));
+  DFM_LAUNCH(k_fill<T>, grid_vec(n), n, (T)0, x);
+  if (rr0 < 1.0e-30) {
+    res = {0.0, 0u, true};
+    return DFM_OK;
+  }
+  KState h;
+  std::memset(&h, 0, sizeof h);
+  h.inv_rr0 = 1.0 / rr0;
+  h.tol2 = tol * tol;
+  *M->h_state = h;
+  DFM_TRY(write_state(M));
+  DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, r, z, st)));
+  DFM_TRY((sweep_upper<B, T, true>(*o.U, *o.planU, z, r, M->partials.p, st, 0)));
+  DFM_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+  const int gv = grid_vec(n);
+  unsigned it = 0;
+  for (; it < maxit;) {
+    DFM_TRY((spmv<B, T, EPI_PCG_PAP>(o, p, z, nullptr, p, nullptr, nullptr, M->partials.p, st)));
+    DFM_LAUNCH(k_pcg_update_xr<T>, gv, n, z, p, r, x, M->partials.p, st);
+    DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, r, z, st)));
+    DFM_TRY((sweep_upper<B, T, true>(*o.U, *o.planU, z, r, M->partials.p, st, 1)));
+    DFM_LAUNCH(k_pcg_update_p<T>, gv, n, z, p, st);
+    it++;
+    if (it % check_every == 0 || it == maxit) {
+      DFM_TRY(read_state(M));
+      if (M->h_state->done) break;
+    }
+  }
+  DFM_TRY(read_state(M));
+  const KState& s = *M->h_state;
+  if (s.done) {
+    res = {std::sqrt(s.conv_rr * s.inv_rr0), (unsigned)s.conv_iter, true};
+  } else {
+    res = {std::sqrt(s.rr * s.inv_rr0), maxit, false};
+  }
+  return DFM_OK;
+}
+
+// LsSol::Solve_PBiCGSTAB (solver_ls_iter.cpp:291-408).
+template <int B, typename T>
+static int bicg_core(dfm_ilu_s* M, const Ops<T>& o, T* r, T* x, T* w, double tol, unsigned maxit,
+                     unsigned check_every, KResult& res, bool& x_zeroed) {
+  const long n = (long)o.nb * B;
+  T *r2 = w, *p = w + n, *Mp = w + 2 * n, *AMp = w + 3 * n, *s = w + 4 * n, *Ms = w + 5 * n, *AMs = w + 6 * n;
+  KState* st = M->state.p;
+  double rr0 = 0.0;
+  DFM_TRY(device_dot<T>(M, n, r, r, &rr0));
+  x_zeroed = false;
+  if (rr0 < 1.0e-60) {
+    res = {std::sqrt(rr0), 0u, true};
+    return DFM_OK;
+  }
+  DFM_LAUNCH(k_fill<T>, grid_vec(n), n, (T)0, x);
+  x_zeroed = true;
+  DFM_HIP(hipMemcpyAsync(r2, r, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+  DFM_HIP(hipMemcpyAsync(p, r, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+  KState h;
+  std::memset(&h, 0, sizeof h);
+  h.inv_rr0 = 1.0 / rr0;
+  h.tol2 = tol * tol;
+  h.iter = 1;
+  *M->h_state = h;
+  DFM_TRY(write_state(M));
+  const int gv = grid_vec(n);
+  unsigned it = 1;
+  for (; it < maxit;) {
+    DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, p, Mp, st)));
+    DFM_TRY((sweep_upper<B, T, false>(*o.U, *o.planU, Mp, nullptr, nullptr, st, 0)));
+    DFM_TRY((spmv<B, T, EPI_BICG_ALPHA>(o, Mp, AMp, nullptr, r2, r, r2, M->partials.p, st)));
+    DFM_LAUNCH(k_bicg_s<T>, gv, n, r, AMp, s, st);
+    DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, s, Ms, st)));
+    DFM_TRY((sweep_upper<B, T, false>(*o.U, *o.planU, Ms, nullptr, nullptr, st, 0)));
+    DFM_TRY((spmv<B, T, EPI_BICG_OMEGA>(o, Ms, AMs, nullptr, s, nullptr, nullptr, M->partials.p, st)));
+    DFM_LAUNCH(k_bicg_xr<T>, gv, n, Mp, Ms, s, AMs, r2, x, r, M->partials.p, st);
+    DFM_LAUNCH(k_bicg_p<T>, gv, n, r, AMp, p, st);
+    it++;
+    if ((it - 1) % check_every == 0 || it == maxit) {
+      DFM_TRY(read_state(M));
+      if (M->h_state->done) break;
+    }
+  }
+  DFM_TRY(read_state(M));
+  const KState& sv = *M->h_state;
+  if (sv.done) {
+    res = {std::sqrt(sv.conv_rr * sv.inv_rr0), (unsigned)sv.conv_iter, true};
+  } else {
+    res = {tol, maxit, false};  // the reference leaves conv_ratio and reports max_iter
+  }
+  return DFM_OK;
+}
+
+// ---------------------------------------------------------------------------
+static int ensure_workspace(dfm_ilu_s* M, size_t n64, size_t n32) {
+  if (M->w64.n < n64) DFM_TRY(M->w64.alloc(n64));
+  if (n32 && M->w32.n < n32) DFM_TRY(M->w32.alloc(n32));
+  size_t need = 0;
+  need = std::max<size_t>(need, (size_t)grid_for(M->A64.nslice, kWaves));
+  need = std::max<size_t>(need, (size_t)upper_nparts(M->U64));
+  need = std::max<size_t>(need, (size_t)2048);
+  need = 3 * need + 64;
+  if (M->partials.n < need) DFM_TRY(M->partials.alloc(need));
+  return DFM_OK;
+}
+
+template <int B>
+static int true_residual64(dfm_ilu_s* M, const double* b, const double* x, double* r, double* rr) {
+  Ops<double> o = ops_of<double>(M);
+  DFM_TRY((spmv<B, double, EPI_STORE>(o, x, r, b, nullptr, nullptr, nullptr, M->partials.p, M->state.p)));
+  DFM_TRY(read_state(M));
+  *rr = M->h_state->d1;
+  return DFM_OK;
+}
+
+template <int B>
+static int solve_dispatch(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r_user, double* x_user, double* conv_ratio,
+                          unsigned* iteration, dfm_solve_opts* opts) {
+  const int nb = (int)M->nb;
+  const long n = (long)nb * B;
+  const bool mixed = opts && opts->precision == DFM_PREC_MIXED;
+  const unsigned check_every = (opts && opts->check_every) ? opts->check_every : 16;
+  const bool want_true = opts && (opts->want_true_relres || mixed);
+  const double tol = *conv_ratio;
+  const unsigned maxit = *iteration;
+  DFM_TRY(ilu_refresh_sell(M, A, mixed));
+  const int* ord = M->permuted ? M->d_order.p : nullptr;
+  // FP64 workspace: r, x (permuted copies), b (rhs copy), then solver vectors
+  const size_t nvec64 = mixed ? 3 : (bicg ? 10 : 5);
+  const size_t nvec32 = mixed ? (bicg ? 9 : 4) : 0;
+  DFM_TRY(ensure_workspace(M, nvec64 * n, nvec32 * n));
+  double* r = M->w64.p;
+  double* x = M->w64.p + n;
+  double* b = M->w64.p + 2 * n;
+  DFM_LAUNCH((k_permute<double, double>), grid_for(n), nb, B, ord, r_user, r, 1.0);
+  if (want_true) DFM_HIP(hipMemcpyAsync(b, r, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+  KResult kr{0.0, 0u, false};
+  int rc = DFM_OK;
+  bool x_written = true;
+  if (!mixed) {
+    Ops<double> o = ops_of<double>(M);
+    if (!bicg) {
+      DFM_TRY((pcg_core<B, double>(M, o, r, x, M->w64.p + 3 * n, M->w64.p + 4 * n, tol, maxit, check_every, kr)));
+    } else {
+      DFM_TRY((bicg_core<B, double>(M, o, r, x, M->w64.p + 3 * n, tol, maxit, check_every, kr, x_written)));
+    }
+    *conv_ratio = kr.conv;
+    if (kr.converged || !bicg) *iteration = kr.iter;
+    rc = kr.converged ? DFM_OK : DFM_NOT_CONVERGED;
+    if (want_true && x_written) {
+      double rr = 0.0, bb2 = 0.0;
+      DFM_TRY(device_dot<double>(M, n, b, b, &bb2));
+      double* tmp = bicg ? M->w64.p + 4 * n : M->w64.p + 4 * n;  // z / Mp slot is free now
+      DFM_TRY(true_residual64<B>(M, b, x, tmp, &rr));
+      opts->true_relres = bb2 > 0.0 ? std::sqrt(rr / bb2) : 0.0;
+    }
+  } else {
+    // Iterative refinement: x += A32^-1 (b - A64 x) with an FP32 inner Krylov solve.
+    Ops<float> o = ops_of<float>(M);
+    float* r32 = M->w32.p;
+    float* d32 = M->w32.p + n;
+    float* ws32 = M->w32.p + 2 * n;
+    const double inner_tol = (opts->inner_tol > 0.0) ? opts->inner_tol : 1.0e-4;
+    const unsigned max_outer = opts->max_outer ? opts->max_outer : 30;
+    double bb2 = 0.0;
+    DFM_TRY(device_dot<double>(M, n, b, b, &bb2));
+    DFM_LAUNCH(k_fill<double>, grid_vec(n), n, 0.0, x);
+    unsigned total = 0, outer = 0;
+    double rel = 0.0;
+    bool conv = false;
+    if (bb2 < 1.0e-30) {
+      conv = true;
+    } else {
+      for (outer = 0; outer <= max_outer; outer++) {
+        double rr = 0.0;
+        DFM_TRY(true_residual64<B>(M, b, x, r, &rr));
+        rel = std::sqrt(rr / bb2);
+        if (rel < tol) {
+          conv = true;
+          break;
+        }
+        if (outer == max_outer || total >= maxit) break;
+        const double nr = std::sqrt(rr);
+        DFM_LAUNCH((k_permute<double, float>), grid_for(n), nb, B, nullptr, r, r32, 1.0 / nr);
+        KResult ir{0.0, 0u, false};
+        const unsigned left = maxit - total;
+        if (!bicg) {
+          DFM_TRY((pcg_core<B, float>(M, o, r32, d32, ws32, ws32 + n, inner_tol, left, check_every, ir)));
+        } else {
+          bool xz = true;
+          DFM_TRY((bicg_core<B, float>(M, o, r32, d32, ws32, inner_tol, left, check_every, ir, xz)));
+        }
+        total += ir.converged ? ir.iter + 1 : left;
+        DFM_LAUNCH(k_refine_update, grid_vec(n), n, nr, d32, x);
+      }
+    }
+    opts->inner_iters = total;
+    opts->outer_iters = outer;
+    opts->true_relres = rel;
+    *conv_ratio = rel;
+    *iteration = total;
+    rc = conv ? DFM_OK : DFM_NOT_CONVERGED;
+  }
+  if (x_written) DFM_LAUNCH(k_unpermute<double>, grid_for(n), nb, B, ord, x, x_user);
+  DFM_LAUNCH(k_unpermute<double>, grid_for(n), nb, B, ord, r, r_user);
+  DFM_HIP(hipStreamSynchronize(stream()));
+  return rc;
+}
+
+static int solve_entry(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r, double* x, double* conv_ratio,
+                       unsigned* iteration, dfm_solve_opts* opts) {
+  DFM_CHECK(A && M && r && x && conv_ratio && iteration, DFM_BAD_ARG, "bad arguments to the solver");
+  DFM_CHECK(A->nb == M->nb && A->b == M->b, DFM_BAD_ARG, "factor and matrix do not match");
+  DFM_CHECK(M->factored, DFM_BAD_ARG, "factor has no values (call dfm_ilu_set_value)");
+  DFM_CHECK(!opts || opts->precision == DFM_PREC_FP64 || opts->precision == DFM_PREC_MIXED, DFM_BAD_ARG,
+            "unknown precision %d", opts ? opts->precision : 0);
+  switch (A->b) {
+    case 1: return solve_dispatch<1>(A, M, bicg, r, x, conv_ratio, iteration, opts);
+    case 2: return solve_dispatch<2>(A, M, bicg, r, x, conv_ratio, iteration, opts);
+    case 3: return solve_dispatch<3>(A, M, bicg, r, x, conv_ratio, iteration, opts);
+    default:
+      set_error("block size %u: the device solvers support 1x1, 2x2 and 3x3 blocks", A->b);
+      return DFM_NOT_IMPLEMENTED;
+  }
+}
+
+template <int B, typename T>
+static int bench_kernel_t(dfm_bcrs_s* A, dfm_ilu_s* M, int which, int nrep, double* avg_ms, double* bytes) {
+  const int nb = (int)M->nb;
+  const long n = (long)nb * B;
+  Ops<T> o = ops_of<T>(M);
+  DevBuf<T> xv, yv;
+  DFM_TRY(xv.alloc(n));
+  DFM_TRY(yv.alloc(n));
+  DFM_LAUNCH(k_fill<T>, grid_vec(n), n, (T)1, xv.p);
+  DFM_LAUNCH(k_fill<T>, grid_vec(n), n, (T)0, yv.p);
+  const double s = sizeof(T), bb = (double)B * B;
+  const double nnzA = (double)A->col.size();
+  double nnzL = 0, nnzU = 0;
+  for (unsigned i = 0; i < M->nb; i++) {
+    nnzL += M->fdiaind[i] - M->frow[i];
+    nnzU += M->frow[i + 1] - M->fdiaind[i];
+  }
+  auto run = [&]() -> int {
+    if (which == 0) return spmv<B, T, EPI_NONE>(o, xv.p, yv.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (which == 1) return sweep_lower<B, T>(*o.L, *o.planL, xv.p, yv.p, nullptr);
+    return sweep_upper<B, T, false>(*o.U, *o.planU, yv.p, nullptr, nullptr, nullptr, 0);
+  };
+  if (which == 0) *bytes = (nnzA + nb) * bb * s + 4.0 * nnzA + 2.0 * n * s;
+  else if (which == 1) *bytes = (nnzL + nb) * bb * s + 4.0 * nnzL + 4.0 * nb + 3.0 * n * s;
+  else *bytes = nnzU * bb * s + 4.0 * nnzU + 4.0 * nb + 3.0 * n * s;
+  DFM_TRY(run());  // warm-up
+  hipEvent_t e0, e1;
+  DFM_HIP(hipEventCreate(&e0));
+  DFM_HIP(hipEventCreate(&e1));
+  DFM_HIP(hipEventRecord(e0, stream()));
+  for (int r = 0; r < nrep; r++) DFM_TRY(run());
+  DFM_HIP(hipEventRecord(e1, stream()));
+  DFM_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  DFM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = (double)ms / nrep;
+  return DFM_OK;
+}
+
+}  // namespace dfm
+
+extern "C" {
+int dfm_bench_kernel(dfm_bcrs* A, dfm_ilu* M, int which, int precision, int nrep, double* avg_ms, double* bytes) {
+  DFM_CHECK(A && M && avg_ms && bytes && nrep > 0 && which >= 0 && which <= 2, DFM_BAD_ARG, "bad arguments");
+  DFM_CHECK(M->factored, DFM_BAD_ARG, "factor has no values (call dfm_ilu_set_value)");
+  const bool f32 = precision == DFM_PREC_MIXED;
+  DFM_TRY(dfm::ilu_refresh_sell(M, A, f32 || M->have32));
+  switch (A->b) {
+    case 1: return f32 ? dfm::bench_kernel_t<1, float>(A, M, which, nrep, avg_ms, bytes)
+                       : dfm::bench_kernel_t<1, double>(A, M, which, nrep, avg_ms, bytes);
+    case 2: return f32 ? dfm::bench_kernel_t<2, float>(A, M, which, nrep, avg_ms, bytes)
+                       : dfm::bench_kernel_t<2, double>(A, M, which, nrep, avg_ms, bytes);
+    case 3: return f32 ? dfm::bench_kernel_t<3, float>(A, M, which, nrep, avg_ms, bytes)
+                       : dfm::bench_kernel_t<3, double>(A, M, which, nrep, avg_ms, bytes);
+    default: return DFM_NOT_IMPLEMENTED;
+  }
+}
+
+int dfm_solve_pcg(dfm_bcrs* A, dfm_ilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration,
+                  dfm_solve_opts* opts) {
+  return dfm::solve_entry(A, M, false, r, x, conv_ratio, iteration, opts);
+}
+int dfm_solve_pbicgstab(dfm_bcrs* A, dfm_ilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration,
+                        dfm_solve_opts* opts) {
+  return dfm::solve_entry(A, M, true, r, x, conv_ratio, iteration, opts);
+}
+}

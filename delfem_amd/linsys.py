@@ -1,0 +1,325 @@
+"""Python mirror of DelFEM's linear-solve API over the delfem_amd C-ABI.
+
+Class and method names follow the reference (include/delfem/matvec/*.h,
+include/delfem/ls/*.h) so tests and scripts read like DelFEM code:
+
+    A = CMatDia_BlkCrs(nblk, len); A.AddPattern(index, array)
+    ...assemble..., A.SetBoundaryCondition(flags)
+    prec = CPreconditioner_ILU(A, lev=0, ordering="multicolour"); prec.SetValue(A)
+    ok, conv, it = Solve_PCG(1e-10, 1000, A, prec, r, x)
+
+Vectors are torch tensors on the GPU (torch is used only for device memory);
+every kernel is the library's own HIP code.  Calls are synchronous.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+
+_ORDER = {"natural": _lib.DFM_ORDER_NATURAL, "given": _lib.DFM_ORDER_GIVEN,
+          "multicolour": _lib.DFM_ORDER_MULTICOLOUR}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def init(device=0):
+    """Select the device; the library then runs on its own stream."""
+    call("dfm_init", device)
+
+
+def _sync_torch():
+    torch = _torch()
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+
+
+def _ptr(a):
+    """Host pointer of a contiguous numpy array."""
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dptr(t):
+    """Device pointer of a contiguous torch tensor."""
+    assert t.is_cuda and t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+def device_vector(n, dtype=None, fill=0.0):
+    torch = _torch()
+    return torch.full((n,), fill, dtype=dtype or torch.float64, device="cuda")
+
+
+def to_device(a):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def fem_pattern(nnode, conn):
+    """CElemAry::MakePattern_FEM: (index, array) of the node adjacency."""
+    conn = np.ascontiguousarray(conn, dtype=np.uint32)
+    nelem, npe = conn.shape
+    rowptr = np.zeros(nnode + 1, dtype=np.uint32)
+    n = C.c_size_t(0)
+    call("dfm_fem_pattern", nnode, nelem, npe, _ptr(conn), _ptr(rowptr), None, 0, C.byref(n))
+    col = np.zeros(max(1, n.value), dtype=np.uint32)
+    call("dfm_fem_pattern", nnode, nelem, npe, _ptr(conn), _ptr(rowptr), _ptr(col), col.size, C.byref(n))
+    return rowptr, col[:n.value]
+
+
+class CMatDia_BlkCrs:
+    """MatVec::CMatDia_BlkCrs (include/delfem/matvec/matdia_blkcrs.h:44-118)."""
+
+    def __init__(self, nblk, len_):
+        h = C.c_void_p()
+        call("dfm_bcrs_create", nblk, len_, C.byref(h))
+        self.h = h
+        self.nblk, self.len = nblk, len_
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_bcrs_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def NBlkMatCol(self):
+        return self.nblk
+
+    def LenBlkCol(self):
+        return self.len
+
+    def NCrs(self):
+        n = C.c_uint()
+        call("dfm_bcrs_info", self.h, None, None, C.byref(n))
+        return n.value
+
+    def AddPattern(self, index, array):
+        index = np.ascontiguousarray(index, dtype=np.uint32)
+        array = np.ascontiguousarray(array, dtype=np.uint32)
+        if array.size == 0:
+            array = np.zeros(1, dtype=np.uint32)
+        call("dfm_bcrs_add_pattern", self.h, index.size - 1, _ptr(index), _ptr(array))
+        return True
+
+    def pattern(self):
+        rowptr = np.zeros(self.nblk + 1, dtype=np.uint32)
+        col = np.zeros(max(1, self.NCrs()), dtype=np.uint32)
+        call("dfm_bcrs_get_pattern", self.h, _ptr(rowptr), _ptr(col))
+        return rowptr, col[:self.NCrs()]
+
+    def SetZero(self):
+        call("dfm_bcrs_set_zero", self.h)
+        return True
+
+    def set_values(self, val, dia):
+        bb = self.len * self.len
+        val = np.ascontiguousarray(val, dtype=np.float64).reshape(-1)
+        dia = np.ascontiguousarray(dia, dtype=np.float64).reshape(-1)
+        assert val.size == self.NCrs() * bb and dia.size == self.nblk * bb
+        call("dfm_bcrs_set_values", self.h, _ptr(val) if val.size else None, _ptr(dia))
+
+    def values(self):
+        bb = self.len * self.len
+        val = np.zeros(max(1, self.NCrs() * bb))
+        dia = np.zeros(max(1, self.nblk * bb))
+        call("dfm_bcrs_get_values", self.h, _ptr(val), _ptr(dia))
+        return val[:self.NCrs() * bb], dia[:self.nblk * bb]
+
+    def Mearge(self, nodes, emat):
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        emat = np.ascontiguousarray(emat, dtype=np.float64).reshape(-1)
+        call("dfm_bcrs_mearge", self.h, nodes.size, _ptr(nodes), self.len * self.len, _ptr(emat))
+        return True
+
+    def SetBoundaryCondition(self, flags):
+        flags = np.ascontiguousarray(flags, dtype=np.int32).reshape(-1)
+        call("dfm_bcrs_set_bc", self.h, _ptr(flags))
+        return True
+
+    def MatVec(self, alpha, x, beta, y):
+        _sync_torch()
+        call("dfm_bcrs_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
+        return True
+
+
+class Assembler:
+    """Element loops of femeqn (AddLinSys_*) gathered by row on the device."""
+
+    def __init__(self, A, conn):
+        conn = np.ascontiguousarray(conn, dtype=np.uint32)
+        h = C.c_void_p()
+        call("dfm_asm_create", A.h, A.nblk, conn.shape[0], conn.shape[1], _ptr(conn), C.byref(h))
+        self.h, self.A = h, A
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_asm_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def poisson2d_p1(self, coord, alpha, source, value, res):
+        _sync_torch()
+        call("dfm_assemble_poisson2d_p1", self.h, _dptr(coord), _dptr(alpha), float(source), _dptr(value),
+             _dptr(res) if res is not None else None)
+
+    def solid3d_p1(self, coord, lam, mu, rho, g, disp, res):
+        _sync_torch()
+        call("dfm_assemble_solid3d_p1", self.h, _dptr(coord), _dptr(lam), _dptr(mu), float(rho), float(g[0]),
+             float(g[1]), float(g[2]), _dptr(disp), _dptr(res) if res is not None else None)
+
+
+class CMatDiaFrac_BlkCrs:
+    """MatVec::CMatDiaFrac_BlkCrs (include/delfem/matvec/matdiafrac_blkcrs.h:40-137):
+    symbolic ILU(lev) at construction, numeric on SetValue."""
+
+    def __init__(self, lev, A, order=None, ordering=None):
+        if ordering is None:
+            ordering = "given" if order is not None else "natural"
+        h = C.c_void_p()
+        ordarr = None
+        if order is not None:
+            ordarr = np.ascontiguousarray(order, dtype=np.int32)
+        call("dfm_ilu_create", A.h, int(lev), _ORDER[ordering], _ptr(ordarr) if ordarr is not None else None,
+             C.byref(h))
+        self.h, self.nblk, self.len = h, A.nblk, A.len
+        self.nsingular = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_ilu_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def info(self):
+        ncrs, nl, nu, nc = C.c_uint(), C.c_int(), C.c_int(), C.c_int()
+        call("dfm_ilu_info", self.h, C.byref(ncrs), C.byref(nl), C.byref(nu), C.byref(nc))
+        return dict(ncrs=ncrs.value, nlevel_lower=nl.value, nlevel_upper=nu.value, ncolour=nc.value)
+
+    def NCrs(self):
+        return self.info()["ncrs"]
+
+    def pattern(self):
+        n = self.NCrs()
+        rowptr = np.zeros(self.nblk + 1, dtype=np.uint32)
+        col = np.zeros(max(1, n), dtype=np.uint32)
+        lev = np.zeros(max(1, n), dtype=np.int32)
+        dia = np.zeros(self.nblk, dtype=np.uint32)
+        order = np.zeros(self.nblk, dtype=np.int32)
+        call("dfm_ilu_get_pattern", self.h, _ptr(rowptr), _ptr(col), _ptr(lev), _ptr(dia), _ptr(order))
+        return dict(rowptr=rowptr, col=col[:n], lev=lev[:n], diaind=dia, order=order)
+
+    def SetValue(self, A):
+        ns = C.c_int()
+        rc = call("dfm_ilu_set_value", self.h, A.h, C.byref(ns), ok=(_lib.DFM_OK, _lib.DFM_SINGULAR))
+        self.nsingular = ns.value
+        return rc == _lib.DFM_OK
+
+    def values(self):
+        bb = self.len * self.len
+        n = self.NCrs()
+        val = np.zeros(max(1, n * bb))
+        dia = np.zeros(self.nblk * bb)
+        call("dfm_ilu_get_values", self.h, _ptr(val), _ptr(dia))
+        return val[:n * bb], dia
+
+    def ForwardSubstitution(self, vec):
+        _sync_torch()
+        call("dfm_ilu_forward", self.h, _dptr(vec))
+        return True
+
+    def BackwardSubstitution(self, vec):
+        _sync_torch()
+        call("dfm_ilu_backward", self.h, _dptr(vec))
+        return True
+
+    def SolvePrecond(self, vec):
+        """CPreconditioner_ILU::SolvePrecond on a vector in the original numbering."""
+        _sync_torch()
+        call("dfm_ilu_solve_precond", self.h, _dptr(vec))
+        return True
+
+
+class CPreconditioner_ILU(CMatDiaFrac_BlkCrs):
+    """LsSol::CPreconditioner_ILU (include/delfem/ls/preconditioner.h:59-108) on one segment."""
+
+    def __init__(self, A, nlev=0, order=None, ordering=None):
+        super().__init__(nlev, A, order=order, ordering=ordering)
+
+
+def _solve(fn, conv_ratio, iteration, A, prec, r, x, precision, opts_kw):
+    _sync_torch()
+    conv = C.c_double(conv_ratio)
+    it = C.c_uint(iteration)
+    opts = _lib.SolveOpts()
+    opts.precision = _lib.DFM_PREC_MIXED if precision == "mixed" else _lib.DFM_PREC_FP64
+    for k, v in (opts_kw or {}).items():
+        setattr(opts, k, v)
+    rc = call(fn, A.h, prec.h, _dptr(r), _dptr(x), C.byref(conv), C.byref(it), C.byref(opts),
+              ok=(_lib.DFM_OK, _lib.DFM_NOT_CONVERGED))
+    info = dict(inner_iters=opts.inner_iters, outer_iters=opts.outer_iters, true_relres=opts.true_relres)
+    return rc == _lib.DFM_OK, conv.value, it.value, info
+
+
+def Solve_PCG(conv_ratio, iteration, A, prec, r, x, precision="fp64", **opts):
+    """LsSol::Solve_PCG (src/ls/solver_ls_iter.cpp:100-172).  Returns
+    (converged, conv_ratio, iteration, info); r is overwritten by the residual."""
+    return _solve("dfm_solve_pcg", conv_ratio, iteration, A, prec, r, x, precision, opts)
+
+
+def Solve_PBiCGSTAB(conv_ratio, iteration, A, prec, r, x, precision="fp64", **opts):
+    """LsSol::Solve_PBiCGSTAB (src/ls/solver_ls_iter.cpp:291-408)."""
+    return _solve("dfm_solve_pbicgstab", conv_ratio, iteration, A, prec, r, x, precision, opts)
+
+
+def dot(x, y):
+    _sync_torch()
+    out = C.c_double()
+    call("dfm_vec_dot", x.numel(), _dptr(x), _dptr(y), C.byref(out))
+    return out.value
+
+
+class FEMSystem:
+    """A Problem (delfem_amd.meshgen) assembled on the device: matrix,
+    residual, Dirichlet conditions -- the InitializeMarge / AddLinSys_* /
+    FinalizeMarge sequence of src/ls/linearsystem.cpp:74-128."""
+
+    def __init__(self, prob):
+        torch = _torch()
+        self.prob = prob
+        rowptr, col = fem_pattern(prob.nnode, prob.conn)
+        self.A = CMatDia_BlkCrs(prob.nnode, prob.b)
+        self.A.AddPattern(rowptr, col)
+        self.asm = Assembler(self.A, prob.conn)
+        self.coord = to_device(prob.coord.ravel())
+        if prob.kind == "poisson2d":
+            self.coef = (to_device(prob.alpha),)
+        else:
+            self.coef = (to_device(prob.lam), to_device(prob.mu))
+        self.res = torch.zeros(prob.ndof, dtype=torch.float64, device="cuda")
+        self.value = torch.zeros(prob.ndof, dtype=torch.float64, device="cuda")
+        self.flags = np.ascontiguousarray(prob.flags, dtype=np.int32)
+        self.flags_dev = to_device(self.flags)
+
+    def assemble(self):
+        """InitializeMarge + element loop + FinalizeMarge; returns ||r||."""
+        self.A.SetZero()
+        self.res.zero_()
+        p = self.prob
+        if p.kind == "poisson2d":
+            self.asm.poisson2d_p1(self.coord, self.coef[0], p.source, self.value, self.res)
+        else:
+            self.asm.solid3d_p1(self.coord, self.coef[0], self.coef[1], p.rho, p.g, self.value, self.res)
+        self.A.SetBoundaryCondition(self.flags)
+        _sync_torch()
+        call("dfm_vec_zero_bc", self.res.numel(), _ptr(self.flags), _dptr(self.res))
+        return dot(self.res, self.res) ** 0.5

@@ -1,0 +1,173 @@
+/*
+ * dfm.h -- C-ABI of delfem_amd, the MI355X (gfx950) implementation of
+ * DelFEM's linear-solve hot path: block-CRS matrices with a separate diagonal,
+ * finite-element assembly, ILU(k) factorisation and preconditioned CG /
+ * BiCGSTAB.
+ *
+ * Conventions (identical to the DelFEM reference):
+ *  - a square matrix has nblk block rows of len x len blocks; the CRS part
+ *    excludes the diagonal, which is held separately; blocks are row-major;
+ *  - "index"/"array" arguments follow Com::CIndexedArray (row pointer of
+ *    nblk+1 entries and column list, self excluded, each row ascending);
+ *  - an ordering is order[new] = old;
+ *  - vectors are block vectors of nblk*len doubles in the reference layout.
+ * Vector arguments named *_dev are device pointers (hipMalloc'd memory of the
+ * current device, or memory from dfm_malloc); arrays without the suffix are
+ * host memory.  All calls are synchronous with respect to the host unless
+ * stated otherwise.  Every call returns a dfm status; dfm_last_error()
+ * describes the last failure.  Handles are thread-compatible (one thread at a
+ * time per handle), not thread-safe.
+ *
+ * Each entry point names the reference interface it replaces
+ * (file:line in the DelFEM tree).
+ */
+#ifndef DFM_H
+#define DFM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DFM_OK = 0,
+    DFM_NOT_CONVERGED = 1,     /* Solve_PCG returning false (solver_ls_iter.cpp:168-171) */
+    DFM_SINGULAR = 2,          /* more than 10 singular pivots (matdiafrac_blkcrs.cpp:774,880-890) */
+    DFM_BAD_ARG = -1,
+    DFM_HIP_ERROR = -2,
+    DFM_UNSORTED = -3,         /* a pattern row is not ascending (reference asserts) */
+    DFM_NOT_IMPLEMENTED = -4,
+    DFM_TOO_LARGE = -5
+};
+
+typedef struct dfm_bcrs_s dfm_bcrs;   /* MatVec::CMatDia_BlkCrs */
+typedef struct dfm_ilu_s dfm_ilu;     /* MatVec::CMatDiaFrac_BlkCrs / LsSol::CPreconditioner_ILU */
+typedef struct dfm_asm_s dfm_asm;     /* element->matrix scatter plan (femeqn AddLinSys_* loops) */
+typedef struct dfm_batch_s dfm_batch; /* many independent small systems */
+
+/* ---- runtime ------------------------------------------------------------ */
+const char *dfm_last_error(void);
+int dfm_init(int device);                       /* select the device, create the library stream */
+int dfm_synchronize(void);
+int dfm_malloc(void **ptr_dev, size_t bytes);
+int dfm_free(void *ptr_dev);
+int dfm_memcpy_htod(void *dst_dev, const void *src, size_t bytes);
+int dfm_memcpy_dtoh(void *dst, const void *src_dev, size_t bytes);
+int dfm_memcpy_dtod(void *dst_dev, const void *src_dev, size_t bytes);
+/* Stream used by the library (hipStream_t cast to void*).  dfm_set_stream makes
+ * the library issue its work on a caller's stream (NULL: the default stream). */
+void *dfm_stream(void);
+int dfm_set_stream(void *stream);
+
+/* ---- pattern (Com::CIndexedArray from element connectivity) -------------- */
+/* CElemAry::MakePattern_FEM, src/femfield/elem_ary.cpp:225-245 (+675-892):
+ * node adjacency without self, rows ascending.  Returns the number of
+ * entries in *ncrs; col may be NULL to query the size (cap = capacity). */
+int dfm_fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned *conn,
+                    unsigned *rowptr, unsigned *col, size_t cap, size_t *ncrs);
+
+/* ---- MatVec::CMatDia_BlkCrs (include/delfem/matvec/matdia_blkcrs.h:44-118) */
+int dfm_bcrs_create(unsigned nblk, unsigned len, dfm_bcrs **out);          /* CMatDia_BlkCrs(nblk,len) matdia_blkcrs.cpp:40 */
+int dfm_bcrs_destroy(dfm_bcrs *A);
+int dfm_bcrs_add_pattern(dfm_bcrs *A, unsigned nrow, const unsigned *index,
+                         const unsigned *array);                           /* AddPattern(CIndexedArray) matdia_blkcrs.cpp:346-505 */
+int dfm_bcrs_info(const dfm_bcrs *A, unsigned *nblk, unsigned *len, unsigned *ncrs); /* NBlkMatCol/LenBlkCol/NCrs mat_blkcrs.h:83-157 */
+int dfm_bcrs_get_pattern(const dfm_bcrs *A, unsigned *rowptr, unsigned *col);        /* GetPtrIndPSuP mat_blkcrs.h:158-165 */
+int dfm_bcrs_set_zero(dfm_bcrs *A);                                        /* SetZero matdia_blkcrs.cpp:139-163 */
+int dfm_bcrs_set_values(dfm_bcrs *A, const double *val, const double *dia); /* host values into the pattern (GetPtrValPSuP/GetPtrValDia writes) */
+int dfm_bcrs_get_values(const dfm_bcrs *A, double *val, double *dia);       /* GetPtrValPSuP/GetPtrValDia reads */
+int dfm_bcrs_device_values(dfm_bcrs *A, double **val_dev, double **dia_dev); /* device arrays of the values (CRS order) */
+int dfm_bcrs_mearge(dfm_bcrs *A, unsigned nblkel, const unsigned *blkel,
+                    unsigned blksize, const double *emat);                  /* Mearge (one element, host emat) matdia_blkcrs.cpp:165-258 */
+int dfm_bcrs_set_bc(dfm_bcrs *A, const int *flag);                          /* SetBoundaryCondition(CBCFlag) matdia_blkcrs.cpp:261-343 */
+int dfm_bcrs_matvec(dfm_bcrs *A, double alpha, const double *x_dev, double beta,
+                    double *y_dev);                                         /* MatVec matdia_blkcrs.cpp:921-1094 */
+
+/* ---- MatVec::CVector_Blk / LsSol::CLinearSystem BLAS-1 (linearsystem.cpp:250-415) */
+int dfm_vec_dot(size_t n, const double *x_dev, const double *y_dev, double *result); /* DOT / operator* vector_blk.cpp:37-47 */
+int dfm_vec_axpy(size_t n, double alpha, const double *x_dev, double *y_dev);        /* AXPY vector_blk.cpp:88-97 */
+int dfm_vec_scal(size_t n, double alpha, double *x_dev);                              /* SCAL / operator*= vector_blk.cpp:54-59 */
+int dfm_vec_copy(size_t n, const double *x_dev, double *y_dev);                       /* COPY / operator= vector_blk.cpp:62-71 */
+int dfm_vec_zero_bc(size_t n, const int *flag, double *x_dev);                        /* CBCFlag::SetZeroToBCDof bcflag_blk.h:102-129 */
+
+/* ---- element assembly (femeqn AddLinSys_* loops + Mearge) ---------------- */
+/* A plan that adds element matrices into A in element order per entry (the
+ * accumulation order of sequential Mearge calls).  conn: nelem*npe host array. */
+int dfm_asm_create(dfm_bcrs *A, unsigned nnode, unsigned nelem, unsigned npe,
+                   const unsigned *conn, dfm_asm **out);
+int dfm_asm_destroy(dfm_asm *p);
+/* AddLinearSystem_Poisson2D_P1, src/femeqn/eqn_poisson.cpp:49-125 (alpha per element).
+ * coord_dev: nnode*2, alpha_dev: nelem, value_dev: nnode, res_dev: nnode (accumulated). */
+int dfm_assemble_poisson2d_p1(dfm_asm *p, const double *coord_dev, const double *alpha_dev,
+                              double source, const double *value_dev, double *res_dev);
+/* AddLinSys_LinearSolid3D_Static_P1, src/femeqn/eqn_linear_solid3d.cpp:241-334
+ * (lambda, myu per element).  coord_dev: nnode*3, disp_dev: nnode*3, res_dev: nnode*3. */
+int dfm_assemble_solid3d_p1(dfm_asm *p, const double *coord_dev, const double *lambda_dev,
+                            const double *myu_dev, double rho, double gx, double gy, double gz,
+                            const double *disp_dev, double *res_dev);
+
+/* ---- ILU (MatVec::CMatDiaFrac_BlkCrs, LsSol::CPreconditioner_ILU) -------- */
+enum { DFM_ORDER_NATURAL = 0, DFM_ORDER_GIVEN = 1, DFM_ORDER_MULTICOLOUR = 2 };
+/* Symbolic ILU(lev_fill) of A's pattern: CMatDiaFrac_BlkCrs(lev,A) / (lev,A,order),
+ * matdiafrac_blkcrs.cpp:227-324 + AddFracPtn :1249-1377; CPreconditioner_ILU::SetLinearSystem
+ * preconditioner.cpp:57-211.  ordering: NATURAL (order ignored), GIVEN (order[new]=old),
+ * MULTICOLOUR (greedy colouring computed here). lev_fill = -1 means unlimited. */
+int dfm_ilu_create(const dfm_bcrs *A, int lev_fill, int ordering, const int *order, dfm_ilu **out);
+int dfm_ilu_destroy(dfm_ilu *M);
+/* ncrs of the factor, number of level sets of L and U, colours (0 unless MULTICOLOUR) */
+int dfm_ilu_info(const dfm_ilu *M, unsigned *ncrs, int *nlevel_lower, int *nlevel_upper, int *ncolour);
+/* Factor pattern in factor numbering; any pointer may be NULL. order: nblk entries. */
+int dfm_ilu_get_pattern(const dfm_ilu *M, unsigned *rowptr, unsigned *col, int *lev,
+                        unsigned *diaind, int *order);
+/* Numeric factorisation on the device: SetValue(A) = SetValue_Initialize + DoILUDecomp,
+ * matdiafrac_blkcrs.cpp:1518-1580, 769-1102; CPreconditioner_ILU::SetValue preconditioner.cpp:217-297.
+ * *nsingular (may be NULL) receives the singular pivot count; DFM_SINGULAR when > 10. */
+int dfm_ilu_set_value(dfm_ilu *M, dfm_bcrs *A, int *nsingular);
+/* Factor values (factor numbering): val: ncrs*len*len (L, D^-1 U), dia: nblk*len*len (D^-1). */
+int dfm_ilu_get_values(const dfm_ilu *M, double *val, double *dia);
+/* In-place solves on a device vector in factor numbering:
+ * ForwardSubstitution / BackwardSubstitution matdiafrac_blkcrs.cpp:332-663. */
+int dfm_ilu_forward(dfm_ilu *M, double *vec_dev);
+int dfm_ilu_backward(dfm_ilu *M, double *vec_dev);
+/* CPreconditioner_ILU::SolvePrecond preconditioner.cpp:301-335 on a vector in the
+ * original numbering (permuted in and out when the factor is ordered). */
+int dfm_ilu_solve_precond(dfm_ilu *M, double *vec_dev);
+
+/* ---- Krylov (LsSol::Solve_PCG / Solve_PBiCGSTAB, solver_ls_iter.cpp:100-172, 291-408) */
+enum { DFM_PREC_FP64 = 0, DFM_PREC_MIXED = 1 };
+typedef struct {
+    int precision;         /* DFM_PREC_FP64: all FP64 (reference semantics).
+                              DFM_PREC_MIXED: FP32 matrix, factor and inner Krylov with FP64
+                              dot accumulators; FP64 outer iterative refinement */
+    double inner_tol;      /* MIXED: relative tolerance of each inner solve (0: default 1e-4) */
+    unsigned max_outer;    /* MIXED: maximum refinement steps (0: default 30) */
+    unsigned check_every;  /* iterations between host convergence checks (0: default 16) */
+    int want_true_relres;  /* nonzero: compute true_relres = ||b - A x|| / ||b|| in FP64 at the end */
+    unsigned inner_iters;  /* out: total inner iterations (MIXED) */
+    unsigned outer_iters;  /* out: refinement steps taken (MIXED) */
+    double true_relres;    /* out: ||b - A x|| / ||b|| in FP64 (when want_true_relres, or MIXED) */
+} dfm_solve_opts;
+/* A x = r with x := 0 start.  r_dev: in = right-hand side (vector id -1), out = final
+ * (recursive) residual; x_dev: out = solution (vector id -2).  conv_ratio: in = tolerance,
+ * out = achieved ||r||/||r0||; iteration: in = max, out = as the reference reports.
+ * opts may be NULL (FP64).  Returns DFM_OK (converged) or DFM_NOT_CONVERGED. */
+int dfm_solve_pcg(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
+                  double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
+int dfm_solve_pbicgstab(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
+                        double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
+
+/* ---- measurement (no reference counterpart) -------------------------------
+ * Times one device kernel of a solve on its real operands: which = 0 the
+ * sliced-ELL SpMV of A, 1 the forward sweep (all lower level sets), 2 the
+ * backward sweep.  precision selects the FP64 or FP32 copies.  Runs nrep
+ * back-to-back launches between HIP events on the library stream; returns the
+ * average milliseconds per launch and the algorithmic HBM bytes of one launch
+ * (values + indices + one pass over each vector operand). */
+int dfm_bench_kernel(dfm_bcrs *A, dfm_ilu *M, int which, int precision, int nrep,
+                     double *avg_ms, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,977 @@
+/*
+ * dfm_oracle.c -- CPU restatement of DelFEM's linear-solve hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg load this library.  The product in
+ * delfem_amd/ neither links nor calls it.
+ *
+ * Written from scratch from the behaviour of the reference routines cited on
+ * each function (paths relative to the DelFEM source tree).  Floating-point
+ * operations are written in the reference's evaluation order and the file is
+ * compiled with -ffp-contract=off, so results are bitwise comparable with the
+ * reference built by g++ for x86-64 (no FMA).
+ *
+ * Pinning (what independently checks each function; tests/test_oracle*.py):
+ *   orc_fem_pattern      golden fixtures from the compiled reference + brute-force set union
+ *   orc_assemble(_vec)   golden fixtures (bitwise) + scipy COO->CSR sum
+ *   orc_set_bc           golden fixtures (bitwise) + closed form
+ *   orc_matvec           golden fixtures (bitwise) + scipy.sparse product
+ *   orc_iluk_symbolic    golden fixtures k=0..3 natural and ordered (bitwise), textbook rule k<=1
+ *   orc_ilu_setvalue/
+ *   orc_ilu_decomp       golden fixtures (bitwise) + (L+D)(I+D^-1U)=A on the pattern
+ *   orc_ilu_forward/
+ *   orc_ilu_backward     golden fixtures (bitwise) + dense triangular solve
+ *   orc_solve_pcg        golden fixtures (iterations, conv_ratio, solution) + scipy direct solve
+ *   orc_solve_pbicgstab  golden fixtures (iterations, conv_ratio, solution) + scipy direct solve
+ *   orc_emat_*           golden fixtures (bitwise) + Laplacian row sums, rigid-body null space
+ *   orc_greedy_colour,
+ *   orc_colour_order,
+ *   orc_level_sets       product algorithms (no reference counterpart): pinned by
+ *                        proper-colouring / topological-order invariants
+ */
+#include "dfm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* a1: src/femfield/elem_ary.cpp:225-245 (MakePattern_FEM),
+ *     675-724 (MakePointSurElem), 821-892 (MakePointSurPoint, isnt_self=true),
+ *     include/delfem/indexed_array.h:113-132 (Sort: ascending per row).      */
+long orc_fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned *conn,
+                     unsigned *rowptr, unsigned *col, long cap)
+{
+    unsigned *elsup_ptr = (unsigned *)calloc((size_t)nnode + 1, sizeof(unsigned));
+    unsigned *elsup = (unsigned *)malloc(sizeof(unsigned) * ((size_t)nelem * npe + 1));
+    int *mark = (int *)malloc(sizeof(int) * ((size_t)nnode + 1));
+    long count = 0;
+    for (unsigned e = 0; e < nelem; e++)
+        for (unsigned k = 0; k < npe; k++) elsup_ptr[conn[(size_t)e * npe + k] + 1]++;
+    for (unsigned i = 0; i < nnode; i++) elsup_ptr[i + 1] += elsup_ptr[i];
+    {
+        unsigned *fill = (unsigned *)malloc(sizeof(unsigned) * ((size_t)nnode + 1));
+        memcpy(fill, elsup_ptr, sizeof(unsigned) * ((size_t)nnode + 1));
+        for (unsigned e = 0; e < nelem; e++)
+            for (unsigned k = 0; k < npe; k++) elsup[fill[conn[(size_t)e * npe + k]]++] = e;
+        free(fill);
+    }
+    for (unsigned i = 0; i < nnode; i++) mark[i] = -1;
+    rowptr[0] = 0;
+    for (unsigned i = 0; i < nnode; i++) {
+        long row0 = count;
+        mark[i] = (int)i;
+        for (unsigned q = elsup_ptr[i]; q < elsup_ptr[i + 1]; q++) {
+            unsigned e = elsup[q];
+            for (unsigned k = 0; k < npe; k++) {
+                unsigned j = conn[(size_t)e * npe + k];
+                if (mark[j] != (int)i) {
+                    mark[j] = (int)i;
+                    if (col && count < cap) col[count] = j;
+                    count++;
+                }
+            }
+        }
+        rowptr[i + 1] = (unsigned)count;
+        if (col && count <= cap) { /* insertion sort of the row (ascending) */
+            for (long a = row0 + 1; a < count; a++) {
+                unsigned v = col[a];
+                long b = a - 1;
+                while (b >= row0 && col[b] > v) { col[b + 1] = col[b]; b--; }
+                col[b + 1] = v;
+            }
+        }
+    }
+    free(elsup_ptr); free(elsup); free(mark);
+    return count;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a5: src/matvec/matdia_blkcrs.cpp:165-258 (CMatDia_BlkCrs::Mearge).
+ * Entries whose column is not in the row's pattern are dropped (:242).     */
+void orc_assemble(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                  double *val, double *dia, unsigned nelem, unsigned npe,
+                  const unsigned *conn, const double *emat)
+{
+    const unsigned bb = b * b;
+    int *pos = (int *)malloc(sizeof(int) * (size_t)nb);
+    for (unsigned i = 0; i < nb; i++) pos[i] = -1;
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *no = conn + (size_t)e * npe;
+        const double *em = emat + (size_t)e * npe * npe * bb;
+        for (unsigned ie = 0; ie < npe; ie++) {
+            const unsigned i = no[ie];
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) pos[col[q]] = (int)q;
+            for (unsigned je = 0; je < npe; je++) {
+                const double *in = em + ((size_t)ie * npe + je) * bb;
+                if (ie == je) {
+                    double *out = dia + (size_t)i * bb;
+                    for (unsigned t = 0; t < bb; t++) out[t] += in[t];
+                } else {
+                    const unsigned j = no[je];
+                    if (pos[j] == -1) continue;
+                    double *out = val + (size_t)pos[j] * bb;
+                    for (unsigned t = 0; t < bb; t++) out[t] += in[t];
+                }
+            }
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) pos[col[q]] = -1;
+        }
+    }
+    free(pos);
+}
+
+/* include/delfem/matvec/vector_blk.h:170-181 (CVector_Blk::AddValue), called
+ * per element node in element order by the femeqn loops
+ * (e.g. src/femeqn/eqn_poisson.cpp:119-121). */
+void orc_assemble_vec(unsigned b, double *vec, unsigned nelem, unsigned npe,
+                      const unsigned *conn, const double *eres)
+{
+    for (unsigned e = 0; e < nelem; e++)
+        for (unsigned k = 0; k < npe; k++)
+            for (unsigned r = 0; r < b; r++)
+                vec[(size_t)conn[(size_t)e * npe + k] * b + r] += eres[((size_t)e * npe + k) * b + r];
+}
+
+/* ------------------------------------------------------------------------- */
+/* a6: src/matvec/matdia_blkcrs.cpp:261-343 (fixed-size branch :318-342).
+ * Rows with flag!=0: diagonal block row and column zeroed, diagonal entry 1,
+ * off-diagonal row zeroed.  Then every off-diagonal column with flag!=0 zeroed. */
+void orc_set_bc(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                double *val, double *dia, const int *flag)
+{
+    const unsigned bb = b * b;
+    for (unsigned i = 0; i < nb; i++) {
+        for (unsigned d = 0; d < b; d++) {
+            if (flag[(size_t)i * b + d] == 0) continue;
+            double *D = dia + (size_t)i * bb;
+            for (unsigned k = 0; k < b; k++) D[k * b + d] = 0.0;
+            for (unsigned k = 0; k < b; k++) D[d * b + k] = 0.0;
+            D[d * b + d] = 1.0;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++)
+                for (unsigned k = 0; k < b; k++) val[(size_t)q * bb + d * b + k] = 0.0;
+        }
+    }
+    const unsigned ncrs = rowptr[nb];
+    for (unsigned q = 0; q < ncrs; q++) {
+        const unsigned j = col[q];
+        for (unsigned d = 0; d < b; d++) {
+            if (flag[(size_t)j * b + d] == 0) continue;
+            for (unsigned k = 0; k < b; k++) val[(size_t)q * bb + k * b + d] = 0.0;
+        }
+    }
+}
+
+/* include/delfem/matvec/bcflag_blk.h:102-129 (fixed-size branch compares == 1). */
+void orc_zero_bc_dof(unsigned ndof, const int *flag, double *vec)
+{
+    for (unsigned i = 0; i < ndof; i++)
+        if (flag[i] == 1) vec[i] = 0.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a10: src/matvec/matdia_blkcrs.cpp:921-1094.  Per row: y *= beta, then each
+ * off-diagonal block in stored order, then the diagonal block.  For b=2,3 the
+ * block-row dot product is summed first and multiplied by alpha. */
+void orc_matvec(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                const double *val, const double *dia, double alpha, const double *x,
+                double beta, double *y)
+{
+    const unsigned bb = b * b;
+    for (unsigned i = 0; i < nb; i++) {
+        double *yi = y + (size_t)i * b;
+        for (unsigned r = 0; r < b; r++) yi[r] *= beta;
+        for (unsigned q = rowptr[i]; q <= rowptr[i + 1]; q++) {
+            const double *A; const double *xj;
+            if (q < rowptr[i + 1]) { A = val + (size_t)q * bb; xj = x + (size_t)col[q] * b; }
+            else { A = dia + (size_t)i * bb; xj = x + (size_t)i * b; }
+            if (b == 1) {
+                yi[0] += alpha * A[0] * xj[0];
+            } else if (b == 2) {
+                yi[0] += alpha * (A[0] * xj[0] + A[1] * xj[1]);
+                yi[1] += alpha * (A[2] * xj[0] + A[3] * xj[1]);
+            } else if (b == 3) {
+                yi[0] += alpha * (A[0] * xj[0] + A[1] * xj[1] + A[2] * xj[2]);
+                yi[1] += alpha * (A[3] * xj[0] + A[4] * xj[1] + A[5] * xj[2]);
+                yi[2] += alpha * (A[6] * xj[0] + A[7] * xj[1] + A[8] * xj[2]);
+            } else {
+                for (unsigned r = 0; r < b; r++)
+                    for (unsigned c = 0; c < b; c++) yi[r] += alpha * A[r * b + c] * xj[c];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* a7: ILU(k) symbolic.
+ *   natural pattern copy: src/matvec/matdiafrac_blkcrs.cpp:227-287, 1587-1638
+ *   ordered pattern:      src/matvec/mat_blkcrs.cpp:493-553 (rows renumbered,
+ *                         columns mapped old->new, each row sorted)
+ *   fill rule:            src/matvec/matdiafrac_blkcrs.cpp:1249-1377 with the
+ *                         sorted list of include/delfem/matvec/matdiafrac_blkcrs.h:185-301:
+ *   for row i, walk the (growing) sorted list; for each k<i with lev_ik+1<=K,
+ *   for each j in U(k) of the finished row k with lev_kj+1<=K, j!=i, insert j
+ *   with level max(lev_ik,lev_kj)+1 unless j is already present (first level
+ *   kept).  Special cases returning the input pattern unchanged: no
+ *   off-diagonal entries, a full matrix, K == 0 (:1260-1272).
+ *   diaind[i] = first position whose column exceeds i (end of row if none). */
+typedef struct { unsigned colv; int lev; int next; } orc_node;
+
+long orc_iluk_symbolic(unsigned nb, const unsigned *rowptr, const unsigned *col,
+                       int lev_fill, const int *order,
+                       unsigned *frowptr, unsigned *fcol, int *flev, unsigned *fdiaind,
+                       long cap)
+{
+    const unsigned ncrs0 = rowptr[nb];
+    unsigned *prow = (unsigned *)malloc(sizeof(unsigned) * ((size_t)nb + 1));
+    unsigned *pcol = (unsigned *)malloc(sizeof(unsigned) * ((size_t)ncrs0 + 1));
+    long result;
+    /* starting pattern (natural copy or reordered + sorted) */
+    if (order) {
+        int *inv = (int *)malloc(sizeof(int) * (size_t)nb);
+        for (unsigned i = 0; i < nb; i++) inv[order[i]] = (int)i;
+        prow[0] = 0;
+        for (unsigned inew = 0; inew < nb; inew++) {
+            const unsigned iold = (unsigned)order[inew];
+            unsigned n0 = prow[inew];
+            unsigned cnt = rowptr[iold + 1] - rowptr[iold];
+            for (unsigned t = 0; t < cnt; t++) pcol[n0 + t] = (unsigned)inv[col[rowptr[iold] + t]];
+            for (unsigned a = n0 + 1; a < n0 + cnt; a++) {
+                unsigned v = pcol[a]; long bq = (long)a - 1;
+                while (bq >= (long)n0 && pcol[bq] > v) { pcol[bq + 1] = pcol[bq]; bq--; }
+                pcol[bq + 1] = v;
+            }
+            prow[inew + 1] = n0 + cnt;
+        }
+        free(inv);
+    } else {
+        memcpy(prow, rowptr, sizeof(unsigned) * ((size_t)nb + 1));
+        memcpy(pcol, col, sizeof(unsigned) * (size_t)ncrs0);
+        for (unsigned i = 0; i < nb; i++)
+            for (unsigned q = prow[i] + 1; q < prow[i + 1]; q++)
+                if (pcol[q] <= pcol[q - 1]) { free(prow); free(pcol); return -1; }
+    }
+    const int unchanged = (ncrs0 == 0) || ((unsigned long)ncrs0 == (unsigned long)nb * (nb - 1)) || lev_fill == 0;
+    if (unchanged) {
+        if (cap < (long)ncrs0) { free(prow); free(pcol); return -(long)ncrs0 - 2; }
+        memcpy(frowptr, prow, sizeof(unsigned) * ((size_t)nb + 1));
+        memcpy(fcol, pcol, sizeof(unsigned) * (size_t)ncrs0);
+        for (unsigned q = 0; q < ncrs0; q++) flev[q] = 0;
+        for (unsigned i = 0; i < nb; i++) {
+            fdiaind[i] = prow[i + 1];
+            for (unsigned q = prow[i]; q < prow[i + 1]; q++)
+                if (pcol[q] > i) { fdiaind[i] = q; break; }
+        }
+        free(prow); free(pcol);
+        return (long)ncrs0;
+    }
+    /* growing output (may exceed cap: then only the size is reported) */
+    long outcap = (long)ncrs0 * 4 + 16;
+    unsigned *ocol = (unsigned *)malloc(sizeof(unsigned) * (size_t)outcap);
+    int *olev = (int *)malloc(sizeof(int) * (size_t)outcap);
+    unsigned *orow = (unsigned *)malloc(sizeof(unsigned) * ((size_t)nb + 1));
+    unsigned *odia = (unsigned *)malloc(sizeof(unsigned) * (size_t)nb);
+    orc_node *list = (orc_node *)malloc(sizeof(orc_node) * ((size_t)nb + 1));
+    long n = 0;
+    orow[0] = 0;
+    for (unsigned i = 0; i < nb; i++) {
+        /* list initialised from row i of the starting pattern, all level 0 */
+        int len = 0, head = -1;
+        const unsigned r0 = prow[i], r1 = prow[i + 1];
+        for (unsigned q = r0; q < r1; q++) {
+            list[len].colv = pcol[q]; list[len].lev = 0; list[len].next = len + 1; len++;
+        }
+        if (len > 0) { list[len - 1].next = -1; head = 0; }
+        for (int kc = head; kc != -1; kc = list[kc].next) {
+            const unsigned k = list[kc].colv;
+            const int lik = list[kc].lev;
+            if (lev_fill != -1 && lik + 1 > lev_fill) continue;
+            if (k >= i) break;
+            int cur = kc; /* insertion cursor: columns only grow along U(k) */
+            for (unsigned q = odia[k]; q < orow[k + 1]; q++) {
+                const int lkj = olev[q];
+                if (lev_fill != -1 && lkj + 1 > lev_fill) continue;
+                const unsigned j = ocol[q];
+                if (j == i) continue;
+                /* advance cursor to the last node with column < j */
+                for (;;) {
+                    const int nx = list[cur].next;
+                    if (nx == -1 || list[nx].colv >= j) break;
+                    cur = nx;
+                }
+                const int nx = list[cur].next;
+                if (nx != -1 && list[nx].colv == j) continue; /* present: first level kept */
+                list[len].colv = j;
+                list[len].lev = (lik > lkj ? lik : lkj) + 1;
+                list[len].next = nx;
+                list[cur].next = len;
+                cur = len;
+                len++;
+            }
+        }
+        /* finalize row i */
+        if (n + len > outcap) {
+            while (n + len > outcap) outcap *= 2;
+            ocol = (unsigned *)realloc(ocol, sizeof(unsigned) * (size_t)outcap);
+            olev = (int *)realloc(olev, sizeof(int) * (size_t)outcap);
+        }
+        const unsigned row_end = (unsigned)(n + len);
+        odia[i] = row_end;
+        for (int t = head; t != -1; t = list[t].next) {
+            if (list[t].colv > i && odia[i] == row_end) odia[i] = (unsigned)n;
+            ocol[n] = list[t].colv; olev[n] = list[t].lev; n++;
+        }
+        orow[i + 1] = (unsigned)n;
+    }
+    if (n > cap) {
+        result = -n - 2;
+    } else {
+        memcpy(frowptr, orow, sizeof(unsigned) * ((size_t)nb + 1));
+        memcpy(fcol, ocol, sizeof(unsigned) * (size_t)n);
+        memcpy(flev, olev, sizeof(int) * (size_t)n);
+        memcpy(fdiaind, odia, sizeof(unsigned) * (size_t)nb);
+        result = n;
+    }
+    free(ocol); free(olev); free(orow); free(odia); free(list); free(prow); free(pcol);
+    return result;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a8 (value copy): include/delfem/matvec/matdiafrac_blkcrs.h:90-117
+ * (SetValue_Initialize) -> src/matvec/mat_blkcrs.cpp:557-694 / 697-751 and
+ * src/matvec/matdia_blkcrs.cpp:673-761: zero the factor, copy each block of
+ * A whose (mapped) column is in the factor row, copy the diagonal blocks. */
+void orc_ilu_setvalue(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                      const double *val, const double *dia, const int *order,
+                      const unsigned *frowptr, const unsigned *fcol,
+                      double *fval, double *fdia)
+{
+    const unsigned bb = b * b;
+    int *inv = NULL;
+    int *pos = (int *)malloc(sizeof(int) * (size_t)nb);
+    if (order) {
+        inv = (int *)malloc(sizeof(int) * (size_t)nb);
+        for (unsigned i = 0; i < nb; i++) inv[order[i]] = (int)i;
+    }
+    memset(fval, 0, sizeof(double) * (size_t)frowptr[nb] * bb);
+    for (unsigned i = 0; i < nb; i++) pos[i] = -1;
+    for (unsigned i = 0; i < nb; i++) {
+        const unsigned io = order ? (unsigned)order[i] : i;
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = (int)q;
+        for (unsigned q = rowptr[io]; q < rowptr[io + 1]; q++) {
+            const unsigned j = inv ? (unsigned)inv[col[q]] : col[q];
+            const int p = pos[j];
+            if (p == -1) continue;
+            memcpy(fval + (size_t)p * bb, val + (size_t)q * bb, sizeof(double) * bb);
+        }
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = -1;
+        memcpy(fdia + (size_t)i * bb, dia + (size_t)io * bb, sizeof(double) * bb);
+    }
+    free(pos); free(inv);
+}
+
+/* src/matvec/matdiafrac_blkcrs.cpp:62-73 (CalcSubMatPr): out -= a*b, one product at a time */
+static void sub_mat_pr(double *out, const double *a, const double *bm, unsigned n)
+{
+    for (unsigned i = 0; i < n; i++)
+        for (unsigned j = 0; j < n; j++)
+            for (unsigned k = 0; k < n; k++) out[i * n + j] -= a[i * n + k] * bm[k * n + j];
+}
+/* :45-60 (CalcMatPr): out = d*out via a temporary */
+static void mat_pr(double *out, const double *d, double *tmp, unsigned n)
+{
+    for (unsigned i = 0; i < n; i++)
+        for (unsigned j = 0; j < n; j++) {
+            tmp[i * n + j] = 0.0;
+            for (unsigned k = 0; k < n; k++) tmp[i * n + j] += d[i * n + k] * out[k * n + j];
+        }
+    for (unsigned i = 0; i < n * n; i++) out[i] = tmp[i];
+}
+/* :107-136 (CalcInvMat): Gauss-Jordan without pivoting; info=1 on |a_ii|<1e-30 */
+static int inv_mat(double *a, unsigned n)
+{
+    for (unsigned i = 0; i < n; i++) {
+        if (fabs(a[i * n + i]) < 1.0e-30) return 1;
+        double t = 1.0 / a[i * n + i];
+        a[i * n + i] = 1.0;
+        for (unsigned k = 0; k < n; k++) a[i * n + k] *= t;
+        for (unsigned j = 0; j < n; j++) {
+            if (j == i) continue;
+            t = a[j * n + i];
+            a[j * n + i] = 0.0;
+            for (unsigned k = 0; k < n; k++) a[j * n + k] -= t * a[i * n + k];
+        }
+    }
+    return 0;
+}
+
+/* a8 (numeric): src/matvec/matdiafrac_blkcrs.cpp:769-1102 (fixed block sizes).
+ * Row-wise IKJ: A_ij -= A_ik * (D^-1 U)_kj for k in L(i) ascending and j in
+ * U(k); invert the diagonal block (left unchanged and counted when singular,
+ * failing after 10); scale the U row by D^-1. */
+int orc_ilu_decomp(unsigned nb, unsigned b, const unsigned *frowptr, const unsigned *fcol,
+                   const unsigned *fdiaind, double *fval, double *fdia)
+{
+    const unsigned bb = b * b;
+    int nsing = 0;
+    int *pos = (int *)malloc(sizeof(int) * (size_t)nb);
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)bb);
+    for (unsigned i = 0; i < nb; i++) pos[i] = -1;
+    for (unsigned i = 0; i < nb; i++) {
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = (int)q;
+        for (unsigned qk = frowptr[i]; qk < fdiaind[i]; qk++) {
+            const unsigned k = fcol[qk];
+            const double *Lik = fval + (size_t)qk * bb;
+            for (unsigned qj = fdiaind[k]; qj < frowptr[k + 1]; qj++) {
+                const unsigned j = fcol[qj];
+                const double *Ukj = fval + (size_t)qj * bb;
+                double *Aij;
+                if (j != i) {
+                    if (pos[j] == -1) continue;
+                    Aij = fval + (size_t)pos[j] * bb;
+                } else {
+                    Aij = fdia + (size_t)i * bb;
+                }
+                if (b == 1) {
+                    Aij[0] -= Lik[0] * Ukj[0];
+                } else if (b == 2) {
+                    Aij[0] -= Lik[0] * Ukj[0] + Lik[1] * Ukj[2];
+                    Aij[1] -= Lik[0] * Ukj[1] + Lik[1] * Ukj[3];
+                    Aij[2] -= Lik[2] * Ukj[0] + Lik[3] * Ukj[2];
+                    Aij[3] -= Lik[2] * Ukj[1] + Lik[3] * Ukj[3];
+                } else if (b == 3) {
+                    for (unsigned r = 0; r < 3; r++) {
+                        Aij[r * 3 + 0] -= Lik[r * 3 + 0] * Ukj[0] + Lik[r * 3 + 1] * Ukj[3] + Lik[r * 3 + 2] * Ukj[6];
+                        Aij[r * 3 + 1] -= Lik[r * 3 + 0] * Ukj[1] + Lik[r * 3 + 1] * Ukj[4] + Lik[r * 3 + 2] * Ukj[7];
+                        Aij[r * 3 + 2] -= Lik[r * 3 + 0] * Ukj[2] + Lik[r * 3 + 1] * Ukj[5] + Lik[r * 3 + 2] * Ukj[8];
+                    }
+                } else {
+                    sub_mat_pr(Aij, Lik, Ukj, b);
+                }
+            }
+        }
+        double *D = fdia + (size_t)i * bb;
+        int singular = 0;
+        if (b == 1) {
+            const double d = D[0];
+            if (fabs(d) > 1.0e-30) D[0] = 1.0 / d; else singular = 1;
+        } else if (b == 2) {
+            const double det = D[0] * D[3] - D[1] * D[2];
+            if (fabs(det) > 1.0e-30) {
+                const double id = 1.0 / det;
+                const double d0 = D[0];
+                D[0] = id * D[3];
+                D[1] = -id * D[1];
+                D[2] = -id * D[2];
+                D[3] = id * d0;
+            } else singular = 1;
+        } else if (b == 3) {
+            const double det = D[0] * D[4] * D[8] + D[3] * D[7] * D[2] + D[6] * D[1] * D[5]
+                             - D[0] * D[7] * D[5] - D[6] * D[4] * D[2] - D[3] * D[1] * D[8];
+            if (fabs(det) > 1.0e-30) {
+                /* :139-158 (CalcInvMat3) recomputes det in the same order */
+                double t[9];
+                for (int s = 0; s < 9; s++) t[s] = D[s];
+                const double id = 1.0 / det;
+                D[0] = id * (t[4] * t[8] - t[5] * t[7]);
+                D[1] = id * (t[2] * t[7] - t[1] * t[8]);
+                D[2] = id * (t[1] * t[5] - t[2] * t[4]);
+                D[3] = id * (t[5] * t[6] - t[3] * t[8]);
+                D[4] = id * (t[0] * t[8] - t[2] * t[6]);
+                D[5] = id * (t[2] * t[3] - t[0] * t[5]);
+                D[6] = id * (t[3] * t[7] - t[4] * t[6]);
+                D[7] = id * (t[1] * t[6] - t[0] * t[7]);
+                D[8] = id * (t[0] * t[4] - t[1] * t[3]);
+            } else singular = 1;
+        } else {
+            singular = inv_mat(D, b);
+        }
+        if (singular) {
+            nsing++;
+            if (nsing > 10) { free(pos); free(tmp); return -1; }
+        }
+        for (unsigned q = fdiaind[i]; q < frowptr[i + 1]; q++) {
+            double *U = fval + (size_t)q * bb;
+            if (b == 1) {
+                U[0] = U[0] * D[0];
+            } else if (b == 2) {
+                double t[4] = {U[0], U[1], U[2], U[3]};
+                U[0] = D[0] * t[0] + D[1] * t[2];
+                U[1] = D[0] * t[1] + D[1] * t[3];
+                U[2] = D[2] * t[0] + D[3] * t[2];
+                U[3] = D[2] * t[1] + D[3] * t[3];
+            } else if (b == 3) {
+                double t[9];
+                for (int s = 0; s < 9; s++) t[s] = U[s];
+                for (unsigned r = 0; r < 3; r++) {
+                    U[r * 3 + 0] = D[r * 3 + 0] * t[0] + D[r * 3 + 1] * t[3] + D[r * 3 + 2] * t[6];
+                    U[r * 3 + 1] = D[r * 3 + 0] * t[1] + D[r * 3 + 1] * t[4] + D[r * 3 + 2] * t[7];
+                    U[r * 3 + 2] = D[r * 3 + 0] * t[2] + D[r * 3 + 1] * t[5] + D[r * 3 + 2] * t[8];
+                }
+            } else {
+                mat_pr(U, D, tmp, b);
+            }
+        }
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = -1;
+    }
+    free(pos); free(tmp);
+    return nsing;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a9: src/matvec/matdiafrac_blkcrs.cpp:332-503 (forward) */
+void orc_ilu_forward(unsigned nb, unsigned b, const unsigned *frowptr, const unsigned *fcol,
+                     const unsigned *fdiaind, const double *fval, const double *fdia,
+                     double *vec)
+{
+    const unsigned bb = b * b;
+    double t[64];
+    for (unsigned i = 0; i < nb; i++) {
+        for (unsigned r = 0; r < b; r++) t[r] = vec[(size_t)i * b + r];
+        for (unsigned q = frowptr[i]; q < fdiaind[i]; q++) {
+            const double *L = fval + (size_t)q * bb;
+            const double *v = vec + (size_t)fcol[q] * b;
+            if (b == 1) {
+                t[0] -= L[0] * v[0];
+            } else if (b == 2) {
+                t[0] -= L[0] * v[0] + L[1] * v[1];
+                t[1] -= L[2] * v[0] + L[3] * v[1];
+            } else if (b == 3) {
+                t[0] -= L[0] * v[0] + L[1] * v[1] + L[2] * v[2];
+                t[1] -= L[3] * v[0] + L[4] * v[1] + L[5] * v[2];
+                t[2] -= L[6] * v[0] + L[7] * v[1] + L[8] * v[2];
+            } else {
+                for (unsigned r = 0; r < b; r++)
+                    for (unsigned c = 0; c < b; c++) t[r] -= L[r * b + c] * v[c];
+            }
+        }
+        const double *D = fdia + (size_t)i * bb;
+        double *o = vec + (size_t)i * b;
+        if (b == 1) {
+            o[0] = D[0] * t[0];
+        } else if (b == 2) {
+            o[0] = D[0] * t[0] + D[1] * t[1];
+            o[1] = D[2] * t[0] + D[3] * t[1];
+        } else if (b == 3) {
+            o[0] = D[0] * t[0] + D[1] * t[1] + D[2] * t[2];
+            o[1] = D[3] * t[0] + D[4] * t[1] + D[5] * t[2];
+            o[2] = D[6] * t[0] + D[7] * t[1] + D[8] * t[2];
+        } else {
+            for (unsigned r = 0; r < b; r++) {
+                double s = 0.0;
+                for (unsigned c = 0; c < b; c++) s += D[r * b + c] * t[c];
+                o[r] = s;
+            }
+        }
+    }
+}
+
+/* a9: src/matvec/matdiafrac_blkcrs.cpp:505-663 (backward) */
+void orc_ilu_backward(unsigned nb, unsigned b, const unsigned *frowptr, const unsigned *fcol,
+                      const unsigned *fdiaind, const double *fval, const double *fdia,
+                      double *vec)
+{
+    const unsigned bb = b * b;
+    (void)fdia;
+    double t[64];
+    for (long ii = (long)nb - 1; ii >= 0; ii--) {
+        const unsigned i = (unsigned)ii;
+        for (unsigned r = 0; r < b; r++) t[r] = vec[(size_t)i * b + r];
+        for (unsigned q = fdiaind[i]; q < frowptr[i + 1]; q++) {
+            const double *U = fval + (size_t)q * bb;
+            const double *v = vec + (size_t)fcol[q] * b;
+            if (b == 1) {
+                t[0] -= U[0] * v[0];
+            } else if (b == 2) {
+                t[0] -= U[0] * v[0] + U[1] * v[1];
+                t[1] -= U[2] * v[0] + U[3] * v[1];
+            } else if (b == 3) {
+                t[0] -= U[0] * v[0] + U[1] * v[1] + U[2] * v[2];
+                t[1] -= U[3] * v[0] + U[4] * v[1] + U[5] * v[2];
+                t[2] -= U[6] * v[0] + U[7] * v[1] + U[8] * v[2];
+            } else {
+                for (unsigned r = 0; r < b; r++)
+                    for (unsigned c = 0; c < b; c++) t[r] -= U[r * b + c] * v[c];
+            }
+        }
+        for (unsigned r = 0; r < b; r++) vec[(size_t)i * b + r] = t[r];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Solver plumbing: src/ls/linearsystem.cpp:250-415 and src/matvec/vector_blk.cpp:37-112 */
+typedef struct {
+    unsigned nb, b;
+    const unsigned *rowptr, *col; const double *val, *dia;
+    const unsigned *frowptr, *fcol, *fdiaind; const double *fval, *fdia;
+    const int *order; int *inv; double *pv;
+} orc_sys;
+
+static double v_dot(size_t n, const double *a, const double *c)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < n; i++) s += a[i] * c[i];
+    return s;
+}
+/* CLinearSystem::AXPY returns without touching y when alpha == 0 (:312) */
+static void v_axpy(size_t n, double alpha, const double *x, double *y)
+{
+    if (alpha == 0) return;
+    for (size_t i = 0; i < n; i++) y[i] += alpha * x[i];
+}
+static void v_scal(size_t n, double alpha, double *x)
+{
+    for (size_t i = 0; i < n; i++) x[i] *= alpha;
+}
+/* src/ls/preconditioner.cpp:301-335 (SolvePrecond; ordered branch :305-312) */
+static void precond(orc_sys *s, double *z)
+{
+    const unsigned b = s->b;
+    if (s->order) {
+        for (unsigned i = 0; i < s->nb; i++)
+            for (unsigned r = 0; r < b; r++) s->pv[(size_t)i * b + r] = z[(size_t)s->order[i] * b + r];
+        orc_ilu_forward(s->nb, b, s->frowptr, s->fcol, s->fdiaind, s->fval, s->fdia, s->pv);
+        orc_ilu_backward(s->nb, b, s->frowptr, s->fcol, s->fdiaind, s->fval, s->fdia, s->pv);
+        for (unsigned i = 0; i < s->nb; i++)
+            for (unsigned r = 0; r < b; r++) z[(size_t)i * b + r] = s->pv[(size_t)s->inv[i] * b + r];
+    } else {
+        orc_ilu_forward(s->nb, b, s->frowptr, s->fcol, s->fdiaind, s->fval, s->fdia, z);
+        orc_ilu_backward(s->nb, b, s->frowptr, s->fcol, s->fdiaind, s->fval, s->fdia, z);
+    }
+}
+static void sys_init(orc_sys *s, unsigned nb, unsigned b,
+                     const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                     const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                     const double *fval, const double *fdia, const int *order)
+{
+    s->nb = nb; s->b = b; s->rowptr = rowptr; s->col = col; s->val = val; s->dia = dia;
+    s->frowptr = frowptr; s->fcol = fcol; s->fdiaind = fdiaind; s->fval = fval; s->fdia = fdia;
+    s->order = order; s->inv = NULL; s->pv = NULL;
+    if (order) {
+        s->inv = (int *)malloc(sizeof(int) * (size_t)nb);
+        for (unsigned i = 0; i < nb; i++) s->inv[order[i]] = (int)i;
+        s->pv = (double *)malloc(sizeof(double) * (size_t)nb * b);
+    }
+}
+static void sys_free(orc_sys *s) { free(s->inv); free(s->pv); }
+static void matvec_ls(orc_sys *s, double alpha, const double *x, double beta, double *y)
+{
+    if (alpha == 0.0) { v_scal((size_t)s->nb * s->b, beta, y); return; }   /* :270-273 */
+    orc_matvec(s->nb, s->b, s->rowptr, s->col, s->val, s->dia, alpha, x, beta, y);
+}
+
+/* a12: src/ls/solver_ls_iter.cpp:100-172 */
+int orc_solve_pcg(unsigned nb, unsigned b,
+                  const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                  const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                  const double *fval, const double *fdia, const int *order,
+                  double *r, double *x, double *conv_ratio, unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = (size_t)nb * b;
+    orc_sys s;
+    sys_init(&s, nb, b, rowptr, col, val, dia, frowptr, fcol, fdiaind, fval, fdia, order);
+    double *p = (double *)calloc(n, sizeof(double));
+    double *z = (double *)calloc(n, sizeof(double));
+    int ret = 0;
+    v_scal(n, 0.0, x);
+    double inv_rr0;
+    {
+        const double rr0 = v_dot(n, r, r);
+        if (rr0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; ret = 1; goto done; }
+        inv_rr0 = 1.0 / rr0;
+    }
+    memcpy(z, r, n * sizeof(double));
+    precond(&s, z);
+    memcpy(p, z, n * sizeof(double));
+    double rz = v_dot(n, r, z);
+    for (unsigned it = 0; it < mx_iter; it++) {
+        double alpha;
+        matvec_ls(&s, 1.0, p, 0.0, z);
+        {
+            const double pAp = v_dot(n, p, z);
+            alpha = rz / pAp;
+        }
+        v_axpy(n, -alpha, z, r);
+        v_axpy(n, alpha, p, x);
+        {
+            const double rr = v_dot(n, r, r);
+            if (rr * inv_rr0 < tol * tol) {
+                *conv_ratio = sqrt(rr * inv_rr0);
+                *iteration = it;
+                ret = 1;
+                goto done;
+            }
+        }
+        double beta;
+        {
+            memcpy(z, r, n * sizeof(double));
+            precond(&s, z);
+            const double rz_new = v_dot(n, r, z);
+            beta = rz_new / rz;
+            rz = rz_new;
+        }
+        v_scal(n, beta, p);
+        v_axpy(n, 1.0, z, p);
+    }
+    {
+        const double rr = v_dot(n, r, r);
+        *conv_ratio = sqrt(rr * inv_rr0);
+        ret = 0;
+    }
+done:
+    free(p); free(z); sys_free(&s);
+    return ret;
+}
+
+/* a13: src/ls/solver_ls_iter.cpp:291-408 */
+int orc_solve_pbicgstab(unsigned nb, unsigned b,
+                        const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                        const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                        const double *fval, const double *fdia, const int *order,
+                        double *r, double *x, double *conv_ratio, unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned max_iter = *iteration;
+    const size_t n = (size_t)nb * b;
+    orc_sys s;
+    sys_init(&s, nb, b, rowptr, col, val, dia, frowptr, fcol, fdiaind, fval, fdia, order);
+    double *vs = (double *)calloc(n, sizeof(double));
+    double *Ms = (double *)calloc(n, sizeof(double));
+    double *AMs = (double *)calloc(n, sizeof(double));
+    double *p = (double *)calloc(n, sizeof(double));
+    double *Mp = (double *)calloc(n, sizeof(double));
+    double *AMp = (double *)calloc(n, sizeof(double));
+    double *r2 = (double *)calloc(n, sizeof(double));
+    double inv_rr0;
+    {
+        const double rr0 = v_dot(n, r, r);
+        if (rr0 < 1.0e-60) { *conv_ratio = sqrt(rr0); *iteration = 0; goto done; }
+        inv_rr0 = 1.0 / rr0;
+    }
+    v_scal(n, 0.0, x);
+    memcpy(r2, r, n * sizeof(double));
+    memcpy(p, r, n * sizeof(double));
+    *iteration = max_iter;
+    for (unsigned it = 1; it < max_iter; it++) {
+        memcpy(Mp, p, n * sizeof(double));
+        precond(&s, Mp);
+        const double r_r2 = v_dot(n, r, r2);
+        matvec_ls(&s, 1.0, Mp, 0.0, AMp);
+        double alpha;
+        {
+            const double den = v_dot(n, AMp, r2);
+            alpha = r_r2 / den;
+        }
+        memcpy(vs, r, n * sizeof(double));
+        v_axpy(n, -alpha, AMp, vs);
+        memcpy(Ms, vs, n * sizeof(double));
+        precond(&s, Ms);
+        matvec_ls(&s, 1.0, Ms, 0.0, AMs);
+        double omega;
+        {
+            const double den = v_dot(n, AMs, AMs);
+            const double num = v_dot(n, vs, AMs);
+            omega = num / den;
+        }
+        v_axpy(n, alpha, Mp, x);
+        v_axpy(n, omega, Ms, x);
+        memcpy(r, vs, n * sizeof(double));
+        v_axpy(n, -omega, AMs, r);
+        {
+            const double rr = v_dot(n, r, r);
+            const double sq = rr * inv_rr0;
+            if (sq < tol * tol) {
+                *conv_ratio = sqrt(rr * inv_rr0);
+                *iteration = it;
+                goto done;
+            }
+        }
+        double beta;
+        {
+            const double tmp1 = v_dot(n, r, r2);
+            beta = tmp1 * alpha / (r_r2 * omega);
+        }
+        v_scal(n, beta, p);
+        v_axpy(n, 1.0, r, p);
+        v_axpy(n, -beta * omega, AMp, p);
+    }
+done:
+    free(vs); free(Ms); free(AMs); free(p); free(Mp); free(AMp); free(r2); sys_free(&s);
+    return 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* a4: element kernels.
+ * src/femeqn/ker_emat_tri.cpp:12-14 (TriArea), 36-69 (TriDlDx) */
+static double tri_area(const double *p0, const double *p1, const double *p2)
+{
+    return 0.5 * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+static void tri_dldx(double dldx[3][2], const double *p0, const double *p1, const double *p2)
+{
+    const double area = tri_area(p0, p1, p2);
+    const double t = 0.5 / area;
+    dldx[0][0] = t * (p1[1] - p2[1]);
+    dldx[1][0] = t * (p2[1] - p0[1]);
+    dldx[2][0] = t * (p0[1] - p1[1]);
+    dldx[0][1] = t * (p2[0] - p1[0]);
+    dldx[1][1] = t * (p0[0] - p2[0]);
+    dldx[2][1] = t * (p1[0] - p0[0]);
+}
+/* include/delfem/femeqn/ker_emat_tet.h:32-77 (TetVolume scales by 0.1666666666666) */
+static double tet_volume(const double *p0, const double *p1, const double *p2, const double *p3)
+{
+    double v = (p1[0] - p0[0]) * ((p2[1] - p0[1]) * (p3[2] - p0[2]) - (p3[1] - p0[1]) * (p2[2] - p0[2]))
+             - (p1[1] - p0[1]) * ((p2[0] - p0[0]) * (p3[2] - p0[2]) - (p3[0] - p0[0]) * (p2[2] - p0[2]))
+             + (p1[2] - p0[2]) * ((p2[0] - p0[0]) * (p3[1] - p0[1]) - (p3[0] - p0[0]) * (p2[1] - p0[1]));
+    v *= 0.1666666666666;
+    return v;
+}
+static void tet_dldx(double d[4][3], const double *p0, const double *p1, const double *p2, const double *p3)
+{
+    const double vol = tet_volume(p0, p1, p2, p3);
+    const double t = 1.0 / (vol * 6.0);
+    d[0][0] = -t * ((p2[1] - p1[1]) * (p3[2] - p1[2]) - (p3[1] - p1[1]) * (p2[2] - p1[2]));
+    d[0][1] = +t * ((p2[0] - p1[0]) * (p3[2] - p1[2]) - (p3[0] - p1[0]) * (p2[2] - p1[2]));
+    d[0][2] = -t * ((p2[0] - p1[0]) * (p3[1] - p1[1]) - (p3[0] - p1[0]) * (p2[1] - p1[1]));
+    d[1][0] = +t * ((p3[1] - p2[1]) * (p0[2] - p2[2]) - (p0[1] - p2[1]) * (p3[2] - p2[2]));
+    d[1][1] = -t * ((p3[0] - p2[0]) * (p0[2] - p2[2]) - (p0[0] - p2[0]) * (p3[2] - p2[2]));
+    d[1][2] = +t * ((p3[0] - p2[0]) * (p0[1] - p2[1]) - (p0[0] - p2[0]) * (p3[1] - p2[1]));
+    d[2][0] = -t * ((p0[1] - p3[1]) * (p1[2] - p3[2]) - (p1[1] - p3[1]) * (p0[2] - p3[2]));
+    d[2][1] = +t * ((p0[0] - p3[0]) * (p1[2] - p3[2]) - (p1[0] - p3[0]) * (p0[2] - p3[2]));
+    d[2][2] = -t * ((p0[0] - p3[0]) * (p1[1] - p3[1]) - (p1[0] - p3[0]) * (p0[1] - p3[1]));
+    d[3][0] = +t * ((p1[1] - p0[1]) * (p2[2] - p0[2]) - (p2[1] - p0[1]) * (p1[2] - p0[2]));
+    d[3][1] = -t * ((p1[0] - p0[0]) * (p2[2] - p0[2]) - (p2[0] - p0[0]) * (p1[2] - p0[2]));
+    d[3][2] = +t * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+
+/* src/femeqn/eqn_poisson.cpp:49-125 (AddLinearSystem_Poisson2D_P1), per-element alpha */
+void orc_emat_poisson2d_p1(unsigned nelem, const double *coord, const unsigned *conn,
+                           const double *alpha, double source, const double *value,
+                           double *emat, double *eres)
+{
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *no = conn + (size_t)e * 3;
+        const double *p0 = coord + (size_t)no[0] * 2, *p1 = coord + (size_t)no[1] * 2, *p2 = coord + (size_t)no[2] * 2;
+        const double area = tri_area(p0, p1, p2);
+        double dl[3][2];
+        tri_dldx(dl, p0, p1, p2);
+        double *em = emat + (size_t)e * 9;
+        double *er = eres + (size_t)e * 3;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                em[i * 3 + j] = alpha[e] * area * (dl[i][0] * dl[j][0] + dl[i][1] * dl[j][1]);
+        for (int i = 0; i < 3; i++) er[i] = source * area * 0.33333333333333333;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) er[i] -= em[i * 3 + j] * value[no[j]];
+    }
+}
+
+/* src/femeqn/eqn_linear_solid3d.cpp:241-334 (AddLinSys_LinearSolid3D_Static_P1),
+ * per-element lambda and myu. emat layout [ino][jno][x][y]. */
+void orc_emat_solid3d_p1(unsigned nelem, const double *coord, const unsigned *conn,
+                         const double *lambda, const double *myu, double rho,
+                         double gx, double gy, double gz, const double *disp,
+                         double *emat, double *eres)
+{
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *no = conn + (size_t)e * 4;
+        const double *p[4];
+        for (int a = 0; a < 4; a++) p[a] = coord + (size_t)no[a] * 3;
+        const double vol = tet_volume(p[0], p[1], p[2], p[3]);
+        double dl[4][3];
+        tet_dldx(dl, p[0], p[1], p[2], p[3]);
+        const double lam = lambda[e], mu = myu[e];
+        double *em = emat + (size_t)e * 144;
+        double *er = eres + (size_t)e * 12;
+        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 4; c++) {
+                double *E = em + (a * 4 + c) * 9;
+                for (int x = 0; x < 3; x++)
+                    for (int y = 0; y < 3; y++)
+                        E[x * 3 + y] = vol * (lam * dl[a][x] * dl[c][y] + mu * dl[c][x] * dl[a][y]);
+                const double d = dl[a][0] * dl[c][0] + dl[a][1] * dl[c][1] + dl[a][2] * dl[c][2];
+                E[0] += vol * mu * d;
+                E[4] += vol * mu * d;
+                E[8] += vol * mu * d;
+            }
+        for (int a = 0; a < 4; a++) {
+            er[a * 3 + 0] = vol * rho * gx * 0.25;
+            er[a * 3 + 1] = vol * rho * gy * 0.25;
+            er[a * 3 + 2] = vol * rho * gz * 0.25;
+            for (int c = 0; c < 4; c++) {
+                const double *E = em + (a * 4 + c) * 9;
+                const double *u = disp + (size_t)no[c] * 3;
+                er[a * 3 + 0] -= E[0] * u[0] + E[1] * u[1] + E[2] * u[2];
+                er[a * 3 + 1] -= E[3] * u[0] + E[4] * u[1] + E[5] * u[2];
+                er[a * 3 + 2] -= E[6] * u[0] + E[7] * u[1] + E[8] * u[2];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Multicolour ordering (product algorithm; no reference counterpart):
+ * greedy colouring in natural order, each node takes the smallest colour not
+ * used by an already-coloured neighbour. */
+int orc_greedy_colour(unsigned nb, const unsigned *rowptr, const unsigned *col, int *colour)
+{
+    int ncol = 0;
+    int *stamp = (int *)malloc(sizeof(int) * ((size_t)nb + 2));
+    for (unsigned i = 0; i < nb + 2; i++) stamp[i] = -1;
+    for (unsigned i = 0; i < nb; i++) colour[i] = -1;
+    for (unsigned i = 0; i < nb; i++) {
+        for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+            const int c = colour[col[q]];
+            if (c >= 0) stamp[c] = (int)i;
+        }
+        int c = 0;
+        while (stamp[c] == (int)i) c++;
+        colour[i] = c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    free(stamp);
+    return ncol;
+}
+
+/* order[new] = old: nodes sorted by colour, natural order within a colour */
+void orc_colour_order(unsigned nb, const int *colour, int ncolour, int *order)
+{
+    unsigned n = 0;
+    for (int c = 0; c < ncolour; c++)
+        for (unsigned i = 0; i < nb; i++)
+            if (colour[i] == c) order[n++] = (int)i;
+}
+
+/* Level sets of the factor DAG.  lower=1: level(i) = 1 + max level(j), j in
+ * L(i) (0 when L(i) is empty), rows taken in increasing order.  lower=0:
+ * the same over U(i), rows taken in decreasing order.  Returns #levels. */
+int orc_level_sets(unsigned nb, const unsigned *frowptr, const unsigned *fcol,
+                   const unsigned *fdiaind, int lower, int *level)
+{
+    int nlev = 0;
+    if (lower) {
+        for (unsigned i = 0; i < nb; i++) {
+            int l = 0;
+            for (unsigned q = frowptr[i]; q < fdiaind[i]; q++) {
+                const int t = level[fcol[q]] + 1;
+                if (t > l) l = t;
+            }
+            level[i] = l;
+            if (l + 1 > nlev) nlev = l + 1;
+        }
+    } else {
+        for (long ii = (long)nb - 1; ii >= 0; ii--) {
+            const unsigned i = (unsigned)ii;
+            int l = 0;
+            for (unsigned q = fdiaind[i]; q < frowptr[i + 1]; q++) {
+                const int t = level[fcol[q]] + 1;
+                if (t > l) l = t;
+            }
+            level[i] = l;
+            if (l + 1 > nlev) nlev = l + 1;
+        }
+    }
+    return nlev;
+}

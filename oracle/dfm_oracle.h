@@ -1,0 +1,120 @@
+/*
+ * dfm_oracle.h -- CPU restatement of DelFEM's linear-solve hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in delfem_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links or calls
+ * it, and it is never the thing measured as the GPU result.
+ *
+ * Every function restates one reference routine, cited as file:line relative
+ * to the DelFEM source tree.  Storage follows the reference exactly:
+ *   rowptr[nb+1]  -- the reference's m_colInd_Blk (row pointer)
+ *   col[ncrs]     -- the reference's m_rowPtr_Blk (column index), diagonal excluded
+ *   val[ncrs*b*b] -- off-diagonal blocks, row-major b x b
+ *   dia[nb*b*b]   -- diagonal blocks, row-major b x b
+ * Orderings use the reference convention order[new] = old
+ * (src/matvec/ordering_blk.cpp:46-60).
+ *
+ * Pinning status of each function is recorded in the header comment of
+ * dfm_oracle.c.
+ */
+#ifndef DFM_ORACLE_H
+#define DFM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* a1: node adjacency from element connectivity, self excluded, rows ascending.
+ * Returns ncrs.  col may be NULL (count only); cap is the capacity of col. */
+long orc_fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned *conn,
+                     unsigned *rowptr, unsigned *col, long cap);
+
+/* a5: Mearge of nelem element matrices in element order.
+ * emat: nelem * (npe*npe) blocks of b*b, layout [(i*npe+j)*b*b + r*b + c]. */
+void orc_assemble(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                  double *val, double *dia, unsigned nelem, unsigned npe,
+                  const unsigned *conn, const double *emat);
+/* residual merge: CVector_Blk::AddValue per element node in element order. */
+void orc_assemble_vec(unsigned b, double *vec, unsigned nelem, unsigned npe,
+                      const unsigned *conn, const double *eres);
+
+/* a6: CMatDia_BlkCrs::SetBoundaryCondition and CBCFlag::SetZeroToBCDof. */
+void orc_set_bc(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                double *val, double *dia, const int *flag);
+void orc_zero_bc_dof(unsigned ndof, const int *flag, double *vec);
+
+/* a10: y = beta*y + alpha*A*x (CMatDia_BlkCrs::MatVec). */
+void orc_matvec(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                const double *val, const double *dia, double alpha, const double *x,
+                double beta, double *y);
+
+/* a7: ILU(k) symbolic pattern (natural or ordered).  lev_fill = -1 means
+ * unlimited.  order may be NULL.  Returns ncrs of the factor, or -(needed)
+ * when cap is too small, or -1 on bad input (unsorted row). */
+long orc_iluk_symbolic(unsigned nb, const unsigned *rowptr, const unsigned *col,
+                       int lev_fill, const int *order,
+                       unsigned *frowptr, unsigned *fcol, int *flev, unsigned *fdiaind,
+                       long cap);
+
+/* a8: copy A (optionally reordered) into the factor pattern, then factor.
+ * orc_ilu_decomp returns the number of singular pivots (0..10), or -1 when
+ * more than 10 were met (the reference stops and returns false). */
+void orc_ilu_setvalue(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                      const double *val, const double *dia, const int *order,
+                      const unsigned *frowptr, const unsigned *fcol,
+                      double *fval, double *fdia);
+int orc_ilu_decomp(unsigned nb, unsigned b, const unsigned *frowptr, const unsigned *fcol,
+                   const unsigned *fdiaind, double *fval, double *fdia);
+
+/* a9: in-place substitutions. */
+void orc_ilu_forward(unsigned nb, unsigned b, const unsigned *frowptr, const unsigned *fcol,
+                     const unsigned *fdiaind, const double *fval, const double *fdia,
+                     double *vec);
+void orc_ilu_backward(unsigned nb, unsigned b, const unsigned *frowptr, const unsigned *fcol,
+                      const unsigned *fdiaind, const double *fval, const double *fdia,
+                      double *vec);
+
+/* a12/a13: LsSol::Solve_PCG and LsSol::Solve_PBiCGSTAB on a one-segment
+ * CLinearSystem with CPreconditioner_ILU (optionally ordered).
+ * r: in = right-hand side (residual vector, id -1), out = final residual.
+ * x: out = update vector (id -2).  conv_ratio and iteration in/out as in the
+ * reference.  Returns 1 (true) / 0 (false) like the reference. */
+int orc_solve_pcg(unsigned nb, unsigned b,
+                  const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                  const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                  const double *fval, const double *fdia, const int *order,
+                  double *r, double *x, double *conv_ratio, unsigned *iteration);
+int orc_solve_pbicgstab(unsigned nb, unsigned b,
+                        const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                        const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                        const double *fval, const double *fdia, const int *order,
+                        double *r, double *x, double *conv_ratio, unsigned *iteration);
+
+/* a4: element kernels (per element arrays). */
+void orc_emat_poisson2d_p1(unsigned nelem, const double *coord, const unsigned *conn,
+                           const double *alpha, double source, const double *value,
+                           double *emat, double *eres);
+void orc_emat_solid3d_p1(unsigned nelem, const double *coord, const unsigned *conn,
+                         const double *lambda, const double *myu, double rho,
+                         double gx, double gy, double gz, const double *disp,
+                         double *emat, double *eres);
+/* positive Helmholtz / implicit diffusion form K*alpha + c*M with P1 consistent mass */
+void orc_emat_helmholtz2d_p1(unsigned nelem, const double *coord, const unsigned *conn,
+                             const double *alpha, const double *c, double source,
+                             const double *value, double *emat, double *eres);
+/* Oseen P1-P1 with PSPG, combined (u,v,p) 3x3 blocks */
+void orc_emat_oseen2d_p1p1(unsigned nelem, const double *coord, const unsigned *conn,
+                           double myu, double rho, const double *adv /* nnode*2 */,
+                           const double *value /* nnode*3 */, double *emat, double *eres);
+
+/* Orderings for the multicolour path (product algorithm, restated). */
+int orc_greedy_colour(unsigned nb, const unsigned *rowptr, const unsigned *col, int *colour);
+void orc_colour_order(unsigned nb, const int *colour, int ncolour, int *order);
+int orc_level_sets(unsigned nb, const unsigned *frowptr, const unsigned *fcol,
+                   const unsigned *fdiaind, int lower, int *level);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,417 @@
+"""ctypes access to the checkers -- TEST INFRASTRUCTURE ONLY.
+
+  Orc  -- liborc.so, the C restatement of the reference hot path (dfm_oracle.c)
+  Ref  -- _ref/libdfmref.so, the DelFEM reference's own sources compiled by
+          oracle/Makefile, driven through ref_driver.cpp (exists only where
+          the DelFEM tree was available at build time)
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module.  The product (delfem_amd/) never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORC_PATH = os.path.join(HERE, "liborc.so")
+REF_PATH = os.path.join(HERE, "_ref", "libdfmref.so")
+REF_TREE = os.environ.get("DELFEM_REF", "/root/reference")
+
+vp = C.c_void_p
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def build(ref=True):
+    """Compile liborc.so (always) and the reference library (when the tree exists)."""
+    targets = ["oracle"]
+    if ref and os.path.isdir(os.path.join(REF_TREE, "src", "matvec")):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE, f"REF={REF_TREE}", "-j8", *targets], check=True)
+
+
+def u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Orc:
+    """The C restatement.  Methods mirror dfm_oracle.h with numpy arrays."""
+
+    def __init__(self, path=ORC_PATH):
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run oracle.build()")
+        L = C.CDLL(path)
+        L.orc_fem_pattern.restype = C.c_long
+        L.orc_fem_pattern.argtypes = [C.c_uint, C.c_uint, C.c_uint, vp, vp, vp, C.c_long]
+        L.orc_assemble.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, C.c_uint, C.c_uint, vp, vp]
+        L.orc_assemble_vec.argtypes = [C.c_uint, vp, C.c_uint, C.c_uint, vp, vp]
+        L.orc_set_bc.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp]
+        L.orc_zero_bc_dof.argtypes = [C.c_uint, vp, vp]
+        L.orc_matvec.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, C.c_double, vp, C.c_double, vp]
+        L.orc_iluk_symbolic.restype = C.c_long
+        L.orc_iluk_symbolic.argtypes = [C.c_uint, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_long]
+        L.orc_ilu_setvalue.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_ilu_decomp.restype = C.c_int
+        L.orc_ilu_decomp.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp]
+        L.orc_ilu_forward.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp]
+        L.orc_ilu_backward.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp]
+        for f in (L.orc_solve_pcg, L.orc_solve_pbicgstab):
+            f.restype = C.c_int
+            f.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                          C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_emat_poisson2d_p1.argtypes = [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]
+        L.orc_emat_solid3d_p1.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, vp, vp, vp]
+        L.orc_greedy_colour.restype = C.c_int
+        L.orc_greedy_colour.argtypes = [C.c_uint, vp, vp, vp]
+        L.orc_colour_order.argtypes = [C.c_uint, vp, C.c_int, vp]
+        L.orc_level_sets.restype = C.c_int
+        L.orc_level_sets.argtypes = [C.c_uint, vp, vp, vp, C.c_int, vp]
+        self.L = L
+
+    # -- patterns ---------------------------------------------------------
+    def fem_pattern(self, nnode, conn):
+        conn = u32(conn)
+        nelem, npe = conn.shape
+        rowptr = np.zeros(nnode + 1, dtype=np.uint32)
+        n = self.L.orc_fem_pattern(nnode, nelem, npe, _p(conn), _p(rowptr), None, 0)
+        col = np.zeros(max(1, n), dtype=np.uint32)
+        self.L.orc_fem_pattern(nnode, nelem, npe, _p(conn), _p(rowptr), _p(col), n)
+        return rowptr, col[:n]
+
+    def iluk_symbolic(self, rowptr, col, lev, order=None):
+        rowptr, col = u32(rowptr), u32(col)
+        nb = rowptr.size - 1
+        order = None if order is None else i32(order)
+        cap = max(16, col.size * 4)
+        while True:
+            frow = np.zeros(nb + 1, dtype=np.uint32)
+            fcol = np.zeros(cap, dtype=np.uint32)
+            flev = np.zeros(cap, dtype=np.int32)
+            fdia = np.zeros(max(1, nb), dtype=np.uint32)
+            n = self.L.orc_iluk_symbolic(nb, _p(rowptr), _p(col), lev, _p(order), _p(frow), _p(fcol), _p(flev),
+                                         _p(fdia), cap)
+            if n == -1:
+                raise ValueError("unsorted pattern row")
+            if n >= 0:
+                return dict(rowptr=frow, col=fcol[:n], lev=flev[:n], diaind=fdia[:nb])
+            cap = -n
+
+    # -- assembly ---------------------------------------------------------
+    def assemble(self, nb, b, rowptr, col, conn, emat, val=None, dia=None):
+        rowptr, col, conn, emat = u32(rowptr), u32(col), u32(conn), f64(emat)
+        val = np.zeros(col.size * b * b) if val is None else f64(val).copy()
+        dia = np.zeros(nb * b * b) if dia is None else f64(dia).copy()
+        self.L.orc_assemble(nb, b, _p(rowptr), _p(col), _p(val), _p(dia), conn.shape[0], conn.shape[1], _p(conn),
+                            _p(emat))
+        return val, dia
+
+    def assemble_vec(self, nb, b, conn, eres, vec=None):
+        conn, eres = u32(conn), f64(eres)
+        vec = np.zeros(nb * b) if vec is None else f64(vec).copy()
+        self.L.orc_assemble_vec(b, _p(vec), conn.shape[0], conn.shape[1], _p(conn), _p(eres))
+        return vec
+
+    def set_bc(self, nb, b, rowptr, col, val, dia, flag):
+        val, dia = f64(val).copy(), f64(dia).copy()
+        self.L.orc_set_bc(nb, b, _p(u32(rowptr)), _p(u32(col)), _p(val), _p(dia), _p(i32(flag)))
+        return val, dia
+
+    def zero_bc(self, flag, vec):
+        vec = f64(vec).copy()
+        self.L.orc_zero_bc_dof(vec.size, _p(i32(flag)), _p(vec))
+        return vec
+
+    def matvec(self, nb, b, rowptr, col, val, dia, alpha, x, beta, y):
+        y = f64(y).copy()
+        self.L.orc_matvec(nb, b, _p(u32(rowptr)), _p(u32(col)), _p(f64(val)), _p(f64(dia)), alpha, _p(f64(x)), beta,
+                          _p(y))
+        return y
+
+    # -- ILU ----------------------------------------------------------------
+    def ilu_values(self, nb, b, rowptr, col, val, dia, F, order=None):
+        """SetValue_Initialize + DoILUDecomp; returns (fval, fdia, nsing)."""
+        nf = F["col"].size
+        fval = np.zeros(nf * b * b)
+        fdia = np.zeros(nb * b * b)
+        order = None if order is None else i32(order)
+        self.L.orc_ilu_setvalue(nb, b, _p(u32(rowptr)), _p(u32(col)), _p(f64(val)), _p(f64(dia)), _p(order),
+                                _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(fval), _p(fdia))
+        ns = self.L.orc_ilu_decomp(nb, b, _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(u32(F["diaind"])), _p(fval),
+                                   _p(fdia))
+        return fval, fdia, ns
+
+    def forward(self, nb, b, F, fval, fdia, vec):
+        vec = f64(vec).copy()
+        self.L.orc_ilu_forward(nb, b, _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(u32(F["diaind"])), _p(f64(fval)),
+                               _p(f64(fdia)), _p(vec))
+        return vec
+
+    def backward(self, nb, b, F, fval, fdia, vec):
+        vec = f64(vec).copy()
+        self.L.orc_ilu_backward(nb, b, _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(u32(F["diaind"])),
+                                _p(f64(fval)), _p(f64(fdia)), _p(vec))
+        return vec
+
+    def solve(self, kind, nb, b, rowptr, col, val, dia, F, fval, fdia, rhs, tol, maxit, order=None):
+        """Solve_PCG / Solve_PBiCGSTAB: returns (ok, conv, iter, x, r_final)."""
+        r = f64(rhs).copy()
+        x = np.zeros(nb * b)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        order = None if order is None else i32(order)
+        f = self.L.orc_solve_pcg if kind == "pcg" else self.L.orc_solve_pbicgstab
+        ok = f(nb, b, _p(u32(rowptr)), _p(u32(col)), _p(f64(val)), _p(f64(dia)), _p(u32(F["rowptr"])),
+               _p(u32(F["col"])), _p(u32(F["diaind"])), _p(f64(fval)), _p(f64(fdia)), _p(order), _p(r), _p(x),
+               C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, x, r
+
+    # -- elements ------------------------------------------------------------
+    def emat_poisson2d(self, coord, conn, alpha, source, value):
+        conn = u32(conn)
+        ne = conn.shape[0]
+        emat = np.zeros(ne * 9)
+        eres = np.zeros(ne * 3)
+        self.L.orc_emat_poisson2d_p1(ne, _p(f64(coord)), _p(conn), _p(f64(alpha)), source, _p(f64(value)), _p(emat),
+                                     _p(eres))
+        return emat, eres
+
+    def emat_solid3d(self, coord, conn, lam, mu, rho, g, disp):
+        conn = u32(conn)
+        ne = conn.shape[0]
+        emat = np.zeros(ne * 144)
+        eres = np.zeros(ne * 12)
+        self.L.orc_emat_solid3d_p1(ne, _p(f64(coord)), _p(conn), _p(f64(lam)), _p(f64(mu)), rho, g[0], g[1], g[2],
+                                   _p(f64(disp)), _p(emat), _p(eres))
+        return emat, eres
+
+    # -- orderings -------------------------------------------------------------
+    def colour_order(self, rowptr, col):
+        rowptr, col = u32(rowptr), u32(col)
+        nb = rowptr.size - 1
+        colour = np.zeros(nb, dtype=np.int32)
+        nc = self.L.orc_greedy_colour(nb, _p(rowptr), _p(col), _p(colour))
+        order = np.zeros(nb, dtype=np.int32)
+        self.L.orc_colour_order(nb, _p(colour), nc, _p(order))
+        return colour, nc, order
+
+    def level_sets(self, F, lower):
+        nb = F["rowptr"].size - 1
+        lev = np.zeros(nb, dtype=np.int32)
+        n = self.L.orc_level_sets(nb, _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(u32(F["diaind"])),
+                                  1 if lower else 0, _p(lev))
+        return lev, n
+
+
+class Ref:
+    """The compiled DelFEM reference (ref_driver.cpp entry points)."""
+
+    def __init__(self, path=REF_PATH):
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: the reference library is built where the DelFEM tree exists")
+        L = C.CDLL(path)
+        sig = {
+            "ref_quiet": (None, [C.c_int]),
+            "ref_sys_create": (vp, [C.c_uint, C.c_uint, vp, vp]),
+            "ref_sys_free": (None, [vp]),
+            "ref_sys_initialize_marge": (None, [vp]),
+            "ref_sys_ncrs": (C.c_uint, [vp]),
+            "ref_sys_merge": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
+            "ref_sys_add_residual": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
+            "ref_sys_set_bc": (None, [vp, vp]),
+            "ref_sys_finalize": (C.c_double, [vp]),
+            "ref_sys_get_matrix": (None, [vp, vp, vp, vp, vp]),
+            "ref_sys_set_matrix": (None, [vp, vp, vp]),
+            "ref_sys_resize_tmp": (None, [vp, C.c_uint]),
+            "ref_sys_set_vector": (None, [vp, C.c_int, vp]),
+            "ref_sys_get_vector": (None, [vp, C.c_int, vp]),
+            "ref_sys_matvec": (None, [vp, C.c_double, C.c_int, C.c_double, C.c_int]),
+            "ref_prec_create": (vp, [vp, C.c_int, vp]),
+            "ref_prec_free": (None, [vp]),
+            "ref_prec_set_value": (C.c_int, [vp, vp]),
+            "ref_prec_solve": (None, [vp, vp, C.c_int]),
+            "ref_solve_pcg": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_solve_pbicgstab": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_frac_create": (vp, [vp, C.c_int, vp]),
+            "ref_frac_free": (None, [vp]),
+            "ref_frac_ncrs": (C.c_uint, [vp]),
+            "ref_frac_get_pattern": (None, [vp, vp, vp, vp]),
+            "ref_frac_set_value": (C.c_int, [vp, vp]),
+            "ref_frac_get_values": (None, [vp, vp, vp]),
+            "ref_frac_forward": (None, [vp, vp]),
+            "ref_frac_backward": (None, [vp, vp]),
+            "ref_frac_solve": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_emat_poisson2d_p1": (None, [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]),
+            "ref_emat_solid3d_p1": (None, [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, vp, vp, vp]),
+        }
+        for k, (res, args) in sig.items():
+            f = getattr(L, k)
+            f.restype = res
+            f.argtypes = args
+        L.ref_quiet(1)
+        self.L = L
+
+
+class RefSystem:
+    """One LsSol::CLinearSystem of the reference with a single segment."""
+
+    def __init__(self, ref, nb, b, index, array):
+        self.ref, self.nb, self.b = ref, nb, b
+        self._index, self._array = u32(index), u32(array)
+        if self._array.size == 0:
+            self._array = np.zeros(1, dtype=np.uint32)
+        self.h = ref.L.ref_sys_create(nb, b, _p(self._index), _p(self._array))
+        ref.L.ref_sys_initialize_marge(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.ref.L.ref_sys_free(self.h)
+            self.h = None
+
+    def merge(self, conn, emat, eres=None):
+        conn = u32(conn)
+        self.ref.L.ref_sys_merge(self.h, conn.shape[0], conn.shape[1], _p(conn), _p(f64(emat)))
+        if eres is not None:
+            self.ref.L.ref_sys_add_residual(self.h, conn.shape[0], conn.shape[1], _p(conn), _p(f64(eres)))
+
+    def set_bc(self, flags):
+        self.ref.L.ref_sys_set_bc(self.h, _p(i32(flags)))
+
+    def finalize(self):
+        return self.ref.L.ref_sys_finalize(self.h)
+
+    def matrix(self):
+        n = self.ref.L.ref_sys_ncrs(self.h)
+        bb = self.b * self.b
+        rowptr = np.zeros(self.nb + 1, dtype=np.uint32)
+        col = np.zeros(max(1, n), dtype=np.uint32)
+        val = np.zeros(max(1, n * bb))
+        dia = np.zeros(self.nb * bb)
+        self.ref.L.ref_sys_get_matrix(self.h, _p(rowptr), _p(col), _p(val), _p(dia))
+        return rowptr, col[:n], val[:n * bb], dia
+
+    def set_matrix(self, val, dia):
+        self.ref.L.ref_sys_set_matrix(self.h, _p(f64(val)), _p(f64(dia)))
+
+    def set_vector(self, iv, v):
+        self.ref.L.ref_sys_set_vector(self.h, iv, _p(f64(v)))
+
+    def get_vector(self, iv):
+        out = np.zeros(self.nb * self.b)
+        self.ref.L.ref_sys_get_vector(self.h, iv, _p(out))
+        return out
+
+    def matvec(self, alpha, iv1, beta, iv2):
+        self.ref.L.ref_sys_matvec(self.h, alpha, iv1, beta, iv2)
+
+    def resize_tmp(self, n):
+        self.ref.L.ref_sys_resize_tmp(self.h, n)
+
+
+class RefILU:
+    """LsSol::CPreconditioner_ILU of the reference."""
+
+    def __init__(self, sysobj, lev, order=None):
+        self.s = sysobj
+        self._order = None if order is None else i32(order)
+        self.h = sysobj.ref.L.ref_prec_create(sysobj.h, lev, _p(self._order))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.s.ref.L.ref_prec_free(self.h)
+            self.h = None
+
+    def set_value(self):
+        return bool(self.s.ref.L.ref_prec_set_value(self.h, self.s.h))
+
+    def solve(self, kind, tol, maxit):
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        f = self.s.ref.L.ref_solve_pcg if kind == "pcg" else self.s.ref.L.ref_solve_pbicgstab
+        ok = f(self.s.h, self.h, C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value
+
+
+class RefFrac:
+    """MatVec::CMatDiaFrac_BlkCrs of the reference (natural or ordered ILU(k))."""
+
+    def __init__(self, sysobj, lev, order=None):
+        self.s = sysobj
+        self._order = None if order is None else i32(order)
+        self.h = sysobj.ref.L.ref_frac_create(sysobj.h, lev, _p(self._order))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.s.ref.L.ref_frac_free(self.h)
+            self.h = None
+
+    def pattern(self):
+        L = self.s.ref.L
+        n = L.ref_frac_ncrs(self.h)
+        nb = self.s.nb
+        rowptr = np.zeros(nb + 1, dtype=np.uint32)
+        col = np.zeros(max(1, n), dtype=np.uint32)
+        dia = np.zeros(nb, dtype=np.uint32)
+        L.ref_frac_get_pattern(self.h, _p(rowptr), _p(col), _p(dia))
+        return dict(rowptr=rowptr, col=col[:n], diaind=dia)
+
+    def set_value(self):
+        return bool(self.s.ref.L.ref_frac_set_value(self.h, self.s.h))
+
+    def values(self):
+        L = self.s.ref.L
+        n = L.ref_frac_ncrs(self.h)
+        bb = self.s.b * self.s.b
+        val = np.zeros(max(1, n * bb))
+        dia = np.zeros(self.s.nb * bb)
+        L.ref_frac_get_values(self.h, _p(val), _p(dia))
+        return val[:n * bb], dia
+
+    def forward(self, vec):
+        v = f64(vec).copy()
+        self.s.ref.L.ref_frac_forward(self.h, _p(v))
+        return v
+
+    def backward(self, vec):
+        v = f64(vec).copy()
+        self.s.ref.L.ref_frac_backward(self.h, _p(v))
+        return v
+
+    def solve(self, kind, tol, maxit):
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.s.ref.L.ref_frac_solve(self.h, self.s.h, 0 if kind == "pcg" else 1, C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value
+
+
+def problem_emats(orc, prob, value=None):
+    """Element matrices / residuals of a meshgen Problem via the C restatement."""
+    if prob.kind == "poisson2d":
+        v = np.zeros(prob.nnode) if value is None else value
+        return orc.emat_poisson2d(prob.coord, prob.conn, prob.alpha, prob.source, v)
+    v = np.zeros(prob.nnode * 3) if value is None else value
+    return orc.emat_solid3d(prob.coord, prob.conn, prob.lam, prob.mu, prob.rho, prob.g, v)
+
+
+def assemble_problem(orc, prob):
+    """Oracle pipeline: pattern, element loop, Mearge, BC, residual (CPU)."""
+    rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)
+    emat, eres = problem_emats(orc, prob)
+    val, dia = orc.assemble(prob.nnode, prob.b, rowptr, col, prob.conn, emat)
+    res = orc.assemble_vec(prob.nnode, prob.b, prob.conn, eres)
+    val, dia = orc.set_bc(prob.nnode, prob.b, rowptr, col, val, dia, prob.flags)
+    res = orc.zero_bc(prob.flags, res)
+    return dict(rowptr=rowptr, col=col, val=val, dia=dia, res=res, emat=emat, eres=eres)

@@ -1,0 +1,367 @@
+// ref_driver.cpp -- C entry points over the compiled DelFEM reference.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles the reference's own
+// matvec/ls sources from the DelFEM tree together with this driver into
+// oracle/_ref/libdfmref.so.  The library is used to (1) pin the C restatement
+// in dfm_oracle.c, (2) write the golden fixtures in tests/golden, and (3) time
+// the reference CPU solver for bench.py's cpu_baseline.  The product never
+// links it.
+//
+// Reference classes driven here (DelFEM include tree):
+//   LsSol::CLinearSystem        include/delfem/ls/linearsystem.h:51-174
+//   LsSol::CPreconditioner_ILU  include/delfem/ls/preconditioner.h:59-108
+//   LsSol::Solve_PCG / Solve_PBiCGSTAB  include/delfem/ls/solver_ls_iter.h:35-45
+//   MatVec::CMatDiaFrac_BlkCrs  include/delfem/matvec/matdiafrac_blkcrs.h:40-137
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "delfem/indexed_array.h"
+#include "delfem/matvec/matdia_blkcrs.h"
+#include "delfem/matvec/matdiafrac_blkcrs.h"
+#include "delfem/matvec/vector_blk.h"
+#include "delfem/matvec/bcflag_blk.h"
+#include "delfem/matvec/ordering_blk.h"
+#include "delfem/ls/linearsystem.h"
+#include "delfem/ls/preconditioner.h"
+#include "delfem/ls/solver_ls_iter.h"
+#include "delfem/femeqn/ker_emat_tri.h"
+#include "delfem/femeqn/ker_emat_tet.h"
+
+namespace {
+
+struct RefSys {
+  LsSol::CLinearSystem ls;
+  unsigned nb = 0, b = 0;
+};
+
+// Factor object with access to the protected pattern arrays.
+class FracPeek : public MatVec::CMatDiaFrac_BlkCrs {
+ public:
+  FracPeek(int lev, const MatVec::CMatDia_BlkCrs& A) : MatVec::CMatDiaFrac_BlkCrs(lev, A) {}
+  FracPeek(int lev, const MatVec::CMatDia_BlkCrs& A, const MatVec::COrdering_Blk& o)
+      : MatVec::CMatDiaFrac_BlkCrs(lev, A, o) {}
+  const unsigned* RowPtr() const { return m_colInd_Blk; }
+  const unsigned* Col() const { return m_rowPtr_Blk; }
+  const unsigned* DiaInd() const { return m_DiaInd; }
+  const double* Val() const { return m_valCrs_Blk; }
+  const double* Dia() const { return m_valDia_Blk; }
+};
+
+struct RefFrac {
+  FracPeek* f = nullptr;
+  MatVec::COrdering_Blk order;
+  bool ordered = false;
+  unsigned nb = 0, b = 0;
+};
+
+// Preconditioner over a (possibly ordered) ILU(k) factor of the matrix layer:
+// the ls-layer CPreconditioner_ILU honours an ordering for ILU(0) only
+// (src/ls/preconditioner.cpp:60-87), this one uses CMatDiaFrac_BlkCrs(lev,A,order).
+class FracPrecond : public LsSol::CPreconditioner {
+ public:
+  explicit FracPrecond(RefFrac* rf) : rf_(rf), tmp_(rf->nb, rf->b) {}
+  void SetLinearSystem(const LsSol::CLinearSystem&) override {}
+  bool SetValue(const LsSol::CLinearSystem& ls) override {
+    if (rf_->ordered) return rf_->f->SetValue(ls.GetMatrix(0), rf_->order);
+    return rf_->f->SetValue(ls.GetMatrix(0));
+  }
+  bool SolvePrecond(LsSol::CLinearSystem& ls, unsigned int iv) override {
+    if (rf_->ordered) {
+      rf_->order.OrderingVector_OldToNew(tmp_, ls.GetVector(iv, 0));
+      rf_->f->ForwardSubstitution(tmp_);
+      rf_->f->BackwardSubstitution(tmp_);
+      rf_->order.OrderingVector_NewToOld(ls.GetVector(iv, 0), tmp_);
+    } else {
+      rf_->f->ForwardSubstitution(ls.GetVector(iv, 0));
+      rf_->f->BackwardSubstitution(ls.GetVector(iv, 0));
+    }
+    return true;
+  }
+
+ private:
+  RefFrac* rf_;
+  MatVec::CVector_Blk tmp_;
+};
+
+std::ofstream& null_stream() {
+  static std::ofstream s("/dev/null");
+  return s;
+}
+
+void copy_vec_in(MatVec::CVector_Blk& v, unsigned nb, unsigned b, const double* src) {
+  for (unsigned i = 0; i < nb; i++)
+    for (unsigned r = 0; r < b; r++) v.SetValue(i, r, src[(size_t)i * b + r]);
+}
+void copy_vec_out(const MatVec::CVector_Blk& v, unsigned nb, unsigned b, double* dst) {
+  for (unsigned i = 0; i < nb; i++)
+    for (unsigned r = 0; r < b; r++) dst[(size_t)i * b + r] = v.GetValue(i, r);
+}
+
+}  // namespace
+
+extern "C" {
+
+// The reference prints progress text to std::cout on the hot path
+// (e.g. src/matvec/matdiafrac_blkcrs.cpp:1364); route it to /dev/null.
+void ref_quiet(int on) {
+  static std::streambuf* saved = nullptr;
+  if (on) {
+    if (!saved) saved = std::cout.rdbuf(null_stream().rdbuf());
+  } else if (saved) {
+    std::cout.rdbuf(saved);
+    saved = nullptr;
+  }
+}
+
+// ---- linear system ---------------------------------------------------------
+void* ref_sys_create(unsigned nb, unsigned b, const unsigned* index, const unsigned* array) {
+  RefSys* s = new RefSys;
+  s->nb = nb;
+  s->b = b;
+  Com::CIndexedArray crs;
+  crs.InitializeSize(nb);
+  for (unsigned i = 0; i <= nb; i++) crs.index[i] = index[i];
+  crs.array.assign(array, array + index[nb]);
+  s->ls.AddLinSysSeg(nb, b);
+  s->ls.AddMat_Dia(0, crs);
+  return s;
+}
+void ref_sys_free(void* p) { delete static_cast<RefSys*>(p); }
+void ref_sys_initialize_marge(void* p) { static_cast<RefSys*>(p)->ls.InitializeMarge(); }
+unsigned ref_sys_ncrs(void* p) { return static_cast<RefSys*>(p)->ls.GetMatrix(0).NCrs(); }
+
+void ref_sys_merge(void* p, unsigned nelem, unsigned npe, const unsigned* conn, const double* emat) {
+  RefSys* s = static_cast<RefSys*>(p);
+  MatVec::CMatDia_BlkCrs& A = s->ls.GetMatrix(0);
+  const unsigned bb = s->b * s->b;
+  for (unsigned e = 0; e < nelem; e++)
+    A.Mearge(npe, conn + (size_t)e * npe, npe, conn + (size_t)e * npe, bb, emat + (size_t)e * npe * npe * bb);
+}
+void ref_sys_add_residual(void* p, unsigned nelem, unsigned npe, const unsigned* conn, const double* eres) {
+  RefSys* s = static_cast<RefSys*>(p);
+  MatVec::CVector_Blk& r = s->ls.GetVector(-1, 0);
+  for (unsigned e = 0; e < nelem; e++)
+    for (unsigned k = 0; k < npe; k++)
+      for (unsigned d = 0; d < s->b; d++)
+        r.AddValue(conn[(size_t)e * npe + k], d, eres[((size_t)e * npe + k) * s->b + d]);
+}
+void ref_sys_set_bc(void* p, const int* flags) {
+  RefSys* s = static_cast<RefSys*>(p);
+  for (unsigned i = 0; i < s->nb; i++)
+    for (unsigned d = 0; d < s->b; d++)
+      if (flags[(size_t)i * s->b + d]) s->ls.GetBCFlag(0).SetBC(i, d);
+}
+double ref_sys_finalize(void* p) { return static_cast<RefSys*>(p)->ls.FinalizeMarge(); }
+
+void ref_sys_get_matrix(void* p, unsigned* rowptr, unsigned* col, double* val, double* dia) {
+  RefSys* s = static_cast<RefSys*>(p);
+  const MatVec::CMatDia_BlkCrs& A = s->ls.GetMatrix(0);
+  const unsigned bb = s->b * s->b;
+  unsigned n = 0;
+  rowptr[0] = 0;
+  for (unsigned i = 0; i < s->nb; i++) {
+    unsigned np = 0;
+    const unsigned* ind = A.GetPtrIndPSuP(i, np);
+    unsigned nv = 0;
+    const double* v = A.GetPtrValPSuP(i, nv);
+    for (unsigned k = 0; k < np; k++) {
+      col[n + k] = ind[k];
+      std::memcpy(val + (size_t)(n + k) * bb, v + (size_t)k * bb, sizeof(double) * bb);
+    }
+    n += np;
+    rowptr[i + 1] = n;
+    std::memcpy(dia + (size_t)i * bb, A.GetPtrValDia(i), sizeof(double) * bb);
+  }
+}
+void ref_sys_set_matrix(void* p, const double* val, const double* dia) {
+  RefSys* s = static_cast<RefSys*>(p);
+  MatVec::CMatDia_BlkCrs& A = s->ls.GetMatrix(0);
+  const unsigned bb = s->b * s->b;
+  unsigned n = 0;
+  for (unsigned i = 0; i < s->nb; i++) {
+    unsigned np = 0;
+    double* v = A.GetPtrValPSuP(i, np);
+    std::memcpy(v, val + (size_t)n * bb, sizeof(double) * bb * np);
+    n += np;
+    std::memcpy(A.GetPtrValDia(i), dia + (size_t)i * bb, sizeof(double) * bb);
+  }
+}
+void ref_sys_resize_tmp(void* p, unsigned ntmp) { static_cast<RefSys*>(p)->ls.ReSizeTmpVecSolver(ntmp); }
+void ref_sys_set_vector(void* p, int iv, const double* src) {
+  RefSys* s = static_cast<RefSys*>(p);
+  copy_vec_in(s->ls.GetVector(iv, 0), s->nb, s->b, src);
+}
+void ref_sys_get_vector(void* p, int iv, double* dst) {
+  RefSys* s = static_cast<RefSys*>(p);
+  copy_vec_out(s->ls.GetVector(iv, 0), s->nb, s->b, dst);
+}
+void ref_sys_matvec(void* p, double alpha, int iv1, double beta, int iv2) {
+  static_cast<RefSys*>(p)->ls.MATVEC(alpha, iv1, beta, iv2);
+}
+
+// ---- ls-layer ILU preconditioner + Krylov ---------------------------------
+void* ref_prec_create(void* p, int lev, const int* order) {
+  RefSys* s = static_cast<RefSys*>(p);
+  LsSol::CPreconditioner_ILU* pr = new LsSol::CPreconditioner_ILU;
+  pr->SetFillInLevel(lev);
+  if (order) pr->SetOrdering(std::vector<int>(order, order + s->nb));
+  pr->SetLinearSystem(s->ls);
+  return pr;
+}
+void ref_prec_free(void* q) { delete static_cast<LsSol::CPreconditioner_ILU*>(q); }
+int ref_prec_set_value(void* q, void* p) {
+  return static_cast<LsSol::CPreconditioner_ILU*>(q)->SetValue(static_cast<RefSys*>(p)->ls) ? 1 : 0;
+}
+void ref_prec_solve(void* q, void* p, int iv) {
+  static_cast<LsSol::CPreconditioner_ILU*>(q)->SolvePrecond(static_cast<RefSys*>(p)->ls, iv);
+}
+int ref_solve_pcg(void* p, void* q, double* conv, unsigned* iter) {
+  LsSol::CLinearSystemPreconditioner lsp(static_cast<RefSys*>(p)->ls, *static_cast<LsSol::CPreconditioner_ILU*>(q));
+  return LsSol::Solve_PCG(*conv, *iter, lsp) ? 1 : 0;
+}
+int ref_solve_pbicgstab(void* p, void* q, double* conv, unsigned* iter) {
+  LsSol::CLinearSystemPreconditioner lsp(static_cast<RefSys*>(p)->ls, *static_cast<LsSol::CPreconditioner_ILU*>(q));
+  return LsSol::Solve_PBiCGSTAB(*conv, *iter, lsp) ? 1 : 0;
+}
+
+// ---- matrix-layer factor (natural or ordered ILU(k)) ----------------------
+void* ref_frac_create(void* p, int lev, const int* order) {
+  RefSys* s = static_cast<RefSys*>(p);
+  RefFrac* rf = new RefFrac;
+  rf->nb = s->nb;
+  rf->b = s->b;
+  if (order) {
+    rf->ordered = true;
+    rf->order.SetOrdering(std::vector<int>(order, order + s->nb));
+    rf->f = new FracPeek(lev, s->ls.GetMatrix(0), rf->order);
+  } else {
+    rf->f = new FracPeek(lev, s->ls.GetMatrix(0));
+  }
+  return rf;
+}
+void ref_frac_free(void* q) {
+  RefFrac* rf = static_cast<RefFrac*>(q);
+  delete rf->f;
+  delete rf;
+}
+unsigned ref_frac_ncrs(void* q) { return static_cast<RefFrac*>(q)->f->NCrs(); }
+void ref_frac_get_pattern(void* q, unsigned* rowptr, unsigned* col, unsigned* diaind) {
+  RefFrac* rf = static_cast<RefFrac*>(q);
+  std::memcpy(rowptr, rf->f->RowPtr(), sizeof(unsigned) * (rf->nb + 1));
+  std::memcpy(col, rf->f->Col(), sizeof(unsigned) * rf->f->NCrs());
+  std::memcpy(diaind, rf->f->DiaInd(), sizeof(unsigned) * rf->nb);
+}
+int ref_frac_set_value(void* q, void* p) {
+  RefFrac* rf = static_cast<RefFrac*>(q);
+  RefSys* s = static_cast<RefSys*>(p);
+  if (rf->ordered) {
+    rf->f->SetValue_Initialize(s->ls.GetMatrix(0), rf->order);
+  } else {
+    rf->f->SetValue_Initialize(s->ls.GetMatrix(0));
+  }
+  return rf->f->DoILUDecomp() ? 1 : 0;
+}
+void ref_frac_get_values(void* q, double* val, double* dia) {
+  RefFrac* rf = static_cast<RefFrac*>(q);
+  const unsigned bb = rf->b * rf->b;
+  std::memcpy(val, rf->f->Val(), sizeof(double) * bb * rf->f->NCrs());
+  std::memcpy(dia, rf->f->Dia(), sizeof(double) * bb * rf->nb);
+}
+void ref_frac_forward(void* q, double* vec) {
+  RefFrac* rf = static_cast<RefFrac*>(q);
+  MatVec::CVector_Blk v(rf->nb, rf->b);
+  copy_vec_in(v, rf->nb, rf->b, vec);
+  rf->f->ForwardSubstitution(v);
+  copy_vec_out(v, rf->nb, rf->b, vec);
+}
+void ref_frac_backward(void* q, double* vec) {
+  RefFrac* rf = static_cast<RefFrac*>(q);
+  MatVec::CVector_Blk v(rf->nb, rf->b);
+  copy_vec_in(v, rf->nb, rf->b, vec);
+  rf->f->BackwardSubstitution(v);
+  copy_vec_out(v, rf->nb, rf->b, vec);
+}
+// Krylov solve preconditioned by the matrix-layer factor (ordered ILU(k)).
+// kind 0 = Solve_PCG, 1 = Solve_PBiCGSTAB.  The factor must already hold values.
+int ref_frac_solve(void* q, void* p, int kind, double* conv, unsigned* iter) {
+  RefFrac* rf = static_cast<RefFrac*>(q);
+  RefSys* s = static_cast<RefSys*>(p);
+  FracPrecond prec(rf);
+  LsSol::CLinearSystemPreconditioner lsp(s->ls, prec);
+  bool ok = kind == 0 ? LsSol::Solve_PCG(*conv, *iter, lsp) : LsSol::Solve_PBiCGSTAB(*conv, *iter, lsp);
+  return ok ? 1 : 0;
+}
+
+// ---- element matrices via the reference kernels ----------------------------
+// Loop bodies follow src/femeqn/eqn_poisson.cpp:99-117 and
+// src/femeqn/eqn_linear_solid3d.cpp:285-320 using the reference's
+// TriArea/TriDlDx (src/femeqn/ker_emat_tri.cpp) and TetVolume/TetDlDx
+// (include/delfem/femeqn/ker_emat_tet.h).
+void ref_emat_poisson2d_p1(unsigned nelem, const double* coord, const unsigned* conn, const double* alpha,
+                           double source, const double* value, double* emat, double* eres) {
+  for (unsigned e = 0; e < nelem; e++) {
+    const unsigned* no = conn + (size_t)e * 3;
+    double c[3][2];
+    for (int k = 0; k < 3; k++) {
+      c[k][0] = coord[no[k] * 2];
+      c[k][1] = coord[no[k] * 2 + 1];
+    }
+    const double area = TriArea(c[0], c[1], c[2]);
+    double dldx[3][2], ct[3];
+    TriDlDx(dldx, ct, c[0], c[1], c[2]);
+    double* em = emat + (size_t)e * 9;
+    double* er = eres + (size_t)e * 3;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        em[i * 3 + j] = alpha[e] * area * (dldx[i][0] * dldx[j][0] + dldx[i][1] * dldx[j][1]);
+    for (int i = 0; i < 3; i++) er[i] = source * area * 0.33333333333333333;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) er[i] -= em[i * 3 + j] * value[no[j]];
+  }
+}
+void ref_emat_solid3d_p1(unsigned nelem, const double* coord, const unsigned* conn, const double* lambda,
+                         const double* myu, double rho, double gx, double gy, double gz, const double* disp,
+                         double* emat, double* eres) {
+  for (unsigned e = 0; e < nelem; e++) {
+    const unsigned* no = conn + (size_t)e * 4;
+    double c[4][3], u[4][3];
+    for (int k = 0; k < 4; k++)
+      for (int d = 0; d < 3; d++) {
+        c[k][d] = coord[no[k] * 3 + d];
+        u[k][d] = disp[no[k] * 3 + d];
+      }
+    const double vol = TetVolume(c[0], c[1], c[2], c[3]);
+    double dldx[4][3], zt[4];
+    TetDlDx(dldx, zt, c[0], c[1], c[2], c[3]);
+    const double lam = lambda[e], mu = myu[e];
+    double em[4][4][3][3];
+    for (int a = 0; a < 4; a++)
+      for (int bq = 0; bq < 4; bq++) {
+        for (int x = 0; x < 3; x++)
+          for (int y = 0; y < 3; y++)
+            em[a][bq][x][y] = vol * (lam * dldx[a][x] * dldx[bq][y] + mu * dldx[bq][x] * dldx[a][y]);
+        const double d = dldx[a][0] * dldx[bq][0] + dldx[a][1] * dldx[bq][1] + dldx[a][2] * dldx[bq][2];
+        em[a][bq][0][0] += vol * mu * d;
+        em[a][bq][1][1] += vol * mu * d;
+        em[a][bq][2][2] += vol * mu * d;
+      }
+    double er[4][3];
+    for (int a = 0; a < 4; a++) {
+      er[a][0] = vol * rho * gx * 0.25;
+      er[a][1] = vol * rho * gy * 0.25;
+      er[a][2] = vol * rho * gz * 0.25;
+      for (int bq = 0; bq < 4; bq++) {
+        er[a][0] -= em[a][bq][0][0] * u[bq][0] + em[a][bq][0][1] * u[bq][1] + em[a][bq][0][2] * u[bq][2];
+        er[a][1] -= em[a][bq][1][0] * u[bq][0] + em[a][bq][1][1] * u[bq][1] + em[a][bq][1][2] * u[bq][2];
+        er[a][2] -= em[a][bq][2][0] * u[bq][0] + em[a][bq][2][1] * u[bq][1] + em[a][bq][2][2] * u[bq][2];
+      }
+    }
+    std::memcpy(emat + (size_t)e * 144, &em[0][0][0][0], sizeof(double) * 144);
+    std::memcpy(eres + (size_t)e * 12, &er[0][0], sizeof(double) * 12);
+  }
+}
+
+}  // extern "C"
