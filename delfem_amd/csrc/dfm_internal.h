@@ -127,8 +127,8 @@ struct KState {
   double alpha, beta, omega, r_r2, rz_new;
   double d0, d1, d2;    // generic reduction outputs
   double conv_rr;       // rr at convergence
-  int done, iter, conv_iter, pad;
-  unsigned tickets[8];
+  int done;             // 0 running, 1 converged (recursive residual), 2 stopped at maxit
+  int iter, conv_iter, maxit;
 };
 
 }  // namespace dfm
@@ -185,4 +185,10 @@ struct dfm_ilu_s {
   dfm::DevBuf<double> partials;
   dfm::DevBuf<dfm::KState> state;
   dfm::KState* h_state = nullptr;       // pinned mirror
+  // instantiated hipGraphs of blocks of Krylov iterations, keyed by their operands
+  std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+  ~dfm_ilu_s() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    if (h_state) (void)hipHostFree(h_state);
+  }
 };

@@ -18,64 +18,100 @@ constexpr int kNT = 256;             // threads per workgroup
 constexpr int kWaves = kNT / 64;     // wavefronts (= slices) per workgroup
 
 // ---------------------------------------------------------------------------
-// Deterministic reductions
+// Deterministic two-phase reductions.  Every workgroup writes its partial sums
+// with plain stores; a one-workgroup finalize kernel launched next adds them in
+// index order and applies the Krylov epilogue.  The kernel boundary makes the
+// partials visible, so no device-scope fence (an L2 write-back on a
+// multi-XCD part) is needed per workgroup.
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
 }
 
-// Sum of v over the workgroup, valid in thread 0.
-__device__ inline double block_sum(double v, double* sh) {
+// Sum over a workgroup of NT threads, valid in thread 0.
+template <int NT>
+__device__ inline double block_sum_n(double v, double* sh) {
   v = wave_sum(v);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) sh[w] = v;
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < kWaves; i++) s += sh[i];
+    for (int i = 0; i < NT / 64; i++) s += sh[i];
   __syncthreads();
   return s;
 }
 
-__device__ inline void store_agent(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline double load_agent(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Each workgroup contributes ND partial sums; the workgroup that finishes last
-// (ticket == nparts-1) adds all partials in index order and returns true in
-// every thread with out[] valid in thread 0.  partials: ND*nparts doubles.
+// partials[d*nparts + part] = workgroup sum of v[d]
 template <int ND>
-__device__ inline bool finish_sums(double (&v)[ND], double* partials, int part, int nparts,
-                                   unsigned* ticket, double (&out)[ND]) {
+__device__ inline void write_partials(const double (&v)[ND], double* partials, int part, int nparts) {
   __shared__ double sh[kWaves];
-  __shared__ int amlast;
 #pragma unroll
   for (int d = 0; d < ND; d++) {
-    const double s = block_sum(v[d], sh);
-    if (threadIdx.x == 0) store_agent(&partials[(size_t)d * nparts + part], s);
+    const double s = block_sum_n<kNT>(v[d], sh);
+    if (threadIdx.x == 0) partials[(size_t)d * nparts + part] = s;
   }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    amlast = (t == (unsigned)nparts - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!amlast) return false;
-  __threadfence();
+}
+
+// Krylov epilogues applied by the finalize kernel to the reduced sums s[0..2].
+enum {
+  FIN_STORE = 0,      // d0,d1,d2 = s            (always runs)
+  FIN_PCG_PAP = 1,    // alpha = rz / <p,Ap>      (solver_ls_iter.cpp:135-139)
+  FIN_PCG_RR = 2,     // convergence test         (:144-154)
+  FIN_PCG_RZ0 = 3,    // rz = <r,z>               (:131)
+  FIN_PCG_RZ = 4,     // beta = <r,z>_new / rz    (:157-161)
+  FIN_BICG_ALPHA = 5, // r_r2, alpha              (:334-347)
+  FIN_BICG_OMEGA = 6, // omega = <s,AMs>/<AMs,AMs> (:366-372)
+  FIN_BICG_XR = 7     // convergence test, beta   (:383-398)
+};
+
+constexpr int kFinNT = 1024;
+
+template <int ND, int FIN>
+__global__ void __launch_bounds__(kFinNT) k_finalize(const double* __restrict__ partials, int nparts,
+                                                    KState* __restrict__ st) {
+  if (FIN != FIN_STORE && st->done) return;
+  __shared__ double sh[kFinNT / 64];
+  double s[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int d = 0; d < ND; d++) {
     double acc = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kNT) acc += load_agent(&partials[(size_t)d * nparts + i]);
-    out[d] = block_sum(acc, sh);
+    for (int i = threadIdx.x; i < nparts; i += kFinNT) acc += partials[(size_t)d * nparts + i];
+    s[d] = block_sum_n<kFinNT>(acc, sh);
   }
-  if (threadIdx.x == 0) atomicExch(ticket, 0u);
-  return true;
+  if (threadIdx.x != 0) return;
+  if constexpr (FIN == FIN_STORE) {
+    st->d0 = s[0];
+    st->d1 = s[1];
+    st->d2 = s[2];
+  } else if constexpr (FIN == FIN_PCG_PAP) {
+    st->pAp = s[0];
+    st->alpha = st->rz / s[0];
+  } else if constexpr (FIN == FIN_PCG_RR || FIN == FIN_BICG_XR) {
+    st->rr = s[0];
+    if (s[0] * st->inv_rr0 < st->tol2) {
+      st->done = 1;
+      st->conv_iter = st->iter;
+      st->conv_rr = s[0];
+    } else if constexpr (FIN == FIN_BICG_XR) {
+      st->beta = s[1] * st->alpha / (st->r_r2 * st->omega);
+    }
+    st->iter += 1;
+    if (!st->done && st->iter >= st->maxit) st->done = 2;
+  } else if constexpr (FIN == FIN_PCG_RZ0) {
+    st->rz = s[0];
+  } else if constexpr (FIN == FIN_PCG_RZ) {
+    st->rz_new = s[0];
+    st->beta = s[0] / st->rz;
+    st->rz = s[0];
+  } else if constexpr (FIN == FIN_BICG_ALPHA) {
+    st->r_r2 = s[2];
+    st->alpha = s[2] / s[0];
+  } else if constexpr (FIN == FIN_BICG_OMEGA) {
+    st->omega = s[0] / s[1];
+  }
 }
-
 // ---------------------------------------------------------------------------
 // Block row kernels in the reference's summation order.
 // acc[r] += sum_c A[r*B+c]*x[c], the B-term sum formed first (matdia_blkcrs.cpp:1027-1058)
@@ -602,22 +638,18 @@ static __global__ void k_sell_gather_dia(long nslot, int bb, const int* __restri
 // SpMV on the sliced-ELL copy of A (rows = slice*64+lane, diagonal last):
 //   y = A x            (res == nullptr)
 //   y = res - A x      (true residual form)
-// Fused reductions (all optional): d0 = <y,w0>, d1 = <y,y>, d2 = <u2,v2>.
-// BiCGSTAB omega = <s,AMs>/<AMs,AMs> uses w0 = s.
-// Reductions accumulate in double.  MODE selects the Krylov epilogue.
-enum { EPI_NONE = 0, EPI_PCG_PAP = 1, EPI_BICG_ALPHA = 2, EPI_BICG_OMEGA = 3, EPI_STORE = 4 };
-
-template <int B, typename T, int MODE>
-static __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __restrict__ sptr,
+// Optional partial sums (DOTS): d0 = <y,w0>, d1 = <y,y>, d2 = <u2,v2>, in double.
+// BiCGSTAB omega = <s,AMs>/<AMs,AMs> uses w0 = s.  GATE: skip once the solver
+// has stopped (Krylov-internal products); the true-residual form runs always.
+template <int B, typename T, bool DOTS, bool GATE>
+__global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __restrict__ sptr,
                                              const int* __restrict__ col, const T* __restrict__ val,
                                              const T* __restrict__ dval, const T* __restrict__ x,
                                              T* __restrict__ y, const T* __restrict__ res,
                                              const T* __restrict__ w0, const T* __restrict__ u2,
                                              const T* __restrict__ v2, double* __restrict__ partials,
-                                             KState* __restrict__ st) {
-  // Krylov-internal products stop once the solver has converged; the
-  // true-residual form (EPI_STORE) always runs.
-  if (MODE != EPI_STORE && st && st->done) return;
+                                             const KState* __restrict__ st) {
+  if (GATE && st->done) return;
   constexpr int BB = B * B;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = blockIdx.x * kWaves + wave;
@@ -652,39 +684,25 @@ static __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const i
         const size_t o = (size_t)row * B + r;
         const T yv = res ? (T)(res[o] - acc[r]) : acc[r];
         y[o] = yv;
-        if (w0) dsum[0] += (double)yv * (double)w0[o];
-        dsum[1] += (double)yv * (double)yv;
-        if (u2) dsum[2] += (double)u2[o] * (double)v2[o];
+        if (DOTS) {
+          if (w0) dsum[0] += (double)yv * (double)w0[o];
+          dsum[1] += (double)yv * (double)yv;
+          if (u2) dsum[2] += (double)u2[o] * (double)v2[o];
+        }
       }
     }
   }
-  if constexpr (MODE != EPI_NONE) {
-    double out[3];
-    if (finish_sums<3>(dsum, partials, blockIdx.x, gridDim.x, &st->tickets[0], out) && threadIdx.x == 0) {
-      if constexpr (MODE == EPI_PCG_PAP) {
-        st->pAp = out[0];
-        st->alpha = st->rz / st->pAp;
-      } else if constexpr (MODE == EPI_BICG_ALPHA) {
-        st->r_r2 = out[2];
-        st->alpha = st->r_r2 / out[0];
-      } else if constexpr (MODE == EPI_BICG_OMEGA) {
-        st->omega = out[0] / out[1];
-      } else {
-        st->d0 = out[0];
-        st->d1 = out[1];
-        st->d2 = out[2];
-      }
-    }
-  }
+  if constexpr (DOTS) write_partials<3>(dsum, partials, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
 // Triangular solves over one level set (ForwardSubstitution / BackwardSubstitution,
 // src/matvec/matdiafrac_blkcrs.cpp:332-663).  Lower: out_i = Dinv_i (rhs_i - sum L_ij out_j);
-// rhs may alias out.  Upper: v_i -= sum U_ij v_j in place; optionally the rows
-// finished here contribute <r,v> to a reduction spanning all level launches.
+// rhs may alias out.  Upper: v_i -= sum U_ij v_j in place; with DOT the rows
+// finished here contribute <r,v> partials (slot part0 + workgroup) to a
+// reduction that spans all level launches of the sweep.
 template <int B, typename T>
-static __global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* __restrict__ rows,
+__global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* __restrict__ rows,
                                                    const int* __restrict__ sptr, const int* __restrict__ col,
                                                    const T* __restrict__ val, const T* __restrict__ dval,
                                                    const T* rhs, T* out, const KState* __restrict__ st) {
@@ -718,11 +736,11 @@ static __global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const
 }
 
 template <int B, typename T, bool DOT>
-static __global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const int* __restrict__ rows,
+__global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const int* __restrict__ rows,
                                                    const int* __restrict__ sptr, const int* __restrict__ col,
                                                    const T* __restrict__ val, T* vec, const T* __restrict__ rvec,
                                                    double* __restrict__ partials, int part0, int nparts,
-                                                   KState* __restrict__ st, int epi) {
+                                                   const KState* __restrict__ st) {
   if (st && st->done) return;
   constexpr int BB = B * B;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -752,27 +770,16 @@ static __global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const
       }
     }
   }
-  if constexpr (DOT) {
-    double out[1];
-    if (finish_sums<1>(dsum, partials, part0 + blockIdx.x, nparts, &st->tickets[1], out) && threadIdx.x == 0) {
-      if (epi == 0) {            // PCG initial: rz
-        st->rz = out[0];
-      } else {                   // PCG iteration: beta = rz_new / rz
-        st->rz_new = out[0];
-        st->beta = out[0] / st->rz;
-        st->rz = out[0];
-      }
-    }
-  }
+  if constexpr (DOT) write_partials<1>(dsum, partials, part0 + blockIdx.x, nparts);
 }
 
 // ---------------------------------------------------------------------------
 // Krylov vector updates (LsSol::Solve_PCG / Solve_PBiCGSTAB, solver_ls_iter.cpp)
-// PCG: r += (-alpha) z ; x += alpha p ; rr = <r,r> ; convergence test (:140-154)
+// PCG: r += (-alpha) z ; x += alpha p ; partial <r,r> (:140-145)
 template <typename T>
-static __global__ void __launch_bounds__(kNT) k_pcg_update_xr(long n, const T* __restrict__ z, const T* __restrict__ p,
+__global__ void __launch_bounds__(kNT) k_pcg_update_xr(long n, const T* __restrict__ z, const T* __restrict__ p,
                                                       T* __restrict__ r, T* __restrict__ x,
-                                                      double* __restrict__ partials, KState* __restrict__ st) {
+                                                      double* __restrict__ partials, const KState* __restrict__ st) {
   if (st->done) return;
   const double alpha = st->alpha;
   const T a = (T)alpha, ma = (T)(-alpha);
@@ -783,20 +790,11 @@ static __global__ void __launch_bounds__(kNT) k_pcg_update_xr(long n, const T* _
     x[i] = x[i] + a * p[i];
     dsum[0] += (double)rv * (double)rv;
   }
-  double out[1];
-  if (finish_sums<1>(dsum, partials, blockIdx.x, gridDim.x, &st->tickets[2], out) && threadIdx.x == 0) {
-    st->rr = out[0];
-    if (out[0] * st->inv_rr0 < st->tol2) {
-      st->done = 1;
-      st->conv_iter = st->iter;
-      st->conv_rr = out[0];
-    }
-    st->iter += 1;
-  }
+  write_partials<1>(dsum, partials, blockIdx.x, gridDim.x);
 }
 // PCG: p = beta p + z (:163-164)
 template <typename T>
-static __global__ void __launch_bounds__(kNT) k_pcg_update_p(long n, const T* __restrict__ z, T* __restrict__ p,
+__global__ void __launch_bounds__(kNT) k_pcg_update_p(long n, const T* __restrict__ z, T* __restrict__ p,
                                                      const KState* __restrict__ st) {
   if (st->done) return;
   const T beta = (T)st->beta;
@@ -804,19 +802,18 @@ static __global__ void __launch_bounds__(kNT) k_pcg_update_p(long n, const T* __
 }
 // BiCGSTAB: s = r + (-alpha) AMp (:355-357)
 template <typename T>
-static __global__ void __launch_bounds__(kNT) k_bicg_s(long n, const T* __restrict__ r, const T* __restrict__ AMp,
+__global__ void __launch_bounds__(kNT) k_bicg_s(long n, const T* __restrict__ r, const T* __restrict__ AMp,
                                                T* __restrict__ s, const KState* __restrict__ st) {
   if (st->done) return;
   const T ma = (T)(-st->alpha);
   for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) s[i] = r[i] + ma * AMp[i];
 }
-// BiCGSTAB: x += alpha Mp; x += omega Ms; r = s + (-omega) AMs; <r,r>, <r,r2>;
-// convergence test and beta (:376-398)
+// BiCGSTAB: x += alpha Mp; x += omega Ms; r = s + (-omega) AMs; partial <r,r>, <r,r2> (:376-392)
 template <typename T>
-static __global__ void __launch_bounds__(kNT) k_bicg_xr(long n, const T* __restrict__ Mp, const T* __restrict__ Ms,
+__global__ void __launch_bounds__(kNT) k_bicg_xr(long n, const T* __restrict__ Mp, const T* __restrict__ Ms,
                                                 const T* __restrict__ s, const T* __restrict__ AMs,
                                                 const T* __restrict__ r2, T* __restrict__ x, T* __restrict__ r,
-                                                double* __restrict__ partials, KState* __restrict__ st) {
+                                                double* __restrict__ partials, const KState* __restrict__ st) {
   if (st->done) return;
   const T a = (T)st->alpha, w = (T)st->omega, mw = (T)(-st->omega);
   double dsum[2] = {0.0, 0.0};
@@ -829,22 +826,11 @@ static __global__ void __launch_bounds__(kNT) k_bicg_xr(long n, const T* __restr
     dsum[0] += (double)rv * (double)rv;
     dsum[1] += (double)rv * (double)r2[i];
   }
-  double out[2];
-  if (finish_sums<2>(dsum, partials, blockIdx.x, gridDim.x, &st->tickets[3], out) && threadIdx.x == 0) {
-    st->rr = out[0];
-    if (out[0] * st->inv_rr0 < st->tol2) {
-      st->done = 1;
-      st->conv_iter = st->iter;
-      st->conv_rr = out[0];
-    } else {
-      st->beta = out[1] * st->alpha / (st->r_r2 * st->omega);
-    }
-    st->iter += 1;
-  }
+  write_partials<2>(dsum, partials, blockIdx.x, gridDim.x);
 }
 // BiCGSTAB: p = beta p; p += r; p += (-beta*omega) AMp (:401-403)
 template <typename T>
-static __global__ void __launch_bounds__(kNT) k_bicg_p(long n, const T* __restrict__ r, const T* __restrict__ AMp,
+__global__ void __launch_bounds__(kNT) k_bicg_p(long n, const T* __restrict__ r, const T* __restrict__ AMp,
                                                T* __restrict__ p, const KState* __restrict__ st) {
   if (st->done) return;
   const T beta = (T)st->beta, mbw = (T)(-st->beta * st->omega);
@@ -859,14 +845,11 @@ static __global__ void __launch_bounds__(kNT) k_bicg_p(long n, const T* __restri
 // ---------------------------------------------------------------------------
 // Plain vector kernels
 template <typename T>
-static __global__ void k_dot(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restrict__ partials,
-                      KState* __restrict__ st) {
+__global__ void k_dot(long n, const T* __restrict__ x, const T* __restrict__ y, double* __restrict__ partials) {
   double dsum[1] = {0.0};
   for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT)
     dsum[0] += (double)x[i] * (double)y[i];
-  double out[1];
-  if (finish_sums<1>(dsum, partials, blockIdx.x, gridDim.x, &st->tickets[4], out) && threadIdx.x == 0)
-    st->d0 = out[0];
+  write_partials<1>(dsum, partials, blockIdx.x, gridDim.x);
 }
 static __global__ void k_axpy(long n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
   for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] += alpha * x[i];
@@ -880,7 +863,7 @@ static __global__ void k_zero_bc(long n, const int* __restrict__ flag, double* _
 }
 // out[new] = in[order[new]] (OrderingVector_OldToNew, ordering_blk.cpp:151-166) with type conversion and scale
 template <typename TI, typename TO>
-static __global__ void k_permute(int nb, int b, const int* __restrict__ order, const TI* __restrict__ in,
+__global__ void k_permute(int nb, int b, const int* __restrict__ order, const TI* __restrict__ in,
                           TO* __restrict__ out, double scale) {
   const long t = (long)blockIdx.x * kNT + threadIdx.x;
   if (t >= (long)nb * b) return;
@@ -891,7 +874,7 @@ static __global__ void k_permute(int nb, int b, const int* __restrict__ order, c
 }
 // out[order[new]] = in[new] (OrderingVector_NewToOld, ordering_blk.cpp:136-149)
 template <typename T>
-static __global__ void k_unpermute(int nb, int b, const int* __restrict__ order, const T* __restrict__ in, T* __restrict__ out) {
+__global__ void k_unpermute(int nb, int b, const int* __restrict__ order, const T* __restrict__ in, T* __restrict__ out) {
   const long t = (long)blockIdx.x * kNT + threadIdx.x;
   if (t >= (long)nb * b) return;
   const long i = t / b;
@@ -899,13 +882,19 @@ static __global__ void k_unpermute(int nb, int b, const int* __restrict__ order,
   const long dst = order ? order[i] : i;
   out[(size_t)dst * b + r] = in[t];
 }
-// x64 += scale * (double)d32 (refinement update)
-static __global__ void k_refine_update(long n, double scale, const float* __restrict__ d, double* __restrict__ x) {
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT)
-    x[i] = x[i] + scale * (double)d[i];
+// Reliable update of the mixed-precision solvers: x64 += (double)x32; x32 = 0.
+static __global__ void k_reliable_update(long n, float* __restrict__ x32, double* __restrict__ x64) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
+    x64[i] = x64[i] + (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+// y32 = (float)x64
+static __global__ void k_to_f32(long n, const double* __restrict__ x, float* __restrict__ y) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] = (float)x[i];
 }
 template <typename T>
-static __global__ void k_fill(long n, T v, T* __restrict__ x) {
+__global__ void k_fill(long n, T v, T* __restrict__ x) {
   for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) x[i] = v;
 }
 

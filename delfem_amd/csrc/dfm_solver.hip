@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <type_traits>
 
 #include "dfm_kernels.h"
@@ -87,6 +88,13 @@ int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32) {
 }
 
 // ---------------------------------------------------------------------------
+#define DFM_LAUNCH_FIN(ND, FIN, partials, nparts, st)                                             \
+  do {                                                                                            \
+    hipLaunchKernelGGL((k_finalize<ND, FIN>), dim3(1), dim3(kFinNT), 0, ::dfm::stream(), partials, \
+                       nparts, st);                                                               \
+    DFM_HIP(hipGetLastError());                                                                   \
+  } while (0)
+
 template <int B, typename T>
 static int sweep_lower(const Sell<T>& L, const Sell<double>& plan, const T* rhs, T* out, const KState* st) {
   const int ng = (int)plan.group_slice_ptr.size() - 1;
@@ -103,9 +111,11 @@ static int upper_nparts(const Sell<double>& plan) {
   for (int g = 0; g < ng; g++) n += grid_for(plan.group_slice_ptr[g + 1] - plan.group_slice_ptr[g], kWaves);
   return n;
 }
+// Backward sweep; with DOT the partial sums of <rvec, vec> over all rows are
+// left in partials[0 .. upper_nparts) for a finalize kernel.
 template <int B, typename T, bool DOT>
 static int sweep_upper(const Sell<T>& U, const Sell<double>& plan, T* vec, const T* rvec, double* partials,
-                       KState* st, int epi) {
+                       const KState* st) {
   const int ng = (int)plan.group_slice_ptr.size() - 1;
   const int nparts = DOT ? upper_nparts(plan) : 0;
   int part0 = 0;
@@ -113,7 +123,7 @@ static int sweep_upper(const Sell<T>& U, const Sell<double>& plan, T* vec, const
     const int s0 = plan.group_slice_ptr[g], s1 = plan.group_slice_ptr[g + 1];
     const int grid = grid_for(s1 - s0, kWaves);
     DFM_LAUNCH((k_trsv_upper<B, T, DOT>), grid, s0, s1, plan.rows.p, plan.slice_ptr.p, plan.col.p, U.val.p, vec,
-               rvec, partials, part0, nparts, st, epi);
+               rvec, partials, part0, nparts, st);
     part0 += grid;
   }
   return DFM_OK;
@@ -123,15 +133,15 @@ int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper) {
   switch (M->b) {
     case 1:
       if (lower) DFM_TRY((sweep_lower<1, double>(M->L64, M->L64, vec, vec, nullptr)));
-      if (upper) DFM_TRY((sweep_upper<1, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, 0)));
+      if (upper) DFM_TRY((sweep_upper<1, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr)));
       break;
     case 2:
       if (lower) DFM_TRY((sweep_lower<2, double>(M->L64, M->L64, vec, vec, nullptr)));
-      if (upper) DFM_TRY((sweep_upper<2, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, 0)));
+      if (upper) DFM_TRY((sweep_upper<2, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr)));
       break;
     case 3:
       if (lower) DFM_TRY((sweep_lower<3, double>(M->L64, M->L64, vec, vec, nullptr)));
-      if (upper) DFM_TRY((sweep_upper<3, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, 0)));
+      if (upper) DFM_TRY((sweep_upper<3, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr)));
       break;
     default:
       set_error("block size %u: the device substitutions support 1x1, 2x2 and 3x3 blocks", M->b);
@@ -164,13 +174,17 @@ Ops<float> ops_of<float>(dfm_ilu_s* M) {
   return {&M->A32, &M->L32, &M->U32, &M->A64, &M->L64, &M->U64, (int)M->nb};
 }
 
-template <int B, typename T, int MODE>
+template <int B, typename T, bool DOTS, bool GATE>
 static int spmv(const Ops<T>& o, const T* x, T* y, const T* res, const T* w0, const T* u2, const T* v2,
-                double* partials, KState* st) {
+                double* partials, const KState* st) {
   const int nslice = o.planA->nslice;
-  DFM_LAUNCH((k_spmv<B, T, MODE>), grid_for(nslice, kWaves), nslice, o.nb, o.planA->slice_ptr.p, o.planA->col.p,
-             o.A->val.p, o.A->dval.p, x, y, res, w0, u2, v2, partials, st);
+  DFM_LAUNCH((k_spmv<B, T, DOTS, GATE>), grid_for(nslice, kWaves), nslice, o.nb, o.planA->slice_ptr.p,
+             o.planA->col.p, o.A->val.p, o.A->dval.p, x, y, res, w0, u2, v2, partials, st);
   return DFM_OK;
+}
+template <typename T>
+static int spmv_nparts(const Ops<T>& o) {
+  return grid_for(o.planA->nslice, kWaves);
 }
 
 static int read_state(dfm_ilu_s* M) {
@@ -184,10 +198,56 @@ static int write_state(dfm_ilu_s* M) {
 }
 template <typename T>
 static int device_dot(dfm_ilu_s* M, long n, const T* x, const T* y, double* out) {
-  DFM_LAUNCH(k_dot<T>, grid_vec(n), n, x, y, M->partials.p, M->state.p);
+  const int g = grid_vec(n);
+  DFM_LAUNCH(k_dot<T>, g, n, x, y, M->partials.p);
+  DFM_LAUNCH_FIN(1, FIN_STORE, M->partials.p, g, M->state.p);
   DFM_TRY(read_state(M));
   *out = M->h_state->d0;
   return DFM_OK;
+}
+
+// Runs `nblock` iterations of `body` as one hipGraph (instantiated once per
+// operand set and cached on the factor), or directly when the stream cannot
+// be captured (the legacy default stream).
+template <class Body>
+static int run_block(dfm_ilu_s* M, const std::string& key, int nblock, Body&& body) {
+  hipStream_t s = stream();
+  if (s == nullptr) {
+    for (int i = 0; i < nblock; i++) DFM_TRY(body());
+    return DFM_OK;
+  }
+  for (auto& g : M->graphs)
+    if (g.first == key) {
+      DFM_HIP(hipGraphLaunch(g.second, s));
+      return DFM_OK;
+    }
+  hipGraph_t graph = nullptr;
+  DFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = DFM_OK;
+  for (int i = 0; i < nblock && rc == DFM_OK; i++) rc = body();
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc != DFM_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  DFM_HIP(e);
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  DFM_HIP(e);
+  M->graphs.emplace_back(key, exec);
+  DFM_HIP(hipGraphLaunch(exec, s));
+  return DFM_OK;
+}
+
+static std::string graph_key(const char* tag, int B, int nblock, std::initializer_list<const void*> ptrs) {
+  std::string k = std::string(tag) + ":" + std::to_string(B) + ":" + std::to_string(nblock);
+  char buf[32];
+  for (const void* p : ptrs) {
+    snprintf(buf, sizeof buf, ":%p", p);
+    k += buf;
+  }
+  return k;
 }
 
 struct KResult {
@@ -196,12 +256,50 @@ struct KResult {
   bool converged;
 };
 
+static void init_state(KState& h, double rr0, double tol, unsigned maxit, int iter0) {
+  std::memset(&h, 0, sizeof h);
+  h.inv_rr0 = 1.0 / rr0;
+  h.tol2 = tol * tol;
+  h.iter = iter0;
+  h.maxit = (int)std::min<unsigned>(maxit, 0x7fffffffu);
+  if (h.iter >= h.maxit) h.done = 2;
+}
+
+// One PCG iteration (solver_ls_iter.cpp:134-165) on the device.
+template <int B, typename T>
+static int pcg_body(dfm_ilu_s* M, const Ops<T>& o, T* r, T* x, T* p, T* z) {
+  const long n = (long)o.nb * B;
+  KState* st = M->state.p;
+  double* part = M->partials.p;
+  const int gv = grid_vec(n);
+  DFM_TRY((spmv<B, T, true, true>(o, p, z, nullptr, p, nullptr, nullptr, part, st)));
+  DFM_LAUNCH_FIN(1, FIN_PCG_PAP, part, spmv_nparts(o), st);
+  DFM_LAUNCH(k_pcg_update_xr<T>, gv, n, z, p, r, x, part, st);
+  DFM_LAUNCH_FIN(1, FIN_PCG_RR, part, gv, st);
+  DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, r, z, st)));
+  DFM_TRY((sweep_upper<B, T, true>(*o.U, *o.planU, z, r, part, st)));
+  DFM_LAUNCH_FIN(1, FIN_PCG_RZ, part, upper_nparts(*o.planU), st);
+  DFM_LAUNCH(k_pcg_update_p<T>, gv, n, z, p, st);
+  return DFM_OK;
+}
+
+// PCG start: z = M^-1 r, rz = <r,z>, p = z (:128-132)
+template <int B, typename T>
+static int pcg_start(dfm_ilu_s* M, const Ops<T>& o, T* r, T* p, T* z) {
+  const long n = (long)o.nb * B;
+  KState* st = M->state.p;
+  DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, r, z, st)));
+  DFM_TRY((sweep_upper<B, T, true>(*o.U, *o.planU, z, r, M->partials.p, st)));
+  DFM_LAUNCH_FIN(1, FIN_PCG_RZ0, M->partials.p, upper_nparts(*o.planU), st);
+  DFM_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+  return DFM_OK;
+}
+
 // LsSol::Solve_PCG (solver_ls_iter.cpp:100-172).  r: in rhs, out final residual; x: out.
 template <int B, typename T>
 static int pcg_core(dfm_ilu_s* M, const Ops<T>& o, T* r, T* x, T* p, T* z, double tol, unsigned maxit,
                     unsigned check_every, KResult& res) {
   const long n = (long)o.nb * B;
-  KState* st = M->state.p;
   double rr0 = 0.0;
   DFM_TRY(device_dot<T>(M, n, r, r, &rr0));
   DFM_LAUNCH(k_fill<T>, grid_vec(n), n, (T)0, x);
@@ -209,36 +307,44 @@ static int pcg_core(dfm_ilu_s* M, const Ops<T>& o, T* r, T* x, T* p, T* z, doubl
     res = {0.0, 0u, true};
     return DFM_OK;
   }
-  KState h;
-  std::memset(&h, 0, sizeof h);
-  h.inv_rr0 = 1.0 / rr0;
-  h.tol2 = tol * tol;
-  *M->h_state = h;
+  init_state(*M->h_state, rr0, tol, maxit, 0);
   DFM_TRY(write_state(M));
-  DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, r, z, st)));
-  DFM_TRY((sweep_upper<B, T, true>(*o.U, *o.planU, z, r, M->partials.p, st, 0)));
-  DFM_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
-  const int gv = grid_vec(n);
-  unsigned it = 0;
-  for (; it < maxit;) {
-    DFM_TRY((spmv<B, T, EPI_PCG_PAP>(o, p, z, nullptr, p, nullptr, nullptr, M->partials.p, st)));
-    DFM_LAUNCH(k_pcg_update_xr<T>, gv, n, z, p, r, x, M->partials.p, st);
-    DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, r, z, st)));
-    DFM_TRY((sweep_upper<B, T, true>(*o.U, *o.planU, z, r, M->partials.p, st, 1)));
-    DFM_LAUNCH(k_pcg_update_p<T>, gv, n, z, p, st);
-    it++;
-    if (it % check_every == 0 || it == maxit) {
-      DFM_TRY(read_state(M));
-      if (M->h_state->done) break;
-    }
+  DFM_TRY((pcg_start<B, T>(M, o, r, p, z)));
+  const std::string key = graph_key(sizeof(T) == 8 ? "pcg64" : "pcg32", B, (int)check_every, {r, x, p, z, M->partials.p, M->state.p});
+  for (;;) {
+    DFM_TRY(run_block(M, key, (int)check_every, [&]() { return pcg_body<B, T>(M, o, r, x, p, z); }));
+    DFM_TRY(read_state(M));
+    if (M->h_state->done) break;
   }
-  DFM_TRY(read_state(M));
   const KState& s = *M->h_state;
-  if (s.done) {
+  if (s.done == 1) {
     res = {std::sqrt(s.conv_rr * s.inv_rr0), (unsigned)s.conv_iter, true};
   } else {
     res = {std::sqrt(s.rr * s.inv_rr0), maxit, false};
   }
+  return DFM_OK;
+}
+
+// One BiCGSTAB iteration (solver_ls_iter.cpp:327-404).
+template <int B, typename T>
+static int bicg_body(dfm_ilu_s* M, const Ops<T>& o, T* r, T* x, T* w) {
+  const long n = (long)o.nb * B;
+  T *r2 = w, *p = w + n, *Mp = w + 2 * n, *AMp = w + 3 * n, *s = w + 4 * n, *Ms = w + 5 * n, *AMs = w + 6 * n;
+  KState* st = M->state.p;
+  double* part = M->partials.p;
+  const int gv = grid_vec(n);
+  DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, p, Mp, st)));
+  DFM_TRY((sweep_upper<B, T, false>(*o.U, *o.planU, Mp, nullptr, nullptr, st)));
+  DFM_TRY((spmv<B, T, true, true>(o, Mp, AMp, nullptr, r2, r, r2, part, st)));
+  DFM_LAUNCH_FIN(3, FIN_BICG_ALPHA, part, spmv_nparts(o), st);
+  DFM_LAUNCH(k_bicg_s<T>, gv, n, r, AMp, s, st);
+  DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, s, Ms, st)));
+  DFM_TRY((sweep_upper<B, T, false>(*o.U, *o.planU, Ms, nullptr, nullptr, st)));
+  DFM_TRY((spmv<B, T, true, true>(o, Ms, AMs, nullptr, s, nullptr, nullptr, part, st)));
+  DFM_LAUNCH_FIN(2, FIN_BICG_OMEGA, part, spmv_nparts(o), st);
+  DFM_LAUNCH(k_bicg_xr<T>, gv, n, Mp, Ms, s, AMs, r2, x, r, part, st);
+  DFM_LAUNCH_FIN(2, FIN_BICG_XR, part, gv, st);
+  DFM_LAUNCH(k_bicg_p<T>, gv, n, r, AMp, p, st);
   return DFM_OK;
 }
 
@@ -247,8 +353,6 @@ template <int B, typename T>
 static int bicg_core(dfm_ilu_s* M, const Ops<T>& o, T* r, T* x, T* w, double tol, unsigned maxit,
                      unsigned check_every, KResult& res, bool& x_zeroed) {
   const long n = (long)o.nb * B;
-  T *r2 = w, *p = w + n, *Mp = w + 2 * n, *AMp = w + 3 * n, *s = w + 4 * n, *Ms = w + 5 * n, *AMs = w + 6 * n;
-  KState* st = M->state.p;
   double rr0 = 0.0;
   DFM_TRY(device_dot<T>(M, n, r, r, &rr0));
   x_zeroed = false;
@@ -258,40 +362,106 @@ static int bicg_core(dfm_ilu_s* M, const Ops<T>& o, T* r, T* x, T* w, double tol
   }
   DFM_LAUNCH(k_fill<T>, grid_vec(n), n, (T)0, x);
   x_zeroed = true;
-  DFM_HIP(hipMemcpyAsync(r2, r, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
-  DFM_HIP(hipMemcpyAsync(p, r, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
-  KState h;
-  std::memset(&h, 0, sizeof h);
-  h.inv_rr0 = 1.0 / rr0;
-  h.tol2 = tol * tol;
-  h.iter = 1;
-  *M->h_state = h;
+  DFM_HIP(hipMemcpyAsync(w, r, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));      // r2 = r
+  DFM_HIP(hipMemcpyAsync(w + n, r, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));  // p = r
+  init_state(*M->h_state, rr0, tol, maxit, 1);
   DFM_TRY(write_state(M));
-  const int gv = grid_vec(n);
-  unsigned it = 1;
-  for (; it < maxit;) {
-    DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, p, Mp, st)));
-    DFM_TRY((sweep_upper<B, T, false>(*o.U, *o.planU, Mp, nullptr, nullptr, st, 0)));
-    DFM_TRY((spmv<B, T, EPI_BICG_ALPHA>(o, Mp, AMp, nullptr, r2, r, r2, M->partials.p, st)));
-    DFM_LAUNCH(k_bicg_s<T>, gv, n, r, AMp, s, st);
-    DFM_TRY((sweep_lower<B, T>(*o.L, *o.planL, s, Ms, st)));
-    DFM_TRY((sweep_upper<B, T, false>(*o.U, *o.planU, Ms, nullptr, nullptr, st, 0)));
-    DFM_TRY((spmv<B, T, EPI_BICG_OMEGA>(o, Ms, AMs, nullptr, s, nullptr, nullptr, M->partials.p, st)));
-    DFM_LAUNCH(k_bicg_xr<T>, gv, n, Mp, Ms, s, AMs, r2, x, r, M->partials.p, st);
-    DFM_LAUNCH(k_bicg_p<T>, gv, n, r, AMp, p, st);
-    it++;
-    if ((it - 1) % check_every == 0 || it == maxit) {
+  const std::string key = graph_key(sizeof(T) == 8 ? "bicg64" : "bicg32", B, (int)check_every, {r, x, w, M->partials.p, M->state.p});
+  if (!M->h_state->done) {
+    for (;;) {
+      DFM_TRY(run_block(M, key, (int)check_every, [&]() { return bicg_body<B, T>(M, o, r, x, w); }));
       DFM_TRY(read_state(M));
       if (M->h_state->done) break;
     }
   }
-  DFM_TRY(read_state(M));
   const KState& sv = *M->h_state;
-  if (sv.done) {
+  if (sv.done == 1) {
     res = {std::sqrt(sv.conv_rr * sv.inv_rr0), (unsigned)sv.conv_iter, true};
   } else {
     res = {tol, maxit, false};  // the reference leaves conv_ratio and reports max_iter
   }
+  return DFM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision with reliable updates: the Krylov recurrence runs in FP32
+// (matrix, factor, vectors) with FP64 reduction accumulators.  Whenever the
+// recursive residual has dropped by kDelta since the last update (or the FP32
+// recurrence reports convergence), the FP32 partial solution is folded into
+// the FP64 solution, the true residual b - A x is recomputed with the FP64
+// matrix and replaces the FP32 residual.  The search direction is kept, so the
+// Krylov information is not lost as in restarted refinement.  Convergence is
+// decided on the FP64 true residual.
+constexpr double kDelta2 = 1.0e-2;  // squared residual drop between reliable updates
+
+template <int B>
+static int true_residual64(dfm_ilu_s* M, const double* b, const double* x, double* r, double* rr) {
+  Ops<double> o = ops_of<double>(M);
+  DFM_TRY((spmv<B, double, true, false>(o, x, r, b, nullptr, nullptr, nullptr, M->partials.p, M->state.p)));
+  DFM_LAUNCH_FIN(3, FIN_STORE, M->partials.p, spmv_nparts(o), M->state.p);
+  DFM_TRY(read_state(M));
+  *rr = M->h_state->d1;
+  return DFM_OK;
+}
+
+template <int B>
+static int mixed_core(dfm_ilu_s* M, bool bicg, const double* b, double* r64, double* x64, float* w32, double tol,
+                      unsigned maxit, unsigned check_every, KResult& res, unsigned& nupdates) {
+  const long n = (long)M->nb * B;
+  Ops<float> o = ops_of<float>(M);
+  float* r32 = w32;
+  float* x32 = w32 + n;
+  float* ws = w32 + 2 * n;  // PCG: p, z ; BiCGSTAB: r2, p, Mp, AMp, s, Ms, AMs
+  nupdates = 0;
+  double rr0 = 0.0;
+  DFM_TRY(device_dot<double>(M, n, b, b, &rr0));
+  DFM_LAUNCH(k_fill<double>, grid_vec(n), n, 0.0, x64);
+  DFM_HIP(hipMemcpyAsync(r64, b, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+  if (rr0 < (bicg ? 1.0e-60 : 1.0e-30)) {
+    res = {bicg ? std::sqrt(rr0) : 0.0, 0u, true};
+    return DFM_OK;
+  }
+  DFM_LAUNCH(k_to_f32, grid_vec(n), n, r64, r32);
+  DFM_LAUNCH(k_fill<float>, grid_vec(n), n, 0.0f, x32);
+  init_state(*M->h_state, rr0, tol, maxit, bicg ? 1 : 0);
+  DFM_TRY(write_state(M));
+  if (!bicg) {
+    DFM_TRY((pcg_start<B, float>(M, o, r32, ws, ws + n)));
+  } else {
+    DFM_HIP(hipMemcpyAsync(ws, r32, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));      // r2
+    DFM_HIP(hipMemcpyAsync(ws + n, r32, n * sizeof(float), hipMemcpyDeviceToDevice, stream()));  // p
+  }
+  const std::string key =
+      graph_key(bicg ? "bicg32r" : "pcg32r", B, (int)check_every, {r32, x32, ws, M->partials.p, M->state.p});
+  double rr_ref = rr0, rr_true = rr0;
+  bool converged = false;
+  for (;;) {
+    if (!M->h_state->done) {
+      DFM_TRY(run_block(M, key, (int)check_every, [&]() {
+        return bicg ? bicg_body<B, float>(M, o, r32, x32, ws) : pcg_body<B, float>(M, o, r32, x32, ws, ws + n);
+      }));
+      DFM_TRY(read_state(M));
+    }
+    KState& h = *M->h_state;
+    if (!(h.done || h.rr < kDelta2 * rr_ref)) continue;
+    // reliable update
+    DFM_LAUNCH(k_reliable_update, grid_vec(n), n, x32, x64);
+    DFM_TRY(true_residual64<B>(M, b, x64, r64, &rr_true));
+    nupdates++;
+    KState& hs = *M->h_state;
+    if (rr_true * hs.inv_rr0 < hs.tol2) {
+      converged = true;
+      break;
+    }
+    if (hs.done == 2 || !std::isfinite(rr_true)) break;
+    DFM_LAUNCH(k_to_f32, grid_vec(n), n, r64, r32);
+    rr_ref = rr_true;
+    hs.done = 0;
+    hs.rr = rr_true;
+    DFM_TRY(write_state(M));
+  }
+  const KState& s = *M->h_state;
+  res = {std::sqrt(rr_true * s.inv_rr0), (unsigned)(bicg ? s.iter - 1 : s.iter), converged};
   return DFM_OK;
 }
 
@@ -309,15 +479,6 @@ static int ensure_workspace(dfm_ilu_s* M, size_t n64, size_t n32) {
 }
 
 template <int B>
-static int true_residual64(dfm_ilu_s* M, const double* b, const double* x, double* r, double* rr) {
-  Ops<double> o = ops_of<double>(M);
-  DFM_TRY((spmv<B, double, EPI_STORE>(o, x, r, b, nullptr, nullptr, nullptr, M->partials.p, M->state.p)));
-  DFM_TRY(read_state(M));
-  *rr = M->h_state->d1;
-  return DFM_OK;
-}
-
-template <int B>
 static int solve_dispatch(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r_user, double* x_user, double* conv_ratio,
                           unsigned* iteration, dfm_solve_opts* opts) {
   const int nb = (int)M->nb;
@@ -329,15 +490,14 @@ static int solve_dispatch(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r_user
   const unsigned maxit = *iteration;
   DFM_TRY(ilu_refresh_sell(M, A, mixed));
   const int* ord = M->permuted ? M->d_order.p : nullptr;
-  // FP64 workspace: r, x (permuted copies), b (rhs copy), then solver vectors
   const size_t nvec64 = mixed ? 3 : (bicg ? 10 : 5);
   const size_t nvec32 = mixed ? (bicg ? 9 : 4) : 0;
   DFM_TRY(ensure_workspace(M, nvec64 * n, nvec32 * n));
   double* r = M->w64.p;
   double* x = M->w64.p + n;
   double* b = M->w64.p + 2 * n;
-  DFM_LAUNCH((k_permute<double, double>), grid_for(n), nb, B, ord, r_user, r, 1.0);
-  if (want_true) DFM_HIP(hipMemcpyAsync(b, r, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+  DFM_LAUNCH((k_permute<double, double>), grid_for(n), nb, B, ord, r_user, mixed ? b : r, 1.0);
+  if (want_true && !mixed) DFM_HIP(hipMemcpyAsync(b, r, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
   KResult kr{0.0, 0u, false};
   int rc = DFM_OK;
   bool x_written = true;
@@ -350,60 +510,24 @@ static int solve_dispatch(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r_user
     }
     *conv_ratio = kr.conv;
     if (kr.converged || !bicg) *iteration = kr.iter;
+    else *iteration = maxit;
     rc = kr.converged ? DFM_OK : DFM_NOT_CONVERGED;
     if (want_true && x_written) {
       double rr = 0.0, bb2 = 0.0;
       DFM_TRY(device_dot<double>(M, n, b, b, &bb2));
-      double* tmp = bicg ? M->w64.p + 4 * n : M->w64.p + 4 * n;  // z / Mp slot is free now
+      double* tmp = M->w64.p + 4 * n;  // free after the solve
       DFM_TRY(true_residual64<B>(M, b, x, tmp, &rr));
       opts->true_relres = bb2 > 0.0 ? std::sqrt(rr / bb2) : 0.0;
     }
   } else {
-    // Iterative refinement: x += A32^-1 (b - A64 x) with an FP32 inner Krylov solve.
-    Ops<float> o = ops_of<float>(M);
-    float* r32 = M->w32.p;
-    float* d32 = M->w32.p + n;
-    float* ws32 = M->w32.p + 2 * n;
-    const double inner_tol = (opts->inner_tol > 0.0) ? opts->inner_tol : 1.0e-4;
-    const unsigned max_outer = opts->max_outer ? opts->max_outer : 30;
-    double bb2 = 0.0;
-    DFM_TRY(device_dot<double>(M, n, b, b, &bb2));
-    DFM_LAUNCH(k_fill<double>, grid_vec(n), n, 0.0, x);
-    unsigned total = 0, outer = 0;
-    double rel = 0.0;
-    bool conv = false;
-    if (bb2 < 1.0e-30) {
-      conv = true;
-    } else {
-      for (outer = 0; outer <= max_outer; outer++) {
-        double rr = 0.0;
-        DFM_TRY(true_residual64<B>(M, b, x, r, &rr));
-        rel = std::sqrt(rr / bb2);
-        if (rel < tol) {
-          conv = true;
-          break;
-        }
-        if (outer == max_outer || total >= maxit) break;
-        const double nr = std::sqrt(rr);
-        DFM_LAUNCH((k_permute<double, float>), grid_for(n), nb, B, nullptr, r, r32, 1.0 / nr);
-        KResult ir{0.0, 0u, false};
-        const unsigned left = maxit - total;
-        if (!bicg) {
-          DFM_TRY((pcg_core<B, float>(M, o, r32, d32, ws32, ws32 + n, inner_tol, left, check_every, ir)));
-        } else {
-          bool xz = true;
-          DFM_TRY((bicg_core<B, float>(M, o, r32, d32, ws32, inner_tol, left, check_every, ir, xz)));
-        }
-        total += ir.converged ? ir.iter + 1 : left;
-        DFM_LAUNCH(k_refine_update, grid_vec(n), n, nr, d32, x);
-      }
-    }
-    opts->inner_iters = total;
-    opts->outer_iters = outer;
-    opts->true_relres = rel;
-    *conv_ratio = rel;
-    *iteration = total;
-    rc = conv ? DFM_OK : DFM_NOT_CONVERGED;
+    unsigned nup = 0;
+    DFM_TRY(mixed_core<B>(M, bicg, b, r, x, M->w32.p, tol, maxit, check_every, kr, nup));
+    opts->inner_iters = kr.iter;
+    opts->outer_iters = nup;
+    opts->true_relres = kr.conv;
+    *conv_ratio = kr.conv;
+    *iteration = kr.iter;
+    rc = kr.converged ? DFM_OK : DFM_NOT_CONVERGED;
   }
   if (x_written) DFM_LAUNCH(k_unpermute<double>, grid_for(n), nb, B, ord, x, x_user);
   DFM_LAUNCH(k_unpermute<double>, grid_for(n), nb, B, ord, r, r_user);
@@ -446,9 +570,9 @@ static int bench_kernel_t(dfm_bcrs_s* A, dfm_ilu_s* M, int which, int nrep, doub
     nnzU += M->frow[i + 1] - M->fdiaind[i];
   }
   auto run = [&]() -> int {
-    if (which == 0) return spmv<B, T, EPI_NONE>(o, xv.p, yv.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (which == 0) return spmv<B, T, false, false>(o, xv.p, yv.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (which == 1) return sweep_lower<B, T>(*o.L, *o.planL, xv.p, yv.p, nullptr);
-    return sweep_upper<B, T, false>(*o.U, *o.planU, yv.p, nullptr, nullptr, nullptr, 0);
+    return sweep_upper<B, T, false>(*o.U, *o.planU, yv.p, nullptr, nullptr, nullptr);
   };
   if (which == 0) *bytes = (nnzA + nb) * bb * s + 4.0 * nnzA + 2.0 * n * s;
   else if (which == 1) *bytes = (nnzL + nb) * bb * s + 4.0 * nnzL + 4.0 * nb + 3.0 * n * s;

@@ -913,15 +913,68 @@ void orc_emat_solid3d_p1(unsigned nelem, const double *coord, const unsigned *co
 
 /* ------------------------------------------------------------------------- */
 /* Multicolour ordering (product algorithm; no reference counterpart):
- * greedy colouring in natural order, each node takes the smallest colour not
- * used by an already-coloured neighbour. */
+ * smallest-last greedy colouring.  Vertices are removed one at a time, the
+ * one of least remaining degree first (ties: lowest index); they are then
+ * coloured in reverse removal order, each with the smallest colour not used
+ * by an already coloured neighbour.  Restated with a lazy binary heap of
+ * (degree, index) pairs. */
+typedef struct { int d; unsigned v; } orc_hent;
+static int hless(orc_hent a, orc_hent b) { return a.d < b.d || (a.d == b.d && a.v < b.v); }
+static void hpush(orc_hent *h, long *n, orc_hent e)
+{
+    long i = (*n)++;
+    h[i] = e;
+    while (i > 0) {
+        long p = (i - 1) / 2;
+        if (!hless(h[i], h[p])) break;
+        orc_hent t = h[i]; h[i] = h[p]; h[p] = t; i = p;
+    }
+}
+static orc_hent hpop(orc_hent *h, long *n)
+{
+    orc_hent top = h[0];
+    h[0] = h[--(*n)];
+    long i = 0;
+    for (;;) {
+        long l = 2 * i + 1, r = l + 1, m = i;
+        if (l < *n && hless(h[l], h[m])) m = l;
+        if (r < *n && hless(h[r], h[m])) m = r;
+        if (m == i) break;
+        orc_hent t = h[i]; h[i] = h[m]; h[m] = t; i = m;
+    }
+    return top;
+}
 int orc_greedy_colour(unsigned nb, const unsigned *rowptr, const unsigned *col, int *colour)
 {
+    int *deg = (int *)malloc(sizeof(int) * ((size_t)nb + 1));
+    char *removed = (char *)calloc((size_t)nb + 1, 1);
+    unsigned *seq = (unsigned *)malloc(sizeof(unsigned) * ((size_t)nb + 1));
+    orc_hent *heap = (orc_hent *)malloc(sizeof(orc_hent) * ((size_t)nb + rowptr[nb] + 1));
+    long hn = 0, ns = 0;
+    for (unsigned i = 0; i < nb; i++) {
+        deg[i] = (int)(rowptr[i + 1] - rowptr[i]);
+        orc_hent e = {deg[i], i};
+        hpush(heap, &hn, e);
+    }
+    while (ns < (long)nb) {
+        orc_hent e = hpop(heap, &hn);
+        if (removed[e.v] || deg[e.v] != e.d) continue;
+        removed[e.v] = 1;
+        seq[ns++] = e.v;
+        for (unsigned q = rowptr[e.v]; q < rowptr[e.v + 1]; q++) {
+            const unsigned w = col[q];
+            if (removed[w]) continue;
+            deg[w]--;
+            orc_hent f = {deg[w], w};
+            hpush(heap, &hn, f);
+        }
+    }
     int ncol = 0;
     int *stamp = (int *)malloc(sizeof(int) * ((size_t)nb + 2));
     for (unsigned i = 0; i < nb + 2; i++) stamp[i] = -1;
     for (unsigned i = 0; i < nb; i++) colour[i] = -1;
-    for (unsigned i = 0; i < nb; i++) {
+    for (long t = (long)nb - 1; t >= 0; t--) {
+        const unsigned i = seq[t];
         for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
             const int c = colour[col[q]];
             if (c >= 0) stamp[c] = (int)i;
@@ -931,7 +984,7 @@ int orc_greedy_colour(unsigned nb, const unsigned *rowptr, const unsigned *col, 
         colour[i] = c;
         if (c + 1 > ncol) ncol = c + 1;
     }
-    free(stamp);
+    free(deg); free(removed); free(seq); free(heap); free(stamp);
     return ncol;
 }
 
