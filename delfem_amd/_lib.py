@@ -25,15 +25,20 @@ DFM_ORDER_MULTICOLOUR = 2
 
 DFM_PREC_FP64 = 0
 DFM_PREC_MIXED = 1
+DFM_PREC_MIXED_BF16 = 2
+DFM_PREC_MIXED_F32 = 3
+DFM_PREC_MIXED_F32_BF16 = 4
+
+PRECISIONS = {"fp64": DFM_PREC_FP64, "mixed": DFM_PREC_MIXED, "mixed_bf16": DFM_PREC_MIXED_BF16,
+              "mixed_f32": DFM_PREC_MIXED_F32, "mixed_f32_bf16": DFM_PREC_MIXED_F32_BF16}
 
 
 class SolveOpts(C.Structure):
     _fields_ = [
         ("precision", C.c_int),
-        ("inner_tol", C.c_double),
-        ("max_outer", C.c_uint),
         ("check_every", C.c_uint),
         ("want_true_relres", C.c_int),
+        ("reliable_delta", C.c_double),
         ("inner_iters", C.c_uint),
         ("outer_iters", C.c_uint),
         ("true_relres", C.c_double),
@@ -91,6 +96,7 @@ _SIGS = {
     "dfm_solve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_solve_pbicgstab": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_bench_kernel": (C.c_int, [H, H, C.c_int, C.c_int, C.c_int, f64p, f64p]),
+    "dfm_bench_copy": (C.c_int, [C.c_size_t, C.c_int, f64p]),
 }
 
 _lib = None

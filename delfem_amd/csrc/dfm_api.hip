@@ -52,7 +52,7 @@ using namespace dfm;
 
 // Implemented in dfm_solver.hip
 namespace dfm {
-int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32);
+int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32, bool want16);
 int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper);
 }
 
@@ -662,7 +662,7 @@ int dfm_ilu_set_value(dfm_ilu* M, dfm_bcrs* A, int* nsingular) {
                    M->d_fval.p, M->d_fdia.p, M->d_nsing.p);
     }
   }
-  DFM_TRY(ilu_refresh_sell(M, A, M->have32));
+  DFM_TRY(ilu_refresh_sell(M, A, M->have32, M->have16));
   int ns = 0;
   DFM_HIP(hipMemcpyAsync(&ns, M->d_nsing.p, sizeof(int), hipMemcpyDeviceToHost, g_stream));
   DFM_HIP(hipStreamSynchronize(g_stream));

@@ -76,6 +76,11 @@ struct DevBuf {
 
 constexpr int kSlice = 64;  // rows per SELL slice = one wavefront
 
+// bfloat16 storage (the upper 16 bits of an IEEE float)
+struct bf16s {
+  unsigned short bits;
+};
+
 // ---------------------------------------------------------------------------
 // Host-side algorithms (dfm_host.cpp)
 long fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned* conn,
@@ -169,7 +174,8 @@ struct dfm_ilu_s {
   dfm::DevBuf<long long> d_srcL, d_srcU;
   dfm::Sell<double> L64, U64;
   dfm::Sell<float> L32, U32;
-  bool have32 = false;
+  dfm::Sell<dfm::bf16s> L16, U16;
+  bool have32 = false, have16 = false;
   // permuted copy of A for the Krylov SpMV
   dfm::SellPlan planA;
   dfm::DevBuf<long long> d_srcA;
@@ -180,8 +186,7 @@ struct dfm_ilu_s {
   const dfm_bcrs_s* A_pattern_owner = nullptr;
   bool factored = false;
   // Krylov workspace (factor numbering)
-  dfm::DevBuf<double> w64;              // several vectors of n doubles
-  dfm::DevBuf<float> w32;
+  dfm::DevBuf<char> work;               // Krylov vectors (typed views carved per solve)
   dfm::DevBuf<double> partials;
   dfm::DevBuf<dfm::KState> state;
   dfm::KState* h_state = nullptr;       // pinned mirror

@@ -113,6 +113,57 @@ __global__ void __launch_bounds__(kFinNT) k_finalize(const double* __restrict__ 
   }
 }
 // ---------------------------------------------------------------------------
+// Storage types of the sliced-ELL copies: double, float, or bfloat16 held as
+// its 16 raw bits (bf16s).  Values are converted to the arithmetic type of the
+// vectors when loaded.
+template <typename TV>
+__device__ inline TV cvt(double v) { return (TV)v; }
+template <typename TV>
+__device__ inline TV cvt(float v) { return (TV)v; }
+template <typename TV>
+__device__ inline TV cvt(bf16s v) {
+  return (TV)__uint_as_float((unsigned)v.bits << 16);
+}
+// double -> bf16 with round-to-nearest-even through float
+__device__ inline bf16s to_bf16(double d) {
+  const unsigned u = __float_as_uint((float)d);
+  bf16s r;
+  r.bits = (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+  return r;
+}
+template <typename TS>
+__device__ inline TS store_as(double v) { return (TS)v; }
+template <>
+__device__ inline bf16s store_as<bf16s>(double v) { return to_bf16(v); }
+
+// Streamed (read-once) operands of the sliced-ELL kernels are loaded with the
+// non-temporal hint so the gathered vectors keep the caches.
+template <typename T>
+__device__ inline T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+template <>
+__device__ inline bf16s ld_stream<bf16s>(const bf16s* p) {
+  bf16s r;
+  r.bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p));
+  return r;
+}
+
+// One block slot of a sliced-ELL row: values a[] (storage TS converted to TV),
+// gathered x block xv[].  Padding slots (column < 0) gather zeros without
+// touching memory.
+template <int B, typename TS, typename TV>
+__device__ inline void sell_slot(const int* __restrict__ col, const TS* __restrict__ val, const TV* x, long k,
+                                 int lane, TV (&a)[B * B], TV (&xv)[B]) {
+  const int c = ld_stream(col + (size_t)k * 64 + lane);
+#pragma unroll
+  for (int e = 0; e < B * B; e++) a[e] = cvt<TV>(ld_stream(val + ((size_t)k * B * B + e) * 64 + lane));
+  const bool ok = c >= 0;
+  const size_t cb = (size_t)(ok ? c : 0) * B;
+#pragma unroll
+  for (int e = 0; e < B; e++) xv[e] = ok ? x[cb + e] : (TV)0;
+}
+// ---------------------------------------------------------------------------
 // Block row kernels in the reference's summation order.
 // acc[r] += sum_c A[r*B+c]*x[c], the B-term sum formed first (matdia_blkcrs.cpp:1027-1058)
 template <int B, typename T>
@@ -619,7 +670,7 @@ static __global__ void k_sell_gather(long nslot, int bb, const long long* __rest
   const int lane = (int)(slot & 63);
   const long long s = src[slot];
   for (int e = 0; e < bb; e++)
-    to[((size_t)k * bb + e) * 64 + lane] = s >= 0 ? (T)from[(size_t)s * bb + e] : (T)0;
+    to[((size_t)k * bb + e) * 64 + lane] = store_as<T>(s >= 0 ? from[(size_t)s * bb + e] : 0.0);
 }
 // Diagonal blocks per slice: to[(s*bb+e)*64+lane] = from[map(row)*bb+e].
 template <typename T>
@@ -631,8 +682,9 @@ static __global__ void k_sell_gather_dia(long nslot, int bb, const int* __restri
   const int lane = (int)(slot & 63);
   const int row = rows[slot];
   const long src = row < 0 ? -1 : (order ? order[row] : row);
-  for (int e = 0; e < bb; e++) to[((size_t)s * bb + e) * 64 + lane] = src >= 0 ? (T)from[(size_t)src * bb + e] : (T)0;
+  for (int e = 0; e < bb; e++) to[((size_t)s * bb + e) * 64 + lane] = store_as<T>(src >= 0 ? from[(size_t)src * bb + e] : 0.0);
 }
+
 
 // ---------------------------------------------------------------------------
 // SpMV on the sliced-ELL copy of A (rows = slice*64+lane, diagonal last):
@@ -641,10 +693,10 @@ static __global__ void k_sell_gather_dia(long nslot, int bb, const int* __restri
 // Optional partial sums (DOTS): d0 = <y,w0>, d1 = <y,y>, d2 = <u2,v2>, in double.
 // BiCGSTAB omega = <s,AMs>/<AMs,AMs> uses w0 = s.  GATE: skip once the solver
 // has stopped (Krylov-internal products); the true-residual form runs always.
-template <int B, typename T, bool DOTS, bool GATE>
+template <int B, typename TS, typename T, bool DOTS, bool GATE>
 __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __restrict__ sptr,
-                                             const int* __restrict__ col, const T* __restrict__ val,
-                                             const T* __restrict__ dval, const T* __restrict__ x,
+                                             const int* __restrict__ col, const TS* __restrict__ val,
+                                             const TS* __restrict__ dval, const T* __restrict__ x,
                                              T* __restrict__ y, const T* __restrict__ res,
                                              const T* __restrict__ w0, const T* __restrict__ u2,
                                              const T* __restrict__ v2, double* __restrict__ partials,
@@ -661,20 +713,23 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
 #pragma unroll
       for (int r = 0; r < B; r++) acc[r] = (T)0;
       const int k0 = sptr[s], k1 = sptr[s + 1];
-      for (int k = k0; k < k1; k++) {
-        const int c = col[(size_t)k * 64 + lane];
-        if (c < 0) break;
-        T a[BB], xv[B];
-#pragma unroll
-        for (int e = 0; e < BB; e++) a[e] = val[((size_t)k * BB + e) * 64 + lane];
-#pragma unroll
-        for (int e = 0; e < B; e++) xv[e] = x[(size_t)c * B + e];
-        blk_madd<B, T>(acc, a, xv);
+      int k = k0;
+      for (; k + 1 < k1; k += 2) {
+        T a0[BB], x0[B], a1[BB], x1[B];
+        sell_slot<B, TS, T>(col, val, x, k, lane, a0, x0);
+        sell_slot<B, TS, T>(col, val, x, k + 1, lane, a1, x1);
+        blk_madd<B, T>(acc, a0, x0);
+        blk_madd<B, T>(acc, a1, x1);
+      }
+      if (k < k1) {
+        T a0[BB], x0[B];
+        sell_slot<B, TS, T>(col, val, x, k, lane, a0, x0);
+        blk_madd<B, T>(acc, a0, x0);
       }
       {
         T a[BB], xv[B];
 #pragma unroll
-        for (int e = 0; e < BB; e++) a[e] = dval[((size_t)s * BB + e) * 64 + lane];
+        for (int e = 0; e < BB; e++) a[e] = cvt<T>(ld_stream(dval + ((size_t)s * BB + e) * 64 + lane));
 #pragma unroll
         for (int e = 0; e < B; e++) xv[e] = x[(size_t)row * B + e];
         blk_madd<B, T>(acc, a, xv);
@@ -701,10 +756,10 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
 // rhs may alias out.  Upper: v_i -= sum U_ij v_j in place; with DOT the rows
 // finished here contribute <r,v> partials (slot part0 + workgroup) to a
 // reduction that spans all level launches of the sweep.
-template <int B, typename T>
+template <int B, typename TS, typename T>
 __global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* __restrict__ rows,
                                                    const int* __restrict__ sptr, const int* __restrict__ col,
-                                                   const T* __restrict__ val, const T* __restrict__ dval,
+                                                   const TS* __restrict__ val, const TS* __restrict__ dval,
                                                    const T* rhs, T* out, const KState* __restrict__ st) {
   if (st && st->done) return;
   constexpr int BB = B * B;
@@ -717,28 +772,31 @@ __global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* _
 #pragma unroll
   for (int r = 0; r < B; r++) t[r] = rhs[(size_t)row * B + r];
   const int k0 = sptr[s], k1 = sptr[s + 1];
-  for (int k = k0; k < k1; k++) {
-    const int c = col[(size_t)k * 64 + lane];
-    if (c < 0) break;
-    T a[BB], v[B];
-#pragma unroll
-    for (int e = 0; e < BB; e++) a[e] = val[((size_t)k * BB + e) * 64 + lane];
-#pragma unroll
-    for (int e = 0; e < B; e++) v[e] = out[(size_t)c * B + e];
-    blk_msub<B, T>(t, a, v);
+  int k = k0;
+  for (; k + 1 < k1; k += 2) {
+    T a0[BB], v0[B], a1[BB], v1[B];
+    sell_slot<B, TS, T>(col, val, out, k, lane, a0, v0);
+    sell_slot<B, TS, T>(col, val, out, k + 1, lane, a1, v1);
+    blk_msub<B, T>(t, a0, v0);
+    blk_msub<B, T>(t, a1, v1);
+  }
+  if (k < k1) {
+    T a0[BB], v0[B];
+    sell_slot<B, TS, T>(col, val, out, k, lane, a0, v0);
+    blk_msub<B, T>(t, a0, v0);
   }
   T d[BB], y[B];
 #pragma unroll
-  for (int e = 0; e < BB; e++) d[e] = dval[((size_t)s * BB + e) * 64 + lane];
+  for (int e = 0; e < BB; e++) d[e] = cvt<T>(ld_stream(dval + ((size_t)s * BB + e) * 64 + lane));
   blk_mul<B, T>(y, d, t);
 #pragma unroll
   for (int r = 0; r < B; r++) out[(size_t)row * B + r] = y[r];
 }
 
-template <int B, typename T, bool DOT>
+template <int B, typename TS, typename T, bool DOT>
 __global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const int* __restrict__ rows,
                                                    const int* __restrict__ sptr, const int* __restrict__ col,
-                                                   const T* __restrict__ val, T* vec, const T* __restrict__ rvec,
+                                                   const TS* __restrict__ val, T* vec, const T* __restrict__ rvec,
                                                    double* __restrict__ partials, int part0, int nparts,
                                                    const KState* __restrict__ st) {
   if (st && st->done) return;
@@ -753,15 +811,18 @@ __global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const int* _
 #pragma unroll
       for (int r = 0; r < B; r++) t[r] = vec[(size_t)row * B + r];
       const int k0 = sptr[s], k1 = sptr[s + 1];
-      for (int k = k0; k < k1; k++) {
-        const int c = col[(size_t)k * 64 + lane];
-        if (c < 0) break;
-        T a[BB], v[B];
-#pragma unroll
-        for (int e = 0; e < BB; e++) a[e] = val[((size_t)k * BB + e) * 64 + lane];
-#pragma unroll
-        for (int e = 0; e < B; e++) v[e] = vec[(size_t)c * B + e];
-        blk_msub<B, T>(t, a, v);
+      int k = k0;
+      for (; k + 1 < k1; k += 2) {
+        T a0[BB], v0[B], a1[BB], v1[B];
+        sell_slot<B, TS, T>(col, val, vec, k, lane, a0, v0);
+        sell_slot<B, TS, T>(col, val, vec, k + 1, lane, a1, v1);
+        blk_msub<B, T>(t, a0, v0);
+        blk_msub<B, T>(t, a1, v1);
+      }
+      if (k < k1) {
+        T a0[BB], v0[B];
+        sell_slot<B, TS, T>(col, val, vec, k, lane, a0, v0);
+        blk_msub<B, T>(t, a0, v0);
       }
 #pragma unroll
       for (int r = 0; r < B; r++) {
@@ -882,16 +943,22 @@ __global__ void k_unpermute(int nb, int b, const int* __restrict__ order, const 
   const long dst = order ? order[i] : i;
   out[(size_t)dst * b + r] = in[t];
 }
-// Reliable update of the mixed-precision solvers: x64 += (double)x32; x32 = 0.
-static __global__ void k_reliable_update(long n, float* __restrict__ x32, double* __restrict__ x64) {
+// Reliable update of the mixed-precision solvers: x64 += (double)xr; xr = 0.
+template <typename T>
+__global__ void k_reliable_update(long n, T* __restrict__ xr, double* __restrict__ x64) {
   for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
-    x64[i] = x64[i] + (double)x32[i];
-    x32[i] = 0.0f;
+    x64[i] = x64[i] + (double)xr[i];
+    xr[i] = (T)0;
   }
 }
-// y32 = (float)x64
-static __global__ void k_to_f32(long n, const double* __restrict__ x, float* __restrict__ y) {
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] = (float)x[i];
+// y = (T)x
+template <typename T>
+__global__ void k_convert(long n, const double* __restrict__ x, T* __restrict__ y) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] = (T)x[i];
+}
+// Plain copy used to calibrate the bandwidth ceiling and the HBM counters.
+static __global__ void __launch_bounds__(kNT) k_copy4(long n4, const float4* __restrict__ in, float4* __restrict__ out) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n4; i += (long)gridDim.x * kNT) out[i] = in[i];
 }
 template <typename T>
 __global__ void k_fill(long n, T v, T* __restrict__ x) {

@@ -261,7 +261,7 @@ def _solve(fn, conv_ratio, iteration, A, prec, r, x, precision, opts_kw):
     conv = C.c_double(conv_ratio)
     it = C.c_uint(iteration)
     opts = _lib.SolveOpts()
-    opts.precision = _lib.DFM_PREC_MIXED if precision == "mixed" else _lib.DFM_PREC_FP64
+    opts.precision = _lib.PRECISIONS[precision]
     for k, v in (opts_kw or {}).items():
         setattr(opts, k, v)
     rc = call(fn, A.h, prec.h, _dptr(r), _dptr(x), C.byref(conv), C.byref(it), C.byref(opts),

@@ -135,17 +135,32 @@ int dfm_ilu_backward(dfm_ilu *M, double *vec_dev);
 int dfm_ilu_solve_precond(dfm_ilu *M, double *vec_dev);
 
 /* ---- Krylov (LsSol::Solve_PCG / Solve_PBiCGSTAB, solver_ls_iter.cpp:100-172, 291-408) */
-enum { DFM_PREC_FP64 = 0, DFM_PREC_MIXED = 1 };
+/* Precision of a solve: the type of the Krylov vectors / recurrence, and the
+ * storage of the matrix copy and of the ILU factors streamed each iteration.
+ *   FP64             f64 vectors, f64 matrix, f64 factors (reference semantics)
+ *   MIXED            f64 vectors, f32 matrix, f32 factors
+ *   MIXED_BF16       f64 vectors, f32 matrix, bf16 factors
+ *   MIXED_F32        f32 vectors, f32 matrix, f32 factors
+ *   MIXED_F32_BF16   f32 vectors, f32 matrix, bf16 factors
+ * All MIXED modes accumulate dot products in f64 and run FP64 reliable
+ * updates: when the recursive residual has dropped by reliable_delta, the
+ * partial solution is added to the f64 solution and the true residual
+ * b - A x is recomputed with the f64 matrix and replaces the recursive one.
+ * Convergence is decided on that f64 true residual. */
+enum {
+    DFM_PREC_FP64 = 0,
+    DFM_PREC_MIXED = 1,
+    DFM_PREC_MIXED_BF16 = 2,
+    DFM_PREC_MIXED_F32 = 3,
+    DFM_PREC_MIXED_F32_BF16 = 4
+};
 typedef struct {
-    int precision;         /* DFM_PREC_FP64: all FP64 (reference semantics).
-                              DFM_PREC_MIXED: FP32 matrix, factor and inner Krylov with FP64
-                              dot accumulators; FP64 outer iterative refinement */
-    double inner_tol;      /* MIXED: relative tolerance of each inner solve (0: default 1e-4) */
-    unsigned max_outer;    /* MIXED: maximum refinement steps (0: default 30) */
+    int precision;         /* DFM_PREC_* (above) */
     unsigned check_every;  /* iterations between host convergence checks (0: default 16) */
     int want_true_relres;  /* nonzero: compute true_relres = ||b - A x|| / ||b|| in FP64 at the end */
-    unsigned inner_iters;  /* out: total inner iterations (MIXED) */
-    unsigned outer_iters;  /* out: refinement steps taken (MIXED) */
+    double reliable_delta; /* MIXED*: residual drop between FP64 reliable updates (0: default 0.1) */
+    unsigned inner_iters;  /* out: Krylov iterations (MIXED*) */
+    unsigned outer_iters;  /* out: FP64 reliable updates (true residual + solution update) (MIXED*) */
     double true_relres;    /* out: ||b - A x|| / ||b|| in FP64 (when want_true_relres, or MIXED) */
 } dfm_solve_opts;
 /* A x = r with x := 0 start.  r_dev: in = right-hand side (vector id -1), out = final
@@ -166,6 +181,9 @@ int dfm_solve_pbicgstab(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
  * (values + indices + one pass over each vector operand). */
 int dfm_bench_kernel(dfm_bcrs *A, dfm_ilu *M, int which, int precision, int nrep,
                      double *avg_ms, double *bytes);
+/* Device-to-device copy of `bytes` (16-byte vector loads and stores) with the
+ * library's launch shape: the practical bandwidth ceiling of a streaming kernel. */
+int dfm_bench_copy(size_t bytes, int nrep, double *avg_ms);
 
 #ifdef __cplusplus
 }

@@ -136,31 +136,31 @@ def test_krylov_fp64(setup, orc, kind, ordering):
     assert true_relres(s, x, ref["res"]) < 1e-9
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed_bf16", "mixed_f32", "mixed_f32_bf16"])
+@pytest.mark.parametrize("kind", ["pcg", "bicg"])
 @pytest.mark.parametrize("ordering", ["natural", "multicolour"])
-def test_krylov_mixed(setup, orc, ordering):
-    """FP32 inner PCG + FP64 refinement reaches the FP64 tolerance on the true
-    residual.  Its error against a 1e-13 FP64 solve is bounded by
-    cond(A) * relres; the stated bound for these small systems is 1e-5
-    (measured ~7e-7 for the 2-D case, whose smooth modes keep a small residual)."""
+def test_krylov_mixed(setup, orc, ordering, kind, precision):
+    """Low-precision recurrence + FP64 reliable updates reaches the FP64
+    tolerance on the true residual (checked independently with the canonical
+    FP64 MatVec).  Its error against a 1e-13 FP64 solve is bounded by
+    cond(A) * relres; the stated bound for these small systems is 1e-5."""
     prob, ref, s, gpu, torch = setup["prob"], setup["ref"], setup["sys"], setup["gpu"], setup["torch"]
-    _case(f"mixed pcg {ordering}")
+    _case(f"{kind} {precision} {ordering}")
     M = gpu.CMatDiaFrac_BlkCrs(0, s.A, ordering=ordering)
     assert M.SetValue(s.A)
+    fn = gpu.Solve_PCG if kind == "pcg" else gpu.Solve_PBiCGSTAB
 
-    def run(tol, precision):
+    def run(tol, prec):
         r = torch.from_numpy(ref["res"].copy()).cuda()
         x = torch.zeros_like(r)
-        out = gpu.Solve_PCG(tol, 4000, s.A, M, r, x, precision=precision, want_true_relres=1)
+        out = fn(tol, 4000, s.A, M, r, x, precision=prec, want_true_relres=1)
         return out, x.cpu().numpy(), true_relres(s, x, ref["res"])
 
-    (ok, conv, it, info), xm, rel = run(1e-10, "mixed")
+    (ok, conv, it, info), xm, rel = run(1e-10, precision)
     assert ok and conv < 1e-10 and info["true_relres"] < 1e-10
     assert rel < 1.5e-10, rel
     assert info["outer_iters"] >= 1
-    (ok64, _, _, info64), xd, _ = run(1e-10, "fp64")
     (okx, _, _, _), xx, _ = run(1e-13, "fp64")
-    assert ok64 and okx
+    assert okx
     err_m = np.linalg.norm(xm - xx) / np.linalg.norm(xx)
-    err_d = np.linalg.norm(xd - xx) / np.linalg.norm(xx)
-    assert err_m <= 1e-5, (err_m, err_d)
-    assert err_d <= 1e-6, (err_m, err_d)
+    assert err_m <= 1e-5, err_m
