@@ -1,0 +1,133 @@
+"""The oracle (C restatement) against the golden fixtures written by the
+compiled DelFEM reference (tests/golden/make_golden.py).  Everything is
+compared bitwise: integer patterns, FP64 element matrices, assembled values,
+ILU factors, substitutions, MatVec and whole PCG / BiCGSTAB runs (iteration
+count, conv_ratio, solution and final residual)."""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = ["poisson2d_8", "solid3d_3", "graph60_b2"]
+
+
+def load(name):
+    return np.load(os.path.join(HERE, "golden", f"{name}.npz"), allow_pickle=False)
+
+
+def orders_of(d):
+    out = {"natural": None}
+    for k in d.files:
+        if k.startswith("order_"):
+            out[k[len("order_"):]] = d[k]
+    return out
+
+
+def system(d):
+    return int(d["nnode"]), int(d["b"]), d["rowptr"], d["col"], d["val"], d["dia"]
+
+
+@pytest.mark.parametrize("name", ["poisson2d_8", "solid3d_3"])
+def test_element_kernels(orc, name):
+    d = load(name)
+    ne = int(d["nemat"])
+    conn = d["conn"][:ne]
+    if str(d["kind"]) == "poisson2d":
+        emat, eres = orc.emat_poisson2d(d["coord"], conn, d["alpha"][:ne], float(d["source"]), np.zeros(int(d["nnode"])))
+    else:
+        emat, eres = orc.emat_solid3d(d["coord"], conn, d["lam"][:ne], d["mu"][:ne], float(d["rho"]), tuple(d["g"]),
+                                      np.zeros(int(d["nnode"]) * 3))
+    assert np.array_equal(emat, d["emat"])
+    assert np.array_equal(eres, d["eres"])
+
+
+@pytest.mark.parametrize("name", ["poisson2d_8", "solid3d_3"])
+def test_pattern_assembly_bc(orc, name):
+    d = load(name)
+    nb, b = int(d["nnode"]), int(d["b"])
+    rowptr, col = orc.fem_pattern(nb, d["conn"])
+    assert np.array_equal(rowptr, d["rowptr"]) and np.array_equal(col, d["col"])
+    if str(d["kind"]) == "poisson2d":
+        emat, eres = orc.emat_poisson2d(d["coord"], d["conn"], d["alpha"], float(d["source"]), np.zeros(nb))
+    else:
+        emat, eres = orc.emat_solid3d(d["coord"], d["conn"], d["lam"], d["mu"], float(d["rho"]), tuple(d["g"]),
+                                      np.zeros(nb * 3))
+    val, dia = orc.assemble(nb, b, rowptr, col, d["conn"], emat)
+    res = orc.assemble_vec(nb, b, d["conn"], eres)
+    val, dia = orc.set_bc(nb, b, rowptr, col, val, dia, d["flags"])
+    res = orc.zero_bc(d["flags"], res)
+    assert np.array_equal(val, d["val"])
+    assert np.array_equal(dia, d["dia"])
+    assert np.array_equal(res, d["res"])
+    assert float(np.sqrt(np.sum(res * res))) == pytest.approx(float(d["norm_res"]), rel=1e-15)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_matvec(orc, name):
+    d = load(name)
+    nb, b, rowptr, col, val, dia = system(d)
+    y = orc.matvec(nb, b, rowptr, col, val, dia, float(d["mv_alpha"]), d["mv_x"], float(d["mv_beta"]), d["mv_y"])
+    assert np.array_equal(y, d["mv_out"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_iluk_patterns(orc, name):
+    d = load(name)
+    nb, b, rowptr, col, val, dia = system(d)
+    n_checked = 0
+    for oname, order in orders_of(d).items():
+        for lev in range(4):
+            tag = f"{oname}_k{lev}"
+            if f"frow_{tag}" not in d.files:
+                continue
+            F = orc.iluk_symbolic(rowptr, col, lev, order)
+            assert np.array_equal(F["rowptr"], d[f"frow_{tag}"]), tag
+            assert np.array_equal(F["col"], d[f"fcol_{tag}"]), tag
+            assert np.array_equal(F["diaind"], d[f"fdia_{tag}"]), tag
+            n_checked += 1
+    assert n_checked >= 8
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_ilu_values_and_substitutions(orc, name):
+    d = load(name)
+    nb, b, rowptr, col, val, dia = system(d)
+    n_checked = 0
+    for oname, order in orders_of(d).items():
+        for lev in range(2):
+            tag = f"{oname}_k{lev}"
+            if f"fval_{tag}" not in d.files:
+                continue
+            F = orc.iluk_symbolic(rowptr, col, lev, order)
+            fv, fd, ns = orc.ilu_values(nb, b, rowptr, col, val, dia, F, order)
+            assert bool(d[f"fok_{tag}"]) == (ns >= 0)
+            assert np.array_equal(fv, d[f"fval_{tag}"]), tag
+            assert np.array_equal(fd, d[f"fdiav_{tag}"]), tag
+            assert np.array_equal(orc.forward(nb, b, F, fv, fd, d["vin"]), d[f"fwd_{tag}"]), tag
+            assert np.array_equal(orc.backward(nb, b, F, fv, fd, d["vin"]), d[f"bwd_{tag}"]), tag
+            n_checked += 1
+    assert n_checked >= 3
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("kind", ["pcg", "bicg"])
+def test_solvers(orc, name, kind):
+    d = load(name)
+    nb, b, rowptr, col, val, dia = system(d)
+    n_checked = 0
+    for oname, order in orders_of(d).items():
+        for lev in range(2):
+            tag = f"{oname}_k{lev}"
+            if f"{kind}_{tag}" not in d.files:
+                continue
+            F = orc.iluk_symbolic(rowptr, col, lev, order)
+            fv, fd, _ = orc.ilu_values(nb, b, rowptr, col, val, dia, F, order)
+            ok, conv, it, x, r = orc.solve(kind, nb, b, rowptr, col, val, dia, F, fv, fd, d["res"], 1e-10, 2000,
+                                           order)
+            want = d[f"{kind}_{tag}"]
+            assert (float(ok), conv, float(it)) == (want[0], want[1], want[2]), tag
+            assert np.array_equal(x, d[f"{kind}_x_{tag}"]), tag
+            assert np.array_equal(r, d[f"{kind}_r_{tag}"]), tag
+            n_checked += 1
+    assert n_checked >= 3

@@ -1,0 +1,293 @@
+"""The oracle pinned by mathematics, independently of the reference: closed
+forms, invariants, library routines (scipy) and brute force on small inputs,
+chosen so that a dropped term, a wrong sign or index or a transposed operand
+fails at least one check."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from delfem_amd import meshgen
+from oracle.oracle import assemble_problem, problem_emats
+
+
+def bsr_of(nb, b, rowptr, col, val, dia):
+    """Dense-able scipy BSR of a block-CRS matrix with separate diagonal."""
+    rows = np.repeat(np.arange(nb), np.diff(rowptr.astype(np.int64)))
+    allr = np.concatenate([rows, np.arange(nb)])
+    allc = np.concatenate([col.astype(np.int64), np.arange(nb)])
+    blocks = np.concatenate([val.reshape(-1, b, b), dia.reshape(-1, b, b)])
+    A = sp.lil_matrix((nb * b, nb * b))
+    coo_r, coo_c, coo_v = [], [], []
+    for (I, J, blk) in zip(allr, allc, blocks):
+        for r in range(b):
+            for c in range(b):
+                coo_r.append(I * b + r)
+                coo_c.append(J * b + c)
+                coo_v.append(blk[r, c])
+    return sp.csr_matrix((coo_v, (coo_r, coo_c)), shape=(nb * b, nb * b))
+
+
+@pytest.fixture(scope="module")
+def poisson():
+    return meshgen.poisson2d(10, seed=5)
+
+
+@pytest.fixture(scope="module")
+def solid():
+    return meshgen.solid3d(3, seed=5, sigma=0.7)
+
+
+def test_fem_pattern_bruteforce(orc, poisson, solid):
+    for prob in (poisson, solid):
+        rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)
+        adj = [set() for _ in range(prob.nnode)]
+        for el in prob.conn:
+            for a in el:
+                for c in el:
+                    if a != c:
+                        adj[a].add(int(c))
+        want_ptr = np.concatenate([[0], np.cumsum([len(s) for s in adj])])
+        want_col = np.concatenate([sorted(s) for s in adj])
+        assert np.array_equal(rowptr, want_ptr) and np.array_equal(col, want_col)
+
+
+def test_poisson_element_closed_form(orc):
+    coord = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    conn = np.array([[0, 1, 2]], dtype=np.uint32)
+    emat, eres = orc.emat_poisson2d(coord, conn, np.array([2.0]), 3.0, np.array([1.0, 2.0, 4.0]))
+    K = 2.0 * 0.5 * np.array([[2.0, -1.0, -1.0], [-1.0, 1.0, 0.0], [-1.0, 0.0, 1.0]])
+    assert np.allclose(emat.reshape(3, 3), K, rtol=1e-14, atol=0)
+    f = np.full(3, 3.0 * 0.5 / 3.0) - K @ np.array([1.0, 2.0, 4.0])
+    assert np.allclose(eres, f, rtol=1e-13)
+
+
+def test_poisson_element_invariants(orc, poisson):
+    emat, _ = orc.emat_poisson2d(poisson.coord, poisson.conn, poisson.alpha, 1.0, np.zeros(poisson.nnode))
+    K = emat.reshape(-1, 3, 3)
+    assert np.allclose(K.sum(axis=2), 0.0, atol=1e-13)          # constants in the null space
+    assert np.allclose(K, K.transpose(0, 2, 1), atol=0)          # symmetric
+    assert np.all(np.linalg.eigvalsh(K)[:, 1:] > 0)              # rank 2, PSD
+
+
+def test_solid_element_invariants(orc, solid):
+    emat, eres = orc.emat_solid3d(solid.coord, solid.conn, solid.lam, solid.mu, 1.0, (0.0, 0.0, -1.0),
+                                  np.zeros(solid.nnode * 3))
+    K = emat.reshape(-1, 4, 4, 3, 3).transpose(0, 1, 3, 2, 4).reshape(-1, 12, 12)
+    assert np.allclose(K, K.transpose(0, 2, 1), rtol=1e-12, atol=1e-15)
+    # rigid-body modes: 3 translations, 3 infinitesimal rotations
+    for e in range(K.shape[0]):
+        X = solid.coord[solid.conn[e]]
+        modes = []
+        for d in range(3):
+            t = np.zeros((4, 3))
+            t[:, d] = 1.0
+            modes.append(t.ravel())
+        for w in np.eye(3):
+            modes.append(np.cross(w, X).ravel())
+        scale = np.abs(K[e]).max()
+        for m in modes:
+            assert np.abs(K[e] @ m).max() < 1e-11 * scale * max(1.0, np.abs(m).max())
+    ev = np.linalg.eigvalsh(K)
+    assert np.all(ev[:, 6:] > 0)                                 # exactly 6 zero modes
+    # body force: equal nodal shares of vol * rho * g
+    vol = np.abs(np.einsum("ij,ij->i", solid.coord[solid.conn[:, 1]] - solid.coord[solid.conn[:, 0]],
+                           np.cross(solid.coord[solid.conn[:, 2]] - solid.coord[solid.conn[:, 0]],
+                                    solid.coord[solid.conn[:, 3]] - solid.coord[solid.conn[:, 0]]))) / 6.0
+    assert np.allclose(eres.reshape(-1, 4, 3)[:, :, 2], -np.repeat(vol[:, None], 4, 1) / 4.0, rtol=1e-10)
+
+
+def test_assembly_vs_scipy(orc, poisson, solid):
+    for prob in (poisson, solid):
+        rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)
+        emat, eres = problem_emats(orc, prob)
+        val, dia = orc.assemble(prob.nnode, prob.b, rowptr, col, prob.conn, emat)
+        A = bsr_of(prob.nnode, prob.b, rowptr, col, val, dia).toarray()
+        b, npe = prob.b, prob.npe
+        E = emat.reshape(-1, npe, npe, b, b)
+        want = np.zeros_like(A)
+        for e, el in enumerate(prob.conn):
+            for i in range(npe):
+                for j in range(npe):
+                    want[el[i] * b:(el[i] + 1) * b, el[j] * b:(el[j] + 1) * b] += E[e, i, j]
+        assert np.allclose(A, want, rtol=1e-13, atol=1e-14 * np.abs(want).max())
+
+
+def test_bc_closed_form(orc, solid):
+    A = assemble_problem(orc, solid)
+    M = bsr_of(solid.nnode, 3, A["rowptr"], A["col"], A["val"], A["dia"]).toarray()
+    fixed = np.nonzero(solid.flags)[0]
+    for k in fixed:
+        row = M[k].copy()
+        assert row[k] == 1.0
+        row[k] = 0.0
+        assert not row.any()
+        colv = M[:, k].copy()
+        colv[k] = 0.0
+        assert not colv.any()
+    assert np.all(A["res"][fixed] == 0.0)
+
+
+def test_matvec_vs_scipy(orc, solid):
+    A = assemble_problem(orc, solid)
+    M = bsr_of(solid.nnode, 3, A["rowptr"], A["col"], A["val"], A["dia"])
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal(solid.ndof), rng.standard_normal(solid.ndof)
+    got = orc.matvec(solid.nnode, 3, A["rowptr"], A["col"], A["val"], A["dia"], 0.7, x, -1.3, y)
+    assert np.allclose(got, 0.7 * (M @ x) - 1.3 * y, rtol=1e-13, atol=1e-14)
+
+
+def dense_factors(nb, b, F, fval, fdia):
+    L = np.zeros((nb * b, nb * b))
+    U = np.eye(nb * b)
+    D = np.zeros((nb * b, nb * b))
+    V = fval.reshape(-1, b, b)
+    for i in range(nb):
+        Dinv = fdia.reshape(-1, b, b)[i]
+        D[i * b:(i + 1) * b, i * b:(i + 1) * b] = np.linalg.inv(Dinv)
+        for q in range(F["rowptr"][i], F["rowptr"][i + 1]):
+            j = F["col"][q]
+            if q < F["diaind"][i]:
+                L[i * b:(i + 1) * b, j * b:(j + 1) * b] = V[q]
+            else:
+                U[i * b:(i + 1) * b, j * b:(j + 1) * b] = V[q]
+    return L, D, U
+
+
+@pytest.mark.parametrize("ordered", [False, True])
+def test_ilu0_factor_identity(orc, solid, ordered):
+    """ILU(0): ((L+D)(I+D^-1U))_ij = A_ij on every position of the pattern."""
+    A = assemble_problem(orc, solid)
+    nb, b = solid.nnode, 3
+    order = orc.colour_order(A["rowptr"], A["col"])[2] if ordered else None
+    F = orc.iluk_symbolic(A["rowptr"], A["col"], 0, order)
+    fv, fd, ns = orc.ilu_values(nb, b, A["rowptr"], A["col"], A["val"], A["dia"], F, order)
+    assert ns == 0
+    L, D, U = dense_factors(nb, b, F, fv, fd)
+    LU = (L + D) @ U
+    Ad = bsr_of(nb, b, A["rowptr"], A["col"], A["val"], A["dia"]).toarray()
+    if ordered:
+        P = np.concatenate([np.arange(o * b, (o + 1) * b) for o in order])
+        Ad = Ad[np.ix_(P, P)]
+    mask = np.kron(bsr_of(nb, 1, F["rowptr"], F["col"], np.ones(F["col"].size), np.ones(nb)).toarray() != 0,
+                   np.ones((b, b))) != 0
+    assert np.allclose(LU[mask], Ad[mask], rtol=1e-11, atol=1e-13 * np.abs(Ad).max())
+    # and the substitutions invert exactly this product
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(nb * b)
+    y = orc.forward(nb, b, F, fv, fd, v)
+    assert np.allclose(y, np.linalg.solve(L + D, v), rtol=1e-10, atol=1e-12)  # block lower triangular
+    z = orc.backward(nb, b, F, fv, fd, y)
+    assert np.allclose(z, sla.solve_triangular(U, y, lower=False, unit_diagonal=True), rtol=1e-10, atol=1e-12)
+
+
+def textbook_iluk(rowptr, col, k):
+    """Textbook symbolic ILU(k): level(i,j) = min over paths of lev_ik + lev_kj + 1."""
+    n = rowptr.size - 1
+    rows = []
+    for i in range(n):
+        lev = {int(j): 0 for j in col[rowptr[i]:rowptr[i + 1]]}
+        done = set()
+        while True:
+            cand = [c for c in sorted(lev) if c < i and c not in done]
+            if not cand:
+                break
+            c = cand[0]
+            done.add(c)
+            for j, lkj in rows[c].items():
+                if j <= c or j == i:
+                    continue
+                v = lev[c] + lkj + 1
+                if v <= k:
+                    lev[j] = min(lev.get(j, 99), v)
+        rows.append({j: l for j, l in lev.items() if l <= k})
+    return [sorted(r) for r in rows]
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_iluk_pattern_textbook_k01(orc, poisson, k):
+    """For k <= 1 the reference's max+1 rule coincides with the textbook sum rule."""
+    A = assemble_problem(orc, poisson)
+    F = orc.iluk_symbolic(A["rowptr"], A["col"], k)
+    want = textbook_iluk(A["rowptr"], A["col"], k)
+    got = [list(F["col"][F["rowptr"][i]:F["rowptr"][i + 1]]) for i in range(poisson.nnode)]
+    assert got == want
+    for i in range(poisson.nnode):
+        r = F["col"][F["rowptr"][i]:F["rowptr"][i + 1]]
+        split = F["diaind"][i] - F["rowptr"][i]
+        assert np.all(r[:split] < i) and np.all(r[split:] > i)
+
+
+def test_pcg_bicg_vs_direct(orc, poisson, solid):
+    for prob in (poisson, solid):
+        A = assemble_problem(orc, prob)
+        nb, b = prob.nnode, prob.b
+        M = bsr_of(nb, b, A["rowptr"], A["col"], A["val"], A["dia"]).tocsc()
+        xd = spla.spsolve(M, A["res"])
+        F = orc.iluk_symbolic(A["rowptr"], A["col"], 0)
+        fv, fd, _ = orc.ilu_values(nb, b, A["rowptr"], A["col"], A["val"], A["dia"], F)
+        for kind in ("pcg", "bicg"):
+            ok, conv, it, x, r = orc.solve(kind, nb, b, A["rowptr"], A["col"], A["val"], A["dia"], F, fv, fd,
+                                           A["res"], 1e-12, 1000)
+            assert ok and conv < 1e-12
+            assert np.linalg.norm(x - xd) <= 1e-9 * np.linalg.norm(xd)
+            assert np.allclose(r, A["res"] - M @ x, atol=1e-9 * np.linalg.norm(A["res"]))
+
+
+def test_solver_return_semantics(orc, poisson):
+    A = assemble_problem(orc, poisson)
+    nb = poisson.nnode
+    F = orc.iluk_symbolic(A["rowptr"], A["col"], 0)
+    fv, fd, _ = orc.ilu_values(nb, 1, A["rowptr"], A["col"], A["val"], A["dia"], F)
+    args = (nb, 1, A["rowptr"], A["col"], A["val"], A["dia"], F, fv, fd)
+    # zero right-hand side: Solve_PCG returns true, iteration 0, conv 0 (solver_ls_iter.cpp:118-124)
+    ok, conv, it, x, _ = orc.solve("pcg", *args, np.zeros(nb), 1e-8, 100)
+    assert ok and it == 0 and conv == 0.0 and not x.any()
+    ok, conv, it, x, _ = orc.solve("bicg", *args, np.zeros(nb), 1e-8, 100)
+    assert ok and it == 0 and conv == 0.0
+    # max_iter reached: PCG returns false with the achieved ratio (:168-171);
+    # BiCGSTAB returns true, iteration = max_iter and conv_ratio unchanged (:327-407)
+    ok, conv, it, _, _ = orc.solve("pcg", *args, A["res"], 1e-14, 3)
+    assert not ok and it == 3 and conv > 1e-14
+    ok, conv, it, _, _ = orc.solve("bicg", *args, A["res"], 1e-14, 3)
+    assert ok and it == 3 and conv == 1e-14
+
+
+@pytest.mark.parametrize("nzero", [3, 10, 11])
+def test_singular_pivot_count(orc, nzero):
+    """Zero pivots are left un-inverted and counted; more than 10 fail
+    (matdiafrac_blkcrs.cpp:774, 880-890)."""
+    nb = 20
+    rowptr = np.arange(nb + 1, dtype=np.uint32) * 0
+    col = np.zeros(0, dtype=np.uint32)
+    F = orc.iluk_symbolic(rowptr, col, 0)
+    dia = np.arange(1, nb + 1, dtype=float)
+    dia[:nzero] = 0.0
+    fv, fd, ns = orc.ilu_values(nb, 1, rowptr, col, np.zeros(0), dia, F)
+    if nzero <= 10:
+        assert ns == nzero
+        assert np.array_equal(fd[nzero:], 1.0 / dia[nzero:])
+    else:
+        assert ns == -1
+
+
+def test_colouring_and_levels(orc, solid, poisson):
+    for prob, ncol_max in ((solid, 4), (poisson, 3)):
+        rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)
+        colour, nc, order = orc.colour_order(rowptr, col)
+        rows = np.repeat(np.arange(prob.nnode), np.diff(rowptr.astype(np.int64)))
+        assert not np.any(colour[rows] == colour[col])             # proper colouring
+        assert nc == ncol_max                                       # the structured meshes' chromatic number
+        assert np.array_equal(np.sort(order), np.arange(prob.nnode))
+        assert np.all(np.diff(colour[order]) >= 0)                  # grouped by colour, stable
+        for lev in (0, 1):
+            F = orc.iluk_symbolic(rowptr, col, lev, order)
+            lv, nl = orc.level_sets(F, True)
+            uv, nu = orc.level_sets(F, False)
+            for i in range(prob.nnode):
+                Lc = F["col"][F["rowptr"][i]:F["diaind"][i]]
+                Uc = F["col"][F["diaind"][i]:F["rowptr"][i + 1]]
+                assert np.all(lv[Lc] < lv[i]) and np.all(uv[Uc] < uv[i])
+            if lev == 0:
+                assert nl == nc and nu == nc                         # one level per colour
