@@ -22,6 +22,7 @@ DFM_TOO_LARGE = -5
 DFM_ORDER_NATURAL = 0
 DFM_ORDER_GIVEN = 1
 DFM_ORDER_MULTICOLOUR = 2
+DFM_ORDER_BLOCK_MULTICOLOUR = 3
 
 DFM_PREC_FP64 = 0
 DFM_PREC_MIXED = 1
@@ -84,7 +85,9 @@ _SIGS = {
     "dfm_asm_destroy": (C.c_int, [H]),
     "dfm_assemble_poisson2d_p1": (C.c_int, [H, vp, vp, C.c_double, vp, vp]),
     "dfm_assemble_solid3d_p1": (C.c_int, [H, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double, vp, vp]),
-    "dfm_ilu_create": (C.c_int, [H, C.c_int, C.c_int, vp, C.POINTER(H)]),
+    "dfm_assemble_oseen2d_p1p1": (C.c_int, [H, vp, C.c_double, C.c_double, vp, vp, vp]),
+    "dfm_assemble_helmholtz2d_p1": (C.c_int, [H, vp, vp, vp, C.c_double, vp, vp]),
+    "dfm_ilu_create": (C.c_int, [H, C.c_int, C.c_int, vp, C.c_uint, C.POINTER(H)]),
     "dfm_ilu_destroy": (C.c_int, [H]),
     "dfm_ilu_info": (C.c_int, [H, u32p, i32p, i32p, i32p]),
     "dfm_ilu_get_pattern": (C.c_int, [H, vp, vp, vp, vp, vp]),

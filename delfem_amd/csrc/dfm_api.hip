@@ -466,6 +466,29 @@ int dfm_assemble_solid3d_p1(dfm_asm* p, const double* coord, const double* lambd
   A->version++;
   return DFM_OK;
 }
+int dfm_assemble_oseen2d_p1p1(dfm_asm* p, const double* coord, double myu, double rho, const double* adv,
+                              const double* value, double* res) {
+  DFM_CHECK(p && p->npe == 3 && p->A->b == 3 && coord && adv && value, DFM_BAD_ARG,
+            "dfm_assemble_oseen2d_p1p1 needs a 3-node plan on a 3x3 block matrix");
+  DFM_CHECK(myu > 0.0 && rho > 0.0, DFM_BAD_ARG, "dfm_assemble_oseen2d_p1p1 needs myu > 0 and rho > 0");
+  dfm_bcrs_s* A = p->A;
+  DFM_LAUNCH(k_asm_oseen2d, grid_for(p->nnode), (int)p->nnode, p->d_conn.p, p->d_elsup_ptr.p, p->d_elsup.p,
+             A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, coord, myu, rho, adv, value, res);
+  DFM_HIP(hipStreamSynchronize(g_stream));
+  A->version++;
+  return DFM_OK;
+}
+int dfm_assemble_helmholtz2d_p1(dfm_asm* p, const double* coord, const double* alpha, const double* cmass,
+                                double source, const double* value, double* res) {
+  DFM_CHECK(p && p->npe == 3 && p->A->b == 1 && coord && alpha && cmass && value, DFM_BAD_ARG,
+            "dfm_assemble_helmholtz2d_p1 needs a 3-node plan on a 1x1 block matrix");
+  dfm_bcrs_s* A = p->A;
+  DFM_LAUNCH(k_asm_helmholtz2d, grid_for(p->nnode), (int)p->nnode, p->d_conn.p, p->d_elsup_ptr.p, p->d_elsup.p,
+             A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, coord, alpha, cmass, source, value, res);
+  DFM_HIP(hipStreamSynchronize(g_stream));
+  A->version++;
+  return DFM_OK;
+}
 
 // ---------------------------------------------------------------------------
 // ILU
@@ -483,9 +506,11 @@ static int upload_sell_plan(const SellPlan& plan, int bb, Sell<double>& s64, Dev
   return DFM_OK;
 }
 
-int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* order, dfm_ilu** out) {
+int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* order, unsigned block_nodes,
+                   dfm_ilu** out) {
   DFM_CHECK(A && out && lev_fill >= -1, DFM_BAD_ARG, "bad arguments to dfm_ilu_create");
-  DFM_CHECK(ordering == DFM_ORDER_NATURAL || ordering == DFM_ORDER_GIVEN || ordering == DFM_ORDER_MULTICOLOUR,
+  DFM_CHECK(ordering == DFM_ORDER_NATURAL || ordering == DFM_ORDER_GIVEN || ordering == DFM_ORDER_MULTICOLOUR ||
+                ordering == DFM_ORDER_BLOCK_MULTICOLOUR,
             DFM_BAD_ARG, "unknown ordering %d", ordering);
   DFM_CHECK(ordering != DFM_ORDER_GIVEN || order, DFM_BAD_ARG, "DFM_ORDER_GIVEN needs an order array");
   DFM_TRY(ensure_init());
@@ -509,6 +534,10 @@ int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* ord
     std::vector<int> colour;
     M->ncolour = greedy_colour(nb, A->rowptr, A->col, colour);
     colour_order(nb, colour, M->ncolour, M->order);
+    M->permuted = true;
+  } else if (ordering == DFM_ORDER_BLOCK_MULTICOLOUR) {
+    int nblock = 0;
+    M->ncolour = bmc_order(nb, A->rowptr, A->col, block_nodes ? block_nodes : 32u, M->order, nblock);
     M->permuted = true;
   } else {
     M->order.resize(nb);

@@ -216,6 +216,68 @@ void colour_order(unsigned nb, const std::vector<int>& colour, int ncolour, std:
   for (unsigned i = 0; i < nb; i++) order[cnt[colour[i]]++] = (int)i;
 }
 
+// Block multicolour ordering: nodes are aggregated into connected blocks of
+// at most block_nodes nodes (breadth-first from the lowest unassigned node,
+// neighbours in ascending order), the block adjacency graph is coloured with
+// greedy_colour, and nodes are ordered by (block colour, block, node).  Rows
+// inside a block keep their natural relative order, so the factor keeps most
+// of the natural-order ILU quality while blocks of one colour are independent.
+int bmc_order(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
+              unsigned block_nodes, std::vector<int>& order, int& nblock) {
+  std::vector<int> blk(nb, -1);
+  std::vector<unsigned> queue;
+  queue.reserve(block_nodes);
+  int nbk = 0;
+  for (unsigned seed = 0; seed < nb; seed++) {
+    if (blk[seed] >= 0) continue;
+    queue.clear();
+    queue.push_back(seed);
+    blk[seed] = nbk;
+    for (size_t h = 0; h < queue.size() && queue.size() < block_nodes; h++) {
+      const unsigned u = queue[h];
+      for (unsigned q = rowptr[u]; q < rowptr[u + 1] && queue.size() < block_nodes; q++) {
+        const unsigned v = col[q];
+        if (blk[v] >= 0) continue;
+        blk[v] = nbk;
+        queue.push_back(v);
+      }
+    }
+    nbk++;
+  }
+  // block adjacency graph, rows ascending, self excluded
+  std::vector<std::vector<unsigned>> adj(nbk);
+  for (unsigned u = 0; u < nb; u++)
+    for (unsigned q = rowptr[u]; q < rowptr[u + 1]; q++)
+      if (blk[col[q]] != blk[u]) adj[blk[u]].push_back((unsigned)blk[col[q]]);
+  std::vector<unsigned> qrow(nbk + 1, 0), qcol;
+  for (int i = 0; i < nbk; i++) {
+    std::sort(adj[i].begin(), adj[i].end());
+    adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+    qrow[i + 1] = qrow[i] + (unsigned)adj[i].size();
+  }
+  qcol.reserve(qrow[nbk]);
+  for (int i = 0; i < nbk; i++) qcol.insert(qcol.end(), adj[i].begin(), adj[i].end());
+  std::vector<int> bcol;
+  const int ncol = greedy_colour((unsigned)nbk, qrow, qcol, bcol);
+  // counting sort by colour, then block, then node
+  std::vector<unsigned> bptr(nbk + 1, 0);
+  for (unsigned u = 0; u < nb; u++) bptr[blk[u] + 1]++;
+  for (int i = 0; i < nbk; i++) bptr[i + 1] += bptr[i];
+  std::vector<int> bynode(nb);
+  {
+    std::vector<unsigned> fill(bptr.begin(), bptr.end() - 1);
+    for (unsigned u = 0; u < nb; u++) bynode[fill[blk[u]]++] = (int)u;
+  }
+  std::vector<int> blocks_by_colour;
+  colour_order((unsigned)nbk, bcol, ncol, blocks_by_colour);
+  order.clear();
+  order.reserve(nb);
+  for (int k : blocks_by_colour)
+    for (unsigned t = bptr[k]; t < bptr[k + 1]; t++) order.push_back(bynode[t]);
+  nblock = nbk;
+  return ncol;
+}
+
 // Level sets of the factor DAG: lower -> longest chain through L(i),
 // upper -> longest chain through U(i) taken from the last row.
 int level_sets(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,

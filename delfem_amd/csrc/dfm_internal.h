@@ -92,6 +92,8 @@ int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::v
 int greedy_colour(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
                   std::vector<int>& colour);
 void colour_order(unsigned nb, const std::vector<int>& colour, int ncolour, std::vector<int>& order);
+int bmc_order(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
+              unsigned block_nodes, std::vector<int>& order, int& nblock);
 int level_sets(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
                const std::vector<unsigned>& fdiaind, bool lower, std::vector<int>& level);
 
@@ -125,6 +127,7 @@ struct Sell {
   DevBuf<T> val, dval;  // val: nk*bb*kSlice, dval: nslice*bb*kSlice
   std::vector<int> group_slice_ptr;
 };
+
 
 // Deterministic reduction state shared by the Krylov kernels.
 struct KState {

@@ -463,6 +463,191 @@ static __global__ void __launch_bounds__(kNT) k_asm_solid3d(int nnode, const uns
     for (int x = 0; x < 3; x++) res[(size_t)i * 3 + x] = racc[x];
 }
 
+// Steady Oseen P1-P1 with PSPG, combined (u,v,p) 3x3 blocks: viscous,
+// Galerkin advection, pressure/continuity and PSPG terms of
+// src/femeqn/eqn_navier_stokes.cpp:73-110, 231-253 with a prescribed advection
+// field, stabilisation parameter of :137-158 without the Courant term; block
+// layout of the combined Stokes assembly (src/femeqn/eqn_stokes.cpp:317-331).
+// Row of node i: the 3 blocks E[a][j] of each element around i, and its
+// residual share -E[a][.] * value.
+__device__ inline void oseen_row(const double* p0, const double* p1, const double* p2, const double (&a)[3][2],
+                                 int ia, double myu, double rho, double (&E)[3][9]) {
+  const double area = d_tri_area(p0, p1, p2);
+  double dl[3][2];
+  d_tri_dldx(dl, p0, p1, p2);
+  const int i = ia;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    E[j][0] = area * myu * dl[i][0] * dl[j][0];
+    E[j][1] = area * myu * dl[i][1] * dl[j][0];
+    E[j][3] = area * myu * dl[i][0] * dl[j][1];
+    E[j][4] = area * myu * dl[i][1] * dl[j][1];
+    const double t = area * myu * (dl[j][0] * dl[i][0] + dl[j][1] * dl[i][1]);
+    E[j][0] += t;
+    E[j][4] += t;
+  }
+  {
+    const double s0 = a[0][0] + a[1][0] + a[2][0], s1 = a[0][1] + a[1][1] + a[2][1];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double t1 = dl[j][0] * s0 + dl[j][1] * s1;
+      double t2 = t1 + (dl[j][0] * a[i][0] + dl[j][1] * a[i][1]);
+      t2 *= area * rho * 0.083333333333333;
+      E[j][0] += t2;
+      E[j][4] += t2;
+    }
+  }
+  {
+    const double t = area * 0.33333333333333333;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      E[j][2] = -t * dl[i][0];
+      E[j][5] = -t * dl[i][1];
+      E[j][6] = t * dl[j][0];
+      E[j][7] = t * dl[j][1];
+    }
+  }
+  double tau;
+  {
+    const double va0 = (a[0][0] + a[1][0] + a[2][0]) / 3.0, va1 = (a[0][1] + a[1][1] + a[2][1]) / 3.0;
+    const double norm_v = sqrt(va0 * va0 + va1 * va1);
+    const double h = sqrt(area / 3.14) * 2;
+    if (norm_v * h * rho * 1.0e-30 > myu) {
+      tau = h * 0.5 / norm_v;
+    } else if (norm_v * h * rho < myu * 1.0e-30) {
+      tau = h * h * rho * 0.5 / myu;
+    } else {
+      const double re_c = 0.5 * norm_v * h * rho / myu;
+      tau = (h * 0.5 / norm_v) / sqrt(1 + 1 / (re_c * re_c));
+    }
+  }
+  {
+    double av0 = a[0][0] + a[1][0] + a[2][0], av1 = a[0][1] + a[1][1] + a[2][1];
+    av0 *= 0.3333333333333333333;
+    av1 *= 0.3333333333333333333;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double t1 = (dl[j][0] * av0 + dl[j][1] * av1) * tau * area;
+      E[j][6] += t1 * dl[i][0];
+      E[j][7] += t1 * dl[i][1];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) E[j][8] = area * tau / rho * (dl[j][0] * dl[i][0] + dl[j][1] * dl[i][1]);
+}
+
+static __global__ void __launch_bounds__(kNT) k_asm_oseen2d(int nnode, const unsigned* __restrict__ conn,
+                                                           const int* __restrict__ eptr, const int* __restrict__ elist,
+                                                           const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                           double* __restrict__ val, double* __restrict__ dia,
+                                                           const double* __restrict__ coord, double myu, double rho,
+                                                           const double* __restrict__ adv,
+                                                           const double* __restrict__ value, double* __restrict__ res) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i >= nnode) return;
+  const int q0 = rowptr[i], q1 = rowptr[i + 1];
+  double racc[3] = {0.0, 0.0, 0.0};
+  if (res)
+    for (int r = 0; r < 3; r++) racc[r] = res[(size_t)i * 3 + r];
+  for (int t = eptr[i]; t < eptr[i + 1]; t++) {
+    const int e = elist[t];
+    unsigned no[3];
+    double p[3][2], a[3][2];
+    int ia = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      no[k] = conn[(size_t)e * 3 + k];
+      p[k][0] = coord[(size_t)no[k] * 2];
+      p[k][1] = coord[(size_t)no[k] * 2 + 1];
+      a[k][0] = adv[(size_t)no[k] * 2];
+      a[k][1] = adv[(size_t)no[k] * 2 + 1];
+      if ((int)no[k] == i) ia = k;
+    }
+    double E[3][9];
+    oseen_row(p[0], p[1], p[2], a, ia, myu, rho, E);
+    double er[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 3; j++) {
+      const double* u = value + (size_t)no[j] * 3;
+#pragma unroll
+      for (int r = 0; r < 3; r++) er[r] -= E[j][r * 3 + 0] * u[0] + E[j][r * 3 + 1] * u[1] + E[j][r * 3 + 2] * u[2];
+      double* out;
+      if (j == ia) {
+        out = dia + (size_t)i * 9;
+      } else {
+        const int q = find_col(col, q0, q1, (int)no[j]);
+        if (q < 0) continue;
+        out = val + (size_t)q * 9;
+      }
+#pragma unroll
+      for (int s = 0; s < 9; s++) out[s] += E[j][s];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) racc[r] += er[r];
+  }
+  if (res)
+    for (int r = 0; r < 3; r++) res[(size_t)i * 3 + r] = racc[r];
+}
+
+// Positive Helmholtz / implicit-diffusion form: alpha*K + c*M with the P1
+// consistent mass of src/femeqn/eqn_diffusion.cpp:302-314 (emat = M + 1*K),
+// per-element alpha and c, source f; eres = f*area/3 - emat * value.
+static __global__ void __launch_bounds__(kNT) k_asm_helmholtz2d(int nnode, const unsigned* __restrict__ conn,
+                                                               const int* __restrict__ eptr,
+                                                               const int* __restrict__ elist,
+                                                               const int* __restrict__ rowptr,
+                                                               const int* __restrict__ col, double* __restrict__ val,
+                                                               double* __restrict__ dia,
+                                                               const double* __restrict__ coord,
+                                                               const double* __restrict__ alpha,
+                                                               const double* __restrict__ cmass, double source,
+                                                               const double* __restrict__ value,
+                                                               double* __restrict__ res) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i >= nnode) return;
+  const int q0 = rowptr[i], q1 = rowptr[i + 1];
+  double racc = res ? res[i] : 0.0;
+  double dacc = dia[i];
+  for (int t = eptr[i]; t < eptr[i + 1]; t++) {
+    const int e = elist[t];
+    unsigned no[3];
+    double p[3][2];
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      no[k] = conn[(size_t)e * 3 + k];
+      p[k][0] = coord[(size_t)no[k] * 2];
+      p[k][1] = coord[(size_t)no[k] * 2 + 1];
+      if ((int)no[k] == i) a = k;
+    }
+    const double area = d_tri_area(p[0], p[1], p[2]);
+    double dl[3][2];
+    d_tri_dldx(dl, p[0], p[1], p[2]);
+    const double m = cmass[e] * area * 0.08333333333333333;
+    double em[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double kc = alpha[e] * area * (dl[a][0] * dl[j][0] + dl[a][1] * dl[j][1]);
+      const double mm = (j == a) ? m + m : m;
+      em[j] = mm + 1.0 * kc;
+    }
+    double er = source * area * 0.333333333333333333;
+#pragma unroll
+    for (int j = 0; j < 3; j++) er -= em[j] * value[no[j]];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (j == a) {
+        dacc += em[j];
+      } else {
+        const int q = find_col(col, q0, q1, (int)no[j]);
+        if (q >= 0) val[q] += em[j];
+      }
+    }
+    racc += er;
+  }
+  dia[i] = dacc;
+  if (res) res[i] = racc;
+}
+
 // ---------------------------------------------------------------------------
 // ILU numeric factorisation (CMatDiaFrac_BlkCrs::SetValue_Initialize + DoILUDecomp,
 // include/delfem/matvec/matdiafrac_blkcrs.h:90-117, src/matvec/matdiafrac_blkcrs.cpp:769-1102).
@@ -756,16 +941,13 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
 // rhs may alias out.  Upper: v_i -= sum U_ij v_j in place; with DOT the rows
 // finished here contribute <r,v> partials (slot part0 + workgroup) to a
 // reduction that spans all level launches of the sweep.
+// One 64-row slice of a lower sweep (lane = row of the slice).
 template <int B, typename TS, typename T>
-__global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* __restrict__ rows,
-                                                   const int* __restrict__ sptr, const int* __restrict__ col,
-                                                   const TS* __restrict__ val, const TS* __restrict__ dval,
-                                                   const T* rhs, T* out, const KState* __restrict__ st) {
-  if (st && st->done) return;
+__device__ __forceinline__ void trsv_lower_slice(int s, int lane, const int* __restrict__ rows,
+                                                 const int* __restrict__ sptr, const int* __restrict__ col,
+                                                 const TS* __restrict__ val, const TS* __restrict__ dval,
+                                                 const T* rhs, T* out) {
   constexpr int BB = B * B;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = s0 + blockIdx.x * kWaves + wave;
-  if (s >= s1) return;
   const int row = rows[(size_t)s * 64 + lane];
   if (row < 0) return;
   T t[B];
@@ -793,6 +975,51 @@ __global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* _
   for (int r = 0; r < B; r++) out[(size_t)row * B + r] = y[r];
 }
 
+// One slice of an upper sweep; with DOT adds <rvec, v> of its rows to dsum.
+template <int B, typename TS, typename T, bool DOT>
+__device__ __forceinline__ void trsv_upper_slice(int s, int lane, const int* __restrict__ rows,
+                                                 const int* __restrict__ sptr, const int* __restrict__ col,
+                                                 const TS* __restrict__ val, T* vec, const T* __restrict__ rvec,
+                                                 double& dsum) {
+  constexpr int BB = B * B;
+  const int row = rows[(size_t)s * 64 + lane];
+  if (row < 0) return;
+  T t[B];
+#pragma unroll
+  for (int r = 0; r < B; r++) t[r] = vec[(size_t)row * B + r];
+  const int k0 = sptr[s], k1 = sptr[s + 1];
+  int k = k0;
+  for (; k + 1 < k1; k += 2) {
+    T a0[BB], v0[B], a1[BB], v1[B];
+    sell_slot<B, TS, T>(col, val, vec, k, lane, a0, v0);
+    sell_slot<B, TS, T>(col, val, vec, k + 1, lane, a1, v1);
+    blk_msub<B, T>(t, a0, v0);
+    blk_msub<B, T>(t, a1, v1);
+  }
+  if (k < k1) {
+    T a0[BB], v0[B];
+    sell_slot<B, TS, T>(col, val, vec, k, lane, a0, v0);
+    blk_msub<B, T>(t, a0, v0);
+  }
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    vec[(size_t)row * B + r] = t[r];
+    if (DOT) dsum += (double)rvec[(size_t)row * B + r] * (double)t[r];
+  }
+}
+
+template <int B, typename TS, typename T>
+__global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* __restrict__ rows,
+                                                   const int* __restrict__ sptr, const int* __restrict__ col,
+                                                   const TS* __restrict__ val, const TS* __restrict__ dval,
+                                                   const T* rhs, T* out, const KState* __restrict__ st) {
+  if (st && st->done) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = s0 + blockIdx.x * kWaves + wave;
+  if (s >= s1) return;
+  trsv_lower_slice<B, TS, T>(s, lane, rows, sptr, col, val, dval, rhs, out);
+}
+
 template <int B, typename TS, typename T, bool DOT>
 __global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const int* __restrict__ rows,
                                                    const int* __restrict__ sptr, const int* __restrict__ col,
@@ -800,37 +1027,10 @@ __global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const int* _
                                                    double* __restrict__ partials, int part0, int nparts,
                                                    const KState* __restrict__ st) {
   if (st && st->done) return;
-  constexpr int BB = B * B;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = s0 + blockIdx.x * kWaves + wave;
   double dsum[1] = {0.0};
-  if (s < s1) {
-    const int row = rows[(size_t)s * 64 + lane];
-    if (row >= 0) {
-      T t[B];
-#pragma unroll
-      for (int r = 0; r < B; r++) t[r] = vec[(size_t)row * B + r];
-      const int k0 = sptr[s], k1 = sptr[s + 1];
-      int k = k0;
-      for (; k + 1 < k1; k += 2) {
-        T a0[BB], v0[B], a1[BB], v1[B];
-        sell_slot<B, TS, T>(col, val, vec, k, lane, a0, v0);
-        sell_slot<B, TS, T>(col, val, vec, k + 1, lane, a1, v1);
-        blk_msub<B, T>(t, a0, v0);
-        blk_msub<B, T>(t, a1, v1);
-      }
-      if (k < k1) {
-        T a0[BB], v0[B];
-        sell_slot<B, TS, T>(col, val, vec, k, lane, a0, v0);
-        blk_msub<B, T>(t, a0, v0);
-      }
-#pragma unroll
-      for (int r = 0; r < B; r++) {
-        vec[(size_t)row * B + r] = t[r];
-        if (DOT) dsum[0] += (double)rvec[(size_t)row * B + r] * (double)t[r];
-      }
-    }
-  }
+  if (s < s1) trsv_upper_slice<B, TS, T, DOT>(s, lane, rows, sptr, col, val, vec, rvec, dsum[0]);
   if constexpr (DOT) write_partials<1>(dsum, partials, part0 + blockIdx.x, nparts);
 }
 

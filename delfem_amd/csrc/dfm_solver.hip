@@ -106,8 +106,8 @@ int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32, bool want16
 }
 
 // ---------------------------------------------------------------------------
-// Level-set sweeps.  `plan` carries the layout (rows, slices, columns) of the
-// FP64 copy; `L`/`U` the values in storage type TS.
+// Level-set sweeps, one launch per level.  `plan` carries the layout (rows,
+// slices, columns) of the FP64 copy; `L`/`U` the values in storage type TS.
 template <int B, typename TS, typename T>
 static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs, T* out, const KState* st) {
   const int ng = (int)plan.group_slice_ptr.size() - 1;
@@ -118,6 +118,7 @@ static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs
   }
   return DFM_OK;
 }
+// Number of partial sums an upper sweep with DOT leaves for the finalize kernel.
 static int upper_nparts(const Sell<double>& plan) {
   int n = 0;
   const int ng = (int)plan.group_slice_ptr.size() - 1;
@@ -223,7 +224,12 @@ static int device_dot(dfm_ilu_s* M, long n, const T* x, const T* y, double* out)
 template <class Body>
 static int run_block(dfm_ilu_s* M, const std::string& key, int nblock, Body&& body) {
   hipStream_t s = stream();
-  if (s == nullptr) {
+  // DFM_GRAPHS=0 launches the iterations directly instead of replaying a graph
+  static const bool use_graphs = [] {
+    const char* e = getenv("DFM_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  if (s == nullptr || !use_graphs) {
     for (int i = 0; i < nblock; i++) DFM_TRY(body());
     return DFM_OK;
   }
@@ -232,16 +238,25 @@ static int run_block(dfm_ilu_s* M, const std::string& key, int nblock, Body&& bo
       DFM_HIP(hipGraphLaunch(g.second, s));
       return DFM_OK;
     }
+  static bool capture_ok = true;
+  if (!capture_ok) {
+    for (int i = 0; i < nblock; i++) DFM_TRY(body());
+    return DFM_OK;
+  }
   hipGraph_t graph = nullptr;
   DFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = DFM_OK;
   for (int i = 0; i < nblock && rc == DFM_OK; i++) rc = body();
   hipError_t e = hipStreamEndCapture(s, &graph);
-  if (rc != DFM_OK) {
+  if (rc != DFM_OK || e != hipSuccess) {
+    // a launch the runtime could not capture: nothing ran; launch the
+    // iterations directly from now on
     if (graph) (void)hipGraphDestroy(graph);
-    return rc;
+    (void)hipGetLastError();
+    capture_ok = false;
+    for (int i = 0; i < nblock; i++) DFM_TRY(body());
+    return DFM_OK;
   }
-  DFM_HIP(e);
   hipGraphExec_t exec = nullptr;
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -474,10 +489,24 @@ static int mixed_core(dfm_ilu_s* M, bool bicg, const double* b, double* r64, dou
     }
     if (hs.done == 2 || !std::isfinite(rr_true)) break;
     DFM_LAUNCH(k_convert<T>, grid_vec(n), n, r64, rr_);
+    // Residual replacement keeps the search directions while the recursive
+    // residual still tracks the true one.  When the recursion has run ahead
+    // by more than 10x in norm (it converged on the low-precision operator
+    // within one check block) its directions no longer fit the replaced
+    // residual: restart the recurrence from the true residual.
+    const bool restart = rr_true > 100.0 * hs.rr;
     rr_ref = rr_true;
     hs.done = 0;
     hs.rr = rr_true;
     DFM_TRY(write_state(M));
+    if (restart) {
+      if (!bicg) {
+        DFM_TRY((pcg_start<B, TA, TF, T>(M, o, rr_, ws, ws + n)));
+      } else {
+        DFM_HIP(hipMemcpyAsync(ws, rr_, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));      // r2
+        DFM_HIP(hipMemcpyAsync(ws + n, rr_, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));  // p
+      }
+    }
   }
   const KState& s = *M->h_state;
   res = {std::sqrt(rr_true * s.inv_rr0), (unsigned)(bicg ? s.iter - 1 : s.iter), converged};

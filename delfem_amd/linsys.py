@@ -19,7 +19,7 @@ from . import _lib
 from ._lib import call
 
 _ORDER = {"natural": _lib.DFM_ORDER_NATURAL, "given": _lib.DFM_ORDER_GIVEN,
-          "multicolour": _lib.DFM_ORDER_MULTICOLOUR}
+          "multicolour": _lib.DFM_ORDER_MULTICOLOUR, "block_multicolour": _lib.DFM_ORDER_BLOCK_MULTICOLOUR}
 
 
 def _torch():
@@ -175,12 +175,22 @@ class Assembler:
         call("dfm_assemble_solid3d_p1", self.h, _dptr(coord), _dptr(lam), _dptr(mu), float(rho), float(g[0]),
              float(g[1]), float(g[2]), _dptr(disp), _dptr(res) if res is not None else None)
 
+    def oseen2d_p1p1(self, coord, myu, rho, adv, value, res):
+        _sync_torch()
+        call("dfm_assemble_oseen2d_p1p1", self.h, _dptr(coord), float(myu), float(rho), _dptr(adv), _dptr(value),
+             _dptr(res) if res is not None else None)
+
+    def helmholtz2d_p1(self, coord, alpha, cmass, source, value, res):
+        _sync_torch()
+        call("dfm_assemble_helmholtz2d_p1", self.h, _dptr(coord), _dptr(alpha), _dptr(cmass), float(source),
+             _dptr(value), _dptr(res) if res is not None else None)
+
 
 class CMatDiaFrac_BlkCrs:
     """MatVec::CMatDiaFrac_BlkCrs (include/delfem/matvec/matdiafrac_blkcrs.h:40-137):
     symbolic ILU(lev) at construction, numeric on SetValue."""
 
-    def __init__(self, lev, A, order=None, ordering=None):
+    def __init__(self, lev, A, order=None, ordering=None, block_nodes=0):
         if ordering is None:
             ordering = "given" if order is not None else "natural"
         h = C.c_void_p()
@@ -188,7 +198,7 @@ class CMatDiaFrac_BlkCrs:
         if order is not None:
             ordarr = np.ascontiguousarray(order, dtype=np.int32)
         call("dfm_ilu_create", A.h, int(lev), _ORDER[ordering], _ptr(ordarr) if ordarr is not None else None,
-             C.byref(h))
+             int(block_nodes), C.byref(h))
         self.h, self.nblk, self.len = h, A.nblk, A.len
         self.nsingular = 0
 
@@ -252,8 +262,8 @@ class CMatDiaFrac_BlkCrs:
 class CPreconditioner_ILU(CMatDiaFrac_BlkCrs):
     """LsSol::CPreconditioner_ILU (include/delfem/ls/preconditioner.h:59-108) on one segment."""
 
-    def __init__(self, A, nlev=0, order=None, ordering=None):
-        super().__init__(nlev, A, order=order, ordering=ordering)
+    def __init__(self, A, nlev=0, order=None, ordering=None, block_nodes=0):
+        super().__init__(nlev, A, order=order, ordering=ordering, block_nodes=block_nodes)
 
 
 def _solve(fn, conv_ratio, iteration, A, prec, r, x, precision, opts_kw):
@@ -303,10 +313,17 @@ class FEMSystem:
         self.coord = to_device(prob.coord.ravel())
         if prob.kind == "poisson2d":
             self.coef = (to_device(prob.alpha),)
+        elif prob.kind == "helmholtz2d":
+            self.coef = (to_device(prob.alpha), to_device(prob.cmass))
+        elif prob.kind == "oseen2d":
+            self.coef = (to_device(prob.adv.ravel()),)
         else:
             self.coef = (to_device(prob.lam), to_device(prob.mu))
         self.res = torch.zeros(prob.ndof, dtype=torch.float64, device="cuda")
-        self.value = torch.zeros(prob.ndof, dtype=torch.float64, device="cuda")
+        if getattr(prob, "value", None) is not None:
+            self.value = to_device(prob.value)
+        else:
+            self.value = torch.zeros(prob.ndof, dtype=torch.float64, device="cuda")
         self.flags = np.ascontiguousarray(prob.flags, dtype=np.int32)
         self.flags_dev = to_device(self.flags)
 
@@ -317,6 +334,10 @@ class FEMSystem:
         p = self.prob
         if p.kind == "poisson2d":
             self.asm.poisson2d_p1(self.coord, self.coef[0], p.source, self.value, self.res)
+        elif p.kind == "helmholtz2d":
+            self.asm.helmholtz2d_p1(self.coord, self.coef[0], self.coef[1], p.source, self.value, self.res)
+        elif p.kind == "oseen2d":
+            self.asm.oseen2d_p1p1(self.coord, p.myu, p.rho, self.coef[0], self.value, self.res)
         else:
             self.asm.solid3d_p1(self.coord, self.coef[0], self.coef[1], p.rho, p.g, self.value, self.res)
         self.A.SetBoundaryCondition(self.flags)

@@ -129,3 +129,82 @@ def solid3d(n=48, seed=12345, sigma=0.0, E=1.0, nu=0.3, g=(0.0, 0.0, -1.0), rho=
     return Problem(kind="solid3d", name=f"solid3d_{n}", n=n, b=3, npe=4, coord=coord, conn=conn,
                    nnode=nnode, nelem=nelem, lam=np.ascontiguousarray(lam), mu=np.ascontiguousarray(mu),
                    rho=rho, g=tuple(g), flags=flags.ravel(), seed=seed, sigma=sigma)
+
+
+def fourier_stream_velocity(coord, seed, modes=4):
+    """Smooth divergence-free advection field on the unit square: the velocity
+    (d psi/dy, -d psi/dx) of psi = sum_{k,l=1..modes} c_kl sin(k pi x) sin(l pi y),
+    c_kl standard normal / (k*k + l*l), scaled so max_nodes |a| = 1.  The normal
+    component vanishes on the walls."""
+    c = normal(seed, modes * modes).reshape(modes, modes)
+    x, y = coord[:, 0], coord[:, 1]
+    ax = np.zeros(x.size)
+    ay = np.zeros(x.size)
+    for k in range(1, modes + 1):
+        sx, cx = np.sin(k * np.pi * x), np.cos(k * np.pi * x)
+        for l in range(1, modes + 1):
+            sy, cy = np.sin(l * np.pi * y), np.cos(l * np.pi * y)
+            w = c[k - 1, l - 1] / (k * k + l * l)
+            ax += w * l * np.pi * sx * cy
+            ay -= w * k * np.pi * cx * sy
+    a = np.stack([ax, ay], axis=1)
+    a /= np.max(np.sqrt(ax * ax + ay * ay))
+    return np.ascontiguousarray(a)
+
+
+def oseen2d(n=256, seed=12345, myu=0.01, rho=1.0, lid=1.0):
+    """Linearised incompressible flow (fluid2d path): lid-driven cavity on the
+    unit square, n x n cells, combined (u, v, p) unknowns per node, viscosity
+    myu, prescribed Fourier advection field.  Velocity is fixed on the whole
+    boundary (u = lid on the top edge without its corners, 0 elsewhere), and the
+    pressure at node 0 is pinned to 0 (with Dirichlet velocity everywhere the
+    pressure is otherwise defined up to a constant).  `value` holds the
+    boundary values: the linear system is solved for the update."""
+    coord, conn = tri_grid(n)
+    nnode = coord.shape[0]
+    adv = fourier_stream_velocity(coord, seed)
+    x, y = coord[:, 0], coord[:, 1]
+    wall = np.isclose(x, 0.0) | np.isclose(x, 1.0) | np.isclose(y, 0.0) | np.isclose(y, 1.0)
+    flags = np.zeros((nnode, 3), dtype=np.int32)
+    flags[wall, 0:2] = 1
+    flags[0, 2] = 1
+    value = np.zeros((nnode, 3))
+    top = np.isclose(y, 1.0) & ~np.isclose(x, 0.0) & ~np.isclose(x, 1.0)
+    value[top, 0] = lid
+    return Problem(kind="oseen2d", name=f"oseen2d_{n}", n=n, b=3, npe=3, coord=coord, conn=conn, nnode=nnode,
+                   nelem=conn.shape[0], myu=myu, rho=rho, adv=adv, flags=flags.ravel(),
+                   value=np.ascontiguousarray(value.ravel()), seed=seed)
+
+
+def helmholtz2d(nx, ny, lx, ly, seed=12345, sigma=0.5, c=1.0, source=1.0, offset=0):
+    """Positive Helmholtz / implicit diffusion alpha*K + c*M on an lx x ly
+    rectangle of nx x ny cells, alpha lognormal(0, sigma) per element drawn
+    from the stream of `seed` at `offset`, f = source, u = 0 on the x = 0 edge."""
+    coord, conn = tri_grid(nx, ny, lx, ly)
+    nelem = conn.shape[0]
+    alpha = lognormal(seed, nelem, 0.0, sigma, offset) if sigma > 0 else np.ones(nelem)
+    nnode = coord.shape[0]
+    flags = np.zeros(nnode, dtype=np.int32)
+    flags[np.isclose(coord[:, 0], 0.0)] = 1
+    return Problem(kind="helmholtz2d", name=f"helmholtz2d_{nx}x{ny}", b=1, npe=3, coord=coord, conn=conn,
+                   nnode=nnode, nelem=nelem, alpha=alpha, cmass=np.full(nelem, float(c)), source=source,
+                   flags=flags, seed=seed)
+
+
+def helmholtz_batch(count=512, seed=12345, nmin=500, nmax=50000, sigma=0.5, cmax=10.0):
+    """Batch of independent rectangles (batched path): node counts log-uniform
+    in [nmin, nmax], aspect ratio lx/ly uniform in [1, 4] (ly = 1, near-square
+    cells), mass coefficient c uniform in [0, cmax], alpha lognormal per
+    element, u = 0 on the short edge x = 0."""
+    u = uniform(seed, 3 * count).reshape(count, 3)
+    out = []
+    off = 3 * count
+    for s in range(count):
+        nn = nmin * (nmax / nmin) ** u[s, 0]
+        ar = 1.0 + 3.0 * u[s, 1]
+        ny = max(2, int(round(np.sqrt(nn / ar))) - 1)
+        nx = max(2, int(round(ar * (ny + 1))) - 1)
+        p = helmholtz2d(nx, ny, ar, 1.0, seed=seed, sigma=sigma, c=cmax * u[s, 2], offset=off)
+        off += p.nelem
+        out.append(p)
+    return out

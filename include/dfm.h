@@ -106,16 +106,37 @@ int dfm_assemble_poisson2d_p1(dfm_asm *p, const double *coord_dev, const double 
 int dfm_assemble_solid3d_p1(dfm_asm *p, const double *coord_dev, const double *lambda_dev,
                             const double *myu_dev, double rho, double gx, double gy, double gz,
                             const double *disp_dev, double *res_dev);
+/* Steady Oseen P1-P1 with PSPG on combined (u,v,p) 3x3 blocks: viscous,
+ * Galerkin advection, pressure/continuity, PSPG advection and PSPG pressure
+ * terms of MakeMat_NavierStokes2D_NonStatic_Newmark_P1P1
+ * (src/femeqn/eqn_navier_stokes.cpp:73-110,136-158,231-253) with a prescribed
+ * advection field and the steady stabilisation parameter (no Courant term),
+ * merged as the combined assembly src/femeqn/eqn_stokes.cpp:231-346 does.
+ * coord_dev: nnode*2, adv_dev: nnode*2, value_dev: nnode*3, res_dev: nnode*3
+ * (accumulated: res -= E*value). */
+int dfm_assemble_oseen2d_p1p1(dfm_asm *p, const double *coord_dev, double myu, double rho,
+                              const double *adv_dev, const double *value_dev, double *res_dev);
+/* Positive Helmholtz / implicit diffusion alpha*K + c*M with the P1 consistent
+ * mass of AddLinearSystem_Diffusion2D_P1 (src/femeqn/eqn_diffusion.cpp:240-345,
+ * rho -> c, gamma*dt -> 1), alpha and c per element.  res_dev: nnode
+ * (accumulated: res += f*area/3 - E*value). */
+int dfm_assemble_helmholtz2d_p1(dfm_asm *p, const double *coord_dev, const double *alpha_dev,
+                                const double *c_dev, double source, const double *value_dev,
+                                double *res_dev);
 
 /* ---- ILU (MatVec::CMatDiaFrac_BlkCrs, LsSol::CPreconditioner_ILU) -------- */
-enum { DFM_ORDER_NATURAL = 0, DFM_ORDER_GIVEN = 1, DFM_ORDER_MULTICOLOUR = 2 };
+enum { DFM_ORDER_NATURAL = 0, DFM_ORDER_GIVEN = 1, DFM_ORDER_MULTICOLOUR = 2, DFM_ORDER_BLOCK_MULTICOLOUR = 3 };
 /* Symbolic ILU(lev_fill) of A's pattern: CMatDiaFrac_BlkCrs(lev,A) / (lev,A,order),
  * matdiafrac_blkcrs.cpp:227-324 + AddFracPtn :1249-1377; CPreconditioner_ILU::SetLinearSystem
  * preconditioner.cpp:57-211.  ordering: NATURAL (order ignored), GIVEN (order[new]=old),
- * MULTICOLOUR (greedy colouring computed here). lev_fill = -1 means unlimited. */
-int dfm_ilu_create(const dfm_bcrs *A, int lev_fill, int ordering, const int *order, dfm_ilu **out);
+ * MULTICOLOUR (greedy smallest-last colouring of A's graph computed here),
+ * BLOCK_MULTICOLOUR (connected blocks of at most block_nodes nodes, block
+ * graph coloured, natural order inside a block; block_nodes 0 means 32).
+ * block_nodes is ignored by the other orderings.  lev_fill = -1 means unlimited. */
+int dfm_ilu_create(const dfm_bcrs *A, int lev_fill, int ordering, const int *order, unsigned block_nodes,
+                   dfm_ilu **out);
 int dfm_ilu_destroy(dfm_ilu *M);
-/* ncrs of the factor, number of level sets of L and U, colours (0 unless MULTICOLOUR) */
+/* ncrs of the factor, number of level sets of L and U, colours (0 for NATURAL / GIVEN) */
 int dfm_ilu_info(const dfm_ilu *M, unsigned *ncrs, int *nlevel_lower, int *nlevel_upper, int *ncolour);
 /* Factor pattern in factor numbering; any pointer may be NULL. order: nblk entries. */
 int dfm_ilu_get_pattern(const dfm_ilu *M, unsigned *rowptr, unsigned *col, int *lev,

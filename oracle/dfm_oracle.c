@@ -23,7 +23,9 @@
  *   orc_ilu_backward     golden fixtures (bitwise) + dense triangular solve
  *   orc_solve_pcg        golden fixtures (iterations, conv_ratio, solution) + scipy direct solve
  *   orc_solve_pbicgstab  golden fixtures (iterations, conv_ratio, solution) + scipy direct solve
- *   orc_emat_*           golden fixtures (bitwise) + Laplacian row sums, rigid-body null space
+ *   orc_emat_*           golden fixtures (bitwise) + Laplacian row sums, rigid-body null space;
+ *                        the Oseen element has no compiled counterpart: pinned by
+ *                        closed-form terms and invariants (parity unpinned against the reference)
  *   orc_greedy_colour,
  *   orc_colour_order,
  *   orc_level_sets       product algorithms (no reference counterpart): pinned by
@@ -911,6 +913,134 @@ void orc_emat_solid3d_p1(unsigned nelem, const double *coord, const unsigned *co
     }
 }
 
+/* Positive Helmholtz / implicit diffusion, per-element alpha and c:
+ * src/femeqn/eqn_diffusion.cpp:302-330 with rho -> c and gamma*dt -> 1:
+ * emat = M + 1*K, M = c*area/12 (diagonal doubled), K = alpha*area*grad.grad;
+ * eres = f*area/3 - emat*value. */
+void orc_emat_helmholtz2d_p1(unsigned nelem, const double *coord, const unsigned *conn,
+                             const double *alpha, const double *cmass, double source,
+                             const double *value, double *emat, double *eres)
+{
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *no = conn + (size_t)e * 3;
+        const double *p0 = coord + (size_t)no[0] * 2, *p1 = coord + (size_t)no[1] * 2, *p2 = coord + (size_t)no[2] * 2;
+        const double area = tri_area(p0, p1, p2);
+        double dl[3][2];
+        tri_dldx(dl, p0, p1, p2);
+        const double m = cmass[e] * area * 0.08333333333333333;
+        double *em = emat + (size_t)e * 9;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const double kc = alpha[e] * area * (dl[i][0] * dl[j][0] + dl[i][1] * dl[j][1]);
+                const double mm = (i == j) ? m + m : m;
+                em[i * 3 + j] = mm + 1.0 * kc;
+            }
+        double *er = eres + (size_t)e * 3;
+        for (int i = 0; i < 3; i++) {
+            er[i] = source * area * 0.333333333333333333;
+            for (int j = 0; j < 3; j++) er[i] -= em[i * 3 + j] * value[no[j]];
+        }
+    }
+}
+
+/* Steady Oseen P1-P1 with PSPG, combined (u,v,p) 3x3 blocks: the viscous,
+ * Galerkin advection, pressure/continuity and PSPG terms of
+ * src/femeqn/eqn_navier_stokes.cpp:73-110 and 231-253 with a prescribed
+ * advection field, no mass/Newmark or SUPG terms, and the stabilisation
+ * parameter of :137-158 without its Courant term (steady).  The block layout
+ * follows the combined Stokes assembly src/femeqn/eqn_stokes.cpp:317-331:
+ * [0..1][0..1] velocity, [0..1][2] pressure gradient, [2][0..1] continuity,
+ * [2][2] pressure.  eres = f - emat * value with f = 0. */
+void orc_emat_oseen2d_p1p1(unsigned nelem, const double *coord, const unsigned *conn,
+                           double myu, double rho, const double *adv,
+                           const double *value, double *emat, double *eres)
+{
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *no = conn + (size_t)e * 3;
+        const double *p0 = coord + (size_t)no[0] * 2, *p1 = coord + (size_t)no[1] * 2, *p2 = coord + (size_t)no[2] * 2;
+        double a[3][2];
+        for (int k = 0; k < 3; k++) { a[k][0] = adv[(size_t)no[k] * 2]; a[k][1] = adv[(size_t)no[k] * 2 + 1]; }
+        const double area = tri_area(p0, p1, p2);
+        double dl[3][2];
+        tri_dldx(dl, p0, p1, p2);
+        double *E = emat + (size_t)e * 81;   /* [i][j][r][c] */
+#define EM(i, j, r, c) E[(((i) * 3 + (j)) * 3 + (r)) * 3 + (c)]
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                EM(i, j, 0, 0) = area * myu * dl[i][0] * dl[j][0];
+                EM(i, j, 0, 1) = area * myu * dl[i][1] * dl[j][0];
+                EM(i, j, 1, 0) = area * myu * dl[i][0] * dl[j][1];
+                EM(i, j, 1, 1) = area * myu * dl[i][1] * dl[j][1];
+                const double t = area * myu * (dl[j][0] * dl[i][0] + dl[j][1] * dl[i][1]);
+                EM(i, j, 0, 0) += t;
+                EM(i, j, 1, 1) += t;
+            }
+        {
+            const double s0 = a[0][0] + a[1][0] + a[2][0], s1 = a[0][1] + a[1][1] + a[2][1];
+            for (int j = 0; j < 3; j++) {
+                const double t1 = dl[j][0] * s0 + dl[j][1] * s1;
+                for (int i = 0; i < 3; i++) {
+                    double t2 = t1 + (dl[j][0] * a[i][0] + dl[j][1] * a[i][1]);
+                    t2 *= area * rho * 0.083333333333333;
+                    EM(i, j, 0, 0) += t2;
+                    EM(i, j, 1, 1) += t2;
+                }
+            }
+        }
+        {
+            const double t = area * 0.33333333333333333;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    EM(i, j, 0, 2) = -t * dl[i][0];
+                    EM(i, j, 1, 2) = -t * dl[i][1];
+                    EM(i, j, 2, 0) = t * dl[j][0];
+                    EM(i, j, 2, 1) = t * dl[j][1];
+                }
+        }
+        double tau;
+        {
+            const double va0 = (a[0][0] + a[1][0] + a[2][0]) / 3.0, va1 = (a[0][1] + a[1][1] + a[2][1]) / 3.0;
+            const double norm_v = sqrt(va0 * va0 + va1 * va1);
+            const double h = sqrt(area / 3.14) * 2;
+            if (norm_v * h * rho * 1.0e-30 > myu) {
+                tau = h * 0.5 / norm_v;
+            } else if (norm_v * h * rho < myu * 1.0e-30) {
+                tau = h * h * rho * 0.5 / myu;
+            } else {
+                const double re_c = 0.5 * norm_v * h * rho / myu;
+                tau = (h * 0.5 / norm_v) / sqrt(1 + 1 / (re_c * re_c));
+            }
+        }
+        {
+            double av0 = a[0][0] + a[1][0] + a[2][0], av1 = a[0][1] + a[1][1] + a[2][1];
+            av0 *= 0.3333333333333333333;
+            av1 *= 0.3333333333333333333;
+            for (int j = 0; j < 3; j++) {
+                const double t1 = (dl[j][0] * av0 + dl[j][1] * av1) * tau * area;
+                for (int i = 0; i < 3; i++) {
+                    EM(i, j, 2, 0) += t1 * dl[i][0];
+                    EM(i, j, 2, 1) += t1 * dl[i][1];
+                }
+            }
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                EM(i, j, 2, 2) = area * tau / rho * (dl[j][0] * dl[i][0] + dl[j][1] * dl[i][1]);
+        double *er = eres + (size_t)e * 9;
+        for (int i = 0; i < 3; i++) {
+            er[i * 3 + 0] = 0.0;
+            er[i * 3 + 1] = 0.0;
+            er[i * 3 + 2] = 0.0;
+            for (int j = 0; j < 3; j++) {
+                const double *u = value + (size_t)no[j] * 3;
+                for (int r = 0; r < 3; r++)
+                    er[i * 3 + r] -= EM(i, j, r, 0) * u[0] + EM(i, j, r, 1) * u[1] + EM(i, j, r, 2) * u[2];
+            }
+        }
+#undef EM
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Multicolour ordering (product algorithm; no reference counterpart):
  * smallest-last greedy colouring.  Vertices are removed one at a time, the
@@ -995,6 +1125,83 @@ void orc_colour_order(unsigned nb, const int *colour, int ncolour, int *order)
     for (int c = 0; c < ncolour; c++)
         for (unsigned i = 0; i < nb; i++)
             if (colour[i] == c) order[n++] = (int)i;
+}
+
+/* Block multicolour ordering (product algorithm; no reference counterpart).
+ * Blocks: for each node in increasing order that has no block yet, start a
+ * new block there and grow it breadth first (neighbours in CRS order), until
+ * it holds block_nodes nodes or its component is exhausted.  The block graph
+ * (blocks adjacent when any of their nodes are) is coloured with
+ * orc_greedy_colour and nodes are listed by (block colour, block, node).
+ * Restated with a sorted pair list and qsort. */
+static const int *g_key_col, *g_key_blk;
+static int bmc_cmp(const void *a, const void *b)
+{
+    const int u = *(const int *)a, v = *(const int *)b;
+    const int cu = g_key_col[g_key_blk[u]], cv = g_key_col[g_key_blk[v]];
+    if (cu != cv) return cu < cv ? -1 : 1;
+    if (g_key_blk[u] != g_key_blk[v]) return g_key_blk[u] < g_key_blk[v] ? -1 : 1;
+    return u < v ? -1 : (u > v);
+}
+static int pair_cmp(const void *a, const void *b)
+{
+    const unsigned *x = (const unsigned *)a, *y = (const unsigned *)b;
+    if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+    return x[1] < y[1] ? -1 : (x[1] > y[1]);
+}
+int orc_bmc_order(unsigned nb, const unsigned *rowptr, const unsigned *col, unsigned block_nodes,
+                  int *order, int *block_of)
+{
+    int nbk = 0;
+    unsigned *queue = (unsigned *)malloc(sizeof(unsigned) * (nb ? nb : 1));
+    for (unsigned i = 0; i < nb; i++) block_of[i] = -1;
+    for (unsigned s = 0; s < nb; s++) {
+        if (block_of[s] >= 0) continue;
+        unsigned qn = 0, h = 0;
+        queue[qn++] = s;
+        block_of[s] = nbk;
+        while (h < qn && qn < block_nodes) {
+            const unsigned u = queue[h++];
+            for (unsigned q = rowptr[u]; q < rowptr[u + 1] && qn < block_nodes; q++) {
+                if (block_of[col[q]] >= 0) continue;
+                block_of[col[q]] = nbk;
+                queue[qn++] = col[q];
+            }
+        }
+        nbk++;
+    }
+    free(queue);
+    const unsigned ne = rowptr[nb];
+    unsigned *pairs = (unsigned *)malloc(sizeof(unsigned) * 2 * (ne ? ne : 1));
+    unsigned np = 0;
+    for (unsigned u = 0; u < nb; u++)
+        for (unsigned q = rowptr[u]; q < rowptr[u + 1]; q++)
+            if (block_of[col[q]] != block_of[u]) {
+                pairs[2 * np] = (unsigned)block_of[u];
+                pairs[2 * np + 1] = (unsigned)block_of[col[q]];
+                np++;
+            }
+    qsort(pairs, np, 2 * sizeof(unsigned), pair_cmp);
+    unsigned *qrow = (unsigned *)calloc((size_t)nbk + 1, sizeof(unsigned));
+    unsigned *qcol = (unsigned *)malloc(sizeof(unsigned) * (np ? np : 1));
+    unsigned nq = 0;
+    for (unsigned t = 0; t < np; t++) {
+        if (t > 0 && pairs[2 * t] == pairs[2 * t - 2] && pairs[2 * t + 1] == pairs[2 * t - 1]) continue;
+        qcol[nq++] = pairs[2 * t + 1];
+        qrow[pairs[2 * t] + 1]++;
+    }
+    for (int k = 0; k < nbk; k++) qrow[k + 1] += qrow[k];
+    int *bcol = (int *)malloc(sizeof(int) * (nbk ? nbk : 1));
+    const int ncol = orc_greedy_colour((unsigned)nbk, qrow, qcol, bcol);
+    for (unsigned i = 0; i < nb; i++) order[i] = (int)i;
+    g_key_col = bcol;
+    g_key_blk = block_of;
+    qsort(order, nb, sizeof(int), bmc_cmp);
+    free(pairs);
+    free(qrow);
+    free(qcol);
+    free(bcol);
+    return ncol;
 }
 
 /* Level sets of the factor DAG.  lower=1: level(i) = 1 + max level(j), j in

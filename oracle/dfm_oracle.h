@@ -99,9 +99,9 @@ void orc_emat_solid3d_p1(unsigned nelem, const double *coord, const unsigned *co
                          const double *lambda, const double *myu, double rho,
                          double gx, double gy, double gz, const double *disp,
                          double *emat, double *eres);
-/* positive Helmholtz / implicit diffusion form K*alpha + c*M with P1 consistent mass */
+/* positive Helmholtz alpha*K + c*M with P1 consistent mass */
 void orc_emat_helmholtz2d_p1(unsigned nelem, const double *coord, const unsigned *conn,
-                             const double *alpha, const double *c, double source,
+                             const double *alpha, const double *cmass, double source,
                              const double *value, double *emat, double *eres);
 /* Oseen P1-P1 with PSPG, combined (u,v,p) 3x3 blocks */
 void orc_emat_oseen2d_p1p1(unsigned nelem, const double *coord, const unsigned *conn,
@@ -111,6 +111,9 @@ void orc_emat_oseen2d_p1p1(unsigned nelem, const double *coord, const unsigned *
 /* Orderings for the multicolour path (product algorithm, restated). */
 int orc_greedy_colour(unsigned nb, const unsigned *rowptr, const unsigned *col, int *colour);
 void orc_colour_order(unsigned nb, const int *colour, int ncolour, int *order);
+/* block multicolour ordering; block_of receives each node's block; returns #colours */
+int orc_bmc_order(unsigned nb, const unsigned *rowptr, const unsigned *col, unsigned block_nodes,
+                  int *order, int *block_of);
 int orc_level_sets(unsigned nb, const unsigned *frowptr, const unsigned *fcol,
                    const unsigned *fdiaind, int lower, int *level);
 

@@ -74,9 +74,13 @@ class Orc:
         L.orc_emat_poisson2d_p1.argtypes = [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]
         L.orc_emat_solid3d_p1.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                           C.c_double, vp, vp, vp]
+        L.orc_emat_oseen2d_p1p1.argtypes = [C.c_uint, vp, vp, C.c_double, C.c_double, vp, vp, vp, vp]
+        L.orc_emat_helmholtz2d_p1.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_double, vp, vp, vp]
         L.orc_greedy_colour.restype = C.c_int
         L.orc_greedy_colour.argtypes = [C.c_uint, vp, vp, vp]
         L.orc_colour_order.argtypes = [C.c_uint, vp, C.c_int, vp]
+        L.orc_bmc_order.restype = C.c_int
+        L.orc_bmc_order.argtypes = [C.c_uint, vp, vp, C.c_uint, vp, vp]
         L.orc_level_sets.restype = C.c_int
         L.orc_level_sets.argtypes = [C.c_uint, vp, vp, vp, C.c_int, vp]
         self.L = L
@@ -197,6 +201,24 @@ class Orc:
                                    _p(f64(disp)), _p(emat), _p(eres))
         return emat, eres
 
+    def emat_oseen2d(self, coord, conn, myu, rho, adv, value):
+        conn = u32(conn)
+        ne = conn.shape[0]
+        emat = np.zeros(ne * 81)
+        eres = np.zeros(ne * 9)
+        self.L.orc_emat_oseen2d_p1p1(ne, _p(f64(coord)), _p(conn), myu, rho, _p(f64(adv)), _p(f64(value)), _p(emat),
+                                     _p(eres))
+        return emat, eres
+
+    def emat_helmholtz2d(self, coord, conn, alpha, cmass, source, value):
+        conn = u32(conn)
+        ne = conn.shape[0]
+        emat = np.zeros(ne * 9)
+        eres = np.zeros(ne * 3)
+        self.L.orc_emat_helmholtz2d_p1(ne, _p(f64(coord)), _p(conn), _p(f64(alpha)), _p(f64(cmass)), source,
+                                       _p(f64(value)), _p(emat), _p(eres))
+        return emat, eres
+
     # -- orderings -------------------------------------------------------------
     def colour_order(self, rowptr, col):
         rowptr, col = u32(rowptr), u32(col)
@@ -206,6 +228,14 @@ class Orc:
         order = np.zeros(nb, dtype=np.int32)
         self.L.orc_colour_order(nb, _p(colour), nc, _p(order))
         return colour, nc, order
+
+    def bmc_order(self, rowptr, col, block_nodes=32):
+        rowptr, col = u32(rowptr), u32(col)
+        nb = rowptr.size - 1
+        order = np.zeros(nb, dtype=np.int32)
+        blk = np.zeros(nb, dtype=np.int32)
+        nc = self.L.orc_bmc_order(nb, _p(rowptr), _p(col), block_nodes, _p(order), _p(blk))
+        return blk, nc, order
 
     def level_sets(self, F, lower):
         nb = F["rowptr"].size - 1
@@ -402,6 +432,12 @@ def problem_emats(orc, prob, value=None):
     if prob.kind == "poisson2d":
         v = np.zeros(prob.nnode) if value is None else value
         return orc.emat_poisson2d(prob.coord, prob.conn, prob.alpha, prob.source, v)
+    if prob.kind == "helmholtz2d":
+        v = np.zeros(prob.nnode) if value is None else value
+        return orc.emat_helmholtz2d(prob.coord, prob.conn, prob.alpha, prob.cmass, prob.source, v)
+    if prob.kind == "oseen2d":
+        v = prob.value if value is None else value
+        return orc.emat_oseen2d(prob.coord, prob.conn, prob.myu, prob.rho, prob.adv, v)
     v = np.zeros(prob.nnode * 3) if value is None else value
     return orc.emat_solid3d(prob.coord, prob.conn, prob.lam, prob.mu, prob.rho, prob.g, v)
 
