@@ -1,38 +1,54 @@
-"""Benchmark: ms per ILU(0)-PCG solve to 1e-10 relative residual on one MI355X.
+"""Benchmark: ms per ILU-preconditioned Krylov solve on one MI355X.
 
-Workload (default): 3-D linear elasticity on a 96^3 hex grid split into Kuhn
-tetrahedra (912,673 nodes, 2,738,019 DOF, 3x3 blocks), per-element Young's
-modulus lognormal(0, 1.0), fixed at z = 0, body force (0, 0, -1); synthetic
-mesh and coefficients from a fixed seed.  One step = ILU(0) numeric
-factorisation + preconditioned CG to 1e-10 on the device, inputs resident in
-HBM.  `value` is the mixed-precision path (FP32 matrix/factor/inner PCG with
-FP64 dot accumulators, FP64 true residual and iterative refinement); the plain
-FP64 path is timed beside it.
+Default workload (the headline): 3-D linear elasticity on a 96^3 hex grid
+split into Kuhn tetrahedra (912,673 nodes, 2,738,019 DOF, 3x3 blocks),
+per-element Young's modulus lognormal(0, 1.0), fixed at z = 0, body force
+(0, 0, -1); synthetic mesh and coefficients from a fixed seed.  One step =
+ILU(0) numeric factorisation + preconditioned CG to 1e-10 relative residual,
+inputs resident in HBM.  `value` is the mixed-precision path (FP32 vectors
+and matrix, BF16 ILU factors, FP64 dot accumulators, FP64 true residual and
+reliable updates, convergence on the FP64 true residual); the plain FP64
+path (reference semantics) is timed beside it.
 
-Prints one JSON line.  python bench.py [--steps K] [--warmup W] [--config cfg3]
+Other workloads: --config cfg1 (2-D Poisson 128^2, ILU(0)-PCG to 1e-8),
+cfg2 (elasticity 48^3, FP64 ILU(0)-PCG to 1e-10), cfg4 (Oseen P1-P1 PSPG
+lid-driven cavity 256^2, ILU(1)-BiCGSTAB to 1e-9).
+
+Prints one JSON line:  python bench.py [--steps K] [--warmup W] [--config cfg3]
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
 import time
 
-import numpy as np
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "ms per ILU-PCG solve to 1e-10 rel-res (mixed vs FP64); HBM GB/s % of peak"
 
 CONFIGS = {
-    "cfg1": dict(kind="poisson2d", n=128, sigma=0.5, tol=1e-8,
-                 workload="2-D Poisson P1, 128x128 cells (16,641 nodes), lognormal(0,0.5) diffusivity"),
-    "cfg2": dict(kind="solid3d", n=48, sigma=0.0, tol=1e-10,
-                 workload="3-D linear elasticity P1 tets, 48^3 grid (117,649 nodes, 352,947 DOF), E=1 nu=0.3"),
-    "cfg3": dict(kind="solid3d", n=96, sigma=1.0, tol=1e-10,
+    "cfg1": dict(kind="poisson2d", n=128, sigma=0.5, tol=1e-8, lev=0, ordering="multicolour", solver="pcg",
+                 mixed="mixed_f32_bf16",
+                 workload="2-D Poisson P1, 128x128 cells (16,641 nodes), lognormal(0,0.5) diffusivity, "
+                          "ILU(0)-PCG to 1e-8"),
+    "cfg2": dict(kind="solid3d", n=48, sigma=0.0, tol=1e-10, lev=0, ordering="multicolour", solver="pcg",
+                 mixed="mixed_f32_bf16",
+                 workload="3-D linear elasticity P1 tets, 48^3 grid (117,649 nodes, 352,947 DOF), E=1 nu=0.3, "
+                          "ILU(0)-PCG to 1e-10"),
+    "cfg3": dict(kind="solid3d", n=96, sigma=1.0, tol=1e-10, lev=0, ordering="multicolour", solver="pcg",
+                 mixed="mixed_f32_bf16",
                  workload="3-D linear elasticity P1 tets, 96^3 grid (912,673 nodes, 2,738,019 DOF), "
-                          "E lognormal(0,1.0), nu=0.3"),
+                          "E lognormal(0,1.0), nu=0.3, ILU(0)-PCG to 1e-10"),
+    "cfg4": dict(kind="oseen2d", n=256, tol=1e-9, lev=1, ordering="block_multicolour", block_nodes=16,
+                 solver="bicg", mixed="mixed",
+                 workload="Oseen P1-P1 PSPG lid-driven cavity, 256x256 cells (66,049 nodes, 198,147 DOF), "
+                          "nu=0.01, Fourier advection, ILU(1)-BiCGSTAB to 1e-9"),
 }
+PREC_NAMES = {"fp64": 0, "mixed": 1, "mixed_bf16": 2, "mixed_f32": 3, "mixed_f32_bf16": 4}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_f32_elasticity96_traffic.txt")
 
 
 def log(*a):
@@ -43,7 +59,23 @@ def make_problem(cfg, seed):
     from delfem_amd import meshgen
     if cfg["kind"] == "poisson2d":
         return meshgen.poisson2d(cfg["n"], seed=seed, sigma=cfg["sigma"])
+    if cfg["kind"] == "oseen2d":
+        return meshgen.oseen2d(cfg["n"], seed=seed)
     return meshgen.solid3d(cfg["n"], seed=seed, sigma=cfg["sigma"])
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("{"):
+                    d = json.loads(line)
+                    if d.get("kernel") == kernel:
+                        return d
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -54,11 +86,11 @@ def main():
     ap.add_argument("--config", default="cfg3", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iters", type=int, default=20, help="PCG iterations in the CPU sample")
+    ap.add_argument("--cpu-iters", type=int, default=40, help="Krylov iterations in the CPU sample")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
-                          "n_gpus": args.gpus}), flush=True)
+        print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1", "n_gpus": args.gpus}),
+              flush=True)
         sys.exit(2)
 
     import torch
@@ -67,30 +99,30 @@ def main():
     cfg = CONFIGS[args.config]
     linsys.init(0)
     stream = torch.cuda.ExternalStream(int(_lib.lib().dfm_stream()))
-    log(f"[bench] building {args.config}: {cfg['workload']}")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    log(f"[bench] {args.config}: {cfg['workload']}")
     t0 = time.time()
     prob = make_problem(cfg, args.seed)
     s = linsys.FEMSystem(prob)
     torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record(stream)
     s.assemble()
     ev[1].record(stream)
     ev[1].synchronize()
     asm_ms = ev[0].elapsed_time(ev[1])
     rhs = s.res.clone()
-    log(f"[bench] mesh+pattern+assembly {time.time() - t0:.1f}s (device assembly {asm_ms:.2f} ms)")
+    log(f"[bench] mesh + pattern + assembly {time.time() - t0:.1f}s (device assembly {asm_ms:.2f} ms)")
     t0 = time.time()
-    M = linsys.CPreconditioner_ILU(s.A, 0, ordering="multicolour")
+    M = linsys.CPreconditioner_ILU(s.A, cfg["lev"], ordering=cfg["ordering"], block_nodes=cfg.get("block_nodes", 0))
     info = M.info()
-    log(f"[bench] ILU(0) symbolic + layouts {time.time() - t0:.1f}s: {info}")
+    log(f"[bench] ILU({cfg['lev']}) symbolic + layouts {time.time() - t0:.1f}s: {info}")
+    solve = linsys.Solve_PCG if cfg["solver"] == "pcg" else linsys.Solve_PBiCGSTAB
 
-    def step(precision):
+    def step(precision, tol=cfg["tol"]):
         ok_f = M.SetValue(s.A)
         r = rhs.clone()
         x = torch.zeros_like(r)
-        ok, conv, it, extra = linsys.Solve_PCG(cfg["tol"], 50000, s.A, M, r, x, precision=precision,
-                                               want_true_relres=1)
+        ok, conv, it, extra = solve(tol, 50000, s.A, M, r, x, precision=precision, want_true_relres=1)
         return ok and ok_f, conv, it, extra, x
 
     def timed(precision, steps, warmup):
@@ -103,46 +135,61 @@ def main():
         ev[1].synchronize()
         return ev[0].elapsed_time(ev[1]) / steps, outs[-1]
 
-    log("[bench] mixed-precision steps")
-    mixed_ms, (ok_m, conv_m, it_m, ex_m, x_m) = timed("mixed", args.steps, args.warmup)
-    log(f"[bench] mixed: {mixed_ms:.2f} ms/solve, inner iters {ex_m['inner_iters']}, outer {ex_m['outer_iters']}, "
-        f"true relres {ex_m['true_relres']:.3e}")
-    log("[bench] fp64 steps")
+    mixed = cfg["mixed"]
+    log(f"[bench] timed {mixed} steps")
+    mixed_ms, (ok_m, conv_m, it_m, ex_m, x_m) = timed(mixed, args.steps, args.warmup)
+    log(f"[bench] {mixed}: {mixed_ms:.2f} ms/solve, inner iters {ex_m['inner_iters']}, "
+        f"outer {ex_m['outer_iters']}, true relres {ex_m['true_relres']:.3e}, ok {ok_m}")
+    log("[bench] timed fp64 steps")
     fp64_ms, (ok_d, conv_d, it_d, ex_d, x_d) = timed("fp64", max(1, args.steps), min(1, args.warmup))
-    log(f"[bench] fp64: {fp64_ms:.2f} ms/solve, iters {it_d + 1}, conv {conv_d:.3e}, "
-        f"true relres {ex_d['true_relres']:.3e}")
-    # FP64 reference solution converged far below the target
-    r = rhs.clone()
-    x_ref = torch.zeros_like(r)
-    M.SetValue(s.A)
-    linsys.Solve_PCG(1e-13, 50000, s.A, M, r, x_ref)
+    iters_d = it_d + 1 if cfg["solver"] == "pcg" else it_d
+    log(f"[bench] fp64: {fp64_ms:.2f} ms/solve, iterations {iters_d}, conv {conv_d:.3e}, "
+        f"true relres {ex_d['true_relres']:.3e}, ok {ok_d}")
+    # other mixed modes, one solve each (reported, not the headline)
+    others = {}
+    for prec in PREC_NAMES:
+        if prec in ("fp64", mixed):
+            continue
+        ms, (ok_o, _, it_o, ex_o, _) = timed(prec, 1, 0)
+        others[prec] = dict(ms=round(ms, 3), inner_iters=ex_o["inner_iters"], outer_iters=ex_o["outer_iters"],
+                            relres=ex_o["true_relres"], ok=bool(ok_o))
+        log(f"[bench] {prec}: {ms:.2f} ms/solve, {ex_o['inner_iters']} iterations, ok {ok_o}")
+    # FP64 solution converged far below the target: the error reference
+    _, _, _, ex_ref, x_ref = step("fp64", tol=1e-13 if cfg["solver"] == "pcg" else 1e-12)
     nref = torch.linalg.norm(x_ref).item()
     err_m = torch.linalg.norm(x_m - x_ref).item() / nref
     err_d = torch.linalg.norm(x_d - x_ref).item() / nref
 
-    # dominant kernels on their real operands
+    # the iteration's kernels on their real operands, timed with HIP events
+    # on the library's stream (dfm_bench_kernel)
     kern = {}
-    for which, name in ((0, "spmv"), (1, "ilu_forward"), (2, "ilu_backward")):
-        for prec in ("mixed", "fp64"):
-            avg = C_double = None
-            import ctypes as C
+    for which, name in ((0, "spmv"), (1, "ilu_lower"), (2, "ilu_upper")):
+        for prec in (mixed, "fp64"):
             a, b = C.c_double(), C.c_double()
-            _lib.call("dfm_bench_kernel", s.A.h, M.h, which, 1 if prec == "mixed" else 0, 20, C.byref(a),
-                      C.byref(b))
+            _lib.call("dfm_bench_kernel", s.A.h, M.h, which, PREC_NAMES[prec], 50, C.byref(a), C.byref(b))
             gbs = b.value / (a.value * 1e-3) / 1e9
-            kern[f"{name}_{'f32' if prec == 'mixed' else 'f64'}"] = dict(ms=a.value, bytes=b.value, gbs=gbs,
-                                                                         frac=gbs / HBM_PEAK_GBS)
+            kern[f"{name}_{prec}"] = dict(ms=round(a.value, 5), bytes=b.value, gbs=round(gbs, 1),
+                                          frac=round(gbs / HBM_PEAK_GBS, 4))
     for k, v in kern.items():
         log(f"[bench] kernel {k}: {v['ms']:.4f} ms, {v['bytes'] / 1e6:.1f} MB, {v['gbs']:.0f} GB/s "
             f"({100 * v['frac']:.1f}% of peak)")
-    dom = kern["spmv_f32"]
+    dom = kern[f"spmv_{mixed}"]
+    kname = "k_spmv<3, float, float, true, true>" if mixed in ("mixed_f32", "mixed_f32_bf16") \
+        else "k_spmv<3, float, double, true, true>"
+    tr = read_traffic(kname) if args.config == "cfg3" else None
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(prob, s, cfg, args, it_d)
+        # the reference runs its own natural ordering where that converges;
+        # otherwise the benchmark's ordering, given to the reference's ordered ILU
+        iters_nat = natural_iterations(linsys, s, cfg, rhs, solve, torch) if cfg["ordering"] != "natural" else iters_d
+        if iters_nat is None:
+            cpu = cpu_baseline(prob, s, cfg, args, iters_d, M.pattern()["order"])
+        else:
+            cpu = cpu_baseline(prob, s, cfg, args, iters_nat, None)
 
     out = {
-        "metric": "ms per ILU-PCG solve to 1e-10 rel-res (mixed vs FP64); HBM GB/s % of peak",
+        "metric": METRIC,
         "value": round(mixed_ms, 3),
         "unit": "ms",
         "n_gpus": 1,
@@ -152,31 +199,52 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner (f64 refinement, f64 reductions)",
+        "dtype": "f32" if mixed in ("mixed_f32", "mixed_f32_bf16") else "f64",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "config": args.config, "seed": args.seed,
-                   "preconditioner": "ILU(0), multicolour ordering", "ncolour": info["ncolour"],
-                   "tol": cfg["tol"]},
+                   "preconditioner": f"ILU({cfg['lev']}), {cfg['ordering']} ordering",
+                   "ncolour": info["ncolour"], "levels": [info["nlevel_lower"], info["nlevel_upper"]],
+                   "tol": cfg["tol"], "precision": mixed},
         "fp64_ms": round(fp64_ms, 3),
         "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
-        "iters_fp64": it_d + 1,
+        "iters_fp64": iters_d,
         "inner_iters_mixed": ex_m["inner_iters"],
         "outer_iters_mixed": ex_m["outer_iters"],
         "relres_mixed": ex_m["true_relres"],
         "relres_fp64": ex_d["true_relres"],
-        "err_mixed_vs_fp64_1e-13": err_m,
-        "err_fp64_vs_fp64_1e-13": err_d,
+        "converged": bool(ok_m and ok_d),
+        "err_mixed_vs_fp64_tight": err_m,
+        "err_fp64_vs_fp64_tight": err_d,
+        "other_precisions": others,
         "assembly_ms": round(asm_ms, 3),
         "kernels": kern,
-        "roofline": {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom["frac"], 4), "traffic": None, "kernel": "k_spmv<3,float> (FP32 SpMV)"},
+        "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["frac"], "traffic": tr["bytes_per_launch"] if tr else None,
+                     "kernel": kname, "algorithmic_bytes": dom["bytes"], "avg_launch_ms": dom["ms"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(prob, s, cfg, args, gpu_iters):
-    """The DelFEM reference's Solve_PCG on the host (one core), sampled."""
+def natural_iterations(linsys, s, cfg, rhs, solve, torch):
+    """Iterations of the reference's own (natural-order) preconditioner on the
+    same system, FP64 on the device: the count the CPU sample is scaled to."""
+    if cfg["kind"] == "oseen2d":
+        return None   # natural-order ILU(1)-BiCGSTAB does not reach 1e-9 here; the CPU runs the same ordering
+    Mn = linsys.CPreconditioner_ILU(s.A, cfg["lev"])
+    Mn.SetValue(s.A)
+    r = rhs.clone()
+    x = torch.zeros_like(r)
+    t0 = time.time()
+    ok, conv, it, _ = solve(cfg["tol"], 50000, s.A, Mn, r, x)
+    log(f"[bench] natural-order FP64: {it + 1} iterations ({time.time() - t0:.1f}s)")
+    return it + 1 if cfg["solver"] == "pcg" else it
+
+
+def cpu_baseline(prob, s, cfg, args, iters, order):
+    """The DelFEM reference's solver (compiled from its sources) on the host,
+    one core, sampled: ILU numeric factorisation + a fixed number of Krylov
+    iterations, scaled to the iteration count of a full solve."""
     from oracle import oracle
     rowptr, col = s.A.pattern()
     val, dia = s.A.values()
@@ -192,33 +260,37 @@ def cpu_baseline(prob, s, cfg, args, gpu_iters):
         rs = oracle.RefSystem(ref, prob.nnode, prob.b, rowptr, col)
         rs.set_matrix(val, dia)
         rs.set_vector(-1, rhs)
-        pr = oracle.RefILU(rs, 0)
+        pr = oracle.RefILU(rs, cfg["lev"], order)
         t1 = time.time()
         pr.set_value()
         t_ilu = time.time() - t1
         t1 = time.time()
-        ok, conv, it = pr.solve("pcg", cfg["tol"], args.cpu_iters)
+        pr.solve(cfg["solver"], cfg["tol"], args.cpu_iters)
         t_it = (time.time() - t1) / args.cpu_iters
     else:
         orc = oracle.Orc()
-        F = orc.iluk_symbolic(rowptr, col, 0)
+        F = orc.iluk_symbolic(rowptr, col, cfg["lev"], order)
         t1 = time.time()
-        fv, fd, _ = orc.ilu_values(prob.nnode, prob.b, rowptr, col, val, dia, F)
+        fv, fd, _ = orc.ilu_values(prob.nnode, prob.b, rowptr, col, val, dia, F, order)
         t_ilu = time.time() - t1
         t1 = time.time()
-        orc.solve("pcg", prob.nnode, prob.b, rowptr, col, val, dia, F, fv, fd, rhs, cfg["tol"], args.cpu_iters)
+        orc.solve(cfg["solver"], prob.nnode, prob.b, rowptr, col, val, dia, F, fv, fd, rhs, cfg["tol"],
+                  args.cpu_iters, order)
         t_it = (time.time() - t1) / args.cpu_iters
-    log(f"[bench] cpu {kind}: ILU numeric {t_ilu:.2f}s, {1e3 * t_it:.1f} ms/iteration ({time.time() - t0:.1f}s)")
-    return None if t_it <= 0 else {
-        "value": round(1e3 * (t_ilu + t_it * gpu_iters), 1),
-        "unit": "ms per solve (estimated)",
+    log(f"[bench] cpu {kind}: ILU numeric {t_ilu:.2f}s, {1e3 * t_it:.1f} ms/iteration, sampled "
+        f"{time.time() - t0:.1f}s, scaled to {iters} iterations")
+    what = "natural-order" if order is None else f"{cfg['ordering']}-ordered"
+    return {
+        "value": round(1e3 * (t_ilu + t_it * iters), 1),
+        "unit": "ms per solve (ILU numeric + iterations x sampled ms/iteration)",
         "cores": 1,
         "kind": kind,
-        "sample": (f"ILU(0) numeric + {args.cpu_iters} natural-order PCG iterations of the same system, "
-                   f"scaled to {gpu_iters} iterations (the device FP64 natural-order count is not used; "
-                   f"scaled with the device FP64 multicolour count)"),
+        "sample": (f"the same assembled system: ILU({cfg['lev']}) numeric ({what}) + {args.cpu_iters} "
+                   f"{'PCG' if cfg['solver'] == 'pcg' else 'BiCGSTAB'} iterations timed on one core, scaled to "
+                   f"{iters} iterations (the device FP64 iteration count of the {what} preconditioner)"),
         "ilu_numeric_s": round(t_ilu, 3),
         "ms_per_iteration": round(1e3 * t_it, 2),
+        "iterations": iters,
     }
 
 

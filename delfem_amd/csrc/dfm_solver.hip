@@ -639,13 +639,21 @@ static int bench_kernel_t(dfm_bcrs_s* A, dfm_ilu_s* M, int which, int nrep, doub
     nnzL += M->fdiaind[i] - M->frow[i];
     nnzU += M->frow[i + 1] - M->fdiaind[i];
   }
+  // the instantiations the PCG iteration launches: SpMV with the fused <p,Ap>
+  // partials, lower sweep, upper sweep with the fused <r,z> partials
+  DFM_TRY(ensure_workspace(M, 0));
+  init_state(*M->h_state, 1.0, 0.0, 1u << 30, 0);
+  DFM_TRY(write_state(M));
+  KState* st = M->state.p;
+  double* part = M->partials.p;
   auto run = [&]() -> int {
-    if (which == 0)
-      return spmv<B, TA, TF, T, false, false>(o, xv.p, yv.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-    if (which == 1) return sweep_lower<B, TF, T>(*o.L, *o.planL, xv.p, yv.p, nullptr);
-    return sweep_upper<B, TF, T, false>(*o.U, *o.planU, yv.p, nullptr, nullptr, nullptr);
+    if (which == 0) return spmv<B, TA, TF, T, true, true>(o, xv.p, yv.p, nullptr, xv.p, nullptr, nullptr, part, st);
+    if (which == 1) return sweep_lower<B, TF, T>(*o.L, *o.planL, xv.p, yv.p, st);
+    return sweep_upper<B, TF, T, true>(*o.U, *o.planU, yv.p, xv.p, part, st);
   };
-  if (which == 0) *bytes = (nnzA + nb) * bb * sa + 4.0 * nnzA + 2.0 * n * sv;
+  // algorithmic bytes: values, column indices, row ids, each vector element
+  // read or written once (x of SpMV and the dot operand are the same vector)
+  if (which == 0) *bytes = (nnzA + nb) * bb * sa + 4.0 * nnzA + 4.0 * nb + 2.0 * n * sv;
   else if (which == 1) *bytes = (nnzL + nb) * bb * sf + 4.0 * nnzL + 4.0 * nb + 3.0 * n * sv;
   else *bytes = nnzU * bb * sf + 4.0 * nnzU + 4.0 * nb + 3.0 * n * sv;
   DFM_TRY(run());  // warm-up
