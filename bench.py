@@ -43,11 +43,12 @@ CONFIGS = {
                  workload="3-D linear elasticity P1 tets, 96^3 grid (912,673 nodes, 2,738,019 DOF), "
                           "E lognormal(0,1.0), nu=0.3, ILU(0)-PCG to 1e-10"),
     "cfg4": dict(kind="oseen2d", n=256, tol=1e-9, lev=1, ordering="block_multicolour", block_nodes=16,
-                 solver="bicg", mixed="mixed",
+                 solver="bicg", mixed="fac32", others=("fac16",),
                  workload="Oseen P1-P1 PSPG lid-driven cavity, 256x256 cells (66,049 nodes, 198,147 DOF), "
                           "nu=0.01, Fourier advection, ILU(1)-BiCGSTAB to 1e-9"),
 }
-PREC_NAMES = {"fp64": 0, "mixed": 1, "mixed_bf16": 2, "mixed_f32": 3, "mixed_f32_bf16": 4}
+PREC_NAMES = {"fp64": 0, "mixed": 1, "mixed_bf16": 2, "mixed_f32": 3, "mixed_f32_bf16": 4, "fac32": 5, "fac16": 6}
+OTHERS = ("mixed", "mixed_bf16", "mixed_f32", "mixed_f32_bf16", "fac32", "fac16")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_f32_elasticity96_traffic.txt")
 
 
@@ -118,19 +119,19 @@ def main():
     log(f"[bench] ILU({cfg['lev']}) symbolic + layouts {time.time() - t0:.1f}s: {info}")
     solve = linsys.Solve_PCG if cfg["solver"] == "pcg" else linsys.Solve_PBiCGSTAB
 
-    def step(precision, tol=cfg["tol"]):
+    def step(precision, tol=cfg["tol"], maxit=50000):
         ok_f = M.SetValue(s.A)
         r = rhs.clone()
         x = torch.zeros_like(r)
-        ok, conv, it, extra = solve(tol, 50000, s.A, M, r, x, precision=precision, want_true_relres=1)
+        ok, conv, it, extra = solve(tol, maxit, s.A, M, r, x, precision=precision, want_true_relres=1)
         return ok and ok_f, conv, it, extra, x
 
-    def timed(precision, steps, warmup):
+    def timed(precision, steps, warmup, maxit=50000):
         for _ in range(warmup):
-            step(precision)
+            step(precision, maxit=maxit)
         torch.cuda.synchronize()
         ev[0].record(stream)
-        outs = [step(precision) for _ in range(steps)]
+        outs = [step(precision, maxit=maxit) for _ in range(steps)]
         ev[1].record(stream)
         ev[1].synchronize()
         return ev[0].elapsed_time(ev[1]) / steps, outs[-1]
@@ -147,10 +148,10 @@ def main():
         f"true relres {ex_d['true_relres']:.3e}, ok {ok_d}")
     # other mixed modes, one solve each (reported, not the headline)
     others = {}
-    for prec in PREC_NAMES:
-        if prec in ("fp64", mixed):
+    for prec in cfg.get("others", OTHERS):
+        if prec == mixed:
             continue
-        ms, (ok_o, _, it_o, ex_o, _) = timed(prec, 1, 0)
+        ms, (ok_o, _, it_o, ex_o, _) = timed(prec, 1, 0, maxit=4 * iters_d)
         others[prec] = dict(ms=round(ms, 3), inner_iters=ex_o["inner_iters"], outer_iters=ex_o["outer_iters"],
                             relres=ex_o["true_relres"], ok=bool(ok_o))
         log(f"[bench] {prec}: {ms:.2f} ms/solve, {ex_o['inner_iters']} iterations, ok {ok_o}")
@@ -174,8 +175,9 @@ def main():
         log(f"[bench] kernel {k}: {v['ms']:.4f} ms, {v['bytes'] / 1e6:.1f} MB, {v['gbs']:.0f} GB/s "
             f"({100 * v['frac']:.1f}% of peak)")
     dom = kern[f"spmv_{mixed}"]
-    kname = "k_spmv<3, float, float, true, true>" if mixed in ("mixed_f32", "mixed_f32_bf16") \
-        else "k_spmv<3, float, double, true, true>"
+    kname = {"mixed_f32": "k_spmv<3, float, float, true, true>", "mixed_f32_bf16": "k_spmv<3, float, float, true, true>",
+             "mixed": "k_spmv<3, float, double, true, true>", "mixed_bf16": "k_spmv<3, float, double, true, true>"
+             }.get(mixed, "k_spmv<3, double, double, true, true>")
     tr = read_traffic(kname) if args.config == "cfg3" else None
 
     cpu = None

@@ -29,9 +29,12 @@ DFM_PREC_MIXED = 1
 DFM_PREC_MIXED_BF16 = 2
 DFM_PREC_MIXED_F32 = 3
 DFM_PREC_MIXED_F32_BF16 = 4
+DFM_PREC_FAC32 = 5
+DFM_PREC_FAC16 = 6
 
 PRECISIONS = {"fp64": DFM_PREC_FP64, "mixed": DFM_PREC_MIXED, "mixed_bf16": DFM_PREC_MIXED_BF16,
-              "mixed_f32": DFM_PREC_MIXED_F32, "mixed_f32_bf16": DFM_PREC_MIXED_F32_BF16}
+              "mixed_f32": DFM_PREC_MIXED_F32, "mixed_f32_bf16": DFM_PREC_MIXED_F32_BF16,
+              "fac32": DFM_PREC_FAC32, "fac16": DFM_PREC_FAC16}
 
 
 class SolveOpts(C.Structure):

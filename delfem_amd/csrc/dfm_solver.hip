@@ -533,7 +533,7 @@ static int solve_dispatch(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r_user
   const int prec = opts ? opts->precision : DFM_PREC_FP64;
   const bool mixed = prec != DFM_PREC_FP64;
   const bool vec32 = prec == DFM_PREC_MIXED_F32 || prec == DFM_PREC_MIXED_F32_BF16;
-  const bool fac16 = prec == DFM_PREC_MIXED_BF16 || prec == DFM_PREC_MIXED_F32_BF16;
+  const bool fac16 = prec == DFM_PREC_MIXED_BF16 || prec == DFM_PREC_MIXED_F32_BF16 || prec == DFM_PREC_FAC16;
   const unsigned check_every = (opts && opts->check_every) ? opts->check_every : 16;
   const bool want_true = opts && (opts->want_true_relres || mixed);
   const double tol = *conv_ratio;
@@ -586,6 +586,14 @@ static int solve_dispatch(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r_user
         DFM_TRY((mixed_core<B, float, float, float>(M, bicg, b, r, x, reinterpret_cast<float*>(wsolver), tol, maxit,
                                                     check_every, delta, kr, nup)));
         break;
+      case DFM_PREC_FAC32:
+        DFM_TRY((mixed_core<B, double, float, double>(M, bicg, b, r, x, reinterpret_cast<double*>(wsolver), tol,
+                                                      maxit, check_every, delta, kr, nup)));
+        break;
+      case DFM_PREC_FAC16:
+        DFM_TRY((mixed_core<B, double, bf16s, double>(M, bicg, b, r, x, reinterpret_cast<double*>(wsolver), tol,
+                                                      maxit, check_every, delta, kr, nup)));
+        break;
       default:
         DFM_TRY((mixed_core<B, float, bf16s, float>(M, bicg, b, r, x, reinterpret_cast<float*>(wsolver), tol, maxit,
                                                     check_every, delta, kr, nup)));
@@ -608,7 +616,7 @@ static int solve_entry(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r, double
   DFM_CHECK(A && M && r && x && conv_ratio && iteration, DFM_BAD_ARG, "bad arguments to the solver");
   DFM_CHECK(A->nb == M->nb && A->b == M->b, DFM_BAD_ARG, "factor and matrix do not match");
   DFM_CHECK(M->factored, DFM_BAD_ARG, "factor has no values (call dfm_ilu_set_value)");
-  DFM_CHECK(!opts || (opts->precision >= DFM_PREC_FP64 && opts->precision <= DFM_PREC_MIXED_F32_BF16), DFM_BAD_ARG,
+  DFM_CHECK(!opts || (opts->precision >= DFM_PREC_FP64 && opts->precision <= DFM_PREC_FAC16), DFM_BAD_ARG,
             "unknown precision %d", opts ? opts->precision : 0);
   switch (A->b) {
     case 1: return solve_dispatch<1>(A, M, bicg, r, x, conv_ratio, iteration, opts);
@@ -679,6 +687,8 @@ static int bench_dispatch(dfm_bcrs_s* A, dfm_ilu_s* M, int which, int prec, int 
     case DFM_PREC_MIXED: return bench_kernel_t<B, float, float, double>(A, M, which, nrep, ms, bytes);
     case DFM_PREC_MIXED_BF16: return bench_kernel_t<B, float, bf16s, double>(A, M, which, nrep, ms, bytes);
     case DFM_PREC_MIXED_F32: return bench_kernel_t<B, float, float, float>(A, M, which, nrep, ms, bytes);
+    case DFM_PREC_FAC32: return bench_kernel_t<B, double, float, double>(A, M, which, nrep, ms, bytes);
+    case DFM_PREC_FAC16: return bench_kernel_t<B, double, bf16s, double>(A, M, which, nrep, ms, bytes);
     default: return bench_kernel_t<B, float, bf16s, float>(A, M, which, nrep, ms, bytes);
   }
 }
@@ -689,9 +699,10 @@ extern "C" {
 int dfm_bench_kernel(dfm_bcrs* A, dfm_ilu* M, int which, int precision, int nrep, double* avg_ms, double* bytes) {
   using namespace dfm;
   DFM_CHECK(A && M && avg_ms && bytes && nrep > 0 && which >= 0 && which <= 2, DFM_BAD_ARG, "bad arguments");
-  DFM_CHECK(precision >= DFM_PREC_FP64 && precision <= DFM_PREC_MIXED_F32_BF16, DFM_BAD_ARG, "bad precision");
+  DFM_CHECK(precision >= DFM_PREC_FP64 && precision <= DFM_PREC_FAC16, DFM_BAD_ARG, "bad precision");
   DFM_CHECK(M->factored, DFM_BAD_ARG, "factor has no values (call dfm_ilu_set_value)");
-  const bool fac16 = precision == DFM_PREC_MIXED_BF16 || precision == DFM_PREC_MIXED_F32_BF16;
+  const bool fac16 = precision == DFM_PREC_MIXED_BF16 || precision == DFM_PREC_MIXED_F32_BF16 ||
+                     precision == DFM_PREC_FAC16;
   DFM_TRY(ilu_refresh_sell(M, A, precision != DFM_PREC_FP64, fac16));
   switch (A->b) {
     case 1: return bench_dispatch<1>(A, M, which, precision, nrep, avg_ms, bytes);

@@ -163,7 +163,12 @@ int dfm_ilu_solve_precond(dfm_ilu *M, double *vec_dev);
  *   MIXED_BF16       f64 vectors, f32 matrix, bf16 factors
  *   MIXED_F32        f32 vectors, f32 matrix, f32 factors
  *   MIXED_F32_BF16   f32 vectors, f32 matrix, bf16 factors
- * All MIXED modes accumulate dot products in f64 and run FP64 reliable
+ *   FAC32            f64 vectors, f64 matrix, f32 factors
+ *   FAC16            f64 vectors, f64 matrix, bf16 factors
+ * FAC modes keep the Krylov operator in FP64 and lower only the
+ * preconditioner: the choice for systems too ill-conditioned for an FP32
+ * matrix (cond(A) * 2^-24 near or above 1).
+ * All modes other than FP64 accumulate dot products in f64 and run FP64 reliable
  * updates: when the recursive residual has dropped by reliable_delta, the
  * partial solution is added to the f64 solution and the true residual
  * b - A x is recomputed with the f64 matrix and replaces the recursive one.
@@ -173,7 +178,9 @@ enum {
     DFM_PREC_MIXED = 1,
     DFM_PREC_MIXED_BF16 = 2,
     DFM_PREC_MIXED_F32 = 3,
-    DFM_PREC_MIXED_F32_BF16 = 4
+    DFM_PREC_MIXED_F32_BF16 = 4,
+    DFM_PREC_FAC32 = 5,
+    DFM_PREC_FAC16 = 6
 };
 typedef struct {
     int precision;         /* DFM_PREC_* (above) */

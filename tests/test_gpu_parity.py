@@ -157,7 +157,7 @@ def test_krylov_fp64(setup, orc, kind, ordering):
     assert true_relres(s, x, ref["res"]) < 1e-9
 
 
-@pytest.mark.parametrize("precision", ["mixed", "mixed_bf16", "mixed_f32", "mixed_f32_bf16"])
+@pytest.mark.parametrize("precision", ["mixed", "mixed_bf16", "mixed_f32", "mixed_f32_bf16", "fac32", "fac16"])
 @pytest.mark.parametrize("kind", ["pcg", "bicg"])
 @pytest.mark.parametrize("ordering", ORDERINGS)
 def test_krylov_mixed(setup, orc, ordering, kind, precision):
