@@ -12,7 +12,8 @@ path (reference semantics) is timed beside it.
 
 Other workloads: --config cfg1 (2-D Poisson 128^2, ILU(0)-PCG to 1e-8),
 cfg2 (elasticity 48^3, FP64 ILU(0)-PCG to 1e-10), cfg4 (Oseen P1-P1 PSPG
-lid-driven cavity 256^2, ILU(1)-BiCGSTAB to 1e-9).
+lid-driven cavity 256^2, ILU(1)-BiCGSTAB to 1e-9), cfg5 (512 independent
+Helmholtz systems, batched ILU(0)-PCG to 1e-8 in one persistent launch).
 
 Prints one JSON line:  python bench.py [--steps K] [--warmup W] [--config cfg3]
 """
@@ -42,6 +43,10 @@ CONFIGS = {
                  mixed="mixed_f32_bf16",
                  workload="3-D linear elasticity P1 tets, 96^3 grid (912,673 nodes, 2,738,019 DOF), "
                           "E lognormal(0,1.0), nu=0.3, ILU(0)-PCG to 1e-10"),
+    "cfg5": dict(kind="batch", count=512, tol=1e-8, maxit=20000,
+                 workload="512 independent 2-D P1 positive Helmholtz systems (rectangles, 500-50,000 nodes "
+                          "log-uniform, aspect 1-4, lognormal(0,0.5) diffusivity, c uniform [0,10]), "
+                          "multicolour ILU(0)-PCG to 1e-8 each, one persistent launch"),
     "cfg4": dict(kind="oseen2d", n=256, tol=1e-9, lev=1, ordering="block_multicolour", block_nodes=16,
                  solver="bicg", mixed="fac32", others=("fac16",),
                  workload="Oseen P1-P1 PSPG lid-driven cavity, 256x256 cells (66,049 nodes, 198,147 DOF), "
@@ -99,6 +104,8 @@ def main():
 
     cfg = CONFIGS[args.config]
     linsys.init(0)
+    if cfg["kind"] == "batch":
+        return bench_batch(cfg, args, torch, linsys, _lib)
     stream = torch.cuda.ExternalStream(int(_lib.lib().dfm_stream()))
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     log(f"[bench] {args.config}: {cfg['workload']}")
@@ -226,6 +233,155 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_batch(cfg, args, torch, linsys, _lib):
+    """Batched workload: every step = ILU(0) numeric + PCG for all systems in
+    one persistent launch (graph replay), FP32 with FP64 reliable updates;
+    the FP64 batch (reference semantics per system) beside it."""
+    import numpy as np
+    from delfem_amd import meshgen
+    stream = torch.cuda.ExternalStream(int(_lib.lib().dfm_stream()))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    log(f"[bench] {args.config}: {cfg['workload']}")
+    t0 = time.time()
+    parts = meshgen.helmholtz_batch(cfg["count"], seed=args.seed)
+    prob = meshgen.concat(parts)
+    fem = linsys.FEMSystem(prob)
+    fem.assemble()
+    B = linsys.BatchSystem(fem, prob.sys_ptr)
+    B.SetValue()
+    rhs = fem.res.clone()
+    x = torch.zeros_like(rhs)
+    nn = np.diff(prob.sys_ptr.astype(np.int64))
+    log(f"[bench] {len(parts)} systems, {prob.nnode} nodes (min {nn.min()}, max {nn.max()}), "
+        f"setup {time.time() - t0:.1f}s, {B.info()}")
+    tune = B.autotune(rhs, x, cfg["tol"], cfg["maxit"], "mixed_f32")
+    shape = min(tune, key=tune.get)
+    log("[bench] autotune (block, workgroups per CU) -> ms: " +
+        ", ".join(f"{k}: {v:.1f}" for k, v in sorted(tune.items())) + f"; chosen {shape}")
+
+    def timed(prec, steps, warmup):
+        for _ in range(warmup):
+            B.SetValue()
+            B.solve(rhs, x, cfg["tol"], cfg["maxit"], prec)
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        for _ in range(steps):
+            B.SetValue()
+            ok, out = B.solve(rhs, x, cfg["tol"], cfg["maxit"], prec)
+        ev[1].record(stream)
+        ev[1].synchronize()
+        return ev[0].elapsed_time(ev[1]) / steps, ok, out
+
+    mixed_ms, ok_m, out_m = timed("mixed_f32", args.steps, args.warmup)
+    x_m = x.clone()
+    fp64_ms, ok_d, out_d = timed("fp64", max(1, args.steps), min(1, args.warmup))
+    x_d = x.clone()
+    B.solve(rhs, x, 1e-12, cfg["maxit"], "fp64")
+    errs = []
+    xm, xd, xt = x_m.cpu().numpy(), x_d.cpu().numpy(), x.cpu().numpy()
+    for s in range(len(parts)):
+        r0, r1 = int(prob.sys_ptr[s]), int(prob.sys_ptr[s + 1])
+        nt = np.linalg.norm(xt[r0:r1])
+        errs.append((np.linalg.norm(xm[r0:r1] - xt[r0:r1]) / nt, np.linalg.norm(xd[r0:r1] - xt[r0:r1]) / nt))
+    errs = np.array(errs)
+    sm, dm = out_m["solve_ms"], out_m["done_ms"]
+    log(f"[bench] mixed: {mixed_ms:.2f} ms per batch, {len(parts) / mixed_ms * 1e3:.0f} solves/s, per-system solve "
+        f"p50 {np.percentile(sm, 50):.2f} p99 {np.percentile(sm, 99):.2f} ms; fp64 {fp64_ms:.2f} ms")
+    # algorithmic bytes of the mixed step: per iteration the FP32 matrix and
+    # factor slots (column + value), diagonals, and ten vector passes
+    w_nnz = np.diff(fem.A.pattern()[0].astype(np.int64))
+    nnz_s = np.add.reduceat(w_nnz, prob.sys_ptr[:-1].astype(np.int64))
+    per_it = 2 * nnz_s * 8 + nn * (4 + 4 + 10 * 4)
+    setup = nnz_s * (8 + 8 + 4 + 4) + nn * (8 + 8 + 4 + 4)
+    total_bytes = float(np.sum(per_it * out_m["iters"].astype(np.int64) + setup))
+    gbs = total_bytes / (mixed_ms * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_baseline_batch(prob, fem, cfg, out_d, args)
+    out = {
+        "metric": METRIC,
+        "value": round(mixed_ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed_ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "config": args.config, "seed": args.seed, "systems": len(parts),
+                   "nodes": int(prob.nnode), "tol": cfg["tol"], "launch": list(shape),
+                   "autotune_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in tune.items()}},
+        "solves_per_s": round(len(parts) / mixed_ms * 1e3, 1),
+        "latency_ms": {"solve_p50": round(float(np.percentile(sm, 50)), 3),
+                       "solve_p99": round(float(np.percentile(sm, 99)), 3),
+                       "solve_max": round(float(sm.max()), 3),
+                       "done_p50": round(float(np.percentile(dm, 50)), 3),
+                       "done_p99": round(float(np.percentile(dm, 99)), 3)},
+        "fp64_ms": round(fp64_ms, 3),
+        "fp64_solves_per_s": round(len(parts) / fp64_ms * 1e3, 1),
+        "converged": bool(ok_m and ok_d),
+        "iters_mixed": {"p50": float(np.percentile(out_m["iters"], 50)), "max": int(out_m["iters"].max())},
+        "iters_fp64": {"p50": float(np.percentile(out_d["iters"], 50)), "max": int(out_d["iters"].max())},
+        "relres_mixed_max": float(out_m["relres"].max()),
+        "relres_fp64_max": float(out_d["relres"].max()),
+        "err_mixed_vs_fp64_tight_max": float(errs[:, 0].max()),
+        "err_fp64_vs_fp64_tight_max": float(errs[:, 1].max()),
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_batch_solve<6, true> (whole step)", "algorithmic_bytes": total_bytes,
+                     "avg_launch_ms": round(mixed_ms, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_batch(prob, fem, cfg, out_d, args):
+    """The reference's natural-order ILU(0) + Solve_PCG on one core for a
+    sample of the systems (every 4th by size), scaled to the whole batch by
+    rows x iterations of the device FP64 solves."""
+    import numpy as np
+    from oracle import oracle
+    try:
+        ref = oracle.Ref()
+        kind = "reference"
+    except RuntimeError:
+        return None
+    rowptr, col = fem.A.pattern()
+    val, dia = fem.A.values()
+    rhs = fem.res.cpu().numpy()
+    nn = np.diff(prob.sys_ptr.astype(np.int64))
+    order = np.argsort(-nn, kind="stable")
+    sample = order[::4]
+    t_total = 0.0
+    for s in sample:
+        r0, r1 = int(prob.sys_ptr[s]), int(prob.sys_ptr[s + 1])
+        q0, q1 = int(rowptr[r0]), int(rowptr[r1])
+        rs = oracle.RefSystem(ref, r1 - r0, 1, (rowptr[r0:r1 + 1] - q0).astype(np.uint32),
+                              (col[q0:q1] - r0).astype(np.uint32))
+        rs.set_matrix(val[q0:q1], dia[r0:r1])
+        rs.set_vector(-1, rhs[r0:r1])
+        t1 = time.time()
+        pr = oracle.RefILU(rs, 0)
+        pr.set_value()
+        pr.solve("pcg", cfg["tol"], cfg["maxit"])
+        t_total += time.time() - t1
+    work = nn * (out_d["iters"].astype(np.int64) + 1)
+    scale = float(work.sum()) / float(work[sample].sum())
+    est = t_total * scale
+    log(f"[bench] cpu reference: {len(sample)} systems in {t_total:.1f}s, whole batch estimated {est:.1f}s")
+    return {
+        "value": round(1e3 * est, 1),
+        "unit": "ms per batch of all systems (estimated from the sample)",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"{len(sample)} of {len(nn)} systems (every 4th by size), each the reference's natural-order "
+                   f"ILU(0) setup + Solve_PCG to {cfg['tol']} on one core, {t_total:.1f}s; scaled by "
+                   f"sum(rows x device FP64 iterations) over all systems / over the sample ({scale:.2f})"),
+        "solves_per_s": round(len(nn) / est, 2),
+    }
 
 
 def natural_iterations(linsys, s, cfg, rhs, solve, torch):

@@ -88,6 +88,13 @@ _SIGS = {
     "dfm_asm_destroy": (C.c_int, [H]),
     "dfm_assemble_poisson2d_p1": (C.c_int, [H, vp, vp, C.c_double, vp, vp]),
     "dfm_assemble_solid3d_p1": (C.c_int, [H, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double, vp, vp]),
+    "dfm_batch_create": (C.c_int, [H, C.c_uint, vp, C.POINTER(H)]),
+    "dfm_batch_destroy": (C.c_int, [H]),
+    "dfm_batch_info": (C.c_int, [H, vp, vp, vp, vp]),
+    "dfm_batch_set_value": (C.c_int, [H, H]),
+    "dfm_batch_set_launch": (C.c_int, [H, C.c_int, C.c_int]),
+    "dfm_batch_solve_pcg": (C.c_int, [H, vp, vp, C.c_double, C.c_uint, C.c_int, vp, vp, vp, vp, vp]),
+    "dfm_batch_get_factor": (C.c_int, [H, C.c_uint, vp, vp, vp]),
     "dfm_assemble_oseen2d_p1p1": (C.c_int, [H, vp, C.c_double, C.c_double, vp, vp, vp]),
     "dfm_assemble_helmholtz2d_p1": (C.c_int, [H, vp, vp, vp, C.c_double, vp, vp]),
     "dfm_ilu_create": (C.c_int, [H, C.c_int, C.c_int, vp, C.c_uint, C.POINTER(H)]),

@@ -1,0 +1,900 @@
+// dfm_batch.hip -- many independent scalar FE systems solved in one
+// persistent launch: per system a multicolour ILU(0) numeric factorisation
+// (DoILUDecomp, src/matvec/matdiafrac_blkcrs.cpp:769-1102, b = 1) and a
+// preconditioned CG (LsSol::Solve_PCG, src/ls/solver_ls_iter.cpp:100-172),
+// in FP64 or in FP32 with FP64 reliable updates.
+//
+// Layout.  The systems are the diagonal blocks of one assembled block-CRS
+// matrix (1x1 blocks), rows [sys_ptr[s], sys_ptr[s+1]).  Each system is
+// renumbered by a greedy colouring of its graph (rows of one colour are
+// independent in both triangular sweeps) and its off-diagonal entries are
+// stored column-major ELL, slot k of new row i at ell0 + k*n + i, slots in
+// ascending new column, so the L part precedes the U part and a wavefront
+// reads consecutive rows.  A workgroup owns one system at a time: it takes the
+// next system from a work counter (largest first), factors it colour by
+// colour, iterates with workgroup barriers only, writes its result and takes
+// the next.  Dot products are workgroup reductions in FP64 in a fixed order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <memory>
+#include <utility>
+
+#include "dfm_internal.h"
+
+namespace dfm {
+
+constexpr int kMaxColourBatch = 16;
+
+struct SysDesc {
+  int row0;       // first row in the concatenated (new) numbering
+  int n;          // rows
+  int w;          // ELL width
+  int ncol;       // colours
+  long long ell0; // first ELL slot
+};
+
+struct BatchOut {
+  int iters;      // PCG iterations (FP64: the reference's 0-based index at convergence)
+  int outer;      // reliable updates (FP32 mode)
+  int status;     // 1 converged, 0 not converged, -1 singular pivots > 10
+  int nsing;      // singular pivots met
+  double relres;  // ||b - A x|| / ||b|| in FP64 at the end
+  unsigned long long t0, t1;  // wall clock at pick-up and at completion
+};
+
+}  // namespace dfm
+
+struct dfm_batch_s {
+  unsigned nsys = 0, nrow = 0;
+  int max_w = 0, max_ncol = 0;
+  std::vector<unsigned> sys_ptr;
+  std::vector<int> perm;                 // new (concatenated) -> old row
+  std::vector<dfm::SysDesc> desc;
+  std::vector<int> cptr;                 // per system kMaxColourBatch+1 colour offsets (local rows)
+  dfm::DevBuf<dfm::SysDesc> d_sys;
+  dfm::DevBuf<int> d_cptr, d_sched, d_perm, d_ell_col;
+  dfm::DevBuf<long long> d_ell_src;
+  dfm::DevBuf<unsigned> d_next;
+  dfm::DevBuf<double> d_a64, d_f64, d_adia64, d_fdia64, d_b64, d_x64, d_r64;
+  dfm::DevBuf<float> d_a32, d_f32, d_adia32, d_fdia32;
+  dfm::DevBuf<char> d_vec;               // solver vectors r, z, p, q, d (T = float or double)
+  dfm::DevBuf<dfm::BatchOut> d_out;
+  const dfm_bcrs_s* A = nullptr;
+  int block = 512, wpc = 1;              // launch shape
+  bool tuned = false;
+  std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+  ~dfm_batch_s() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+  }
+};
+
+namespace dfm {
+
+// ---------------------------------------------------------------------------
+// Device side
+struct BatchArgs {
+  int nsys;
+  const SysDesc* sys;
+  const int* cptr;
+  const int* sched;
+  unsigned* next;
+  const int* ecol;
+  const double* a64;
+  double* f64;
+  const double* adia64;
+  double* fdia64;
+  float* a32;
+  float* f32;
+  float* adia32;
+  float* fdia32;
+  const double* b64;
+  double* x64;
+  double* r64;
+  void* vec;          // 5 vectors of nrow entries, type T
+  long nrow;
+  double tol2;
+  int maxit;
+  BatchOut* out;
+};
+
+// all-reduce over the workgroup, result in every thread; alternating LDS
+// buffers make one barrier per reduction enough
+__device__ inline double wg_sum(double v, double (*sh)[16], int& buf) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[buf][w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < nw; i++) s += sh[buf][i];
+  buf ^= 1;
+  return s;
+}
+
+// Row access.  A row's W column slots are loaded together (predicated on the
+// system's width, padding -1), then the value and vector loads of every slot
+// are issued together, so a wavefront has 2W independent loads in flight
+// instead of a chain of dependent ones; the arithmetic then runs in slot
+// (ascending column) order.
+template <int W>
+__device__ inline void ell_cols(const int* __restrict__ ecol, const SysDesc& d, int i, int (&c)[W]) {
+#pragma unroll
+  for (int k = 0; k < W; k++) c[k] = k < d.w ? ecol[d.ell0 + (long long)k * d.n + i] : -1;
+}
+
+// Rows are taken U at a time per thread (rows i, i+T, .., i+(U-1)T), their
+// loads issued together before any arithmetic.
+template <int W>
+constexpr int rows_per_pass() { return W <= 8 ? 2 : 1; }
+
+// y = A x over rows [0, n) (ELL + diagonal); returns the <x, y> partial
+template <int W, typename V, typename T>
+__device__ inline double ell_spmv(const SysDesc& d, const int* __restrict__ ecol, const V* __restrict__ a,
+                                  const V* __restrict__ adia, const T* x, T* y) {
+  constexpr int U = rows_per_pass<W>();
+  const int NT = blockDim.x;
+  double part = 0.0;
+  for (int i0 = threadIdx.x; i0 < d.n; i0 += U * NT) {
+    int c[U][W];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * NT;
+      if (i < d.n) {
+        ell_cols<W>(ecol, d, i, c[u]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < W; k++) c[u][k] = -1;
+      }
+    }
+    T v[U][W], xi[U], di[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * NT;
+#pragma unroll
+      for (int k = 0; k < W; k++)
+        v[u][k] = c[u][k] >= 0 ? (T)a[d.ell0 + (long long)k * d.n + i] * x[c[u][k]] : (T)0;
+      xi[u] = i < d.n ? x[i] : (T)0;
+      di[u] = i < d.n ? (T)adia[d.row0 + i] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * NT;
+      if (i >= d.n) continue;
+      T acc = 0;
+#pragma unroll
+      for (int k = 0; k < W; k++) acc += v[u][k];
+      acc += di[u] * xi[u];
+      y[i] = acc;
+      part += (double)xi[u] * (double)acc;
+    }
+  }
+  return part;
+}
+
+// One substitution step over rows [lo, hi) of one colour.  Forward
+// (LOWER): z_i = Dinv_i (r_i - sum_{j<i} L_ij z_j); backward: z_i -= sum_{j>i}
+// U_ij z_j, returning the <r, z> partial of the finished rows.  Each sum in
+// ascending column order, as the reference's substitutions.
+template <int W, bool LOWER, typename V, typename T>
+__device__ inline double ell_subst(const SysDesc& d, int lo, int hi, const int* __restrict__ ecol,
+                                   const V* __restrict__ f, const V* __restrict__ fdia, const T* r, T* z) {
+  constexpr int U = rows_per_pass<W>();
+  const int NT = blockDim.x;
+  double part = 0.0;
+  for (int i0 = lo + threadIdx.x; i0 < hi; i0 += U * NT) {
+    int c[U][W];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * NT;
+      if (i < hi) {
+        ell_cols<W>(ecol, d, i, c[u]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < W; k++) c[u][k] = -1;
+      }
+    }
+    T v[U][W], t[U], di[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * NT;
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        const bool use = LOWER ? (c[u][k] >= 0 && c[u][k] < i) : (c[u][k] > i);
+        v[u][k] = use ? (T)f[d.ell0 + (long long)k * d.n + i] * z[c[u][k]] : (T)0;
+      }
+      t[u] = i < hi ? (LOWER ? r[i] : z[i]) : (T)0;
+      di[u] = (LOWER && i < hi) ? (T)fdia[d.row0 + i] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * NT;
+      if (i >= hi) continue;
+      T tt = t[u];
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        const bool use = LOWER ? (c[u][k] >= 0 && c[u][k] < i) : (c[u][k] > i);
+        if (use) tt -= v[u][k];
+      }
+      if (LOWER) {
+        z[i] = di[u] * tt;
+      } else {
+        z[i] = tt;
+        part += (double)r[i] * (double)tt;
+      }
+    }
+  }
+  return part;
+}
+
+// z = (LU)^-1 r colour by colour (forward), then in reverse (backward); returns <r, z>
+template <int W, typename V, typename T>
+__device__ inline double ell_precond(const SysDesc& d, const int* __restrict__ cp, const int* __restrict__ ecol,
+                                     const V* __restrict__ f, const V* __restrict__ fdia, const T* r, T* z) {
+  for (int c = 0; c < d.ncol; c++) {
+    ell_subst<W, true, V, T>(d, cp[c], cp[c + 1], ecol, f, fdia, r, z);
+    __syncthreads();
+  }
+  double part = 0.0;
+  for (int c = d.ncol - 1; c >= 0; c--) {
+    part += ell_subst<W, false, V, T>(d, cp[c], cp[c + 1], ecol, f, fdia, r, z);
+    __syncthreads();
+  }
+  return part;
+}
+
+// The reference's PCG loop on this system (FP64 copies, x from 0).  Stops
+// when <r,r> < tol2 <r0,r0>; returns the reference's iteration value and
+// leaves <r,r>/<r0,r0> in rel2.
+template <int W>
+__device__ inline int wg_pcg64(const SysDesc& d, const int* cp, const int* ecol, const double* a, const double* adia,
+                               const double* f, const double* fdia, double* r, double* x, double* z, double* p,
+                               double* q, double tol2, int maxit, double (*sh)[16], int& buf, double& rel2) {
+  double rr0 = 0.0;
+  for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+    x[i] = 0.0;
+    rr0 += r[i] * r[i];
+  }
+  rr0 = wg_sum(rr0, sh, buf);
+  rel2 = 0.0;
+  if (rr0 < 1.0e-30) return 0;
+  const double inv_rr0 = 1.0 / rr0;
+  double rz = wg_sum(ell_precond<W, double, double>(d, cp, ecol, f, fdia, r, z), sh, buf);
+  for (int i = threadIdx.x; i < d.n; i += blockDim.x) p[i] = z[i];
+  __syncthreads();
+  rel2 = 1.0;
+  for (int it = 0; it < maxit; it++) {
+    const double pap = wg_sum(ell_spmv<W, double, double>(d, ecol, a, adia, p, q), sh, buf);
+    const double alpha = rz / pap;
+    double rr = 0.0;
+    constexpr int U = rows_per_pass<W>();
+    for (int i0 = threadIdx.x; i0 < d.n; i0 += U * (int)blockDim.x) {
+      double xv[U], pv[U], rv[U], qv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i < d.n) {
+          xv[u] = x[i];
+          pv[u] = p[i];
+          rv[u] = r[i];
+          qv[u] = q[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i >= d.n) continue;
+        x[i] = xv[u] + alpha * pv[u];
+        const double rn = rv[u] + (-alpha) * qv[u];
+        r[i] = rn;
+        rr += rn * rn;
+      }
+    }
+    rr = wg_sum(rr, sh, buf);
+    rel2 = rr * inv_rr0;
+    if (rel2 < tol2) return it;
+    const double rz_new = wg_sum(ell_precond<W, double, double>(d, cp, ecol, f, fdia, r, z), sh, buf);
+    const double beta = rz_new / rz;
+    rz = rz_new;
+    for (int i0 = threadIdx.x; i0 < d.n; i0 += U * (int)blockDim.x) {
+      double pv[U], zv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i < d.n) {
+          pv[u] = p[i];
+          zv[u] = z[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i < d.n) p[i] = beta * pv[u] + zv[u];
+      }
+    }
+    __syncthreads();
+  }
+  return maxit;
+}
+
+// r64 = b - A64 x64; returns <r64, r64>
+template <int W>
+__device__ inline double true_res(const SysDesc& d, const int* ecol, const double* a64, const double* adia64,
+                                  const double* b, const double* x, double* r64, double (*sh)[16], int& buf) {
+  double part = 0.0;
+  for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+    int c[W];
+    ell_cols<W>(ecol, d, i, c);
+    double v[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) v[k] = c[k] >= 0 ? a64[d.ell0 + (long long)k * d.n + i] * x[c[k]] : 0.0;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < W; k++) acc += v[k];
+    acc += adia64[d.row0 + i] * x[i];
+    const double rv = b[i] - acc;
+    if (r64) r64[i] = rv;
+    part += rv * rv;
+  }
+  return wg_sum(part, sh, buf);
+}
+
+// FP32 PCG with FP64 reliable updates: the FP32 recurrence runs on the FP32
+// matrix and factor; when its residual has dropped by delta^2 since the last
+// update (or reached the tolerance) the partial solution is added to the FP64
+// solution, the FP64 true residual replaces the recursive one, and the
+// iteration continues (restarted from the true residual when the recursion
+// had run more than 10x ahead of it).  Converged on the FP64 true residual.
+template <int W>
+__device__ inline void wg_pcg_mixed(const SysDesc& d, const int* cp, const int* ecol, const BatchArgs& g, const double* b,
+                                    double* x64, double* r64, float* r, float* xr, float* z, float* p, float* q,
+                                    double (*sh)[16], int& buf, int& iters, int& nupd, int& status, double& relres) {
+  const float* a32 = g.a32;
+  const float* adia32 = g.adia32;
+  const float* f32 = g.f32;
+  const float* fdia32 = g.fdia32;
+  double bb = 0.0;
+  for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+    x64[i] = 0.0;
+    xr[i] = 0.0f;
+    r[i] = (float)b[i];
+    bb += b[i] * b[i];
+  }
+  bb = wg_sum(bb, sh, buf);
+  iters = 0;
+  nupd = 0;
+  if (bb < 1.0e-30) {
+    status = 1;
+    relres = 0.0;
+    return;
+  }
+  const double delta2 = 0.01;
+  double rz = wg_sum(ell_precond<W, float, float>(d, cp, ecol, f32, fdia32, r, z), sh, buf);
+  for (int i = threadIdx.x; i < d.n; i += blockDim.x) p[i] = z[i];
+  __syncthreads();
+  double rr_ref = bb, rr_true = bb;
+  bool done = false;
+  int it = 0;
+  for (; it < g.maxit && !done; it++) {
+    const double pap = wg_sum(ell_spmv<W, float, float>(d, ecol, a32, adia32, p, q), sh, buf);
+    const float alpha = (float)(rz / pap), malpha = -alpha;
+    double rr = 0.0;
+    constexpr int U = rows_per_pass<W>();
+    for (int i0 = threadIdx.x; i0 < d.n; i0 += U * (int)blockDim.x) {
+      float xv[U], pv[U], rv[U], qv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i < d.n) {
+          xv[u] = xr[i];
+          pv[u] = p[i];
+          rv[u] = r[i];
+          qv[u] = q[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i >= d.n) continue;
+        xr[i] = xv[u] + alpha * pv[u];
+        const float rn = rv[u] + malpha * qv[u];
+        r[i] = rn;
+        rr += (double)rn * (double)rn;
+      }
+    }
+    rr = wg_sum(rr, sh, buf);
+    bool restart = false;
+    if (rr < delta2 * rr_ref || rr < g.tol2 * bb) {
+      for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+        x64[i] += (double)xr[i];
+        xr[i] = 0.0f;
+      }
+      __syncthreads();
+      rr_true = true_res<W>(d, ecol, g.a64, g.adia64, b, x64, r64, sh, buf);
+      nupd++;
+      if (rr_true < g.tol2 * bb) {
+        done = true;
+        continue;
+      }
+      for (int i = threadIdx.x; i < d.n; i += blockDim.x) r[i] = (float)r64[i];
+      __syncthreads();
+      restart = rr_true > 100.0 * rr;
+      rr_ref = rr_true;
+    }
+    const double rz_new = wg_sum(ell_precond<W, float, float>(d, cp, ecol, f32, fdia32, r, z), sh, buf);
+    const float beta = restart ? 0.0f : (float)(rz_new / rz);
+    rz = rz_new;
+    for (int i0 = threadIdx.x; i0 < d.n; i0 += U * (int)blockDim.x) {
+      float pv[U], zv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i < d.n) {
+          pv[u] = p[i];
+          zv[u] = z[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i < d.n) p[i] = beta * pv[u] + zv[u];
+      }
+    }
+    __syncthreads();
+  }
+  iters = it;
+  if (!done) {
+    for (int i = threadIdx.x; i < d.n; i += blockDim.x) x64[i] += (double)xr[i];
+    __syncthreads();
+    rr_true = true_res<W>(d, ecol, g.a64, g.adia64, b, x64, r64, sh, buf);
+  }
+  status = rr_true < g.tol2 * bb ? 1 : 0;
+  relres = sqrt(rr_true / bb);
+}
+
+// ILU(0) numeric of one system in place (b = 1 branch of DoILUDecomp: for
+// each k of L(i) in ascending order, with row k finished, a_ij -= a_ik u_kj
+// over U(k); pivot inverted; U row scaled by it).  Row i and row k are held
+// in registers, the fill positions matched by column.  Returns the number
+// of singular pivots.
+template <int W>
+__device__ inline int wg_ilu0(const SysDesc& d, const int* cp, const int* __restrict__ ecol,
+                              const double* __restrict__ a64, double* f64, const double* __restrict__ adia64,
+                              double* fdia64, double (*sh)[16], int& buf) {
+  double nsing = 0.0;
+  for (int c = 0; c < d.ncol; c++) {
+    for (int i = cp[c] + threadIdx.x; i < cp[c + 1]; i += blockDim.x) {
+      int ci[W];
+      double fi[W];
+      ell_cols<W>(ecol, d, i, ci);
+#pragma unroll
+      for (int k = 0; k < W; k++) fi[k] = ci[k] >= 0 ? a64[d.ell0 + (long long)k * d.n + i] : 0.0;
+      double dii = adia64[d.row0 + i];
+#pragma unroll
+      for (int qk = 0; qk < W; qk++) {
+        const int k = ci[qk];
+        if (k < 0 || k > i) continue;
+        const double L = fi[qk];
+        int ck[W];
+        double uk[W];
+        ell_cols<W>(ecol, d, k, ck);
+#pragma unroll
+        for (int qj = 0; qj < W; qj++) uk[qj] = ck[qj] > k ? f64[d.ell0 + (long long)qj * d.n + k] : 0.0;
+#pragma unroll
+        for (int qj = 0; qj < W; qj++) {
+          const int j = ck[qj];
+          if (j <= k) continue;
+          if (j == i) {
+            dii -= L * uk[qj];
+            continue;
+          }
+#pragma unroll
+          for (int pp = 0; pp < W; pp++)
+            if (pp > qk && ci[pp] == j) fi[pp] -= L * uk[qj];
+        }
+      }
+      if (fabs(dii) > 1.0e-30) dii = 1.0 / dii;
+      else nsing += 1.0;
+      fdia64[d.row0 + i] = dii;
+#pragma unroll
+      for (int k = 0; k < W; k++)
+        if (ci[k] >= 0) f64[d.ell0 + (long long)k * d.n + i] = ci[k] > i ? fi[k] * dii : fi[k];
+    }
+    __syncthreads();
+  }
+  return (int)wg_sum(nsing, sh, buf);
+}
+
+template <int W, bool MIXED>
+__global__ void __launch_bounds__(1024) k_batch_solve(BatchArgs g) {
+  __shared__ double sh[2][16];
+  __shared__ int s_sys;
+  __shared__ unsigned long long s_t0;
+  int buf = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const unsigned t = atomicAdd(g.next, 1u);
+      s_sys = t < (unsigned)g.nsys ? g.sched[t] : -1;
+      s_t0 = wall_clock64();
+    }
+    __syncthreads();
+    const int s = s_sys;
+    const unsigned long long t0 = s_t0;
+    __syncthreads();
+    if (s < 0) break;
+    const SysDesc d = g.sys[s];
+    const int* cp = g.cptr + (size_t)s * (kMaxColourBatch + 1);
+    const int nsing = wg_ilu0<W>(d, cp, g.ecol, g.a64, g.f64, g.adia64, g.fdia64, sh, buf);
+    int iters = 0, outer = 0, status = 0;
+    double relres = 0.0;
+    const double* b = g.b64 + d.row0;
+    double* x = g.x64 + d.row0;
+    if (nsing > 10) {
+      status = -1;
+    } else if (!MIXED) {
+      double* vb = reinterpret_cast<double*>(g.vec);
+      double *r = vb + d.row0, *z = vb + g.nrow + d.row0, *p = vb + 2 * g.nrow + d.row0,
+             *q = vb + 3 * g.nrow + d.row0;
+      for (int i = threadIdx.x; i < d.n; i += blockDim.x) r[i] = b[i];
+      __syncthreads();
+      double rel2 = 0.0;
+      iters = wg_pcg64<W>(d, cp, g.ecol, g.a64, g.adia64, g.f64, g.fdia64, r, x, z, p, q, g.tol2, g.maxit, sh, buf,
+                          rel2);
+      status = rel2 < g.tol2 ? 1 : 0;
+      double bb = 0.0;
+      for (int i = threadIdx.x; i < d.n; i += blockDim.x) bb += b[i] * b[i];
+      bb = wg_sum(bb, sh, buf);
+      const double rt = true_res<W>(d, g.ecol, g.a64, g.adia64, b, x, nullptr, sh, buf);
+      relres = bb > 0.0 ? sqrt(rt / bb) : 0.0;
+    } else {
+      // FP32 copies of this system's matrix and factor, then the refined solve
+      for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+        for (int k = 0; k < d.w; k++) {
+          const long long qq = d.ell0 + (long long)k * d.n + i;
+          g.a32[qq] = (float)g.a64[qq];
+          g.f32[qq] = (float)g.f64[qq];
+        }
+        g.adia32[d.row0 + i] = (float)g.adia64[d.row0 + i];
+        g.fdia32[d.row0 + i] = (float)g.fdia64[d.row0 + i];
+      }
+      __syncthreads();
+      float* vf = reinterpret_cast<float*>(g.vec);
+      float *r = vf + d.row0, *z = vf + g.nrow + d.row0, *p = vf + 2 * g.nrow + d.row0, *q = vf + 3 * g.nrow + d.row0,
+            *xr = vf + 4 * g.nrow + d.row0;
+      wg_pcg_mixed<W>(d, cp, g.ecol, g, b, x, g.r64 + d.row0, r, xr, z, p, q, sh, buf, iters, outer, status, relres);
+    }
+    if (threadIdx.x == 0) {
+      BatchOut o;
+      o.iters = iters;
+      o.outer = outer;
+      o.status = status;
+      o.nsing = nsing;
+      o.relres = relres;
+      o.t0 = t0;
+      o.t1 = wall_clock64();
+      g.out[s] = o;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_batch_reset(unsigned* next) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *next = 0u;
+}
+__global__ void k_batch_gather(long long nslot, const long long* __restrict__ src, const double* __restrict__ val,
+                               double* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nslot) return;
+  const long long s = src[t];
+  out[t] = s >= 0 ? val[s] : 0.0;
+}
+__global__ void k_batch_permute(long n, const int* __restrict__ perm, const double* __restrict__ in,
+                                double* __restrict__ out) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) out[t] = in[perm[t]];
+}
+__global__ void k_batch_unpermute(long n, const int* __restrict__ perm, const double* __restrict__ in,
+                                  double* __restrict__ out) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) out[perm[t]] = in[t];
+}
+
+static int grid256(long long n) { return (int)std::max<long long>(1, (n + 255) / 256); }
+
+// ---------------------------------------------------------------------------
+// Host side
+static int batch_args(dfm_batch_s* B, bool mixed, double tol, unsigned maxit, BatchArgs& g) {
+  g.nsys = (int)B->nsys;
+  g.sys = B->d_sys.p;
+  g.cptr = B->d_cptr.p;
+  g.sched = B->d_sched.p;
+  g.next = B->d_next.p;
+  g.ecol = B->d_ell_col.p;
+  g.a64 = B->d_a64.p;
+  g.f64 = B->d_f64.p;
+  g.adia64 = B->d_adia64.p;
+  g.fdia64 = B->d_fdia64.p;
+  g.a32 = B->d_a32.p;
+  g.f32 = B->d_f32.p;
+  g.adia32 = B->d_adia32.p;
+  g.fdia32 = B->d_fdia32.p;
+  g.b64 = B->d_b64.p;
+  g.x64 = B->d_x64.p;
+  g.r64 = B->d_r64.p;
+  g.nrow = (long)B->nrow;
+  const size_t vbytes = (size_t)B->nrow * 5 * (mixed ? sizeof(float) : sizeof(double));
+  if (B->d_vec.n < vbytes) DFM_TRY(B->d_vec.alloc(vbytes));
+  g.vec = B->d_vec.p;
+  g.tol2 = tol * tol;
+  g.maxit = (int)std::min<unsigned>(maxit, 0x7fffffffu);
+  g.out = B->d_out.p;
+  return DFM_OK;
+}
+
+static int ncu_count() {
+  static int ncu = 0;
+  if (ncu == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+      ncu = 1;
+  }
+  return ncu;
+}
+
+// The solve step (work counter reset + persistent launch), replayed from a
+// graph per (precision, launch shape, tolerance, iteration limit).
+static int batch_launch(dfm_batch_s* B, bool mixed, double tol, unsigned maxit) {
+  BatchArgs g;
+  DFM_TRY(batch_args(B, mixed, tol, maxit, g));
+  const int grid = std::min<int>((int)B->nsys, ncu_count() * B->wpc);
+  char key[160];
+  snprintf(key, sizeof key, "%d:%d:%d:%.17g:%u:%p", (int)mixed, B->block, B->wpc, tol, maxit, g.vec);
+  hipStream_t s = stream();
+  for (auto& e : B->graphs)
+    if (e.first == key) {
+      DFM_HIP(hipGraphLaunch(e.second, s));
+      return DFM_OK;
+    }
+  auto body = [&]() -> int {
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, B->d_next.p);
+    DFM_HIP(hipGetLastError());
+    if (B->max_w <= 6) {
+      if (mixed) hipLaunchKernelGGL((k_batch_solve<6, true>), dim3(grid), dim3(B->block), 0, s, g);
+      else hipLaunchKernelGGL((k_batch_solve<6, false>), dim3(grid), dim3(B->block), 0, s, g);
+    } else if (B->max_w <= 8) {
+      if (mixed) hipLaunchKernelGGL((k_batch_solve<8, true>), dim3(grid), dim3(B->block), 0, s, g);
+      else hipLaunchKernelGGL((k_batch_solve<8, false>), dim3(grid), dim3(B->block), 0, s, g);
+    } else {
+      if (mixed) hipLaunchKernelGGL((k_batch_solve<16, true>), dim3(grid), dim3(B->block), 0, s, g);
+      else hipLaunchKernelGGL((k_batch_solve<16, false>), dim3(grid), dim3(B->block), 0, s, g);
+    }
+    DFM_HIP(hipGetLastError());
+    return DFM_OK;
+  };
+  if (s == nullptr) return body();
+  hipGraph_t graph = nullptr;
+  DFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = body();
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc != DFM_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return body();
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  DFM_HIP(e);
+  B->graphs.emplace_back(key, exec);
+  DFM_HIP(hipGraphLaunch(exec, s));
+  return DFM_OK;
+}
+
+}  // namespace dfm
+
+using namespace dfm;
+
+extern "C" {
+
+int dfm_batch_create(const dfm_bcrs* A, unsigned nsys, const unsigned* sys_ptr, dfm_batch** out) {
+  DFM_CHECK(A && sys_ptr && out && nsys > 0, DFM_BAD_ARG, "bad arguments to dfm_batch_create");
+  DFM_CHECK(A->b == 1, DFM_NOT_IMPLEMENTED, "the batched solver takes 1x1 blocks (b = %u)", A->b);
+  DFM_CHECK(sys_ptr[0] == 0 && sys_ptr[nsys] == A->nb, DFM_BAD_ARG,
+            "sys_ptr must start at 0 and end at the block count %u", A->nb);
+  for (unsigned s = 0; s < nsys; s++)
+    DFM_CHECK(sys_ptr[s + 1] > sys_ptr[s], DFM_BAD_ARG, "system %u is empty", s);
+  DFM_TRY(ensure_init());
+  std::unique_ptr<dfm_batch_s> B(new dfm_batch_s);
+  B->A = A;
+  B->nsys = nsys;
+  B->nrow = A->nb;
+  B->sys_ptr.assign(sys_ptr, sys_ptr + nsys + 1);
+  B->perm.resize(A->nb);
+  B->desc.resize(nsys);
+  B->cptr.assign((size_t)nsys * (kMaxColourBatch + 1), 0);
+  long long ell_total = 0;
+  // first pass: colouring, widths
+  std::vector<std::vector<int>> orders(nsys);
+  for (unsigned s = 0; s < nsys; s++) {
+    const unsigned r0 = sys_ptr[s], n = sys_ptr[s + 1] - r0;
+    std::vector<unsigned> lrow(n + 1, 0), lcol;
+    lcol.reserve(A->rowptr[r0 + n] - A->rowptr[r0]);
+    int w = 0;
+    for (unsigned i = 0; i < n; i++) {
+      for (unsigned q = A->rowptr[r0 + i]; q < A->rowptr[r0 + i + 1]; q++) {
+        const unsigned c = A->col[q];
+        DFM_CHECK(c >= r0 && c < r0 + n, DFM_BAD_ARG,
+                  "row %u couples system %u to row %u outside it: the systems must be diagonal blocks", r0 + i, s, c);
+        lcol.push_back(c - r0);
+      }
+      lrow[i + 1] = (unsigned)lcol.size();
+      w = std::max(w, (int)(lrow[i + 1] - lrow[i]));
+    }
+    std::vector<int> colour;
+    const int nc = greedy_colour(n, lrow, lcol, colour);
+    DFM_CHECK(nc <= kMaxColourBatch, DFM_NOT_IMPLEMENTED, "system %u needs %d colours (at most %d)", s, nc,
+              kMaxColourBatch);
+    colour_order(n, colour, nc, orders[s]);
+    int* cp = B->cptr.data() + (size_t)s * (kMaxColourBatch + 1);
+    for (unsigned i = 0; i < n; i++) cp[colour[i] + 1]++;
+    for (int c = 0; c < nc; c++) cp[c + 1] += cp[c];
+    for (int c = nc; c < kMaxColourBatch; c++) cp[c + 1] = cp[c];
+    DFM_CHECK(w <= 16, DFM_NOT_IMPLEMENTED, "system %u has a row with %d off-diagonal entries (at most 16)", s, w);
+    B->desc[s] = SysDesc{(int)r0, (int)n, w, nc, ell_total};
+    ell_total += (long long)w * n;
+    B->max_w = std::max(B->max_w, w);
+    B->max_ncol = std::max(B->max_ncol, nc);
+  }
+  // second pass: ELL columns (new numbering, ascending) and sources in A
+  std::vector<int> ecol((size_t)std::max<long long>(1, ell_total), -1);
+  std::vector<long long> esrc((size_t)std::max<long long>(1, ell_total), -1);
+  for (unsigned s = 0; s < nsys; s++) {
+    const SysDesc& d = B->desc[s];
+    const std::vector<int>& ord = orders[s];
+    std::vector<int> inv(d.n);
+    for (int i = 0; i < d.n; i++) inv[ord[i]] = i;
+    std::vector<std::pair<int, long long>> ent;
+    for (int i = 0; i < d.n; i++) {
+      const unsigned orow = (unsigned)d.row0 + (unsigned)ord[i];
+      B->perm[d.row0 + i] = (int)orow;
+      ent.clear();
+      for (unsigned q = A->rowptr[orow]; q < A->rowptr[orow + 1]; q++)
+        ent.emplace_back(inv[A->col[q] - d.row0], (long long)q);
+      std::sort(ent.begin(), ent.end());
+      for (size_t k = 0; k < ent.size(); k++) {
+        const size_t slot = (size_t)d.ell0 + k * (size_t)d.n + i;
+        ecol[slot] = ent[k].first;
+        esrc[slot] = ent[k].second;
+      }
+    }
+  }
+  // schedule: most work (rows * width) first
+  std::vector<int> sched(nsys);
+  std::iota(sched.begin(), sched.end(), 0);
+  std::stable_sort(sched.begin(), sched.end(), [&](int a, int b) {
+    return (long long)B->desc[a].n * (B->desc[a].w + 1) > (long long)B->desc[b].n * (B->desc[b].w + 1);
+  });
+  const size_t nslot = (size_t)std::max<long long>(1, ell_total), nr = A->nb;
+  DFM_TRY(B->d_sys.upload(B->desc));
+  DFM_TRY(B->d_cptr.upload(B->cptr));
+  DFM_TRY(B->d_sched.upload(sched));
+  DFM_TRY(B->d_perm.upload(B->perm));
+  DFM_TRY(B->d_ell_col.upload(ecol));
+  DFM_TRY(B->d_ell_src.upload(esrc));
+  DFM_TRY(B->d_next.alloc(1));
+  DFM_TRY(B->d_a64.alloc(nslot));
+  DFM_TRY(B->d_f64.alloc(nslot));
+  DFM_TRY(B->d_a32.alloc(nslot));
+  DFM_TRY(B->d_f32.alloc(nslot));
+  DFM_TRY(B->d_adia64.alloc(nr));
+  DFM_TRY(B->d_fdia64.alloc(nr));
+  DFM_TRY(B->d_adia32.alloc(nr));
+  DFM_TRY(B->d_fdia32.alloc(nr));
+  DFM_TRY(B->d_b64.alloc(nr));
+  DFM_TRY(B->d_x64.alloc(nr));
+  DFM_TRY(B->d_r64.alloc(nr));
+  DFM_TRY(B->d_out.alloc(nsys));
+  DFM_HIP(hipStreamSynchronize(stream()));
+  *out = B.release();
+  return DFM_OK;
+}
+
+int dfm_batch_destroy(dfm_batch* B) {
+  delete B;
+  return DFM_OK;
+}
+
+int dfm_batch_info(const dfm_batch* B, unsigned* max_width, int* max_colours, int* block, int* wg_per_cu) {
+  DFM_CHECK(B, DFM_BAD_ARG, "NULL batch");
+  if (max_width) *max_width = (unsigned)B->max_w;
+  if (max_colours) *max_colours = B->max_ncol;
+  if (block) *block = B->block;
+  if (wg_per_cu) *wg_per_cu = B->wpc;
+  return DFM_OK;
+}
+
+int dfm_batch_set_value(dfm_batch* B, const dfm_bcrs* A) {
+  DFM_CHECK(B && A && A == B->A, DFM_BAD_ARG, "dfm_batch_set_value needs the matrix the batch was made from");
+  const long long nslot = (long long)B->d_a64.n;
+  hipLaunchKernelGGL(k_batch_gather, dim3(grid256(nslot)), dim3(256), 0, stream(), nslot, B->d_ell_src.p,
+                     A->d_val.p, B->d_a64.p);
+  DFM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_batch_permute, dim3(grid256(B->nrow)), dim3(256), 0, stream(), (long)B->nrow, B->d_perm.p,
+                     A->d_dia.p, B->d_adia64.p);
+  DFM_HIP(hipGetLastError());
+  DFM_HIP(hipStreamSynchronize(stream()));
+  return DFM_OK;
+}
+
+int dfm_batch_set_launch(dfm_batch* B, int block, int wg_per_cu) {
+  DFM_CHECK(B && (block == 256 || block == 512 || block == 1024) && wg_per_cu >= 1 && wg_per_cu <= 8 &&
+                block * wg_per_cu <= 2048,
+            DFM_BAD_ARG, "launch shape: block 256/512/1024, 1..8 workgroups per CU, at most 2048 threads per CU");
+  B->block = block;
+  B->wpc = wg_per_cu;
+  return DFM_OK;
+}
+
+int dfm_batch_solve_pcg(dfm_batch* B, const double* rhs, double* x, double tol, unsigned maxit, int precision,
+                        unsigned* iters, int* outer, int* status, double* relres, double* ms) {
+  DFM_CHECK(B && rhs && x && tol > 0.0 && maxit > 0, DFM_BAD_ARG, "bad arguments to dfm_batch_solve_pcg");
+  DFM_CHECK(precision == DFM_PREC_FP64 || precision == DFM_PREC_MIXED_F32, DFM_BAD_ARG,
+            "the batched solver runs DFM_PREC_FP64 or DFM_PREC_MIXED_F32 (FP32 with FP64 refinement)");
+  const bool mixed = precision == DFM_PREC_MIXED_F32;
+  hipLaunchKernelGGL(k_batch_permute, dim3(grid256(B->nrow)), dim3(256), 0, stream(), (long)B->nrow, B->d_perm.p,
+                     rhs, B->d_b64.p);
+  DFM_HIP(hipGetLastError());
+  DFM_TRY(batch_launch(B, mixed, tol, maxit));
+  hipLaunchKernelGGL(k_batch_unpermute, dim3(grid256(B->nrow)), dim3(256), 0, stream(), (long)B->nrow,
+                     B->d_perm.p, B->d_x64.p, x);
+  DFM_HIP(hipGetLastError());
+  std::vector<BatchOut> o(B->nsys);
+  DFM_TRY(B->d_out.download(o.data(), B->nsys));
+  int dev = 0, khz = 0;
+  DFM_HIP(hipGetDevice(&dev));
+  DFM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  unsigned long long tmin = ~0ull;
+  for (const BatchOut& e : o) tmin = std::min(tmin, e.t0);
+  bool all = true;
+  for (unsigned s = 0; s < B->nsys; s++) {
+    if (iters) iters[s] = (unsigned)o[s].iters;
+    if (outer) outer[s] = o[s].outer;
+    if (status) status[s] = o[s].status;
+    if (relres) relres[s] = o[s].relres;
+    if (ms) {
+      ms[2 * s] = khz > 0 ? (double)(o[s].t1 - o[s].t0) / khz : 0.0;     // solve time on its workgroup
+      ms[2 * s + 1] = khz > 0 ? (double)(o[s].t1 - tmin) / khz : 0.0;   // completion since the launch began
+    }
+    all = all && o[s].status == 1;
+  }
+  return all ? DFM_OK : DFM_NOT_CONVERGED;
+}
+
+int dfm_batch_get_factor(const dfm_batch* B, unsigned sys, double* val, double* dia, int* order) {
+  DFM_CHECK(B && sys < B->nsys, DFM_BAD_ARG, "bad arguments to dfm_batch_get_factor");
+  const SysDesc& d = B->desc[sys];
+  std::vector<double> f((size_t)d.w * d.n), fd(d.n);
+  std::vector<int> c((size_t)d.w * d.n);
+  if (!f.empty()) {
+    DFM_HIP(hipMemcpy(f.data(), B->d_f64.p + d.ell0, f.size() * sizeof(double), hipMemcpyDeviceToHost));
+    DFM_HIP(hipMemcpy(c.data(), B->d_ell_col.p + d.ell0, c.size() * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  DFM_HIP(hipMemcpy(fd.data(), B->d_fdia64.p + d.row0, fd.size() * sizeof(double), hipMemcpyDeviceToHost));
+  size_t t = 0;
+  for (int i = 0; i < d.n; i++)
+    for (int k = 0; k < d.w; k++) {
+      const size_t slot = (size_t)k * d.n + i;
+      if (c[slot] < 0) break;
+      if (val) val[t] = f[slot];
+      t++;
+    }
+  if (dia) std::copy(fd.begin(), fd.end(), dia);
+  if (order)
+    for (int i = 0; i < d.n; i++) order[i] = B->perm[d.row0 + i] - d.row0;
+  return DFM_OK;
+}
+
+}  // extern "C"

@@ -344,3 +344,88 @@ class FEMSystem:
         _sync_torch()
         call("dfm_vec_zero_bc", self.res.numel(), _ptr(self.flags), _dptr(self.res))
         return dot(self.res, self.res) ** 0.5
+
+
+class BatchSystem:
+    """Many independent scalar systems (the diagonal blocks of one assembled
+    FEMSystem) solved together: dfm_batch_* -- per system the reference's
+    ordered ILU(0) and Solve_PCG, all systems in one persistent launch."""
+
+    SHAPES = ((256, 1), (256, 2), (256, 4), (512, 1), (512, 2), (512, 4), (1024, 1), (1024, 2))
+
+    def __init__(self, fem, sys_ptr):
+        self.fem = fem
+        self.sys_ptr = np.ascontiguousarray(sys_ptr, dtype=np.uint32)
+        self.nsys = self.sys_ptr.size - 1
+        h = C.c_void_p()
+        call("dfm_batch_create", fem.A.h, self.nsys, _ptr(self.sys_ptr), C.byref(h))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_batch_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def info(self):
+        w, nc, bl, wp = C.c_uint(), C.c_int(), C.c_int(), C.c_int()
+        call("dfm_batch_info", self.h, C.byref(w), C.byref(nc), C.byref(bl), C.byref(wp))
+        return dict(max_width=w.value, max_colours=nc.value, block=bl.value, wg_per_cu=wp.value)
+
+    def SetValue(self):
+        _sync_torch()
+        call("dfm_batch_set_value", self.h, self.fem.A.h)
+
+    def set_launch(self, block, wg_per_cu):
+        call("dfm_batch_set_launch", self.h, int(block), int(wg_per_cu))
+
+    def solve(self, rhs, x, tol, maxit, precision="mixed_f32"):
+        """Returns (all_converged, dict of per-system arrays)."""
+        _sync_torch()
+        n = self.nsys
+        it = np.zeros(n, dtype=np.uint32)
+        outer = np.zeros(n, dtype=np.int32)
+        st = np.zeros(n, dtype=np.int32)
+        rel = np.zeros(n)
+        ms = np.zeros(2 * n)
+        rc = call("dfm_batch_solve_pcg", self.h, _dptr(rhs), _dptr(x), float(tol), int(maxit),
+                  _lib.PRECISIONS[precision], _ptr(it), _ptr(outer), _ptr(st), _ptr(rel), _ptr(ms),
+                  ok=(_lib.DFM_OK, _lib.DFM_NOT_CONVERGED))
+        return rc == _lib.DFM_OK, dict(iters=it, outer=outer, status=st, relres=rel, solve_ms=ms[0::2],
+                                       done_ms=ms[1::2])
+
+    def factor(self, s):
+        """(val, dia, order) of system s after a solve (colour numbering)."""
+        n = int(self.sys_ptr[s + 1] - self.sys_ptr[s])
+        rp, _ = self.fem.A.pattern()
+        nnz = int(rp[self.sys_ptr[s + 1]] - rp[self.sys_ptr[s]])
+        val = np.zeros(max(nnz, 1))
+        dia = np.zeros(n)
+        order = np.zeros(n, dtype=np.int32)
+        call("dfm_batch_get_factor", self.h, int(s), _ptr(val), _ptr(dia), _ptr(order))
+        return val[:nnz], dia, order
+
+    def autotune(self, rhs, x, tol, maxit, precision="mixed_f32", reps=2):
+        """Time each launch shape on this batch (graph replay, HIP events on
+        the library stream) and keep the fastest.  Returns {shape: ms}."""
+        torch = _torch()
+        stream = torch.cuda.ExternalStream(int(_lib.lib().dfm_stream()))
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        times = {}
+        for bl, wp in self.SHAPES:
+            self.set_launch(bl, wp)
+            self.solve(rhs, x, tol, maxit, precision)
+            best = float("inf")
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                ev[0].record(stream)
+                self.solve(rhs, x, tol, maxit, precision)
+                ev[1].record(stream)
+                ev[1].synchronize()
+                best = min(best, ev[0].elapsed_time(ev[1]))
+            times[(bl, wp)] = best
+        bl, wp = min(times, key=times.get)
+        self.set_launch(bl, wp)
+        return times

@@ -208,3 +208,24 @@ def helmholtz_batch(count=512, seed=12345, nmin=500, nmax=50000, sigma=0.5, cmax
         off += p.nelem
         out.append(p)
     return out
+
+
+def concat(problems):
+    """Disjoint union of scalar Helmholtz problems as one problem whose
+    matrix has the systems as diagonal blocks; sys_ptr[s] is the first node
+    of system s."""
+    kinds = {p.kind for p in problems}
+    assert kinds == {"helmholtz2d"}, kinds
+    sys_ptr = np.zeros(len(problems) + 1, dtype=np.uint32)
+    for s, p in enumerate(problems):
+        sys_ptr[s + 1] = sys_ptr[s] + p.nnode
+    conn = np.concatenate([p.conn + np.uint32(sys_ptr[s]) for s, p in enumerate(problems)]).astype(np.uint32)
+    nnode = int(sys_ptr[-1])
+    return Problem(kind="helmholtz2d", name=f"batch_{len(problems)}", b=1, npe=3,
+                   coord=np.ascontiguousarray(np.concatenate([p.coord for p in problems])),
+                   conn=np.ascontiguousarray(conn), nnode=nnode, nelem=conn.shape[0],
+                   alpha=np.ascontiguousarray(np.concatenate([p.alpha for p in problems])),
+                   cmass=np.ascontiguousarray(np.concatenate([p.cmass for p in problems])),
+                   source=problems[0].source,
+                   flags=np.ascontiguousarray(np.concatenate([p.flags for p in problems]).astype(np.int32)),
+                   sys_ptr=sys_ptr, parts=problems)

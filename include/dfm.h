@@ -155,6 +155,40 @@ int dfm_ilu_backward(dfm_ilu *M, double *vec_dev);
  * original numbering (permuted in and out when the factor is ordered). */
 int dfm_ilu_solve_precond(dfm_ilu *M, double *vec_dev);
 
+/* ---- Batched solves: many independent scalar systems, one persistent launch */
+/* No single reference counterpart: each system gets the reference's ordered
+ * CPreconditioner_ILU(0) (preconditioner.cpp:57-297, DoILUDecomp b = 1,
+ * matdiafrac_blkcrs.cpp:769-1102) and Solve_PCG (solver_ls_iter.cpp:100-172).
+ * A is one assembled 1x1-block matrix whose diagonal blocks, rows
+ * [sys_ptr[s], sys_ptr[s+1]) (host array, nsys+1 entries), are the systems;
+ * any coupling between systems is rejected.  Each system is ordered by a
+ * greedy colouring of its graph. */
+int dfm_batch_create(const dfm_bcrs *A, unsigned nsys, const unsigned *sys_ptr, dfm_batch **out);
+int dfm_batch_destroy(dfm_batch *B);
+int dfm_batch_info(const dfm_batch *B, unsigned *max_width, int *max_colours, int *block, int *wg_per_cu);
+/* Take A's current values (after assembly and boundary conditions). */
+int dfm_batch_set_value(dfm_batch *B, const dfm_bcrs *A);
+/* Launch shape of the persistent kernel: threads per workgroup (256, 512,
+ * 1024) and workgroups per compute unit (block * wg_per_cu <= 2048). */
+int dfm_batch_set_launch(dfm_batch *B, int block, int wg_per_cu);
+/* One step for every system: ILU(0) numeric, then PCG from x = 0 on rhs
+ * (concatenated, A's numbering) to ||r|| < tol ||rhs||.  precision:
+ * DFM_PREC_FP64 (the reference's PCG) or DFM_PREC_MIXED_F32 (FP32 matrix,
+ * factors and recurrence; FP64 solution, FP64 true residual replacing the
+ * recursive one at every 10x residual drop, convergence on the FP64 true
+ * residual).  Per system (host arrays, any may be NULL): iterations (FP64:
+ * the reference's 0-based index at convergence; FP32: iterations run),
+ * FP64 reliable updates, status
+ * (1 converged, 0 not, -1 more than 10 singular pivots), FP64 relative
+ * residual, and ms[2s] the solve time on its workgroup, ms[2s+1] the time
+ * from the launch to its completion.  DFM_NOT_CONVERGED unless all converged. */
+int dfm_batch_solve_pcg(dfm_batch *B, const double *rhs_dev, double *x_dev, double tol, unsigned maxit,
+                        int precision, unsigned *iters, int *outer, int *status, double *relres, double *ms);
+/* Factor of system sys after a solve, in its colour numbering: val in CRS
+ * order (rows ascending, columns ascending; L unscaled, U scaled by D^-1),
+ * dia = D^-1, order[new] = old (local rows). */
+int dfm_batch_get_factor(const dfm_batch *B, unsigned sys, double *val, double *dia, int *order);
+
 /* ---- Krylov (LsSol::Solve_PCG / Solve_PBiCGSTAB, solver_ls_iter.cpp:100-172, 291-408) */
 /* Precision of a solve: the type of the Krylov vectors / recurrence, and the
  * storage of the matrix copy and of the ILU factors streamed each iteration.
