@@ -83,6 +83,7 @@ _SIGS = {
     "dfm_vec_axpy": (C.c_int, [C.c_size_t, C.c_double, vp, vp]),
     "dfm_vec_scal": (C.c_int, [C.c_size_t, C.c_double, vp]),
     "dfm_vec_copy": (C.c_int, [C.c_size_t, vp, vp]),
+    "dfm_vec_fill": (C.c_int, [C.c_size_t, C.c_double, vp]),
     "dfm_vec_zero_bc": (C.c_int, [C.c_size_t, vp, vp]),
     "dfm_asm_create": (C.c_int, [H, C.c_uint, C.c_uint, C.c_uint, vp, C.POINTER(H)]),
     "dfm_asm_destroy": (C.c_int, [H]),

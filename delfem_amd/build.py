@@ -1,4 +1,5 @@
-"""Builds libdelfem_amd.so (HIP, gfx950) in-tree with hipcc."""
+"""Builds libdelfem_amd.so (HIP, gfx950) in-tree with hipcc, and the C++
+program that drives the drop-in classes of include/delfem_amd/ (tests/cpp/)."""
 import os
 import subprocess
 import sys
@@ -31,6 +32,24 @@ def build(verbose=False):
     return LIB
 
 
+SHIM_SRC = os.path.join(ROOT, "tests", "cpp", "shim_test.cpp")
+SHIM_BIN = os.path.join(ROOT, "tests", "cpp", "shim_test")
+
+
+def build_shim_test(verbose=False):
+    """The C++ program of the drop-in test, linked against the library."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = SHIM_BIN + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), SHIM_SRC, "-o", tmp,
+           "-L", HERE, "-ldelfem_amd", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../delfem_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, SHIM_BIN)
+    return SHIM_BIN
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
+    build_shim_test(verbose="-v" in sys.argv)
     print(LIB)

@@ -390,6 +390,12 @@ int dfm_vec_scal(size_t n, double alpha, double* x) {
   DFM_HIP(hipStreamSynchronize(g_stream));
   return DFM_OK;
 }
+int dfm_vec_fill(size_t n, double value, double* x) {
+  DFM_TRY(ensure_init());
+  DFM_LAUNCH(k_fill<double>, grid_vec((long)n), (long)n, value, x);
+  DFM_HIP(hipStreamSynchronize(g_stream));
+  return DFM_OK;
+}
 int dfm_vec_copy(size_t n, const double* x, double* y) {
   DFM_TRY(ensure_init());
   if (x == y || n == 0) return DFM_OK;

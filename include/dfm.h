@@ -89,6 +89,7 @@ int dfm_vec_dot(size_t n, const double *x_dev, const double *y_dev, double *resu
 int dfm_vec_axpy(size_t n, double alpha, const double *x_dev, double *y_dev);        /* AXPY vector_blk.cpp:88-97 */
 int dfm_vec_scal(size_t n, double alpha, double *x_dev);                              /* SCAL / operator*= vector_blk.cpp:54-59 */
 int dfm_vec_copy(size_t n, const double *x_dev, double *y_dev);                       /* COPY / operator= vector_blk.cpp:62-71 */
+int dfm_vec_fill(size_t n, double value, double *x_dev);                             /* SetVectorZero (value 0) vector_blk.cpp:100-105 */
 int dfm_vec_zero_bc(size_t n, const int *flag, double *x_dev);                        /* CBCFlag::SetZeroToBCDof bcflag_blk.h:102-129 */
 
 /* ---- element assembly (femeqn AddLinSys_* loops + Mearge) ---------------- */

@@ -48,3 +48,15 @@ def test_device_calls_without_device_fail_cleanly():
     h = C.c_void_p()
     rc = L.dfm_bcrs_create(10, 3, C.byref(h))
     assert rc != _lib.DFM_OK and L.dfm_last_error()
+
+
+def test_cpp_dropin_headers_compile(tmp_path):
+    """include/delfem_amd/{matvec,ls}.hpp are valid C++17 on their own (the
+    GPU test runs the program built from tests/cpp/shim_test.cpp)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(root, "tests", "cpp", "shim_test.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(root, "include"),
+                        src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

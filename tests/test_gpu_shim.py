@@ -1,0 +1,67 @@
+"""The C++ drop-in classes (include/delfem_amd/matvec.hpp, ls.hpp) driven the
+way DelFEM's equation systems drive the reference classes, checked against
+the oracle: assembly by per-element Mearge and FinalizeMarge bit-exact,
+ILU(0)-PCG / ILU(0)-BiCGSTAB within one iteration and 1e-8 of the oracle's
+solution.  The program is tests/cpp/shim_test (built by build())."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from delfem_amd import meshgen
+from oracle.oracle import assemble_problem, problem_emats
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "shim_test")
+
+CASES = {
+    "poisson2d_10_pcg": (lambda: meshgen.poisson2d(10, seed=3), 0, 1e-10),
+    "solid3d_3_pcg": (lambda: meshgen.solid3d(3, seed=3, sigma=0.5), 0, 1e-10),
+    "oseen2d_8_bicg": (lambda: meshgen.oseen2d(8, seed=3), 1, 1e-10),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_cpp_dropin(orc, gpu, tmp_path, name):
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/shim_test is not built (run __graft_entry__.build())")
+    make, kind, tol = CASES[name]
+    prob = make()
+    print(f"[case] {name}", flush=True)
+    emat, eres = problem_emats(orc, prob)
+    b = prob.b
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([prob.nnode, prob.nelem, b, prob.npe, kind, 2000], dtype=np.int32).tofile(f)
+        np.array([tol]).tofile(f)
+        np.ascontiguousarray(prob.conn, dtype=np.uint32).tofile(f)
+        np.ascontiguousarray(emat).tofile(f)
+        np.ascontiguousarray(eres).tofile(f)
+        np.ascontiguousarray(prob.flags, dtype=np.int32).tofile(f)
+    run = subprocess.run([BIN, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    print(run.stdout, run.stderr, flush=True)
+    assert run.returncode == 0
+    ref = assemble_problem(orc, prob)
+    ndof = prob.nnode * b
+    raw = open(out, "rb").read()
+    d2 = np.frombuffer(raw[:16], dtype=np.float64)
+    i2 = np.frombuffer(raw[16:24], dtype=np.int32)
+    rest = np.frombuffer(raw[24:], dtype=np.float64)
+    nv, nd = ref["val"].size, ref["dia"].size
+    val, dia = rest[:nv], rest[nv:nv + nd]
+    res, x = rest[nv + nd:nv + nd + ndof], rest[nv + nd + ndof:nv + nd + 2 * ndof]
+    rr = rest[nv + nd + 2 * ndof]
+    assert np.array_equal(val, ref["val"]) and np.array_equal(dia, ref["dia"])
+    assert np.array_equal(res, ref["res"])
+    assert d2[0] == pytest.approx(np.linalg.norm(ref["res"]), rel=1e-12)
+    F = orc.iluk_symbolic(ref["rowptr"], ref["col"], 0)
+    fv, fd, _ = orc.ilu_values(prob.nnode, b, ref["rowptr"], ref["col"], ref["val"], ref["dia"], F)
+    ok_o, conv_o, it_o, x_o, _ = orc.solve("pcg" if kind == 0 else "bicg", prob.nnode, b, ref["rowptr"], ref["col"],
+                                           ref["val"], ref["dia"], F, fv, fd, ref["res"], tol, 2000)
+    assert i2[0] == 1 and ok_o
+    assert abs(int(i2[1]) - it_o) <= 1, (i2[1], it_o)
+    assert np.linalg.norm(x - x_o) <= 1e-8 * np.linalg.norm(x_o)
+    assert np.sqrt(rr) <= 10 * tol * np.linalg.norm(ref["res"])
