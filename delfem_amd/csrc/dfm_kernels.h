@@ -10,12 +10,19 @@
 // workgroup sums the per-workgroup partials in index order).
 #pragma once
 
+#include <type_traits>
+
 #include "dfm_internal.h"
 
 namespace dfm {
 
 constexpr int kNT = 256;             // threads per workgroup
 constexpr int kWaves = kNT / 64;     // wavefronts (= slices) per workgroup
+// Triangular sweeps launch one small workgroup per slice: a colour of a
+// multicolour ordering holds only a few hundred slices, and one-wave
+// workgroups spread them over all CUs.
+constexpr int kTrsvWaves = 1;
+constexpr int kTrsvNT = 64 * kTrsvWaves;
 
 // ---------------------------------------------------------------------------
 // Deterministic two-phase reductions.  Every workgroup writes its partial sums
@@ -44,12 +51,12 @@ __device__ inline double block_sum_n(double v, double* sh) {
 }
 
 // partials[d*nparts + part] = workgroup sum of v[d]
-template <int ND>
+template <int ND, int NT = kNT>
 __device__ inline void write_partials(const double (&v)[ND], double* partials, int part, int nparts) {
-  __shared__ double sh[kWaves];
+  __shared__ double sh[NT / 64];
 #pragma unroll
   for (int d = 0; d < ND; d++) {
-    const double s = block_sum_n<kNT>(v[d], sh);
+    const double s = block_sum_n<NT>(v[d], sh);
     if (threadIdx.x == 0) partials[(size_t)d * nparts + part] = s;
   }
 }
@@ -149,6 +156,50 @@ __device__ inline bf16s ld_stream<bf16s>(const bf16s* p) {
   return r;
 }
 
+// Slot layout of a block in a sliced-ELL array.  float / double: entry e of
+// the block of lane l in slot k at (k*bb + e)*64 + l, one value per lane per
+// load.  bf16: entries paired into 32-bit words, word w = entries (2w, 2w+1)
+// at (k*W + w)*64 + l with W = (bb+1)/2, so each lane loads 4 bytes.
+template <typename TS>
+__host__ __device__ constexpr int slot_elems(int bb) {
+  return std::is_same<TS, bf16s>::value ? 2 * ((bb + 1) / 2) : bb;
+}
+template <typename TS>
+__device__ inline size_t slot_index(size_t k, int bb, int e, int lane) {
+  if constexpr (std::is_same<TS, bf16s>::value)
+    return ((k * (size_t)((bb + 1) / 2) + (size_t)(e >> 1)) * 64 + lane) * 2 + (e & 1);
+  else
+    return (k * bb + e) * 64 + lane;
+}
+// All BB values of one block slot, converted to TV.
+template <int BB, typename TS, typename TV>
+__device__ inline void load_block(const TS* __restrict__ val, size_t k, int lane, TV (&a)[BB]) {
+  if constexpr (std::is_same<TS, bf16s>::value) {
+    constexpr int W = (BB + 1) / 2;
+    const unsigned* wv = reinterpret_cast<const unsigned*>(val);
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      const unsigned u = __builtin_nontemporal_load(wv + (k * W + w) * 64 + lane);
+      a[2 * w] = (TV)__uint_as_float(u << 16);
+      if (2 * w + 1 < BB) a[2 * w + 1] = (TV)__uint_as_float(u & 0xFFFF0000u);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < BB; e++) a[e] = cvt<TV>(ld_stream(val + (k * BB + e) * 64 + lane));
+  }
+}
+
+// Workgroup index remapped so that the workgroups dispatched to one XCD
+// (dispatch is round-robin over the 8 XCDs) cover one contiguous range of
+// slices: neighbouring rows, which gather overlapping parts of x, then share
+// that XCD's L2 instead of being spread over all eight.
+__device__ inline int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  if (G < 16) return b;
+  const int x = b & 7, i = b >> 3, q = G >> 3, r = G & 7;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // One block slot of a sliced-ELL row: values a[] (storage TS converted to TV),
 // gathered x block xv[].  Padding slots (column < 0) gather zeros without
 // touching memory.
@@ -156,8 +207,7 @@ template <int B, typename TS, typename TV>
 __device__ inline void sell_slot(const int* __restrict__ col, const TS* __restrict__ val, const TV* x, long k,
                                  int lane, TV (&a)[B * B], TV (&xv)[B]) {
   const int c = ld_stream(col + (size_t)k * 64 + lane);
-#pragma unroll
-  for (int e = 0; e < B * B; e++) a[e] = cvt<TV>(ld_stream(val + ((size_t)k * B * B + e) * 64 + lane));
+  load_block<B * B, TS, TV>(val, (size_t)k, lane, a);
   const bool ok = c >= 0;
   const size_t cb = (size_t)(ok ? c : 0) * B;
 #pragma unroll
@@ -854,8 +904,8 @@ static __global__ void k_sell_gather(long nslot, int bb, const long long* __rest
   const long k = slot >> 6;
   const int lane = (int)(slot & 63);
   const long long s = src[slot];
-  for (int e = 0; e < bb; e++)
-    to[((size_t)k * bb + e) * 64 + lane] = store_as<T>(s >= 0 ? from[(size_t)s * bb + e] : 0.0);
+  for (int e = 0; e < bb; e++) to[slot_index<T>(k, bb, e, lane)] = store_as<T>(s >= 0 ? from[(size_t)s * bb + e] : 0.0);
+  if (slot_elems<T>(bb) > bb) to[slot_index<T>(k, bb, bb, lane)] = store_as<T>(0.0);
 }
 // Diagonal blocks per slice: to[(s*bb+e)*64+lane] = from[map(row)*bb+e].
 template <typename T>
@@ -867,7 +917,9 @@ static __global__ void k_sell_gather_dia(long nslot, int bb, const int* __restri
   const int lane = (int)(slot & 63);
   const int row = rows[slot];
   const long src = row < 0 ? -1 : (order ? order[row] : row);
-  for (int e = 0; e < bb; e++) to[((size_t)s * bb + e) * 64 + lane] = store_as<T>(src >= 0 ? from[(size_t)src * bb + e] : 0.0);
+  for (int e = 0; e < bb; e++)
+    to[slot_index<T>(s, bb, e, lane)] = store_as<T>(src >= 0 ? from[(size_t)src * bb + e] : 0.0);
+  if (slot_elems<T>(bb) > bb) to[slot_index<T>(s, bb, bb, lane)] = store_as<T>(0.0);
 }
 
 
@@ -889,7 +941,7 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
   if (GATE && st->done) return;
   constexpr int BB = B * B;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = blockIdx.x * kWaves + wave;
+  const int s = xcd_block() * kWaves + wave;
   double dsum[3] = {0.0, 0.0, 0.0};
   if (s < nslice) {
     const int row = s * 64 + lane;
@@ -913,8 +965,7 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
       }
       {
         T a[BB], xv[B];
-#pragma unroll
-        for (int e = 0; e < BB; e++) a[e] = cvt<T>(ld_stream(dval + ((size_t)s * BB + e) * 64 + lane));
+        load_block<BB, TS, T>(dval, (size_t)s, lane, a);
 #pragma unroll
         for (int e = 0; e < B; e++) xv[e] = x[(size_t)row * B + e];
         blk_madd<B, T>(acc, a, xv);
@@ -968,8 +1019,7 @@ __device__ __forceinline__ void trsv_lower_slice(int s, int lane, const int* __r
     blk_msub<B, T>(t, a0, v0);
   }
   T d[BB], y[B];
-#pragma unroll
-  for (int e = 0; e < BB; e++) d[e] = cvt<T>(ld_stream(dval + ((size_t)s * BB + e) * 64 + lane));
+  load_block<BB, TS, T>(dval, (size_t)s, lane, d);
   blk_mul<B, T>(y, d, t);
 #pragma unroll
   for (int r = 0; r < B; r++) out[(size_t)row * B + r] = y[r];
@@ -1009,29 +1059,29 @@ __device__ __forceinline__ void trsv_upper_slice(int s, int lane, const int* __r
 }
 
 template <int B, typename TS, typename T>
-__global__ void __launch_bounds__(kNT) k_trsv_lower(int s0, int s1, const int* __restrict__ rows,
+__global__ void __launch_bounds__(kTrsvNT) k_trsv_lower(int s0, int s1, const int* __restrict__ rows,
                                                    const int* __restrict__ sptr, const int* __restrict__ col,
                                                    const TS* __restrict__ val, const TS* __restrict__ dval,
                                                    const T* rhs, T* out, const KState* __restrict__ st) {
   if (st && st->done) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = s0 + blockIdx.x * kWaves + wave;
+  const int s = s0 + xcd_block() * kTrsvWaves + wave;
   if (s >= s1) return;
   trsv_lower_slice<B, TS, T>(s, lane, rows, sptr, col, val, dval, rhs, out);
 }
 
 template <int B, typename TS, typename T, bool DOT>
-__global__ void __launch_bounds__(kNT) k_trsv_upper(int s0, int s1, const int* __restrict__ rows,
+__global__ void __launch_bounds__(kTrsvNT) k_trsv_upper(int s0, int s1, const int* __restrict__ rows,
                                                    const int* __restrict__ sptr, const int* __restrict__ col,
                                                    const TS* __restrict__ val, T* vec, const T* __restrict__ rvec,
                                                    double* __restrict__ partials, int part0, int nparts,
                                                    const KState* __restrict__ st) {
   if (st && st->done) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = s0 + blockIdx.x * kWaves + wave;
+  const int s = s0 + xcd_block() * kTrsvWaves + wave;
   double dsum[1] = {0.0};
   if (s < s1) trsv_upper_slice<B, TS, T, DOT>(s, lane, rows, sptr, col, val, vec, rvec, dsum[0]);
-  if constexpr (DOT) write_partials<1>(dsum, partials, part0 + blockIdx.x, nparts);
+  if constexpr (DOT) write_partials<1, kTrsvNT>(dsum, partials, part0 + blockIdx.x, nparts);
 }
 
 // ---------------------------------------------------------------------------

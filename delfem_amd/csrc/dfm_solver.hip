@@ -29,6 +29,14 @@ static inline int grid_vec(long n) { return (int)std::min<long>(std::max<long>(1
     }                                                                                       \
   } while (0)
 
+#define DFM_LAUNCH_NT(kernel, grid, nt, ...)                                              \
+  do {                                                                                    \
+    if ((grid) > 0) {                                                                     \
+      hipLaunchKernelGGL((kernel), dim3(grid), dim3(nt), 0, ::dfm::stream(), __VA_ARGS__); \
+      DFM_HIP(hipGetLastError());                                                         \
+    }                                                                                     \
+  } while (0)
+
 #define DFM_LAUNCH_FIN(ND, FIN, partials, nparts, st)                                             \
   do {                                                                                            \
     hipLaunchKernelGGL((k_finalize<ND, FIN>), dim3(1), dim3(kFinNT), 0, ::dfm::stream(), partials, \
@@ -44,8 +52,8 @@ static int sell_alloc_like(const Sell<double>& src, Sell<T>& dst) {
   dst.nk = src.nk;
   dst.bb = src.bb;
   dst.group_slice_ptr = src.group_slice_ptr;
-  DFM_TRY(dst.val.alloc(std::max<size_t>(1, (size_t)src.nk * src.bb * kSlice)));
-  DFM_TRY(dst.dval.alloc(std::max<size_t>(1, (size_t)src.nslice * src.bb * kSlice)));
+  DFM_TRY(dst.val.alloc(std::max<size_t>(1, (size_t)src.nk * slot_elems<T>(src.bb) * kSlice)));
+  DFM_TRY(dst.dval.alloc(std::max<size_t>(1, (size_t)src.nslice * slot_elems<T>(src.bb) * kSlice)));
   return DFM_OK;
 }
 
@@ -113,8 +121,8 @@ static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs
   const int ng = (int)plan.group_slice_ptr.size() - 1;
   for (int g = 0; g < ng; g++) {
     const int s0 = plan.group_slice_ptr[g], s1 = plan.group_slice_ptr[g + 1];
-    DFM_LAUNCH((k_trsv_lower<B, TS, T>), grid_for(s1 - s0, kWaves), s0, s1, plan.rows.p, plan.slice_ptr.p,
-               plan.col.p, L.val.p, L.dval.p, rhs, out, st);
+    DFM_LAUNCH_NT((k_trsv_lower<B, TS, T>), grid_for(s1 - s0, kTrsvWaves), kTrsvNT, s0, s1, plan.rows.p,
+                  plan.slice_ptr.p, plan.col.p, L.val.p, L.dval.p, rhs, out, st);
   }
   return DFM_OK;
 }
@@ -122,7 +130,7 @@ static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs
 static int upper_nparts(const Sell<double>& plan) {
   int n = 0;
   const int ng = (int)plan.group_slice_ptr.size() - 1;
-  for (int g = 0; g < ng; g++) n += grid_for(plan.group_slice_ptr[g + 1] - plan.group_slice_ptr[g], kWaves);
+  for (int g = 0; g < ng; g++) n += grid_for(plan.group_slice_ptr[g + 1] - plan.group_slice_ptr[g], kTrsvWaves);
   return n;
 }
 // Backward sweep; with DOT the partial sums of <rvec, vec> over all rows are
@@ -135,9 +143,9 @@ static int sweep_upper(const Sell<TS>& U, const Sell<double>& plan, T* vec, cons
   int part0 = 0;
   for (int g = 0; g < ng; g++) {
     const int s0 = plan.group_slice_ptr[g], s1 = plan.group_slice_ptr[g + 1];
-    const int grid = grid_for(s1 - s0, kWaves);
-    DFM_LAUNCH((k_trsv_upper<B, TS, T, DOT>), grid, s0, s1, plan.rows.p, plan.slice_ptr.p, plan.col.p, U.val.p,
-               vec, rvec, partials, part0, nparts, st);
+    const int grid = grid_for(s1 - s0, kTrsvWaves);
+    DFM_LAUNCH_NT((k_trsv_upper<B, TS, T, DOT>), grid, kTrsvNT, s0, s1, plan.rows.p, plan.slice_ptr.p, plan.col.p,
+                  U.val.p, vec, rvec, partials, part0, nparts, st);
     part0 += grid;
   }
   return DFM_OK;
@@ -659,9 +667,10 @@ static int bench_kernel_t(dfm_bcrs_s* A, dfm_ilu_s* M, int which, int nrep, doub
     if (which == 1) return sweep_lower<B, TF, T>(*o.L, *o.planL, xv.p, yv.p, st);
     return sweep_upper<B, TF, T, true>(*o.U, *o.planU, yv.p, xv.p, part, st);
   };
-  // algorithmic bytes: values, column indices, row ids, each vector element
-  // read or written once (x of SpMV and the dot operand are the same vector)
-  if (which == 0) *bytes = (nnzA + nb) * bb * sa + 4.0 * nnzA + 4.0 * nb + 2.0 * n * sv;
+  // algorithmic bytes: values, column indices, row ids (sweeps), each vector
+  // element read or written once (x of SpMV and the dot operand are the same
+  // vector)
+  if (which == 0) *bytes = (nnzA + nb) * bb * sa + 4.0 * nnzA + 2.0 * n * sv;
   else if (which == 1) *bytes = (nnzL + nb) * bb * sf + 4.0 * nnzL + 4.0 * nb + 3.0 * n * sv;
   else *bytes = nnzU * bb * sf + 4.0 * nnzU + 4.0 * nb + 3.0 * n * sv;
   DFM_TRY(run());  // warm-up
