@@ -94,6 +94,7 @@ _SIGS = {
     "dfm_batch_info": (C.c_int, [H, vp, vp, vp, vp]),
     "dfm_batch_set_value": (C.c_int, [H, H]),
     "dfm_batch_set_launch": (C.c_int, [H, C.c_int, C.c_int]),
+    "dfm_batch_set_teams": (C.c_int, [H, C.c_uint, C.c_int]),
     "dfm_batch_solve_pcg": (C.c_int, [H, vp, vp, C.c_double, C.c_uint, C.c_int, vp, vp, vp, vp, vp]),
     "dfm_batch_get_factor": (C.c_int, [H, C.c_uint, vp, vp, vp]),
     "dfm_assemble_oseen2d_p1p1": (C.c_int, [H, vp, C.c_double, C.c_double, vp, vp, vp]),

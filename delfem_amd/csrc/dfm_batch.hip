@@ -55,7 +55,8 @@ struct dfm_batch_s {
   std::vector<dfm::SysDesc> desc;
   std::vector<int> cptr;                 // per system kMaxColourBatch+1 colour offsets (local rows)
   dfm::DevBuf<dfm::SysDesc> d_sys;
-  dfm::DevBuf<int> d_cptr, d_sched, d_perm, d_ell_col;
+  dfm::DevBuf<int> d_cptr, d_perm, d_ell_col;
+  std::vector<int> sched;                // systems by decreasing work
   dfm::DevBuf<long long> d_ell_src;
   dfm::DevBuf<unsigned> d_next;
   dfm::DevBuf<double> d_a64, d_f64, d_adia64, d_fdia64, d_b64, d_x64, d_r64;
@@ -64,6 +65,12 @@ struct dfm_batch_s {
   dfm::DevBuf<dfm::BatchOut> d_out;
   const dfm_bcrs_s* A = nullptr;
   int block = 512, wpc = 1;              // launch shape
+  unsigned team_min_rows = 16384;        // systems at least this large get a team of workgroups
+  int team_max = 1;                      // workgroups per team (1: no teams, the default)
+  std::string items_key;                 // launch shape the work items were built for
+  dfm::DevBuf<char> d_items, d_teams;
+  dfm::DevBuf<int> d_fail;
+  int nitem = 0;
   bool tuned = false;
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
   ~dfm_batch_s() {
@@ -75,12 +82,30 @@ namespace dfm {
 
 // ---------------------------------------------------------------------------
 // Device side
+constexpr int kMaxTeam = 16;               // workgroups sharing one system
+constexpr unsigned kTeamSpinLimit = 1u << 21;
+
+// Per system, reset every launch: the team barrier counter (alone on its
+// cache line) and double-buffered slots for the members' partial sums.
+struct TeamState {
+  unsigned arrive;
+  unsigned pad[31];
+  double part[2][kMaxTeam];
+};
+
+// One unit of work: member `member` of a team of `team` workgroups on system `sys`.
+struct Item {
+  int sys, member, team, pad;
+};
+
 struct BatchArgs {
-  int nsys;
+  int nitem;
+  const Item* items;
+  unsigned* next;
+  TeamState* teams;
+  int* fail;
   const SysDesc* sys;
   const int* cptr;
-  const int* sched;
-  unsigned* next;
   const int* ecol;
   const double* a64;
   double* f64;
@@ -114,6 +139,80 @@ __device__ inline double wg_sum(double v, double (*sh)[16], int& buf) {
   return s;
 }
 
+// The workgroups working on one system.  With K = 1 it is the workgroup
+// alone (barriers are workgroup barriers).  With K > 1 every member owns a
+// contiguous share of each row range; a team barrier is: an agent-scope
+// release in every wave, a workgroup barrier, an increment of the system's
+// counter, a wait until all K members arrived (relaxed polls, bounded: a
+// timeout sets *fail and the launch completes with the result marked
+// invalid), a workgroup barrier and an agent-scope acquire in every wave.
+// The per-wave cache maintenance makes a team barrier cost far more than the
+// work it saves on systems of up to 50 k rows, so teams are off by default
+// (team_max = 1); they pay only where a system's iteration is much longer
+// than a barrier.  Team reductions write each member's workgroup sum to a
+// slot, pass a team barrier and add the K slots in member order: identical
+// in every member.
+struct Team {
+  int K, m;
+  TeamState* ts;
+  int* fail;
+  unsigned nsync;
+  int tbuf;
+  double (*sh)[16];
+  int buf;
+  double* bc;   // LDS broadcast slot
+};
+
+__device__ inline void tsync(Team& t) {
+  if (t.K == 1) {
+    __syncthreads();
+    return;
+  }
+  // every wave releases its own stores at agent scope before the workgroup
+  // barrier and acquires after it: with the release and acquire made by one
+  // wave only, other members have read stale rows on this device
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  t.nsync++;
+  if (threadIdx.x == 0 && !__hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    __hip_atomic_fetch_add(&t.ts->arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned target = t.nsync * (unsigned)t.K;
+    unsigned spins = 0;
+    while (__hip_atomic_load(&t.ts->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kTeamSpinLimit) {
+        __hip_atomic_store(t.fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+__device__ inline double tsum(Team& t, double v) {
+  const double s = wg_sum(v, t.sh, t.buf);
+  if (t.K == 1) return s;
+  if (threadIdx.x == 0) t.ts->part[t.tbuf][t.m] = s;
+  tsync(t);
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    for (int k = 0; k < t.K; k++) a += t.ts->part[t.tbuf][k];
+    *t.bc = a;
+  }
+  __syncthreads();
+  const double r = *t.bc;
+  t.tbuf ^= 1;
+  return r;
+}
+
+// this member's share [a, b) of the rows [lo, hi)
+__device__ inline void tshare(const Team& t, int lo, int hi, int& a, int& b) {
+  const long long len = hi - lo;
+  a = lo + (int)(len * t.m / t.K);
+  b = lo + (int)(len * (t.m + 1) / t.K);
+}
+
 // Row access.  A row's W column slots are loaded together (predicated on the
 // system's width, padding -1), then the value and vector loads of every slot
 // are issued together, so a wavefront has 2W independent loads in flight
@@ -127,22 +226,22 @@ __device__ inline void ell_cols(const int* __restrict__ ecol, const SysDesc& d, 
 
 // Rows are taken U at a time per thread (rows i, i+T, .., i+(U-1)T), their
 // loads issued together before any arithmetic.
-template <int W>
+template <int W, typename T = double>
 constexpr int rows_per_pass() { return W <= 8 ? 2 : 1; }
 
-// y = A x over rows [0, n) (ELL + diagonal); returns the <x, y> partial
+// y = A x over rows [lo, hi) (ELL + diagonal); returns the <x, y> partial
 template <int W, typename V, typename T>
-__device__ inline double ell_spmv(const SysDesc& d, const int* __restrict__ ecol, const V* __restrict__ a,
-                                  const V* __restrict__ adia, const T* x, T* y) {
-  constexpr int U = rows_per_pass<W>();
+__device__ inline double ell_spmv(const SysDesc& d, int lo, int hi, const int* __restrict__ ecol,
+                                  const V* __restrict__ a, const V* __restrict__ adia, const T* x, T* y) {
+  constexpr int U = rows_per_pass<W, T>();
   const int NT = blockDim.x;
   double part = 0.0;
-  for (int i0 = threadIdx.x; i0 < d.n; i0 += U * NT) {
+  for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += U * NT) {
     int c[U][W];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = i0 + u * NT;
-      if (i < d.n) {
+      if (i < hi) {
         ell_cols<W>(ecol, d, i, c[u]);
       } else {
 #pragma unroll
@@ -156,13 +255,13 @@ __device__ inline double ell_spmv(const SysDesc& d, const int* __restrict__ ecol
 #pragma unroll
       for (int k = 0; k < W; k++)
         v[u][k] = c[u][k] >= 0 ? (T)a[d.ell0 + (long long)k * d.n + i] * x[c[u][k]] : (T)0;
-      xi[u] = i < d.n ? x[i] : (T)0;
-      di[u] = i < d.n ? (T)adia[d.row0 + i] : (T)0;
+      xi[u] = i < hi ? x[i] : (T)0;
+      di[u] = i < hi ? (T)adia[d.row0 + i] : (T)0;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = i0 + u * NT;
-      if (i >= d.n) continue;
+      if (i >= hi) continue;
       T acc = 0;
 #pragma unroll
       for (int k = 0; k < W; k++) acc += v[u][k];
@@ -181,10 +280,10 @@ __device__ inline double ell_spmv(const SysDesc& d, const int* __restrict__ ecol
 template <int W, bool LOWER, typename V, typename T>
 __device__ inline double ell_subst(const SysDesc& d, int lo, int hi, const int* __restrict__ ecol,
                                    const V* __restrict__ f, const V* __restrict__ fdia, const T* r, T* z) {
-  constexpr int U = rows_per_pass<W>();
+  constexpr int U = rows_per_pass<W, T>();
   const int NT = blockDim.x;
   double part = 0.0;
-  for (int i0 = lo + threadIdx.x; i0 < hi; i0 += U * NT) {
+  for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += U * NT) {
     int c[U][W];
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -229,102 +328,125 @@ __device__ inline double ell_subst(const SysDesc& d, int lo, int hi, const int* 
   return part;
 }
 
-// z = (LU)^-1 r colour by colour (forward), then in reverse (backward); returns <r, z>
+// z = (LU)^-1 r colour by colour (forward), then in reverse (backward);
+// returns <r, z> over the team
 template <int W, typename V, typename T>
-__device__ inline double ell_precond(const SysDesc& d, const int* __restrict__ cp, const int* __restrict__ ecol,
-                                     const V* __restrict__ f, const V* __restrict__ fdia, const T* r, T* z) {
+__device__ inline double ell_precond(Team& tm, const SysDesc& d, const int* __restrict__ cp,
+                                     const int* __restrict__ ecol, const V* __restrict__ f,
+                                     const V* __restrict__ fdia, const T* r, T* z) {
+  int a, b;
   for (int c = 0; c < d.ncol; c++) {
-    ell_subst<W, true, V, T>(d, cp[c], cp[c + 1], ecol, f, fdia, r, z);
-    __syncthreads();
+    tshare(tm, cp[c], cp[c + 1], a, b);
+    ell_subst<W, true, V, T>(d, a, b, ecol, f, fdia, r, z);
+    tsync(tm);
   }
   double part = 0.0;
   for (int c = d.ncol - 1; c >= 0; c--) {
-    part += ell_subst<W, false, V, T>(d, cp[c], cp[c + 1], ecol, f, fdia, r, z);
-    __syncthreads();
+    tshare(tm, cp[c], cp[c + 1], a, b);
+    part += ell_subst<W, false, V, T>(d, a, b, ecol, f, fdia, r, z);
+    if (c > 0) tsync(tm);
   }
-  return part;
+  return tsum(tm, part);
+}
+
+// p = beta p + z over [a, b)
+template <int W, typename T>
+__device__ inline void p_update(int a, int b, T beta, const T* z, T* p) {
+  constexpr int U = rows_per_pass<W, T>();
+  for (int i0 = a + (int)threadIdx.x; i0 < b; i0 += U * (int)blockDim.x) {
+    T pv[U], zv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < b) {
+        pv[u] = p[i];
+        zv[u] = z[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < b) p[i] = beta * pv[u] + zv[u];
+    }
+  }
+}
+
+// x += alpha p; r -= alpha q over [a, b); returns the <r, r> partial
+template <int W, typename T>
+__device__ inline double xr_update(int a, int b, T alpha, const T* p, const T* q, T* x, T* r) {
+  constexpr int U = rows_per_pass<W, T>();
+  const T malpha = -alpha;
+  double rr = 0.0;
+  for (int i0 = a + (int)threadIdx.x; i0 < b; i0 += U * (int)blockDim.x) {
+    T xv[U], pv[U], rv[U], qv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < b) {
+        xv[u] = x[i];
+        pv[u] = p[i];
+        rv[u] = r[i];
+        qv[u] = q[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i >= b) continue;
+      x[i] = xv[u] + alpha * pv[u];
+      const T rn = rv[u] + malpha * qv[u];
+      r[i] = rn;
+      rr += (double)rn * (double)rn;
+    }
+  }
+  return rr;
 }
 
 // The reference's PCG loop on this system (FP64 copies, x from 0).  Stops
 // when <r,r> < tol2 <r0,r0>; returns the reference's iteration value and
 // leaves <r,r>/<r0,r0> in rel2.
 template <int W>
-__device__ inline int wg_pcg64(const SysDesc& d, const int* cp, const int* ecol, const double* a, const double* adia,
-                               const double* f, const double* fdia, double* r, double* x, double* z, double* p,
-                               double* q, double tol2, int maxit, double (*sh)[16], int& buf, double& rel2) {
+__device__ inline int wg_pcg64(Team& tm, const SysDesc& d, const int* cp, const int* ecol, const double* a,
+                               const double* adia, const double* f, const double* fdia, double* r, double* x, double* z,
+                               double* p, double* q, double tol2, int maxit, double& rel2) {
+  int ra, rb;
+  tshare(tm, 0, d.n, ra, rb);
   double rr0 = 0.0;
-  for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+  for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) {
     x[i] = 0.0;
     rr0 += r[i] * r[i];
   }
-  rr0 = wg_sum(rr0, sh, buf);
+  rr0 = tsum(tm, rr0);
   rel2 = 0.0;
   if (rr0 < 1.0e-30) return 0;
   const double inv_rr0 = 1.0 / rr0;
-  double rz = wg_sum(ell_precond<W, double, double>(d, cp, ecol, f, fdia, r, z), sh, buf);
-  for (int i = threadIdx.x; i < d.n; i += blockDim.x) p[i] = z[i];
-  __syncthreads();
+  double rz = ell_precond<W, double, double>(tm, d, cp, ecol, f, fdia, r, z);
+  for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) p[i] = z[i];
+  tsync(tm);
   rel2 = 1.0;
   for (int it = 0; it < maxit; it++) {
-    const double pap = wg_sum(ell_spmv<W, double, double>(d, ecol, a, adia, p, q), sh, buf);
+    const double pap = tsum(tm, ell_spmv<W, double, double>(d, ra, rb, ecol, a, adia, p, q));
     const double alpha = rz / pap;
-    double rr = 0.0;
-    constexpr int U = rows_per_pass<W>();
-    for (int i0 = threadIdx.x; i0 < d.n; i0 += U * (int)blockDim.x) {
-      double xv[U], pv[U], rv[U], qv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = i0 + u * (int)blockDim.x;
-        if (i < d.n) {
-          xv[u] = x[i];
-          pv[u] = p[i];
-          rv[u] = r[i];
-          qv[u] = q[i];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = i0 + u * (int)blockDim.x;
-        if (i >= d.n) continue;
-        x[i] = xv[u] + alpha * pv[u];
-        const double rn = rv[u] + (-alpha) * qv[u];
-        r[i] = rn;
-        rr += rn * rn;
-      }
-    }
-    rr = wg_sum(rr, sh, buf);
+    const double rr = tsum(tm, xr_update<W, double>(ra, rb, alpha, p, q, x, r));
     rel2 = rr * inv_rr0;
     if (rel2 < tol2) return it;
-    const double rz_new = wg_sum(ell_precond<W, double, double>(d, cp, ecol, f, fdia, r, z), sh, buf);
+    const double rz_new = ell_precond<W, double, double>(tm, d, cp, ecol, f, fdia, r, z);
     const double beta = rz_new / rz;
     rz = rz_new;
-    for (int i0 = threadIdx.x; i0 < d.n; i0 += U * (int)blockDim.x) {
-      double pv[U], zv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = i0 + u * (int)blockDim.x;
-        if (i < d.n) {
-          pv[u] = p[i];
-          zv[u] = z[i];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = i0 + u * (int)blockDim.x;
-        if (i < d.n) p[i] = beta * pv[u] + zv[u];
-      }
-    }
-    __syncthreads();
+    p_update<W, double>(ra, rb, beta, z, p);
+    tsync(tm);
   }
   return maxit;
 }
 
-// r64 = b - A64 x64; returns <r64, r64>
+// r64 = b - A64 x64 over this member's rows (r64 may be null); returns <r64, r64> over the team
 template <int W>
-__device__ inline double true_res(const SysDesc& d, const int* ecol, const double* a64, const double* adia64,
-                                  const double* b, const double* x, double* r64, double (*sh)[16], int& buf) {
+__device__ inline double true_res(Team& tm, const SysDesc& d, const int* ecol, const double* a64,
+                                  const double* adia64, const double* b, const double* x, double* r64) {
+  int ra, rb;
+  tshare(tm, 0, d.n, ra, rb);
   double part = 0.0;
-  for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+  for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) {
     int c[W];
     ell_cols<W>(ecol, d, i, c);
     double v[W];
@@ -338,7 +460,7 @@ __device__ inline double true_res(const SysDesc& d, const int* ecol, const doubl
     if (r64) r64[i] = rv;
     part += rv * rv;
   }
-  return wg_sum(part, sh, buf);
+  return tsum(tm, part);
 }
 
 // FP32 PCG with FP64 reliable updates: the FP32 recurrence runs on the FP32
@@ -348,21 +470,19 @@ __device__ inline double true_res(const SysDesc& d, const int* ecol, const doubl
 // iteration continues (restarted from the true residual when the recursion
 // had run more than 10x ahead of it).  Converged on the FP64 true residual.
 template <int W>
-__device__ inline void wg_pcg_mixed(const SysDesc& d, const int* cp, const int* ecol, const BatchArgs& g, const double* b,
-                                    double* x64, double* r64, float* r, float* xr, float* z, float* p, float* q,
-                                    double (*sh)[16], int& buf, int& iters, int& nupd, int& status, double& relres) {
-  const float* a32 = g.a32;
-  const float* adia32 = g.adia32;
-  const float* f32 = g.f32;
-  const float* fdia32 = g.fdia32;
+__device__ inline void wg_pcg_mixed(Team& tm, const SysDesc& d, const int* cp, const int* ecol, const BatchArgs& g,
+                                    const double* b, double* x64, double* r64, float* r, float* xr, float* z,
+                                    float* p, float* q, int& iters, int& nupd, int& status, double& relres) {
+  int ra, rb;
+  tshare(tm, 0, d.n, ra, rb);
   double bb = 0.0;
-  for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+  for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) {
     x64[i] = 0.0;
     xr[i] = 0.0f;
     r[i] = (float)b[i];
     bb += b[i] * b[i];
   }
-  bb = wg_sum(bb, sh, buf);
+  bb = tsum(tm, bb);
   iters = 0;
   nupd = 0;
   if (bb < 1.0e-30) {
@@ -371,84 +491,45 @@ __device__ inline void wg_pcg_mixed(const SysDesc& d, const int* cp, const int* 
     return;
   }
   const double delta2 = 0.01;
-  double rz = wg_sum(ell_precond<W, float, float>(d, cp, ecol, f32, fdia32, r, z), sh, buf);
-  for (int i = threadIdx.x; i < d.n; i += blockDim.x) p[i] = z[i];
-  __syncthreads();
+  double rz = ell_precond<W, float, float>(tm, d, cp, ecol, g.f32, g.fdia32, r, z);
+  for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) p[i] = z[i];
+  tsync(tm);
   double rr_ref = bb, rr_true = bb;
   bool done = false;
   int it = 0;
   for (; it < g.maxit && !done; it++) {
-    const double pap = wg_sum(ell_spmv<W, float, float>(d, ecol, a32, adia32, p, q), sh, buf);
-    const float alpha = (float)(rz / pap), malpha = -alpha;
-    double rr = 0.0;
-    constexpr int U = rows_per_pass<W>();
-    for (int i0 = threadIdx.x; i0 < d.n; i0 += U * (int)blockDim.x) {
-      float xv[U], pv[U], rv[U], qv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = i0 + u * (int)blockDim.x;
-        if (i < d.n) {
-          xv[u] = xr[i];
-          pv[u] = p[i];
-          rv[u] = r[i];
-          qv[u] = q[i];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = i0 + u * (int)blockDim.x;
-        if (i >= d.n) continue;
-        xr[i] = xv[u] + alpha * pv[u];
-        const float rn = rv[u] + malpha * qv[u];
-        r[i] = rn;
-        rr += (double)rn * (double)rn;
-      }
-    }
-    rr = wg_sum(rr, sh, buf);
+    const double pap = tsum(tm, ell_spmv<W, float, float>(d, ra, rb, ecol, g.a32, g.adia32, p, q));
+    const float alpha = (float)(rz / pap);
+    const double rr = tsum(tm, xr_update<W, float>(ra, rb, alpha, p, q, xr, r));
     bool restart = false;
     if (rr < delta2 * rr_ref || rr < g.tol2 * bb) {
-      for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+      for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) {
         x64[i] += (double)xr[i];
         xr[i] = 0.0f;
       }
-      __syncthreads();
-      rr_true = true_res<W>(d, ecol, g.a64, g.adia64, b, x64, r64, sh, buf);
+      tsync(tm);
+      rr_true = true_res<W>(tm, d, ecol, g.a64, g.adia64, b, x64, r64);
       nupd++;
       if (rr_true < g.tol2 * bb) {
         done = true;
         continue;
       }
-      for (int i = threadIdx.x; i < d.n; i += blockDim.x) r[i] = (float)r64[i];
-      __syncthreads();
+      for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) r[i] = (float)r64[i];
+      tsync(tm);
       restart = rr_true > 100.0 * rr;
       rr_ref = rr_true;
     }
-    const double rz_new = wg_sum(ell_precond<W, float, float>(d, cp, ecol, f32, fdia32, r, z), sh, buf);
+    const double rz_new = ell_precond<W, float, float>(tm, d, cp, ecol, g.f32, g.fdia32, r, z);
     const float beta = restart ? 0.0f : (float)(rz_new / rz);
     rz = rz_new;
-    for (int i0 = threadIdx.x; i0 < d.n; i0 += U * (int)blockDim.x) {
-      float pv[U], zv[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = i0 + u * (int)blockDim.x;
-        if (i < d.n) {
-          pv[u] = p[i];
-          zv[u] = z[i];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int i = i0 + u * (int)blockDim.x;
-        if (i < d.n) p[i] = beta * pv[u] + zv[u];
-      }
-    }
-    __syncthreads();
+    p_update<W, float>(ra, rb, beta, z, p);
+    tsync(tm);
   }
   iters = it;
   if (!done) {
-    for (int i = threadIdx.x; i < d.n; i += blockDim.x) x64[i] += (double)xr[i];
-    __syncthreads();
-    rr_true = true_res<W>(d, ecol, g.a64, g.adia64, b, x64, r64, sh, buf);
+    for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) x64[i] += (double)xr[i];
+    tsync(tm);
+    rr_true = true_res<W>(tm, d, ecol, g.a64, g.adia64, b, x64, r64);
   }
   status = rr_true < g.tol2 * bb ? 1 : 0;
   relres = sqrt(rr_true / bb);
@@ -460,12 +541,14 @@ __device__ inline void wg_pcg_mixed(const SysDesc& d, const int* cp, const int* 
 // in registers, the fill positions matched by column.  Returns the number
 // of singular pivots.
 template <int W>
-__device__ inline int wg_ilu0(const SysDesc& d, const int* cp, const int* __restrict__ ecol,
+__device__ inline int wg_ilu0(Team& tm, const SysDesc& d, const int* cp, const int* __restrict__ ecol,
                               const double* __restrict__ a64, double* f64, const double* __restrict__ adia64,
-                              double* fdia64, double (*sh)[16], int& buf) {
+                              double* fdia64) {
   double nsing = 0.0;
+  int ra, rb;
   for (int c = 0; c < d.ncol; c++) {
-    for (int i = cp[c] + threadIdx.x; i < cp[c + 1]; i += blockDim.x) {
+    tshare(tm, cp[c], cp[c + 1], ra, rb);
+    for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) {
       int ci[W];
       double fi[W];
       ell_cols<W>(ecol, d, i, ci);
@@ -502,55 +585,59 @@ __device__ inline int wg_ilu0(const SysDesc& d, const int* cp, const int* __rest
       for (int k = 0; k < W; k++)
         if (ci[k] >= 0) f64[d.ell0 + (long long)k * d.n + i] = ci[k] > i ? fi[k] * dii : fi[k];
     }
-    __syncthreads();
+    tsync(tm);
   }
-  return (int)wg_sum(nsing, sh, buf);
+  return (int)tsum(tm, nsing);
 }
 
 template <int W, bool MIXED>
 __global__ void __launch_bounds__(1024) k_batch_solve(BatchArgs g) {
   __shared__ double sh[2][16];
-  __shared__ int s_sys;
+  __shared__ double s_bc;
+  __shared__ Item s_item;
   __shared__ unsigned long long s_t0;
   int buf = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       const unsigned t = atomicAdd(g.next, 1u);
-      s_sys = t < (unsigned)g.nsys ? g.sched[t] : -1;
+      s_item = t < (unsigned)g.nitem ? g.items[t] : Item{-1, 0, 1, 0};
       s_t0 = wall_clock64();
     }
     __syncthreads();
-    const int s = s_sys;
+    const Item it = s_item;
     const unsigned long long t0 = s_t0;
     __syncthreads();
-    if (s < 0) break;
+    if (it.sys < 0) break;
+    const int s = it.sys;
+    Team tm{it.team, it.member, g.teams + s, g.fail, 0u, 0, sh, buf, &s_bc};
     const SysDesc d = g.sys[s];
     const int* cp = g.cptr + (size_t)s * (kMaxColourBatch + 1);
-    const int nsing = wg_ilu0<W>(d, cp, g.ecol, g.a64, g.f64, g.adia64, g.fdia64, sh, buf);
+    const int nsing = wg_ilu0<W>(tm, d, cp, g.ecol, g.a64, g.f64, g.adia64, g.fdia64);
     int iters = 0, outer = 0, status = 0;
     double relres = 0.0;
     const double* b = g.b64 + d.row0;
     double* x = g.x64 + d.row0;
+    int ra, rb;
+    tshare(tm, 0, d.n, ra, rb);
     if (nsing > 10) {
       status = -1;
     } else if (!MIXED) {
       double* vb = reinterpret_cast<double*>(g.vec);
       double *r = vb + d.row0, *z = vb + g.nrow + d.row0, *p = vb + 2 * g.nrow + d.row0,
              *q = vb + 3 * g.nrow + d.row0;
-      for (int i = threadIdx.x; i < d.n; i += blockDim.x) r[i] = b[i];
-      __syncthreads();
+      for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) r[i] = b[i];
+      tsync(tm);
       double rel2 = 0.0;
-      iters = wg_pcg64<W>(d, cp, g.ecol, g.a64, g.adia64, g.f64, g.fdia64, r, x, z, p, q, g.tol2, g.maxit, sh, buf,
-                          rel2);
+      iters = wg_pcg64<W>(tm, d, cp, g.ecol, g.a64, g.adia64, g.f64, g.fdia64, r, x, z, p, q, g.tol2, g.maxit, rel2);
       status = rel2 < g.tol2 ? 1 : 0;
       double bb = 0.0;
-      for (int i = threadIdx.x; i < d.n; i += blockDim.x) bb += b[i] * b[i];
-      bb = wg_sum(bb, sh, buf);
-      const double rt = true_res<W>(d, g.ecol, g.a64, g.adia64, b, x, nullptr, sh, buf);
+      for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) bb += b[i] * b[i];
+      bb = tsum(tm, bb);
+      const double rt = true_res<W>(tm, d, g.ecol, g.a64, g.adia64, b, x, nullptr);
       relres = bb > 0.0 ? sqrt(rt / bb) : 0.0;
     } else {
       // FP32 copies of this system's matrix and factor, then the refined solve
-      for (int i = threadIdx.x; i < d.n; i += blockDim.x) {
+      for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) {
         for (int k = 0; k < d.w; k++) {
           const long long qq = d.ell0 + (long long)k * d.n + i;
           g.a32[qq] = (float)g.a64[qq];
@@ -559,13 +646,14 @@ __global__ void __launch_bounds__(1024) k_batch_solve(BatchArgs g) {
         g.adia32[d.row0 + i] = (float)g.adia64[d.row0 + i];
         g.fdia32[d.row0 + i] = (float)g.fdia64[d.row0 + i];
       }
-      __syncthreads();
+      tsync(tm);
       float* vf = reinterpret_cast<float*>(g.vec);
       float *r = vf + d.row0, *z = vf + g.nrow + d.row0, *p = vf + 2 * g.nrow + d.row0, *q = vf + 3 * g.nrow + d.row0,
             *xr = vf + 4 * g.nrow + d.row0;
-      wg_pcg_mixed<W>(d, cp, g.ecol, g, b, x, g.r64 + d.row0, r, xr, z, p, q, sh, buf, iters, outer, status, relres);
+      wg_pcg_mixed<W>(tm, d, cp, g.ecol, g, b, x, g.r64 + d.row0, r, xr, z, p, q, iters, outer, status, relres);
     }
-    if (threadIdx.x == 0) {
+    buf = tm.buf;
+    if (threadIdx.x == 0 && tm.m == 0) {
       BatchOut o;
       o.iters = iters;
       o.outer = outer;
@@ -580,8 +668,14 @@ __global__ void __launch_bounds__(1024) k_batch_solve(BatchArgs g) {
   }
 }
 
-__global__ void k_batch_reset(unsigned* next) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *next = 0u;
+// Work counter, failure flag and every system's team counter back to zero.
+__global__ void k_batch_reset(unsigned* next, int* fail, TeamState* teams, int nsys) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    *next = 0u;
+    *fail = 0;
+  }
+  if (t < nsys) teams[t].arrive = 0u;
 }
 __global__ void k_batch_gather(long long nslot, const long long* __restrict__ src, const double* __restrict__ val,
                                double* __restrict__ out) {
@@ -605,11 +699,53 @@ static int grid256(long long n) { return (int)std::max<long long>(1, (n + 255) /
 
 // ---------------------------------------------------------------------------
 // Host side
+static int ncu_count();
+
+// Work items for a launch shape: the largest systems (at least team_min_rows
+// rows) get teams of 2..team_max workgroups while the team members fit in
+// the workgroups resident at once less a reserve; every other system is one
+// item.  Teams come first, so all their members are taken by resident
+// workgroups at the start of the launch and no member waits for a partner
+// that cannot start.
+template <int W, bool MIXED>
+static int build_items(dfm_batch_s* B) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_solve<W, MIXED>),
+                                                   B->block, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const int resident = per_cu * ncu_count();
+  char key[96];
+  snprintf(key, sizeof key, "%d:%d:%d:%u:%d", W, (int)MIXED, B->block, B->team_min_rows, B->team_max);
+  if (B->items_key == key) return DFM_OK;
+  int budget = resident - std::max(16, resident / 4);
+  std::vector<Item> teams, singles;
+  for (int s : B->sched) {
+    const int n = B->desc[s].n;
+    int K = 1;
+    if (B->team_max > 1 && (unsigned)n >= B->team_min_rows)
+      K = std::min({B->team_max, kMaxTeam, std::max(2, n / (int)std::max(1u, B->team_min_rows / 4))});
+    if (K > 1 && K <= budget) {
+      budget -= K;
+      for (int m = 0; m < K; m++) teams.push_back(Item{s, m, K, 0});
+    } else {
+      singles.push_back(Item{s, 0, 1, 0});
+    }
+  }
+  teams.insert(teams.end(), singles.begin(), singles.end());
+  B->nitem = (int)teams.size();
+  DFM_TRY(B->d_items.upload(reinterpret_cast<const char*>(teams.data()), teams.size() * sizeof(Item)));
+  DFM_HIP(hipStreamSynchronize(stream()));
+  B->items_key = key;
+  return DFM_OK;
+}
+
 static int batch_args(dfm_batch_s* B, bool mixed, double tol, unsigned maxit, BatchArgs& g) {
-  g.nsys = (int)B->nsys;
+  g.nitem = B->nitem;
+  g.items = reinterpret_cast<const Item*>(B->d_items.p);
+  g.teams = reinterpret_cast<TeamState*>(B->d_teams.p);
+  g.fail = B->d_fail.p;
   g.sys = B->d_sys.p;
   g.cptr = B->d_cptr.p;
-  g.sched = B->d_sched.p;
   g.next = B->d_next.p;
   g.ecol = B->d_ell_col.p;
   g.a64 = B->d_a64.p;
@@ -647,11 +783,16 @@ static int ncu_count() {
 // The solve step (work counter reset + persistent launch), replayed from a
 // graph per (precision, launch shape, tolerance, iteration limit).
 static int batch_launch(dfm_batch_s* B, bool mixed, double tol, unsigned maxit) {
+  const int W = B->max_w <= 6 ? 6 : (B->max_w <= 8 ? 8 : 16);
+  if (W == 6) DFM_TRY(mixed ? (build_items<6, true>(B)) : (build_items<6, false>(B)));
+  else if (W == 8) DFM_TRY(mixed ? (build_items<8, true>(B)) : (build_items<8, false>(B)));
+  else DFM_TRY(mixed ? (build_items<16, true>(B)) : (build_items<16, false>(B)));
   BatchArgs g;
   DFM_TRY(batch_args(B, mixed, tol, maxit, g));
-  const int grid = std::min<int>((int)B->nsys, ncu_count() * B->wpc);
-  char key[160];
-  snprintf(key, sizeof key, "%d:%d:%d:%.17g:%u:%p", (int)mixed, B->block, B->wpc, tol, maxit, g.vec);
+  const int grid = std::min<int>(B->nitem, ncu_count() * B->wpc);
+  char key[200];
+  snprintf(key, sizeof key, "%d:%d:%d:%.17g:%u:%p:%s", (int)mixed, B->block, B->wpc, tol, maxit, g.vec,
+           B->items_key.c_str());
   hipStream_t s = stream();
   for (auto& e : B->graphs)
     if (e.first == key) {
@@ -659,7 +800,8 @@ static int batch_launch(dfm_batch_s* B, bool mixed, double tol, unsigned maxit) 
       return DFM_OK;
     }
   auto body = [&]() -> int {
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, B->d_next.p);
+    hipLaunchKernelGGL(k_batch_reset, dim3((B->nsys + 255) / 256), dim3(256), 0, s, B->d_next.p, B->d_fail.p,
+                       reinterpret_cast<TeamState*>(B->d_teams.p), (int)B->nsys);
     DFM_HIP(hipGetLastError());
     if (B->max_w <= 6) {
       if (mixed) hipLaunchKernelGGL((k_batch_solve<6, true>), dim3(grid), dim3(B->block), 0, s, g);
@@ -780,7 +922,9 @@ int dfm_batch_create(const dfm_bcrs* A, unsigned nsys, const unsigned* sys_ptr, 
   const size_t nslot = (size_t)std::max<long long>(1, ell_total), nr = A->nb;
   DFM_TRY(B->d_sys.upload(B->desc));
   DFM_TRY(B->d_cptr.upload(B->cptr));
-  DFM_TRY(B->d_sched.upload(sched));
+  B->sched = sched;
+  DFM_TRY(B->d_teams.alloc((size_t)nsys * sizeof(TeamState)));
+  DFM_TRY(B->d_fail.alloc(1));
   DFM_TRY(B->d_perm.upload(B->perm));
   DFM_TRY(B->d_ell_col.upload(ecol));
   DFM_TRY(B->d_ell_src.upload(esrc));
@@ -838,6 +982,14 @@ int dfm_batch_set_launch(dfm_batch* B, int block, int wg_per_cu) {
   return DFM_OK;
 }
 
+int dfm_batch_set_teams(dfm_batch* B, unsigned min_rows, int max_team) {
+  DFM_CHECK(B && max_team >= 1 && max_team <= kMaxTeam, DFM_BAD_ARG, "max_team must be 1..%d", kMaxTeam);
+  B->team_min_rows = std::max(1u, min_rows);
+  B->team_max = max_team;
+  B->items_key.clear();
+  return DFM_OK;
+}
+
 int dfm_batch_solve_pcg(dfm_batch* B, const double* rhs, double* x, double tol, unsigned maxit, int precision,
                         unsigned* iters, int* outer, int* status, double* relres, double* ms) {
   DFM_CHECK(B && rhs && x && tol > 0.0 && maxit > 0, DFM_BAD_ARG, "bad arguments to dfm_batch_solve_pcg");
@@ -853,6 +1005,9 @@ int dfm_batch_solve_pcg(dfm_batch* B, const double* rhs, double* x, double tol, 
   DFM_HIP(hipGetLastError());
   std::vector<BatchOut> o(B->nsys);
   DFM_TRY(B->d_out.download(o.data(), B->nsys));
+  int fail = 0;
+  DFM_TRY(B->d_fail.download(&fail, 1));
+  DFM_CHECK(!fail, DFM_HIP_ERROR, "a team barrier of the batched solve timed out (results invalid)");
   int dev = 0, khz = 0;
   DFM_HIP(hipGetDevice(&dev));
   DFM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));

@@ -381,6 +381,10 @@ class BatchSystem:
     def set_launch(self, block, wg_per_cu):
         call("dfm_batch_set_launch", self.h, int(block), int(wg_per_cu))
 
+    def set_teams(self, min_rows, max_team):
+        """Systems of at least min_rows rows get teams of up to max_team workgroups."""
+        call("dfm_batch_set_teams", self.h, int(min_rows), int(max_team))
+
     def solve(self, rhs, x, tol, maxit, precision="mixed_f32"):
         """Returns (all_converged, dict of per-system arrays)."""
         _sync_torch()

@@ -172,6 +172,12 @@ int dfm_batch_set_value(dfm_batch *B, const dfm_bcrs *A);
 /* Launch shape of the persistent kernel: threads per workgroup (256, 512,
  * 1024) and workgroups per compute unit (block * wg_per_cu <= 2048). */
 int dfm_batch_set_launch(dfm_batch *B, int block, int wg_per_cu);
+/* Systems of at least min_rows rows are solved by a team of up to max_team
+ * workgroups (row ranges split, team barriers between phases), as many as fit
+ * in the workgroups resident at once; max_team 1 (the default) keeps one
+ * workgroup per system.  A team barrier costs tens of microseconds of cache
+ * maintenance, so teams pay only for systems far larger than 50 k rows. */
+int dfm_batch_set_teams(dfm_batch *B, unsigned min_rows, int max_team);
 /* One step for every system: ILU(0) numeric, then PCG from x = 0 on rhs
  * (concatenated, A's numbering) to ||r|| < tol ||rhs||.  precision:
  * DFM_PREC_FP64 (the reference's PCG) or DFM_PREC_MIXED_F32 (FP32 matrix,

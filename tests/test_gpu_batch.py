@@ -120,3 +120,39 @@ def test_batch_rejects_coupled_systems(gpu):
     fem.assemble()
     with pytest.raises(gpu._lib.DfmError):
         gpu.BatchSystem(fem, np.array([0, 7, p.nnode], dtype=np.uint32))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_f32"])
+def test_batch_teams(batch, orc, prec):
+    """Systems of at least 100 rows solved by teams of 2-4 workgroups (row
+    ranges split, team barriers): factors still bit-exact, FP64 iterations
+    within one of the oracle, FP32 refinement to the FP64 tolerance."""
+    prob, ref, fem, B, torch = batch["prob"], batch["ref"], batch["fem"], batch["B"], batch["torch"]
+    _case(f"batch teams {prec}")
+    B.set_teams(100, 4)
+    try:
+        rhs = fem.res.clone()
+        x = torch.zeros_like(rhs)
+        tol = 1e-10 if prec == "fp64" else 1e-8
+        ok, out = B.solve(rhs, x, tol, 4000, precision=prec)
+        assert ok and np.all(out["status"] == 1) and np.all(out["relres"] < 1.01 * tol)
+        xg = x.cpu().numpy()
+        for s in range(B.nsys):
+            r0, r1 = int(prob.sys_ptr[s]), int(prob.sys_ptr[s + 1])
+            S = sub_system(ref, r0, r1)
+            n = r1 - r0
+            _, _, order = orc.colour_order(S["rowptr"], S["col"])
+            F = orc.iluk_symbolic(S["rowptr"], S["col"], 0, order)
+            ov, od, _ = orc.ilu_values(n, 1, S["rowptr"], S["col"], S["val"], S["dia"], F, order)
+            fval, fdia, _ = B.factor(s)
+            assert np.array_equal(fval, ov) and np.array_equal(fdia, od), s
+            if prec == "fp64":
+                _, _, it_o, x_o, _ = orc.solve("pcg", n, 1, S["rowptr"], S["col"], S["val"], S["dia"], F, ov, od,
+                                               S["res"], 1e-10, 2000, order)
+                assert abs(int(out["iters"][s]) - it_o) <= 1, (s, out["iters"][s], it_o)
+                assert np.linalg.norm(xg[r0:r1] - x_o) <= 1e-8 * np.linalg.norm(x_o), s
+            else:
+                y = orc.matvec(n, 1, S["rowptr"], S["col"], S["val"], S["dia"], -1.0, xg[r0:r1], 1.0, S["res"])
+                assert np.linalg.norm(y) <= 1.01e-8 * np.linalg.norm(S["res"]), s
+    finally:
+        B.set_teams(16384, 1)
