@@ -181,8 +181,8 @@ def main():
     for k, v in kern.items():
         log(f"[bench] kernel {k}: {v['ms']:.4f} ms, {v['bytes'] / 1e6:.1f} MB, {v['gbs']:.0f} GB/s "
             f"({100 * v['frac']:.1f}% of peak)")
-    ceiling = read_ceiling(torch, stream)
-    log(f"[bench] streaming read of 2 GiB (torch.sum): {ceiling:.0f} GB/s")
+    ceiling = read_ceiling(_lib)
+    log(f"[bench] streaming read of 2 GiB (library read kernel): {ceiling:.0f} GB/s")
     dom = kern[f"spmv_{mixed}"]
     kname = {"mixed_f32": "k_spmv<3, float, float, true, true>", "mixed_f32_bf16": "k_spmv<3, float, float, true, true>",
              "mixed": "k_spmv<3, float, double, true, true>", "mixed_bf16": "k_spmv<3, float, double, true, true>"
@@ -408,22 +408,14 @@ def cpu_baseline_batch(prob, fem, cfg, out_d, args):
     }
 
 
-def read_ceiling(torch, stream, gib=2, reps=10):
-    """Streaming-read rate of this device: a 2 GiB float32 sum (torch's
-    reduction), timed with HIP events.  Context for the roofline fraction,
+def read_ceiling(lib, gib=2, reps=10):
+    """Streaming-read rate of this device: the library's 16-byte-load read
+    kernel over 2 GiB, timed with HIP events on the library stream.  Context
+    for the roofline fraction (what a read-dominated kernel can reach here),
     not its peak."""
-    a = torch.ones(gib << 28, dtype=torch.float32, device="cuda")
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    with torch.cuda.stream(stream):
-        a.sum()
-        ev[0].record(stream)
-        for _ in range(reps):
-            a.sum()
-        ev[1].record(stream)
-    ev[1].synchronize()
-    ms = ev[0].elapsed_time(ev[1]) / reps
-    del a
-    return (gib << 30) / (ms * 1e-3) / 1e9
+    ms = C.c_double(0.0)
+    lib.call("dfm_bench_read", C.c_size_t(gib << 30), reps, C.byref(ms))
+    return (gib << 30) / (ms.value * 1e-3) / 1e9
 
 
 def natural_iterations(linsys, s, cfg, rhs, solve, torch):

@@ -112,6 +112,7 @@ _SIGS = {
     "dfm_solve_pbicgstab": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_bench_kernel": (C.c_int, [H, H, C.c_int, C.c_int, C.c_int, f64p, f64p]),
     "dfm_bench_copy": (C.c_int, [C.c_size_t, C.c_int, f64p]),
+    "dfm_bench_read": (C.c_int, [C.c_size_t, C.c_int, f64p]),
 }
 
 _lib = None

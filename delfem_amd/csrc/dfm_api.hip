@@ -592,6 +592,16 @@ int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* ord
     }
     build_sell_plan(rowsL, gptrL, begL, endL, M->fcol, nullptr, M->planL);
     build_sell_plan(rowsU, gptrU, begU, endU, M->fcol, nullptr, M->planU);
+    // rows of the first backward level (no U entries), flagged per forward
+    // slice lane for the fused PCG sweep
+    const SellPlan& pu = M->planU;
+    M->upper0_empty = pu.group_slice_ptr.size() >= 2 && pu.slice_ptr[pu.group_slice_ptr[1]] == pu.slice_ptr[0];
+    std::vector<unsigned char> mask(M->planL.rows.size(), 0);
+    for (size_t i = 0; i < mask.size(); i++) {
+      const int row = M->planL.rows[i];
+      mask[i] = row >= 0 && levU[row] == 0;
+    }
+    DFM_TRY(M->d_rzmaskL.upload(mask));
   }
   // permuted copy of A: rows in factor numbering, columns renumbered and sorted
   {

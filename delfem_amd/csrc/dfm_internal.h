@@ -116,7 +116,6 @@ void build_sell_plan(const std::vector<int>& rows_by_group, const std::vector<in
                      const std::vector<unsigned>& beg, const std::vector<unsigned>& end,
                      const std::vector<unsigned>& colv, const std::vector<long long>* srcv,
                      SellPlan& plan);
-
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Sell {
@@ -174,6 +173,8 @@ struct dfm_ilu_s {
   dfm::DevBuf<int> d_nsing;
   // triangular-solve layouts (value copies refreshed after each factorisation)
   dfm::SellPlan planL, planU;
+  bool upper0_empty = false;            // first backward level has no U entries (true for level sets)
+  dfm::DevBuf<unsigned char> d_rzmaskL; // per forward-sweep slice lane: row lies in the first backward level
   dfm::DevBuf<long long> d_srcL, d_srcU;
   dfm::Sell<double> L64, U64;
   dfm::Sell<float> L32, U32;

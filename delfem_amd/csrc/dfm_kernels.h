@@ -75,6 +75,25 @@ enum {
 
 constexpr int kFinNT = 1024;
 
+// This thread's share of sum(p[0..n)): p[t] + p[t + NT] + p[t + 2 NT] + ...
+// added in that order, with U loads issued before their adds so that a long
+// partial array costs a few memory round trips rather than one per element.
+template <int NT, int U = 8>
+__device__ inline double strided_sum(const double* __restrict__ p, int n) {
+  double acc = 0.0;
+  for (int base = (int)threadIdx.x; base < n; base += U * NT) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = base + u * NT;
+      v[u] = i < n ? p[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) acc += v[u];
+  }
+  return acc;
+}
+
 template <int ND, int FIN>
 __global__ void __launch_bounds__(kFinNT) k_finalize(const double* __restrict__ partials, int nparts,
                                                     KState* __restrict__ st) {
@@ -83,9 +102,7 @@ __global__ void __launch_bounds__(kFinNT) k_finalize(const double* __restrict__ 
   double s[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int d = 0; d < ND; d++) {
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kFinNT) acc += partials[(size_t)d * nparts + i];
-    s[d] = block_sum_n<kFinNT>(acc, sh);
+    s[d] = block_sum_n<kFinNT>(strided_sum<kFinNT>(partials + (size_t)d * nparts, nparts), sh);
   }
   if (threadIdx.x != 0) return;
   if constexpr (FIN == FIN_STORE) {
@@ -986,6 +1003,28 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
   if constexpr (DOTS) write_partials<3>(dsum, partials, blockIdx.x, gridDim.x);
 }
 
+// t -= sum over slots k0 <= k < k1 (in that order) of block(k) * v[col(k)]:
+// four slots' loads are issued before their products, so a lane waits for a
+// few memory round trips per row rather than one per slot.
+template <int B, typename TS, typename T>
+__device__ __forceinline__ void sell_msub_range(const int* __restrict__ col, const TS* __restrict__ val,
+                                                const T* v, int k0, int k1, int lane, T (&t)[B]) {
+  constexpr int BB = B * B;
+  int k = k0;
+  for (; k + 3 < k1; k += 4) {
+    T a[4][BB], x[4][B];
+#pragma unroll
+    for (int u = 0; u < 4; u++) sell_slot<B, TS, T>(col, val, v, k + u, lane, a[u], x[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++) blk_msub<B, T>(t, a[u], x[u]);
+  }
+  for (; k < k1; k++) {
+    T a[BB], x[B];
+    sell_slot<B, TS, T>(col, val, v, k, lane, a, x);
+    blk_msub<B, T>(t, a, x);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Triangular solves over one level set (ForwardSubstitution / BackwardSubstitution,
 // src/matvec/matdiafrac_blkcrs.cpp:332-663).  Lower: out_i = Dinv_i (rhs_i - sum L_ij out_j);
@@ -1004,20 +1043,7 @@ __device__ __forceinline__ void trsv_lower_slice(int s, int lane, const int* __r
   T t[B];
 #pragma unroll
   for (int r = 0; r < B; r++) t[r] = rhs[(size_t)row * B + r];
-  const int k0 = sptr[s], k1 = sptr[s + 1];
-  int k = k0;
-  for (; k + 1 < k1; k += 2) {
-    T a0[BB], v0[B], a1[BB], v1[B];
-    sell_slot<B, TS, T>(col, val, out, k, lane, a0, v0);
-    sell_slot<B, TS, T>(col, val, out, k + 1, lane, a1, v1);
-    blk_msub<B, T>(t, a0, v0);
-    blk_msub<B, T>(t, a1, v1);
-  }
-  if (k < k1) {
-    T a0[BB], v0[B];
-    sell_slot<B, TS, T>(col, val, out, k, lane, a0, v0);
-    blk_msub<B, T>(t, a0, v0);
-  }
+  sell_msub_range<B, TS, T>(col, val, out, sptr[s], sptr[s + 1], lane, t);
   T d[BB], y[B];
   load_block<BB, TS, T>(dval, (size_t)s, lane, d);
   blk_mul<B, T>(y, d, t);
@@ -1031,26 +1057,12 @@ __device__ __forceinline__ void trsv_upper_slice(int s, int lane, const int* __r
                                                  const int* __restrict__ sptr, const int* __restrict__ col,
                                                  const TS* __restrict__ val, T* vec, const T* __restrict__ rvec,
                                                  double& dsum) {
-  constexpr int BB = B * B;
   const int row = rows[(size_t)s * 64 + lane];
   if (row < 0) return;
   T t[B];
 #pragma unroll
   for (int r = 0; r < B; r++) t[r] = vec[(size_t)row * B + r];
-  const int k0 = sptr[s], k1 = sptr[s + 1];
-  int k = k0;
-  for (; k + 1 < k1; k += 2) {
-    T a0[BB], v0[B], a1[BB], v1[B];
-    sell_slot<B, TS, T>(col, val, vec, k, lane, a0, v0);
-    sell_slot<B, TS, T>(col, val, vec, k + 1, lane, a1, v1);
-    blk_msub<B, T>(t, a0, v0);
-    blk_msub<B, T>(t, a1, v1);
-  }
-  if (k < k1) {
-    T a0[BB], v0[B];
-    sell_slot<B, TS, T>(col, val, vec, k, lane, a0, v0);
-    blk_msub<B, T>(t, a0, v0);
-  }
+  sell_msub_range<B, TS, T>(col, val, vec, sptr[s], sptr[s + 1], lane, t);
 #pragma unroll
   for (int r = 0; r < B; r++) {
     vec[(size_t)row * B + r] = t[r];
@@ -1082,6 +1094,86 @@ __global__ void __launch_bounds__(kTrsvNT) k_trsv_upper(int s0, int s1, const in
   double dsum[1] = {0.0};
   if (s < s1) trsv_upper_slice<B, TS, T, DOT>(s, lane, rows, sptr, col, val, vec, rvec, dsum[0]);
   if constexpr (DOT) write_partials<1, kTrsvNT>(dsum, partials, part0 + blockIdx.x, nparts);
+}
+
+// Forward sweep of one level fused with the PCG update that precedes it
+// (solver_ls_iter.cpp:140-145, then :158): for each row of the level
+//   r_i += (-alpha) Ap_i ; x_i += alpha p_i ; z_i = Dinv_i (r_i - sum L_ij z_j).
+// Ap arrives in z.  A row's own r, x, p and Ap are all the update needs and the
+// substitution reads r only at its own row, so every row is updated just
+// before it is solved and the separate vector pass disappears; rows of later
+// levels still hold Ap in z, rows of earlier levels already hold their z.
+// Partial <r,r> of the level's rows go to prr[part0 + workgroup].  With RZ,
+// rows flagged in rzmask (those without U entries, which the backward sweep
+// leaves unchanged) also add <r,z> to prz[part0 + workgroup], so the backward
+// sweep need not visit them.
+template <int B, typename TS, typename T, bool RZ>
+__global__ void __launch_bounds__(kTrsvNT) k_trsv_lower_xr(int s0, int s1, const int* __restrict__ rows,
+                                                      const int* __restrict__ sptr, const int* __restrict__ col,
+                                                      const TS* __restrict__ val, const TS* __restrict__ dval,
+                                                      const T* __restrict__ p, T* __restrict__ r,
+                                                      T* __restrict__ x, T* z,
+                                                      const unsigned char* __restrict__ rzmask,
+                                                      double* __restrict__ prr, int part0,
+                                                      double* __restrict__ prz, const KState* __restrict__ st) {
+  if (st->done) return;
+  constexpr int BB = B * B;
+  const T a = (T)st->alpha, ma = (T)(-st->alpha);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = s0 + xcd_block() * kTrsvWaves + wave;
+  double drr[1] = {0.0}, drz[1] = {0.0};
+  const int row = s < s1 ? rows[(size_t)s * 64 + lane] : -1;
+  if (row >= 0) {
+    const bool rz = RZ && rzmask[(size_t)s * 64 + lane];
+    T t[B], rin[B];
+#pragma unroll
+    for (int e = 0; e < B; e++) {
+      const size_t o = (size_t)row * B + e;
+      const T rv = r[o] + ma * z[o];
+      r[o] = rv;
+      x[o] = x[o] + a * p[o];
+      drr[0] += (double)rv * (double)rv;
+      t[e] = rv;
+      rin[e] = rv;
+    }
+    sell_msub_range<B, TS, T>(col, val, z, sptr[s], sptr[s + 1], lane, t);
+    T d[BB], y[B];
+    load_block<BB, TS, T>(dval, (size_t)s, lane, d);
+    blk_mul<B, T>(y, d, t);
+#pragma unroll
+    for (int e = 0; e < B; e++) {
+      z[(size_t)row * B + e] = y[e];
+      if (rz) drz[0] += (double)rin[e] * (double)y[e];
+    }
+  }
+  write_partials<1, kTrsvNT>(drr, prr, part0 + blockIdx.x, 0);
+  if constexpr (RZ) write_partials<1, kTrsvNT>(drz, prz, part0 + blockIdx.x, 0);
+}
+
+// Finalize of a PCG iteration whose <r,r> and <r,z> partials were left by
+// different launches: the convergence test of solver_ls_iter.cpp:144-154 on
+// sum(prr), then beta = <r,z>_new / rz (:157-161) on sum(prz) unless the
+// iteration ended the solve.
+static __global__ void __launch_bounds__(kFinNT) k_finalize_rrz(const double* __restrict__ prr, int nrr,
+                                                        const double* __restrict__ prz, int nrz,
+                                                        KState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ double sh[kFinNT / 64];
+  const double rr = block_sum_n<kFinNT>(strided_sum<kFinNT>(prr, nrr), sh);
+  const double rz = block_sum_n<kFinNT>(strided_sum<kFinNT>(prz, nrz), sh);
+  if (threadIdx.x != 0) return;
+  st->rr = rr;
+  if (rr * st->inv_rr0 < st->tol2) {
+    st->done = 1;
+    st->conv_iter = st->iter;
+    st->conv_rr = rr;
+  }
+  st->iter += 1;
+  if (!st->done && st->iter >= st->maxit) st->done = 2;
+  if (st->done) return;
+  st->rz_new = rz;
+  st->beta = rz / st->rz;
+  st->rz = rz;
 }
 
 // ---------------------------------------------------------------------------
@@ -1209,6 +1301,29 @@ __global__ void k_convert(long n, const double* __restrict__ x, T* __restrict__ 
 // Plain copy used to calibrate the bandwidth ceiling and the HBM counters.
 static __global__ void __launch_bounds__(kNT) k_copy4(long n4, const float4* __restrict__ in, float4* __restrict__ out) {
   for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n4; i += (long)gridDim.x * kNT) out[i] = in[i];
+}
+// Streaming read of n4 16-byte words, four independent loads in flight per
+// lane; the sum is stored only on an impossible value so that the loads stay
+// live without a write stream.
+static __global__ void __launch_bounds__(kNT) k_read4(long n4, const float4* __restrict__ in, float* __restrict__ out) {
+  constexpr int U = 4;
+  float4 acc = {0.f, 0.f, 0.f, 0.f};
+  const long stride = (long)gridDim.x * kNT;
+  long i = (long)blockIdx.x * kNT + threadIdx.x;
+  for (; i + (U - 1) * stride < n4; i += U * stride) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = in[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      acc.x += v[u].x;
+      acc.y += v[u].y;
+      acc.z += v[u].z;
+      acc.w += v[u].w;
+    }
+  }
+  for (; i < n4; i += stride) acc.x += in[i].x;
+  if (acc.x + acc.y + acc.z + acc.w == 12345.f) out[0] = 1.f;
 }
 template <typename T>
 __global__ void k_fill(long n, T v, T* __restrict__ x) {

@@ -151,6 +151,9 @@ static int sweep_upper(const Sell<TS>& U, const Sell<double>& plan, T* vec, cons
   return DFM_OK;
 }
 
+// Partial sums a fused lower sweep (k_trsv_lower_xr) leaves: one per workgroup.
+static int lower_nparts(const Sell<double>& plan) { return upper_nparts(plan); }
+
 int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper) {
   switch (M->b) {
     case 1:
@@ -305,7 +308,14 @@ static void init_state(KState& h, double rr0, double tol, unsigned maxit, int it
   if (h.iter >= h.maxit) h.done = 2;
 }
 
-// One PCG iteration (solver_ls_iter.cpp:134-165) on the device.
+// One PCG iteration (solver_ls_iter.cpp:134-165) on the device:
+//   Ap (+ partial <p,Ap>) -> alpha -> forward sweep fused with the r, x update
+//   (+ partial <r,r>) -> backward sweep (+ partial <r,z>) -> convergence test
+//   and beta -> p = beta p + z.
+// The convergence test reads <r,r> after the sweeps rather than before them;
+// it sees the same value, and the sweeps of the final iteration are the only
+// extra work.  The first backward level (rows without U entries, which it
+// leaves unchanged) is not launched: the forward sweep adds their <r,z>.
 template <int B, typename TA, typename TF, typename T>
 static int pcg_body(dfm_ilu_s* M, const Ops<TA, TF>& o, T* r, T* x, T* p, T* z) {
   const long n = (long)o.nb * B;
@@ -314,11 +324,36 @@ static int pcg_body(dfm_ilu_s* M, const Ops<TA, TF>& o, T* r, T* x, T* p, T* z) 
   const int gv = grid_vec(n);
   DFM_TRY((spmv<B, TA, TF, T, true, true>(o, p, z, nullptr, p, nullptr, nullptr, part, st)));
   DFM_LAUNCH_FIN(1, FIN_PCG_PAP, part, spmv_nparts(*o.planA), st);
-  DFM_LAUNCH(k_pcg_update_xr<T>, gv, n, z, p, r, x, part, st);
-  DFM_LAUNCH_FIN(1, FIN_PCG_RR, part, gv, st);
-  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, r, z, st)));
-  DFM_TRY((sweep_upper<B, TF, T, true>(*o.U, *o.planU, z, r, part, st)));
-  DFM_LAUNCH_FIN(1, FIN_PCG_RZ, part, upper_nparts(*o.planU), st);
+  const Sell<double>& pl = *o.planL;
+  const Sell<double>& pu = *o.planU;
+  const int ngL = (int)pl.group_slice_ptr.size() - 1, ngU = (int)pu.group_slice_ptr.size() - 1;
+  const bool skip_u0 = M->upper0_empty;
+  const int nrr = lower_nparts(pl);
+  double* prr = part;
+  double* prz = part + nrr;
+  int part0 = 0;
+  for (int g = 0; g < ngL; g++) {
+    const int s0 = pl.group_slice_ptr[g], s1 = pl.group_slice_ptr[g + 1];
+    const int grid = grid_for(s1 - s0, kTrsvWaves);
+    if (skip_u0) {
+      DFM_LAUNCH_NT((k_trsv_lower_xr<B, TF, T, true>), grid, kTrsvNT, s0, s1, pl.rows.p, pl.slice_ptr.p, pl.col.p,
+                    o.L->val.p, o.L->dval.p, p, r, x, z, M->d_rzmaskL.p, prr, part0, prz, st);
+    } else {
+      DFM_LAUNCH_NT((k_trsv_lower_xr<B, TF, T, false>), grid, kTrsvNT, s0, s1, pl.rows.p, pl.slice_ptr.p,
+                    pl.col.p, o.L->val.p, o.L->dval.p, p, r, x, z, nullptr, prr, part0, nullptr, st);
+    }
+    part0 += grid;
+  }
+  int nrz = skip_u0 ? nrr : 0;
+  for (int g = skip_u0 ? 1 : 0; g < ngU; g++) {
+    const int s0 = pu.group_slice_ptr[g], s1 = pu.group_slice_ptr[g + 1];
+    const int grid = grid_for(s1 - s0, kTrsvWaves);
+    DFM_LAUNCH_NT((k_trsv_upper<B, TF, T, true>), grid, kTrsvNT, s0, s1, pu.rows.p, pu.slice_ptr.p, pu.col.p,
+                  o.U->val.p, z, r, prz, nrz, 0, st);
+    nrz += grid;
+  }
+  hipLaunchKernelGGL(k_finalize_rrz, dim3(1), dim3(kFinNT), 0, stream(), prr, nrr, prz, nrz, st);
+  DFM_HIP(hipGetLastError());
   DFM_LAUNCH(k_pcg_update_p<T>, gv, n, z, p, st);
   return DFM_OK;
 }
@@ -527,6 +562,7 @@ static int ensure_workspace(dfm_ilu_s* M, size_t bytes) {
   size_t need = 0;
   need = std::max<size_t>(need, (size_t)spmv_nparts(M->A64));
   need = std::max<size_t>(need, (size_t)upper_nparts(M->U64));
+  need = std::max<size_t>(need, (size_t)lower_nparts(M->L64));
   need = std::max<size_t>(need, (size_t)2048);
   need = 3 * need + 64;
   if (M->partials.n < need) DFM_TRY(M->partials.alloc(need));
@@ -737,6 +773,33 @@ int dfm_bench_copy(size_t bytes, int nrep, double* avg_ms) {
   DFM_HIP(hipEventCreate(&e1));
   DFM_HIP(hipEventRecord(e0, stream()));
   for (int r = 0; r < nrep; r++) DFM_LAUNCH(k_copy4, grid, n4, a.p, b.p);
+  DFM_HIP(hipEventRecord(e1, stream()));
+  DFM_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  DFM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = (double)ms / nrep;
+  return DFM_OK;
+}
+
+int dfm_bench_read(size_t bytes, int nrep, double* avg_ms) {
+  using namespace dfm;
+  DFM_CHECK(avg_ms && nrep > 0 && bytes >= 16, DFM_BAD_ARG, "bad arguments");
+  DFM_TRY(ensure_init());
+  const long n4 = (long)(bytes / 16);
+  DevBuf<float4> a;
+  DevBuf<float> out;
+  DFM_TRY(a.alloc(n4));
+  DFM_TRY(out.alloc(1));
+  DFM_HIP(hipMemsetAsync(a.p, 0, n4 * 16, stream()));
+  const int grid = (int)std::min<long>((n4 + kNT - 1) / kNT, 1 << 16);
+  DFM_LAUNCH(k_read4, grid, n4, a.p, out.p);
+  hipEvent_t e0, e1;
+  DFM_HIP(hipEventCreate(&e0));
+  DFM_HIP(hipEventCreate(&e1));
+  DFM_HIP(hipEventRecord(e0, stream()));
+  for (int r = 0; r < nrep; r++) DFM_LAUNCH(k_read4, grid, n4, a.p, out.p);
   DFM_HIP(hipEventRecord(e1, stream()));
   DFM_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
