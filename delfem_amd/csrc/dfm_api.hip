@@ -1,6 +1,7 @@
 // dfm_api.hip -- C-ABI of delfem_amd: runtime, CMatDia_BlkCrs, assembly and
 // ILU (symbolic on the host, numeric and substitutions on the device).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -140,6 +141,30 @@ int dfm_fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned
 
 // ---------------------------------------------------------------------------
 // MatVec::CMatDia_BlkCrs
+// Value versions are unique across all matrices, so a factor's mirrored copy
+// of A (tagged with the version it copied) is refreshed whenever it is given
+// a different matrix or changed values.
+static unsigned long long next_version() {
+  static std::atomic<unsigned long long> counter{1};
+  return ++counter;
+}
+
+// FNV-1a over the block-CRS pattern: a factor or copy built on one pattern
+// checks the matrix it is later given against it.
+static unsigned long long hash_pattern(const dfm_bcrs_s* A) {
+  unsigned long long h = 1469598103934665603ull;
+  auto mix = [&h](unsigned v) {
+    for (int k = 0; k < 4; k++) {
+      h ^= (v >> (8 * k)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  };
+  mix(A->nb);
+  for (unsigned v : A->rowptr) mix(v);
+  for (unsigned v : A->col) mix(v);
+  return h;
+}
+
 int dfm_bcrs_create(unsigned nblk, unsigned len, dfm_bcrs** out) {
   DFM_CHECK(out && len > 0 && len <= 8, DFM_BAD_ARG, "bad arguments to dfm_bcrs_create (len=%u)", len);
   DFM_TRY(ensure_init());
@@ -147,6 +172,8 @@ int dfm_bcrs_create(unsigned nblk, unsigned len, dfm_bcrs** out) {
   A->nb = nblk;
   A->b = len;
   A->rowptr.assign(nblk + 1, 0);
+  A->pattern_hash = hash_pattern(A);
+  A->version = next_version();
   int rc = A->d_rowptr.alloc(nblk + 1);
   if (rc == DFM_OK) rc = A->d_dia.alloc((size_t)nblk * len * len);
   if (rc == DFM_OK && nblk) {
@@ -220,6 +247,7 @@ int dfm_bcrs_add_pattern(dfm_bcrs* A, unsigned nrow, const unsigned* index, cons
   }
   A->rowptr.swap(rp);
   A->col.swap(cl);
+  A->pattern_hash = hash_pattern(A);
   std::vector<int> rpi(A->rowptr.begin(), A->rowptr.end()), cli(A->col.begin(), A->col.end());
   DFM_TRY(A->d_rowptr.upload(rpi));
   DFM_TRY(A->d_col.upload(cli));
@@ -228,7 +256,7 @@ int dfm_bcrs_add_pattern(dfm_bcrs* A, unsigned nrow, const unsigned* index, cons
   if (nval) DFM_HIP(hipMemsetAsync(A->d_val.p, 0, nval * sizeof(double), g_stream));
   DFM_HIP(hipMemsetAsync(A->d_dia.p, 0, A->d_dia.n * sizeof(double), g_stream));
   DFM_HIP(hipStreamSynchronize(g_stream));
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 
@@ -252,7 +280,7 @@ int dfm_bcrs_set_zero(dfm_bcrs* A) {
   if (A->d_val.n) DFM_HIP(hipMemsetAsync(A->d_val.p, 0, A->d_val.n * sizeof(double), g_stream));
   if (A->d_dia.n) DFM_HIP(hipMemsetAsync(A->d_dia.p, 0, A->d_dia.n * sizeof(double), g_stream));
   DFM_HIP(hipStreamSynchronize(g_stream));
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 
@@ -263,7 +291,7 @@ int dfm_bcrs_set_values(dfm_bcrs* A, const double* val, const double* dia) {
   if (dia && A->d_dia.n)
     DFM_HIP(hipMemcpyAsync(A->d_dia.p, dia, A->d_dia.n * sizeof(double), hipMemcpyHostToDevice, g_stream));
   DFM_HIP(hipStreamSynchronize(g_stream));
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 
@@ -281,7 +309,7 @@ int dfm_bcrs_device_values(dfm_bcrs* A, double** val, double** dia) {
   DFM_CHECK(A, DFM_BAD_ARG, "NULL matrix");
   if (val) *val = A->d_val.p;
   if (dia) *dia = A->d_dia.p;
-  A->version++;  // the caller may write through the pointers
+  A->version = next_version();  // the caller may write through the pointers
   return DFM_OK;
 }
 
@@ -319,7 +347,7 @@ int dfm_bcrs_mearge(dfm_bcrs* A, unsigned nblkel, const unsigned* blkel, unsigne
     DFM_HIP(hipMemcpyAsync(A->d_dia.p + (size_t)i * bb, d.data(), bb * sizeof(double), hipMemcpyHostToDevice, g_stream));
     DFM_HIP(hipStreamSynchronize(g_stream));
   }
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 
@@ -331,7 +359,7 @@ int dfm_bcrs_set_bc(dfm_bcrs* A, const int* flag) {
   const long ncrs = A->rowptr[A->nb];
   DFM_LAUNCH(k_bc_cols, grid_for(ncrs), ncrs, (int)A->b, A->d_col.p, A->d_val.p, f.p);
   DFM_HIP(hipStreamSynchronize(g_stream));
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 
@@ -458,7 +486,7 @@ int dfm_assemble_poisson2d_p1(dfm_asm* p, const double* coord, const double* alp
   DFM_LAUNCH(k_asm_poisson2d, grid_for(p->nnode), (int)p->nnode, p->d_conn.p, p->d_elsup_ptr.p, p->d_elsup.p,
              A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, coord, alpha, source, value, res);
   DFM_HIP(hipStreamSynchronize(g_stream));
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 int dfm_assemble_solid3d_p1(dfm_asm* p, const double* coord, const double* lambda, const double* myu, double rho,
@@ -469,7 +497,7 @@ int dfm_assemble_solid3d_p1(dfm_asm* p, const double* coord, const double* lambd
   DFM_LAUNCH(k_asm_solid3d, grid_for(p->nnode), (int)p->nnode, p->d_conn.p, p->d_elsup_ptr.p, p->d_elsup.p,
              A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, coord, lambda, myu, rho, gx, gy, gz, disp, res);
   DFM_HIP(hipStreamSynchronize(g_stream));
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 int dfm_assemble_oseen2d_p1p1(dfm_asm* p, const double* coord, double myu, double rho, const double* adv,
@@ -481,7 +509,7 @@ int dfm_assemble_oseen2d_p1p1(dfm_asm* p, const double* coord, double myu, doubl
   DFM_LAUNCH(k_asm_oseen2d, grid_for(p->nnode), (int)p->nnode, p->d_conn.p, p->d_elsup_ptr.p, p->d_elsup.p,
              A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, coord, myu, rho, adv, value, res);
   DFM_HIP(hipStreamSynchronize(g_stream));
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 int dfm_assemble_helmholtz2d_p1(dfm_asm* p, const double* coord, const double* alpha, const double* cmass,
@@ -492,7 +520,7 @@ int dfm_assemble_helmholtz2d_p1(dfm_asm* p, const double* coord, const double* a
   DFM_LAUNCH(k_asm_helmholtz2d, grid_for(p->nnode), (int)p->nnode, p->d_conn.p, p->d_elsup_ptr.p, p->d_elsup.p,
              A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, coord, alpha, cmass, source, value, res);
   DFM_HIP(hipStreamSynchronize(g_stream));
-  A->version++;
+  A->version = next_version();
   return DFM_OK;
 }
 
@@ -526,7 +554,7 @@ int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* ord
   M->b = b;
   M->lev_fill = lev_fill;
   M->ordering = ordering;
-  M->A_pattern_owner = A;
+  M->pattern_hash = A->pattern_hash;
   if (ordering == DFM_ORDER_GIVEN) {
     M->order.assign(order, order + nb);
     std::vector<int> seen(nb, 0);
@@ -678,7 +706,7 @@ int dfm_ilu_get_pattern(const dfm_ilu* M, unsigned* rowptr, unsigned* col, int* 
 
 int dfm_ilu_set_value(dfm_ilu* M, dfm_bcrs* A, int* nsingular) {
   DFM_CHECK(M && A && A->nb == M->nb && A->b == M->b, DFM_BAD_ARG, "factor and matrix do not match");
-  DFM_CHECK(A == M->A_pattern_owner || A->rowptr == M->A_pattern_owner->rowptr, DFM_BAD_ARG,
+  DFM_CHECK(A->pattern_hash == M->pattern_hash, DFM_BAD_ARG,
             "matrix pattern differs from the one the factor was built on");
   const int nb = (int)M->nb, b = (int)M->b, bb = b * b;
   const long nf = (long)M->fcol.size();

@@ -145,7 +145,8 @@ struct dfm_bcrs_s {
   std::vector<unsigned> rowptr, col;   // host pattern (CRS without diagonal)
   dfm::DevBuf<int> d_rowptr, d_col;
   dfm::DevBuf<double> d_val, d_dia;
-  unsigned long long version = 1;       // bumped whenever values change
+  unsigned long long version = 0;       // unique across matrices, renewed whenever values change
+  unsigned long long pattern_hash = 0;  // FNV-1a of (nb, rowptr, col), refreshed when the pattern changes
 };
 
 struct dfm_asm_s {
@@ -187,7 +188,7 @@ struct dfm_ilu_s {
   dfm::Sell<float> A32;
   unsigned long long a_version = 0;     // version of A mirrored in A64 / A32
   bool a32_valid = false;
-  const dfm_bcrs_s* A_pattern_owner = nullptr;
+  unsigned long long pattern_hash = 0;  // pattern of the matrix the factor was built on
   bool factored = false;
   // Krylov workspace (factor numbering)
   dfm::DevBuf<char> work;               // Krylov vectors (typed views carved per solve)

@@ -659,6 +659,8 @@ static int solve_entry(dfm_bcrs_s* A, dfm_ilu_s* M, bool bicg, double* r, double
                        unsigned* iteration, dfm_solve_opts* opts) {
   DFM_CHECK(A && M && r && x && conv_ratio && iteration, DFM_BAD_ARG, "bad arguments to the solver");
   DFM_CHECK(A->nb == M->nb && A->b == M->b, DFM_BAD_ARG, "factor and matrix do not match");
+  DFM_CHECK(A->pattern_hash == M->pattern_hash, DFM_BAD_ARG,
+            "matrix pattern differs from the one the factor was built on");
   DFM_CHECK(M->factored, DFM_BAD_ARG, "factor has no values (call dfm_ilu_set_value)");
   DFM_CHECK(!opts || (opts->precision >= DFM_PREC_FP64 && opts->precision <= DFM_PREC_FAC16), DFM_BAD_ARG,
             "unknown precision %d", opts ? opts->precision : 0);

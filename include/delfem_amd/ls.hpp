@@ -165,7 +165,9 @@ inline bool solve(bool bicg, double& conv_ratio, unsigned& iteration, CLinearSys
   const int rc = (bicg ? dfm_solve_pbicgstab : dfm_solve_pcg)(
       lsp.ls.GetMatrix().handle(), lsp.prec.handle(), lsp.ls.GetResidual().device_ptr(),
       lsp.ls.GetUpdate().device_ptr(), &conv_ratio, &iteration, &o);
-  return check(rc, bicg ? "Solve_PBiCGSTAB" : "Solve_PCG") == DFM_OK;
+  // Solve_PCG is false at max_iter; Solve_PBiCGSTAB is true either way
+  // (solver_ls_iter.cpp:168-171, 407)
+  return check(rc, bicg ? "Solve_PBiCGSTAB" : "Solve_PCG") == DFM_OK || bicg;
 }
 }  // namespace detail
 

@@ -7,6 +7,9 @@
 //   MatVec::CBCFlag            include/delfem/matvec/bcflag_blk.h:36-140
 //   MatVec::CMatDia_BlkCrs     include/delfem/matvec/matdia_blkcrs.h:44-118
 //   MatVec::CMatDiaFrac_BlkCrs include/delfem/matvec/matdiafrac_blkcrs.h:40-137
+//   MatVec::CPrecond_Blk       include/delfem/matvec/matprecond_blk.h:37-41
+//   MatVec::Sol::Solve_CG / Solve_PCG / Solve_BiCGSTAB / Solve_PBiCGSTAB
+//                              include/delfem/matvec/solver_mat_iter.h:41-60
 //
 // The classes are declared in namespace delfem_amd::MatVec; unless
 // DELFEM_AMD_NO_ALIASES is defined, namespace MatVec aliases it so that code
@@ -16,6 +19,8 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -64,6 +69,8 @@ class CVector_Blk {
     check(dfm_vec_axpy(size(), alpha, rhs.v_, v_), "CVector_Blk::AXPY");
     return *this;
   }
+  // operator+= (vector_blk.cpp:73-81): y + 1.0 * x is y + x exactly
+  CVector_Blk& operator+=(const CVector_Blk& rhs) { return AXPY(1.0, rhs); }
   void SetVectorZero() { check(dfm_vec_fill(size(), 0.0, v_), "CVector_Blk::SetVectorZero"); }
   double GetSquaredVectorNorm() const { return (*this) * (*this); }
   unsigned NBlk() const { return nblk_; }
@@ -167,21 +174,32 @@ class CMatDia_BlkCrs {
   dfm_bcrs* h_ = nullptr;
 };
 
-class CMatDiaFrac_BlkCrs {
+// Abstract preconditioner of the matrix layer (matprecond_blk.h:37-41).
+class CPrecond_Blk {
  public:
+  virtual ~CPrecond_Blk() {}
+  virtual bool SolvePrecond(const CMatDia_BlkCrs& mat, CVector_Blk& vec) const = 0;
+};
+
+class CMatDiaFrac_BlkCrs : public CPrecond_Blk {
+ public:
+  // CMatDiaFrac_BlkCrs(A): ILU(0) pattern of A
+  explicit CMatDiaFrac_BlkCrs(const CMatDia_BlkCrs& A) : CMatDiaFrac_BlkCrs(0, A) {}
   // CMatDiaFrac_BlkCrs(lev, A): symbolic ILU(lev) of A's pattern (lev = -1: unlimited)
-  CMatDiaFrac_BlkCrs(int lev_fill, const CMatDia_BlkCrs& A) {
+  CMatDiaFrac_BlkCrs(int lev_fill, const CMatDia_BlkCrs& A) : src_(&A) {
     check(dfm_ilu_create(A.handle(), lev_fill, DFM_ORDER_NATURAL, nullptr, 0, &h_), "CMatDiaFrac_BlkCrs");
   }
   // ordered variant, order[new] = old
-  CMatDiaFrac_BlkCrs(int lev_fill, const CMatDia_BlkCrs& A, const std::vector<int>& order) {
+  CMatDiaFrac_BlkCrs(int lev_fill, const CMatDia_BlkCrs& A, const std::vector<int>& order)
+      : src_(&A), ordering_(DFM_ORDER_GIVEN) {
     check(dfm_ilu_create(A.handle(), lev_fill, DFM_ORDER_GIVEN, order.data(), 0, &h_), "CMatDiaFrac_BlkCrs");
   }
   // device orderings: DFM_ORDER_MULTICOLOUR, DFM_ORDER_BLOCK_MULTICOLOUR
-  CMatDiaFrac_BlkCrs(int lev_fill, const CMatDia_BlkCrs& A, int ordering, unsigned block_nodes = 0) {
+  CMatDiaFrac_BlkCrs(int lev_fill, const CMatDia_BlkCrs& A, int ordering, unsigned block_nodes = 0)
+      : src_(&A), ordering_(ordering) {
     check(dfm_ilu_create(A.handle(), lev_fill, ordering, nullptr, block_nodes, &h_), "CMatDiaFrac_BlkCrs");
   }
-  ~CMatDiaFrac_BlkCrs() {
+  ~CMatDiaFrac_BlkCrs() override {
     if (h_) dfm_ilu_destroy(h_);
   }
   CMatDiaFrac_BlkCrs(const CMatDiaFrac_BlkCrs&) = delete;
@@ -198,6 +216,20 @@ class CMatDiaFrac_BlkCrs {
   bool BackwardSubstitution(CVector_Blk& v) const {
     return check(dfm_ilu_backward(h_, v.device_ptr()), "BackwardSubstitution") == DFM_OK;
   }
+  // Solve (matdiafrac_blkcrs.h:60-64): forward then backward substitution in
+  // the factor's own numbering
+  bool Solve(CVector_Blk& v) const {
+    ForwardSubstitution(v);
+    BackwardSubstitution(v);
+    return true;
+  }
+  // CPrecond_Blk interface (matdiafrac_blkcrs.h:69-71): the matrix argument is not used
+  bool SolvePrecond(const CMatDia_BlkCrs& mat, CVector_Blk& v) const override {
+    (void)mat;
+    return Solve(v);
+  }
+  const CMatDia_BlkCrs* source() const { return src_; }
+  int ordering() const { return ordering_; }
   unsigned NCrs() const {
     unsigned n = 0;
     int a, b, c;
@@ -207,9 +239,209 @@ class CMatDiaFrac_BlkCrs {
   dfm_ilu* handle() const { return h_; }
 
  private:
+  const CMatDia_BlkCrs* src_ = nullptr;
+  int ordering_ = DFM_ORDER_NATURAL;
   dfm_ilu* h_ = nullptr;
 };
 
+// ---------------------------------------------------------------------------
+// Matrix-layer Krylov solvers (src/matvec/solver_mat_iter.cpp).  They solve
+// mat * u = r from u = 0; r is overwritten with the final residual.  With a
+// natural-order CMatDiaFrac_BlkCrs built on `mat` as the preconditioner the
+// preconditioned solvers run the library's device-resident solver; any other
+// CPrecond_Blk (or an ordered factor, whose Solve works in its own numbering)
+// runs the reference's loop over device vectors, calling SolvePrecond.
+namespace Sol {
+namespace detail {
+inline const CMatDiaFrac_BlkCrs* device_factor(const CMatDia_BlkCrs& mat, const CPrecond_Blk& precond) {
+  const auto* f = dynamic_cast<const CMatDiaFrac_BlkCrs*>(&precond);
+  return f && f->source() == &mat && f->ordering() == DFM_ORDER_NATURAL ? f : nullptr;
+}
+}  // namespace detail
+
+// Solve_CG (solver_mat_iter.cpp:37-109).  As in the reference, the
+// convergence test reads the residual norm from before the current step, and
+// a converged return leaves that step out of u (it is already in r).
+inline bool Solve_CG(double& conv_ratio, unsigned& iteration, const CMatDia_BlkCrs& mat, CVector_Blk& r_vec,
+                     CVector_Blk& u_vec) {
+  const double tol = conv_ratio;
+  const unsigned mx_iter = iteration;
+  const unsigned nblk = r_vec.NBlk(), len = r_vec.Len();
+  u_vec.SetVectorZero();
+  double rr = r_vec * r_vec;
+  if (rr < 1.0e-30) {
+    conv_ratio = 0.0;
+    iteration = 0;
+    return true;
+  }
+  const double inv_rr0 = 1.0 / rr;
+  CVector_Blk p_vec(nblk, len), Ap_vec(nblk, len);
+  p_vec = r_vec;
+  iteration = mx_iter;
+  for (unsigned it = 1; it < mx_iter; it++) {
+    mat.MatVec(1.0, p_vec, 0.0, Ap_vec);
+    const double alpha = rr / (p_vec * Ap_vec);
+    r_vec.AXPY(-alpha, Ap_vec);
+    const double rr1 = r_vec * r_vec;
+    if (rr * inv_rr0 < tol * tol) {
+      conv_ratio = std::sqrt(rr * inv_rr0);
+      iteration = it;
+      return true;
+    }
+    u_vec.AXPY(alpha, p_vec);
+    const double beta = rr1 / rr;
+    rr = rr1;
+    p_vec *= beta;
+    p_vec += r_vec;
+  }
+  return true;
+}
+
+// Solve_PCG (solver_mat_iter.cpp:111-184): true when converged (conv_ratio =
+// ||r||/||r0||, iteration = 0-based index); false at max_iter with both left
+// at their inputs.
+inline bool Solve_PCG(double& conv_ratio, unsigned& iteration, const CMatDia_BlkCrs& mat, CVector_Blk& r_vec,
+                      CVector_Blk& u_vec, const CPrecond_Blk& precond, const CMatDia_BlkCrs& mat_p) {
+  const double tol = conv_ratio;
+  const unsigned mx_iter = iteration;
+  if (const CMatDiaFrac_BlkCrs* f = detail::device_factor(mat, precond)) {
+    const int rc = check(dfm_solve_pcg(mat.handle(), f->handle(), r_vec.device_ptr(), u_vec.device_ptr(),
+                                       &conv_ratio, &iteration, nullptr),
+                         "Sol::Solve_PCG");
+    if (rc == DFM_OK) return true;
+    conv_ratio = tol;
+    iteration = mx_iter;
+    return false;
+  }
+  const unsigned nblk = r_vec.NBlk(), len = r_vec.Len();
+  u_vec.SetVectorZero();
+  const double rr0 = r_vec * r_vec;
+  if (rr0 < 1.0e-30) {
+    conv_ratio = 0.0;
+    iteration = 0;
+    return true;
+  }
+  const double inv_rr0 = 1.0 / rr0;
+  CVector_Blk p_vec(nblk, len), z_vec(nblk, len);
+  z_vec = r_vec;
+  precond.SolvePrecond(mat_p, z_vec);
+  p_vec = z_vec;
+  for (unsigned it = 0; it < mx_iter; it++) {
+    const double rz = r_vec * z_vec;
+    mat.MatVec(1.0, p_vec, 0.0, z_vec);
+    const double alpha = rz / (p_vec * z_vec);
+    r_vec.AXPY(-alpha, z_vec);
+    u_vec.AXPY(alpha, p_vec);
+    const double rr = r_vec * r_vec;
+    if (rr * inv_rr0 < tol * tol) {
+      conv_ratio = std::sqrt(rr * inv_rr0);
+      iteration = it;
+      return true;
+    }
+    z_vec = r_vec;
+    precond.SolvePrecond(mat_p, z_vec);
+    const double beta = (r_vec * z_vec) / rz;
+    p_vec *= beta;
+    p_vec += z_vec;
+  }
+  return false;
+}
+
+namespace detail {
+// Solve_BiCGSTAB / Solve_PBiCGSTAB loop (solver_mat_iter.cpp:190-288, 294-423);
+// precond == nullptr is the unpreconditioned form.  Always true; at max_iter
+// conv_ratio keeps its input and iteration = max_iter.
+inline bool bicgstab_loop(double& conv_ratio, unsigned& iteration, const CMatDia_BlkCrs& mat, CVector_Blk& r_vec,
+                          CVector_Blk& u_vec, const CPrecond_Blk* precond) {
+  const double tol = conv_ratio;
+  const unsigned mx_iter = iteration;
+  const unsigned nblk = r_vec.NBlk(), len = r_vec.Len();
+  u_vec.SetVectorZero();
+  const double rr0 = r_vec * r_vec;
+  if (rr0 < 1.0e-30) {
+    conv_ratio = 0.0;
+    iteration = 0;
+    return true;
+  }
+  const double inv_rr0 = 1.0 / rr0;
+  CVector_Blk s_vec(nblk, len), p_vec(nblk, len), AMp_vec(nblk, len), AMs_vec(nblk, len), r0_vec(nblk, len);
+  std::unique_ptr<CVector_Blk> Mp_vec, Ms_vec;
+  if (precond) {
+    Mp_vec.reset(new CVector_Blk(nblk, len));
+    Ms_vec.reset(new CVector_Blk(nblk, len));
+  }
+  r0_vec = r_vec;
+  p_vec = r_vec;
+  iteration = mx_iter;
+  for (unsigned it = 1; it < mx_iter; it++) {
+    CVector_Blk* mp = &p_vec;
+    if (precond) {
+      *Mp_vec = p_vec;
+      precond->SolvePrecond(mat, *Mp_vec);
+      mp = Mp_vec.get();
+    }
+    const double r_r0 = r_vec * r0_vec;
+    mat.MatVec(1.0, *mp, 0.0, AMp_vec);
+    const double alpha = r_r0 / (AMp_vec * r0_vec);
+    s_vec = r_vec;
+    s_vec.AXPY(-alpha, AMp_vec);
+    CVector_Blk* ms = &s_vec;
+    if (precond) {
+      *Ms_vec = s_vec;
+      precond->SolvePrecond(mat, *Ms_vec);
+      ms = Ms_vec.get();
+    }
+    mat.MatVec(1.0, *ms, 0.0, AMs_vec);
+    const double den = AMs_vec * AMs_vec;
+    const double omega = (s_vec * AMs_vec) / den;
+    u_vec.AXPY(alpha, *mp);
+    u_vec.AXPY(omega, *ms);
+    r_vec = s_vec;
+    r_vec.AXPY(-omega, AMs_vec);
+    const double rr = r_vec * r_vec;
+    if (rr * inv_rr0 < tol * tol) {
+      conv_ratio = std::sqrt(rr * inv_rr0);
+      iteration = it;
+      return true;
+    }
+    const double beta = (r_vec * r0_vec) * alpha / (r_r0 * omega);
+    p_vec *= beta;
+    p_vec += r_vec;
+    p_vec.AXPY(-beta * omega, AMp_vec);
+  }
+  return true;
+}
+}  // namespace detail
+
+// Solve_BiCGSTAB (solver_mat_iter.cpp:190-288)
+inline bool Solve_BiCGSTAB(double& conv_ratio, unsigned& iteration, const CMatDia_BlkCrs& mat, CVector_Blk& r_vec,
+                           CVector_Blk& u_vec) {
+  return detail::bicgstab_loop(conv_ratio, iteration, mat, r_vec, u_vec, nullptr);
+}
+
+// Solve_PBiCGSTAB (solver_mat_iter.cpp:294-423).  As in the reference the
+// preconditioner is applied with `mat`; mat_p is not used.
+inline bool Solve_PBiCGSTAB(double& conv_ratio, unsigned& iteration, const CMatDia_BlkCrs& mat,
+                            CVector_Blk& r_vec, CVector_Blk& u_vec, const CPrecond_Blk& precond,
+                            const CMatDia_BlkCrs& mat_p) {
+  (void)mat_p;
+  if (const CMatDiaFrac_BlkCrs* f = detail::device_factor(mat, precond)) {
+    // the matrix layer returns early below 1e-30 (the ls layer: 1e-60)
+    if (r_vec * r_vec < 1.0e-30) {
+      u_vec.SetVectorZero();
+      conv_ratio = 0.0;
+      iteration = 0;
+      return true;
+    }
+    check(dfm_solve_pbicgstab(mat.handle(), f->handle(), r_vec.device_ptr(), u_vec.device_ptr(), &conv_ratio,
+                              &iteration, nullptr),
+          "Sol::Solve_PBiCGSTAB");
+    return true;
+  }
+  return detail::bicgstab_loop(conv_ratio, iteration, mat, r_vec, u_vec, &precond);
+}
+
+}  // namespace Sol
 }  // namespace MatVec
 }  // namespace delfem_amd
 

@@ -23,6 +23,8 @@
  *   orc_ilu_backward     golden fixtures (bitwise) + dense triangular solve
  *   orc_solve_pcg        golden fixtures (iterations, conv_ratio, solution) + scipy direct solve
  *   orc_solve_pbicgstab  golden fixtures (iterations, conv_ratio, solution) + scipy direct solve
+ *   orc_solve_mat        golden fixtures (iterations, conv_ratio, solution, final residual) + textbook
+ *                        CG / BiCGSTAB in numpy (iterates, the stale-norm test of Solve_CG)
  *   orc_emat_*           golden fixtures (bitwise) + Laplacian row sums, rigid-body null space;
  *                        the Oseen element has no compiled counterpart: pinned by
  *                        closed-form terms and invariants (parity unpinned against the reference)
@@ -801,6 +803,197 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
 done:
     free(vs); free(Ms); free(AMs); free(p); free(Mp); free(AMp); free(r2); sys_free(&s);
     return 1;
+}
+
+/* a17: the matrix-layer twins MatVec::Sol::Solve_CG / Solve_PCG /
+ * Solve_BiCGSTAB / Solve_PBiCGSTAB (src/matvec/solver_mat_iter.cpp:37-109,
+ * 111-184, 190-288, 294-423).  They drive CVector_Blk directly, whose AXPY
+ * (src/matvec/vector_blk.cpp:83-93) has no alpha == 0 shortcut, and the
+ * preconditioner is CMatDiaFrac_BlkCrs::SolvePrecond = Solve = forward then
+ * backward substitution in the factor's own numbering
+ * (include/delfem/matvec/matdiafrac_blkcrs.h:60-70).  kind: 0 Solve_CG,
+ * 1 Solve_PCG, 2 Solve_BiCGSTAB, 3 Solve_PBiCGSTAB (kinds 0 and 2 ignore the
+ * factor arguments, which may be NULL).  Returns the reference's bool. */
+static void cv_axpy(size_t n, double alpha, const double *x, double *y)
+{
+    for (size_t i = 0; i < n; i++) y[i] += alpha * x[i];
+}
+static void cv_add(size_t n, const double *x, double *y)
+{
+    for (size_t i = 0; i < n; i++) y[i] += x[i];
+}
+static void mat_precond(orc_sys *s, double *z)
+{
+    orc_ilu_forward(s->nb, s->b, s->frowptr, s->fcol, s->fdiaind, s->fval, s->fdia, z);
+    orc_ilu_backward(s->nb, s->b, s->frowptr, s->fcol, s->fdiaind, s->fval, s->fdia, z);
+}
+static void mat_mv(orc_sys *s, const double *x, double *y)
+{
+    orc_matvec(s->nb, s->b, s->rowptr, s->col, s->val, s->dia, 1.0, x, 0.0, y);
+}
+
+/* Solve_CG (:37-109).  The convergence test reads the squared norm from
+ * before this iteration's residual update, and a converged return skips this
+ * iteration's solution update. */
+static int mat_cg(orc_sys *s, double *r, double *u, double *conv_ratio, unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = (size_t)s->nb * s->b;
+    v_scal(n, 0.0, u);
+    double rr = v_dot(n, r, r);
+    if (rr < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; return 1; }
+    const double inv_rr0 = 1.0 / rr;
+    double *p = (double *)malloc(n * sizeof(double));
+    double *Ap = (double *)calloc(n, sizeof(double));
+    memcpy(p, r, n * sizeof(double));
+    *iteration = mx_iter;
+    for (unsigned it = 1; it < mx_iter; it++) {
+        mat_mv(s, p, Ap);
+        const double alpha = rr / v_dot(n, p, Ap);
+        cv_axpy(n, -alpha, Ap, r);
+        const double rr1 = v_dot(n, r, r);
+        if (rr * inv_rr0 < tol * tol) {
+            *conv_ratio = sqrt(rr * inv_rr0);
+            *iteration = it;
+            break;
+        }
+        cv_axpy(n, alpha, p, u);
+        const double beta = rr1 / rr;
+        rr = rr1;
+        v_scal(n, beta, p);
+        cv_add(n, r, p);
+    }
+    free(p); free(Ap);
+    return 1;
+}
+
+/* Solve_PCG (:111-184): the ls-layer loop with <r,z> formed at the top of
+ * every iteration; at max_iter conv_ratio and iteration keep their inputs. */
+static int mat_pcg(orc_sys *s, double *r, double *u, double *conv_ratio, unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = (size_t)s->nb * s->b;
+    v_scal(n, 0.0, u);
+    const double rr0 = v_dot(n, r, r);
+    if (rr0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; return 1; }
+    const double inv_rr0 = 1.0 / rr0;
+    double *p = (double *)malloc(n * sizeof(double));
+    double *z = (double *)malloc(n * sizeof(double));
+    int ret = 0;
+    memcpy(z, r, n * sizeof(double));
+    mat_precond(s, z);
+    memcpy(p, z, n * sizeof(double));
+    for (unsigned it = 0; it < mx_iter; it++) {
+        const double rz = v_dot(n, r, z);
+        mat_mv(s, p, z);
+        const double alpha = rz / v_dot(n, p, z);
+        cv_axpy(n, -alpha, z, r);
+        cv_axpy(n, alpha, p, u);
+        const double rr = v_dot(n, r, r);
+        if (rr * inv_rr0 < tol * tol) {
+            *conv_ratio = sqrt(rr * inv_rr0);
+            *iteration = it;
+            ret = 1;
+            break;
+        }
+        memcpy(z, r, n * sizeof(double));
+        mat_precond(s, z);
+        const double beta = v_dot(n, r, z) / rz;
+        v_scal(n, beta, p);
+        cv_add(n, z, p);
+    }
+    free(p); free(z);
+    return ret;
+}
+
+/* Solve_BiCGSTAB (:190-288) and Solve_PBiCGSTAB (:294-423); always true, at
+ * max_iter conv_ratio keeps its input and iteration = max_iter. */
+static int mat_bicgstab(orc_sys *s, int precondition, double *r, double *u, double *conv_ratio,
+                        unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = (size_t)s->nb * s->b;
+    double *vs = (double *)calloc(n, sizeof(double));
+    double *Ms = (double *)calloc(n, sizeof(double));
+    double *AMs = (double *)calloc(n, sizeof(double));
+    double *p = (double *)calloc(n, sizeof(double));
+    double *Mp = (double *)calloc(n, sizeof(double));
+    double *AMp = (double *)calloc(n, sizeof(double));
+    double *r0 = (double *)calloc(n, sizeof(double));
+    v_scal(n, 0.0, u);
+    double inv_rr0;
+    {
+        const double rr0 = v_dot(n, r, r);
+        if (rr0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; goto done; }
+        inv_rr0 = 1.0 / rr0;
+    }
+    memcpy(r0, r, n * sizeof(double));
+    memcpy(p, r, n * sizeof(double));
+    *iteration = mx_iter;
+    for (unsigned it = 1; it < mx_iter; it++) {
+        /* unpreconditioned: M = I, so Mp is p and Ms is s */
+        const double *mp = p;
+        if (precondition) {
+            memcpy(Mp, p, n * sizeof(double));
+            mat_precond(s, Mp);
+            mp = Mp;
+        }
+        const double r_r0 = v_dot(n, r, r0);
+        mat_mv(s, mp, AMp);
+        const double alpha = r_r0 / v_dot(n, AMp, r0);
+        memcpy(vs, r, n * sizeof(double));
+        cv_axpy(n, -alpha, AMp, vs);
+        const double *ms = vs;
+        if (precondition) {
+            memcpy(Ms, vs, n * sizeof(double));
+            mat_precond(s, Ms);
+            ms = Ms;
+        }
+        mat_mv(s, ms, AMs);
+        const double omega = v_dot(n, vs, AMs) / v_dot(n, AMs, AMs);
+        cv_axpy(n, alpha, mp, u);
+        cv_axpy(n, omega, ms, u);
+        memcpy(r, vs, n * sizeof(double));
+        cv_axpy(n, -omega, AMs, r);
+        {
+            const double rr = v_dot(n, r, r);
+            const double sq = rr * inv_rr0;
+            if (sq < tol * tol) {
+                *conv_ratio = sqrt(rr * inv_rr0);
+                *iteration = it;
+                goto done;
+            }
+        }
+        const double beta = v_dot(n, r, r0) * alpha / (r_r0 * omega);
+        v_scal(n, beta, p);
+        cv_add(n, r, p);
+        cv_axpy(n, -beta * omega, AMp, p);
+    }
+done:
+    free(vs); free(Ms); free(AMs); free(p); free(Mp); free(AMp); free(r0);
+    return 1;
+}
+
+int orc_solve_mat(int kind, unsigned nb, unsigned b,
+                  const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                  const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                  const double *fval, const double *fdia,
+                  double *r, double *u, double *conv_ratio, unsigned *iteration)
+{
+    orc_sys s;
+    sys_init(&s, nb, b, rowptr, col, val, dia, frowptr, fcol, fdiaind, fval, fdia, NULL);
+    int ret;
+    switch (kind) {
+        case 0: ret = mat_cg(&s, r, u, conv_ratio, iteration); break;
+        case 1: ret = mat_pcg(&s, r, u, conv_ratio, iteration); break;
+        case 2: ret = mat_bicgstab(&s, 0, r, u, conv_ratio, iteration); break;
+        default: ret = mat_bicgstab(&s, 1, r, u, conv_ratio, iteration); break;
+    }
+    sys_free(&s);
+    return ret;
 }
 
 /* ------------------------------------------------------------------------- */

@@ -91,6 +91,14 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
                         const double *fval, const double *fdia, const int *order,
                         double *r, double *x, double *conv_ratio, unsigned *iteration);
 
+/* a17: MatVec::Sol::Solve_CG (kind 0) / Solve_PCG (1) / Solve_BiCGSTAB (2) /
+ * Solve_PBiCGSTAB (3), src/matvec/solver_mat_iter.cpp; natural-order factor. */
+int orc_solve_mat(int kind, unsigned nb, unsigned b,
+                  const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                  const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                  const double *fval, const double *fdia,
+                  double *r, double *u, double *conv_ratio, unsigned *iteration);
+
 /* a4: element kernels (per element arrays). */
 void orc_emat_poisson2d_p1(unsigned nelem, const double *coord, const unsigned *conn,
                            const double *alpha, double source, const double *value,

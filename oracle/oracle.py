@@ -71,6 +71,9 @@ class Orc:
             f.restype = C.c_int
             f.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                           C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_solve_mat.restype = C.c_int
+        L.orc_solve_mat.argtypes = [C.c_int, C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_emat_poisson2d_p1.argtypes = [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]
         L.orc_emat_solid3d_p1.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                           C.c_double, vp, vp, vp]
@@ -182,6 +185,22 @@ class Orc:
                C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value, x, r
 
+    MAT_KINDS = {"cg": 0, "pcg": 1, "bicgstab": 2, "pbicgstab": 3}
+
+    def solve_mat(self, kind, nb, b, rowptr, col, val, dia, rhs, tol, maxit, F=None, fval=None, fdia=None):
+        """MatVec::Sol::Solve_CG / Solve_PCG / Solve_BiCGSTAB / Solve_PBiCGSTAB
+        (natural-order factor for the preconditioned kinds): returns
+        (ok, conv, iter, u, r_final)."""
+        r = f64(rhs).copy()
+        u = np.zeros(nb * b)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        fp = (None, None, None, None, None) if F is None else (u32(F["rowptr"]), u32(F["col"]), u32(F["diaind"]),
+                                                             f64(fval), f64(fdia))
+        ok = self.L.orc_solve_mat(self.MAT_KINDS[kind], nb, b, _p(u32(rowptr)), _p(u32(col)), _p(f64(val)),
+                                  _p(f64(dia)), *[_p(a) for a in fp], _p(r), _p(u), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
+
     # -- elements ------------------------------------------------------------
     def emat_poisson2d(self, coord, conn, alpha, source, value):
         conn = u32(conn)
@@ -283,6 +302,7 @@ class Ref:
             "ref_frac_forward": (None, [vp, vp]),
             "ref_frac_backward": (None, [vp, vp]),
             "ref_frac_solve": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_mat_solve": (C.c_int, [vp, vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_emat_poisson2d_p1": (None, [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]),
             "ref_emat_solid3d_p1": (None, [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                            C.c_double, vp, vp, vp]),
@@ -349,6 +369,18 @@ class RefSystem:
 
     def resize_tmp(self, n):
         self.ref.L.ref_sys_resize_tmp(self.h, n)
+
+    def solve_mat(self, kind, rhs, tol, maxit, frac=None):
+        """MatVec::Sol::Solve_* on this system's matrix (frac: a natural-order
+        RefFrac holding values, for the preconditioned kinds): returns
+        (ok, conv, iter, u, r_final)."""
+        r = f64(rhs).copy()
+        u = np.zeros(self.nb * self.b)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_mat_solve(self.h, None if frac is None else frac.h, Orc.MAT_KINDS[kind], _p(r), _p(u),
+                                      C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
 
 
 class RefILU:

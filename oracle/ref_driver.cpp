@@ -12,6 +12,7 @@
 //   LsSol::CPreconditioner_ILU  include/delfem/ls/preconditioner.h:59-108
 //   LsSol::Solve_PCG / Solve_PBiCGSTAB  include/delfem/ls/solver_ls_iter.h:35-45
 //   MatVec::CMatDiaFrac_BlkCrs  include/delfem/matvec/matdiafrac_blkcrs.h:40-137
+//   MatVec::Sol::Solve_*        include/delfem/matvec/solver_mat_iter.h:41-60
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -27,6 +28,7 @@
 #include "delfem/ls/linearsystem.h"
 #include "delfem/ls/preconditioner.h"
 #include "delfem/ls/solver_ls_iter.h"
+#include "delfem/matvec/solver_mat_iter.h"
 #include "delfem/femeqn/ker_emat_tri.h"
 #include "delfem/femeqn/ker_emat_tet.h"
 
@@ -292,6 +294,31 @@ int ref_frac_solve(void* q, void* p, int kind, double* conv, unsigned* iter) {
   FracPrecond prec(rf);
   LsSol::CLinearSystemPreconditioner lsp(s->ls, prec);
   bool ok = kind == 0 ? LsSol::Solve_PCG(*conv, *iter, lsp) : LsSol::Solve_PBiCGSTAB(*conv, *iter, lsp);
+  return ok ? 1 : 0;
+}
+
+// Matrix-layer solvers on the system's matrix (MatVec::Sol, solver_mat_iter.cpp):
+// kind 0 Solve_CG, 1 Solve_PCG, 2 Solve_BiCGSTAB, 3 Solve_PBiCGSTAB; q is a
+// natural-order factor holding values (kinds 1 and 3), used as the
+// CPrecond_Blk with the system matrix as mat_p.  r: in right-hand side,
+// out final residual; u: out solution.
+int ref_mat_solve(void* p, void* q, int kind, double* r, double* u, double* conv, unsigned* iter) {
+  RefSys* s = static_cast<RefSys*>(p);
+  const MatVec::CMatDia_BlkCrs& A = s->ls.GetMatrix(0);
+  MatVec::CVector_Blk rv(s->nb, s->b), uv(s->nb, s->b);
+  copy_vec_in(rv, s->nb, s->b, r);
+  bool ok = false;
+  if (kind == 0) {
+    ok = MatVec::Sol::Solve_CG(*conv, *iter, A, rv, uv);
+  } else if (kind == 2) {
+    ok = MatVec::Sol::Solve_BiCGSTAB(*conv, *iter, A, rv, uv);
+  } else {
+    const MatVec::CPrecond_Blk& pc = *static_cast<RefFrac*>(q)->f;
+    ok = kind == 1 ? MatVec::Sol::Solve_PCG(*conv, *iter, A, rv, uv, pc, A)
+                   : MatVec::Sol::Solve_PBiCGSTAB(*conv, *iter, A, rv, uv, pc, A);
+  }
+  copy_vec_out(rv, s->nb, s->b, r);
+  copy_vec_out(uv, s->nb, s->b, u);
   return ok ? 1 : 0;
 }
 

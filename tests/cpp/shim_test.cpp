@@ -1,20 +1,43 @@
 // Drives the C++ drop-in classes (include/delfem_amd/{matvec,ls}.hpp) the way
 // DelFEM's equation systems do: pattern from the element list, InitializeMarge,
 // one Mearge per element, residual, Dirichlet flags, FinalizeMarge, ILU(0),
-// Solve_PCG or Solve_PBiCGSTAB.  Input and output are raw binary files
-// written / read by tests/test_gpu_shim.py.
+// then one solver: LsSol::Solve_PCG / Solve_PBiCGSTAB, or the matrix-layer
+// MatVec::Sol::Solve_PCG / Solve_PBiCGSTAB / Solve_CG / Solve_BiCGSTAB on the
+// assembled matrix (with the ILU(0) factor itself, which takes the device
+// solver, or wrapped in another CPrecond_Blk, which takes the generic loop).
+// Input and output are raw binary files written / read by tests/test_gpu_shim.py.
 //
-// input : i32 nnode nelem len npe kind(0 PCG, 1 BiCGSTAB) maxit | f64 tol |
+// input : i32 nnode nelem len npe kind maxit | f64 tol |
 //         u32 conn[nelem*npe] | f64 emat[nelem*npe*npe*len*len] |
 //         f64 eres[nelem*npe*len] | i32 flags[nnode*len]
+//         kind: 0 LsSol PCG, 1 LsSol BiCGSTAB, 2 Sol::Solve_PCG, 3 Sol::Solve_PBiCGSTAB,
+//         4 / 5 the same through a wrapping CPrecond_Blk, 6 Sol::Solve_CG, 7 Sol::Solve_BiCGSTAB
 // output: f64 norm_res conv | i32 ok iteration | f64 val[ncrs*len*len]
-//         dia[nnode*len*len] res[ndof] x[ndof]
+//         dia[nnode*len*len] res[ndof] x[ndof] |b-Ax|^2 r_final[ndof]
 #include <cstdio>
 #include <cstdlib>
 #include <set>
 #include <vector>
 
 #include "delfem_amd/ls.hpp"
+
+// A preconditioner that is not the factor class itself: the matrix-layer
+// solvers then run their generic loop.
+struct WrappedPrecond : MatVec::CPrecond_Blk {
+  explicit WrappedPrecond(const MatVec::CMatDiaFrac_BlkCrs& f) : f(f) {}
+  bool SolvePrecond(const MatVec::CMatDia_BlkCrs& mat, MatVec::CVector_Blk& v) const override {
+    return f.SolvePrecond(mat, v);
+  }
+  const MatVec::CMatDiaFrac_BlkCrs& f;
+};
+
+static const char* kind_name(int kind) {
+  static const char* names[] = {"LsSol::Solve_PCG",          "LsSol::Solve_PBiCGSTAB",
+                                "Sol::Solve_PCG",            "Sol::Solve_PBiCGSTAB",
+                                "Sol::Solve_PCG (wrapped)",  "Sol::Solve_PBiCGSTAB (wrapped)",
+                                "Sol::Solve_CG",             "Sol::Solve_BiCGSTAB"};
+  return kind >= 0 && kind < 8 ? names[kind] : "?";
+}
 
 template <typename T>
 static void rd(FILE* f, T* p, size_t n) {
@@ -88,13 +111,36 @@ int main(int argc, char** argv) {
     std::vector<double> val, dia;
     A.GetValues(val, dia);
     const std::vector<double> rfin = ls.GetResidual(0).GetValues();
-    LsSol::CPreconditioner_ILU prec(ls, 0);
-    if (!prec.SetValue(ls)) return 1;
-    LsSol::CLinearSystemPreconditioner lsp(ls, prec);
     double conv = tol;
-    const bool ok = kind == 0 ? LsSol::Solve_PCG(conv, maxit, lsp) : LsSol::Solve_PBiCGSTAB(conv, maxit, lsp);
-    const std::vector<double> x = ls.GetUpdate(0).GetValues();
+    bool ok = false;
+    std::vector<double> x, rout;
+    if (kind <= 1) {
+      LsSol::CPreconditioner_ILU prec(ls, 0);
+      if (!prec.SetValue(ls)) return 1;
+      LsSol::CLinearSystemPreconditioner lsp(ls, prec);
+      ok = kind == 0 ? LsSol::Solve_PCG(conv, maxit, lsp) : LsSol::Solve_PBiCGSTAB(conv, maxit, lsp);
+      x = ls.GetUpdate(0).GetValues();
+      rout = ls.GetResidual(0).GetValues();
+    } else {
+      MatVec::CVector_Blk r(nnode, len), u(nnode, len);
+      r.SetValues(rfin.data());
+      MatVec::CMatDiaFrac_BlkCrs fac(0, A);
+      if (!fac.SetValue(A)) return 1;
+      WrappedPrecond wrapped(fac);
+      switch (kind) {
+        case 2: ok = MatVec::Sol::Solve_PCG(conv, maxit, A, r, u, fac, A); break;
+        case 3: ok = MatVec::Sol::Solve_PBiCGSTAB(conv, maxit, A, r, u, fac, A); break;
+        case 4: ok = MatVec::Sol::Solve_PCG(conv, maxit, A, r, u, wrapped, A); break;
+        case 5: ok = MatVec::Sol::Solve_PBiCGSTAB(conv, maxit, A, r, u, wrapped, A); break;
+        case 6: ok = MatVec::Sol::Solve_CG(conv, maxit, A, r, u); break;
+        case 7: ok = MatVec::Sol::Solve_BiCGSTAB(conv, maxit, A, r, u); break;
+        default: return 2;
+      }
+      x = u.GetValues();
+      rout = r.GetValues();
+    }
     // the API's own check: ||b - A x|| with MATVEC on a scratch vector
+    ls.GetUpdate(0).SetValues(x.data());
     ls.ReSizeTmpVecSolver(1);
     ls.GetTmpVectorArySize();
     ls.vec(0).SetValues(rfin.data());
@@ -111,9 +157,10 @@ int main(int argc, char** argv) {
     wr(o, rfin.data(), rfin.size());
     wr(o, x.data(), x.size());
     wr(o, &rr, 1);
+    wr(o, rout.data(), rout.size());
     fclose(o);
-    printf("shim_test: %s ok=%d iteration=%u conv=%.3e |b-Ax|=%.3e\n", kind == 0 ? "PCG" : "BiCGSTAB", ok ? 1 : 0,
-           maxit, conv, rr > 0 ? __builtin_sqrt(rr) : 0.0);
+    printf("shim_test: %s ok=%d iteration=%u conv=%.3e |b-Ax|=%.3e\n", kind_name(kind), ok ? 1 : 0, maxit, conv,
+           rr > 0 ? __builtin_sqrt(rr) : 0.0);
   } catch (const delfem_amd::Error& e) {
     fprintf(stderr, "delfem_amd error: %s\n", e.what());
     return 1;

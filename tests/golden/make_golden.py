@@ -8,8 +8,10 @@ Each fixture holds inputs (mesh or graph, coefficients, vectors, orderings)
 and the reference's outputs for them: element matrices, assembled block-CRS
 matrix and residual after Dirichlet conditions, ILU(k) patterns (natural and
 ordered, k = 0..3), ILU factor values, one MatVec, one forward/backward
-substitution, and the Solve_PCG / Solve_PBiCGSTAB results (flag, iterations,
-conv_ratio, solution, final residual).  Floating-point arrays are stored as
+substitution, the Solve_PCG / Solve_PBiCGSTAB results (flag, iterations,
+conv_ratio, solution, final residual), and the same for the matrix-layer
+MatVec::Sol::Solve_CG / Solve_PCG / Solve_BiCGSTAB / Solve_PBiCGSTAB with the
+natural ILU(0) factor.  Floating-point arrays are stored as
 float64 and compared bitwise by the tests.
 """
 import os
@@ -107,6 +109,17 @@ def matvec_fixture(out, rs, nb, b):
     out.update(mv_x=x, mv_y=y, mv_alpha=np.array(-0.5), mv_beta=np.array(2.0), mv_out=rs.get_vector(1))
 
 
+def mat_solver_fixture(out, rs, res, tol=1e-10, maxit=2000):
+    """MatVec::Sol solvers on the assembled matrix and residual."""
+    rf = O.RefFrac(rs, 0)
+    rf.set_value()
+    for kind in ("cg", "pcg", "bicgstab", "pbicgstab"):
+        ok, conv, it, u, r = rs.solve_mat(kind, res, tol, maxit, rf if kind.startswith("p") else None)
+        out[f"mat_{kind}"] = np.array([float(ok), conv, float(it)])
+        out[f"mat_{kind}_u"] = u
+        out[f"mat_{kind}_r"] = r
+
+
 def random_graph_fixture(ref, n=60, b=2, seed=7):
     """A random symmetric graph with 2x2 blocks: diagonally dominant values."""
     rng = np.random.default_rng(seed)
@@ -149,6 +162,7 @@ def main():
         matvec_fixture(out, rs, prob.nnode, prob.b)
         ilu_fixture(out, rs, prob.nnode, prob.b, res, orders, (0, 1, 2, 3), 1e-10,
                     max_value_lev=1 if prob.b == 1 else 0)
+        mat_solver_fixture(out, rs, res)
         fixtures[name] = out
     out, rs, res = random_graph_fixture(ref)
     rng = np.random.default_rng(13)

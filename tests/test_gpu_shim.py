@@ -2,7 +2,9 @@
 way DelFEM's equation systems drive the reference classes, checked against
 the oracle: assembly by per-element Mearge and FinalizeMarge bit-exact,
 ILU(0)-PCG / ILU(0)-BiCGSTAB within one iteration and 1e-8 of the oracle's
-solution.  The program is tests/cpp/shim_test (built by build())."""
+solution; the matrix-layer MatVec::Sol solvers (device path and generic
+loop, preconditioned and not) against the oracle's restatement of
+solver_mat_iter.cpp.  The program is tests/cpp/shim_test (built by build())."""
 import os
 import subprocess
 
@@ -24,18 +26,15 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
-def test_cpp_dropin(orc, gpu, tmp_path, name):
+def run_shim(orc, prob, kind, tol, tmp_path, maxit=2000):
+    """Runs tests/cpp/shim_test; returns (ref, d2, i2, val, dia, res, x, rr, r_final)."""
     if not os.path.exists(BIN):
         pytest.fail("tests/cpp/shim_test is not built (run __graft_entry__.build())")
-    make, kind, tol = CASES[name]
-    prob = make()
-    print(f"[case] {name}", flush=True)
     emat, eres = problem_emats(orc, prob)
     b = prob.b
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
-        np.array([prob.nnode, prob.nelem, b, prob.npe, kind, 2000], dtype=np.int32).tofile(f)
+        np.array([prob.nnode, prob.nelem, b, prob.npe, kind, maxit], dtype=np.int32).tofile(f)
         np.array([tol]).tofile(f)
         np.ascontiguousarray(prob.conn, dtype=np.uint32).tofile(f)
         np.ascontiguousarray(emat).tofile(f)
@@ -54,6 +53,17 @@ def test_cpp_dropin(orc, gpu, tmp_path, name):
     val, dia = rest[:nv], rest[nv:nv + nd]
     res, x = rest[nv + nd:nv + nd + ndof], rest[nv + nd + ndof:nv + nd + 2 * ndof]
     rr = rest[nv + nd + 2 * ndof]
+    r_final = rest[nv + nd + 2 * ndof + 1:nv + nd + 3 * ndof + 1]
+    return ref, d2, i2, val, dia, res, x, rr, r_final
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_cpp_dropin(orc, gpu, tmp_path, name):
+    make, kind, tol = CASES[name]
+    prob = make()
+    print(f"[case] {name}", flush=True)
+    b = prob.b
+    ref, d2, i2, val, dia, res, x, rr, _ = run_shim(orc, prob, kind, tol, tmp_path)
     assert np.array_equal(val, ref["val"]) and np.array_equal(dia, ref["dia"])
     assert np.array_equal(res, ref["res"])
     assert d2[0] == pytest.approx(np.linalg.norm(ref["res"]), rel=1e-12)
@@ -65,3 +75,49 @@ def test_cpp_dropin(orc, gpu, tmp_path, name):
     assert abs(int(i2[1]) - it_o) <= 1, (i2[1], it_o)
     assert np.linalg.norm(x - x_o) <= 1e-8 * np.linalg.norm(x_o)
     assert np.sqrt(rr) <= 10 * tol * np.linalg.norm(ref["res"])
+
+
+MAT_KINDS = {2: "pcg", 3: "pbicgstab", 4: "pcg", 5: "pbicgstab", 6: "cg", 7: "bicgstab"}
+MAT_CASES = {
+    "poisson2d_10": (lambda: meshgen.poisson2d(10, seed=3), (2, 3, 4, 5, 6, 7)),
+    "solid3d_3": (lambda: meshgen.solid3d(3, seed=3, sigma=0.5), (2, 3, 4, 5, 6, 7)),
+    "oseen2d_8": (lambda: meshgen.oseen2d(8, seed=3), (3, 5, 7)),
+}
+
+
+@pytest.mark.parametrize("kind", [2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("name", sorted(MAT_CASES))
+def test_cpp_matrix_layer_solvers(orc, gpu, tmp_path, name, kind):
+    """MatVec::Sol::Solve_* through the C++ classes against the oracle's
+    restatement of solver_mat_iter.cpp: same return value, iterations within
+    one (two without a preconditioner), solution within 1e-8 relative, and
+    the final residual the solver leaves (for Solve_CG not b - A u)."""
+    make, kinds = MAT_CASES[name]
+    if kind not in kinds:
+        pytest.skip("CG needs a symmetric matrix")
+    prob = make()
+    print(f"[case] {name} kind {kind}", flush=True)
+    tol = 1e-10
+    ref, d2, i2, val, dia, res, x, rr, r_final = run_shim(orc, prob, kind, tol, tmp_path)
+    nb, b = prob.nnode, prob.b
+    F = orc.iluk_symbolic(ref["rowptr"], ref["col"], 0)
+    fv, fd, _ = orc.ilu_values(nb, b, ref["rowptr"], ref["col"], ref["val"], ref["dia"], F)
+    ok_o, conv_o, it_o, u_o, r_o = orc.solve_mat(MAT_KINDS[kind], nb, b, ref["rowptr"], ref["col"], ref["val"],
+                                                 ref["dia"], ref["res"], tol, 2000, F, fv, fd)
+    assert bool(i2[0]) == ok_o and ok_o
+    assert abs(int(i2[1]) - it_o) <= (2 if kind >= 6 else 1), (i2[1], it_o)
+    assert d2[1] < tol
+    assert np.linalg.norm(x - u_o) <= 1e-8 * np.linalg.norm(u_o)
+    n0 = np.linalg.norm(ref["res"])
+    assert np.linalg.norm(r_final) < tol * n0
+    if kind != 6:
+        assert np.sqrt(rr) <= 10 * tol * n0
+
+
+@pytest.mark.parametrize("kind", [2, 3, 6])
+def test_cpp_matrix_layer_max_iter(orc, gpu, tmp_path, kind):
+    """Stopped by max_iter: Solve_PCG false, Solve_PBiCGSTAB and Solve_CG
+    true, conv_ratio left at the tolerance and iteration at max_iter."""
+    prob = meshgen.solid3d(3, seed=3, sigma=0.5)
+    _, d2, i2, _, _, _, _, _, _ = run_shim(orc, prob, kind, 1e-14, tmp_path, maxit=5)
+    assert (int(i2[0]), int(i2[1]), d2[1]) == (0 if kind == 2 else 1, 5, 1e-14)

@@ -131,3 +131,18 @@ def test_solvers(orc, name, kind):
             assert np.array_equal(r, d[f"{kind}_r_{tag}"]), tag
             n_checked += 1
     assert n_checked >= 3
+
+
+@pytest.mark.parametrize("name", ["poisson2d_8", "solid3d_3"])
+@pytest.mark.parametrize("kind", ["cg", "pcg", "bicgstab", "pbicgstab"])
+def test_matrix_layer_solvers(orc, name, kind):
+    """MatVec::Sol::Solve_* with the natural ILU(0) factor: flag, iterations,
+    conv_ratio, solution and final residual bit for bit."""
+    d = load(name)
+    nb, b, rowptr, col, val, dia = system(d)
+    F = orc.iluk_symbolic(rowptr, col, 0)
+    fv, fd, _ = orc.ilu_values(nb, b, rowptr, col, val, dia, F)
+    ok, conv, it, u, r = orc.solve_mat(kind, nb, b, rowptr, col, val, dia, d["res"], 1e-10, 2000, F, fv, fd)
+    want = d[f"mat_{kind}"]
+    assert (float(ok), conv, float(it)) == (want[0], want[1], want[2])
+    assert np.array_equal(u, d[f"mat_{kind}_u"]) and np.array_equal(r, d[f"mat_{kind}_r"])

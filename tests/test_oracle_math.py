@@ -291,3 +291,117 @@ def test_colouring_and_levels(orc, solid, poisson):
                 assert np.all(lv[Lc] < lv[i]) and np.all(uv[Uc] < uv[i])
             if lev == 0:
                 assert nl == nc and nu == nc                         # one level per colour
+
+
+def _textbook_cg(A, b, nstep):
+    """Plain CG (Hestenes-Stiefel) from x0 = 0: iterates x_k and residuals r_k, k = 0..nstep."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = r.copy()
+    xs, rs = [x.copy()], [r.copy()]
+    for _ in range(nstep):
+        Ap = A @ p
+        a = (r @ r) / (p @ Ap)
+        x = x + a * p
+        rn = r - a * Ap
+        p = rn + ((rn @ rn) / (r @ r)) * p
+        r = rn
+        xs.append(x.copy())
+        rs.append(r.copy())
+    return xs, rs
+
+
+def _textbook_bicgstab(A, b, tol, maxit):
+    """Unpreconditioned BiCGSTAB (van der Vorst 1992) from x0 = 0, shadow r0 = b:
+    returns the first k with ||r_k|| < tol ||b||, x_k, r_k."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    r0 = b.copy()
+    p = r.copy()
+    nb0 = b @ b
+    for k in range(1, maxit):
+        Ap = A @ p
+        rho = r @ r0
+        a = rho / (Ap @ r0)
+        s = r - a * Ap
+        As = A @ s
+        w = (As @ s) / (As @ As)
+        x = x + a * p + w * s
+        r = s - w * As
+        if (r @ r) / nb0 < tol * tol:
+            return k, x, r
+        beta = (r @ r0) / rho * (a / w)
+        p = r + beta * (p - w * Ap)
+    return None, x, r
+
+
+def test_matrix_layer_cg_textbook(orc, poisson, solid):
+    """MatVec::Sol::Solve_CG is textbook CG whose convergence test reads the
+    residual from before the current step and which returns before adding
+    that step to the solution: iteration = K + 1 for the first K with
+    ||r_K|| < tol ||r_0||, u = x_K, r_final = r_{K+1}, conv = ||r_K|| / ||r_0||."""
+    for prob in (poisson, solid):
+        A = assemble_problem(orc, prob)
+        nb, b = prob.nnode, prob.b
+        M = bsr_of(nb, b, A["rowptr"], A["col"], A["val"], A["dia"])
+        rhs = A["res"]
+        tol = 1e-8
+        ok, conv, it, u, rfin = orc.solve_mat("cg", nb, b, A["rowptr"], A["col"], A["val"], A["dia"], rhs, tol, 5000)
+        xs, rs = _textbook_cg(M, rhs, it + 1)
+        n0 = np.linalg.norm(rhs)
+        K = next(k for k in range(1, len(rs)) if np.linalg.norm(rs[k]) < tol * n0)
+        exact = prob is poisson   # well conditioned: rounding moves nothing
+        assert ok and (it == K + 1 if exact else abs(it - (K + 1)) <= 1)
+        K = it - 1
+        assert conv == pytest.approx(np.linalg.norm(rs[K]) / n0, rel=1e-6 if exact else 0.1)
+        assert np.linalg.norm(u - xs[K]) <= (1e-9 if exact else 1e-6) * np.linalg.norm(xs[K])
+        if exact:   # r_final is r_{K+1}, not r_K
+            assert np.linalg.norm(rfin - rs[K + 1]) <= 0.05 * np.linalg.norm(rs[K + 1])
+            assert np.linalg.norm(rfin - rs[K]) > 0.2 * np.linalg.norm(rs[K + 1])
+        # the state it leaves: r_final is not b - A u (the last step is in r only)
+        assert np.linalg.norm(rhs - M @ u) > np.linalg.norm(rfin)
+
+
+def test_matrix_layer_bicgstab_textbook(orc, poisson, solid):
+    """MatVec::Sol::Solve_BiCGSTAB is unpreconditioned BiCGSTAB: same
+    iteration count, solution and residual as the textbook recurrence; the
+    preconditioned twin with an ILU(0) factor solves to the tolerance."""
+    for prob in (poisson, solid):
+        A = assemble_problem(orc, prob)
+        nb, b = prob.nnode, prob.b
+        M = bsr_of(nb, b, A["rowptr"], A["col"], A["val"], A["dia"])
+        rhs = A["res"]
+        ok, conv, it, u, rfin = orc.solve_mat("bicgstab", nb, b, A["rowptr"], A["col"], A["val"], A["dia"], rhs,
+                                              1e-9, 5000)
+        k, x, r = _textbook_bicgstab(M, rhs, 1e-9, 5000)
+        assert ok and (it == k if prob is poisson else abs(it - k) <= 1)
+        assert np.linalg.norm(u - x) <= 1e-7 * np.linalg.norm(x)
+        assert conv < 1e-9
+        if prob is poisson:   # elsewhere the last BiCGSTAB residual is erratic under rounding
+            assert conv == pytest.approx(np.linalg.norm(r) / np.linalg.norm(rhs), rel=0.25)
+        F = orc.iluk_symbolic(A["rowptr"], A["col"], 0)
+        fv, fd, _ = orc.ilu_values(nb, b, A["rowptr"], A["col"], A["val"], A["dia"], F)
+        for kind in ("pcg", "pbicgstab"):
+            ok, conv, it, u, rfin = orc.solve_mat(kind, nb, b, A["rowptr"], A["col"], A["val"], A["dia"], rhs, 1e-9,
+                                                  5000, F, fv, fd)
+            xd = spla.spsolve(M.tocsc(), rhs)
+            assert ok and conv < 1e-9 and it < 200
+            assert np.linalg.norm(rhs - M @ u) <= 1.01e-9 * np.linalg.norm(rhs)
+            assert np.linalg.norm(u - xd) <= 1e-6 * np.linalg.norm(xd)
+
+
+def test_matrix_layer_return_semantics(orc, poisson):
+    """At max_iter: Solve_PCG returns false and Solve_CG / Solve_(P)BiCGSTAB
+    true, all leaving conv_ratio at the input tolerance; a zero right-hand
+    side returns conv 0, iteration 0 (solver_mat_iter.cpp:55-62, 131-137)."""
+    A = assemble_problem(orc, poisson)
+    nb = poisson.nnode
+    F = orc.iluk_symbolic(A["rowptr"], A["col"], 0)
+    fv, fd, _ = orc.ilu_values(nb, 1, A["rowptr"], A["col"], A["val"], A["dia"], F)
+    for kind, ok_exp in (("cg", True), ("pcg", False), ("bicgstab", True), ("pbicgstab", True)):
+        ok, conv, it, _, _ = orc.solve_mat(kind, nb, 1, A["rowptr"], A["col"], A["val"], A["dia"], A["res"], 1e-12,
+                                           3, F, fv, fd)
+        assert (ok, conv, it) == (ok_exp, 1e-12, 3), kind
+        ok, conv, it, u, _ = orc.solve_mat(kind, nb, 1, A["rowptr"], A["col"], A["val"], A["dia"],
+                                           np.zeros(nb), 1e-12, 30, F, fv, fd)
+        assert (ok, conv, it) == (True, 0.0, 0) and not u.any(), kind

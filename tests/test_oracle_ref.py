@@ -52,3 +52,39 @@ def test_iluk_random_graphs(orc, ref, seed, lev):
         F = orc.iluk_symbolic(rowptr, col, lev, order)
         for k in ("rowptr", "col", "diaind"):
             assert np.array_equal(P[k], F[k]), (k, lev)
+
+
+def _ref_system(orc, ref, prob):
+    A = assemble_problem(orc, prob)
+    rs = RefSystem(ref, prob.nnode, prob.b, A["rowptr"], A["col"])
+    rs.merge(prob.conn, A["emat"], A["eres"])
+    rs.set_bc(prob.flags)
+    rs.finalize()
+    return A, rs
+
+
+@pytest.mark.parametrize("kind", ["cg", "pcg", "bicgstab", "pbicgstab"])
+@pytest.mark.parametrize("case", ["poisson2d_24", "solid3d_4", "oseen2d_8"])
+def test_matrix_layer_solvers(orc, ref, kind, case):
+    """MatVec::Sol::Solve_* (solver_mat_iter.cpp) of the oracle against the
+    reference: return value, iteration count, conv_ratio, solution and final
+    residual bit for bit, including a run stopped by max_iter."""
+    prob = {"poisson2d_24": lambda: meshgen.poisson2d(24, seed=5),
+            "solid3d_4": lambda: meshgen.solid3d(4, seed=5, sigma=0.5),
+            "oseen2d_8": lambda: meshgen.oseen2d(8, seed=5)}[case]()
+    if prob.kind == "oseen2d" and kind in ("cg", "pcg"):
+        pytest.skip("CG needs a symmetric matrix")
+    A, rs = _ref_system(orc, ref, prob)
+    nb, b = prob.nnode, prob.b
+    F = fv = fd = None
+    frac = None
+    if kind.startswith("p"):
+        frac = RefFrac(rs, 0)
+        frac.set_value()
+        F = orc.iluk_symbolic(A["rowptr"], A["col"], 0)
+        fv, fd, _ = orc.ilu_values(nb, b, A["rowptr"], A["col"], A["val"], A["dia"], F)
+    for tol, maxit in ((1e-9, 3000), (1e-9, 7)):
+        got = orc.solve_mat(kind, nb, b, A["rowptr"], A["col"], A["val"], A["dia"], A["res"], tol, maxit, F, fv, fd)
+        exp = rs.solve_mat(kind, A["res"], tol, maxit, frac)
+        assert got[:3] == exp[:3], (kind, maxit, got[:3], exp[:3])
+        assert np.array_equal(got[3], exp[3]) and np.array_equal(got[4], exp[4])
