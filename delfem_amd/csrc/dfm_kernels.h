@@ -173,36 +173,71 @@ __device__ inline bf16s ld_stream<bf16s>(const bf16s* p) {
   return r;
 }
 
-// Slot layout of a block in a sliced-ELL array.  float / double: entry e of
-// the block of lane l in slot k at (k*bb + e)*64 + l, one value per lane per
-// load.  bf16: entries paired into 32-bit words, word w = entries (2w, 2w+1)
-// at (k*W + w)*64 + l with W = (bb+1)/2, so each lane loads 4 bytes.
+// Slot layout of a block in a sliced-ELL array.  Each lane's block of one
+// slot is E 32-bit words (float: bb words, double: 2 bb, bf16: entries paired
+// into ceil(bb/2) words).  A slot stores the first 4 floor(E/4) words of all
+// 64 lanes as 16-byte quads, quad-major ((q*64 + lane)*4 + j), then the
+// remaining E mod 4 words of each lane contiguously (Q*256 + lane*R + r), so a
+// lane fetches its block with 16-byte loads plus one 4- or 8-byte load and
+// every load instruction of a wave reads one contiguous 1 KiB or 256/512 B
+// run.  A slot occupies 256 E bytes, a multiple of 256.
+template <typename TS>
+__host__ __device__ constexpr int slot_words(int bb) {
+  return std::is_same<TS, bf16s>::value ? (bb + 1) / 2 : bb * (int)sizeof(TS) / 4;
+}
+// TS elements allocated per lane per slot (bf16 pads an odd bb by one)
 template <typename TS>
 __host__ __device__ constexpr int slot_elems(int bb) {
-  return std::is_same<TS, bf16s>::value ? 2 * ((bb + 1) / 2) : bb;
+  return slot_words<TS>(bb) * 4 / (int)sizeof(TS);
 }
+// Position (in TS elements) of entry e of the block of `lane` in slot k.
 template <typename TS>
-__device__ inline size_t slot_index(size_t k, int bb, int e, int lane) {
-  if constexpr (std::is_same<TS, bf16s>::value)
-    return ((k * (size_t)((bb + 1) / 2) + (size_t)(e >> 1)) * 64 + lane) * 2 + (e & 1);
-  else
-    return (k * bb + e) * 64 + lane;
+__host__ __device__ inline size_t slot_index(size_t k, int bb, int e, int lane) {
+  const int E = slot_words<TS>(bb), Q = E / 4, R = E % 4;
+  const bool half = std::is_same<TS, bf16s>::value;
+  const int w = half ? (e >> 1) : e * (int)sizeof(TS) / 4;  // first word of the entry
+  size_t word = k * 64 * (size_t)E;
+  if (w < 4 * Q) word += ((size_t)(w >> 2) * 64 + lane) * 4 + (w & 3);
+  else word += (size_t)Q * 256 + (size_t)lane * R + (w - 4 * Q);
+  if (half) return word * 2 + (e & 1);
+  return word * 4 / sizeof(TS);
 }
-// All BB values of one block slot, converted to TV.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+// All BB values of one block slot, converted to TV (non-temporal loads: the
+// matrix streams once per use, the gathered vectors keep the caches).
 template <int BB, typename TS, typename TV>
 __device__ inline void load_block(const TS* __restrict__ val, size_t k, int lane, TV (&a)[BB]) {
-  if constexpr (std::is_same<TS, bf16s>::value) {
-    constexpr int W = (BB + 1) / 2;
-    const unsigned* wv = reinterpret_cast<const unsigned*>(val);
+  constexpr int E = slot_words<TS>(BB), Q = E / 4, R = E % 4;
+  const unsigned* base = reinterpret_cast<const unsigned*>(val) + k * 64 * E;
+  unsigned w[E];
 #pragma unroll
-    for (int w = 0; w < W; w++) {
-      const unsigned u = __builtin_nontemporal_load(wv + (k * W + w) * 64 + lane);
-      a[2 * w] = (TV)__uint_as_float(u << 16);
-      if (2 * w + 1 < BB) a[2 * w + 1] = (TV)__uint_as_float(u & 0xFFFF0000u);
-    }
+  for (int q = 0; q < Q; q++) {
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(base + ((size_t)q * 64 + lane) * 4));
+    w[4 * q] = v.x;
+    w[4 * q + 1] = v.y;
+    w[4 * q + 2] = v.z;
+    w[4 * q + 3] = v.w;
+  }
+  const unsigned* rem = base + Q * 256 + (size_t)lane * R;
+  if constexpr (R == 2) {
+    const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(rem));
+    w[4 * Q] = v.x;
+    w[4 * Q + 1] = v.y;
   } else {
 #pragma unroll
-    for (int e = 0; e < BB; e++) a[e] = cvt<TV>(ld_stream(val + (k * BB + e) * 64 + lane));
+    for (int r = 0; r < R; r++) w[4 * Q + r] = __builtin_nontemporal_load(rem + r);
+  }
+#pragma unroll
+  for (int e = 0; e < BB; e++) {
+    if constexpr (std::is_same<TS, bf16s>::value) {
+      const unsigned u = w[e >> 1];
+      a[e] = (TV)__uint_as_float((e & 1) ? (u & 0xFFFF0000u) : (u << 16));
+    } else if constexpr (std::is_same<TS, double>::value) {
+      a[e] = (TV)__hiloint2double((int)w[2 * e + 1], (int)w[2 * e]);
+    } else {
+      a[e] = (TV)__uint_as_float(w[e]);
+    }
   }
 }
 
@@ -940,6 +975,27 @@ static __global__ void k_sell_gather_dia(long nslot, int bb, const int* __restri
 }
 
 
+// acc += sum over slots k0 <= k < k1 (in that order) of block(k) * x[col(k)],
+// four slots' loads issued before their products.
+template <int B, typename TS, typename T>
+__device__ __forceinline__ void sell_madd_range(const int* __restrict__ col, const TS* __restrict__ val,
+                                                const T* __restrict__ x, int k0, int k1, int lane, T (&acc)[B]) {
+  constexpr int BB = B * B;
+  int k = k0;
+  for (; k + 3 < k1; k += 4) {
+    T a[4][BB], v[4][B];
+#pragma unroll
+    for (int u = 0; u < 4; u++) sell_slot<B, TS, T>(col, val, x, k + u, lane, a[u], v[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++) blk_madd<B, T>(acc, a[u], v[u]);
+  }
+  for (; k < k1; k++) {
+    T a[BB], v[B];
+    sell_slot<B, TS, T>(col, val, x, k, lane, a, v);
+    blk_madd<B, T>(acc, a, v);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // SpMV on the sliced-ELL copy of A (rows = slice*64+lane, diagonal last):
 //   y = A x            (res == nullptr)
@@ -966,20 +1022,7 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
       T acc[B];
 #pragma unroll
       for (int r = 0; r < B; r++) acc[r] = (T)0;
-      const int k0 = sptr[s], k1 = sptr[s + 1];
-      int k = k0;
-      for (; k + 1 < k1; k += 2) {
-        T a0[BB], x0[B], a1[BB], x1[B];
-        sell_slot<B, TS, T>(col, val, x, k, lane, a0, x0);
-        sell_slot<B, TS, T>(col, val, x, k + 1, lane, a1, x1);
-        blk_madd<B, T>(acc, a0, x0);
-        blk_madd<B, T>(acc, a1, x1);
-      }
-      if (k < k1) {
-        T a0[BB], x0[B];
-        sell_slot<B, TS, T>(col, val, x, k, lane, a0, x0);
-        blk_madd<B, T>(acc, a0, x0);
-      }
+      sell_madd_range<B, TS, T>(col, val, x, sptr[s], sptr[s + 1], lane, acc);
       {
         T a[BB], xv[B];
         load_block<BB, TS, T>(dval, (size_t)s, lane, a);
@@ -1004,19 +1047,21 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
 }
 
 // t -= sum over slots k0 <= k < k1 (in that order) of block(k) * v[col(k)]:
-// four slots' loads are issued before their products, so a lane waits for a
-// few memory round trips per row rather than one per slot.
+// U slots' loads are issued before their products, so a lane waits for a few
+// memory round trips per row rather than one per slot (eight slots per batch
+// measured no faster).
 template <int B, typename TS, typename T>
 __device__ __forceinline__ void sell_msub_range(const int* __restrict__ col, const TS* __restrict__ val,
                                                 const T* v, int k0, int k1, int lane, T (&t)[B]) {
   constexpr int BB = B * B;
+  constexpr int U = 4;
   int k = k0;
-  for (; k + 3 < k1; k += 4) {
-    T a[4][BB], x[4][B];
+  for (; k + U - 1 < k1; k += U) {
+    T a[U][BB], x[U][B];
 #pragma unroll
-    for (int u = 0; u < 4; u++) sell_slot<B, TS, T>(col, val, v, k + u, lane, a[u], x[u]);
+    for (int u = 0; u < U; u++) sell_slot<B, TS, T>(col, val, v, k + u, lane, a[u], x[u]);
 #pragma unroll
-    for (int u = 0; u < 4; u++) blk_msub<B, T>(t, a[u], x[u]);
+    for (int u = 0; u < U; u++) blk_msub<B, T>(t, a[u], x[u]);
   }
   for (; k < k1; k++) {
     T a[BB], x[B];
@@ -1302,27 +1347,25 @@ __global__ void k_convert(long n, const double* __restrict__ x, T* __restrict__ 
 static __global__ void __launch_bounds__(kNT) k_copy4(long n4, const float4* __restrict__ in, float4* __restrict__ out) {
   for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n4; i += (long)gridDim.x * kNT) out[i] = in[i];
 }
-// Streaming read of n4 16-byte words, four independent loads in flight per
-// lane; the sum is stored only on an impossible value so that the loads stay
-// live without a write stream.
+// Streaming read of n4 16-byte words: eight non-temporal 16-byte loads in
+// flight per lane (the fastest plain read measured on this part); the sum is
+// stored only on an impossible value so that the loads stay live without a
+// write stream.
 static __global__ void __launch_bounds__(kNT) k_read4(long n4, const float4* __restrict__ in, float* __restrict__ out) {
-  constexpr int U = 4;
-  float4 acc = {0.f, 0.f, 0.f, 0.f};
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  constexpr int U = 8;
+  const f32x4* p = reinterpret_cast<const f32x4*>(in);
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
   const long stride = (long)gridDim.x * kNT;
   long i = (long)blockIdx.x * kNT + threadIdx.x;
   for (; i + (U - 1) * stride < n4; i += U * stride) {
-    float4 v[U];
+    f32x4 v[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) v[u] = in[i + u * stride];
+    for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(p + i + u * stride);
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      acc.x += v[u].x;
-      acc.y += v[u].y;
-      acc.z += v[u].z;
-      acc.w += v[u].w;
-    }
+    for (int u = 0; u < U; u++) acc += v[u];
   }
-  for (; i < n4; i += stride) acc.x += in[i].x;
+  for (; i < n4; i += stride) acc += __builtin_nontemporal_load(p + i);
   if (acc.x + acc.y + acc.z + acc.w == 12345.f) out[0] = 1.f;
 }
 template <typename T>

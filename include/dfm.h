@@ -253,8 +253,8 @@ int dfm_bench_kernel(dfm_bcrs *A, dfm_ilu *M, int which, int precision, int nrep
 /* Device-to-device copy of `bytes` (16-byte vector loads and stores) with the
  * library's launch shape: the practical bandwidth ceiling of a streaming kernel. */
 int dfm_bench_copy(size_t bytes, int nrep, double *avg_ms);
-/* Streaming read of `bytes` (16-byte vector loads, four in flight per lane, no
- * write stream): the practical read-bandwidth ceiling that SpMV and the
+/* Streaming read of `bytes` (non-temporal 16-byte loads, eight in flight per
+ * lane, no write stream): the practical read-bandwidth ceiling that SpMV and the
  * triangular sweeps, which read far more than they write, are held against. */
 int dfm_bench_read(size_t bytes, int nrep, double *avg_ms);
 
