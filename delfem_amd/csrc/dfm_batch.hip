@@ -66,7 +66,7 @@ struct dfm_batch_s {
   const dfm_bcrs_s* A = nullptr;
   int block = 512, wpc = 1;              // launch shape
   unsigned team_min_rows = 16384;        // systems at least this large get a team of workgroups
-  int team_max = 1;                      // workgroups per team (1: no teams, the default)
+  int team_max = 4;                      // workgroups per team (1: no teams)
   std::string items_key;                 // launch shape the work items were built for
   dfm::DevBuf<char> d_items, d_teams;
   dfm::DevBuf<int> d_fail;
@@ -141,15 +141,12 @@ __device__ inline double wg_sum(double v, double (*sh)[16], int& buf) {
 
 // The workgroups working on one system.  With K = 1 it is the workgroup
 // alone (barriers are workgroup barriers).  With K > 1 every member owns a
-// contiguous share of each row range; a team barrier is: an agent-scope
-// release in every wave, a workgroup barrier, an increment of the system's
-// counter, a wait until all K members arrived (relaxed polls, bounded: a
-// timeout sets *fail and the launch completes with the result marked
-// invalid), a workgroup barrier and an agent-scope acquire in every wave.
-// The per-wave cache maintenance makes a team barrier cost far more than the
-// work it saves on systems of up to 50 k rows, so teams are off by default
-// (team_max = 1); they pay only where a system's iteration is much longer
-// than a barrier.  Team reductions write each member's workgroup sum to a
+// contiguous share of each row range; a team barrier is: every wave waits for
+// its stores, a workgroup barrier, then wave 0 makes an agent-scope release,
+// increments the system's counter, waits until all K members arrived
+// (relaxed polls, bounded: a timeout sets *fail and the launch completes with
+// the result marked invalid) and makes an agent-scope acquire, and a second
+// workgroup barrier.  Team reductions write each member's workgroup sum to a
 // slot, pass a team barrier and add the K slots in member order: identical
 // in every member.
 struct Team {
@@ -168,14 +165,19 @@ __device__ inline void tsync(Team& t) {
     __syncthreads();
     return;
   }
-  // every wave releases its own stores at agent scope before the workgroup
-  // barrier and acquires after it: with the release and acquire made by one
-  // wave only, other members have read stale rows on this device
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // Every wave waits until L2 has acknowledged its own stores (a
+  // workgroup-scope barrier does not wait for them on this device), so that
+  // after the workgroup barrier one agent-scope release -- an L2 write-back --
+  // by wave 0 covers the stores of all waves.  Wave 0 then arrives, polls
+  // and makes the agent-scope acquire, which invalidates the CU's L1 (shared
+  // by all waves of the workgroup) and the XCD's L2 before the other waves
+  // pass the second workgroup barrier.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   t.nsync++;
   if (threadIdx.x == 0 && !__hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-    __hip_atomic_fetch_add(&t.ts->arrive, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(&t.ts->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned target = t.nsync * (unsigned)t.K;
     unsigned spins = 0;
     while (__hip_atomic_load(&t.ts->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -185,9 +187,9 @@ __device__ inline void tsync(Team& t) {
         break;
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 __device__ inline double tsum(Team& t, double v) {

@@ -155,4 +155,47 @@ def test_batch_teams(batch, orc, prec):
                 y = orc.matvec(n, 1, S["rowptr"], S["col"], S["val"], S["dia"], -1.0, xg[r0:r1], 1.0, S["res"])
                 assert np.linalg.norm(y) <= 1.01e-8 * np.linalg.norm(S["res"]), s
     finally:
-        B.set_teams(16384, 1)
+        B.set_teams(16384, 4)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_f32"])
+def test_batch_default_teams_large(orc, gpu, prec):
+    """A system above the default team threshold (16,384 rows) beside small
+    ones: solved by a team of workgroups with the default settings; factor
+    bit-exact, FP64 iterations within one of the oracle and 1e-8 in the
+    solution, FP32 refinement to the FP64 tolerance on the true residual."""
+    import torch
+    _case(f"batch default teams {prec}")
+    parts = [meshgen.helmholtz2d(150, 130, 2.0, 1.0, seed=5, c=2.0),
+             meshgen.helmholtz2d(12, 9, 1.5, 1.0, seed=6, c=0.5, offset=50),
+             meshgen.helmholtz2d(20, 20, 1.0, 1.0, seed=7, c=4.0, offset=90)]
+    prob = meshgen.concat(parts)
+    assert int(prob.sys_ptr[1]) >= 16384
+    ref = assemble_problem(orc, prob)
+    fem = gpu.FEMSystem(prob)
+    fem.assemble()
+    B = gpu.BatchSystem(fem, prob.sys_ptr)
+    B.SetValue()
+    rhs = fem.res.clone()
+    x = torch.zeros_like(rhs)
+    tol = 1e-10 if prec == "fp64" else 1e-8
+    ok, out = B.solve(rhs, x, tol, 5000, precision=prec)
+    assert ok and np.all(out["status"] == 1) and np.all(out["relres"] < 1.01 * tol)
+    xg = x.cpu().numpy()
+    for s in range(B.nsys):
+        r0, r1 = int(prob.sys_ptr[s]), int(prob.sys_ptr[s + 1])
+        S = sub_system(ref, r0, r1)
+        n = r1 - r0
+        _, _, order = orc.colour_order(S["rowptr"], S["col"])
+        F = orc.iluk_symbolic(S["rowptr"], S["col"], 0, order)
+        ov, od, _ = orc.ilu_values(n, 1, S["rowptr"], S["col"], S["val"], S["dia"], F, order)
+        fval, fdia, _ = B.factor(s)
+        assert np.array_equal(fval, ov) and np.array_equal(fdia, od), s
+        if prec == "fp64":
+            _, _, it_o, x_o, _ = orc.solve("pcg", n, 1, S["rowptr"], S["col"], S["val"], S["dia"], F, ov, od,
+                                           S["res"], 1e-10, 5000, order)
+            assert abs(int(out["iters"][s]) - it_o) <= 1, (s, out["iters"][s], it_o)
+            assert np.linalg.norm(xg[r0:r1] - x_o) <= 1e-8 * np.linalg.norm(x_o), s
+        else:
+            y = orc.matvec(n, 1, S["rowptr"], S["col"], S["val"], S["dia"], -1.0, xg[r0:r1], 1.0, S["res"])
+            assert np.linalg.norm(y) <= 1.01e-8 * np.linalg.norm(S["res"]), s
