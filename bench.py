@@ -32,11 +32,11 @@ METRIC = "ms per ILU-PCG solve to 1e-10 rel-res (mixed vs FP64); HBM GB/s % of p
 
 CONFIGS = {
     "cfg1": dict(kind="poisson2d", n=128, sigma=0.5, tol=1e-8, lev=0, ordering="multicolour", solver="pcg",
-                 mixed="mixed_f32_bf16",
+                 mixed="mixed_f32_bf16", headline="fp64",
                  workload="2-D Poisson P1, 128x128 cells (16,641 nodes), lognormal(0,0.5) diffusivity, "
                           "ILU(0)-PCG to 1e-8"),
     "cfg2": dict(kind="solid3d", n=48, sigma=0.0, tol=1e-10, lev=0, ordering="multicolour", solver="pcg",
-                 mixed="mixed_f32_bf16",
+                 mixed="mixed_f32_bf16", headline="fp64",
                  workload="3-D linear elasticity P1 tets, 48^3 grid (117,649 nodes, 352,947 DOF), E=1 nu=0.3, "
                           "ILU(0)-PCG to 1e-10"),
     "cfg3": dict(kind="solid3d", n=96, sigma=1.0, tol=1e-10, lev=0, ordering="multicolour", solver="pcg",
@@ -183,10 +183,14 @@ def main():
             f"({100 * v['frac']:.1f}% of peak)")
     ceiling = read_ceiling(_lib)
     log(f"[bench] streaming read of 2 GiB (library read kernel): {ceiling:.0f} GB/s")
-    dom = kern[f"spmv_{mixed}"]
+    # the configuration's own precision: FP64 where it asks for plain FP64
+    # (cfg1, cfg2), the mixed mode otherwise
+    head = cfg.get("headline", mixed)
+    head_ms = fp64_ms if head == "fp64" else mixed_ms
+    dom = kern[f"spmv_{head}"]
     kname = {"mixed_f32": "k_spmv<3, float, float, true, true>", "mixed_f32_bf16": "k_spmv<3, float, float, true, true>",
              "mixed": "k_spmv<3, float, double, true, true>", "mixed_bf16": "k_spmv<3, float, double, true, true>"
-             }.get(mixed, "k_spmv<3, double, double, true, true>")
+             }.get(head, "k_spmv<3, double, double, true, true>")
     tr = read_traffic(kname) if args.config == "cfg3" else None
 
     cpu = None
@@ -201,21 +205,23 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": round(mixed_ms, 3),
+        "value": round(head_ms, 3),
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(mixed_ms, 3),
+        "ms_per_step": round(head_ms, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if mixed in ("mixed_f32", "mixed_f32_bf16") else "f64",
+        "dtype": "f32" if head in ("mixed_f32", "mixed_f32_bf16") else "f64",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "config": args.config, "seed": args.seed,
                    "preconditioner": f"ILU({cfg['lev']}), {cfg['ordering']} ordering",
                    "ncolour": info["ncolour"], "levels": [info["nlevel_lower"], info["nlevel_upper"]],
-                   "tol": cfg["tol"], "precision": mixed},
+                   "tol": cfg["tol"], "precision": head},
+        "mixed_ms": round(mixed_ms, 3),
+        "mixed_precision": mixed,
         "fp64_ms": round(fp64_ms, 3),
         "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
         "iters_fp64": iters_d,

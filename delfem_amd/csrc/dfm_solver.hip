@@ -511,6 +511,7 @@ static int mixed_core(dfm_ilu_s* M, bool bicg, const double* b, double* r64, dou
   const std::string tag = std::string(bicg ? "bicgR" : "pcgR") + tname<T>() + tname<TA>() + tname<TF>();
   const std::string key = graph_key(tag, B, (int)check_every, {rr_, xr, ws, M->partials.p, M->state.p});
   double rr_ref = rr0, rr_true = rr0;
+  double rr_low = rr0;  // smallest recursive <r,r> seen since the last update
   bool converged = false;
   for (;;) {
     if (!M->h_state->done) {
@@ -520,11 +521,22 @@ static int mixed_core(dfm_ilu_s* M, bool bicg, const double* b, double* r64, dou
       DFM_TRY(read_state(M));
     }
     KState& h = *M->h_state;
-    if (!(h.done || h.rr < delta2 * rr_ref)) continue;
-    // reliable update
-    DFM_LAUNCH(k_reliable_update<T>, grid_vec(n), n, xr, x64);
-    DFM_TRY(true_residual64<B>(M, b, x64, r64, &rr_true));
-    nupdates++;
+    // A low-precision recurrence that has lost its way (BiCGSTAB on FP32
+    // vectors can: the residual grows by orders of magnitude or overflows)
+    // is stopped by an update and restarted from the FP64 true residual.
+    const bool diverged = !std::isfinite(h.rr) || h.rr > 1.0e4 * rr_low;
+    if (std::isfinite(h.rr)) rr_low = std::min(rr_low, h.rr);
+    if (!(h.done || h.rr < delta2 * rr_ref || diverged)) continue;
+    if (std::isfinite(h.rr)) {
+      // reliable update
+      DFM_LAUNCH(k_reliable_update<T>, grid_vec(n), n, xr, x64);
+      DFM_TRY(true_residual64<B>(M, b, x64, r64, &rr_true));
+      nupdates++;
+    } else {
+      // the partial solution overflowed: drop it and restart from the last
+      // true residual (x64 and r64 still hold it)
+      DFM_LAUNCH(k_fill<T>, grid_vec(n), n, (T)0, xr);
+    }
     KState& hs = *M->h_state;
     if (rr_true * hs.inv_rr0 < hs.tol2) {
       converged = true;
@@ -537,8 +549,9 @@ static int mixed_core(dfm_ilu_s* M, bool bicg, const double* b, double* r64, dou
     // by more than 10x in norm (it converged on the low-precision operator
     // within one check block) its directions no longer fit the replaced
     // residual: restart the recurrence from the true residual.
-    const bool restart = rr_true > 100.0 * hs.rr;
+    const bool restart = diverged || rr_true > 100.0 * hs.rr;
     rr_ref = rr_true;
+    rr_low = rr_true;
     hs.done = 0;
     hs.rr = rr_true;
     DFM_TRY(write_state(M));

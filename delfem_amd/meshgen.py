@@ -131,6 +131,51 @@ def solid3d(n=48, seed=12345, sigma=0.0, E=1.0, nu=0.3, g=(0.0, 0.0, -1.0), rho=
                    rho=rho, g=tuple(g), flags=flags.ravel(), seed=seed, sigma=sigma)
 
 
+def read_gid_msh(path):
+    """Nodes and elements of a GiD ASCII mesh (one MESH block: a header line,
+    `Coordinates` ... `end coordinates` with `id x y [z]` lines, `Elements` ...
+    `end elements` with `id n1 .. nk [material]` lines; ids 1-based), the
+    format Msh::CMesh3D::ReadFromFile_GiDMsh reads (src/msh/mesh3d.cpp:601-700).
+    Returns (coord[nnode, dim], conn[nelem, k]) with 0-based node indices in
+    the file's order."""
+    with open(path) as f:
+        lines = [ln.strip() for ln in f]
+    head = lines[0].split()
+    dim = int(head[head.index("dimension") + 1])
+    npe = int(head[head.index("Nnode") + 1])
+    c0, c1 = lines.index("Coordinates"), lines.index("end coordinates")
+    e0, e1 = lines.index("Elements"), lines.index("end elements")
+    rows = [ln.split() for ln in lines[c0 + 1:c1] if ln]
+    ids = np.array([int(r[0]) for r in rows])
+    coord = np.array([[float(v) for v in r[1:1 + dim]] for r in rows])
+    if not np.array_equal(ids, np.arange(1, ids.size + 1)):
+        raise ValueError(f"{path}: node ids are not 1..n in order")
+    conn = np.array([[int(v) for v in ln.split()[1:1 + npe]] for ln in lines[e0 + 1:e1] if ln], dtype=np.int64) - 1
+    return np.ascontiguousarray(coord), np.ascontiguousarray(conn.astype(np.uint32))
+
+
+def solid3d_mesh(coord, conn, E=200.0, nu=0.43, g=(0.0, 0.0, -1.0), rho=1.0, fix_fraction=0.01, name="mesh"):
+    """Linear elasticity on a given (positively oriented) tetrahedral mesh: E,
+    nu uniform, body force rho*g, displacement fixed at the nodes within
+    fix_fraction of the mesh's length from one end of its principal axis."""
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    conn = np.ascontiguousarray(conn, dtype=np.uint32)
+    p = coord[conn.astype(np.int64)]
+    vol = np.einsum("ij,ij->i", np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), p[:, 3] - p[:, 0])
+    if np.any(vol <= 0.0):
+        raise ValueError("tetrahedra must be positively oriented")
+    c = coord - coord.mean(axis=0)
+    axis = np.linalg.eigh(c.T @ c)[1][:, -1]
+    t = c @ axis
+    nnode, nelem = coord.shape[0], conn.shape[0]
+    flags = np.zeros((nnode, 3), dtype=np.int32)
+    flags[t <= t.min() + fix_fraction * (t.max() - t.min()), :] = 1
+    lam, mu = lame(np.full(nelem, float(E)), nu)
+    return Problem(kind="solid3d", name=name, n=0, b=3, npe=4, coord=coord, conn=conn, nnode=nnode, nelem=nelem,
+                   lam=np.ascontiguousarray(lam), mu=np.ascontiguousarray(mu), rho=rho, g=tuple(g),
+                   flags=flags.ravel(), seed=0, sigma=0.0)
+
+
 def fourier_stream_velocity(coord, seed, modes=4):
     """Smooth divergence-free advection field on the unit square: the velocity
     (d psi/dy, -d psi/dx) of psi = sum_{k,l=1..modes} c_kl sin(k pi x) sin(l pi y),

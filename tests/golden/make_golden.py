@@ -120,6 +120,41 @@ def mat_solver_fixture(out, rs, res, tol=1e-10, maxit=2000):
         out[f"mat_{kind}_r"] = r
 
 
+MSH = "/root/reference/test_glut/input_file/cylinder_tet.msh"
+
+
+def gid_mesh_fixture(ref):
+    """The reference's unstructured tetrahedral mesh (its solid3d demo input),
+    elasticity with E = 200, nu = 0.43 as in that demo: mesh arrays, the
+    reference's assembled system, natural ILU(0) and Solve_PCG results."""
+    coord, conn = meshgen.read_gid_msh(MSH)
+    prob = meshgen.solid3d_mesh(coord, conn, name="cylinder_tet")
+    out, rs, rp, cl, val, dia, res = fem_fixture(ref, prob)
+    rf = O.RefFrac(rs, 0)
+    ok = rf.set_value()
+    fv, fd = rf.values()
+    out.update(fval_natural_k0=fv, fdiav_natural_k0=fd, fok_natural_k0=np.array(ok))
+    for kind in ("pcg", "bicg"):
+        rs.set_vector(-1, res)
+        ok2, conv, it = rf.solve(kind, 1e-10, 3000)
+        out[f"{kind}_natural_k0"] = np.array([float(ok2), conv, float(it)])
+        out[f"{kind}_x_natural_k0"] = rs.get_vector(-2)
+        out[f"{kind}_r_natural_k0"] = rs.get_vector(-1)
+    rs.set_vector(-1, res)
+    # the large outputs are kept as SHA-256 digests of their bytes (bitwise
+    # comparison at a fraction of the size)
+    for k in ("rowptr", "col", "val", "dia", "emat", "eres", "fval_natural_k0", "fdiav_natural_k0",
+              "pcg_r_natural_k0", "bicg_r_natural_k0"):
+        out[f"sha_{k}"] = np.array(digest(out.pop(k)))
+    return out
+
+
+def digest(a):
+    """SHA-256 of an array's bytes (little-endian, C order)."""
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
 def random_graph_fixture(ref, n=60, b=2, seed=7):
     """A random symmetric graph with 2x2 blocks: diagonally dominant values."""
     rng = np.random.default_rng(seed)
@@ -164,6 +199,7 @@ def main():
                     max_value_lev=1 if prob.b == 1 else 0)
         mat_solver_fixture(out, rs, res)
         fixtures[name] = out
+    fixtures["cylinder_tet"] = gid_mesh_fixture(ref)
     out, rs, res = random_graph_fixture(ref)
     rng = np.random.default_rng(13)
     orders = {"natural": None, "random": rng.permutation(60).astype(np.int32)}

@@ -146,3 +146,38 @@ def test_matrix_layer_solvers(orc, name, kind):
     want = d[f"mat_{kind}"]
     assert (float(ok), conv, float(it)) == (want[0], want[1], want[2])
     assert np.array_equal(u, d[f"mat_{kind}_u"]) and np.array_equal(r, d[f"mat_{kind}_r"])
+
+
+def sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_unstructured_mesh(orc):
+    """The reference's own tetrahedral mesh (747 nodes, 3,114 elements,
+    E = 200, nu = 0.43): element matrices, pattern, assembled system, natural
+    ILU(0) factor and the PCG / BiCGSTAB runs bit for bit (large arrays by
+    SHA-256 digest)."""
+    d = load("cylinder_tet")
+    nb = int(d["nnode"])
+    conn = d["conn"]
+    emat, eres = orc.emat_solid3d(d["coord"], conn, d["lam"], d["mu"], float(d["rho"]), tuple(d["g"]),
+                                  np.zeros(nb * 3))
+    ne = int(d["nemat"])
+    assert sha(emat[:ne * 144]) == str(d["sha_emat"]) and sha(eres[:ne * 12]) == str(d["sha_eres"])
+    rowptr, col = orc.fem_pattern(nb, conn)
+    assert sha(rowptr) == str(d["sha_rowptr"]) and sha(col) == str(d["sha_col"])
+    val, dia = orc.assemble(nb, 3, rowptr, col, conn, emat)
+    res = orc.assemble_vec(nb, 3, conn, eres)
+    val, dia = orc.set_bc(nb, 3, rowptr, col, val, dia, d["flags"])
+    res = orc.zero_bc(d["flags"], res)
+    assert sha(val) == str(d["sha_val"]) and sha(dia) == str(d["sha_dia"])
+    assert np.array_equal(res, d["res"])
+    F = orc.iluk_symbolic(rowptr, col, 0)
+    fv, fd, _ = orc.ilu_values(nb, 3, rowptr, col, val, dia, F)
+    assert sha(fv) == str(d["sha_fval_natural_k0"]) and sha(fd) == str(d["sha_fdiav_natural_k0"])
+    for kind in ("pcg", "bicg"):
+        ok, conv, it, x, r = orc.solve(kind, nb, 3, rowptr, col, val, dia, F, fv, fd, res, 1e-10, 3000)
+        want = d[f"{kind}_natural_k0"]
+        assert (float(ok), conv, float(it)) == (want[0], want[1], want[2]), kind
+        assert np.array_equal(x, d[f"{kind}_x_natural_k0"]) and sha(r) == str(d[f"sha_{kind}_r_natural_k0"])
