@@ -345,6 +345,16 @@ class FEMSystem:
         call("dfm_vec_zero_bc", self.res.numel(), _ptr(self.flags), _dptr(self.res))
         return dot(self.res, self.res) ** 0.5
 
+    def UpdateValueOfField(self, update):
+        """CLinearSystem_Field::UpdateValueOfField (src/femls/linearsystem_field.cpp:715-793)
+        for the node values: value += update (CNodeAry::AddValueToNodeSegment with
+        factor 1.0; a combined (u, v, p) segment updates its components in block
+        order).  The next assemble() forms the residual at the new values."""
+        _sync_torch()
+        if update.numel() != self.value.numel():
+            raise ValueError("update and field sizes differ")
+        call("dfm_vec_axpy", self.value.numel(), 1.0, _dptr(update), _dptr(self.value))
+
 
 class BatchSystem:
     """Many independent scalar systems (the diagonal blocks of one assembled

@@ -241,3 +241,27 @@ def test_unstructured_mesh_vs_reference(gpu, orc):
         pcg = kind == "pcg"
         assert abs(it - int(want[2])) <= (1 if pcg else max(2, int(want[2]) // 4)), (kind, it, want[2])
         assert np.linalg.norm(x.cpu().numpy() - x_ref) <= (1e-8 if pcg else 1e-6) * np.linalg.norm(x_ref)
+
+
+def test_field_update_cycle(setup):
+    """The femls loop around the solver: assemble, solve for the update in
+    FP64, UpdateValueOfField (value += update), assemble again at the new
+    values.  For these linear problems the new residual is the solver's
+    final residual: below 1e-9 of the first, with the matrix unchanged."""
+    prob, gpu, torch = setup["prob"], setup["gpu"], setup["torch"]
+    _case(f"field update {prob.name}")
+    s = gpu.FEMSystem(prob)
+    n0 = s.assemble()
+    val0, dia0 = s.A.values()
+    M = gpu.CMatDiaFrac_BlkCrs(1 if prob.kind == "oseen2d" else 0, s.A)
+    assert M.SetValue(s.A)
+    fn = gpu.Solve_PBiCGSTAB if prob.kind == "oseen2d" else gpu.Solve_PCG
+    r = s.res.clone()
+    x = torch.zeros_like(r)
+    ok, conv, it, _ = fn(1e-11, 5000, s.A, M, r, x)
+    assert ok
+    s.UpdateValueOfField(x)
+    n1 = s.assemble()
+    val1, dia1 = s.A.values()
+    assert np.array_equal(val0, val1) and np.array_equal(dia0, dia1)
+    assert n1 <= 1e-9 * n0, (n1, n0)
