@@ -521,10 +521,12 @@ static int mixed_core(dfm_ilu_s* M, bool bicg, const double* b, double* r64, dou
       DFM_TRY(read_state(M));
     }
     KState& h = *M->h_state;
-    // A low-precision recurrence that has lost its way (BiCGSTAB on FP32
-    // vectors can: the residual grows by orders of magnitude or overflows)
-    // is stopped by an update and restarted from the FP64 true residual.
-    const bool diverged = !std::isfinite(h.rr) || h.rr > 1.0e4 * rr_low;
+    // A recurrence on FP32 vectors that has lost its way (BiCGSTAB can: the
+    // residual grows by orders of magnitude or overflows) is stopped by an
+    // update and restarted from the FP64 true residual.  With FP64 vectors
+    // only an overflow counts: BiCGSTAB's residual swings by such factors on
+    // its way down, and a restart throws away its Krylov space.
+    const bool diverged = !std::isfinite(h.rr) || (std::is_same<T, float>::value && h.rr > 1.0e4 * rr_low);
     if (std::isfinite(h.rr)) rr_low = std::min(rr_low, h.rr);
     if (!(h.done || h.rr < delta2 * rr_ref || diverged)) continue;
     if (std::isfinite(h.rr)) {
