@@ -176,6 +176,31 @@ def solid3d_mesh(coord, conn, E=200.0, nu=0.43, g=(0.0, 0.0, -1.0), rho=1.0, fix
                    flags=flags.ravel(), seed=0, sigma=0.0)
 
 
+def zhelmholtz2d(nx, ny=None, lx=1.0, ly=1.0, wave_length=0.25):
+    """Complex Helmholtz (acoustic) problem of the reference's complex twin:
+    -Lap u - k^2 u on an lx x ly rectangle of nx x ny cells (P1 triangles,
+    src/femeqn/eqn_helmholtz.cpp:51-133) with Sommerfeld radiation line
+    elements on the bottom, right and top edges (:437-497) and the incoming
+    wave u = 1 prescribed on the x = 0 edge.  `value` holds that boundary
+    value (complex); the linear system is solved for the update.  Scalar
+    unknowns (b = 1), complex symmetric matrix."""
+    ny = nx if ny is None else ny
+    coord, conn = tri_grid(nx, ny, lx, ly)
+    nnode = coord.shape[0]
+    node = lambda i, j: j * (nx + 1) + i
+    lines = [(node(i, 0), node(i + 1, 0)) for i in range(nx)]
+    lines += [(node(nx, j), node(nx, j + 1)) for j in range(ny)]
+    lines += [(node(i + 1, ny), node(i, ny)) for i in range(nx)]
+    lconn = np.ascontiguousarray(np.array(lines, dtype=np.uint32))
+    flags = np.zeros(nnode, dtype=np.int32)
+    flags[np.isclose(coord[:, 0], 0.0)] = 1
+    value = np.zeros(nnode, dtype=np.complex128)
+    value[flags == 1] = 1.0
+    return Problem(kind="zhelmholtz2d", name=f"zhelmholtz2d_{nx}x{ny}", n=nx, b=1, npe=3, coord=coord, conn=conn,
+                   lconn=lconn, nnode=nnode, nelem=conn.shape[0], wave_length=float(wave_length), flags=flags,
+                   value=value, seed=0)
+
+
 def fourier_stream_velocity(coord, seed, modes=4):
     """Smooth divergence-free advection field on the unit square: the velocity
     (d psi/dy, -d psi/dx) of psi = sum_{k,l=1..modes} c_kl sin(k pi x) sin(l pi y),

@@ -997,6 +997,269 @@ int orc_solve_mat(int kind, unsigned nb, unsigned b,
 }
 
 /* ------------------------------------------------------------------------- */
+/* Complex twin (scalar blocks only, as the reference's complex ILU):
+ * Com::Complex arithmetic (include/delfem/complex.h:82-160) written out in
+ * the same operation order; a complex value is two doubles (re, im). */
+typedef struct { double re, im; } zc;
+static zc zmul(zc a, zc b) { zc r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
+static zc zsub(zc a, zc b) { zc r = {a.re - b.re, a.im - b.im}; return r; }
+static zc zadd(zc a, zc b) { zc r = {a.re + b.re, a.im + b.im}; return r; }
+static zc zdiv(zc a, zc b)
+{
+    const double f = 1.0 / (b.re * b.re + b.im * b.im);
+    zc r = {(a.re * b.re + a.im * b.im) * f, (-a.re * b.im + a.im * b.re) * f};
+    return r;
+}
+static zc zrdiv(double d, zc c)   /* double / Complex */
+{
+    const double f = d / (c.re * c.re + c.im * c.im);
+    zc r = {f * c.re, -f * c.im};
+    return r;
+}
+static zc zdscale(double d, zc c) { zc r = {d * c.re, d * c.im}; return r; }   /* double * Complex */
+
+/* CZMatDia_BlkCrs::Mearge (src/matvec/zmatdia_blkcrs.cpp:198-237) of all
+ * elements in order; emat per element npe*npe complex, row-major. */
+void orc_zassemble(unsigned nb, const unsigned *rowptr, const unsigned *col, unsigned nelem, unsigned npe,
+                   const unsigned *conn, const double *emat, double *val, double *dia)
+{
+    int *pos = (int *)malloc(sizeof(int) * (size_t)nb);
+    for (unsigned i = 0; i < nb; i++) pos[i] = -1;
+    zc *v = (zc *)val, *d = (zc *)dia;
+    const zc *em = (const zc *)emat;
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *no = conn + (size_t)e * npe;
+        for (unsigned a = 0; a < npe; a++) {
+            const unsigned i = no[a];
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) pos[col[q]] = (int)q;
+            for (unsigned bq = 0; bq < npe; bq++) {
+                const zc x = em[((size_t)e * npe + a) * npe + bq];
+                if (a == bq) d[i] = zadd(d[i], x);
+                else v[pos[no[bq]]] = zadd(v[pos[no[bq]]], x);
+            }
+        }
+    }
+    free(pos);
+}
+/* CZVector_Blk::AddValue of element residuals, element order */
+void orc_zassemble_vec(double *res, unsigned nelem, unsigned npe, const unsigned *conn, const double *eres)
+{
+    zc *r = (zc *)res;
+    const zc *er = (const zc *)eres;
+    for (unsigned e = 0; e < nelem; e++)
+        for (unsigned a = 0; a < npe; a++) {
+            const unsigned i = conn[(size_t)e * npe + a];
+            r[i] = zadd(r[i], er[(size_t)e * npe + a]);
+        }
+}
+/* CZMatDia_BlkCrs::SetBoundaryCondition (zmatdia_blkcrs.cpp:239-268), len 1,
+ * and the residual zeroed at fixed nodes */
+void orc_zset_bc(unsigned nb, const unsigned *rowptr, const unsigned *col, double *val, double *dia,
+                 const int *flag, double *res)
+{
+    zc *v = (zc *)val, *d = (zc *)dia, *r = (zc *)res;
+    const zc zero = {0.0, 0.0}, one = {1.0, 0.0};
+    for (unsigned i = 0; i < nb; i++) {
+        if (flag[i] == 0) continue;
+        d[i] = one;
+        for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) v[q] = zero;
+        if (r) r[i] = zero;
+    }
+    for (unsigned q = 0; q < rowptr[nb]; q++)
+        if (flag[col[q]]) v[q] = zero;
+}
+/* CZMatDia_BlkCrs::MatVec, BlkSize 1 (zmatdia_blkcrs.cpp:737-770) */
+void orc_zmatvec(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                 double alpha, const double *x, double beta, double *y)
+{
+    const zc *v = (const zc *)val, *d = (const zc *)dia, *xv = (const zc *)x;
+    zc *yv = (zc *)y;
+    for (unsigned i = 0; i < nb; i++) {
+        double re = 0.0, im = 0.0;
+        for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+            const zc a = v[q], b = xv[col[q]];
+            re += alpha * (a.re * b.re - a.im * b.im);
+            im += alpha * (a.re * b.im + a.im * b.re);
+        }
+        zc t = {yv[i].re * beta, yv[i].im * beta};
+        t.re += re;
+        t.im += im;
+        yv[i] = zadd(t, zmul(zdscale(alpha, d[i]), xv[i]));
+    }
+}
+/* CZMatDiaFrac_BlkCrs::SetValue (copy of A into the factor pattern, fill
+ * zero) with an optional order[new] = old, as the real ordered copy */
+void orc_zilu_setvalue(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val,
+                       const double *dia, const int *order, const unsigned *frowptr, const unsigned *fcol,
+                       double *fval, double *fdia)
+{
+    int *inv = NULL;
+    int *pos = (int *)malloc(sizeof(int) * (size_t)nb);
+    if (order) {
+        inv = (int *)malloc(sizeof(int) * (size_t)nb);
+        for (unsigned i = 0; i < nb; i++) inv[order[i]] = (int)i;
+    }
+    memset(fval, 0, sizeof(double) * 2 * (size_t)frowptr[nb]);
+    for (unsigned i = 0; i < nb; i++) pos[i] = -1;
+    for (unsigned i = 0; i < nb; i++) {
+        const unsigned io = order ? (unsigned)order[i] : i;
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = (int)q;
+        for (unsigned q = rowptr[io]; q < rowptr[io + 1]; q++) {
+            const unsigned j = inv ? (unsigned)inv[col[q]] : col[q];
+            if (pos[j] == -1) continue;
+            fval[2 * (size_t)pos[j]] = val[2 * (size_t)q];
+            fval[2 * (size_t)pos[j] + 1] = val[2 * (size_t)q + 1];
+        }
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = -1;
+        fdia[2 * (size_t)i] = dia[2 * (size_t)io];
+        fdia[2 * (size_t)i + 1] = dia[2 * (size_t)io + 1];
+    }
+    free(pos); free(inv);
+}
+/* CZMatDiaFrac_BlkCrs::DoILUDecomp (zmatdiafrac_blkcrs.cpp:192-264): returns
+ * the number of pivots with |d|^2 < 1e-20 (the reference prints and waits
+ * for a key on such a pivot, then divides anyway) */
+int orc_zilu_decomp(unsigned nb, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                    double *fval, double *fdia)
+{
+    zc *v = (zc *)fval, *d = (zc *)fdia;
+    int *r2c = (int *)malloc(sizeof(int) * (size_t)nb);
+    int nsmall = 0;
+    for (unsigned j = 0; j < nb; j++) r2c[j] = -1;
+    for (unsigned i = 0; i < nb; i++) {
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) r2c[fcol[q]] = (int)q;
+        for (unsigned ik = frowptr[i]; ik < frowptr[i + 1]; ik++) {
+            const unsigned k = fcol[ik];
+            if (k >= i) break;
+            const zc ikv = v[ik];
+            for (unsigned kj = fdiaind[k]; kj < frowptr[k + 1]; kj++) {
+                const unsigned j = fcol[kj];
+                const zc kjv = v[kj];
+                if (j != i) {
+                    const int ij = r2c[j];
+                    if (ij == -1) continue;
+                    v[ij] = zsub(v[ij], zmul(ikv, kjv));
+                } else {
+                    d[i] = zsub(d[i], zmul(ikv, kjv));
+                }
+            }
+        }
+        const zc ii = d[i];
+        if (ii.re * ii.re + ii.im * ii.im < 1.0e-20) nsmall++;
+        d[i] = zrdiv(1.0, ii);
+        for (unsigned q = fdiaind[i]; q < frowptr[i + 1]; q++) v[q] = zmul(v[q], d[i]);
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) r2c[fcol[q]] = -1;
+    }
+    free(r2c);
+    return nsmall;
+}
+/* ForwardSubstitution / BackwardSubstitution, len 1 (zmatdiafrac_blkcrs.cpp:116-157) */
+void orc_zilu_forward(unsigned nb, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                      const double *fval, const double *fdia, double *vec)
+{
+    const zc *v = (const zc *)fval, *d = (const zc *)fdia;
+    zc *x = (zc *)vec;
+    for (unsigned i = 0; i < nb; i++) {
+        zc t = x[i];
+        for (unsigned q = frowptr[i]; q < fdiaind[i]; q++) t = zsub(t, zmul(v[q], x[fcol[q]]));
+        x[i] = zmul(d[i], t);
+    }
+}
+void orc_zilu_backward(unsigned nb, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                       const double *fval, const double *fdia, double *vec)
+{
+    const zc *v = (const zc *)fval;
+    zc *x = (zc *)vec;
+    (void)fdia;
+    for (long i = (long)nb - 1; i >= 0; i--) {
+        zc t = x[i];
+        for (unsigned q = fdiaind[i]; q < frowptr[i + 1]; q++) t = zsub(t, zmul(v[q], x[fcol[q]]));
+        x[i] = t;
+    }
+}
+static zc zdot(size_t n, const zc *a, const zc *b)   /* CZVector_Blk operator* (zvector_blk.cpp:36-50) */
+{
+    double re = 0.0, im = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        re += a[i].re * b[i].re - a[i].im * b[i].im;
+        im += a[i].re * b[i].im + a[i].im * b[i].re;
+    }
+    zc r = {re, im};
+    return r;
+}
+static double znorm2(size_t n, const zc *a)   /* GetSquaredVectorNorm (:150-157) */
+{
+    double s = 0.0;
+    for (size_t i = 0; i < n; i++) s += a[i].re * a[i].re + a[i].im * a[i].im;
+    return s;
+}
+static void zprecond(unsigned nb, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                     const double *fval, const double *fdia, const int *order, const int *inv, zc *tmp, zc *w)
+{
+    if (!order) {
+        orc_zilu_forward(nb, frowptr, fcol, fdiaind, fval, fdia, (double *)w);
+        orc_zilu_backward(nb, frowptr, fcol, fdiaind, fval, fdia, (double *)w);
+        return;
+    }
+    for (unsigned i = 0; i < nb; i++) tmp[i] = w[order[i]];
+    orc_zilu_forward(nb, frowptr, fcol, fdiaind, fval, fdia, (double *)tmp);
+    orc_zilu_backward(nb, frowptr, fcol, fdiaind, fval, fdia, (double *)tmp);
+    for (unsigned i = 0; i < nb; i++) w[i] = tmp[inv[i]];
+}
+/* MatVec::Sol::Solve_PCOCG (src/matvec/zsolver_mat_iter.cpp:303-392) with the
+ * ILU factor as preconditioner (order: applied in the factor's numbering
+ * through the permutation, NULL for the reference's natural order).  r: in
+ * right-hand side, out final residual; u: out solution.  Always returns 1
+ * (the reference returns true also at max_iter, conv_ratio then unchanged). */
+int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                     const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const double *fval,
+                     const double *fdia, const int *order, double *r, double *u, double *conv_ratio,
+                     unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = nb;
+    zc *rv = (zc *)r, *uv = (zc *)u;
+    memset(u, 0, sizeof(double) * 2 * n);
+    const double rr0 = znorm2(n, rv);
+    if (rr0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; return 1; }
+    const double inv_rr0 = 1.0 / rr0;
+    zc *p = (zc *)malloc(sizeof(zc) * n), *Ap = (zc *)malloc(sizeof(zc) * n), *w = (zc *)malloc(sizeof(zc) * n);
+    zc *tmp = (zc *)malloc(sizeof(zc) * n);
+    int *inv = NULL;
+    if (order) {
+        inv = (int *)malloc(sizeof(int) * n);
+        for (unsigned i = 0; i < nb; i++) inv[order[i]] = (int)i;
+    }
+    memcpy(w, rv, sizeof(zc) * n);
+    zprecond(nb, frowptr, fcol, fdiaind, fval, fdia, order, inv, tmp, w);
+    memcpy(p, w, sizeof(zc) * n);
+    zc rw = zdot(n, rv, w);
+    *iteration = mx_iter;
+    for (unsigned it = 1; it < mx_iter; it++) {
+        orc_zmatvec(nb, rowptr, col, val, dia, 1.0, (const double *)p, 0.0, (double *)Ap);
+        const zc alpha = zdiv(rw, zdot(n, p, Ap));
+        const zc malpha = {-alpha.re, -alpha.im};
+        for (size_t i = 0; i < n; i++) rv[i] = zadd(rv[i], zmul(malpha, Ap[i]));
+        const double ratio = sqrt(znorm2(n, rv) * inv_rr0);
+        if (ratio < tol) {
+            *conv_ratio = ratio;
+            *iteration = it;
+            break;
+        }
+        for (size_t i = 0; i < n; i++) uv[i] = zadd(uv[i], zmul(alpha, p[i]));
+        memcpy(w, rv, sizeof(zc) * n);
+        zprecond(nb, frowptr, fcol, fdiaind, fval, fdia, order, inv, tmp, w);
+        const zc rw2 = zdot(n, rv, w);
+        const zc beta = zdiv(rw2, rw);
+        rw = rw2;
+        for (size_t i = 0; i < n; i++) p[i] = zadd(zmul(p[i], beta), w[i]);
+    }
+    free(p); free(Ap); free(w); free(tmp); free(inv);
+    return 1;
+}
+
+/* ------------------------------------------------------------------------- */
 /* a4: element kernels.
  * src/femeqn/ker_emat_tri.cpp:12-14 (TriArea), 36-69 (TriDlDx) */
 static double tri_area(const double *p0, const double *p1, const double *p2)
@@ -1427,4 +1690,70 @@ int orc_level_sets(unsigned nb, const unsigned *frowptr, const unsigned *fcol,
         }
     }
     return nlev;
+}
+
+/* Complex Helmholtz elements (src/femeqn/eqn_helmholtz.cpp).  Triangle
+ * (:51-133): emat = area grad.grad - k^2 area/12 (1 + delta), k = 2*3.1416 /
+ * wave_length as written there, eres = -emat value.  Sommerfeld radiation
+ * line (:437-497): emat = i k elen/6 [2 1; 1 2] + i/(2 elen k) [-1 1; 1 -1]. */
+void orc_zemat_helmholtz2d_p1(unsigned nelem, const double *coord, const unsigned *conn, double wave_length,
+                              const double *value, double *emat, double *eres)
+{
+    const zc *val = (const zc *)value;
+    zc *em_all = (zc *)emat, *er_all = (zc *)eres;
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *no = conn + (size_t)e * 3;
+        const double *p0 = coord + 2 * (size_t)no[0], *p1 = coord + 2 * (size_t)no[1], *p2 = coord + 2 * (size_t)no[2];
+        const double area = tri_area(p0, p1, p2);
+        double dldx[3][2];
+        tri_dldx(dldx, p0, p1, p2);
+        zc *em = em_all + (size_t)e * 9;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                em[i * 3 + j].re = area * (dldx[i][0] * dldx[j][0] + dldx[i][1] * dldx[j][1]);
+                em[i * 3 + j].im = 0.0;
+            }
+        const double k = 2 * 3.1416 / wave_length;
+        const double t = k * k * area / 12.0;
+        for (int i = 0; i < 3; i++) {
+            em[i * 3 + i].re -= t;
+            em[i * 3 + i].im -= 0.0;
+            for (int j = 0; j < 3; j++) {
+                em[i * 3 + j].re -= t;
+                em[i * 3 + j].im -= 0.0;
+            }
+        }
+        zc *er = er_all + (size_t)e * 3;
+        for (int i = 0; i < 3; i++) {
+            er[i].re = 0.0;
+            er[i].im = 0.0;
+            for (int j = 0; j < 3; j++) er[i] = zsub(er[i], zmul(em[i * 3 + j], val[no[j]]));
+        }
+    }
+}
+void orc_zemat_sommerfeld_b1(unsigned nline, const double *coord, const unsigned *conn, double wave_length,
+                             const double *value, double *emat, double *eres)
+{
+    const zc *val = (const zc *)value;
+    zc *em_all = (zc *)emat, *er_all = (zc *)eres;
+    const zc unit_i = {0.0, 1.0};
+    for (unsigned e = 0; e < nline; e++) {
+        const unsigned *no = conn + (size_t)e * 2;
+        const double *p0 = coord + 2 * (size_t)no[0], *p1 = coord + 2 * (size_t)no[1];
+        const double elen = sqrt((p0[0] - p1[0]) * (p0[0] - p1[0]) + (p0[1] - p1[1]) * (p0[1] - p1[1]));
+        const double k = 2 * 3.1416 / wave_length;
+        const zc t1 = zdscale(k / 6.0 * elen, unit_i);
+        const zc t2 = zdscale(-1 / (2.0 * elen * k), unit_i);
+        zc *em = em_all + (size_t)e * 4;
+        em[0] = zadd(zdscale(2, t1), t2);
+        em[1] = zsub(t1, t2);
+        em[2] = zsub(t1, t2);
+        em[3] = zadd(zdscale(2, t1), t2);
+        zc *er = er_all + (size_t)e * 2;
+        for (int i = 0; i < 2; i++) {
+            er[i].re = 0.0;
+            er[i].im = 0.0;
+            for (int j = 0; j < 2; j++) er[i] = zsub(er[i], zmul(em[i * 2 + j], val[no[j]]));
+        }
+    }
 }

@@ -128,4 +128,31 @@ int orc_level_sets(unsigned nb, const unsigned *frowptr, const unsigned *fcol,
 #ifdef __cplusplus
 }
 #endif
+
+/* Complex twin, scalar (len 1) blocks; complex arrays are (re, im) double pairs. */
+void orc_zassemble(unsigned nb, const unsigned *rowptr, const unsigned *col, unsigned nelem, unsigned npe,
+                   const unsigned *conn, const double *emat, double *val, double *dia);
+void orc_zassemble_vec(double *res, unsigned nelem, unsigned npe, const unsigned *conn, const double *eres);
+void orc_zset_bc(unsigned nb, const unsigned *rowptr, const unsigned *col, double *val, double *dia,
+                 const int *flag, double *res);
+void orc_zmatvec(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                 double alpha, const double *x, double beta, double *y);
+void orc_zilu_setvalue(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val,
+                       const double *dia, const int *order, const unsigned *frowptr, const unsigned *fcol,
+                       double *fval, double *fdia);
+int orc_zilu_decomp(unsigned nb, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                    double *fval, double *fdia);
+void orc_zilu_forward(unsigned nb, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                      const double *fval, const double *fdia, double *vec);
+void orc_zilu_backward(unsigned nb, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                       const double *fval, const double *fdia, double *vec);
+int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                     const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const double *fval,
+                     const double *fdia, const int *order, double *r, double *u, double *conv_ratio,
+                     unsigned *iteration);
+void orc_zemat_helmholtz2d_p1(unsigned nelem, const double *coord, const unsigned *conn, double wave_length,
+                              const double *value, double *emat, double *eres);
+void orc_zemat_sommerfeld_b1(unsigned nline, const double *coord, const unsigned *conn, double wave_length,
+                             const double *value, double *emat, double *eres);
+
 #endif

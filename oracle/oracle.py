@@ -42,6 +42,10 @@ def i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
 
 
+def c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
 def f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -74,6 +78,20 @@ class Orc:
         L.orc_solve_mat.restype = C.c_int
         L.orc_solve_mat.argtypes = [C.c_int, C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                     C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_zassemble.argtypes = [C.c_uint, vp, vp, C.c_uint, C.c_uint, vp, vp, vp, vp]
+        L.orc_zassemble_vec.argtypes = [vp, C.c_uint, C.c_uint, vp, vp]
+        L.orc_zset_bc.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp]
+        L.orc_zmatvec.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_double, vp, C.c_double, vp]
+        L.orc_zilu_setvalue.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_zilu_decomp.restype = C.c_int
+        L.orc_zilu_decomp.argtypes = [C.c_uint, vp, vp, vp, vp, vp]
+        L.orc_zilu_forward.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp]
+        L.orc_zilu_backward.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp]
+        L.orc_zsolve_pcocg.restype = C.c_int
+        L.orc_zsolve_pcocg.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_zemat_helmholtz2d_p1.argtypes = [C.c_uint, vp, vp, C.c_double, vp, vp, vp]
+        L.orc_zemat_sommerfeld_b1.argtypes = [C.c_uint, vp, vp, C.c_double, vp, vp, vp]
         L.orc_emat_poisson2d_p1.argtypes = [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]
         L.orc_emat_solid3d_p1.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                           C.c_double, vp, vp, vp]
@@ -184,6 +202,80 @@ class Orc:
                _p(u32(F["col"])), _p(u32(F["diaind"])), _p(f64(fval)), _p(f64(fdia)), _p(order), _p(r), _p(x),
                C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value, x, r
+
+    # -- complex twin (len 1; complex128 arrays) ------------------------------
+    def zemat_helmholtz2d(self, coord, conn, wave_length, value):
+        conn = u32(conn)
+        ne = conn.shape[0]
+        emat = np.zeros(ne * 9, dtype=np.complex128)
+        eres = np.zeros(ne * 3, dtype=np.complex128)
+        self.L.orc_zemat_helmholtz2d_p1(ne, _p(f64(coord)), _p(conn), float(wave_length), _p(c128(value)), _p(emat),
+                                        _p(eres))
+        return emat, eres
+
+    def zemat_sommerfeld(self, coord, lconn, wave_length, value):
+        lconn = u32(lconn)
+        ne = lconn.shape[0]
+        emat = np.zeros(ne * 4, dtype=np.complex128)
+        eres = np.zeros(ne * 2, dtype=np.complex128)
+        self.L.orc_zemat_sommerfeld_b1(ne, _p(f64(coord)), _p(lconn), float(wave_length), _p(c128(value)), _p(emat),
+                                       _p(eres))
+        return emat, eres
+
+    def zassemble(self, nb, rowptr, col, conn, emat, val=None, dia=None):
+        conn = u32(conn)
+        val = np.zeros(int(rowptr[-1]), dtype=np.complex128) if val is None else c128(val).copy()
+        dia = np.zeros(nb, dtype=np.complex128) if dia is None else c128(dia).copy()
+        self.L.orc_zassemble(nb, _p(u32(rowptr)), _p(u32(col)), conn.shape[0], conn.shape[1], _p(conn),
+                             _p(c128(emat)), _p(val), _p(dia))
+        return val, dia
+
+    def zassemble_vec(self, res, conn, eres):
+        conn = u32(conn)
+        res = c128(res).copy()
+        self.L.orc_zassemble_vec(_p(res), conn.shape[0], conn.shape[1], _p(conn), _p(c128(eres)))
+        return res
+
+    def zset_bc(self, nb, rowptr, col, val, dia, flags, res):
+        val, dia, res = c128(val).copy(), c128(dia).copy(), c128(res).copy()
+        self.L.orc_zset_bc(nb, _p(u32(rowptr)), _p(u32(col)), _p(val), _p(dia), _p(i32(flags)), _p(res))
+        return val, dia, res
+
+    def zmatvec(self, nb, rowptr, col, val, dia, alpha, x, beta, y):
+        y = c128(y).copy()
+        self.L.orc_zmatvec(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)), float(alpha),
+                           _p(c128(x)), float(beta), _p(y))
+        return y
+
+    def zilu_values(self, nb, rowptr, col, val, dia, F, order=None):
+        """Complex ILU on the pattern F (orc_iluk_symbolic): (fval, fdia, nsmall)."""
+        fval = np.zeros(int(F["rowptr"][-1]), dtype=np.complex128)
+        fdia = np.zeros(nb, dtype=np.complex128)
+        order = None if order is None else i32(order)
+        self.L.orc_zilu_setvalue(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)), _p(order),
+                                 _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(fval), _p(fdia))
+        ns = self.L.orc_zilu_decomp(nb, _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(u32(F["diaind"])), _p(fval),
+                                    _p(fdia))
+        return fval, fdia, ns
+
+    def zilu_solve(self, nb, F, fval, fdia, vec):
+        v = c128(vec).copy()
+        args = (nb, _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(u32(F["diaind"])), _p(c128(fval)), _p(c128(fdia)))
+        self.L.orc_zilu_forward(*args, _p(v))
+        self.L.orc_zilu_backward(*args, _p(v))
+        return v
+
+    def zsolve_pcocg(self, nb, rowptr, col, val, dia, F, fval, fdia, rhs, tol, maxit, order=None):
+        """MatVec::Sol::Solve_PCOCG: (ok, conv, iter, u, r_final)."""
+        r = c128(rhs).copy()
+        u = np.zeros(nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        order = None if order is None else i32(order)
+        ok = self.L.orc_zsolve_pcocg(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)),
+                                     _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(u32(F["diaind"])), _p(c128(fval)),
+                                     _p(c128(fdia)), _p(order), _p(r), _p(u), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
 
     MAT_KINDS = {"cg": 0, "pcg": 1, "bicgstab": 2, "pbicgstab": 3}
 
@@ -303,6 +395,20 @@ class Ref:
             "ref_frac_backward": (None, [vp, vp]),
             "ref_frac_solve": (C.c_int, [vp, vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_mat_solve": (C.c_int, [vp, vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_zsys_create": (vp, [C.c_uint, vp, vp]),
+            "ref_zsys_free": (None, [vp]),
+            "ref_zsys_merge": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
+            "ref_zsys_set_bc": (None, [vp, vp]),
+            "ref_zsys_get": (None, [vp, vp, vp]),
+            "ref_zsys_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
+            "ref_zfrac_create": (vp, [vp, C.c_int]),
+            "ref_zfrac_free": (None, [vp]),
+            "ref_zfrac_ncrs": (C.c_uint, [vp]),
+            "ref_zfrac_get_pattern": (None, [vp, C.c_uint, vp, vp]),
+            "ref_zfrac_set_value": (C.c_int, [vp, vp]),
+            "ref_zfrac_get_values": (None, [vp, C.c_uint, vp, vp]),
+            "ref_zfrac_solve": (None, [vp, C.c_uint, vp]),
+            "ref_zsolve_pcocg": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_emat_poisson2d_p1": (None, [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]),
             "ref_emat_solid3d_p1": (None, [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                            C.c_double, vp, vp, vp]),
@@ -457,6 +563,84 @@ class RefFrac:
         it = C.c_uint(maxit)
         ok = self.s.ref.L.ref_frac_solve(self.h, self.s.h, 0 if kind == "pcg" else 1, C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value
+
+
+class RefZSystem:
+    """MatVec::CZMatDia_BlkCrs (len 1) of the reference with its ILU(lev) factor."""
+
+    def __init__(self, ref, nb, rowptr, col):
+        self.ref, self.nb = ref, nb
+        self._rp, self._cl = u32(rowptr), u32(col)
+        self.h = ref.L.ref_zsys_create(nb, _p(self._rp), _p(self._cl))
+        self.f = None
+
+    def __del__(self):
+        if getattr(self, "f", None):
+            self.ref.L.ref_zfrac_free(self.f)
+        if getattr(self, "h", None):
+            self.ref.L.ref_zsys_free(self.h)
+
+    def merge(self, conn, emat):
+        conn = u32(conn)
+        self.ref.L.ref_zsys_merge(self.h, conn.shape[0], conn.shape[1], _p(conn), _p(c128(emat)))
+
+    def set_bc(self, flags):
+        self.ref.L.ref_zsys_set_bc(self.h, _p(i32(flags)))
+
+    def matrix(self):
+        val = np.zeros(int(self._rp[-1]), dtype=np.complex128)
+        dia = np.zeros(self.nb, dtype=np.complex128)
+        self.ref.L.ref_zsys_get(self.h, _p(val), _p(dia))
+        return val, dia
+
+    def matvec(self, alpha, x, beta, y):
+        y = c128(y).copy()
+        self.ref.L.ref_zsys_matvec(self.h, float(alpha), _p(c128(x)), float(beta), _p(y))
+        return y
+
+    def factor(self, lev):
+        if self.f:
+            self.ref.L.ref_zfrac_free(self.f)
+        self.f = self.ref.L.ref_zfrac_create(self.h, int(lev))
+        n = self.ref.L.ref_zfrac_ncrs(self.f)
+        rowptr = np.zeros(self.nb + 1, dtype=np.uint32)
+        col = np.zeros(max(n, 1), dtype=np.uint32)
+        self.ref.L.ref_zfrac_get_pattern(self.f, self.nb, _p(rowptr), _p(col))
+        ok = bool(self.ref.L.ref_zfrac_set_value(self.f, self.h))
+        fval = np.zeros(max(n, 1), dtype=np.complex128)
+        fdia = np.zeros(self.nb, dtype=np.complex128)
+        self.ref.L.ref_zfrac_get_values(self.f, self.nb, _p(fval), _p(fdia))
+        return dict(rowptr=rowptr, col=col[:n]), fval[:n], fdia, ok
+
+    def precond(self, vec):
+        v = c128(vec).copy()
+        self.ref.L.ref_zfrac_solve(self.f, self.nb, _p(v))
+        return v
+
+    def pcocg(self, rhs, tol, maxit):
+        r = c128(rhs).copy()
+        u = np.zeros(self.nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_zsolve_pcocg(self.h, self.f, _p(r), _p(u), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
+
+
+def assemble_zproblem(orc, prob):
+    """Oracle pipeline of the complex Helmholtz problem: pattern (triangles),
+    triangle and Sommerfeld line elements merged in that order, Dirichlet
+    rows/columns and residual."""
+    rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)
+    et, rt = orc.zemat_helmholtz2d(prob.coord, prob.conn, prob.wave_length, prob.value)
+    el, rl = orc.zemat_sommerfeld(prob.coord, prob.lconn, prob.wave_length, prob.value)
+    val, dia = orc.zassemble(prob.nnode, rowptr, col, prob.conn, et)
+    val, dia = orc.zassemble(prob.nnode, rowptr, col, prob.lconn, el, val, dia)
+    res = np.zeros(prob.nnode, dtype=np.complex128)
+    res = orc.zassemble_vec(res, prob.conn, rt)
+    res = orc.zassemble_vec(res, prob.lconn, rl)
+    val, dia, res = orc.zset_bc(prob.nnode, rowptr, col, val, dia, prob.flags, res)
+    return dict(rowptr=rowptr, col=col, val=val, dia=dia, res=res, emat_tri=et, eres_tri=rt, emat_line=el,
+                eres_line=rl)
 
 
 def problem_emats(orc, prob, value=None):

@@ -13,6 +13,8 @@
 //   LsSol::Solve_PCG / Solve_PBiCGSTAB  include/delfem/ls/solver_ls_iter.h:35-45
 //   MatVec::CMatDiaFrac_BlkCrs  include/delfem/matvec/matdiafrac_blkcrs.h:40-137
 //   MatVec::Sol::Solve_*        include/delfem/matvec/solver_mat_iter.h:41-60
+//   MatVec::CZMatDia_BlkCrs, CZMatDiaFrac_BlkCrs, CZVector_Blk, Sol::Solve_PCOCG
+//                               include/delfem/matvec/z*.h (complex twin, len 1)
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -29,6 +31,10 @@
 #include "delfem/ls/preconditioner.h"
 #include "delfem/ls/solver_ls_iter.h"
 #include "delfem/matvec/solver_mat_iter.h"
+#include "delfem/matvec/zmatdia_blkcrs.h"
+#include "delfem/matvec/zmatdiafrac_blkcrs.h"
+#include "delfem/matvec/zvector_blk.h"
+#include "delfem/matvec/zsolver_mat_iter.h"
 #include "delfem/femeqn/ker_emat_tri.h"
 #include "delfem/femeqn/ker_emat_tet.h"
 
@@ -88,6 +94,29 @@ class FracPrecond : public LsSol::CPreconditioner {
   MatVec::CVector_Blk tmp_;
 };
 
+// Complex factor with access to its pattern arrays.
+class ZFracPeek : public MatVec::CZMatDiaFrac_BlkCrs {
+ public:
+  ZFracPeek(int lev, const MatVec::CZMatDia_BlkCrs& A) : MatVec::CZMatDiaFrac_BlkCrs(lev, A) {}
+  const unsigned* RowPtr() const { return m_colInd_Blk; }
+  const unsigned* Col() const { return m_rowPtr_Blk; }
+  const Com::Complex* Val() const { return m_valCrs_Blk; }
+  const Com::Complex* Dia() const { return m_valDia_Blk; }
+};
+struct RefZSys {
+  MatVec::CZMatDia_BlkCrs* A = nullptr;
+  unsigned nb = 0;
+  std::vector<int> tmp;
+};
+void zcopy_in(MatVec::CZVector_Blk& v, unsigned n, const double* src) {
+  for (unsigned i = 0; i < n; i++) v.SetValue(i, 0, Com::Complex(src[2 * i], src[2 * i + 1]));
+}
+void zcopy_out(const MatVec::CZVector_Blk& v, unsigned n, double* dst) {
+  for (unsigned i = 0; i < n; i++) {
+    dst[2 * i] = v.GetValue(i, 0).Real();
+    dst[2 * i + 1] = v.GetValue(i, 0).Imag();
+  }
+}
 std::ofstream& null_stream() {
   static std::ofstream s("/dev/null");
   return s;
@@ -319,6 +348,110 @@ int ref_mat_solve(void* p, void* q, int kind, double* r, double* u, double* conv
   }
   copy_vec_out(rv, s->nb, s->b, r);
   copy_vec_out(uv, s->nb, s->b, u);
+  return ok ? 1 : 0;
+}
+
+// ---- complex twin (len 1) ------------------------------------------------------
+void* ref_zsys_create(unsigned nb, const unsigned* index, const unsigned* array) {
+  RefZSys* z = new RefZSys;
+  z->nb = nb;
+  z->A = new MatVec::CZMatDia_BlkCrs(nb, 1);
+  Com::CIndexedArray crs;
+  crs.InitializeSize(nb);
+  crs.index.assign(index, index + nb + 1);
+  crs.array.assign(array, array + index[nb]);
+  z->A->AddPattern(crs);
+  z->A->SetZero();
+  z->tmp.assign(nb, -1);
+  return z;
+}
+void ref_zsys_free(void* p) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  delete z->A;
+  delete z;
+}
+// CZMatDia_BlkCrs::Mearge of each element in order (emat: npe*npe complex per element)
+void ref_zsys_merge(void* p, unsigned nelem, unsigned npe, const unsigned* conn, const double* emat) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  std::vector<Com::Complex> em(npe * npe);
+  for (unsigned e = 0; e < nelem; e++) {
+    for (unsigned k = 0; k < npe * npe; k++)
+      em[k] = Com::Complex(emat[2 * ((size_t)e * npe * npe + k)], emat[2 * ((size_t)e * npe * npe + k) + 1]);
+    z->A->Mearge(npe, conn + (size_t)e * npe, npe, conn + (size_t)e * npe, 1, em.data(), z->tmp.data());
+  }
+}
+void ref_zsys_set_bc(void* p, const int* flags) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  MatVec::CBCFlag bc(z->nb, 1);
+  for (unsigned i = 0; i < z->nb; i++)
+    if (flags[i]) bc.SetBC(i, 0);
+  z->A->SetBoundaryCondition(bc);
+}
+void ref_zsys_get(void* p, double* val, double* dia) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  const unsigned nb = z->nb;
+  for (unsigned i = 0; i < nb; i++) {
+    unsigned n = 0;
+    const Com::Complex* v = z->A->GetPtrValPSuP(i, n);
+    unsigned n0 = 0;
+    const unsigned* c0 = z->A->GetPtrIndPSuP(0, n0);
+    unsigned ni = 0;
+    const unsigned* ci = z->A->GetPtrIndPSuP(i, ni);
+    const size_t off = (size_t)(ci - c0);
+    for (unsigned k = 0; k < n; k++) {
+      val[2 * (off + k)] = v[k].Real();
+      val[2 * (off + k) + 1] = v[k].Imag();
+    }
+    const Com::Complex* d = z->A->GetPtrValDia(i);
+    dia[2 * i] = d->Real();
+    dia[2 * i + 1] = d->Imag();
+  }
+}
+void ref_zsys_matvec(void* p, double alpha, const double* x, double beta, double* y) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  MatVec::CZVector_Blk xv(z->nb, 1), yv(z->nb, 1);
+  zcopy_in(xv, z->nb, x);
+  zcopy_in(yv, z->nb, y);
+  z->A->MatVec(alpha, xv, beta, yv);
+  zcopy_out(yv, z->nb, y);
+}
+void* ref_zfrac_create(void* p, int lev) { return new ZFracPeek(lev, *static_cast<RefZSys*>(p)->A); }
+void ref_zfrac_free(void* f) { delete static_cast<ZFracPeek*>(f); }
+unsigned ref_zfrac_ncrs(void* f) { return static_cast<ZFracPeek*>(f)->NCrs(); }
+void ref_zfrac_get_pattern(void* f, unsigned nb, unsigned* rowptr, unsigned* col) {
+  ZFracPeek* z = static_cast<ZFracPeek*>(f);
+  std::memcpy(rowptr, z->RowPtr(), sizeof(unsigned) * (nb + 1));
+  std::memcpy(col, z->Col(), sizeof(unsigned) * z->NCrs());
+}
+// CZMatDiaFrac_BlkCrs::SetValue: copy of A and DoILUDecomp
+int ref_zfrac_set_value(void* f, void* p) {
+  return static_cast<ZFracPeek*>(f)->SetValue(*static_cast<RefZSys*>(p)->A) ? 1 : 0;
+}
+void ref_zfrac_get_values(void* f, unsigned nb, double* val, double* dia) {
+  ZFracPeek* z = static_cast<ZFracPeek*>(f);
+  for (unsigned k = 0; k < z->NCrs(); k++) {
+    val[2 * k] = z->Val()[k].Real();
+    val[2 * k + 1] = z->Val()[k].Imag();
+  }
+  for (unsigned i = 0; i < nb; i++) {
+    dia[2 * i] = z->Dia()[i].Real();
+    dia[2 * i + 1] = z->Dia()[i].Imag();
+  }
+}
+void ref_zfrac_solve(void* f, unsigned nb, double* vec) {
+  MatVec::CZVector_Blk v(nb, 1);
+  zcopy_in(v, nb, vec);
+  static_cast<ZFracPeek*>(f)->Solve(v);
+  zcopy_out(v, nb, vec);
+}
+// MatVec::Sol::Solve_PCOCG with the factor as preconditioner; r in/out, u out
+int ref_zsolve_pcocg(void* p, void* f, double* r, double* u, double* conv, unsigned* iter) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  MatVec::CZVector_Blk rv(z->nb, 1), uv(z->nb, 1);
+  zcopy_in(rv, z->nb, r);
+  const bool ok = MatVec::Sol::Solve_PCOCG(*conv, *iter, *z->A, rv, uv, *static_cast<ZFracPeek*>(f), *z->A);
+  zcopy_out(rv, z->nb, r);
+  zcopy_out(uv, z->nb, u);
   return ok ? 1 : 0;
 }
 

@@ -11,7 +11,8 @@ ordered, k = 0..3), ILU factor values, one MatVec, one forward/backward
 substitution, the Solve_PCG / Solve_PBiCGSTAB results (flag, iterations,
 conv_ratio, solution, final residual), and the same for the matrix-layer
 MatVec::Sol::Solve_CG / Solve_PCG / Solve_BiCGSTAB / Solve_PBiCGSTAB with the
-natural ILU(0) factor.  Floating-point arrays are stored as
+natural ILU(0) factor; the reference's unstructured tetrahedral mesh; and the
+complex twin (CZMatDia_BlkCrs, CZMatDiaFrac_BlkCrs, Solve_PCOCG).  Floating-point arrays are stored as
 float64 and compared bitwise by the tests.
 """
 import os
@@ -149,6 +150,36 @@ def gid_mesh_fixture(ref):
     return out
 
 
+def zhelmholtz_fixture(ref):
+    """Complex twin: the reference's CZMatDia_BlkCrs assembled from the
+    oracle's element matrices (eqn_helmholtz.cpp is not compiled: it needs the
+    field layer), its ILU(0)/ILU(1) factors, one MatVec and one
+    preconditioner apply, and MatVec::Sol::Solve_PCOCG."""
+    orc = O.Orc()
+    prob = meshgen.zhelmholtz2d(10, 8, 1.0, 0.8, wave_length=0.3)
+    Z = O.assemble_zproblem(orc, prob)
+    rz = O.RefZSystem(ref, prob.nnode, Z["rowptr"], Z["col"])
+    rz.merge(prob.conn, Z["emat_tri"])
+    rz.merge(prob.lconn, Z["emat_line"])
+    rz.set_bc(prob.flags)
+    val, dia = rz.matrix()
+    out = dict(coord=prob.coord, conn=prob.conn, lconn=prob.lconn, wave_length=np.array(prob.wave_length),
+               flags=prob.flags, value=prob.value, nnode=np.array(prob.nnode), rowptr=Z["rowptr"], col=Z["col"],
+               emat_tri=Z["emat_tri"][:9 * 16], emat_line=Z["emat_line"][:4 * 8], zval=val, zdia=dia, zres=Z["res"])
+    rng = np.random.default_rng(17)
+    x = rng.standard_normal(prob.nnode) + 1j * rng.standard_normal(prob.nnode)
+    out.update(zmv_x=x, zmv_out=rz.matvec(1.5, x, -0.25, x))
+    for lev in (0, 1):
+        P, fv, fd, ok = rz.factor(lev)
+        out[f"zfrow_k{lev}"], out[f"zfcol_k{lev}"] = P["rowptr"], P["col"]
+        out[f"zfval_k{lev}"], out[f"zfdia_k{lev}"] = fv, fd
+        out[f"zprec_k{lev}"] = rz.precond(x)
+        ok2, conv, it, u, r = rz.pcocg(Z["res"], 1e-10, 2000)
+        out[f"zpcocg_k{lev}"] = np.array([float(ok2), conv, float(it)])
+        out[f"zpcocg_u_k{lev}"], out[f"zpcocg_r_k{lev}"] = u, r
+    return out
+
+
 def digest(a):
     """SHA-256 of an array's bytes (little-endian, C order)."""
     import hashlib
@@ -200,6 +231,7 @@ def main():
         mat_solver_fixture(out, rs, res)
         fixtures[name] = out
     fixtures["cylinder_tet"] = gid_mesh_fixture(ref)
+    fixtures["zhelmholtz2d_10x8"] = zhelmholtz_fixture(ref)
     out, rs, res = random_graph_fixture(ref)
     rng = np.random.default_rng(13)
     orders = {"natural": None, "random": rng.permutation(60).astype(np.int32)}

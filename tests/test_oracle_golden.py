@@ -181,3 +181,32 @@ def test_unstructured_mesh(orc):
         want = d[f"{kind}_natural_k0"]
         assert (float(ok), conv, float(it)) == (want[0], want[1], want[2]), kind
         assert np.array_equal(x, d[f"{kind}_x_natural_k0"]) and sha(r) == str(d[f"sha_{kind}_r_natural_k0"])
+
+
+def test_complex_twin(orc):
+    """Complex twin against the reference's outputs: assembly from the stored
+    element matrices' problem, MatVec, ILU(0)/ILU(1) pattern and factor, one
+    preconditioner apply and Solve_PCOCG, bit for bit."""
+    from delfem_amd import meshgen
+    from oracle.oracle import assemble_zproblem
+    d = load("zhelmholtz2d_10x8")
+    prob = meshgen.zhelmholtz2d(10, 8, 1.0, 0.8, wave_length=float(d["wave_length"]))
+    assert np.array_equal(prob.coord, d["coord"]) and np.array_equal(prob.lconn, d["lconn"])
+    Z = assemble_zproblem(orc, prob)
+    assert np.array_equal(Z["emat_tri"][:9 * 16], d["emat_tri"]) and np.array_equal(Z["emat_line"][:32], d["emat_line"])
+    assert np.array_equal(Z["val"], d["zval"]) and np.array_equal(Z["dia"], d["zdia"])
+    assert np.array_equal(Z["res"], d["zres"])
+    nb = int(d["nnode"])
+    assert np.array_equal(orc.zmatvec(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], 1.5, d["zmv_x"], -0.25,
+                                      d["zmv_x"]), d["zmv_out"])
+    for lev in (0, 1):
+        F = orc.iluk_symbolic(Z["rowptr"], Z["col"], lev)
+        assert np.array_equal(F["rowptr"], d[f"zfrow_k{lev}"]) and np.array_equal(F["col"], d[f"zfcol_k{lev}"])
+        fv, fd, _ = orc.zilu_values(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F)
+        assert np.array_equal(fv, d[f"zfval_k{lev}"]) and np.array_equal(fd, d[f"zfdia_k{lev}"])
+        assert np.array_equal(orc.zilu_solve(nb, F, fv, fd, d["zmv_x"]), d[f"zprec_k{lev}"])
+        ok, conv, it, u, r = orc.zsolve_pcocg(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F, fv, fd, Z["res"],
+                                              1e-10, 2000)
+        want = d[f"zpcocg_k{lev}"]
+        assert (float(ok), conv, float(it)) == (want[0], want[1], want[2])
+        assert np.array_equal(u, d[f"zpcocg_u_k{lev}"]) and np.array_equal(r, d[f"zpcocg_r_k{lev}"])

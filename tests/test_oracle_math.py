@@ -405,3 +405,66 @@ def test_matrix_layer_return_semantics(orc, poisson):
         ok, conv, it, u, _ = orc.solve_mat(kind, nb, 1, A["rowptr"], A["col"], A["val"], A["dia"],
                                            np.zeros(nb), 1e-12, 30, F, fv, fd)
         assert (ok, conv, it) == (True, 0.0, 0) and not u.any(), kind
+
+
+def _zdense(nb, rowptr, col, val, dia):
+    A = np.zeros((nb, nb), dtype=np.complex128)
+    for i in range(nb):
+        A[i, i] = dia[i]
+        for q in range(int(rowptr[i]), int(rowptr[i + 1])):
+            A[i, col[q]] = val[q]
+    return A
+
+
+def test_complex_helmholtz_elements(orc):
+    """Triangle: stiffness (the real Poisson element with unit diffusivity)
+    minus k^2 area/12 (1 + delta_ij), k = 2*3.1416/wave_length as in the
+    reference, zero imaginary part; Sommerfeld line: i k h/6 [2 1; 1 2] -
+    i/(2 h k) [1 -1; -1 1]; residuals -emat value."""
+    prob = meshgen.zhelmholtz2d(5, 4, 1.0, 0.8, wave_length=0.4)
+    et, rt = orc.zemat_helmholtz2d(prob.coord, prob.conn, prob.wave_length, prob.value)
+    ks, _ = orc.emat_poisson2d(prob.coord, prob.conn, np.ones(prob.nelem), 0.0, np.zeros(prob.nnode))
+    k = 2 * 3.1416 / prob.wave_length
+    p = prob.coord[prob.conn.astype(np.int64)]
+    area = 0.5 * ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1]) - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+    mass = (area[:, None, None] / 12.0) * (np.ones((3, 3)) + np.eye(3))
+    want = ks.reshape(-1, 3, 3) - k * k * mass
+    assert np.allclose(et.reshape(-1, 3, 3).real, want, rtol=1e-13, atol=1e-13)
+    assert not et.imag.any()
+    assert np.allclose(rt.reshape(-1, 3), -np.einsum("eij,ej->ei", et.reshape(-1, 3, 3), prob.value[prob.conn]))
+    el, rl = orc.zemat_sommerfeld(prob.coord, prob.lconn, prob.wave_length, prob.value)
+    q = prob.coord[prob.lconn.astype(np.int64)]
+    h = np.linalg.norm(q[:, 0] - q[:, 1], axis=1)
+    want = (1j * k * h / 6)[:, None, None] * np.array([[2, 1], [1, 2]]) \
+        - (1j / (2 * h * k))[:, None, None] * np.array([[1, -1], [-1, 1]])
+    assert np.allclose(el.reshape(-1, 2, 2), want, rtol=1e-13, atol=1e-15)
+    assert np.allclose(rl.reshape(-1, 2), -np.einsum("eij,ej->ei", el.reshape(-1, 2, 2), prob.value[prob.lconn]))
+
+
+def test_complex_ilu_and_pcocg(orc):
+    """Complex ILU with unlimited fill is the exact LU (its apply equals a
+    dense solve); the assembled matrix is complex symmetric; PCOCG with
+    ILU(0) reaches the solution of a direct solve."""
+    from oracle.oracle import assemble_zproblem
+    prob = meshgen.zhelmholtz2d(8, 6, 1.0, 0.75, wave_length=0.35)
+    Z = assemble_zproblem(orc, prob)
+    nb = prob.nnode
+    A = _zdense(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"])
+    free = prob.flags == 0
+    assert np.allclose(A[np.ix_(free, free)], A[np.ix_(free, free)].T, rtol=0, atol=1e-13)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(nb) + 1j * rng.standard_normal(nb)
+    Ffull = orc.iluk_symbolic(Z["rowptr"], Z["col"], -1)
+    fv, fd, ns = orc.zilu_values(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Ffull)
+    assert ns == 0
+    assert np.allclose(orc.zilu_solve(nb, Ffull, fv, fd, b), np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    y = orc.zmatvec(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], 0.7, b, -0.3, b)
+    assert np.allclose(y, 0.7 * (A @ b) - 0.3 * b, rtol=1e-13, atol=1e-13)
+    F = orc.iluk_symbolic(Z["rowptr"], Z["col"], 0)
+    fv, fd, _ = orc.zilu_values(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F)
+    ok, conv, it, u, r = orc.zsolve_pcocg(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F, fv, fd, Z["res"], 1e-11,
+                                          2000)
+    xd = np.linalg.solve(A, Z["res"])
+    assert ok and conv < 1e-11 and 0 < it < 2000
+    assert np.linalg.norm(u - xd) <= 1e-8 * np.linalg.norm(xd)
+    assert np.linalg.norm(Z["res"] - A @ u) <= 1e-10 * np.linalg.norm(Z["res"])

@@ -88,3 +88,35 @@ def test_matrix_layer_solvers(orc, ref, kind, case):
         exp = rs.solve_mat(kind, A["res"], tol, maxit, frac)
         assert got[:3] == exp[:3], (kind, maxit, got[:3], exp[:3])
         assert np.array_equal(got[3], exp[3]) and np.array_equal(got[4], exp[4])
+
+
+@pytest.mark.parametrize("lev", [0, 1])
+def test_complex_twin_vs_reference(orc, ref, lev):
+    """Complex twin (len 1) against the reference's CZMatDia_BlkCrs /
+    CZMatDiaFrac_BlkCrs / MatVec::Sol::Solve_PCOCG on the complex Helmholtz
+    problem: assembled matrix, MatVec, ILU(lev) pattern and factor, one
+    preconditioner apply and the whole PCOCG run, bit for bit."""
+    from oracle.oracle import RefZSystem, assemble_zproblem
+    prob = meshgen.zhelmholtz2d(14, 9, 1.2, 0.8, wave_length=0.3)
+    Z = assemble_zproblem(orc, prob)
+    rz = RefZSystem(ref, prob.nnode, Z["rowptr"], Z["col"])
+    rz.merge(prob.conn, Z["emat_tri"])
+    rz.merge(prob.lconn, Z["emat_line"])
+    rz.set_bc(prob.flags)
+    val, dia = rz.matrix()
+    assert np.array_equal(val, Z["val"]) and np.array_equal(dia, Z["dia"])
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(prob.nnode) + 1j * rng.standard_normal(prob.nnode)
+    y = rng.standard_normal(prob.nnode) + 1j * rng.standard_normal(prob.nnode)
+    assert np.array_equal(rz.matvec(-0.5, x, 2.0, y),
+                          orc.zmatvec(prob.nnode, Z["rowptr"], Z["col"], Z["val"], Z["dia"], -0.5, x, 2.0, y))
+    P, fv_r, fd_r, ok = rz.factor(lev)
+    F = orc.iluk_symbolic(Z["rowptr"], Z["col"], lev)
+    assert np.array_equal(P["rowptr"], F["rowptr"]) and np.array_equal(P["col"], F["col"])
+    fv, fd, ns = orc.zilu_values(prob.nnode, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F)
+    assert ok and ns == 0
+    assert np.array_equal(fv, fv_r) and np.array_equal(fd, fd_r)
+    assert np.array_equal(rz.precond(x), orc.zilu_solve(prob.nnode, F, fv, fd, x))
+    got = orc.zsolve_pcocg(prob.nnode, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F, fv, fd, Z["res"], 1e-10, 2000)
+    exp = rz.pcocg(Z["res"], 1e-10, 2000)
+    assert got[:3] == exp[:3] and np.array_equal(got[3], exp[3]) and np.array_equal(got[4], exp[4])
