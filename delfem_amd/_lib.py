@@ -110,6 +110,34 @@ _SIGS = {
     "dfm_ilu_solve_precond": (C.c_int, [H, vp]),
     "dfm_solve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_solve_pbicgstab": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
+    "dfm_zbcrs_create": (C.c_int, [C.c_uint, C.c_uint, C.POINTER(H)]),
+    "dfm_zbcrs_destroy": (C.c_int, [H]),
+    "dfm_zbcrs_add_pattern": (C.c_int, [H, C.c_uint, vp, vp]),
+    "dfm_zbcrs_info": (C.c_int, [H, u32p, u32p, u32p]),
+    "dfm_zbcrs_get_pattern": (C.c_int, [H, vp, vp]),
+    "dfm_zbcrs_set_zero": (C.c_int, [H]),
+    "dfm_zbcrs_set_values": (C.c_int, [H, vp, vp]),
+    "dfm_zbcrs_get_values": (C.c_int, [H, vp, vp]),
+    "dfm_zbcrs_device_values": (C.c_int, [H, C.POINTER(vp), C.POINTER(vp)]),
+    "dfm_zbcrs_mearge": (C.c_int, [H, C.c_uint, vp, C.c_uint, vp]),
+    "dfm_zbcrs_set_bc": (C.c_int, [H, vp]),
+    "dfm_zbcrs_matvec": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
+    "dfm_zvec_dot": (C.c_int, [C.c_size_t, vp, vp, f64p]),
+    "dfm_zvec_norm2": (C.c_int, [C.c_size_t, vp, f64p]),
+    "dfm_zvec_zero_bc": (C.c_int, [C.c_size_t, vp, vp]),
+    "dfm_zasm_create": (C.c_int, [H, C.c_uint, C.c_uint, C.c_uint, vp, C.POINTER(H)]),
+    "dfm_zassemble_helmholtz2d_p1": (C.c_int, [H, H, vp, C.c_double, vp, vp]),
+    "dfm_zassemble_sommerfeld_b1": (C.c_int, [H, H, vp, C.c_double, vp, vp]),
+    "dfm_zilu_create": (C.c_int, [H, C.c_int, C.c_int, vp, C.POINTER(H)]),
+    "dfm_zilu_destroy": (C.c_int, [H]),
+    "dfm_zilu_info": (C.c_int, [H, u32p, i32p, i32p, i32p]),
+    "dfm_zilu_get_pattern": (C.c_int, [H, vp, vp, vp, vp, vp]),
+    "dfm_zilu_set_value": (C.c_int, [H, H, i32p]),
+    "dfm_zilu_get_values": (C.c_int, [H, vp, vp]),
+    "dfm_zilu_forward": (C.c_int, [H, vp]),
+    "dfm_zilu_backward": (C.c_int, [H, vp]),
+    "dfm_zilu_solve_precond": (C.c_int, [H, vp]),
+    "dfm_zsolve_pcocg": (C.c_int, [H, H, vp, vp, f64p, u32p]),
     "dfm_bench_kernel": (C.c_int, [H, H, C.c_int, C.c_int, C.c_int, f64p, f64p]),
     "dfm_bench_copy": (C.c_int, [C.c_size_t, C.c_int, f64p]),
     "dfm_bench_read": (C.c_int, [C.c_size_t, C.c_int, f64p]),

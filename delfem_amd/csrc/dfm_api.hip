@@ -57,6 +57,14 @@ int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32, bool want16
 int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper);
 }
 
+// Value versions are unique across all matrices, so a factor's mirrored copy
+// of A (tagged with the version it copied) is refreshed whenever it is given
+// a different matrix or changed values.
+unsigned long long dfm::next_version() {
+  static std::atomic<unsigned long long> counter{1};
+  return ++counter;
+}
+
 extern "C" {
 
 const char* dfm_last_error(void) { return dfm::g_err.c_str(); }
@@ -141,13 +149,6 @@ int dfm_fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned
 
 // ---------------------------------------------------------------------------
 // MatVec::CMatDia_BlkCrs
-// Value versions are unique across all matrices, so a factor's mirrored copy
-// of A (tagged with the version it copied) is refreshed whenever it is given
-// a different matrix or changed values.
-static unsigned long long next_version() {
-  static std::atomic<unsigned long long> counter{1};
-  return ++counter;
-}
 
 // FNV-1a over the block-CRS pattern: a factor or copy built on one pattern
 // checks the matrix it is later given against it.

@@ -15,6 +15,8 @@ namespace dfm {
 void set_error(const char* fmt, ...);
 hipStream_t stream();
 int ensure_init();
+// Value versions, unique across all matrices (real and complex).
+unsigned long long next_version();
 
 #define DFM_HIP(call)                                                                    \
   do {                                                                                   \

@@ -443,3 +443,247 @@ class BatchSystem:
         bl, wp = min(times, key=times.get)
         self.set_launch(bl, wp)
         return times
+
+
+# ---------------------------------------------------------------------------
+# Complex twin.  Complex vectors are torch complex128 tensors on the GPU
+# (interleaved re, im doubles, the layout of Com::Complex); host arrays are
+# numpy complex128.
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128).reshape(-1)
+
+
+class CZMatDia_BlkCrs:
+    """MatVec::CZMatDia_BlkCrs (include/delfem/matvec/zmatdia_blkcrs.h:39-100),
+    scalar blocks (len 1) as the reference's complex solvers use it."""
+
+    def __init__(self, nblk, len_=1):
+        h = C.c_void_p()
+        call("dfm_zbcrs_create", nblk, len_, C.byref(h))
+        self.h = h
+        self.nblk, self.len = nblk, len_
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_zbcrs_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def NBlkMatCol(self):
+        return self.nblk
+
+    def LenBlkCol(self):
+        return self.len
+
+    def NCrs(self):
+        n = C.c_uint()
+        call("dfm_zbcrs_info", self.h, None, None, C.byref(n))
+        return n.value
+
+    def AddPattern(self, index, array):
+        index = np.ascontiguousarray(index, dtype=np.uint32)
+        array = np.ascontiguousarray(array, dtype=np.uint32)
+        if array.size == 0:
+            array = np.zeros(1, dtype=np.uint32)
+        call("dfm_zbcrs_add_pattern", self.h, index.size - 1, _ptr(index), _ptr(array))
+        return True
+
+    def pattern(self):
+        rowptr = np.zeros(self.nblk + 1, dtype=np.uint32)
+        col = np.zeros(max(1, self.NCrs()), dtype=np.uint32)
+        call("dfm_zbcrs_get_pattern", self.h, _ptr(rowptr), _ptr(col))
+        return rowptr, col[:self.NCrs()]
+
+    def SetZero(self):
+        call("dfm_zbcrs_set_zero", self.h)
+        return True
+
+    def set_values(self, val, dia):
+        val, dia = _c128(val), _c128(dia)
+        assert val.size == self.NCrs() and dia.size == self.nblk
+        call("dfm_zbcrs_set_values", self.h, _ptr(val) if val.size else None, _ptr(dia))
+
+    def values(self):
+        n = self.NCrs()
+        val = np.zeros(max(1, n), dtype=np.complex128)
+        dia = np.zeros(max(1, self.nblk), dtype=np.complex128)
+        call("dfm_zbcrs_get_values", self.h, _ptr(val), _ptr(dia))
+        return val[:n], dia[:self.nblk]
+
+    def Mearge(self, nodes, emat):
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        emat = _c128(emat)
+        call("dfm_zbcrs_mearge", self.h, nodes.size, _ptr(nodes), 1, _ptr(emat))
+        return True
+
+    def SetBoundaryCondition(self, flags):
+        flags = np.ascontiguousarray(flags, dtype=np.int32).reshape(-1)
+        call("dfm_zbcrs_set_bc", self.h, _ptr(flags))
+        return True
+
+    def MatVec(self, alpha, x, beta, y):
+        _sync_torch()
+        call("dfm_zbcrs_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
+        return True
+
+
+class ZAssembler:
+    """Element loops of src/femeqn/eqn_helmholtz.cpp gathered by row on the device."""
+
+    def __init__(self, A, conn):
+        conn = np.ascontiguousarray(conn, dtype=np.uint32)
+        h = C.c_void_p()
+        call("dfm_zasm_create", A.h, A.nblk, conn.shape[0], conn.shape[1], _ptr(conn), C.byref(h))
+        self.h, self.A = h, A
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_asm_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def helmholtz2d_p1(self, coord, wave_length, value, res):
+        _sync_torch()
+        call("dfm_zassemble_helmholtz2d_p1", self.A.h, self.h, _dptr(coord), float(wave_length), _dptr(value),
+             _dptr(res) if res is not None else None)
+
+    def sommerfeld_b1(self, coord, wave_length, value, res):
+        _sync_torch()
+        call("dfm_zassemble_sommerfeld_b1", self.A.h, self.h, _dptr(coord), float(wave_length), _dptr(value),
+             _dptr(res) if res is not None else None)
+
+
+class CZMatDiaFrac_BlkCrs:
+    """MatVec::CZMatDiaFrac_BlkCrs (include/delfem/matvec/zmatdiafrac_blkcrs.h:44-96):
+    symbolic ILU(lev) at construction, numeric on SetValue.  ordering
+    "natural" is the reference's; "multicolour" / "given" reorder for the
+    level-scheduled device sweeps."""
+
+    def __init__(self, lev, A, order=None, ordering=None):
+        if ordering is None:
+            ordering = "given" if order is not None else "natural"
+        ordarr = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+        h = C.c_void_p()
+        call("dfm_zilu_create", A.h, int(lev), _ORDER[ordering], _ptr(ordarr) if ordarr is not None else None,
+             C.byref(h))
+        self.h, self.nblk = h, A.nblk
+        self.nsmall = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_zilu_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def info(self):
+        ncrs, nl, nu, nc = C.c_uint(), C.c_int(), C.c_int(), C.c_int()
+        call("dfm_zilu_info", self.h, C.byref(ncrs), C.byref(nl), C.byref(nu), C.byref(nc))
+        return dict(ncrs=ncrs.value, nlevel_lower=nl.value, nlevel_upper=nu.value, ncolour=nc.value)
+
+    def NCrs(self):
+        return self.info()["ncrs"]
+
+    def pattern(self):
+        n = self.NCrs()
+        rowptr = np.zeros(self.nblk + 1, dtype=np.uint32)
+        col = np.zeros(max(1, n), dtype=np.uint32)
+        lev = np.zeros(max(1, n), dtype=np.int32)
+        dia = np.zeros(self.nblk, dtype=np.uint32)
+        order = np.zeros(self.nblk, dtype=np.int32)
+        call("dfm_zilu_get_pattern", self.h, _ptr(rowptr), _ptr(col), _ptr(lev), _ptr(dia), _ptr(order))
+        return dict(rowptr=rowptr, col=col[:n], lev=lev[:n], diaind=dia, order=order)
+
+    def SetValue(self, A):
+        """SetValue + DoILUDecomp; self.nsmall counts pivots with |d|^2 < 1e-20."""
+        ns = C.c_int()
+        call("dfm_zilu_set_value", self.h, A.h, C.byref(ns))
+        self.nsmall = ns.value
+        return True
+
+    def values(self):
+        n = self.NCrs()
+        val = np.zeros(max(1, n), dtype=np.complex128)
+        dia = np.zeros(self.nblk, dtype=np.complex128)
+        call("dfm_zilu_get_values", self.h, _ptr(val), _ptr(dia))
+        return val[:n], dia
+
+    def ForwardSubstitution(self, vec):
+        _sync_torch()
+        call("dfm_zilu_forward", self.h, _dptr(vec))
+        return True
+
+    def BackwardSubstitution(self, vec):
+        _sync_torch()
+        call("dfm_zilu_backward", self.h, _dptr(vec))
+        return True
+
+    def SolvePrecond(self, vec):
+        """Solve (forward + backward) on a vector in the original numbering."""
+        _sync_torch()
+        call("dfm_zilu_solve_precond", self.h, _dptr(vec))
+        return True
+
+
+def Solve_PCOCG(conv_ratio, iteration, A, prec, r, u):
+    """MatVec::Sol::Solve_PCOCG (src/matvec/zsolver_mat_iter.cpp:303-392).
+    Returns (True, conv_ratio, iteration) as the reference (true also at
+    max_iter); r is overwritten by the final residual, u by the solution."""
+    _sync_torch()
+    conv = C.c_double(conv_ratio)
+    it = C.c_uint(iteration)
+    call("dfm_zsolve_pcocg", A.h, prec.h, _dptr(r), _dptr(u), C.byref(conv), C.byref(it))
+    return True, conv.value, it.value
+
+
+def zdot(x, y):
+    """CZVector_Blk operator* (zvector_blk.cpp:39-60): sum x_i y_i, no conjugate."""
+    _sync_torch()
+    out = (C.c_double * 2)()
+    call("dfm_zvec_dot", x.numel(), _dptr(x), _dptr(y), out)
+    return complex(out[0], out[1])
+
+
+def znorm2(x):
+    """GetSquaredVectorNorm (zvector_blk.cpp:154-161)."""
+    _sync_torch()
+    out = C.c_double()
+    call("dfm_zvec_norm2", x.numel(), _dptr(x), C.byref(out))
+    return out.value
+
+
+class ZHelmholtzSystem:
+    """The complex Helmholtz problem (meshgen.zhelmholtz2d) assembled on the
+    device: pattern from the triangles, triangle then Sommerfeld line
+    elements, Dirichlet rows / columns and residual."""
+
+    def __init__(self, prob):
+        torch = _torch()
+        self.prob = prob
+        rowptr, col = fem_pattern(prob.nnode, prob.conn)
+        self.A = CZMatDia_BlkCrs(prob.nnode, 1)
+        self.A.AddPattern(rowptr, col)
+        self.asm_tri = ZAssembler(self.A, prob.conn)
+        self.asm_line = ZAssembler(self.A, prob.lconn)
+        self.coord = to_device(prob.coord.ravel())
+        self.value = to_device(_c128(prob.value))
+        self.res = torch.zeros(prob.nnode, dtype=torch.complex128, device="cuda")
+        self.flags = np.ascontiguousarray(prob.flags, dtype=np.int32)
+
+    def assemble(self):
+        """Returns ||r|| after the boundary conditions."""
+        self.A.SetZero()
+        self.res.zero_()
+        p = self.prob
+        self.asm_tri.helmholtz2d_p1(self.coord, p.wave_length, self.value, self.res)
+        self.asm_line.sommerfeld_b1(self.coord, p.wave_length, self.value, self.res)
+        self.A.SetBoundaryCondition(self.flags)
+        _sync_torch()
+        call("dfm_zvec_zero_bc", self.res.numel(), _ptr(self.flags), _dptr(self.res))
+        return znorm2(self.res) ** 0.5

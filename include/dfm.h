@@ -240,6 +240,69 @@ int dfm_solve_pcg(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
 int dfm_solve_pbicgstab(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
                         double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
 
+/* ---- complex twin (scalar blocks, as the reference's complex classes) ----
+ * Complex values are interleaved (re, im) doubles: the layout of
+ * Com::Complex and of std::complex<double>.  Every complex operation keeps
+ * the operation order of Com::Complex (include/delfem/complex.h:82-160), so
+ * assembly, MatVec, factorisation and substitutions match the reference bit
+ * for bit in natural ordering. */
+typedef struct dfm_zbcrs_s dfm_zbcrs;   /* MatVec::CZMatDia_BlkCrs */
+typedef struct dfm_zilu_s dfm_zilu;     /* MatVec::CZMatDiaFrac_BlkCrs */
+int dfm_zbcrs_create(unsigned nblk, unsigned len, dfm_zbcrs **out);        /* CZMatDia_BlkCrs(nblk,len) zmatdia_blkcrs.cpp:44-49; len must be 1 */
+int dfm_zbcrs_destroy(dfm_zbcrs *A);
+int dfm_zbcrs_add_pattern(dfm_zbcrs *A, unsigned nrow, const unsigned *index,
+                          const unsigned *array);                          /* AddPattern(CIndexedArray) zmatdia_blkcrs.cpp:406-551 */
+int dfm_zbcrs_info(const dfm_zbcrs *A, unsigned *nblk, unsigned *len, unsigned *ncrs);
+int dfm_zbcrs_get_pattern(const dfm_zbcrs *A, unsigned *rowptr, unsigned *col);
+int dfm_zbcrs_set_zero(dfm_zbcrs *A);                                      /* SetZero zmatdia_blkcrs.cpp:185-190 */
+int dfm_zbcrs_set_values(dfm_zbcrs *A, const double *val, const double *dia); /* host values: val 2*ncrs, dia 2*nblk doubles */
+int dfm_zbcrs_get_values(const dfm_zbcrs *A, double *val, double *dia);
+int dfm_zbcrs_device_values(dfm_zbcrs *A, double **val_dev, double **dia_dev);
+int dfm_zbcrs_mearge(dfm_zbcrs *A, unsigned nblkel, const unsigned *blkel,
+                     unsigned blksize, const double *emat);                /* Mearge (one element, host emat nblkel^2 complex) zmatdia_blkcrs.cpp:198-237 */
+int dfm_zbcrs_set_bc(dfm_zbcrs *A, const int *flag);                       /* SetBoundaryCondition zmatdia_blkcrs.cpp:239-268 (host flags) */
+int dfm_zbcrs_matvec(dfm_zbcrs *A, double alpha, const double *x_dev, double beta,
+                     double *y_dev);                                       /* MatVec zmatdia_blkcrs.cpp:737-770 */
+int dfm_zvec_dot(size_t n, const double *x_dev, const double *y_dev, double *result); /* CZVector_Blk operator* (no conjugate) zvector_blk.cpp:39-60; result[2] */
+int dfm_zvec_norm2(size_t n, const double *x_dev, double *result);                    /* GetSquaredVectorNorm zvector_blk.cpp:154-161 */
+int dfm_zvec_zero_bc(size_t n, const int *flag, double *x_dev);                        /* CBCFlag::SetZeroToBCDof on a complex vector */
+/* Element assembly (src/femeqn/eqn_helmholtz.cpp AddLinSys_Helmholtz loops):
+ * plans made on the complex matrix's pattern add element rows in element
+ * order per entry, as sequential Mearge calls do. */
+int dfm_zasm_create(dfm_zbcrs *A, unsigned nnode, unsigned nelem, unsigned npe,
+                    const unsigned *conn, dfm_asm **out);
+/* Helmholtz P1 triangle, eqn_helmholtz.cpp:51-133 (k = 2*3.1416/wave_length);
+ * res_dev (may be NULL) += -emat value. */
+int dfm_zassemble_helmholtz2d_p1(dfm_zbcrs *A, dfm_asm *p, const double *coord_dev, double wave_length,
+                                 const double *value_dev, double *res_dev);
+/* Sommerfeld radiation line element, eqn_helmholtz.cpp:437-497. */
+int dfm_zassemble_sommerfeld_b1(dfm_zbcrs *A, dfm_asm *p, const double *coord_dev, double wave_length,
+                                const double *value_dev, double *res_dev);
+/* Symbolic ILU(lev_fill): CZMatDiaFrac_BlkCrs(lev,A) zmatdiafrac_blkcrs.cpp:77-108 + AddFracPtn
+ * :263-432; orderings as dfm_ilu_create (NATURAL is the reference's). */
+int dfm_zilu_create(const dfm_zbcrs *A, int lev_fill, int ordering, const int *order, dfm_zilu **out);
+int dfm_zilu_destroy(dfm_zilu *M);
+int dfm_zilu_info(const dfm_zilu *M, unsigned *ncrs, int *nlevel_lower, int *nlevel_upper, int *ncolour);
+int dfm_zilu_get_pattern(const dfm_zilu *M, unsigned *rowptr, unsigned *col, int *lev,
+                         unsigned *diaind, int *order);
+/* SetValue + DoILUDecomp zmatdiafrac_blkcrs.cpp:466-486, 192-262.  *nsmall (may be NULL):
+ * pivots with |d|^2 < 1e-20 (the reference reports them and divides anyway). */
+int dfm_zilu_set_value(dfm_zilu *M, dfm_zbcrs *A, int *nsmall);
+int dfm_zilu_get_values(const dfm_zilu *M, double *val, double *dia);
+/* ForwardSubstitution / BackwardSubstitution zmatdiafrac_blkcrs.cpp:116-157 (factor numbering) */
+int dfm_zilu_forward(dfm_zilu *M, double *vec_dev);
+int dfm_zilu_backward(dfm_zilu *M, double *vec_dev);
+/* Solve / SolvePrecond zmatdiafrac_blkcrs.cpp:164-170 on a vector in the original numbering */
+int dfm_zilu_solve_precond(dfm_zilu *M, double *vec_dev);
+/* MatVec::Sol::Solve_PCOCG zsolver_mat_iter.cpp:303-392 with the factor as
+ * preconditioner.  r_dev: in right-hand side, out final residual; u_dev: out
+ * solution (without the last step's update, as the reference).  conv_ratio: in
+ * tolerance, out sqrt(||r||^2/||r0||^2) at convergence (unchanged at max_iter);
+ * iteration: in max_iter, out iterations.  Returns DFM_OK in both cases, as
+ * the reference returns true. */
+int dfm_zsolve_pcocg(dfm_zbcrs *A, dfm_zilu *M, double *r_dev, double *u_dev,
+                     double *conv_ratio, unsigned *iteration);
+
 /* ---- measurement (no reference counterpart) -------------------------------
  * Times one device kernel of a solve on its real operands: which = 0 the
  * sliced-ELL SpMV of A, 1 the forward sweep (all lower level sets), 2 the

@@ -1224,7 +1224,7 @@ int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, c
     const double rr0 = znorm2(n, rv);
     if (rr0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; return 1; }
     const double inv_rr0 = 1.0 / rr0;
-    zc *p = (zc *)malloc(sizeof(zc) * n), *Ap = (zc *)malloc(sizeof(zc) * n), *w = (zc *)malloc(sizeof(zc) * n);
+    zc *p = (zc *)malloc(sizeof(zc) * n), *Ap = (zc *)calloc(n, sizeof(zc)), *w = (zc *)malloc(sizeof(zc) * n);
     zc *tmp = (zc *)malloc(sizeof(zc) * n);
     int *inv = NULL;
     if (order) {

@@ -1,0 +1,247 @@
+"""Complex twin on the device (through the C-ABI) against the oracle's
+restatement of the reference's complex classes, which the compiled reference
+pins bit for bit (tests/test_oracle_ref.py, tests/test_oracle_golden.py).
+
+Bitwise: pattern, device element assembly (Helmholtz triangles, then
+Sommerfeld lines), per-element Mearge, Dirichlet rows and columns, MatVec,
+the ILU(k) fill pattern, factor values and substitutions, in natural and
+multicolour order.  Solve_PCOCG uses parallel reductions for its dot
+products, so its iterates differ from the oracle's by rounding; COCG on
+these indefinite systems converges irregularly, so the bar is the iteration
+count within 5 % (at least 2) and the solution within 1e-8 relative error at
+the 1e-10 tolerance, plus the reference's return semantics (true at
+max_iter with conv_ratio unchanged, zero right-hand side)."""
+import os
+
+import numpy as np
+import pytest
+
+from delfem_amd import meshgen
+from oracle.oracle import assemble_zproblem
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+PROBLEMS = {
+    "zhelmholtz2d_10x8": lambda: meshgen.zhelmholtz2d(10, 8, 1.0, 0.8, wave_length=0.3),
+    "zhelmholtz2d_40x30": lambda: meshgen.zhelmholtz2d(40, 30, 2.0, 1.5, wave_length=0.6),
+    "zhelmholtz2d_48x48": lambda: meshgen.zhelmholtz2d(48, 48, 1.0, 1.0, wave_length=0.25),
+}
+
+
+def crandn(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).to("cuda")
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def system(gpu, prob):
+    S = gpu.ZHelmholtzSystem(prob)
+    nres = S.assemble()
+    return S, nres
+
+
+def colour_order(orc, Z, ordering):
+    if ordering == "multicolour":
+        return orc.colour_order(Z["rowptr"], Z["col"])[2]
+    return None
+
+
+@pytest.mark.parametrize("name", sorted(PROBLEMS))
+def test_zassembly(orc, gpu, name):
+    prob = PROBLEMS[name]()
+    print(f"[case] {name}", flush=True)
+    S, nres = system(gpu, prob)
+    Z = assemble_zproblem(orc, prob)
+    rowptr, col = S.A.pattern()
+    assert np.array_equal(rowptr, Z["rowptr"]) and np.array_equal(col, Z["col"])
+    val, dia = S.A.values()
+    assert np.array_equal(val, Z["val"]) and np.array_equal(dia, Z["dia"])
+    assert np.array_equal(host(S.res), Z["res"])
+    assert nres == pytest.approx(np.linalg.norm(Z["res"]), rel=1e-13)
+
+
+def test_zassembly_golden(gpu):
+    """The reference's own assembled complex system (fixture written by the
+    compiled reference), bit for bit."""
+    d = np.load(os.path.join(GOLDEN, "zhelmholtz2d_10x8.npz"), allow_pickle=False)
+    prob = meshgen.zhelmholtz2d(10, 8, 1.0, 0.8, wave_length=float(d["wave_length"]))
+    S, _ = system(gpu, prob)
+    val, dia = S.A.values()
+    assert np.array_equal(val, d["zval"]) and np.array_equal(dia, d["zdia"])
+    assert np.array_equal(host(S.res), d["zres"])
+
+
+def test_zmearge(orc, gpu):
+    """Element-by-element Mearge of the oracle's element matrices (triangles,
+    then lines) and SetBoundaryCondition equal the oracle's assembly."""
+    prob = PROBLEMS["zhelmholtz2d_10x8"]()
+    Z = assemble_zproblem(orc, prob)
+    A = gpu.CZMatDia_BlkCrs(prob.nnode, 1)
+    A.AddPattern(Z["rowptr"], Z["col"])
+    for conn, em, npe in ((prob.conn, Z["emat_tri"], 3), (prob.lconn, Z["emat_line"], 2)):
+        for e in range(conn.shape[0]):
+            A.Mearge(conn[e], em[e * npe * npe:(e + 1) * npe * npe])
+    A.SetBoundaryCondition(prob.flags)
+    val, dia = A.values()
+    assert np.array_equal(val, Z["val"]) and np.array_equal(dia, Z["dia"])
+
+
+@pytest.mark.parametrize("name", ["zhelmholtz2d_10x8", "zhelmholtz2d_40x30"])
+def test_zmatvec_and_dots(orc, gpu, name):
+    prob = PROBLEMS[name]()
+    S, _ = system(gpu, prob)
+    Z = assemble_zproblem(orc, prob)
+    n = prob.nnode
+    x, y = crandn(n, 1), crandn(n, 2)
+    yd = dev(y)
+    S.A.MatVec(1.5, dev(x), -0.25, yd)
+    want = orc.zmatvec(n, Z["rowptr"], Z["col"], Z["val"], Z["dia"], 1.5, x, -0.25, y)
+    assert np.array_equal(host(yd), want)
+    # CZVector_Blk operator* has no conjugate
+    assert gpu.zdot(dev(x), dev(y)) == pytest.approx(np.sum(x * y), rel=1e-13)
+    assert gpu.znorm2(dev(x)) == pytest.approx(np.sum(np.abs(x) ** 2), rel=1e-13)
+
+
+@pytest.mark.parametrize("ordering", ["natural", "multicolour"])
+@pytest.mark.parametrize("lev", [0, 1, 2])
+@pytest.mark.parametrize("name", ["zhelmholtz2d_10x8", "zhelmholtz2d_40x30"])
+def test_zilu(orc, gpu, name, lev, ordering):
+    """Fill pattern, factor values, forward / backward substitution (factor
+    numbering) and SolvePrecond (original numbering) bit for bit."""
+    prob = PROBLEMS[name]()
+    print(f"[case] {name} lev {lev} {ordering}", flush=True)
+    S, _ = system(gpu, prob)
+    Z = assemble_zproblem(orc, prob)
+    n = prob.nnode
+    order = colour_order(orc, Z, ordering)
+    M = gpu.CZMatDiaFrac_BlkCrs(lev, S.A, ordering=ordering)
+    P = M.pattern()
+    F = orc.iluk_symbolic(Z["rowptr"], Z["col"], lev, order)
+    for k in ("rowptr", "col", "diaind"):
+        assert np.array_equal(P[k], F[k]), k
+    if order is not None:
+        assert np.array_equal(P["order"], order)
+    assert M.SetValue(S.A)
+    fv, fd = M.values()
+    ofv, ofd, ns = orc.zilu_values(n, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F, order)
+    assert M.nsmall == ns == 0
+    assert np.array_equal(fv, ofv) and np.array_equal(fd, ofd)
+    v = crandn(n, 3)
+    vd = dev(v)
+    M.ForwardSubstitution(vd)
+    M.BackwardSubstitution(vd)
+    assert np.array_equal(host(vd), orc.zilu_solve(n, F, ofv, ofd, v))
+    vd = dev(v)
+    M.SolvePrecond(vd)
+    if order is None:
+        want = orc.zilu_solve(n, F, ofv, ofd, v)
+    else:
+        want = np.empty_like(v)
+        want[order] = orc.zilu_solve(n, F, ofv, ofd, v[order])
+    assert np.array_equal(host(vd), want)
+
+
+def pcocg_case(orc, gpu, prob, lev, ordering, tol=1e-10, maxit=3000):
+    import torch
+    S, _ = system(gpu, prob)
+    Z = assemble_zproblem(orc, prob)
+    n = prob.nnode
+    order = colour_order(orc, Z, ordering)
+    M = gpu.CZMatDiaFrac_BlkCrs(lev, S.A, ordering=ordering)
+    M.SetValue(S.A)
+    r = S.res.clone()
+    u = torch.full((n,), 7.0 + 0j, dtype=torch.complex128, device="cuda")
+    ok, conv, it = gpu.Solve_PCOCG(tol, maxit, S.A, M, r, u)
+    F = orc.iluk_symbolic(Z["rowptr"], Z["col"], lev, order)
+    fv, fd, _ = orc.zilu_values(n, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F, order)
+    want = orc.zsolve_pcocg(n, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F, fv, fd, Z["res"], tol, maxit, order)
+    return (ok, conv, it, host(u), host(r)), want, Z
+
+
+@pytest.mark.parametrize("ordering", ["natural", "multicolour"])
+@pytest.mark.parametrize("lev", [0, 1])
+@pytest.mark.parametrize("name", sorted(PROBLEMS))
+def test_zpcocg(orc, gpu, name, lev, ordering):
+    prob = PROBLEMS[name]()
+    print(f"[case] {name} lev {lev} {ordering}", flush=True)
+    (ok, conv, it, u, r), (ok_o, conv_o, it_o, u_o, r_o), Z = pcocg_case(orc, gpu, prob, lev, ordering)
+    print(f"  device it={it} conv={conv:.3e}  oracle it={it_o} conv={conv_o:.3e}  "
+          f"|u-u_o|/|u_o|={np.linalg.norm(u - u_o) / np.linalg.norm(u_o):.2e}", flush=True)
+    assert ok and ok_o
+    assert conv < 1e-10
+    assert abs(it - it_o) <= max(2, int(np.ceil(0.05 * it_o))), (it, it_o)
+    assert np.linalg.norm(u - u_o) <= 1e-8 * np.linalg.norm(u_o)
+    # the final residual is the recursive one, of norm conv * ||b||
+    b = np.linalg.norm(Z["res"])
+    assert np.linalg.norm(r) == pytest.approx(conv * b, rel=1e-6)
+
+
+def test_zpcocg_golden(gpu):
+    """The reference's Solve_PCOCG runs of the fixture (natural ILU(0) and
+    ILU(1)) on the fixture's matrix: iterations within 2, solution 1e-8."""
+    import torch
+    d = np.load(os.path.join(GOLDEN, "zhelmholtz2d_10x8.npz"), allow_pickle=False)
+    prob = meshgen.zhelmholtz2d(10, 8, 1.0, 0.8, wave_length=float(d["wave_length"]))
+    S, _ = system(gpu, prob)
+    n = prob.nnode
+    for lev in (0, 1):
+        M = gpu.CZMatDiaFrac_BlkCrs(lev, S.A)
+        M.SetValue(S.A)
+        fv, fd = M.values()
+        assert np.array_equal(fv, d[f"zfval_k{lev}"]) and np.array_equal(fd, d[f"zfdia_k{lev}"])
+        r = dev(d["zres"])
+        u = torch.zeros(n, dtype=torch.complex128, device="cuda")
+        ok, conv, it = gpu.Solve_PCOCG(1e-10, 2000, S.A, M, r, u)
+        want = d[f"zpcocg_k{lev}"]
+        assert ok and want[0] == 1.0
+        assert abs(it - int(want[2])) <= 2 and conv < 1e-10
+        u_ref = d[f"zpcocg_u_k{lev}"]
+        assert np.linalg.norm(host(u) - u_ref) <= 1e-8 * np.linalg.norm(u_ref)
+
+
+@pytest.mark.parametrize("maxit", [1, 2, 5])
+def test_zpcocg_max_iter(orc, gpu, maxit):
+    """Stopped by max_iter: true, iteration = max_iter, conv_ratio unchanged;
+    the partial solution follows the oracle's."""
+    prob = PROBLEMS["zhelmholtz2d_40x30"]()
+    (ok, conv, it, u, r), (ok_o, conv_o, it_o, u_o, r_o), _ = pcocg_case(orc, gpu, prob, 0, "natural", tol=1e-14,
+                                                                         maxit=maxit)
+    assert (ok, conv, it) == (True, 1e-14, maxit) == (ok_o, conv_o, it_o)
+    scale = max(np.linalg.norm(u_o), 1e-300)
+    assert np.linalg.norm(u - u_o) <= 1e-12 * scale
+    assert np.linalg.norm(r - r_o) <= 1e-12 * np.linalg.norm(r_o)
+
+
+def test_zpcocg_zero_rhs(gpu):
+    import torch
+    prob = PROBLEMS["zhelmholtz2d_10x8"]()
+    S, _ = system(gpu, prob)
+    M = gpu.CZMatDiaFrac_BlkCrs(0, S.A)
+    M.SetValue(S.A)
+    n = prob.nnode
+    r = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    u = torch.full((n,), 3.0 + 1j, dtype=torch.complex128, device="cuda")
+    assert gpu.Solve_PCOCG(1e-10, 100, S.A, M, r, u) == (True, 0.0, 0)
+    assert torch.count_nonzero(u).item() == 0
+
+
+def test_zpattern_mismatch(gpu):
+    """A factor refuses a matrix of another pattern."""
+    from delfem_amd._lib import DfmError
+    a = gpu.ZHelmholtzSystem(PROBLEMS["zhelmholtz2d_10x8"]())
+    a.assemble()
+    b = gpu.ZHelmholtzSystem(meshgen.zhelmholtz2d(8, 10, 1.0, 0.8, wave_length=0.3))
+    b.assemble()
+    M = gpu.CZMatDiaFrac_BlkCrs(0, a.A)
+    with pytest.raises(DfmError):
+        M.SetValue(b.A)
