@@ -99,6 +99,40 @@ __device__ inline double2 zrdiv(double d, double2 c) {  // double / Complex
 __device__ inline double2 zdscale(double d, double2 c) { return make_double2(d * c.x, d * c.y); }
 __device__ inline double2 zzero() { return make_double2(0.0, 0.0); }
 
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+// streamed once per pass: non-temporal 16-byte load
+__device__ inline double2 zload_nt(const double2* p) {
+  const f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(p));
+  return make_double2(v.x, v.y);
+}
+
+// One lane's row of a sliced-ELL matrix, slots [k0, k1): t -= a_k x[c_k]
+// in slot order (padding slots, column < 0, are skipped), with the column,
+// value and gathered x loads of four slots issued before their operations.
+__device__ inline double2 zrow_sub(double2 t, int k0, int k1, int lane, const int* __restrict__ col,
+                                   const double2* __restrict__ val, const double2* x) {
+  int k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    int c[4];
+    double2 a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) c[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * 64 + lane);
+#pragma unroll
+    for (int u = 0; u < 4; u++) a[u] = zload_nt(val + (size_t)(k + u) * 64 + lane);
+#pragma unroll
+    for (int u = 0; u < 4; u++) b[u] = c[u] >= 0 ? x[c[u]] : zzero();
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (c[u] >= 0) t = zsub(t, zmul(a[u], b[u]));
+  }
+  for (; k < k1; k++) {
+    const int c = __builtin_nontemporal_load(col + (size_t)k * 64 + lane);
+    if (c < 0) break;
+    t = zsub(t, zmul(zload_nt(val + (size_t)k * 64 + lane), x[c]));
+  }
+  return t;
+}
+
 // ---------------------------------------------------------------------------
 // Element assembly, gathered per row: thread i adds the row-i entries of the
 // elements around node i in ascending element order, the order in which
@@ -319,45 +353,54 @@ __global__ void __launch_bounds__(kNT) k_zsell_dia(long nlane, const int* __rest
 
 // Forward substitution (zmatdiafrac_blkcrs.cpp:116-138) of the slices
 // [s0, s1) of one level, one wavefront per slice, lane = row:
-// x_i = d_i (x_i - sum_q L_iq x_col(q)), the sum taken in CRS order.
+// x_i = d_i (src_i - sum_q L_iq x_col(q)), the sum taken in CRS order.  src
+// may be x (in place) or another vector (the Krylov loop reads r and writes
+// w = M^-1 r without a copy: every row belongs to exactly one level).
 __global__ void __launch_bounds__(64) k_zfwd(int s0, int s1, const int* __restrict__ rows,
                                             const int* __restrict__ sptr, const int* __restrict__ col,
                                             const double2* __restrict__ val, const double2* __restrict__ dval,
-                                            double2* __restrict__ x, const ZState* __restrict__ st) {
+                                            const double2* src, double2* x, const ZState* __restrict__ st) {
   if (st && st->done) return;
   const int s = s0 + xcd_block();
   if (s >= s1) return;
   const int lane = threadIdx.x;
   const int row = rows[(size_t)s * 64 + lane];
   if (row < 0) return;
-  double2 t = x[row];
-  const int k1 = sptr[s + 1];
-  for (int k = sptr[s]; k < k1; k++) {
-    const int c = col[(size_t)k * 64 + lane];
-    if (c < 0) break;
-    t = zsub(t, zmul(val[(size_t)k * 64 + lane], x[c]));
-  }
+  const double2 t = zrow_sub(src[row], sptr[s], sptr[s + 1], lane, col, val, x);
   x[row] = zmul(dval[(size_t)s * 64 + lane], t);
 }
-// Backward substitution (:140-157): x_i -= sum_q U_iq x_col(q).
+// Backward substitution (:140-157): x_i -= sum_q U_iq x_col(q).  DOT: also
+// the slice's share of (r, x) for the final x (partials[s], partials[nparts+s]):
+// every row is finished by exactly one slice.
+template <bool DOT>
 __global__ void __launch_bounds__(64) k_zbwd(int s0, int s1, const int* __restrict__ rows,
                                             const int* __restrict__ sptr, const int* __restrict__ col,
                                             const double2* __restrict__ val, double2* __restrict__ x,
+                                            const double2* __restrict__ r, double* __restrict__ partials, int nparts,
                                             const ZState* __restrict__ st) {
   if (st && st->done) return;
   const int s = s0 + xcd_block();
   if (s >= s1) return;
   const int lane = threadIdx.x;
   const int row = rows[(size_t)s * 64 + lane];
-  if (row < 0) return;
-  double2 t = x[row];
-  const int k1 = sptr[s + 1];
-  for (int k = sptr[s]; k < k1; k++) {
-    const int c = col[(size_t)k * 64 + lane];
-    if (c < 0) break;
-    t = zsub(t, zmul(val[(size_t)k * 64 + lane], x[c]));
+  double v0 = 0.0, v1 = 0.0;
+  if (row >= 0) {
+    const double2 t = zrow_sub(x[row], sptr[s], sptr[s + 1], lane, col, val, x);
+    x[row] = t;
+    if (DOT) {
+      const double2 ri = r[row];
+      v0 = ri.x * t.x - ri.y * t.y;
+      v1 = ri.x * t.y + ri.y * t.x;
+    }
   }
-  x[row] = t;
+  if (DOT) {
+    v0 = wave_sum(v0);
+    v1 = wave_sum(v1);
+    if (lane == 0) {
+      partials[s] = v0;
+      partials[(size_t)nparts + s] = v1;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -419,6 +462,10 @@ __global__ void __launch_bounds__(kFinNT) k_zfinalize(const double* __restrict__
   } else if constexpr (FIN == ZFIN_RW0) {
     st->rw = make_double2(s[0], s[1]);
   } else if constexpr (FIN == ZFIN_ALPHA) {
+    if (st->iter >= st->maxit) {  // the loop runs iterations 1 .. max_iter-1
+      st->done = 2;
+      return;
+    }
     st->alpha = zdiv(st->rw, make_double2(s[0], s[1]));
   } else if constexpr (FIN == ZFIN_RATIO) {
     const double ratio = sqrt(s[0] * st->inv_rr0);
@@ -431,7 +478,7 @@ __global__ void __launch_bounds__(kFinNT) k_zfinalize(const double* __restrict__
     const double2 rw2 = make_double2(s[0], s[1]);
     st->beta = zdiv(rw2, st->rw);
     st->rw = rw2;
-    if (++st->iter >= st->maxit) st->done = 2;
+    st->iter++;
   }
 }
 
@@ -452,10 +499,27 @@ __global__ void __launch_bounds__(kNT) k_zspmv_pap(int nslice, const int* __rest
     if (row >= 0) {
       double re = 0.0, im = 0.0;
       const int k1 = sptr[s + 1];
-      for (int k = sptr[s]; k < k1; k++) {
-        const int c = col[(size_t)k * 64 + lane];
+      int k = sptr[s];
+      for (; k + 4 <= k1; k += 4) {
+        int c[4];
+        double2 a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) c[u] = __builtin_nontemporal_load(col + (size_t)(k + u) * 64 + lane);
+#pragma unroll
+        for (int u = 0; u < 4; u++) a[u] = zload_nt(val + (size_t)(k + u) * 64 + lane);
+#pragma unroll
+        for (int u = 0; u < 4; u++) b[u] = c[u] >= 0 ? p[c[u]] : zzero();
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (c[u] >= 0) {
+            re += a[u].x * b[u].x - a[u].y * b[u].y;
+            im += a[u].x * b[u].y + a[u].y * b[u].x;
+          }
+      }
+      for (; k < k1; k++) {
+        const int c = __builtin_nontemporal_load(col + (size_t)k * 64 + lane);
         if (c < 0) break;
-        const double2 a = val[(size_t)k * 64 + lane], b = p[c];
+        const double2 a = zload_nt(val + (size_t)k * 64 + lane), b = p[c];
         re += a.x * b.x - a.y * b.y;
         im += a.x * b.y + a.y * b.x;
       }
@@ -481,23 +545,17 @@ __global__ void __launch_bounds__(kNT) k_zupd_r(long n, const double2* __restric
   }
   write_partials<1>(v, partials, blockIdx.x, gridDim.x);
 }
-// u += alpha p (after the convergence test, as the reference); w = r
-__global__ void __launch_bounds__(kNT) k_zupd_uw(long n, const double2* __restrict__ p, double2* __restrict__ u,
-                                                const double2* __restrict__ r, double2* __restrict__ w,
-                                                const ZState* __restrict__ st) {
-  if (st->done) return;
-  const double2 al = st->alpha;
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
-    u[i] = zadd(u[i], zmul(al, p[i]));
-    w[i] = r[i];
-  }
-}
+// u += alpha p (after the convergence test, as the reference), then
 // p = p beta + w
-__global__ void __launch_bounds__(kNT) k_zupd_p(long n, const double2* __restrict__ w, double2* __restrict__ p,
-                                               const ZState* __restrict__ st) {
+__global__ void __launch_bounds__(kNT) k_zupd_up(long n, const double2* __restrict__ w, double2* __restrict__ p,
+                                                double2* __restrict__ u, const ZState* __restrict__ st) {
   if (st->done) return;
-  const double2 be = st->beta;
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) p[i] = zadd(zmul(p[i], be), w[i]);
+  const double2 al = st->alpha, be = st->beta;
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
+    const double2 pi = p[i];
+    u[i] = zadd(u[i], zmul(al, pi));
+    p[i] = zadd(zmul(pi, be), w[i]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -513,20 +571,27 @@ int zrefresh_a(dfm_zilu_s* Z, const dfm_zbcrs_s* A) {
   return DFM_OK;
 }
 
-// Forward and / or backward substitution of x (factor numbering), one launch
-// per level.
-int zsweep(dfm_zilu_s* Z, double2* x, const ZState* st, bool fwd, bool bwd) {
+// M^-1 in factor numbering, one launch per level: forward from src into x
+// (src may be x), then backward on x.  dot_r: also (r, x) into partials
+// (one entry per backward slice).
+int zsweep(dfm_zilu_s* Z, const double2* src, double2* x, const ZState* st, bool fwd, bool bwd,
+           const double2* dot_r = nullptr, double* partials = nullptr) {
   dfm_ilu_s* M = Z->M;
   if (fwd)
     for (int l = 0; l < M->nlev_lower; l++) {
       const int s0 = M->L64.group_slice_ptr[l], s1 = M->L64.group_slice_ptr[l + 1];
-      DFM_ZLAUNCH(k_zfwd, s1 - s0, 64, s0, s1, M->L64.rows.p, M->L64.slice_ptr.p, M->L64.col.p, Z->zL.p, Z->zLd.p, x,
-                  st);
+      DFM_ZLAUNCH(k_zfwd, s1 - s0, 64, s0, s1, M->L64.rows.p, M->L64.slice_ptr.p, M->L64.col.p, Z->zL.p, Z->zLd.p,
+                  src, x, st);
     }
   if (bwd)
     for (int l = 0; l < M->nlev_upper; l++) {
       const int s0 = M->U64.group_slice_ptr[l], s1 = M->U64.group_slice_ptr[l + 1];
-      DFM_ZLAUNCH(k_zbwd, s1 - s0, 64, s0, s1, M->U64.rows.p, M->U64.slice_ptr.p, M->U64.col.p, Z->zU.p, x, st);
+      if (dot_r)
+        DFM_ZLAUNCH(k_zbwd<true>, s1 - s0, 64, s0, s1, M->U64.rows.p, M->U64.slice_ptr.p, M->U64.col.p, Z->zU.p, x,
+                    dot_r, partials, M->U64.nslice, st);
+      else
+        DFM_ZLAUNCH(k_zbwd<false>, s1 - s0, 64, s0, s1, M->U64.rows.p, M->U64.slice_ptr.p, M->U64.col.p, Z->zU.p, x,
+                    (const double2*)nullptr, (double*)nullptr, 0, st);
     }
   return DFM_OK;
 }
@@ -858,13 +923,13 @@ int dfm_zilu_get_values(const dfm_zilu* Z, double* val, double* dia) {
 }
 int dfm_zilu_forward(dfm_zilu* Z, double* vec) {
   DFM_CHECK(Z && vec && Z->factored, DFM_BAD_ARG, "factor has no values (call dfm_zilu_set_value)");
-  DFM_TRY(zsweep(Z, (double2*)vec, nullptr, true, false));
+  DFM_TRY(zsweep(Z, (const double2*)vec, (double2*)vec, nullptr, true, false));
   DFM_HIP(hipStreamSynchronize(stream()));
   return DFM_OK;
 }
 int dfm_zilu_backward(dfm_zilu* Z, double* vec) {
   DFM_CHECK(Z && vec && Z->factored, DFM_BAD_ARG, "factor has no values (call dfm_zilu_set_value)");
-  DFM_TRY(zsweep(Z, (double2*)vec, nullptr, false, true));
+  DFM_TRY(zsweep(Z, (const double2*)vec, (double2*)vec, nullptr, false, true));
   DFM_HIP(hipStreamSynchronize(stream()));
   return DFM_OK;
 }
@@ -874,12 +939,12 @@ int dfm_zilu_solve_precond(dfm_zilu* Z, double* vec) {
   const long n = M->nb;
   double2* v = (double2*)vec;
   if (!M->permuted) {
-    DFM_TRY(zsweep(Z, v, nullptr, true, true));
+    DFM_TRY(zsweep(Z, v, v, nullptr, true, true));
   } else {
     DevBuf<double2> tmp;
     DFM_TRY(tmp.alloc(std::max<long>(1, n)));
     DFM_ZLAUNCH(k_zgather, zgrid_vec(n), kNT, n, M->d_order.p, (const double2*)v, tmp.p);
-    DFM_TRY(zsweep(Z, tmp.p, nullptr, true, true));
+    DFM_TRY(zsweep(Z, tmp.p, tmp.p, nullptr, true, true));
     DFM_ZLAUNCH(k_zgather, zgrid_vec(n), kNT, n, M->d_inv.p, (const double2*)tmp.p, v);
   }
   DFM_HIP(hipStreamSynchronize(stream()));
@@ -904,9 +969,10 @@ int dfm_zsolve_pcocg(dfm_zbcrs* A, dfm_zilu* Z, double* r_dev, double* u_dev, do
   const unsigned mx_iter = *iteration;
   DFM_TRY(zrefresh_a(Z, A));
   if (Z->work.n < (size_t)(5 * n)) DFM_TRY(Z->work.alloc(std::max<long>(1, 5 * n)));
-  double2 *r = Z->work.p, *u = r + n, *p = u + n, *Ap = p + n, *w = Ap + n;
+  double2 *r = Z->work.p, *u = r + n, *p = u + n, *Ap = p + n, *w = Ap + n;  // factor numbering
   const int gv = zgrid_vec(n), nsA = M->A64.nslice, gA = std::max(1, (nsA + kWaves - 1) / kWaves);
-  const int nparts_max = std::max(gv, gA);
+  const int nsU = M->U64.nslice;
+  const int nparts_max = std::max(std::max(gv, gA), nsU);
   if (Z->partials.n < (size_t)(2 * nparts_max)) DFM_TRY(Z->partials.alloc(2 * nparts_max));
   double* parts = Z->partials.p;
   ZState* st = Z->state.p;
@@ -930,12 +996,10 @@ int dfm_zsolve_pcocg(dfm_zbcrs* A, dfm_zilu* Z, double* r_dev, double* u_dev, do
     *iteration = 0;
     return DFM_OK;
   }
-  // w = M^-1 r, p = w, rw = (r, w)
-  DFM_ZLAUNCH(k_zgather, gv, kNT, n, (const int*)nullptr, (const double2*)r, w);
-  DFM_TRY(zsweep(Z, w, nullptr, true, true));
+  // w = M^-1 r (with the partials of (r, w)), p = w
+  DFM_TRY(zsweep(Z, r, w, nullptr, true, true, r, parts));
+  DFM_ZLAUNCH((k_zfinalize<2, ZFIN_RW0>), 1, kFinNT, parts, nsU, st);
   DFM_ZLAUNCH(k_zgather, gv, kNT, n, (const int*)nullptr, (const double2*)w, p);
-  DFM_ZLAUNCH(k_zdot, gv, kNT, n, (const double2*)r, (const double2*)w, parts, (const ZState*)nullptr);
-  DFM_ZLAUNCH((k_zfinalize<2, ZFIN_RW0>), 1, kFinNT, parts, gv, st);
   if (mx_iter > 1) {
     auto body = [&]() -> int {
       DFM_ZLAUNCH(k_zspmv_pap, gA, kNT, nsA, M->A64.rows.p, M->A64.slice_ptr.p, M->A64.col.p,
@@ -944,11 +1008,9 @@ int dfm_zsolve_pcocg(dfm_zbcrs* A, dfm_zilu* Z, double* r_dev, double* u_dev, do
       DFM_ZLAUNCH((k_zfinalize<2, ZFIN_ALPHA>), 1, kFinNT, parts, gA, st);
       DFM_ZLAUNCH(k_zupd_r, gv, kNT, n, (const double2*)Ap, r, parts, (const ZState*)st);
       DFM_ZLAUNCH((k_zfinalize<1, ZFIN_RATIO>), 1, kFinNT, parts, gv, st);
-      DFM_ZLAUNCH(k_zupd_uw, gv, kNT, n, (const double2*)p, u, (const double2*)r, w, (const ZState*)st);
-      DFM_TRY(zsweep(Z, w, st, true, true));
-      DFM_ZLAUNCH(k_zdot, gv, kNT, n, (const double2*)r, (const double2*)w, parts, (const ZState*)st);
-      DFM_ZLAUNCH((k_zfinalize<2, ZFIN_BETA>), 1, kFinNT, parts, gv, st);
-      DFM_ZLAUNCH(k_zupd_p, gv, kNT, n, (const double2*)w, p, (const ZState*)st);
+      DFM_TRY(zsweep(Z, r, w, st, true, true, r, parts));
+      DFM_ZLAUNCH((k_zfinalize<2, ZFIN_BETA>), 1, kFinNT, parts, nsU, st);
+      DFM_ZLAUNCH(k_zupd_up, gv, kNT, n, (const double2*)w, p, u, (const ZState*)st);
       return DFM_OK;
     };
     const int check_every = 16;
