@@ -138,6 +138,9 @@ _SIGS = {
     "dfm_zilu_backward": (C.c_int, [H, vp]),
     "dfm_zilu_solve_precond": (C.c_int, [H, vp]),
     "dfm_zsolve_pcocg": (C.c_int, [H, H, vp, vp, f64p, u32p]),
+    "dfm_zsolve_cocg": (C.c_int, [H, vp, vp, f64p, u32p]),
+    "dfm_zsolve_bicgstab": (C.c_int, [H, vp, vp, f64p, u32p]),
+    "dfm_zsolve_bicgstabp": (C.c_int, [H, H, vp, vp, f64p, u32p]),
     "dfm_bench_kernel": (C.c_int, [H, H, C.c_int, C.c_int, C.c_int, f64p, f64p]),
     "dfm_bench_copy": (C.c_int, [C.c_size_t, C.c_int, f64p]),
     "dfm_bench_read": (C.c_int, [C.c_size_t, C.c_int, f64p]),

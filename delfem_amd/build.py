@@ -32,21 +32,26 @@ def build(verbose=False):
     return LIB
 
 
-SHIM_SRC = os.path.join(ROOT, "tests", "cpp", "shim_test.cpp")
-SHIM_BIN = os.path.join(ROOT, "tests", "cpp", "shim_test")
+SHIM_PROGRAMS = ["shim_test", "zshim_test"]  # tests/cpp/<name>.cpp -> tests/cpp/<name>
 
 
 def build_shim_test(verbose=False):
-    """The C++ program of the drop-in test, linked against the library."""
+    """The C++ programs of the drop-in tests (real and complex classes),
+    linked against the library."""
     cxx = os.environ.get("CXX", "g++")
-    tmp = SHIM_BIN + ".tmp"
-    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), SHIM_SRC, "-o", tmp,
-           "-L", HERE, "-ldelfem_amd", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../delfem_amd"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, SHIM_BIN)
-    return SHIM_BIN
+    out = []
+    for name in SHIM_PROGRAMS:
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        binp = os.path.join(ROOT, "tests", "cpp", name)
+        tmp = binp + ".tmp"
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-o", tmp,
+               "-L", HERE, "-ldelfem_amd", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../delfem_amd"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, binp)
+        out.append(binp)
+    return out
 
 
 if __name__ == "__main__":

@@ -28,13 +28,39 @@ using namespace dfm;
 
 namespace {
 
-// Device state of one Solve_PCOCG run.
+// Device state of one complex Krylov run.
 struct ZState {
-  double2 rw, alpha, beta;  // (r, w) of the current iteration, step sizes
+  double2 rw, alpha, beta;  // (r, w) of the current iteration (COCG / PCOCG), step sizes
+  double2 omega, r_r0;      // BiCGSTAB: omega, InnerProduct(r, r0*)
   double inv_rr0, tol, ratio;
   double d0, d1;            // generic reduction outputs
   int done;                 // 0 running, 1 converged, 2 stopped at max_iter, 3 zero right-hand side
   int iter, conv_iter, maxit;
+};
+
+// Workspace of the device-resident solvers: vectors, reduction partials,
+// state (with a pinned host mirror) and the instantiated hipGraphs of
+// 16-iteration blocks, keyed by their operands.
+struct ZCtx {
+  dfm::DevBuf<double2> work;
+  dfm::DevBuf<double> partials;
+  dfm::DevBuf<ZState> state;
+  ZState* h = nullptr;
+  std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+  ZCtx() = default;
+  ZCtx(const ZCtx&) = delete;
+  ~ZCtx() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    if (h) (void)hipHostFree(h);
+  }
+};
+
+// A sliced-ELL operator: slot k of lane l at col / val[k*64+l], the lane's
+// row at rows[s*64+l], its diagonal at dval[s*64+l].
+struct ZOp {
+  int nslice;
+  const int *rows, *sptr, *col;
+  const double2 *val, *dval;
 };
 
 }  // namespace
@@ -44,6 +70,13 @@ struct dfm_zbcrs_s {
   DevBuf<double2> d_val, d_dia;         // CRS order, diagonal separate
   unsigned long long version = 0;       // renewed whenever values change
   unsigned long long val_hash = 0;      // pattern d_val was sized for
+  // natural-order sliced-ELL copy for the unpreconditioned solvers (built on first use)
+  unsigned long long sell_hash = 0, sell_version = 0;
+  int sell_nslice = 0;
+  DevBuf<int> s_sptr, s_rows, s_col;
+  DevBuf<long long> s_src;
+  DevBuf<double2> s_val, s_dval;
+  ZCtx ctx;
   ~dfm_zbcrs_s() {
     if (pat) dfm_bcrs_destroy(pat);
   }
@@ -56,14 +89,8 @@ struct dfm_zilu_s {
   DevBuf<double2> zA, zAd;              // sliced-ELL copy of A in factor numbering
   unsigned long long a_version = 0;     // version of A mirrored in zA
   bool factored = false;
-  DevBuf<double2> work;                 // Krylov vectors
-  DevBuf<double> partials;
-  DevBuf<ZState> state;
-  ZState* h_state = nullptr;            // pinned mirror
-  std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+  ZCtx ctx;
   ~dfm_zilu_s() {
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
-    if (h_state) (void)hipHostFree(h_state);
     if (M) dfm_ilu_destroy(M);
   }
 };
@@ -427,6 +454,17 @@ __global__ void __launch_bounds__(kNT) k_zdot(long n, const double2* __restrict_
   }
   write_partials<2>(v, partials, blockIdx.x, gridDim.x);
 }
+// InnerProduct (zvector_blk.cpp:62-80): sum conj(a_i) b_i
+__global__ void __launch_bounds__(kNT) k_zdotc(long n, const double2* __restrict__ a, const double2* __restrict__ b,
+                                              double* __restrict__ partials) {
+  double v[2] = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
+    const double2 x = a[i], y = b[i];
+    v[0] += x.x * y.x + x.y * y.y;
+    v[1] += x.x * y.y - x.y * y.x;
+  }
+  write_partials<2>(v, partials, blockIdx.x, gridDim.x);
+}
 // GetSquaredVectorNorm (:154-161)
 __global__ void __launch_bounds__(kNT) k_znorm2(long n, const double2* __restrict__ a, double* __restrict__ partials) {
   double v[1] = {0.0};
@@ -437,7 +475,11 @@ __global__ void __launch_bounds__(kNT) k_znorm2(long n, const double2* __restric
   write_partials<1>(v, partials, blockIdx.x, gridDim.x);
 }
 
-enum { ZFIN_STORE = 0, ZFIN_RR0 = 1, ZFIN_RW0 = 2, ZFIN_ALPHA = 3, ZFIN_RATIO = 4, ZFIN_BETA = 5 };
+enum {
+  ZFIN_STORE = 0, ZFIN_RR0 = 1, ZFIN_RW0 = 2, ZFIN_RR0C = 3,             // run also after the stop
+  ZFIN_ALPHA = 4, ZFIN_RATIO = 5, ZFIN_BETA = 6,                         // COCG / PCOCG
+  ZFIN_BALPHA = 7, ZFIN_BOMEGA = 8, ZFIN_BCONV = 9, ZFIN_BCONVP = 10     // BiCGSTAB / BiCGStabP
+};
 
 // Epilogues of Solve_PCOCG (zsolver_mat_iter.cpp:303-392) on the reduced sums.
 template <int ND, int FIN>
@@ -445,7 +487,7 @@ __global__ void __launch_bounds__(kFinNT) k_zfinalize(const double* __restrict__
                                                      ZState* __restrict__ st) {
   if (FIN >= ZFIN_ALPHA && st->done) return;
   __shared__ double sh[kFinNT / 64];
-  double s[2] = {0.0, 0.0};
+  double s[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int d = 0; d < ND; d++) s[d] = block_sum_n<kFinNT>(strided_sum<kFinNT>(partials + (size_t)d * nparts, nparts), sh);
   if (threadIdx.x != 0) return;
@@ -461,6 +503,31 @@ __global__ void __launch_bounds__(kFinNT) k_zfinalize(const double* __restrict__
     }
   } else if constexpr (FIN == ZFIN_RW0) {
     st->rw = make_double2(s[0], s[1]);
+  } else if constexpr (FIN == ZFIN_RR0C) {
+    st->r_r0 = make_double2(s[0], s[1]);
+  } else if constexpr (FIN == ZFIN_BALPHA) {
+    if (st->iter >= st->maxit) {
+      st->done = 2;
+      return;
+    }
+    st->alpha = zdiv(st->r_r0, make_double2(s[0], s[1]));
+  } else if constexpr (FIN == ZFIN_BOMEGA) {
+    // Complex / double (complex.h:148-150)
+    st->omega = make_double2(s[1] / s[0], s[2] / s[0]);
+  } else if constexpr (FIN == ZFIN_BCONV || FIN == ZFIN_BCONVP) {
+    const double sq_ratio = s[0] * st->inv_rr0;
+    if (FIN == ZFIN_BCONVP) st->ratio = sqrt(s[0] * st->inv_rr0);
+    if (sq_ratio < st->tol * st->tol) {
+      if (FIN == ZFIN_BCONV) st->ratio = sqrt(s[0] * st->inv_rr0);
+      st->conv_iter = st->iter;
+      st->done = 1;
+      return;
+    }
+    const double2 t1 = make_double2(s[1], s[2]);
+    st->beta = FIN == ZFIN_BCONVP ? zmul(zdiv(st->alpha, st->omega), zdiv(t1, st->r_r0))
+                                  : zmul(zdiv(t1, st->r_r0), zdiv(st->alpha, st->omega));
+    st->r_r0 = t1;
+    st->iter++;
   } else if constexpr (FIN == ZFIN_ALPHA) {
     if (st->iter >= st->maxit) {  // the loop runs iterations 1 .. max_iter-1
       st->done = 2;
@@ -482,18 +549,26 @@ __global__ void __launch_bounds__(kFinNT) k_zfinalize(const double* __restrict__
   }
 }
 
-// Ap = A p over the sliced-ELL copy of A (factor numbering), one wavefront
-// per slice, with this workgroup's partial of (p, Ap).  As MatVec(1, p, 0, Ap)
-// except that the old Ap is not read (beta = 0 multiplies a finite value).
-__global__ void __launch_bounds__(kNT) k_zspmv_pap(int nslice, const int* __restrict__ rows,
-                                                  const int* __restrict__ sptr, const int* __restrict__ col,
-                                                  const double2* __restrict__ val, const double2* __restrict__ dval,
-                                                  const double2* __restrict__ p, double2* __restrict__ Ap,
-                                                  double* __restrict__ partials, const ZState* __restrict__ st) {
+// y = A x over a sliced-ELL operator, one wavefront per slice (as MatVec(1,
+// x, 0, y) except that the old y is not read: 0 times a finite value), with
+// this workgroup's partials of
+//   EP_PAP     x . y                          (Solve_COCG / Solve_PCOCG)
+//   EP_CONJ    InnerProduct(y, aux) = sum conj(y) aux       (BiCGSTAB alpha)
+//   EP_OMEGA   ||y||^2 and InnerProduct(aux, y)             (BiCGSTAB omega)
+enum { EP_PAP = 0, EP_CONJ = 1, EP_OMEGA = 2 };
+template <int EP>
+__global__ void __launch_bounds__(kNT) k_zspmv(int nslice, const int* __restrict__ rows, const int* __restrict__ sptr,
+                                              const int* __restrict__ col, const double2* __restrict__ val,
+                                              const double2* __restrict__ dval, const double2* __restrict__ x,
+                                              double2* __restrict__ y, const double2* __restrict__ aux,
+                                              double* __restrict__ partials, const ZState* __restrict__ st) {
   if (st->done) return;
+  constexpr int ND = EP == EP_OMEGA ? 3 : 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int s = xcd_block() * kWaves + wave;
-  double v[2] = {0.0, 0.0};
+  double v[ND];
+#pragma unroll
+  for (int d = 0; d < ND; d++) v[d] = 0.0;
   if (s < nslice) {
     const int row = rows[(size_t)s * 64 + lane];
     if (row >= 0) {
@@ -508,7 +583,7 @@ __global__ void __launch_bounds__(kNT) k_zspmv_pap(int nslice, const int* __rest
 #pragma unroll
         for (int u = 0; u < 4; u++) a[u] = zload_nt(val + (size_t)(k + u) * 64 + lane);
 #pragma unroll
-        for (int u = 0; u < 4; u++) b[u] = c[u] >= 0 ? p[c[u]] : zzero();
+        for (int u = 0; u < 4; u++) b[u] = c[u] >= 0 ? x[c[u]] : zzero();
 #pragma unroll
         for (int u = 0; u < 4; u++)
           if (c[u] >= 0) {
@@ -519,18 +594,29 @@ __global__ void __launch_bounds__(kNT) k_zspmv_pap(int nslice, const int* __rest
       for (; k < k1; k++) {
         const int c = __builtin_nontemporal_load(col + (size_t)k * 64 + lane);
         if (c < 0) break;
-        const double2 a = zload_nt(val + (size_t)k * 64 + lane), b = p[c];
+        const double2 a = zload_nt(val + (size_t)k * 64 + lane), b = x[c];
         re += a.x * b.x - a.y * b.y;
         im += a.x * b.y + a.y * b.x;
       }
-      const double2 pr = p[row];
-      const double2 y = zadd(make_double2(re, im), zmul(dval[(size_t)s * 64 + lane], pr));
-      Ap[row] = y;
-      v[0] = pr.x * y.x - pr.y * y.y;
-      v[1] = pr.x * y.y + pr.y * y.x;
+      const double2 xr = x[row];
+      const double2 yr = zadd(make_double2(re, im), zmul(dval[(size_t)s * 64 + lane], xr));
+      y[row] = yr;
+      if constexpr (EP == EP_PAP) {
+        v[0] = xr.x * yr.x - xr.y * yr.y;
+        v[1] = xr.x * yr.y + xr.y * yr.x;
+      } else if constexpr (EP == EP_CONJ) {
+        const double2 a = aux[row];
+        v[0] = yr.x * a.x + yr.y * a.y;
+        v[1] = yr.x * a.y - yr.y * a.x;
+      } else {
+        const double2 a = aux[row];
+        v[0] = yr.x * yr.x + yr.y * yr.y;
+        v[1] = a.x * yr.x + a.y * yr.y;
+        v[2] = a.x * yr.y - a.y * yr.x;
+      }
     }
   }
-  write_partials<2>(v, partials, blockIdx.x, gridDim.x);
+  write_partials<ND>(v, partials, blockIdx.x, gridDim.x);
 }
 // r += (-alpha) Ap, with the partial of ||r||^2
 __global__ void __launch_bounds__(kNT) k_zupd_r(long n, const double2* __restrict__ Ap, double2* __restrict__ r,
@@ -556,6 +642,56 @@ __global__ void __launch_bounds__(kNT) k_zupd_up(long n, const double2* __restri
     u[i] = zadd(u[i], zmul(al, pi));
     p[i] = zadd(zmul(pi, be), w[i]);
   }
+}
+
+// BiCGSTAB (zsolver_mat_iter.cpp:487-616, 622-739) vector steps
+// s = r + (-alpha) AMp
+__global__ void __launch_bounds__(kNT) k_zbicg_s(long n, const double2* __restrict__ r,
+                                                const double2* __restrict__ AMp, double2* __restrict__ sv,
+                                                const ZState* __restrict__ st) {
+  if (st->done) return;
+  const double2 ma = make_double2(-st->alpha.x, -st->alpha.y);
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT)
+    sv[i] = zadd(r[i], zmul(ma, AMp[i]));
+}
+// u += alpha Mp; u += omega Ms; r = s + (-omega) AMs; partials of ||r||^2
+// and InnerProduct(r, r0*)
+__global__ void __launch_bounds__(kNT) k_zbicg_ur(long n, const double2* __restrict__ Mp,
+                                                 const double2* __restrict__ Ms, const double2* __restrict__ sv,
+                                                 const double2* __restrict__ AMs, const double2* __restrict__ r0,
+                                                 double2* __restrict__ u, double2* __restrict__ r,
+                                                 double* __restrict__ partials, const ZState* __restrict__ st) {
+  if (st->done) return;
+  const double2 al = st->alpha, om = st->omega, mo = make_double2(-st->omega.x, -st->omega.y);
+  double v[3] = {0.0, 0.0, 0.0};
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
+    double2 ui = zadd(u[i], zmul(al, Mp[i]));
+    u[i] = zadd(ui, zmul(om, Ms[i]));
+    const double2 ri = zadd(sv[i], zmul(mo, AMs[i]));
+    r[i] = ri;
+    const double2 a = r0[i];
+    v[0] += ri.x * ri.x + ri.y * ri.y;
+    v[1] += ri.x * a.x + ri.y * a.y;
+    v[2] += ri.x * a.y - ri.y * a.x;
+  }
+  write_partials<3>(v, partials, blockIdx.x, gridDim.x);
+}
+// p = p beta; p += r; p += (-beta omega) AMp
+__global__ void __launch_bounds__(kNT) k_zbicg_p(long n, const double2* __restrict__ r,
+                                                const double2* __restrict__ AMp, double2* __restrict__ p,
+                                                const ZState* __restrict__ st) {
+  if (st->done) return;
+  const double2 be = st->beta;
+  const double2 c = zmul(make_double2(-be.x, -be.y), st->omega);
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) {
+    double2 t = zmul(p[i], be);
+    t = zadd(t, r[i]);
+    p[i] = zadd(t, zmul(c, AMp[i]));
+  }
+}
+// x = conj(x) (SetVectorConjugate, zvector_blk.cpp:149-152)
+__global__ void __launch_bounds__(kNT) k_zconj(long n, double2* __restrict__ x) {
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) x[i].y = -x[i].y;
 }
 
 // ---------------------------------------------------------------------------
@@ -596,17 +732,31 @@ int zsweep(dfm_zilu_s* Z, const double2* src, double2* x, const ZState* st, bool
   return DFM_OK;
 }
 
-int zread_state(dfm_zilu_s* Z) {
-  DFM_HIP(hipMemcpyAsync(Z->h_state, Z->state.p, sizeof(ZState), hipMemcpyDeviceToHost, stream()));
+int zread_state(ZCtx& c) {
+  DFM_HIP(hipMemcpyAsync(c.h, c.state.p, sizeof(ZState), hipMemcpyDeviceToHost, stream()));
   DFM_HIP(hipStreamSynchronize(stream()));
   return DFM_OK;
 }
 
+// Sizes the workspace for nvec vectors of n entries and nparts partials per
+// reduction (three reductions at most), and writes the initial state.
+int zctx_begin(ZCtx& c, long n, int nvec, int nparts, double tol, unsigned mx_iter) {
+  if (c.work.n < (size_t)(nvec * n)) DFM_TRY(c.work.alloc(std::max<long>(1, nvec * n)));
+  if (c.partials.n < (size_t)(3 * nparts)) DFM_TRY(c.partials.alloc(3 * nparts));
+  if (!c.state.n) DFM_TRY(c.state.alloc(1));
+  if (!c.h) DFM_HIP(hipHostMalloc((void**)&c.h, sizeof(ZState), hipHostMallocDefault));
+  std::memset(c.h, 0, sizeof(ZState));
+  c.h->tol = tol;
+  c.h->maxit = (int)std::min<unsigned>(mx_iter, 0x7fffffffu);
+  c.h->iter = 1;
+  DFM_HIP(hipMemcpyAsync(c.state.p, c.h, sizeof(ZState), hipMemcpyHostToDevice, stream()));
+  return DFM_OK;
+}
+
 // Runs `nblock` iterations of `body` as one hipGraph, instantiated once per
-// operand set and cached on the factor (directly where the stream cannot be
-// captured).
+// operand set and cached (directly where the stream cannot be captured).
 template <class Body>
-int zrun_block(dfm_zilu_s* Z, const std::string& key, int nblock, Body&& body) {
+int zrun_block(ZCtx& c, const std::string& key, int nblock, Body&& body) {
   hipStream_t s = stream();
   static const bool use_graphs = [] {
     const char* e = getenv("DFM_GRAPHS");
@@ -616,7 +766,7 @@ int zrun_block(dfm_zilu_s* Z, const std::string& key, int nblock, Body&& body) {
     for (int i = 0; i < nblock; i++) DFM_TRY(body());
     return DFM_OK;
   }
-  for (auto& g : Z->graphs)
+  for (auto& g : c.graphs)
     if (g.first == key) {
       DFM_HIP(hipGraphLaunch(g.second, s));
       return DFM_OK;
@@ -636,8 +786,201 @@ int zrun_block(dfm_zilu_s* Z, const std::string& key, int nblock, Body&& body) {
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   DFM_HIP(e);
-  Z->graphs.emplace_back(key, exec);
+  c.graphs.emplace_back(key, exec);
   DFM_HIP(hipGraphLaunch(exec, s));
+  return DFM_OK;
+}
+
+// Replays 16-iteration blocks until the state says stop.
+template <class Body>
+int zloop(ZCtx& c, const char* tag, const void* k1, const void* k2, Body&& body) {
+  const int check_every = 16;
+  char key[192];
+  snprintf(key, sizeof key, "%s:%p:%p:%p:%p:%d", tag, (void*)c.work.p, (void*)c.partials.p, k1, k2, check_every);
+  for (;;) {
+    DFM_TRY(zrun_block(c, key, check_every, body));
+    DFM_TRY(zread_state(c));
+    if (c.h->done) return DFM_OK;
+  }
+}
+
+// The natural-order sliced-ELL copy of A (one group, rows in order, entries
+// in CRS order), for the unpreconditioned solvers.
+int znatural_op(dfm_zbcrs_s* A, ZOp& op) {
+  const dfm_bcrs_s* P = A->pat;
+  const unsigned nb = P->nb;
+  if (A->sell_hash != P->pattern_hash || !A->s_rows.n) {
+    std::vector<int> rows(nb), gptr = {0, (int)nb};
+    std::vector<unsigned> beg(nb), end(nb);
+    for (unsigned i = 0; i < nb; i++) {
+      rows[i] = (int)i;
+      beg[i] = P->rowptr[i];
+      end[i] = P->rowptr[i + 1];
+    }
+    SellPlan plan;
+    build_sell_plan(rows, gptr, beg, end, P->col, nullptr, plan);
+    DFM_TRY(A->s_sptr.upload(plan.slice_ptr));
+    DFM_TRY(A->s_rows.upload(plan.rows));
+    DFM_TRY(A->s_col.upload(plan.col));
+    DFM_TRY(A->s_src.upload(plan.src));
+    DFM_TRY(A->s_val.alloc(std::max<size_t>(1, (size_t)plan.nk * kSlice)));
+    DFM_TRY(A->s_dval.alloc(std::max<size_t>(1, (size_t)plan.nslice * kSlice)));
+    A->sell_nslice = plan.nslice;
+    A->sell_hash = P->pattern_hash;
+    A->sell_version = 0;
+  }
+  if (A->sell_version != A->version) {
+    const long nslot = (long)A->s_src.n, nlane = (long)A->sell_nslice * kSlice;
+    if (nslot) DFM_ZLAUNCH(k_zsell_gather, zgrid_for(nslot), kNT, nslot, A->s_src.p, A->d_val.p, A->s_val.p);
+    if (nlane)
+      DFM_ZLAUNCH(k_zsell_dia, zgrid_for(nlane), kNT, nlane, A->s_rows.p, (const int*)nullptr, A->d_dia.p, A->s_dval.p);
+    A->sell_version = A->version;
+  }
+  op = ZOp{A->sell_nslice, A->s_rows.p, A->s_sptr.p, A->s_col.p, A->s_val.p, A->s_dval.p};
+  return DFM_OK;
+}
+
+ZOp zfactor_op(const dfm_zilu_s* Z) {
+  const dfm_ilu_s* M = Z->M;
+  return ZOp{M->A64.nslice, M->A64.rows.p, M->A64.slice_ptr.p, M->A64.col.p, Z->zA.p, Z->zAd.p};
+}
+
+template <int EP>
+int zspmv(const ZOp& A, const double2* x, double2* y, const double2* aux, double* parts, const ZState* st) {
+  const int g = std::max(1, (A.nslice + kWaves - 1) / kWaves);
+  DFM_ZLAUNCH(k_zspmv<EP>, g, kNT, A.nslice, A.rows, A.sptr, A.col, A.val, A.dval, x, y, aux, parts, st);
+  return DFM_OK;
+}
+inline int zspmv_grid(const ZOp& A) { return std::max(1, (A.nslice + kWaves - 1) / kWaves); }
+
+// MatVec::Sol::Solve_PCOCG (zsolver_mat_iter.cpp:303-392) with Z's factor, or
+// Solve_COCG (:220-299) when Z is null (w = r).  Vectors in the operator's
+// numbering: perm / inv map it to the caller's (null: identity).
+int zcocg_core(ZCtx& c, const ZOp& A, dfm_zilu_s* Z, long n, const int* perm, const int* inv, double2* r_dev,
+               double2* u_dev, double* conv_ratio, unsigned* iteration) {
+  const double tol = *conv_ratio;
+  const unsigned mx_iter = *iteration;
+  const int gv = zgrid_vec(n), gA = zspmv_grid(A), nsU = Z ? Z->M->U64.nslice : 0;
+  DFM_TRY(zctx_begin(c, n, Z ? 5 : 4, std::max(std::max(gv, gA), nsU), tol, mx_iter));
+  double2 *r = c.work.p, *u = r + n, *p = u + n, *Ap = p + n, *w = Z ? Ap + n : r;
+  double* parts = c.partials.p;
+  ZState* st = c.state.p;
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, perm, (const double2*)r_dev, r);
+  DFM_ZLAUNCH(k_zfill, gv, kNT, n, u);
+  DFM_ZLAUNCH(k_znorm2, gv, kNT, n, (const double2*)r, parts);
+  DFM_ZLAUNCH((k_zfinalize<1, ZFIN_RR0>), 1, kFinNT, parts, gv, st);
+  DFM_TRY(zread_state(c));
+  if (c.h->done == 3) {  // ||r||^2 < 1e-30: conv_ratio 0, iteration 0, u = 0
+    DFM_ZLAUNCH(k_zfill, gv, kNT, n, u_dev);
+    DFM_HIP(hipStreamSynchronize(stream()));
+    *conv_ratio = 0.0;
+    *iteration = 0;
+    return DFM_OK;
+  }
+  // w = M^-1 r (with the partials of (r, w)), p = w
+  if (Z) {
+    DFM_TRY(zsweep(Z, r, w, nullptr, true, true, r, parts));
+    DFM_ZLAUNCH((k_zfinalize<2, ZFIN_RW0>), 1, kFinNT, parts, nsU, st);
+  } else {
+    DFM_ZLAUNCH(k_zdot, gv, kNT, n, (const double2*)r, (const double2*)r, parts, (const ZState*)nullptr);
+    DFM_ZLAUNCH((k_zfinalize<2, ZFIN_RW0>), 1, kFinNT, parts, gv, st);
+  }
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, (const int*)nullptr, (const double2*)w, p);
+  if (mx_iter > 1) {
+    auto body = [&]() -> int {
+      DFM_TRY(zspmv<EP_PAP>(A, p, Ap, nullptr, parts, st));
+      DFM_ZLAUNCH((k_zfinalize<2, ZFIN_ALPHA>), 1, kFinNT, parts, gA, st);
+      DFM_ZLAUNCH(k_zupd_r, gv, kNT, n, (const double2*)Ap, r, parts, (const ZState*)st);
+      DFM_ZLAUNCH((k_zfinalize<1, ZFIN_RATIO>), 1, kFinNT, parts, gv, st);
+      if (Z) {
+        DFM_TRY(zsweep(Z, r, w, st, true, true, r, parts));
+        DFM_ZLAUNCH((k_zfinalize<2, ZFIN_BETA>), 1, kFinNT, parts, nsU, st);
+      } else {
+        DFM_ZLAUNCH(k_zdot, gv, kNT, n, (const double2*)r, (const double2*)r, parts, (const ZState*)st);
+        DFM_ZLAUNCH((k_zfinalize<2, ZFIN_BETA>), 1, kFinNT, parts, gv, st);
+      }
+      DFM_ZLAUNCH(k_zupd_up, gv, kNT, n, (const double2*)w, p, u, (const ZState*)st);
+      return DFM_OK;
+    };
+    DFM_TRY(zloop(c, Z ? "pcocg" : "cocg", A.val, A.rows, body));
+  }
+  DFM_TRY(zread_state(c));
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, inv, (const double2*)r, r_dev);
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, inv, (const double2*)u, u_dev);
+  DFM_HIP(hipStreamSynchronize(stream()));
+  if (c.h->done == 1) {
+    *conv_ratio = c.h->ratio;
+    *iteration = (unsigned)c.h->conv_iter;
+  } else {
+    *iteration = mx_iter;
+  }
+  return DFM_OK;
+}
+
+// MatVec::Sol::Solve_BiCGStabP (zsolver_mat_iter.cpp:622-739) with Z's factor,
+// or Solve_BiCGSTAB (:487-616) when Z is null.  The preconditioned form
+// starts from r = conj(b) with r0* = b and stores conv_ratio at every
+// iteration (at max_iter it holds the last ratio); both return true.
+int zbicg_core(ZCtx& c, const ZOp& A, dfm_zilu_s* Z, long n, const int* perm, const int* inv, double2* r_dev,
+               double2* u_dev, double* conv_ratio, unsigned* iteration) {
+  const double tol = *conv_ratio;
+  const unsigned mx_iter = *iteration;
+  const int gv = zgrid_vec(n), gA = zspmv_grid(A);
+  DFM_TRY(zctx_begin(c, n, Z ? 9 : 7, std::max(gv, gA), tol, mx_iter));
+  double2 *r = c.work.p, *u = r + n, *p = u + n, *sv = p + n, *r0 = sv + n, *AMp = r0 + n, *AMs = AMp + n;
+  double2 *Mp = Z ? AMs + n : p, *Ms = Z ? Mp + n : sv;
+  double* parts = c.partials.p;
+  ZState* st = c.state.p;
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, perm, (const double2*)r_dev, r);
+  DFM_ZLAUNCH(k_zfill, gv, kNT, n, u);
+  DFM_ZLAUNCH(k_znorm2, gv, kNT, n, (const double2*)r, parts);
+  DFM_ZLAUNCH((k_zfinalize<1, ZFIN_RR0>), 1, kFinNT, parts, gv, st);
+  DFM_TRY(zread_state(c));
+  if (c.h->done == 3) {
+    DFM_ZLAUNCH(k_zfill, gv, kNT, n, u_dev);
+    DFM_HIP(hipStreamSynchronize(stream()));
+    *conv_ratio = 0.0;
+    *iteration = 0;
+    return DFM_OK;
+  }
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, (const int*)nullptr, (const double2*)r, r0);
+  if (Z) DFM_ZLAUNCH(k_zconj, gv, kNT, n, r);
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, (const int*)nullptr, (const double2*)r, p);
+  // InnerProduct(r, r0*) = (r0*, r) with a conjugate on r: k_zspmv's EP_CONJ form
+  // over the vectors, here as a plain reduction
+  DFM_ZLAUNCH(k_zdotc, gv, kNT, n, (const double2*)r, (const double2*)r0, parts);
+  DFM_ZLAUNCH((k_zfinalize<2, ZFIN_RR0C>), 1, kFinNT, parts, gv, st);
+  if (mx_iter > 1) {
+    auto body = [&]() -> int {
+      if (Z) DFM_TRY(zsweep(Z, p, Mp, st, true, true));
+      DFM_TRY(zspmv<EP_CONJ>(A, Mp, AMp, r0, parts, st));
+      DFM_ZLAUNCH((k_zfinalize<2, ZFIN_BALPHA>), 1, kFinNT, parts, gA, st);
+      DFM_ZLAUNCH(k_zbicg_s, gv, kNT, n, (const double2*)r, (const double2*)AMp, sv, (const ZState*)st);
+      if (Z) DFM_TRY(zsweep(Z, sv, Ms, st, true, true));
+      DFM_TRY(zspmv<EP_OMEGA>(A, Ms, AMs, sv, parts, st));
+      DFM_ZLAUNCH((k_zfinalize<3, ZFIN_BOMEGA>), 1, kFinNT, parts, gA, st);
+      DFM_ZLAUNCH(k_zbicg_ur, gv, kNT, n, (const double2*)Mp, (const double2*)Ms, (const double2*)sv,
+                  (const double2*)AMs, (const double2*)r0, u, r, parts, (const ZState*)st);
+      if (Z)
+        DFM_ZLAUNCH((k_zfinalize<3, ZFIN_BCONVP>), 1, kFinNT, parts, gv, st);
+      else
+        DFM_ZLAUNCH((k_zfinalize<3, ZFIN_BCONV>), 1, kFinNT, parts, gv, st);
+      DFM_ZLAUNCH(k_zbicg_p, gv, kNT, n, (const double2*)r, (const double2*)AMp, p, (const ZState*)st);
+      return DFM_OK;
+    };
+    DFM_TRY(zloop(c, Z ? "bicgstabp" : "bicgstab", A.val, A.rows, body));
+  }
+  DFM_TRY(zread_state(c));
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, inv, (const double2*)r, r_dev);
+  DFM_ZLAUNCH(k_zgather, gv, kNT, n, inv, (const double2*)u, u_dev);
+  DFM_HIP(hipStreamSynchronize(stream()));
+  if (c.h->done == 1) {
+    *conv_ratio = c.h->ratio;
+    *iteration = (unsigned)c.h->conv_iter;
+  } else {
+    if (Z && mx_iter > 1 && c.h->iter > 1) *conv_ratio = c.h->ratio;
+    *iteration = mx_iter;
+  }
   return DFM_OK;
 }
 
@@ -868,8 +1211,6 @@ int dfm_zilu_create(const dfm_zbcrs* A, int lev_fill, int ordering, const int* o
   DFM_TRY(Z->zU.alloc(std::max<size_t>(1, (size_t)M->U64.nk * kSlice)));
   DFM_TRY(Z->zA.alloc(std::max<size_t>(1, (size_t)M->A64.nk * kSlice)));
   DFM_TRY(Z->zAd.alloc(std::max<size_t>(1, (size_t)M->A64.nslice * kSlice)));
-  DFM_TRY(Z->state.alloc(1));
-  DFM_HIP(hipHostMalloc((void**)&Z->h_state, sizeof(ZState), hipHostMallocDefault));
   *out = Z.release();
   return DFM_OK;
 }
@@ -952,85 +1293,46 @@ int dfm_zilu_solve_precond(dfm_zilu* Z, double* vec) {
 }
 
 // ---------------------------------------------------------------------------
-// MatVec::Sol::Solve_PCOCG (zsolver_mat_iter.cpp:303-392).  r: right-hand side
-// in, final residual out; u: solution out.  In factor numbering with the
-// permuted copy of A.  Returns DFM_OK as the reference returns true, also when
-// max_iter stops it (iteration = max_iter, conv_ratio unchanged).
-int dfm_zsolve_pcocg(dfm_zbcrs* A, dfm_zilu* Z, double* r_dev, double* u_dev, double* conv_ratio,
-                     unsigned* iteration) {
-  DFM_CHECK(A && Z && r_dev && u_dev && conv_ratio && iteration && r_dev != u_dev, DFM_BAD_ARG,
-            "bad arguments to dfm_zsolve_pcocg");
-  DFM_CHECK(Z->factored, DFM_BAD_ARG, "factor has no values (call dfm_zilu_set_value)");
-  dfm_ilu_s* M = Z->M;
-  DFM_CHECK(A->pat->nb == M->nb && A->pat->pattern_hash == M->pattern_hash, DFM_BAD_ARG,
-            "matrix pattern differs from the one the factor was built on");
-  const long n = M->nb;
-  const double tol = *conv_ratio;
-  const unsigned mx_iter = *iteration;
-  DFM_TRY(zrefresh_a(Z, A));
-  if (Z->work.n < (size_t)(5 * n)) DFM_TRY(Z->work.alloc(std::max<long>(1, 5 * n)));
-  double2 *r = Z->work.p, *u = r + n, *p = u + n, *Ap = p + n, *w = Ap + n;  // factor numbering
-  const int gv = zgrid_vec(n), nsA = M->A64.nslice, gA = std::max(1, (nsA + kWaves - 1) / kWaves);
-  const int nsU = M->U64.nslice;
-  const int nparts_max = std::max(std::max(gv, gA), nsU);
-  if (Z->partials.n < (size_t)(2 * nparts_max)) DFM_TRY(Z->partials.alloc(2 * nparts_max));
-  double* parts = Z->partials.p;
-  ZState* st = Z->state.p;
-  ZState& h = *Z->h_state;
-  std::memset(&h, 0, sizeof(ZState));
-  h.tol = tol;
-  h.maxit = (int)std::min<unsigned>(mx_iter, 0x7fffffffu);
-  h.iter = 1;
-  DFM_HIP(hipMemcpyAsync(st, &h, sizeof(ZState), hipMemcpyHostToDevice, stream()));
-  const int* perm = M->permuted ? M->d_order.p : nullptr;
-  const int* inv = M->permuted ? M->d_inv.p : nullptr;
-  DFM_ZLAUNCH(k_zgather, gv, kNT, n, perm, (const double2*)r_dev, r);
-  DFM_ZLAUNCH(k_zfill, gv, kNT, n, u);
-  DFM_ZLAUNCH(k_znorm2, gv, kNT, n, (const double2*)r, parts);
-  DFM_ZLAUNCH((k_zfinalize<1, ZFIN_RR0>), 1, kFinNT, parts, gv, st);
-  DFM_TRY(zread_state(Z));
-  if (h.done == 3) {  // ||r||^2 < 1e-30: conv_ratio 0, iteration 0, u = 0
-    DFM_ZLAUNCH(k_zfill, gv, kNT, n, (double2*)u_dev);
-    DFM_HIP(hipStreamSynchronize(stream()));
-    *conv_ratio = 0.0;
-    *iteration = 0;
-    return DFM_OK;
-  }
-  // w = M^-1 r (with the partials of (r, w)), p = w
-  DFM_TRY(zsweep(Z, r, w, nullptr, true, true, r, parts));
-  DFM_ZLAUNCH((k_zfinalize<2, ZFIN_RW0>), 1, kFinNT, parts, nsU, st);
-  DFM_ZLAUNCH(k_zgather, gv, kNT, n, (const int*)nullptr, (const double2*)w, p);
-  if (mx_iter > 1) {
-    auto body = [&]() -> int {
-      DFM_ZLAUNCH(k_zspmv_pap, gA, kNT, nsA, M->A64.rows.p, M->A64.slice_ptr.p, M->A64.col.p,
-                  (const double2*)Z->zA.p, (const double2*)Z->zAd.p, (const double2*)p, Ap, parts,
-                  (const ZState*)st);
-      DFM_ZLAUNCH((k_zfinalize<2, ZFIN_ALPHA>), 1, kFinNT, parts, gA, st);
-      DFM_ZLAUNCH(k_zupd_r, gv, kNT, n, (const double2*)Ap, r, parts, (const ZState*)st);
-      DFM_ZLAUNCH((k_zfinalize<1, ZFIN_RATIO>), 1, kFinNT, parts, gv, st);
-      DFM_TRY(zsweep(Z, r, w, st, true, true, r, parts));
-      DFM_ZLAUNCH((k_zfinalize<2, ZFIN_BETA>), 1, kFinNT, parts, nsU, st);
-      DFM_ZLAUNCH(k_zupd_up, gv, kNT, n, (const double2*)w, p, u, (const ZState*)st);
-      return DFM_OK;
-    };
-    const int check_every = 16;
-    char key[160];
-    snprintf(key, sizeof key, "pcocg:%p:%p:%p:%d", (void*)Z->work.p, (void*)parts, (void*)Z->zA.p, check_every);
-    for (;;) {
-      DFM_TRY(zrun_block(Z, key, check_every, body));
-      DFM_TRY(zread_state(Z));
-      if (h.done) break;
-    }
-  }
-  DFM_TRY(zread_state(Z));
-  DFM_ZLAUNCH(k_zgather, gv, kNT, n, inv, (const double2*)r, (double2*)r_dev);
-  DFM_ZLAUNCH(k_zgather, gv, kNT, n, inv, (const double2*)u, (double2*)u_dev);
-  DFM_HIP(hipStreamSynchronize(stream()));
-  if (h.done == 1) {
-    *conv_ratio = h.ratio;
-    *iteration = (unsigned)h.conv_iter;
-  } else {
-    *iteration = mx_iter;
+// MatVec::Sol complex solvers.  r: right-hand side in, final residual out;
+// u: solution out.  The preconditioned ones run in the factor's numbering
+// with its permuted copy of A.  All return DFM_OK, as the reference returns
+// true also when max_iter stops them.
+static int zsolve_checks(dfm_zbcrs* A, dfm_zilu* Z, double* r, double* u, double* conv, unsigned* iter) {
+  DFM_CHECK(A && r && u && conv && iter && r != u, DFM_BAD_ARG, "bad arguments to a complex solver");
+  if (Z) {
+    DFM_CHECK(Z->factored, DFM_BAD_ARG, "factor has no values (call dfm_zilu_set_value)");
+    DFM_CHECK(A->pat->nb == Z->M->nb && A->pat->pattern_hash == Z->M->pattern_hash, DFM_BAD_ARG,
+              "matrix pattern differs from the one the factor was built on");
   }
   return DFM_OK;
+}
+int dfm_zsolve_pcocg(dfm_zbcrs* A, dfm_zilu* Z, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  DFM_CHECK(Z, DFM_BAD_ARG, "dfm_zsolve_pcocg needs a factor");
+  DFM_TRY(zsolve_checks(A, Z, r, u, conv_ratio, iteration));
+  DFM_TRY(zrefresh_a(Z, A));
+  const dfm_ilu_s* M = Z->M;
+  return zcocg_core(Z->ctx, zfactor_op(Z), Z, M->nb, M->permuted ? M->d_order.p : nullptr,
+                    M->permuted ? M->d_inv.p : nullptr, (double2*)r, (double2*)u, conv_ratio, iteration);
+}
+int dfm_zsolve_cocg(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  DFM_TRY(zsolve_checks(A, nullptr, r, u, conv_ratio, iteration));
+  ZOp op;
+  DFM_TRY(znatural_op(A, op));
+  return zcocg_core(A->ctx, op, nullptr, A->pat->nb, nullptr, nullptr, (double2*)r, (double2*)u, conv_ratio,
+                    iteration);
+}
+int dfm_zsolve_bicgstab(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  DFM_TRY(zsolve_checks(A, nullptr, r, u, conv_ratio, iteration));
+  ZOp op;
+  DFM_TRY(znatural_op(A, op));
+  return zbicg_core(A->ctx, op, nullptr, A->pat->nb, nullptr, nullptr, (double2*)r, (double2*)u, conv_ratio,
+                    iteration);
+}
+int dfm_zsolve_bicgstabp(dfm_zbcrs* A, dfm_zilu* Z, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  DFM_CHECK(Z, DFM_BAD_ARG, "dfm_zsolve_bicgstabp needs a factor");
+  DFM_TRY(zsolve_checks(A, Z, r, u, conv_ratio, iteration));
+  DFM_TRY(zrefresh_a(Z, A));
+  const dfm_ilu_s* M = Z->M;
+  return zbicg_core(Z->ctx, zfactor_op(Z), Z, M->nb, M->permuted ? M->d_order.p : nullptr,
+                    M->permuted ? M->d_inv.p : nullptr, (double2*)r, (double2*)u, conv_ratio, iteration);
 }

@@ -642,6 +642,31 @@ def Solve_PCOCG(conv_ratio, iteration, A, prec, r, u):
     return True, conv.value, it.value
 
 
+def _zsolve(fn, conv_ratio, iteration, A, prec, r, u):
+    _sync_torch()
+    conv = C.c_double(conv_ratio)
+    it = C.c_uint(iteration)
+    args = (A.h,) + ((prec.h,) if prec is not None else ()) + (_dptr(r), _dptr(u), C.byref(conv), C.byref(it))
+    call(fn, *args)
+    return True, conv.value, it.value
+
+
+def Solve_COCG(conv_ratio, iteration, A, r, u):
+    """MatVec::Sol::Solve_COCG (src/matvec/zsolver_mat_iter.cpp:220-299)."""
+    return _zsolve("dfm_zsolve_cocg", conv_ratio, iteration, A, None, r, u)
+
+
+def Solve_BiCGSTAB(conv_ratio, iteration, A, r, u):
+    """MatVec::Sol::Solve_BiCGSTAB for the complex matrix (zsolver_mat_iter.cpp:487-616),
+    with the reference's conjugated inner products (see include/dfm.h)."""
+    return _zsolve("dfm_zsolve_bicgstab", conv_ratio, iteration, A, None, r, u)
+
+
+def Solve_BiCGStabP(conv_ratio, iteration, A, prec, r, u):
+    """MatVec::Sol::Solve_BiCGStabP (zsolver_mat_iter.cpp:622-739) with the complex ILU factor."""
+    return _zsolve("dfm_zsolve_bicgstabp", conv_ratio, iteration, A, prec, r, u)
+
+
 def zdot(x, y):
     """CZVector_Blk operator* (zvector_blk.cpp:39-60): sum x_i y_i, no conjugate."""
     _sync_torch()

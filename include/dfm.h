@@ -302,6 +302,16 @@ int dfm_zilu_solve_precond(dfm_zilu *M, double *vec_dev);
  * the reference returns true. */
 int dfm_zsolve_pcocg(dfm_zbcrs *A, dfm_zilu *M, double *r_dev, double *u_dev,
                      double *conv_ratio, unsigned *iteration);
+/* MatVec::Sol::Solve_COCG zsolver_mat_iter.cpp:220-299 (unpreconditioned PCOCG). */
+int dfm_zsolve_cocg(dfm_zbcrs *A, double *r_dev, double *u_dev, double *conv_ratio, unsigned *iteration);
+/* MatVec::Sol::Solve_BiCGSTAB zsolver_mat_iter.cpp:487-616 and Solve_BiCGStabP :622-739 (with
+ * the factor).  As the reference: the inner products are InnerProduct (sum conj(a) b,
+ * zvector_blk.cpp:62-80), so alpha and omega are the conjugates of the textbook ones and the
+ * pair converges only on real-valued data; BiCGStabP starts from r = conj(b) with shadow
+ * residual b and leaves the last ratio in conv_ratio also at max_iter.  Return DFM_OK. */
+int dfm_zsolve_bicgstab(dfm_zbcrs *A, double *r_dev, double *u_dev, double *conv_ratio, unsigned *iteration);
+int dfm_zsolve_bicgstabp(dfm_zbcrs *A, dfm_zilu *M, double *r_dev, double *u_dev,
+                         double *conv_ratio, unsigned *iteration);
 
 /* ---- measurement (no reference counterpart) -------------------------------
  * Times one device kernel of a solve on its real operands: which = 0 the

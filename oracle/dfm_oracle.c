@@ -1259,6 +1259,133 @@ int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, c
     return 1;
 }
 
+/* InnerProduct (zvector_blk.cpp:62-80): sum conj(a_i) b_i */
+static zc zconjdot(size_t n, const zc *a, const zc *b)
+{
+    double re = 0.0, im = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        re += a[i].re * b[i].re + a[i].im * b[i].im;
+        im += a[i].re * b[i].im - a[i].im * b[i].re;
+    }
+    zc r = {re, im};
+    return r;
+}
+static zc zneg(zc a) { zc r = {-a.re, -a.im}; return r; }
+static zc zdivd(zc a, double d) { zc r = {a.re / d, a.im / d}; return r; }   /* Complex / double */
+/* MatVec::Sol::Solve_COCG (zsolver_mat_iter.cpp:220-299): r in rhs / out
+ * final residual, u out.  Always returns 1; conv_ratio unchanged at max_iter. */
+int orc_zsolve_cocg(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                    double *r, double *u, double *conv_ratio, unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = nb;
+    zc *rv = (zc *)r, *uv = (zc *)u;
+    memset(u, 0, sizeof(double) * 2 * n);
+    const double rr0 = znorm2(n, rv);
+    if (rr0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; return 1; }
+    const double inv_rr0 = 1.0 / rr0;
+    zc *p = (zc *)malloc(sizeof(zc) * n), *Ap = (zc *)calloc(n, sizeof(zc));
+    memcpy(p, rv, sizeof(zc) * n);
+    zc rr = zdot(n, rv, rv);
+    *iteration = mx_iter;
+    for (unsigned it = 1; it < mx_iter; it++) {
+        orc_zmatvec(nb, rowptr, col, val, dia, 1.0, (const double *)p, 0.0, (double *)Ap);
+        const zc alpha = zdiv(rr, zdot(n, Ap, p));
+        const zc malpha = zneg(alpha);
+        for (size_t i = 0; i < n; i++) rv[i] = zadd(rv[i], zmul(malpha, Ap[i]));
+        const double ratio = sqrt(znorm2(n, rv) * inv_rr0);
+        if (ratio < tol) {
+            *conv_ratio = ratio;
+            *iteration = it;
+            break;
+        }
+        for (size_t i = 0; i < n; i++) uv[i] = zadd(uv[i], zmul(alpha, p[i]));
+        const zc rr1 = zdot(n, rv, rv);
+        const zc beta = zdiv(rr1, rr);
+        rr = rr1;
+        for (size_t i = 0; i < n; i++) p[i] = zadd(zmul(p[i], beta), rv[i]);
+    }
+    free(p); free(Ap);
+    return 1;
+}
+/* MatVec::Sol::Solve_BiCGSTAB (zsolver_mat_iter.cpp:487-616, unpreconditioned)
+ * and Solve_BiCGStabP (:622-739, with the ILU factor; order as in
+ * orc_zsolve_pcocg).  precond = 0 selects the first.  The preconditioned
+ * form starts from r = conj(b) with r0* = b, and stores conv_ratio at every
+ * iteration (so at max_iter it holds the last ratio); both return 1. */
+int orc_zsolve_bicgstab(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val,
+                        const double *dia, int precond, const unsigned *frowptr, const unsigned *fcol,
+                        const unsigned *fdiaind, const double *fval, const double *fdia, const int *order,
+                        double *r, double *u, double *conv_ratio, unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = nb;
+    zc *rv = (zc *)r, *uv = (zc *)u;
+    memset(u, 0, sizeof(double) * 2 * n);
+    const double sq0 = znorm2(n, rv);
+    if (sq0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; return 1; }
+    const double inv = 1.0 / sq0;
+    zc *r0 = (zc *)malloc(sizeof(zc) * n), *p = (zc *)malloc(sizeof(zc) * n), *s = (zc *)malloc(sizeof(zc) * n);
+    zc *Mp = (zc *)malloc(sizeof(zc) * n), *Ms = (zc *)malloc(sizeof(zc) * n);
+    zc *AMp = (zc *)calloc(n, sizeof(zc)), *AMs = (zc *)calloc(n, sizeof(zc)), *tmp = (zc *)malloc(sizeof(zc) * n);
+    int *inv_order = NULL;
+    if (order) {
+        inv_order = (int *)malloc(sizeof(int) * n);
+        for (unsigned i = 0; i < nb; i++) inv_order[order[i]] = (int)i;
+    }
+    memcpy(r0, rv, sizeof(zc) * n);
+    if (precond)
+        for (size_t i = 0; i < n; i++) rv[i].im = -rv[i].im;   /* SetVectorConjugate */
+    memcpy(p, rv, sizeof(zc) * n);
+    zc r_r0 = zconjdot(n, rv, r0);
+    *iteration = mx_iter;
+    for (unsigned it = 1; it < mx_iter; it++) {
+        const zc *mp = p, *ms = s;
+        if (precond) {
+            memcpy(Mp, p, sizeof(zc) * n);
+            zprecond(nb, frowptr, fcol, fdiaind, fval, fdia, order, inv_order, tmp, Mp);
+            mp = Mp;
+        }
+        orc_zmatvec(nb, rowptr, col, val, dia, 1.0, (const double *)mp, 0.0, (double *)AMp);
+        const zc alpha = zdiv(r_r0, zconjdot(n, AMp, r0));
+        const zc malpha = zneg(alpha);
+        for (size_t i = 0; i < n; i++) s[i] = zadd(rv[i], zmul(malpha, AMp[i]));
+        if (precond) {
+            memcpy(Ms, s, sizeof(zc) * n);
+            zprecond(nb, frowptr, fcol, fdiaind, fval, fdia, order, inv_order, tmp, Ms);
+            ms = Ms;
+        }
+        orc_zmatvec(nb, rowptr, col, val, dia, 1.0, (const double *)ms, 0.0, (double *)AMs);
+        const double den = znorm2(n, AMs);
+        const zc omega = zdivd(zconjdot(n, s, AMs), den);
+        for (size_t i = 0; i < n; i++) uv[i] = zadd(uv[i], zmul(alpha, mp[i]));
+        for (size_t i = 0; i < n; i++) uv[i] = zadd(uv[i], zmul(omega, ms[i]));
+        const zc momega = zneg(omega);
+        for (size_t i = 0; i < n; i++) rv[i] = zadd(s[i], zmul(momega, AMs[i]));
+        const double sq = znorm2(n, rv);
+        const double sq_ratio = sq * inv;
+        if (precond) *conv_ratio = sqrt(sq * inv);
+        if (sq_ratio < tol * tol) {
+            if (!precond) *conv_ratio = sqrt(sq * inv);
+            *iteration = it;
+            break;
+        }
+        const zc t1 = zconjdot(n, rv, r0);
+        const zc beta = precond ? zmul(zdiv(alpha, omega), zdiv(t1, r_r0)) : zmul(zdiv(t1, r_r0), zdiv(alpha, omega));
+        r_r0 = t1;
+        const zc c = zmul(zneg(beta), omega);
+        for (size_t i = 0; i < n; i++) {
+            zc t = zmul(p[i], beta);
+            t = zadd(t, rv[i]);
+            p[i] = zadd(t, zmul(c, AMp[i]));
+        }
+    }
+    free(r0); free(p); free(s); free(Mp); free(Ms); free(AMp); free(AMs); free(tmp); free(inv_order);
+    return 1;
+}
+
 /* ------------------------------------------------------------------------- */
 /* a4: element kernels.
  * src/femeqn/ker_emat_tri.cpp:12-14 (TriArea), 36-69 (TriDlDx) */

@@ -150,6 +150,12 @@ int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, c
                      const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const double *fval,
                      const double *fdia, const int *order, double *r, double *u, double *conv_ratio,
                      unsigned *iteration);
+int orc_zsolve_cocg(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                    double *r, double *u, double *conv_ratio, unsigned *iteration);
+int orc_zsolve_bicgstab(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val,
+                        const double *dia, int precond, const unsigned *frowptr, const unsigned *fcol,
+                        const unsigned *fdiaind, const double *fval, const double *fdia, const int *order,
+                        double *r, double *u, double *conv_ratio, unsigned *iteration);
 void orc_zemat_helmholtz2d_p1(unsigned nelem, const double *coord, const unsigned *conn, double wave_length,
                               const double *value, double *emat, double *eres);
 void orc_zemat_sommerfeld_b1(unsigned nline, const double *coord, const unsigned *conn, double wave_length,

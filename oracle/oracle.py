@@ -90,6 +90,11 @@ class Orc:
         L.orc_zsolve_pcocg.restype = C.c_int
         L.orc_zsolve_pcocg.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                        C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_zsolve_cocg.restype = C.c_int
+        L.orc_zsolve_cocg.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_zsolve_bicgstab.restype = C.c_int
+        L.orc_zsolve_bicgstab.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zemat_helmholtz2d_p1.argtypes = [C.c_uint, vp, vp, C.c_double, vp, vp, vp]
         L.orc_zemat_sommerfeld_b1.argtypes = [C.c_uint, vp, vp, C.c_double, vp, vp, vp]
         L.orc_emat_poisson2d_p1.argtypes = [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]
@@ -277,6 +282,31 @@ class Orc:
                                      _p(c128(fdia)), _p(order), _p(r), _p(u), C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value, u, r
 
+    def zsolve_cocg(self, nb, rowptr, col, val, dia, rhs, tol, maxit):
+        """MatVec::Sol::Solve_COCG: (ok, conv, iter, u, r_final)."""
+        r = c128(rhs).copy()
+        u = np.zeros(nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.L.orc_zsolve_cocg(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)), _p(r), _p(u),
+                                    C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
+
+    def zsolve_bicgstab(self, nb, rowptr, col, val, dia, rhs, tol, maxit, F=None, fval=None, fdia=None,
+                        order=None):
+        """MatVec::Sol::Solve_BiCGSTAB (F None) or Solve_BiCGStabP with the
+        factor: (ok, conv, iter, u, r_final)."""
+        r = c128(rhs).copy()
+        u = np.zeros(nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        order = None if order is None else i32(order)
+        fa = (None,) * 5 if F is None else (u32(F["rowptr"]), u32(F["col"]), u32(F["diaind"]), c128(fval), c128(fdia))
+        ok = self.L.orc_zsolve_bicgstab(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)),
+                                        0 if F is None else 1, *[_p(a) for a in fa], _p(order), _p(r), _p(u),
+                                        C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
+
     MAT_KINDS = {"cg": 0, "pcg": 1, "bicgstab": 2, "pbicgstab": 3}
 
     def solve_mat(self, kind, nb, b, rowptr, col, val, dia, rhs, tol, maxit, F=None, fval=None, fdia=None):
@@ -409,6 +439,8 @@ class Ref:
             "ref_zfrac_get_values": (None, [vp, C.c_uint, vp, vp]),
             "ref_zfrac_solve": (None, [vp, C.c_uint, vp]),
             "ref_zsolve_pcocg": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_zsolve_cocg": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_zsolve_bicgstab": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_emat_poisson2d_p1": (None, [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]),
             "ref_emat_solid3d_p1": (None, [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                            C.c_double, vp, vp, vp]),
@@ -623,6 +655,24 @@ class RefZSystem:
         conv = C.c_double(tol)
         it = C.c_uint(maxit)
         ok = self.ref.L.ref_zsolve_pcocg(self.h, self.f, _p(r), _p(u), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
+
+    def cocg(self, rhs, tol, maxit):
+        r = c128(rhs).copy()
+        u = np.zeros(self.nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_zsolve_cocg(self.h, _p(r), _p(u), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
+
+    def bicgstab(self, rhs, tol, maxit, precond):
+        """Solve_BiCGSTAB, or Solve_BiCGStabP with the factor when precond."""
+        r = c128(rhs).copy()
+        u = np.zeros(self.nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_zsolve_bicgstab(self.h, self.f if precond else None, _p(r), _p(u), C.byref(conv),
+                                            C.byref(it))
         return bool(ok), conv.value, it.value, u, r
 
 

@@ -15,6 +15,7 @@
 //   MatVec::Sol::Solve_*        include/delfem/matvec/solver_mat_iter.h:41-60
 //   MatVec::CZMatDia_BlkCrs, CZMatDiaFrac_BlkCrs, CZVector_Blk, Sol::Solve_PCOCG
 //                               include/delfem/matvec/z*.h (complex twin, len 1)
+#include <malloc.h>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -450,6 +451,36 @@ int ref_zsolve_pcocg(void* p, void* f, double* r, double* u, double* conv, unsig
   MatVec::CZVector_Blk rv(z->nb, 1), uv(z->nb, 1);
   zcopy_in(rv, z->nb, r);
   const bool ok = MatVec::Sol::Solve_PCOCG(*conv, *iter, *z->A, rv, uv, *static_cast<ZFracPeek*>(f), *z->A);
+  zcopy_out(rv, z->nb, r);
+  zcopy_out(uv, z->nb, u);
+  return ok ? 1 : 0;
+}
+
+// MatVec::Sol::Solve_COCG / Solve_BiCGSTAB / Solve_BiCGStabP (the last with the factor).
+// Their work vectors are CZVector_Blk, whose constructor leaves the values
+// uninitialised (zvector_blk.h:48-51), and MatVec(1, x, 0, y) multiplies the
+// old y by 0 (zmatdia_blkcrs.cpp:765): heap garbage that is Inf or NaN turns
+// into NaN.  glibc's M_PERTURB = 0xff makes every malloc'd block start as
+// zero bytes (and fills freed blocks with 0xff), so these runs see the
+// zero-initialised vectors the algorithm assumes and are reproducible.
+static void zero_new_blocks() { mallopt(M_PERTURB, 0xff); }
+int ref_zsolve_cocg(void* p, double* r, double* u, double* conv, unsigned* iter) {
+  zero_new_blocks();
+  RefZSys* z = static_cast<RefZSys*>(p);
+  MatVec::CZVector_Blk rv(z->nb, 1), uv(z->nb, 1);
+  zcopy_in(rv, z->nb, r);
+  const bool ok = MatVec::Sol::Solve_COCG(*conv, *iter, *z->A, rv, uv);
+  zcopy_out(rv, z->nb, r);
+  zcopy_out(uv, z->nb, u);
+  return ok ? 1 : 0;
+}
+int ref_zsolve_bicgstab(void* p, void* f, double* r, double* u, double* conv, unsigned* iter) {
+  zero_new_blocks();
+  RefZSys* z = static_cast<RefZSys*>(p);
+  MatVec::CZVector_Blk rv(z->nb, 1), uv(z->nb, 1);
+  zcopy_in(rv, z->nb, r);
+  const bool ok = f ? MatVec::Sol::Solve_BiCGStabP(*conv, *iter, *z->A, rv, uv, *static_cast<ZFracPeek*>(f), *z->A)
+                    : MatVec::Sol::Solve_BiCGSTAB(*conv, *iter, *z->A, rv, uv);
   zcopy_out(rv, z->nb, r);
   zcopy_out(uv, z->nb, u);
   return ok ? 1 : 0;

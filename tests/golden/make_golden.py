@@ -177,6 +177,36 @@ def zhelmholtz_fixture(ref):
         ok2, conv, it, u, r = rz.pcocg(Z["res"], 1e-10, 2000)
         out[f"zpcocg_k{lev}"] = np.array([float(ok2), conv, float(it)])
         out[f"zpcocg_u_k{lev}"], out[f"zpcocg_r_k{lev}"] = u, r
+        # BiCGStabP with this factor: the first iterations (the reference's
+        # complex BiCGSTAB conjugates alpha and omega and does not converge on
+        # this complex system)
+        ok2, conv, it, u, r = rz.bicgstab(Z["res"], 1e-10, 8, True)
+        out[f"zbicgp_k{lev}"] = np.array([float(ok2), conv, float(it)])
+        out[f"zbicgp_u_k{lev}"], out[f"zbicgp_r_k{lev}"] = u, r
+    ok2, conv, it, u, r = rz.cocg(Z["res"], 1e-10, 2000)
+    out["zcocg"] = np.array([float(ok2), conv, float(it)])
+    out["zcocg_u"], out["zcocg_r"] = u, r
+    ok2, conv, it, u, r = rz.bicgstab(Z["res"], 1e-10, 8, False)
+    out["zbicg"] = np.array([float(ok2), conv, float(it)])
+    out["zbicg_u"], out["zbicg_r"] = u, r
+    # a real-valued system in complex storage (triangles only, long wave):
+    # there all four solvers converge
+    prob2 = meshgen.zhelmholtz2d(10, 8, 1.0, 0.8, wave_length=2.0)
+    et, rt = orc.zemat_helmholtz2d(prob2.coord, prob2.conn, prob2.wave_length, prob2.value)
+    rz2 = O.RefZSystem(ref, prob2.nnode, Z["rowptr"], Z["col"])
+    rz2.merge(prob2.conn, et)
+    rz2.set_bc(prob2.flags)
+    res2 = orc.zassemble_vec(np.zeros(prob2.nnode, dtype=np.complex128), prob2.conn, rt)
+    res2[prob2.flags == 1] = 0.0
+    val2, dia2 = rz2.matrix()
+    out.update(rval=val2, rdia=dia2, rres=res2)
+    rz2.factor(0)
+    for name, run in (("pcocg", lambda: rz2.pcocg(res2, 1e-10, 2000)), ("cocg", lambda: rz2.cocg(res2, 1e-10, 2000)),
+                      ("bicg", lambda: rz2.bicgstab(res2, 1e-10, 2000, False)),
+                      ("bicgp", lambda: rz2.bicgstab(res2, 1e-10, 2000, True))):
+        ok2, conv, it, u, r = run()
+        out[f"r{name}"] = np.array([float(ok2), conv, float(it)])
+        out[f"r{name}_u"], out[f"r{name}_r"] = u, r
     return out
 
 

@@ -245,3 +245,154 @@ def test_zpattern_mismatch(gpu):
     M = gpu.CZMatDiaFrac_BlkCrs(0, a.A)
     with pytest.raises(DfmError):
         M.SetValue(b.A)
+
+
+def real_system(gpu, orc, nx, ny, lx, ly, wave_length=2.0):
+    """Triangles only, long wave: a real-valued system in complex storage,
+    where the reference's complex BiCGSTAB pair converges."""
+    import torch
+    prob = meshgen.zhelmholtz2d(nx, ny, lx, ly, wave_length=wave_length)
+    rowptr, col = gpu.fem_pattern(prob.nnode, prob.conn)
+    A = gpu.CZMatDia_BlkCrs(prob.nnode, 1)
+    A.AddPattern(rowptr, col)
+    asm = gpu.ZAssembler(A, prob.conn)
+    res = torch.zeros(prob.nnode, dtype=torch.complex128, device="cuda")
+    asm.helmholtz2d_p1(gpu.to_device(prob.coord.ravel()), prob.wave_length, dev(prob.value), res)
+    A.SetBoundaryCondition(prob.flags)
+    rh = host(res)
+    rh[prob.flags == 1] = 0.0
+    val, dia = A.values()
+    return prob, A, dev(rh), dict(rowptr=rowptr, col=col, val=val, dia=dia, res=rh)
+
+
+def test_real_system_assembly(orc, gpu):
+    prob, A, res, R = real_system(gpu, orc, 10, 8, 1.0, 0.8)
+    d = np.load(os.path.join(GOLDEN, "zhelmholtz2d_10x8.npz"), allow_pickle=False)
+    assert np.array_equal(R["val"], d["rval"]) and np.array_equal(R["dia"], d["rdia"])
+    assert np.array_equal(R["res"], d["rres"])
+
+
+def close_run(dev_run, ora_run, it_frac, err):
+    ok, conv, it, u, r = dev_run
+    ok_o, conv_o, it_o, u_o, r_o = ora_run
+    print(f"  device it={it} conv={conv:.3e}  oracle it={it_o} conv={conv_o:.3e}  "
+          f"|u-u_o|/|u_o|={np.linalg.norm(u - u_o) / np.linalg.norm(u_o):.2e}", flush=True)
+    assert ok and ok_o
+    assert abs(it - it_o) <= max(2, int(np.ceil(it_frac * it_o))), (it, it_o)
+    assert np.linalg.norm(u - u_o) <= err * np.linalg.norm(u_o)
+
+
+@pytest.mark.parametrize("name", ["zhelmholtz2d_10x8", "zhelmholtz2d_40x30"])
+def test_zcocg(orc, gpu, name):
+    """Solve_COCG on the complex Helmholtz system (natural-order SELL copy of
+    A, no preconditioner): iterations within 5 %, solution within 1e-8."""
+    import torch
+    prob = PROBLEMS[name]()
+    S, _ = system(gpu, prob)
+    Z = assemble_zproblem(orc, prob)
+    r = S.res.clone()
+    u = torch.zeros_like(r)
+    ok, conv, it = gpu.Solve_COCG(1e-10, 4000, S.A, r, u)
+    want = orc.zsolve_cocg(prob.nnode, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Z["res"], 1e-10, 4000)
+    assert conv < 1e-10
+    close_run((ok, conv, it, host(u), host(r)), want, 0.05, 1e-8)
+
+
+@pytest.mark.parametrize("maxit", [2, 3, 5])
+@pytest.mark.parametrize("precond", [False, True])
+def test_zbicgstab_first_iterations(orc, gpu, maxit, precond):
+    """The reference's complex BiCGSTAB pair on the complex Helmholtz system,
+    where it does not converge: the first iterations follow the oracle's
+    (conv_ratio of BiCGStabP included) to rounding."""
+    import torch
+    prob = PROBLEMS["zhelmholtz2d_40x30"]()
+    S, _ = system(gpu, prob)
+    Z = assemble_zproblem(orc, prob)
+    n = prob.nnode
+    r = S.res.clone()
+    u = torch.zeros_like(r)
+    if precond:
+        M = gpu.CZMatDiaFrac_BlkCrs(0, S.A)
+        M.SetValue(S.A)
+        ok, conv, it = gpu.Solve_BiCGStabP(1e-10, maxit, S.A, M, r, u)
+        F = orc.iluk_symbolic(Z["rowptr"], Z["col"], 0)
+        fv, fd, _ = orc.zilu_values(n, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F)
+        want = orc.zsolve_bicgstab(n, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Z["res"], 1e-10, maxit, F, fv, fd)
+    else:
+        ok, conv, it = gpu.Solve_BiCGSTAB(1e-10, maxit, S.A, r, u)
+        want = orc.zsolve_bicgstab(n, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Z["res"], 1e-10, maxit)
+    uh, rh = host(u), host(r)
+    print(f"  maxit {maxit} precond {precond}: conv {conv} / {want[1]}, "
+          f"u err {np.linalg.norm(uh - want[3]) / max(np.linalg.norm(want[3]), 1e-300):.2e}", flush=True)
+    assert (ok, it) == (True, maxit) == (want[0], want[2])
+    assert conv == pytest.approx(want[1], rel=1e-9)
+    if maxit > 1:
+        assert np.linalg.norm(uh - want[3]) <= 1e-9 * np.linalg.norm(want[3])
+        assert np.linalg.norm(rh - want[4]) <= 1e-9 * np.linalg.norm(want[4])
+
+
+@pytest.mark.parametrize("solver", ["cocg", "pcocg", "bicgstab", "bicgstabp", "bicgstabp_mc"])
+def test_zsolvers_real_data(orc, gpu, solver):
+    """All complex solvers on a real-valued system in complex storage, where
+    each converges (the wave is long enough for a positive definite
+    operator): COCG-type iterations within 5 % and solution 1e-8;
+    BiCGSTAB-type (rounding-sensitive) within 25 % and 1e-6."""
+    import torch
+    prob, A, res, R = real_system(gpu, orc, 40, 30, 2.0, 1.5, wave_length=10.0)
+    n = prob.nnode
+    r = res.clone()
+    u = torch.zeros_like(r)
+    order = None
+    args = (n, R["rowptr"], R["col"], R["val"], R["dia"])
+    if solver in ("pcocg", "bicgstabp", "bicgstabp_mc"):
+        ordering = "multicolour" if solver.endswith("_mc") else "natural"
+        order = orc.colour_order(R["rowptr"], R["col"])[2] if ordering == "multicolour" else None
+        M = gpu.CZMatDiaFrac_BlkCrs(0, A, ordering=ordering)
+        M.SetValue(A)
+        F = orc.iluk_symbolic(R["rowptr"], R["col"], 0, order)
+        fv, fd, _ = orc.zilu_values(*args, F, order)
+    if solver == "cocg":
+        got = gpu.Solve_COCG(1e-10, 4000, A, r, u)
+        want = orc.zsolve_cocg(*args, R["res"], 1e-10, 4000)
+    elif solver == "pcocg":
+        got = gpu.Solve_PCOCG(1e-10, 4000, A, M, r, u)
+        want = orc.zsolve_pcocg(*args, F, fv, fd, R["res"], 1e-10, 4000)
+    elif solver == "bicgstab":
+        got = gpu.Solve_BiCGSTAB(1e-10, 4000, A, r, u)
+        want = orc.zsolve_bicgstab(*args, R["res"], 1e-10, 4000)
+    else:
+        got = gpu.Solve_BiCGStabP(1e-10, 4000, A, M, r, u)
+        want = orc.zsolve_bicgstab(*args, R["res"], 1e-10, 4000, F, fv, fd, order)
+    assert got[1] < 1e-10
+    bicg = solver.startswith("bicg")
+    close_run((got[0], got[1], got[2], host(u), host(r)), want, 0.25 if bicg else 0.05, 1e-6 if bicg else 1e-8)
+    assert not host(u).imag.any()
+
+
+def test_zsolvers_golden(gpu):
+    """The reference's runs of the fixture's real-valued system and its COCG
+    run on the complex system, on the device."""
+    import torch
+    d = np.load(os.path.join(GOLDEN, "zhelmholtz2d_10x8.npz"), allow_pickle=False)
+    prob = meshgen.zhelmholtz2d(10, 8, 1.0, 0.8, wave_length=float(d["wave_length"]))
+    S, _ = system(gpu, prob)
+    n = prob.nnode
+    r, u = dev(d["zres"]), torch.zeros(n, dtype=torch.complex128, device="cuda")
+    ok, conv, it = gpu.Solve_COCG(1e-10, 2000, S.A, r, u)
+    assert abs(it - int(d["zcocg"][2])) <= 2 and conv < 1e-10
+    assert np.linalg.norm(host(u) - d["zcocg_u"]) <= 1e-8 * np.linalg.norm(d["zcocg_u"])
+    A = gpu.CZMatDia_BlkCrs(n, 1)
+    A.AddPattern(d["rowptr"], d["col"])
+    A.set_values(d["rval"], d["rdia"])
+    M = gpu.CZMatDiaFrac_BlkCrs(0, A)
+    M.SetValue(A)
+    for key, run, frac, err in (("rpcocg", lambda r, u: gpu.Solve_PCOCG(1e-10, 2000, A, M, r, u), 0.05, 1e-8),
+                                ("rcocg", lambda r, u: gpu.Solve_COCG(1e-10, 2000, A, r, u), 0.05, 1e-8),
+                                ("rbicg", lambda r, u: gpu.Solve_BiCGSTAB(1e-10, 2000, A, r, u), 0.25, 1e-6),
+                                ("rbicgp", lambda r, u: gpu.Solve_BiCGStabP(1e-10, 2000, A, M, r, u), 0.25, 1e-6)):
+        r, u = dev(d["rres"]), torch.zeros(n, dtype=torch.complex128, device="cuda")
+        ok, conv, it = run(r, u)
+        want = d[key]
+        assert ok and conv < 1e-10, key
+        assert abs(it - int(want[2])) <= max(2, int(np.ceil(frac * want[2]))), (key, it, want[2])
+        assert np.linalg.norm(host(u) - d[key + "_u"]) <= err * np.linalg.norm(d[key + "_u"]), key

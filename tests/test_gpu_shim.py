@@ -123,3 +123,64 @@ def test_cpp_matrix_layer_max_iter(orc, gpu, tmp_path, kind):
     prob = meshgen.solid3d(3, seed=3, sigma=0.5)
     _, d2, i2, _, _, _, _, _, _ = run_shim(orc, prob, kind, 1e-14, tmp_path, maxit=5)
     assert (int(i2[0]), int(i2[1]), d2[1]) == (0 if kind == 2 else 1, 5, 1e-14)
+
+
+ZBIN = os.path.join(HERE, "cpp", "zshim_test")
+ZKINDS = {0: "cocg", 1: "pcocg", 2: "bicgstab", 3: "bicgstabp"}
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_cpp_complex_dropin(orc, gpu, tmp_path, kind):
+    """The complex classes (include/delfem_amd/zmatvec.hpp) driven as the
+    reference's Helmholtz code drives its own: per-element Mearge and
+    SetBoundaryCondition bit-exact; Solve_COCG / Solve_PCOCG (ILU(1)) within
+    5 % of the oracle's iterations and 1e-8 of its solution; the
+    BiCGSTAB pair, which the reference's conjugated inner products keep from
+    converging on this complex system, through its first five iterations
+    within 1e-9."""
+    from oracle.oracle import assemble_zproblem
+    if not os.path.exists(ZBIN):
+        pytest.fail("tests/cpp/zshim_test is not built (run __graft_entry__.build())")
+    prob = meshgen.zhelmholtz2d(40, 30, 2.0, 1.5, wave_length=0.6)
+    Z = assemble_zproblem(orc, prob)
+    n, lev = prob.nnode, 1
+    maxit, tol = (3000, 1e-10) if kind <= 1 else (6, 1e-10)
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([n, prob.conn.shape[0], prob.lconn.shape[0], kind, maxit, lev], dtype=np.int32).tofile(f)
+        np.array([tol]).tofile(f)
+        np.ascontiguousarray(prob.conn, dtype=np.uint32).tofile(f)
+        np.ascontiguousarray(prob.lconn, dtype=np.uint32).tofile(f)
+        np.ascontiguousarray(Z["emat_tri"]).tofile(f)
+        np.ascontiguousarray(Z["emat_line"]).tofile(f)
+        np.ascontiguousarray(Z["res"]).tofile(f)
+        np.ascontiguousarray(prob.flags, dtype=np.int32).tofile(f)
+    run = subprocess.run([ZBIN, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    print(run.stdout, run.stderr, flush=True)
+    assert run.returncode == 0
+    raw = open(out, "rb").read()
+    conv = float(np.frombuffer(raw[:8], dtype=np.float64)[0])
+    ok, it = np.frombuffer(raw[8:16], dtype=np.int32)
+    rest = np.frombuffer(raw[16:], dtype=np.complex128)
+    nv = Z["val"].size
+    val, dia, u, r = rest[:nv], rest[nv:nv + n], rest[nv + n:nv + 2 * n], rest[nv + 2 * n:nv + 3 * n]
+    assert np.array_equal(val, Z["val"]) and np.array_equal(dia, Z["dia"])
+    F = orc.iluk_symbolic(Z["rowptr"], Z["col"], lev)
+    fv, fd, _ = orc.zilu_values(n, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F)
+    args = (n, Z["rowptr"], Z["col"], Z["val"], Z["dia"])
+    if kind == 0:
+        want = orc.zsolve_cocg(*args, Z["res"], tol, maxit)
+    elif kind == 1:
+        want = orc.zsolve_pcocg(*args, F, fv, fd, Z["res"], tol, maxit)
+    elif kind == 2:
+        want = orc.zsolve_bicgstab(*args, Z["res"], tol, maxit)
+    else:
+        want = orc.zsolve_bicgstab(*args, Z["res"], tol, maxit, F, fv, fd)
+    assert ok == 1 and want[0]
+    if kind <= 1:
+        assert conv < tol and abs(int(it) - want[2]) <= max(2, int(np.ceil(0.05 * want[2])))
+        assert np.linalg.norm(u - want[3]) <= 1e-8 * np.linalg.norm(want[3])
+    else:
+        assert int(it) == want[2] == maxit and conv == pytest.approx(want[1], rel=1e-9)
+        assert np.linalg.norm(u - want[3]) <= 1e-9 * np.linalg.norm(want[3])
+        assert np.linalg.norm(r - want[4]) <= 1e-9 * np.linalg.norm(want[4])

@@ -210,3 +210,38 @@ def test_complex_twin(orc):
         want = d[f"zpcocg_k{lev}"]
         assert (float(ok), conv, float(it)) == (want[0], want[1], want[2])
         assert np.array_equal(u, d[f"zpcocg_u_k{lev}"]) and np.array_equal(r, d[f"zpcocg_r_k{lev}"])
+
+
+def test_complex_solvers(orc):
+    """MatVec::Sol::Solve_COCG, Solve_BiCGSTAB and Solve_BiCGStabP against the
+    reference's runs, bit for bit: on the complex Helmholtz system (COCG to
+    convergence, the BiCGSTAB pair for their first iterations, which is all
+    they do there: the reference conjugates alpha and omega) and on a
+    real-valued system in complex storage, where all four solvers converge."""
+    from delfem_amd import meshgen
+    from oracle.oracle import assemble_zproblem
+    d = load("zhelmholtz2d_10x8")
+    prob = meshgen.zhelmholtz2d(10, 8, 1.0, 0.8, wave_length=float(d["wave_length"]))
+    Z = assemble_zproblem(orc, prob)
+    nb = int(d["nnode"])
+    args = (nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"])
+
+    def same(got, key, ukey=None, rkey=None):
+        want = d[key]
+        assert (float(got[0]), got[1], float(got[2])) == (want[0], want[1], want[2]), key
+        assert np.array_equal(got[3], d[ukey or key + "_u"]) and np.array_equal(got[4], d[rkey or key + "_r"]), key
+
+    same(orc.zsolve_cocg(*args, Z["res"], 1e-10, 2000), "zcocg")
+    same(orc.zsolve_bicgstab(*args, Z["res"], 1e-10, 8), "zbicg")
+    for lev in (0, 1):
+        F = orc.iluk_symbolic(Z["rowptr"], Z["col"], lev)
+        fv, fd, _ = orc.zilu_values(*args, F)
+        same(orc.zsolve_bicgstab(*args, Z["res"], 1e-10, 8, F, fv, fd), f"zbicgp_k{lev}", f"zbicgp_u_k{lev}",
+             f"zbicgp_r_k{lev}")
+    rargs = (nb, Z["rowptr"], Z["col"], d["rval"], d["rdia"])
+    F = orc.iluk_symbolic(Z["rowptr"], Z["col"], 0)
+    fv, fd, _ = orc.zilu_values(*rargs, F)
+    same(orc.zsolve_pcocg(*rargs, F, fv, fd, d["rres"], 1e-10, 2000), "rpcocg")
+    same(orc.zsolve_cocg(*rargs, d["rres"], 1e-10, 2000), "rcocg")
+    same(orc.zsolve_bicgstab(*rargs, d["rres"], 1e-10, 2000), "rbicg")
+    same(orc.zsolve_bicgstab(*rargs, d["rres"], 1e-10, 2000, F, fv, fd), "rbicgp")

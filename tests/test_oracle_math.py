@@ -468,3 +468,34 @@ def test_complex_ilu_and_pcocg(orc):
     assert ok and conv < 1e-11 and 0 < it < 2000
     assert np.linalg.norm(u - xd) <= 1e-8 * np.linalg.norm(xd)
     assert np.linalg.norm(Z["res"] - A @ u) <= 1e-10 * np.linalg.norm(Z["res"])
+
+
+def test_complex_solvers_real_data(orc):
+    """On a real-valued system in complex storage every complex solver is
+    its real counterpart: COCG, BiCGSTAB and BiCGStabP (ILU(0)) reach the
+    direct solution; on the complex Helmholtz system COCG does too."""
+    from oracle.oracle import assemble_zproblem
+    prob = meshgen.zhelmholtz2d(9, 7, 1.0, 0.8, wave_length=2.0)
+    rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)
+    et, rt = orc.zemat_helmholtz2d(prob.coord, prob.conn, prob.wave_length, prob.value)
+    val, dia = orc.zassemble(prob.nnode, rowptr, col, prob.conn, et)
+    res = orc.zassemble_vec(np.zeros(prob.nnode, dtype=np.complex128), prob.conn, rt)
+    val, dia, res = orc.zset_bc(prob.nnode, rowptr, col, val, dia, prob.flags, res)
+    assert not val.imag.any() and not dia.imag.any() and not res.imag.any()
+    nb = prob.nnode
+    A = _zdense(nb, rowptr, col, val, dia)
+    xd = np.linalg.solve(A, res)
+    F = orc.iluk_symbolic(rowptr, col, 0)
+    fv, fd, _ = orc.zilu_values(nb, rowptr, col, val, dia, F)
+    args = (nb, rowptr, col, val, dia, res, 1e-11, 2000)
+    for ok, conv, it, u, r in (orc.zsolve_cocg(*args), orc.zsolve_bicgstab(*args),
+                               orc.zsolve_bicgstab(*args, F, fv, fd)):
+        assert ok and conv < 1e-11 and 0 < it < 2000
+        assert np.linalg.norm(u - xd) <= 1e-8 * np.linalg.norm(xd)
+        assert not u.imag.any()
+    Z = assemble_zproblem(orc, meshgen.zhelmholtz2d(8, 6, 1.0, 0.75, wave_length=0.35))
+    nb = Z["res"].size
+    A = _zdense(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"])
+    ok, conv, it, u, r = orc.zsolve_cocg(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Z["res"], 1e-11, 2000)
+    xd = np.linalg.solve(A, Z["res"])
+    assert ok and conv < 1e-11 and np.linalg.norm(u - xd) <= 1e-8 * np.linalg.norm(xd)

@@ -120,3 +120,28 @@ def test_complex_twin_vs_reference(orc, ref, lev):
     got = orc.zsolve_pcocg(prob.nnode, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F, fv, fd, Z["res"], 1e-10, 2000)
     exp = rz.pcocg(Z["res"], 1e-10, 2000)
     assert got[:3] == exp[:3] and np.array_equal(got[3], exp[3]) and np.array_equal(got[4], exp[4])
+
+
+@pytest.mark.parametrize("maxit", [2, 7, 3000])
+def test_complex_solvers_vs_reference(orc, ref, maxit):
+    """Solve_COCG, Solve_BiCGSTAB and Solve_BiCGStabP (ILU(0)) against the
+    reference on the complex Helmholtz problem, bit for bit (NaN equal to
+    NaN: the reference's complex BiCGSTAB pair breaks down there)."""
+    from oracle.oracle import RefZSystem, assemble_zproblem
+    prob = meshgen.zhelmholtz2d(14, 9, 1.2, 0.8, wave_length=0.3)
+    Z = assemble_zproblem(orc, prob)
+    nb = prob.nnode
+    rz = RefZSystem(ref, nb, Z["rowptr"], Z["col"])
+    rz.merge(prob.conn, Z["emat_tri"])
+    rz.merge(prob.lconn, Z["emat_line"])
+    rz.set_bc(prob.flags)
+    rz.factor(0)
+    F = orc.iluk_symbolic(Z["rowptr"], Z["col"], 0)
+    fv, fd, _ = orc.zilu_values(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F)
+    args = (nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Z["res"], 1e-10, maxit)
+    for got, exp in ((orc.zsolve_cocg(*args), rz.cocg(Z["res"], 1e-10, maxit)),
+                     (orc.zsolve_bicgstab(*args), rz.bicgstab(Z["res"], 1e-10, maxit, False)),
+                     (orc.zsolve_bicgstab(*args, F, fv, fd), rz.bicgstab(Z["res"], 1e-10, maxit, True))):
+        assert got[0] == exp[0] and got[2] == exp[2]
+        assert got[1] == exp[1] or (np.isnan(got[1]) and np.isnan(exp[1]))
+        assert np.array_equal(got[3], exp[3], equal_nan=True) and np.array_equal(got[4], exp[4], equal_nan=True)
