@@ -459,7 +459,7 @@ int ref_zsolve_pcocg(void* p, void* f, double* r, double* u, double* conv, unsig
 // MatVec::Sol::Solve_COCG / Solve_BiCGSTAB / Solve_BiCGStabP (the last with the factor).
 // Their work vectors are CZVector_Blk, whose constructor leaves the values
 // uninitialised (zvector_blk.h:48-51), and MatVec(1, x, 0, y) multiplies the
-// old y by 0 (zmatdia_blkcrs.cpp:765): heap garbage that is Inf or NaN turns
+// old y by 0 (zmatdia_blkcrs.cpp:767): heap garbage that is Inf or NaN turns
 // into NaN.  glibc's M_PERTURB = 0xff makes every malloc'd block start as
 // zero bytes (and fills freed blocks with 0xff), so these runs see the
 // zero-initialised vectors the algorithm assumes and are reproducible.
