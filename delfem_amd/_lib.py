@@ -110,6 +110,19 @@ _SIGS = {
     "dfm_ilu_solve_precond": (C.c_int, [H, vp]),
     "dfm_solve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_solve_pbicgstab": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
+    "dfm_bmat_create": (C.c_int, [C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.POINTER(H)]),
+    "dfm_bmat_destroy": (C.c_int, [H]),
+    "dfm_bmat_set_pattern_boundary": (C.c_int, [H, H, vp, vp]),
+    "dfm_bmat_info": (C.c_int, [H, u32p, u32p, u32p]),
+    "dfm_bmat_get_pattern": (C.c_int, [H, vp, vp]),
+    "dfm_bmat_set_zero": (C.c_int, [H]),
+    "dfm_bmat_get_values": (C.c_int, [H, vp]),
+    "dfm_bmat_mearge": (C.c_int, [H, C.c_uint, vp, C.c_uint, vp]),
+    "dfm_bmat_copy_from": (C.c_int, [H, H]),
+    "dfm_bmat_set_bc_inverse_col": (C.c_int, [H, vp]),
+    "dfm_bmat_set_bc_row": (C.c_int, [H, vp]),
+    "dfm_bmat_matvec": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
+    "dfm_vec_copy_free": (C.c_int, [C.c_size_t, vp, vp, vp]),
     "dfm_zbcrs_create": (C.c_int, [C.c_uint, C.c_uint, C.POINTER(H)]),
     "dfm_zbcrs_destroy": (C.c_int, [H]),
     "dfm_zbcrs_add_pattern": (C.c_int, [H, C.c_uint, vp, vp]),

@@ -356,6 +356,138 @@ class FEMSystem:
         call("dfm_vec_axpy", self.value.numel(), 1.0, _dptr(update), _dptr(self.value))
 
 
+class CMat_BlkCrs:
+    """MatVec::CMat_BlkCrs (include/delfem/matvec/mat_blkcrs.h:45-197) as the
+    boundary matrix of the saved-stiffness system: square, square blocks."""
+
+    def __init__(self, nblk_col, len_col, nblk_row, len_row):
+        h = C.c_void_p()
+        call("dfm_bmat_create", nblk_col, len_col, nblk_row, len_row, C.byref(h))
+        self.h, self.nblk, self.len = h, nblk_col, len_col
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_bmat_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def NCrs(self):
+        n = C.c_uint()
+        call("dfm_bmat_info", self.h, None, None, C.byref(n))
+        return n.value
+
+    def SetPatternBoundary(self, A, flag_col, flag_row):
+        fc = np.ascontiguousarray(flag_col, dtype=np.int32).reshape(-1)
+        fr = np.ascontiguousarray(flag_row, dtype=np.int32).reshape(-1)
+        call("dfm_bmat_set_pattern_boundary", self.h, A.h, _ptr(fc), _ptr(fr))
+        return True
+
+    def pattern(self):
+        n = self.NCrs()
+        rowptr = np.zeros(self.nblk + 1, dtype=np.uint32)
+        col = np.zeros(max(1, n), dtype=np.uint32)
+        call("dfm_bmat_get_pattern", self.h, _ptr(rowptr), _ptr(col))
+        return rowptr, col[:n]
+
+    def SetZero(self):
+        call("dfm_bmat_set_zero", self.h)
+        return True
+
+    def values(self):
+        n = self.NCrs() * self.len * self.len
+        val = np.zeros(max(1, n))
+        call("dfm_bmat_get_values", self.h, _ptr(val))
+        return val[:n]
+
+    def Mearge(self, nodes, emat):
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        emat = np.ascontiguousarray(emat, dtype=np.float64).reshape(-1)
+        call("dfm_bmat_mearge", self.h, nodes.size, _ptr(nodes), self.len * self.len, _ptr(emat))
+        return True
+
+    def CopyFrom(self, A):
+        """The entries of the assembled diagonal-block matrix A (before its
+        boundary conditions): the same sums as merging the elements here."""
+        call("dfm_bmat_copy_from", self.h, A.h)
+        return True
+
+    def SetBoundaryConditionInverse_Colum(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32).reshape(-1)
+        call("dfm_bmat_set_bc_inverse_col", self.h, _ptr(f))
+        return True
+
+    def SetBoundaryCondition_Row(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32).reshape(-1)
+        call("dfm_bmat_set_bc_row", self.h, _ptr(f))
+        return True
+
+    def MatVec(self, alpha, x, beta, y):
+        _sync_torch()
+        call("dfm_bmat_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
+        return True
+
+
+class FEMSystemSave(FEMSystem):
+    """Fem::Ls::CLinearSystem_Save (src/femls/linearsystem_fieldsave.cpp) for
+    a linear static Problem: the assembly also keeps the force f (element
+    forces alone) and the boundary matrix K_b (the stiffness between free rows
+    and fixed columns); MakeResidual forms r = f - K_b u for the current
+    field u without re-assembly, the solve gives the free values directly and
+    UpdateValueOfField sets them (fixed dofs keep their prescribed values)."""
+
+    def __init__(self, prob):
+        torch = _torch()
+        super().__init__(prob)
+        self.Kb = CMat_BlkCrs(prob.nnode, prob.b, prob.nnode, prob.b)
+        self.Kb.SetPatternBoundary(self.A, self.flags, self.flags)
+        self.force = torch.zeros(prob.ndof, dtype=torch.float64, device="cuda")
+
+    def assemble(self):
+        """InitializeMarge + element loop + FinalizeMarge of the saved system
+        (linearsystem_fieldsave.cpp:166-204): the element loop runs at u = 0,
+        so the assembled residual is the force."""
+        torch = _torch()
+        value = self.value
+        self.value = torch.zeros_like(value)
+        try:
+            self.A.SetZero()
+            self.res.zero_()
+            p = self.prob
+            if p.kind == "poisson2d":
+                self.asm.poisson2d_p1(self.coord, self.coef[0], p.source, self.value, self.res)
+            elif p.kind == "helmholtz2d":
+                self.asm.helmholtz2d_p1(self.coord, self.coef[0], self.coef[1], p.source, self.value, self.res)
+            elif p.kind == "solid3d":
+                self.asm.solid3d_p1(self.coord, self.coef[0], self.coef[1], p.rho, p.g, self.value, self.res)
+            else:
+                raise ValueError(f"no saved-stiffness assembly for {p.kind}")
+        finally:
+            self.value = value
+        self.force.copy_(self.res)
+        self.Kb.CopyFrom(self.A)
+        self.A.SetBoundaryCondition(self.flags)
+        self.Kb.SetBoundaryConditionInverse_Colum(self.flags)
+        self.Kb.SetBoundaryCondition_Row(self.flags)
+        return 0.0
+
+    def MakeResidual(self):
+        """r = f - K_b u at the current field values (linearsystem_fieldsave.cpp:206-247); returns ||r||."""
+        _sync_torch()
+        call("dfm_vec_copy", self.res.numel(), _dptr(self.force), _dptr(self.res))
+        self.Kb.MatVec(-1.0, self.value, 1.0, self.res)
+        return dot(self.res, self.res) ** 0.5
+
+    def UpdateValueOfField(self, update):
+        """CLinearSystem_Save::UpdateValueOfField (linearsystem_fieldsave.cpp:249-300):
+        the field takes the solution at free dofs; fixed dofs keep their values."""
+        _sync_torch()
+        if update.numel() != self.value.numel():
+            raise ValueError("update and field sizes differ")
+        call("dfm_vec_copy_free", self.value.numel(), _ptr(self.flags.reshape(-1)), _dptr(update), _dptr(self.value))
+
+
 class BatchSystem:
     """Many independent scalar systems (the diagonal blocks of one assembled
     FEMSystem) solved together: dfm_batch_* -- per system the reference's

@@ -240,6 +240,38 @@ int dfm_solve_pcg(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
 int dfm_solve_pbicgstab(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
                         double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
 
+/* ---- saved-stiffness residual (Fem::Ls::CLinearSystem_Save) --------------
+ * The boundary matrix K_b of src/femls/linearsystem_fieldsave.cpp: a
+ * MatVec::CMat_BlkCrs (include/delfem/matvec/mat_blkcrs.h:45-197) whose rows
+ * are the block rows with a free dof and whose columns are the blocks with a
+ * fixed dof, so that MakeResidual (linearsystem_fieldsave.cpp:206-247) is
+ * r = f - K_b u without re-assembly.  Square matrices of square blocks. */
+typedef struct dfm_bmat_s dfm_bmat;     /* MatVec::CMat_BlkCrs */
+int dfm_bmat_create(unsigned nblk_col, unsigned len_col, unsigned nblk_row, unsigned len_row,
+                    dfm_bmat **out);                                       /* CMat_BlkCrs(nblk_col,len_col,nblk_row,len_row) */
+int dfm_bmat_destroy(dfm_bmat *M);
+/* SetPatternBoundary(A, bc_col, bc_row) mat_blkcrs.cpp:1177-1265 (host flags, nblk*len each):
+ * per row with a free dof (bc_col), its own block first, then its CRS neighbours, those with
+ * a fixed dof (bc_row); values zero */
+int dfm_bmat_set_pattern_boundary(dfm_bmat *M, const dfm_bcrs *A, const int *flag_col, const int *flag_row);
+int dfm_bmat_info(const dfm_bmat *M, unsigned *nblk, unsigned *len, unsigned *ncrs);
+int dfm_bmat_get_pattern(const dfm_bmat *M, unsigned *rowptr, unsigned *col);
+int dfm_bmat_set_zero(dfm_bmat *M);                                        /* SetZero mat_blkcrs.cpp:227 */
+int dfm_bmat_get_values(const dfm_bmat *M, double *val);                   /* ncrs*len*len, row-major blocks */
+int dfm_bmat_mearge(dfm_bmat *M, unsigned nblkel, const unsigned *blkel,
+                    unsigned blksize, const double *emat);                 /* Mearge (one square element, host emat) mat_blkcrs.cpp:754-828 */
+/* The values of A (assembled, before its boundary conditions) at K_b's entries: equal to
+ * merging the same element matrices into K_b (the assembly loops merge every element into
+ * both, eqn_linear_solid3d.cpp:511-512); A must have the pattern K_b was made from. */
+int dfm_bmat_copy_from(dfm_bmat *M, const dfm_bcrs *A);
+int dfm_bmat_set_bc_inverse_col(dfm_bmat *M, const int *flag);             /* SetBoundaryConditionInverse_Colum mat_blkcrs.cpp:876-907 */
+int dfm_bmat_set_bc_row(dfm_bmat *M, const int *flag);                     /* SetBoundaryCondition_Row mat_blkcrs.cpp:910-952 */
+int dfm_bmat_matvec(dfm_bmat *M, double alpha, const double *x_dev, double beta,
+                    double *y_dev);                                        /* MatVec(..., isnt_trans=true) mat_blkcrs.cpp:955-1176 */
+/* y[i] = x[i] where flag[i] == 0 (host flags): the field update of CLinearSystem_Save::
+ * UpdateValueOfField (linearsystem_fieldsave.cpp:249-300), fixed dofs keep their values */
+int dfm_vec_copy_free(size_t n, const int *flag, const double *x_dev, double *y_dev);
+
 /* ---- complex twin (scalar blocks, as the reference's complex classes) ----
  * Complex values are interleaved (re, im) doubles: the layout of
  * Com::Complex and of std::complex<double>.  Every complex operation keeps

@@ -997,6 +997,109 @@ int orc_solve_mat(int kind, unsigned nb, unsigned b,
 }
 
 /* ------------------------------------------------------------------------- */
+/* Saved-stiffness residual (CLinearSystem_Save, src/femls/linearsystem_fieldsave.cpp):
+ * the boundary matrix is a CMat_BlkCrs (square b x b blocks, the diagonal
+ * block kept inside the CRS rows). */
+
+/* CMat_BlkCrs::SetPatternBoundary (src/matvec/mat_blkcrs.cpp:1177-1265) from
+ * the CRS of a diagonal-block matrix: rows with at least one free dof, and
+ * in them the columns with at least one fixed dof -- the row's own block
+ * first, then its neighbours in CRS order.  brow: nb+1; bcol capacity
+ * rowptr[nb] + nb.  Returns the number of entries. */
+unsigned orc_bmat_pattern_boundary(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                                   const int *flag_col, const int *flag_row, unsigned *brow, unsigned *bcol)
+{
+    unsigned char *nz_col = (unsigned char *)calloc(nb ? nb : 1, 1), *nz_row = (unsigned char *)calloc(nb ? nb : 1, 1);
+    for (unsigned i = 0; i < nb; i++) {
+        for (unsigned d = 0; d < b; d++) {
+            if (flag_col[(size_t)i * b + d] == 0) { nz_col[i] = 1; break; }
+        }
+        for (unsigned d = 0; d < b; d++) {
+            if (flag_row[(size_t)i * b + d] != 0) { nz_row[i] = 1; break; }
+        }
+    }
+    unsigned n = 0;
+    brow[0] = 0;
+    for (unsigned i = 0; i < nb; i++) {
+        if (nz_col[i]) {
+            if (nz_row[i]) bcol[n++] = i;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++)
+                if (nz_row[col[q]]) bcol[n++] = col[q];
+        }
+        brow[i + 1] = n;
+    }
+    free(nz_col); free(nz_row);
+    return n;
+}
+/* CMat_BlkCrs::Mearge (mat_blkcrs.cpp:754-828) of all elements in order:
+ * blocks whose column is not in the row are skipped. */
+void orc_bmat_mearge(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, unsigned nelem,
+                     unsigned npe, const unsigned *conn, const double *emat, double *val)
+{
+    const size_t bb = (size_t)b * b;
+    int *pos = (int *)malloc(sizeof(int) * (nb ? nb : 1));
+    for (unsigned i = 0; i < nb; i++) pos[i] = -1;
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *no = conn + (size_t)e * npe;
+        for (unsigned a = 0; a < npe; a++) {
+            const unsigned i = no[a];
+            for (unsigned q = brow[i]; q < brow[i + 1]; q++) pos[bcol[q]] = (int)q;
+            for (unsigned c = 0; c < npe; c++) {
+                const int q = pos[no[c]];
+                if (q < 0) continue;
+                const double *in = emat + (((size_t)e * npe + a) * npe + c) * bb;
+                for (size_t t = 0; t < bb; t++) val[(size_t)q * bb + t] += in[t];
+            }
+            for (unsigned q = brow[i]; q < brow[i + 1]; q++) pos[bcol[q]] = -1;
+        }
+    }
+    free(pos);
+}
+/* SetBoundaryConditionInverse_Colum (:876-907): zero the columns of free dofs */
+void orc_bmat_bc_inverse_col(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, const int *flag,
+                             double *val)
+{
+    const size_t bb = (size_t)b * b;
+    for (unsigned q = 0; q < brow[nb]; q++) {
+        const unsigned j = bcol[q];
+        for (unsigned jd = 0; jd < b; jd++) {
+            if (flag[(size_t)j * b + jd] != 0) continue;
+            for (unsigned id = 0; id < b; id++) val[(size_t)q * bb + (size_t)id * b + jd] = 0.0;
+        }
+    }
+}
+/* SetBoundaryCondition_Row (:910-952): zero the rows of fixed dofs */
+void orc_bmat_bc_row(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, const int *flag, double *val)
+{
+    const size_t bb = (size_t)b * b;
+    (void)bcol;
+    for (unsigned i = 0; i < nb; i++)
+        for (unsigned id = 0; id < b; id++) {
+            if (flag[(size_t)i * b + id] == 0) continue;
+            for (unsigned q = brow[i]; q < brow[i + 1]; q++)
+                for (unsigned jd = 0; jd < b; jd++) val[(size_t)q * bb + (size_t)id * b + jd] = 0.0;
+        }
+}
+/* CMat_BlkCrs::MatVec (:955-1176), isnt_trans = true, square blocks:
+ * y = beta y + alpha A x, each product added as (alpha a) x straight into y;
+ * scalar blocks with beta = 0 start from y = 0 */
+void orc_bmat_matvec(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, const double *val,
+                     double alpha, const double *x, double beta, double *y)
+{
+    const size_t bb = (size_t)b * b;
+    for (unsigned i = 0; i < nb; i++) {
+        double *yi = y + (size_t)i * b;
+        if (b == 1 && beta == 0.0) yi[0] = 0.0;
+        else for (unsigned id = 0; id < b; id++) yi[id] *= beta;
+        for (unsigned q = brow[i]; q < brow[i + 1]; q++) {
+            const double *xj = x + (size_t)bcol[q] * b;
+            for (unsigned id = 0; id < b; id++)
+                for (unsigned jd = 0; jd < b; jd++) yi[id] += alpha * val[(size_t)q * bb + (size_t)id * b + jd] * xj[jd];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Complex twin (scalar blocks only, as the reference's complex ILU):
  * Com::Complex arithmetic (include/delfem/complex.h:82-160) written out in
  * the same operation order; a complex value is two doubles (re, im). */

@@ -146,6 +146,15 @@ void orc_zilu_forward(unsigned nb, const unsigned *frowptr, const unsigned *fcol
                       const double *fval, const double *fdia, double *vec);
 void orc_zilu_backward(unsigned nb, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
                        const double *fval, const double *fdia, double *vec);
+unsigned orc_bmat_pattern_boundary(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned *col,
+                                   const int *flag_col, const int *flag_row, unsigned *brow, unsigned *bcol);
+void orc_bmat_mearge(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, unsigned nelem,
+                     unsigned npe, const unsigned *conn, const double *emat, double *val);
+void orc_bmat_bc_inverse_col(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, const int *flag,
+                             double *val);
+void orc_bmat_bc_row(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, const int *flag, double *val);
+void orc_bmat_matvec(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, const double *val,
+                     double alpha, const double *x, double beta, double *y);
 int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
                      const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const double *fval,
                      const double *fdia, const int *order, double *r, double *u, double *conv_ratio,

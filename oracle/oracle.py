@@ -90,6 +90,12 @@ class Orc:
         L.orc_zsolve_pcocg.restype = C.c_int
         L.orc_zsolve_pcocg.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                        C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_bmat_pattern_boundary.restype = C.c_uint
+        L.orc_bmat_pattern_boundary.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp]
+        L.orc_bmat_mearge.argtypes = [C.c_uint, C.c_uint, vp, vp, C.c_uint, C.c_uint, vp, vp, vp]
+        L.orc_bmat_bc_inverse_col.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp]
+        L.orc_bmat_bc_row.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp]
+        L.orc_bmat_matvec.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, C.c_double, vp, C.c_double, vp]
         L.orc_zsolve_cocg.restype = C.c_int
         L.orc_zsolve_cocg.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zsolve_bicgstab.restype = C.c_int
@@ -282,6 +288,35 @@ class Orc:
                                      _p(c128(fdia)), _p(order), _p(r), _p(u), C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value, u, r
 
+    def bmat_pattern_boundary(self, nb, b, rowptr, col, flag_col, flag_row):
+        """CMat_BlkCrs::SetPatternBoundary: (brow, bcol)."""
+        rowptr, col = u32(rowptr), u32(col)
+        brow = np.zeros(nb + 1, dtype=np.uint32)
+        bcol = np.zeros(int(rowptr[-1]) + nb + 1, dtype=np.uint32)
+        n = self.L.orc_bmat_pattern_boundary(nb, b, _p(rowptr), _p(col), _p(i32(flag_col)), _p(i32(flag_row)),
+                                             _p(brow), _p(bcol))
+        return brow, bcol[:n].copy()
+
+    def bmat_mearge(self, nb, b, brow, bcol, conn, emat, val=None):
+        conn = u32(conn)
+        val = np.zeros(int(brow[-1]) * b * b) if val is None else f64(val).copy()
+        self.L.orc_bmat_mearge(nb, b, _p(u32(brow)), _p(u32(bcol)), conn.shape[0], conn.shape[1], _p(conn),
+                               _p(f64(emat)), _p(val))
+        return val
+
+    def bmat_bc(self, nb, b, brow, bcol, flag, val):
+        """SetBoundaryConditionInverse_Colum then SetBoundaryCondition_Row with the same flags."""
+        val = f64(val).copy()
+        self.L.orc_bmat_bc_inverse_col(nb, b, _p(u32(brow)), _p(u32(bcol)), _p(i32(flag)), _p(val))
+        self.L.orc_bmat_bc_row(nb, b, _p(u32(brow)), _p(u32(bcol)), _p(i32(flag)), _p(val))
+        return val
+
+    def bmat_matvec(self, nb, b, brow, bcol, val, alpha, x, beta, y):
+        y = f64(y).copy()
+        self.L.orc_bmat_matvec(nb, b, _p(u32(brow)), _p(u32(bcol)), _p(f64(val)), float(alpha), _p(f64(x)),
+                               float(beta), _p(y))
+        return y
+
     def zsolve_cocg(self, nb, rowptr, col, val, dia, rhs, tol, maxit):
         """MatVec::Sol::Solve_COCG: (ok, conv, iter, u, r_final)."""
         r = c128(rhs).copy()
@@ -440,6 +475,13 @@ class Ref:
             "ref_zfrac_solve": (None, [vp, C.c_uint, vp]),
             "ref_zsolve_pcocg": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_zsolve_cocg": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_bmat_create_boundary": (vp, [vp, vp, vp]),
+            "ref_bmat_free": (None, [vp]),
+            "ref_bmat_ncrs": (C.c_uint, [vp]),
+            "ref_bmat_get": (None, [vp, vp, vp, vp]),
+            "ref_bmat_merge": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
+            "ref_bmat_bc": (None, [vp, vp, C.c_int]),
+            "ref_bmat_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
             "ref_zsolve_bicgstab": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_emat_poisson2d_p1": (None, [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]),
             "ref_emat_solid3d_p1": (None, [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
@@ -595,6 +637,40 @@ class RefFrac:
         it = C.c_uint(maxit)
         ok = self.s.ref.L.ref_frac_solve(self.h, self.s.h, 0 if kind == "pcg" else 1, C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value
+
+
+class RefBMat:
+    """MatVec::CMat_BlkCrs of the reference, patterned by SetPatternBoundary
+    on a RefSystem's matrix (the boundary matrix of CLinearSystem_Save)."""
+
+    def __init__(self, rs, flag_col, flag_row):
+        self.ref, self.nb, self.b = rs.ref, rs.nb, rs.b
+        self.h = self.ref.L.ref_bmat_create_boundary(rs.h, _p(i32(flag_col)), _p(i32(flag_row)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.ref.L.ref_bmat_free(self.h)
+
+    def get(self):
+        n = self.ref.L.ref_bmat_ncrs(self.h)
+        brow = np.zeros(self.nb + 1, dtype=np.uint32)
+        bcol = np.zeros(max(n, 1), dtype=np.uint32)
+        val = np.zeros(max(n, 1) * self.b * self.b)
+        self.ref.L.ref_bmat_get(self.h, _p(brow), _p(bcol), _p(val))
+        return brow, bcol[:n], val[:n * self.b * self.b]
+
+    def merge(self, conn, emat):
+        conn = u32(conn)
+        self.ref.L.ref_bmat_merge(self.h, conn.shape[0], conn.shape[1], _p(conn), _p(f64(emat)))
+
+    def bc(self, flag):
+        self.ref.L.ref_bmat_bc(self.h, _p(i32(flag)), 1)
+        self.ref.L.ref_bmat_bc(self.h, _p(i32(flag)), 0)
+
+    def matvec(self, alpha, x, beta, y):
+        y = f64(y).copy()
+        self.ref.L.ref_bmat_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
+        return y
 
 
 class RefZSystem:

@@ -456,6 +456,81 @@ int ref_zsolve_pcocg(void* p, void* f, double* r, double* u, double* conv, unsig
   return ok ? 1 : 0;
 }
 
+// ---- CMat_BlkCrs as CLinearSystem_Save's boundary matrix ---------------------
+struct RefBMat {
+  MatVec::CMat_BlkCrs* M = nullptr;
+  unsigned nb = 0, b = 0;
+};
+// a CMat_BlkCrs with SetPatternBoundary(A of the system, bc_col, bc_row)
+void* ref_bmat_create_boundary(void* sys, const int* flag_col, const int* flag_row) {
+  RefSys* s = static_cast<RefSys*>(sys);
+  RefBMat* m = new RefBMat;
+  m->nb = s->nb;
+  m->b = s->b;
+  m->M = new MatVec::CMat_BlkCrs(s->nb, s->b, s->nb, s->b);
+  MatVec::CBCFlag bc_col(s->nb, s->b), bc_row(s->nb, s->b);
+  for (unsigned i = 0; i < s->nb; i++)
+    for (unsigned d = 0; d < s->b; d++) {
+      if (flag_col[i * s->b + d]) bc_col.SetBC(i, d);
+      if (flag_row[i * s->b + d]) bc_row.SetBC(i, d);
+    }
+  m->M->SetPatternBoundary(s->ls.GetMatrix(0), bc_col, bc_row);
+  m->M->SetZero();
+  return m;
+}
+void ref_bmat_free(void* p) {
+  RefBMat* m = static_cast<RefBMat*>(p);
+  delete m->M;
+  delete m;
+}
+unsigned ref_bmat_ncrs(void* p) { return static_cast<RefBMat*>(p)->M->NCrs(); }
+// rowptr: nb+1, col: ncrs, val: ncrs*b*b (any may be NULL)
+void ref_bmat_get(void* p, unsigned* rowptr, unsigned* col, double* val) {
+  RefBMat* m = static_cast<RefBMat*>(p);
+  const unsigned bb = m->b * m->b;
+  unsigned n = 0;
+  if (rowptr) rowptr[0] = 0;
+  for (unsigned i = 0; i < m->nb; i++) {
+    unsigned np = 0;
+    const unsigned* ind = m->M->GetPtrIndPSuP(i, np);
+    const double* v = m->M->GetPtrValPSuP(i, np);
+    for (unsigned k = 0; k < np; k++) {
+      if (col) col[n + k] = ind[k];
+      if (val)
+        for (unsigned t = 0; t < bb; t++) val[(size_t)(n + k) * bb + t] = v[k * bb + t];
+    }
+    n += np;
+    if (rowptr) rowptr[i + 1] = n;
+  }
+}
+void ref_bmat_merge(void* p, unsigned nelem, unsigned npe, const unsigned* conn, const double* emat) {
+  RefBMat* m = static_cast<RefBMat*>(p);
+  const unsigned bb = m->b * m->b;
+  for (unsigned e = 0; e < nelem; e++)
+    m->M->Mearge(npe, conn + (size_t)e * npe, npe, conn + (size_t)e * npe, bb, emat + (size_t)e * npe * npe * bb);
+}
+void ref_bmat_bc(void* p, const int* flag, int inverse_col) {
+  RefBMat* m = static_cast<RefBMat*>(p);
+  MatVec::CBCFlag bc(m->nb, m->b);
+  for (unsigned i = 0; i < m->nb; i++)
+    for (unsigned d = 0; d < m->b; d++)
+      if (flag[i * m->b + d]) bc.SetBC(i, d);
+  if (inverse_col) m->M->SetBoundaryConditionInverse_Colum(bc);
+  else m->M->SetBoundaryCondition_Row(bc);
+}
+void ref_bmat_matvec(void* p, double alpha, const double* x, double beta, double* y) {
+  RefBMat* m = static_cast<RefBMat*>(p);
+  MatVec::CVector_Blk xv(m->nb, m->b), yv(m->nb, m->b);
+  for (unsigned i = 0; i < m->nb; i++)
+    for (unsigned d = 0; d < m->b; d++) {
+      xv.SetValue(i, d, x[i * m->b + d]);
+      yv.SetValue(i, d, y[i * m->b + d]);
+    }
+  m->M->MatVec(alpha, xv, beta, yv, true);
+  for (unsigned i = 0; i < m->nb; i++)
+    for (unsigned d = 0; d < m->b; d++) y[i * m->b + d] = yv.GetValue(i, d);
+}
+
 // MatVec::Sol::Solve_COCG / Solve_BiCGSTAB / Solve_BiCGStabP (the last with the factor).
 // Their work vectors are CZVector_Blk, whose constructor leaves the values
 // uninitialised (zvector_blk.h:48-51), and MatVec(1, x, 0, y) multiplies the

@@ -499,3 +499,30 @@ def test_complex_solvers_real_data(orc):
     ok, conv, it, u, r = orc.zsolve_cocg(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Z["res"], 1e-11, 2000)
     xd = np.linalg.solve(A, Z["res"])
     assert ok and conv < 1e-11 and np.linalg.norm(u - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+def test_boundary_matrix_is_free_fixed_block(orc):
+    """The boundary matrix after its two conditions is K[free rows, fixed
+    columns] of the assembled stiffness (dense check), so f - K_b u is the
+    right-hand side of the free equations for prescribed values u."""
+    from oracle.oracle import problem_emats
+    for prob in (meshgen.poisson2d(5, seed=2), meshgen.solid3d(2, seed=2, sigma=0.3)):
+        nb, b = prob.nnode, prob.b
+        n = nb * b
+        flags = np.ascontiguousarray(prob.flags, dtype=np.int32).reshape(-1)
+        rowptr, col = orc.fem_pattern(nb, prob.conn)
+        emat, _ = problem_emats(orc, prob)
+        brow, bcol = orc.bmat_pattern_boundary(nb, b, rowptr, col, flags, flags)
+        val = orc.bmat_bc(nb, b, brow, bcol, flags, orc.bmat_mearge(nb, b, brow, bcol, prob.conn, emat))
+        K = np.zeros((n, n))
+        ee = prob.npe * b
+        for e in range(prob.nelem):
+            dofs = (prob.conn[e][:, None] * b + np.arange(b)[None, :]).reshape(-1)
+            K[np.ix_(dofs, dofs)] += emat[e * ee * ee:(e + 1) * ee * ee].reshape(prob.npe, prob.npe, b, b) \
+                .transpose(0, 2, 1, 3).reshape(ee, ee)
+        want = K * (flags == 0)[:, None] * (flags != 0)[None, :]
+        Kb = np.zeros((n, n))
+        for i in range(nb):
+            for q in range(brow[i], brow[i + 1]):
+                Kb[i * b:(i + 1) * b, bcol[q] * b:(bcol[q] + 1) * b] = val[q * b * b:(q + 1) * b * b].reshape(b, b)
+        assert np.allclose(Kb, want, rtol=1e-13, atol=1e-13)

@@ -145,3 +145,31 @@ def test_complex_solvers_vs_reference(orc, ref, maxit):
         assert got[0] == exp[0] and got[2] == exp[2]
         assert got[1] == exp[1] or (np.isnan(got[1]) and np.isnan(exp[1]))
         assert np.array_equal(got[3], exp[3], equal_nan=True) and np.array_equal(got[4], exp[4], equal_nan=True)
+
+
+@pytest.mark.parametrize("kind", ["poisson2d", "solid3d"])
+def test_boundary_matrix_vs_reference(orc, ref, kind):
+    """CLinearSystem_Save's boundary matrix: the reference's CMat_BlkCrs
+    SetPatternBoundary, Mearge of every element, SetBoundaryConditionInverse_Colum
+    + SetBoundaryCondition_Row and MatVec(isnt_trans) against the oracle, bit
+    for bit."""
+    from oracle.oracle import RefBMat, RefSystem, problem_emats
+    prob = meshgen.poisson2d(8, seed=3) if kind == "poisson2d" else meshgen.solid3d(3, seed=3, sigma=0.5)
+    nb, b = prob.nnode, prob.b
+    flags = np.ascontiguousarray(prob.flags, dtype=np.int32).reshape(-1)
+    rowptr, col = orc.fem_pattern(nb, prob.conn)
+    emat, _ = problem_emats(orc, prob)
+    M = RefBMat(RefSystem(ref, nb, b, rowptr, col), flags, flags)
+    brow, bcol = orc.bmat_pattern_boundary(nb, b, rowptr, col, flags, flags)
+    rp, cl, _ = M.get()
+    assert np.array_equal(rp, brow) and np.array_equal(cl, bcol) and cl.size > 0
+    M.merge(prob.conn, emat)
+    val = orc.bmat_mearge(nb, b, brow, bcol, prob.conn, emat)
+    assert np.array_equal(M.get()[2], val)
+    M.bc(flags)
+    val = orc.bmat_bc(nb, b, brow, bcol, flags, val)
+    assert np.array_equal(M.get()[2], val)
+    rng = np.random.default_rng(2)
+    x, y = rng.standard_normal(nb * b), rng.standard_normal(nb * b)
+    for alpha, beta in ((-1.0, 1.0), (1.0, 0.0), (0.7, -0.3)):
+        assert np.array_equal(M.matvec(alpha, x, beta, y), orc.bmat_matvec(nb, b, brow, bcol, val, alpha, x, beta, y))
