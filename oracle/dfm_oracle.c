@@ -603,12 +603,17 @@ void orc_ilu_backward(unsigned nb, unsigned b, const unsigned *frowptr, const un
 
 /* ------------------------------------------------------------------------- */
 /* Solver plumbing: src/ls/linearsystem.cpp:250-415 and src/matvec/vector_blk.cpp:37-112 */
+struct orc_ms_s;
 typedef struct {
     unsigned nb, b;
     const unsigned *rowptr, *col; const double *val, *dia;
     const unsigned *frowptr, *fcol, *fdiaind; const double *fval, *fdia;
     const int *order; int *inv; double *pv;
+    struct orc_ms_s *ms;   /* several segments (then the fields above are unused) */
 } orc_sys;
+static void ms_matvec(struct orc_ms_s *m, double alpha, const double *x, double beta, double *y);
+static void ms_precond(struct orc_ms_s *m, double *z);
+static double ms_dot(struct orc_ms_s *m, const double *a, const double *c);
 
 static double v_dot(size_t n, const double *a, const double *c)
 {
@@ -629,6 +634,7 @@ static void v_scal(size_t n, double alpha, double *x)
 /* src/ls/preconditioner.cpp:301-335 (SolvePrecond; ordered branch :305-312) */
 static void precond(orc_sys *s, double *z)
 {
+    if (s->ms) { ms_precond(s->ms, z); return; }
     const unsigned b = s->b;
     if (s->order) {
         for (unsigned i = 0; i < s->nb; i++)
@@ -649,7 +655,7 @@ static void sys_init(orc_sys *s, unsigned nb, unsigned b,
 {
     s->nb = nb; s->b = b; s->rowptr = rowptr; s->col = col; s->val = val; s->dia = dia;
     s->frowptr = frowptr; s->fcol = fcol; s->fdiaind = fdiaind; s->fval = fval; s->fdia = fdia;
-    s->order = order; s->inv = NULL; s->pv = NULL;
+    s->order = order; s->inv = NULL; s->pv = NULL; s->ms = NULL;
     if (order) {
         s->inv = (int *)malloc(sizeof(int) * (size_t)nb);
         for (unsigned i = 0; i < nb; i++) s->inv[order[i]] = (int)i;
@@ -659,47 +665,47 @@ static void sys_init(orc_sys *s, unsigned nb, unsigned b,
 static void sys_free(orc_sys *s) { free(s->inv); free(s->pv); }
 static void matvec_ls(orc_sys *s, double alpha, const double *x, double beta, double *y)
 {
+    if (s->ms) { ms_matvec(s->ms, alpha, x, beta, y); return; }
     if (alpha == 0.0) { v_scal((size_t)s->nb * s->b, beta, y); return; }   /* :270-273 */
     orc_matvec(s->nb, s->b, s->rowptr, s->col, s->val, s->dia, alpha, x, beta, y);
 }
+/* CLinearSystem::DOT (:335-364): the segments' dot products added in order */
+static double s_dot(orc_sys *s, size_t n, const double *a, const double *c)
+{
+    return s->ms ? ms_dot(s->ms, a, c) : v_dot(n, a, c);
+}
 
 /* a12: src/ls/solver_ls_iter.cpp:100-172 */
-int orc_solve_pcg(unsigned nb, unsigned b,
-                  const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
-                  const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
-                  const double *fval, const double *fdia, const int *order,
-                  double *r, double *x, double *conv_ratio, unsigned *iteration)
+static int pcg_core(orc_sys *sp, size_t n, double *r, double *x, double *conv_ratio, unsigned *iteration)
 {
     const double tol = *conv_ratio;
     const unsigned mx_iter = *iteration;
-    const size_t n = (size_t)nb * b;
-    orc_sys s;
-    sys_init(&s, nb, b, rowptr, col, val, dia, frowptr, fcol, fdiaind, fval, fdia, order);
+    orc_sys s = *sp;
     double *p = (double *)calloc(n, sizeof(double));
     double *z = (double *)calloc(n, sizeof(double));
     int ret = 0;
     v_scal(n, 0.0, x);
     double inv_rr0;
     {
-        const double rr0 = v_dot(n, r, r);
+        const double rr0 = s_dot(&s, n, r, r);
         if (rr0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; ret = 1; goto done; }
         inv_rr0 = 1.0 / rr0;
     }
     memcpy(z, r, n * sizeof(double));
     precond(&s, z);
     memcpy(p, z, n * sizeof(double));
-    double rz = v_dot(n, r, z);
+    double rz = s_dot(&s, n, r, z);
     for (unsigned it = 0; it < mx_iter; it++) {
         double alpha;
         matvec_ls(&s, 1.0, p, 0.0, z);
         {
-            const double pAp = v_dot(n, p, z);
+            const double pAp = s_dot(&s, n, p, z);
             alpha = rz / pAp;
         }
         v_axpy(n, -alpha, z, r);
         v_axpy(n, alpha, p, x);
         {
-            const double rr = v_dot(n, r, r);
+            const double rr = s_dot(&s, n, r, r);
             if (rr * inv_rr0 < tol * tol) {
                 *conv_ratio = sqrt(rr * inv_rr0);
                 *iteration = it;
@@ -711,7 +717,7 @@ int orc_solve_pcg(unsigned nb, unsigned b,
         {
             memcpy(z, r, n * sizeof(double));
             precond(&s, z);
-            const double rz_new = v_dot(n, r, z);
+            const double rz_new = s_dot(&s, n, r, z);
             beta = rz_new / rz;
             rz = rz_new;
         }
@@ -719,27 +725,33 @@ int orc_solve_pcg(unsigned nb, unsigned b,
         v_axpy(n, 1.0, z, p);
     }
     {
-        const double rr = v_dot(n, r, r);
+        const double rr = s_dot(&s, n, r, r);
         *conv_ratio = sqrt(rr * inv_rr0);
         ret = 0;
     }
 done:
-    free(p); free(z); sys_free(&s);
+    free(p); free(z);
+    return ret;
+}
+int orc_solve_pcg(unsigned nb, unsigned b,
+                  const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                  const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                  const double *fval, const double *fdia, const int *order,
+                  double *r, double *x, double *conv_ratio, unsigned *iteration)
+{
+    orc_sys s;
+    sys_init(&s, nb, b, rowptr, col, val, dia, frowptr, fcol, fdiaind, fval, fdia, order);
+    const int ret = pcg_core(&s, (size_t)nb * b, r, x, conv_ratio, iteration);
+    sys_free(&s);
     return ret;
 }
 
 /* a13: src/ls/solver_ls_iter.cpp:291-408 */
-int orc_solve_pbicgstab(unsigned nb, unsigned b,
-                        const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
-                        const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
-                        const double *fval, const double *fdia, const int *order,
-                        double *r, double *x, double *conv_ratio, unsigned *iteration)
+static int bicg_core(orc_sys *sp, size_t n, double *r, double *x, double *conv_ratio, unsigned *iteration)
 {
     const double tol = *conv_ratio;
     const unsigned max_iter = *iteration;
-    const size_t n = (size_t)nb * b;
-    orc_sys s;
-    sys_init(&s, nb, b, rowptr, col, val, dia, frowptr, fcol, fdiaind, fval, fdia, order);
+    orc_sys s = *sp;
     double *vs = (double *)calloc(n, sizeof(double));
     double *Ms = (double *)calloc(n, sizeof(double));
     double *AMs = (double *)calloc(n, sizeof(double));
@@ -749,7 +761,7 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
     double *r2 = (double *)calloc(n, sizeof(double));
     double inv_rr0;
     {
-        const double rr0 = v_dot(n, r, r);
+        const double rr0 = s_dot(&s, n, r, r);
         if (rr0 < 1.0e-60) { *conv_ratio = sqrt(rr0); *iteration = 0; goto done; }
         inv_rr0 = 1.0 / rr0;
     }
@@ -760,11 +772,11 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
     for (unsigned it = 1; it < max_iter; it++) {
         memcpy(Mp, p, n * sizeof(double));
         precond(&s, Mp);
-        const double r_r2 = v_dot(n, r, r2);
+        const double r_r2 = s_dot(&s, n, r, r2);
         matvec_ls(&s, 1.0, Mp, 0.0, AMp);
         double alpha;
         {
-            const double den = v_dot(n, AMp, r2);
+            const double den = s_dot(&s, n, AMp, r2);
             alpha = r_r2 / den;
         }
         memcpy(vs, r, n * sizeof(double));
@@ -774,8 +786,8 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
         matvec_ls(&s, 1.0, Ms, 0.0, AMs);
         double omega;
         {
-            const double den = v_dot(n, AMs, AMs);
-            const double num = v_dot(n, vs, AMs);
+            const double den = s_dot(&s, n, AMs, AMs);
+            const double num = s_dot(&s, n, vs, AMs);
             omega = num / den;
         }
         v_axpy(n, alpha, Mp, x);
@@ -783,7 +795,7 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
         memcpy(r, vs, n * sizeof(double));
         v_axpy(n, -omega, AMs, r);
         {
-            const double rr = v_dot(n, r, r);
+            const double rr = s_dot(&s, n, r, r);
             const double sq = rr * inv_rr0;
             if (sq < tol * tol) {
                 *conv_ratio = sqrt(rr * inv_rr0);
@@ -793,7 +805,7 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
         }
         double beta;
         {
-            const double tmp1 = v_dot(n, r, r2);
+            const double tmp1 = s_dot(&s, n, r, r2);
             beta = tmp1 * alpha / (r_r2 * omega);
         }
         v_scal(n, beta, p);
@@ -801,8 +813,20 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
         v_axpy(n, -beta * omega, AMp, p);
     }
 done:
-    free(vs); free(Ms); free(AMs); free(p); free(Mp); free(AMp); free(r2); sys_free(&s);
+    free(vs); free(Ms); free(AMs); free(p); free(Mp); free(AMp); free(r2);
     return 1;
+}
+int orc_solve_pbicgstab(unsigned nb, unsigned b,
+                        const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                        const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                        const double *fval, const double *fdia, const int *order,
+                        double *r, double *x, double *conv_ratio, unsigned *iteration)
+{
+    orc_sys s;
+    sys_init(&s, nb, b, rowptr, col, val, dia, frowptr, fcol, fdiaind, fval, fdia, order);
+    const int ret = bicg_core(&s, (size_t)nb * b, r, x, conv_ratio, iteration);
+    sys_free(&s);
+    return ret;
 }
 
 /* a17: the matrix-layer twins MatVec::Sol::Solve_CG / Solve_PCG /
@@ -1097,6 +1121,403 @@ void orc_bmat_matvec(unsigned nb, unsigned b, const unsigned *brow, const unsign
                 for (unsigned jd = 0; jd < b; jd++) yi[id] += alpha * val[(size_t)q * bb + (size_t)id * b + jd] * xj[jd];
         }
     }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Multi-segment systems (LsSol::CLinearSystem with several segments,
+ * src/ls/linearsystem.cpp; CPreconditioner_ILU across segments,
+ * src/ls/preconditioner.cpp:57-206, 217-297, 301-335; CMatFrac_BlkCrs,
+ * src/matvec/matfrac_blkcrs.cpp).  Off-diagonal blocks are CMat_BlkCrs with
+ * rectangular blocks: rows of segment i (nbc blocks of bc dofs), columns of
+ * segment j (nbr blocks of br dofs).  The preconditioner is ILU(0) across
+ * segments (no fill between them: the reference's default). */
+
+/* CMat_BlkCrs::MatVec (mat_blkcrs.cpp:955-1176), isnt_trans = true, with the
+ * reference's shape-specific operation orders */
+void orc_bmat2_matvec(unsigned nbc, unsigned bc, unsigned nbr, unsigned br, const unsigned *rowptr,
+                      const unsigned *col, const double *val, double alpha, const double *x, double beta, double *y)
+{
+    (void)nbr;
+    const size_t bs = (size_t)bc * br;
+    for (unsigned i = 0; i < nbc; i++) {
+        double *yi = y + (size_t)i * bc;
+        if (bc == 1 && br == 1) {
+            if (beta != 0.0) yi[0] *= beta;
+            else yi[0] = 0.0;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) yi[0] += alpha * val[q] * x[col[q]];
+        } else if (bc == 1 && br == 2) {
+            yi[0] *= beta;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+                const double *xj = x + (size_t)col[q] * 2;
+                yi[0] += alpha * (val[q * 2] * xj[0] + val[q * 2 + 1] * xj[1]);
+            }
+        } else if (bc == 1 && br == 3) {
+            yi[0] *= beta;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+                const double *xj = x + (size_t)col[q] * 3;
+                yi[0] += alpha * (val[q * 3] * xj[0] + val[q * 3 + 1] * xj[1] + val[q * 3 + 2] * xj[2]);
+            }
+        } else if (bc == 1) {
+            yi[0] *= beta;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+                const double *xj = x + (size_t)col[q] * br;
+                for (unsigned jd = 0; jd < br; jd++) yi[0] += alpha * val[(size_t)q * br + jd] * xj[jd];
+            }
+        } else if (br == 1) {
+            for (unsigned id = 0; id < bc; id++) yi[id] *= beta;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+                const double ajx = x[col[q]] * alpha;
+                for (unsigned id = 0; id < bc; id++) yi[id] += val[(size_t)q * bc + id] * ajx;
+            }
+        } else {
+            for (unsigned id = 0; id < bc; id++) yi[id] *= beta;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+                const double *xj = x + (size_t)col[q] * br;
+                for (unsigned id = 0; id < bc; id++)
+                    for (unsigned jd = 0; jd < br; jd++) yi[id] += alpha * val[q * bs + (size_t)id * br + jd] * xj[jd];
+            }
+        }
+    }
+}
+/* Mearge of one rectangular element block set (mat_blkcrs.cpp:754-828):
+ * emat[(a*nrow_el + c)*bc*br] for row node a, column node c */
+void orc_bmat2_mearge(unsigned nbc, unsigned bc, unsigned nbr, unsigned br, const unsigned *rowptr,
+                      const unsigned *col, unsigned nelem, unsigned npe_col, const unsigned *conn_col,
+                      unsigned npe_row, const unsigned *conn_row, const double *emat, double *val)
+{
+    const size_t bs = (size_t)bc * br;
+    int *pos = (int *)malloc(sizeof(int) * (nbr ? nbr : 1));
+    (void)nbc;
+    for (unsigned j = 0; j < nbr; j++) pos[j] = -1;
+    for (unsigned e = 0; e < nelem; e++) {
+        const unsigned *nc = conn_col + (size_t)e * npe_col, *nr = conn_row + (size_t)e * npe_row;
+        for (unsigned a = 0; a < npe_col; a++) {
+            const unsigned i = nc[a];
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) pos[col[q]] = (int)q;
+            for (unsigned c = 0; c < npe_row; c++) {
+                const int q = pos[nr[c]];
+                if (q < 0) continue;
+                const double *in = emat + (((size_t)e * npe_col + a) * npe_row + c) * bs;
+                for (size_t t = 0; t < bs; t++) val[(size_t)q * bs + t] += in[t];
+            }
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) pos[col[q]] = -1;
+        }
+    }
+    free(pos);
+}
+/* SetBoundaryCondition_Row (:910-952) / SetBoundaryCondition_Colum (:829-875) */
+void orc_bmat2_bc(unsigned nbc, unsigned bc, unsigned nbr, unsigned br, const unsigned *rowptr, const unsigned *col,
+                  const int *flag_row_dofs, const int *flag_col_dofs, double *val)
+{
+    const size_t bs = (size_t)bc * br;
+    (void)nbr;
+    if (flag_row_dofs)
+        for (unsigned i = 0; i < nbc; i++)
+            for (unsigned id = 0; id < bc; id++) {
+                if (flag_row_dofs[(size_t)i * bc + id] == 0) continue;
+                for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++)
+                    for (unsigned jd = 0; jd < br; jd++) val[q * bs + (size_t)id * br + jd] = 0.0;
+            }
+    if (flag_col_dofs)
+        for (unsigned q = 0; q < rowptr[nbc]; q++)
+            for (unsigned jd = 0; jd < br; jd++) {
+                if (flag_col_dofs[(size_t)col[q] * br + jd] == 0) continue;
+                for (unsigned id = 0; id < bc; id++) val[q * bs + (size_t)id * br + jd] = 0.0;
+            }
+}
+/* CalcSubMatPr / CalcMatPr (matfrac_blkcrs.cpp:43-69) */
+static void rsub_mat_pr(double *out, const double *a, const double *b, unsigned ni, unsigned nk, unsigned nj)
+{
+    for (unsigned i = 0; i < ni; i++)
+        for (unsigned j = 0; j < nj; j++)
+            for (unsigned k = 0; k < nk; k++) out[i * nj + j] -= a[i * nk + k] * b[k * nj + j];
+}
+static void rmat_pr(double *out, const double *d, unsigned ni, unsigned nj)
+{
+    double tmp[64];
+    for (unsigned i = 0; i < ni; i++)
+        for (unsigned j = 0; j < nj; j++) {
+            tmp[i * nj + j] = 0.0;
+            for (unsigned k = 0; k < ni; k++) tmp[i * nj + j] += d[i * ni + k] * out[k * nj + j];
+        }
+    for (unsigned i = 0; i < ni * nj; i++) out[i] = tmp[i];
+}
+
+#define ORC_MS_MAX 4
+typedef struct {
+    unsigned nbc, bc, nbr, br;
+    const unsigned *rowptr, *col;
+    const double *val;
+    double *F;
+} orc_msblk;
+struct orc_ms_s {
+    unsigned nseg, nb[ORC_MS_MAX], b[ORC_MS_MAX];
+    size_t off[ORC_MS_MAX + 1];
+    const unsigned *rowptr[ORC_MS_MAX], *col[ORC_MS_MAX];
+    const double *val[ORC_MS_MAX], *dia[ORC_MS_MAX];
+    unsigned *diaind[ORC_MS_MAX];
+    double *fval[ORC_MS_MAX], *fdia[ORC_MS_MAX];
+    orc_msblk *nd[ORC_MS_MAX][ORC_MS_MAX];
+};
+typedef struct orc_ms_s orc_ms;
+
+void *orc_ms_create(unsigned nseg, const unsigned *nb, const unsigned *b)
+{
+    if (nseg == 0 || nseg > ORC_MS_MAX) return NULL;
+    orc_ms *m = (orc_ms *)calloc(1, sizeof(orc_ms));
+    m->nseg = nseg;
+    m->off[0] = 0;
+    for (unsigned i = 0; i < nseg; i++) {
+        m->nb[i] = nb[i];
+        m->b[i] = b[i];
+        m->off[i + 1] = m->off[i] + (size_t)nb[i] * b[i];
+    }
+    return m;
+}
+void orc_ms_free(void *p)
+{
+    orc_ms *m = (orc_ms *)p;
+    if (!m) return;
+    for (unsigned i = 0; i < m->nseg; i++) {
+        free(m->diaind[i]); free(m->fval[i]); free(m->fdia[i]);
+        for (unsigned j = 0; j < m->nseg; j++)
+            if (m->nd[i][j]) { free(m->nd[i][j]->F); free(m->nd[i][j]); }
+    }
+    free(m);
+}
+/* the arrays are borrowed: they must outlive the handle */
+void orc_ms_set_dia(void *p, unsigned i, const unsigned *rowptr, const unsigned *col, const double *val,
+                    const double *dia)
+{
+    orc_ms *m = (orc_ms *)p;
+    m->rowptr[i] = rowptr; m->col[i] = col; m->val[i] = val; m->dia[i] = dia;
+}
+void orc_ms_set_nondia(void *p, unsigned i, unsigned j, const unsigned *rowptr, const unsigned *col, const double *val)
+{
+    orc_ms *m = (orc_ms *)p;
+    orc_msblk *k = (orc_msblk *)calloc(1, sizeof(orc_msblk));
+    k->nbc = m->nb[i]; k->bc = m->b[i]; k->nbr = m->nb[j]; k->br = m->b[j];
+    k->rowptr = rowptr; k->col = col; k->val = val;
+    m->nd[i][j] = k;
+}
+/* CLinearSystem::MATVEC (:250-296) on concatenated segment vectors */
+static void ms_matvec(orc_ms *m, double alpha, const double *x, double beta, double *y)
+{
+    if (alpha == 0.0) { v_scal(m->off[m->nseg], beta, y); return; }
+    for (unsigned i = 0; i < m->nseg; i++) {
+        orc_matvec(m->nb[i], m->b[i], m->rowptr[i], m->col[i], m->val[i], m->dia[i], alpha, x + m->off[i], beta,
+                   y + m->off[i]);
+        for (unsigned j = 0; j < m->nseg; j++) {
+            const orc_msblk *k = m->nd[i][j];
+            if (!k) continue;
+            orc_bmat2_matvec(k->nbc, k->bc, k->nbr, k->br, k->rowptr, k->col, k->val, alpha, x + m->off[j], 1.0,
+                             y + m->off[i]);
+        }
+    }
+}
+void orc_ms_matvec(void *p, double alpha, const double *x, double beta, double *y)
+{
+    ms_matvec((orc_ms *)p, alpha, x, beta, y);
+}
+static double ms_dot(orc_ms *m, const double *a, const double *c)
+{
+    double s = 0.0;
+    for (unsigned i = 0; i < m->nseg; i++) s += v_dot(m->off[i + 1] - m->off[i], a + m->off[i], c + m->off[i]);
+    return s;
+}
+/* CMatFrac_BlkCrs::DoILUDecompUp (matfrac_blkcrs.cpp:601-701): rows of U_ij
+ * eliminated with the L part of factor i, then scaled by its inverted pivots */
+static void ms_up(orc_ms *m, unsigned si, orc_msblk *k)
+{
+    const unsigned bi = k->bc, bj = k->br;
+    const size_t bs = (size_t)bi * bj, bii = (size_t)bi * bi;
+    int *pos = (int *)malloc(sizeof(int) * (k->nbr ? k->nbr : 1));
+    for (unsigned j = 0; j < k->nbr; j++) pos[j] = -1;
+    for (unsigned i = 0; i < k->nbc; i++) {
+        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++) pos[k->col[q]] = (int)q;
+        for (unsigned ik = m->rowptr[si][i]; ik < m->diaind[si][i]; ik++) {
+            const unsigned kb = m->col[si][ik];
+            for (unsigned kj = k->rowptr[kb]; kj < k->rowptr[kb + 1]; kj++) {
+                const int ij = pos[k->col[kj]];
+                if (ij < 0) continue;
+                rsub_mat_pr(k->F + (size_t)ij * bs, m->fval[si] + (size_t)ik * bii, k->F + (size_t)kj * bs, bi, bi, bj);
+            }
+        }
+        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++)
+            rmat_pr(k->F + (size_t)q * bs, m->fdia[si] + (size_t)i * bii, bi, bj);
+        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++) pos[k->col[q]] = -1;
+    }
+    free(pos);
+}
+/* DoILUDecompLow (:703-783): rows of L_ij eliminated with the U part of factor j */
+static void ms_low(orc_ms *m, unsigned sj, orc_msblk *k)
+{
+    const unsigned bi = k->bc, bj = k->br;
+    const size_t bs = (size_t)bi * bj, bjj = (size_t)bj * bj;
+    int *pos = (int *)malloc(sizeof(int) * (k->nbr ? k->nbr : 1));
+    for (unsigned j = 0; j < k->nbr; j++) pos[j] = -1;
+    for (unsigned i = 0; i < k->nbc; i++) {
+        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++) pos[k->col[q]] = (int)q;
+        for (unsigned ik = k->rowptr[i]; ik < k->rowptr[i + 1]; ik++) {
+            const unsigned kb = k->col[ik];
+            for (unsigned kj = m->diaind[sj][kb]; kj < m->rowptr[sj][kb + 1]; kj++) {
+                const int ij = pos[m->col[sj][kj]];
+                if (ij < 0) continue;
+                rsub_mat_pr(k->F + (size_t)ij * bs, k->F + (size_t)ik * bs, m->fval[sj] + (size_t)kj * bjj, bi, bj, bj);
+            }
+        }
+        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++) pos[k->col[q]] = -1;
+    }
+    free(pos);
+}
+/* CMatFrac_BlkCrs::DoILUDecompLowUp (:540-599): A_ij -= L_ik U_kj on A_ij's pattern */
+static void ms_lowup(orc_msblk *t, const orc_msblk *lo, const orc_msblk *up)
+{
+    const unsigned bi = t->bc, bj = t->br, bk = lo->br;
+    int *pos = (int *)malloc(sizeof(int) * (t->nbr ? t->nbr : 1));
+    for (unsigned j = 0; j < t->nbr; j++) pos[j] = -1;
+    for (unsigned i = 0; i < t->nbc; i++) {
+        for (unsigned q = t->rowptr[i]; q < t->rowptr[i + 1]; q++) pos[t->col[q]] = (int)q;
+        for (unsigned ik = lo->rowptr[i]; ik < lo->rowptr[i + 1]; ik++) {
+            const unsigned kb = lo->col[ik];
+            for (unsigned kj = up->rowptr[kb]; kj < up->rowptr[kb + 1]; kj++) {
+                const int ij = pos[up->col[kj]];
+                if (ij < 0) continue;
+                rsub_mat_pr(t->F + (size_t)ij * bi * bj, lo->F + (size_t)ik * bi * bk, up->F + (size_t)kj * bk * bj, bi,
+                           bk, bj);
+            }
+        }
+        for (unsigned q = t->rowptr[i]; q < t->rowptr[i + 1]; q++) pos[t->col[q]] = -1;
+    }
+    free(pos);
+}
+/* CMatDiaFrac_BlkCrs::DoILUDecompLowUp (matdiafrac_blkcrs.cpp:665-767): the
+ * diagonal factor of segment i, before its own factorisation, -= L_ik U_ki */
+static void ms_lowup_dia(orc_ms *m, unsigned si, const orc_msblk *lo, const orc_msblk *up)
+{
+    const unsigned bi = m->b[si], bk = lo->br, nb = m->nb[si];
+    const size_t bii = (size_t)bi * bi;
+    int *pos = (int *)malloc(sizeof(int) * (nb ? nb : 1));
+    for (unsigned j = 0; j < nb; j++) pos[j] = -1;
+    for (unsigned i = 0; i < nb; i++) {
+        for (unsigned q = m->rowptr[si][i]; q < m->rowptr[si][i + 1]; q++) pos[m->col[si][q]] = (int)q;
+        for (unsigned ik = lo->rowptr[i]; ik < lo->rowptr[i + 1]; ik++) {
+            const unsigned kb = lo->col[ik];
+            for (unsigned kj = up->rowptr[kb]; kj < up->rowptr[kb + 1]; kj++) {
+                const unsigned j = up->col[kj];
+                const double *a = lo->F + (size_t)ik * bi * bk, *c = up->F + (size_t)kj * bk * bi;
+                if (j != i) {
+                    const int ij = pos[j];
+                    if (ij < 0) continue;
+                    rsub_mat_pr(m->fval[si] + (size_t)ij * bii, a, c, bi, bk, bi);
+                } else {
+                    rsub_mat_pr(m->fdia[si] + (size_t)i * bii, a, c, bi, bk, bi);
+                }
+            }
+        }
+        for (unsigned q = m->rowptr[si][i]; q < m->rowptr[si][i + 1]; q++) pos[m->col[si][q]] = -1;
+    }
+    free(pos);
+}
+/* CPreconditioner_ILU::SetValue across segments (preconditioner.cpp:217-297):
+ * copies first, then blocks in (i, j) order.  Returns 0, or -1 when a
+ * diagonal factorisation met more than 10 singular pivots. */
+int orc_ms_ilu(void *p)
+{
+    orc_ms *m = (orc_ms *)p;
+    const unsigned n = m->nseg;
+    for (unsigned i = 0; i < n; i++) {
+        const unsigned nb = m->nb[i], bb = m->b[i] * m->b[i];
+        const unsigned ncrs = m->rowptr[i][nb];
+        free(m->diaind[i]); free(m->fval[i]); free(m->fdia[i]);
+        m->diaind[i] = (unsigned *)malloc(sizeof(unsigned) * (nb ? nb : 1));
+        for (unsigned r = 0; r < nb; r++) {
+            unsigned q = m->rowptr[i][r];
+            while (q < m->rowptr[i][r + 1] && m->col[i][q] < r) q++;
+            m->diaind[i][r] = q;
+        }
+        m->fval[i] = (double *)calloc((size_t)ncrs * bb + 1, sizeof(double));
+        m->fdia[i] = (double *)calloc((size_t)nb * bb + 1, sizeof(double));
+        orc_ilu_setvalue(nb, m->b[i], m->rowptr[i], m->col[i], m->val[i], m->dia[i], NULL, m->rowptr[i], m->col[i],
+                         m->fval[i], m->fdia[i]);
+        for (unsigned j = 0; j < n; j++) {
+            orc_msblk *k = m->nd[i][j];
+            if (!k) continue;
+            const size_t nv = (size_t)k->rowptr[k->nbc] * k->bc * k->br;
+            free(k->F);
+            k->F = (double *)malloc(sizeof(double) * (nv + 1));
+            memcpy(k->F, k->val, sizeof(double) * nv);
+        }
+    }
+    for (unsigned i = 0; i < n; i++)
+        for (unsigned j = 0; j < n; j++) {
+            if (i == j) {
+                for (unsigned kk = 0; kk < i; kk++)
+                    if (m->nd[i][kk] && m->nd[kk][i]) ms_lowup_dia(m, i, m->nd[i][kk], m->nd[kk][i]);
+                if (orc_ilu_decomp(m->nb[i], m->b[i], m->rowptr[i], m->col[i], m->diaind[i], m->fval[i], m->fdia[i]) < 0)
+                    return -1;
+                continue;
+            }
+            orc_msblk *k = m->nd[i][j];
+            if (!k) continue;
+            const unsigned kmax = i < j ? i : j;
+            for (unsigned kk = 0; kk < kmax + 1; kk++) {
+                if (kk == i && kk < j) { ms_up(m, i, k); continue; }
+                if (kk == j && kk < i) { ms_low(m, j, k); continue; }
+                if (!m->nd[i][kk] || !m->nd[kk][j]) continue;
+                ms_lowup(k, m->nd[i][kk], m->nd[kk][j]);
+            }
+        }
+    return 0;
+}
+/* factor values: segment i's diagonal factor (fval ncrs*b*b, fdia nb*b*b) or
+ * the off-diagonal block (i, j) (fval only) */
+void orc_ms_get_factor(void *p, unsigned i, unsigned j, double *fval, double *fdia)
+{
+    orc_ms *m = (orc_ms *)p;
+    if (i == j) {
+        const size_t bb = (size_t)m->b[i] * m->b[i];
+        memcpy(fval, m->fval[i], sizeof(double) * m->rowptr[i][m->nb[i]] * bb);
+        memcpy(fdia, m->fdia[i], sizeof(double) * m->nb[i] * bb);
+    } else if (m->nd[i][j]) {
+        const orc_msblk *k = m->nd[i][j];
+        memcpy(fval, k->F, sizeof(double) * k->rowptr[k->nbc] * k->bc * k->br);
+    }
+}
+/* SolvePrecond across segments (preconditioner.cpp:301-335) */
+static void ms_precond(orc_ms *m, double *z)
+{
+    const unsigned n = m->nseg;
+    for (unsigned i = 0; i < n; i++) {
+        for (unsigned j = 0; j < i; j++) {
+            const orc_msblk *k = m->nd[i][j];
+            if (!k) continue;
+            orc_bmat2_matvec(k->nbc, k->bc, k->nbr, k->br, k->rowptr, k->col, k->F, -1.0, z + m->off[j], 1.0,
+                             z + m->off[i]);
+        }
+        orc_ilu_forward(m->nb[i], m->b[i], m->rowptr[i], m->col[i], m->diaind[i], m->fval[i], m->fdia[i], z + m->off[i]);
+    }
+    for (int i = (int)n - 1; i >= 0; i--) {
+        for (unsigned j = (unsigned)i + 1; j < n; j++) {
+            const orc_msblk *k = m->nd[i][j];
+            if (!k) continue;
+            orc_bmat2_matvec(k->nbc, k->bc, k->nbr, k->br, k->rowptr, k->col, k->F, -1.0, z + m->off[j], 1.0,
+                             z + m->off[i]);
+        }
+        orc_ilu_backward(m->nb[i], m->b[i], m->rowptr[i], m->col[i], m->diaind[i], m->fval[i], m->fdia[i],
+                         z + m->off[i]);
+    }
+}
+void orc_ms_precond(void *p, double *z) { ms_precond((orc_ms *)p, z); }
+/* LsSol::Solve_PCG / Solve_PBiCGSTAB on the multi-segment system */
+int orc_ms_solve(void *p, int bicg, double *r, double *x, double *conv_ratio, unsigned *iteration)
+{
+    orc_ms *m = (orc_ms *)p;
+    orc_sys s;
+    memset(&s, 0, sizeof s);
+    s.ms = m;
+    return bicg ? bicg_core(&s, m->off[m->nseg], r, x, conv_ratio, iteration)
+                : pcg_core(&s, m->off[m->nseg], r, x, conv_ratio, iteration);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -155,6 +155,23 @@ void orc_bmat_bc_inverse_col(unsigned nb, unsigned b, const unsigned *brow, cons
 void orc_bmat_bc_row(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, const int *flag, double *val);
 void orc_bmat_matvec(unsigned nb, unsigned b, const unsigned *brow, const unsigned *bcol, const double *val,
                      double alpha, const double *x, double beta, double *y);
+void orc_bmat2_matvec(unsigned nbc, unsigned bc, unsigned nbr, unsigned br, const unsigned *rowptr,
+                      const unsigned *col, const double *val, double alpha, const double *x, double beta, double *y);
+void orc_bmat2_mearge(unsigned nbc, unsigned bc, unsigned nbr, unsigned br, const unsigned *rowptr,
+                      const unsigned *col, unsigned nelem, unsigned npe_col, const unsigned *conn_col,
+                      unsigned npe_row, const unsigned *conn_row, const double *emat, double *val);
+void orc_bmat2_bc(unsigned nbc, unsigned bc, unsigned nbr, unsigned br, const unsigned *rowptr, const unsigned *col,
+                  const int *flag_row_dofs, const int *flag_col_dofs, double *val);
+void *orc_ms_create(unsigned nseg, const unsigned *nb, const unsigned *b);
+void orc_ms_free(void *p);
+void orc_ms_set_dia(void *p, unsigned i, const unsigned *rowptr, const unsigned *col, const double *val,
+                    const double *dia);
+void orc_ms_set_nondia(void *p, unsigned i, unsigned j, const unsigned *rowptr, const unsigned *col, const double *val);
+void orc_ms_matvec(void *p, double alpha, const double *x, double beta, double *y);
+int orc_ms_ilu(void *p);
+void orc_ms_get_factor(void *p, unsigned i, unsigned j, double *fval, double *fdia);
+void orc_ms_precond(void *p, double *z);
+int orc_ms_solve(void *p, int bicg, double *r, double *x, double *conv_ratio, unsigned *iteration);
 int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
                      const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const double *fval,
                      const double *fdia, const int *order, double *r, double *u, double *conv_ratio,

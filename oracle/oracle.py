@@ -96,6 +96,21 @@ class Orc:
         L.orc_bmat_bc_inverse_col.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp]
         L.orc_bmat_bc_row.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp]
         L.orc_bmat_matvec.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, C.c_double, vp, C.c_double, vp]
+        L.orc_bmat2_matvec.argtypes = [C.c_uint] * 4 + [vp, vp, vp, C.c_double, vp, C.c_double, vp]
+        L.orc_bmat2_mearge.argtypes = [C.c_uint] * 4 + [vp, vp, C.c_uint, C.c_uint, vp, C.c_uint, vp, vp, vp]
+        L.orc_bmat2_bc.argtypes = [C.c_uint] * 4 + [vp, vp, vp, vp, vp]
+        L.orc_ms_create.restype = vp
+        L.orc_ms_create.argtypes = [C.c_uint, vp, vp]
+        L.orc_ms_free.argtypes = [vp]
+        L.orc_ms_set_dia.argtypes = [vp, C.c_uint, vp, vp, vp, vp]
+        L.orc_ms_set_nondia.argtypes = [vp, C.c_uint, C.c_uint, vp, vp, vp]
+        L.orc_ms_matvec.argtypes = [vp, C.c_double, vp, C.c_double, vp]
+        L.orc_ms_ilu.restype = C.c_int
+        L.orc_ms_ilu.argtypes = [vp]
+        L.orc_ms_get_factor.argtypes = [vp, C.c_uint, C.c_uint, vp, vp]
+        L.orc_ms_precond.argtypes = [vp, vp]
+        L.orc_ms_solve.restype = C.c_int
+        L.orc_ms_solve.argtypes = [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zsolve_cocg.restype = C.c_int
         L.orc_zsolve_cocg.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zsolve_bicgstab.restype = C.c_int
@@ -476,6 +491,20 @@ class Ref:
             "ref_zsolve_pcocg": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_zsolve_cocg": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_bmat_create_boundary": (vp, [vp, vp, vp]),
+            "ref_ms_create": (vp, [C.c_uint, vp, vp]),
+            "ref_ms_free": (None, [vp]),
+            "ref_ms_add_pattern": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
+            "ref_ms_initialize": (None, [vp]),
+            "ref_ms_merge": (None, [vp, C.c_uint, C.c_uint, C.c_uint, C.c_uint, vp, vp]),
+            "ref_ms_set_bc": (None, [vp, C.c_uint, vp]),
+            "ref_ms_set_vector": (None, [vp, C.c_int, vp]),
+            "ref_ms_get_vector": (None, [vp, C.c_int, vp]),
+            "ref_ms_finalize": (C.c_double, [vp]),
+            "ref_ms_get_block": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
+            "ref_ms_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
+            "ref_ms_ilu": (C.c_int, [vp]),
+            "ref_ms_precond": (None, [vp, vp]),
+            "ref_ms_solve": (C.c_int, [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_bmat_free": (None, [vp]),
             "ref_bmat_ncrs": (C.c_uint, [vp]),
             "ref_bmat_get": (None, [vp, vp, vp, vp]),
@@ -637,6 +666,187 @@ class RefFrac:
         it = C.c_uint(maxit)
         ok = self.s.ref.L.ref_frac_solve(self.h, self.s.h, 0 if kind == "pcg" else 1, C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value
+
+
+def split_problem(orc, prob, groups):
+    """A combined-unknown Problem (b dofs per node) split into segments of the
+    given dof groups (e.g. [[0, 1], [2]]: velocity and pressure) the way the
+    reference's split fields assemble it: per block (i, j) the element
+    matrices restricted to the dofs of groups i and j; diagonal patterns are
+    the node adjacency, off-diagonal ones the adjacency with the node itself.
+    Returns dict(nb, b=[...], rowptr/col for 'dia' and 'nd', emat[(i, j)],
+    res (concatenated, before the Dirichlet rows), flags[i])."""
+    emat, eres = problem_emats(orc, prob)
+    nb, B, npe, ne = prob.nnode, prob.b, prob.npe, prob.nelem
+    em = emat.reshape(ne, npe, npe, B, B)
+    er = eres.reshape(ne, npe, B)
+    rowptr, col = orc.fem_pattern(nb, prob.conn)
+    # off-diagonal pattern: adjacency plus the node itself, ascending
+    rp2 = np.zeros(nb + 1, dtype=np.uint32)
+    cols = []
+    for i in range(nb):
+        c = np.sort(np.append(col[rowptr[i]:rowptr[i + 1]], i)).astype(np.uint32)
+        cols.append(c)
+        rp2[i + 1] = rp2[i] + c.size
+    col2 = np.concatenate(cols).astype(np.uint32)
+    flags = np.ascontiguousarray(prob.flags, dtype=np.int32).reshape(nb, B)
+    out = dict(nb=nb, b=[len(g) for g in groups], rowptr=rowptr, col=col, rowptr_nd=rp2, col_nd=col2, emat={},
+               flags=[np.ascontiguousarray(flags[:, g]).reshape(-1) for g in groups])
+    for i, gi in enumerate(groups):
+        for j, gj in enumerate(groups):
+            out["emat"][(i, j)] = np.ascontiguousarray(em[:, :, :, gi][:, :, :, :, gj]).reshape(-1)
+    res = np.zeros(nb * B)
+    res = orc.assemble_vec(nb, B, prob.conn, eres).reshape(nb, B)
+    out["res"] = np.concatenate([np.ascontiguousarray(res[:, g]).reshape(-1) for g in groups])
+    return out
+
+
+class OrcMS:
+    """The oracle's multi-segment system: blocks assembled (Mearge of the
+    split element matrices), Dirichlet conditions as CLinearSystem::
+    FinalizeMarge sets them, MATVEC, cross-segment ILU(0), SolvePrecond and
+    the Krylov solvers."""
+
+    def __init__(self, orc, sp, conn):
+        self.orc, self.sp = orc, sp
+        L = orc.L
+        nseg = len(sp["b"])
+        nb = sp["nb"]
+        self.nseg, self.nb, self.b = nseg, nb, sp["b"]
+        self.keep = []
+        self.val, self.dia = {}, {}
+        conn = u32(conn)
+        for i in range(nseg):
+            bi = self.b[i]
+            val, dia = orc.assemble(nb, bi, sp["rowptr"], sp["col"], conn, sp["emat"][(i, i)])
+            val, dia = orc.set_bc(nb, bi, sp["rowptr"], sp["col"], val, dia, sp["flags"][i])
+            self.val[(i, i)], self.dia[i] = val, dia
+            for j in range(nseg):
+                if i == j:
+                    continue
+                bj = self.b[j]
+                v = np.zeros(int(sp["rowptr_nd"][-1]) * bi * bj)
+                L.orc_bmat2_mearge(nb, bi, nb, bj, _p(u32(sp["rowptr_nd"])), _p(u32(sp["col_nd"])), conn.shape[0],
+                                   conn.shape[1], _p(conn), conn.shape[1], _p(conn), _p(f64(sp["emat"][(i, j)])), _p(v))
+                L.orc_bmat2_bc(nb, bi, nb, bj, _p(u32(sp["rowptr_nd"])), _p(u32(sp["col_nd"])),
+                               _p(i32(sp["flags"][i])), _p(i32(sp["flags"][j])), _p(v))
+                self.val[(i, j)] = v
+        self.nbs = np.array([nb] * nseg, dtype=np.uint32)
+        self.bs = np.array(self.b, dtype=np.uint32)
+        self.h = L.orc_ms_create(nseg, _p(self.nbs), _p(self.bs))
+        rp, cl = u32(sp["rowptr"]), u32(sp["col"])
+        rp2, cl2 = u32(sp["rowptr_nd"]), u32(sp["col_nd"])
+        self.keep += [rp, cl, rp2, cl2]
+        for i in range(nseg):
+            L.orc_ms_set_dia(self.h, i, _p(rp), _p(cl), _p(self.val[(i, i)]), _p(self.dia[i]))
+            for j in range(nseg):
+                if i != j:
+                    L.orc_ms_set_nondia(self.h, i, j, _p(rp2), _p(cl2), _p(self.val[(i, j)]))
+        flags = np.concatenate(sp["flags"])
+        self.res = np.where(flags != 0, 0.0, sp["res"])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.orc.L.orc_ms_free(self.h)
+
+    def matvec(self, alpha, x, beta, y):
+        y = f64(y).copy()
+        self.orc.L.orc_ms_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
+        return y
+
+    def ilu(self):
+        return self.orc.L.orc_ms_ilu(self.h) == 0
+
+    def factor(self, i, j):
+        bb = self.b[i] * self.b[j]
+        if i == j:
+            fv = np.zeros(int(self.sp["rowptr"][-1]) * bb)
+            fd = np.zeros(self.nb * bb)
+            self.orc.L.orc_ms_get_factor(self.h, i, j, _p(fv), _p(fd))
+            return fv, fd
+        fv = np.zeros(int(self.sp["rowptr_nd"][-1]) * bb)
+        self.orc.L.orc_ms_get_factor(self.h, i, j, _p(fv), None)
+        return fv
+
+    def precond(self, z):
+        z = f64(z).copy()
+        self.orc.L.orc_ms_precond(self.h, _p(z))
+        return z
+
+    def solve(self, bicg, rhs, tol, maxit):
+        r = f64(rhs).copy()
+        x = np.zeros_like(r)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.orc.L.orc_ms_solve(self.h, 1 if bicg else 0, _p(r), _p(x), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, x, r
+
+
+class RefMS:
+    """The reference's multi-segment LsSol::CLinearSystem assembled from a
+    split problem, with CPreconditioner_ILU(0) across the segments."""
+
+    def __init__(self, ref, sp, conn):
+        self.ref, self.sp = ref, sp
+        L = ref.L
+        nseg = len(sp["b"])
+        self.nbs = np.array([sp["nb"]] * nseg, dtype=np.uint32)
+        self.bs = np.array(sp["b"], dtype=np.uint32)
+        self.h = L.ref_ms_create(nseg, _p(self.nbs), _p(self.bs))
+        for i in range(nseg):
+            for j in range(nseg):
+                if i == j:
+                    L.ref_ms_add_pattern(self.h, i, j, _p(u32(sp["rowptr"])), _p(u32(sp["col"])))
+                else:
+                    L.ref_ms_add_pattern(self.h, i, j, _p(u32(sp["rowptr_nd"])), _p(u32(sp["col_nd"])))
+        L.ref_ms_initialize(self.h)
+        conn = u32(conn)
+        for i in range(nseg):
+            for j in range(nseg):
+                L.ref_ms_merge(self.h, i, j, conn.shape[0], conn.shape[1], _p(conn), _p(f64(sp["emat"][(i, j)])))
+        L.ref_ms_set_vector(self.h, -1, _p(f64(sp["res"])))
+        for i in range(nseg):
+            L.ref_ms_set_bc(self.h, i, _p(i32(sp["flags"][i])))
+        self.norm_res = L.ref_ms_finalize(self.h)
+        self.nseg = nseg
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.ref.L.ref_ms_free(self.h)
+
+    def block(self, i, j):
+        nb, bb = self.sp["nb"], self.sp["b"][i] * self.sp["b"][j]
+        n = int((self.sp["rowptr"] if i == j else self.sp["rowptr_nd"])[-1])
+        val = np.zeros(n * bb)
+        dia = np.zeros(nb * bb) if i == j else None
+        self.ref.L.ref_ms_get_block(self.h, i, j, _p(val), _p(dia))
+        return (val, dia) if i == j else val
+
+    def vector(self, iv):
+        v = np.zeros(sum(self.sp["b"]) * self.sp["nb"])
+        self.ref.L.ref_ms_get_vector(self.h, iv, _p(v))
+        return v
+
+    def matvec(self, alpha, x, beta, y):
+        y = f64(y).copy()
+        self.ref.L.ref_ms_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
+        return y
+
+    def ilu(self):
+        return bool(self.ref.L.ref_ms_ilu(self.h))
+
+    def precond(self, z):
+        z = f64(z).copy()
+        self.ref.L.ref_ms_precond(self.h, _p(z))
+        return z
+
+    def solve(self, bicg, rhs, tol, maxit):
+        r = f64(rhs).copy()
+        x = np.zeros_like(r)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_ms_solve(self.h, 1 if bicg else 0, _p(r), _p(x), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, x, r
 
 
 class RefBMat:

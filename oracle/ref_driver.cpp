@@ -456,6 +456,125 @@ int ref_zsolve_pcocg(void* p, void* f, double* r, double* u, double* conv, unsig
   return ok ? 1 : 0;
 }
 
+// ---- multi-segment LsSol::CLinearSystem --------------------------------------
+struct RefMS {
+  LsSol::CLinearSystem ls;
+  std::vector<unsigned> nb, b;
+  LsSol::CPreconditioner_ILU* prec = nullptr;
+  ~RefMS() { delete prec; }
+};
+static Com::CIndexedArray make_crs(unsigned nrow, const unsigned* index, const unsigned* array) {
+  Com::CIndexedArray crs;
+  crs.InitializeSize(nrow);
+  for (unsigned i = 0; i <= nrow; i++) crs.index[i] = index[i];
+  crs.array.assign(array, array + index[nrow]);
+  return crs;
+}
+void* ref_ms_create(unsigned nseg, const unsigned* nb, const unsigned* b) {
+  RefMS* m = new RefMS;
+  for (unsigned i = 0; i < nseg; i++) {
+    m->nb.push_back(nb[i]);
+    m->b.push_back(b[i]);
+    m->ls.AddLinSysSeg(nb[i], b[i]);
+  }
+  return m;
+}
+void ref_ms_free(void* p) { delete static_cast<RefMS*>(p); }
+void ref_ms_add_pattern(void* p, unsigned i, unsigned j, const unsigned* index, const unsigned* array) {
+  RefMS* m = static_cast<RefMS*>(p);
+  const Com::CIndexedArray crs = make_crs(m->nb[i], index, array);
+  if (i == j) m->ls.AddMat_Dia(i, crs);
+  else m->ls.AddMat_NonDia(i, j, crs);
+}
+void ref_ms_initialize(void* p) { static_cast<RefMS*>(p)->ls.InitializeMarge(); }
+// Mearge of every element into block (i, j): emat[e][a][c] blocks of b_i x b_j
+void ref_ms_merge(void* p, unsigned i, unsigned j, unsigned nelem, unsigned npe, const unsigned* conn,
+                  const double* emat) {
+  RefMS* m = static_cast<RefMS*>(p);
+  const unsigned bs = m->b[i] * m->b[j];
+  for (unsigned e = 0; e < nelem; e++) {
+    const unsigned* no = conn + (size_t)e * npe;
+    const double* em = emat + (size_t)e * npe * npe * bs;
+    if (i == j) m->ls.GetMatrix(i).Mearge(npe, no, npe, no, bs, em);
+    else m->ls.GetMatrix(i, j).Mearge(npe, no, npe, no, bs, em);
+  }
+}
+void ref_ms_set_bc(void* p, unsigned i, const int* flags) {
+  RefMS* m = static_cast<RefMS*>(p);
+  for (unsigned k = 0; k < m->nb[i]; k++)
+    for (unsigned d = 0; d < m->b[i]; d++)
+      if (flags[(size_t)k * m->b[i] + d]) m->ls.GetBCFlag(i).SetBC(k, d);
+}
+// residual (vector id -1) from / to a concatenated vector
+void ref_ms_set_vector(void* p, int iv, const double* src) {
+  RefMS* m = static_cast<RefMS*>(p);
+  size_t off = 0;
+  for (unsigned i = 0; i < m->nb.size(); i++) {
+    copy_vec_in(m->ls.GetVector(iv, i), m->nb[i], m->b[i], src + off);
+    off += (size_t)m->nb[i] * m->b[i];
+  }
+}
+void ref_ms_get_vector(void* p, int iv, double* dst) {
+  RefMS* m = static_cast<RefMS*>(p);
+  size_t off = 0;
+  for (unsigned i = 0; i < m->nb.size(); i++) {
+    copy_vec_out(m->ls.GetVector(iv, i), m->nb[i], m->b[i], dst + off);
+    off += (size_t)m->nb[i] * m->b[i];
+  }
+}
+double ref_ms_finalize(void* p) { return static_cast<RefMS*>(p)->ls.FinalizeMarge(); }
+// values of block (i, j) in CRS order (dia: off-diagonal part + diagonal blocks)
+void ref_ms_get_block(void* p, unsigned i, unsigned j, double* val, double* dia) {
+  RefMS* m = static_cast<RefMS*>(p);
+  const MatVec::CMat_BlkCrs& A = i == j ? static_cast<const MatVec::CMat_BlkCrs&>(m->ls.GetMatrix(i))
+                                        : m->ls.GetMatrix(i, j);
+  const unsigned bs = m->b[i] * m->b[j];
+  size_t n = 0;
+  for (unsigned k = 0; k < m->nb[i]; k++) {
+    unsigned np = 0;
+    const double* v = A.GetPtrValPSuP(k, np);
+    std::memcpy(val + n * bs, v, sizeof(double) * bs * np);
+    n += np;
+  }
+  if (i == j && dia)
+    for (unsigned k = 0; k < m->nb[i]; k++)
+      std::memcpy(dia + (size_t)k * bs, m->ls.GetMatrix(i).GetPtrValDia(k), sizeof(double) * bs);
+}
+void ref_ms_matvec(void* p, double alpha, const double* x, double beta, double* y) {
+  RefMS* m = static_cast<RefMS*>(p);
+  m->ls.ReSizeTmpVecSolver(2);
+  ref_ms_set_vector(p, 0, x);
+  ref_ms_set_vector(p, 1, y);
+  m->ls.MATVEC(alpha, 0, beta, 1);
+  ref_ms_get_vector(p, 1, y);
+}
+// CPreconditioner_ILU(ls, 0) across the segments, SetValue; returns its bool
+int ref_ms_ilu(void* p) {
+  RefMS* m = static_cast<RefMS*>(p);
+  delete m->prec;
+  m->prec = new LsSol::CPreconditioner_ILU;
+  m->prec->SetFillInLevel(0);
+  m->prec->SetLinearSystem(m->ls);
+  return m->prec->SetValue(m->ls) ? 1 : 0;
+}
+void ref_ms_precond(void* p, double* z) {
+  RefMS* m = static_cast<RefMS*>(p);
+  m->ls.ReSizeTmpVecSolver(1);
+  ref_ms_set_vector(p, 0, z);
+  m->prec->SolvePrecond(m->ls, 0);
+  ref_ms_get_vector(p, 0, z);
+}
+// LsSol::Solve_PCG / Solve_PBiCGSTAB: r in = right-hand side, out = residual; x out
+int ref_ms_solve(void* p, int bicg, double* r, double* x, double* conv, unsigned* iter) {
+  RefMS* m = static_cast<RefMS*>(p);
+  ref_ms_set_vector(p, -1, r);
+  LsSol::CLinearSystemPreconditioner lsp(m->ls, *m->prec);
+  const bool ok = bicg ? LsSol::Solve_PBiCGSTAB(*conv, *iter, lsp) : LsSol::Solve_PCG(*conv, *iter, lsp);
+  ref_ms_get_vector(p, -1, r);
+  ref_ms_get_vector(p, -2, x);
+  return ok ? 1 : 0;
+}
+
 // ---- CMat_BlkCrs as CLinearSystem_Save's boundary matrix ---------------------
 struct RefBMat {
   MatVec::CMat_BlkCrs* M = nullptr;

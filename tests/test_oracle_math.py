@@ -526,3 +526,34 @@ def test_boundary_matrix_is_free_fixed_block(orc):
             for q in range(brow[i], brow[i + 1]):
                 Kb[i * b:(i + 1) * b, bcol[q] * b:(bcol[q] + 1) * b] = val[q * b * b:(q + 1) * b * b].reshape(b, b)
         assert np.allclose(Kb, want, rtol=1e-13, atol=1e-13)
+
+
+def test_multisegment_is_permuted_combined_system(orc):
+    """A split system is the combined one with its unknowns regrouped: the
+    segment MATVEC equals the dense combined product, and the split PCG
+    solution equals the direct solution of the combined system; with every
+    segment a whole field, the cross-segment ILU(0) equals the combined
+    ILU(0) only where no coupling is dropped, so only the solution is compared."""
+    from oracle.oracle import OrcMS, assemble_problem, split_problem
+    prob = meshgen.solid3d(2, seed=4, sigma=0.3)
+    groups = [[0, 1], [2]]
+    sp = split_problem(orc, prob, groups)
+    O = OrcMS(orc, sp, prob.conn)
+    nb, B = prob.nnode, prob.b
+    perm = np.concatenate([(np.arange(nb)[:, None] * B + np.array(g)[None, :]).reshape(-1) for g in groups])
+    ref = assemble_problem(orc, prob)
+    n = nb * B
+    A = np.zeros((n, n))
+    for i in range(nb):
+        A[i * B:(i + 1) * B, i * B:(i + 1) * B] = ref["dia"][i * B * B:(i + 1) * B * B].reshape(B, B)
+        for q in range(ref["rowptr"][i], ref["rowptr"][i + 1]):
+            j = ref["col"][q]
+            A[i * B:(i + 1) * B, j * B:(j + 1) * B] = ref["val"][q * B * B:(q + 1) * B * B].reshape(B, B)
+    Ap = A[np.ix_(perm, perm)]
+    x = np.random.default_rng(3).standard_normal(n)
+    assert np.allclose(O.matvec(1.0, x, 0.0, np.zeros(n)), Ap @ x, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(O.res, ref["res"][perm])
+    assert O.ilu()
+    ok, conv, it, u, r = O.solve(False, O.res, 1e-12, 2000)
+    xd = np.linalg.solve(Ap, O.res)
+    assert ok and np.linalg.norm(u - xd) <= 1e-9 * np.linalg.norm(xd)

@@ -173,3 +173,38 @@ def test_boundary_matrix_vs_reference(orc, ref, kind):
     x, y = rng.standard_normal(nb * b), rng.standard_normal(nb * b)
     for alpha, beta in ((-1.0, 1.0), (1.0, 0.0), (0.7, -0.3)):
         assert np.array_equal(M.matvec(alpha, x, beta, y), orc.bmat_matvec(nb, b, brow, bcol, val, alpha, x, beta, y))
+
+
+MS_CASES = {
+    "solid3d_uv_w": (lambda: meshgen.solid3d(3, seed=3, sigma=0.5), [[0, 1], [2]], False),
+    "solid3d_u_v_w": (lambda: meshgen.solid3d(3, seed=3, sigma=0.5), [[0], [1], [2]], False),
+    "oseen2d_uv_p": (lambda: meshgen.oseen2d(8, seed=3), [[0, 1], [2]], True),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MS_CASES))
+def test_multisegment_vs_reference(orc, ref, name):
+    """Multi-segment LsSol::CLinearSystem (split fields): per-block Mearge and
+    FinalizeMarge, MATVEC, CPreconditioner_ILU(0) across the segments (one
+    SolvePrecond) and the whole PCG / BiCGSTAB run, bit for bit."""
+    from oracle.oracle import OrcMS, RefMS, split_problem
+    make, groups, bicg = MS_CASES[name]
+    prob = make()
+    sp = split_problem(orc, prob, groups)
+    O, R = OrcMS(orc, sp, prob.conn), RefMS(ref, sp, prob.conn)
+    for i in range(len(groups)):
+        for j in range(len(groups)):
+            if i == j:
+                v, d = R.block(i, i)
+                assert np.array_equal(v, O.val[(i, i)]) and np.array_equal(d, O.dia[i])
+            else:
+                assert np.array_equal(R.block(i, j), O.val[(i, j)])
+    assert np.array_equal(R.vector(-1), O.res)
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal(O.res.size), rng.standard_normal(O.res.size)
+    assert np.array_equal(R.matvec(-0.7, x, 0.3, y), O.matvec(-0.7, x, 0.3, y))
+    assert R.ilu() and O.ilu()
+    assert np.array_equal(R.precond(x), O.precond(x))
+    a, b = R.solve(bicg, O.res, 1e-10, 3000), O.solve(bicg, O.res, 1e-10, 3000)
+    assert a[:3] == b[:3] and a[0]
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
