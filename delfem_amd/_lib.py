@@ -123,6 +123,24 @@ _SIGS = {
     "dfm_bmat_set_bc_row": (C.c_int, [H, vp]),
     "dfm_bmat_matvec": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
     "dfm_vec_copy_free": (C.c_int, [C.c_size_t, vp, vp, vp]),
+    "dfm_bmat_add_pattern": (C.c_int, [H, C.c_uint, vp, vp]),
+    "dfm_bmat_set_bc_col": (C.c_int, [H, vp]),
+    "dfm_bmat_extract": (C.c_int, [H, H, vp, vp]),
+    "dfm_bcrs_extract": (C.c_int, [H, H, vp]),
+    "dfm_vec_gather": (C.c_int, [C.c_size_t, vp, vp, vp]),
+    "dfm_mseg_create": (C.c_int, [C.c_uint, vp, vp, C.POINTER(H)]),
+    "dfm_mseg_destroy": (C.c_int, [H]),
+    "dfm_mseg_set_dia": (C.c_int, [H, C.c_uint, H]),
+    "dfm_mseg_set_nondia": (C.c_int, [H, C.c_uint, C.c_uint, H]),
+    "dfm_mseg_matvec": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
+    "dfm_mseg_dot": (C.c_int, [H, vp, vp, f64p]),
+    "dfm_mseg_ilu_create": (C.c_int, [H, C.POINTER(H)]),
+    "dfm_mseg_ilu_destroy": (C.c_int, [H]),
+    "dfm_mseg_ilu_set_value": (C.c_int, [H, i32p]),
+    "dfm_mseg_ilu_get_values": (C.c_int, [H, C.c_uint, C.c_uint, vp, vp]),
+    "dfm_mseg_ilu_precond": (C.c_int, [H, vp]),
+    "dfm_mseg_solve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p]),
+    "dfm_mseg_solve_pbicgstab": (C.c_int, [H, H, vp, vp, f64p, u32p]),
     "dfm_zbcrs_create": (C.c_int, [C.c_uint, C.c_uint, C.POINTER(H)]),
     "dfm_zbcrs_destroy": (C.c_int, [H]),
     "dfm_zbcrs_add_pattern": (C.c_int, [H, C.c_uint, vp, vp]),

@@ -705,15 +705,33 @@ int dfm_ilu_get_pattern(const dfm_ilu* M, unsigned* rowptr, unsigned* col, int* 
   return DFM_OK;
 }
 
-int dfm_ilu_set_value(dfm_ilu* M, dfm_bcrs* A, int* nsingular) {
-  DFM_CHECK(M && A && A->nb == M->nb && A->b == M->b, DFM_BAD_ARG, "factor and matrix do not match");
-  DFM_CHECK(A->pattern_hash == M->pattern_hash, DFM_BAD_ARG,
-            "matrix pattern differs from the one the factor was built on");
+}  // extern "C"
+
+// SetValue_Initialize: A's values into the factor pattern (fill zero)
+int dfm::ilu_numeric_copy(dfm_ilu_s* M, const dfm_bcrs_s* A) {
   const int nb = (int)M->nb, b = (int)M->b, bb = b * b;
   const long nf = (long)M->fcol.size();
   DFM_LAUNCH(k_ilu_copy, grid_for(nf * bb), nf, bb, M->d_fsrc.p, A->d_val.p, M->d_fval.p);
   DFM_LAUNCH(k_ilu_copy_dia, grid_for((long)nb * bb), nb, bb, M->permuted ? M->d_order.p : nullptr, A->d_dia.p,
              M->d_fdia.p);
+  return DFM_OK;
+}
+
+extern "C" {
+
+int dfm_ilu_set_value(dfm_ilu* M, dfm_bcrs* A, int* nsingular) {
+  DFM_CHECK(M && A && A->nb == M->nb && A->b == M->b, DFM_BAD_ARG, "factor and matrix do not match");
+  DFM_CHECK(A->pattern_hash == M->pattern_hash, DFM_BAD_ARG,
+            "matrix pattern differs from the one the factor was built on");
+  DFM_TRY(ilu_numeric_copy(M, A));
+  return ilu_numeric_factor(M, A, nsingular);
+}
+
+}  // extern "C"
+
+// DoILUDecomp on the values in the factor, then the sliced-ELL copies
+int dfm::ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular) {
+  const int b = (int)M->b;
   DFM_HIP(hipMemsetAsync(M->d_nsing.p, 0, sizeof(int), g_stream));
   for (int l = 0; l < M->nlev_lower; l++) {
     const int off = M->lev_ptr_L[l], nrow = M->lev_ptr_L[l + 1] - off;
@@ -748,6 +766,8 @@ int dfm_ilu_set_value(dfm_ilu* M, dfm_bcrs* A, int* nsingular) {
   }
   return DFM_OK;
 }
+
+extern "C" {
 
 int dfm_ilu_get_values(const dfm_ilu* M, double* val, double* dia) {
   DFM_CHECK(M, DFM_BAD_ARG, "NULL factor");

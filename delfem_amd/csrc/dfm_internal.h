@@ -140,6 +140,12 @@ struct KState {
   int iter, conv_iter, maxit;
 };
 
+// Numeric ILU in two steps (dfm_ilu_set_value = copy + factor): between
+// them a caller may change the copied values (cross-segment updates).
+int ilu_numeric_copy(dfm_ilu_s* M, const dfm_bcrs_s* A);
+int ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular);
+int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper);
+
 }  // namespace dfm
 
 struct dfm_bcrs_s {
@@ -157,6 +163,16 @@ struct dfm_asm_s {
   dfm::DevBuf<unsigned> d_conn;
   dfm::DevBuf<int> d_elsup_ptr, d_elsup;  // node -> elements (ascending)
   dfm::DevBuf<int> d_slot;                 // per (node, element-slot, local j): CRS index, -1 diag, -2 dropped
+};
+
+struct dfm_bmat_s {
+  unsigned nb = 0, b = 0;               // rows: nb blocks of b dofs
+  unsigned nbr = 0, br = 0;             // columns: nbr blocks of br dofs
+  std::vector<unsigned> rowptr, col;    // host pattern (the diagonal block, if any, inside the rows)
+  dfm::DevBuf<int> d_rowptr, d_col;
+  dfm::DevBuf<double> d_val;            // ncrs * b * br, row-major blocks
+  dfm::DevBuf<long long> d_src;         // per entry: CRS index in the source matrix, or -1-i for its diagonal block i
+  unsigned long long src_hash = 0;      // pattern hash of the matrix the pattern was made from
 };
 
 struct dfm_ilu_s {

@@ -1,0 +1,481 @@
+// dfm_mseg.hip -- multi-segment linear systems (LsSol::CLinearSystem with
+// several segments, src/ls/linearsystem.cpp): diagonal blocks are the
+// library's block-CRS matrices (one block size per segment), off-diagonal
+// blocks rectangular CMat_BlkCrs (dfm_bmat); vectors are the segments'
+// vectors one after the other.  The preconditioner is CPreconditioner_ILU(0)
+// across the segments (src/ls/preconditioner.cpp:57-335): per segment the
+// natural-order ILU(0) of its diagonal block, per off-diagonal block a
+// CMatFrac_BlkCrs copy updated by DoILUDecompUp / Low / LowUp
+// (src/matvec/matfrac_blkcrs.cpp:540-783) and the diagonal blocks by
+// CMatDiaFrac_BlkCrs::DoILUDecompLowUp (matdiafrac_blkcrs.cpp:665-767), in
+// the reference's block order and operation order.  The Krylov loops
+// (LsSol::Solve_PCG / Solve_PBiCGSTAB, src/ls/solver_ls_iter.cpp) run on the
+// host over device kernels; the dot products add the segments' sums in
+// segment order as CLinearSystem::DOT does.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "dfm_kernels.h"
+
+using namespace dfm;
+
+struct dfm_mseg_s {
+  unsigned nseg = 0;
+  std::vector<unsigned> nb, b;
+  std::vector<size_t> off;                          // dof offset of each segment (nseg + 1)
+  std::vector<dfm_bcrs_s*> dia;                     // borrowed
+  std::vector<std::vector<dfm_bmat_s*>> nd;         // borrowed, null where there is no block
+};
+
+struct dfm_msilu_s {
+  dfm_mseg_s* S = nullptr;
+  std::vector<dfm_ilu_s*> dia;                      // owned: ILU(0), natural order
+  std::vector<std::vector<dfm_bmat_s*>> F;          // owned: copies of the off-diagonal blocks
+  std::vector<double*> work;                        // Krylov vectors
+  ~dfm_msilu_s() {
+    for (auto* m : dia)
+      if (m) dfm_ilu_destroy(m);
+    for (auto& row : F)
+      for (auto* f : row)
+        if (f) dfm_bmat_destroy(f);
+    for (double* w : work)
+      if (w) dfm_free(w);
+  }
+};
+
+#define DFM_MLAUNCH(kernel, grid, ...)                                                      \
+  do {                                                                                      \
+    if ((grid) > 0) {                                                                       \
+      hipLaunchKernelGGL((kernel), dim3(grid), dim3(kNT), 0, ::dfm::stream(), __VA_ARGS__); \
+      DFM_HIP(hipGetLastError());                                                           \
+    }                                                                                       \
+  } while (0)
+
+namespace {
+
+inline int mgrid(long n) { return (int)std::max<long>(1, (n + kNT - 1) / kNT); }
+
+// CalcSubMatPr (matfrac_blkcrs.cpp:43-54): out -= a b, one product at a time
+__device__ inline void msub(double* out, const double* a, const double* c, int ni, int nk, int nj) {
+  for (int i = 0; i < ni; i++)
+    for (int j = 0; j < nj; j++)
+      for (int k = 0; k < nk; k++) out[i * nj + j] -= a[i * nk + k] * c[k * nj + j];
+}
+// CalcMatPr (:56-69): out = d out through a temporary
+__device__ inline void mmul(double* out, const double* d, int ni, int nj) {
+  double tmp[64];
+  for (int i = 0; i < ni; i++)
+    for (int j = 0; j < nj; j++) {
+      tmp[i * nj + j] = 0.0;
+      for (int k = 0; k < ni; k++) tmp[i * nj + j] += d[i * ni + k] * out[k * nj + j];
+    }
+  for (int i = 0; i < ni * nj; i++) out[i] = tmp[i];
+}
+
+// DoILUDecompUp (matfrac_blkcrs.cpp:601-701) for the rows of one lower level
+// of factor i: U_ij row r -= L_rk U_ij row k, then scaled by the inverted pivot
+__global__ void __launch_bounds__(kNT) k_ms_up(int nrow, const int* __restrict__ rows, const int* __restrict__ frow,
+                                              const int* __restrict__ fcol, const int* __restrict__ fdiaind,
+                                              const double* __restrict__ fval, const double* __restrict__ fdia,
+                                              const int* __restrict__ rp, const int* __restrict__ cl,
+                                              double* __restrict__ F, int bi, int bj) {
+  const int t = blockIdx.x * kNT + threadIdx.x;
+  if (t >= nrow) return;
+  const int r = rows[t];
+  const int bs = bi * bj, bii = bi * bi;
+  for (int ik = frow[r]; ik < fdiaind[r]; ik++) {
+    const int k = fcol[ik];
+    for (int kj = rp[k]; kj < rp[k + 1]; kj++) {
+      const int ij = find_col(cl, rp[r], rp[r + 1], cl[kj]);
+      if (ij < 0) continue;
+      msub(F + (long)ij * bs, fval + (long)ik * bii, F + (long)kj * bs, bi, bi, bj);
+    }
+  }
+  for (int q = rp[r]; q < rp[r + 1]; q++) mmul(F + (long)q * bs, fdia + (long)r * bii, bi, bj);
+}
+// DoILUDecompLow (:703-783): L_ij row r -= L_ij(r, k) U_j row k, rows independent
+__global__ void __launch_bounds__(kNT) k_ms_low(int nrow, const int* __restrict__ rp, const int* __restrict__ cl,
+                                               double* __restrict__ F, const int* __restrict__ frow,
+                                               const int* __restrict__ fcol, const int* __restrict__ fdiaind,
+                                               const double* __restrict__ fval, int bi, int bj) {
+  const int r = blockIdx.x * kNT + threadIdx.x;
+  if (r >= nrow) return;
+  const int bs = bi * bj, bjj = bj * bj;
+  for (int ik = rp[r]; ik < rp[r + 1]; ik++) {
+    const int k = cl[ik];
+    for (int kj = fdiaind[k]; kj < frow[k + 1]; kj++) {
+      const int ij = find_col(cl, rp[r], rp[r + 1], fcol[kj]);
+      if (ij < 0) continue;
+      msub(F + (long)ij * bs, F + (long)ik * bs, fval + (long)kj * bjj, bi, bj, bj);
+    }
+  }
+}
+// CMatFrac_BlkCrs::DoILUDecompLowUp (:540-599): T_ij -= L_ik U_kj on T's pattern
+__global__ void __launch_bounds__(kNT) k_ms_lowup(int nrow, const int* __restrict__ trp, const int* __restrict__ tcl,
+                                                 double* __restrict__ T, const int* __restrict__ lrp,
+                                                 const int* __restrict__ lcl, const double* __restrict__ Lv,
+                                                 const int* __restrict__ urp, const int* __restrict__ ucl,
+                                                 const double* __restrict__ Uv, int bi, int bk, int bj) {
+  const int r = blockIdx.x * kNT + threadIdx.x;
+  if (r >= nrow) return;
+  for (int ik = lrp[r]; ik < lrp[r + 1]; ik++) {
+    const int k = lcl[ik];
+    for (int kj = urp[k]; kj < urp[k + 1]; kj++) {
+      const int ij = find_col(tcl, trp[r], trp[r + 1], ucl[kj]);
+      if (ij < 0) continue;
+      msub(T + (long)ij * bi * bj, Lv + (long)ik * bi * bk, Uv + (long)kj * bk * bj, bi, bk, bj);
+    }
+  }
+}
+// CMatDiaFrac_BlkCrs::DoILUDecompLowUp (matdiafrac_blkcrs.cpp:665-767): the
+// copied diagonal block of segment i -= L_ik U_ki before its factorisation
+__global__ void __launch_bounds__(kNT) k_ms_lowup_dia(int nrow, const int* __restrict__ frow,
+                                                     const int* __restrict__ fcol, double* __restrict__ fval,
+                                                     double* __restrict__ fdia, const int* __restrict__ lrp,
+                                                     const int* __restrict__ lcl, const double* __restrict__ Lv,
+                                                     const int* __restrict__ urp, const int* __restrict__ ucl,
+                                                     const double* __restrict__ Uv, int bi, int bk) {
+  const int r = blockIdx.x * kNT + threadIdx.x;
+  if (r >= nrow) return;
+  const int bii = bi * bi;
+  for (int ik = lrp[r]; ik < lrp[r + 1]; ik++) {
+    const int k = lcl[ik];
+    for (int kj = urp[k]; kj < urp[k + 1]; kj++) {
+      const int j = ucl[kj];
+      const double* a = Lv + (long)ik * bi * bk;
+      const double* c = Uv + (long)kj * bk * bi;
+      if (j != r) {
+        const int ij = find_col(fcol, frow[r], frow[r + 1], j);
+        if (ij < 0) continue;
+        msub(fval + (long)ij * bii, a, c, bi, bk, bi);
+      } else {
+        msub(fdia + (long)r * bii, a, c, bi, bk, bi);
+      }
+    }
+  }
+}
+
+int seg_dot(dfm_mseg_s* S, const double* x, const double* y, double* out) {
+  double s = 0.0;
+  for (unsigned i = 0; i < S->nseg; i++) {
+    double d = 0.0;
+    DFM_TRY(dfm_vec_dot(S->off[i + 1] - S->off[i], x + S->off[i], y + S->off[i], &d));
+    s += d;
+  }
+  *out = s;
+  return DFM_OK;
+}
+// CLinearSystem::AXPY returns without touching y when alpha == 0 (linearsystem.cpp:302)
+int ls_axpy(size_t n, double alpha, const double* x, double* y) {
+  if (alpha == 0) return DFM_OK;
+  return dfm_vec_axpy(n, alpha, x, y);
+}
+
+}  // namespace
+
+int dfm_mseg_create(unsigned nseg, const unsigned* nblk, const unsigned* len, dfm_mseg** out) {
+  DFM_CHECK(out && nblk && len && nseg > 0 && nseg <= 16, DFM_BAD_ARG, "bad arguments to dfm_mseg_create");
+  DFM_TRY(ensure_init());
+  std::unique_ptr<dfm_mseg_s> S(new dfm_mseg_s);
+  S->nseg = nseg;
+  S->off.assign(1, 0);
+  for (unsigned i = 0; i < nseg; i++) {
+    DFM_CHECK(len[i] > 0 && len[i] <= 8, DFM_BAD_ARG, "segment %u: block size %u", i, len[i]);
+    S->nb.push_back(nblk[i]);
+    S->b.push_back(len[i]);
+    S->off.push_back(S->off.back() + (size_t)nblk[i] * len[i]);
+  }
+  S->dia.assign(nseg, nullptr);
+  S->nd.assign(nseg, std::vector<dfm_bmat_s*>(nseg, nullptr));
+  *out = S.release();
+  return DFM_OK;
+}
+int dfm_mseg_destroy(dfm_mseg* S) {
+  delete S;
+  return DFM_OK;
+}
+int dfm_mseg_set_dia(dfm_mseg* S, unsigned i, dfm_bcrs* A) {
+  DFM_CHECK(S && A && i < S->nseg && A->nb == S->nb[i] && A->b == S->b[i], DFM_BAD_ARG,
+            "dfm_mseg_set_dia: segment %u does not match the matrix", i);
+  S->dia[i] = A;
+  return DFM_OK;
+}
+int dfm_mseg_set_nondia(dfm_mseg* S, unsigned i, unsigned j, dfm_bmat* B) {
+  DFM_CHECK(S && B && i < S->nseg && j < S->nseg && i != j && B->nb == S->nb[i] && B->b == S->b[i] &&
+                B->nbr == S->nb[j] && B->br == S->b[j],
+            DFM_BAD_ARG, "dfm_mseg_set_nondia: block (%u, %u) does not match the segments", i, j);
+  S->nd[i][j] = B;
+  return DFM_OK;
+}
+int dfm_mseg_matvec(dfm_mseg* S, double alpha, const double* x, double beta, double* y) {
+  DFM_CHECK(S && x && y && x != y, DFM_BAD_ARG, "bad arguments to dfm_mseg_matvec");
+  if (alpha == 0.0) return dfm_vec_scal(S->off[S->nseg], beta, y);
+  for (unsigned i = 0; i < S->nseg; i++) {
+    DFM_CHECK(S->dia[i], DFM_BAD_ARG, "segment %u has no diagonal block", i);
+    DFM_TRY(dfm_bcrs_matvec(S->dia[i], alpha, x + S->off[i], beta, y + S->off[i]));
+    for (unsigned j = 0; j < S->nseg; j++)
+      if (S->nd[i][j]) DFM_TRY(dfm_bmat_matvec(S->nd[i][j], alpha, x + S->off[j], 1.0, y + S->off[i]));
+  }
+  return DFM_OK;
+}
+int dfm_mseg_dot(dfm_mseg* S, const double* x, const double* y, double* out) {
+  DFM_CHECK(S && x && y && out, DFM_BAD_ARG, "bad arguments to dfm_mseg_dot");
+  return seg_dot(S, x, y, out);
+}
+
+int dfm_mseg_ilu_create(dfm_mseg* S, dfm_msilu** out) {
+  DFM_CHECK(S && out, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_create");
+  std::unique_ptr<dfm_msilu_s> M(new dfm_msilu_s);
+  M->S = S;
+  M->dia.assign(S->nseg, nullptr);
+  M->F.assign(S->nseg, std::vector<dfm_bmat_s*>(S->nseg, nullptr));
+  for (unsigned i = 0; i < S->nseg; i++) {
+    DFM_CHECK(S->dia[i], DFM_BAD_ARG, "segment %u has no diagonal block", i);
+    DFM_TRY(dfm_ilu_create(S->dia[i], 0, DFM_ORDER_NATURAL, nullptr, 0, &M->dia[i]));
+    for (unsigned j = 0; j < S->nseg; j++) {
+      const dfm_bmat_s* B = S->nd[i][j];
+      if (!B) continue;
+      DFM_TRY(dfm_bmat_create(B->nb, B->b, B->nbr, B->br, &M->F[i][j]));
+      std::vector<unsigned> rp = B->rowptr, cl = B->col;
+      if (cl.empty()) cl.push_back(0);
+      DFM_TRY(dfm_bmat_add_pattern(M->F[i][j], B->nb, rp.data(), cl.data()));
+    }
+  }
+  *out = M.release();
+  return DFM_OK;
+}
+int dfm_mseg_ilu_destroy(dfm_msilu* M) {
+  delete M;
+  return DFM_OK;
+}
+int dfm_mseg_ilu_set_value(dfm_msilu* M, int* nsingular) {
+  DFM_CHECK(M, DFM_BAD_ARG, "NULL factor");
+  dfm_mseg_s* S = M->S;
+  const unsigned n = S->nseg;
+  int nsing_total = 0;
+  // SetValue_Initialize of every block first (preconditioner.cpp:231-251)
+  for (unsigned i = 0; i < n; i++) {
+    DFM_CHECK(S->dia[i]->pattern_hash == M->dia[i]->pattern_hash, DFM_BAD_ARG,
+              "segment %u: matrix pattern differs from the one the factor was built on", i);
+    DFM_TRY(ilu_numeric_copy(M->dia[i], S->dia[i]));
+    for (unsigned j = 0; j < n; j++) {
+      dfm_bmat_s* F = M->F[i][j];
+      if (!F) continue;
+      const size_t nv = F->col.size() * F->b * F->br;
+      DFM_CHECK(S->nd[i][j]->col == F->col, DFM_BAD_ARG, "block (%u, %u): pattern changed", i, j);
+      if (nv)
+        DFM_HIP(hipMemcpyAsync(F->d_val.p, S->nd[i][j]->d_val.p, nv * sizeof(double), hipMemcpyDeviceToDevice,
+                               stream()));
+    }
+  }
+  // the blocks in (i, j) order (:253-297)
+  for (unsigned i = 0; i < n; i++)
+    for (unsigned j = 0; j < n; j++) {
+      if (i == j) {
+        dfm_ilu_s* D = M->dia[i];
+        for (unsigned k = 0; k < i; k++) {
+          const dfm_bmat_s *lo = M->F[i][k], *up = M->F[k][i];
+          if (!lo || !up) continue;
+          DFM_MLAUNCH(k_ms_lowup_dia, mgrid(S->nb[i]), (int)S->nb[i], D->d_frow.p, D->d_fcol.p, D->d_fval.p,
+                      D->d_fdia.p, lo->d_rowptr.p, lo->col.empty() ? nullptr : lo->d_col.p, lo->d_val.p,
+                      up->d_rowptr.p, up->col.empty() ? nullptr : up->d_col.p, up->d_val.p, (int)S->b[i],
+                      (int)S->b[k]);
+        }
+        int ns = 0;
+        const int rc = ilu_numeric_factor(D, S->dia[i], &ns);
+        nsing_total += ns;
+        if (rc != DFM_OK) {
+          if (nsingular) *nsingular = nsing_total;
+          return rc;
+        }
+        continue;
+      }
+      dfm_bmat_s* T = M->F[i][j];
+      if (!T) continue;
+      const int* tcl = T->col.empty() ? nullptr : T->d_col.p;
+      const unsigned kmax = std::min(i, j);
+      for (unsigned k = 0; k < kmax + 1; k++) {
+        if (k == i && k < j) {  // DoILUDecompUp with factor i, one launch per lower level
+          const dfm_ilu_s* D = M->dia[i];
+          for (int l = 0; l < D->nlev_lower; l++) {
+            const int o = D->lev_ptr_L[l], nr = D->lev_ptr_L[l + 1] - o;
+            DFM_MLAUNCH(k_ms_up, mgrid(nr), nr, D->d_rows_L.p + o, D->d_frow.p, D->d_fcol.p, D->d_fdiaind.p,
+                        D->d_fval.p, D->d_fdia.p, T->d_rowptr.p, tcl, T->d_val.p, (int)T->b, (int)T->br);
+          }
+          continue;
+        }
+        if (k == j && k < i) {  // DoILUDecompLow with factor j
+          const dfm_ilu_s* D = M->dia[j];
+          DFM_MLAUNCH(k_ms_low, mgrid(T->nb), (int)T->nb, T->d_rowptr.p, tcl, T->d_val.p, D->d_frow.p, D->d_fcol.p,
+                      D->d_fdiaind.p, D->d_fval.p, (int)T->b, (int)T->br);
+          continue;
+        }
+        const dfm_bmat_s *lo = M->F[i][k], *up = M->F[k][j];
+        if (!lo || !up) continue;
+        DFM_MLAUNCH(k_ms_lowup, mgrid(T->nb), (int)T->nb, T->d_rowptr.p, tcl, T->d_val.p, lo->d_rowptr.p,
+                    lo->col.empty() ? nullptr : lo->d_col.p, lo->d_val.p, up->d_rowptr.p,
+                    up->col.empty() ? nullptr : up->d_col.p, up->d_val.p, (int)T->b, (int)lo->br, (int)T->br);
+      }
+    }
+  DFM_HIP(hipStreamSynchronize(stream()));
+  if (nsingular) *nsingular = nsing_total;
+  return DFM_OK;
+}
+int dfm_mseg_ilu_get_values(const dfm_msilu* M, unsigned i, unsigned j, double* val, double* dia) {
+  DFM_CHECK(M && i < M->S->nseg && j < M->S->nseg, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_get_values");
+  if (i == j) return dfm_ilu_get_values(M->dia[i], val, dia);
+  DFM_CHECK(M->F[i][j], DFM_BAD_ARG, "no block (%u, %u)", i, j);
+  return dfm_bmat_get_values(M->F[i][j], val);
+}
+// SolvePrecond across segments (preconditioner.cpp:317-333)
+int dfm_mseg_ilu_precond(dfm_msilu* M, double* vec) {
+  DFM_CHECK(M && vec, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_precond");
+  dfm_mseg_s* S = M->S;
+  const unsigned n = S->nseg;
+  for (unsigned i = 0; i < n; i++) {
+    for (unsigned j = 0; j < i; j++)
+      if (M->F[i][j]) DFM_TRY(dfm_bmat_matvec(M->F[i][j], -1.0, vec + S->off[j], 1.0, vec + S->off[i]));
+    DFM_TRY(ilu_sweep64(M->dia[i], vec + S->off[i], true, false));
+  }
+  for (int i = (int)n - 1; i >= 0; i--) {
+    for (unsigned j = (unsigned)i + 1; j < n; j++)
+      if (M->F[i][j]) DFM_TRY(dfm_bmat_matvec(M->F[i][j], -1.0, vec + S->off[j], 1.0, vec + S->off[i]));
+    DFM_TRY(ilu_sweep64(M->dia[i], vec + S->off[i], false, true));
+  }
+  DFM_HIP(hipStreamSynchronize(stream()));
+  return DFM_OK;
+}
+
+// Work vectors start at zero, as the reference's: MATVEC(1, x, 0, y) scales
+// the old y by 0, which keeps a NaN that uninitialised memory may hold.
+static int msilu_work(dfm_msilu_s* M, int nvec) {
+  const size_t n = M->S->off[M->S->nseg];
+  while ((int)M->work.size() < nvec) {
+    double* p = nullptr;
+    DFM_TRY(dfm_malloc((void**)&p, std::max<size_t>(1, n) * sizeof(double)));
+    M->work.push_back(p);
+    DFM_TRY(dfm_vec_fill(std::max<size_t>(1, n), 0.0, p));
+  }
+  return DFM_OK;
+}
+
+// LsSol::Solve_PCG (solver_ls_iter.cpp:100-172) on the segments: r in =
+// right-hand side, out = residual; x out.  DFM_OK when converged,
+// DFM_NOT_CONVERGED at max_iter (conv_ratio then the final ratio).
+int dfm_mseg_solve_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration) {
+  DFM_CHECK(S && M && M->S == S && r && x && conv_ratio && iteration && r != x, DFM_BAD_ARG,
+            "bad arguments to dfm_mseg_solve_pcg");
+  const double tol = *conv_ratio;
+  const unsigned mx_iter = *iteration;
+  const size_t n = S->off[S->nseg];
+  DFM_TRY(msilu_work(M, 2));
+  double *p = M->work[0], *z = M->work[1];
+  DFM_TRY(dfm_vec_scal(n, 0.0, x));
+  double rr0 = 0.0;
+  DFM_TRY(seg_dot(S, r, r, &rr0));
+  if (rr0 < 1.0e-30) {
+    *conv_ratio = 0.0;
+    *iteration = 0;
+    return DFM_OK;
+  }
+  const double inv_rr0 = 1.0 / rr0;
+  DFM_TRY(dfm_vec_copy(n, r, z));
+  DFM_TRY(dfm_mseg_ilu_precond(M, z));
+  DFM_TRY(dfm_vec_copy(n, z, p));
+  double rz = 0.0;
+  DFM_TRY(seg_dot(S, r, z, &rz));
+  for (unsigned it = 0; it < mx_iter; it++) {
+    DFM_TRY(dfm_mseg_matvec(S, 1.0, p, 0.0, z));
+    double pAp = 0.0;
+    DFM_TRY(seg_dot(S, p, z, &pAp));
+    const double alpha = rz / pAp;
+    DFM_TRY(ls_axpy(n, -alpha, z, r));
+    DFM_TRY(ls_axpy(n, alpha, p, x));
+    double rr = 0.0;
+    DFM_TRY(seg_dot(S, r, r, &rr));
+    if (rr * inv_rr0 < tol * tol) {
+      *conv_ratio = std::sqrt(rr * inv_rr0);
+      *iteration = it;
+      return DFM_OK;
+    }
+    DFM_TRY(dfm_vec_copy(n, r, z));
+    DFM_TRY(dfm_mseg_ilu_precond(M, z));
+    double rz_new = 0.0;
+    DFM_TRY(seg_dot(S, r, z, &rz_new));
+    const double beta = rz_new / rz;
+    rz = rz_new;
+    DFM_TRY(dfm_vec_scal(n, beta, p));
+    DFM_TRY(ls_axpy(n, 1.0, z, p));
+  }
+  double rr = 0.0;
+  DFM_TRY(seg_dot(S, r, r, &rr));
+  *conv_ratio = std::sqrt(rr * inv_rr0);
+  return DFM_NOT_CONVERGED;
+}
+
+// LsSol::Solve_PBiCGSTAB (solver_ls_iter.cpp:291-408) on the segments.
+// Returns DFM_OK when converged, DFM_NOT_CONVERGED at max_iter (the
+// reference returns true in both cases; iteration = max_iter then).
+int dfm_mseg_solve_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio,
+                             unsigned* iteration) {
+  DFM_CHECK(S && M && M->S == S && r && x && conv_ratio && iteration && r != x, DFM_BAD_ARG,
+            "bad arguments to dfm_mseg_solve_pbicgstab");
+  const double tol = *conv_ratio;
+  const unsigned max_iter = *iteration;
+  const size_t n = S->off[S->nseg];
+  DFM_TRY(msilu_work(M, 7));
+  double *vs = M->work[0], *Ms = M->work[1], *AMs = M->work[2], *p = M->work[3], *Mp = M->work[4],
+         *AMp = M->work[5], *r2 = M->work[6];
+  double rr0 = 0.0;
+  DFM_TRY(seg_dot(S, r, r, &rr0));
+  if (rr0 < 1.0e-60) {
+    *conv_ratio = std::sqrt(rr0);
+    *iteration = 0;
+    return DFM_OK;
+  }
+  const double inv_rr0 = 1.0 / rr0;
+  DFM_TRY(dfm_vec_scal(n, 0.0, x));
+  DFM_TRY(dfm_vec_copy(n, r, r2));
+  DFM_TRY(dfm_vec_copy(n, r, p));
+  *iteration = max_iter;
+  for (unsigned it = 1; it < max_iter; it++) {
+    DFM_TRY(dfm_vec_copy(n, p, Mp));
+    DFM_TRY(dfm_mseg_ilu_precond(M, Mp));
+    double r_r2 = 0.0;
+    DFM_TRY(seg_dot(S, r, r2, &r_r2));
+    DFM_TRY(dfm_mseg_matvec(S, 1.0, Mp, 0.0, AMp));
+    double den = 0.0;
+    DFM_TRY(seg_dot(S, AMp, r2, &den));
+    const double alpha = r_r2 / den;
+    DFM_TRY(dfm_vec_copy(n, r, vs));
+    DFM_TRY(ls_axpy(n, -alpha, AMp, vs));
+    DFM_TRY(dfm_vec_copy(n, vs, Ms));
+    DFM_TRY(dfm_mseg_ilu_precond(M, Ms));
+    DFM_TRY(dfm_mseg_matvec(S, 1.0, Ms, 0.0, AMs));
+    double den2 = 0.0, num = 0.0;
+    DFM_TRY(seg_dot(S, AMs, AMs, &den2));
+    DFM_TRY(seg_dot(S, vs, AMs, &num));
+    const double omega = num / den2;
+    DFM_TRY(ls_axpy(n, alpha, Mp, x));
+    DFM_TRY(ls_axpy(n, omega, Ms, x));
+    DFM_TRY(dfm_vec_copy(n, vs, r));
+    DFM_TRY(ls_axpy(n, -omega, AMs, r));
+    double rr = 0.0;
+    DFM_TRY(seg_dot(S, r, r, &rr));
+    if (rr * inv_rr0 < tol * tol) {
+      *conv_ratio = std::sqrt(rr * inv_rr0);
+      *iteration = it;
+      return DFM_OK;
+    }
+    double tmp1 = 0.0;
+    DFM_TRY(seg_dot(S, r, r2, &tmp1));
+    const double beta = tmp1 * alpha / (r_r2 * omega);
+    DFM_TRY(dfm_vec_scal(n, beta, p));
+    DFM_TRY(ls_axpy(n, 1.0, r, p));
+    DFM_TRY(ls_axpy(n, -beta * omega, AMp, p));
+  }
+  return DFM_NOT_CONVERGED;
+}

@@ -327,8 +327,9 @@ class FEMSystem:
         self.flags = np.ascontiguousarray(prob.flags, dtype=np.int32)
         self.flags_dev = to_device(self.flags)
 
-    def assemble(self):
-        """InitializeMarge + element loop + FinalizeMarge; returns ||r||."""
+    def assemble_elements(self):
+        """InitializeMarge + the element loop (matrix and residual, no
+        boundary conditions yet)."""
         self.A.SetZero()
         self.res.zero_()
         p = self.prob
@@ -340,6 +341,10 @@ class FEMSystem:
             self.asm.oseen2d_p1p1(self.coord, p.myu, p.rho, self.coef[0], self.value, self.res)
         else:
             self.asm.solid3d_p1(self.coord, self.coef[0], self.coef[1], p.rho, p.g, self.value, self.res)
+
+    def assemble(self):
+        """InitializeMarge + element loop + FinalizeMarge; returns ||r||."""
+        self.assemble_elements()
         self.A.SetBoundaryCondition(self.flags)
         _sync_torch()
         call("dfm_vec_zero_bc", self.res.numel(), _ptr(self.flags), _dptr(self.res))
@@ -363,7 +368,7 @@ class CMat_BlkCrs:
     def __init__(self, nblk_col, len_col, nblk_row, len_row):
         h = C.c_void_p()
         call("dfm_bmat_create", nblk_col, len_col, nblk_row, len_row, C.byref(h))
-        self.h, self.nblk, self.len = h, nblk_col, len_col
+        self.h, self.nblk, self.len, self.len_row = h, nblk_col, len_col, len_row
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -396,7 +401,7 @@ class CMat_BlkCrs:
         return True
 
     def values(self):
-        n = self.NCrs() * self.len * self.len
+        n = self.NCrs() * self.len * self.len_row
         val = np.zeros(max(1, n))
         call("dfm_bmat_get_values", self.h, _ptr(val))
         return val[:n]
@@ -486,6 +491,158 @@ class FEMSystemSave(FEMSystem):
         if update.numel() != self.value.numel():
             raise ValueError("update and field sizes differ")
         call("dfm_vec_copy_free", self.value.numel(), _ptr(self.flags.reshape(-1)), _dptr(update), _dptr(self.value))
+
+
+class MultiSegmentSystem:
+    """LsSol::CLinearSystem with several segments (src/ls/linearsystem.cpp):
+    a combined-unknown Problem split into the given dof groups (e.g.
+    [[0, 1], [2]] for velocity and pressure), as the reference's split-field
+    equation systems assemble it.  The combined matrix is assembled on the
+    device and its sub-blocks taken into the segments' diagonal blocks
+    (CMatDia_BlkCrs) and off-diagonal blocks (CMat_BlkCrs, pattern: the node
+    adjacency with the node itself); FinalizeMarge sets the boundary
+    conditions per block.  Vectors are the segments' vectors one after the
+    other (torch tensors on the device)."""
+
+    def __init__(self, prob, groups):
+        torch = _torch()
+        self.prob, self.groups = prob, [list(g) for g in groups]
+        self.fem = FEMSystem(prob)
+        nb, B = prob.nnode, prob.b
+        rowptr, col = fem_pattern(nb, prob.conn)
+        rp2 = np.zeros(nb + 1, dtype=np.uint32)
+        cols = []
+        for i in range(nb):
+            c = np.sort(np.append(col[rowptr[i]:rowptr[i + 1]], i)).astype(np.uint32)
+            cols.append(c)
+            rp2[i + 1] = rp2[i] + c.size
+        col2 = np.concatenate(cols).astype(np.uint32)
+        nseg = len(self.groups)
+        self.lens = [len(g) for g in self.groups]
+        self.dia = []
+        for g in self.groups:
+            D = CMatDia_BlkCrs(nb, len(g))
+            D.AddPattern(rowptr, col)
+            self.dia.append(D)
+        self.nd = {}
+        for i, gi in enumerate(self.groups):
+            for j, gj in enumerate(self.groups):
+                if i != j:
+                    M = CMat_BlkCrs(nb, len(gi), nb, len(gj))
+                    call("dfm_bmat_add_pattern", M.h, nb, _ptr(rp2), _ptr(col2))
+                    self.nd[(i, j)] = M
+        nbs = np.array([nb] * nseg, dtype=np.uint32)
+        lens = np.array(self.lens, dtype=np.uint32)
+        h = C.c_void_p()
+        call("dfm_mseg_create", nseg, _ptr(nbs), _ptr(lens), C.byref(h))
+        self.h = h
+        for i in range(nseg):
+            call("dfm_mseg_set_dia", h, i, self.dia[i].h)
+        for (i, j), M in self.nd.items():
+            call("dfm_mseg_set_nondia", h, i, j, M.h)
+        flags = np.ascontiguousarray(prob.flags, dtype=np.int32).reshape(nb, B)
+        self.flags = [np.ascontiguousarray(flags[:, g]).reshape(-1) for g in self.groups]
+        self.flags_all = np.concatenate(self.flags)
+        # combined dof of each segment-ordered dof
+        self.perm = np.concatenate([(np.arange(nb)[:, None] * B + np.array(g)[None, :]).reshape(-1)
+                                    for g in self.groups]).astype(np.int64)
+        self.res = torch.zeros(nb * B, dtype=torch.float64, device="cuda")
+        self.rowptr_nd, self.col_nd = rp2, col2
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_mseg_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def assemble(self):
+        """InitializeMarge, the element loop of the combined problem split into
+        the blocks, FinalizeMarge (diagonal SetBoundaryCondition, off-diagonal
+        SetBoundaryCondition_Row then _Colum, residual zeroed at fixed dofs);
+        returns ||r||."""
+        self.fem.assemble_elements()
+        A = self.fem.A
+        for i, g in enumerate(self.groups):
+            d = np.array(g, dtype=np.uint32)
+            call("dfm_bcrs_extract", self.dia[i].h, A.h, _ptr(d))
+        for (i, j), M in self.nd.items():
+            di, dj = np.array(self.groups[i], dtype=np.uint32), np.array(self.groups[j], dtype=np.uint32)
+            call("dfm_bmat_extract", M.h, A.h, _ptr(di), _ptr(dj))
+        for i in range(len(self.groups)):
+            self.dia[i].SetBoundaryCondition(self.flags[i])
+        for (i, j), M in self.nd.items():
+            M.SetBoundaryCondition_Row(self.flags[i])
+        for (i, j), M in self.nd.items():
+            call("dfm_bmat_set_bc_col", M.h, _ptr(self.flags[j]))
+        _sync_torch()
+        call("dfm_vec_gather", self.perm.size, _ptr(self.perm), _dptr(self.fem.res), _dptr(self.res))
+        call("dfm_vec_zero_bc", self.res.numel(), _ptr(self.flags_all), _dptr(self.res))
+        return self.dot(self.res, self.res) ** 0.5
+
+    def dot(self, x, y):
+        _sync_torch()
+        out = C.c_double()
+        call("dfm_mseg_dot", self.h, _dptr(x), _dptr(y), C.byref(out))
+        return out.value
+
+    def MATVEC(self, alpha, x, beta, y):
+        _sync_torch()
+        call("dfm_mseg_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
+
+    def preconditioner(self):
+        return MultiSegmentILU(self)
+
+
+class MultiSegmentILU:
+    """CPreconditioner_ILU(0) across the segments (src/ls/preconditioner.cpp:57-335)."""
+
+    def __init__(self, S):
+        h = C.c_void_p()
+        call("dfm_mseg_ilu_create", S.h, C.byref(h))
+        self.h, self.S = h, S
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_mseg_ilu_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def SetValue(self):
+        ns = C.c_int()
+        rc = call("dfm_mseg_ilu_set_value", self.h, C.byref(ns), ok=(_lib.DFM_OK, _lib.DFM_SINGULAR))
+        return rc == _lib.DFM_OK
+
+    def values(self, i, j):
+        S = self.S
+        nb, bi, bj = S.prob.nnode, S.lens[i], S.lens[j]
+        if i == j:
+            n = int(S.dia[i].NCrs())
+            val, dia = np.zeros(max(1, n * bi * bi)), np.zeros(nb * bi * bi)
+            call("dfm_mseg_ilu_get_values", self.h, i, j, _ptr(val), _ptr(dia))
+            return val[:n * bi * bi], dia
+        n = int(S.rowptr_nd[-1])
+        val = np.zeros(max(1, n * bi * bj))
+        call("dfm_mseg_ilu_get_values", self.h, i, j, _ptr(val), None)
+        return val[:n * bi * bj]
+
+    def SolvePrecond(self, vec):
+        _sync_torch()
+        call("dfm_mseg_ilu_precond", self.h, _dptr(vec))
+
+    def solve(self, kind, conv_ratio, iteration, r, x):
+        """LsSol::Solve_PCG (kind "pcg") or Solve_PBiCGSTAB ("bicg") on the
+        segments; returns (converged, conv_ratio, iteration)."""
+        _sync_torch()
+        conv = C.c_double(conv_ratio)
+        it = C.c_uint(iteration)
+        fn = "dfm_mseg_solve_pcg" if kind == "pcg" else "dfm_mseg_solve_pbicgstab"
+        rc = call(fn, self.S.h, self.h, _dptr(r), _dptr(x), C.byref(conv), C.byref(it),
+                  ok=(_lib.DFM_OK, _lib.DFM_NOT_CONVERGED))
+        return rc == _lib.DFM_OK, conv.value, it.value
 
 
 class BatchSystem:

@@ -268,9 +268,56 @@ int dfm_bmat_set_bc_inverse_col(dfm_bmat *M, const int *flag);             /* Se
 int dfm_bmat_set_bc_row(dfm_bmat *M, const int *flag);                     /* SetBoundaryCondition_Row mat_blkcrs.cpp:910-952 */
 int dfm_bmat_matvec(dfm_bmat *M, double alpha, const double *x_dev, double beta,
                     double *y_dev);                                        /* MatVec(..., isnt_trans=true) mat_blkcrs.cpp:955-1176 */
+/* General block CRS (off-diagonal blocks of multi-segment systems): rows of nblk_col blocks
+ * of len_col dofs, columns of nblk_row blocks of len_row dofs (dfm_bmat_create takes any).
+ * AddPattern(CIndexedArray) of a new matrix (mat_blkcrs.cpp:286-370): rows of column blocks. */
+int dfm_bmat_add_pattern(dfm_bmat *M, unsigned nrow, const unsigned *index, const unsigned *array);
+int dfm_bmat_set_bc_col(dfm_bmat *M, const int *flag);                     /* SetBoundaryCondition_Colum mat_blkcrs.cpp:829-875 */
+/* The block's values from a combined matrix A (b dofs per node, same nodes) assembled by the
+ * same element loop, before its boundary conditions: entry (i, j)'s sub-block is A's block
+ * (i, j) restricted to rows dofs_row[0..len_col) and columns dofs_col[0..len_row) -- the sums
+ * Mearge of the split element matrices forms.  Every entry must be in A's pattern. */
+int dfm_bmat_extract(dfm_bmat *M, const dfm_bcrs *A, const unsigned *dofs_row, const unsigned *dofs_col);
+/* The same for a segment's diagonal block D (D has A's pattern, block size len(dofs)). */
+int dfm_bcrs_extract(dfm_bcrs *D, const dfm_bcrs *A, const unsigned *dofs);
+/* y[k] = x[idx[k]] for k < n (host idx): a combined vector regrouped into segments, or back */
+int dfm_vec_gather(size_t n, const long long *idx, const double *x_dev, double *y_dev);
 /* y[i] = x[i] where flag[i] == 0 (host flags): the field update of CLinearSystem_Save::
  * UpdateValueOfField (linearsystem_fieldsave.cpp:249-300), fixed dofs keep their values */
 int dfm_vec_copy_free(size_t n, const int *flag, const double *x_dev, double *y_dev);
+
+/* ---- multi-segment systems (LsSol::CLinearSystem with several segments) --
+ * src/ls/linearsystem.cpp:138-490: segment i has nblk[i] blocks of len[i] dofs, a
+ * diagonal block (dfm_bcrs) and off-diagonal blocks (dfm_bmat, rows of segment i,
+ * columns of segment j); vectors are the segments' vectors one after the other
+ * (device pointers).  Blocks are borrowed: they must outlive the handle. */
+typedef struct dfm_mseg_s dfm_mseg;
+typedef struct dfm_msilu_s dfm_msilu;
+int dfm_mseg_create(unsigned nseg, const unsigned *nblk, const unsigned *len, dfm_mseg **out); /* AddLinSysSeg x nseg */
+int dfm_mseg_destroy(dfm_mseg *S);
+int dfm_mseg_set_dia(dfm_mseg *S, unsigned i, dfm_bcrs *A);                 /* AddMat_Dia :419-448 */
+int dfm_mseg_set_nondia(dfm_mseg *S, unsigned i, unsigned j, dfm_bmat *B);  /* AddMat_NonDia :450-520 */
+int dfm_mseg_matvec(dfm_mseg *S, double alpha, const double *x_dev, double beta,
+                    double *y_dev);                                         /* MATVEC :250-296 */
+int dfm_mseg_dot(dfm_mseg *S, const double *x_dev, const double *y_dev, double *result); /* DOT :335-364 */
+/* CPreconditioner_ILU(0) across the segments (preconditioner.cpp:57-206): natural-order
+ * ILU(0) of each diagonal block and a CMatFrac_BlkCrs copy of each off-diagonal block,
+ * no fill between segments (the reference's default level 0). */
+int dfm_mseg_ilu_create(dfm_mseg *S, dfm_msilu **out);
+int dfm_mseg_ilu_destroy(dfm_msilu *M);
+/* SetValue (:217-297): copies, then per block in (i, j) order DoILUDecompLowUp + DoILUDecomp
+ * (diagonal), DoILUDecompUp / Low / LowUp (off-diagonal); DFM_SINGULAR as dfm_ilu_set_value */
+int dfm_mseg_ilu_set_value(dfm_msilu *M, int *nsingular);
+/* factor values of block (i, j): diagonal as dfm_ilu_get_values, off-diagonal ncrs*len_i*len_j */
+int dfm_mseg_ilu_get_values(const dfm_msilu *M, unsigned i, unsigned j, double *val, double *dia);
+int dfm_mseg_ilu_precond(dfm_msilu *M, double *vec_dev);                    /* SolvePrecond :317-333 */
+/* LsSol::Solve_PCG / Solve_PBiCGSTAB (solver_ls_iter.cpp:100-172, 291-408) on the segments:
+ * r_dev in right-hand side, out residual; x_dev out.  DFM_OK converged, DFM_NOT_CONVERGED
+ * at max_iter (Solve_PBiCGSTAB's true in both cases). */
+int dfm_mseg_solve_pcg(dfm_mseg *S, dfm_msilu *M, double *r_dev, double *x_dev,
+                       double *conv_ratio, unsigned *iteration);
+int dfm_mseg_solve_pbicgstab(dfm_mseg *S, dfm_msilu *M, double *r_dev, double *x_dev,
+                             double *conv_ratio, unsigned *iteration);
 
 /* ---- complex twin (scalar blocks, as the reference's complex classes) ----
  * Complex values are interleaved (re, im) doubles: the layout of

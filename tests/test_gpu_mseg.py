@@ -1,0 +1,92 @@
+"""Multi-segment systems on the device (through the C-ABI) against the
+oracle's restatement, which the compiled reference pins bit for bit
+(tests/test_oracle_ref.py): the split blocks, residual and MATVEC, the
+cross-segment ILU(0) factor values and SolvePrecond bit for bit; Solve_PCG
+within one iteration and 1e-8, Solve_PBiCGSTAB within 25 % and 1e-6 (the
+dot products are device reductions)."""
+import numpy as np
+import pytest
+
+from delfem_amd import meshgen
+from oracle.oracle import OrcMS, split_problem
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "solid3d_uv_w": (lambda: meshgen.solid3d(4, seed=7, sigma=0.5), [[0, 1], [2]], "pcg"),
+    "solid3d_u_v_w": (lambda: meshgen.solid3d(4, seed=7, sigma=0.5), [[0], [1], [2]], "pcg"),
+    "oseen2d_uv_p": (lambda: meshgen.oseen2d(12, seed=7), [[0, 1], [2]], "bicg"),
+}
+
+
+def setup(orc, gpu, name):
+    make, groups, kind = CASES[name]
+    prob = make()
+    print(f"[case] {name}", flush=True)
+    S = gpu.MultiSegmentSystem(prob, groups)
+    nres = S.assemble()
+    sp = split_problem(orc, prob, groups)
+    O = OrcMS(orc, sp, prob.conn)
+    return prob, groups, kind, S, nres, O
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_blocks_residual_matvec(orc, gpu, name):
+    import torch
+    prob, groups, kind, S, nres, O = setup(orc, gpu, name)
+    for i in range(len(groups)):
+        val, dia = S.dia[i].values()
+        assert np.array_equal(val, O.val[(i, i)]) and np.array_equal(dia, O.dia[i]), i
+        for j in range(len(groups)):
+            if i != j:
+                assert np.array_equal(S.nd[(i, j)].values(), O.val[(i, j)]), (i, j)
+    assert np.array_equal(S.res.cpu().numpy(), O.res)
+    assert nres == pytest.approx(np.linalg.norm(O.res), rel=1e-13)
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal(O.res.size), rng.standard_normal(O.res.size)
+    yd = torch.from_numpy(y.copy()).cuda()
+    S.MATVEC(-0.7, torch.from_numpy(x).cuda(), 0.3, yd)
+    assert np.array_equal(yd.cpu().numpy(), O.matvec(-0.7, x, 0.3, y))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_cross_segment_ilu(orc, gpu, name):
+    import torch
+    prob, groups, kind, S, nres, O = setup(orc, gpu, name)
+    M = S.preconditioner()
+    assert M.SetValue() and O.ilu()
+    for i in range(len(groups)):
+        for j in range(len(groups)):
+            if i == j:
+                fv, fd = M.values(i, i)
+                ofv, ofd = O.factor(i, i)
+                assert np.array_equal(fv, ofv) and np.array_equal(fd, ofd), (i, j)
+            else:
+                assert np.array_equal(M.values(i, j), O.factor(i, j)), (i, j)
+    x = np.random.default_rng(2).standard_normal(O.res.size)
+    xd = torch.from_numpy(x.copy()).cuda()
+    M.SolvePrecond(xd)
+    assert np.array_equal(xd.cpu().numpy(), O.precond(x))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_segment_solvers(orc, gpu, name):
+    import torch
+    prob, groups, kind, S, nres, O = setup(orc, gpu, name)
+    M = S.preconditioner()
+    M.SetValue()
+    O.ilu()
+    r = S.res.clone()
+    x = torch.zeros_like(r)
+    ok, conv, it = M.solve(kind, 1e-10, 3000, r, x)
+    ok_o, conv_o, it_o, x_o, r_o = O.solve(kind == "bicg", O.res, 1e-10, 3000)
+    xh = x.cpu().numpy()
+    print(f"  device it={it} conv={conv:.3e}  oracle it={it_o} conv={conv_o:.3e}  "
+          f"|x-x_o|/|x_o|={np.linalg.norm(xh - x_o) / np.linalg.norm(x_o):.2e}", flush=True)
+    assert ok and ok_o and conv < 1e-10
+    if kind == "pcg":
+        assert abs(it - it_o) <= 1
+        assert np.linalg.norm(xh - x_o) <= 1e-8 * np.linalg.norm(x_o)
+    else:
+        assert abs(it - it_o) <= max(2, int(np.ceil(0.25 * it_o)))
+        assert np.linalg.norm(xh - x_o) <= 1e-6 * np.linalg.norm(x_o)
