@@ -15,7 +15,8 @@ cfg2 (elasticity 48^3, FP64 ILU(0)-PCG to 1e-10), cfg4 (Oseen P1-P1 PSPG
 lid-driven cavity 256^2, ILU(1)-BiCGSTAB to 1e-9), cfg5 (512 independent
 Helmholtz systems, batched ILU(0)-PCG to 1e-8 in one persistent launch),
 zhelm (complex twin: complex Helmholtz with Sommerfeld lines 1024^2,
-ILU(0)-PCOCG to 1e-8).
+ILU(0)-PCOCG to 1e-8), mseg (elasticity 48^3 split into two segments,
+ILU(0) across segments, PCG to 1e-10).
 
 Prints one JSON line:  python bench.py [--steps K] [--warmup W] [--config cfg3]
 """
@@ -54,6 +55,10 @@ CONFIGS = {
                   workload="complex twin: Helmholtz P1 triangles + Sommerfeld radiation lines on the unit square, "
                            "1024x1024 cells (1,050,625 nodes), wave_length 1.0, incoming wave u = 1 on x = 0, "
                            "ILU(0)-PCOCG to 1e-8"),
+    "mseg": dict(kind="mseg", n=48, sigma=0.0, groups=[[0, 1], [2]], tol=1e-10, maxit=5000,
+                 workload="multi-segment system: 3-D linear elasticity 48^3 (117,649 nodes) split into an (ux, uy) "
+                          "segment and a uz segment, ILU(0) across segments (nodes multicolour-numbered), "
+                          "PCG to 1e-10"),
     "cfg4": dict(kind="oseen2d", n=256, tol=1e-9, lev=1, ordering="block_multicolour", block_nodes=16,
                  solver="bicg", mixed="fac32", others=("fac16",),
                  workload="Oseen P1-P1 PSPG lid-driven cavity, 256x256 cells (66,049 nodes, 198,147 DOF), "
@@ -115,6 +120,8 @@ def main():
         return bench_batch(cfg, args, torch, linsys, _lib)
     if cfg["kind"] == "zhelmholtz":
         return bench_complex(cfg, args, torch, linsys, _lib)
+    if cfg["kind"] == "mseg":
+        return bench_mseg(cfg, args, torch, linsys, _lib)
     stream = torch.cuda.ExternalStream(int(_lib.lib().dfm_stream()))
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     log(f"[bench] {args.config}: {cfg['workload']}")
@@ -443,6 +450,108 @@ def bench_complex(cfg, args, torch, linsys, _lib):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_mseg(cfg, args, torch, linsys, _lib):
+    """Multi-segment workload: every step = cross-segment ILU(0) SetValue +
+    Solve_PCG on the segments (host-driven loop over device kernels)."""
+    import numpy as np
+    from delfem_amd import meshgen
+    log(f"[bench] {args.config}: {cfg['workload']}")
+    t0 = time.time()
+    prob = meshgen.solid3d(cfg["n"], seed=args.seed, sigma=cfg["sigma"])
+    runs = {}
+    for ordering in ("multicolour", "natural"):
+        S = linsys.MultiSegmentSystem(prob, cfg["groups"], ordering=ordering)
+        S.assemble()
+        rhs = S.res.clone()
+        M = S.preconditioner()
+        log(f"[bench] {ordering}: mesh + split assembly + symbolic {time.time() - t0:.1f}s")
+
+        def step():
+            M.SetValue()
+            r = rhs.clone()
+            x = torch.zeros_like(r)
+            return M.solve("pcg", cfg["tol"], cfg["maxit"], r, x)
+
+        steps = args.steps if ordering == "multicolour" else 1
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            ok, conv, it = step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t1) * 1e3 / steps
+        runs[ordering] = (ms, ok, conv, it + 1)
+        log(f"[bench] {ordering} ILU(0) across segments + PCG: {ms:.2f} ms/solve, {it + 1} iterations, "
+            f"conv {conv:.3e}, ok {ok}")
+        t0 = time.time()
+    ms, ok, conv, iters = runs["multicolour"]
+    it = iters - 1
+    cpu = None
+    if not args.no_cpu:
+        # the reference's natural numbering, scaled to its own iteration count
+        cpu = cpu_baseline_mseg(prob, cfg, args, runs["natural"][3])
+    out = {
+        "metric": "ms per multi-segment ILU(0)-PCG solve to 1e-10 (host-driven loop)",
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "config": args.config, "tol": cfg["tol"],
+                   "segments": [len(g) for g in cfg["groups"]],
+                   "ordering": "nodes renumbered by multicolouring (value); natural beside it"},
+        "natural_ms": round(runs["natural"][0], 3),
+        "natural_iterations": runs["natural"][3],
+        "iterations": it + 1,
+        "converged": bool(ok),
+        "conv_ratio": conv,
+        "ms_per_iteration": round(ms / (it + 1), 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_mseg(prob, cfg, args, iters):
+    """The reference's multi-segment CLinearSystem (compiled from its sources)
+    on one core: CPreconditioner_ILU(0) across segments (symbolic + numeric)
+    and a sample of Solve_PCG iterations, scaled to the device's count."""
+    from oracle import oracle
+    try:
+        ref = oracle.Ref()
+    except RuntimeError:
+        return None
+    orc = oracle.Orc()
+    sp = oracle.split_problem(orc, prob, cfg["groups"])
+    R = oracle.RefMS(ref, sp, prob.conn)
+    rhs = R.vector(-1)
+    t1 = time.time()
+    R.ilu()
+    t_ilu = time.time() - t1
+    t1 = time.time()
+    R.solve(False, rhs, cfg["tol"], args.cpu_iters)
+    t_it = (time.time() - t1) / args.cpu_iters
+    log(f"[bench] cpu reference: ILU {t_ilu:.2f}s, {1e3 * t_it:.1f} ms/iteration, scaled to {iters} iterations")
+    return {
+        "value": round(1e3 * (t_ilu + t_it * iters), 1),
+        "unit": "ms per solve (ILU + iterations x sampled ms/iteration)",
+        "cores": 1,
+        "kind": "reference",
+        "sample": (f"the same split system: CPreconditioner_ILU(0) across the segments (symbolic + numeric), then "
+                   f"{args.cpu_iters} Solve_PCG iterations on one core, scaled to {iters} iterations (the device's "
+                   f"count with the reference's natural numbering)"),
+        "ilu_s": round(t_ilu, 3),
+        "ms_per_iteration": round(1e3 * t_it, 2),
+        "iterations": iters,
+    }
 
 
 def cpu_baseline_complex(prob, S, cfg, args, iters):

@@ -504,8 +504,23 @@ class MultiSegmentSystem:
     conditions per block.  Vectors are the segments' vectors one after the
     other (torch tensors on the device)."""
 
-    def __init__(self, prob, groups):
+    def __init__(self, prob, groups, ordering="natural"):
+        """ordering "multicolour": the mesh nodes are first renumbered by the
+        greedy multicolouring of the node graph (the same numbering for every
+        segment), so that each segment's natural-order ILU(0) is a colour-
+        ordered one with one level set per colour; vectors are then in that
+        numbering (self.node_order[new] = old node)."""
         torch = _torch()
+        self.node_order = None
+        if ordering == "multicolour":
+            rp0, cl0 = fem_pattern(prob.nnode, prob.conn)
+            A0 = CMatDia_BlkCrs(prob.nnode, 1)
+            A0.AddPattern(rp0, cl0)
+            self.node_order = CMatDiaFrac_BlkCrs(0, A0, ordering="multicolour").pattern()["order"].copy()
+            from . import meshgen
+            prob = meshgen.relabel(prob, self.node_order)
+        elif ordering != "natural":
+            raise ValueError(f"unknown ordering {ordering}")
         self.prob, self.groups = prob, [list(g) for g in groups]
         self.fem = FEMSystem(prob)
         nb, B = prob.nnode, prob.b

@@ -103,6 +103,28 @@ class Problem:
         return self.nnode * self.b
 
 
+NODAL_FIELDS = ("coord", "flags", "value", "adv")
+
+
+def relabel(prob, order):
+    """The same problem with its nodes renumbered, new node k = old node
+    order[k]: nodal arrays permuted, connectivity mapped.  Element data and
+    element order are unchanged, so every element matrix is the same."""
+    order = np.asarray(order, dtype=np.int64)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    kw = dict(prob.__dict__)
+    for k in NODAL_FIELDS:
+        if k in kw and kw[k] is not None:
+            a = np.asarray(kw[k])            # (nnode, ...) or raveled (nnode * b)
+            kw[k] = np.ascontiguousarray(a.reshape(prob.nnode, -1)[order].reshape(a.shape))
+    for k in ("conn", "lconn"):
+        if k in kw and kw[k] is not None:
+            kw[k] = np.ascontiguousarray(inv[kw[k]].astype(np.asarray(kw[k]).dtype))
+    kw["name"] = prob.name + "_relabelled"
+    return Problem(**kw)
+
+
 def poisson2d(n=128, seed=12345, sigma=0.5, source=1.0):
     """2-D Poisson (scalar2d path): unit square, n x n cells, lognormal diffusivity
     per element, f = source, u = 0 on x = 0."""
