@@ -541,9 +541,18 @@ static int upload_sell_plan(const SellPlan& plan, int bb, Sell<double>& s64, Dev
   return DFM_OK;
 }
 
-int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* order, unsigned block_nodes,
-                   dfm_ilu** out) {
+}  // extern "C"
+
+namespace dfm {
+// dfm_ilu_create; with given_row / given_col (natural order only) the factor
+// pattern is that one instead of the ILU(k) symbolic of A's: it must hold
+// A's pattern, and its extra entries start at zero
+int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* order, unsigned block_nodes,
+                    const std::vector<unsigned>* given_row, const std::vector<unsigned>* given_col, dfm_ilu** out) {
   DFM_CHECK(A && out && lev_fill >= -1, DFM_BAD_ARG, "bad arguments to dfm_ilu_create");
+  DFM_CHECK(!given_row || (given_col && ordering == DFM_ORDER_NATURAL && given_row->size() == A->nb + 1 &&
+                           given_col->size() == given_row->back()),
+            DFM_BAD_ARG, "bad given factor pattern");
   DFM_CHECK(ordering == DFM_ORDER_NATURAL || ordering == DFM_ORDER_GIVEN || ordering == DFM_ORDER_MULTICOLOUR ||
                 ordering == DFM_ORDER_BLOCK_MULTICOLOUR,
             DFM_BAD_ARG, "unknown ordering %d", ordering);
@@ -581,8 +590,24 @@ int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* ord
   }
   M->inv.assign(nb, 0);
   for (unsigned i = 0; i < nb; i++) M->inv[M->order[i]] = (int)i;
-  DFM_TRY(iluk_symbolic(nb, A->rowptr, A->col, lev_fill, M->permuted ? &M->order : nullptr, M->frow, M->fcol,
-                        M->flev, M->fdiaind));
+  if (given_row) {
+    M->frow = *given_row;
+    M->fcol = *given_col;
+    M->flev.assign(M->fcol.size(), 0);
+    M->fdiaind.assign(nb, 0);
+    for (unsigned i = 0; i < nb; i++) {
+      const auto b0 = M->fcol.begin() + M->frow[i], b1 = M->fcol.begin() + M->frow[i + 1];
+      DFM_CHECK(std::is_sorted(b0, b1) && std::adjacent_find(b0, b1) == b1 && std::find(b0, b1, i) == b1,
+                DFM_BAD_ARG, "given factor pattern: row %u not strictly ascending or holds the diagonal", i);
+      for (unsigned q = A->rowptr[i]; q < A->rowptr[i + 1]; q++)
+        DFM_CHECK(std::binary_search(b0, b1, A->col[q]), DFM_BAD_ARG,
+                  "given factor pattern: row %u lacks column %u of the matrix", i, A->col[q]);
+      M->fdiaind[i] = (unsigned)(std::upper_bound(b0, b1, i) - M->fcol.begin());
+    }
+  } else {
+    DFM_TRY(iluk_symbolic(nb, A->rowptr, A->col, lev_fill, M->permuted ? &M->order : nullptr, M->frow, M->fcol,
+                          M->flev, M->fdiaind));
+  }
   const size_t nf = M->fcol.size();
   // factor entry -> block of A
   std::vector<long long> fsrc(nf, -1);
@@ -679,6 +704,14 @@ int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* ord
   }
   *out = M.release();
   return DFM_OK;
+}
+}  // namespace dfm
+
+extern "C" {
+
+int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* order, unsigned block_nodes,
+                   dfm_ilu** out) {
+  return dfm::ilu_create_impl(A, lev_fill, ordering, order, block_nodes, nullptr, nullptr, out);
 }
 
 int dfm_ilu_destroy(dfm_ilu* M) {

@@ -140,6 +140,10 @@ struct KState {
   int iter, conv_iter, maxit;
 };
 
+// dfm_ilu_create, or (given_row / given_col, natural order) a factor on a
+// pattern computed by the caller that holds A's (cross-segment fill).
+int ilu_create_impl(const dfm_bcrs_s* A, int lev_fill, int ordering, const int* order, unsigned block_nodes,
+                    const std::vector<unsigned>* given_row, const std::vector<unsigned>* given_col, dfm_ilu_s** out);
 // Numeric ILU in two steps (dfm_ilu_set_value = copy + factor): between
 // them a caller may change the copied values (cross-segment updates).
 int ilu_numeric_copy(dfm_ilu_s* M, const dfm_bcrs_s* A);

@@ -2,10 +2,11 @@
 // several segments, src/ls/linearsystem.cpp): diagonal blocks are the
 // library's block-CRS matrices (one block size per segment), off-diagonal
 // blocks rectangular CMat_BlkCrs (dfm_bmat); vectors are the segments'
-// vectors one after the other.  The preconditioner is CPreconditioner_ILU(0)
+// vectors one after the other.  The preconditioner is CPreconditioner_ILU(k)
 // across the segments (src/ls/preconditioner.cpp:57-335): per segment the
-// natural-order ILU(0) of its diagonal block, per off-diagonal block a
-// CMatFrac_BlkCrs copy updated by DoILUDecompUp / Low / LowUp
+// natural-order ILU of its diagonal block, per off-diagonal block a
+// CMatFrac_BlkCrs copy (fill patterns from the host symbolic below) updated
+// by DoILUDecompUp / Low / LowUp
 // (src/matvec/matfrac_blkcrs.cpp:540-783) and the diagonal blocks by
 // CMatDiaFrac_BlkCrs::DoILUDecompLowUp (matdiafrac_blkcrs.cpp:665-767), in
 // the reference's block order and operation order.  The Krylov loops
@@ -32,8 +33,11 @@ struct dfm_mseg_s {
 
 struct dfm_msilu_s {
   dfm_mseg_s* S = nullptr;
-  std::vector<dfm_ilu_s*> dia;                      // owned: ILU(0), natural order
-  std::vector<std::vector<dfm_bmat_s*>> F;          // owned: copies of the off-diagonal blocks
+  int lev_fill = 0;
+  std::vector<dfm_ilu_s*> dia;                      // owned: natural order, pattern with cross-segment fill
+  std::vector<std::vector<dfm_bmat_s*>> F;          // owned: off-diagonal factor blocks (block pattern + fill)
+  std::vector<std::vector<std::vector<unsigned>>> src_col;  // the blocks' patterns the factor was built on
+  std::vector<std::vector<DevBuf<long long>>> src;  // factor entry -> block entry, -1 for fill
   std::vector<double*> work;                        // Krylov vectors
   ~dfm_msilu_s() {
     for (auto* m : dia)
@@ -158,6 +162,183 @@ __global__ void __launch_bounds__(kNT) k_ms_lowup_dia(int nrow, const int* __res
   }
 }
 
+// SetValue_Initialize of an off-diagonal factor block (CMat_BlkCrs::SetValue,
+// mat_blkcrs.cpp:557-694): the block's values where it has the entry, 0 on fill
+__global__ void __launch_bounds__(kNT) k_ms_init(long n, int bs, const long long* __restrict__ src,
+                                                const double* __restrict__ bval, double* __restrict__ fval) {
+  const long t = (long)blockIdx.x * kNT + threadIdx.x;
+  if (t >= n) return;
+  const long q = t / bs, e = t % bs;
+  const long long s = src[q];
+  fval[t] = s >= 0 ? bval[s * bs + e] : 0.0;
+}
+
+// ---- symbolic factorisation across segments (host) ---------------------------
+// A block pattern with fill levels (the CRowLev lists of the reference's
+// CMatFrac_BlkCrs / CMatDiaFrac_BlkCrs); diagonal patterns hold no diagonal.
+struct LevPat {
+  unsigned nrow = 0;
+  std::vector<unsigned> rp, cl;
+  std::vector<int> lev;
+  void from(unsigned n, const std::vector<unsigned>& r, const std::vector<unsigned>& c) {
+    nrow = n;
+    rp = r;
+    cl = c;
+    lev.assign(c.size(), 0);
+  }
+  bool full(unsigned ncol) const { return (unsigned long long)cl.size() == (unsigned long long)nrow * ncol; }
+  void diaind(std::vector<unsigned>& d) const {
+    d.resize(nrow);
+    for (unsigned r = 0; r < nrow; r++)
+      d[r] = (unsigned)(std::upper_bound(cl.begin() + rp[r], cl.begin() + rp[r + 1], r) - cl.begin());
+  }
+};
+// One row being extended, as a sorted linked list: an inserted column keeps
+// the level it was first given (CListRLN::Insert, matfrac_blkcrs.h:153-255).
+struct RowList {
+  struct Node {
+    unsigned c;
+    int lev, next;
+  };
+  std::vector<Node> v;
+  int head = -1;
+  void init(const LevPat& P, unsigned r) {
+    v.clear();
+    for (unsigned q = P.rp[r]; q < P.rp[r + 1]; q++) v.push_back({P.cl[q], P.lev[q], (int)v.size() + 1});
+    head = v.empty() ? -1 : 0;
+    if (!v.empty()) v.back().next = -1;
+  }
+  void insert(unsigned j, int lev, int& cur) {
+    if (head < 0 || j < v[head].c) {
+      v.push_back({j, lev, head});
+      head = cur = (int)v.size() - 1;
+      return;
+    }
+    int c = (cur < 0 || v[cur].c > j) ? head : cur;
+    while (v[c].next >= 0 && v[v[c].next].c <= j) c = v[c].next;
+    if (v[c].c == j) {
+      cur = c;
+      return;
+    }
+    v.push_back({j, lev, v[c].next});
+    v[c].next = (int)v.size() - 1;
+    cur = v[c].next;
+  }
+  void emit(LevPat& P) const {
+    for (int t = head; t >= 0; t = v[t].next) {
+      P.cl.push_back(v[t].c);
+      P.lev.push_back(v[t].lev);
+    }
+    P.rp.push_back((unsigned)P.cl.size());
+  }
+};
+inline bool lev_skip(int l, int K) { return K != -1 && l + 1 > K; }
+inline LevPat lp_start(unsigned nrow, size_t hint) {
+  LevPat P;
+  P.nrow = nrow;
+  P.rp.reserve(nrow + 1);
+  P.rp.push_back(0);
+  P.cl.reserve(hint);
+  P.lev.reserve(hint);
+  return P;
+}
+// AddFracPtnLowUp: T gains L_ik U_kj fill (matfrac_blkcrs.cpp:156-249; the
+// diagonal-block form, matdiafrac_blkcrs.cpp:1105-1247, leaves out column r)
+void lp_lowup(LevPat& T, unsigned ncol, const LevPat& L, const LevPat& U, int K, bool skip_diag, RowList& rl) {
+  if (T.full(ncol) || K == 0) return;
+  LevPat P = lp_start(T.nrow, T.cl.size() * 2);
+  for (unsigned r = 0; r < T.nrow; r++) {
+    rl.init(T, r);
+    for (unsigned ik = L.rp[r]; ik < L.rp[r + 1]; ik++) {
+      const int lik = L.lev[ik];
+      if (lev_skip(lik, K)) continue;
+      const unsigned k = L.cl[ik];
+      int cur = rl.head;
+      for (unsigned kj = U.rp[k]; kj < U.rp[k + 1]; kj++) {
+        const int lkj = U.lev[kj];
+        if (lev_skip(lkj, K) || (skip_diag && U.cl[kj] == r)) continue;
+        rl.insert(U.cl[kj], std::max(lik, lkj) + 1, cur);
+      }
+    }
+    rl.emit(P);
+  }
+  T = std::move(P);
+}
+// AddFracPtnUp (matfrac_blkcrs.cpp:251-362): U_ij row r gains L_ii(r, k)
+// U_ij(k, :) fill from the rows k < r already extended
+void lp_up(LevPat& T, unsigned ncol, const LevPat& D, int K, RowList& rl) {
+  if (T.full(ncol) || K == 0) return;
+  std::vector<unsigned> dind;
+  D.diaind(dind);
+  LevPat P = lp_start(T.nrow, T.cl.size() * 2);
+  for (unsigned r = 0; r < T.nrow; r++) {
+    rl.init(T, r);
+    for (unsigned ik = D.rp[r]; ik < dind[r]; ik++) {
+      const int lik = D.lev[ik];
+      if (lev_skip(lik, K)) continue;
+      const unsigned k = D.cl[ik];
+      int cur = rl.head;
+      for (unsigned kj = P.rp[k]; kj < P.rp[k + 1]; kj++) {
+        const int lkj = P.lev[kj];
+        if (lev_skip(lkj, K)) continue;
+        rl.insert(P.cl[kj], std::max(lik, lkj) + 1, cur);
+      }
+    }
+    rl.emit(P);
+  }
+  T = std::move(P);
+}
+// AddFracPtnLow (matfrac_blkcrs.cpp:364-495): L_ij row r, walked in column
+// order while it grows, gains L_ij(r, k) U_jj(k, :) fill
+void lp_low(LevPat& T, unsigned ncol, const LevPat& D, int K, RowList& rl) {
+  if (T.full(ncol) || K == 0) return;
+  std::vector<unsigned> dind;
+  D.diaind(dind);
+  LevPat P = lp_start(T.nrow, T.cl.size() * 2);
+  for (unsigned r = 0; r < T.nrow; r++) {
+    rl.init(T, r);
+    for (int t = rl.head; t >= 0; t = rl.v[t].next) {
+      const int lik = rl.v[t].lev;
+      if (lev_skip(lik, K)) continue;
+      const unsigned k = rl.v[t].c;
+      int cur = t;
+      for (unsigned kj = dind[k]; kj < D.rp[k + 1]; kj++) {
+        const int lkj = D.lev[kj];
+        if (lev_skip(lkj, K)) continue;
+        rl.insert(D.cl[kj], std::max(lik, lkj) + 1, cur);
+      }
+    }
+    rl.emit(P);
+  }
+  T = std::move(P);
+}
+// AddFracPtn of a diagonal block (matdiafrac_blkcrs.cpp:1249-1377) from the
+// levels it already holds
+void lp_dia_fill(LevPat& D, int K, RowList& rl) {
+  const unsigned nb = D.nrow;
+  if (D.cl.empty() || D.full(nb - 1) || K == 0) return;
+  LevPat P = lp_start(nb, D.cl.size() * 4);
+  std::vector<unsigned> dind(nb);
+  for (unsigned r = 0; r < nb; r++) {
+    rl.init(D, r);
+    for (int t = rl.head; t >= 0; t = rl.v[t].next) {
+      const int lik = rl.v[t].lev;
+      if (lev_skip(lik, K)) continue;
+      const unsigned k = rl.v[t].c;
+      if (k >= r) break;
+      int cur = t;
+      for (unsigned kj = dind[k]; kj < P.rp[k + 1]; kj++) {
+        const int lkj = P.lev[kj];
+        if (lev_skip(lkj, K) || P.cl[kj] == r) continue;
+        rl.insert(P.cl[kj], std::max(lik, lkj) + 1, cur);
+      }
+    }
+    rl.emit(P);
+    dind[r] = (unsigned)(std::upper_bound(P.cl.begin() + P.rp[r], P.cl.end(), r) - P.cl.begin());
+  }
+  D = std::move(P);
+}
+
 int seg_dot(dfm_mseg_s* S, const double* x, const double* y, double* out) {
   double s = 0.0;
   for (unsigned i = 0; i < S->nseg; i++) {
@@ -226,22 +407,77 @@ int dfm_mseg_dot(dfm_mseg* S, const double* x, const double* y, double* out) {
   return seg_dot(S, x, y, out);
 }
 
-int dfm_mseg_ilu_create(dfm_mseg* S, dfm_msilu** out) {
-  DFM_CHECK(S && out, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_create");
+// CPreconditioner_ILU(ls, lev_fill) across the segments: the symbolic part
+// of SetLinearSystem (preconditioner.cpp:57-211).  Every pattern starts as
+// its block's with level 0; for lev_fill != 0 they are extended in (i, j)
+// order -- diagonal block: LowUp with each k < i, then its own fill; above
+// the diagonal: LowUp with k < i, then Up with factor i; below: LowUp with
+// k < j, then Low with factor j.
+int dfm_mseg_ilu_create(dfm_mseg* S, int lev_fill, dfm_msilu** out) {
+  DFM_CHECK(S && out && lev_fill >= -1, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_create");
+  const unsigned n = S->nseg;
   std::unique_ptr<dfm_msilu_s> M(new dfm_msilu_s);
   M->S = S;
-  M->dia.assign(S->nseg, nullptr);
-  M->F.assign(S->nseg, std::vector<dfm_bmat_s*>(S->nseg, nullptr));
-  for (unsigned i = 0; i < S->nseg; i++) {
+  M->lev_fill = lev_fill;
+  M->dia.assign(n, nullptr);
+  M->F.assign(n, std::vector<dfm_bmat_s*>(n, nullptr));
+  M->src_col.assign(n, std::vector<std::vector<unsigned>>(n));
+  M->src.resize(n);
+  for (auto& row : M->src) row = std::vector<DevBuf<long long>>(n);
+  std::vector<LevPat> D(n);
+  std::vector<std::vector<LevPat>> F(n, std::vector<LevPat>(n));
+  for (unsigned i = 0; i < n; i++) {
     DFM_CHECK(S->dia[i], DFM_BAD_ARG, "segment %u has no diagonal block", i);
-    DFM_TRY(dfm_ilu_create(S->dia[i], 0, DFM_ORDER_NATURAL, nullptr, 0, &M->dia[i]));
-    for (unsigned j = 0; j < S->nseg; j++) {
+    D[i].from(S->nb[i], S->dia[i]->rowptr, S->dia[i]->col);
+    for (unsigned j = 0; j < n; j++)
+      if (S->nd[i][j]) F[i][j].from(S->nb[i], S->nd[i][j]->rowptr, S->nd[i][j]->col);
+  }
+  if (lev_fill != 0) {
+    RowList rl;
+    for (unsigned i = 0; i < n; i++)
+      for (unsigned j = 0; j < n; j++) {
+        if (i == j) {
+          for (unsigned k = 0; k < i; k++)
+            if (S->nd[i][k] && S->nd[k][i]) lp_lowup(D[i], S->nb[i] - 1, F[i][k], F[k][i], lev_fill, true, rl);
+          lp_dia_fill(D[i], lev_fill, rl);
+          continue;
+        }
+        const unsigned kmax = std::min(i, j);
+        for (unsigned k = 0; k < kmax + 1; k++) {
+          const bool have = S->nd[i][j] != nullptr;
+          if (k == i && k < j) {
+            if (have) lp_up(F[i][j], S->nb[j], D[k], lev_fill, rl);
+            continue;
+          }
+          if (k == j && k < i) {
+            if (have) lp_low(F[i][j], S->nb[j], D[k], lev_fill, rl);
+            continue;
+          }
+          if (!S->nd[i][k] || !S->nd[k][j]) continue;
+          DFM_CHECK(have, DFM_BAD_ARG, "fill needs block (%u, %u), which the system does not have", i, j);
+          lp_lowup(F[i][j], S->nb[j], F[i][k], F[k][j], lev_fill, false, rl);
+        }
+      }
+  }
+  for (unsigned i = 0; i < n; i++) {
+    DFM_TRY(ilu_create_impl(S->dia[i], lev_fill, DFM_ORDER_NATURAL, nullptr, 0, &D[i].rp, &D[i].cl, &M->dia[i]));
+    for (unsigned j = 0; j < n; j++) {
       const dfm_bmat_s* B = S->nd[i][j];
       if (!B) continue;
+      const LevPat& P = F[i][j];
       DFM_TRY(dfm_bmat_create(B->nb, B->b, B->nbr, B->br, &M->F[i][j]));
-      std::vector<unsigned> rp = B->rowptr, cl = B->col;
+      std::vector<unsigned> rp = P.rp, cl = P.cl;
       if (cl.empty()) cl.push_back(0);
       DFM_TRY(dfm_bmat_add_pattern(M->F[i][j], B->nb, rp.data(), cl.data()));
+      DFM_CHECK(M->F[i][j]->col == P.cl, DFM_BAD_ARG, "block (%u, %u): factor pattern not kept", i, j);
+      std::vector<long long> src(P.cl.size(), -1);
+      for (unsigned r = 0; r < B->nb; r++)
+        for (unsigned q = B->rowptr[r]; q < B->rowptr[r + 1]; q++) {
+          const auto it = std::lower_bound(P.cl.begin() + P.rp[r], P.cl.begin() + P.rp[r + 1], B->col[q]);
+          src[it - P.cl.begin()] = q;
+        }
+      DFM_TRY(M->src[i][j].upload(src));
+      M->src_col[i][j] = B->col;
     }
   }
   *out = M.release();
@@ -264,11 +500,9 @@ int dfm_mseg_ilu_set_value(dfm_msilu* M, int* nsingular) {
     for (unsigned j = 0; j < n; j++) {
       dfm_bmat_s* F = M->F[i][j];
       if (!F) continue;
-      const size_t nv = F->col.size() * F->b * F->br;
-      DFM_CHECK(S->nd[i][j]->col == F->col, DFM_BAD_ARG, "block (%u, %u): pattern changed", i, j);
-      if (nv)
-        DFM_HIP(hipMemcpyAsync(F->d_val.p, S->nd[i][j]->d_val.p, nv * sizeof(double), hipMemcpyDeviceToDevice,
-                               stream()));
+      const long bs = (long)F->b * F->br, nv = (long)F->col.size() * bs;
+      DFM_CHECK(S->nd[i][j]->col == M->src_col[i][j], DFM_BAD_ARG, "block (%u, %u): pattern changed", i, j);
+      if (nv) DFM_MLAUNCH(k_ms_init, mgrid(nv), nv, (int)bs, M->src[i][j].p, S->nd[i][j]->d_val.p, F->d_val.p);
     }
   }
   // the blocks in (i, j) order (:253-297)
@@ -322,6 +556,24 @@ int dfm_mseg_ilu_set_value(dfm_msilu* M, int* nsingular) {
     }
   DFM_HIP(hipStreamSynchronize(stream()));
   if (nsingular) *nsingular = nsing_total;
+  return DFM_OK;
+}
+int dfm_mseg_ilu_get_pattern(const dfm_msilu* M, unsigned i, unsigned j, unsigned* ncrs, unsigned* rowptr,
+                             unsigned* col) {
+  DFM_CHECK(M && i < M->S->nseg && j < M->S->nseg && ncrs, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_get_pattern");
+  const std::vector<unsigned>* rp;
+  const std::vector<unsigned>* cl;
+  if (i == j) {
+    rp = &M->dia[i]->frow;
+    cl = &M->dia[i]->fcol;
+  } else {
+    DFM_CHECK(M->F[i][j], DFM_BAD_ARG, "no block (%u, %u)", i, j);
+    rp = &M->F[i][j]->rowptr;
+    cl = &M->F[i][j]->col;
+  }
+  *ncrs = (unsigned)cl->size();
+  if (rowptr) std::memcpy(rowptr, rp->data(), sizeof(unsigned) * rp->size());
+  if (col && !cl->empty()) std::memcpy(col, cl->data(), sizeof(unsigned) * cl->size());
   return DFM_OK;
 }
 int dfm_mseg_ilu_get_values(const dfm_msilu* M, unsigned i, unsigned j, double* val, double* dia) {

@@ -606,17 +606,18 @@ class MultiSegmentSystem:
         _sync_torch()
         call("dfm_mseg_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
 
-    def preconditioner(self):
-        return MultiSegmentILU(self)
+    def preconditioner(self, lev=0):
+        return MultiSegmentILU(self, lev)
 
 
 class MultiSegmentILU:
-    """CPreconditioner_ILU(0) across the segments (src/ls/preconditioner.cpp:57-335)."""
+    """CPreconditioner_ILU(ls, lev) across the segments
+    (src/ls/preconditioner.cpp:57-335): lev 0 no fill, -1 unlimited."""
 
-    def __init__(self, S):
+    def __init__(self, S, lev=0):
         h = C.c_void_p()
-        call("dfm_mseg_ilu_create", S.h, C.byref(h))
-        self.h, self.S = h, S
+        call("dfm_mseg_ilu_create", S.h, int(lev), C.byref(h))
+        self.h, self.S, self.lev = h, S, lev
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -631,15 +632,25 @@ class MultiSegmentILU:
         rc = call("dfm_mseg_ilu_set_value", self.h, C.byref(ns), ok=(_lib.DFM_OK, _lib.DFM_SINGULAR))
         return rc == _lib.DFM_OK
 
+    def pattern(self, i, j):
+        """Factor pattern (rowptr, col) of block (i, j), fill included."""
+        n = C.c_uint()
+        call("dfm_mseg_ilu_get_pattern", self.h, i, j, C.byref(n), None, None)
+        rp = np.zeros(self.S.prob.nnode + 1, dtype=np.uint32)
+        cl = np.zeros(max(1, n.value), dtype=np.uint32)
+        call("dfm_mseg_ilu_get_pattern", self.h, i, j, C.byref(n), _ptr(rp), _ptr(cl))
+        return rp, cl[:n.value]
+
     def values(self, i, j):
         S = self.S
         nb, bi, bj = S.prob.nnode, S.lens[i], S.lens[j]
+        nc = C.c_uint()
+        call("dfm_mseg_ilu_get_pattern", self.h, i, j, C.byref(nc), None, None)
+        n = int(nc.value)
         if i == j:
-            n = int(S.dia[i].NCrs())
             val, dia = np.zeros(max(1, n * bi * bi)), np.zeros(nb * bi * bi)
             call("dfm_mseg_ilu_get_values", self.h, i, j, _ptr(val), _ptr(dia))
             return val[:n * bi * bi], dia
-        n = int(S.rowptr_nd[-1])
         val = np.zeros(max(1, n * bi * bj))
         call("dfm_mseg_ilu_get_values", self.h, i, j, _ptr(val), None)
         return val[:n * bi * bj]

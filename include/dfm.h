@@ -300,14 +300,20 @@ int dfm_mseg_set_nondia(dfm_mseg *S, unsigned i, unsigned j, dfm_bmat *B);  /* A
 int dfm_mseg_matvec(dfm_mseg *S, double alpha, const double *x_dev, double beta,
                     double *y_dev);                                         /* MATVEC :250-296 */
 int dfm_mseg_dot(dfm_mseg *S, const double *x_dev, const double *y_dev, double *result); /* DOT :335-364 */
-/* CPreconditioner_ILU(0) across the segments (preconditioner.cpp:57-206): natural-order
- * ILU(0) of each diagonal block and a CMatFrac_BlkCrs copy of each off-diagonal block,
- * no fill between segments (the reference's default level 0). */
-int dfm_mseg_ilu_create(dfm_mseg *S, dfm_msilu **out);
+/* CPreconditioner_ILU(ls, lev_fill) across the segments (preconditioner.cpp:57-206, with
+ * SetFillInLevel(lev_fill) for every block): natural-order ILU of each diagonal block and a
+ * CMatFrac_BlkCrs copy of each off-diagonal block, patterns extended by AddFracPtnLowUp /
+ * AddFracPtnUp / AddFracPtnLow / AddFracPtn (matfrac_blkcrs.cpp:156-495,
+ * matdiafrac_blkcrs.cpp:1105-1377) in the reference's block order; lev_fill 0 = no fill,
+ * -1 = unlimited.  Fill that would need a block the system lacks is DFM_BAD_ARG. */
+int dfm_mseg_ilu_create(dfm_mseg *S, int lev_fill, dfm_msilu **out);
 int dfm_mseg_ilu_destroy(dfm_msilu *M);
 /* SetValue (:217-297): copies, then per block in (i, j) order DoILUDecompLowUp + DoILUDecomp
  * (diagonal), DoILUDecompUp / Low / LowUp (off-diagonal); DFM_SINGULAR as dfm_ilu_set_value */
 int dfm_mseg_ilu_set_value(dfm_msilu *M, int *nsingular);
+/* factor pattern of block (i, j): *ncrs blocks; rowptr (nblk_i + 1) and col (ncrs) may be NULL */
+int dfm_mseg_ilu_get_pattern(const dfm_msilu *M, unsigned i, unsigned j, unsigned *ncrs, unsigned *rowptr,
+                             unsigned *col);
 /* factor values of block (i, j): diagonal as dfm_ilu_get_values, off-diagonal ncrs*len_i*len_j */
 int dfm_mseg_ilu_get_values(const dfm_msilu *M, unsigned i, unsigned j, double *val, double *dia);
 int dfm_mseg_ilu_precond(dfm_msilu *M, double *vec_dev);                    /* SolvePrecond :317-333 */

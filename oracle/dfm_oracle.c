@@ -1244,10 +1244,18 @@ static void rmat_pr(double *out, const double *d, unsigned ni, unsigned nj)
 }
 
 #define ORC_MS_MAX 4
+/* a block pattern with fill levels (CRowLev, include/delfem/matvec/matfrac_blkcrs.h:132-150) */
+typedef struct {
+    unsigned nrow;
+    unsigned *rp, *cl;
+    int *lev;
+    size_t cap;
+} orc_lpat;
 typedef struct {
     unsigned nbc, bc, nbr, br;
     const unsigned *rowptr, *col;
     const double *val;
+    orc_lpat fp;   /* factor pattern (the block's pattern plus fill) */
     double *F;
 } orc_msblk;
 struct orc_ms_s {
@@ -1255,11 +1263,252 @@ struct orc_ms_s {
     size_t off[ORC_MS_MAX + 1];
     const unsigned *rowptr[ORC_MS_MAX], *col[ORC_MS_MAX];
     const double *val[ORC_MS_MAX], *dia[ORC_MS_MAX];
+    orc_lpat fp[ORC_MS_MAX];   /* diagonal factors' patterns */
     unsigned *diaind[ORC_MS_MAX];
     double *fval[ORC_MS_MAX], *fdia[ORC_MS_MAX];
     orc_msblk *nd[ORC_MS_MAX][ORC_MS_MAX];
 };
 typedef struct orc_ms_s orc_ms;
+
+static void lp_free(orc_lpat *P) { free(P->rp); free(P->cl); free(P->lev); memset(P, 0, sizeof *P); }
+static void lp_begin(orc_lpat *P, unsigned nrow, size_t cap)
+{
+    P->nrow = nrow;
+    P->cap = cap ? cap : 16;
+    P->rp = (unsigned *)malloc(sizeof(unsigned) * ((size_t)nrow + 1));
+    P->cl = (unsigned *)malloc(sizeof(unsigned) * P->cap);
+    P->lev = (int *)malloc(sizeof(int) * P->cap);
+    P->rp[0] = 0;
+}
+static void lp_push(orc_lpat *P, size_t n, unsigned c, int lev)
+{
+    if (n >= P->cap) {
+        P->cap *= 2;
+        P->cl = (unsigned *)realloc(P->cl, sizeof(unsigned) * P->cap);
+        P->lev = (int *)realloc(P->lev, sizeof(int) * P->cap);
+    }
+    P->cl[n] = c;
+    P->lev[n] = lev;
+}
+/* a CRS pattern with every level 0 (MakePatternInitialize, matfrac_blkcrs.cpp:136-154) */
+static void lp_from(orc_lpat *P, unsigned nrow, const unsigned *rp, const unsigned *cl)
+{
+    lp_begin(P, nrow, rp[nrow] + 1);
+    memcpy(P->rp, rp, sizeof(unsigned) * ((size_t)nrow + 1));
+    memcpy(P->cl, cl, sizeof(unsigned) * rp[nrow]);
+    for (unsigned q = 0; q < rp[nrow]; q++) P->lev[q] = 0;
+}
+/* The sorted row list of the fill routines (CListRLN,
+ * include/delfem/matvec/matfrac_blkcrs.h:153-255): Insert adds (j, lev)
+ * unless j is present, so an entry keeps the level it was first given. */
+typedef struct { orc_node *v; int len, cap, head; } orc_rowlist;
+static void rl_init(orc_rowlist *L, const orc_lpat *P, unsigned r)
+{
+    const int n = (int)(P->rp[r + 1] - P->rp[r]);
+    if (L->cap < n + 1) { L->cap = 2 * n + 16; L->v = (orc_node *)realloc(L->v, sizeof(orc_node) * L->cap); }
+    L->len = n;
+    for (int t = 0; t < n; t++) {
+        L->v[t].colv = P->cl[P->rp[r] + t];
+        L->v[t].lev = P->lev[P->rp[r] + t];
+        L->v[t].next = t + 1;
+    }
+    if (n > 0) { L->v[n - 1].next = -1; L->head = 0; } else L->head = -1;
+}
+static void rl_insert(orc_rowlist *L, unsigned j, int lev, int *cur)
+{
+    if (L->len + 1 > L->cap) { L->cap = 2 * L->cap + 16; L->v = (orc_node *)realloc(L->v, sizeof(orc_node) * L->cap); }
+    if (L->head < 0 || j < L->v[L->head].colv) {
+        L->v[L->len].colv = j; L->v[L->len].lev = lev; L->v[L->len].next = L->head;
+        L->head = *cur = L->len++;
+        return;
+    }
+    int c = *cur;
+    if (c < 0 || L->v[c].colv > j) c = L->head;
+    while (L->v[c].next >= 0 && L->v[L->v[c].next].colv <= j) c = L->v[c].next;
+    if (L->v[c].colv == j) { *cur = c; return; }
+    L->v[L->len].colv = j; L->v[L->len].lev = lev; L->v[L->len].next = L->v[c].next;
+    L->v[c].next = L->len;
+    *cur = L->len++;
+}
+static void rl_emit(const orc_rowlist *L, orc_lpat *P, unsigned r)
+{
+    size_t n = P->rp[r];
+    for (int t = L->head; t >= 0; t = L->v[t].next) lp_push(P, n++, L->v[t].colv, L->v[t].lev);
+    P->rp[r + 1] = (unsigned)n;
+}
+#define LEV_SKIP(l, K) ((K) != -1 && (l) + 1 > (K))
+/* AddFracPtnLowUp: T gains L_ik U_kj fill (CMatFrac_BlkCrs, matfrac_blkcrs.cpp:156-249;
+ * with skip_diag the CMatDiaFrac_BlkCrs one, matdiafrac_blkcrs.cpp:1105-1247,
+ * which leaves out the diagonal).  Full matrices and level 0 are left alone. */
+static void lp_lowup(orc_lpat *T, unsigned ncol_full, const orc_lpat *L, const orc_lpat *U, int K, int skip_diag,
+                     orc_rowlist *rl)
+{
+    if ((unsigned long)T->rp[T->nrow] == (unsigned long)T->nrow * ncol_full || K == 0) return;
+    orc_lpat P;
+    lp_begin(&P, T->nrow, T->rp[T->nrow] * 2 + 16);
+    for (unsigned r = 0; r < T->nrow; r++) {
+        rl_init(rl, T, r);
+        for (unsigned ik = L->rp[r]; ik < L->rp[r + 1]; ik++) {
+            const unsigned k = L->cl[ik];
+            const int lik = L->lev[ik];
+            if (LEV_SKIP(lik, K)) continue;
+            int cur = rl->head;
+            for (unsigned kj = U->rp[k]; kj < U->rp[k + 1]; kj++) {
+                const int lkj = U->lev[kj];
+                if (LEV_SKIP(lkj, K)) continue;
+                const unsigned j = U->cl[kj];
+                if (skip_diag && j == r) continue;
+                rl_insert(rl, j, (lik > lkj ? lik : lkj) + 1, &cur);
+            }
+        }
+        rl_emit(rl, &P, r);
+    }
+    lp_free(T);
+    *T = P;
+}
+/* diaind: first position of a row whose column exceeds the row */
+static void lp_diaind(const orc_lpat *P, unsigned *diaind)
+{
+    for (unsigned r = 0; r < P->nrow; r++) {
+        diaind[r] = P->rp[r + 1];
+        for (unsigned q = P->rp[r]; q < P->rp[r + 1]; q++)
+            if (P->cl[q] > r) { diaind[r] = q; break; }
+    }
+}
+/* AddFracPtnUp (matfrac_blkcrs.cpp:251-362): U_ij row r gains L_ii(r, k) U_ij(k, :)
+ * fill, k from the lower part of factor i, rows k < r already extended */
+static void lp_up(orc_lpat *T, unsigned ncol_full, const orc_lpat *D, int K, orc_rowlist *rl)
+{
+    if ((unsigned long)T->rp[T->nrow] == (unsigned long)T->nrow * ncol_full || K == 0) return;
+    unsigned *dind = (unsigned *)malloc(sizeof(unsigned) * (D->nrow + 1));
+    lp_diaind(D, dind);
+    orc_lpat P;
+    lp_begin(&P, T->nrow, T->rp[T->nrow] * 2 + 16);
+    for (unsigned r = 0; r < T->nrow; r++) {
+        rl_init(rl, T, r);
+        for (unsigned ik = D->rp[r]; ik < dind[r]; ik++) {
+            const unsigned k = D->cl[ik];
+            const int lik = D->lev[ik];
+            if (LEV_SKIP(lik, K)) continue;
+            int cur = rl->head;
+            for (unsigned kj = P.rp[k]; kj < P.rp[k + 1]; kj++) {
+                const int lkj = P.lev[kj];
+                if (LEV_SKIP(lkj, K)) continue;
+                rl_insert(rl, P.cl[kj], (lik > lkj ? lik : lkj) + 1, &cur);
+            }
+        }
+        rl_emit(rl, &P, r);
+    }
+    free(dind);
+    lp_free(T);
+    *T = P;
+}
+/* AddFracPtnLow (matfrac_blkcrs.cpp:364-495): L_ij row r, walked in column
+ * order as it grows, gains L_ij(r, k) U_jj(k, :) fill */
+static void lp_low(orc_lpat *T, unsigned ncol_full, const orc_lpat *D, int K, orc_rowlist *rl)
+{
+    if ((unsigned long)T->rp[T->nrow] == (unsigned long)T->nrow * ncol_full || K == 0) return;
+    unsigned *dind = (unsigned *)malloc(sizeof(unsigned) * (D->nrow + 1));
+    lp_diaind(D, dind);
+    orc_lpat P;
+    lp_begin(&P, T->nrow, T->rp[T->nrow] * 2 + 16);
+    for (unsigned r = 0; r < T->nrow; r++) {
+        rl_init(rl, T, r);
+        for (int t = rl->head; t >= 0; t = rl->v[t].next) {
+            const unsigned k = rl->v[t].colv;
+            const int lik = rl->v[t].lev;
+            if (LEV_SKIP(lik, K)) continue;
+            int cur = t;
+            for (unsigned kj = dind[k]; kj < D->rp[k + 1]; kj++) {
+                const int lkj = D->lev[kj];
+                if (LEV_SKIP(lkj, K)) continue;
+                rl_insert(rl, D->cl[kj], (lik > lkj ? lik : lkj) + 1, &cur);
+            }
+        }
+        rl_emit(rl, &P, r);
+    }
+    free(dind);
+    lp_free(T);
+    *T = P;
+}
+/* AddFracPtn of a diagonal factor (matdiafrac_blkcrs.cpp:1249-1377) starting
+ * from the levels it already has; the same rule as orc_iluk_symbolic */
+static void lp_dia_fill(orc_lpat *D, int K, orc_rowlist *rl)
+{
+    const unsigned nb = D->nrow;
+    if (D->rp[nb] == 0 || (unsigned long)D->rp[nb] == (unsigned long)nb * (nb - 1) || K == 0) return;
+    orc_lpat P;
+    lp_begin(&P, nb, D->rp[nb] * 4 + 16);
+    unsigned *dind = (unsigned *)malloc(sizeof(unsigned) * (nb + 1));
+    for (unsigned r = 0; r < nb; r++) {
+        rl_init(rl, D, r);
+        for (int t = rl->head; t >= 0; t = rl->v[t].next) {
+            const unsigned k = rl->v[t].colv;
+            const int lik = rl->v[t].lev;
+            if (LEV_SKIP(lik, K)) continue;
+            if (k >= r) break;
+            int cur = t;
+            for (unsigned kj = dind[k]; kj < P.rp[k + 1]; kj++) {
+                const int lkj = P.lev[kj];
+                if (LEV_SKIP(lkj, K)) continue;
+                const unsigned j = P.cl[kj];
+                if (j == r) continue;
+                rl_insert(rl, j, (lik > lkj ? lik : lkj) + 1, &cur);
+            }
+        }
+        rl_emit(rl, &P, r);
+        dind[r] = P.rp[r + 1];
+        for (unsigned q = P.rp[r]; q < P.rp[r + 1]; q++)
+            if (P.cl[q] > r) { dind[r] = q; break; }
+    }
+    free(dind);
+    lp_free(D);
+    *D = P;
+}
+/* CPreconditioner_ILU::SetLinearSystem with fill level K for every block
+ * (preconditioner.cpp:57-211): patterns start as the blocks' own with level
+ * 0; for K != 0 they are extended in (i, j) order -- diagonal: LowUp with
+ * every k < i, then its own fill; above: LowUp with k < i, then Up with
+ * factor i; below: LowUp with k < j, then Low with factor j.  Returns -1 when
+ * a block that fill would create is missing. */
+static int ms_symbolic(orc_ms *m, int K)
+{
+    const unsigned n = m->nseg;
+    for (unsigned i = 0; i < n; i++) {
+        lp_free(&m->fp[i]);
+        lp_from(&m->fp[i], m->nb[i], m->rowptr[i], m->col[i]);
+        for (unsigned j = 0; j < n; j++) {
+            orc_msblk *k = m->nd[i][j];
+            if (!k) continue;
+            lp_free(&k->fp);
+            lp_from(&k->fp, k->nbc, k->rowptr, k->col);
+        }
+    }
+    if (K == 0) return 0;
+    orc_rowlist rl = {NULL, 0, 0, -1};
+    int rc = 0;
+    for (unsigned i = 0; i < n && rc == 0; i++)
+        for (unsigned j = 0; j < n && rc == 0; j++) {
+            if (i == j) {
+                for (unsigned kk = 0; kk < i; kk++)
+                    if (m->nd[i][kk] && m->nd[kk][i])
+                        lp_lowup(&m->fp[i], m->nb[i] - 1, &m->nd[i][kk]->fp, &m->nd[kk][i]->fp, K, 1, &rl);
+                lp_dia_fill(&m->fp[i], K, &rl);
+                continue;
+            }
+            const unsigned kmax = i < j ? i : j;
+            for (unsigned kk = 0; kk < kmax + 1; kk++) {
+                orc_msblk *t = m->nd[i][j];
+                if (kk == i && kk < j) { if (t) lp_up(&t->fp, t->nbr, &m->fp[kk], K, &rl); continue; }
+                if (kk == j && kk < i) { if (t) lp_low(&t->fp, t->nbr, &m->fp[kk], K, &rl); continue; }
+                if (!m->nd[i][kk] || !m->nd[kk][j]) continue;
+                if (!t) { rc = -1; break; }
+                lp_lowup(&t->fp, t->nbr, &m->nd[i][kk]->fp, &m->nd[kk][j]->fp, K, 0, &rl);
+            }
+        }
+    free(rl.v);
+    return rc;
+}
 
 void *orc_ms_create(unsigned nseg, const unsigned *nb, const unsigned *b)
 {
@@ -1280,8 +1529,9 @@ void orc_ms_free(void *p)
     if (!m) return;
     for (unsigned i = 0; i < m->nseg; i++) {
         free(m->diaind[i]); free(m->fval[i]); free(m->fdia[i]);
+        lp_free(&m->fp[i]);
         for (unsigned j = 0; j < m->nseg; j++)
-            if (m->nd[i][j]) { free(m->nd[i][j]->F); free(m->nd[i][j]); }
+            if (m->nd[i][j]) { free(m->nd[i][j]->F); lp_free(&m->nd[i][j]->fp); free(m->nd[i][j]); }
     }
     free(m);
 }
@@ -1331,21 +1581,22 @@ static void ms_up(orc_ms *m, unsigned si, orc_msblk *k)
 {
     const unsigned bi = k->bc, bj = k->br;
     const size_t bs = (size_t)bi * bj, bii = (size_t)bi * bi;
+    const unsigned *rp = k->fp.rp, *cl = k->fp.cl, *frow = m->fp[si].rp, *fcol = m->fp[si].cl;
     int *pos = (int *)malloc(sizeof(int) * (k->nbr ? k->nbr : 1));
     for (unsigned j = 0; j < k->nbr; j++) pos[j] = -1;
     for (unsigned i = 0; i < k->nbc; i++) {
-        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++) pos[k->col[q]] = (int)q;
-        for (unsigned ik = m->rowptr[si][i]; ik < m->diaind[si][i]; ik++) {
-            const unsigned kb = m->col[si][ik];
-            for (unsigned kj = k->rowptr[kb]; kj < k->rowptr[kb + 1]; kj++) {
-                const int ij = pos[k->col[kj]];
+        for (unsigned q = rp[i]; q < rp[i + 1]; q++) pos[cl[q]] = (int)q;
+        for (unsigned ik = frow[i]; ik < m->diaind[si][i]; ik++) {
+            const unsigned kb = fcol[ik];
+            for (unsigned kj = rp[kb]; kj < rp[kb + 1]; kj++) {
+                const int ij = pos[cl[kj]];
                 if (ij < 0) continue;
                 rsub_mat_pr(k->F + (size_t)ij * bs, m->fval[si] + (size_t)ik * bii, k->F + (size_t)kj * bs, bi, bi, bj);
             }
         }
-        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++)
+        for (unsigned q = rp[i]; q < rp[i + 1]; q++)
             rmat_pr(k->F + (size_t)q * bs, m->fdia[si] + (size_t)i * bii, bi, bj);
-        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++) pos[k->col[q]] = -1;
+        for (unsigned q = rp[i]; q < rp[i + 1]; q++) pos[cl[q]] = -1;
     }
     free(pos);
 }
@@ -1354,19 +1605,20 @@ static void ms_low(orc_ms *m, unsigned sj, orc_msblk *k)
 {
     const unsigned bi = k->bc, bj = k->br;
     const size_t bs = (size_t)bi * bj, bjj = (size_t)bj * bj;
+    const unsigned *rp = k->fp.rp, *cl = k->fp.cl, *frow = m->fp[sj].rp, *fcol = m->fp[sj].cl;
     int *pos = (int *)malloc(sizeof(int) * (k->nbr ? k->nbr : 1));
     for (unsigned j = 0; j < k->nbr; j++) pos[j] = -1;
     for (unsigned i = 0; i < k->nbc; i++) {
-        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++) pos[k->col[q]] = (int)q;
-        for (unsigned ik = k->rowptr[i]; ik < k->rowptr[i + 1]; ik++) {
-            const unsigned kb = k->col[ik];
-            for (unsigned kj = m->diaind[sj][kb]; kj < m->rowptr[sj][kb + 1]; kj++) {
-                const int ij = pos[m->col[sj][kj]];
+        for (unsigned q = rp[i]; q < rp[i + 1]; q++) pos[cl[q]] = (int)q;
+        for (unsigned ik = rp[i]; ik < rp[i + 1]; ik++) {
+            const unsigned kb = cl[ik];
+            for (unsigned kj = m->diaind[sj][kb]; kj < frow[kb + 1]; kj++) {
+                const int ij = pos[fcol[kj]];
                 if (ij < 0) continue;
                 rsub_mat_pr(k->F + (size_t)ij * bs, k->F + (size_t)ik * bs, m->fval[sj] + (size_t)kj * bjj, bi, bj, bj);
             }
         }
-        for (unsigned q = k->rowptr[i]; q < k->rowptr[i + 1]; q++) pos[k->col[q]] = -1;
+        for (unsigned q = rp[i]; q < rp[i + 1]; q++) pos[cl[q]] = -1;
     }
     free(pos);
 }
@@ -1377,17 +1629,17 @@ static void ms_lowup(orc_msblk *t, const orc_msblk *lo, const orc_msblk *up)
     int *pos = (int *)malloc(sizeof(int) * (t->nbr ? t->nbr : 1));
     for (unsigned j = 0; j < t->nbr; j++) pos[j] = -1;
     for (unsigned i = 0; i < t->nbc; i++) {
-        for (unsigned q = t->rowptr[i]; q < t->rowptr[i + 1]; q++) pos[t->col[q]] = (int)q;
-        for (unsigned ik = lo->rowptr[i]; ik < lo->rowptr[i + 1]; ik++) {
-            const unsigned kb = lo->col[ik];
-            for (unsigned kj = up->rowptr[kb]; kj < up->rowptr[kb + 1]; kj++) {
-                const int ij = pos[up->col[kj]];
+        for (unsigned q = t->fp.rp[i]; q < t->fp.rp[i + 1]; q++) pos[t->fp.cl[q]] = (int)q;
+        for (unsigned ik = lo->fp.rp[i]; ik < lo->fp.rp[i + 1]; ik++) {
+            const unsigned kb = lo->fp.cl[ik];
+            for (unsigned kj = up->fp.rp[kb]; kj < up->fp.rp[kb + 1]; kj++) {
+                const int ij = pos[up->fp.cl[kj]];
                 if (ij < 0) continue;
                 rsub_mat_pr(t->F + (size_t)ij * bi * bj, lo->F + (size_t)ik * bi * bk, up->F + (size_t)kj * bk * bj, bi,
                            bk, bj);
             }
         }
-        for (unsigned q = t->rowptr[i]; q < t->rowptr[i + 1]; q++) pos[t->col[q]] = -1;
+        for (unsigned q = t->fp.rp[i]; q < t->fp.rp[i + 1]; q++) pos[t->fp.cl[q]] = -1;
     }
     free(pos);
 }
@@ -1397,14 +1649,15 @@ static void ms_lowup_dia(orc_ms *m, unsigned si, const orc_msblk *lo, const orc_
 {
     const unsigned bi = m->b[si], bk = lo->br, nb = m->nb[si];
     const size_t bii = (size_t)bi * bi;
+    const unsigned *frow = m->fp[si].rp, *fcol = m->fp[si].cl;
     int *pos = (int *)malloc(sizeof(int) * (nb ? nb : 1));
     for (unsigned j = 0; j < nb; j++) pos[j] = -1;
     for (unsigned i = 0; i < nb; i++) {
-        for (unsigned q = m->rowptr[si][i]; q < m->rowptr[si][i + 1]; q++) pos[m->col[si][q]] = (int)q;
-        for (unsigned ik = lo->rowptr[i]; ik < lo->rowptr[i + 1]; ik++) {
-            const unsigned kb = lo->col[ik];
-            for (unsigned kj = up->rowptr[kb]; kj < up->rowptr[kb + 1]; kj++) {
-                const unsigned j = up->col[kj];
+        for (unsigned q = frow[i]; q < frow[i + 1]; q++) pos[fcol[q]] = (int)q;
+        for (unsigned ik = lo->fp.rp[i]; ik < lo->fp.rp[i + 1]; ik++) {
+            const unsigned kb = lo->fp.cl[ik];
+            for (unsigned kj = up->fp.rp[kb]; kj < up->fp.rp[kb + 1]; kj++) {
+                const unsigned j = up->fp.cl[kj];
                 const double *a = lo->F + (size_t)ik * bi * bk, *c = up->F + (size_t)kj * bk * bi;
                 if (j != i) {
                     const int ij = pos[j];
@@ -1415,38 +1668,47 @@ static void ms_lowup_dia(orc_ms *m, unsigned si, const orc_msblk *lo, const orc_
                 }
             }
         }
-        for (unsigned q = m->rowptr[si][i]; q < m->rowptr[si][i + 1]; q++) pos[m->col[si][q]] = -1;
+        for (unsigned q = frow[i]; q < frow[i + 1]; q++) pos[fcol[q]] = -1;
     }
     free(pos);
 }
-/* CPreconditioner_ILU::SetValue across segments (preconditioner.cpp:217-297):
- * copies first, then blocks in (i, j) order.  Returns 0, or -1 when a
- * diagonal factorisation met more than 10 singular pivots. */
-int orc_ms_ilu(void *p)
+/* CPreconditioner_ILU across segments with fill level lev_fill for every
+ * block: the patterns (ms_symbolic), then SetValue (preconditioner.cpp:
+ * 217-297): copies first (SetValue_Initialize: zero, then the blocks'
+ * values where their columns are in the factor rows), then blocks in (i, j)
+ * order.  Returns 0, -1 when a diagonal factorisation met more than 10
+ * singular pivots, -2 when fill would need a block that is not there. */
+int orc_ms_ilu(void *p, int lev_fill)
 {
     orc_ms *m = (orc_ms *)p;
     const unsigned n = m->nseg;
+    if (ms_symbolic(m, lev_fill) < 0) return -2;
     for (unsigned i = 0; i < n; i++) {
         const unsigned nb = m->nb[i], bb = m->b[i] * m->b[i];
-        const unsigned ncrs = m->rowptr[i][nb];
+        const unsigned ncrs = m->fp[i].rp[nb];
         free(m->diaind[i]); free(m->fval[i]); free(m->fdia[i]);
         m->diaind[i] = (unsigned *)malloc(sizeof(unsigned) * (nb ? nb : 1));
-        for (unsigned r = 0; r < nb; r++) {
-            unsigned q = m->rowptr[i][r];
-            while (q < m->rowptr[i][r + 1] && m->col[i][q] < r) q++;
-            m->diaind[i][r] = q;
-        }
+        lp_diaind(&m->fp[i], m->diaind[i]);
         m->fval[i] = (double *)calloc((size_t)ncrs * bb + 1, sizeof(double));
         m->fdia[i] = (double *)calloc((size_t)nb * bb + 1, sizeof(double));
-        orc_ilu_setvalue(nb, m->b[i], m->rowptr[i], m->col[i], m->val[i], m->dia[i], NULL, m->rowptr[i], m->col[i],
+        orc_ilu_setvalue(nb, m->b[i], m->rowptr[i], m->col[i], m->val[i], m->dia[i], NULL, m->fp[i].rp, m->fp[i].cl,
                          m->fval[i], m->fdia[i]);
         for (unsigned j = 0; j < n; j++) {
             orc_msblk *k = m->nd[i][j];
             if (!k) continue;
-            const size_t nv = (size_t)k->rowptr[k->nbc] * k->bc * k->br;
+            const size_t bs = (size_t)k->bc * k->br;
             free(k->F);
-            k->F = (double *)malloc(sizeof(double) * (nv + 1));
-            memcpy(k->F, k->val, sizeof(double) * nv);
+            k->F = (double *)calloc((size_t)k->fp.rp[k->nbc] * bs + 1, sizeof(double));
+            int *pos = (int *)malloc(sizeof(int) * (k->nbr ? k->nbr : 1));
+            for (unsigned c = 0; c < k->nbr; c++) pos[c] = -1;
+            for (unsigned r = 0; r < k->nbc; r++) {
+                for (unsigned q = k->fp.rp[r]; q < k->fp.rp[r + 1]; q++) pos[k->fp.cl[q]] = (int)q;
+                for (unsigned q = k->rowptr[r]; q < k->rowptr[r + 1]; q++)
+                    if (pos[k->col[q]] >= 0)
+                        memcpy(k->F + (size_t)pos[k->col[q]] * bs, k->val + (size_t)q * bs, sizeof(double) * bs);
+                for (unsigned q = k->fp.rp[r]; q < k->fp.rp[r + 1]; q++) pos[k->fp.cl[q]] = -1;
+            }
+            free(pos);
         }
     }
     for (unsigned i = 0; i < n; i++)
@@ -1454,7 +1716,7 @@ int orc_ms_ilu(void *p)
             if (i == j) {
                 for (unsigned kk = 0; kk < i; kk++)
                     if (m->nd[i][kk] && m->nd[kk][i]) ms_lowup_dia(m, i, m->nd[i][kk], m->nd[kk][i]);
-                if (orc_ilu_decomp(m->nb[i], m->b[i], m->rowptr[i], m->col[i], m->diaind[i], m->fval[i], m->fdia[i]) < 0)
+                if (orc_ilu_decomp(m->nb[i], m->b[i], m->fp[i].rp, m->fp[i].cl, m->diaind[i], m->fval[i], m->fdia[i]) < 0)
                     return -1;
                 continue;
             }
@@ -1470,6 +1732,22 @@ int orc_ms_ilu(void *p)
         }
     return 0;
 }
+/* number of blocks in the factor pattern of block (i, j); 0 when absent */
+unsigned orc_ms_factor_ncrs(void *p, unsigned i, unsigned j)
+{
+    orc_ms *m = (orc_ms *)p;
+    if (i == j) return m->fp[i].rp ? m->fp[i].rp[m->nb[i]] : 0;
+    const orc_msblk *k = m->nd[i][j];
+    return k && k->fp.rp ? k->fp.rp[k->nbc] : 0;
+}
+/* factor pattern of block (i, j): rowptr (nb_i + 1), col (ncrs) */
+void orc_ms_get_pattern(void *p, unsigned i, unsigned j, unsigned *rowptr, unsigned *col)
+{
+    orc_ms *m = (orc_ms *)p;
+    const orc_lpat *P = i == j ? &m->fp[i] : &m->nd[i][j]->fp;
+    memcpy(rowptr, P->rp, sizeof(unsigned) * ((size_t)P->nrow + 1));
+    memcpy(col, P->cl, sizeof(unsigned) * P->rp[P->nrow]);
+}
 /* factor values: segment i's diagonal factor (fval ncrs*b*b, fdia nb*b*b) or
  * the off-diagonal block (i, j) (fval only) */
 void orc_ms_get_factor(void *p, unsigned i, unsigned j, double *fval, double *fdia)
@@ -1477,11 +1755,11 @@ void orc_ms_get_factor(void *p, unsigned i, unsigned j, double *fval, double *fd
     orc_ms *m = (orc_ms *)p;
     if (i == j) {
         const size_t bb = (size_t)m->b[i] * m->b[i];
-        memcpy(fval, m->fval[i], sizeof(double) * m->rowptr[i][m->nb[i]] * bb);
+        memcpy(fval, m->fval[i], sizeof(double) * m->fp[i].rp[m->nb[i]] * bb);
         memcpy(fdia, m->fdia[i], sizeof(double) * m->nb[i] * bb);
     } else if (m->nd[i][j]) {
         const orc_msblk *k = m->nd[i][j];
-        memcpy(fval, k->F, sizeof(double) * k->rowptr[k->nbc] * k->bc * k->br);
+        memcpy(fval, k->F, sizeof(double) * k->fp.rp[k->nbc] * k->bc * k->br);
     }
 }
 /* SolvePrecond across segments (preconditioner.cpp:301-335) */
@@ -1492,19 +1770,19 @@ static void ms_precond(orc_ms *m, double *z)
         for (unsigned j = 0; j < i; j++) {
             const orc_msblk *k = m->nd[i][j];
             if (!k) continue;
-            orc_bmat2_matvec(k->nbc, k->bc, k->nbr, k->br, k->rowptr, k->col, k->F, -1.0, z + m->off[j], 1.0,
+            orc_bmat2_matvec(k->nbc, k->bc, k->nbr, k->br, k->fp.rp, k->fp.cl, k->F, -1.0, z + m->off[j], 1.0,
                              z + m->off[i]);
         }
-        orc_ilu_forward(m->nb[i], m->b[i], m->rowptr[i], m->col[i], m->diaind[i], m->fval[i], m->fdia[i], z + m->off[i]);
+        orc_ilu_forward(m->nb[i], m->b[i], m->fp[i].rp, m->fp[i].cl, m->diaind[i], m->fval[i], m->fdia[i], z + m->off[i]);
     }
     for (int i = (int)n - 1; i >= 0; i--) {
         for (unsigned j = (unsigned)i + 1; j < n; j++) {
             const orc_msblk *k = m->nd[i][j];
             if (!k) continue;
-            orc_bmat2_matvec(k->nbc, k->bc, k->nbr, k->br, k->rowptr, k->col, k->F, -1.0, z + m->off[j], 1.0,
+            orc_bmat2_matvec(k->nbc, k->bc, k->nbr, k->br, k->fp.rp, k->fp.cl, k->F, -1.0, z + m->off[j], 1.0,
                              z + m->off[i]);
         }
-        orc_ilu_backward(m->nb[i], m->b[i], m->rowptr[i], m->col[i], m->diaind[i], m->fval[i], m->fdia[i],
+        orc_ilu_backward(m->nb[i], m->b[i], m->fp[i].rp, m->fp[i].cl, m->diaind[i], m->fval[i], m->fdia[i],
                          z + m->off[i]);
     }
 }

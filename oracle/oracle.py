@@ -106,7 +106,10 @@ class Orc:
         L.orc_ms_set_nondia.argtypes = [vp, C.c_uint, C.c_uint, vp, vp, vp]
         L.orc_ms_matvec.argtypes = [vp, C.c_double, vp, C.c_double, vp]
         L.orc_ms_ilu.restype = C.c_int
-        L.orc_ms_ilu.argtypes = [vp]
+        L.orc_ms_ilu.argtypes = [vp, C.c_int]
+        L.orc_ms_factor_ncrs.restype = C.c_uint
+        L.orc_ms_factor_ncrs.argtypes = [vp, C.c_uint, C.c_uint]
+        L.orc_ms_get_pattern.argtypes = [vp, C.c_uint, C.c_uint, vp, vp]
         L.orc_ms_get_factor.argtypes = [vp, C.c_uint, C.c_uint, vp, vp]
         L.orc_ms_precond.argtypes = [vp, vp]
         L.orc_ms_solve.restype = C.c_int
@@ -502,7 +505,7 @@ class Ref:
             "ref_ms_finalize": (C.c_double, [vp]),
             "ref_ms_get_block": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
             "ref_ms_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
-            "ref_ms_ilu": (C.c_int, [vp]),
+            "ref_ms_ilu": (C.c_int, [vp, C.c_int]),
             "ref_ms_precond": (None, [vp, vp]),
             "ref_ms_solve": (C.c_int, [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_bmat_free": (None, [vp]),
@@ -754,17 +757,27 @@ class OrcMS:
         self.orc.L.orc_ms_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
         return y
 
-    def ilu(self):
-        return self.orc.L.orc_ms_ilu(self.h) == 0
+    def ilu(self, lev=0):
+        """CPreconditioner_ILU(ls, lev) across the segments; True on success."""
+        return self.orc.L.orc_ms_ilu(self.h, int(lev)) == 0
+
+    def pattern(self, i, j):
+        """Factor pattern of block (i, j) (rowptr, col) after ilu()."""
+        n = self.orc.L.orc_ms_factor_ncrs(self.h, i, j)
+        rp = np.zeros(self.nb + 1, dtype=np.uint32)
+        cl = np.zeros(max(n, 1), dtype=np.uint32)
+        self.orc.L.orc_ms_get_pattern(self.h, i, j, _p(rp), _p(cl))
+        return rp, cl[:n]
 
     def factor(self, i, j):
         bb = self.b[i] * self.b[j]
+        n = int(self.orc.L.orc_ms_factor_ncrs(self.h, i, j))
         if i == j:
-            fv = np.zeros(int(self.sp["rowptr"][-1]) * bb)
+            fv = np.zeros(n * bb)
             fd = np.zeros(self.nb * bb)
             self.orc.L.orc_ms_get_factor(self.h, i, j, _p(fv), _p(fd))
             return fv, fd
-        fv = np.zeros(int(self.sp["rowptr_nd"][-1]) * bb)
+        fv = np.zeros(n * bb)
         self.orc.L.orc_ms_get_factor(self.h, i, j, _p(fv), None)
         return fv
 
@@ -832,8 +845,8 @@ class RefMS:
         self.ref.L.ref_ms_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
         return y
 
-    def ilu(self):
-        return bool(self.ref.L.ref_ms_ilu(self.h))
+    def ilu(self, lev=0):
+        return bool(self.ref.L.ref_ms_ilu(self.h, int(lev)))
 
     def precond(self, z):
         z = f64(z).copy()

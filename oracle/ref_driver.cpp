@@ -16,6 +16,17 @@
 //   MatVec::CZMatDia_BlkCrs, CZMatDiaFrac_BlkCrs, CZVector_Blk, Sol::Solve_PCOCG
 //                               include/delfem/matvec/z*.h (complex twin, len 1)
 #include <malloc.h>
+
+// The work vectors of the reference's solvers are CVector_Blk / CZVector_Blk,
+// whose constructors leave the values uninitialised (vector_blk.h:53-56,
+// zvector_blk.h:48-51), and MatVec(1, x, 0, y) multiplies the old y by 0
+// (matdia_blkcrs.cpp:939, zmatdia_blkcrs.cpp:767): heap garbage that is Inf
+// or NaN turns into NaN, depending on what the process freed before.
+// glibc's M_PERTURB = 0xff makes every malloc'd block start as zero bytes (and
+// fills freed blocks with 0xff), so every run sees the zero-initialised
+// vectors the algorithms assume and is reproducible.  Set when this library
+// is loaded.
+__attribute__((constructor)) static void ref_zero_new_blocks() { mallopt(M_PERTURB, 0xff); }
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -548,12 +559,12 @@ void ref_ms_matvec(void* p, double alpha, const double* x, double beta, double* 
   m->ls.MATVEC(alpha, 0, beta, 1);
   ref_ms_get_vector(p, 1, y);
 }
-// CPreconditioner_ILU(ls, 0) across the segments, SetValue; returns its bool
-int ref_ms_ilu(void* p) {
+// CPreconditioner_ILU(ls, lev) across the segments, SetValue; returns its bool
+int ref_ms_ilu(void* p, int lev) {
   RefMS* m = static_cast<RefMS*>(p);
   delete m->prec;
   m->prec = new LsSol::CPreconditioner_ILU;
-  m->prec->SetFillInLevel(0);
+  m->prec->SetFillInLevel(lev);
   m->prec->SetLinearSystem(m->ls);
   return m->prec->SetValue(m->ls) ? 1 : 0;
 }
@@ -651,12 +662,6 @@ void ref_bmat_matvec(void* p, double alpha, const double* x, double beta, double
 }
 
 // MatVec::Sol::Solve_COCG / Solve_BiCGSTAB / Solve_BiCGStabP (the last with the factor).
-// Their work vectors are CZVector_Blk, whose constructor leaves the values
-// uninitialised (zvector_blk.h:48-51), and MatVec(1, x, 0, y) multiplies the
-// old y by 0 (zmatdia_blkcrs.cpp:767): heap garbage that is Inf or NaN turns
-// into NaN.  glibc's M_PERTURB = 0xff makes every malloc'd block start as
-// zero bytes (and fills freed blocks with 0xff), so these runs see the
-// zero-initialised vectors the algorithm assumes and are reproducible.
 static void zero_new_blocks() { mallopt(M_PERTURB, 0xff); }
 int ref_zsolve_cocg(void* p, double* r, double* u, double* conv, unsigned* iter) {
   zero_new_blocks();

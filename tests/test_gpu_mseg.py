@@ -1,9 +1,10 @@
 """Multi-segment systems on the device (through the C-ABI) against the
 oracle's restatement, which the compiled reference pins bit for bit
 (tests/test_oracle_ref.py): the split blocks, residual and MATVEC, the
-cross-segment ILU(0) factor values and SolvePrecond bit for bit; Solve_PCG
-within one iteration and 1e-8, Solve_PBiCGSTAB within 25 % and 1e-6 (the
-dot products are device reductions)."""
+cross-segment ILU(k) factor patterns (k = 0, 1, 2, unlimited), factor
+values and SolvePrecond bit for bit; Solve_PCG within one iteration and
+1e-8, Solve_PBiCGSTAB within 25 % and 1e-6 (the dot products are device
+reductions)."""
 import numpy as np
 import pytest
 
@@ -95,8 +96,8 @@ def test_segment_solvers(orc, gpu, name):
 def test_multicolour_numbering(orc, gpu):
     """Nodes renumbered by multicolouring: the relabelled problem's blocks and
     cross-segment factor equal the oracle's on the same relabelled problem,
-    the factor sweeps have one level per colour, and the PCG solution mapped
-    back equals the natural-numbering one within 1e-8."""
+    and the PCG solution mapped back equals the natural-numbering one within
+    1e-8."""
     import torch
     from delfem_amd import meshgen
     prob = meshgen.solid3d(4, seed=7, sigma=0.5)
@@ -142,3 +143,57 @@ def test_multicolour_numbering(orc, gpu):
         off += nb * len(g)
     print(f"  multicolour {it} iterations, natural {itn}", flush=True)
     assert np.linalg.norm(back - xn) <= 1e-8 * np.linalg.norm(xn)
+
+
+FILL_CASES = {
+    "solid3d_uv_w": (lambda: meshgen.solid3d(3, seed=7, sigma=0.5), [[0, 1], [2]], "pcg"),
+    "solid3d_u_v_w": (lambda: meshgen.solid3d(3, seed=7, sigma=0.5), [[0], [1], [2]], "pcg"),
+    "oseen2d_uv_p": (lambda: meshgen.oseen2d(8, seed=7), [[0, 1], [2]], "bicg"),
+}
+
+
+@pytest.mark.parametrize("lev", [1, 2, -1])
+@pytest.mark.parametrize("name", sorted(FILL_CASES))
+def test_cross_segment_fill(orc, gpu, name, lev):
+    """CPreconditioner_ILU(ls, lev) across segments: every block's fill
+    pattern, factor values and SolvePrecond equal the oracle's bit for bit;
+    the solvers as for ILU(0)."""
+    import torch
+    make, groups, kind = FILL_CASES[name]
+    prob = make()
+    print(f"[case] {name} lev {lev}", flush=True)
+    S = gpu.MultiSegmentSystem(prob, groups)
+    S.assemble()
+    O = OrcMS(orc, split_problem(orc, prob, groups), prob.conn)
+    M = S.preconditioner(lev)
+    assert M.SetValue() and O.ilu(lev)
+    n = len(groups)
+    for i in range(n):
+        for j in range(n):
+            rp, cl = M.pattern(i, j)
+            orp, ocl = O.pattern(i, j)
+            assert np.array_equal(rp, orp) and np.array_equal(cl, ocl), (i, j)
+            if i == j:
+                fv, fd = M.values(i, i)
+                ofv, ofd = O.factor(i, i)
+                assert np.array_equal(fv, ofv) and np.array_equal(fd, ofd), (i, j)
+            else:
+                assert np.array_equal(M.values(i, j), O.factor(i, j)), (i, j)
+    x = np.random.default_rng(2).standard_normal(O.res.size)
+    xd = torch.from_numpy(x.copy()).cuda()
+    M.SolvePrecond(xd)
+    assert np.array_equal(xd.cpu().numpy(), O.precond(x))
+    r = S.res.clone()
+    xs = torch.zeros_like(r)
+    ok, conv, it = M.solve(kind, 1e-10, 3000, r, xs)
+    ok_o, conv_o, it_o, x_o, _ = O.solve(kind == "bicg", O.res, 1e-10, 3000)
+    xh = xs.cpu().numpy()
+    print(f"  device it={it} oracle it={it_o} |x-x_o|/|x_o|={np.linalg.norm(xh - x_o) / np.linalg.norm(x_o):.2e}",
+          flush=True)
+    assert ok and ok_o
+    if kind == "pcg":
+        assert abs(it - it_o) <= 1
+        assert np.linalg.norm(xh - x_o) <= 1e-8 * np.linalg.norm(x_o)
+    else:
+        assert abs(it - it_o) <= max(2, int(np.ceil(0.25 * it_o)))
+        assert np.linalg.norm(xh - x_o) <= 1e-6 * np.linalg.norm(x_o)

@@ -557,3 +557,50 @@ def test_multisegment_is_permuted_combined_system(orc):
     ok, conv, it, u, r = O.solve(False, O.res, 1e-12, 2000)
     xd = np.linalg.solve(Ap, O.res)
     assert ok and np.linalg.norm(u - xd) <= 1e-9 * np.linalg.norm(xd)
+
+
+def _global_pattern(O, groups, lev=None):
+    """Boolean node-block pattern of the whole segment-ordered system: the
+    blocks of the system (lev None) or of the cross-segment factor."""
+    n, nb = len(groups), O.nb
+    G = np.zeros((n * nb, n * nb), dtype=bool)
+    for i in range(n):
+        G[np.arange(i * nb, (i + 1) * nb), np.arange(i * nb, (i + 1) * nb)] = True
+        for j in range(n):
+            if lev is None:
+                rp, cl = (O.sp["rowptr"], O.sp["col"]) if i == j else (O.sp["rowptr_nd"], O.sp["col_nd"])
+            else:
+                rp, cl = O.pattern(i, j)
+            rows = np.repeat(np.arange(nb), np.diff(np.asarray(rp, dtype=np.int64)))
+            G[i * nb + rows, j * nb + np.asarray(cl, dtype=np.int64)] = True
+    return G
+
+
+@pytest.mark.parametrize("case", ["solid3d_u_v_w", "oseen2d_uv_p"])
+def test_multisegment_fill_is_global_ilu(orc, case):
+    """Fill between segments is the ILU of the whole system in segment order:
+    at level 1 the factor pattern of every block is the system's pattern plus
+    {(a, b): a_ak, a_kb != 0 for some k < min(a, b)} (first-level fill, where
+    the reference's keep-first-level rule cannot matter); with unlimited fill
+    the factorisation is an exact LU, so SolvePrecond(A x) = x."""
+    from oracle.oracle import OrcMS, split_problem
+    if case == "solid3d_u_v_w":
+        prob, groups = meshgen.solid3d(2, seed=3, sigma=0.5), [[0], [1], [2]]
+    else:
+        prob, groups = meshgen.oseen2d(5, seed=3), [[0, 1], [2]]
+    sp = split_problem(orc, prob, groups)
+    O = OrcMS(orc, sp, prob.conn)
+    G = _global_pattern(O, groups)
+    want = G.copy()
+    for k in range(G.shape[0]):
+        a = np.nonzero(G[k + 1:, k])[0] + k + 1
+        b = np.nonzero(G[k, k + 1:])[0] + k + 1
+        want[np.ix_(a, b)] = True
+    assert O.ilu(1)
+    got = _global_pattern(O, groups, lev=1)
+    assert np.array_equal(got, want)
+    assert (want & ~G).any()           # the case has fill
+    assert O.ilu(-1)
+    x = np.random.default_rng(5).standard_normal(O.res.size)
+    z = O.precond(O.matvec(1.0, x, 0.0, np.zeros_like(x)))
+    assert np.linalg.norm(z - x) <= 1e-11 * np.linalg.norm(x)

@@ -208,3 +208,25 @@ def test_multisegment_vs_reference(orc, ref, name):
     a, b = R.solve(bicg, O.res, 1e-10, 3000), O.solve(bicg, O.res, 1e-10, 3000)
     assert a[:3] == b[:3] and a[0]
     assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+
+
+@pytest.mark.parametrize("lev", [1, 2, 3])
+@pytest.mark.parametrize("name", sorted(MS_CASES))
+def test_multisegment_fill_vs_reference(orc, ref, name, lev):
+    """CPreconditioner_ILU(ls, lev) across the segments (fill between
+    segments: AddFracPtnLowUp / Up / Low and the diagonal blocks' AddFracPtn
+    in the reference's block order): SolvePrecond on two vectors and the
+    whole PCG / BiCGSTAB run, bit for bit."""
+    from oracle.oracle import OrcMS, RefMS, split_problem
+    make, groups, bicg = MS_CASES[name]
+    prob = make()
+    sp = split_problem(orc, prob, groups)
+    O, R = OrcMS(orc, sp, prob.conn), RefMS(ref, sp, prob.conn)
+    assert R.ilu(lev) and O.ilu(lev)
+    rng = np.random.default_rng(lev)
+    for _ in range(2):
+        x = rng.standard_normal(O.res.size)
+        assert np.array_equal(R.precond(x), O.precond(x))
+    a, b = R.solve(bicg, O.res, 1e-10, 3000), O.solve(bicg, O.res, 1e-10, 3000)
+    assert a[:3] == b[:3] and a[0]
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
