@@ -16,7 +16,9 @@ lid-driven cavity 256^2, ILU(1)-BiCGSTAB to 1e-9), cfg5 (512 independent
 Helmholtz systems, batched ILU(0)-PCG to 1e-8 in one persistent launch),
 zhelm (complex twin: complex Helmholtz with Sommerfeld lines 1024^2,
 ILU(0)-PCOCG to 1e-8), mseg (elasticity 48^3 split into two segments,
-ILU(0) across segments, PCG to 1e-10).
+ILU(0) across segments, PCG to 1e-10), mseg_fluid (Oseen 256^2 split into
+velocity and pressure segments, ILU(1) with fill between segments,
+BiCGSTAB to 1e-9).
 
 Prints one JSON line:  python bench.py [--steps K] [--warmup W] [--config cfg3]
 """
@@ -55,6 +57,12 @@ CONFIGS = {
                   workload="complex twin: Helmholtz P1 triangles + Sommerfeld radiation lines on the unit square, "
                            "1024x1024 cells (1,050,625 nodes), wave_length 1.0, incoming wave u = 1 on x = 0, "
                            "ILU(0)-PCOCG to 1e-8"),
+    "mseg_fluid": dict(kind="mseg", problem="oseen2d", n=256, groups=[[0, 1], [2]], lev=1, solver="bicg", tol=1e-5,
+                       maxit=100, ordering="tile_multicolour",
+                       workload="multi-segment system: Oseen P1-P1 PSPG 256x256 cells (66,049 nodes) split into a "
+                                "velocity segment and a pressure segment, ILU(1) across segments (fill between "
+                                "segments; nodes numbered by coloured 4x4 tiles), BiCGSTAB to 1e-5 or 100 "
+                                "iterations (the fluid solver's settings)"),
     "mseg": dict(kind="mseg", n=48, sigma=0.0, groups=[[0, 1], [2]], tol=1e-10, maxit=5000,
                  workload="multi-segment system: 3-D linear elasticity 48^3 (117,649 nodes) split into an (ux, uy) "
                           "segment and a uz segment, ILU(0) across segments (nodes multicolour-numbered), "
@@ -105,6 +113,8 @@ def main():
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=40, help="Krylov iterations in the CPU sample")
+    ap.add_argument("--skip-natural", action="store_true",
+                    help="multi-segment configs: leave out the natural-numbering comparison solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1", "n_gpus": args.gpus}),
@@ -453,28 +463,34 @@ def bench_complex(cfg, args, torch, linsys, _lib):
 
 
 def bench_mseg(cfg, args, torch, linsys, _lib):
-    """Multi-segment workload: every step = cross-segment ILU(0) SetValue +
-    Solve_PCG on the segments (host-driven loop over device kernels)."""
-    import numpy as np
+    """Multi-segment workload: every step = cross-segment ILU(k) SetValue +
+    Solve_PCG or Solve_PBiCGSTAB on the segments (host-driven loop over
+    device kernels)."""
     from delfem_amd import meshgen
     log(f"[bench] {args.config}: {cfg['workload']}")
     t0 = time.time()
-    prob = meshgen.solid3d(cfg["n"], seed=args.seed, sigma=cfg["sigma"])
+    if cfg.get("problem") == "oseen2d":
+        prob = meshgen.oseen2d(cfg["n"], seed=args.seed)
+    else:
+        prob = meshgen.solid3d(cfg["n"], seed=args.seed, sigma=cfg["sigma"])
+    lev, solver = cfg.get("lev", 0), cfg.get("solver", "pcg")
+    names = f"ILU({lev}) across segments + {'BiCGSTAB' if solver == 'bicg' else 'PCG'}"
     runs = {}
-    for ordering in ("multicolour", "natural"):
+    dev_ordering = cfg.get("ordering", "multicolour")
+    for ordering in (dev_ordering,) if args.skip_natural else (dev_ordering, "natural"):
         S = linsys.MultiSegmentSystem(prob, cfg["groups"], ordering=ordering)
         S.assemble()
         rhs = S.res.clone()
-        M = S.preconditioner()
+        M = S.preconditioner(lev)
         log(f"[bench] {ordering}: mesh + split assembly + symbolic {time.time() - t0:.1f}s")
 
         def step():
             M.SetValue()
             r = rhs.clone()
             x = torch.zeros_like(r)
-            return M.solve("pcg", cfg["tol"], cfg["maxit"], r, x)
+            return M.solve(solver, cfg["tol"], cfg["maxit"], r, x)
 
-        steps = args.steps if ordering == "multicolour" else 1
+        steps = args.steps if ordering == dev_ordering else 1
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
@@ -483,18 +499,19 @@ def bench_mseg(cfg, args, torch, linsys, _lib):
             ok, conv, it = step()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t1) * 1e3 / steps
-        runs[ordering] = (ms, ok, conv, it + 1)
-        log(f"[bench] {ordering} ILU(0) across segments + PCG: {ms:.2f} ms/solve, {it + 1} iterations, "
-            f"conv {conv:.3e}, ok {ok}")
+        # Solve_PCG reports the 0-based index of its last iteration, Solve_PBiCGSTAB a count from 1
+        runs[ordering] = (ms, ok, conv, it + 1 if solver == "pcg" else it)
+        log(f"[bench] {ordering} {names}: {ms:.2f} ms/solve, {runs[ordering][3]} iterations, conv {conv:.3e}, "
+            f"ok {ok}")
         t0 = time.time()
-    ms, ok, conv, iters = runs["multicolour"]
+    ms, ok, conv, iters = runs[dev_ordering]
     it = iters - 1
     cpu = None
     if not args.no_cpu:
         # the reference's natural numbering, scaled to its own iteration count
-        cpu = cpu_baseline_mseg(prob, cfg, args, runs["natural"][3])
+        cpu = cpu_baseline_mseg(prob, cfg, args, runs.get("natural", runs[dev_ordering])[3])
     out = {
-        "metric": "ms per multi-segment ILU(0)-PCG solve to 1e-10 (host-driven loop)",
+        "metric": f"ms per multi-segment {names} solve to {cfg['tol']:g} (host-driven loop)",
         "value": round(ms, 3),
         "unit": "ms",
         "n_gpus": 1,
@@ -508,9 +525,9 @@ def bench_mseg(cfg, args, torch, linsys, _lib):
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "config": args.config, "tol": cfg["tol"],
                    "segments": [len(g) for g in cfg["groups"]],
-                   "ordering": "nodes renumbered by multicolouring (value); natural beside it"},
-        "natural_ms": round(runs["natural"][0], 3),
-        "natural_iterations": runs["natural"][3],
+                   "ordering": f"nodes renumbered ({dev_ordering}) for the value; natural beside it"},
+        "natural_ms": round(runs["natural"][0], 3) if "natural" in runs else None,
+        "natural_iterations": runs["natural"][3] if "natural" in runs else None,
         "iterations": it + 1,
         "converged": bool(ok),
         "conv_ratio": conv,
@@ -522,8 +539,9 @@ def bench_mseg(cfg, args, torch, linsys, _lib):
 
 def cpu_baseline_mseg(prob, cfg, args, iters):
     """The reference's multi-segment CLinearSystem (compiled from its sources)
-    on one core: CPreconditioner_ILU(0) across segments (symbolic + numeric)
-    and a sample of Solve_PCG iterations, scaled to the device's count."""
+    on one core: CPreconditioner_ILU(k) across segments (symbolic + numeric)
+    and a sample of Solve_PCG / Solve_PBiCGSTAB iterations, scaled to the
+    device's count."""
     from oracle import oracle
     try:
         ref = oracle.Ref()
@@ -533,11 +551,12 @@ def cpu_baseline_mseg(prob, cfg, args, iters):
     sp = oracle.split_problem(orc, prob, cfg["groups"])
     R = oracle.RefMS(ref, sp, prob.conn)
     rhs = R.vector(-1)
+    lev, bicg = cfg.get("lev", 0), cfg.get("solver", "pcg") == "bicg"
     t1 = time.time()
-    R.ilu()
+    R.ilu(lev)
     t_ilu = time.time() - t1
     t1 = time.time()
-    R.solve(False, rhs, cfg["tol"], args.cpu_iters)
+    R.solve(bicg, rhs, cfg["tol"], args.cpu_iters)
     t_it = (time.time() - t1) / args.cpu_iters
     log(f"[bench] cpu reference: ILU {t_ilu:.2f}s, {1e3 * t_it:.1f} ms/iteration, scaled to {iters} iterations")
     return {
@@ -545,9 +564,9 @@ def cpu_baseline_mseg(prob, cfg, args, iters):
         "unit": "ms per solve (ILU + iterations x sampled ms/iteration)",
         "cores": 1,
         "kind": "reference",
-        "sample": (f"the same split system: CPreconditioner_ILU(0) across the segments (symbolic + numeric), then "
-                   f"{args.cpu_iters} Solve_PCG iterations on one core, scaled to {iters} iterations (the device's "
-                   f"count with the reference's natural numbering)"),
+        "sample": (f"the same split system: CPreconditioner_ILU({lev}) across the segments (symbolic + numeric), "
+                   f"then {args.cpu_iters} {'Solve_PBiCGSTAB' if bicg else 'Solve_PCG'} iterations on one core, "
+                   f"scaled to {iters} iterations (the device's count with the reference's natural numbering)"),
         "ilu_s": round(t_ilu, 3),
         "ms_per_iteration": round(1e3 * t_it, 2),
         "iterations": iters,

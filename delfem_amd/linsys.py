@@ -504,19 +504,27 @@ class MultiSegmentSystem:
     conditions per block.  Vectors are the segments' vectors one after the
     other (torch tensors on the device)."""
 
-    def __init__(self, prob, groups, ordering="natural"):
+    def __init__(self, prob, groups, ordering="natural", block_nodes=16):
         """ordering "multicolour": the mesh nodes are first renumbered by the
         greedy multicolouring of the node graph (the same numbering for every
         segment), so that each segment's natural-order ILU(0) is a colour-
-        ordered one with one level set per colour; vectors are then in that
+        ordered one with one level set per colour; "block_multicolour": by
+        colours of node blocks (about block_nodes each); "tile_multicolour":
+        by colours of geometric tiles (meshgen.tile_colour_order), which keeps
+        the level count low also with fill (ILU(1)).  Vectors are then in that
         numbering (self.node_order[new] = old node)."""
         torch = _torch()
         self.node_order = None
-        if ordering == "multicolour":
+        if ordering == "tile_multicolour":
+            from . import meshgen
+            self.node_order = meshgen.tile_colour_order(prob, block_nodes)
+            prob = meshgen.relabel(prob, self.node_order)
+        elif ordering in ("multicolour", "block_multicolour"):
             rp0, cl0 = fem_pattern(prob.nnode, prob.conn)
             A0 = CMatDia_BlkCrs(prob.nnode, 1)
             A0.AddPattern(rp0, cl0)
-            self.node_order = CMatDiaFrac_BlkCrs(0, A0, ordering="multicolour").pattern()["order"].copy()
+            F0 = CMatDiaFrac_BlkCrs(0, A0, ordering=ordering, block_nodes=block_nodes)
+            self.node_order = F0.pattern()["order"].copy()
             from . import meshgen
             prob = meshgen.relabel(prob, self.node_order)
         elif ordering != "natural":

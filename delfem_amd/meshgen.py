@@ -125,6 +125,41 @@ def relabel(prob, order):
     return Problem(**kw)
 
 
+def tile_colour_order(prob, tile_nodes=16):
+    """A node numbering for level-scheduled ILU sweeps: the nodes are cut
+    into square (cubic) tiles of about tile_nodes nodes by their
+    coordinates, the tiles coloured greedily on the tile adjacency of the
+    mesh, and numbered by (colour, tile, node).  Tiles of one colour are
+    not adjacent, and for tiles of a few nodes a side not at distance two
+    either, so ILU(0) and ILU(1) sweeps need about (colours x levels within
+    a tile) level sets.  Returns order (order[new] = old node)."""
+    coord = np.asarray(prob.coord, dtype=np.float64)
+    nn, dim = coord.shape
+    conn = np.asarray(prob.conn, dtype=np.int64)
+    # mesh spacing: median edge length over the elements' first edges
+    h = float(np.median(np.linalg.norm(coord[conn[:, 1]] - coord[conn[:, 0]], axis=1)))
+    side = h * max(1.0, round(tile_nodes ** (1.0 / dim)))
+    cell = np.floor((coord - coord.min(axis=0)) / side + 1e-9).astype(np.int64)
+    dims = cell.max(axis=0) + 1
+    tile = np.ravel_multi_index(tuple(cell.T), tuple(dims))
+    tiles, tid = np.unique(tile, return_inverse=True)
+    nt = tiles.size
+    # tile adjacency from the element connectivity
+    a = tid[conn]
+    pairs = np.concatenate([np.stack([a[:, i], a[:, j]], axis=1)
+                            for i in range(a.shape[1]) for j in range(a.shape[1]) if i != j])
+    pairs = np.unique(pairs[pairs[:, 0] != pairs[:, 1]], axis=0)
+    ptr = np.searchsorted(pairs[:, 0], np.arange(nt + 1))
+    colour = -np.ones(nt, dtype=np.int64)
+    for t in range(nt):
+        used = set(colour[pairs[ptr[t]:ptr[t + 1], 1]].tolist())
+        c = 0
+        while c in used:
+            c += 1
+        colour[t] = c
+    return np.lexsort((np.arange(nn), tid, colour[tid]))
+
+
 def poisson2d(n=128, seed=12345, sigma=0.5, source=1.0):
     """2-D Poisson (scalar2d path): unit square, n x n cells, lognormal diffusivity
     per element, f = source, u = 0 on x = 0."""
