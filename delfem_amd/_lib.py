@@ -154,6 +154,7 @@ _SIGS = {
     "dfm_zbcrs_mearge": (C.c_int, [H, C.c_uint, vp, C.c_uint, vp]),
     "dfm_zbcrs_set_bc": (C.c_int, [H, vp]),
     "dfm_zbcrs_matvec": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
+    "dfm_zbcrs_matvec_hermitian": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
     "dfm_zvec_dot": (C.c_int, [C.c_size_t, vp, vp, f64p]),
     "dfm_zvec_norm2": (C.c_int, [C.c_size_t, vp, f64p]),
     "dfm_zvec_zero_bc": (C.c_int, [C.c_size_t, vp, vp]),
@@ -171,6 +172,7 @@ _SIGS = {
     "dfm_zilu_solve_precond": (C.c_int, [H, vp]),
     "dfm_zsolve_pcocg": (C.c_int, [H, H, vp, vp, f64p, u32p]),
     "dfm_zsolve_cocg": (C.c_int, [H, vp, vp, f64p, u32p]),
+    "dfm_zsolve_cgnr": (C.c_int, [H, vp, vp, f64p, u32p]),
     "dfm_zsolve_bicgstab": (C.c_int, [H, vp, vp, f64p, u32p]),
     "dfm_zsolve_bicgstabp": (C.c_int, [H, H, vp, vp, f64p, u32p]),
     "dfm_bench_kernel": (C.c_int, [H, H, C.c_int, C.c_int, C.c_int, f64p, f64p]),

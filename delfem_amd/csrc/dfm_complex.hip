@@ -76,6 +76,10 @@ struct dfm_zbcrs_s {
   DevBuf<int> s_sptr, s_rows, s_col;
   DevBuf<long long> s_src;
   DevBuf<double2> s_val, s_dval;
+  // transposed pattern for MatVec_Hermitian (built on first use): for column
+  // i the rows j holding it, ascending, and the entry positions
+  unsigned long long t_hash = 0;
+  DevBuf<int> t_ptr, t_row, t_src;
   ZCtx ctx;
   ~dfm_zbcrs_s() {
     if (pat) dfm_bcrs_destroy(pat);
@@ -306,6 +310,38 @@ __global__ void __launch_bounds__(kNT) k_zcrs_matvec(int nb, const int* __restri
   t.x += re;
   t.y += im;
   y[i] = zadd(t, zmul(zdscale(alpha, dia[i]), x[i]));
+}
+
+// MatVec_Hermitian (zmatdia_blkcrs.cpp:803-847): y = alpha A^H x + beta y.
+// The reference scales y by beta, then walks the rows j of A adding
+// alpha*InnerProduct(x_j, a_ji) = alpha*(x_j conj(a_ji)) to y_i, and the
+// diagonal term to y_j after row j's entries.  Per y_i that order is: rows
+// j < i holding column i (ascending), the diagonal, rows j > i; one thread
+// per y_i gathers them so through the transposed pattern.
+__device__ inline double2 zinner(double2 a, double2 b) {  // a conj(b), complex.h:153-156
+  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__global__ void __launch_bounds__(kNT) k_zcrs_matvec_h(int nb, const int* __restrict__ tptr,
+                                                      const int* __restrict__ trow, const int* __restrict__ tsrc,
+                                                      const double2* __restrict__ val,
+                                                      const double2* __restrict__ dia, double alpha,
+                                                      const double2* __restrict__ x, double beta,
+                                                      double2* __restrict__ y) {
+  const int i = blockIdx.x * kNT + threadIdx.x;
+  if (i >= nb) return;
+  const double2 yi = y[i];
+  double2 t = make_double2(yi.x * beta, yi.y * beta);
+  bool diag = false;
+  for (int q = tptr[i]; q < tptr[i + 1]; q++) {
+    const int j = trow[q];
+    if (!diag && j > i) {
+      t = zadd(t, zdscale(alpha, zinner(x[i], dia[i])));
+      diag = true;
+    }
+    t = zadd(t, zdscale(alpha, zinner(x[j], val[tsrc[q]])));
+  }
+  if (!diag) t = zadd(t, zdscale(alpha, zinner(x[i], dia[i])));
+  y[i] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -1122,6 +1158,37 @@ int dfm_zbcrs_matvec(dfm_zbcrs* A, double alpha, const double* x, double beta, d
   return DFM_OK;
 }
 
+static int ztranspose(dfm_zbcrs* A) {
+  const dfm_bcrs_s* P = A->pat;
+  if (A->t_hash == P->pattern_hash && A->t_ptr.p) return DFM_OK;
+  const unsigned nb = P->nb;
+  std::vector<int> ptr(nb + 1, 0), row(std::max<size_t>(1, P->col.size())), src(row.size());
+  for (unsigned q = 0; q < P->col.size(); q++) ptr[P->col[q] + 1]++;
+  for (unsigned i = 0; i < nb; i++) ptr[i + 1] += ptr[i];
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (unsigned j = 0; j < nb; j++)  // rows in ascending order: each column's list ascends
+    for (unsigned q = P->rowptr[j]; q < P->rowptr[j + 1]; q++) {
+      const int k = fill[P->col[q]]++;
+      row[k] = (int)j;
+      src[k] = (int)q;
+    }
+  DFM_TRY(A->t_ptr.upload(ptr));
+  DFM_TRY(A->t_row.upload(row));
+  DFM_TRY(A->t_src.upload(src));
+  A->t_hash = P->pattern_hash;
+  return DFM_OK;
+}
+int dfm_zbcrs_matvec_hermitian(dfm_zbcrs* A, double alpha, const double* x, double beta, double* y) {
+  DFM_CHECK(A && x && y && x != y, DFM_BAD_ARG, "bad arguments to dfm_zbcrs_matvec_hermitian");
+  DFM_TRY(ensure_init());
+  DFM_TRY(ztranspose(A));
+  const dfm_bcrs_s* P = A->pat;
+  DFM_ZLAUNCH(k_zcrs_matvec_h, zgrid_for(P->nb), kNT, (int)P->nb, A->t_ptr.p, A->t_row.p, A->t_src.p, A->d_val.p,
+              A->d_dia.p, alpha, (const double2*)x, beta, (double2*)y);
+  DFM_HIP(hipStreamSynchronize(stream()));
+  return DFM_OK;
+}
+
 // CZVector_Blk
 static int zreduce(int nd, size_t n, const double2* x, const double2* y, double* out) {
   static DevBuf<double> partials;
@@ -1320,6 +1387,60 @@ int dfm_zsolve_cocg(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsi
   DFM_TRY(znatural_op(A, op));
   return zcocg_core(A->ctx, op, nullptr, A->pat->nb, nullptr, nullptr, (double2*)r, (double2*)u, conv_ratio,
                     iteration);
+}
+// MatVec::Sol::Solve_CGNR (zsolver_mat_iter.cpp:394-483): CG on the normal
+// equations with z = A^H r; alpha and beta are ratios of squared norms.  A
+// host-driven loop over the matrix-vector kernels (squared norms are device
+// reductions); the vector updates are AXPY on the interleaved doubles, which
+// is the reference's complex AXPY with a real factor, component by component.
+int dfm_zsolve_cgnr(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  DFM_TRY(zsolve_checks(A, nullptr, r, u, conv_ratio, iteration));
+  DFM_TRY(ensure_init());
+  const size_t nb = A->pat->nb, n2 = 2 * nb;
+  const double tol = *conv_ratio;
+  const unsigned mx_iter = *iteration;
+  DFM_TRY(dfm_vec_fill(n2, 0.0, u));
+  double sq0 = 0.0;
+  DFM_TRY(dfm_zvec_norm2(nb, r, &sq0));
+  if (sq0 < 1.0e-30) {
+    *conv_ratio = 0.0;
+    *iteration = 0;
+    return DFM_OK;
+  }
+  const double inv = 1.0 / sq0;
+  DevBuf<double> w;
+  DFM_TRY(w.alloc(3 * std::max<size_t>(1, n2)));
+  double *p = w.p, *z = w.p + n2, *Ap = w.p + 2 * n2;
+  DFM_TRY(dfm_vec_fill(3 * n2, 0.0, w.p));
+  DFM_TRY(dfm_zbcrs_matvec_hermitian(A, 1.0, r, 0.0, z));
+  double sqz0 = 0.0;
+  DFM_TRY(dfm_zvec_norm2(nb, z, &sqz0));
+  DFM_TRY(dfm_vec_copy(n2, z, p));
+  *iteration = mx_iter;
+  for (unsigned it = 1; it < mx_iter; it++) {
+    DFM_TRY(dfm_zbcrs_matvec(A, 1.0, p, 0.0, Ap));
+    double sqAp = 0.0;
+    DFM_TRY(dfm_zvec_norm2(nb, Ap, &sqAp));
+    const double alpha = sqz0 / sqAp;
+    DFM_TRY(dfm_vec_axpy(n2, alpha, p, u));
+    DFM_TRY(dfm_vec_axpy(n2, -alpha, Ap, r));
+    double sqr = 0.0;
+    DFM_TRY(dfm_zvec_norm2(nb, r, &sqr));
+    const double ratio = std::sqrt(sqr * inv);
+    if (ratio < tol) {
+      *conv_ratio = ratio;
+      *iteration = it;
+      return DFM_OK;
+    }
+    DFM_TRY(dfm_zbcrs_matvec_hermitian(A, 1.0, r, 0.0, z));
+    double sqz1 = 0.0;
+    DFM_TRY(dfm_zvec_norm2(nb, z, &sqz1));
+    const double beta = sqz1 / sqz0;
+    sqz0 = sqz1;
+    DFM_TRY(dfm_vec_scal(n2, beta, p));
+    DFM_TRY(dfm_vec_axpy(n2, 1.0, z, p));
+  }
+  return DFM_OK;
 }
 int dfm_zsolve_bicgstab(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration) {
   DFM_TRY(zsolve_checks(A, nullptr, r, u, conv_ratio, iteration));

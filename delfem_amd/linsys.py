@@ -852,6 +852,12 @@ class CZMatDia_BlkCrs:
         call("dfm_zbcrs_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
         return True
 
+    def MatVec_Hermitian(self, alpha, x, beta, y):
+        """y = alpha A^H x + beta y (src/matvec/zmatdia_blkcrs.cpp:803-847)."""
+        _sync_torch()
+        call("dfm_zbcrs_matvec_hermitian", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
+        return True
+
 
 class ZAssembler:
     """Element loops of src/femeqn/eqn_helmholtz.cpp gathered by row on the device."""
@@ -972,6 +978,12 @@ def _zsolve(fn, conv_ratio, iteration, A, prec, r, u):
     args = (A.h,) + ((prec.h,) if prec is not None else ()) + (_dptr(r), _dptr(u), C.byref(conv), C.byref(it))
     call(fn, *args)
     return True, conv.value, it.value
+
+
+def Solve_CGNR(conv_ratio, iteration, A, r, u):
+    """MatVec::Sol::Solve_CGNR (src/matvec/zsolver_mat_iter.cpp:394-483): CG on
+    the normal equations, for any nonsingular complex matrix."""
+    return _zsolve("dfm_zsolve_cgnr", conv_ratio, iteration, A, None, r, u)
 
 
 def Solve_COCG(conv_ratio, iteration, A, r, u):

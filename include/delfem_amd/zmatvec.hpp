@@ -8,7 +8,7 @@
 //   MatVec::CZMatDia_BlkCrs     include/delfem/matvec/zmatdia_blkcrs.h:39-100
 //   MatVec::CZMatPrecond_Blk    include/delfem/matvec/zmatprecond_blk.h:34-38
 //   MatVec::CZMatDiaFrac_BlkCrs include/delfem/matvec/zmatdiafrac_blkcrs.h:44-96
-//   MatVec::Sol::Solve_COCG / Solve_PCOCG / Solve_BiCGSTAB / Solve_BiCGStabP
+//   MatVec::Sol::Solve_CGNR / Solve_COCG / Solve_PCOCG / Solve_BiCGSTAB / Solve_BiCGStabP
 //                               include/delfem/matvec/zsolver_mat_iter.h:51-80
 //
 // The preconditioned solvers take the factor class itself (built on the same
@@ -110,6 +110,11 @@ class CZMatDia_BlkCrs {
   bool MatVec(double alpha, const CZVector_Blk& x, double beta, CZVector_Blk& y) const {
     return check(dfm_zbcrs_matvec(h_, alpha, x.device_ptr(), beta, y.device_ptr()), "MatVec") == DFM_OK;
   }
+  // y = beta*y + alpha*A^H*x (zmatdia_blkcrs.cpp:803-847)
+  bool MatVec_Hermitian(double alpha, const CZVector_Blk& x, double beta, CZVector_Blk& y) const {
+    return check(dfm_zbcrs_matvec_hermitian(h_, alpha, x.device_ptr(), beta, y.device_ptr()),
+                 "MatVec_Hermitian") == DFM_OK;
+  }
   void GetValues(std::vector<std::complex<double>>& val, std::vector<std::complex<double>>& dia) const {
     const auto i = info();
     val.resize(i.ncrs);
@@ -197,6 +202,13 @@ inline const CZMatDiaFrac_BlkCrs& device_zfactor(const CZMatDia_BlkCrs& mat, con
 }
 }  // namespace detail
 
+// Solve_CGNR (zsolver_mat_iter.cpp:394-483)
+inline bool Solve_CGNR(double& conv_ratio, unsigned& iteration, const CZMatDia_BlkCrs& mat, CZVector_Blk& r_vec,
+                       CZVector_Blk& u_vec) {
+  check(dfm_zsolve_cgnr(mat.handle(), r_vec.device_ptr(), u_vec.device_ptr(), &conv_ratio, &iteration),
+        "Sol::Solve_CGNR");
+  return true;
+}
 // Solve_COCG (zsolver_mat_iter.cpp:220-299)
 inline bool Solve_COCG(double& conv_ratio, unsigned& iteration, const CZMatDia_BlkCrs& mat, CZVector_Blk& r_vec,
                        CZVector_Blk& u_vec) {

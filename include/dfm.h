@@ -348,6 +348,8 @@ int dfm_zbcrs_mearge(dfm_zbcrs *A, unsigned nblkel, const unsigned *blkel,
 int dfm_zbcrs_set_bc(dfm_zbcrs *A, const int *flag);                       /* SetBoundaryCondition zmatdia_blkcrs.cpp:239-268 (host flags) */
 int dfm_zbcrs_matvec(dfm_zbcrs *A, double alpha, const double *x_dev, double beta,
                      double *y_dev);                                       /* MatVec zmatdia_blkcrs.cpp:737-770 */
+int dfm_zbcrs_matvec_hermitian(dfm_zbcrs *A, double alpha, const double *x_dev, double beta,
+                               double *y_dev);                             /* MatVec_Hermitian zmatdia_blkcrs.cpp:803-847: alpha A^H x + beta y */
 int dfm_zvec_dot(size_t n, const double *x_dev, const double *y_dev, double *result); /* CZVector_Blk operator* (no conjugate) zvector_blk.cpp:39-60; result[2] */
 int dfm_zvec_norm2(size_t n, const double *x_dev, double *result);                    /* GetSquaredVectorNorm zvector_blk.cpp:154-161 */
 int dfm_zvec_zero_bc(size_t n, const int *flag, double *x_dev);                        /* CBCFlag::SetZeroToBCDof on a complex vector */
@@ -387,6 +389,10 @@ int dfm_zilu_solve_precond(dfm_zilu *M, double *vec_dev);
  * the reference returns true. */
 int dfm_zsolve_pcocg(dfm_zbcrs *A, dfm_zilu *M, double *r_dev, double *u_dev,
                      double *conv_ratio, unsigned *iteration);
+/* MatVec::Sol::Solve_CGNR zsolver_mat_iter.cpp:394-483: CG on A^H A u = A^H b (any nonsingular
+ * complex A).  r_dev in right-hand side, out final residual; u_dev out; conv_ratio / iteration
+ * as dfm_zsolve_pcocg (unchanged tolerance and max_iter when not reached).  Returns DFM_OK. */
+int dfm_zsolve_cgnr(dfm_zbcrs *A, double *r_dev, double *u_dev, double *conv_ratio, unsigned *iteration);
 /* MatVec::Sol::Solve_COCG zsolver_mat_iter.cpp:220-299 (unpreconditioned PCOCG). */
 int dfm_zsolve_cocg(dfm_zbcrs *A, double *r_dev, double *u_dev, double *conv_ratio, unsigned *iteration);
 /* MatVec::Sol::Solve_BiCGSTAB zsolver_mat_iter.cpp:487-616 and Solve_BiCGStabP :622-739 (with

@@ -1889,6 +1889,25 @@ void orc_zmatvec(unsigned nb, const unsigned *rowptr, const unsigned *col, const
         yv[i] = zadd(t, zmul(zdscale(alpha, d[i]), xv[i]));
     }
 }
+/* InnerProduct(a, b) = a conj(b) (include/delfem/complex.h:153-156) */
+static zc zip(zc a, zc b) { zc r = {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im}; return r; }
+/* CZMatDia_BlkCrs::MatVec_Hermitian, BlkSize 1 (zmatdia_blkcrs.cpp:803-847):
+ * y *= beta, then row by row j: y_i += alpha InnerProduct(x_j, a_ji) for the
+ * row's entries, then y_j += alpha InnerProduct(x_j, d_j) -- y = alpha A^H x + beta y. */
+void orc_zmatvec_hermitian(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val,
+                           const double *dia, double alpha, const double *x, double beta, double *y)
+{
+    const zc *v = (const zc *)val, *d = (const zc *)dia, *xv = (const zc *)x;
+    zc *yv = (zc *)y;
+    for (unsigned i = 0; i < nb; i++) { yv[i].re *= beta; yv[i].im *= beta; }
+    for (unsigned j = 0; j < nb; j++) {
+        for (unsigned q = rowptr[j]; q < rowptr[j + 1]; q++) {
+            const unsigned i = col[q];
+            yv[i] = zadd(yv[i], zdscale(alpha, zip(xv[j], v[q])));
+        }
+        yv[j] = zadd(yv[j], zdscale(alpha, zip(xv[j], d[j])));
+    }
+}
 /* CZMatDiaFrac_BlkCrs::SetValue (copy of A into the factor pattern, fill
  * zero) with an optional order[new] = old, as the real ordered copy */
 void orc_zilu_setvalue(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val,
@@ -2109,6 +2128,48 @@ int orc_zsolve_cocg(unsigned nb, const unsigned *rowptr, const unsigned *col, co
         for (size_t i = 0; i < n; i++) p[i] = zadd(zmul(p[i], beta), rv[i]);
     }
     free(p); free(Ap);
+    return 1;
+}
+/* MatVec::Sol::Solve_CGNR (zsolver_mat_iter.cpp:394-483): CG on the normal
+ * equations A^H A u = A^H b with z = A^H r; squared norms only, so alpha and
+ * beta are real.  r in rhs / out final residual, u out (updated also in the
+ * last step).  Always returns 1; conv_ratio unchanged and iteration =
+ * max_iter when the tolerance is not reached. */
+int orc_zsolve_cgnr(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                    double *r, double *u, double *conv_ratio, unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = nb;
+    zc *rv = (zc *)r, *uv = (zc *)u;
+    memset(u, 0, sizeof(double) * 2 * n);
+    const double sq0 = znorm2(n, rv);
+    if (sq0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; return 1; }
+    const double inv = 1.0 / sq0;
+    zc *p = (zc *)calloc(n, sizeof(zc)), *z = (zc *)calloc(n, sizeof(zc)), *Ap = (zc *)calloc(n, sizeof(zc));
+    orc_zmatvec_hermitian(nb, rowptr, col, val, dia, 1.0, r, 0.0, (double *)z);
+    double sqz0 = znorm2(n, z);
+    memcpy(p, z, sizeof(zc) * n);
+    *iteration = mx_iter;
+    for (unsigned it = 1; it < mx_iter; it++) {
+        orc_zmatvec(nb, rowptr, col, val, dia, 1.0, (const double *)p, 0.0, (double *)Ap);
+        const double alpha = sqz0 / znorm2(n, Ap);
+        for (size_t i = 0; i < n; i++) uv[i] = zadd(uv[i], zdscale(alpha, p[i]));
+        for (size_t i = 0; i < n; i++) rv[i] = zadd(rv[i], zdscale(-alpha, Ap[i]));
+        const double ratio = sqrt(znorm2(n, rv) * inv);
+        if (ratio < tol) {
+            *conv_ratio = ratio;
+            *iteration = it;
+            break;
+        }
+        orc_zmatvec_hermitian(nb, rowptr, col, val, dia, 1.0, r, 0.0, (double *)z);
+        const double sqz1 = znorm2(n, z);
+        const double beta = sqz1 / sqz0;
+        sqz0 = sqz1;
+        for (size_t i = 0; i < n; i++) { p[i].re *= beta; p[i].im *= beta; }
+        for (size_t i = 0; i < n; i++) p[i] = zadd(p[i], z[i]);
+    }
+    free(p); free(z); free(Ap);
     return 1;
 }
 /* MatVec::Sol::Solve_BiCGSTAB (zsolver_mat_iter.cpp:487-616, unpreconditioned)

@@ -174,6 +174,10 @@ void orc_ms_get_pattern(void *p, unsigned i, unsigned j, unsigned *rowptr, unsig
 void orc_ms_get_factor(void *p, unsigned i, unsigned j, double *fval, double *fdia);
 void orc_ms_precond(void *p, double *z);
 int orc_ms_solve(void *p, int bicg, double *r, double *x, double *conv_ratio, unsigned *iteration);
+void orc_zmatvec_hermitian(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val,
+                           const double *dia, double alpha, const double *x, double beta, double *y);
+int orc_zsolve_cgnr(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                    double *r, double *u, double *conv_ratio, unsigned *iteration);
 int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
                      const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const double *fval,
                      const double *fdia, const int *order, double *r, double *u, double *conv_ratio,

@@ -114,6 +114,9 @@ class Orc:
         L.orc_ms_precond.argtypes = [vp, vp]
         L.orc_ms_solve.restype = C.c_int
         L.orc_ms_solve.argtypes = [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_zmatvec_hermitian.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_double, vp, C.c_double, vp]
+        L.orc_zsolve_cgnr.restype = C.c_int
+        L.orc_zsolve_cgnr.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zsolve_cocg.restype = C.c_int
         L.orc_zsolve_cocg.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zsolve_bicgstab.restype = C.c_int
@@ -275,6 +278,23 @@ class Orc:
         self.L.orc_zmatvec(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)), float(alpha),
                            _p(c128(x)), float(beta), _p(y))
         return y
+
+    def zmatvec_hermitian(self, nb, rowptr, col, val, dia, alpha, x, beta, y):
+        """CZMatDia_BlkCrs::MatVec_Hermitian: alpha A^H x + beta y."""
+        y = c128(y).copy()
+        self.L.orc_zmatvec_hermitian(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)),
+                                     float(alpha), _p(c128(x)), float(beta), _p(y))
+        return y
+
+    def zsolve_cgnr(self, nb, rowptr, col, val, dia, rhs, tol, maxit):
+        """MatVec::Sol::Solve_CGNR: (ok, conv, iter, u, r_final)."""
+        r = c128(rhs).copy()
+        u = np.zeros(nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.L.orc_zsolve_cgnr(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)), _p(r), _p(u),
+                                    C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
 
     def zilu_values(self, nb, rowptr, col, val, dia, F, order=None):
         """Complex ILU on the pattern F (orc_iluk_symbolic): (fval, fdia, nsmall)."""
@@ -484,6 +504,8 @@ class Ref:
             "ref_zsys_set_bc": (None, [vp, vp]),
             "ref_zsys_get": (None, [vp, vp, vp]),
             "ref_zsys_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
+            "ref_zsys_matvec_hermitian": (None, [vp, C.c_double, vp, C.c_double, vp]),
+            "ref_zsolve_cgnr": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_zfrac_create": (vp, [vp, C.c_int]),
             "ref_zfrac_free": (None, [vp]),
             "ref_zfrac_ncrs": (C.c_uint, [vp]),
@@ -962,6 +984,19 @@ class RefZSystem:
         conv = C.c_double(tol)
         it = C.c_uint(maxit)
         ok = self.ref.L.ref_zsolve_cocg(self.h, _p(r), _p(u), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
+
+    def matvec_hermitian(self, alpha, x, beta, y):
+        y = c128(y).copy()
+        self.ref.L.ref_zsys_matvec_hermitian(self.h, float(alpha), _p(c128(x)), float(beta), _p(y))
+        return y
+
+    def cgnr(self, rhs, tol, maxit):
+        r = c128(rhs).copy()
+        u = np.zeros(self.nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_zsolve_cgnr(self.h, _p(r), _p(u), C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value, u, r
 
     def bicgstab(self, rhs, tol, maxit, precond):

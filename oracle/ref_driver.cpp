@@ -427,6 +427,14 @@ void ref_zsys_matvec(void* p, double alpha, const double* x, double beta, double
   z->A->MatVec(alpha, xv, beta, yv);
   zcopy_out(yv, z->nb, y);
 }
+void ref_zsys_matvec_hermitian(void* p, double alpha, const double* x, double beta, double* y) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  MatVec::CZVector_Blk xv(z->nb, 1), yv(z->nb, 1);
+  zcopy_in(xv, z->nb, x);
+  zcopy_in(yv, z->nb, y);
+  z->A->MatVec_Hermitian(alpha, xv, beta, yv);
+  zcopy_out(yv, z->nb, y);
+}
 void* ref_zfrac_create(void* p, int lev) { return new ZFracPeek(lev, *static_cast<RefZSys*>(p)->A); }
 void ref_zfrac_free(void* f) { delete static_cast<ZFracPeek*>(f); }
 unsigned ref_zfrac_ncrs(void* f) { return static_cast<ZFracPeek*>(f)->NCrs(); }
@@ -663,6 +671,16 @@ void ref_bmat_matvec(void* p, double alpha, const double* x, double beta, double
 
 // MatVec::Sol::Solve_COCG / Solve_BiCGSTAB / Solve_BiCGStabP (the last with the factor).
 static void zero_new_blocks() { mallopt(M_PERTURB, 0xff); }
+// MatVec::Sol::Solve_CGNR; r in/out, u out
+int ref_zsolve_cgnr(void* p, double* r, double* u, double* conv, unsigned* iter) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  MatVec::CZVector_Blk rv(z->nb, 1), uv(z->nb, 1);
+  zcopy_in(rv, z->nb, r);
+  const bool ok = MatVec::Sol::Solve_CGNR(*conv, *iter, *z->A, rv, uv);
+  zcopy_out(rv, z->nb, r);
+  zcopy_out(uv, z->nb, u);
+  return ok ? 1 : 0;
+}
 int ref_zsolve_cocg(void* p, double* r, double* u, double* conv, unsigned* iter) {
   zero_new_blocks();
   RefZSys* z = static_cast<RefZSys*>(p);

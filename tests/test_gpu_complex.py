@@ -396,3 +396,35 @@ def test_zsolvers_golden(gpu):
         assert ok and conv < 1e-10, key
         assert abs(it - int(want[2])) <= max(2, int(np.ceil(frac * want[2]))), (key, it, want[2])
         assert np.linalg.norm(host(u) - d[key + "_u"]) <= err * np.linalg.norm(d[key + "_u"]), key
+
+
+def test_hermitian_matvec_and_cgnr(orc, gpu):
+    """MatVec_Hermitian bit for bit against the oracle (which equals the
+    compiled reference bit for bit) for beta 0, 1 and another; Solve_CGNR
+    within 5 % of the oracle's iterations and 1e-8 of its solution, on the
+    complex Helmholtz system (CGNR needs no real-valued data)."""
+    import torch
+    from oracle.oracle import assemble_zproblem
+    prob = meshgen.zhelmholtz2d(20, 15, 1.0, 0.75, wave_length=0.35)
+    print("[case] zhelmholtz2d 20x15 CGNR", flush=True)
+    Z = assemble_zproblem(orc, prob)
+    nb = prob.nnode
+    A = gpu.CZMatDia_BlkCrs(nb, 1)
+    A.AddPattern(Z["rowptr"], Z["col"])
+    A.set_values(Z["val"], Z["dia"])
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(nb) + 1j * rng.standard_normal(nb)
+    y = rng.standard_normal(nb) + 1j * rng.standard_normal(nb)
+    for alpha, beta in ((1.0, 0.0), (-0.7, 1.0), (0.3, -1.9)):
+        yd = torch.from_numpy(y.copy()).cuda()
+        A.MatVec_Hermitian(alpha, torch.from_numpy(x).cuda(), beta, yd)
+        want = orc.zmatvec_hermitian(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], alpha, x, beta, y)
+        assert np.array_equal(yd.cpu().numpy(), want)
+    r = torch.from_numpy(Z["res"].copy()).cuda()
+    u = torch.zeros_like(r)
+    ok, conv, it = gpu.Solve_CGNR(1e-10, 5000, A, r, u)
+    want = orc.zsolve_cgnr(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Z["res"], 1e-10, 5000)
+    print(f"  device {it} iterations, oracle {want[2]}", flush=True)
+    assert ok and conv < 1e-10 and abs(it - want[2]) <= max(2, int(np.ceil(0.05 * want[2])))
+    uh = u.cpu().numpy()
+    assert np.linalg.norm(uh - want[3]) <= 1e-8 * np.linalg.norm(want[3])

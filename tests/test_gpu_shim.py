@@ -129,12 +129,13 @@ ZBIN = os.path.join(HERE, "cpp", "zshim_test")
 ZKINDS = {0: "cocg", 1: "pcocg", 2: "bicgstab", 3: "bicgstabp"}
 
 
-@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4])
 def test_cpp_complex_dropin(orc, gpu, tmp_path, kind):
     """The complex classes (include/delfem_amd/zmatvec.hpp) driven as the
     reference's Helmholtz code drives its own: per-element Mearge and
-    SetBoundaryCondition bit-exact; Solve_COCG / Solve_PCOCG (ILU(1)) within
-    5 % of the oracle's iterations and 1e-8 of its solution; the
+    SetBoundaryCondition bit-exact; Solve_COCG / Solve_PCOCG (ILU(1)) /
+    Solve_CGNR within 5 % of the oracle's iterations and 1e-8 of its
+    solution; the
     BiCGSTAB pair, which the reference's conjugated inner products keep from
     converging on this complex system, through its first five iterations
     within 1e-9."""
@@ -144,7 +145,7 @@ def test_cpp_complex_dropin(orc, gpu, tmp_path, kind):
     prob = meshgen.zhelmholtz2d(40, 30, 2.0, 1.5, wave_length=0.6)
     Z = assemble_zproblem(orc, prob)
     n, lev = prob.nnode, 1
-    maxit, tol = (3000, 1e-10) if kind <= 1 else (6, 1e-10)
+    maxit, tol = (3000, 1e-10) if kind <= 1 else (6, 1e-10) if kind <= 3 else (10000, 1e-10)
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
         np.array([n, prob.conn.shape[0], prob.lconn.shape[0], kind, maxit, lev], dtype=np.int32).tofile(f)
@@ -174,10 +175,12 @@ def test_cpp_complex_dropin(orc, gpu, tmp_path, kind):
         want = orc.zsolve_pcocg(*args, F, fv, fd, Z["res"], tol, maxit)
     elif kind == 2:
         want = orc.zsolve_bicgstab(*args, Z["res"], tol, maxit)
-    else:
+    elif kind == 3:
         want = orc.zsolve_bicgstab(*args, Z["res"], tol, maxit, F, fv, fd)
+    else:
+        want = orc.zsolve_cgnr(*args, Z["res"], tol, maxit)
     assert ok == 1 and want[0]
-    if kind <= 1:
+    if kind in (0, 1, 4):
         assert conv < tol and abs(int(it) - want[2]) <= max(2, int(np.ceil(0.05 * want[2])))
         assert np.linalg.norm(u - want[3]) <= 1e-8 * np.linalg.norm(want[3])
     else:

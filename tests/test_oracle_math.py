@@ -604,3 +604,45 @@ def test_multisegment_fill_is_global_ilu(orc, case):
     x = np.random.default_rng(5).standard_normal(O.res.size)
     z = O.precond(O.matvec(1.0, x, 0.0, np.zeros_like(x)))
     assert np.linalg.norm(z - x) <= 1e-11 * np.linalg.norm(x)
+
+
+def test_complex_cgnr_and_hermitian_matvec(orc):
+    """MatVec_Hermitian is alpha A^H x + beta y of the dense matrix; Solve_CGNR
+    is CG on A^H A u = A^H b: a textbook CGNR in numpy takes the same number
+    of iterations (its residual ratios agree to round-off) and both reach
+    the direct solution of A u = b."""
+    from oracle.oracle import assemble_zproblem
+    prob = meshgen.zhelmholtz2d(8, 6, 1.0, 0.75, wave_length=0.35)
+    Z = assemble_zproblem(orc, prob)
+    nb = prob.nnode
+    A = _zdense(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"])
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(nb) + 1j * rng.standard_normal(nb)
+    y = rng.standard_normal(nb) + 1j * rng.standard_normal(nb)
+    got = orc.zmatvec_hermitian(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], 0.6, x, -1.3, y)
+    assert np.allclose(got, 0.6 * (A.conj().T @ x) - 1.3 * y, rtol=1e-13, atol=1e-13)
+    b = Z["res"]
+    tol = 1e-9
+    ok, conv, it, u, r = orc.zsolve_cgnr(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], b, tol, 5000)
+    # textbook CGNR (Saad, Iterative Methods, Alg. 8.4) with the same stopping rule
+    uu, rr = np.zeros(nb, dtype=complex), b.copy()
+    z = A.conj().T @ rr
+    p = z.copy()
+    zz = np.vdot(z, z).real
+    it_t = None
+    for k in range(1, 5000):
+        w = A @ p
+        a = zz / np.vdot(w, w).real
+        uu += a * p
+        rr -= a * w
+        if np.sqrt(np.vdot(rr, rr).real / np.vdot(b, b).real) < tol:
+            it_t = k
+            break
+        z = A.conj().T @ rr
+        zz1 = np.vdot(z, z).real
+        p = z + (zz1 / zz) * p
+        zz = zz1
+    assert ok and conv < tol and it_t is not None and abs(it - it_t) <= 1
+    xd = np.linalg.solve(A, b)
+    assert np.linalg.norm(u - xd) <= 1e-6 * np.linalg.norm(xd)
+    assert np.linalg.norm(uu - xd) <= 1e-6 * np.linalg.norm(xd)

@@ -124,9 +124,10 @@ def test_complex_twin_vs_reference(orc, ref, lev):
 
 @pytest.mark.parametrize("maxit", [2, 7, 3000])
 def test_complex_solvers_vs_reference(orc, ref, maxit):
-    """Solve_COCG, Solve_BiCGSTAB and Solve_BiCGStabP (ILU(0)) against the
-    reference on the complex Helmholtz problem, bit for bit (NaN equal to
-    NaN: the reference's complex BiCGSTAB pair breaks down there)."""
+    """Solve_CGNR, Solve_COCG, Solve_BiCGSTAB and Solve_BiCGStabP (ILU(0))
+    against the reference on the complex Helmholtz problem, bit for bit (NaN
+    equal to NaN: the reference's complex BiCGSTAB pair breaks down there);
+    MatVec_Hermitian with beta 0, 1 and another."""
     from oracle.oracle import RefZSystem, assemble_zproblem
     prob = meshgen.zhelmholtz2d(14, 9, 1.2, 0.8, wave_length=0.3)
     Z = assemble_zproblem(orc, prob)
@@ -139,12 +140,19 @@ def test_complex_solvers_vs_reference(orc, ref, maxit):
     F = orc.iluk_symbolic(Z["rowptr"], Z["col"], 0)
     fv, fd, _ = orc.zilu_values(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], F)
     args = (nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], Z["res"], 1e-10, maxit)
-    for got, exp in ((orc.zsolve_cocg(*args), rz.cocg(Z["res"], 1e-10, maxit)),
+    for got, exp in ((orc.zsolve_cgnr(*args), rz.cgnr(Z["res"], 1e-10, maxit)),
+                     (orc.zsolve_cocg(*args), rz.cocg(Z["res"], 1e-10, maxit)),
                      (orc.zsolve_bicgstab(*args), rz.bicgstab(Z["res"], 1e-10, maxit, False)),
                      (orc.zsolve_bicgstab(*args, F, fv, fd), rz.bicgstab(Z["res"], 1e-10, maxit, True))):
         assert got[0] == exp[0] and got[2] == exp[2]
         assert got[1] == exp[1] or (np.isnan(got[1]) and np.isnan(exp[1]))
         assert np.array_equal(got[3], exp[3], equal_nan=True) and np.array_equal(got[4], exp[4], equal_nan=True)
+    rng = np.random.default_rng(maxit)
+    x = rng.standard_normal(nb) + 1j * rng.standard_normal(nb)
+    y = rng.standard_normal(nb) + 1j * rng.standard_normal(nb)
+    for alpha, beta in ((1.0, 0.0), (-0.7, 1.0), (0.3, -1.9)):
+        got = orc.zmatvec_hermitian(nb, Z["rowptr"], Z["col"], Z["val"], Z["dia"], alpha, x, beta, y)
+        assert np.array_equal(got, rz.matvec_hermitian(alpha, x, beta, y))
 
 
 @pytest.mark.parametrize("kind", ["poisson2d", "solid3d"])
