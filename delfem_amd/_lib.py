@@ -173,6 +173,9 @@ _SIGS = {
     "dfm_zsolve_pcocg": (C.c_int, [H, H, vp, vp, f64p, u32p]),
     "dfm_zsolve_cocg": (C.c_int, [H, vp, vp, f64p, u32p]),
     "dfm_zsolve_cgnr": (C.c_int, [H, vp, vp, f64p, u32p]),
+    "dfm_zsolve_cg": (C.c_int, [H, vp, vp, f64p, u32p]),
+    "dfm_zsolve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p]),
+    "dfm_zvec_inner": (C.c_int, [C.c_size_t, vp, vp, vp]),
     "dfm_zsolve_bicgstab": (C.c_int, [H, vp, vp, f64p, u32p]),
     "dfm_zsolve_bicgstabp": (C.c_int, [H, H, vp, vp, f64p, u32p]),
     "dfm_bench_kernel": (C.c_int, [H, H, C.c_int, C.c_int, C.c_int, f64p, f64p]),

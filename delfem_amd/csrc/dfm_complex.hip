@@ -1202,6 +1202,9 @@ static int zreduce(int nd, size_t n, const double2* x, const double2* y, double*
   if (nd == 2) {
     DFM_ZLAUNCH(k_zdot, g, kNT, (long)n, x, y, partials.p, (const ZState*)nullptr);
     DFM_ZLAUNCH((k_zfinalize<2, ZFIN_STORE>), 1, kFinNT, partials.p, g, st.p);
+  } else if (nd == 3) {  // conjugated: sum conj(x_i) y_i
+    DFM_ZLAUNCH(k_zdotc, g, kNT, (long)n, x, y, partials.p);
+    DFM_ZLAUNCH((k_zfinalize<2, ZFIN_STORE>), 1, kFinNT, partials.p, g, st.p);
   } else {
     DFM_ZLAUNCH(k_znorm2, g, kNT, (long)n, x, partials.p);
     DFM_ZLAUNCH((k_zfinalize<1, ZFIN_STORE>), 1, kFinNT, partials.p, g, st.p);
@@ -1209,8 +1212,12 @@ static int zreduce(int nd, size_t n, const double2* x, const double2* y, double*
   DFM_HIP(hipMemcpyAsync(h, st.p, sizeof(ZState), hipMemcpyDeviceToHost, stream()));
   DFM_HIP(hipStreamSynchronize(stream()));
   out[0] = h->d0;
-  if (nd == 2) out[1] = h->d1;
+  if (nd >= 2) out[1] = h->d1;
   return DFM_OK;
+}
+int dfm_zvec_inner(size_t n, const double* x, const double* y, double* result) {
+  DFM_CHECK(x && y && result, DFM_BAD_ARG, "bad arguments to dfm_zvec_inner");
+  return zreduce(3, n, (const double2*)x, (const double2*)y, result);
 }
 int dfm_zvec_dot(size_t n, const double* x, const double* y, double* result) {
   DFM_CHECK(x && y && result, DFM_BAD_ARG, "bad arguments to dfm_zvec_dot");
@@ -1387,6 +1394,91 @@ int dfm_zsolve_cocg(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsi
   DFM_TRY(znatural_op(A, op));
   return zcocg_core(A->ctx, op, nullptr, A->pat->nb, nullptr, nullptr, (double2*)r, (double2*)u, conv_ratio,
                     iteration);
+}
+// MatVec::Sol::Solve_CG / Solve_PCG for a Hermitian matrix
+// (zsolver_mat_iter.cpp:32-119, 121-218): CG with the real part of the
+// conjugated inner product; host-driven loops as Solve_CGNR.  Solve_CG stops
+// when |r|^2/|r0|^2 < tol^2 and counts from 1; Solve_PCG applies the factor
+// (SolvePrecond), counts from 0 and reports DFM_NOT_CONVERGED at max_iter
+// (the reference's false) with conv_ratio and iteration as given.
+static int zcg_herm(dfm_zbcrs* A, dfm_zilu* Z, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  DFM_TRY(zsolve_checks(A, Z, r, u, conv_ratio, iteration));
+  DFM_TRY(ensure_init());
+  const size_t nb = A->pat->nb, n2 = 2 * nb;
+  const double tol = *conv_ratio;
+  const unsigned mx_iter = *iteration;
+  DFM_TRY(dfm_vec_fill(n2, 0.0, u));
+  double sq0 = 0.0;
+  DFM_TRY(dfm_zvec_norm2(nb, r, &sq0));
+  if (sq0 < 1.0e-30) {
+    *conv_ratio = 0.0;
+    *iteration = 0;
+    return DFM_OK;
+  }
+  const double inv = 1.0 / sq0;
+  DevBuf<double> w;
+  DFM_TRY(w.alloc(3 * std::max<size_t>(1, n2)));
+  double *p = w.p, *Ap = w.p + n2, *z = w.p + 2 * n2;
+  DFM_TRY(dfm_vec_fill(3 * n2, 0.0, w.p));
+  double c[2];
+  if (!Z) {
+    double sq_res0 = sq0;
+    DFM_TRY(dfm_vec_copy(n2, r, p));
+    *iteration = mx_iter;
+    for (unsigned it = 1; it < mx_iter; it++) {
+      DFM_TRY(dfm_zbcrs_matvec(A, 1.0, p, 0.0, Ap));
+      DFM_TRY(dfm_zvec_inner(nb, p, Ap, c));
+      const double alpha = sq_res0 / c[0];
+      DFM_TRY(dfm_vec_axpy(n2, alpha, p, u));
+      DFM_TRY(dfm_vec_axpy(n2, -alpha, Ap, r));
+      double sq1 = 0.0;
+      DFM_TRY(dfm_zvec_norm2(nb, r, &sq1));
+      if (sq1 * inv < tol * tol) {
+        *conv_ratio = std::sqrt(sq1 * inv);
+        *iteration = it;
+        return DFM_OK;
+      }
+      const double beta = sq1 / sq_res0;
+      sq_res0 = sq1;
+      DFM_TRY(dfm_vec_scal(n2, beta, p));
+      DFM_TRY(dfm_vec_axpy(n2, 1.0, r, p));
+    }
+    return DFM_OK;
+  }
+  DFM_TRY(dfm_vec_copy(n2, r, z));
+  DFM_TRY(dfm_zilu_solve_precond(Z, z));
+  DFM_TRY(dfm_vec_copy(n2, z, p));
+  DFM_TRY(dfm_zvec_inner(nb, r, z, c));
+  double rz0 = c[0];
+  for (unsigned it = 0; it < mx_iter; it++) {
+    DFM_TRY(dfm_zbcrs_matvec(A, 1.0, p, 0.0, Ap));
+    DFM_TRY(dfm_zvec_inner(nb, p, Ap, c));
+    const double alpha = rz0 / c[0];
+    DFM_TRY(dfm_vec_axpy(n2, alpha, p, u));
+    DFM_TRY(dfm_vec_axpy(n2, -alpha, Ap, r));
+    double sq = 0.0;
+    DFM_TRY(dfm_zvec_norm2(nb, r, &sq));
+    if (sq * inv < tol * tol) {
+      *conv_ratio = std::sqrt(sq * inv);
+      *iteration = it;
+      return DFM_OK;
+    }
+    DFM_TRY(dfm_vec_copy(n2, r, z));
+    DFM_TRY(dfm_zilu_solve_precond(Z, z));
+    DFM_TRY(dfm_zvec_inner(nb, r, z, c));
+    const double beta = c[0] / rz0;
+    rz0 = c[0];
+    DFM_TRY(dfm_vec_scal(n2, beta, p));
+    DFM_TRY(dfm_vec_axpy(n2, 1.0, z, p));
+  }
+  return DFM_NOT_CONVERGED;
+}
+int dfm_zsolve_cg(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  return zcg_herm(A, nullptr, r, u, conv_ratio, iteration);
+}
+int dfm_zsolve_pcg(dfm_zbcrs* A, dfm_zilu* Z, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  DFM_CHECK(Z, DFM_BAD_ARG, "dfm_zsolve_pcg needs a factor");
+  return zcg_herm(A, Z, r, u, conv_ratio, iteration);
 }
 // MatVec::Sol::Solve_CGNR (zsolver_mat_iter.cpp:394-483): CG on the normal
 // equations with z = A^H r; alpha and beta are ratios of squared norms.  A

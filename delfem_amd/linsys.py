@@ -980,6 +980,24 @@ def _zsolve(fn, conv_ratio, iteration, A, prec, r, u):
     return True, conv.value, it.value
 
 
+def Solve_CG_Hermitian(conv_ratio, iteration, A, r, u):
+    """MatVec::Sol::Solve_CG of the complex classes (src/matvec/zsolver_mat_iter.cpp:32-119):
+    CG for a Hermitian complex matrix."""
+    return _zsolve("dfm_zsolve_cg", conv_ratio, iteration, A, None, r, u)
+
+
+def Solve_PCG_Hermitian(conv_ratio, iteration, A, prec, r, u):
+    """MatVec::Sol::Solve_PCG of the complex classes (zsolver_mat_iter.cpp:121-218) with
+    the complex ILU factor; returns (converged, conv_ratio, iteration) -- False at
+    max_iter, as the reference."""
+    _sync_torch()
+    conv = C.c_double(conv_ratio)
+    it = C.c_uint(iteration)
+    rc = call("dfm_zsolve_pcg", A.h, prec.h, _dptr(r), _dptr(u), C.byref(conv), C.byref(it),
+              ok=(_lib.DFM_OK, _lib.DFM_NOT_CONVERGED))
+    return rc == _lib.DFM_OK, conv.value, it.value
+
+
 def Solve_CGNR(conv_ratio, iteration, A, r, u):
     """MatVec::Sol::Solve_CGNR (src/matvec/zsolver_mat_iter.cpp:394-483): CG on
     the normal equations, for any nonsingular complex matrix."""

@@ -8,7 +8,8 @@
 //   MatVec::CZMatDia_BlkCrs     include/delfem/matvec/zmatdia_blkcrs.h:39-100
 //   MatVec::CZMatPrecond_Blk    include/delfem/matvec/zmatprecond_blk.h:34-38
 //   MatVec::CZMatDiaFrac_BlkCrs include/delfem/matvec/zmatdiafrac_blkcrs.h:44-96
-//   MatVec::Sol::Solve_CGNR / Solve_COCG / Solve_PCOCG / Solve_BiCGSTAB / Solve_BiCGStabP
+//   MatVec::Sol::Solve_CG / Solve_PCG (Hermitian) / Solve_CGNR / Solve_COCG / Solve_PCOCG /
+//   Solve_BiCGSTAB / Solve_BiCGStabP
 //                               include/delfem/matvec/zsolver_mat_iter.h:51-80
 //
 // The preconditioned solvers take the factor class itself (built on the same
@@ -202,6 +203,21 @@ inline const CZMatDiaFrac_BlkCrs& device_zfactor(const CZMatDia_BlkCrs& mat, con
 }
 }  // namespace detail
 
+// Solve_CG for a Hermitian matrix (zsolver_mat_iter.cpp:32-119)
+inline bool Solve_CG(double& conv_ratio, unsigned& iteration, const CZMatDia_BlkCrs& mat, CZVector_Blk& r_vec,
+                     CZVector_Blk& u_vec) {
+  check(dfm_zsolve_cg(mat.handle(), r_vec.device_ptr(), u_vec.device_ptr(), &conv_ratio, &iteration),
+        "Sol::Solve_CG");
+  return true;
+}
+// Solve_PCG for a Hermitian matrix (:121-218); false at max_iter
+inline bool Solve_PCG(double& conv_ratio, unsigned& iteration, const CZMatDia_BlkCrs& mat, CZVector_Blk& r_vec,
+                      CZVector_Blk& u_vec, const CZMatPrecond_Blk& precond, const CZMatDia_BlkCrs& mat_p) {
+  (void)mat_p;
+  return check(dfm_zsolve_pcg(mat.handle(), detail::device_zfactor(mat, precond).handle(), r_vec.device_ptr(),
+                              u_vec.device_ptr(), &conv_ratio, &iteration),
+               "Sol::Solve_PCG") == DFM_OK;
+}
 // Solve_CGNR (zsolver_mat_iter.cpp:394-483)
 inline bool Solve_CGNR(double& conv_ratio, unsigned& iteration, const CZMatDia_BlkCrs& mat, CZVector_Blk& r_vec,
                        CZVector_Blk& u_vec) {

@@ -352,6 +352,7 @@ int dfm_zbcrs_matvec_hermitian(dfm_zbcrs *A, double alpha, const double *x_dev, 
                                double *y_dev);                             /* MatVec_Hermitian zmatdia_blkcrs.cpp:803-847: alpha A^H x + beta y */
 int dfm_zvec_dot(size_t n, const double *x_dev, const double *y_dev, double *result); /* CZVector_Blk operator* (no conjugate) zvector_blk.cpp:39-60; result[2] */
 int dfm_zvec_norm2(size_t n, const double *x_dev, double *result);                    /* GetSquaredVectorNorm zvector_blk.cpp:154-161 */
+int dfm_zvec_inner(size_t n, const double *x_dev, const double *y_dev, double *result); /* InnerProduct (sum conj(x) y) zvector_blk.cpp:62-80; result[2] */
 int dfm_zvec_zero_bc(size_t n, const int *flag, double *x_dev);                        /* CBCFlag::SetZeroToBCDof on a complex vector */
 /* Element assembly (src/femeqn/eqn_helmholtz.cpp AddLinSys_Helmholtz loops):
  * plans made on the complex matrix's pattern add element rows in element
@@ -393,6 +394,14 @@ int dfm_zsolve_pcocg(dfm_zbcrs *A, dfm_zilu *M, double *r_dev, double *u_dev,
  * complex A).  r_dev in right-hand side, out final residual; u_dev out; conv_ratio / iteration
  * as dfm_zsolve_pcocg (unchanged tolerance and max_iter when not reached).  Returns DFM_OK. */
 int dfm_zsolve_cgnr(dfm_zbcrs *A, double *r_dev, double *u_dev, double *conv_ratio, unsigned *iteration);
+/* MatVec::Sol::Solve_CG / Solve_PCG of the complex classes, zsolver_mat_iter.cpp:32-119 and
+ * 121-218: CG for a Hermitian matrix (real part of the conjugated inner product); Solve_PCG
+ * with the factor's SolvePrecond.  Solve_CG returns DFM_OK also at max_iter (conv_ratio
+ * unchanged); Solve_PCG returns DFM_NOT_CONVERGED there (the reference's false), its
+ * iteration count starts at 0 as the reference's. */
+int dfm_zsolve_cg(dfm_zbcrs *A, double *r_dev, double *u_dev, double *conv_ratio, unsigned *iteration);
+int dfm_zsolve_pcg(dfm_zbcrs *A, dfm_zilu *M, double *r_dev, double *u_dev, double *conv_ratio,
+                   unsigned *iteration);
 /* MatVec::Sol::Solve_COCG zsolver_mat_iter.cpp:220-299 (unpreconditioned PCOCG). */
 int dfm_zsolve_cocg(dfm_zbcrs *A, double *r_dev, double *u_dev, double *conv_ratio, unsigned *iteration);
 /* MatVec::Sol::Solve_BiCGSTAB zsolver_mat_iter.cpp:487-616 and Solve_BiCGStabP :622-739 (with

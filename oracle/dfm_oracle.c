@@ -2130,6 +2130,92 @@ int orc_zsolve_cocg(unsigned nb, const unsigned *rowptr, const unsigned *col, co
     free(p); free(Ap);
     return 1;
 }
+/* InnerProduct of vectors (zvector_blk.cpp:62-80): sum conj(a_i) b_i */
+static zc zdotc(size_t n, const zc *a, const zc *b)
+{
+    double re = 0.0, im = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        re += a[i].re * b[i].re + a[i].im * b[i].im;
+        im += a[i].re * b[i].im - a[i].im * b[i].re;
+    }
+    zc r = {re, im};
+    return r;
+}
+/* MatVec::Sol::Solve_CG / Solve_PCG for a Hermitian complex matrix
+ * (zsolver_mat_iter.cpp:32-119, 121-218): CG with the conjugated inner
+ * product's real part.  precond = 0 selects the first (stop when
+ * |r|^2/|r0|^2 < tol^2, returns 1 always); the second applies the ILU factor
+ * (loop from 0, returns 0 at max_iter with conv_ratio and iteration as given). */
+int orc_zsolve_cg_herm(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                       int precond, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                       const double *fval, const double *fdia, const int *order, double *r, double *u,
+                       double *conv_ratio, unsigned *iteration)
+{
+    const double tol = *conv_ratio;
+    const unsigned mx_iter = *iteration;
+    const size_t n = nb;
+    zc *rv = (zc *)r, *uv = (zc *)u;
+    memset(u, 0, sizeof(double) * 2 * n);
+    const double sq0 = znorm2(n, rv);
+    if (sq0 < 1.0e-30) { *conv_ratio = 0.0; *iteration = 0; return 1; }
+    const double inv = 1.0 / sq0;
+    zc *p = (zc *)calloc(n, sizeof(zc)), *Ap = (zc *)calloc(n, sizeof(zc)), *z = (zc *)calloc(n, sizeof(zc));
+    zc *tmp = (zc *)malloc(sizeof(zc) * n);
+    int *invo = NULL, rc = 1;
+    if (order) {
+        invo = (int *)malloc(sizeof(int) * n);
+        for (unsigned i = 0; i < nb; i++) invo[order[i]] = (int)i;
+    }
+    if (!precond) {
+        double sq_res0 = sq0;
+        memcpy(p, rv, sizeof(zc) * n);
+        *iteration = mx_iter;
+        for (unsigned it = 1; it < mx_iter; it++) {
+            orc_zmatvec(nb, rowptr, col, val, dia, 1.0, (const double *)p, 0.0, (double *)Ap);
+            const double alpha = sq_res0 / zdotc(n, p, Ap).re;
+            for (size_t i = 0; i < n; i++) uv[i] = zadd(uv[i], zdscale(alpha, p[i]));
+            for (size_t i = 0; i < n; i++) rv[i] = zadd(rv[i], zdscale(-alpha, Ap[i]));
+            const double sq1 = znorm2(n, rv);
+            if (sq1 * inv < tol * tol) {
+                *conv_ratio = sqrt(sq1 * inv);
+                *iteration = it;
+                break;
+            }
+            const double beta = sq1 / sq_res0;
+            sq_res0 = sq1;
+            for (size_t i = 0; i < n; i++) { p[i].re *= beta; p[i].im *= beta; }
+            for (size_t i = 0; i < n; i++) p[i] = zadd(p[i], zdscale(1.0, rv[i]));
+        }
+    } else {
+        memcpy(z, rv, sizeof(zc) * n);
+        zprecond(nb, frowptr, fcol, fdiaind, fval, fdia, order, invo, tmp, z);
+        memcpy(p, z, sizeof(zc) * n);
+        double rz0 = zdotc(n, rv, z).re;
+        rc = 0;
+        for (unsigned it = 0; it < mx_iter; it++) {
+            orc_zmatvec(nb, rowptr, col, val, dia, 1.0, (const double *)p, 0.0, (double *)Ap);
+            const double alpha = rz0 / zdotc(n, p, Ap).re;
+            for (size_t i = 0; i < n; i++) uv[i] = zadd(uv[i], zdscale(alpha, p[i]));
+            for (size_t i = 0; i < n; i++) rv[i] = zadd(rv[i], zdscale(-alpha, Ap[i]));
+            const double sq = znorm2(n, rv);
+            if (sq * inv < tol * tol) {
+                *conv_ratio = sqrt(sq * inv);
+                *iteration = it;
+                rc = 1;
+                break;
+            }
+            memcpy(z, rv, sizeof(zc) * n);
+            zprecond(nb, frowptr, fcol, fdiaind, fval, fdia, order, invo, tmp, z);
+            const double rz1 = zdotc(n, rv, z).re;
+            const double beta = rz1 / rz0;
+            rz0 = rz1;
+            for (size_t i = 0; i < n; i++) { p[i].re *= beta; p[i].im *= beta; }
+            for (size_t i = 0; i < n; i++) p[i] = zadd(p[i], z[i]);
+        }
+    }
+    free(p); free(Ap); free(z); free(tmp); free(invo);
+    return rc;
+}
 /* MatVec::Sol::Solve_CGNR (zsolver_mat_iter.cpp:394-483): CG on the normal
  * equations A^H A u = A^H b with z = A^H r; squared norms only, so alpha and
  * beta are real.  r in rhs / out final residual, u out (updated also in the

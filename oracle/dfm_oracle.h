@@ -178,6 +178,10 @@ void orc_zmatvec_hermitian(unsigned nb, const unsigned *rowptr, const unsigned *
                            const double *dia, double alpha, const double *x, double beta, double *y);
 int orc_zsolve_cgnr(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
                     double *r, double *u, double *conv_ratio, unsigned *iteration);
+int orc_zsolve_cg_herm(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                       int precond, const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
+                       const double *fval, const double *fdia, const int *order, double *r, double *u,
+                       double *conv_ratio, unsigned *iteration);
 int orc_zsolve_pcocg(unsigned nb, const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
                      const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const double *fval,
                      const double *fdia, const int *order, double *r, double *u, double *conv_ratio,

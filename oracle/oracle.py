@@ -115,6 +115,9 @@ class Orc:
         L.orc_ms_solve.restype = C.c_int
         L.orc_ms_solve.argtypes = [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zmatvec_hermitian.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_double, vp, C.c_double, vp]
+        L.orc_zsolve_cg_herm.restype = C.c_int
+        L.orc_zsolve_cg_herm.argtypes = [C.c_uint, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zsolve_cgnr.restype = C.c_int
         L.orc_zsolve_cgnr.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]
         L.orc_zsolve_cocg.restype = C.c_int
@@ -285,6 +288,22 @@ class Orc:
         self.L.orc_zmatvec_hermitian(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)),
                                      float(alpha), _p(c128(x)), float(beta), _p(y))
         return y
+
+    def zsolve_cg_herm(self, nb, rowptr, col, val, dia, rhs, tol, maxit, F=None, fval=None, fdia=None, order=None):
+        """MatVec::Sol::Solve_CG (F None) or Solve_PCG with the ILU factor, for a
+        Hermitian complex matrix: (ok, conv, iter, u, r_final)."""
+        r = c128(rhs).copy()
+        u = np.zeros(nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        pre = F is not None
+        order = None if order is None else i32(order)
+        ok = self.L.orc_zsolve_cg_herm(nb, _p(u32(rowptr)), _p(u32(col)), _p(c128(val)), _p(c128(dia)), int(pre),
+                                       _p(u32(F["rowptr"])) if pre else None, _p(u32(F["col"])) if pre else None,
+                                       _p(u32(F["diaind"])) if pre else None, _p(c128(fval)) if pre else None,
+                                       _p(c128(fdia)) if pre else None, _p(order), _p(r), _p(u), C.byref(conv),
+                                       C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
 
     def zsolve_cgnr(self, nb, rowptr, col, val, dia, rhs, tol, maxit):
         """MatVec::Sol::Solve_CGNR: (ok, conv, iter, u, r_final)."""
@@ -506,6 +525,7 @@ class Ref:
             "ref_zsys_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
             "ref_zsys_matvec_hermitian": (None, [vp, C.c_double, vp, C.c_double, vp]),
             "ref_zsolve_cgnr": (C.c_int, [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_zsolve_cg_herm": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_zfrac_create": (vp, [vp, C.c_int]),
             "ref_zfrac_free": (None, [vp]),
             "ref_zfrac_ncrs": (C.c_uint, [vp]),
@@ -991,6 +1011,16 @@ class RefZSystem:
         self.ref.L.ref_zsys_matvec_hermitian(self.h, float(alpha), _p(c128(x)), float(beta), _p(y))
         return y
 
+    def cg_herm(self, rhs, tol, maxit, precond):
+        """Solve_CG, or Solve_PCG with the factor when precond."""
+        r = c128(rhs).copy()
+        u = np.zeros(self.nb, dtype=np.complex128)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_zsolve_cg_herm(self.h, self.f if precond else None, _p(r), _p(u), C.byref(conv),
+                                           C.byref(it))
+        return bool(ok), conv.value, it.value, u, r
+
     def cgnr(self, rhs, tol, maxit):
         r = c128(rhs).copy()
         u = np.zeros(self.nb, dtype=np.complex128)
@@ -1025,6 +1055,31 @@ def assemble_zproblem(orc, prob):
     val, dia, res = orc.zset_bc(prob.nnode, rowptr, col, val, dia, prob.flags, res)
     return dict(rowptr=rowptr, col=col, val=val, dia=dia, res=res, emat_tri=et, eres_tri=rt, emat_line=el,
                 eres_line=rl)
+
+
+def hermitian_zproblem(orc, prob, seed=0, skew=0.1):
+    """A Hermitian positive definite complex system on the mesh of a complex
+    Helmholtz problem (for the Hermitian Solve_CG / Solve_PCG): per triangle
+    the real part of its Helmholtz element (a long wave length makes it
+    positive definite) plus i*S with S real antisymmetric (random, scaled by
+    skew against the element's diagonal), merged, Dirichlet rows/columns;
+    right-hand side from the triangles' residuals."""
+    rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)
+    et, rt = orc.zemat_helmholtz2d(prob.coord, prob.conn, prob.wave_length, prob.value)
+    ne = prob.conn.shape[0]
+    E = et.reshape(ne, 3, 3).real.astype(np.complex128)
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(-1.0, 1.0, (ne, 3))
+    S = np.zeros((ne, 3, 3))
+    S[:, 0, 1], S[:, 0, 2], S[:, 1, 2] = a[:, 0], a[:, 1], a[:, 2]
+    S = S - S.transpose(0, 2, 1)
+    scale = skew * np.abs(np.diagonal(E.real, axis1=1, axis2=2)).min(axis=1)
+    E = E + 1j * S * scale[:, None, None]
+    emat = np.ascontiguousarray(E.reshape(-1))
+    val, dia = orc.zassemble(prob.nnode, rowptr, col, prob.conn, emat)
+    res = orc.zassemble_vec(np.zeros(prob.nnode, dtype=np.complex128), prob.conn, rt)
+    val, dia, res = orc.zset_bc(prob.nnode, rowptr, col, val, dia, prob.flags, res)
+    return dict(rowptr=rowptr, col=col, val=val, dia=dia, res=res, emat_tri=emat)
 
 
 def problem_emats(orc, prob, value=None):

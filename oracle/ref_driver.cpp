@@ -671,6 +671,17 @@ void ref_bmat_matvec(void* p, double alpha, const double* x, double beta, double
 
 // MatVec::Sol::Solve_COCG / Solve_BiCGSTAB / Solve_BiCGStabP (the last with the factor).
 static void zero_new_blocks() { mallopt(M_PERTURB, 0xff); }
+// MatVec::Sol::Solve_CG / Solve_PCG (with the factor f, or f null) for a Hermitian matrix
+int ref_zsolve_cg_herm(void* p, void* f, double* r, double* u, double* conv, unsigned* iter) {
+  RefZSys* z = static_cast<RefZSys*>(p);
+  MatVec::CZVector_Blk rv(z->nb, 1), uv(z->nb, 1);
+  zcopy_in(rv, z->nb, r);
+  const bool ok = f ? MatVec::Sol::Solve_PCG(*conv, *iter, *z->A, rv, uv, *static_cast<ZFracPeek*>(f), *z->A)
+                    : MatVec::Sol::Solve_CG(*conv, *iter, *z->A, rv, uv);
+  zcopy_out(rv, z->nb, r);
+  zcopy_out(uv, z->nb, u);
+  return ok ? 1 : 0;
+}
 // MatVec::Sol::Solve_CGNR; r in/out, u out
 int ref_zsolve_cgnr(void* p, double* r, double* u, double* conv, unsigned* iter) {
   RefZSys* z = static_cast<RefZSys*>(p);

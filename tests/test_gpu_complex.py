@@ -428,3 +428,36 @@ def test_hermitian_matvec_and_cgnr(orc, gpu):
     assert ok and conv < 1e-10 and abs(it - want[2]) <= max(2, int(np.ceil(0.05 * want[2])))
     uh = u.cpu().numpy()
     assert np.linalg.norm(uh - want[3]) <= 1e-8 * np.linalg.norm(want[3])
+
+
+def test_hermitian_cg_pcg(orc, gpu):
+    """Solve_CG and Solve_PCG (ILU(0)) of the complex classes on a Hermitian
+    positive definite system: within 5 % of the oracle's iterations (which
+    equal the compiled reference's bit for bit) and 1e-8 of its solution."""
+    import torch
+    from oracle.oracle import hermitian_zproblem
+    prob = meshgen.zhelmholtz2d(30, 20, 1.2, 0.8, wave_length=1000.0)
+    print("[case] hermitian 30x20 CG / PCG", flush=True)
+    H = hermitian_zproblem(orc, prob, seed=1)
+    nb = prob.nnode
+    A = gpu.CZMatDia_BlkCrs(nb, 1)
+    A.AddPattern(H["rowptr"], H["col"])
+    A.set_values(H["val"], H["dia"])
+    M = gpu.CZMatDiaFrac_BlkCrs(0, A)
+    M.SetValue(A)
+    F = orc.iluk_symbolic(H["rowptr"], H["col"], 0)
+    fv, fd, _ = orc.zilu_values(nb, H["rowptr"], H["col"], H["val"], H["dia"], F)
+    for pre in (False, True):
+        r = torch.from_numpy(H["res"].copy()).cuda()
+        u = torch.zeros_like(r)
+        if pre:
+            ok, conv, it = gpu.Solve_PCG_Hermitian(1e-10, 3000, A, M, r, u)
+            want = orc.zsolve_cg_herm(nb, H["rowptr"], H["col"], H["val"], H["dia"], H["res"], 1e-10, 3000, F, fv, fd)
+        else:
+            ok, conv, it = gpu.Solve_CG_Hermitian(1e-10, 3000, A, r, u)
+            want = orc.zsolve_cg_herm(nb, H["rowptr"], H["col"], H["val"], H["dia"], H["res"], 1e-10, 3000)
+        print(f"  precond {pre}: device {it} iterations, oracle {want[2]}", flush=True)
+        assert ok and want[0] and conv < 1e-10
+        assert abs(it - want[2]) <= max(2, int(np.ceil(0.05 * want[2])))
+        uh = u.cpu().numpy()
+        assert np.linalg.norm(uh - want[3]) <= 1e-8 * np.linalg.norm(want[3])

@@ -646,3 +646,39 @@ def test_complex_cgnr_and_hermitian_matvec(orc):
     xd = np.linalg.solve(A, b)
     assert np.linalg.norm(u - xd) <= 1e-6 * np.linalg.norm(xd)
     assert np.linalg.norm(uu - xd) <= 1e-6 * np.linalg.norm(xd)
+
+
+def test_complex_hermitian_cg(orc):
+    """On a Hermitian positive definite system Solve_CG is textbook CG with the
+    Hermitian inner product (same iteration count within one, numpy) and
+    Solve_PCG with ILU(0) reaches the direct solution."""
+    from oracle.oracle import hermitian_zproblem
+    prob = meshgen.zhelmholtz2d(14, 9, 1.2, 0.8, wave_length=1000.0)
+    H = hermitian_zproblem(orc, prob, seed=1)
+    nb = prob.nnode
+    A = _zdense(nb, H["rowptr"], H["col"], H["val"], H["dia"])
+    assert np.array_equal(A, A.conj().T) and np.linalg.eigvalsh(A).min() > 0
+    b = H["res"]
+    ok, conv, it, u, r = orc.zsolve_cg_herm(nb, H["rowptr"], H["col"], H["val"], H["dia"], b, 1e-10, 3000)
+    x, rr = np.zeros(nb, dtype=complex), b.copy()
+    p = rr.copy()
+    s0 = np.vdot(rr, rr).real
+    it_t = None
+    for k in range(1, 3000):
+        w = A @ p
+        a = s0 / np.vdot(p, w).real
+        x += a * p
+        rr -= a * w
+        s1 = np.vdot(rr, rr).real
+        if s1 / np.vdot(b, b).real < 1e-20:
+            it_t = k
+            break
+        p = rr + (s1 / s0) * p
+        s0 = s1
+    xd = np.linalg.solve(A, b)
+    assert ok and it_t is not None and abs(it - it_t) <= 1
+    assert np.linalg.norm(u - xd) <= 1e-8 * np.linalg.norm(xd)
+    F = orc.iluk_symbolic(H["rowptr"], H["col"], 0)
+    fv, fd, _ = orc.zilu_values(nb, H["rowptr"], H["col"], H["val"], H["dia"], F)
+    ok, conv, itp, u, r = orc.zsolve_cg_herm(nb, H["rowptr"], H["col"], H["val"], H["dia"], b, 1e-10, 3000, F, fv, fd)
+    assert ok and itp < it and np.linalg.norm(u - xd) <= 1e-8 * np.linalg.norm(xd)

@@ -155,6 +155,30 @@ def test_complex_solvers_vs_reference(orc, ref, maxit):
         assert np.array_equal(got, rz.matvec_hermitian(alpha, x, beta, y))
 
 
+@pytest.mark.parametrize("maxit", [3, 3000])
+def test_complex_hermitian_cg_vs_reference(orc, ref, maxit):
+    """Solve_CG and Solve_PCG (ILU(0)) of the complex classes on a Hermitian
+    positive definite system against the reference, bit for bit (return
+    value, conv_ratio, iterations, solution, residual)."""
+    from oracle.oracle import RefZSystem, hermitian_zproblem
+    prob = meshgen.zhelmholtz2d(14, 9, 1.2, 0.8, wave_length=1000.0)
+    H = hermitian_zproblem(orc, prob, seed=1)
+    nb = prob.nnode
+    rz = RefZSystem(ref, nb, H["rowptr"], H["col"])
+    rz.merge(prob.conn, H["emat_tri"])
+    rz.set_bc(prob.flags)
+    v, d = rz.matrix()
+    assert np.array_equal(v, H["val"]) and np.array_equal(d, H["dia"])
+    rz.factor(0)
+    F = orc.iluk_symbolic(H["rowptr"], H["col"], 0)
+    fv, fd, _ = orc.zilu_values(nb, H["rowptr"], H["col"], H["val"], H["dia"], F)
+    args = (nb, H["rowptr"], H["col"], H["val"], H["dia"], H["res"], 1e-10, maxit)
+    for got, exp in ((orc.zsolve_cg_herm(*args), rz.cg_herm(H["res"], 1e-10, maxit, False)),
+                     (orc.zsolve_cg_herm(*args, F, fv, fd), rz.cg_herm(H["res"], 1e-10, maxit, True))):
+        assert got[:3] == exp[:3]
+        assert np.array_equal(got[3], exp[3]) and np.array_equal(got[4], exp[4])
+
+
 @pytest.mark.parametrize("kind", ["poisson2d", "solid3d"])
 def test_boundary_matrix_vs_reference(orc, ref, kind):
     """CLinearSystem_Save's boundary matrix: the reference's CMat_BlkCrs
