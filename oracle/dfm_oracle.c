@@ -32,6 +32,22 @@
  *   orc_colour_order,
  *   orc_level_sets       product algorithms (no reference counterpart): pinned by
  *                        proper-colouring / topological-order invariants
+ *   orc_zemat_*, orc_zassemble, orc_zset_bc, orc_zmatvec, orc_zilu_*
+ *                        compiled reference (bitwise) + golden fixture zhelmholtz2d_10x8 +
+ *                        complex symmetry, exact LU with unlimited fill, dense product
+ *   orc_zsolve_pcocg / cocg / bicgstab
+ *                        compiled reference (bitwise) + direct solve; real-valued data reduce
+ *                        them to the real CG / BiCGSTAB
+ *   orc_zmatvec_hermitian, orc_zsolve_cgnr, orc_zsolve_cg_herm
+ *                        compiled reference (bitwise) + dense A^H x, textbook CGNR / CG in
+ *                        numpy (iterations +-1), direct solve
+ *   orc_bmat_*, orc_bmat2_*
+ *                        compiled reference (bitwise) + the free-by-fixed block of the dense
+ *                        assembled matrix
+ *   orc_ms_*             compiled reference (bitwise: blocks, MATVEC, SolvePrecond, PCG /
+ *                        BiCGSTAB runs, fill levels 0-3) + split system = permuted combined
+ *                        system (direct solve), level-1 fill = global first-level fill,
+ *                        unlimited fill = exact LU
  */
 #include "dfm_oracle.h"
 
