@@ -32,7 +32,7 @@ def build(verbose=False):
     return LIB
 
 
-SHIM_PROGRAMS = ["shim_test", "zshim_test"]  # tests/cpp/<name>.cpp -> tests/cpp/<name>
+SHIM_PROGRAMS = ["shim_test", "zshim_test", "msshim_test"]  # tests/cpp/<name>.cpp -> tests/cpp/<name>
 
 
 def build_shim_test(verbose=False):

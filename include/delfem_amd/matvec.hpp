@@ -51,11 +51,14 @@ class CVector_Blk {
     check(dfm_malloc(reinterpret_cast<void**>(&v_), sizeof(double) * size()), "CVector_Blk");
     SetVectorZero();
   }
+  // a view of nblk*len device doubles owned elsewhere (one segment of a
+  // multi-segment system's vector); it does not free them
+  CVector_Blk(unsigned nblk, unsigned len, double* view) : nblk_(nblk), len_(len), v_(view), owns_(false) {}
   ~CVector_Blk() {
-    if (v_) dfm_free(v_);
+    if (v_ && owns_) dfm_free(v_);
   }
   CVector_Blk(const CVector_Blk&) = delete;
-  CVector_Blk(CVector_Blk&& o) noexcept : nblk_(o.nblk_), len_(o.len_), v_(o.v_) { o.v_ = nullptr; }
+  CVector_Blk(CVector_Blk&& o) noexcept : nblk_(o.nblk_), len_(o.len_), v_(o.v_), owns_(o.owns_) { o.v_ = nullptr; }
   // operator= copies the values (vector_blk.cpp:62-71)
   CVector_Blk& operator=(const CVector_Blk& rhs) {
     check(dfm_vec_copy(size(), rhs.v_, v_), "CVector_Blk::operator=");
@@ -96,6 +99,7 @@ class CVector_Blk {
  private:
   unsigned nblk_, len_;
   double* v_ = nullptr;
+  bool owns_ = true;
 };
 
 class CBCFlag {
@@ -172,6 +176,76 @@ class CMatDia_BlkCrs {
     return i;
   }
   dfm_bcrs* h_ = nullptr;
+};
+
+// MatVec::CMat_BlkCrs (include/delfem/matvec/mat_blkcrs.h:45-195): a
+// rectangular block CRS matrix, nblk_col block rows of len_col dofs by
+// nblk_row block columns of len_row dofs; the off-diagonal blocks of a
+// multi-segment system and CLinearSystem_Save's boundary matrix.
+class CMat_BlkCrs {
+ public:
+  CMat_BlkCrs(unsigned nblk_col, unsigned len_col, unsigned nblk_row, unsigned len_row)
+      : nblk_col_(nblk_col), len_col_(len_col), nblk_row_(nblk_row), len_row_(len_row) {
+    check(dfm_bmat_create(nblk_col, len_col, nblk_row, len_row, &h_), "CMat_BlkCrs");
+  }
+  ~CMat_BlkCrs() {
+    if (h_) dfm_bmat_destroy(h_);
+  }
+  CMat_BlkCrs(const CMat_BlkCrs&) = delete;
+  CMat_BlkCrs& operator=(const CMat_BlkCrs&) = delete;
+  unsigned NBlkMatCol() const { return nblk_col_; }
+  unsigned NBlkMatRow() const { return nblk_row_; }
+  unsigned LenBlkCol() const { return len_col_; }
+  unsigned LenBlkRow() const { return len_row_; }
+  unsigned NCrs() const {
+    unsigned nb = 0, len = 0, ncrs = 0;
+    check(dfm_bmat_info(h_, &nb, &len, &ncrs), "CMat_BlkCrs info");
+    return ncrs;
+  }
+  // AddPattern(const Com::CIndexedArray&) (mat_blkcrs.cpp:280-414)
+  bool AddPattern(const std::vector<unsigned>& index, const std::vector<unsigned>& array) {
+    static const unsigned none = 0;
+    return check(dfm_bmat_add_pattern(h_, (unsigned)index.size() - 1, index.data(),
+                                      array.empty() ? &none : array.data()),
+                 "CMat_BlkCrs::AddPattern") == DFM_OK;
+  }
+  bool SetZero() { return check(dfm_bmat_set_zero(h_), "CMat_BlkCrs::SetZero") == DFM_OK; }
+  // Mearge of one element (mat_blkcrs.cpp:754-828); square elements only
+  bool Mearge(unsigned nblkel_col, const unsigned* blkel_col, unsigned nblkel_row, const unsigned* blkel_row,
+              unsigned blksize, const double* emat) {
+    if (nblkel_col != nblkel_row) return false;
+    for (unsigned i = 0; i < nblkel_col; i++)
+      if (blkel_col[i] != blkel_row[i]) return false;
+    return check(dfm_bmat_mearge(h_, nblkel_col, blkel_col, blksize, emat), "CMat_BlkCrs::Mearge") == DFM_OK;
+  }
+  // y = beta*y + alpha*A*x (mat_blkcrs.cpp:955-1174, isnt_trans = true)
+  bool MatVec(double alpha, const CVector_Blk& x, double beta, CVector_Blk& y, bool isnt_trans = true) const {
+    if (!isnt_trans) throw Error("CMat_BlkCrs::MatVec: the transposed product is not provided", DFM_NOT_IMPLEMENTED);
+    return check(dfm_bmat_matvec(h_, alpha, x.device_ptr(), beta, y.device_ptr()), "CMat_BlkCrs::MatVec") == DFM_OK;
+  }
+  bool SetBoundaryCondition_Row(const CBCFlag& bc) {
+    return check(dfm_bmat_set_bc_row(h_, bc.data()), "SetBoundaryCondition_Row") == DFM_OK;
+  }
+  bool SetBoundaryCondition_Colum(const CBCFlag& bc) {
+    return check(dfm_bmat_set_bc_col(h_, bc.data()), "SetBoundaryCondition_Colum") == DFM_OK;
+  }
+  bool SetBoundaryConditionInverse_Colum(const CBCFlag& bc) {
+    return check(dfm_bmat_set_bc_inverse_col(h_, bc.data()), "SetBoundaryConditionInverse_Colum") == DFM_OK;
+  }
+  // SetPatternBoundary (mat_blkcrs.cpp:1177-1265): the boundary matrix pattern of CLinearSystem_Save
+  bool SetPatternBoundary(const CMatDia_BlkCrs& A, const CBCFlag& bc_col, const CBCFlag& bc_row) {
+    return check(dfm_bmat_set_pattern_boundary(h_, A.handle(), bc_col.data(), bc_row.data()),
+                 "SetPatternBoundary") == DFM_OK;
+  }
+  void GetValues(std::vector<double>& val) const {
+    val.resize((size_t)NCrs() * len_col_ * len_row_);
+    check(dfm_bmat_get_values(h_, val.data()), "CMat_BlkCrs::GetValues");
+  }
+  dfm_bmat* handle() const { return h_; }
+
+ private:
+  unsigned nblk_col_, len_col_, nblk_row_, len_row_;
+  dfm_bmat* h_ = nullptr;
 };
 
 // Abstract preconditioner of the matrix layer (matprecond_blk.h:37-41).
