@@ -60,6 +60,13 @@ int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper);
 // Value versions are unique across all matrices, so a factor's mirrored copy
 // of A (tagged with the version it copied) is refreshed whenever it is given
 // a different matrix or changed values.
+thread_local int dfm::g_sync_defer = 0;
+int dfm::sync_stream() {
+  if (g_sync_defer) return DFM_OK;
+  DFM_HIP(hipStreamSynchronize(stream()));
+  return DFM_OK;
+}
+
 unsigned long long dfm::next_version() {
   static std::atomic<unsigned long long> counter{1};
   return ++counter;
@@ -375,8 +382,7 @@ int dfm_bcrs_matvec(dfm_bcrs* A, double alpha, const double* x, double beta, dou
       DFM_LAUNCH(k_crs_matvec_gen, g, nb, (int)A->b, A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, alpha, x,
                  beta, y);
   }
-  DFM_HIP(hipStreamSynchronize(g_stream));
-  return DFM_OK;
+  return sync_stream();
 }
 
 // ---------------------------------------------------------------------------
@@ -410,27 +416,23 @@ int dfm_vec_axpy(size_t n, double alpha, const double* x, double* y) {
   DFM_TRY(ensure_init());
   if (alpha == 0.0) return DFM_OK;  // CLinearSystem::AXPY (linearsystem.cpp:312)
   DFM_LAUNCH(k_axpy, grid_vec((long)n), (long)n, alpha, x, y);
-  DFM_HIP(hipStreamSynchronize(g_stream));
-  return DFM_OK;
+  return sync_stream();
 }
 int dfm_vec_scal(size_t n, double alpha, double* x) {
   DFM_TRY(ensure_init());
   DFM_LAUNCH(k_scal, grid_vec((long)n), (long)n, alpha, x);
-  DFM_HIP(hipStreamSynchronize(g_stream));
-  return DFM_OK;
+  return sync_stream();
 }
 int dfm_vec_fill(size_t n, double value, double* x) {
   DFM_TRY(ensure_init());
   DFM_LAUNCH(k_fill<double>, grid_vec((long)n), (long)n, value, x);
-  DFM_HIP(hipStreamSynchronize(g_stream));
-  return DFM_OK;
+  return sync_stream();
 }
 int dfm_vec_copy(size_t n, const double* x, double* y) {
   DFM_TRY(ensure_init());
   if (x == y || n == 0) return DFM_OK;
   DFM_HIP(hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
-  DFM_HIP(hipStreamSynchronize(g_stream));
-  return DFM_OK;
+  return sync_stream();
 }
 int dfm_vec_zero_bc(size_t n, const int* flag, double* x) {
   DFM_TRY(ensure_init());

@@ -290,8 +290,7 @@ int dfm_bmat_matvec(dfm_bmat* M, double alpha, const double* x, double beta, dou
   if (ndof)
     DFM_BLAUNCH(k_bmat_matvec, bgrid(ndof), ndof, (int)M->b, (int)M->br, M->d_rowptr.p,
                 M->col.empty() ? nullptr : M->d_col.p, M->d_val.p, alpha, x, beta, y);
-  DFM_HIP(hipStreamSynchronize(stream()));
-  return DFM_OK;
+  return sync_stream();
 }
 int dfm_vec_copy_free(size_t n, const int* flag, const double* x, double* y) {
   DFM_CHECK(flag && x && y, DFM_BAD_ARG, "bad arguments to dfm_vec_copy_free");

@@ -1154,8 +1154,7 @@ int dfm_zbcrs_matvec(dfm_zbcrs* A, double alpha, const double* x, double beta, d
   const dfm_bcrs_s* P = A->pat;
   DFM_ZLAUNCH(k_zcrs_matvec, zgrid_for(P->nb), kNT, (int)P->nb, P->d_rowptr.p, P->d_col.p, A->d_val.p, A->d_dia.p,
               alpha, (const double2*)x, beta, (double2*)y);
-  DFM_HIP(hipStreamSynchronize(stream()));
-  return DFM_OK;
+  return sync_stream();
 }
 
 static int ztranspose(dfm_zbcrs* A) {
@@ -1185,8 +1184,7 @@ int dfm_zbcrs_matvec_hermitian(dfm_zbcrs* A, double alpha, const double* x, doub
   const dfm_bcrs_s* P = A->pat;
   DFM_ZLAUNCH(k_zcrs_matvec_h, zgrid_for(P->nb), kNT, (int)P->nb, A->t_ptr.p, A->t_row.p, A->t_src.p, A->d_val.p,
               A->d_dia.p, alpha, (const double2*)x, beta, (double2*)y);
-  DFM_HIP(hipStreamSynchronize(stream()));
-  return DFM_OK;
+  return sync_stream();
 }
 
 // CZVector_Blk
@@ -1473,19 +1471,34 @@ static int zcg_herm(dfm_zbcrs* A, dfm_zilu* Z, double* r, double* u, double* con
   }
   return DFM_NOT_CONVERGED;
 }
+// a host loop with deferred synchronisation, synchronised once at its end
+template <typename F>
+static int deferred(F&& body) {
+  int rc;
+  {
+    SyncDefer defer;
+    rc = body();
+  }
+  if (hipStreamSynchronize(stream()) != hipSuccess && rc >= 0) rc = DFM_HIP_ERROR;
+  return rc;
+}
 int dfm_zsolve_cg(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration) {
-  return zcg_herm(A, nullptr, r, u, conv_ratio, iteration);
+  return deferred([&] { return zcg_herm(A, nullptr, r, u, conv_ratio, iteration); });
 }
 int dfm_zsolve_pcg(dfm_zbcrs* A, dfm_zilu* Z, double* r, double* u, double* conv_ratio, unsigned* iteration) {
   DFM_CHECK(Z, DFM_BAD_ARG, "dfm_zsolve_pcg needs a factor");
-  return zcg_herm(A, Z, r, u, conv_ratio, iteration);
+  return deferred([&] { return zcg_herm(A, Z, r, u, conv_ratio, iteration); });
 }
 // MatVec::Sol::Solve_CGNR (zsolver_mat_iter.cpp:394-483): CG on the normal
 // equations with z = A^H r; alpha and beta are ratios of squared norms.  A
 // host-driven loop over the matrix-vector kernels (squared norms are device
 // reductions); the vector updates are AXPY on the interleaved doubles, which
 // is the reference's complex AXPY with a real factor, component by component.
+static int zcgnr(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration);
 int dfm_zsolve_cgnr(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration) {
+  return deferred([&] { return zcgnr(A, r, u, conv_ratio, iteration); });
+}
+static int zcgnr(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigned* iteration) {
   DFM_TRY(zsolve_checks(A, nullptr, r, u, conv_ratio, iteration));
   DFM_TRY(ensure_init());
   const size_t nb = A->pat->nb, n2 = 2 * nb;

@@ -14,6 +14,17 @@ namespace dfm {
 
 void set_error(const char* fmt, ...);
 hipStream_t stream();
+// Host-driven solver loops (the multi-segment and complex CGNR / CG ones)
+// defer the stream synchronisation that each entry point otherwise ends
+// with: every call enqueues on the one stream, reductions read back by the
+// host still wait for their own results, and the loop synchronises once at
+// its end.
+extern thread_local int g_sync_defer;
+struct SyncDefer {
+  SyncDefer() { ++g_sync_defer; }
+  ~SyncDefer() { --g_sync_defer; }
+};
+int sync_stream();  // hipStreamSynchronize unless deferred
 int ensure_init();
 // Value versions, unique across all matrices (real and complex).
 unsigned long long next_version();

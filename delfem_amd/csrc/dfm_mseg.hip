@@ -597,8 +597,19 @@ int dfm_mseg_ilu_precond(dfm_msilu* M, double* vec) {
       if (M->F[i][j]) DFM_TRY(dfm_bmat_matvec(M->F[i][j], -1.0, vec + S->off[j], 1.0, vec + S->off[i]));
     DFM_TRY(ilu_sweep64(M->dia[i], vec + S->off[i], false, true));
   }
-  DFM_HIP(hipStreamSynchronize(stream()));
-  return DFM_OK;
+  return sync_stream();
+}
+
+// a host loop with deferred synchronisation, synchronised once at its end
+template <typename F>
+static int deferred(F&& body) {
+  int rc;
+  {
+    SyncDefer defer;
+    rc = body();
+  }
+  if (hipStreamSynchronize(stream()) != hipSuccess && rc >= 0) rc = DFM_HIP_ERROR;
+  return rc;
 }
 
 // Work vectors start at zero, as the reference's: MATVEC(1, x, 0, y) scales
@@ -617,7 +628,7 @@ static int msilu_work(dfm_msilu_s* M, int nvec) {
 // LsSol::Solve_PCG (solver_ls_iter.cpp:100-172) on the segments: r in =
 // right-hand side, out = residual; x out.  DFM_OK when converged,
 // DFM_NOT_CONVERGED at max_iter (conv_ratio then the final ratio).
-int dfm_mseg_solve_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration) {
+static int ms_solve_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration) {
   DFM_CHECK(S && M && M->S == S && r && x && conv_ratio && iteration && r != x, DFM_BAD_ARG,
             "bad arguments to dfm_mseg_solve_pcg");
   const double tol = *conv_ratio;
@@ -667,11 +678,14 @@ int dfm_mseg_solve_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* 
   *conv_ratio = std::sqrt(rr * inv_rr0);
   return DFM_NOT_CONVERGED;
 }
+int dfm_mseg_solve_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration) {
+  return deferred([&] { return ms_solve_pcg(S, M, r, x, conv_ratio, iteration); });
+}
 
 // LsSol::Solve_PBiCGSTAB (solver_ls_iter.cpp:291-408) on the segments.
 // Returns DFM_OK when converged, DFM_NOT_CONVERGED at max_iter (the
 // reference returns true in both cases; iteration = max_iter then).
-int dfm_mseg_solve_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio,
+static int ms_solve_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio,
                              unsigned* iteration) {
   DFM_CHECK(S && M && M->S == S && r && x && conv_ratio && iteration && r != x, DFM_BAD_ARG,
             "bad arguments to dfm_mseg_solve_pbicgstab");
@@ -730,4 +744,8 @@ int dfm_mseg_solve_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, do
     DFM_TRY(ls_axpy(n, -beta * omega, AMp, p));
   }
   return DFM_NOT_CONVERGED;
+}
+int dfm_mseg_solve_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio,
+                             unsigned* iteration) {
+  return deferred([&] { return ms_solve_pbicgstab(S, M, r, x, conv_ratio, iteration); });
 }
