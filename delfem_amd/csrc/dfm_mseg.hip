@@ -29,6 +29,13 @@ struct dfm_mseg_s {
   std::vector<size_t> off;                          // dof offset of each segment (nseg + 1)
   std::vector<dfm_bcrs_s*> dia;                     // borrowed
   std::vector<std::vector<dfm_bmat_s*>> nd;         // borrowed, null where there is no block
+  // DOT: per-segment partial sums and results, read back with one copy
+  DevBuf<double> dot_partials;
+  DevBuf<KState> dot_state;
+  KState* dot_host = nullptr;
+  ~dfm_mseg_s() {
+    if (dot_host) (void)hipHostFree(dot_host);
+  }
 };
 
 struct dfm_msilu_s {
@@ -61,6 +68,8 @@ struct dfm_msilu_s {
 namespace {
 
 inline int mgrid(long n) { return (int)std::max<long>(1, (n + kNT - 1) / kNT); }
+// the grid dfm_vec_dot uses, so a segment's dot is the same sum
+inline int grid_vec(long n) { return (int)std::min<long>(std::max<long>(1, (n + kNT - 1) / kNT), 2048); }
 
 // CalcSubMatPr (matfrac_blkcrs.cpp:43-54): out -= a b, one product at a time
 __device__ inline void msub(double* out, const double* a, const double* c, int ni, int nk, int nj) {
@@ -339,13 +348,33 @@ void lp_dia_fill(LevPat& D, int K, RowList& rl) {
   D = std::move(P);
 }
 
+// CLinearSystem::DOT: each segment's dot product (the same two-phase
+// reduction as dfm_vec_dot), read back together, added in segment order
 int seg_dot(dfm_mseg_s* S, const double* x, const double* y, double* out) {
-  double s = 0.0;
-  for (unsigned i = 0; i < S->nseg; i++) {
-    double d = 0.0;
-    DFM_TRY(dfm_vec_dot(S->off[i + 1] - S->off[i], x + S->off[i], y + S->off[i], &d));
-    s += d;
+  const unsigned n = S->nseg;
+  std::vector<int> g(n);
+  int gmax = 1;
+  for (unsigned i = 0; i < n; i++) {
+    g[i] = grid_vec((long)(S->off[i + 1] - S->off[i]));
+    gmax = std::max(gmax, g[i]);
   }
+  if (S->dot_partials.n < (size_t)gmax * n) DFM_TRY(S->dot_partials.alloc((size_t)gmax * n));
+  if (S->dot_state.n < n) {
+    DFM_TRY(S->dot_state.alloc(n));
+    DFM_HIP(hipMemsetAsync(S->dot_state.p, 0, sizeof(KState) * n, stream()));
+  }
+  if (!S->dot_host) DFM_HIP(hipHostMalloc((void**)&S->dot_host, sizeof(KState) * 16, hipHostMallocDefault));
+  for (unsigned i = 0; i < n; i++) {
+    double* part = S->dot_partials.p + (size_t)i * gmax;
+    DFM_MLAUNCH(k_dot<double>, g[i], (long)(S->off[i + 1] - S->off[i]), x + S->off[i], y + S->off[i], part);
+    hipLaunchKernelGGL((k_finalize<1, FIN_STORE>), dim3(1), dim3(kFinNT), 0, stream(), part, g[i],
+                       S->dot_state.p + i);
+    DFM_HIP(hipGetLastError());
+  }
+  DFM_HIP(hipMemcpyAsync(S->dot_host, S->dot_state.p, sizeof(KState) * n, hipMemcpyDeviceToHost, stream()));
+  DFM_HIP(hipStreamSynchronize(stream()));
+  double s = 0.0;
+  for (unsigned i = 0; i < n; i++) s += S->dot_host[i].d0;
   *out = s;
   return DFM_OK;
 }
