@@ -51,3 +51,72 @@ def test_problems():
     assert np.all(p.lam > 0) and np.all(p.mu > 0)
     q = meshgen.poisson2d(8, seed=1)
     assert q.flags.sum() == 9 and q.alpha.size == 128
+
+
+def test_relabel_keeps_element_matrices():
+    """meshgen.relabel renumbers nodes only: every element matrix and residual
+    is unchanged (the element data and order are kept), the nodal arrays
+    (raveled or not) follow the permutation."""
+    from oracle import oracle
+    orc = oracle.Orc()
+    for prob in (meshgen.solid3d(3, seed=2, sigma=0.5), meshgen.oseen2d(6, seed=2),
+                 meshgen.zhelmholtz2d(6, 4, 1.0, 0.7, wave_length=0.5)):
+        order = np.random.default_rng(1).permutation(prob.nnode)
+        q = meshgen.relabel(prob, order)
+        assert np.array_equal(q.coord, prob.coord[order])
+        assert np.array_equal(np.asarray(q.flags).reshape(prob.nnode, -1),
+                              np.asarray(prob.flags).reshape(prob.nnode, -1)[order])
+        inv = np.empty_like(order)
+        inv[order] = np.arange(order.size)
+        assert np.array_equal(q.conn, inv[prob.conn])
+        if prob.kind != "zhelmholtz2d":
+            e1, r1 = oracle.problem_emats(orc, prob)
+            e2, r2 = oracle.problem_emats(orc, q)
+            assert np.array_equal(e1, e2) and np.array_equal(r1, r2)
+        else:
+            assert np.array_equal(q.lconn, inv[prob.lconn])
+
+
+def test_tile_colour_order():
+    """tile_colour_order is a permutation; each tile's nodes are numbered
+    consecutively; and with tiles of one colour never adjacent, the natural
+    ILU(0) sweep of the renumbered mesh has at most (colours x tile nodes)
+    level sets (8 colours bound the greedy colouring of these tile grids)."""
+    for prob, tn in ((meshgen.oseen2d(32, seed=1), 16), (meshgen.solid3d(8, seed=1), 8)):
+        order = meshgen.tile_colour_order(prob, tn)
+        assert np.array_equal(np.sort(order), np.arange(prob.nnode))
+        # the tiles, recomputed as the numbering defines them
+        coord = np.asarray(prob.coord)
+        conn = np.asarray(prob.conn)
+        h = float(np.median(np.linalg.norm(coord[conn[:, 1]] - coord[conn[:, 0]], axis=1)))
+        side = h * max(1.0, round(tn ** (1.0 / coord.shape[1])))
+        cell = np.floor((coord - coord.min(axis=0)) / side + 1e-9).astype(np.int64)
+        tid = np.unique(np.ravel_multi_index(tuple(cell.T), tuple(cell.max(axis=0) + 1)), return_inverse=True)[1]
+        pos = np.empty_like(order)
+        pos[order] = np.arange(order.size)
+        # nodes of one tile are numbered consecutively
+        for t in np.unique(tid)[:50]:
+            p = np.sort(pos[tid == t])
+            assert p[-1] - p[0] == p.size - 1
+        rp, cl = fem_pattern_numpy(prob.nnode, inv_conn(conn, order))
+        lv = np.zeros(prob.nnode, dtype=np.int64)
+        for r in range(prob.nnode):
+            c = cl[rp[r]:rp[r + 1]]
+            c = c[c < r]
+            lv[r] = lv[c].max() + 1 if c.size else 0
+        assert lv.max() + 1 <= 8 * tn
+
+
+def inv_conn(conn, order):
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    return inv[conn]
+
+
+def fem_pattern_numpy(nnode, conn):
+    """Node adjacency (without the node itself) as CRS, ascending."""
+    npe = conn.shape[1]
+    a = np.concatenate([np.stack([conn[:, i], conn[:, j]], axis=1) for i in range(npe) for j in range(npe) if i != j])
+    a = np.unique(a, axis=0)
+    rp = np.searchsorted(a[:, 0], np.arange(nnode + 1))
+    return rp, a[:, 1]
