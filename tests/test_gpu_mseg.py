@@ -93,23 +93,25 @@ def test_segment_solvers(orc, gpu, name):
         assert np.linalg.norm(xh - x_o) <= 1e-6 * np.linalg.norm(x_o)
 
 
-def test_multicolour_numbering(orc, gpu):
-    """Nodes renumbered by multicolouring: the relabelled problem's blocks and
-    cross-segment factor equal the oracle's on the same relabelled problem,
-    and the PCG solution mapped back equals the natural-numbering one within
-    1e-8."""
+@pytest.mark.parametrize("ordering,lev", [("multicolour", 0), ("tile_multicolour", 1)])
+def test_multicolour_numbering(orc, gpu, ordering, lev):
+    """Nodes renumbered (multicolouring, or coloured tiles with ILU(1)): the
+    relabelled problem's blocks and cross-segment factor equal the oracle's
+    on the same relabelled problem, and the PCG solution mapped back equals
+    the natural-numbering one within 1e-8."""
     import torch
     from delfem_amd import meshgen
     prob = meshgen.solid3d(4, seed=7, sigma=0.5)
+    print(f"[case] solid3d_4 {ordering} ILU({lev})", flush=True)
     groups = [[0, 1], [2]]
-    S = gpu.MultiSegmentSystem(prob, groups, ordering="multicolour")
+    S = gpu.MultiSegmentSystem(prob, groups, ordering=ordering)
     S.assemble()
     order = S.node_order
     sp = split_problem(orc, meshgen.relabel(prob, order), groups)
     O = OrcMS(orc, sp, S.prob.conn)
     assert np.array_equal(S.res.cpu().numpy(), O.res)
-    M = S.preconditioner()
-    assert M.SetValue() and O.ilu()
+    M = S.preconditioner(lev)
+    assert M.SetValue() and O.ilu(lev)
     for i in range(2):
         for j in range(2):
             got = M.values(i, j)
@@ -141,7 +143,7 @@ def test_multicolour_numbering(orc, gpu):
         out[order] = blk
         back[off:off + nb * len(g)] = out.reshape(-1)
         off += nb * len(g)
-    print(f"  multicolour {it} iterations, natural {itn}", flush=True)
+    print(f"  {ordering} {it} iterations, natural {itn}", flush=True)
     assert np.linalg.norm(back - xn) <= 1e-8 * np.linalg.norm(xn)
 
 
