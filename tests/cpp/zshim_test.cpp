@@ -9,7 +9,7 @@
 //         u32 line[nline*2] | c128 emat_tri[ntri*9] | c128 emat_line[nline*4] |
 //         c128 rhs[nnode] | i32 flags[nnode]
 //         kind: 0 Solve_COCG, 1 Solve_PCOCG, 2 Solve_BiCGSTAB, 3 Solve_BiCGStabP,
-//               4 Solve_CGNR
+//               4 Solve_CGNR, 5 Solve_CG and 6 Solve_PCG (Hermitian matrices)
 // output: f64 conv | i32 ok iteration | c128 val[ncrs] dia[nnode] u[nnode] r[nnode]
 #include <complex>
 #include <cstdio>
@@ -98,6 +98,8 @@ int main(int argc, char** argv) {
       case 2: ok = MatVec::Sol::Solve_BiCGSTAB(conv, maxit, A, r, u); break;
       case 3: ok = MatVec::Sol::Solve_BiCGStabP(conv, maxit, A, r, u, prec, A); break;
       case 4: ok = MatVec::Sol::Solve_CGNR(conv, maxit, A, r, u); break;
+      case 5: ok = MatVec::Sol::Solve_CG(conv, maxit, A, r, u); break;
+      case 6: ok = MatVec::Sol::Solve_PCG(conv, maxit, A, r, u, prec, A); break;
       default: return 2;
     }
     const std::vector<cplx> uh = u.GetValues(), rh = r.GetValues();

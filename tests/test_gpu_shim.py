@@ -257,3 +257,46 @@ def test_cpp_multisegment_dropin(orc, gpu, tmp_path, name, lev):
     else:
         assert abs(int(it) - want[2]) <= max(2, int(np.ceil(0.25 * want[2])))
         assert np.linalg.norm(u - want[3]) <= 1e-6 * np.linalg.norm(want[3])
+
+
+@pytest.mark.parametrize("kind", [5, 6])
+def test_cpp_complex_hermitian_cg(orc, gpu, tmp_path, kind):
+    """The complex classes' Solve_CG / Solve_PCG (ILU(1)) overloads on a
+    Hermitian positive definite system assembled by per-element Mearge:
+    matrix bit-exact, iterations within 5 % of the oracle's and solution
+    within 1e-8."""
+    from oracle.oracle import hermitian_zproblem
+    if not os.path.exists(ZBIN):
+        pytest.fail("tests/cpp/zshim_test is not built (run __graft_entry__.build())")
+    prob = meshgen.zhelmholtz2d(30, 20, 1.5, 1.0, wave_length=1000.0)
+    H = hermitian_zproblem(orc, prob, seed=2)
+    n, lev, maxit, tol = prob.nnode, 1, 3000, 1e-10
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([n, prob.conn.shape[0], 0, kind, maxit, lev], dtype=np.int32).tofile(f)
+        np.array([tol]).tofile(f)
+        np.ascontiguousarray(prob.conn, dtype=np.uint32).tofile(f)
+        np.ascontiguousarray(H["emat_tri"]).tofile(f)
+        np.ascontiguousarray(H["res"]).tofile(f)
+        np.ascontiguousarray(prob.flags, dtype=np.int32).tofile(f)
+    run = subprocess.run([ZBIN, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    print(run.stdout, run.stderr, flush=True)
+    assert run.returncode == 0
+    raw = open(out, "rb").read()
+    conv = float(np.frombuffer(raw[:8], dtype=np.float64)[0])
+    ok, it = np.frombuffer(raw[8:16], dtype=np.int32)
+    rest = np.frombuffer(raw[16:], dtype=np.complex128)
+    nv = H["val"].size
+    val, dia, u = rest[:nv], rest[nv:nv + n], rest[nv + n:nv + 2 * n]
+    assert np.array_equal(val, H["val"]) and np.array_equal(dia, H["dia"])
+    args = (n, H["rowptr"], H["col"], H["val"], H["dia"], H["res"], tol, maxit)
+    if kind == 5:
+        want = orc.zsolve_cg_herm(*args)
+    else:
+        F = orc.iluk_symbolic(H["rowptr"], H["col"], lev)
+        fv, fd, _ = orc.zilu_values(n, H["rowptr"], H["col"], H["val"], H["dia"], F)
+        want = orc.zsolve_cg_herm(*args, F, fv, fd)
+    print(f"  C++ {it} iterations, oracle {want[2]}", flush=True)
+    assert ok == 1 and want[0] and conv < tol
+    assert abs(int(it) - want[2]) <= max(2, int(np.ceil(0.05 * want[2])))
+    assert np.linalg.norm(u - want[3]) <= 1e-8 * np.linalg.norm(want[3])
