@@ -696,6 +696,8 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
     DFM_TRY(M->d_order.upload(M->order));
     DFM_TRY(M->d_inv.upload(M->inv));
     DFM_TRY(M->d_nsing.alloc(1));
+    DFM_TRY(M->d_singrow.alloc(std::max<size_t>(1, nb)));
+    DFM_TRY(M->d_fixrow.alloc(1));
     DFM_TRY(upload_sell_plan(M->planL, (int)bb, M->L64, M->d_srcL));
     DFM_TRY(upload_sell_plan(M->planU, (int)bb, M->U64, M->d_srcU));
     DFM_TRY(upload_sell_plan(M->planA, (int)bb, M->A64, M->d_srcA));
@@ -743,12 +745,16 @@ int dfm_ilu_get_pattern(const dfm_ilu* M, unsigned* rowptr, unsigned* col, int* 
 }  // extern "C"
 
 // SetValue_Initialize: A's values into the factor pattern (fill zero)
-int dfm::ilu_numeric_copy(dfm_ilu_s* M, const dfm_bcrs_s* A) {
+int dfm::ilu_numeric_copy(dfm_ilu_s* M, const dfm_bcrs_s* A) { return ilu_numeric_copy_rows(M, A, 0); }
+
+int dfm::ilu_numeric_copy_rows(dfm_ilu_s* M, const dfm_bcrs_s* A, int row0) {
   const int nb = (int)M->nb, b = (int)M->b, bb = b * b;
-  const long nf = (long)M->fcol.size();
-  DFM_LAUNCH(k_ilu_copy, grid_for(nf * bb), nf, bb, M->d_fsrc.p, A->d_val.p, M->d_fval.p);
-  DFM_LAUNCH(k_ilu_copy_dia, grid_for((long)nb * bb), nb, bb, M->permuted ? M->d_order.p : nullptr, A->d_dia.p,
-             M->d_fdia.p);
+  if (row0 >= nb) return DFM_OK;
+  const long q0 = (long)M->frow[row0];
+  const long nf = (long)M->fcol.size() - q0;
+  DFM_LAUNCH(k_ilu_copy, grid_for(nf * bb), nf, bb, M->d_fsrc.p + q0, A->d_val.p, M->d_fval.p + q0 * bb);
+  DFM_LAUNCH(k_ilu_copy_dia, grid_for((long)(nb - row0) * bb), nb, row0, bb, M->permuted ? M->d_order.p : nullptr,
+             A->d_dia.p, M->d_fdia.p);
   return DFM_OK;
 }
 
@@ -764,30 +770,44 @@ int dfm_ilu_set_value(dfm_ilu* M, dfm_bcrs* A, int* nsingular) {
 
 }  // extern "C"
 
-// DoILUDecomp on the values in the factor, then the sliced-ELL copies
-int dfm::ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular) {
+namespace {
+// one launch of the factor kernel over nrow rows listed at rows_dev
+int ilu_factor_launch(dfm_ilu_s* M, int nrow, const int* rows_dev, int partial) {
   const int b = (int)M->b;
+  const int g = grid_for(nrow);
+  switch (b) {
+    case 1:
+      DFM_LAUNCH(k_ilu_factor<1>, g, nrow, rows_dev, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p, M->d_fval.p,
+                 M->d_fdia.p, M->d_nsing.p, M->d_singrow.p, partial);
+      break;
+    case 2:
+      DFM_LAUNCH(k_ilu_factor<2>, g, nrow, rows_dev, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p, M->d_fval.p,
+                 M->d_fdia.p, M->d_nsing.p, M->d_singrow.p, partial);
+      break;
+    case 3:
+      DFM_LAUNCH(k_ilu_factor<3>, g, nrow, rows_dev, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p, M->d_fval.p,
+                 M->d_fdia.p, M->d_nsing.p, M->d_singrow.p, partial);
+      break;
+    default:
+      DFM_LAUNCH(k_ilu_factor_gen, g, nrow, b, rows_dev, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p, M->d_fval.p,
+                 M->d_fdia.p, M->d_nsing.p, M->d_singrow.p, partial);
+  }
+  return DFM_OK;
+}
+}  // namespace
+
+// DoILUDecomp on the values in the factor, then the sliced-ELL copies.  The
+// levels run in parallel, so every row is factored and the singular pivots
+// counted; a row depends only on rows before it, so when there are more than
+// 10 the rows before the 11th singular one (in factor numbering) already hold
+// what the reference's row-by-row loop leaves there, and the rest is put
+// back: later rows restored to their copied values, the stopping row redone
+// without its U scaling.
+int dfm::ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular, ilu_restore_fn restore, void* restore_ctx) {
   DFM_HIP(hipMemsetAsync(M->d_nsing.p, 0, sizeof(int), g_stream));
   for (int l = 0; l < M->nlev_lower; l++) {
     const int off = M->lev_ptr_L[l], nrow = M->lev_ptr_L[l + 1] - off;
-    const int g = grid_for(nrow);
-    switch (b) {
-      case 1:
-        DFM_LAUNCH(k_ilu_factor<1>, g, nrow, M->d_rows_L.p + off, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p,
-                   M->d_fval.p, M->d_fdia.p, M->d_nsing.p);
-        break;
-      case 2:
-        DFM_LAUNCH(k_ilu_factor<2>, g, nrow, M->d_rows_L.p + off, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p,
-                   M->d_fval.p, M->d_fdia.p, M->d_nsing.p);
-        break;
-      case 3:
-        DFM_LAUNCH(k_ilu_factor<3>, g, nrow, M->d_rows_L.p + off, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p,
-                   M->d_fval.p, M->d_fdia.p, M->d_nsing.p);
-        break;
-      default:
-        DFM_LAUNCH(k_ilu_factor_gen, g, nrow, b, M->d_rows_L.p + off, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p,
-                   M->d_fval.p, M->d_fdia.p, M->d_nsing.p);
-    }
+    DFM_TRY(ilu_factor_launch(M, nrow, M->d_rows_L.p + off, 0));
   }
   DFM_TRY(ilu_refresh_sell(M, A, M->have32, M->have16));
   int ns = 0;
@@ -796,7 +816,19 @@ int dfm::ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular) {
   if (nsingular) *nsingular = ns;
   M->factored = true;
   if (ns > 10) {
-    set_error("ILU factorisation met %d singular pivots (more than 10)", ns);
+    std::vector<int> sing(M->nb);
+    DFM_TRY(M->d_singrow.download(sing.data(), M->nb));
+    int stop = -1, cnt = 0;
+    for (unsigned i = 0; i < M->nb && stop < 0; i++)
+      if (sing[i] && ++cnt == 11) stop = (int)i;
+    DFM_CHECK(stop >= 0, DFM_HIP_ERROR, "singular pivot count and row flags disagree");
+    if (restore) DFM_TRY(restore(restore_ctx, stop));
+    else DFM_TRY(ilu_numeric_copy_rows(M, A, stop));
+    DFM_TRY(M->d_fixrow.upload(&stop, 1));
+    DFM_TRY(ilu_factor_launch(M, 1, M->d_fixrow.p, 1));
+    DFM_TRY(ilu_refresh_sell(M, A, M->have32, M->have16));
+    DFM_HIP(hipStreamSynchronize(g_stream));
+    set_error("ILU factorisation met more than 10 singular pivots (stopped in factor row %d)", stop);
     return DFM_SINGULAR;
   }
   return DFM_OK;

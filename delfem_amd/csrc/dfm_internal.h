@@ -158,7 +158,16 @@ int ilu_create_impl(const dfm_bcrs_s* A, int lev_fill, int ordering, const int* 
 // Numeric ILU in two steps (dfm_ilu_set_value = copy + factor): between
 // them a caller may change the copied values (cross-segment updates).
 int ilu_numeric_copy(dfm_ilu_s* M, const dfm_bcrs_s* A);
-int ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular);
+// the copy of factor rows [row0, nb) only
+int ilu_numeric_copy_rows(dfm_ilu_s* M, const dfm_bcrs_s* A, int row0);
+// DoILUDecomp.  With more than 10 singular pivots the factor is left as the
+// reference leaves it: rows before the 11th singular pivot (in factor
+// numbering) factored, that row updated but not scaled, the rows after it as
+// they were before the factorisation -- restore(row0) must put rows
+// [row0, nb) back in that state (NULL: ilu_numeric_copy_rows from A).
+typedef int (*ilu_restore_fn)(void* ctx, int row0);
+int ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular, ilu_restore_fn restore = nullptr,
+                       void* restore_ctx = nullptr);
 int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper);
 
 }  // namespace dfm
@@ -205,6 +214,8 @@ struct dfm_ilu_s {
   dfm::DevBuf<int> d_rows_L;            // rows sorted by lower level
   std::vector<int> lev_ptr_L;
   dfm::DevBuf<int> d_nsing;
+  dfm::DevBuf<int> d_singrow;           // per factor row: the pivot was singular
+  dfm::DevBuf<int> d_fixrow;            // the row DoILUDecomp stopped in (one entry)
   // triangular-solve layouts (value copies refreshed after each factorisation)
   dfm::SellPlan planL, planU;
   bool upper0_empty = false;            // first backward level has no U entries (true for level sets)

@@ -762,23 +762,29 @@ static __global__ void k_ilu_copy(long nf, int bb, const long long* __restrict__
   const long long s = fsrc[q];
   fval[t] = s >= 0 ? aval[(size_t)s * bb + e] : 0.0;
 }
-static __global__ void k_ilu_copy_dia(int nb, int bb, const int* __restrict__ order, const double* __restrict__ adia,
-                               double* __restrict__ fdia) {
+// rows [row0, nb) of the factor's diagonal
+static __global__ void k_ilu_copy_dia(int nb, int row0, int bb, const int* __restrict__ order,
+                                      const double* __restrict__ adia, double* __restrict__ fdia) {
   const long t = (long)blockIdx.x * kNT + threadIdx.x;
-  if (t >= (long)nb * bb) return;
-  const long i = t / bb;
-  const int e = (int)(t - i * bb);
+  if (t >= (long)(nb - row0) * bb) return;
+  const long i = row0 + t / bb;
+  const int e = (int)(t - (i - row0) * bb);
   const long io = order ? order[i] : i;
   fdia[t] = adia[(size_t)io * bb + e];
 }
 
 // One thread per row of a level of the lower DAG: IKJ update with the
 // finished rows k in L(i), then invert the pivot block and scale the U row.
+// singrow[i] records whether row i's pivot was singular.  partial != 0
+// reproduces the state DoILUDecomp leaves in the row where it gives up (the
+// 11th singular pivot, matdiafrac_blkcrs.cpp:880-890): the row is updated,
+// its pivot left as the inversion attempt left it, its U part not scaled.
 template <int B>
 static __global__ void __launch_bounds__(kNT) k_ilu_factor(int nrow, const int* __restrict__ rows,
                                                    const int* __restrict__ frow, const int* __restrict__ fcol,
                                                    const int* __restrict__ fdiaind, double* __restrict__ fval,
-                                                   double* __restrict__ fdia, int* __restrict__ nsing) {
+                                                   double* __restrict__ fdia, int* __restrict__ nsing,
+                                                   int* __restrict__ singrow, int partial) {
   const int t = blockIdx.x * kNT + threadIdx.x;
   if (t >= nrow) return;
   constexpr int BB = B * B;
@@ -862,6 +868,8 @@ static __global__ void __launch_bounds__(kNT) k_ilu_factor(int nrow, const int* 
     }
   }
   if (singular) atomicAdd(nsing, 1);
+  singrow[i] = singular ? 1 : 0;
+  if (partial && singular) return;  // the pivot block is unchanged when singular (b <= 3)
 #pragma unroll
   for (int s = 0; s < BB; s++) D[s] = d[s];
   for (int q = rd; q < r1; q++) {
@@ -890,7 +898,8 @@ static __global__ void __launch_bounds__(kNT) k_ilu_factor(int nrow, const int* 
 static __global__ void __launch_bounds__(kNT) k_ilu_factor_gen(int nrow, int b, const int* __restrict__ rows,
                                                        const int* __restrict__ frow, const int* __restrict__ fcol,
                                                        const int* __restrict__ fdiaind, double* __restrict__ fval,
-                                                       double* __restrict__ fdia, int* __restrict__ nsing) {
+                                                       double* __restrict__ fdia, int* __restrict__ nsing,
+                                                       int* __restrict__ singrow, int partial) {
   const int t = blockIdx.x * kNT + threadIdx.x;
   if (t >= nrow) return;
   const int bb = b * b;
@@ -934,6 +943,9 @@ static __global__ void __launch_bounds__(kNT) k_ilu_factor_gen(int nrow, int b, 
     }
   }
   if (singular) atomicAdd(nsing, 1);
+  singrow[i] = singular ? 1 : 0;
+  // CalcInvMat leaves the pivot block half-inverted where it stops (in place, as here)
+  if (partial && singular) return;
   double tmp[64];
   for (int q = rd; q < r1; q++) {
     double* U = fval + (size_t)q * bb;

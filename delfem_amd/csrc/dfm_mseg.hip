@@ -145,13 +145,13 @@ __global__ void __launch_bounds__(kNT) k_ms_lowup(int nrow, const int* __restric
 }
 // CMatDiaFrac_BlkCrs::DoILUDecompLowUp (matdiafrac_blkcrs.cpp:665-767): the
 // copied diagonal block of segment i -= L_ik U_ki before its factorisation
-__global__ void __launch_bounds__(kNT) k_ms_lowup_dia(int nrow, const int* __restrict__ frow,
+__global__ void __launch_bounds__(kNT) k_ms_lowup_dia(int nrow, int row0, const int* __restrict__ frow,
                                                      const int* __restrict__ fcol, double* __restrict__ fval,
                                                      double* __restrict__ fdia, const int* __restrict__ lrp,
                                                      const int* __restrict__ lcl, const double* __restrict__ Lv,
                                                      const int* __restrict__ urp, const int* __restrict__ ucl,
                                                      const double* __restrict__ Uv, int bi, int bk) {
-  const int r = blockIdx.x * kNT + threadIdx.x;
+  const int r = row0 + (int)(blockIdx.x * kNT + threadIdx.x);
   if (r >= nrow) return;
   const int bii = bi * bi;
   for (int ik = lrp[r]; ik < lrp[r + 1]; ik++) {
@@ -516,6 +516,36 @@ int dfm_mseg_ilu_destroy(dfm_msilu* M) {
   delete M;
   return DFM_OK;
 }
+namespace {
+struct DiaPrep {
+  dfm_msilu* M;
+  unsigned i;
+};
+// DoILUDecompLowUp of segment i's diagonal factor with each pair of
+// off-diagonal factors (i, k), (k, i), k < i, on rows [row0, nb_i)
+int dia_lowup(void* ctx, int row0) {
+  const DiaPrep* P = static_cast<const DiaPrep*>(ctx);
+  dfm_msilu* M = P->M;
+  dfm_mseg_s* S = M->S;
+  const unsigned i = P->i;
+  dfm_ilu_s* D = M->dia[i];
+  for (unsigned k = 0; k < i; k++) {
+    const dfm_bmat_s *lo = M->F[i][k], *up = M->F[k][i];
+    if (!lo || !up) continue;
+    DFM_MLAUNCH(k_ms_lowup_dia, mgrid((long)S->nb[i] - row0), (int)S->nb[i], row0, D->d_frow.p, D->d_fcol.p,
+                D->d_fval.p, D->d_fdia.p, lo->d_rowptr.p, lo->col.empty() ? nullptr : lo->d_col.p, lo->d_val.p,
+                up->d_rowptr.p, up->col.empty() ? nullptr : up->d_col.p, up->d_val.p, (int)S->b[i], (int)S->b[k]);
+  }
+  return DFM_OK;
+}
+// the state of rows [row0, nb_i) before DoILUDecomp: copied values, then the LowUp updates
+int dia_restore(void* ctx, int row0) {
+  const DiaPrep* P = static_cast<const DiaPrep*>(ctx);
+  DFM_TRY(ilu_numeric_copy_rows(P->M->dia[P->i], P->M->S->dia[P->i], row0));
+  return dia_lowup(ctx, row0);
+}
+}  // namespace
+
 int dfm_mseg_ilu_set_value(dfm_msilu* M, int* nsingular) {
   DFM_CHECK(M, DFM_BAD_ARG, "NULL factor");
   dfm_mseg_s* S = M->S;
@@ -538,17 +568,10 @@ int dfm_mseg_ilu_set_value(dfm_msilu* M, int* nsingular) {
   for (unsigned i = 0; i < n; i++)
     for (unsigned j = 0; j < n; j++) {
       if (i == j) {
-        dfm_ilu_s* D = M->dia[i];
-        for (unsigned k = 0; k < i; k++) {
-          const dfm_bmat_s *lo = M->F[i][k], *up = M->F[k][i];
-          if (!lo || !up) continue;
-          DFM_MLAUNCH(k_ms_lowup_dia, mgrid(S->nb[i]), (int)S->nb[i], D->d_frow.p, D->d_fcol.p, D->d_fval.p,
-                      D->d_fdia.p, lo->d_rowptr.p, lo->col.empty() ? nullptr : lo->d_col.p, lo->d_val.p,
-                      up->d_rowptr.p, up->col.empty() ? nullptr : up->d_col.p, up->d_val.p, (int)S->b[i],
-                      (int)S->b[k]);
-        }
+        DiaPrep prep{M, i};
+        DFM_TRY(dia_lowup(&prep, 0));
         int ns = 0;
-        const int rc = ilu_numeric_factor(D, S->dia[i], &ns);
+        const int rc = ilu_numeric_factor(M->dia[i], S->dia[i], &ns, dia_restore, &prep);
         nsing_total += ns;
         if (rc != DFM_OK) {
           if (nsingular) *nsingular = nsing_total;

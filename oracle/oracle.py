@@ -1097,6 +1097,19 @@ def problem_emats(orc, prob, value=None):
     return orc.emat_solid3d(prob.coord, prob.conn, prob.lam, prob.mu, prob.rho, prob.g, v)
 
 
+def zero_block_rows(rowptr, col, val, dia, b, rows):
+    """Copies of (val, dia) with whole block rows set to zero (off-diagonal
+    blocks and diagonal block): the pivot of such a row is exactly zero after
+    elimination, a singular pivot for DoILUDecomp whatever the ordering."""
+    val, dia = f64(val).copy(), f64(dia).copy()
+    bb = b * b
+    for r in rows:
+        r = int(r)
+        val[int(rowptr[r]) * bb:int(rowptr[r + 1]) * bb] = 0.0
+        dia[r * bb:(r + 1) * bb] = 0.0
+    return val, dia
+
+
 def assemble_problem(orc, prob):
     """Oracle pipeline: pattern, element loop, Mearge, BC, residual (CPU)."""
     rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from delfem_amd import meshgen
-from oracle.oracle import RefFrac, RefILU, RefSystem, assemble_problem
+from oracle.oracle import RefFrac, RefILU, RefSystem, assemble_problem, zero_block_rows
 
 
 def test_poisson_128_pcg_trajectory(orc, ref):
@@ -262,3 +262,30 @@ def test_multisegment_fill_vs_reference(orc, ref, name, lev):
     a, b = R.solve(bicg, O.res, 1e-10, 3000), O.solve(bicg, O.res, 1e-10, 3000)
     assert a[:3] == b[:3] and a[0]
     assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+
+
+@pytest.mark.parametrize("b,nzero", [(1, 10), (1, 11), (1, 14), (3, 10), (3, 11)])
+@pytest.mark.parametrize("ordered", [False, True])
+def test_singular_pivot_stop_state(orc, ref, b, nzero, ordered):
+    """DoILUDecomp (matdiafrac_blkcrs.cpp:774-775, 880-890, 968-977): up to 10
+    singular pivots are left un-inverted and the factorisation goes on; at the
+    11th, in factor row order, it returns false with that row updated but its U
+    part not scaled and every later row as SetValue_Initialize left it.  The
+    oracle leaves the same bits (natural and ordered factors)."""
+    prob = meshgen.poisson2d(10, seed=5) if b == 1 else meshgen.solid3d(3, seed=5, sigma=0.5)
+    A = assemble_problem(orc, prob)
+    nb = prob.nnode
+    rng = np.random.default_rng(nzero)
+    order = rng.permutation(nb).astype(np.int32) if ordered else None
+    pos = np.linspace(3, nb - 4, nzero).astype(int)             # rows in factor numbering
+    rows = order[pos] if ordered else pos
+    val, dia = zero_block_rows(A["rowptr"], A["col"], A["val"], A["dia"], b, rows)
+    rs = RefSystem(ref, nb, b, A["rowptr"], A["col"])
+    rs.set_matrix(val, dia)
+    rf = RefFrac(rs, 0, order)
+    ok_r = rf.set_value()
+    fv_r, fd_r = rf.values()
+    F = orc.iluk_symbolic(A["rowptr"], A["col"], 0, order)
+    fv, fd, ns = orc.ilu_values(nb, b, A["rowptr"], A["col"], val, dia, F, order)
+    assert ok_r == (nzero <= 10) and (ns == nzero if nzero <= 10 else ns == -1)
+    assert np.array_equal(fv, fv_r) and np.array_equal(fd, fd_r)
