@@ -156,8 +156,8 @@ def main():
         ok_f = M.SetValue(s.A)
         r = rhs.clone()
         x = torch.zeros_like(r)
-        ok, conv, it, extra = solve(tol, maxit, s.A, M, r, x, precision=precision, want_true_relres=1)
-        return ok and ok_f, conv, it, extra, x
+        _, conv, it, extra = solve(tol, maxit, s.A, M, r, x, precision=precision, want_true_relres=1)
+        return extra["converged"] and ok_f, conv, it, extra, x
 
     def timed(precision, steps, warmup, maxit=50000):
         for _ in range(warmup):

@@ -100,6 +100,7 @@ _SIGS = {
     "dfm_assemble_oseen2d_p1p1": (C.c_int, [H, vp, C.c_double, C.c_double, vp, vp, vp]),
     "dfm_assemble_helmholtz2d_p1": (C.c_int, [H, vp, vp, vp, C.c_double, vp, vp]),
     "dfm_ilu_create": (C.c_int, [H, C.c_int, C.c_int, vp, C.c_uint, C.POINTER(H)]),
+    "dfm_ilu_create_fill": (C.c_int, [H, C.c_int, vp, C.c_uint, C.POINTER(H)]),
     "dfm_ilu_destroy": (C.c_int, [H]),
     "dfm_ilu_info": (C.c_int, [H, u32p, i32p, i32p, i32p]),
     "dfm_ilu_get_pattern": (C.c_int, [H, vp, vp, vp, vp, vp]),
@@ -117,6 +118,7 @@ _SIGS = {
     "dfm_bmat_get_pattern": (C.c_int, [H, vp, vp]),
     "dfm_bmat_set_zero": (C.c_int, [H]),
     "dfm_bmat_get_values": (C.c_int, [H, vp]),
+    "dfm_bmat_set_values": (C.c_int, [H, vp]),
     "dfm_bmat_mearge": (C.c_int, [H, C.c_uint, vp, C.c_uint, vp]),
     "dfm_bmat_copy_from": (C.c_int, [H, H]),
     "dfm_bmat_set_bc_inverse_col": (C.c_int, [H, vp]),

@@ -20,11 +20,31 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose=False):
+OBJ_DIR = os.path.join(ROOT, "_build")
+
+
+def build(verbose=False, jobs=None):
+    """Compiles every source to an object (in parallel, `jobs` at a time) and
+    links the shared library; always recompiles."""
+    from concurrent.futures import ThreadPoolExecutor
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+
+    def compile_one(src):
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc, *cflags, *inc, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        return obj
+
+    jobs = jobs or min(len(SOURCES), int(os.environ.get("MAX_JOBS", "0") or 0) or os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp]
+    cmd = [hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
@@ -32,7 +52,7 @@ def build(verbose=False):
     return LIB
 
 
-SHIM_PROGRAMS = ["shim_test", "zshim_test", "msshim_test"]  # tests/cpp/<name>.cpp -> tests/cpp/<name>
+SHIM_PROGRAMS = ["shim_test", "zshim_test", "msshim_test", "poisson_dropin"]  # tests/cpp/<name>.cpp -> tests/cpp/<name>
 
 
 def build_shim_test(verbose=False):

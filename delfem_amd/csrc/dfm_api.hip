@@ -550,7 +550,8 @@ namespace dfm {
 // pattern is that one instead of the ILU(k) symbolic of A's: it must hold
 // A's pattern, and its extra entries start at zero
 int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* order, unsigned block_nodes,
-                    const std::vector<unsigned>* given_row, const std::vector<unsigned>* given_col, dfm_ilu** out) {
+                    const std::vector<unsigned>* given_row, const std::vector<unsigned>* given_col, dfm_ilu** out,
+                    const std::vector<unsigned>* fill_blk) {
   DFM_CHECK(A && out && lev_fill >= -1, DFM_BAD_ARG, "bad arguments to dfm_ilu_create");
   DFM_CHECK(!given_row || (given_col && ordering == DFM_ORDER_NATURAL && given_row->size() == A->nb + 1 &&
                            given_col->size() == given_row->back()),
@@ -607,8 +608,10 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
       M->fdiaind[i] = (unsigned)(std::upper_bound(b0, b1, i) - M->fcol.begin());
     }
   } else {
+    DFM_CHECK(!fill_blk || fill_blk->empty() || !M->permuted, DFM_BAD_ARG,
+              "fill blocks are given for natural-order factors only");
     DFM_TRY(iluk_symbolic(nb, A->rowptr, A->col, lev_fill, M->permuted ? &M->order : nullptr, M->frow, M->fcol,
-                          M->flev, M->fdiaind));
+                          M->flev, M->fdiaind, fill_blk));
   }
   const size_t nf = M->fcol.size();
   // factor entry -> block of A
@@ -716,6 +719,12 @@ extern "C" {
 int dfm_ilu_create(const dfm_bcrs* A, int lev_fill, int ordering, const int* order, unsigned block_nodes,
                    dfm_ilu** out) {
   return dfm::ilu_create_impl(A, lev_fill, ordering, order, block_nodes, nullptr, nullptr, out);
+}
+
+int dfm_ilu_create_fill(const dfm_bcrs* A, int lev_fill, const unsigned* fill_blk, unsigned nfill, dfm_ilu** out) {
+  DFM_CHECK(nfill == 0 || fill_blk, DFM_BAD_ARG, "NULL fill block list");
+  const std::vector<unsigned> blk(fill_blk, fill_blk + nfill);
+  return dfm::ilu_create_impl(A, lev_fill, DFM_ORDER_NATURAL, nullptr, 0, nullptr, nullptr, out, &blk);
 }
 
 int dfm_ilu_destroy(dfm_ilu* M) {

@@ -227,6 +227,13 @@ int dfm_bmat_get_values(const dfm_bmat* M, double* val) {
   DFM_HIP(hipStreamSynchronize(stream()));
   return DFM_OK;
 }
+int dfm_bmat_set_values(dfm_bmat* M, const double* val) {
+  DFM_CHECK(M && (val || M->col.empty()), DFM_BAD_ARG, "bad arguments to dfm_bmat_set_values");
+  const size_t n = M->col.size() * M->b * M->br;
+  if (n) DFM_HIP(hipMemcpyAsync(M->d_val.p, val, n * sizeof(double), hipMemcpyHostToDevice, stream()));
+  DFM_HIP(hipStreamSynchronize(stream()));
+  return DFM_OK;
+}
 int dfm_bmat_mearge(dfm_bmat* M, unsigned nblkel, const unsigned* blkel, unsigned blksize, const double* emat) {
   DFM_CHECK(M && blkel && emat && blksize == M->b * M->br, DFM_BAD_ARG, "bad arguments to dfm_bmat_mearge");
   const unsigned bb = blksize;

@@ -47,9 +47,26 @@ long fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned* c
 // ILU(k) symbolic with DelFEM's level rule (src/matvec/matdiafrac_blkcrs.cpp:1249-1377):
 // new fill gets max(lev_ik, lev_kj)+1, an entry keeps the first level it was
 // given, k is taken from the growing sorted row in ascending order.
+// With fill_blk (AddFracPtn(lev, aBlkFill), :1379-1516) the rows and columns
+// listed there take fill of any level: fill (i, j) of level l is kept when
+// l <= lev for both i and j, or when i or j is listed; operands are not
+// filtered by their own levels.
 int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
                   int lev_fill, const std::vector<int>* order, std::vector<unsigned>& frow,
-                  std::vector<unsigned>& fcol, std::vector<int>& flev, std::vector<unsigned>& fdiaind) {
+                  std::vector<unsigned>& fcol, std::vector<int>& flev, std::vector<unsigned>& fdiaind,
+                  const std::vector<unsigned>* fill_blk) {
+  const bool blk = fill_blk && !fill_blk->empty();
+  std::vector<int> rowlev;  // per row: its fill limit, -1 unlimited
+  if (blk) {
+    rowlev.assign(nb, lev_fill);
+    for (unsigned r : *fill_blk) {
+      if (r >= nb) {
+        set_error("fill block %u out of range (%u rows)", r, nb);
+        return DFM_BAD_ARG;
+      }
+      rowlev[r] = -1;
+    }
+  }
   const unsigned ncrs0 = rowptr[nb];
   std::vector<unsigned> prow(nb + 1), pcol(ncrs0);
   if (order) {  // src/matvec/mat_blkcrs.cpp:493-553
@@ -73,7 +90,8 @@ int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::v
           return DFM_UNSORTED;
         }
   }
-  const bool unchanged = ncrs0 == 0 || (unsigned long long)ncrs0 == (unsigned long long)nb * (nb - 1) || lev_fill == 0;
+  const bool unchanged = ncrs0 == 0 || (unsigned long long)ncrs0 == (unsigned long long)nb * (nb - 1) ||
+                         (lev_fill == 0 && !blk);
   fdiaind.assign(nb, 0);
   if (unchanged) {
     frow = prow;
@@ -111,14 +129,18 @@ int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::v
     for (int kc = head; kc != -1; kc = list[kc].next) {
       const unsigned k = list[kc].c;
       const int lik = list[kc].lev;
-      if (lev_fill != -1 && lik + 1 > lev_fill) continue;
+      if (!blk && lev_fill != -1 && lik + 1 > lev_fill) continue;
       if (k >= i) break;
       int cur = kc;
       for (unsigned q = fdiaind[k]; q < frow[k + 1]; q++) {
         const int lkj = flev[q];
-        if (lev_fill != -1 && lkj + 1 > lev_fill) continue;
+        if (!blk && lev_fill != -1 && lkj + 1 > lev_fill) continue;
         const unsigned j = fcol[q];
         if (j == i) continue;
+        if (blk) {
+          const int l = std::max(lik, lkj) + 1, li = rowlev[i], lj = rowlev[j];
+          if (li != -1 && lj != -1 && (l > li || l > lj)) continue;
+        }
         for (;;) {
           const int nx = list[cur].next;
           if (nx == -1 || list[nx].c >= j) break;

@@ -99,9 +99,11 @@ struct bf16s {
 long fem_pattern(unsigned nnode, unsigned nelem, unsigned npe, const unsigned* conn,
                  std::vector<unsigned>& rowptr, std::vector<unsigned>& col);
 // returns DFM_OK or DFM_UNSORTED
+// fill_blk (may be NULL): AddFracPtn(lev, aBlkFill) -- rows and columns that take fill of any level
 int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
                   int lev_fill, const std::vector<int>* order, std::vector<unsigned>& frow,
-                  std::vector<unsigned>& fcol, std::vector<int>& flev, std::vector<unsigned>& fdiaind);
+                  std::vector<unsigned>& fcol, std::vector<int>& flev, std::vector<unsigned>& fdiaind,
+                  const std::vector<unsigned>* fill_blk = nullptr);
 int greedy_colour(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
                   std::vector<int>& colour);
 void colour_order(unsigned nb, const std::vector<int>& colour, int ncolour, std::vector<int>& order);
@@ -154,7 +156,8 @@ struct KState {
 // dfm_ilu_create, or (given_row / given_col, natural order) a factor on a
 // pattern computed by the caller that holds A's (cross-segment fill).
 int ilu_create_impl(const dfm_bcrs_s* A, int lev_fill, int ordering, const int* order, unsigned block_nodes,
-                    const std::vector<unsigned>* given_row, const std::vector<unsigned>* given_col, dfm_ilu_s** out);
+                    const std::vector<unsigned>* given_row, const std::vector<unsigned>* given_col, dfm_ilu_s** out,
+                    const std::vector<unsigned>* fill_blk = nullptr);
 // Numeric ILU in two steps (dfm_ilu_set_value = copy + factor): between
 // them a caller may change the copied values (cross-segment updates).
 int ilu_numeric_copy(dfm_ilu_s* M, const dfm_bcrs_s* A);

@@ -770,7 +770,7 @@ static __global__ void k_ilu_copy_dia(int nb, int row0, int bb, const int* __res
   const long i = row0 + t / bb;
   const int e = (int)(t - (i - row0) * bb);
   const long io = order ? order[i] : i;
-  fdia[t] = adia[(size_t)io * bb + e];
+  fdia[(size_t)i * bb + e] = adia[(size_t)io * bb + e];
 }
 
 // One thread per row of a level of the lower DAG: IKJ update with the

@@ -190,15 +190,24 @@ class CMatDiaFrac_BlkCrs:
     """MatVec::CMatDiaFrac_BlkCrs (include/delfem/matvec/matdiafrac_blkcrs.h:40-137):
     symbolic ILU(lev) at construction, numeric on SetValue."""
 
-    def __init__(self, lev, A, order=None, ordering=None, block_nodes=0):
+    def __init__(self, lev, A, order=None, ordering=None, block_nodes=0, fill_blk=None):
+        """fill_blk (natural order only): block rows and columns that take fill
+        of any level -- AddFracPtn(lev, aBlkFill) (matdiafrac_blkcrs.cpp:1379-1516),
+        what CPreconditioner_ILU::SetFillBlk asks for."""
         if ordering is None:
             ordering = "given" if order is not None else "natural"
         h = C.c_void_p()
         ordarr = None
         if order is not None:
             ordarr = np.ascontiguousarray(order, dtype=np.int32)
-        call("dfm_ilu_create", A.h, int(lev), _ORDER[ordering], _ptr(ordarr) if ordarr is not None else None,
-             int(block_nodes), C.byref(h))
+        if fill_blk is not None:
+            if ordering != "natural":
+                raise ValueError("fill blocks need the natural order")
+            blk = np.ascontiguousarray(fill_blk, dtype=np.uint32)
+            call("dfm_ilu_create_fill", A.h, int(lev), _ptr(blk), blk.size, C.byref(h))
+        else:
+            call("dfm_ilu_create", A.h, int(lev), _ORDER[ordering], _ptr(ordarr) if ordarr is not None else None,
+                 int(block_nodes), C.byref(h))
         self.h, self.nblk, self.len = h, A.nblk, A.len
         self.nsingular = 0
 
@@ -266,29 +275,36 @@ class CPreconditioner_ILU(CMatDiaFrac_BlkCrs):
         super().__init__(nlev, A, order=order, ordering=ordering, block_nodes=block_nodes)
 
 
-def _solve(fn, conv_ratio, iteration, A, prec, r, x, precision, opts_kw):
+def _solve(fn, conv, it, A, prec, r, x, precision, opts_kw):
     _sync_torch()
-    conv = C.c_double(conv_ratio)
-    it = C.c_uint(iteration)
     opts = _lib.SolveOpts()
     opts.precision = _lib.PRECISIONS[precision]
     for k, v in (opts_kw or {}).items():
         setattr(opts, k, v)
     rc = call(fn, A.h, prec.h, _dptr(r), _dptr(x), C.byref(conv), C.byref(it), C.byref(opts),
               ok=(_lib.DFM_OK, _lib.DFM_NOT_CONVERGED))
-    info = dict(inner_iters=opts.inner_iters, outer_iters=opts.outer_iters, true_relres=opts.true_relres)
-    return rc == _lib.DFM_OK, conv.value, it.value, info
+    info = dict(inner_iters=opts.inner_iters, outer_iters=opts.outer_iters, true_relres=opts.true_relres,
+                converged=rc == _lib.DFM_OK)
+    return info
 
 
 def Solve_PCG(conv_ratio, iteration, A, prec, r, x, precision="fp64", **opts):
     """LsSol::Solve_PCG (src/ls/solver_ls_iter.cpp:100-172).  Returns
-    (converged, conv_ratio, iteration, info); r is overwritten by the residual."""
-    return _solve("dfm_solve_pcg", conv_ratio, iteration, A, prec, r, x, precision, opts)
+    (ok, conv_ratio, iteration, info): ok as the reference's result, true when
+    converged, false at max_iter; info["converged"] the same; r is
+    overwritten by the residual."""
+    conv, it = C.c_double(conv_ratio), C.c_uint(iteration)
+    info = _solve("dfm_solve_pcg", conv, it, A, prec, r, x, precision, opts)
+    return info["converged"], conv.value, it.value, info
 
 
 def Solve_PBiCGSTAB(conv_ratio, iteration, A, prec, r, x, precision="fp64", **opts):
-    """LsSol::Solve_PBiCGSTAB (src/ls/solver_ls_iter.cpp:291-408)."""
-    return _solve("dfm_solve_pbicgstab", conv_ratio, iteration, A, prec, r, x, precision, opts)
+    """LsSol::Solve_PBiCGSTAB (src/ls/solver_ls_iter.cpp:291-408).  As the
+    reference it returns true also when stopped by max_iter (then iteration
+    = max_iter and conv_ratio unchanged); info["converged"] tells them apart."""
+    conv, it = C.c_double(conv_ratio), C.c_uint(iteration)
+    info = _solve("dfm_solve_pbicgstab", conv, it, A, prec, r, x, precision, opts)
+    return True, conv.value, it.value, info
 
 
 def dot(x, y):

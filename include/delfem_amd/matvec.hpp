@@ -1,21 +1,32 @@
 // delfem_amd/matvec.hpp -- C++ classes with the names and semantics of
 // DelFEM's MatVec layer over the dfm.h C-ABI: block vectors, Dirichlet flags,
-// the block-CRS matrix with separate diagonal and its ILU(k) factor.  Values
-// live in device memory; host arrays are copied in and out explicitly.
+// the block-CRS matrix with separate diagonal and its ILU(k) factor.
 //
 //   MatVec::CVector_Blk        include/delfem/matvec/vector_blk.h:41-215
 //   MatVec::CBCFlag            include/delfem/matvec/bcflag_blk.h:36-140
 //   MatVec::CMatDia_BlkCrs     include/delfem/matvec/matdia_blkcrs.h:44-118
+//   MatVec::CMat_BlkCrs        include/delfem/matvec/mat_blkcrs.h:45-195
 //   MatVec::CMatDiaFrac_BlkCrs include/delfem/matvec/matdiafrac_blkcrs.h:40-137
 //   MatVec::CPrecond_Blk       include/delfem/matvec/matprecond_blk.h:37-41
 //   MatVec::Sol::Solve_CG / Solve_PCG / Solve_BiCGSTAB / Solve_PBiCGSTAB
 //                              include/delfem/matvec/solver_mat_iter.h:41-60
 //
+// Values live in device memory.  Element loops written against the reference
+// work unchanged: per-entry vector access (GetValue / SetValue / AddValue /
+// GetValuePtr) and per-element Mearge act on host copies that are kept in
+// step lazily -- a vector's host copy is fetched whole on the first host
+// access after device work and sent back whole before the next device
+// operation; a matrix's Mearge calls accumulate into a host copy of its values
+// that is sent to the device, once, when the matrix is next used there.
+// Host pointers handed out (GetValue references, GetValuePtr, GetPtrValPSuP)
+// stay valid, but see device results only after being fetched again.
+//
 // The classes are declared in namespace delfem_amd::MatVec; unless
 // DELFEM_AMD_NO_ALIASES is defined, namespace MatVec aliases it so that code
 // written against the reference compiles unchanged.  Failures of the library
 // (bad arguments, device errors) throw delfem_amd::Error; numerical outcomes
-// are returned as bool like the reference.
+// are returned as bool like the reference.  Variable ("flex") block sizes are
+// not provided.
 #pragma once
 
 #include <algorithm>
@@ -27,6 +38,7 @@
 #include <vector>
 
 #include "../dfm.h"
+#include "indexed_array.hpp"
 
 namespace delfem_amd {
 
@@ -43,73 +55,173 @@ inline int check(int rc, const char* what) {
 
 namespace MatVec {
 
+namespace detail {
+// One device array of doubles and its host copy.  host_ok / dev_ok say which
+// side holds the current values (at least one does); the other side is
+// brought up to date, whole, when it is next used.  A multi-segment system's
+// whole-vector and per-segment vectors share one store.
+class VecStore {
+ public:
+  explicit VecStore(size_t n) : n_(n) {
+    if (n_) check(dfm_malloc(reinterpret_cast<void**>(&dev_), sizeof(double) * n_), "CVector_Blk");
+  }
+  ~VecStore() {
+    if (dev_) dfm_free(dev_);
+  }
+  VecStore(const VecStore&) = delete;
+  VecStore& operator=(const VecStore&) = delete;
+  size_t size() const { return n_; }
+  const double* host_read() {
+    pull();
+    return host_.data();
+  }
+  double* host_write() {
+    pull();
+    dev_ok_ = false;
+    return host_.data();
+  }
+  const double* dev_read() {
+    push();
+    return dev_;
+  }
+  double* dev_write() {
+    push();
+    host_ok_ = false;
+    return dev_;
+  }
+  // the device side, about to be overwritten whole: nothing is sent over
+  double* dev_overwrite() {
+    dev_ok_ = true;
+    host_ok_ = false;
+    return dev_;
+  }
+
+ private:
+  void pull() {
+    if (host_ok_) return;
+    host_.resize(n_);
+    if (n_) check(dfm_memcpy_dtoh(host_.data(), dev_, sizeof(double) * n_), "CVector_Blk (device to host)");
+    host_ok_ = true;
+  }
+  void push() {
+    if (dev_ok_) return;
+    if (n_) check(dfm_memcpy_htod(dev_, host_.data(), sizeof(double) * n_), "CVector_Blk (host to device)");
+    dev_ok_ = true;
+  }
+  size_t n_;
+  double* dev_ = nullptr;
+  std::vector<double> host_;
+  bool host_ok_ = false, dev_ok_ = true;
+};
+}  // namespace detail
+
 class CMatDia_BlkCrs;
 
 class CVector_Blk {
  public:
-  CVector_Blk(unsigned nblk, unsigned len) : nblk_(nblk), len_(len) {
-    check(dfm_malloc(reinterpret_cast<void**>(&v_), sizeof(double) * size()), "CVector_Blk");
+  CVector_Blk(unsigned nblk, unsigned len)
+      : nblk_(nblk), len_(len), st_(std::make_shared<detail::VecStore>((size_t)nblk * len)) {
     SetVectorZero();
   }
-  // a view of nblk*len device doubles owned elsewhere (one segment of a
-  // multi-segment system's vector); it does not free them
-  CVector_Blk(unsigned nblk, unsigned len, double* view) : nblk_(nblk), len_(len), v_(view), owns_(false) {}
-  ~CVector_Blk() {
-    if (v_ && owns_) dfm_free(v_);
-  }
-  CVector_Blk(const CVector_Blk&) = delete;
-  CVector_Blk(CVector_Blk&& o) noexcept : nblk_(o.nblk_), len_(o.len_), v_(o.v_), owns_(o.owns_) { o.v_ = nullptr; }
+  // a copy of the values (vector_blk.h:62-72)
+  CVector_Blk(const CVector_Blk& v) : CVector_Blk(v.nblk_, v.len_) { *this = v; }
+  // a view of nblk*len entries of `st` from `off` (one segment of a
+  // multi-segment system's vector)
+  CVector_Blk(unsigned nblk, unsigned len, std::shared_ptr<detail::VecStore> st, size_t off)
+      : nblk_(nblk), len_(len), st_(std::move(st)), off_(off) {}
+  virtual ~CVector_Blk() = default;
   // operator= copies the values (vector_blk.cpp:62-71)
   CVector_Blk& operator=(const CVector_Blk& rhs) {
-    check(dfm_vec_copy(size(), rhs.v_, v_), "CVector_Blk::operator=");
+    if (this != &rhs) {
+      const double* x = rhs.dread();
+      check(dfm_vec_copy(size(), x, dwrite()), "CVector_Blk::operator=");
+    }
     return *this;
   }
   CVector_Blk& operator*=(double d) {
-    check(dfm_vec_scal(size(), d, v_), "CVector_Blk::operator*=");
+    check(dfm_vec_scal(size(), d, dwrite()), "CVector_Blk::operator*=");
     return *this;
   }
   CVector_Blk& AXPY(const double& alpha, const CVector_Blk& rhs) {
-    check(dfm_vec_axpy(size(), alpha, rhs.v_, v_), "CVector_Blk::AXPY");
+    const double* x = rhs.dread();
+    check(dfm_vec_axpy(size(), alpha, x, dwrite()), "CVector_Blk::AXPY");
     return *this;
   }
   // operator+= (vector_blk.cpp:73-81): y + 1.0 * x is y + x exactly
   CVector_Blk& operator+=(const CVector_Blk& rhs) { return AXPY(1.0, rhs); }
-  void SetVectorZero() { check(dfm_vec_fill(size(), 0.0, v_), "CVector_Blk::SetVectorZero"); }
+  void SetVectorZero() {
+    double* p = whole() ? st_->dev_overwrite() : st_->dev_write() + off_;
+    check(dfm_vec_fill(size(), 0.0, p), "CVector_Blk::SetVectorZero");
+  }
   double GetSquaredVectorNorm() const { return (*this) * (*this); }
   unsigned NBlk() const { return nblk_; }
-  unsigned Len() const { return len_; }
+  int Len() const { return (int)len_; }
+  unsigned Len(unsigned /*iblk*/) const { return len_; }
   unsigned GetTotalDofSize() const { return nblk_ * len_; }
   friend double operator*(const CVector_Blk& a, const CVector_Blk& b) {
     double r = 0.0;
-    check(dfm_vec_dot(a.size(), a.v_, b.v_, &r), "CVector_Blk dot");
+    const double* x = a.dread();
+    const double* y = b.dread();
+    check(dfm_vec_dot(a.size(), x, y, &r), "CVector_Blk dot");
     return r;
   }
-  // host copies of the whole vector (GetValue / SetValue over all entries)
-  void SetValues(const double* host) { check(dfm_memcpy_htod(v_, host, sizeof(double) * size()), "SetValues"); }
-  void GetValues(double* host) const { check(dfm_memcpy_dtoh(host, v_, sizeof(double) * size()), "GetValues"); }
+  // per-entry access on the host copy (vector_blk.h:135-181)
+  const double& GetValue(unsigned iblk, unsigned idofblk) const {
+    return st_->host_read()[off_ + (size_t)iblk * len_ + idofblk];
+  }
+  void SetValue(unsigned iblk, unsigned idofblk, double val) {
+    st_->host_write()[off_ + (size_t)iblk * len_ + idofblk] = val;
+  }
+  void AddValue(unsigned iblk, unsigned idofblk, double val) {
+    st_->host_write()[off_ + (size_t)iblk * len_ + idofblk] += val;
+  }
+  double* GetValuePtr(unsigned iblk) { return st_->host_write() + off_ + (size_t)iblk * len_; }
+  const double* GetValuePtr(unsigned iblk) const { return st_->host_read() + off_ + (size_t)iblk * len_; }
+  // host copies of the whole vector
+  void SetValues(const double* host) {
+    double* p = whole() ? st_->dev_overwrite() : st_->dev_write() + off_;
+    check(dfm_memcpy_htod(p, host, sizeof(double) * size()), "SetValues");
+  }
+  void GetValues(double* host) const {
+    check(dfm_memcpy_dtoh(host, dread(), sizeof(double) * size()), "GetValues");
+  }
   std::vector<double> GetValues() const {
     std::vector<double> h(size());
     GetValues(h.data());
     return h;
   }
-  double* device_ptr() { return v_; }
-  const double* device_ptr() const { return v_; }
+  // device memory of the values, current; the non-const form counts as a write
+  double* device_ptr() { return dwrite(); }
+  const double* device_ptr() const { return dread(); }
   size_t size() const { return (size_t)nblk_ * len_; }
+  // re-point this vector at another store (the vector keeps its identity);
+  // the caller has copied the values over
+  void rebind(std::shared_ptr<detail::VecStore> st, size_t off) {
+    st_ = std::move(st);
+    off_ = off;
+  }
+  const std::shared_ptr<detail::VecStore>& store() const { return st_; }
+  size_t offset() const { return off_; }
 
  private:
+  bool whole() const { return off_ == 0 && size() == st_->size(); }
+  const double* dread() const { return st_->dev_read() + off_; }
+  double* dwrite() const { return st_->dev_write() + off_; }
   unsigned nblk_, len_;
-  double* v_ = nullptr;
-  bool owns_ = true;
+  std::shared_ptr<detail::VecStore> st_;
+  size_t off_ = 0;
 };
 
 class CBCFlag {
  public:
   CBCFlag(unsigned nblk, unsigned len) : nblk_(nblk), len_(len), flag_((size_t)nblk * len, 0) {}
+  virtual ~CBCFlag() = default;
   void SetAllFlagZero() { std::fill(flag_.begin(), flag_.end(), 0); }
   void SetBC(unsigned iblk, unsigned idof, int flag = 1) { flag_[(size_t)iblk * len_ + idof] = flag; }
   int GetBCFlag(unsigned iblk, unsigned idof) const { return flag_[(size_t)iblk * len_ + idof]; }
   unsigned NBlk() const { return nblk_; }
-  unsigned LenBlk() const { return len_; }
+  int LenBlk() const { return (int)len_; }
+  unsigned LenBlk(unsigned /*iblk*/) const { return len_; }
   // CBCFlag::SetZeroToBCDof (bcflag_blk.h:102-129)
   bool SetZeroToBCDof(CVector_Blk& v) const {
     check(dfm_vec_zero_bc(v.size(), flag_.data(), v.device_ptr()), "SetZeroToBCDof");
@@ -122,60 +234,159 @@ class CBCFlag {
   std::vector<int> flag_;
 };
 
+namespace detail {
+// Host copy of a block-CRS matrix's values for element loops: filled from
+// the device on the first host access (or set to zero by SetZero), written
+// back whole before the matrix is next used on the device.
+struct ValMirror {
+  std::vector<unsigned> rowptr, col;  // pattern (host)
+  std::vector<double> val, dia;
+  bool valid = false;                 // val / dia hold the current values
+  bool dirty = false;                 // ... and the device does not
+};
+}  // namespace detail
+
 class CMatDia_BlkCrs {
  public:
-  CMatDia_BlkCrs(unsigned nblk, unsigned len) { check(dfm_bcrs_create(nblk, len, &h_), "CMatDia_BlkCrs"); }
-  ~CMatDia_BlkCrs() {
+  CMatDia_BlkCrs(unsigned nblk, unsigned len) : nblk_(nblk), len_(len) {
+    check(dfm_bcrs_create(nblk, len, &h_), "CMatDia_BlkCrs");
+    m_.rowptr.assign(nblk + 1, 0);
+  }
+  virtual ~CMatDia_BlkCrs() {
     if (h_) dfm_bcrs_destroy(h_);
   }
   CMatDia_BlkCrs(const CMatDia_BlkCrs&) = delete;
   CMatDia_BlkCrs& operator=(const CMatDia_BlkCrs&) = delete;
-  unsigned NBlkMatCol() const { return info().nblk; }
-  unsigned NBlkMatRow() const { return info().nblk; }
-  unsigned LenBlkCol() const { return info().len; }
-  unsigned LenBlkRow() const { return info().len; }
-  unsigned NCrs() const { return info().ncrs; }
-  // AddPattern(const Com::CIndexedArray&): index = row pointer, array = columns
+  unsigned NBlkMatCol() const { return nblk_; }
+  unsigned NBlkMatRow() const { return nblk_; }
+  int LenBlkCol() const { return (int)len_; }
+  int LenBlkRow() const { return (int)len_; }
+  unsigned NCrs() const { return (unsigned)m_.col.size(); }
+  // AddPattern(const Com::CIndexedArray&) (matdia_blkcrs.cpp:346-505): the
+  // diagonal is excluded, rows are merged with the existing pattern
+  bool AddPattern(const Com::CIndexedArray& crs) { return AddPattern(crs.index, crs.array); }
   bool AddPattern(const std::vector<unsigned>& index, const std::vector<unsigned>& array) {
-    return check(dfm_bcrs_add_pattern(h_, (unsigned)index.size() - 1, index.data(), array.data()),
-                 "AddPattern") == DFM_OK;
+    flush();
+    static const unsigned none = 0;
+    const bool ok = check(dfm_bcrs_add_pattern(h_, (unsigned)index.size() - 1, index.data(),
+                                               array.empty() ? &none : array.data()),
+                          "AddPattern") == DFM_OK;
+    unsigned nb = 0, len = 0, ncrs = 0;
+    check(dfm_bcrs_info(h_, &nb, &len, &ncrs), "AddPattern");
+    m_.col.resize(ncrs);
+    check(dfm_bcrs_get_pattern(h_, m_.rowptr.data(), m_.col.data()), "AddPattern");
+    m_.valid = m_.dirty = false;
+    return ok;
   }
-  bool SetZero() { return check(dfm_bcrs_set_zero(h_), "SetZero") == DFM_OK; }
-  // Mearge of one element matrix; the diagonal-block matrix takes square
-  // elements (the same node list for rows and columns)
+  // SetZero (matdia_blkcrs.cpp:139-163): on the host copy, which the element
+  // loop that follows fills
+  bool SetZero() {
+    const size_t bb = (size_t)len_ * len_;
+    m_.val.assign(m_.col.size() * bb, 0.0);
+    m_.dia.assign((size_t)nblk_ * bb, 0.0);
+    m_.valid = m_.dirty = true;
+    return true;
+  }
+  // Mearge (matdia_blkcrs.cpp:165-258): element block (i, j) at
+  // emat[(i*nblkel_row + j)*blksize]; position i == j goes to the diagonal
+  // block of node blkel_col[i], other blocks to their CRS entry of row
+  // blkel_col[i], and blocks outside the pattern are dropped.
   bool Mearge(unsigned nblkel_col, const unsigned* blkel_col, unsigned nblkel_row, const unsigned* blkel_row,
               unsigned blksize, const double* emat) {
-    if (nblkel_col != nblkel_row) return false;
+    const size_t bb = (size_t)len_ * len_;
+    if (blksize != bb) return false;
     for (unsigned i = 0; i < nblkel_col; i++)
-      if (blkel_col[i] != blkel_row[i]) return false;
-    return check(dfm_bcrs_mearge(h_, nblkel_col, blkel_col, blksize, emat), "Mearge") == DFM_OK;
+      if (blkel_col[i] >= nblk_) return false;
+    for (unsigned j = 0; j < nblkel_row; j++)
+      if (blkel_row[j] >= nblk_) return false;
+    host_values(true);
+    for (unsigned ie = 0; ie < nblkel_col; ie++) {
+      const unsigned i = blkel_col[ie];
+      const unsigned* c0 = m_.col.data() + m_.rowptr[i];
+      const unsigned* c1 = m_.col.data() + m_.rowptr[i + 1];
+      for (unsigned je = 0; je < nblkel_row; je++) {
+        const double* in = emat + ((size_t)ie * nblkel_row + je) * bb;
+        double* out;
+        if (ie == je) {
+          out = m_.dia.data() + (size_t)i * bb;
+        } else {
+          const unsigned* q = std::lower_bound(c0, c1, blkel_row[je]);
+          if (q == c1 || *q != blkel_row[je]) continue;
+          out = m_.val.data() + (size_t)(q - m_.col.data()) * bb;
+        }
+        for (size_t t = 0; t < bb; t++) out[t] += in[t];
+      }
+    }
+    return true;
   }
-  // y = beta*y + alpha*A*x
+  // y = beta*y + alpha*A*x (matdia_blkcrs.cpp:921-1094)
   bool MatVec(double alpha, const CVector_Blk& x, double beta, CVector_Blk& y) const {
-    return check(dfm_bcrs_matvec(h_, alpha, x.device_ptr(), beta, y.device_ptr()), "MatVec") == DFM_OK;
+    dfm_bcrs* h = handle();
+    const double* xp = x.device_ptr();
+    return check(dfm_bcrs_matvec(h, alpha, xp, beta, y.device_ptr()), "MatVec") == DFM_OK;
   }
   bool SetBoundaryCondition(const CBCFlag& flag) {
-    return check(dfm_bcrs_set_bc(h_, flag.data()), "SetBoundaryCondition") == DFM_OK;
+    return check(dfm_bcrs_set_bc(handle(), flag.data()), "SetBoundaryCondition") == DFM_OK;
   }
-  // host copies of the values (GetPtrValPSuP / GetPtrValDia)
+  // the pattern and values of one row on the host (mat_blkcrs.h:150-173,
+  // matdia_blkcrs.h:102-111); the non-const forms count as writes
+  const unsigned* GetPtrIndPSuP(unsigned ipoin, unsigned& npsup) const {
+    npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
+    return m_.col.data() + m_.rowptr[ipoin];
+  }
+  const double* GetPtrValPSuP(unsigned ipoin, unsigned& npsup) const {
+    npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
+    return host_values(false).data() + (size_t)m_.rowptr[ipoin] * len_ * len_;
+  }
+  double* GetPtrValPSuP(unsigned ipoin, unsigned& npsup) {
+    npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
+    return host_values(true).data() + (size_t)m_.rowptr[ipoin] * len_ * len_;
+  }
+  const double* GetPtrValDia(unsigned ipoin) const {
+    host_values(false);
+    return m_.dia.data() + (size_t)ipoin * len_ * len_;
+  }
+  double* GetPtrValDia(unsigned ipoin) {
+    host_values(true);
+    return m_.dia.data() + (size_t)ipoin * len_ * len_;
+  }
+  // host copies of the values (CRS order) and of the pattern
   void GetValues(std::vector<double>& val, std::vector<double>& dia) const {
-    const auto i = info();
-    val.resize((size_t)i.ncrs * i.len * i.len);
-    dia.resize((size_t)i.nblk * i.len * i.len);
-    check(dfm_bcrs_get_values(h_, val.data(), dia.data()), "GetValues");
+    host_values(false);
+    val = m_.val;
+    dia = m_.dia;
   }
-  dfm_bcrs* handle() const { return h_; }
+  const std::vector<unsigned>& RowPtr() const { return m_.rowptr; }
+  const std::vector<unsigned>& Col() const { return m_.col; }
+  // the C-ABI handle with the device values current; the caller may change
+  // them, so the host copy is dropped
+  dfm_bcrs* handle() const {
+    flush();
+    m_.valid = false;
+    return h_;
+  }
 
  private:
-  struct Info {
-    unsigned nblk, len, ncrs;
-  };
-  Info info() const {
-    Info i{};
-    check(dfm_bcrs_info(h_, &i.nblk, &i.len, &i.ncrs), "CMatDia_BlkCrs info");
-    return i;
+  std::vector<double>& host_values(bool write) const {
+    if (!m_.valid) {
+      const size_t bb = (size_t)len_ * len_;
+      m_.val.resize(m_.col.size() * bb);
+      m_.dia.resize((size_t)nblk_ * bb);
+      check(dfm_bcrs_get_values(h_, m_.val.data(), m_.dia.data()), "CMatDia_BlkCrs values");
+      m_.valid = true;
+      m_.dirty = false;
+    }
+    if (write) m_.dirty = true;
+    return m_.val;
   }
+  void flush() const {
+    if (!m_.dirty) return;
+    check(dfm_bcrs_set_values(h_, m_.val.data(), m_.dia.data()), "CMatDia_BlkCrs values");
+    m_.dirty = false;
+  }
+  unsigned nblk_, len_;
   dfm_bcrs* h_ = nullptr;
+  mutable detail::ValMirror m_;
 };
 
 // MatVec::CMat_BlkCrs (include/delfem/matvec/mat_blkcrs.h:45-195): a
@@ -187,65 +398,129 @@ class CMat_BlkCrs {
   CMat_BlkCrs(unsigned nblk_col, unsigned len_col, unsigned nblk_row, unsigned len_row)
       : nblk_col_(nblk_col), len_col_(len_col), nblk_row_(nblk_row), len_row_(len_row) {
     check(dfm_bmat_create(nblk_col, len_col, nblk_row, len_row, &h_), "CMat_BlkCrs");
+    m_.rowptr.assign(nblk_col + 1, 0);
   }
-  ~CMat_BlkCrs() {
+  virtual ~CMat_BlkCrs() {
     if (h_) dfm_bmat_destroy(h_);
   }
   CMat_BlkCrs(const CMat_BlkCrs&) = delete;
   CMat_BlkCrs& operator=(const CMat_BlkCrs&) = delete;
   unsigned NBlkMatCol() const { return nblk_col_; }
   unsigned NBlkMatRow() const { return nblk_row_; }
-  unsigned LenBlkCol() const { return len_col_; }
-  unsigned LenBlkRow() const { return len_row_; }
-  unsigned NCrs() const {
-    unsigned nb = 0, len = 0, ncrs = 0;
-    check(dfm_bmat_info(h_, &nb, &len, &ncrs), "CMat_BlkCrs info");
-    return ncrs;
-  }
+  int LenBlkCol() const { return (int)len_col_; }
+  int LenBlkRow() const { return (int)len_row_; }
+  unsigned NCrs() const { return (unsigned)m_.col.size(); }
   // AddPattern(const Com::CIndexedArray&) (mat_blkcrs.cpp:280-414)
+  bool AddPattern(const Com::CIndexedArray& crs) { return AddPattern(crs.index, crs.array); }
   bool AddPattern(const std::vector<unsigned>& index, const std::vector<unsigned>& array) {
+    flush();
     static const unsigned none = 0;
-    return check(dfm_bmat_add_pattern(h_, (unsigned)index.size() - 1, index.data(),
-                                      array.empty() ? &none : array.data()),
-                 "CMat_BlkCrs::AddPattern") == DFM_OK;
+    const bool ok = check(dfm_bmat_add_pattern(h_, (unsigned)index.size() - 1, index.data(),
+                                               array.empty() ? &none : array.data()),
+                          "CMat_BlkCrs::AddPattern") == DFM_OK;
+    refresh_pattern();
+    return ok;
   }
-  bool SetZero() { return check(dfm_bmat_set_zero(h_), "CMat_BlkCrs::SetZero") == DFM_OK; }
-  // Mearge of one element (mat_blkcrs.cpp:754-828); square elements only
+  bool SetZero() {
+    m_.val.assign(m_.col.size() * len_col_ * len_row_, 0.0);
+    m_.valid = m_.dirty = true;
+    return true;
+  }
+  // Mearge (mat_blkcrs.cpp:754-828): element block (i, j) at
+  // emat[(i*nblkel_row + j)*blksize] into the entry (blkel_col[i],
+  // blkel_row[j]) where the pattern has it
   bool Mearge(unsigned nblkel_col, const unsigned* blkel_col, unsigned nblkel_row, const unsigned* blkel_row,
               unsigned blksize, const double* emat) {
-    if (nblkel_col != nblkel_row) return false;
+    const size_t bs = (size_t)len_col_ * len_row_;
+    if (blksize != bs) return false;
     for (unsigned i = 0; i < nblkel_col; i++)
-      if (blkel_col[i] != blkel_row[i]) return false;
-    return check(dfm_bmat_mearge(h_, nblkel_col, blkel_col, blksize, emat), "CMat_BlkCrs::Mearge") == DFM_OK;
+      if (blkel_col[i] >= nblk_col_) return false;
+    for (unsigned j = 0; j < nblkel_row; j++)
+      if (blkel_row[j] >= nblk_row_) return false;
+    host_values(true);
+    for (unsigned ie = 0; ie < nblkel_col; ie++) {
+      const unsigned i = blkel_col[ie];
+      const unsigned* c0 = m_.col.data() + m_.rowptr[i];
+      const unsigned* c1 = m_.col.data() + m_.rowptr[i + 1];
+      for (unsigned je = 0; je < nblkel_row; je++) {
+        const unsigned* q = std::lower_bound(c0, c1, blkel_row[je]);
+        if (q == c1 || *q != blkel_row[je]) continue;
+        const double* in = emat + ((size_t)ie * nblkel_row + je) * bs;
+        double* out = m_.val.data() + (size_t)(q - m_.col.data()) * bs;
+        for (size_t t = 0; t < bs; t++) out[t] += in[t];
+      }
+    }
+    return true;
   }
   // y = beta*y + alpha*A*x (mat_blkcrs.cpp:955-1174, isnt_trans = true)
   bool MatVec(double alpha, const CVector_Blk& x, double beta, CVector_Blk& y, bool isnt_trans = true) const {
     if (!isnt_trans) throw Error("CMat_BlkCrs::MatVec: the transposed product is not provided", DFM_NOT_IMPLEMENTED);
-    return check(dfm_bmat_matvec(h_, alpha, x.device_ptr(), beta, y.device_ptr()), "CMat_BlkCrs::MatVec") == DFM_OK;
+    dfm_bmat* h = handle();
+    const double* xp = x.device_ptr();
+    return check(dfm_bmat_matvec(h, alpha, xp, beta, y.device_ptr()), "CMat_BlkCrs::MatVec") == DFM_OK;
   }
   bool SetBoundaryCondition_Row(const CBCFlag& bc) {
-    return check(dfm_bmat_set_bc_row(h_, bc.data()), "SetBoundaryCondition_Row") == DFM_OK;
+    return check(dfm_bmat_set_bc_row(handle(), bc.data()), "SetBoundaryCondition_Row") == DFM_OK;
   }
   bool SetBoundaryCondition_Colum(const CBCFlag& bc) {
-    return check(dfm_bmat_set_bc_col(h_, bc.data()), "SetBoundaryCondition_Colum") == DFM_OK;
+    return check(dfm_bmat_set_bc_col(handle(), bc.data()), "SetBoundaryCondition_Colum") == DFM_OK;
   }
   bool SetBoundaryConditionInverse_Colum(const CBCFlag& bc) {
-    return check(dfm_bmat_set_bc_inverse_col(h_, bc.data()), "SetBoundaryConditionInverse_Colum") == DFM_OK;
+    return check(dfm_bmat_set_bc_inverse_col(handle(), bc.data()), "SetBoundaryConditionInverse_Colum") == DFM_OK;
   }
   // SetPatternBoundary (mat_blkcrs.cpp:1177-1265): the boundary matrix pattern of CLinearSystem_Save
   bool SetPatternBoundary(const CMatDia_BlkCrs& A, const CBCFlag& bc_col, const CBCFlag& bc_row) {
-    return check(dfm_bmat_set_pattern_boundary(h_, A.handle(), bc_col.data(), bc_row.data()),
-                 "SetPatternBoundary") == DFM_OK;
+    flush();
+    const bool ok = check(dfm_bmat_set_pattern_boundary(h_, A.handle(), bc_col.data(), bc_row.data()),
+                          "SetPatternBoundary") == DFM_OK;
+    refresh_pattern();
+    return ok;
   }
-  void GetValues(std::vector<double>& val) const {
-    val.resize((size_t)NCrs() * len_col_ * len_row_);
-    check(dfm_bmat_get_values(h_, val.data()), "CMat_BlkCrs::GetValues");
+  const unsigned* GetPtrIndPSuP(unsigned ipoin, unsigned& npsup) const {
+    npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
+    return m_.col.data() + m_.rowptr[ipoin];
   }
-  dfm_bmat* handle() const { return h_; }
+  const double* GetPtrValPSuP(unsigned ipoin, unsigned& npsup) const {
+    npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
+    return host_values(false).data() + (size_t)m_.rowptr[ipoin] * len_col_ * len_row_;
+  }
+  double* GetPtrValPSuP(unsigned ipoin, unsigned& npsup) {
+    npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
+    return host_values(true).data() + (size_t)m_.rowptr[ipoin] * len_col_ * len_row_;
+  }
+  void GetValues(std::vector<double>& val) const { val = host_values(false); }
+  dfm_bmat* handle() const {
+    flush();
+    m_.valid = false;
+    return h_;
+  }
 
  private:
+  void refresh_pattern() {
+    unsigned nb = 0, len = 0, ncrs = 0;
+    check(dfm_bmat_info(h_, &nb, &len, &ncrs), "CMat_BlkCrs pattern");
+    m_.col.resize(ncrs);
+    check(dfm_bmat_get_pattern(h_, m_.rowptr.data(), m_.col.data()), "CMat_BlkCrs pattern");
+    m_.valid = m_.dirty = false;
+  }
+  std::vector<double>& host_values(bool write) const {
+    if (!m_.valid) {
+      m_.val.resize(m_.col.size() * len_col_ * len_row_);
+      if (!m_.val.empty()) check(dfm_bmat_get_values(h_, m_.val.data()), "CMat_BlkCrs values");
+      m_.valid = true;
+      m_.dirty = false;
+    }
+    if (write) m_.dirty = true;
+    return m_.val;
+  }
+  void flush() const {
+    if (!m_.dirty) return;
+    check(dfm_bmat_set_values(h_, m_.val.empty() ? nullptr : m_.val.data()), "CMat_BlkCrs values");
+    m_.dirty = false;
+  }
   unsigned nblk_col_, len_col_, nblk_row_, len_row_;
   dfm_bmat* h_ = nullptr;
+  mutable detail::ValMirror m_;
 };
 
 // Abstract preconditioner of the matrix layer (matprecond_blk.h:37-41).
@@ -278,12 +553,20 @@ class CMatDiaFrac_BlkCrs : public CPrecond_Blk {
   }
   CMatDiaFrac_BlkCrs(const CMatDiaFrac_BlkCrs&) = delete;
   CMatDiaFrac_BlkCrs& operator=(const CMatDiaFrac_BlkCrs&) = delete;
-  // SetValue(A) + DoILUDecomp(): false when more than 10 pivots were singular
+  // SetValue(A) (matdiafrac_blkcrs.cpp:1518-1537): the values of A, then the
+  // factorisation; true whatever the factorisation met, as the reference,
+  // which ignores DoILUDecomp's result there
   bool SetValue(const CMatDia_BlkCrs& A) {
-    int nsing = 0;
-    return check(dfm_ilu_set_value(h_, A.handle(), &nsing), "SetValue") == DFM_OK;
+    nsing_ = 0;
+    factored_ = check(dfm_ilu_set_value(h_, A.handle(), &nsing_), "SetValue") == DFM_OK;
+    return true;
   }
-  bool DoILUDecomp() { return true; }  // the factorisation runs in SetValue
+  // DoILUDecomp (matdiafrac_blkcrs.cpp:769-1102): the device factorises in
+  // SetValue; this reports its outcome -- false when it stopped at the 11th
+  // singular pivot, leaving the factor as the reference leaves it
+  bool DoILUDecomp() { return factored_; }
+  // singular pivots met by the last factorisation
+  int NSingular() const { return nsing_; }
   bool ForwardSubstitution(CVector_Blk& v) const {
     return check(dfm_ilu_forward(h_, v.device_ptr()), "ForwardSubstitution") == DFM_OK;
   }
@@ -316,6 +599,8 @@ class CMatDiaFrac_BlkCrs : public CPrecond_Blk {
   const CMatDia_BlkCrs* src_ = nullptr;
   int ordering_ = DFM_ORDER_NATURAL;
   dfm_ilu* h_ = nullptr;
+  int nsing_ = 0;
+  bool factored_ = false;
 };
 
 // ---------------------------------------------------------------------------

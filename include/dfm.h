@@ -136,6 +136,12 @@ enum { DFM_ORDER_NATURAL = 0, DFM_ORDER_GIVEN = 1, DFM_ORDER_MULTICOLOUR = 2, DF
  * block_nodes is ignored by the other orderings.  lev_fill = -1 means unlimited. */
 int dfm_ilu_create(const dfm_bcrs *A, int lev_fill, int ordering, const int *order, unsigned block_nodes,
                    dfm_ilu **out);
+/* Natural-order symbolic ILU(lev_fill) in which the nfill block rows listed in fill_blk
+ * (and the same block columns) take fill of any level: CMatDiaFrac_BlkCrs::AddFracPtn(lev,
+ * aBlkFill), matdiafrac_blkcrs.cpp:1379-1516, reached through CPreconditioner_ILU::SetFillBlk
+ * (include/delfem/ls/preconditioner.h:80, preconditioner.cpp:149-150).  Fill (i, j) of level l
+ * is kept when l <= lev_fill for both i and j (lev_fill -1: unlimited) or when i or j is listed. */
+int dfm_ilu_create_fill(const dfm_bcrs *A, int lev_fill, const unsigned *fill_blk, unsigned nfill, dfm_ilu **out);
 int dfm_ilu_destroy(dfm_ilu *M);
 /* ncrs of the factor, number of level sets of L and U, colours (0 for NATURAL / GIVEN) */
 int dfm_ilu_info(const dfm_ilu *M, unsigned *ncrs, int *nlevel_lower, int *nlevel_upper, int *ncolour);
@@ -258,6 +264,7 @@ int dfm_bmat_info(const dfm_bmat *M, unsigned *nblk, unsigned *len, unsigned *nc
 int dfm_bmat_get_pattern(const dfm_bmat *M, unsigned *rowptr, unsigned *col);
 int dfm_bmat_set_zero(dfm_bmat *M);                                        /* SetZero mat_blkcrs.cpp:227 */
 int dfm_bmat_get_values(const dfm_bmat *M, double *val);                   /* ncrs*len*len, row-major blocks */
+int dfm_bmat_set_values(dfm_bmat *M, const double *val);                   /* host values into the pattern (GetPtrValPSuP writes) mat_blkcrs.h:166-173 */
 int dfm_bmat_mearge(dfm_bmat *M, unsigned nblkel, const unsigned *blkel,
                     unsigned blksize, const double *emat);                 /* Mearge (one square element, host emat) mat_blkcrs.cpp:754-828 */
 /* The values of A (assembled, before its boundary conditions) at K_b's entries: equal to

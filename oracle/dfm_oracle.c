@@ -236,10 +236,16 @@ void orc_matvec(unsigned nb, unsigned b, const unsigned *rowptr, const unsigned 
  *   diaind[i] = first position whose column exceeds i (end of row if none). */
 typedef struct { unsigned colv; int lev; int next; } orc_node;
 
-long orc_iluk_symbolic(unsigned nb, const unsigned *rowptr, const unsigned *col,
-                       int lev_fill, const int *order,
-                       unsigned *frowptr, unsigned *fcol, int *flev, unsigned *fdiaind,
-                       long cap)
+/* rowlev (may be NULL): AddFracPtn(lev, aBlkFill), matdiafrac_blkcrs.cpp:1379-1516 --
+ * per row its fill limit (lev_fill, or -1 for the listed blocks); no operand
+ * is skipped for its own level and fill (i, j) of level l is inserted unless
+ * both limits are finite and l exceeds one of them (:1478-1482).  The input
+ * is returned unchanged only for no off-diagonal entries, a full matrix, or
+ * K == 0 without listed blocks (:1392-1407). */
+static long iluk_core(unsigned nb, const unsigned *rowptr, const unsigned *col,
+                      int lev_fill, const int *order, const int *rowlev,
+                      unsigned *frowptr, unsigned *fcol, int *flev, unsigned *fdiaind,
+                      long cap)
 {
     const unsigned ncrs0 = rowptr[nb];
     unsigned *prow = (unsigned *)malloc(sizeof(unsigned) * ((size_t)nb + 1));
@@ -270,7 +276,8 @@ long orc_iluk_symbolic(unsigned nb, const unsigned *rowptr, const unsigned *col,
             for (unsigned q = prow[i] + 1; q < prow[i + 1]; q++)
                 if (pcol[q] <= pcol[q - 1]) { free(prow); free(pcol); return -1; }
     }
-    const int unchanged = (ncrs0 == 0) || ((unsigned long)ncrs0 == (unsigned long)nb * (nb - 1)) || lev_fill == 0;
+    const int unchanged = (ncrs0 == 0) || ((unsigned long)ncrs0 == (unsigned long)nb * (nb - 1)) ||
+                          (lev_fill == 0 && !rowlev);
     if (unchanged) {
         if (cap < (long)ncrs0) { free(prow); free(pcol); return -(long)ncrs0 - 2; }
         memcpy(frowptr, prow, sizeof(unsigned) * ((size_t)nb + 1));
@@ -304,14 +311,18 @@ long orc_iluk_symbolic(unsigned nb, const unsigned *rowptr, const unsigned *col,
         for (int kc = head; kc != -1; kc = list[kc].next) {
             const unsigned k = list[kc].colv;
             const int lik = list[kc].lev;
-            if (lev_fill != -1 && lik + 1 > lev_fill) continue;
+            if (!rowlev && lev_fill != -1 && lik + 1 > lev_fill) continue;
             if (k >= i) break;
             int cur = kc; /* insertion cursor: columns only grow along U(k) */
             for (unsigned q = odia[k]; q < orow[k + 1]; q++) {
                 const int lkj = olev[q];
-                if (lev_fill != -1 && lkj + 1 > lev_fill) continue;
+                if (!rowlev && lev_fill != -1 && lkj + 1 > lev_fill) continue;
                 const unsigned j = ocol[q];
                 if (j == i) continue;
+                if (rowlev) {
+                    const int l = (lik > lkj ? lik : lkj) + 1;
+                    if (rowlev[i] != -1 && rowlev[j] != -1 && (l > rowlev[i] || l > rowlev[j])) continue;
+                }
                 /* advance cursor to the last node with column < j */
                 for (;;) {
                     const int nx = list[cur].next;
@@ -353,6 +364,35 @@ long orc_iluk_symbolic(unsigned nb, const unsigned *rowptr, const unsigned *col,
     }
     free(ocol); free(olev); free(orow); free(odia); free(list); free(prow); free(pcol);
     return result;
+}
+
+long orc_iluk_symbolic(unsigned nb, const unsigned *rowptr, const unsigned *col,
+                       int lev_fill, const int *order,
+                       unsigned *frowptr, unsigned *fcol, int *flev, unsigned *fdiaind,
+                       long cap)
+{
+    return iluk_core(nb, rowptr, col, lev_fill, order, NULL, frowptr, fcol, flev, fdiaind, cap);
+}
+
+/* CPreconditioner_ILU::SetFillBlk + SetLinearSystem on one segment
+ * (src/ls/preconditioner.cpp:149-150, 198-210): natural-order ILU(0) pattern,
+ * AddFracPtn(lev, aBlkFill), MakePatternFinalize.  Returns as orc_iluk_symbolic
+ * (-1 also for a block out of range). */
+long orc_iluk_symbolic_blk(unsigned nb, const unsigned *rowptr, const unsigned *col, int lev_fill,
+                           const unsigned *fill_blk, unsigned nfill,
+                           unsigned *frowptr, unsigned *fcol, int *flev, unsigned *fdiaind, long cap)
+{
+    if (nfill == 0)
+        return iluk_core(nb, rowptr, col, lev_fill, NULL, NULL, frowptr, fcol, flev, fdiaind, cap);
+    int *rowlev = (int *)malloc(sizeof(int) * ((size_t)nb + 1));
+    for (unsigned i = 0; i < nb; i++) rowlev[i] = lev_fill;
+    for (unsigned t = 0; t < nfill; t++) {
+        if (fill_blk[t] >= nb) { free(rowlev); return -1; }
+        rowlev[fill_blk[t]] = -1;
+    }
+    const long r = iluk_core(nb, rowptr, col, lev_fill, NULL, rowlev, frowptr, fcol, flev, fdiaind, cap);
+    free(rowlev);
+    return r;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -841,6 +881,97 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
     orc_sys s;
     sys_init(&s, nb, b, rowptr, col, val, dia, frowptr, fcol, fdiaind, fval, fdia, order);
     const int ret = bicg_core(&s, (size_t)nb * b, r, x, conv_ratio, iteration);
+    sys_free(&s);
+    return ret;
+}
+
+/* a12/a13, unpreconditioned: LsSol::Solve_CG (src/ls/solver_ls_iter.cpp:31-97)
+ * and LsSol::Solve_BiCGSTAB (:177-286) over CLinearSystem's BLAS-1 (AXPY
+ * skips alpha == 0, DOT adds the segments' products).  Solve_CG returns at
+ * ||r||^2 < 1e-60 after zeroing x, stops with num_iter = the 0-based index of
+ * the converging step (max_iter when none; conv_ratio then unchanged);
+ * Solve_BiCGSTAB zeroes x first, counts from 1, and reports max_iter with
+ * conv_ratio unchanged when it does not converge.  Both return true. */
+static int ls_cg_core(orc_sys *sp, size_t n, double *r, double *x, double *conv_ratio, unsigned *num_iter)
+{
+    const unsigned max_iter = *num_iter;
+    const double tol = *conv_ratio;
+    orc_sys s = *sp;
+    v_scal(n, 0.0, x);
+    double rr = s_dot(&s, n, r, r);
+    if (rr < 1.0e-60) { *num_iter = 0; *conv_ratio = sqrt(rr); return 1; }
+    const double inv_rr0 = 1.0 / rr;
+    double *p = (double *)calloc(n, sizeof(double));
+    double *Ap = (double *)calloc(n, sizeof(double));
+    memcpy(p, r, n * sizeof(double));
+    unsigned it;
+    for (it = 0; it < max_iter; it++) {
+        matvec_ls(&s, 1.0, p, 0.0, Ap);
+        const double alpha = rr / s_dot(&s, n, p, Ap);
+        v_axpy(n, alpha, p, x);
+        v_axpy(n, -alpha, Ap, r);
+        const double rr_new = s_dot(&s, n, r, r);
+        if (rr_new * inv_rr0 < tol * tol) { *conv_ratio = sqrt(rr_new * inv_rr0); break; }
+        const double beta = rr_new / rr;
+        rr = rr_new;
+        v_scal(n, beta, p);
+        v_axpy(n, 1.0, r, p);
+    }
+    *num_iter = it;
+    free(p); free(Ap);
+    return 1;
+}
+static int ls_bicg_core(orc_sys *sp, size_t n, double *r, double *x, double *conv_ratio, unsigned *num_iter)
+{
+    const unsigned max_iter = *num_iter;
+    const double tol = *conv_ratio;
+    orc_sys s = *sp;
+    v_scal(n, 0.0, x);
+    const double rr0 = s_dot(&s, n, r, r);
+    if (rr0 < 1.0e-60) { *conv_ratio = sqrt(rr0); *num_iter = 0; return 1; }
+    const double inv_rr0 = 1.0 / rr0;
+    double *vs = (double *)calloc(n, sizeof(double));
+    double *As = (double *)calloc(n, sizeof(double));
+    double *p = (double *)calloc(n, sizeof(double));
+    double *Ap = (double *)calloc(n, sizeof(double));
+    double *r2 = (double *)calloc(n, sizeof(double));
+    memcpy(r2, r, n * sizeof(double));
+    memcpy(p, r, n * sizeof(double));
+    double r_r2 = s_dot(&s, n, r, r2);
+    *num_iter = max_iter;
+    for (unsigned it = 1; it < max_iter; it++) {
+        matvec_ls(&s, 1.0, p, 0.0, Ap);
+        const double alpha = r_r2 / s_dot(&s, n, Ap, r2);
+        memcpy(vs, r, n * sizeof(double));
+        v_axpy(n, -alpha, Ap, vs);
+        matvec_ls(&s, 1.0, vs, 0.0, As);
+        const double den = s_dot(&s, n, As, As);
+        const double num = s_dot(&s, n, As, vs);
+        const double omega = num / den;
+        v_axpy(n, alpha, p, x);
+        v_axpy(n, omega, vs, x);
+        memcpy(r, vs, n * sizeof(double));
+        v_axpy(n, -omega, As, r);
+        const double sq = s_dot(&s, n, r, r) * inv_rr0;
+        if (sq < tol * tol) { *conv_ratio = sqrt(sq); *num_iter = it; break; }
+        const double tmp1 = s_dot(&s, n, r, r2);
+        const double beta = (tmp1 * alpha) / (r_r2 * omega);
+        r_r2 = tmp1;
+        v_scal(n, beta, p);
+        v_axpy(n, 1.0, r, p);
+        v_axpy(n, -beta * omega, Ap, p);
+    }
+    free(vs); free(As); free(p); free(Ap); free(r2);
+    return 1;
+}
+int orc_solve_ls(int kind, unsigned nb, unsigned b,
+                 const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                 double *r, double *x, double *conv_ratio, unsigned *num_iter)
+{
+    orc_sys s;
+    sys_init(&s, nb, b, rowptr, col, val, dia, NULL, NULL, NULL, NULL, NULL, NULL);
+    const int ret = kind == 0 ? ls_cg_core(&s, (size_t)nb * b, r, x, conv_ratio, num_iter)
+                              : ls_bicg_core(&s, (size_t)nb * b, r, x, conv_ratio, num_iter);
     sys_free(&s);
     return ret;
 }

@@ -56,6 +56,10 @@ long orc_iluk_symbolic(unsigned nb, const unsigned *rowptr, const unsigned *col,
                        int lev_fill, const int *order,
                        unsigned *frowptr, unsigned *fcol, int *flev, unsigned *fdiaind,
                        long cap);
+/* the same with fill blocks (AddFracPtn(lev, aBlkFill), natural order) */
+long orc_iluk_symbolic_blk(unsigned nb, const unsigned *rowptr, const unsigned *col, int lev_fill,
+                           const unsigned *fill_blk, unsigned nfill,
+                           unsigned *frowptr, unsigned *fcol, int *flev, unsigned *fdiaind, long cap);
 
 /* a8: copy A (optionally reordered) into the factor pattern, then factor.
  * orc_ilu_decomp returns the number of singular pivots (0..10), or -1 when
@@ -90,6 +94,11 @@ int orc_solve_pbicgstab(unsigned nb, unsigned b,
                         const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind,
                         const double *fval, const double *fdia, const int *order,
                         double *r, double *x, double *conv_ratio, unsigned *iteration);
+/* LsSol::Solve_CG (kind 0) / Solve_BiCGSTAB (kind 2), unpreconditioned
+ * (src/ls/solver_ls_iter.cpp:31-97, 177-286) */
+int orc_solve_ls(int kind, unsigned nb, unsigned b,
+                 const unsigned *rowptr, const unsigned *col, const double *val, const double *dia,
+                 double *r, double *x, double *conv_ratio, unsigned *num_iter);
 
 /* a17: MatVec::Sol::Solve_CG (kind 0) / Solve_PCG (1) / Solve_BiCGSTAB (2) /
  * Solve_PBiCGSTAB (3), src/matvec/solver_mat_iter.cpp; natural-order factor. */

@@ -66,6 +66,8 @@ class Orc:
         L.orc_matvec.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, C.c_double, vp, C.c_double, vp]
         L.orc_iluk_symbolic.restype = C.c_long
         L.orc_iluk_symbolic.argtypes = [C.c_uint, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_long]
+        L.orc_iluk_symbolic_blk.restype = C.c_long
+        L.orc_iluk_symbolic_blk.argtypes = [C.c_uint, vp, vp, C.c_int, vp, C.c_uint, vp, vp, vp, vp, C.c_long]
         L.orc_ilu_setvalue.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_ilu_decomp.restype = C.c_int
         L.orc_ilu_decomp.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp]
@@ -75,6 +77,9 @@ class Orc:
             f.restype = C.c_int
             f.argtypes = [C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                           C.POINTER(C.c_double), C.POINTER(C.c_uint)]
+        L.orc_solve_ls.restype = C.c_int
+        L.orc_solve_ls.argtypes = [C.c_int, C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_uint)]
         L.orc_solve_mat.restype = C.c_int
         L.orc_solve_mat.argtypes = [C.c_int, C.c_uint, C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                     C.POINTER(C.c_double), C.POINTER(C.c_uint)]
@@ -151,20 +156,28 @@ class Orc:
         self.L.orc_fem_pattern(nnode, nelem, npe, _p(conn), _p(rowptr), _p(col), n)
         return rowptr, col[:n]
 
-    def iluk_symbolic(self, rowptr, col, lev, order=None):
+    def iluk_symbolic(self, rowptr, col, lev, order=None, fill_blk=None):
+        """ILU(lev) pattern; fill_blk (natural order only): the block rows and
+        columns that take fill of any level (AddFracPtn(lev, aBlkFill))."""
         rowptr, col = u32(rowptr), u32(col)
         nb = rowptr.size - 1
         order = None if order is None else i32(order)
+        blk = None if fill_blk is None else u32(fill_blk)
+        assert blk is None or order is None
         cap = max(16, col.size * 4)
         while True:
             frow = np.zeros(nb + 1, dtype=np.uint32)
             fcol = np.zeros(cap, dtype=np.uint32)
             flev = np.zeros(cap, dtype=np.int32)
             fdia = np.zeros(max(1, nb), dtype=np.uint32)
-            n = self.L.orc_iluk_symbolic(nb, _p(rowptr), _p(col), lev, _p(order), _p(frow), _p(fcol), _p(flev),
-                                         _p(fdia), cap)
+            if blk is None:
+                n = self.L.orc_iluk_symbolic(nb, _p(rowptr), _p(col), lev, _p(order), _p(frow), _p(fcol), _p(flev),
+                                             _p(fdia), cap)
+            else:
+                n = self.L.orc_iluk_symbolic_blk(nb, _p(rowptr), _p(col), lev, _p(blk), blk.size, _p(frow), _p(fcol),
+                                                 _p(flev), _p(fdia), cap)
             if n == -1:
-                raise ValueError("unsorted pattern row")
+                raise ValueError("unsorted pattern row or fill block out of range")
             if n >= 0:
                 return dict(rowptr=frow, col=fcol[:n], lev=flev[:n], diaind=fdia[:nb])
             cap = -n
@@ -401,6 +414,17 @@ class Orc:
 
     MAT_KINDS = {"cg": 0, "pcg": 1, "bicgstab": 2, "pbicgstab": 3}
 
+    def solve_ls(self, kind, nb, b, rowptr, col, val, dia, rhs, tol, maxit):
+        """LsSol::Solve_CG ("cg") / Solve_BiCGSTAB ("bicgstab"), unpreconditioned:
+        returns (ok, conv, iter, x, r_final)."""
+        r = f64(rhs).copy()
+        x = np.zeros(nb * b)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.L.orc_solve_ls(self.MAT_KINDS[kind], nb, b, _p(u32(rowptr)), _p(u32(col)), _p(f64(val)),
+                                 _p(f64(dia)), _p(r), _p(x), C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, x, r
+
     def solve_mat(self, kind, nb, b, rowptr, col, val, dia, rhs, tol, maxit, F=None, fval=None, fdia=None):
         """MatVec::Sol::Solve_CG / Solve_PCG / Solve_BiCGSTAB / Solve_PBiCGSTAB
         (natural-order factor for the preconditioned kinds): returns
@@ -507,7 +531,9 @@ class Ref:
             "ref_prec_solve": (None, [vp, vp, C.c_int]),
             "ref_solve_pcg": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_solve_pbicgstab": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_ls_solve": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_frac_create": (vp, [vp, C.c_int, vp]),
+            "ref_frac_create_fill": (vp, [vp, C.c_int, vp, C.c_uint]),
             "ref_frac_free": (None, [vp]),
             "ref_frac_ncrs": (C.c_uint, [vp]),
             "ref_frac_get_pattern": (None, [vp, vp, vp, vp]),
@@ -624,6 +650,15 @@ class RefSystem:
     def resize_tmp(self, n):
         self.ref.L.ref_sys_resize_tmp(self.h, n)
 
+    def solve_ls(self, kind, rhs, tol, maxit):
+        """LsSol::Solve_CG ("cg") / Solve_BiCGSTAB ("bicgstab") on this system:
+        returns (ok, conv, iter, x, r_final)."""
+        self.set_vector(-1, rhs)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_ls_solve(self.h, Orc.MAT_KINDS[kind], C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, self.get_vector(-2), self.get_vector(-1)
+
     def solve_mat(self, kind, rhs, tol, maxit, frac=None):
         """MatVec::Sol::Solve_* on this system's matrix (frac: a natural-order
         RefFrac holding values, for the preconditioned kinds): returns
@@ -662,12 +697,18 @@ class RefILU:
 
 
 class RefFrac:
-    """MatVec::CMatDiaFrac_BlkCrs of the reference (natural or ordered ILU(k))."""
+    """MatVec::CMatDiaFrac_BlkCrs of the reference (natural or ordered ILU(k));
+    with fill_blk: CMatDiaFrac_BlkCrs(0, A) + AddFracPtn(lev, aBlkFill) +
+    MakePatternFinalize, as CPreconditioner_ILU builds it after SetFillBlk."""
 
-    def __init__(self, sysobj, lev, order=None):
+    def __init__(self, sysobj, lev, order=None, fill_blk=None):
         self.s = sysobj
         self._order = None if order is None else i32(order)
-        self.h = sysobj.ref.L.ref_frac_create(sysobj.h, lev, _p(self._order))
+        if fill_blk is None:
+            self.h = sysobj.ref.L.ref_frac_create(sysobj.h, lev, _p(self._order))
+        else:
+            blk = u32(fill_blk)
+            self.h = sysobj.ref.L.ref_frac_create_fill(sysobj.h, lev, _p(blk), blk.size)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -265,6 +265,12 @@ int ref_solve_pcg(void* p, void* q, double* conv, unsigned* iter) {
   LsSol::CLinearSystemPreconditioner lsp(static_cast<RefSys*>(p)->ls, *static_cast<LsSol::CPreconditioner_ILU*>(q));
   return LsSol::Solve_PCG(*conv, *iter, lsp) ? 1 : 0;
 }
+// LsSol::Solve_CG (kind 0) / Solve_BiCGSTAB (kind 2) on the system itself
+int ref_ls_solve(void* p, int kind, double* conv, unsigned* iter) {
+  LsSol::CLinearSystem& ls = static_cast<RefSys*>(p)->ls;
+  const bool ok = kind == 0 ? LsSol::Solve_CG(*conv, *iter, ls) : LsSol::Solve_BiCGSTAB(*conv, *iter, ls);
+  return ok ? 1 : 0;
+}
 int ref_solve_pbicgstab(void* p, void* q, double* conv, unsigned* iter) {
   LsSol::CLinearSystemPreconditioner lsp(static_cast<RefSys*>(p)->ls, *static_cast<LsSol::CPreconditioner_ILU*>(q));
   return LsSol::Solve_PBiCGSTAB(*conv, *iter, lsp) ? 1 : 0;
@@ -283,6 +289,18 @@ void* ref_frac_create(void* p, int lev, const int* order) {
   } else {
     rf->f = new FracPeek(lev, s->ls.GetMatrix(0));
   }
+  return rf;
+}
+// CPreconditioner_ILU after SetFillBlk(blk) on one segment (preconditioner.cpp:88-90,
+// 149-150, 198-210): ILU(0) pattern, AddFracPtn(lev, blk), MakePatternFinalize
+void* ref_frac_create_fill(void* p, int lev, const unsigned* blk, unsigned nblk) {
+  RefSys* s = static_cast<RefSys*>(p);
+  RefFrac* rf = new RefFrac;
+  rf->nb = s->nb;
+  rf->b = s->b;
+  rf->f = new FracPeek(0, s->ls.GetMatrix(0));
+  rf->f->AddFracPtn(lev, std::vector<unsigned>(blk, blk + nblk));
+  rf->f->MakePatternFinalize();
   return rf;
 }
 void ref_frac_free(void* q) {

@@ -168,12 +168,12 @@ def test_cfg4_oseen_full_size(orc, gpu):
     gv, gd = M.values()
     assert ns == 0 and np.array_equal(gv, fv) and np.array_equal(gd, fd), "ILU(1) factor differs"
     rhs = ref["res"]
-    ok_t, _, _, _, x_t = device_solve(gpu, s, M, rhs, "bicg", 1e-12, 20000)
-    assert ok_t and true_relres(s.A, x_t, rhs) <= 1e-10
+    _, _, _, info_t, x_t = device_solve(gpu, s, M, rhs, "bicg", 1e-12, 20000)
+    assert info_t["converged"] and true_relres(s.A, x_t, rhs) <= 1e-10
     for prec in ("fp64", "fac32"):
         ok, conv, it, info, x = device_solve(gpu, s, M, rhs, "bicg", 1e-9, 20000, prec)
         rr = true_relres(s.A, x, rhs)
-        assert ok and it < 20000 and rr <= 2e-9, (prec, rr)
+        assert ok and info["converged"] and rr <= 2e-9, (prec, rr)
         assert relerr(x, x_t) <= 1e-5, prec
 
 

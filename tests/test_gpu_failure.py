@@ -118,12 +118,12 @@ def test_max_iter_return(orc, gpu, poisson, kind):
     x = torch.zeros_like(r)
     fn = gpu.Solve_PCG if kind == "pcg" else gpu.Solve_PBiCGSTAB
     tol, maxit = 1e-14, 5
-    ok, conv, it, _ = fn(tol, maxit, Mdev, ilu, r, x)
+    ok, conv, it, info = fn(tol, maxit, Mdev, ilu, r, x)
     F = orc.iluk_symbolic(A["rowptr"], A["col"], 0)
     fv, fd, _ = orc.ilu_values(prob.nnode, 1, A["rowptr"], A["col"], A["val"], A["dia"], F)
     ok_o, conv_o, it_o, x_o, _ = orc.solve(kind, prob.nnode, 1, A["rowptr"], A["col"], A["val"], A["dia"], F, fv, fd,
                                            A["res"], tol, maxit)
-    assert it == it_o == maxit
+    assert it == it_o == maxit and not info["converged"]
     if kind == "pcg":
         assert not ok and not ok_o
         assert abs(conv - conv_o) <= 1e-8 * conv_o   # the ratio reached, reductions in another order

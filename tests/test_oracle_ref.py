@@ -289,3 +289,92 @@ def test_singular_pivot_stop_state(orc, ref, b, nzero, ordered):
     fv, fd, ns = orc.ilu_values(nb, b, A["rowptr"], A["col"], val, dia, F, order)
     assert ok_r == (nzero <= 10) and (ns == nzero if nzero <= 10 else ns == -1)
     assert np.array_equal(fv, fv_r) and np.array_equal(fd, fd_r)
+
+
+@pytest.mark.parametrize("kind", ["cg", "bicgstab"])
+@pytest.mark.parametrize("case", ["converge", "max_iter", "zero_rhs"])
+@pytest.mark.parametrize("name", ["poisson", "oseen"])
+def test_ls_unpreconditioned_solvers(orc, ref, kind, case, name):
+    """LsSol::Solve_CG / Solve_BiCGSTAB (src/ls/solver_ls_iter.cpp:31-97,
+    177-286) of the oracle against the compiled reference: flag, count,
+    conv_ratio and both vectors bit for bit."""
+    if name == "oseen" and kind == "cg":
+        pytest.skip("CG needs a symmetric matrix")
+    prob = meshgen.poisson2d(9, seed=4) if name == "poisson" else meshgen.oseen2d(6, seed=4)
+    A = assemble_problem(orc, prob)
+    rs = RefSystem(ref, prob.nnode, prob.b, A["rowptr"], A["col"])
+    rs.set_matrix(A["val"], A["dia"])
+    rhs = A["res"] * (0.0 if case == "zero_rhs" else 1.0)
+    tol, maxit = (1e-14, 6) if case == "max_iter" else (1e-10, 2000)
+    got = orc.solve_ls(kind, prob.nnode, prob.b, A["rowptr"], A["col"], A["val"], A["dia"], rhs, tol, maxit)
+    want = rs.solve_ls(kind, rhs, tol, maxit)
+    assert got[:3] == want[:3]
+    assert np.array_equal(got[3], want[3]) and np.array_equal(got[4], want[4])
+    if case == "converge":
+        assert got[1] < tol
+    if case == "max_iter":
+        assert got[2] == maxit and got[1] == tol
+
+
+@pytest.mark.parametrize("lev", [0, 1, 2, -1])
+@pytest.mark.parametrize("name", ["poisson", "solid", "graph"])
+def test_fill_blocks(orc, ref, lev, name):
+    """CPreconditioner_ILU::SetFillBlk -> AddFracPtn(lev, aBlkFill)
+    (src/matvec/matdiafrac_blkcrs.cpp:1379-1516): the listed block rows and
+    columns take fill of any level.  Pattern and factor values of the oracle
+    against the compiled reference, and the pattern's defining properties:
+    it contains the plain ILU(lev) pattern, it equals it when no block is
+    listed, and with every block listed it is the unlimited-fill pattern.
+    It is not a superset of ILU(lev) in general: operands are not filtered by
+    their level, so an entry can first enter a listed row at a higher level
+    than ILU(lev) gives it, and the first level is the one kept -- which then
+    blocks fill below (the reference's rule, seen here with lev = 2)."""
+    if name == "graph":
+        rng = np.random.default_rng(11)
+        n = 90
+        adj = [set() for _ in range(n)]
+        for i in range(n):
+            for _ in range(2):
+                j = int(rng.integers(n))
+                if j != i:
+                    adj[i].add(j)
+                    adj[j].add(i)
+        rowptr = np.concatenate([[0], np.cumsum([len(a) for a in adj])]).astype(np.uint32)
+        col = np.array([j for a in adj for j in sorted(a)], dtype=np.uint32)
+        nb, b = n, 1
+        val = rng.standard_normal(col.size)
+        dia = 10.0 + rng.random(n)
+    else:
+        prob = meshgen.poisson2d(8, seed=2) if name == "poisson" else meshgen.solid3d(3, seed=2, sigma=0.3)
+        A = assemble_problem(orc, prob)
+        rowptr, col, val, dia, nb, b = A["rowptr"], A["col"], A["val"], A["dia"], prob.nnode, prob.b
+    blk = np.array(sorted({3, nb // 2, nb - 2}), dtype=np.uint32)
+    rs = RefSystem(ref, nb, b, rowptr, col)
+    rs.set_matrix(val, dia)
+    rf = RefFrac(rs, lev, fill_blk=blk)
+    P = rf.pattern()
+    F = orc.iluk_symbolic(rowptr, col, lev, fill_blk=blk)
+    for k in ("rowptr", "col", "diaind"):
+        assert np.array_equal(P[k], F[k]), k
+    assert rf.set_value() is True
+    fv_r, fd_r = rf.values()
+    fv, fd, ns = orc.ilu_values(nb, b, rowptr, col, val, dia, F)
+    assert ns == 0 and np.array_equal(fv, fv_r) and np.array_equal(fd, fd_r)
+    # properties that do not rest on the reference
+    plain = orc.iluk_symbolic(rowptr, col, lev)
+    full = orc.iluk_symbolic(rowptr, col, -1)
+    as_set = lambda G: {(i, int(c)) for i in range(nb) for c in G["col"][G["rowptr"][i]:G["rowptr"][i + 1]]}
+    assert as_set(F) <= as_set(full)
+    if lev in (0, -1):
+        assert as_set(plain) <= as_set(F)
+    empty = orc.iluk_symbolic(rowptr, col, lev, fill_blk=np.zeros(0, dtype=np.uint32))
+    assert np.array_equal(empty["col"], plain["col"])
+    every = orc.iluk_symbolic(rowptr, col, lev, fill_blk=np.arange(nb, dtype=np.uint32))
+    assert np.array_equal(every["col"], full["col"])
+    if lev == 0:  # a listed row gains its whole unlimited-fill row; an unlisted row only fill in listed columns
+        for i in range(nb):
+            row = set(F["col"][F["rowptr"][i]:F["rowptr"][i + 1]].tolist())
+            rp = set(plain["col"][plain["rowptr"][i]:plain["rowptr"][i + 1]].tolist())
+            extra = row - rp
+            if i not in blk:
+                assert extra <= set(blk.tolist()), i
