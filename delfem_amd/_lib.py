@@ -109,6 +109,7 @@ _SIGS = {
     "dfm_ilu_forward": (C.c_int, [H, vp]),
     "dfm_ilu_backward": (C.c_int, [H, vp]),
     "dfm_ilu_solve_precond": (C.c_int, [H, vp]),
+    "dfm_set_sweep_mode": (C.c_int, [C.c_int, C.c_int]),
     "dfm_solve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_solve_pbicgstab": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_bmat_create": (C.c_int, [C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.POINTER(H)]),

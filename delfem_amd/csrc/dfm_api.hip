@@ -529,6 +529,26 @@ int dfm_assemble_helmholtz2d_p1(dfm_asm* p, const double* coord, const double* a
 
 // ---------------------------------------------------------------------------
 // ILU
+// slice -> level map and zeroed counters of a plan's persistent sweep
+static int build_sweep_sync(const SellPlan& plan, SweepSync& ss) {
+  const int ng = (int)plan.group_slice_ptr.size() - 1;
+  std::vector<int> slev((size_t)std::max(1, plan.nslice), 0);
+  int maxlev = 0;
+  for (int g = 0; g < ng; g++) {
+    const int s0 = plan.group_slice_ptr[g], s1 = plan.group_slice_ptr[g + 1];
+    for (int sl = s0; sl < s1; sl++) slev[sl] = g;
+    maxlev = std::max(maxlev, s1 - s0);
+  }
+  ss.nslice = plan.nslice;
+  ss.ngroup = ng;
+  ss.maxlev = maxlev;
+  DFM_TRY(ss.slev.upload(slev));
+  DFM_TRY(ss.gsp.upload(plan.group_slice_ptr));
+  DFM_TRY(ss.sync.alloc((size_t)ng + 2));
+  DFM_HIP(hipMemsetAsync(ss.sync.p, 0, sizeof(int) * ((size_t)ng + 2), g_stream));
+  return DFM_OK;
+}
+
 static int upload_sell_plan(const SellPlan& plan, int bb, Sell<double>& s64, DevBuf<long long>& dsrc) {
   s64.nslice = plan.nslice;
   s64.nk = plan.nk;
@@ -703,6 +723,8 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
     DFM_TRY(M->d_fixrow.alloc(1));
     DFM_TRY(upload_sell_plan(M->planL, (int)bb, M->L64, M->d_srcL));
     DFM_TRY(upload_sell_plan(M->planU, (int)bb, M->U64, M->d_srcU));
+    DFM_TRY(build_sweep_sync(M->planL, M->syncL));
+    DFM_TRY(build_sweep_sync(M->planU, M->syncU));
     DFM_TRY(upload_sell_plan(M->planA, (int)bb, M->A64, M->d_srcA));
     DFM_TRY(M->state.alloc(1));
     DFM_HIP(hipHostMalloc((void**)&M->h_state, sizeof(KState), hipHostMallocDefault));

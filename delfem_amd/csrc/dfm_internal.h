@@ -151,6 +151,15 @@ struct KState {
   double conv_rr;       // rr at convergence
   int done;             // 0 running, 1 converged (recursive residual), 2 stopped at maxit
   int iter, conv_iter, maxit;
+  int sweep_fault;      // a persistent sweep gave up waiting for a level (see k_trsv_persist)
+};
+
+// Work and level counters of the persistent sweep over one sliced-ELL plan:
+// slev[slice] = its level, gsp = the plan's group_slice_ptr, sync = work
+// counter, exit counter, one done-count per level (kept zero between launches).
+struct SweepSync {
+  DevBuf<int> slev, gsp, sync;
+  int nslice = 0, ngroup = 0, maxlev = 0;  // maxlev: slices of the largest level
 };
 
 // dfm_ilu_create, or (given_row / given_col, natural order) a factor on a
@@ -225,6 +234,7 @@ struct dfm_ilu_s {
   dfm::DevBuf<unsigned char> d_rzmaskL; // per forward-sweep slice lane: row lies in the first backward level
   dfm::DevBuf<long long> d_srcL, d_srcU;
   dfm::Sell<double> L64, U64;
+  dfm::SweepSync syncL, syncU;          // persistent-sweep counters of planL / planU
   dfm::Sell<float> L32, U32;
   dfm::Sell<dfm::bf16s> L16, U16;
   bool have32 = false, have16 = false;

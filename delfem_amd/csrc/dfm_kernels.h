@@ -1153,6 +1153,115 @@ __global__ void __launch_bounds__(kTrsvNT) k_trsv_upper(int s0, int s1, const in
   if constexpr (DOT) write_partials<1, kTrsvNT>(dsum, partials, part0 + blockIdx.x, nparts);
 }
 
+// ---------------------------------------------------------------------------
+// Persistent sweeps: every level set of a sweep in one launch, for factors
+// with many small levels (ILU(k >= 1) under a block ordering), where a launch
+// per level costs more than the level's work.  One-wave workgroups take
+// 64-row slices in level order from a work counter (sync[0]).  A wave first
+// loads what no earlier level writes -- its rows' right-hand side, the
+// columns and blocks of its first kPersistPF slots and its diagonal block --
+// then waits until all slices of the previous level are done (the count
+// sync[2 + g - 1] reaching that level's slice count), gathers, solves, and
+// counts its slice (sync[2 + g]).  Slices are taken in level order by waves
+// that are running, so a wave waits only on slices held by running waves:
+// the sweep cannot deadlock whatever the grid and residency.  The wait is
+// bounded: after kPersistMaxPolls polls the wave records st->sweep_fault and
+// goes on (and every other wave stops waiting when it sees the fault), so
+// the kernel always ends and the host reports the fault.  The
+// count is published after an agent-scope release fence and read before an
+// agent-scope acquire fence (the XCDs' L2 caches are not coherent with each
+// other).  The last wave to leave (sync[1] counts them) resets the counters
+// for the next launch.  Slots are subtracted in the same order as in the
+// per-level kernels, so FP64 results are bitwise the same.
+constexpr int kPersistPF = 4;
+constexpr int kPersistMaxPolls = 1 << 18;
+
+template <int B, typename TS, typename T, bool UPPER, bool DOT>
+__global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, const int* __restrict__ slev,
+                                                     const int* __restrict__ gsp, const int* __restrict__ rows,
+                                                     const int* __restrict__ sptr, const int* __restrict__ col,
+                                                     const TS* __restrict__ val, const TS* __restrict__ dval,
+                                                     const T* rhs, T* out, const T* __restrict__ rvec,
+                                                     double* __restrict__ partials, KState* __restrict__ st,
+                                                     int* __restrict__ sync) {
+  constexpr int BB = B * B;
+  const int lane = threadIdx.x;
+  double dsum[1] = {0.0};
+  if (!st->done) {
+    for (;;) {
+      int s = 0;
+      if (lane == 0) s = __hip_atomic_fetch_add(&sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s = __shfl(s, 0, 64);
+      if (s >= nslice) break;
+      const int g = slev[s];
+      const int row = rows[(size_t)s * 64 + lane];
+      const int k0 = sptr[s], k1 = sptr[s + 1];
+      T t[B], d[BB], a[kPersistPF][BB];
+      int c[kPersistPF];
+      if (row >= 0) {
+#pragma unroll
+        for (int e = 0; e < B; e++) t[e] = rhs[(size_t)row * B + e];
+#pragma unroll
+        for (int u = 0; u < kPersistPF; u++) {
+          c[u] = -1;
+          if (k0 + u < k1) {
+            c[u] = ld_stream(col + (size_t)(k0 + u) * 64 + lane);
+            load_block<BB, TS, T>(val, (size_t)(k0 + u), lane, a[u]);
+          }
+        }
+        if constexpr (!UPPER) load_block<BB, TS, T>(dval, (size_t)s, lane, d);
+      }
+      if (g > 0) {
+        const int need = gsp[g] - gsp[g - 1];
+        int polls = 0;
+        while (__hip_atomic_load(&sync[1 + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          if (++polls > kPersistMaxPolls) {
+            __hip_atomic_store(&st->sweep_fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          // once any wave has given up, the others stop waiting too
+          if (__hip_atomic_load(&st->sweep_fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+          __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      if (row >= 0) {
+#pragma unroll
+        for (int u = 0; u < kPersistPF; u++) {
+          if (k0 + u < k1) {
+            T xv[B];
+            const size_t cb = (size_t)(c[u] >= 0 ? c[u] : 0) * B;
+#pragma unroll
+            for (int e = 0; e < B; e++) xv[e] = c[u] >= 0 ? out[cb + e] : (T)0;
+            blk_msub<B, T>(t, a[u], xv);
+          }
+        }
+        if (k0 + kPersistPF < k1) sell_msub_range<B, TS, T>(col, val, out, k0 + kPersistPF, k1, lane, t);
+        if constexpr (UPPER) {
+#pragma unroll
+          for (int e = 0; e < B; e++) {
+            out[(size_t)row * B + e] = t[e];
+            if (DOT) dsum[0] += (double)rvec[(size_t)row * B + e] * (double)t[e];
+          }
+        } else {
+          T y[B];
+          blk_mul<B, T>(y, d, t);
+#pragma unroll
+          for (int e = 0; e < B; e++) out[(size_t)row * B + e] = y[e];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (lane == 0) __hip_atomic_fetch_add(&sync[2 + g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if constexpr (DOT) write_partials<1, 64>(dsum, partials, blockIdx.x, 0);
+  if (lane == 0) {
+    const int left = __hip_atomic_fetch_add(&sync[1], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (left == (int)gridDim.x - 1)
+      for (int i = 0; i < ngroup + 2; i++) __hip_atomic_store(&sync[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Forward sweep of one level fused with the PCG update that precedes it
 // (solver_ls_iter.cpp:140-145, then :158): for each row of the level
 //   r_i += (-alpha) Ap_i ; x_i += alpha p_i ; z_i = Dinv_i (r_i - sum L_ij z_j).

@@ -116,8 +116,38 @@ int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32, bool want16
 // ---------------------------------------------------------------------------
 // Level-set sweeps, one launch per level.  `plan` carries the layout (rows,
 // slices, columns) of the FP64 copy; `L`/`U` the values in storage type TS.
+// Persistent sweeps (k_trsv_persist) serve plans with many levels; they need
+// the solver state (for the stop flag and the fault report).  DFM_PERSIST=0
+// turns them off, =2 uses them for every plan; by default a plan of at least
+// kPersistMinLevels levels takes them.  Grid: two waves per slice of the
+// largest level (waves of later levels load their factor slots while the
+// level before them finishes), at least 64; DFM_PERSIST_GRID overrides it.
+constexpr int kPersistMinLevels = 8;
+constexpr int kPersistMaxGrid = 4096;
+static int env_int(const char* name, int dflt) {
+  const char* e = getenv(name);
+  return e ? atoi(e) : dflt;
+}
+static int g_persist_mode = env_int("DFM_PERSIST", 1);      // dfm_set_sweep_mode
+static int g_persist_grid = env_int("DFM_PERSIST_GRID", 0);
+static bool use_persist(const SweepSync* ps, const KState* st) {
+  if (!ps || !st || ps->nslice == 0) return false;
+  return g_persist_mode == 2 || (g_persist_mode == 1 && ps->ngroup >= kPersistMinLevels);
+}
+static int persist_grid(const SweepSync& ps) {
+  const int g = g_persist_grid > 0 ? g_persist_grid : std::max(64, 2 * ps.maxlev);
+  return std::max(1, std::min(std::min(g, ps.nslice), kPersistMaxGrid));
+}
+
 template <int B, typename TS, typename T>
-static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs, T* out, const KState* st) {
+static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs, T* out, KState* st,
+                       const SweepSync* ps = nullptr) {
+  if (use_persist(ps, st)) {
+    DFM_LAUNCH_NT((k_trsv_persist<B, TS, T, false, false>), persist_grid(*ps), 64, ps->nslice, ps->ngroup,
+                  ps->slev.p, ps->gsp.p, plan.rows.p, plan.slice_ptr.p, plan.col.p, L.val.p, L.dval.p, rhs, out,
+                  nullptr, nullptr, st, ps->sync.p);
+    return DFM_OK;
+  }
   const int ng = (int)plan.group_slice_ptr.size() - 1;
   for (int g = 0; g < ng; g++) {
     const int s0 = plan.group_slice_ptr[g], s1 = plan.group_slice_ptr[g + 1];
@@ -127,7 +157,8 @@ static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs
   return DFM_OK;
 }
 // Number of partial sums an upper sweep with DOT leaves for the finalize kernel.
-static int upper_nparts(const Sell<double>& plan) {
+static int upper_nparts(const Sell<double>& plan, const SweepSync* ps = nullptr, const KState* st = nullptr) {
+  if (use_persist(ps, st)) return persist_grid(*ps);
   int n = 0;
   const int ng = (int)plan.group_slice_ptr.size() - 1;
   for (int g = 0; g < ng; g++) n += grid_for(plan.group_slice_ptr[g + 1] - plan.group_slice_ptr[g], kTrsvWaves);
@@ -137,7 +168,13 @@ static int upper_nparts(const Sell<double>& plan) {
 // left in partials[0 .. upper_nparts) for a finalize kernel.
 template <int B, typename TS, typename T, bool DOT>
 static int sweep_upper(const Sell<TS>& U, const Sell<double>& plan, T* vec, const T* rvec, double* partials,
-                       const KState* st) {
+                       KState* st, const SweepSync* ps = nullptr) {
+  if (use_persist(ps, st)) {
+    DFM_LAUNCH_NT((k_trsv_persist<B, TS, T, true, DOT>), persist_grid(*ps), 64, ps->nslice, ps->ngroup, ps->slev.p,
+                  ps->gsp.p, plan.rows.p, plan.slice_ptr.p, plan.col.p, U.val.p, U.val.p, vec, vec, rvec, partials,
+                  st, ps->sync.p);
+    return DFM_OK;
+  }
   const int ng = (int)plan.group_slice_ptr.size() - 1;
   const int nparts = DOT ? upper_nparts(plan) : 0;
   int part0 = 0;
@@ -186,6 +223,8 @@ struct Ops {
   const Sell<double>* planL;
   const Sell<double>* planU;
   int nb;
+  const SweepSync* syncL;
+  const SweepSync* syncU;
 };
 template <typename TS>
 static const Sell<TS>& sel(const Sell<double>& s64, const Sell<float>& s32, const Sell<bf16s>& s16) {
@@ -197,7 +236,7 @@ template <typename TA, typename TF>
 static Ops<TA, TF> ops_of(dfm_ilu_s* M) {
   static const Sell<bf16s> none{};
   return {&sel<TA>(M->A64, M->A32, none), &sel<TF>(M->L64, M->L32, M->L16), &sel<TF>(M->U64, M->U32, M->U16),
-          &M->A64, &M->L64, &M->U64, (int)M->nb};
+          &M->A64, &M->L64, &M->U64, (int)M->nb, &M->syncL, &M->syncU};
 }
 
 template <int B, typename TA, typename TF, typename T, bool DOTS, bool GATE>
@@ -213,6 +252,8 @@ static int spmv_nparts(const Sell<double>& planA) { return grid_for(planA.nslice
 static int read_state(dfm_ilu_s* M) {
   DFM_HIP(hipMemcpyAsync(M->h_state, M->state.p, sizeof(KState), hipMemcpyDeviceToHost, stream()));
   DFM_HIP(hipStreamSynchronize(stream()));
+  DFM_CHECK(!M->h_state->sweep_fault, DFM_HIP_ERROR,
+            "a persistent triangular sweep gave up waiting for a level set (results invalid)");
   return DFM_OK;
 }
 static int write_state(dfm_ilu_s* M) {
@@ -278,7 +319,9 @@ static int run_block(dfm_ilu_s* M, const std::string& key, int nblock, Body&& bo
 }
 
 static std::string graph_key(const std::string& tag, int B, int nblock, std::initializer_list<const void*> ptrs) {
-  std::string k = tag + ":" + std::to_string(B) + ":" + std::to_string(nblock);
+  // the sweep mode is part of what a graph captured
+  std::string k = tag + ":" + std::to_string(B) + ":" + std::to_string(nblock) + ":p" +
+                  std::to_string(g_persist_mode) + "." + std::to_string(g_persist_grid);
   char buf[32];
   for (const void* p : ptrs) {
     snprintf(buf, sizeof buf, ":%p", p);
@@ -363,9 +406,9 @@ template <int B, typename TA, typename TF, typename T>
 static int pcg_start(dfm_ilu_s* M, const Ops<TA, TF>& o, T* r, T* p, T* z) {
   const long n = (long)o.nb * B;
   KState* st = M->state.p;
-  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, r, z, st)));
-  DFM_TRY((sweep_upper<B, TF, T, true>(*o.U, *o.planU, z, r, M->partials.p, st)));
-  DFM_LAUNCH_FIN(1, FIN_PCG_RZ0, M->partials.p, upper_nparts(*o.planU), st);
+  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, r, z, st, o.syncL)));
+  DFM_TRY((sweep_upper<B, TF, T, true>(*o.U, *o.planU, z, r, M->partials.p, st, o.syncU)));
+  DFM_LAUNCH_FIN(1, FIN_PCG_RZ0, M->partials.p, upper_nparts(*o.planU, o.syncU, st), st);
   DFM_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
   return DFM_OK;
 }
@@ -378,13 +421,13 @@ static int bicg_body(dfm_ilu_s* M, const Ops<TA, TF>& o, T* r, T* x, T* w) {
   KState* st = M->state.p;
   double* part = M->partials.p;
   const int gv = grid_vec(n);
-  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, p, Mp, st)));
-  DFM_TRY((sweep_upper<B, TF, T, false>(*o.U, *o.planU, Mp, nullptr, nullptr, st)));
+  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, p, Mp, st, o.syncL)));
+  DFM_TRY((sweep_upper<B, TF, T, false>(*o.U, *o.planU, Mp, nullptr, nullptr, st, o.syncU)));
   DFM_TRY((spmv<B, TA, TF, T, true, true>(o, Mp, AMp, nullptr, r2, r, r2, part, st)));
   DFM_LAUNCH_FIN(3, FIN_BICG_ALPHA, part, spmv_nparts(*o.planA), st);
   DFM_LAUNCH(k_bicg_s<T>, gv, n, r, AMp, s, st);
-  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, s, Ms, st)));
-  DFM_TRY((sweep_upper<B, TF, T, false>(*o.U, *o.planU, Ms, nullptr, nullptr, st)));
+  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, s, Ms, st, o.syncL)));
+  DFM_TRY((sweep_upper<B, TF, T, false>(*o.U, *o.planU, Ms, nullptr, nullptr, st, o.syncU)));
   DFM_TRY((spmv<B, TA, TF, T, true, true>(o, Ms, AMs, nullptr, s, nullptr, nullptr, part, st)));
   DFM_LAUNCH_FIN(2, FIN_BICG_OMEGA, part, spmv_nparts(*o.planA), st);
   DFM_LAUNCH(k_bicg_xr<T>, gv, n, Mp, Ms, s, AMs, r2, x, r, part, st);
@@ -578,7 +621,7 @@ static int ensure_workspace(dfm_ilu_s* M, size_t bytes) {
   need = std::max<size_t>(need, (size_t)spmv_nparts(M->A64));
   need = std::max<size_t>(need, (size_t)upper_nparts(M->U64));
   need = std::max<size_t>(need, (size_t)lower_nparts(M->L64));
-  need = std::max<size_t>(need, (size_t)2048);
+  need = std::max<size_t>(need, (size_t)kPersistMaxGrid);
   need = 3 * need + 64;
   if (M->partials.n < need) DFM_TRY(M->partials.alloc(need));
   return DFM_OK;
@@ -717,8 +760,8 @@ static int bench_kernel_t(dfm_bcrs_s* A, dfm_ilu_s* M, int which, int nrep, doub
   double* part = M->partials.p;
   auto run = [&]() -> int {
     if (which == 0) return spmv<B, TA, TF, T, true, true>(o, xv.p, yv.p, nullptr, xv.p, nullptr, nullptr, part, st);
-    if (which == 1) return sweep_lower<B, TF, T>(*o.L, *o.planL, xv.p, yv.p, st);
-    return sweep_upper<B, TF, T, true>(*o.U, *o.planU, yv.p, xv.p, part, st);
+    if (which == 1) return sweep_lower<B, TF, T>(*o.L, *o.planL, xv.p, yv.p, st, o.syncL);
+    return sweep_upper<B, TF, T, true>(*o.U, *o.planU, yv.p, xv.p, part, st, o.syncU);
   };
   // algorithmic bytes: values, column indices, row ids (sweeps), each vector
   // element read or written once (x of SpMV and the dot operand are the same
@@ -739,7 +782,7 @@ static int bench_kernel_t(dfm_bcrs_s* A, dfm_ilu_s* M, int which, int nrep, doub
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *avg_ms = (double)ms / nrep;
-  return DFM_OK;
+  return read_state(M);  // reports a persistent sweep's fault
 }
 
 template <int B>
@@ -824,6 +867,14 @@ int dfm_bench_read(size_t bytes, int nrep, double* avg_ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *avg_ms = (double)ms / nrep;
+  return DFM_OK;
+}
+
+int dfm_set_sweep_mode(int mode, int grid) {
+  DFM_CHECK(mode >= 0 && mode <= 2 && grid >= 0 && grid <= dfm::kPersistMaxGrid, DFM_BAD_ARG,
+            "bad sweep mode %d / grid %d", mode, grid);
+  dfm::g_persist_mode = mode;
+  dfm::g_persist_grid = grid;
   return DFM_OK;
 }
 

@@ -307,6 +307,13 @@ def Solve_PBiCGSTAB(conv_ratio, iteration, A, prec, r, x, precision="fp64", **op
     return True, conv.value, it.value, info
 
 
+def set_sweep_mode(mode, grid=0):
+    """Level sets of the solvers' triangular sweeps: mode 0 one launch per
+    level, 1 (default) one persistent launch for factors of 8 or more levels,
+    2 persistent always; grid = workgroups of a persistent launch (0: automatic)."""
+    call("dfm_set_sweep_mode", int(mode), int(grid))
+
+
 def dot(x, y):
     _sync_torch()
     out = C.c_double()

@@ -243,6 +243,13 @@ typedef struct {
  * opts may be NULL (FP64).  Returns DFM_OK (converged) or DFM_NOT_CONVERGED. */
 int dfm_solve_pcg(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
                   double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
+/* How the solvers run the level sets of a triangular sweep (no reference
+ * counterpart; a tuning knob): mode 0 one launch per level, 1 (default) one
+ * persistent launch for factors of 8 or more levels, 2 persistent always;
+ * grid = workgroups of a persistent launch (0: twice the slices of the
+ * largest level, at least 64; at most 4096).  FP64 sweeps give the same bits
+ * either way.  Process-wide; initial values from DFM_PERSIST / DFM_PERSIST_GRID. */
+int dfm_set_sweep_mode(int mode, int grid);
 int dfm_solve_pbicgstab(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
                         double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
 
