@@ -29,6 +29,7 @@ void set_error(const char* fmt, ...) {
 
 hipStream_t stream() { return g_stream; }
 
+
 int ensure_init() {
   if (g_inited) return DFM_OK;
   int dev = 0;
@@ -544,8 +545,9 @@ static int build_sweep_sync(const SellPlan& plan, SweepSync& ss) {
   ss.maxlev = maxlev;
   DFM_TRY(ss.slev.upload(slev));
   DFM_TRY(ss.gsp.upload(plan.group_slice_ptr));
-  DFM_TRY(ss.sync.alloc((size_t)ng + 2));
-  DFM_HIP(hipMemsetAsync(ss.sync.p, 0, sizeof(int) * ((size_t)ng + 2), g_stream));
+  const size_t nsync = ((size_t)ng + 2) * kSyncStride;
+  DFM_TRY(ss.sync.alloc(nsync));
+  DFM_HIP(hipMemsetAsync(ss.sync.p, 0, sizeof(int) * nsync, g_stream));
   return DFM_OK;
 }
 

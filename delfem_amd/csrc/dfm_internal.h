@@ -156,7 +156,8 @@ struct KState {
 
 // Work and level counters of the persistent sweep over one sliced-ELL plan:
 // slev[slice] = its level, gsp = the plan's group_slice_ptr, sync = work
-// counter, exit counter, one done-count per level (kept zero between launches).
+// counter, exit counter, one done-count per level, each on its own 256-byte
+// line (kept zero between launches).
 struct SweepSync {
   DevBuf<int> slev, gsp, sync;
   int nslice = 0, ngroup = 0, maxlev = 0;  // maxlev: slices of the largest level

@@ -204,13 +204,13 @@ __host__ __device__ inline size_t slot_index(size_t k, int bb, int e, int lane) 
 }
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-// All BB values of one block slot, converted to TV (non-temporal loads: the
-// matrix streams once per use, the gathered vectors keep the caches).
-template <int BB, typename TS, typename TV>
-__device__ inline void load_block(const TS* __restrict__ val, size_t k, int lane, TV (&a)[BB]) {
+// The E raw 32-bit words of one lane's block in slot k (non-temporal loads:
+// the matrix streams once per use, the gathered vectors keep the caches).
+template <int BB, typename TS>
+__device__ inline void load_words(const TS* __restrict__ val, size_t k, int lane,
+                                  unsigned (&w)[slot_words<TS>(BB)]) {
   constexpr int E = slot_words<TS>(BB), Q = E / 4, R = E % 4;
   const unsigned* base = reinterpret_cast<const unsigned*>(val) + k * 64 * E;
-  unsigned w[E];
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(base + ((size_t)q * 64 + lane) * 4));
@@ -228,6 +228,10 @@ __device__ inline void load_block(const TS* __restrict__ val, size_t k, int lane
 #pragma unroll
     for (int r = 0; r < R; r++) w[4 * Q + r] = __builtin_nontemporal_load(rem + r);
   }
+}
+// ... and those words as BB values of type TV
+template <int BB, typename TS, typename TV>
+__device__ inline void words_to_block(const unsigned (&w)[slot_words<TS>(BB)], TV (&a)[BB]) {
 #pragma unroll
   for (int e = 0; e < BB; e++) {
     if constexpr (std::is_same<TS, bf16s>::value) {
@@ -239,6 +243,13 @@ __device__ inline void load_block(const TS* __restrict__ val, size_t k, int lane
       a[e] = (TV)__uint_as_float(w[e]);
     }
   }
+}
+// All BB values of one block slot, converted to TV.
+template <int BB, typename TS, typename TV>
+__device__ inline void load_block(const TS* __restrict__ val, size_t k, int lane, TV (&a)[BB]) {
+  unsigned w[slot_words<TS>(BB)];
+  load_words<BB, TS>(val, k, lane, w);
+  words_to_block<BB, TS, TV>(w, a);
 }
 
 // Workgroup index remapped so that the workgroups dispatched to one XCD
@@ -1157,24 +1168,41 @@ __global__ void __launch_bounds__(kTrsvNT) k_trsv_upper(int s0, int s1, const in
 // Persistent sweeps: every level set of a sweep in one launch, for factors
 // with many small levels (ILU(k >= 1) under a block ordering), where a launch
 // per level costs more than the level's work.  One-wave workgroups take
-// 64-row slices in level order from a work counter (sync[0]).  A wave first
-// loads what no earlier level writes -- its rows' right-hand side, the
-// columns and blocks of its first kPersistPF slots and its diagonal block --
-// then waits until all slices of the previous level are done (the count
-// sync[2 + g - 1] reaching that level's slice count), gathers, solves, and
-// counts its slice (sync[2 + g]).  Slices are taken in level order by waves
-// that are running, so a wave waits only on slices held by running waves:
-// the sweep cannot deadlock whatever the grid and residency.  The wait is
-// bounded: after kPersistMaxPolls polls the wave records st->sweep_fault and
-// goes on (and every other wave stops waiting when it sees the fault), so
-// the kernel always ends and the host reports the fault.  The
-// count is published after an agent-scope release fence and read before an
-// agent-scope acquire fence (the XCDs' L2 caches are not coherent with each
-// other).  The last wave to leave (sync[1] counts them) resets the counters
-// for the next launch.  Slots are subtracted in the same order as in the
-// per-level kernels, so FP64 results are bitwise the same.
-constexpr int kPersistPF = 4;
+// 64-row slices in level order from a work counter (counter 0; counter i is
+// sync[i * kSyncStride]).  A wave first
+// loads what no earlier level writes -- its rows' right-hand side, the column
+// of each of its first persist_cols() slots, the blocks of its first
+// persist_blocks() slots (held as raw words) and its diagonal block --
+// then waits until all slices of the previous level are done (counter
+// 2 + g - 1 reaching that level's slice count).  After the wait one
+// batch of loads -- every gather, and the blocks not yet held -- is issued at
+// once, so the dependent part of a slice is one memory round trip; the
+// subtractions then run in slot order, as in the per-level kernels, so FP64
+// results are bitwise the same.  The slice is counted done in counter 2 + g.
+// Slices are taken in level order by waves that are running, so a wave waits
+// only on slices held by running waves: the sweep cannot deadlock whatever
+// the grid and residency.  The wait is bounded: after kPersistMaxPolls polls
+// the wave records st->sweep_fault and goes on (and every other wave stops
+// waiting when it sees the fault), so the kernel always ends and the host
+// reports the fault.  The count is published after an agent-scope release
+// fence and read before an agent-scope acquire fence (the XCDs' L2 caches are
+// not coherent with each other).  The last wave to leave (counter 1 counts
+// them) resets the counters for the next launch.
 constexpr int kPersistMaxPolls = 1 << 18;
+// Counters of a persistent sweep sit kSyncStride ints apart (256 bytes), so
+// that pollers of one level and producers of another do not share a line.
+constexpr int kSyncStride = 64;
+// Slots whose columns a wave loads before the wait (their gathered values,
+// about 72 words, are held after it), and slots whose blocks it holds across
+// the wait (about 96 words): register budgets that leave no spills.
+template <int B, typename T>
+__host__ __device__ constexpr int persist_cols() {
+  return 72 / (B * (int)sizeof(T) / 4) < 16 ? 72 / (B * (int)sizeof(T) / 4) : 16;
+}
+template <typename TS, int BB, int PC>
+__host__ __device__ constexpr int persist_blocks() {
+  return 96 / slot_words<TS>(BB) < PC ? 96 / slot_words<TS>(BB) : PC;
+}
 
 template <int B, typename TS, typename T, bool UPPER, bool DOT>
 __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, const int* __restrict__ slev,
@@ -1185,6 +1213,9 @@ __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, con
                                                      double* __restrict__ partials, KState* __restrict__ st,
                                                      int* __restrict__ sync) {
   constexpr int BB = B * B;
+  constexpr int E = slot_words<TS>(BB);
+  constexpr int PC = persist_cols<B, T>();
+  constexpr int PB = persist_blocks<TS, BB, PC>();
   const int lane = threadIdx.x;
   double dsum[1] = {0.0};
   if (!st->done) {
@@ -1195,26 +1226,24 @@ __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, con
       if (s >= nslice) break;
       const int g = slev[s];
       const int row = rows[(size_t)s * 64 + lane];
-      const int k0 = sptr[s], k1 = sptr[s + 1];
-      T t[B], d[BB], a[kPersistPF][BB];
-      int c[kPersistPF];
+      const int k0 = sptr[s], nk = sptr[s + 1] - k0;  // nk is the same for the whole wave
+      T t[B], d[BB];
+      int c[PC];
+      unsigned w[PB][E];
       if (row >= 0) {
 #pragma unroll
         for (int e = 0; e < B; e++) t[e] = rhs[(size_t)row * B + e];
 #pragma unroll
-        for (int u = 0; u < kPersistPF; u++) {
-          c[u] = -1;
-          if (k0 + u < k1) {
-            c[u] = ld_stream(col + (size_t)(k0 + u) * 64 + lane);
-            load_block<BB, TS, T>(val, (size_t)(k0 + u), lane, a[u]);
-          }
-        }
+        for (int u = 0; u < PC; u++) c[u] = u < nk ? ld_stream(col + (size_t)(k0 + u) * 64 + lane) : -1;
+#pragma unroll
+        for (int u = 0; u < PB; u++)
+          if (u < nk) load_words<BB, TS>(val, (size_t)(k0 + u), lane, w[u]);
         if constexpr (!UPPER) load_block<BB, TS, T>(dval, (size_t)s, lane, d);
       }
       if (g > 0) {
         const int need = gsp[g] - gsp[g - 1];
         int polls = 0;
-        while (__hip_atomic_load(&sync[1 + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        while (__hip_atomic_load(&sync[(1 + g) * kSyncStride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
           if (++polls > kPersistMaxPolls) {
             __hip_atomic_store(&st->sweep_fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
@@ -1226,17 +1255,28 @@ __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, con
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
       if (row >= 0) {
+        // one batch: the gathers of the first PC slots and the blocks of slots PB..PC-1
+        T xv[PC][B];
+        unsigned w2[PC > PB ? PC - PB : 1][E];
 #pragma unroll
-        for (int u = 0; u < kPersistPF; u++) {
-          if (k0 + u < k1) {
-            T xv[B];
+        for (int u = 0; u < PC; u++) {
+          if (u < nk) {
             const size_t cb = (size_t)(c[u] >= 0 ? c[u] : 0) * B;
 #pragma unroll
-            for (int e = 0; e < B; e++) xv[e] = c[u] >= 0 ? out[cb + e] : (T)0;
-            blk_msub<B, T>(t, a[u], xv);
+            for (int e = 0; e < B; e++) xv[u][e] = c[u] >= 0 ? out[cb + e] : (T)0;
+            if (u >= PB) load_words<BB, TS>(val, (size_t)(k0 + u), lane, w2[u - PB]);
           }
         }
-        if (k0 + kPersistPF < k1) sell_msub_range<B, TS, T>(col, val, out, k0 + kPersistPF, k1, lane, t);
+#pragma unroll
+        for (int u = 0; u < PC; u++) {
+          if (u < nk) {
+            T a[BB];
+            if (u < PB) words_to_block<BB, TS, T>(w[u], a);
+            else words_to_block<BB, TS, T>(w2[u - PB], a);
+            blk_msub<B, T>(t, a, xv[u]);
+          }
+        }
+        if (nk > PC) sell_msub_range<B, TS, T>(col, val, out, k0 + PC, k0 + nk, lane, t);
         if constexpr (UPPER) {
 #pragma unroll
           for (int e = 0; e < B; e++) {
@@ -1251,14 +1291,15 @@ __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, con
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      if (lane == 0) __hip_atomic_fetch_add(&sync[2 + g], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_fetch_add(&sync[(2 + g) * kSyncStride], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   if constexpr (DOT) write_partials<1, 64>(dsum, partials, blockIdx.x, 0);
   if (lane == 0) {
-    const int left = __hip_atomic_fetch_add(&sync[1], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const int left = __hip_atomic_fetch_add(&sync[kSyncStride], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (left == (int)gridDim.x - 1)
-      for (int i = 0; i < ngroup + 2; i++) __hip_atomic_store(&sync[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int i = 0; i < ngroup + 2; i++)
+        __hip_atomic_store(&sync[i * kSyncStride], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
