@@ -108,6 +108,7 @@ _SIGS = {
     "dfm_ilu_get_values": (C.c_int, [H, vp, vp]),
     "dfm_ilu_forward": (C.c_int, [H, vp]),
     "dfm_ilu_backward": (C.c_int, [H, vp]),
+    "dfm_ilu_solve_precond_noalloc": (C.c_int, [H, vp]),
     "dfm_ilu_solve_precond": (C.c_int, [H, vp]),
     "dfm_set_sweep_mode": (C.c_int, [C.c_int, C.c_int]),
     "dfm_solve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),

@@ -29,6 +29,24 @@ void set_error(const char* fmt, ...) {
 
 hipStream_t stream() { return g_stream; }
 
+// Scratch of the allocation-free entry points: a file-scope device array,
+// one copy per device, whose address each device gets at set-up (dfm_init)
+// through hipGetSymbolAddress.  Host code uses only those addresses, never
+// the array's name.  Calls that overlap on one device share it and race.
+constexpr size_t kScratchDoubles = (size_t)1 << 22;  // 4 Mi doubles = 32 MiB per device
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxScratchDevices = 64;
+static double* g_scratch[kMaxScratchDevices] = {};
+
+// with `device` current
+static int scratch_setup(int device) {
+  if (device >= kMaxScratchDevices || g_scratch[device]) return DFM_OK;
+  void* p = nullptr;
+  DFM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  DFM_CHECK(p, DFM_HIP_ERROR, "no device address for the scratch array");
+  g_scratch[device] = static_cast<double*>(p);
+  return DFM_OK;
+}
 
 int ensure_init() {
   if (g_inited) return DFM_OK;
@@ -83,6 +101,7 @@ int dfm_init(int device) {
   DFM_CHECK(device >= 0 && device < count, DFM_BAD_ARG, "device %d out of range (%d devices)", device, count);
   if (g_inited && g_device == device) return DFM_OK;
   DFM_HIP(hipSetDevice(device));
+  DFM_TRY(scratch_setup(device));
   if (g_stream && g_own_stream) (void)hipStreamDestroy(g_stream);
   g_own_stream = true;
   DFM_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
@@ -891,6 +910,27 @@ int dfm_ilu_backward(dfm_ilu* M, double* vec) {
   DFM_HIP(hipStreamSynchronize(g_stream));
   return DFM_OK;
 }
+int dfm_ilu_solve_precond_noalloc(dfm_ilu* M, double* vec) {
+  DFM_CHECK(M && vec && M->factored, DFM_BAD_ARG, "factor has no values (call dfm_ilu_set_value)");
+  DFM_CHECK(g_inited, DFM_BAD_ARG, "dfm_init has not been called");
+  const int nb = (int)M->nb, b = (int)M->b;
+  const long n = (long)nb * b;
+  if (!M->permuted) {
+    DFM_TRY(ilu_sweep64(M, vec, true, true));
+  } else {
+    DFM_CHECK((size_t)n <= kScratchDoubles, DFM_TOO_LARGE,
+              "ordered factor of %ld values: the allocation-free SolvePrecond holds at most %zu", n,
+              kScratchDoubles);
+    double* tmp = g_device < kMaxScratchDevices ? g_scratch[g_device] : nullptr;
+    DFM_CHECK(tmp, DFM_BAD_ARG, "device %d has no scratch array", g_device);
+    DFM_LAUNCH((k_permute<double, double>), grid_for(n), nb, b, M->d_order.p, vec, tmp, 1.0);
+    DFM_TRY(ilu_sweep64(M, tmp, true, true));
+    DFM_LAUNCH(k_unpermute<double>, grid_for(n), nb, b, M->d_order.p, tmp, vec);
+  }
+  DFM_HIP(hipStreamSynchronize(g_stream));
+  return DFM_OK;
+}
+
 int dfm_ilu_solve_precond(dfm_ilu* M, double* vec) {
   DFM_CHECK(M && vec && M->factored, DFM_BAD_ARG, "factor has no values (call dfm_ilu_set_value)");
   const int nb = (int)M->nb, b = (int)M->b;

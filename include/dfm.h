@@ -161,6 +161,14 @@ int dfm_ilu_backward(dfm_ilu *M, double *vec_dev);
 /* CPreconditioner_ILU::SolvePrecond preconditioner.cpp:301-335 on a vector in the
  * original numbering (permuted in and out when the factor is ordered). */
 int dfm_ilu_solve_precond(dfm_ilu *M, double *vec_dev);
+/* The same without allocating (no counterpart in the reference): for callers
+ * that cannot allocate device memory -- code under graph capture, serving
+ * loops, harnesses that forbid hipMalloc.  An ordered factor's permuted copy
+ * lives in a per-device scratch array of 4 Mi doubles (32 MiB, held until the
+ * process ends): DFM_TOO_LARGE for an ordered factor of more values; natural-
+ * order factors need no scratch.  Calls that overlap on one device share the
+ * scratch and race.  Needs dfm_init. */
+int dfm_ilu_solve_precond_noalloc(dfm_ilu *M, double *vec_dev);
 
 /* ---- Batched solves: many independent scalar systems, one persistent launch */
 /* No single reference counterpart: each system gets the reference's ordered
