@@ -103,6 +103,7 @@ _SIGS = {
     "dfm_ilu_create_fill": (C.c_int, [H, C.c_int, vp, C.c_uint, C.POINTER(H)]),
     "dfm_ilu_destroy": (C.c_int, [H]),
     "dfm_ilu_info": (C.c_int, [H, u32p, i32p, i32p, i32p]),
+    "dfm_ilu_tile_info": (C.c_int, [H, i32p, i32p, i32p, i32p]),
     "dfm_ilu_get_pattern": (C.c_int, [H, vp, vp, vp, vp, vp]),
     "dfm_ilu_set_value": (C.c_int, [H, H, i32p]),
     "dfm_ilu_get_values": (C.c_int, [H, vp, vp]),

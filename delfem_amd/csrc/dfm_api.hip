@@ -584,9 +584,38 @@ static int upload_sell_plan(const SellPlan& plan, int bb, Sell<double>& s64, Dev
   return DFM_OK;
 }
 
+static int upload_tile_plan(const TilePlan& tp, int bb, TileSweep& ts) {
+  ts.nt = tp.nt;
+  ts.ntile = tp.ntile;
+  ts.phase_tile = tp.phase_tile;
+  Sell<double>& v = ts.v64;
+  v.nslice = tp.ntile * tp.nt / kSlice;
+  v.nk = tp.nk;
+  v.bb = bb;
+  v.group_slice_ptr = {0, v.nslice};
+  DFM_TRY(v.slice_ptr.upload(tp.slice_ptr));
+  DFM_TRY(v.rows.upload(tp.rows));
+  DFM_TRY(v.col.upload(tp.col));
+  DFM_TRY(ts.src.upload(tp.src));
+  DFM_TRY(ts.rlev.upload(tp.rlev));
+  DFM_TRY(ts.depth.upload(tp.depth));
+  DFM_TRY(ts.split.upload(tp.split));
+  DFM_TRY(v.val.alloc(std::max<size_t>(1, (size_t)tp.nk * bb * kSlice)));
+  DFM_TRY(v.dval.alloc(std::max<size_t>(1, (size_t)v.nslice * bb * kSlice)));
+  return DFM_OK;
+}
+
 }  // extern "C"
 
 namespace dfm {
+// Rows a tile of the phase-tiled sweeps is filled to (DFM_TILE_ROWS, 1..256):
+// smaller tiles spread a phase over more compute units.
+static int g_tile_rows = [] {
+  const char* e = getenv("DFM_TILE_ROWS");
+  const int v = e ? atoi(e) : 64;
+  return v >= 1 && v <= kTileMaxRows ? v : 64;
+}();
+
 // dfm_ilu_create; with given_row / given_col (natural order only) the factor
 // pattern is that one instead of the ILU(k) symbolic of A's: it must hold
 // A's pattern, and its extra entries start at zero
@@ -622,10 +651,13 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
     std::vector<int> colour;
     M->ncolour = greedy_colour(nb, A->rowptr, A->col, colour);
     colour_order(nb, colour, M->ncolour, M->order);
+    M->phase_ptr.assign(M->ncolour + 1, 0);
+    for (unsigned i = 0; i < nb; i++) M->phase_ptr[colour[i] + 1]++;
+    for (int c = 0; c < M->ncolour; c++) M->phase_ptr[c + 1] += M->phase_ptr[c];
     M->permuted = true;
   } else if (ordering == DFM_ORDER_BLOCK_MULTICOLOUR) {
     int nblock = 0;
-    M->ncolour = bmc_order(nb, A->rowptr, A->col, block_nodes ? block_nodes : 32u, M->order, nblock);
+    M->ncolour = bmc_order(nb, A->rowptr, A->col, block_nodes ? block_nodes : 32u, M->order, nblock, &M->phase_ptr);
     M->permuted = true;
   } else {
     M->order.resize(nb);
@@ -703,6 +735,11 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
     }
     DFM_TRY(M->d_rzmaskL.upload(mask));
   }
+  // phase-tiled sweep layouts, for factors under a colour ordering
+  TilePlan tpL, tpU;
+  const bool tiled = M->phase_ptr.size() >= 2 &&
+                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, M->phase_ptr, false, g_tile_rows, tpL) &&
+                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, M->phase_ptr, true, g_tile_rows, tpU);
   // permuted copy of A: rows in factor numbering, columns renumbered and sorted
   {
     std::vector<unsigned> beg(nb), end(nb), colv(A->col.size());
@@ -746,6 +783,10 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
     DFM_TRY(upload_sell_plan(M->planU, (int)bb, M->U64, M->d_srcU));
     DFM_TRY(build_sweep_sync(M->planL, M->syncL));
     DFM_TRY(build_sweep_sync(M->planU, M->syncU));
+    if (tiled) {
+      DFM_TRY(upload_tile_plan(tpL, (int)bb, M->tileL));
+      DFM_TRY(upload_tile_plan(tpU, (int)bb, M->tileU));
+    }
     DFM_TRY(upload_sell_plan(M->planA, (int)bb, M->A64, M->d_srcA));
     DFM_TRY(M->state.alloc(1));
     DFM_HIP(hipHostMalloc((void**)&M->h_state, sizeof(KState), hipHostMallocDefault));
@@ -781,6 +822,15 @@ int dfm_ilu_info(const dfm_ilu* M, unsigned* ncrs, int* nlevel_lower, int* nleve
   if (nlevel_lower) *nlevel_lower = M->nlev_lower;
   if (nlevel_upper) *nlevel_upper = M->nlev_upper;
   if (ncolour) *ncolour = M->ncolour;
+  return DFM_OK;
+}
+
+int dfm_ilu_tile_info(const dfm_ilu* M, int* nphase, int* ntile_lower, int* ntile_upper, int* tile_threads) {
+  DFM_CHECK(M, DFM_BAD_ARG, "NULL factor");
+  if (nphase) *nphase = M->tileL.ntile ? (int)M->tileL.phase_tile.size() - 1 : 0;
+  if (ntile_lower) *ntile_lower = M->tileL.ntile;
+  if (ntile_upper) *ntile_upper = M->tileU.ntile;
+  if (tile_threads) *tile_threads = std::max(M->tileL.nt, M->tileU.nt);
   return DFM_OK;
 }
 

@@ -245,7 +245,7 @@ void colour_order(unsigned nb, const std::vector<int>& colour, int ncolour, std:
 // inside a block keep their natural relative order, so the factor keeps most
 // of the natural-order ILU quality while blocks of one colour are independent.
 int bmc_order(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
-              unsigned block_nodes, std::vector<int>& order, int& nblock) {
+              unsigned block_nodes, std::vector<int>& order, int& nblock, std::vector<int>* colour_ptr) {
   std::vector<int> blk(nb, -1);
   std::vector<unsigned> queue;
   queue.reserve(block_nodes);
@@ -297,6 +297,11 @@ int bmc_order(unsigned nb, const std::vector<unsigned>& rowptr, const std::vecto
   for (int k : blocks_by_colour)
     for (unsigned t = bptr[k]; t < bptr[k + 1]; t++) order.push_back(bynode[t]);
   nblock = nbk;
+  if (colour_ptr) {
+    colour_ptr->assign(ncol + 1, 0);
+    for (int k = 0; k < nbk; k++) (*colour_ptr)[bcol[k] + 1] += (int)(bptr[k + 1] - bptr[k]);
+    for (int c = 0; c < ncol; c++) (*colour_ptr)[c + 1] += (*colour_ptr)[c];
+  }
   return ncol;
 }
 
@@ -368,6 +373,143 @@ void build_sell_plan(const std::vector<int>& rows_by_group, const std::vector<in
       }
     }
   }
+}
+
+// Phase-tiled sweep plan (TilePlan).  Components by union-find over the
+// in-phase entries, local levels in sweep order, components packed into tiles
+// in the order of their first rows, rows of a tile in factor order.
+bool build_tile_plan(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
+                     const std::vector<unsigned>& fdiaind, const std::vector<int>& phase_ptr, bool upper,
+                     int pack_rows, TilePlan& tp) {
+  tp = TilePlan();
+  const int np = (int)phase_ptr.size() - 1;
+  if (np < 1 || phase_ptr[0] != 0 || phase_ptr[np] != (int)nb) return false;
+  for (int p = 0; p < np; p++)
+    if (phase_ptr[p + 1] < phase_ptr[p]) return false;
+  auto beg = [&](unsigned i) { return upper ? fdiaind[i] : frow[i]; };
+  auto end = [&](unsigned i) { return upper ? frow[i + 1] : fdiaind[i]; };
+  std::vector<int> phase_of(nb);
+  for (int p = 0; p < np; p++)
+    for (int i = phase_ptr[p]; i < phase_ptr[p + 1]; i++) phase_of[i] = p;
+  // components and local levels, phase by phase
+  std::vector<int> parent(nb), level(nb, 0), comp_size(nb, 0);
+  std::function<int(int)> find = [&](int x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (unsigned i = 0; i < nb; i++) parent[i] = (int)i;
+  int maxcomp = 0;
+  for (int p = 0; p < np; p++) {
+    const int a = phase_ptr[p], b = phase_ptr[p + 1];
+    for (int i = a; i < b; i++)
+      for (unsigned q = beg(i); q < end(i); q++) {
+        const int j = (int)fcol[q];
+        if (j < a || j >= b) continue;
+        const int ri = find(i), rj = find(j);
+        if (ri != rj) parent[std::max(ri, rj)] = std::min(ri, rj);  // root = smallest row
+      }
+    for (int ii = 0; ii < b - a; ii++) {
+      const int i = upper ? b - 1 - ii : a + ii;
+      int l = 0;
+      for (unsigned q = beg(i); q < end(i); q++) {
+        const int j = (int)fcol[q];
+        if (j >= a && j < b) l = std::max(l, level[j] + 1);
+      }
+      level[i] = l;
+    }
+    for (int i = a; i < b; i++) maxcomp = std::max(maxcomp, ++comp_size[find(i)]);
+  }
+  if (maxcomp > kTileMaxRows) return false;
+  const int nt = maxcomp <= 64 ? 64 : (maxcomp <= 128 ? 128 : 256);
+  const int cap = std::min(nt, std::max(1, pack_rows));
+  tp.nt = nt;
+  // tiles, phases in launch order
+  std::vector<std::vector<int>> members(nb);
+  std::vector<int> tile_rows;  // concatenated rows of all tiles, nt per tile
+  tp.phase_tile.assign(1, 0);
+  std::vector<int> cur;
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    std::sort(cur.begin(), cur.end());
+    for (int t = 0; t < nt; t++) tile_rows.push_back(t < (int)cur.size() ? cur[t] : -1);
+    tp.ntile++;
+    cur.clear();
+  };
+  for (int pp = 0; pp < np; pp++) {
+    const int p = upper ? np - 1 - pp : pp;
+    const int a = phase_ptr[p], b = phase_ptr[p + 1];
+    for (int i = a; i < b; i++) members[find(i)].push_back(i);
+    for (int i = a; i < b; i++) {
+      if (members[i].empty()) continue;  // not a root
+      const int sz = (int)members[i].size();
+      if (!cur.empty() && (int)cur.size() + sz > cap) flush();
+      cur.insert(cur.end(), members[i].begin(), members[i].end());
+      if ((int)cur.size() >= cap) flush();
+      std::vector<int>().swap(members[i]);
+    }
+    flush();
+    tp.phase_tile.push_back(tp.ntile);
+  }
+  // per tile row: level; per slice: parts and widths
+  const int nslice = tp.ntile * nt / kSlice;
+  tp.rows = tile_rows;
+  tp.rlev.assign(tile_rows.size(), -1);
+  tp.depth.assign(tp.ntile, 0);
+  tp.slice_ptr.assign(nslice + 1, 0);
+  tp.split.assign(nslice, 0);
+  std::vector<int> nfirst(tile_rows.size(), 0), nsecond(tile_rows.size(), 0);
+  // in-phase columns: lower rows -> the columns at or after the phase start;
+  // upper rows -> the columns before the phase end
+  auto in_phase = [&](int i, int j) { return phase_of[j] == phase_of[i]; };
+  for (size_t r = 0; r < tile_rows.size(); r++) {
+    const int i = tile_rows[r];
+    if (i < 0) continue;
+    tp.rlev[r] = level[i];
+    int nin = 0, nout = 0;
+    for (unsigned q = beg(i); q < end(i); q++) (in_phase(i, (int)fcol[q]) ? nin : nout)++;
+    nfirst[r] = upper ? nin : nout;
+    nsecond[r] = upper ? nout : nin;
+    const int t = (int)(r / nt);
+    tp.depth[t] = std::max(tp.depth[t], level[i] + 1);
+  }
+  for (int s = 0; s < nslice; s++) {
+    int wa = 0, wb = 0;
+    for (int l = 0; l < kSlice; l++) {
+      wa = std::max(wa, nfirst[(size_t)s * kSlice + l]);
+      wb = std::max(wb, nsecond[(size_t)s * kSlice + l]);
+    }
+    tp.split[s] = tp.slice_ptr[s] + wa;
+    tp.slice_ptr[s + 1] = tp.split[s] + wb;
+  }
+  tp.nk = tp.slice_ptr[nslice];
+  tp.col.assign((size_t)tp.nk * kSlice, -1);
+  tp.src.assign((size_t)tp.nk * kSlice, -1);
+  std::vector<int> local(nb, -1);
+  for (int t = 0; t < tp.ntile; t++) {
+    for (int r = 0; r < nt; r++) {
+      const int i = tile_rows[(size_t)t * nt + r];
+      if (i >= 0) local[i] = r;
+    }
+    for (int r = 0; r < nt; r++) {
+      const int i = tile_rows[(size_t)t * nt + r];
+      if (i < 0) continue;
+      const int s = (t * nt + r) / kSlice, l = r % kSlice;
+      int ka = tp.slice_ptr[s], kb = tp.split[s];
+      for (unsigned q = beg(i); q < end(i); q++) {
+        const int j = (int)fcol[q];
+        const bool in = in_phase(i, j);
+        const bool first = upper ? in : !in;
+        int& k = first ? ka : kb;
+        tp.col[(size_t)k * kSlice + l] = in ? local[j] : j;
+        tp.src[(size_t)k * kSlice + l] = (long long)q;
+        k++;
+      }
+    }
+  }
+  return true;
 }
 
 }  // namespace dfm

@@ -107,10 +107,47 @@ int iluk_symbolic(unsigned nb, const std::vector<unsigned>& rowptr, const std::v
 int greedy_colour(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
                   std::vector<int>& colour);
 void colour_order(unsigned nb, const std::vector<int>& colour, int ncolour, std::vector<int>& order);
+// colour_ptr (may be NULL): ncolour + 1 offsets of the colours in the new order
 int bmc_order(unsigned nb, const std::vector<unsigned>& rowptr, const std::vector<unsigned>& col,
-              unsigned block_nodes, std::vector<int>& order, int& nblock);
+              unsigned block_nodes, std::vector<int>& order, int& nblock, std::vector<int>* colour_ptr = nullptr);
 int level_sets(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
                const std::vector<unsigned>& fdiaind, bool lower, std::vector<int>& level);
+
+// Phase-tiled sweep plan of one triangle of a factor whose rows fall into
+// phases: contiguous row ranges in factor order (the colours of a multicolour
+// or block-multicolour ordering), swept one launch per phase (the lower
+// triangle first phase first, the upper last phase first).  Inside a phase,
+// rows depend on each other only through small connected components of the
+// phase's own entries; a tile (one workgroup of nt threads, thread t = tile
+// row t) holds whole components, at most nt rows, and runs them level by
+// level (a row's local level: 1 + the largest local level of the in-phase
+// rows it reads) with a workgroup barrier between levels.  A tile row's slots
+// keep the factor's column order and come in two parts, laid out as a
+// sliced-ELL array whose slices are the tiles' 64-row waves: lower rows first
+// their columns in earlier phases (col = factor row), then their in-phase
+// columns (col = tile row of that column); upper rows first their in-phase
+// columns, then those in later phases.  split[s] is the first slot of slice
+// s's second part; -1 columns pad each part to the slice's widest row.
+struct TilePlan {
+  int nt = 0;                       // threads (= rows) per tile: 64, 128 or 256
+  int ntile = 0;
+  std::vector<int> phase_tile;      // launch p covers tiles [phase_tile[p], phase_tile[p + 1])
+  std::vector<int> rows, rlev;      // ntile * nt: factor row and local level per tile row (-1: padding)
+  std::vector<int> depth;           // per tile: number of local levels
+  std::vector<int> slice_ptr;       // ntile * nt / 64 + 1
+  std::vector<int> split;           // per slice
+  long nk = 0;                      // slots per lane over all slices
+  std::vector<int> col;             // nk * 64
+  std::vector<long long> src;       // nk * 64: factor entry, -1 padding
+};
+// Largest component a tile can hold.
+constexpr int kTileMaxRows = 256;
+// Builds the plan; pack_rows: components are packed into one tile while it
+// holds at most that many rows (a larger component takes a tile of its own).
+// Returns false (plan unusable) when a component exceeds kTileMaxRows rows.
+bool build_tile_plan(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
+                     const std::vector<unsigned>& fdiaind, const std::vector<int>& phase_ptr, bool upper,
+                     int pack_rows, TilePlan& tp);
 
 // Sliced-ELL layout plan: rows grouped into "groups" (level sets, or one group
 // for a plain SpMV), each group cut into slices of kSlice rows.  For slice s
@@ -161,6 +198,19 @@ struct KState {
 struct SweepSync {
   DevBuf<int> slev, gsp, sync;
   int nslice = 0, ngroup = 0, maxlev = 0;  // maxlev: slices of the largest level
+};
+
+// Device copy of a TilePlan: the layout (slice_ptr, rows, col of v64) and the
+// value copies in each storage type (v32 / v16 made with the factor's
+// FP32 / BF16 copies).  ntile == 0: the factor has no usable phase tiling.
+struct TileSweep {
+  int nt = 0, ntile = 0;
+  std::vector<int> phase_tile;
+  DevBuf<int> rlev, depth, split;
+  DevBuf<long long> src;
+  Sell<double> v64;
+  Sell<float> v32;
+  Sell<bf16s> v16;
 };
 
 // dfm_ilu_create, or (given_row / given_col, natural order) a factor on a
@@ -236,6 +286,8 @@ struct dfm_ilu_s {
   dfm::DevBuf<long long> d_srcL, d_srcU;
   dfm::Sell<double> L64, U64;
   dfm::SweepSync syncL, syncU;          // persistent-sweep counters of planL / planU
+  std::vector<int> phase_ptr;           // colour offsets in factor order (multicolour orderings)
+  dfm::TileSweep tileL, tileU;          // phase-tiled sweep layouts (ntile 0: none)
   dfm::Sell<float> L32, U32;
   dfm::Sell<dfm::bf16s> L16, U16;
   bool have32 = false, have16 = false;

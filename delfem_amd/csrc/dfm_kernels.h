@@ -1303,6 +1303,153 @@ __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// Phase-tiled sweeps: one launch per phase (a colour of the ordering), one
+// workgroup per tile of whole dependency components, thread t = tile row t
+// (layout: TilePlan).  Before the first barrier every thread loads what does
+// not depend on this launch: its right-hand side, for a lower row the
+// subtraction of its columns in earlier phases (complete in memory; they come
+// first in column order) and its diagonal block, for an upper row the products
+// of its columns in later phases (subtracted last, in order), and the blocks
+// and tile positions of its first kTileInSlots in-phase columns.  Then the
+// tile runs its local levels: the rows of level l subtract their in-phase
+// columns, read from LDS where the rows of earlier levels left them, finish,
+// and publish to LDS and to the vector; a barrier ends the level.  Every
+// subtraction is the one of the per-level kernels in the same order (a
+// product formed first and then subtracted, blk_msub), and padding slots
+// subtract exact zeros, so FP64 results are bitwise those of the per-level
+// sweeps.  With DOT the tile's <rvec, v> partial goes to partials[tile].
+constexpr int kTileInSlots = 4;
+template <int B, typename T>
+__host__ __device__ constexpr int tile_out_slots() {
+  return 48 / (B * (int)sizeof(T) / 4) < 8 ? 48 / (B * (int)sizeof(T) / 4) : 8;
+}
+// s[r] = sum_c A[r*B+c]*x[c], the sum blk_msub forms before subtracting it
+template <int B, typename T>
+__device__ inline void blk_prod(T* s, const T* a, const T* x) {
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    T v = a[r * B] * x[0];
+#pragma unroll
+    for (int c = 1; c < B; c++) v = v + a[r * B + c] * x[c];
+    s[r] = v;
+  }
+}
+
+template <int B, typename TS, typename T, int NT, bool UPPER, bool DOT>
+__global__ void __launch_bounds__(NT) k_trsv_tile(int tile0, const int* __restrict__ rows,
+                                                  const int* __restrict__ rlev, const int* __restrict__ depth,
+                                                  const int* __restrict__ sptr, const int* __restrict__ split,
+                                                  const int* __restrict__ col, const TS* __restrict__ val,
+                                                  const TS* __restrict__ dval, const T* rhs, T* out,
+                                                  const T* __restrict__ rvec, double* __restrict__ partials,
+                                                  const KState* __restrict__ st) {
+  if (st && st->done) return;
+  constexpr int BB = B * B;
+  constexpr int E = slot_words<TS>(BB);
+  constexpr int PI = kTileInSlots;
+  constexpr int PO = UPPER ? tile_out_slots<B, T>() : 1;
+  __shared__ T xs[NT * B];
+  const int tile = tile0 + blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int s = tile * (NT / 64) + (tid >> 6);
+  const size_t rix = (size_t)tile * NT + tid;
+  const int row = rows[rix];
+  const int lev = rlev[rix];
+  const int nlev = depth[tile];
+  const int k0 = sptr[s], km = split[s], k1 = sptr[s + 1];
+  const int ki = UPPER ? k0 : km, ni = UPPER ? km - k0 : k1 - km;  // in-phase part
+  const int ko = UPPER ? km : k0, no = UPPER ? k1 - km : km - k0;  // the other phases
+  T t[B], d[BB];
+  int ci[PI];
+  unsigned wi[PI][E];
+  T so[PO][B];  // upper, B > 1: products of the later-phase slots
+  T ao[PO], xo[PO];  // upper, B == 1: their factors (blk_msub<1> fuses them into the subtraction)
+  double dsum[1] = {0.0};
+  if (row >= 0) {
+#pragma unroll
+    for (int e = 0; e < B; e++) t[e] = rhs[(size_t)row * B + e];
+    if constexpr (!UPPER) {
+      sell_msub_range<B, TS, T>(col, val, out, ko, ko + no, lane, t);
+      load_block<BB, TS, T>(dval, (size_t)s, lane, d);
+    }
+#pragma unroll
+    for (int u = 0; u < PI; u++) {
+      if (u < ni) {
+        ci[u] = col[(size_t)(ki + u) * 64 + lane];
+        load_words<BB, TS>(val, (size_t)(ki + u), lane, wi[u]);
+      }
+    }
+    if constexpr (UPPER) {
+#pragma unroll
+      for (int u = 0; u < PO; u++) {
+        if (u < no) {
+          T a[BB], xv[B];
+          sell_slot<B, TS, T>(col, val, out, ko + u, lane, a, xv);
+          if constexpr (B == 1) {
+            ao[u] = a[0];
+            xo[u] = xv[0];
+          } else {
+            blk_prod<B, T>(so[u], a, xv);
+          }
+        }
+      }
+    }
+  }
+  for (int l = 0; l < nlev; l++) {
+    if (row >= 0 && lev == l) {
+#pragma unroll
+      for (int u = 0; u < PI; u++) {
+        if (u < ni) {
+          T a[BB], xv[B];
+          words_to_block<BB, TS, T>(wi[u], a);
+          const int c = ci[u];
+#pragma unroll
+          for (int e = 0; e < B; e++) xv[e] = c >= 0 ? xs[c * B + e] : (T)0;
+          blk_msub<B, T>(t, a, xv);
+        }
+      }
+      for (int u = PI; u < ni; u++) {
+        T a[BB], xv[B];
+        const int c = col[(size_t)(ki + u) * 64 + lane];
+        load_block<BB, TS, T>(val, (size_t)(ki + u), lane, a);
+#pragma unroll
+        for (int e = 0; e < B; e++) xv[e] = c >= 0 ? xs[c * B + e] : (T)0;
+        blk_msub<B, T>(t, a, xv);
+      }
+      if constexpr (UPPER) {
+#pragma unroll
+        for (int u = 0; u < PO; u++) {
+          if (u < no) {
+            if constexpr (B == 1) {
+              t[0] -= ao[u] * xo[u];
+            } else {
+#pragma unroll
+              for (int r = 0; r < B; r++) t[r] -= so[u][r];
+            }
+          }
+        }
+        if (no > PO) sell_msub_range<B, TS, T>(col, val, out, ko + PO, ko + no, lane, t);
+      }
+      T y[B];
+      if constexpr (UPPER) {
+#pragma unroll
+        for (int e = 0; e < B; e++) y[e] = t[e];
+      } else {
+        blk_mul<B, T>(y, d, t);
+      }
+#pragma unroll
+      for (int e = 0; e < B; e++) {
+        xs[tid * B + e] = y[e];
+        out[(size_t)row * B + e] = y[e];
+        if (DOT) dsum[0] += (double)rvec[(size_t)row * B + e] * (double)y[e];
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (DOT) write_partials<1, NT>(dsum, partials, tile, 0);
+}
+
 // Forward sweep of one level fused with the PCG update that precedes it
 // (solver_ls_iter.cpp:140-145, then :158): for each row of the level
 //   r_i += (-alpha) Ap_i ; x_i += alpha p_i ; z_i = Dinv_i (r_i - sum L_ij z_j).

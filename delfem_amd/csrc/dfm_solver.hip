@@ -99,6 +99,22 @@ int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32, bool want16
     DFM_TRY(gather_dia(M->L16, M->L64.rows.p, nullptr, M->d_fdia.p, bb));
     DFM_TRY(gather(M->U16, M->d_srcU, M->d_fval.p, bb));
   }
+  for (TileSweep* ts : {&M->tileL, &M->tileU}) {
+    if (!ts->ntile) continue;
+    const bool lower = ts == &M->tileL;
+    DFM_TRY(gather(ts->v64, ts->src, M->d_fval.p, bb));
+    if (lower) DFM_TRY(gather_dia(ts->v64, ts->v64.rows.p, nullptr, M->d_fdia.p, bb));
+    if (M->have32) {
+      if (ts->v32.nslice != ts->v64.nslice) DFM_TRY(sell_alloc_like(ts->v64, ts->v32));
+      DFM_TRY(gather(ts->v32, ts->src, M->d_fval.p, bb));
+      if (lower) DFM_TRY(gather_dia(ts->v32, ts->v64.rows.p, nullptr, M->d_fdia.p, bb));
+    }
+    if (M->have16) {
+      if (ts->v16.nslice != ts->v64.nslice) DFM_TRY(sell_alloc_like(ts->v64, ts->v16));
+      DFM_TRY(gather(ts->v16, ts->src, M->d_fval.p, bb));
+      if (lower) DFM_TRY(gather_dia(ts->v16, ts->v64.rows.p, nullptr, M->d_fdia.p, bb));
+    }
+  }
   if (A && (M->a_version != A->version || (M->have32 && !M->a32_valid))) {
     const int* ord = M->permuted ? M->d_order.p : nullptr;
     DFM_TRY(gather(M->A64, M->d_srcA, A->d_val.p, bb));
@@ -116,12 +132,17 @@ int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32, bool want16
 // ---------------------------------------------------------------------------
 // Level-set sweeps, one launch per level.  `plan` carries the layout (rows,
 // slices, columns) of the FP64 copy; `L`/`U` the values in storage type TS.
-// Persistent sweeps (k_trsv_persist) serve plans with many levels; they need
-// the solver state (for the stop flag and the fault report).  DFM_PERSIST=0
-// turns them off, =2 uses them for every plan; by default a plan of at least
-// kPersistMinLevels levels takes them.  Grid: two waves per slice of the
-// largest level (waves of later levels load their factor slots while the
-// level before them finishes), at least 64; DFM_PERSIST_GRID overrides it.
+// Plans with many levels take one of two other forms, both given the solver
+// state (for the stop flag and, persistent sweeps, the fault report): the
+// phase-tiled sweep (k_trsv_tile, one launch per colour) where the factor
+// has a phase tiling, else the persistent sweep (k_trsv_persist, one launch).
+// Sweep mode (DFM_PERSIST, dfm_set_sweep_mode): 0 one launch per level
+// always; 1 (default) plans of at least kPersistMinLevels levels take the
+// tiled form if there is one, else the persistent one; 2 the persistent form
+// for every plan; 3 the tiled form wherever there is one (per level
+// elsewhere).  Persistent grid: two waves per slice of the largest level
+// (waves of later levels load their factor slots while the level before them
+// finishes), at least 64; DFM_PERSIST_GRID overrides it.
 constexpr int kPersistMinLevels = 8;
 constexpr int kPersistMaxGrid = 4096;
 static int env_int(const char* name, int dflt) {
@@ -130,9 +151,42 @@ static int env_int(const char* name, int dflt) {
 }
 static int g_persist_mode = env_int("DFM_PERSIST", 1);      // dfm_set_sweep_mode
 static int g_persist_grid = env_int("DFM_PERSIST_GRID", 0);
-static bool use_persist(const SweepSync* ps, const KState* st) {
-  if (!ps || !st || ps->nslice == 0) return false;
+static bool use_tile(const TileSweep* ts, const SweepSync* ps, const KState* st) {
+  if (!ts || !st || ts->ntile == 0) return false;
+  return g_persist_mode == 3 || (g_persist_mode == 1 && ps && ps->ngroup >= kPersistMinLevels);
+}
+static bool use_persist(const SweepSync* ps, const KState* st, const TileSweep* ts = nullptr) {
+  if (!ps || !st || ps->nslice == 0 || use_tile(ts, ps, st)) return false;
   return g_persist_mode == 2 || (g_persist_mode == 1 && ps->ngroup >= kPersistMinLevels);
+}
+template <typename TS>
+static const Sell<TS>& tile_vals(const TileSweep& ts) {
+  if constexpr (std::is_same<TS, double>::value) return ts.v64;
+  else if constexpr (std::is_same<TS, float>::value) return ts.v32;
+  else return ts.v16;
+}
+// One launch per phase, in the plan's order.
+template <int B, typename TS, typename T, int NT, bool UPPER, bool DOT>
+static int tile_sweep_nt(const TileSweep& ts, const T* rhs, T* out, const T* rvec, double* partials,
+                         const KState* st) {
+  const Sell<TS>& v = tile_vals<TS>(ts);
+  const Sell<double>& lay = ts.v64;
+  for (size_t p = 0; p + 1 < ts.phase_tile.size(); p++) {
+    const int t0 = ts.phase_tile[p], nt = ts.phase_tile[p + 1] - t0;
+    if (nt == 0) continue;
+    DFM_LAUNCH_NT((k_trsv_tile<B, TS, T, NT, UPPER, DOT>), nt, NT, t0, lay.rows.p, ts.rlev.p, ts.depth.p,
+                  lay.slice_ptr.p, ts.split.p, lay.col.p, v.val.p, v.dval.p, rhs, out, rvec, partials, st);
+  }
+  return DFM_OK;
+}
+template <int B, typename TS, typename T, bool UPPER, bool DOT>
+static int tile_sweep(const TileSweep& ts, const T* rhs, T* out, const T* rvec, double* partials,
+                      const KState* st) {
+  switch (ts.nt) {
+    case 64: return tile_sweep_nt<B, TS, T, 64, UPPER, DOT>(ts, rhs, out, rvec, partials, st);
+    case 128: return tile_sweep_nt<B, TS, T, 128, UPPER, DOT>(ts, rhs, out, rvec, partials, st);
+    default: return tile_sweep_nt<B, TS, T, 256, UPPER, DOT>(ts, rhs, out, rvec, partials, st);
+  }
 }
 static int persist_grid(const SweepSync& ps) {
   const int g = g_persist_grid > 0 ? g_persist_grid : std::max(64, 2 * ps.maxlev);
@@ -141,7 +195,8 @@ static int persist_grid(const SweepSync& ps) {
 
 template <int B, typename TS, typename T>
 static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs, T* out, KState* st,
-                       const SweepSync* ps = nullptr) {
+                       const SweepSync* ps = nullptr, const TileSweep* ts = nullptr) {
+  if (use_tile(ts, ps, st)) return tile_sweep<B, TS, T, false, false>(*ts, rhs, out, nullptr, nullptr, st);
   if (use_persist(ps, st)) {
     DFM_LAUNCH_NT((k_trsv_persist<B, TS, T, false, false>), persist_grid(*ps), 64, ps->nslice, ps->ngroup,
                   ps->slev.p, ps->gsp.p, plan.rows.p, plan.slice_ptr.p, plan.col.p, L.val.p, L.dval.p, rhs, out,
@@ -157,7 +212,9 @@ static int sweep_lower(const Sell<TS>& L, const Sell<double>& plan, const T* rhs
   return DFM_OK;
 }
 // Number of partial sums an upper sweep with DOT leaves for the finalize kernel.
-static int upper_nparts(const Sell<double>& plan, const SweepSync* ps = nullptr, const KState* st = nullptr) {
+static int upper_nparts(const Sell<double>& plan, const SweepSync* ps = nullptr, const KState* st = nullptr,
+                        const TileSweep* ts = nullptr) {
+  if (use_tile(ts, ps, st)) return ts->ntile;
   if (use_persist(ps, st)) return persist_grid(*ps);
   int n = 0;
   const int ng = (int)plan.group_slice_ptr.size() - 1;
@@ -168,7 +225,8 @@ static int upper_nparts(const Sell<double>& plan, const SweepSync* ps = nullptr,
 // left in partials[0 .. upper_nparts) for a finalize kernel.
 template <int B, typename TS, typename T, bool DOT>
 static int sweep_upper(const Sell<TS>& U, const Sell<double>& plan, T* vec, const T* rvec, double* partials,
-                       KState* st, const SweepSync* ps = nullptr) {
+                       KState* st, const SweepSync* ps = nullptr, const TileSweep* ts = nullptr) {
+  if (use_tile(ts, ps, st)) return tile_sweep<B, TS, T, true, DOT>(*ts, vec, vec, rvec, partials, st);
   if (use_persist(ps, st)) {
     DFM_LAUNCH_NT((k_trsv_persist<B, TS, T, true, DOT>), persist_grid(*ps), 64, ps->nslice, ps->ngroup, ps->slev.p,
                   ps->gsp.p, plan.rows.p, plan.slice_ptr.p, plan.col.p, U.val.p, U.val.p, vec, vec, rvec, partials,
@@ -225,6 +283,8 @@ struct Ops {
   int nb;
   const SweepSync* syncL;
   const SweepSync* syncU;
+  const TileSweep* tileL;
+  const TileSweep* tileU;
 };
 template <typename TS>
 static const Sell<TS>& sel(const Sell<double>& s64, const Sell<float>& s32, const Sell<bf16s>& s16) {
@@ -236,7 +296,7 @@ template <typename TA, typename TF>
 static Ops<TA, TF> ops_of(dfm_ilu_s* M) {
   static const Sell<bf16s> none{};
   return {&sel<TA>(M->A64, M->A32, none), &sel<TF>(M->L64, M->L32, M->L16), &sel<TF>(M->U64, M->U32, M->U16),
-          &M->A64, &M->L64, &M->U64, (int)M->nb, &M->syncL, &M->syncU};
+          &M->A64, &M->L64, &M->U64, (int)M->nb, &M->syncL, &M->syncU, &M->tileL, &M->tileU};
 }
 
 template <int B, typename TA, typename TF, typename T, bool DOTS, bool GATE>
@@ -406,9 +466,9 @@ template <int B, typename TA, typename TF, typename T>
 static int pcg_start(dfm_ilu_s* M, const Ops<TA, TF>& o, T* r, T* p, T* z) {
   const long n = (long)o.nb * B;
   KState* st = M->state.p;
-  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, r, z, st, o.syncL)));
-  DFM_TRY((sweep_upper<B, TF, T, true>(*o.U, *o.planU, z, r, M->partials.p, st, o.syncU)));
-  DFM_LAUNCH_FIN(1, FIN_PCG_RZ0, M->partials.p, upper_nparts(*o.planU, o.syncU, st), st);
+  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, r, z, st, o.syncL, o.tileL)));
+  DFM_TRY((sweep_upper<B, TF, T, true>(*o.U, *o.planU, z, r, M->partials.p, st, o.syncU, o.tileU)));
+  DFM_LAUNCH_FIN(1, FIN_PCG_RZ0, M->partials.p, upper_nparts(*o.planU, o.syncU, st, o.tileU), st);
   DFM_HIP(hipMemcpyAsync(p, z, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
   return DFM_OK;
 }
@@ -421,13 +481,13 @@ static int bicg_body(dfm_ilu_s* M, const Ops<TA, TF>& o, T* r, T* x, T* w) {
   KState* st = M->state.p;
   double* part = M->partials.p;
   const int gv = grid_vec(n);
-  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, p, Mp, st, o.syncL)));
-  DFM_TRY((sweep_upper<B, TF, T, false>(*o.U, *o.planU, Mp, nullptr, nullptr, st, o.syncU)));
+  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, p, Mp, st, o.syncL, o.tileL)));
+  DFM_TRY((sweep_upper<B, TF, T, false>(*o.U, *o.planU, Mp, nullptr, nullptr, st, o.syncU, o.tileU)));
   DFM_TRY((spmv<B, TA, TF, T, true, true>(o, Mp, AMp, nullptr, r2, r, r2, part, st)));
   DFM_LAUNCH_FIN(3, FIN_BICG_ALPHA, part, spmv_nparts(*o.planA), st);
   DFM_LAUNCH(k_bicg_s<T>, gv, n, r, AMp, s, st);
-  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, s, Ms, st, o.syncL)));
-  DFM_TRY((sweep_upper<B, TF, T, false>(*o.U, *o.planU, Ms, nullptr, nullptr, st, o.syncU)));
+  DFM_TRY((sweep_lower<B, TF, T>(*o.L, *o.planL, s, Ms, st, o.syncL, o.tileL)));
+  DFM_TRY((sweep_upper<B, TF, T, false>(*o.U, *o.planU, Ms, nullptr, nullptr, st, o.syncU, o.tileU)));
   DFM_TRY((spmv<B, TA, TF, T, true, true>(o, Ms, AMs, nullptr, s, nullptr, nullptr, part, st)));
   DFM_LAUNCH_FIN(2, FIN_BICG_OMEGA, part, spmv_nparts(*o.planA), st);
   DFM_LAUNCH(k_bicg_xr<T>, gv, n, Mp, Ms, s, AMs, r2, x, r, part, st);
@@ -622,6 +682,7 @@ static int ensure_workspace(dfm_ilu_s* M, size_t bytes) {
   need = std::max<size_t>(need, (size_t)upper_nparts(M->U64));
   need = std::max<size_t>(need, (size_t)lower_nparts(M->L64));
   need = std::max<size_t>(need, (size_t)kPersistMaxGrid);
+  need = std::max<size_t>(need, (size_t)M->tileU.ntile);
   need = 3 * need + 64;
   if (M->partials.n < need) DFM_TRY(M->partials.alloc(need));
   return DFM_OK;
@@ -760,8 +821,8 @@ static int bench_kernel_t(dfm_bcrs_s* A, dfm_ilu_s* M, int which, int nrep, doub
   double* part = M->partials.p;
   auto run = [&]() -> int {
     if (which == 0) return spmv<B, TA, TF, T, true, true>(o, xv.p, yv.p, nullptr, xv.p, nullptr, nullptr, part, st);
-    if (which == 1) return sweep_lower<B, TF, T>(*o.L, *o.planL, xv.p, yv.p, st, o.syncL);
-    return sweep_upper<B, TF, T, true>(*o.U, *o.planU, yv.p, xv.p, part, st, o.syncU);
+    if (which == 1) return sweep_lower<B, TF, T>(*o.L, *o.planL, xv.p, yv.p, st, o.syncL, o.tileL);
+    return sweep_upper<B, TF, T, true>(*o.U, *o.planU, yv.p, xv.p, part, st, o.syncU, o.tileU);
   };
   // algorithmic bytes: values, column indices, row ids (sweeps), each vector
   // element read or written once (x of SpMV and the dot operand are the same
@@ -871,7 +932,7 @@ int dfm_bench_read(size_t bytes, int nrep, double* avg_ms) {
 }
 
 int dfm_set_sweep_mode(int mode, int grid) {
-  DFM_CHECK(mode >= 0 && mode <= 2 && grid >= 0 && grid <= dfm::kPersistMaxGrid, DFM_BAD_ARG,
+  DFM_CHECK(mode >= 0 && mode <= 3 && grid >= 0 && grid <= dfm::kPersistMaxGrid, DFM_BAD_ARG,
             "bad sweep mode %d / grid %d", mode, grid);
   dfm::g_persist_mode = mode;
   dfm::g_persist_grid = grid;

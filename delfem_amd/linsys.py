@@ -224,6 +224,12 @@ class CMatDiaFrac_BlkCrs:
         call("dfm_ilu_info", self.h, C.byref(ncrs), C.byref(nl), C.byref(nu), C.byref(nc))
         return dict(ncrs=ncrs.value, nlevel_lower=nl.value, nlevel_upper=nu.value, ncolour=nc.value)
 
+    def tile_info(self):
+        """Phase tiling of the sweeps: phases (colours), tiles per sweep, threads per tile."""
+        v = [C.c_int() for _ in range(4)]
+        call("dfm_ilu_tile_info", self.h, *(C.byref(x) for x in v))
+        return dict(nphase=v[0].value, ntile_lower=v[1].value, ntile_upper=v[2].value, tile_threads=v[3].value)
+
     def NCrs(self):
         return self.info()["ncrs"]
 
@@ -309,8 +315,10 @@ def Solve_PBiCGSTAB(conv_ratio, iteration, A, prec, r, x, precision="fp64", **op
 
 def set_sweep_mode(mode, grid=0):
     """Level sets of the solvers' triangular sweeps: mode 0 one launch per
-    level, 1 (default) one persistent launch for factors of 8 or more levels,
-    2 persistent always; grid = workgroups of a persistent launch (0: automatic)."""
+    level; 1 (default) for factors of 8 or more levels one launch per colour
+    (dependent rows tiled into workgroups) where the ordering allows, else one
+    persistent launch; 2 persistent always; 3 per colour wherever possible;
+    grid = workgroups of a persistent launch (0: automatic)."""
     call("dfm_set_sweep_mode", int(mode), int(grid))
 
 

@@ -145,6 +145,11 @@ int dfm_ilu_create_fill(const dfm_bcrs *A, int lev_fill, const unsigned *fill_bl
 int dfm_ilu_destroy(dfm_ilu *M);
 /* ncrs of the factor, number of level sets of L and U, colours (0 for NATURAL / GIVEN) */
 int dfm_ilu_info(const dfm_ilu *M, unsigned *ncrs, int *nlevel_lower, int *nlevel_upper, int *ncolour);
+/* Phase tiling of the triangular sweeps (no reference counterpart; a
+ * diagnostic): phases = colours swept one launch each, tiles per sweep, and
+ * the threads of a tile; all 0 when the factor has none (natural or given
+ * order, or a dependency component of more than 256 rows). */
+int dfm_ilu_tile_info(const dfm_ilu *M, int *nphase, int *ntile_lower, int *ntile_upper, int *tile_threads);
 /* Factor pattern in factor numbering; any pointer may be NULL. order: nblk entries. */
 int dfm_ilu_get_pattern(const dfm_ilu *M, unsigned *rowptr, unsigned *col, int *lev,
                         unsigned *diaind, int *order);
@@ -252,11 +257,16 @@ typedef struct {
 int dfm_solve_pcg(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
                   double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
 /* How the solvers run the level sets of a triangular sweep (no reference
- * counterpart; a tuning knob): mode 0 one launch per level, 1 (default) one
- * persistent launch for factors of 8 or more levels, 2 persistent always;
- * grid = workgroups of a persistent launch (0: twice the slices of the
- * largest level, at least 64; at most 4096).  FP64 sweeps give the same bits
- * either way.  Process-wide; initial values from DFM_PERSIST / DFM_PERSIST_GRID. */
+ * counterpart; a tuning knob): mode 0 one launch per level; 1 (default) for
+ * factors of 8 or more levels one launch per colour of a multicolour or
+ * block-multicolour ordering (tiles of dependent rows run level by level
+ * inside one workgroup) where every tile fits 256 rows, else one persistent
+ * launch; 2 persistent always; 3 per colour wherever possible (per level
+ * elsewhere).  grid = workgroups of a persistent launch (0: twice the slices
+ * of the largest level, at least 64; at most 4096).  FP64 sweeps give the
+ * same bits in every mode.  Process-wide; initial values from DFM_PERSIST /
+ * DFM_PERSIST_GRID (DFM_TILE_ROWS, read at factor creation: rows a tile is
+ * filled to, default 64). */
 int dfm_set_sweep_mode(int mode, int grid);
 int dfm_solve_pbicgstab(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
                         double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);

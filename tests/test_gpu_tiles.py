@@ -1,0 +1,109 @@
+"""The phase-tiled triangular sweep (one launch per colour of a multicolour
+or block-multicolour ordering; tiles of dependent rows run level by level
+inside a workgroup) against the one-launch-per-level sweep.  A tile row
+subtracts its slots in the factor's column order, products formed as in the
+per-level kernels, so BiCGSTAB (whose reductions do not involve the sweeps)
+must give the same bits in both modes for every storage type of the factors.
+PCG's first preconditioner application reduces <r,z> over the tiles'
+partials, so PCG is held to one iteration and 1e-8 of the per-level run.
+Block sizes of the ordering select tiles of 64, 128 and 256 threads; a
+factor whose components exceed 256 rows has no tiling and keeps the other
+sweep forms."""
+import numpy as np
+import pytest
+
+from delfem_amd import meshgen
+
+pytestmark = pytest.mark.gpu
+
+# name: (problem, ILU level, ordering, block nodes, solver)
+CASES = {
+    "oseen2d_16_ilu1_bmc8": (lambda: meshgen.oseen2d(16, seed=3), 1, "block_multicolour", 8, "bicg"),
+    "oseen2d_24_ilu1_bmc48": (lambda: meshgen.oseen2d(24, seed=4), 1, "block_multicolour", 48, "bicg"),
+    "oseen2d_32_ilu2_bmc40": (lambda: meshgen.oseen2d(32, seed=5), 2, "block_multicolour", 40, "bicg"),
+    # a dependency component above 256 rows: no tiling, the modes fall back
+    "oseen2d_32_ilu1_bmc100_untiled": (lambda: meshgen.oseen2d(32, seed=5), 1, "block_multicolour", 100, "bicg"),
+    "solid3d_4_ilu1_mc": (lambda: meshgen.solid3d(4, seed=3, sigma=0.5), 1, "multicolour", 0, "pcg"),
+    "poisson2d_24_ilu2_bmc16": (lambda: meshgen.poisson2d(24, seed=3), 2, "block_multicolour", 16, "pcg"),
+    "poisson2d_24_ilu0_mc": (lambda: meshgen.poisson2d(24, seed=6), 0, "multicolour", 0, "pcg"),
+}
+
+
+def _case(name):
+    print(f"[case] {name}", flush=True)
+
+
+@pytest.fixture(params=sorted(CASES), scope="module")
+def system(request, gpu):
+    _case(request.param)
+    make, lev, ordering, bn, _ = CASES[request.param]
+    prob = make()
+    s = gpu.FEMSystem(prob)
+    s.assemble()
+    M = gpu.CPreconditioner_ILU(s.A, lev, ordering=ordering, block_nodes=bn)
+    assert M.SetValue(s.A)
+    yield request.param, prob, s, M
+    gpu.set_sweep_mode(1, 0)
+
+
+def _solve(gpu, s, M, kind, precision, tol=1e-10):
+    import torch
+    r = s.res.clone()
+    x = torch.zeros_like(r)
+    fn = gpu.Solve_PCG if kind == "pcg" else gpu.Solve_PBiCGSTAB
+    ok, conv, it, info = fn(tol, 5000, s.A, M, r, x, precision=precision)
+    return info["converged"], conv, it, x.cpu().numpy(), r.cpu().numpy()
+
+
+def test_tiling_present(system):
+    name, prob, s, M = system
+    ti, info = M.tile_info(), M.info()
+    print(name, ti, info, flush=True)
+    if name.endswith("untiled"):
+        assert ti == dict(nphase=0, ntile_lower=0, ntile_upper=0, tile_threads=0)
+        return
+    assert ti["nphase"] == info["ncolour"] and ti["ntile_lower"] > 0 and ti["ntile_upper"] > 0
+    assert ti["tile_threads"] in (64, 128, 256)
+    if name.endswith("bmc40"):
+        assert ti["tile_threads"] == 256
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fac32", "fac16", "mixed", "mixed_f32"])
+def test_bicgstab_bitwise_tiled(system, gpu, precision):
+    name, prob, s, M = system
+    if CASES[name][4] != "bicg":
+        pytest.skip("symmetric system: PCG")
+    gpu.set_sweep_mode(0, 0)
+    base = _solve(gpu, s, M, "bicg", precision)
+    for mode in (3, 1):
+        _case(f"{name} bicg {precision} sweep mode {mode}")
+        gpu.set_sweep_mode(mode, 0)
+        got = _solve(gpu, s, M, "bicg", precision)
+        assert got[:3] == base[:3], (mode, got[:3], base[:3])
+        assert np.array_equal(got[3], base[3]) and np.array_equal(got[4], base[4]), mode
+    assert base[0] or precision != "fp64"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pcg_tiled(system, gpu, precision):
+    name, prob, s, M = system
+    if CASES[name][4] != "pcg":
+        pytest.skip("nonsymmetric system: BiCGSTAB")
+    gpu.set_sweep_mode(0, 0)
+    ok0, _, it0, x0, _ = _solve(gpu, s, M, "pcg", precision)
+    _case(f"{name} pcg {precision} tiled")
+    gpu.set_sweep_mode(3, 0)
+    ok, conv, it, x, _ = _solve(gpu, s, M, "pcg", precision)
+    assert ok and ok0 and conv < 1e-10 and abs(it - it0) <= 1
+    assert np.linalg.norm(x - x0) <= 1e-8 * np.linalg.norm(x0)
+
+
+def test_kernel_timer_runs_tiled(system, gpu):
+    import ctypes as C
+    from delfem_amd import _lib
+    name, prob, s, M = system
+    gpu.set_sweep_mode(3, 0)
+    for which in (1, 2):
+        ms, nbytes = C.c_double(), C.c_double()
+        _lib.call("dfm_bench_kernel", s.A.h, M.h, which, 0, 5, C.byref(ms), C.byref(nbytes))
+        assert ms.value > 0 and nbytes.value > 0
