@@ -600,6 +600,9 @@ static int upload_tile_plan(const TilePlan& tp, int bb, TileSweep& ts) {
   DFM_TRY(ts.rlev.upload(tp.rlev));
   DFM_TRY(ts.depth.upload(tp.depth));
   DFM_TRY(ts.split.upload(tp.split));
+  DFM_TRY(ts.rov.upload(tp.rov));
+  ts.ovf_in = tp.ovf_in;
+  ts.ovf_out = tp.ovf_out;
   DFM_TRY(v.val.alloc(std::max<size_t>(1, (size_t)tp.nk * bb * kSlice)));
   DFM_TRY(v.dval.alloc(std::max<size_t>(1, (size_t)v.nslice * bb * kSlice)));
   return DFM_OK;
@@ -738,8 +741,8 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
   // phase-tiled sweep layouts, for factors under a colour ordering
   TilePlan tpL, tpU;
   const bool tiled = M->phase_ptr.size() >= 2 &&
-                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, M->phase_ptr, false, g_tile_rows, tpL) &&
-                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, M->phase_ptr, true, g_tile_rows, tpU);
+                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, M->phase_ptr, false, g_tile_rows, (int)b, tpL) &&
+                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, M->phase_ptr, true, g_tile_rows, (int)b, tpU);
   // permuted copy of A: rows in factor numbering, columns renumbered and sorted
   {
     std::vector<unsigned> beg(nb), end(nb), colv(A->col.size());

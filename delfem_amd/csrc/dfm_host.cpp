@@ -380,7 +380,7 @@ void build_sell_plan(const std::vector<int>& rows_by_group, const std::vector<in
 // in the order of their first rows, rows of a tile in factor order.
 bool build_tile_plan(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
                      const std::vector<unsigned>& fdiaind, const std::vector<int>& phase_ptr, bool upper,
-                     int pack_rows, TilePlan& tp) {
+                     int pack_rows, int b, TilePlan& tp) {
   tp = TilePlan();
   const int np = (int)phase_ptr.size() - 1;
   if (np < 1 || phase_ptr[0] != 0 || phase_ptr[np] != (int)nb) return false;
@@ -485,6 +485,31 @@ bool build_tile_plan(unsigned nb, const std::vector<unsigned>& frow, const std::
     tp.slice_ptr[s + 1] = tp.split[s] + wb;
   }
   tp.nk = tp.slice_ptr[nslice];
+  // overflow LDS: per tile row, its in-phase slots beyond kTileInSlots and
+  // (upper) its later-phase slots beyond kTileOutSlots, packed row after row
+  tp.rov.assign(tile_rows.size() * 4, 0);
+  for (int t = 0; t < tp.ntile; t++) {
+    int oi = 0, oo = 0;
+    for (int r = 0; r < nt; r++) {
+      const size_t rix = (size_t)t * nt + r;
+      if (tile_rows[rix] < 0) continue;
+      const int nin = upper ? nfirst[rix] : nsecond[rix], nout = upper ? nsecond[rix] : 0;
+      const int ci = std::max(0, nin - kTileInSlots), co = std::max(0, nout - kTileOutSlots);
+      tp.rov[rix * 4 + 0] = oi;
+      tp.rov[rix * 4 + 1] = ci;
+      tp.rov[rix * 4 + 2] = oo;
+      tp.rov[rix * 4 + 3] = co;
+      oi += ci;
+      oo += co;
+    }
+    tp.ovf_in = std::max(tp.ovf_in, oi);
+    tp.ovf_out = std::max(tp.ovf_out, oo);
+  }
+  const long bytes = (long)tp.ovf_in * (b * b * 8 + 4) + (long)tp.ovf_out * b * 8;
+  if (bytes > kTileOverflowBytes) {
+    tp = TilePlan();
+    return false;
+  }
   tp.col.assign((size_t)tp.nk * kSlice, -1);
   tp.src.assign((size_t)tp.nk * kSlice, -1);
   std::vector<int> local(nb, -1);

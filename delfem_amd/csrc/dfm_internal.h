@@ -136,18 +136,29 @@ struct TilePlan {
   std::vector<int> depth;           // per tile: number of local levels
   std::vector<int> slice_ptr;       // ntile * nt / 64 + 1
   std::vector<int> split;           // per slice
+  std::vector<int> rov;             // ntile * nt * 4: per tile row, in the tile's overflow LDS, the first
+                                    // slot and count of its in-phase slots beyond kTileInSlots, then of
+                                    // (upper) its later-phase products beyond kTileOutSlots
+  int ovf_in = 0, ovf_out = 0;      // largest overflow of a tile, slots
   long nk = 0;                      // slots per lane over all slices
   std::vector<int> col;             // nk * 64
   std::vector<long long> src;       // nk * 64: factor entry, -1 padding
 };
 // Largest component a tile can hold.
 constexpr int kTileMaxRows = 256;
-// Builds the plan; pack_rows: components are packed into one tile while it
-// holds at most that many rows (a larger component takes a tile of its own).
-// Returns false (plan unusable) when a component exceeds kTileMaxRows rows.
+// In-phase slots a tile row holds in registers, and (upper) later-phase
+// products; a row's further slots go to the workgroup's overflow LDS, of at
+// most kTileOverflowBytes (FP64 values) per tile.
+constexpr int kTileInSlots = 4;
+constexpr int kTileOutSlots = 8;
+constexpr int kTileOverflowBytes = 48 * 1024;
+// Builds the plan for b x b blocks; pack_rows: components are packed into one
+// tile while it holds at most that many rows (a larger component takes a tile
+// of its own).  Returns false (plan unusable) when a component exceeds
+// kTileMaxRows rows or a tile's overflow exceeds kTileOverflowBytes.
 bool build_tile_plan(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
                      const std::vector<unsigned>& fdiaind, const std::vector<int>& phase_ptr, bool upper,
-                     int pack_rows, TilePlan& tp);
+                     int pack_rows, int b, TilePlan& tp);
 
 // Sliced-ELL layout plan: rows grouped into "groups" (level sets, or one group
 // for a plain SpMV), each group cut into slices of kSlice rows.  For slice s
@@ -204,9 +215,9 @@ struct SweepSync {
 // value copies in each storage type (v32 / v16 made with the factor's
 // FP32 / BF16 copies).  ntile == 0: the factor has no usable phase tiling.
 struct TileSweep {
-  int nt = 0, ntile = 0;
+  int nt = 0, ntile = 0, ovf_in = 0, ovf_out = 0;
   std::vector<int> phase_tile;
-  DevBuf<int> rlev, depth, split;
+  DevBuf<int> rlev, depth, split, rov;
   DevBuf<long long> src;
   Sell<double> v64;
   Sell<float> v32;

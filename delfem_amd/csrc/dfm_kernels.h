@@ -1307,23 +1307,28 @@ __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, con
 // Phase-tiled sweeps: one launch per phase (a colour of the ordering), one
 // workgroup per tile of whole dependency components, thread t = tile row t
 // (layout: TilePlan).  Before the first barrier every thread loads what does
-// not depend on this launch: its right-hand side, for a lower row the
-// subtraction of its columns in earlier phases (complete in memory; they come
-// first in column order) and its diagonal block, for an upper row the products
-// of its columns in later phases (subtracted last, in order), and the blocks
-// and tile positions of its first kTileInSlots in-phase columns.  Then the
-// tile runs its local levels: the rows of level l subtract their in-phase
-// columns, read from LDS where the rows of earlier levels left them, finish,
-// and publish to LDS and to the vector; a barrier ends the level.  Every
-// subtraction is the one of the per-level kernels in the same order (a
-// product formed first and then subtracted, blk_msub), and padding slots
-// subtract exact zeros, so FP64 results are bitwise those of the per-level
-// sweeps.  With DOT the tile's <rvec, v> partial goes to partials[tile].
-constexpr int kTileInSlots = 4;
-template <int B, typename T>
-__host__ __device__ constexpr int tile_out_slots() {
-  return 48 / (B * (int)sizeof(T) / 4) < 8 ? 48 / (B * (int)sizeof(T) / 4) : 8;
-}
+// not depend on this launch: its right-hand side, the blocks and tile
+// positions of its in-phase columns -- the first kTileInSlots in registers,
+// the rest in the workgroup's overflow LDS (the row's places there: rov) --
+// and, for a lower row, the subtraction of its columns in earlier phases
+// (complete in memory; they come first in column order) and its diagonal
+// block, for an upper row the products of its columns in later phases
+// (subtracted last, in order; kTileOutSlots in registers, the rest in the
+// overflow LDS).  Those columns' indices are fetched ahead of their blocks
+// and gathers, so the prologue costs two dependent memory round trips per
+// chunk of slots only once.  Then the tile runs its local levels with no
+// global memory access: the rows of level l gather their in-phase columns
+// from LDS (where the rows of earlier levels left them; padding slots read a
+// row of zeros) in one batch, subtract them, finish and publish to LDS; a
+// barrier ends the level.  Slice bounds are wave-uniform (a slice is one
+// wave), so the per-slot branches are scalar.  The results go to the vector
+// after the last level.  Every subtraction is the one of the per-level
+// kernels in the same order (the library is built without contraction, so a
+// block product formed first and subtracted after is exactly blk_msub's), and
+// padding slots subtract exact zeros, so FP64 results are bitwise those of
+// the per-level sweeps.  With DOT the tile's <rvec, v> partial goes to
+// partials[tile].
+
 // s[r] = sum_c A[r*B+c]*x[c], the sum blk_msub forms before subtracting it
 template <int B, typename T>
 __device__ inline void blk_prod(T* s, const T* a, const T* x) {
@@ -1336,20 +1341,95 @@ __device__ inline void blk_prod(T* s, const T* a, const T* x) {
   }
 }
 
+// Slots [k0 + base, k0 + base + m) (m <= 16, wave-uniform) of a lane whose
+// columns lie in other phases (complete in x): with PROD=false
+// t -= block * x slot after slot (blk_msub); with PROD=true the products of
+// slot k (counted from k0) are left in prod[k] for k < NP and in
+// ovf[(ovf0 + k - NP) * B ..] for NP <= k < nreal (the lane's real slots).
+// The m column indices are fetched at once, then blocks and gathers in
+// chunks of CH slots.  FIRST: base == 0, so slot numbers are constants.
+template <int B, typename TS, typename T, bool PROD, int NP, bool FIRST>
+__device__ __forceinline__ void tile_other16(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
+                                             int k0, int base, int m, int lane, T (&t)[B], T (&prod)[NP][B],
+                                             T* ovf, int ovf0, int nreal) {
+  constexpr int BB = B * B;
+  constexpr int E = slot_words<TS>(BB);
+  constexpr int CH = E * 4 <= 40 ? 8 : 4;
+  int cc[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++)
+    if (u < m) cc[u] = ld_stream(col + (size_t)(k0 + base + u) * 64 + lane);
+#pragma unroll
+  for (int c0 = 0; c0 < 16; c0 += CH) {
+    if (c0 < m) {
+      unsigned w[CH][E];
+      T xv[CH][B];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        if (c0 + u < m) {
+          load_words<BB, TS>(val, (size_t)(k0 + base + c0 + u), lane, w[u]);
+          const int c = cc[c0 + u];
+          const size_t cb = (size_t)(c >= 0 ? c : 0) * B;
+#pragma unroll
+          for (int e = 0; e < B; e++) xv[u][e] = c >= 0 ? x[cb + e] : (T)0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        if (c0 + u < m) {
+          T a[BB];
+          words_to_block<BB, TS, T>(w[u], a);
+          if constexpr (PROD) {
+            const int k = (FIRST ? 0 : base) + c0 + u;
+            if (FIRST && c0 + u < NP) {
+              blk_prod<B, T>(prod[(c0 + u) % NP], a, xv[u]);
+            } else if (k < nreal) {
+              T p[B];
+              blk_prod<B, T>(p, a, xv[u]);
+#pragma unroll
+              for (int r = 0; r < B; r++) ovf[(size_t)(ovf0 + k - NP) * B + r] = p[r];
+            }
+          } else {
+            blk_msub<B, T>(t, a, xv[u]);
+          }
+        }
+      }
+    }
+  }
+}
+// All n (wave-uniform) other-phase slots from k0, 16 at a time.
+template <int B, typename TS, typename T, bool PROD, int NP>
+__device__ __forceinline__ void tile_other(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
+                                           int k0, int n, int lane, T (&t)[B], T (&prod)[NP][B], T* ovf,
+                                           int ovf0, int nreal) {
+  static_assert(NP <= 16, "register products must come from the first 16 slots");
+  if (n > 0) tile_other16<B, TS, T, PROD, NP, true>(col, val, x, k0, 0, n < 16 ? n : 16, lane, t, prod, ovf, ovf0, nreal);
+  for (int base = 16; base < n; base += 16)
+    tile_other16<B, TS, T, PROD, NP, false>(col, val, x, k0, base, n - base < 16 ? n - base : 16, lane, t, prod, ovf,
+                                            ovf0, nreal);
+}
+
 template <int B, typename TS, typename T, int NT, bool UPPER, bool DOT>
 __global__ void __launch_bounds__(NT) k_trsv_tile(int tile0, const int* __restrict__ rows,
                                                   const int* __restrict__ rlev, const int* __restrict__ depth,
                                                   const int* __restrict__ sptr, const int* __restrict__ split,
                                                   const int* __restrict__ col, const TS* __restrict__ val,
-                                                  const TS* __restrict__ dval, const T* rhs, T* out,
+                                                  const TS* __restrict__ dval, const int* __restrict__ rov,
+                                                  int ovf_in, int ovf_out, const T* rhs, T* out,
                                                   const T* __restrict__ rvec, double* __restrict__ partials,
                                                   const KState* __restrict__ st) {
   if (st && st->done) return;
   constexpr int BB = B * B;
   constexpr int E = slot_words<TS>(BB);
   constexpr int PI = kTileInSlots;
-  constexpr int PO = UPPER ? tile_out_slots<B, T>() : 1;
-  __shared__ T xs[NT * B];
+  constexpr int PO = UPPER ? kTileOutSlots : 1;
+  __shared__ T xs[(NT + 1) * B];  // tile row NT: zeros
+  // overflow: ovf_in in-phase blocks, ovf_out later-phase products (upper),
+  // then the in-phase blocks' tile positions
+  extern __shared__ __align__(16) unsigned char tile_lds[];
+  T* ov_val = reinterpret_cast<T*>(tile_lds);
+  T* ov_prod = ov_val + (size_t)ovf_in * BB;
+  int* ov_col = reinterpret_cast<int*>(ov_prod + (size_t)ovf_out * B);
   const int tile = tile0 + blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63;
   const int s = tile * (NT / 64) + (tid >> 6);
@@ -1357,79 +1437,83 @@ __global__ void __launch_bounds__(NT) k_trsv_tile(int tile0, const int* __restri
   const int row = rows[rix];
   const int lev = rlev[rix];
   const int nlev = depth[tile];
-  const int k0 = sptr[s], km = split[s], k1 = sptr[s + 1];
+  // a slice is one wave: its bounds are uniform
+  const int k0 = __builtin_amdgcn_readfirstlane(sptr[s]);
+  const int km = __builtin_amdgcn_readfirstlane(split[s]);
+  const int k1 = __builtin_amdgcn_readfirstlane(sptr[s + 1]);
   const int ki = UPPER ? k0 : km, ni = UPPER ? km - k0 : k1 - km;  // in-phase part
   const int ko = UPPER ? km : k0, no = UPPER ? k1 - km : km - k0;  // the other phases
+  const int4 ov = reinterpret_cast<const int4*>(rov)[rix];  // in: first, count; out: first, count
+  if (tid < B) xs[NT * B + tid] = (T)0;
   T t[B], d[BB];
   int ci[PI];
   unsigned wi[PI][E];
-  T so[PO][B];  // upper, B > 1: products of the later-phase slots
-  T ao[PO], xo[PO];  // upper, B == 1: their factors (blk_msub<1> fuses them into the subtraction)
+  T so[PO][B];  // upper: products of the first later-phase slots
   double dsum[1] = {0.0};
+#pragma unroll
+  for (int u = 0; u < PI; u++) ci[u] = NT;
   if (row >= 0) {
 #pragma unroll
     for (int e = 0; e < B; e++) t[e] = rhs[(size_t)row * B + e];
-    if constexpr (!UPPER) {
-      sell_msub_range<B, TS, T>(col, val, out, ko, ko + no, lane, t);
-      load_block<BB, TS, T>(dval, (size_t)s, lane, d);
-    }
 #pragma unroll
     for (int u = 0; u < PI; u++) {
       if (u < ni) {
-        ci[u] = col[(size_t)(ki + u) * 64 + lane];
+        const int c = col[(size_t)(ki + u) * 64 + lane];
+        ci[u] = c >= 0 ? c : NT;
         load_words<BB, TS>(val, (size_t)(ki + u), lane, wi[u]);
       }
     }
-    if constexpr (UPPER) {
+    if constexpr (!UPPER) load_block<BB, TS, T>(dval, (size_t)s, lane, d);
+    for (int u = 0; u < ov.y; u++) {
+      T a[BB];
+      load_block<BB, TS, T>(val, (size_t)(ki + PI + u), lane, a);
+      const int o = ov.x + u;
 #pragma unroll
-      for (int u = 0; u < PO; u++) {
-        if (u < no) {
-          T a[BB], xv[B];
-          sell_slot<B, TS, T>(col, val, out, ko + u, lane, a, xv);
-          if constexpr (B == 1) {
-            ao[u] = a[0];
-            xo[u] = xv[0];
-          } else {
-            blk_prod<B, T>(so[u], a, xv);
-          }
-        }
-      }
+      for (int e = 0; e < BB; e++) ov_val[(size_t)o * BB + e] = a[e];
+      const int c = col[(size_t)(ki + PI + u) * 64 + lane];
+      ov_col[o] = c >= 0 ? c : NT;
     }
+    tile_other<B, TS, T, UPPER, PO>(col, val, out, ko, no, lane, t, so, ov_prod, ov.z, PO + ov.w);
   }
+  __syncthreads();  // the zero row
   for (int l = 0; l < nlev; l++) {
     if (row >= 0 && lev == l) {
+      T xv[PI][B];
+#pragma unroll
+      for (int u = 0; u < PI; u++)
+#pragma unroll
+        for (int e = 0; e < B; e++) xv[u][e] = xs[ci[u] * B + e];
 #pragma unroll
       for (int u = 0; u < PI; u++) {
         if (u < ni) {
-          T a[BB], xv[B];
+          T a[BB];
           words_to_block<BB, TS, T>(wi[u], a);
-          const int c = ci[u];
-#pragma unroll
-          for (int e = 0; e < B; e++) xv[e] = c >= 0 ? xs[c * B + e] : (T)0;
-          blk_msub<B, T>(t, a, xv);
+          blk_msub<B, T>(t, a, xv[u]);
         }
       }
-      for (int u = PI; u < ni; u++) {
-        T a[BB], xv[B];
-        const int c = col[(size_t)(ki + u) * 64 + lane];
-        load_block<BB, TS, T>(val, (size_t)(ki + u), lane, a);
+      for (int u = 0; u < ov.y; u++) {
+        T a[BB], x2[B];
+        const int o = ov.x + u;
+        const int c = ov_col[o];
 #pragma unroll
-        for (int e = 0; e < B; e++) xv[e] = c >= 0 ? xs[c * B + e] : (T)0;
-        blk_msub<B, T>(t, a, xv);
+        for (int e = 0; e < BB; e++) a[e] = ov_val[(size_t)o * BB + e];
+#pragma unroll
+        for (int e = 0; e < B; e++) x2[e] = xs[c * B + e];
+        blk_msub<B, T>(t, a, x2);
       }
       if constexpr (UPPER) {
 #pragma unroll
         for (int u = 0; u < PO; u++) {
           if (u < no) {
-            if constexpr (B == 1) {
-              t[0] -= ao[u] * xo[u];
-            } else {
 #pragma unroll
-              for (int r = 0; r < B; r++) t[r] -= so[u][r];
-            }
+            for (int r = 0; r < B; r++) t[r] -= so[u][r];
           }
         }
-        if (no > PO) sell_msub_range<B, TS, T>(col, val, out, ko + PO, ko + no, lane, t);
+        for (int u = 0; u < ov.w; u++) {
+          const int o = ov.z + u;
+#pragma unroll
+          for (int r = 0; r < B; r++) t[r] -= ov_prod[(size_t)o * B + r];
+        }
       }
       T y[B];
       if constexpr (UPPER) {
@@ -1441,11 +1525,17 @@ __global__ void __launch_bounds__(NT) k_trsv_tile(int tile0, const int* __restri
 #pragma unroll
       for (int e = 0; e < B; e++) {
         xs[tid * B + e] = y[e];
-        out[(size_t)row * B + e] = y[e];
-        if (DOT) dsum[0] += (double)rvec[(size_t)row * B + e] * (double)y[e];
+        t[e] = y[e];
       }
     }
     __syncthreads();
+  }
+  if (row >= 0) {
+#pragma unroll
+    for (int e = 0; e < B; e++) {
+      out[(size_t)row * B + e] = t[e];
+      if (DOT) dsum[0] += (double)rvec[(size_t)row * B + e] * (double)t[e];
+    }
   }
   if constexpr (DOT) write_partials<1, NT>(dsum, partials, tile, 0);
 }

@@ -171,11 +171,14 @@ static int tile_sweep_nt(const TileSweep& ts, const T* rhs, T* out, const T* rve
                          const KState* st) {
   const Sell<TS>& v = tile_vals<TS>(ts);
   const Sell<double>& lay = ts.v64;
+  const size_t shmem = (size_t)ts.ovf_in * (B * B * sizeof(T) + sizeof(int)) + (size_t)ts.ovf_out * B * sizeof(T);
   for (size_t p = 0; p + 1 < ts.phase_tile.size(); p++) {
     const int t0 = ts.phase_tile[p], nt = ts.phase_tile[p + 1] - t0;
     if (nt == 0) continue;
-    DFM_LAUNCH_NT((k_trsv_tile<B, TS, T, NT, UPPER, DOT>), nt, NT, t0, lay.rows.p, ts.rlev.p, ts.depth.p,
-                  lay.slice_ptr.p, ts.split.p, lay.col.p, v.val.p, v.dval.p, rhs, out, rvec, partials, st);
+    hipLaunchKernelGGL((k_trsv_tile<B, TS, T, NT, UPPER, DOT>), dim3(nt), dim3(NT), shmem, stream(), t0,
+                       lay.rows.p, ts.rlev.p, ts.depth.p, lay.slice_ptr.p, ts.split.p, lay.col.p, v.val.p, v.dval.p,
+                       ts.rov.p, ts.ovf_in, ts.ovf_out, rhs, out, rvec, partials, st);
+    DFM_HIP(hipGetLastError());
   }
   return DFM_OK;
 }

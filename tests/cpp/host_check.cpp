@@ -7,7 +7,8 @@
 //
 // usage: host_check <in> <out>
 //   in : int32 nnode, nelem, npe, lev, ordering (0 natural, 2 multicolour,
-//        3 block multicolour), block_nodes, pack_rows; uint32 conn[nelem*npe]
+//        3 block multicolour), block_nodes, pack_rows, block size b;
+//        uint32 conn[nelem*npe]
 //   out: int32 status (0 ok), nnz, nf, ncolour, nlev_lower, nlev_upper,
 //        tile threads L / U, tiles L / U; uint32 rowptr[nnode+1], col[nnz];
 //        int32 order[nnode]; uint32 frow[nnode+1], fcol[nf]; int32 flev[nf];
@@ -127,6 +128,27 @@ static void check_tiles(const TilePlan& tp, unsigned nb, const std::vector<unsig
       EXPECT(q == qend, "slice %d lane %d: %u of the row's entries unplaced", s, l, qend - q);
     }
   }
+  // overflow LDS: each row's slots beyond the register budgets, packed row
+  // after row inside the largest tile's area
+  EXPECT(tp.rov.size() == tp.rows.size() * 4, "overflow places");
+  if (g_fail) return;
+  for (int t = 0; t < tp.ntile; t++) {
+    int oi = 0, oo = 0;
+    for (int r = 0; r < tp.nt; r++) {
+      const size_t rix = (size_t)t * tp.nt + r;
+      const int i = tp.rows[rix];
+      if (i < 0) continue;
+      int nin = 0, nout = 0;
+      const unsigned q0 = upper ? fdia[i] : frow[i], q1 = upper ? frow[i + 1] : fdia[i];
+      for (unsigned q = q0; q < q1; q++) (phase_of[fcol[q]] == phase_of[i] ? nin : nout)++;
+      const int ci = std::max(0, nin - kTileInSlots), co = upper ? std::max(0, nout - kTileOutSlots) : 0;
+      EXPECT(tp.rov[rix * 4] == oi && tp.rov[rix * 4 + 1] == ci, "tile %d row %d in-phase overflow", t, r);
+      EXPECT(tp.rov[rix * 4 + 2] == oo && tp.rov[rix * 4 + 3] == co, "tile %d row %d later-phase overflow", t, r);
+      oi += ci;
+      oo += co;
+    }
+    EXPECT(oi <= tp.ovf_in && oo <= tp.ovf_out, "tile %d overflow beyond the launch's area", t);
+  }
 }
 
 int main(int argc, char** argv) {
@@ -136,10 +158,10 @@ int main(int argc, char** argv) {
   }
   FILE* f = fopen(argv[1], "rb");
   if (!f) return 2;
-  int hdr[7];
-  if (fread(hdr, sizeof(int), 7, f) != 7) return 2;
+  int hdr[8];
+  if (fread(hdr, sizeof(int), 8, f) != 8) return 2;
   const unsigned nnode = hdr[0], nelem = hdr[1], npe = hdr[2];
-  const int lev = hdr[3], ordering = hdr[4], block_nodes = hdr[5], pack_rows = hdr[6];
+  const int lev = hdr[3], ordering = hdr[4], block_nodes = hdr[5], pack_rows = hdr[6], bsize = hdr[7];
   std::vector<unsigned> conn((size_t)nelem * npe);
   if (fread(conn.data(), sizeof(unsigned), conn.size(), f) != conn.size()) return 2;
   fclose(f);
@@ -192,12 +214,12 @@ int main(int argc, char** argv) {
   int ntL = 0, ntU = 0, tiL = 0, tiU = 0;
   if (phase_ptr.size() >= 2) {
     TilePlan tL, tU;
-    if (build_tile_plan(nnode, frow, fcol, fdia, phase_ptr, false, pack_rows, tL)) {
+    if (build_tile_plan(nnode, frow, fcol, fdia, phase_ptr, false, pack_rows, bsize, tL)) {
       check_tiles(tL, nnode, frow, fcol, fdia, phase_ptr, false);
       ntL = tL.nt;
       tiL = tL.ntile;
     }
-    if (build_tile_plan(nnode, frow, fcol, fdia, phase_ptr, true, pack_rows, tU)) {
+    if (build_tile_plan(nnode, frow, fcol, fdia, phase_ptr, true, pack_rows, bsize, tU)) {
       check_tiles(tU, nnode, frow, fcol, fdia, phase_ptr, true);
       ntU = tU.nt;
       tiU = tU.ntile;

@@ -41,7 +41,7 @@ def run(exe, tmp_path, prob, lev, ordering, block_nodes=16, pack_rows=64):
     nelem, npe = conn.shape
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(inp, "wb") as f:
-        f.write(np.array([prob.nnode, nelem, npe, lev, ORDER[ordering], block_nodes, pack_rows],
+        f.write(np.array([prob.nnode, nelem, npe, lev, ORDER[ordering], block_nodes, pack_rows, prob.b],
                          dtype=np.int32).tobytes())
         f.write(conn.tobytes())
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
