@@ -1364,6 +1364,13 @@ int dfm_zilu_solve_precond(dfm_zilu* Z, double* vec) {
   return DFM_OK;
 }
 
+// Synchronises the stream when it goes out of scope: declared after a local
+// workspace of a loop that defers synchronisation, it keeps the workspace
+// alive until the kernels queued on it have run.
+struct StreamDrain {
+  ~StreamDrain() { (void)hipStreamSynchronize(stream()); }
+};
+
 // ---------------------------------------------------------------------------
 // MatVec::Sol complex solvers.  r: right-hand side in, final residual out;
 // u: solution out.  The preconditioned ones run in the factor's numbering
@@ -1415,6 +1422,7 @@ static int zcg_herm(dfm_zbcrs* A, dfm_zilu* Z, double* r, double* u, double* con
   }
   const double inv = 1.0 / sq0;
   DevBuf<double> w;
+  StreamDrain drain_w;  // destroyed before w: queued kernels finish before w is freed
   DFM_TRY(w.alloc(3 * std::max<size_t>(1, n2)));
   double *p = w.p, *Ap = w.p + n2, *z = w.p + 2 * n2;
   DFM_TRY(dfm_vec_fill(3 * n2, 0.0, w.p));
@@ -1514,6 +1522,7 @@ static int zcgnr(dfm_zbcrs* A, double* r, double* u, double* conv_ratio, unsigne
   }
   const double inv = 1.0 / sq0;
   DevBuf<double> w;
+  StreamDrain drain_w;  // destroyed before w: queued kernels finish before w is freed
   DFM_TRY(w.alloc(3 * std::max<size_t>(1, n2)));
   double *p = w.p, *z = w.p + n2, *Ap = w.p + 2 * n2;
   DFM_TRY(dfm_vec_fill(3 * n2, 0.0, w.p));

@@ -32,7 +32,8 @@ struct dfm_mseg_s {
   // DOT: per-segment partial sums and results, read back with one copy
   DevBuf<double> dot_partials;
   DevBuf<KState> dot_state;
-  KState* dot_host = nullptr;
+  KState* dot_host = nullptr;  // pinned readback of the per-segment dots, dot_host_n slots
+  unsigned dot_host_n = 0;
   ~dfm_mseg_s() {
     if (dot_host) (void)hipHostFree(dot_host);
   }
@@ -363,7 +364,13 @@ int seg_dot(dfm_mseg_s* S, const double* x, const double* y, double* out) {
     DFM_TRY(S->dot_state.alloc(n));
     DFM_HIP(hipMemsetAsync(S->dot_state.p, 0, sizeof(KState) * n, stream()));
   }
-  if (!S->dot_host) DFM_HIP(hipHostMalloc((void**)&S->dot_host, sizeof(KState) * 16, hipHostMallocDefault));
+  if (S->dot_host_n < n) {
+    if (S->dot_host) DFM_HIP(hipHostFree(S->dot_host));
+    S->dot_host = nullptr;
+    S->dot_host_n = 0;
+    DFM_HIP(hipHostMalloc((void**)&S->dot_host, sizeof(KState) * n, hipHostMallocDefault));
+    S->dot_host_n = n;
+  }
   for (unsigned i = 0; i < n; i++) {
     double* part = S->dot_partials.p + (size_t)i * gmax;
     DFM_MLAUNCH(k_dot<double>, g[i], (long)(S->off[i + 1] - S->off[i]), x + S->off[i], y + S->off[i], part);
