@@ -21,8 +21,22 @@ velocity and pressure segments, ILU(1) with fill between segments,
 BiCGSTAB to 1e-9).
 
 Prints one JSON line:  python bench.py [--steps K] [--warmup W] [--config cfg3]
+
+Profiling aids (no JSON line):
+  python bench.py --kernel-loop WHICH --kernel-precision P [--nrep N] [--copy-mib M]
+      the default workload's system, then one of the iteration's kernels
+      (WHICH 0 SpMV, 1 lower sweep, 2 upper sweep) launched N times on its
+      real operands, and a copy kernel of M MiB: the program to run under
+      rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE.
+  python bench.py --summarize-pmc FETCH_CSV WRITE_CSV --kernel NAME --out FILE
+      HBM bytes per launch of NAME from the two counter passes, each counter
+      scaled by what it reads for the copy kernel of known size.
+  python bench.py --summarize-trace KERNEL_TRACE_CSV --kernel NAME --out FILE
+      per-kernel statistics of a rocprofv3 --kernel-trace run, with NAME's
+      average over its launches that did work (at least half the median).
 """
 import argparse
+import csv
 import ctypes as C
 import json
 import os
@@ -104,6 +118,109 @@ def read_traffic(kernel):
     return None
 
 
+def kernel_loop(args):
+    """One kernel of the default workload's iteration, isolated, for counter runs."""
+    import torch
+    from delfem_amd import linsys, _lib
+    linsys.init(0)
+    cfg = CONFIGS["cfg3"]
+    t0 = time.time()
+    s = linsys.FEMSystem(make_problem(cfg, args.seed))
+    s.assemble()
+    M = linsys.CPreconditioner_ILU(s.A, cfg["lev"], ordering=cfg["ordering"])
+    M.SetValue(s.A)
+    torch.cuda.synchronize()
+    log(f"[loop] {cfg['workload']}: ready in {time.time() - t0:.1f}s")
+    a, b = C.c_double(), C.c_double()
+    _lib.call("dfm_bench_kernel", s.A.h, M.h, args.kernel_loop, args.kernel_precision, args.nrep, C.byref(a),
+              C.byref(b))
+    log(f"[loop] kernel which={args.kernel_loop} precision={args.kernel_precision}: {a.value:.5f} ms/launch, "
+        f"{b.value / 1e6:.2f} MB algorithmic, {b.value / a.value / 1e6:.1f} GB/s")
+    nbytes = args.copy_mib << 20
+    _lib.call("dfm_bench_copy", C.c_size_t(nbytes), args.nrep, C.byref(a))
+    log(f"[loop] copy {args.copy_mib} MiB: {a.value:.5f} ms/launch, {2 * nbytes / a.value / 1e6:.1f} GB/s "
+        f"(read + write)")
+
+
+def _mean_counter(path, kernel_substr, counter):
+    vals = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            if kernel_substr not in row.get("Kernel_Name", "") or row.get("Counter_Name") != counter:
+                continue
+            did = row.get("Dispatch_Id")
+            vals[did] = vals.get(did, 0.0) + float(row["Counter_Value"])
+    if not vals:
+        raise SystemExit(f"{path}: no {counter} rows for a kernel matching {kernel_substr!r}")
+    return sum(vals.values()) / len(vals), len(vals)
+
+
+def summarize_pmc(args):
+    """HBM bytes per launch from a FETCH_SIZE pass and a WRITE_SIZE pass."""
+    fetch_csv, write_csv = args.summarize_pmc
+    nbytes = float(args.copy_mib << 20)
+    fk, nf = _mean_counter(fetch_csv, args.kernel, "FETCH_SIZE")
+    fc, _ = _mean_counter(fetch_csv, "k_copy4", "FETCH_SIZE")
+    wk, nw = _mean_counter(write_csv, args.kernel, "WRITE_SIZE")
+    wc, _ = _mean_counter(write_csv, "k_copy4", "WRITE_SIZE")
+    f_fetch, f_write = nbytes / fc, nbytes / wc  # bytes per counter unit, from the copy kernel
+    out = dict(kernel=args.kernel, bytes_per_launch=round(fk * f_fetch + wk * f_write),
+               fetch_bytes=round(fk * f_fetch), write_bytes=round(wk * f_write),
+               fetch_counter=fk, write_counter=wk, copy_fetch_counter=fc, copy_write_counter=wc,
+               fetch_scale=f_fetch, write_scale=f_write, dispatches=[nf, nw], copy_bytes=nbytes)
+    with open(args.out, "w") as f:
+        f.write(f"# HBM traffic per launch of {args.kernel}\n")
+        f.write(f"# {args.note}\n" if args.note else "")
+        f.write("# FETCH_SIZE and WRITE_SIZE (rocprofv3 --pmc, one pass each) averaged over the kernel's\n")
+        f.write("# dispatches, each scaled by copy_bytes / (the same counter for k_copy4 moving copy_bytes).\n")
+        f.write(json.dumps(out) + "\n")
+    print(json.dumps(out))
+
+
+def summarize_trace(args):
+    """Per-kernel statistics of a kernel trace; the named kernel also over its working launches."""
+    import re
+    per = {}
+    rows = []
+    with open(args.summarize_trace, newline="") as f:
+        for row in csv.DictReader(f):
+            rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]),
+                         re.sub(r"\(.*", "", row["Kernel_Name"]).strip()))
+    rows.sort()
+    for t0, t1, name in rows:
+        per.setdefault(name, []).append((t1 - t0) / 1e3)
+    total = sum(sum(v) for v in per.values())
+    lines = []
+    for name, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"{100 * sum(v) / total:6.2f}% {len(v):8d} {sum(v) / len(v):10.2f} {min(v):10.2f} / "
+                     f"{max(v):9.2f}  {name}")
+    key = None
+    for name in per:
+        if args.kernel and name.endswith(args.kernel):
+            key = name
+    with open(args.out, "w") as f:
+        f.write(f"# {args.note}\n" if args.note else "")
+        f.write("# columns: share of GPU time, calls, average duration (us), min / max duration (us), kernel\n")
+        for ln in lines[:40]:
+            f.write(ln + "\n")
+        def working(durs):
+            half = 0.5 * sorted(durs)[len(durs) // 2]
+            return [d for d in durs if d >= half]
+
+        if key:
+            v = per[key]
+            w = working(v)
+            f.write(f"# {key}: {len(v)} launches, average {sum(v) / len(v):.2f} us; the {len(w)} that did work "
+                    f"(>= half the median) average {sum(w) / len(w):.2f} us\n")
+            if args.tail and args.tail <= len(v):
+                t = v[-args.tail:]
+                tw = working(t)
+                f.write(f"# the last {len(t)} launches (bench.py's in-solve timing solve, launched directly with "
+                        f"start / stop events): average {sum(t) / len(t):.2f} us; the {len(tw)} that did work "
+                        f"average {sum(tw) / len(tw):.2f} us\n")
+    print(open(args.out).read())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,12 +232,29 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=40, help="Krylov iterations in the CPU sample")
     ap.add_argument("--skip-natural", action="store_true",
                     help="multi-segment configs: leave out the natural-numbering comparison solve")
+    ap.add_argument("--kernel-loop", type=int, default=None, metavar="WHICH",
+                    help="profiling aid: run one iteration kernel of the default workload in a loop")
+    ap.add_argument("--kernel-precision", type=int, default=4)
+    ap.add_argument("--nrep", type=int, default=50)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV")
+    ap.add_argument("--tail", type=int, default=0, help="--summarize-trace: also average the kernel's last N launches")
+    ap.add_argument("--kernel", default="")
+    ap.add_argument("--out", default="")
+    ap.add_argument("--note", default="")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        return summarize_pmc(args)
+    if args.summarize_trace:
+        return summarize_trace(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1", "n_gpus": args.gpus}),
               flush=True)
         sys.exit(2)
 
+    if args.kernel_loop is not None:
+        return kernel_loop(args)
     import torch
     from delfem_amd import linsys, _lib
 
@@ -214,6 +348,22 @@ def main():
     head = cfg.get("headline", mixed)
     head_ms = fp64_ms if head == "fp64" else mixed_ms
     dom = kern[f"spmv_{head}"]
+    # the same kernel inside a solve: one more solve of the headline mode whose
+    # iteration SpMVs carry kernel start / stop events (launched directly, not
+    # from graphs); the launches after convergence inside the last block of
+    # iterations return at once and are left out
+    _lib.call("dfm_set_kernel_timing", 1)
+    try:
+        step(head)
+        t_all, t_work, n_all, n_work = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        _lib.call("dfm_get_kernel_timing", C.byref(t_all), C.byref(t_work), C.byref(n_all), C.byref(n_work))
+    finally:
+        _lib.call("dfm_set_kernel_timing", 0)
+    in_solve = dict(avg_ms=round(t_work.value, 5), launches=n_work.value, all_avg_ms=round(t_all.value, 5),
+                    all_launches=n_all.value)
+    in_gbs = dom["bytes"] / (t_work.value * 1e-3) / 1e9 if t_work.value > 0 else 0.0
+    log(f"[bench] in-solve SpMV ({head}): {t_work.value:.4f} ms over {n_work.value} working launches "
+        f"({n_all.value} in all, {t_all.value:.4f} ms), {in_gbs:.0f} GB/s ({100 * in_gbs / HBM_PEAK_GBS:.1f}% of peak)")
     kname = {"mixed_f32": "k_spmv<3, float, float, true, true>", "mixed_f32_bf16": "k_spmv<3, float, float, true, true>",
              "mixed": "k_spmv<3, float, double, true, true>", "mixed_bf16": "k_spmv<3, float, double, true, true>"
              }.get(head, "k_spmv<3, double, double, true, true>")
@@ -261,9 +411,10 @@ def main():
         "other_precisions": others,
         "assembly_ms": round(asm_ms, 3),
         "kernels": kern,
-        "roofline": {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": dom["frac"], "traffic": tr["bytes_per_launch"] if tr else None,
-                     "kernel": kname, "algorithmic_bytes": dom["bytes"], "avg_launch_ms": dom["ms"],
+        "roofline": {"bound": "hbm", "achieved": round(in_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(in_gbs / HBM_PEAK_GBS, 4), "traffic": tr["bytes_per_launch"] if tr else None,
+                     "kernel": kname, "algorithmic_bytes": dom["bytes"], "avg_launch_ms": in_solve["avg_ms"],
+                     "in_solve": in_solve, "isolated_loop": {"avg_ms": dom["ms"], "frac": dom["frac"]},
                      "measured_read_ceiling_gbs": round(ceiling, 1)},
         "cpu_baseline": cpu,
     }

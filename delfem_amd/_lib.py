@@ -112,6 +112,8 @@ _SIGS = {
     "dfm_ilu_solve_precond_noalloc": (C.c_int, [H, vp]),
     "dfm_ilu_solve_precond": (C.c_int, [H, vp]),
     "dfm_set_sweep_mode": (C.c_int, [C.c_int, C.c_int]),
+    "dfm_set_kernel_timing": (C.c_int, [C.c_int]),
+    "dfm_get_kernel_timing": (C.c_int, [f64p, f64p, i32p, i32p]),
     "dfm_solve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_solve_pbicgstab": (C.c_int, [H, H, vp, vp, f64p, u32p, C.POINTER(SolveOpts)]),
     "dfm_bmat_create": (C.c_int, [C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.POINTER(H)]),
@@ -140,6 +142,7 @@ _SIGS = {
     "dfm_mseg_matvec": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
     "dfm_mseg_dot": (C.c_int, [H, vp, vp, f64p]),
     "dfm_mseg_ilu_create": (C.c_int, [H, C.c_int, C.POINTER(H)]),
+    "dfm_mseg_ilu_create_levels": (C.c_int, [H, vp, C.POINTER(H)]),
     "dfm_mseg_ilu_get_pattern": (C.c_int, [H, C.c_uint, C.c_uint, u32p, vp, vp]),
     "dfm_mseg_ilu_destroy": (C.c_int, [H]),
     "dfm_mseg_ilu_set_value": (C.c_int, [H, i32p]),

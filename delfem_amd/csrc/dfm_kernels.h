@@ -998,25 +998,100 @@ static __global__ void k_sell_gather_dia(long nslot, int bb, const int* __restri
 }
 
 
-// acc += sum over slots k0 <= k < k1 (in that order) of block(k) * x[col(k)],
-// four slots' loads issued before their products.
+// s[r] = sum_c A[r*B+c]*x[c], the sum blk_msub forms before subtracting it
+template <int B, typename T>
+__device__ inline void blk_prod(T* s, const T* a, const T* x) {
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    T v = a[r * B] * x[0];
+#pragma unroll
+    for (int c = 1; c < B; c++) v = v + a[r * B + c] * x[c];
+    s[r] = v;
+  }
+}
+
+// What the slot helpers do with each slot's block a and gathered x.
+enum { SLOT_MSUB = 0, SLOT_PROD = 1, SLOT_MADD = 2 };
+// Slots [k0 + base, k0 + base + m) (m <= 16, wave-uniform) of a lane whose
+// columns are complete in x, in order: SLOT_MSUB t -= block * x (blk_msub),
+// SLOT_MADD t += block * x (blk_madd), SLOT_PROD the products of slot k
+// (counted from k0) left in prod[k] for k < NP and in
+// ovf[(ovf0 + k - NP) * B ..] for NP <= k < nreal (the lane's real slots).
+// The m column indices are fetched at once, then blocks and gathers in chunks
+// of CH slots, so a lane waits for two memory round trips only once.
+// FIRST: base == 0, so slot numbers are constants.
+template <int B, typename TS, typename T, int OP, int NP, bool FIRST>
+__device__ __forceinline__ void sell_slots16(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
+                                             int k0, int base, int m, int lane, T (&t)[B], T (&prod)[NP][B],
+                                             T* ovf, int ovf0, int nreal) {
+  constexpr int BB = B * B;
+  constexpr int E = slot_words<TS>(BB);
+  constexpr int CH = E * 4 <= 40 ? 8 : 4;
+  int cc[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++)
+    if (u < m) cc[u] = ld_stream(col + (size_t)(k0 + base + u) * 64 + lane);
+#pragma unroll
+  for (int c0 = 0; c0 < 16; c0 += CH) {
+    if (c0 < m) {
+      unsigned w[CH][E];
+      T xv[CH][B];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        if (c0 + u < m) {
+          load_words<BB, TS>(val, (size_t)(k0 + base + c0 + u), lane, w[u]);
+          const int c = cc[c0 + u];
+          const size_t cb = (size_t)(c >= 0 ? c : 0) * B;
+#pragma unroll
+          for (int e = 0; e < B; e++) xv[u][e] = c >= 0 ? x[cb + e] : (T)0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        if (c0 + u < m) {
+          T a[BB];
+          words_to_block<BB, TS, T>(w[u], a);
+          if constexpr (OP == SLOT_PROD) {
+            const int k = (FIRST ? 0 : base) + c0 + u;
+            if (FIRST && c0 + u < NP) {
+              blk_prod<B, T>(prod[(c0 + u) % NP], a, xv[u]);
+            } else if (k < nreal) {
+              T p[B];
+              blk_prod<B, T>(p, a, xv[u]);
+#pragma unroll
+              for (int r = 0; r < B; r++) ovf[(size_t)(ovf0 + k - NP) * B + r] = p[r];
+            }
+          } else if constexpr (OP == SLOT_MADD) {
+            blk_madd<B, T>(t, a, xv[u]);
+          } else {
+            blk_msub<B, T>(t, a, xv[u]);
+          }
+        }
+      }
+    }
+  }
+}
+// All n (wave-uniform) slots from k0, 16 at a time.
+template <int B, typename TS, typename T, int OP, int NP>
+__device__ __forceinline__ void sell_slots(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
+                                           int k0, int n, int lane, T (&t)[B], T (&prod)[NP][B], T* ovf,
+                                           int ovf0, int nreal) {
+  static_assert(NP <= 16, "register products must come from the first 16 slots");
+  if (n > 0) sell_slots16<B, TS, T, OP, NP, true>(col, val, x, k0, 0, n < 16 ? n : 16, lane, t, prod, ovf, ovf0, nreal);
+  for (int base = 16; base < n; base += 16)
+    sell_slots16<B, TS, T, OP, NP, false>(col, val, x, k0, base, n - base < 16 ? n - base : 16, lane, t, prod, ovf,
+                                          ovf0, nreal);
+}
+
+// acc += sum over slots k0 <= k < k1 (in that order) of block(k) * x[col(k)];
+// k0 and k1 are the same across the wave (one slice per wave).
 template <int B, typename TS, typename T>
 __device__ __forceinline__ void sell_madd_range(const int* __restrict__ col, const TS* __restrict__ val,
                                                 const T* __restrict__ x, int k0, int k1, int lane, T (&acc)[B]) {
-  constexpr int BB = B * B;
-  int k = k0;
-  for (; k + 3 < k1; k += 4) {
-    T a[4][BB], v[4][B];
-#pragma unroll
-    for (int u = 0; u < 4; u++) sell_slot<B, TS, T>(col, val, x, k + u, lane, a[u], v[u]);
-#pragma unroll
-    for (int u = 0; u < 4; u++) blk_madd<B, T>(acc, a[u], v[u]);
-  }
-  for (; k < k1; k++) {
-    T a[BB], v[B];
-    sell_slot<B, TS, T>(col, val, x, k, lane, a, v);
-    blk_madd<B, T>(acc, a, v);
-  }
+  T none[1][B];
+  k0 = __builtin_amdgcn_readfirstlane(k0);
+  k1 = __builtin_amdgcn_readfirstlane(k1);
+  sell_slots<B, TS, T, SLOT_MADD, 1>(col, val, x, k0, k1 - k0, lane, acc, none, nullptr, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1069,28 +1144,15 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
   if constexpr (DOTS) write_partials<3>(dsum, partials, blockIdx.x, gridDim.x);
 }
 
-// t -= sum over slots k0 <= k < k1 (in that order) of block(k) * v[col(k)]:
-// U slots' loads are issued before their products, so a lane waits for a few
-// memory round trips per row rather than one per slot (eight slots per batch
-// measured no faster).
+// t -= sum over slots k0 <= k < k1 (in that order) of block(k) * v[col(k)];
+// k0 and k1 are the same across the wave (one slice per wave).
 template <int B, typename TS, typename T>
 __device__ __forceinline__ void sell_msub_range(const int* __restrict__ col, const TS* __restrict__ val,
                                                 const T* v, int k0, int k1, int lane, T (&t)[B]) {
-  constexpr int BB = B * B;
-  constexpr int U = 4;
-  int k = k0;
-  for (; k + U - 1 < k1; k += U) {
-    T a[U][BB], x[U][B];
-#pragma unroll
-    for (int u = 0; u < U; u++) sell_slot<B, TS, T>(col, val, v, k + u, lane, a[u], x[u]);
-#pragma unroll
-    for (int u = 0; u < U; u++) blk_msub<B, T>(t, a[u], x[u]);
-  }
-  for (; k < k1; k++) {
-    T a[BB], x[B];
-    sell_slot<B, TS, T>(col, val, v, k, lane, a, x);
-    blk_msub<B, T>(t, a, x);
-  }
+  T none[1][B];
+  k0 = __builtin_amdgcn_readfirstlane(k0);
+  k1 = __builtin_amdgcn_readfirstlane(k1);
+  sell_slots<B, TS, T, SLOT_MSUB, 1>(col, val, v, k0, k1 - k0, lane, t, none, nullptr, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1329,86 +1391,6 @@ __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, con
 // the per-level sweeps.  With DOT the tile's <rvec, v> partial goes to
 // partials[tile].
 
-// s[r] = sum_c A[r*B+c]*x[c], the sum blk_msub forms before subtracting it
-template <int B, typename T>
-__device__ inline void blk_prod(T* s, const T* a, const T* x) {
-#pragma unroll
-  for (int r = 0; r < B; r++) {
-    T v = a[r * B] * x[0];
-#pragma unroll
-    for (int c = 1; c < B; c++) v = v + a[r * B + c] * x[c];
-    s[r] = v;
-  }
-}
-
-// Slots [k0 + base, k0 + base + m) (m <= 16, wave-uniform) of a lane whose
-// columns lie in other phases (complete in x): with PROD=false
-// t -= block * x slot after slot (blk_msub); with PROD=true the products of
-// slot k (counted from k0) are left in prod[k] for k < NP and in
-// ovf[(ovf0 + k - NP) * B ..] for NP <= k < nreal (the lane's real slots).
-// The m column indices are fetched at once, then blocks and gathers in
-// chunks of CH slots.  FIRST: base == 0, so slot numbers are constants.
-template <int B, typename TS, typename T, bool PROD, int NP, bool FIRST>
-__device__ __forceinline__ void tile_other16(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
-                                             int k0, int base, int m, int lane, T (&t)[B], T (&prod)[NP][B],
-                                             T* ovf, int ovf0, int nreal) {
-  constexpr int BB = B * B;
-  constexpr int E = slot_words<TS>(BB);
-  constexpr int CH = E * 4 <= 40 ? 8 : 4;
-  int cc[16];
-#pragma unroll
-  for (int u = 0; u < 16; u++)
-    if (u < m) cc[u] = ld_stream(col + (size_t)(k0 + base + u) * 64 + lane);
-#pragma unroll
-  for (int c0 = 0; c0 < 16; c0 += CH) {
-    if (c0 < m) {
-      unsigned w[CH][E];
-      T xv[CH][B];
-#pragma unroll
-      for (int u = 0; u < CH; u++) {
-        if (c0 + u < m) {
-          load_words<BB, TS>(val, (size_t)(k0 + base + c0 + u), lane, w[u]);
-          const int c = cc[c0 + u];
-          const size_t cb = (size_t)(c >= 0 ? c : 0) * B;
-#pragma unroll
-          for (int e = 0; e < B; e++) xv[u][e] = c >= 0 ? x[cb + e] : (T)0;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < CH; u++) {
-        if (c0 + u < m) {
-          T a[BB];
-          words_to_block<BB, TS, T>(w[u], a);
-          if constexpr (PROD) {
-            const int k = (FIRST ? 0 : base) + c0 + u;
-            if (FIRST && c0 + u < NP) {
-              blk_prod<B, T>(prod[(c0 + u) % NP], a, xv[u]);
-            } else if (k < nreal) {
-              T p[B];
-              blk_prod<B, T>(p, a, xv[u]);
-#pragma unroll
-              for (int r = 0; r < B; r++) ovf[(size_t)(ovf0 + k - NP) * B + r] = p[r];
-            }
-          } else {
-            blk_msub<B, T>(t, a, xv[u]);
-          }
-        }
-      }
-    }
-  }
-}
-// All n (wave-uniform) other-phase slots from k0, 16 at a time.
-template <int B, typename TS, typename T, bool PROD, int NP>
-__device__ __forceinline__ void tile_other(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
-                                           int k0, int n, int lane, T (&t)[B], T (&prod)[NP][B], T* ovf,
-                                           int ovf0, int nreal) {
-  static_assert(NP <= 16, "register products must come from the first 16 slots");
-  if (n > 0) tile_other16<B, TS, T, PROD, NP, true>(col, val, x, k0, 0, n < 16 ? n : 16, lane, t, prod, ovf, ovf0, nreal);
-  for (int base = 16; base < n; base += 16)
-    tile_other16<B, TS, T, PROD, NP, false>(col, val, x, k0, base, n - base < 16 ? n - base : 16, lane, t, prod, ovf,
-                                            ovf0, nreal);
-}
-
 template <int B, typename TS, typename T, int NT, bool UPPER, bool DOT>
 __global__ void __launch_bounds__(NT) k_trsv_tile(int tile0, const int* __restrict__ rows,
                                                   const int* __restrict__ rlev, const int* __restrict__ depth,
@@ -1473,7 +1455,8 @@ __global__ void __launch_bounds__(NT) k_trsv_tile(int tile0, const int* __restri
       const int c = col[(size_t)(ki + PI + u) * 64 + lane];
       ov_col[o] = c >= 0 ? c : NT;
     }
-    tile_other<B, TS, T, UPPER, PO>(col, val, out, ko, no, lane, t, so, ov_prod, ov.z, PO + ov.w);
+    sell_slots<B, TS, T, UPPER ? SLOT_PROD : SLOT_MSUB, PO>(col, val, out, ko, no, lane, t, so, ov_prod, ov.z,
+                                                       PO + ov.w);
   }
   __syncthreads();  // the zero row
   for (int l = 0; l < nlev; l++) {

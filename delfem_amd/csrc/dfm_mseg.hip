@@ -451,7 +451,24 @@ int dfm_mseg_dot(dfm_mseg* S, const double* x, const double* y, double* out) {
 // k < j, then Low with factor j.
 int dfm_mseg_ilu_create(dfm_mseg* S, int lev_fill, dfm_msilu** out) {
   DFM_CHECK(S && out && lev_fill >= -1, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_create");
+  std::vector<int> alev((size_t)S->nseg * S->nseg, lev_fill);
+  return dfm_mseg_ilu_create_levels(S, alev.data(), out);
+}
+
+// The same with a fill level per block (i, j) (row-major nseg x nseg), what
+// SetFillInLevel(lev, ilss) calls resolve to (preconditioner.cpp:108-122);
+// all levels 0: no fill (:124-130).  A block the fill would need but the
+// system lacks stays out when its own level is 0 (the reference creates it
+// empty, :171-185).
+int dfm_mseg_ilu_create_levels(dfm_mseg* S, const int* alev, dfm_msilu** out) {
+  DFM_CHECK(S && out && alev, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_create_levels");
   const unsigned n = S->nseg;
+  bool any = false;
+  for (unsigned t = 0; t < n * n; t++) {
+    DFM_CHECK(alev[t] >= -1, DFM_BAD_ARG, "fill level %d of block %u below -1", alev[t], t);
+    any = any || alev[t] != 0;
+  }
+  const int lev_fill = alev[0];
   std::unique_ptr<dfm_msilu_s> M(new dfm_msilu_s);
   M->S = S;
   M->lev_fill = lev_fill;
@@ -468,35 +485,38 @@ int dfm_mseg_ilu_create(dfm_mseg* S, int lev_fill, dfm_msilu** out) {
     for (unsigned j = 0; j < n; j++)
       if (S->nd[i][j]) F[i][j].from(S->nb[i], S->nd[i][j]->rowptr, S->nd[i][j]->col);
   }
-  if (lev_fill != 0) {
+  if (any) {
     RowList rl;
     for (unsigned i = 0; i < n; i++)
       for (unsigned j = 0; j < n; j++) {
+        const int K = alev[i * n + j];
         if (i == j) {
           for (unsigned k = 0; k < i; k++)
-            if (S->nd[i][k] && S->nd[k][i]) lp_lowup(D[i], S->nb[i] - 1, F[i][k], F[k][i], lev_fill, true, rl);
-          lp_dia_fill(D[i], lev_fill, rl);
+            if (S->nd[i][k] && S->nd[k][i]) lp_lowup(D[i], S->nb[i] - 1, F[i][k], F[k][i], K, true, rl);
+          lp_dia_fill(D[i], K, rl);
           continue;
         }
         const unsigned kmax = std::min(i, j);
         for (unsigned k = 0; k < kmax + 1; k++) {
           const bool have = S->nd[i][j] != nullptr;
           if (k == i && k < j) {
-            if (have) lp_up(F[i][j], S->nb[j], D[k], lev_fill, rl);
+            if (have) lp_up(F[i][j], S->nb[j], D[k], K, rl);
             continue;
           }
           if (k == j && k < i) {
-            if (have) lp_low(F[i][j], S->nb[j], D[k], lev_fill, rl);
+            if (have) lp_low(F[i][j], S->nb[j], D[k], K, rl);
             continue;
           }
           if (!S->nd[i][k] || !S->nd[k][j]) continue;
+          if (!have && K == 0) continue;
           DFM_CHECK(have, DFM_BAD_ARG, "fill needs block (%u, %u), which the system does not have", i, j);
-          lp_lowup(F[i][j], S->nb[j], F[i][k], F[k][j], lev_fill, false, rl);
+          lp_lowup(F[i][j], S->nb[j], F[i][k], F[k][j], K, false, rl);
         }
       }
   }
   for (unsigned i = 0; i < n; i++) {
-    DFM_TRY(ilu_create_impl(S->dia[i], lev_fill, DFM_ORDER_NATURAL, nullptr, 0, &D[i].rp, &D[i].cl, &M->dia[i]));
+    DFM_TRY(ilu_create_impl(S->dia[i], alev[i * n + i], DFM_ORDER_NATURAL, nullptr, 0, &D[i].rp, &D[i].cl,
+                            &M->dia[i]));
     for (unsigned j = 0; j < n; j++) {
       const dfm_bmat_s* B = S->nd[i][j];
       if (!B) continue;

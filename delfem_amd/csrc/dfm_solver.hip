@@ -13,6 +13,9 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <vector>
+
+#include <hip/hip_ext.h>
 
 #include "dfm_kernels.h"
 
@@ -302,10 +305,39 @@ static Ops<TA, TF> ops_of(dfm_ilu_s* M) {
           &M->A64, &M->L64, &M->U64, (int)M->nb, &M->syncL, &M->syncU, &M->tileL, &M->tileU};
 }
 
+// In-solve timing of the Krylov iteration's SpMV (dfm_set_kernel_timing):
+// while it is on, solves launch their iterations directly instead of
+// replaying graphs, and each iteration SpMV is launched with a start / stop
+// event pair that the runtime stamps when the kernel itself starts and ends.
+struct KernelTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // start, stop per launch
+  size_t used = 0;             // launches recorded since it was switched on
+};
+static KernelTimer g_ktimer;
+constexpr size_t kTimerMaxLaunches = 1u << 16;
+
 template <int B, typename TA, typename TF, typename T, bool DOTS, bool GATE>
 static int spmv(const Ops<TA, TF>& o, const T* x, T* y, const T* res, const T* w0, const T* u2, const T* v2,
                 double* partials, const KState* st) {
   const int nslice = o.planA->nslice;
+  if constexpr (DOTS && GATE) {
+    if (g_ktimer.on && g_ktimer.used < kTimerMaxLaunches && grid_for(nslice, kWaves) > 0) {
+      while (g_ktimer.ev.size() < 2 * (g_ktimer.used + 1)) {
+        hipEvent_t e = nullptr;
+        DFM_HIP(hipEventCreate(&e));
+        g_ktimer.ev.push_back(e);
+      }
+      hipEvent_t* e = &g_ktimer.ev[2 * g_ktimer.used];
+      hipExtLaunchKernelGGL((k_spmv<B, TA, T, DOTS, GATE>), dim3(grid_for(nslice, kWaves)), dim3(kNT), 0, stream(),
+                            e[0], e[1], 0, nslice, (int)o.nb, (const int*)o.planA->slice_ptr.p,
+                            (const int*)o.planA->col.p, (const TA*)o.A->val.p, (const TA*)o.A->dval.p, x, y, res, w0,
+                            u2, v2, partials, st);
+      DFM_HIP(hipGetLastError());
+      g_ktimer.used++;
+      return DFM_OK;
+    }
+  }
   DFM_LAUNCH((k_spmv<B, TA, T, DOTS, GATE>), grid_for(nslice, kWaves), nslice, o.nb, o.planA->slice_ptr.p,
              o.planA->col.p, o.A->val.p, o.A->dval.p, x, y, res, w0, u2, v2, partials, st);
   return DFM_OK;
@@ -344,7 +376,7 @@ static int run_block(dfm_ilu_s* M, const std::string& key, int nblock, Body&& bo
     const char* e = getenv("DFM_GRAPHS");
     return !(e && e[0] == '0');
   }();
-  if (s == nullptr || !use_graphs) {
+  if (s == nullptr || !use_graphs || g_ktimer.on) {
     for (int i = 0; i < nblock; i++) DFM_TRY(body());
     return DFM_OK;
   }
@@ -931,6 +963,44 @@ int dfm_bench_read(size_t bytes, int nrep, double* avg_ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *avg_ms = (double)ms / nrep;
+  return DFM_OK;
+}
+
+int dfm_set_kernel_timing(int on) {
+  DFM_HIP(hipStreamSynchronize(dfm::stream()));
+  dfm::g_ktimer.on = on != 0;
+  if (on) dfm::g_ktimer.used = 0;
+  return DFM_OK;
+}
+
+int dfm_get_kernel_timing(double* avg_all_ms, double* avg_working_ms, int* nlaunch, int* nworking) {
+  DFM_CHECK(avg_all_ms && avg_working_ms && nlaunch && nworking, DFM_BAD_ARG, "NULL output");
+  DFM_HIP(hipStreamSynchronize(dfm::stream()));
+  const size_t n = dfm::g_ktimer.used;
+  std::vector<double> ms(n);
+  for (size_t i = 0; i < n; i++) {
+    float t = 0.f;
+    DFM_HIP(hipEventElapsedTime(&t, dfm::g_ktimer.ev[2 * i], dfm::g_ktimer.ev[2 * i + 1]));
+    ms[i] = t;
+  }
+  double all = 0.0, work = 0.0;
+  int nw = 0;
+  if (n) {
+    std::vector<double> sorted(ms);
+    std::sort(sorted.begin(), sorted.end());
+    const double half_median = 0.5 * sorted[n / 2];
+    for (double v : ms) {
+      all += v;
+      if (v >= half_median) {
+        work += v;
+        nw++;
+      }
+    }
+  }
+  *avg_all_ms = n ? all / (double)n : 0.0;
+  *avg_working_ms = nw ? work / nw : 0.0;
+  *nlaunch = (int)n;
+  *nworking = nw;
   return DFM_OK;
 }
 

@@ -645,17 +645,38 @@ class MultiSegmentSystem:
         _sync_torch()
         call("dfm_mseg_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
 
-    def preconditioner(self, lev=0):
-        return MultiSegmentILU(self, lev)
+    def preconditioner(self, lev=0, inputs=None):
+        return MultiSegmentILU(self, lev, inputs)
+
+
+def fill_levels(nseg, inputs):
+    """The fill level of each block (i, j) that SetFillInLevel(lev, ilss)
+    requests resolve to, in order (src/ls/preconditioner.cpp:108-122): ilss
+    -1 sets every block, otherwise segment ilss's block row and column."""
+    alev = np.zeros((nseg, nseg), dtype=np.int32)
+    for lev, ilss in inputs:
+        if ilss == -1:
+            alev[:, :] = lev
+        elif 0 <= ilss < nseg:
+            alev[:, ilss] = lev
+            alev[ilss, :] = lev
+        else:
+            raise ValueError(f"SetFillInLevel: segment {ilss} beyond the {nseg} segments")
+    return alev
 
 
 class MultiSegmentILU:
     """CPreconditioner_ILU(ls, lev) across the segments
-    (src/ls/preconditioner.cpp:57-335): lev 0 no fill, -1 unlimited."""
+    (src/ls/preconditioner.cpp:57-335): lev 0 no fill, -1 unlimited; inputs:
+    SetFillInLevel(lev, ilss) requests [(lev, ilss), ...] instead of lev."""
 
-    def __init__(self, S, lev=0):
+    def __init__(self, S, lev=0, inputs=None):
         h = C.c_void_p()
-        call("dfm_mseg_ilu_create", S.h, int(lev), C.byref(h))
+        if inputs is None:
+            call("dfm_mseg_ilu_create", S.h, int(lev), C.byref(h))
+        else:
+            alev = np.ascontiguousarray(fill_levels(len(S.groups), inputs), dtype=np.int32)
+            call("dfm_mseg_ilu_create_levels", S.h, _ptr(alev), C.byref(h))
         self.h, self.S, self.lev = h, S, lev
 
     def __del__(self):

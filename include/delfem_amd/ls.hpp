@@ -24,8 +24,10 @@
 // runs the reference's loop over its virtual BLAS-1 calls.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
+#include <utility>
 #include <vector>
 
 #include "indexed_array.hpp"
@@ -324,23 +326,38 @@ class CPreconditioner_ILU : public CPreconditioner {
     SetFillInLevel((int)nlev);
     SetLinearSystem(ls);
   }
-  ~CPreconditioner_ILU() override { Clear(); }
+  ~CPreconditioner_ILU() override { release(); }
   CPreconditioner_ILU(const CPreconditioner_ILU&) = delete;
   CPreconditioner_ILU& operator=(const CPreconditioner_ILU&) = delete;
+  // Clear (preconditioner.cpp:37-54): the factors, the fill-level requests
+  // and the ordering
   void Clear() {
-    if (h_) dfm_ilu_destroy(h_);
-    if (ms_) dfm_mseg_ilu_destroy(ms_);
-    h_ = nullptr;
-    ms_ = nullptr;
-    mseg_.reset();
-    ls_ = nullptr;
+    release();
+    lev_inputs_.clear();
+    order_.clear();
+    ordering_ = DFM_ORDER_NATURAL;
   }
-  // SetFillInLevel(lev, ilss): the fill level of every block (ilss = -1);
-  // the levels are kept, as the reference keeps them, for the next
-  // SetLinearSystem.  Per-segment levels are not provided.
-  void SetFillInLevel(int lev, int ilss0 = -1) {
-    if (ilss0 != -1) throw Error("SetFillInLevel: per-segment levels are not provided", DFM_NOT_IMPLEMENTED);
-    lev_ = lev;
+  // SetFillInLevel(lev, ilss) (preconditioner.h:77): a request kept until
+  // Clear(); at SetLinearSystem the requests, in order, set the fill level of
+  // every block (ilss = -1) or of segment ilss's block row and column
+  // (preconditioner.cpp:108-122); no request means level 0.
+  void SetFillInLevel(int lev, int ilss0 = -1) { lev_inputs_.push_back({lev, ilss0}); }
+  // the level of each block (i, j), row-major nseg x nseg, the requests resolve to
+  std::vector<int> FillLevels(unsigned nseg) const {
+    std::vector<int> alev((size_t)nseg * nseg, 0);
+    for (const auto& in : lev_inputs_) {
+      if (in.second == -1) {
+        std::fill(alev.begin(), alev.end(), in.first);
+        continue;
+      }
+      if (in.second < 0 || (unsigned)in.second >= nseg)
+        throw Error("SetFillInLevel: segment index beyond the linear system", DFM_BAD_ARG);
+      for (unsigned i = 0; i < nseg; i++) {
+        alev[(size_t)i * nseg + in.second] = in.first;
+        alev[(size_t)in.second * nseg + i] = in.first;
+      }
+    }
+    return alev;
   }
   // SetFillBlk(aBlk) (preconditioner.h:80): block rows (and columns) of the
   // first segment's factor that take fill of any level (AddFracPtn(lev, aBlk))
@@ -362,14 +379,15 @@ class CPreconditioner_ILU : public CPreconditioner {
   // symbolic factorisation (preconditioner.cpp:57-211); several segments:
   // ILU(lev) across them in natural order
   void SetLinearSystem(const CLinearSystem& ls) override {
-    Clear();
+    release();
+    const std::vector<int> alev = FillLevels(ls.GetNLinSysSeg());
     if (ls.GetNLinSysSeg() == 1) {
       const dfm_bcrs* A = ls.GetMatrix(0).handle();
       if (!fill_blk_.empty() && ordering_ == DFM_ORDER_NATURAL) {
-        check(dfm_ilu_create_fill(A, lev_, fill_blk_.data(), (unsigned)fill_blk_.size(), &h_),
+        check(dfm_ilu_create_fill(A, alev[0], fill_blk_.data(), (unsigned)fill_blk_.size(), &h_),
               "CPreconditioner_ILU::SetLinearSystem");
       } else {
-        const int lev = ordering_ == DFM_ORDER_GIVEN ? 0 : lev_;
+        const int lev = ordering_ == DFM_ORDER_GIVEN ? 0 : alev[0];
         check(dfm_ilu_create(A, lev, ordering_, order_.empty() ? nullptr : order_.data(), block_nodes_, &h_),
               "CPreconditioner_ILU::SetLinearSystem");
       }
@@ -379,7 +397,7 @@ class CPreconditioner_ILU : public CPreconditioner {
       if (!fill_blk_.empty())
         throw Error("CPreconditioner_ILU: fill blocks with several segments are not provided", DFM_NOT_IMPLEMENTED);
       mseg_ = ls.mseg();
-      check(dfm_mseg_ilu_create(mseg_.get(), lev_, &ms_), "CPreconditioner_ILU::SetLinearSystem");
+      check(dfm_mseg_ilu_create_levels(mseg_.get(), alev.data(), &ms_), "CPreconditioner_ILU::SetLinearSystem");
     }
     ls_ = &ls;
     gen_ = ls.structure();
@@ -414,7 +432,16 @@ class CPreconditioner_ILU : public CPreconditioner {
                   DFM_BAD_ARG);
     (void)what;
   }
-  int lev_ = 0, ordering_ = DFM_ORDER_NATURAL;
+  void release() {
+    if (h_) dfm_ilu_destroy(h_);
+    if (ms_) dfm_mseg_ilu_destroy(ms_);
+    h_ = nullptr;
+    ms_ = nullptr;
+    mseg_.reset();
+    ls_ = nullptr;
+  }
+  std::vector<std::pair<int, int>> lev_inputs_;  // SetFillInLevel requests (lev, ilss)
+  int ordering_ = DFM_ORDER_NATURAL;
   unsigned block_nodes_ = 0;
   std::vector<int> order_;
   std::vector<unsigned> fill_blk_;

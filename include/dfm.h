@@ -268,6 +268,17 @@ int dfm_solve_pcg(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
  * DFM_PERSIST_GRID (DFM_TILE_ROWS, read at factor creation: rows a tile is
  * filled to, default 64). */
 int dfm_set_sweep_mode(int mode, int grid);
+/* In-solve timing of the Krylov iteration's SpMV (no reference counterpart;
+ * a measurement aid for the roofline figure).  on != 0 starts recording:
+ * solves then launch their iterations directly instead of replaying graphs
+ * (same kernels, same order) and every iteration SpMV gets a start / stop
+ * event pair stamped at the kernel's own start and end (at most 65,536
+ * launches); on == 0 stops.  dfm_get_kernel_timing returns the average
+ * duration over all recorded launches and over the launches that did work
+ * (at least half the median duration: launches after convergence inside a
+ * block of iterations return at once), and both counts. */
+int dfm_set_kernel_timing(int on);
+int dfm_get_kernel_timing(double *avg_all_ms, double *avg_working_ms, int *nlaunch, int *nworking);
 int dfm_solve_pbicgstab(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
                         double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
 
@@ -339,6 +350,12 @@ int dfm_mseg_dot(dfm_mseg *S, const double *x_dev, const double *y_dev, double *
  * matdiafrac_blkcrs.cpp:1105-1377) in the reference's block order; lev_fill 0 = no fill,
  * -1 = unlimited.  Fill that would need a block the system lacks is DFM_BAD_ARG. */
 int dfm_mseg_ilu_create(dfm_mseg *S, int lev_fill, dfm_msilu **out);
+/* The same with a fill level per block (i, j), lev[i * nseg + j] (-1 unlimited): what
+ * CPreconditioner_ILU::SetFillInLevel(lev, ilss) requests resolve to (preconditioner.h:77,
+ * preconditioner.cpp:108-131; a request sets segment ilss's block row and column, ilss -1
+ * every block, later requests win; all zero: no fill).  A missing block that the fill would
+ * need stays out when its own level is 0 (the reference creates it empty). */
+int dfm_mseg_ilu_create_levels(dfm_mseg *S, const int *lev, dfm_msilu **out);
 int dfm_mseg_ilu_destroy(dfm_msilu *M);
 /* SetValue (:217-297): copies, then per block in (i, j) order DoILUDecompLowUp + DoILUDecomp
  * (diagonal), DoILUDecompUp / Low / LowUp (off-diagonal); DFM_SINGULAR as dfm_ilu_set_value */

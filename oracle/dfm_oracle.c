@@ -1618,9 +1618,11 @@ static void lp_dia_fill(orc_lpat *D, int K, orc_rowlist *rl)
  * every k < i, then its own fill; above: LowUp with k < i, then Up with
  * factor i; below: LowUp with k < j, then Low with factor j.  Returns -1 when
  * a block that fill would create is missing. */
-static int ms_symbolic(orc_ms *m, int K)
+/* alev: the fill level of each block (i, j), row-major n x n (preconditioner.cpp:108-131) */
+static int ms_symbolic(orc_ms *m, const int *alev)
 {
     const unsigned n = m->nseg;
+    int any = 0;
     for (unsigned i = 0; i < n; i++) {
         lp_free(&m->fp[i]);
         lp_from(&m->fp[i], m->nb[i], m->rowptr[i], m->col[i]);
@@ -1631,11 +1633,14 @@ static int ms_symbolic(orc_ms *m, int K)
             lp_from(&k->fp, k->nbc, k->rowptr, k->col);
         }
     }
-    if (K == 0) return 0;
+    for (unsigned t = 0; t < n * n; t++)
+        if (alev[t] != 0) any = 1;
+    if (!any) return 0; /* preconditioner.cpp:124-130 */
     orc_rowlist rl = {NULL, 0, 0, -1};
     int rc = 0;
     for (unsigned i = 0; i < n && rc == 0; i++)
         for (unsigned j = 0; j < n && rc == 0; j++) {
+            const int K = alev[i * n + j];
             if (i == j) {
                 for (unsigned kk = 0; kk < i; kk++)
                     if (m->nd[i][kk] && m->nd[kk][i])
@@ -1649,7 +1654,9 @@ static int ms_symbolic(orc_ms *m, int K)
                 if (kk == i && kk < j) { if (t) lp_up(&t->fp, t->nbr, &m->fp[kk], K, &rl); continue; }
                 if (kk == j && kk < i) { if (t) lp_low(&t->fp, t->nbr, &m->fp[kk], K, &rl); continue; }
                 if (!m->nd[i][kk] || !m->nd[kk][j]) continue;
-                if (!t) { rc = -1; break; }
+                /* the reference creates the missing block (:171-185); at level 0
+                   it stays empty, which is the same as leaving it out */
+                if (!t) { if (K == 0) continue; rc = -1; break; }
                 lp_lowup(&t->fp, t->nbr, &m->nd[i][kk]->fp, &m->nd[kk][j]->fp, K, 0, &rl);
             }
         }
@@ -1828,8 +1835,17 @@ static void ms_lowup_dia(orc_ms *m, unsigned si, const orc_msblk *lo, const orc_
 int orc_ms_ilu(void *p, int lev_fill)
 {
     orc_ms *m = (orc_ms *)p;
+    int alev[ORC_MS_MAX * ORC_MS_MAX];
+    for (unsigned t = 0; t < m->nseg * m->nseg; t++) alev[t] = lev_fill;
+    return orc_ms_ilu_levels(p, alev);
+}
+/* the same with a fill level per block (i, j), row-major nseg x nseg: what
+ * SetFillInLevel(lev, ilss) calls resolve to (preconditioner.cpp:108-122) */
+int orc_ms_ilu_levels(void *p, const int *alev)
+{
+    orc_ms *m = (orc_ms *)p;
     const unsigned n = m->nseg;
-    if (ms_symbolic(m, lev_fill) < 0) return -2;
+    if (ms_symbolic(m, alev) < 0) return -2;
     for (unsigned i = 0; i < n; i++) {
         const unsigned nb = m->nb[i], bb = m->b[i] * m->b[i];
         const unsigned ncrs = m->fp[i].rp[nb];

@@ -112,6 +112,8 @@ class Orc:
         L.orc_ms_matvec.argtypes = [vp, C.c_double, vp, C.c_double, vp]
         L.orc_ms_ilu.restype = C.c_int
         L.orc_ms_ilu.argtypes = [vp, C.c_int]
+        L.orc_ms_ilu_levels.restype = C.c_int
+        L.orc_ms_ilu_levels.argtypes = [vp, vp]
         L.orc_ms_factor_ncrs.restype = C.c_uint
         L.orc_ms_factor_ncrs.argtypes = [vp, C.c_uint, C.c_uint]
         L.orc_ms_get_pattern.argtypes = [vp, C.c_uint, C.c_uint, vp, vp]
@@ -574,6 +576,7 @@ class Ref:
             "ref_ms_get_block": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
             "ref_ms_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
             "ref_ms_ilu": (C.c_int, [vp, C.c_int]),
+            "ref_ms_ilu_inputs": (C.c_int, [vp, C.c_int, vp, vp]),
             "ref_ms_precond": (None, [vp, vp]),
             "ref_ms_solve": (C.c_int, [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_bmat_free": (None, [vp]),
@@ -787,6 +790,20 @@ def split_problem(orc, prob, groups):
     return out
 
 
+def fill_levels(nseg, inputs):
+    """The fill level of each block (i, j) after SetFillInLevel(lev, ilss)
+    calls, in order (preconditioner.cpp:108-122): ilss -1 sets every block,
+    otherwise the row and the column of segment ilss; later calls win."""
+    alev = np.zeros((nseg, nseg), dtype=np.int32)
+    for lev, ilss in inputs:
+        if ilss == -1:
+            alev[:, :] = lev
+        else:
+            alev[:, ilss] = lev
+            alev[ilss, :] = lev
+    return alev
+
+
 class OrcMS:
     """The oracle's multi-segment system: blocks assembled (Mearge of the
     split element matrices), Dirichlet conditions as CLinearSystem::
@@ -840,8 +857,12 @@ class OrcMS:
         self.orc.L.orc_ms_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
         return y
 
-    def ilu(self, lev=0):
-        """CPreconditioner_ILU(ls, lev) across the segments; True on success."""
+    def ilu(self, lev=0, inputs=None):
+        """CPreconditioner_ILU(ls, lev) across the segments; True on success.
+        inputs: SetFillInLevel(lev, ilss) calls [(lev, ilss), ...] instead."""
+        if inputs is not None:
+            alev = np.ascontiguousarray(fill_levels(self.nseg, inputs), dtype=np.int32)
+            return self.orc.L.orc_ms_ilu_levels(self.h, _p(alev)) == 0
         return self.orc.L.orc_ms_ilu(self.h, int(lev)) == 0
 
     def pattern(self, i, j):
@@ -928,7 +949,11 @@ class RefMS:
         self.ref.L.ref_ms_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
         return y
 
-    def ilu(self, lev=0):
+    def ilu(self, lev=0, inputs=None):
+        if inputs is not None:
+            lv = np.ascontiguousarray([l for l, _ in inputs], dtype=np.int32)
+            ss = np.ascontiguousarray([i for _, i in inputs], dtype=np.int32)
+            return bool(self.ref.L.ref_ms_ilu_inputs(self.h, len(inputs), _p(lv), _p(ss)))
         return bool(self.ref.L.ref_ms_ilu(self.h, int(lev)))
 
     def precond(self, z):

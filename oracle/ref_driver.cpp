@@ -594,6 +594,15 @@ int ref_ms_ilu(void* p, int lev) {
   m->prec->SetLinearSystem(m->ls);
   return m->prec->SetValue(m->ls) ? 1 : 0;
 }
+// The same with SetFillInLevel(lev[t], ilss[t]) for t < nin, in that order
+int ref_ms_ilu_inputs(void* p, int nin, const int* lev, const int* ilss) {
+  RefMS* m = static_cast<RefMS*>(p);
+  delete m->prec;
+  m->prec = new LsSol::CPreconditioner_ILU;
+  for (int t = 0; t < nin; t++) m->prec->SetFillInLevel(lev[t], ilss[t]);
+  m->prec->SetLinearSystem(m->ls);
+  return m->prec->SetValue(m->ls) ? 1 : 0;
+}
 void ref_ms_precond(void* p, double* z) {
   RefMS* m = static_cast<RefMS*>(p);
   m->ls.ReSizeTmpVecSolver(1);

@@ -104,7 +104,12 @@ int main(int argc, char** argv) {
       off += (size_t)nnode * len[i];
     }
     const double norm = ls.FinalizeMarge();
-    LsSol::CPreconditioner_ILU prec(ls, (unsigned)lev);
+    // lev -2: a per-segment request, level 1 for the last segment's block
+    // row and column, as a caller of SetFillInLevel(lev, ilss) writes it
+    LsSol::CPreconditioner_ILU prec;
+    if (lev == -2) prec.SetFillInLevel(1, (int)nseg - 1);
+    else prec.SetFillInLevel(lev);
+    prec.SetLinearSystem(ls);
     if (!prec.SetValue(ls)) return 7;
     LsSol::CLinearSystemPreconditioner lsp(ls, prec);
     double conv = tol;

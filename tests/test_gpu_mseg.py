@@ -199,3 +199,43 @@ def test_cross_segment_fill(orc, gpu, name, lev):
     else:
         assert abs(it - it_o) <= max(2, int(np.ceil(0.25 * it_o)))
         assert np.linalg.norm(xh - x_o) <= 1e-6 * np.linalg.norm(x_o)
+
+
+SEG_LEVEL_INPUTS = [[(1, 0)], [(2, -1), (0, 1)], [(1, 1), (2, 0)], [(-1, 1)], [(0, -1), (1, 2)]]
+
+
+@pytest.mark.parametrize("inputs", SEG_LEVEL_INPUTS, ids=[str(i) for i in SEG_LEVEL_INPUTS])
+@pytest.mark.parametrize("name", sorted(FILL_CASES))
+def test_segment_fill_levels(orc, gpu, name, inputs):
+    """SetFillInLevel(lev, ilss) per segment (preconditioner.h:77,
+    preconditioner.cpp:108-131): every block's fill pattern, factor values
+    and SolvePrecond equal the oracle's bit for bit (the oracle's level
+    resolution and symbolic are pinned to the compiled reference in
+    tests/test_oracle_ref.py)."""
+    import torch
+    make, groups, kind = FILL_CASES[name]
+    if any(i >= len(groups) for _, i in inputs):
+        pytest.skip("segment index beyond this split")
+    prob = make()
+    print(f"[case] {name} levels {inputs}", flush=True)
+    S = gpu.MultiSegmentSystem(prob, groups)
+    S.assemble()
+    O = OrcMS(orc, split_problem(orc, prob, groups), prob.conn)
+    M = S.preconditioner(inputs=inputs)
+    assert M.SetValue() and O.ilu(inputs=inputs)
+    n = len(groups)
+    for i in range(n):
+        for j in range(n):
+            rp, cl = M.pattern(i, j)
+            orp, ocl = O.pattern(i, j)
+            assert np.array_equal(rp, orp) and np.array_equal(cl, ocl), (i, j)
+            if i == j:
+                fv, fd = M.values(i, i)
+                ofv, ofd = O.factor(i, i)
+                assert np.array_equal(fv, ofv) and np.array_equal(fd, ofd), (i, j)
+            else:
+                assert np.array_equal(M.values(i, j), O.factor(i, j)), (i, j)
+    x = np.random.default_rng(3).standard_normal(O.res.size)
+    xd = torch.from_numpy(x.copy()).cuda()
+    M.SolvePrecond(xd)
+    assert np.array_equal(xd.cpu().numpy(), O.precond(x))

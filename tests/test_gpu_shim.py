@@ -196,15 +196,16 @@ MS_DROPIN = {
 }
 
 
-@pytest.mark.parametrize("lev", [0, 1])
+@pytest.mark.parametrize("lev", [0, 1, -2])
 @pytest.mark.parametrize("name", sorted(MS_DROPIN))
 def test_cpp_multisegment_dropin(orc, gpu, tmp_path, name, lev):
     """The C++ LsSol classes with several segments (include/delfem_amd/ls.hpp)
     driven as the reference's split-field systems drive its own: per-block
     Mearge and FinalizeMarge bit-exact against the oracle (which equals the
-    compiled reference bit for bit), CPreconditioner_ILU(ls, lev) across the
-    segments, Solve_PCG within one iteration and 1e-8 of the oracle,
-    Solve_PBiCGSTAB within 25 % and 1e-6."""
+    compiled reference bit for bit), CPreconditioner_ILU across the segments
+    with SetFillInLevel(lev) or (lev -2) SetFillInLevel(1, last segment),
+    Solve_PCG within one iteration and 1e-8 of the oracle, Solve_PBiCGSTAB
+    within 25 % and 1e-6."""
     from oracle.oracle import OrcMS, split_problem
     if not os.path.exists(MSBIN):
         pytest.fail("tests/cpp/msshim_test is not built (run __graft_entry__.build())")
@@ -247,7 +248,7 @@ def test_cpp_multisegment_dropin(orc, gpu, tmp_path, name, lev):
                 rest = rest[O.dia[i].size:]
     assert rest.size == 0
     assert norm == pytest.approx(np.linalg.norm(O.res), rel=1e-13)
-    assert O.ilu(lev)
+    assert O.ilu(inputs=[(1, nseg - 1)]) if lev == -2 else O.ilu(lev)
     want = O.solve(kind == 1, O.res, tol, maxit)
     print(f"  C++ {it} iterations, oracle {want[2]}", flush=True)
     assert ok == 1 and want[0] and conv < tol

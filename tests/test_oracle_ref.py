@@ -264,6 +264,52 @@ def test_multisegment_fill_vs_reference(orc, ref, name, lev):
     assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
 
 
+MS_LEVEL_INPUTS = [
+    [(1, 0)],
+    [(2, -1), (0, 1)],
+    [(1, 1), (2, 0)],
+    [(-1, 1)],
+    [(0, -1), (1, 2)],
+]
+
+
+@pytest.mark.parametrize("inputs", MS_LEVEL_INPUTS, ids=[str(i) for i in MS_LEVEL_INPUTS])
+@pytest.mark.parametrize("name", sorted(MS_CASES))
+def test_multisegment_segment_levels_vs_reference(orc, ref, name, inputs):
+    """CPreconditioner_ILU::SetFillInLevel(lev, ilss) per segment
+    (preconditioner.h:77, preconditioner.cpp:108-131: a call sets the fill
+    level of segment ilss's block row and column, -1 every block, later calls
+    win; all-zero levels add no fill): the oracle, given the level matrix
+    those calls resolve to, against the reference given the calls --
+    SolvePrecond and the whole solve bit for bit, and the factor differs from
+    both the ILU(0) and the uniform-level one where the levels differ."""
+    from oracle.oracle import OrcMS, RefMS, split_problem, fill_levels
+    make, groups, bicg = MS_CASES[name]
+    nseg = len(groups)
+    if any(i >= nseg for _, i in inputs):
+        pytest.skip("segment index beyond this split")
+    prob = make()
+    sp = split_problem(orc, prob, groups)
+    O, R = OrcMS(orc, sp, prob.conn), RefMS(ref, sp, prob.conn)
+    assert R.ilu(inputs=inputs) and O.ilu(inputs=inputs)
+    rng = np.random.default_rng(len(inputs))
+    x = rng.standard_normal(O.res.size)
+    z = R.precond(x)
+    assert np.array_equal(z, O.precond(x))
+    a, b = R.solve(bicg, O.res, 1e-10, 3000), O.solve(bicg, O.res, 1e-10, 3000)
+    assert a[:3] == b[:3] and a[0]
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+    alev = fill_levels(nseg, inputs)
+    O0 = OrcMS(orc, sp, prob.conn)
+    O0.ilu(0)
+    if np.any(alev != 0):
+        assert not np.array_equal(O0.precond(x), z)
+    if len(np.unique(alev)) > 1:
+        Ou = OrcMS(orc, sp, prob.conn)
+        Ou.ilu(int(alev.max()) if alev.min() >= 0 else -1)
+        assert not np.array_equal(Ou.precond(x), z)
+
+
 @pytest.mark.parametrize("b,nzero", [(1, 10), (1, 11), (1, 14), (3, 10), (3, 11)])
 @pytest.mark.parametrize("ordered", [False, True])
 def test_singular_pivot_stop_state(orc, ref, b, nzero, ordered):
