@@ -444,7 +444,7 @@ def bench_batch(cfg, args, torch, linsys, _lib):
         f"setup {time.time() - t0:.1f}s, {B.info()}")
     tune = B.autotune(rhs, x, cfg["tol"], cfg["maxit"], "mixed_f32")
     shape = min(tune, key=tune.get)
-    log("[bench] autotune (block, workgroups per CU) -> ms: " +
+    log("[bench] autotune (block, workgroups per CU, team rows, team size) -> ms: " +
         ", ".join(f"{k}: {v:.1f}" for k, v in sorted(tune.items())) + f"; chosen {shape}")
 
     def timed(prec, steps, warmup):

@@ -796,27 +796,35 @@ class BatchSystem:
         call("dfm_batch_get_factor", self.h, int(s), _ptr(val), _ptr(dia), _ptr(order))
         return val[:nnz], dia, order
 
-    def autotune(self, rhs, x, tol, maxit, precision="mixed_f32", reps=2):
-        """Time each launch shape on this batch (graph replay, HIP events on
-        the library stream) and keep the fastest.  Returns {shape: ms}."""
+    # team axis: (rows from which a system gets a team, most workgroups per team)
+    TEAMS = ((16384, 4), (16384, 8), (8192, 4))
+
+    def autotune(self, rhs, x, tol, maxit, precision="mixed_f32", reps=2, shapes=None, teams=None):
+        """Time each launch shape (threads per workgroup, workgroups per CU)
+        with each team setting on this batch (graph replay, HIP events on the
+        library stream) and keep the fastest.  Returns {(block, wg_per_cu,
+        team_min_rows, team_max): ms}."""
         torch = _torch()
         stream = torch.cuda.ExternalStream(int(_lib.lib().dfm_stream()))
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         times = {}
-        for bl, wp in self.SHAPES:
-            self.set_launch(bl, wp)
-            self.solve(rhs, x, tol, maxit, precision)
-            best = float("inf")
-            for _ in range(reps):
-                torch.cuda.synchronize()
-                ev[0].record(stream)
+        for bl, wp in (shapes or self.SHAPES):
+            for tr, tk in (teams or self.TEAMS):
+                self.set_launch(bl, wp)
+                self.set_teams(tr, tk)
                 self.solve(rhs, x, tol, maxit, precision)
-                ev[1].record(stream)
-                ev[1].synchronize()
-                best = min(best, ev[0].elapsed_time(ev[1]))
-            times[(bl, wp)] = best
-        bl, wp = min(times, key=times.get)
+                best = float("inf")
+                for _ in range(reps):
+                    torch.cuda.synchronize()
+                    ev[0].record(stream)
+                    self.solve(rhs, x, tol, maxit, precision)
+                    ev[1].record(stream)
+                    ev[1].synchronize()
+                    best = min(best, ev[0].elapsed_time(ev[1]))
+                times[(bl, wp, tr, tk)] = best
+        bl, wp, tr, tk = min(times, key=times.get)
         self.set_launch(bl, wp)
+        self.set_teams(tr, tk)
         return times
 
 

@@ -199,3 +199,38 @@ def test_batch_default_teams_large(orc, gpu, prec):
         else:
             y = orc.matvec(n, 1, S["rowptr"], S["col"], S["val"], S["dia"], -1.0, xg[r0:r1], 1.0, S["res"])
             assert np.linalg.norm(y) <= 1.01e-8 * np.linalg.norm(S["res"]), s
+
+
+def test_batch_autotune(batch, orc):
+    """BatchSystem.autotune times every (launch shape, team setting) pair on
+    the batch and keeps the fastest; the tuned batch still solves to the FP64
+    tolerance on every system's true residual, with factors bit-exact."""
+    prob, ref, fem, B, torch = batch["prob"], batch["ref"], batch["fem"], batch["B"], batch["torch"]
+    _case("batch autotune")
+    rhs = fem.res.clone()
+    x = torch.zeros_like(rhs)
+    shapes, teams = ((256, 1), (512, 2), (1024, 1)), ((16384, 4), (100, 2), (100, 4))
+    try:
+        times = B.autotune(rhs, x, 1e-8, 4000, "mixed_f32", reps=1, shapes=shapes, teams=teams)
+        assert set(times) == {(bl, wp, tr, tk) for bl, wp in shapes for tr, tk in teams}
+        assert all(v > 0 for v in times.values())
+        best = min(times, key=times.get)
+        info = B.info()
+        assert (info["block"], info["wg_per_cu"]) == best[:2]
+        ok, out = B.solve(rhs, x, 1e-8, 4000, precision="mixed_f32")
+        assert ok and np.all(out["status"] == 1) and np.all(out["relres"] < 1.01e-8)
+        xg = x.cpu().numpy()
+        for s in range(B.nsys):
+            r0, r1 = int(prob.sys_ptr[s]), int(prob.sys_ptr[s + 1])
+            S = sub_system(ref, r0, r1)
+            n = r1 - r0
+            y = orc.matvec(n, 1, S["rowptr"], S["col"], S["val"], S["dia"], -1.0, xg[r0:r1], 1.0, S["res"])
+            assert np.linalg.norm(y) <= 1.01e-8 * np.linalg.norm(S["res"]), s
+            _, _, order = orc.colour_order(S["rowptr"], S["col"])
+            F = orc.iluk_symbolic(S["rowptr"], S["col"], 0, order)
+            ov, od, _ = orc.ilu_values(n, 1, S["rowptr"], S["col"], S["val"], S["dia"], F, order)
+            fval, fdia, _ = B.factor(s)
+            assert np.array_equal(fval, ov) and np.array_equal(fdia, od), s
+    finally:
+        B.set_launch(512, 1)
+        B.set_teams(16384, 4)
