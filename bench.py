@@ -81,8 +81,11 @@ CONFIGS = {
                  workload="multi-segment system: 3-D linear elasticity 48^3 (117,649 nodes) split into an (ux, uy) "
                           "segment and a uz segment, ILU(0) across segments (nodes multicolour-numbered), "
                           "PCG to 1e-10"),
+    # FAC32 BiCGSTAB keeps FP64 vectors: its recurrence needs no intermediate
+    # reliable updates (each would restart BiCGSTAB's recurrences), so the FP64
+    # true residual is checked only when the recurrence has converged
     "cfg4": dict(kind="oseen2d", n=256, tol=1e-9, lev=1, ordering="block_multicolour", block_nodes=16,
-                 solver="bicg", mixed="fac32", others=("fac16",),
+                 solver="bicg", mixed="fac32", others=("fac16",), mixed_opts={"fac32": {"reliable_delta": 1e-150}},
                  workload="Oseen P1-P1 PSPG lid-driven cavity, 256x256 cells (66,049 nodes, 198,147 DOF), "
                           "nu=0.01, Fourier advection, ILU(1)-BiCGSTAB to 1e-9"),
 }
@@ -290,7 +293,8 @@ def main():
         ok_f = M.SetValue(s.A)
         r = rhs.clone()
         x = torch.zeros_like(r)
-        _, conv, it, extra = solve(tol, maxit, s.A, M, r, x, precision=precision, want_true_relres=1)
+        extra_opts = cfg.get("mixed_opts", {}).get(precision, {})
+        _, conv, it, extra = solve(tol, maxit, s.A, M, r, x, precision=precision, want_true_relres=1, **extra_opts)
         return extra["converged"] and ok_f, conv, it, extra, x
 
     def timed(precision, steps, warmup, maxit=50000):
@@ -395,7 +399,7 @@ def main():
         "config": {"workload": cfg["workload"], "config": args.config, "seed": args.seed,
                    "preconditioner": f"ILU({cfg['lev']}), {cfg['ordering']} ordering",
                    "ncolour": info["ncolour"], "levels": [info["nlevel_lower"], info["nlevel_upper"]],
-                   "tol": cfg["tol"], "precision": head},
+                   "tol": cfg["tol"], "precision": head, "solve_options": cfg.get("mixed_opts", {}).get(head, {})},
         "mixed_ms": round(mixed_ms, 3),
         "mixed_precision": mixed,
         "fp64_ms": round(fp64_ms, 3),
