@@ -584,7 +584,8 @@ static int upload_sell_plan(const SellPlan& plan, int bb, Sell<double>& s64, Dev
   return DFM_OK;
 }
 
-static int upload_tile_plan(const TilePlan& tp, int bb, TileSweep& ts) {
+static int upload_tile_plan(const TilePlan& tp, int b, TileSweep& ts) {
+  const int bb = b * b;
   ts.nt = tp.nt;
   ts.ntile = tp.ntile;
   ts.phase_tile = tp.phase_tile;
@@ -603,6 +604,23 @@ static int upload_tile_plan(const TilePlan& tp, int bb, TileSweep& ts) {
   DFM_TRY(ts.rov.upload(tp.rov));
   ts.ovf_in = tp.ovf_in;
   ts.ovf_out = tp.ovf_out;
+  if (b == 2 || b == 3) {
+    TileSplit sp;
+    build_tile_split(tp, b, sp);
+    ts.g = sp.g;
+    Sell<double>& w = ts.s64;
+    w.nslice = sp.nslice;
+    w.nk = sp.nk;
+    w.bb = b;
+    w.group_slice_ptr = {0, w.nslice};
+    DFM_TRY(w.slice_ptr.upload(sp.slice_ptr));
+    DFM_TRY(w.rows.upload(sp.rows));
+    DFM_TRY(w.col.upload(sp.col));
+    DFM_TRY(ts.ssplit.upload(sp.split));
+    DFM_TRY(ts.ssrc.upload(sp.src));
+    DFM_TRY(w.val.alloc(std::max<size_t>(1, (size_t)sp.nk * b * kSlice)));
+    DFM_TRY(w.dval.alloc(std::max<size_t>(1, (size_t)w.nslice * b * kSlice)));
+  }
   DFM_TRY(v.val.alloc(std::max<size_t>(1, (size_t)tp.nk * bb * kSlice)));
   DFM_TRY(v.dval.alloc(std::max<size_t>(1, (size_t)v.nslice * bb * kSlice)));
   return DFM_OK;
@@ -787,8 +805,8 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
     DFM_TRY(build_sweep_sync(M->planL, M->syncL));
     DFM_TRY(build_sweep_sync(M->planU, M->syncU));
     if (tiled) {
-      DFM_TRY(upload_tile_plan(tpL, (int)bb, M->tileL));
-      DFM_TRY(upload_tile_plan(tpU, (int)bb, M->tileU));
+      DFM_TRY(upload_tile_plan(tpL, (int)b, M->tileL));
+      DFM_TRY(upload_tile_plan(tpU, (int)b, M->tileU));
     }
     DFM_TRY(upload_sell_plan(M->planA, (int)bb, M->A64, M->d_srcA));
     DFM_TRY(M->state.alloc(1));

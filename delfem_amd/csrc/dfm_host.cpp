@@ -537,4 +537,68 @@ bool build_tile_plan(unsigned nb, const std::vector<unsigned>& frow, const std::
   return true;
 }
 
+// TileSplit from a TilePlan: each tile row's slots, read back in order from
+// the row's lane of the TilePlan layout, laid out again for g lanes per row.
+void build_tile_split(const TilePlan& tp, int b, TileSplit& ts) {
+  ts = TileSplit();
+  const int g = b <= 2 ? 2 : 4;
+  const int nt = tp.nt, rpw = kSlice / g;  // tile rows per wave
+  ts.g = g;
+  ts.b = b;
+  ts.nslice = tp.ntile * nt * g / kSlice;
+  // per tile row: its first- and second-part slots (col, factor entry), in order
+  const size_t nrow = tp.rows.size();
+  std::vector<std::vector<std::pair<int, long long>>> first(nrow), second(nrow);
+  const int nslice0 = tp.ntile * nt / kSlice;
+  for (int s = 0; s < nslice0; s++)
+    for (int l = 0; l < kSlice; l++) {
+      const size_t rix = (size_t)s * kSlice + l;
+      if (tp.rows[rix] < 0) continue;
+      for (int k = tp.slice_ptr[s]; k < tp.slice_ptr[s + 1]; k++) {
+        const long long q = tp.src[(size_t)k * kSlice + l];
+        if (q < 0) continue;
+        (k < tp.split[s] ? first : second)[rix].push_back({tp.col[(size_t)k * kSlice + l], q});
+      }
+    }
+  ts.slice_ptr.assign(ts.nslice + 1, 0);
+  ts.split.assign(ts.nslice, 0);
+  for (int s = 0; s < ts.nslice; s++) {
+    // rows of this wave: tile t, tile rows [w*rpw, (w+1)*rpw)
+    const int waves_per_tile = nt * g / kSlice;
+    const int t = s / waves_per_tile, w = s % waves_per_tile;
+    int wa = 0, wb = 0;
+    for (int j = 0; j < rpw; j++) {
+      const size_t rix = (size_t)t * nt + w * rpw + j;
+      wa = std::max(wa, (int)first[rix].size());
+      wb = std::max(wb, (int)second[rix].size());
+    }
+    ts.split[s] = ts.slice_ptr[s] + wa;
+    ts.slice_ptr[s + 1] = ts.split[s] + wb;
+  }
+  ts.nk = ts.slice_ptr[ts.nslice];
+  ts.col.assign((size_t)ts.nk * kSlice, -1);
+  ts.src.assign((size_t)ts.nk * kSlice, -1);
+  ts.rows.assign((size_t)ts.nslice * kSlice, -1);
+  for (int s = 0; s < ts.nslice; s++) {
+    const int waves_per_tile = nt * g / kSlice;
+    const int t = s / waves_per_tile, w = s % waves_per_tile;
+    for (int l = 0; l < kSlice; l++) {
+      const int j = l / g, r = l % g;
+      const size_t rix = (size_t)t * nt + w * rpw + j;
+      const int row = tp.rows[rix];
+      if (row < 0 || r >= b) continue;
+      ts.rows[(size_t)s * kSlice + l] = row * b + r;
+      for (int part = 0; part < 2; part++) {
+        const auto& v = part ? second[rix] : first[rix];
+        int k = part ? ts.split[s] : ts.slice_ptr[s];
+        for (const auto& cq : v) {
+          ts.col[(size_t)k * kSlice + l] = cq.first;
+          ts.src[(size_t)k * kSlice + l] = cq.second * b + r;
+          k++;
+        }
+      }
+    }
+  }
+}
+
 }  // namespace dfm

@@ -144,6 +144,21 @@ struct TilePlan {
   std::vector<int> col;             // nk * 64
   std::vector<long long> src;       // nk * 64: factor entry, -1 padding
 };
+// Split-lane form of a TilePlan for b x b blocks (b > 1): g = 2 (b = 2) or
+// 4 (b = 3) lanes per tile row, lane t*g + r holding row r of the row's
+// blocks (lanes r >= b idle).  Waves (64 lanes, 64/g tile rows) are the
+// slices; per slice the two parts of TilePlan, padded to the widest of its
+// rows.  Per slot-lane: col (the TilePlan encoding), src = q*b + r (the
+// b values of block row r of factor entry q); rows[lane] = row*b + r (the
+// diagonal block's row r), -1 for padding and idle lanes.
+struct TileSplit {
+  int g = 0, b = 0, nslice = 0;
+  long nk = 0;
+  std::vector<int> slice_ptr, split, rows, col;
+  std::vector<long long> src;
+};
+void build_tile_split(const TilePlan& tp, int b, TileSplit& ts);
+
 // Largest component a tile can hold.
 constexpr int kTileMaxRows = 256;
 // In-phase slots a tile row holds in registers, and (upper) later-phase
@@ -222,6 +237,14 @@ struct TileSweep {
   Sell<double> v64;
   Sell<float> v32;
   Sell<bf16s> v16;
+  // split-lane layout (TileSplit; g = 0: none): slice_ptr, rows, col of s64,
+  // values per storage type
+  int g = 0;
+  DevBuf<int> ssplit;
+  DevBuf<long long> ssrc;
+  Sell<double> s64;
+  Sell<float> s32;
+  Sell<bf16s> s16;
 };
 
 // dfm_ilu_create, or (given_row / given_col, natural order) a factor on a

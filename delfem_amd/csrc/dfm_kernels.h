@@ -1523,6 +1523,229 @@ __global__ void __launch_bounds__(NT) k_trsv_tile(int tile0, const int* __restri
   if constexpr (DOT) write_partials<1, NT>(dsum, partials, tile, 0);
 }
 
+// ---------------------------------------------------------------------------
+// Split-lane phase-tiled sweeps (b x b blocks, b = 2 or 3; layout: TileSplit):
+// G lanes per tile row, lane r computing row r of every block product
+// (s = sum_c A[r][c] x[c], then t_r -= s, as blk_msub does for its row r),
+// so a level costs a wave a third of the arithmetic of one lane per row.  The
+// lower sweep's D^-1 product needs all of the row's t: the G lanes of a row
+// are one quad of the wave, and each reads the others' values through DPP
+// quad permutes.  Slot order, products and padding are those of k_trsv_tile,
+// so FP64 results are bitwise the per-level sweeps'.  An alternative form
+// (sweep mode 4): G times the threads per tile make its prologue and barriers
+// cost more than the arithmetic it saves on the Oseen factors.
+template <int C>
+__device__ inline double quad_lane(double v) {
+  constexpr int ctrl = C | (C << 2) | (C << 4) | (C << 6);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int C>
+__device__ inline float quad_lane(float v) {
+  constexpr int ctrl = C | (C << 2) | (C << 4) | (C << 6);
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+// the value of v in lane c of this lane's group of G (a quad, or a half quad for G = 2)
+template <int G, typename T>
+__device__ inline T group_lane(T v, int c, int lane) {
+  if constexpr (G == 4) {
+    return c == 0 ? quad_lane<0>(v) : (c == 1 ? quad_lane<1>(v) : (c == 2 ? quad_lane<2>(v) : quad_lane<3>(v)));
+  } else {
+    // lanes (0,1) and (2,3) of a quad: lane c of the pair
+    const T a = (lane & 2) ? quad_lane<2>(v) : quad_lane<0>(v);
+    const T b = (lane & 2) ? quad_lane<3>(v) : quad_lane<1>(v);
+    return c == 0 ? a : b;
+  }
+}
+// t -= sum over B-value row slots [k0, k0 + n) (wave-uniform n) of
+// a_r . x[col]; PROD: the products of the first NP slots to prod[], the
+// lane's further real ones (k < nreal) to ovf[(ovf0 + k - NP) * B], nothing to t
+template <int B, typename TS, typename T, bool PROD, int NP>
+__device__ __forceinline__ void row_slots(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
+                                          int k0, int n, int lane, T& t, T (&prod)[NP], T* ovf, int ovf0,
+                                          int nreal) {
+  constexpr int E = slot_words<TS>(B);
+  constexpr int CH = 8;
+  for (int base = 0; base < n; base += 16) {
+    const int m = n - base < 16 ? n - base : 16;
+    int cc[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++)
+      if (u < m) cc[u] = ld_stream(col + (size_t)(k0 + base + u) * 64 + lane);
+#pragma unroll
+    for (int c0 = 0; c0 < 16; c0 += CH) {
+      if (c0 < m) {
+        unsigned w[CH][E];
+        T xv[CH][B];
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+          if (c0 + u < m) {
+            load_words<B, TS>(val, (size_t)(k0 + base + c0 + u), lane, w[u]);
+            const int c = cc[c0 + u];
+            const size_t cb = (size_t)(c >= 0 ? c : 0) * B;
+#pragma unroll
+            for (int e = 0; e < B; e++) xv[u][e] = c >= 0 ? x[cb + e] : (T)0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+          if (c0 + u < m) {
+            T a[B];
+            words_to_block<B, TS, T>(w[u], a);
+            T v = a[0] * xv[u][0];
+#pragma unroll
+            for (int e = 1; e < B; e++) v = v + a[e] * xv[u][e];
+            if constexpr (PROD) {
+              const int k = base + c0 + u;
+              if (k < NP) {
+#pragma unroll
+                for (int q = 0; q < NP; q++)
+                  if (q == k) prod[q] = v;
+              } else if (k < nreal) {
+                ovf[(size_t)(ovf0 + k - NP) * B] = v;
+              }
+            } else {
+              t -= v;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int B, typename TS, typename T, int NTS, bool UPPER, bool DOT>
+__global__ void __launch_bounds__(NTS) k_trsv_tile_split(int tile0, const int* __restrict__ rows,
+                                                         const int* __restrict__ rlev, const int* __restrict__ depth,
+                                                         const int* __restrict__ sptr, const int* __restrict__ split,
+                                                         const int* __restrict__ col, const TS* __restrict__ val,
+                                                         const TS* __restrict__ dval, const int* __restrict__ rov,
+                                                         int ovf_in, int ovf_out, const T* rhs, T* out,
+                                                         const T* __restrict__ rvec, double* __restrict__ partials,
+                                                         const KState* __restrict__ st) {
+  if (st && st->done) return;
+  constexpr int G = B == 2 ? 2 : 4;
+  constexpr int NT = NTS / G;  // tile rows
+  constexpr int E = slot_words<TS>(B);
+  constexpr int PI = 2 * kTileInSlots;
+  constexpr int PO = UPPER ? kTileOutSlots : 1;
+  __shared__ T xs[(NT + 1) * B];  // tile row NT: zeros
+  // overflow: ovf_in in-phase blocks (B rows of B values), ovf_out products
+  // (B per slot), then the in-phase blocks' tile positions
+  extern __shared__ __align__(16) unsigned char tile_lds[];
+  T* ov_val = reinterpret_cast<T*>(tile_lds);
+  T* ov_prod = ov_val + (size_t)ovf_in * B * B;
+  int* ov_col = reinterpret_cast<int*>(ov_prod + (size_t)ovf_out * B);
+  const int tile = tile0 + blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int trow = tid / G, r = tid % G;
+  const bool act = r < B;
+  const int s = tile * (NTS / 64) + (tid >> 6);
+  const size_t rix = (size_t)tile * NT + trow;
+  const int row = rows[rix];
+  const int lev = rlev[rix];
+  const int nlev = depth[tile];
+  const int k0 = __builtin_amdgcn_readfirstlane(sptr[s]);
+  const int km = __builtin_amdgcn_readfirstlane(split[s]);
+  const int k1 = __builtin_amdgcn_readfirstlane(sptr[s + 1]);
+  const int ki = UPPER ? k0 : km, ni = UPPER ? km - k0 : k1 - km;  // in-phase part
+  const int ko = UPPER ? km : k0, no = UPPER ? k1 - km : km - k0;  // the other phases
+  const int4 ov = reinterpret_cast<const int4*>(rov)[rix];
+  // the plan places a row's in-phase slots beyond kTileInSlots at overflow
+  // slots [ov.x, ov.x + ov.y); this kernel holds PI in registers, so it
+  // stages only the ones beyond PI, at their places in that range
+  const int ovx = ov.x + (PI - kTileInSlots), ovn = ov.y > PI - kTileInSlots ? ov.y - (PI - kTileInSlots) : 0;
+  if (tid < B) xs[NT * B + tid] = (T)0;
+  T t = (T)0, d[B];
+  int ci[PI];
+  unsigned wi[PI][E];
+  T so[PO];
+  double dsum[1] = {0.0};
+#pragma unroll
+  for (int u = 0; u < PI; u++) ci[u] = NT;
+  const bool live = row >= 0 && act;
+  if (live) {
+    t = rhs[(size_t)row * B + r];
+#pragma unroll
+    for (int u = 0; u < PI; u++) {
+      if (u < ni) {
+        const int c = col[(size_t)(ki + u) * 64 + lane];
+        ci[u] = c >= 0 ? c : NT;
+        load_words<B, TS>(val, (size_t)(ki + u), lane, wi[u]);
+      }
+    }
+    if constexpr (!UPPER) load_block<B, TS, T>(dval, (size_t)s, lane, d);
+    for (int u = 0; u < ovn; u++) {
+      T a[B];
+      load_block<B, TS, T>(val, (size_t)(ki + PI + u), lane, a);
+      const int o = ovx + u;
+#pragma unroll
+      for (int e = 0; e < B; e++) ov_val[((size_t)o * B + r) * B + e] = a[e];
+      if (r == 0) {
+        const int c = col[(size_t)(ki + PI + u) * 64 + lane];
+        ov_col[o] = c >= 0 ? c : NT;
+      }
+    }
+    row_slots<B, TS, T, UPPER, PO>(col, val, out, ko, no, lane, t, so, ov_prod + r, ov.z, PO + ov.w);
+  }
+  __syncthreads();  // the zero row, the staged overflow
+  for (int l = 0; l < nlev; l++) {
+    const bool mine = row >= 0 && lev == l;  // the same for the G lanes of a row
+    if (mine) {
+      if (act) {
+        T xv[PI][B];
+#pragma unroll
+        for (int u = 0; u < PI; u++)
+#pragma unroll
+          for (int e = 0; e < B; e++) xv[u][e] = xs[ci[u] * B + e];
+#pragma unroll
+        for (int u = 0; u < PI; u++) {
+          if (u < ni) {
+            T a[B];
+            words_to_block<B, TS, T>(wi[u], a);
+            T v = a[0] * xv[u][0];
+#pragma unroll
+            for (int e = 1; e < B; e++) v = v + a[e] * xv[u][e];
+            t -= v;
+          }
+        }
+        for (int u = 0; u < ovn; u++) {
+          const int o = ovx + u;
+          const int c = ov_col[o];
+          T v = ov_val[((size_t)o * B + r) * B] * xs[c * B];
+#pragma unroll
+          for (int e = 1; e < B; e++) v = v + ov_val[((size_t)o * B + r) * B + e] * xs[c * B + e];
+          t -= v;
+        }
+        if constexpr (UPPER) {
+#pragma unroll
+          for (int u = 0; u < PO; u++)
+            if (u < no) t -= so[u];
+          for (int u = 0; u < ov.w; u++) t -= ov_prod[((size_t)ov.z + u) * B + r];
+        }
+      }
+      if constexpr (!UPPER) {
+        // y_r = sum_c D[r][c] t_c, t_c from lane c of the row's group
+        T tc[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) tc[c] = group_lane<G, T>(t, c, lane);
+        T y = d[0] * tc[0];
+#pragma unroll
+        for (int c = 1; c < B; c++) y = y + d[c] * tc[c];
+        t = y;
+      }
+      if (act) xs[trow * B + r] = t;
+    }
+    __syncthreads();
+  }
+  if (live) {
+    out[(size_t)row * B + r] = t;
+    if (DOT) dsum[0] += (double)rvec[(size_t)row * B + r] * (double)t;
+  }
+  if constexpr (DOT) write_partials<1, NTS>(dsum, partials, tile, 0);
+}
+
 // Forward sweep of one level fused with the PCG update that precedes it
 // (solver_ls_iter.cpp:140-145, then :158): for each row of the level
 //   r_i += (-alpha) Ap_i ; x_i += alpha p_i ; z_i = Dinv_i (r_i - sum L_ij z_j).

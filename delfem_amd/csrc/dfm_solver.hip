@@ -117,6 +117,22 @@ int ilu_refresh_sell(dfm_ilu_s* M, const dfm_bcrs_s* A, bool want32, bool want16
       DFM_TRY(gather(ts->v16, ts->src, M->d_fval.p, bb));
       if (lower) DFM_TRY(gather_dia(ts->v16, ts->v64.rows.p, nullptr, M->d_fdia.p, bb));
     }
+    if (ts->g) {
+      // block row r of entry q: the b values at from[(q*b + r) * b]
+      const int b = (int)M->b;
+      DFM_TRY(gather(ts->s64, ts->ssrc, M->d_fval.p, b));
+      if (lower) DFM_TRY(gather_dia(ts->s64, ts->s64.rows.p, nullptr, M->d_fdia.p, b));
+      if (M->have32) {
+        if (ts->s32.nslice != ts->s64.nslice) DFM_TRY(sell_alloc_like(ts->s64, ts->s32));
+        DFM_TRY(gather(ts->s32, ts->ssrc, M->d_fval.p, b));
+        if (lower) DFM_TRY(gather_dia(ts->s32, ts->s64.rows.p, nullptr, M->d_fdia.p, b));
+      }
+      if (M->have16) {
+        if (ts->s16.nslice != ts->s64.nslice) DFM_TRY(sell_alloc_like(ts->s64, ts->s16));
+        DFM_TRY(gather(ts->s16, ts->ssrc, M->d_fval.p, b));
+        if (lower) DFM_TRY(gather_dia(ts->s16, ts->s64.rows.p, nullptr, M->d_fdia.p, b));
+      }
+    }
   }
   if (A && (M->a_version != A->version || (M->have32 && !M->a32_valid))) {
     const int* ord = M->permuted ? M->d_order.p : nullptr;
@@ -156,7 +172,7 @@ static int g_persist_mode = env_int("DFM_PERSIST", 1);      // dfm_set_sweep_mod
 static int g_persist_grid = env_int("DFM_PERSIST_GRID", 0);
 static bool use_tile(const TileSweep* ts, const SweepSync* ps, const KState* st) {
   if (!ts || !st || ts->ntile == 0) return false;
-  return g_persist_mode == 3 || (g_persist_mode == 1 && ps && ps->ngroup >= kPersistMinLevels);
+  return g_persist_mode >= 3 || (g_persist_mode == 1 && ps && ps->ngroup >= kPersistMinLevels);
 }
 static bool use_persist(const SweepSync* ps, const KState* st, const TileSweep* ts = nullptr) {
   if (!ps || !st || ps->nslice == 0 || use_tile(ts, ps, st)) return false;
@@ -168,6 +184,17 @@ static const Sell<TS>& tile_vals(const TileSweep& ts) {
   else if constexpr (std::is_same<TS, float>::value) return ts.v32;
   else return ts.v16;
 }
+template <typename TS>
+static const Sell<TS>& split_vals(const TileSweep& ts) {
+  if constexpr (std::is_same<TS, double>::value) return ts.s64;
+  else if constexpr (std::is_same<TS, float>::value) return ts.s32;
+  else return ts.s16;
+}
+// Sweep mode 4 takes the split-lane tiled kernels where the layout has them
+// (2x2 and 3x3 blocks); modes 1 and 3 one lane per tile row, which measured
+// faster (Oseen 256^2 FAC32: 51 / 61 us per lower / upper sweep against 59 /
+// 69 us at the split form's best tile size).
+static int g_tile_split = 0;
 // One launch per phase, in the plan's order.
 template <int B, typename TS, typename T, int NT, bool UPPER, bool DOT>
 static int tile_sweep_nt(const TileSweep& ts, const T* rhs, T* out, const T* rvec, double* partials,
@@ -185,9 +212,42 @@ static int tile_sweep_nt(const TileSweep& ts, const T* rhs, T* out, const T* rve
   }
   return DFM_OK;
 }
+// Split-lane form: G lanes per tile row, G * nt threads per tile.
+template <int B, typename TS, typename T, int NTS, bool UPPER, bool DOT>
+static int tile_split_nt(const TileSweep& ts, const T* rhs, T* out, const T* rvec, double* partials,
+                         const KState* st) {
+  const Sell<TS>& v = split_vals<TS>(ts);
+  const Sell<double>& lay = ts.s64;
+  const size_t shmem = (size_t)ts.ovf_in * (B * B * sizeof(T) + sizeof(int)) + (size_t)ts.ovf_out * B * sizeof(T);
+  for (size_t p = 0; p + 1 < ts.phase_tile.size(); p++) {
+    const int t0 = ts.phase_tile[p], nt = ts.phase_tile[p + 1] - t0;
+    if (nt == 0) continue;
+    hipLaunchKernelGGL((k_trsv_tile_split<B, TS, T, NTS, UPPER, DOT>), dim3(nt), dim3(NTS), shmem, stream(), t0,
+                       ts.v64.rows.p, ts.rlev.p, ts.depth.p, lay.slice_ptr.p, ts.ssplit.p, lay.col.p, v.val.p,
+                       v.dval.p, ts.rov.p, ts.ovf_in, ts.ovf_out, rhs, out, rvec, partials, st);
+    DFM_HIP(hipGetLastError());
+  }
+  return DFM_OK;
+}
 template <int B, typename TS, typename T, bool UPPER, bool DOT>
 static int tile_sweep(const TileSweep& ts, const T* rhs, T* out, const T* rvec, double* partials,
                       const KState* st) {
+  if constexpr (B == 2 || B == 3) {
+    if (g_tile_split && ts.g) {
+      constexpr int G = B == 2 ? 2 : 4;
+      switch (ts.nt) {
+        case 64: return tile_split_nt<B, TS, T, 64 * G, UPPER, DOT>(ts, rhs, out, rvec, partials, st);
+        case 128: return tile_split_nt<B, TS, T, 128 * G, UPPER, DOT>(ts, rhs, out, rvec, partials, st);
+        default:
+          if constexpr (G == 4) {
+            // 256 rows x 4 lanes would exceed a workgroup: one lane per row
+            break;
+          } else {
+            return tile_split_nt<B, TS, T, 256 * G, UPPER, DOT>(ts, rhs, out, rvec, partials, st);
+          }
+      }
+    }
+  }
   switch (ts.nt) {
     case 64: return tile_sweep_nt<B, TS, T, 64, UPPER, DOT>(ts, rhs, out, rvec, partials, st);
     case 128: return tile_sweep_nt<B, TS, T, 128, UPPER, DOT>(ts, rhs, out, rvec, partials, st);
@@ -1005,9 +1065,10 @@ int dfm_get_kernel_timing(double* avg_all_ms, double* avg_working_ms, int* nlaun
 }
 
 int dfm_set_sweep_mode(int mode, int grid) {
-  DFM_CHECK(mode >= 0 && mode <= 3 && grid >= 0 && grid <= dfm::kPersistMaxGrid, DFM_BAD_ARG,
+  DFM_CHECK(mode >= 0 && mode <= 4 && grid >= 0 && grid <= dfm::kPersistMaxGrid, DFM_BAD_ARG,
             "bad sweep mode %d / grid %d", mode, grid);
   dfm::g_persist_mode = mode;
+  dfm::g_tile_split = mode == 4;
   dfm::g_persist_grid = grid;
   return DFM_OK;
 }

@@ -318,7 +318,8 @@ def set_sweep_mode(mode, grid=0):
     level; 1 (default) for factors of 8 or more levels one launch per colour
     (dependent rows tiled into workgroups) where the ordering allows, else one
     persistent launch; 2 persistent always; 3 per colour wherever possible;
-    grid = workgroups of a persistent launch (0: automatic)."""
+    4 as 3 with one lane per block row of 2x2 / 3x3 blocks (3: one per tile
+    row); grid = workgroups of a persistent launch (0: automatic)."""
     call("dfm_set_sweep_mode", int(mode), int(grid))
 
 

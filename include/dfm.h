@@ -262,7 +262,8 @@ int dfm_solve_pcg(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
  * block-multicolour ordering (tiles of dependent rows run level by level
  * inside one workgroup) where every tile fits 256 rows, else one persistent
  * launch; 2 persistent always; 3 per colour wherever possible (per level
- * elsewhere).  grid = workgroups of a persistent launch (0: twice the slices
+ * elsewhere); 4 as 3 with one lane per block row of 2x2 / 3x3 blocks instead of
+ * one per tile row (slower here; kept as an alternative).  grid = workgroups of a persistent launch (0: twice the slices
  * of the largest level, at least 64; at most 4096).  FP64 sweeps give the
  * same bits in every mode.  Process-wide; initial values from DFM_PERSIST /
  * DFM_PERSIST_GRID (DFM_TILE_ROWS, read at factor creation: rows a tile is

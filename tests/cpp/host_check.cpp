@@ -151,6 +151,53 @@ static void check_tiles(const TilePlan& tp, unsigned nb, const std::vector<unsig
   }
 }
 
+// TileSplit: every live lane (tile row t, block row r < b) holds the row's
+// first-part then second-part slots of the TilePlan, in order, with
+// src = q*b + r; idle and padding lanes hold nothing.
+static void check_split(const TilePlan& tp, int b) {
+  TileSplit sp;
+  build_tile_split(tp, b, sp);
+  const int g = sp.g, nt = tp.nt, rpw = kSlice / g, wpt = nt * g / kSlice;
+  EXPECT(g == (b == 2 ? 2 : 4) && sp.nslice == tp.ntile * wpt, "split shape");
+  EXPECT((int)sp.slice_ptr.size() == sp.nslice + 1 && (int)sp.split.size() == sp.nslice, "split slices");
+  EXPECT(sp.col.size() == (size_t)sp.nk * kSlice && sp.src.size() == sp.col.size(), "split slots");
+  if (g_fail) return;
+  for (int s = 0; s < sp.nslice; s++) {
+    const int t = s / wpt, w = s % wpt;
+    for (int l = 0; l < kSlice; l++) {
+      const int j = l / g, r = l % g;
+      const int trow = w * rpw + j;
+      const size_t rix = (size_t)t * nt + trow;
+      const int row = tp.rows[rix];
+      const bool live = row >= 0 && r < b;
+      EXPECT(sp.rows[(size_t)s * kSlice + l] == (live ? row * b + r : -1), "split rows slice %d lane %d", s, l);
+      // the TilePlan's slots of this row
+      const int s0 = (int)(rix / kSlice), l0 = (int)(rix % kSlice);
+      for (int part = 0; part < 2; part++) {
+        std::vector<std::pair<int, long long>> want;
+        const int a0 = part ? tp.split[s0] : tp.slice_ptr[s0], a1 = part ? tp.slice_ptr[s0 + 1] : tp.split[s0];
+        for (int k = a0; k < a1; k++)
+          if (tp.src[(size_t)k * kSlice + l0] >= 0)
+            want.push_back({tp.col[(size_t)k * kSlice + l0], tp.src[(size_t)k * kSlice + l0]});
+        const int k0 = part ? sp.split[s] : sp.slice_ptr[s], k1 = part ? sp.slice_ptr[s + 1] : sp.split[s];
+        size_t n = 0;
+        for (int k = k0; k < k1; k++) {
+          const long long q = sp.src[(size_t)k * kSlice + l];
+          const int c = sp.col[(size_t)k * kSlice + l];
+          if (q < 0) {
+            EXPECT(c == -1, "split padding with a column");
+            continue;
+          }
+          EXPECT(live && n < want.size() && c == want[n].first && q == want[n].second * b + r,
+                 "split slice %d lane %d slot %d", s, l, k);
+          n++;
+        }
+        EXPECT(n == (live ? want.size() : 0), "split slice %d lane %d: %zu of %zu slots", s, l, n, want.size());
+      }
+    }
+  }
+}
+
 int main(int argc, char** argv) {
   if (argc != 3) {
     fprintf(stderr, "usage: host_check <in> <out>\n");
@@ -216,11 +263,13 @@ int main(int argc, char** argv) {
     TilePlan tL, tU;
     if (build_tile_plan(nnode, frow, fcol, fdia, phase_ptr, false, pack_rows, bsize, tL)) {
       check_tiles(tL, nnode, frow, fcol, fdia, phase_ptr, false);
+      if (bsize == 2 || bsize == 3) check_split(tL, bsize);
       ntL = tL.nt;
       tiL = tL.ntile;
     }
     if (build_tile_plan(nnode, frow, fcol, fdia, phase_ptr, true, pack_rows, bsize, tU)) {
       check_tiles(tU, nnode, frow, fcol, fdia, phase_ptr, true);
+      if (bsize == 2 || bsize == 3) check_split(tU, bsize);
       ntU = tU.nt;
       tiU = tU.ntile;
     }

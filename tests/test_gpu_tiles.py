@@ -1,9 +1,11 @@
 """The phase-tiled triangular sweep (one launch per colour of a multicolour
 or block-multicolour ordering; tiles of dependent rows run level by level
-inside a workgroup) against the one-launch-per-level sweep.  A tile row
+inside a workgroup) against the one-launch-per-level sweep, in both forms:
+sweep mode 3 (one lane per tile row) and mode 4 (2x2 and 3x3 blocks: one
+lane per block row, the D^-1 product through quad permutes).  A tile row
 subtracts its slots in the factor's column order, products formed as in the
 per-level kernels, so BiCGSTAB (whose reductions do not involve the sweeps)
-must give the same bits in both modes for every storage type of the factors.
+must give the same bits in every mode for every storage type of the factors.
 PCG's first preconditioner application reduces <r,z> over the tiles'
 partials, so PCG is held to one iteration and 1e-8 of the per-level run.
 Block sizes of the ordering select tiles of 64, 128 and 256 threads; a
@@ -75,7 +77,7 @@ def test_bicgstab_bitwise_tiled(system, gpu, precision):
         pytest.skip("symmetric system: PCG")
     gpu.set_sweep_mode(0, 0)
     base = _solve(gpu, s, M, "bicg", precision)
-    for mode in (3, 1):
+    for mode in (3, 4, 1):
         _case(f"{name} bicg {precision} sweep mode {mode}")
         gpu.set_sweep_mode(mode, 0)
         got = _solve(gpu, s, M, "bicg", precision)
@@ -91,19 +93,21 @@ def test_pcg_tiled(system, gpu, precision):
         pytest.skip("nonsymmetric system: BiCGSTAB")
     gpu.set_sweep_mode(0, 0)
     ok0, _, it0, x0, _ = _solve(gpu, s, M, "pcg", precision)
-    _case(f"{name} pcg {precision} tiled")
-    gpu.set_sweep_mode(3, 0)
-    ok, conv, it, x, _ = _solve(gpu, s, M, "pcg", precision)
-    assert ok and ok0 and conv < 1e-10 and abs(it - it0) <= 1
-    assert np.linalg.norm(x - x0) <= 1e-8 * np.linalg.norm(x0)
+    for mode in (3, 4):
+        _case(f"{name} pcg {precision} tiled mode {mode}")
+        gpu.set_sweep_mode(mode, 0)
+        ok, conv, it, x, _ = _solve(gpu, s, M, "pcg", precision)
+        assert ok and ok0 and conv < 1e-10 and abs(it - it0) <= 1
+        assert np.linalg.norm(x - x0) <= 1e-8 * np.linalg.norm(x0)
 
 
 def test_kernel_timer_runs_tiled(system, gpu):
     import ctypes as C
     from delfem_amd import _lib
     name, prob, s, M = system
-    gpu.set_sweep_mode(3, 0)
-    for which in (1, 2):
-        ms, nbytes = C.c_double(), C.c_double()
-        _lib.call("dfm_bench_kernel", s.A.h, M.h, which, 0, 5, C.byref(ms), C.byref(nbytes))
-        assert ms.value > 0 and nbytes.value > 0
+    for mode in (3, 4):
+        gpu.set_sweep_mode(mode, 0)
+        for which in (1, 2):
+            ms, nbytes = C.c_double(), C.c_double()
+            _lib.call("dfm_bench_kernel", s.A.h, M.h, which, 0, 5, C.byref(ms), C.byref(nbytes))
+            assert ms.value > 0 and nbytes.value > 0
