@@ -142,7 +142,7 @@ _SIGS = {
     "dfm_mseg_matvec": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
     "dfm_mseg_dot": (C.c_int, [H, vp, vp, f64p]),
     "dfm_mseg_ilu_create": (C.c_int, [H, C.c_int, C.POINTER(H)]),
-    "dfm_mseg_ilu_create_levels": (C.c_int, [H, vp, C.POINTER(H)]),
+    "dfm_mseg_ilu_create_levels": (C.c_int, [H, vp, vp, C.c_uint, C.POINTER(H)]),
     "dfm_mseg_ilu_get_pattern": (C.c_int, [H, C.c_uint, C.c_uint, u32p, vp, vp]),
     "dfm_mseg_ilu_destroy": (C.c_int, [H]),
     "dfm_mseg_ilu_set_value": (C.c_int, [H, i32p]),

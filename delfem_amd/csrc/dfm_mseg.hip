@@ -452,18 +452,22 @@ int dfm_mseg_dot(dfm_mseg* S, const double* x, const double* y, double* out) {
 int dfm_mseg_ilu_create(dfm_mseg* S, int lev_fill, dfm_msilu** out) {
   DFM_CHECK(S && out && lev_fill >= -1, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_create");
   std::vector<int> alev((size_t)S->nseg * S->nseg, lev_fill);
-  return dfm_mseg_ilu_create_levels(S, alev.data(), out);
+  return dfm_mseg_ilu_create_levels(S, alev.data(), nullptr, 0, out);
 }
 
 // The same with a fill level per block (i, j) (row-major nseg x nseg), what
 // SetFillInLevel(lev, ilss) calls resolve to (preconditioner.cpp:108-122);
-// all levels 0: no fill (:124-130).  A block the fill would need but the
-// system lacks stays out when its own level is 0 (the reference creates it
-// empty, :171-185).
-int dfm_mseg_ilu_create_levels(dfm_mseg* S, const int* alev, dfm_msilu** out) {
-  DFM_CHECK(S && out && alev, DFM_BAD_ARG, "bad arguments to dfm_mseg_ilu_create_levels");
+// all levels 0 and no fill blocks: no fill (:105, 124-130).  fill_blk:
+// SetFillBlk's rows of segment 0, which take fill of any level in block
+// (0, 0) (AddFracPtn(lev, aBlkFill), :143-148).  A block the fill would need
+// but the system lacks stays out when its own level is 0 (the reference
+// creates it empty, :171-185).
+int dfm_mseg_ilu_create_levels(dfm_mseg* S, const int* alev, const unsigned* fill_blk, unsigned nfill,
+                               dfm_msilu** out) {
+  DFM_CHECK(S && out && alev && (nfill == 0 || fill_blk), DFM_BAD_ARG,
+            "bad arguments to dfm_mseg_ilu_create_levels");
   const unsigned n = S->nseg;
-  bool any = false;
+  bool any = nfill > 0;
   for (unsigned t = 0; t < n * n; t++) {
     DFM_CHECK(alev[t] >= -1, DFM_BAD_ARG, "fill level %d of block %u below -1", alev[t], t);
     any = any || alev[t] != 0;
@@ -493,6 +497,18 @@ int dfm_mseg_ilu_create_levels(dfm_mseg* S, const int* alev, dfm_msilu** out) {
         if (i == j) {
           for (unsigned k = 0; k < i; k++)
             if (S->nd[i][k] && S->nd[k][i]) lp_lowup(D[i], S->nb[i] - 1, F[i][k], F[k][i], K, true, rl);
+          if (i == 0 && nfill > 0) {
+            // block (0, 0) still holds A_00's pattern at level 0: AddFracPtn(K, aBlkFill) is the
+            // fill-block symbolic of that pattern
+            const std::vector<unsigned> blk(fill_blk, fill_blk + nfill);
+            std::vector<unsigned> fr, fc, fd;
+            std::vector<int> fl;
+            DFM_TRY(iluk_symbolic(S->nb[0], S->dia[0]->rowptr, S->dia[0]->col, K, nullptr, fr, fc, fl, fd, &blk));
+            D[0].rp = std::move(fr);
+            D[0].cl = std::move(fc);
+            D[0].lev = std::move(fl);
+            continue;
+          }
           lp_dia_fill(D[i], K, rl);
           continue;
         }

@@ -646,8 +646,8 @@ class MultiSegmentSystem:
         _sync_torch()
         call("dfm_mseg_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
 
-    def preconditioner(self, lev=0, inputs=None):
-        return MultiSegmentILU(self, lev, inputs)
+    def preconditioner(self, lev=0, inputs=None, fill_blk=None):
+        return MultiSegmentILU(self, lev, inputs, fill_blk)
 
 
 def fill_levels(nseg, inputs):
@@ -669,15 +669,19 @@ def fill_levels(nseg, inputs):
 class MultiSegmentILU:
     """CPreconditioner_ILU(ls, lev) across the segments
     (src/ls/preconditioner.cpp:57-335): lev 0 no fill, -1 unlimited; inputs:
-    SetFillInLevel(lev, ilss) requests [(lev, ilss), ...] instead of lev."""
+    SetFillInLevel(lev, ilss) requests [(lev, ilss), ...] instead of lev;
+    fill_blk: SetFillBlk's rows of segment 0 (fill of any level there)."""
 
-    def __init__(self, S, lev=0, inputs=None):
+    def __init__(self, S, lev=0, inputs=None, fill_blk=None):
         h = C.c_void_p()
-        if inputs is None:
+        if inputs is None and fill_blk is None:
             call("dfm_mseg_ilu_create", S.h, int(lev), C.byref(h))
         else:
+            inputs = [(lev, -1)] if inputs is None else inputs
             alev = np.ascontiguousarray(fill_levels(len(S.groups), inputs), dtype=np.int32)
-            call("dfm_mseg_ilu_create_levels", S.h, _ptr(alev), C.byref(h))
+            blk = np.ascontiguousarray(fill_blk if fill_blk is not None else [], dtype=np.uint32)
+            call("dfm_mseg_ilu_create_levels", S.h, _ptr(alev), _ptr(blk) if blk.size else None, blk.size,
+                 C.byref(h))
         self.h, self.S, self.lev = h, S, lev
 
     def __del__(self):

@@ -360,7 +360,8 @@ class CPreconditioner_ILU : public CPreconditioner {
     return alev;
   }
   // SetFillBlk(aBlk) (preconditioner.h:80): block rows (and columns) of the
-  // first segment's factor that take fill of any level (AddFracPtn(lev, aBlk))
+  // first segment's factor that take fill of any level (AddFracPtn(lev, aBlk)),
+  // with one segment or several
   void SetFillBlk(const std::vector<unsigned>& aBlk) { fill_blk_ = aBlk; }
   // SetOrdering(order[new] = old) (preconditioner.h:99-103); empty = natural.
   // As the reference, the ordered factor is ILU(0) whatever the fill level
@@ -394,10 +395,10 @@ class CPreconditioner_ILU : public CPreconditioner {
     } else {
       if (ordering_ != DFM_ORDER_NATURAL)
         throw Error("CPreconditioner_ILU: orderings need a single segment", DFM_NOT_IMPLEMENTED);
-      if (!fill_blk_.empty())
-        throw Error("CPreconditioner_ILU: fill blocks with several segments are not provided", DFM_NOT_IMPLEMENTED);
       mseg_ = ls.mseg();
-      check(dfm_mseg_ilu_create_levels(mseg_.get(), alev.data(), &ms_), "CPreconditioner_ILU::SetLinearSystem");
+      check(dfm_mseg_ilu_create_levels(mseg_.get(), alev.data(), fill_blk_.empty() ? nullptr : fill_blk_.data(),
+                                       (unsigned)fill_blk_.size(), &ms_),
+            "CPreconditioner_ILU::SetLinearSystem");
     }
     ls_ = &ls;
     gen_ = ls.structure();

@@ -354,9 +354,12 @@ int dfm_mseg_ilu_create(dfm_mseg *S, int lev_fill, dfm_msilu **out);
 /* The same with a fill level per block (i, j), lev[i * nseg + j] (-1 unlimited): what
  * CPreconditioner_ILU::SetFillInLevel(lev, ilss) requests resolve to (preconditioner.h:77,
  * preconditioner.cpp:108-131; a request sets segment ilss's block row and column, ilss -1
- * every block, later requests win; all zero: no fill).  A missing block that the fill would
- * need stays out when its own level is 0 (the reference creates it empty). */
-int dfm_mseg_ilu_create_levels(dfm_mseg *S, const int *lev, dfm_msilu **out);
+ * every block, later requests win; all zero and no fill blocks: no fill), and fill_blk:
+ * SetFillBlk's rows of segment 0 (preconditioner.h:80, preconditioner.cpp:143-148), which
+ * take fill of any level in block (0, 0) (nfill 0: none).  A missing block that the fill
+ * would need stays out when its own level is 0 (the reference creates it empty). */
+int dfm_mseg_ilu_create_levels(dfm_mseg *S, const int *lev, const unsigned *fill_blk, unsigned nfill,
+                               dfm_msilu **out);
 int dfm_mseg_ilu_destroy(dfm_msilu *M);
 /* SetValue (:217-297): copies, then per block in (i, j) order DoILUDecompLowUp + DoILUDecomp
  * (diagonal), DoILUDecompUp / Low / LowUp (off-diagonal); DFM_SINGULAR as dfm_ilu_set_value */

@@ -1618,11 +1618,13 @@ static void lp_dia_fill(orc_lpat *D, int K, orc_rowlist *rl)
  * every k < i, then its own fill; above: LowUp with k < i, then Up with
  * factor i; below: LowUp with k < j, then Low with factor j.  Returns -1 when
  * a block that fill would create is missing. */
-/* alev: the fill level of each block (i, j), row-major n x n (preconditioner.cpp:108-131) */
-static int ms_symbolic(orc_ms *m, const int *alev)
+/* alev: the fill level of each block (i, j), row-major n x n (preconditioner.cpp:108-131);
+ * fill_blk (nfill entries): SetFillBlk's rows of segment 0 that take fill of any
+ * level, AddFracPtn(lev, aBlkFill) on block (0, 0) (:143-148) */
+static int ms_symbolic(orc_ms *m, const int *alev, const unsigned *fill_blk, unsigned nfill)
 {
     const unsigned n = m->nseg;
-    int any = 0;
+    int any = nfill > 0;
     for (unsigned i = 0; i < n; i++) {
         lp_free(&m->fp[i]);
         lp_from(&m->fp[i], m->nb[i], m->rowptr[i], m->col[i]);
@@ -1635,7 +1637,7 @@ static int ms_symbolic(orc_ms *m, const int *alev)
     }
     for (unsigned t = 0; t < n * n; t++)
         if (alev[t] != 0) any = 1;
-    if (!any) return 0; /* preconditioner.cpp:124-130 */
+    if (!any) return 0; /* preconditioner.cpp:105, 124-130 */
     orc_rowlist rl = {NULL, 0, 0, -1};
     int rc = 0;
     for (unsigned i = 0; i < n && rc == 0; i++)
@@ -1645,6 +1647,31 @@ static int ms_symbolic(orc_ms *m, const int *alev)
                 for (unsigned kk = 0; kk < i; kk++)
                     if (m->nd[i][kk] && m->nd[kk][i])
                         lp_lowup(&m->fp[i], m->nb[i] - 1, &m->nd[i][kk]->fp, &m->nd[kk][i]->fp, K, 1, &rl);
+                if (i == 0 && nfill > 0) {
+                    /* block (0, 0) is still A_00's pattern at level 0 here */
+                    const unsigned nb0 = m->nb[0];
+                    long cap = (long)m->rowptr[0][nb0] * 4 + 16;
+                    for (;;) {
+                        unsigned *fr = (unsigned *)malloc(sizeof(unsigned) * (nb0 + 1));
+                        unsigned *fc = (unsigned *)malloc(sizeof(unsigned) * (size_t)cap);
+                        int *fl = (int *)malloc(sizeof(int) * (size_t)cap);
+                        unsigned *fd = (unsigned *)malloc(sizeof(unsigned) * (nb0 ? nb0 : 1));
+                        const long r = orc_iluk_symbolic_blk(nb0, m->rowptr[0], m->col[0], K, fill_blk, nfill, fr, fc,
+                                                             fl, fd, cap);
+                        if (r >= 0) {
+                            lp_free(&m->fp[0]);
+                            lp_begin(&m->fp[0], nb0, (size_t)r + 1);
+                            memcpy(m->fp[0].rp, fr, sizeof(unsigned) * (nb0 + 1));
+                            memcpy(m->fp[0].cl, fc, sizeof(unsigned) * (size_t)r);
+                            memcpy(m->fp[0].lev, fl, sizeof(int) * (size_t)r);
+                        }
+                        free(fr); free(fc); free(fl); free(fd);
+                        if (r == -1) { rc = -1; break; }
+                        if (r >= 0) break;
+                        cap = -r;
+                    }
+                    continue;
+                }
                 lp_dia_fill(&m->fp[i], K, &rl);
                 continue;
             }
@@ -1837,15 +1864,16 @@ int orc_ms_ilu(void *p, int lev_fill)
     orc_ms *m = (orc_ms *)p;
     int alev[ORC_MS_MAX * ORC_MS_MAX];
     for (unsigned t = 0; t < m->nseg * m->nseg; t++) alev[t] = lev_fill;
-    return orc_ms_ilu_levels(p, alev);
+    return orc_ms_ilu_levels(p, alev, NULL, 0);
 }
 /* the same with a fill level per block (i, j), row-major nseg x nseg: what
- * SetFillInLevel(lev, ilss) calls resolve to (preconditioner.cpp:108-122) */
-int orc_ms_ilu_levels(void *p, const int *alev)
+ * SetFillInLevel(lev, ilss) calls resolve to (preconditioner.cpp:108-122);
+ * fill_blk: SetFillBlk's rows of segment 0 (NULL / 0: none) */
+int orc_ms_ilu_levels(void *p, const int *alev, const unsigned *fill_blk, unsigned nfill)
 {
     orc_ms *m = (orc_ms *)p;
     const unsigned n = m->nseg;
-    if (ms_symbolic(m, alev) < 0) return -2;
+    if (ms_symbolic(m, alev, fill_blk, nfill) < 0) return -2;
     for (unsigned i = 0; i < n; i++) {
         const unsigned nb = m->nb[i], bb = m->b[i] * m->b[i];
         const unsigned ncrs = m->fp[i].rp[nb];

@@ -178,7 +178,7 @@ void orc_ms_set_dia(void *p, unsigned i, const unsigned *rowptr, const unsigned 
 void orc_ms_set_nondia(void *p, unsigned i, unsigned j, const unsigned *rowptr, const unsigned *col, const double *val);
 void orc_ms_matvec(void *p, double alpha, const double *x, double beta, double *y);
 int orc_ms_ilu(void *p, int lev_fill);
-int orc_ms_ilu_levels(void *p, const int *alev);
+int orc_ms_ilu_levels(void *p, const int *alev, const unsigned *fill_blk, unsigned nfill);
 unsigned orc_ms_factor_ncrs(void *p, unsigned i, unsigned j);
 void orc_ms_get_pattern(void *p, unsigned i, unsigned j, unsigned *rowptr, unsigned *col);
 void orc_ms_get_factor(void *p, unsigned i, unsigned j, double *fval, double *fdia);

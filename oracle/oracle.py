@@ -113,7 +113,7 @@ class Orc:
         L.orc_ms_ilu.restype = C.c_int
         L.orc_ms_ilu.argtypes = [vp, C.c_int]
         L.orc_ms_ilu_levels.restype = C.c_int
-        L.orc_ms_ilu_levels.argtypes = [vp, vp]
+        L.orc_ms_ilu_levels.argtypes = [vp, vp, vp, C.c_uint]
         L.orc_ms_factor_ncrs.restype = C.c_uint
         L.orc_ms_factor_ncrs.argtypes = [vp, C.c_uint, C.c_uint]
         L.orc_ms_get_pattern.argtypes = [vp, C.c_uint, C.c_uint, vp, vp]
@@ -576,7 +576,7 @@ class Ref:
             "ref_ms_get_block": (None, [vp, C.c_uint, C.c_uint, vp, vp]),
             "ref_ms_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
             "ref_ms_ilu": (C.c_int, [vp, C.c_int]),
-            "ref_ms_ilu_inputs": (C.c_int, [vp, C.c_int, vp, vp]),
+            "ref_ms_ilu_inputs": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int]),
             "ref_ms_precond": (None, [vp, vp]),
             "ref_ms_solve": (C.c_int, [vp, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_bmat_free": (None, [vp]),
@@ -857,12 +857,15 @@ class OrcMS:
         self.orc.L.orc_ms_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
         return y
 
-    def ilu(self, lev=0, inputs=None):
+    def ilu(self, lev=0, inputs=None, fill_blk=None):
         """CPreconditioner_ILU(ls, lev) across the segments; True on success.
-        inputs: SetFillInLevel(lev, ilss) calls [(lev, ilss), ...] instead."""
-        if inputs is not None:
+        inputs: SetFillInLevel(lev, ilss) calls [(lev, ilss), ...] instead;
+        fill_blk: SetFillBlk's rows of segment 0."""
+        if inputs is not None or fill_blk is not None:
+            inputs = [(lev, -1)] if inputs is None else inputs
             alev = np.ascontiguousarray(fill_levels(self.nseg, inputs), dtype=np.int32)
-            return self.orc.L.orc_ms_ilu_levels(self.h, _p(alev)) == 0
+            blk = u32(fill_blk if fill_blk is not None else [])
+            return self.orc.L.orc_ms_ilu_levels(self.h, _p(alev), _p(blk) if blk.size else None, blk.size) == 0
         return self.orc.L.orc_ms_ilu(self.h, int(lev)) == 0
 
     def pattern(self, i, j):
@@ -949,11 +952,14 @@ class RefMS:
         self.ref.L.ref_ms_matvec(self.h, float(alpha), _p(f64(x)), float(beta), _p(y))
         return y
 
-    def ilu(self, lev=0, inputs=None):
-        if inputs is not None:
+    def ilu(self, lev=0, inputs=None, fill_blk=None):
+        if inputs is not None or fill_blk is not None:
+            inputs = [(lev, -1)] if inputs is None else inputs
             lv = np.ascontiguousarray([l for l, _ in inputs], dtype=np.int32)
             ss = np.ascontiguousarray([i for _, i in inputs], dtype=np.int32)
-            return bool(self.ref.L.ref_ms_ilu_inputs(self.h, len(inputs), _p(lv), _p(ss)))
+            blk = u32(fill_blk if fill_blk is not None else [])
+            return bool(self.ref.L.ref_ms_ilu_inputs(self.h, len(inputs), _p(lv), _p(ss), _p(blk) if blk.size else None,
+                                                     blk.size))
         return bool(self.ref.L.ref_ms_ilu(self.h, int(lev)))
 
     def precond(self, z):

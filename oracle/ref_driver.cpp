@@ -594,12 +594,14 @@ int ref_ms_ilu(void* p, int lev) {
   m->prec->SetLinearSystem(m->ls);
   return m->prec->SetValue(m->ls) ? 1 : 0;
 }
-// The same with SetFillInLevel(lev[t], ilss[t]) for t < nin, in that order
-int ref_ms_ilu_inputs(void* p, int nin, const int* lev, const int* ilss) {
+// The same with SetFillInLevel(lev[t], ilss[t]) for t < nin, in that order,
+// and SetFillBlk(fill_blk[0 .. nfill)) when nfill > 0
+int ref_ms_ilu_inputs(void* p, int nin, const int* lev, const int* ilss, const unsigned* fill_blk, int nfill) {
   RefMS* m = static_cast<RefMS*>(p);
   delete m->prec;
   m->prec = new LsSol::CPreconditioner_ILU;
   for (int t = 0; t < nin; t++) m->prec->SetFillInLevel(lev[t], ilss[t]);
+  if (nfill > 0) m->prec->SetFillBlk(std::vector<unsigned>(fill_blk, fill_blk + nfill));
   m->prec->SetLinearSystem(m->ls);
   return m->prec->SetValue(m->ls) ? 1 : 0;
 }

@@ -239,3 +239,42 @@ def test_segment_fill_levels(orc, gpu, name, inputs):
     xd = torch.from_numpy(x.copy()).cuda()
     M.SolvePrecond(xd)
     assert np.array_equal(xd.cpu().numpy(), O.precond(x))
+
+
+@pytest.mark.parametrize("inputs", [None, [(1, 1)], [(2, -1)]], ids=["lev0", "lev1seg1", "lev2"])
+@pytest.mark.parametrize("name", sorted(FILL_CASES))
+def test_segment_fill_blocks(orc, gpu, name, inputs):
+    """SetFillBlk with several segments (preconditioner.h:80,
+    preconditioner.cpp:143-148): listed rows of segment 0 take fill of any
+    level in block (0, 0); every block's pattern, factor values and
+    SolvePrecond equal the oracle's bit for bit (the oracle is pinned to the
+    compiled reference in tests/test_oracle_ref.py)."""
+    import torch
+    make, groups, kind = FILL_CASES[name]
+    prob = make()
+    print(f"[case] {name} fill blocks, levels {inputs}", flush=True)
+    S = gpu.MultiSegmentSystem(prob, groups)
+    S.assemble()
+    sp = split_problem(orc, prob, groups)
+    nb = sp["nb"]
+    blk = [1, nb // 3, nb // 2 + 1, nb - 2]
+    O = OrcMS(orc, sp, prob.conn)
+    kw = dict(inputs=inputs) if inputs is not None else dict(lev=0)
+    M = S.preconditioner(fill_blk=blk, **kw)
+    assert M.SetValue() and O.ilu(fill_blk=blk, **kw)
+    n = len(groups)
+    for i in range(n):
+        for j in range(n):
+            rp, cl = M.pattern(i, j)
+            orp, ocl = O.pattern(i, j)
+            assert np.array_equal(rp, orp) and np.array_equal(cl, ocl), (i, j)
+            if i == j:
+                fv, fd = M.values(i, i)
+                ofv, ofd = O.factor(i, i)
+                assert np.array_equal(fv, ofv) and np.array_equal(fd, ofd), (i, j)
+            else:
+                assert np.array_equal(M.values(i, j), O.factor(i, j)), (i, j)
+    x = np.random.default_rng(4).standard_normal(O.res.size)
+    xd = torch.from_numpy(x.copy()).cuda()
+    M.SolvePrecond(xd)
+    assert np.array_equal(xd.cpu().numpy(), O.precond(x))

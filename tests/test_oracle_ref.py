@@ -310,6 +310,35 @@ def test_multisegment_segment_levels_vs_reference(orc, ref, name, inputs):
         assert not np.array_equal(Ou.precond(x), z)
 
 
+@pytest.mark.parametrize("inputs", [None, [(1, 1)], [(2, -1)]], ids=["lev0", "lev1seg1", "lev2"])
+@pytest.mark.parametrize("name", sorted(MS_CASES))
+def test_multisegment_fill_blocks_vs_reference(orc, ref, name, inputs):
+    """CPreconditioner_ILU::SetFillBlk with several segments
+    (preconditioner.h:80, preconditioner.cpp:105, 143-148): the listed rows
+    of segment 0 take fill of any level in block (0, 0) (AddFracPtn(lev,
+    aBlkFill)), the other blocks follow their levels; SolvePrecond and the
+    whole solve bit for bit, and the factor differs from the one without
+    fill blocks."""
+    from oracle.oracle import OrcMS, RefMS, split_problem
+    make, groups, bicg = MS_CASES[name]
+    prob = make()
+    sp = split_problem(orc, prob, groups)
+    nb = sp["nb"]
+    blk = [1, nb // 3, nb // 2 + 1, nb - 2]
+    O, R = OrcMS(orc, sp, prob.conn), RefMS(ref, sp, prob.conn)
+    kw = dict(inputs=inputs) if inputs is not None else dict(lev=0)
+    assert R.ilu(fill_blk=blk, **kw) and O.ilu(fill_blk=blk, **kw)
+    x = np.random.default_rng(11).standard_normal(O.res.size)
+    z = R.precond(x)
+    assert np.array_equal(z, O.precond(x))
+    a, b = R.solve(bicg, O.res, 1e-10, 3000), O.solve(bicg, O.res, 1e-10, 3000)
+    assert a[:3] == b[:3] and a[0]
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+    O0 = OrcMS(orc, sp, prob.conn)
+    assert O0.ilu(**kw)
+    assert not np.array_equal(O0.precond(x), z)
+
+
 @pytest.mark.parametrize("b,nzero", [(1, 10), (1, 11), (1, 14), (3, 10), (3, 11)])
 @pytest.mark.parametrize("ordered", [False, True])
 def test_singular_pivot_stop_state(orc, ref, b, nzero, ordered):
