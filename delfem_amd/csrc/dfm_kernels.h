@@ -263,18 +263,34 @@ __device__ inline int xcd_block() {
   return x * q + (x < r ? x : r) + i;
 }
 
+// Holds three wave-uniform values that were loaded independently at one
+// point of the program, so that the compiler neither sinks a load below a
+// branch nor waits for each of them in turn: the loads leave together and
+// the wave waits for them once.
+__device__ __forceinline__ void issue_together(int a, int b, int c) {
+  asm volatile("" ::"s"(a), "s"(b), "s"(c));
+}
+// The x block of column c (B entries), zeros for a padding slot (c < 0).  The
+// loads are issued whatever c is (block 0 for padding, always in bounds), so
+// that the B words become one multi-word load and no branch splits the wave;
+// the zero is selected afterwards.
+template <int B, typename T>
+__device__ __forceinline__ void gather_x(const T* x, int c, T (&xv)[B]) {
+  const size_t cb = (size_t)(c >= 0 ? c : 0) * B;
+  T v[B];
+#pragma unroll
+  for (int e = 0; e < B; e++) v[e] = x[cb + e];
+#pragma unroll
+  for (int e = 0; e < B; e++) xv[e] = c >= 0 ? v[e] : (T)0;
+}
 // One block slot of a sliced-ELL row: values a[] (storage TS converted to TV),
-// gathered x block xv[].  Padding slots (column < 0) gather zeros without
-// touching memory.
+// gathered x block xv[] (zeros for a padding slot, column < 0).
 template <int B, typename TS, typename TV>
 __device__ inline void sell_slot(const int* __restrict__ col, const TS* __restrict__ val, const TV* x, long k,
                                  int lane, TV (&a)[B * B], TV (&xv)[B]) {
   const int c = ld_stream(col + (size_t)k * 64 + lane);
   load_block<B * B, TS, TV>(val, (size_t)k, lane, a);
-  const bool ok = c >= 0;
-  const size_t cb = (size_t)(ok ? c : 0) * B;
-#pragma unroll
-  for (int e = 0; e < B; e++) xv[e] = ok ? x[cb + e] : (TV)0;
+  gather_x<B, TV>(x, c, xv);
 }
 // ---------------------------------------------------------------------------
 // Block row kernels in the reference's summation order.
@@ -1019,11 +1035,18 @@ enum { SLOT_MSUB = 0, SLOT_PROD = 1, SLOT_MADD = 2 };
 // ovf[(ovf0 + k - NP) * B ..] for NP <= k < nreal (the lane's real slots).
 // The m column indices are fetched at once, then blocks and gathers in chunks
 // of CH slots, so a lane waits for two memory round trips only once.
-// FIRST: base == 0, so slot numbers are constants.
-template <int B, typename TS, typename T, int OP, int NP, bool FIRST>
+// FIRST: base == 0, so slot numbers are constants.  init(t) runs once, in
+// the FIRST call after the first chunk's loads have been issued and before
+// its first block product, so that a caller's own arithmetic for t does not
+// hold back the slot loads.
+struct NoInit {
+  template <typename A>
+  __device__ __forceinline__ void operator()(A&) const {}
+};
+template <int B, typename TS, typename T, int OP, int NP, bool FIRST, typename Init = NoInit>
 __device__ __forceinline__ void sell_slots16(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
                                              int k0, int base, int m, int lane, T (&t)[B], T (&prod)[NP][B],
-                                             T* ovf, int ovf0, int nreal) {
+                                             T* ovf, int ovf0, int nreal, const Init& init = Init()) {
   constexpr int BB = B * B;
   constexpr int E = slot_words<TS>(BB);
   constexpr int CH = E * 4 <= 40 ? 8 : 4;
@@ -1040,12 +1063,11 @@ __device__ __forceinline__ void sell_slots16(const int* __restrict__ col, const 
       for (int u = 0; u < CH; u++) {
         if (c0 + u < m) {
           load_words<BB, TS>(val, (size_t)(k0 + base + c0 + u), lane, w[u]);
-          const int c = cc[c0 + u];
-          const size_t cb = (size_t)(c >= 0 ? c : 0) * B;
-#pragma unroll
-          for (int e = 0; e < B; e++) xv[u][e] = c >= 0 ? x[cb + e] : (T)0;
+          gather_x<B, T>(x, cc[c0 + u], xv[u]);
         }
       }
+      if constexpr (FIRST)
+        if (c0 == 0) init(t);
 #pragma unroll
       for (int u = 0; u < CH; u++) {
         if (c0 + u < m) {
@@ -1071,13 +1093,18 @@ __device__ __forceinline__ void sell_slots16(const int* __restrict__ col, const 
     }
   }
 }
-// All n (wave-uniform) slots from k0, 16 at a time.
-template <int B, typename TS, typename T, int OP, int NP>
+// All n (wave-uniform) slots from k0, 16 at a time (init(t) as above; with
+// no slots it runs at once).
+template <int B, typename TS, typename T, int OP, int NP, typename Init = NoInit>
 __device__ __forceinline__ void sell_slots(const int* __restrict__ col, const TS* __restrict__ val, const T* x,
                                            int k0, int n, int lane, T (&t)[B], T (&prod)[NP][B], T* ovf,
-                                           int ovf0, int nreal) {
+                                           int ovf0, int nreal, const Init& init = Init()) {
   static_assert(NP <= 16, "register products must come from the first 16 slots");
-  if (n > 0) sell_slots16<B, TS, T, OP, NP, true>(col, val, x, k0, 0, n < 16 ? n : 16, lane, t, prod, ovf, ovf0, nreal);
+  if (n > 0)
+    sell_slots16<B, TS, T, OP, NP, true, Init>(col, val, x, k0, 0, n < 16 ? n : 16, lane, t, prod, ovf, ovf0, nreal,
+                                               init);
+  else
+    init(t);
   for (int base = 16; base < n; base += 16)
     sell_slots16<B, TS, T, OP, NP, false>(col, val, x, k0, base, n - base < 16 ? n - base : 16, lane, t, prod, ovf,
                                           ovf0, nreal);
@@ -1109,10 +1136,16 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
                                              const T* __restrict__ w0, const T* __restrict__ u2,
                                              const T* __restrict__ v2, double* __restrict__ partials,
                                              const KState* __restrict__ st) {
-  if (GATE && st->done) return;
   constexpr int BB = B * B;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int s = xcd_block() * kWaves + wave;
+  // the slice's slot range (scalar loads) is fetched in the same round trip
+  // as the stop flag
+  const int sl = s < nslice ? s : nslice - 1;
+  const int k0 = sptr[sl], k1 = sptr[sl + 1];
+  const int done = GATE ? st->done : 0;
+  issue_together(k0, k1, done);
+  if (done) return;
   double dsum[3] = {0.0, 0.0, 0.0};
   if (s < nslice) {
     const int row = s * 64 + lane;
@@ -1120,7 +1153,7 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
       T acc[B];
 #pragma unroll
       for (int r = 0; r < B; r++) acc[r] = (T)0;
-      sell_madd_range<B, TS, T>(col, val, x, sptr[s], sptr[s + 1], lane, acc);
+      sell_madd_range<B, TS, T>(col, val, x, k0, k1, lane, acc);
       {
         T a[BB], xv[B];
         load_block<BB, TS, T>(dval, (size_t)s, lane, a);
@@ -1146,13 +1179,32 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
 
 // t -= sum over slots k0 <= k < k1 (in that order) of block(k) * v[col(k)];
 // k0 and k1 are the same across the wave (one slice per wave).
-template <int B, typename TS, typename T>
+template <int B, typename TS, typename T, typename Init = NoInit>
 __device__ __forceinline__ void sell_msub_range(const int* __restrict__ col, const TS* __restrict__ val,
-                                                const T* v, int k0, int k1, int lane, T (&t)[B]) {
+                                                const T* v, int k0, int k1, int lane, T (&t)[B],
+                                                const Init& init = Init()) {
   T none[1][B];
   k0 = __builtin_amdgcn_readfirstlane(k0);
   k1 = __builtin_amdgcn_readfirstlane(k1);
-  sell_slots<B, TS, T, SLOT_MSUB, 1>(col, val, v, k0, k1 - k0, lane, t, none, nullptr, 0, 0);
+  sell_slots<B, TS, T, SLOT_MSUB, 1, Init>(col, val, v, k0, k1 - k0, lane, t, none, nullptr, 0, 0, init);
+}
+
+// What a one-slice wave of a level kernel reads before anything else: its
+// lane's row, the slice's slot range and the stop flag (false without a
+// state), fetched in one round trip rather than one after the other.
+struct SliceHead {
+  int row, k0, k1, done;
+};
+__device__ __forceinline__ SliceHead slice_head(int s, int lane, const int* __restrict__ rows,
+                                                const int* __restrict__ sptr, const KState* __restrict__ st) {
+  SliceHead h;
+  h.k0 = sptr[s];
+  h.k1 = sptr[s + 1];
+  h.done = st ? st->done : 0;
+  h.row = rows[(size_t)s * 64 + lane];
+  issue_together(h.k0, h.k1, h.done);
+  asm volatile("" ::"v"(h.row));
+  return h;
 }
 
 // ---------------------------------------------------------------------------
@@ -1162,18 +1214,17 @@ __device__ __forceinline__ void sell_msub_range(const int* __restrict__ col, con
 // finished here contribute <r,v> partials (slot part0 + workgroup) to a
 // reduction that spans all level launches of the sweep.
 // One 64-row slice of a lower sweep (lane = row of the slice).
+// row: the lane's row (-1 padding); [k0, k1): the slice's slots.
 template <int B, typename TS, typename T>
-__device__ __forceinline__ void trsv_lower_slice(int s, int lane, const int* __restrict__ rows,
-                                                 const int* __restrict__ sptr, const int* __restrict__ col,
-                                                 const TS* __restrict__ val, const TS* __restrict__ dval,
-                                                 const T* rhs, T* out) {
+__device__ __forceinline__ void trsv_lower_slice(int s, int lane, int row, int k0, int k1,
+                                                 const int* __restrict__ col, const TS* __restrict__ val,
+                                                 const TS* __restrict__ dval, const T* rhs, T* out) {
   constexpr int BB = B * B;
-  const int row = rows[(size_t)s * 64 + lane];
   if (row < 0) return;
   T t[B];
 #pragma unroll
   for (int r = 0; r < B; r++) t[r] = rhs[(size_t)row * B + r];
-  sell_msub_range<B, TS, T>(col, val, out, sptr[s], sptr[s + 1], lane, t);
+  sell_msub_range<B, TS, T>(col, val, out, k0, k1, lane, t);
   T d[BB], y[B];
   load_block<BB, TS, T>(dval, (size_t)s, lane, d);
   blk_mul<B, T>(y, d, t);
@@ -1183,16 +1234,14 @@ __device__ __forceinline__ void trsv_lower_slice(int s, int lane, const int* __r
 
 // One slice of an upper sweep; with DOT adds <rvec, v> of its rows to dsum.
 template <int B, typename TS, typename T, bool DOT>
-__device__ __forceinline__ void trsv_upper_slice(int s, int lane, const int* __restrict__ rows,
-                                                 const int* __restrict__ sptr, const int* __restrict__ col,
+__device__ __forceinline__ void trsv_upper_slice(int lane, int row, int k0, int k1, const int* __restrict__ col,
                                                  const TS* __restrict__ val, T* vec, const T* __restrict__ rvec,
                                                  double& dsum) {
-  const int row = rows[(size_t)s * 64 + lane];
   if (row < 0) return;
   T t[B];
 #pragma unroll
   for (int r = 0; r < B; r++) t[r] = vec[(size_t)row * B + r];
-  sell_msub_range<B, TS, T>(col, val, vec, sptr[s], sptr[s + 1], lane, t);
+  sell_msub_range<B, TS, T>(col, val, vec, k0, k1, lane, t);
 #pragma unroll
   for (int r = 0; r < B; r++) {
     vec[(size_t)row * B + r] = t[r];
@@ -1205,11 +1254,12 @@ __global__ void __launch_bounds__(kTrsvNT) k_trsv_lower(int s0, int s1, const in
                                                    const int* __restrict__ sptr, const int* __restrict__ col,
                                                    const TS* __restrict__ val, const TS* __restrict__ dval,
                                                    const T* rhs, T* out, const KState* __restrict__ st) {
-  if (st && st->done) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int s = s0 + xcd_block() * kTrsvWaves + wave;
-  if (s >= s1) return;
-  trsv_lower_slice<B, TS, T>(s, lane, rows, sptr, col, val, dval, rhs, out);
+  const int sl = s < s1 ? s : s1 - 1;
+  const SliceHead h = slice_head(sl, lane, rows, sptr, st);
+  if (h.done || s >= s1) return;
+  trsv_lower_slice<B, TS, T>(s, lane, h.row, h.k0, h.k1, col, val, dval, rhs, out);
 }
 
 template <int B, typename TS, typename T, bool DOT>
@@ -1218,11 +1268,13 @@ __global__ void __launch_bounds__(kTrsvNT) k_trsv_upper(int s0, int s1, const in
                                                    const TS* __restrict__ val, T* vec, const T* __restrict__ rvec,
                                                    double* __restrict__ partials, int part0, int nparts,
                                                    const KState* __restrict__ st) {
-  if (st && st->done) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int s = s0 + xcd_block() * kTrsvWaves + wave;
+  const int sl = s < s1 ? s : s1 - 1;
+  const SliceHead h = slice_head(sl, lane, rows, sptr, st);
+  if (h.done) return;
   double dsum[1] = {0.0};
-  if (s < s1) trsv_upper_slice<B, TS, T, DOT>(s, lane, rows, sptr, col, val, vec, rvec, dsum[0]);
+  if (s < s1) trsv_upper_slice<B, TS, T, DOT>(lane, h.row, h.k0, h.k1, col, val, vec, rvec, dsum[0]);
   if constexpr (DOT) write_partials<1, kTrsvNT>(dsum, partials, part0 + blockIdx.x, nparts);
 }
 
@@ -1323,9 +1375,7 @@ __global__ void __launch_bounds__(64) k_trsv_persist(int nslice, int ngroup, con
 #pragma unroll
         for (int u = 0; u < PC; u++) {
           if (u < nk) {
-            const size_t cb = (size_t)(c[u] >= 0 ? c[u] : 0) * B;
-#pragma unroll
-            for (int e = 0; e < B; e++) xv[u][e] = c[u] >= 0 ? out[cb + e] : (T)0;
+            gather_x<B, T>(out, c[u], xv[u]);
             if (u >= PB) load_words<BB, TS>(val, (size_t)(k0 + u), lane, w2[u - PB]);
           }
         }
@@ -1582,10 +1632,7 @@ __device__ __forceinline__ void row_slots(const int* __restrict__ col, const TS*
         for (int u = 0; u < CH; u++) {
           if (c0 + u < m) {
             load_words<B, TS>(val, (size_t)(k0 + base + c0 + u), lane, w[u]);
-            const int c = cc[c0 + u];
-            const size_t cb = (size_t)(c >= 0 ? c : 0) * B;
-#pragma unroll
-            for (int e = 0; e < B; e++) xv[u][e] = c >= 0 ? x[cb + e] : (T)0;
+            gather_x<B, T>(x, cc[c0 + u], xv[u]);
           }
         }
 #pragma unroll
@@ -1766,33 +1813,51 @@ __global__ void __launch_bounds__(kTrsvNT) k_trsv_lower_xr(int s0, int s1, const
                                                       const unsigned char* __restrict__ rzmask,
                                                       double* __restrict__ prr, int part0,
                                                       double* __restrict__ prz, const KState* __restrict__ st) {
-  if (st->done) return;
   constexpr int BB = B * B;
-  const T a = (T)st->alpha, ma = (T)(-st->alpha);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int s = s0 + xcd_block() * kTrsvWaves + wave;
+  const int sl = s < s1 ? s : s1 - 1;
+  const SliceHead h = slice_head(sl, lane, rows, sptr, st);
+  if (h.done) return;
+  const T a = (T)st->alpha, ma = (T)(-st->alpha);
   double drr[1] = {0.0}, drz[1] = {0.0};
-  const int row = s < s1 ? rows[(size_t)s * 64 + lane] : -1;
+  const int row = s < s1 ? h.row : -1;
   if (row >= 0) {
     const bool rz = RZ && rzmask[(size_t)s * 64 + lane];
-    T t[B], rin[B];
+    // the row's r, z (= Ap), x, p are loaded here and combined only after the
+    // slot loads have left (init), and r, x stored at the end
+    T rr[B], zz[B], xx[B], pp[B];
 #pragma unroll
     for (int e = 0; e < B; e++) {
       const size_t o = (size_t)row * B + e;
-      const T rv = r[o] + ma * z[o];
-      r[o] = rv;
-      x[o] = x[o] + a * p[o];
-      drr[0] += (double)rv * (double)rv;
-      t[e] = rv;
-      rin[e] = rv;
+      rr[e] = r[o];
+      zz[e] = z[o];
+      xx[e] = x[o];
+      pp[e] = p[o];
     }
-    sell_msub_range<B, TS, T>(col, val, z, sptr[s], sptr[s + 1], lane, t);
+    T t[B], rin[B];
+    auto init = [&](T(&tt)[B]) {
+#pragma unroll
+      for (int e = 0; e < B; e++) {
+        // pins the arithmetic here: without it the compiler moves it ahead
+        // of the slot loads, which then wait for r and z
+        asm volatile("" : "+v"(rr[e]), "+v"(zz[e]));
+        const T rv = rr[e] + ma * zz[e];
+        tt[e] = rv;
+        rin[e] = rv;
+      }
+    };
+    sell_msub_range<B, TS, T>(col, val, z, h.k0, h.k1, lane, t, init);
     T d[BB], y[B];
     load_block<BB, TS, T>(dval, (size_t)s, lane, d);
     blk_mul<B, T>(y, d, t);
 #pragma unroll
     for (int e = 0; e < B; e++) {
-      z[(size_t)row * B + e] = y[e];
+      const size_t o = (size_t)row * B + e;
+      r[o] = rin[e];
+      x[o] = xx[e] + a * pp[e];
+      drr[0] += (double)rin[e] * (double)rin[e];
+      z[o] = y[e];
       if (rz) drz[0] += (double)rin[e] * (double)y[e];
     }
   }
