@@ -18,6 +18,8 @@ namespace dfm {
 
 constexpr int kNT = 256;             // threads per workgroup
 constexpr int kWaves = kNT / 64;     // wavefronts (= slices) per workgroup
+constexpr int kSpmvNT = 256;         // threads per workgroup of the solver SpMV
+constexpr int kSpmvWaves = kSpmvNT / 64;
 // Triangular sweeps launch one small workgroup per slice: a colour of a
 // multicolour ordering holds only a few hundred slices, and one-wave
 // workgroups spread them over all CUs.
@@ -1049,7 +1051,7 @@ __device__ __forceinline__ void sell_slots16(const int* __restrict__ col, const 
                                              T* ovf, int ovf0, int nreal, const Init& init = Init()) {
   constexpr int BB = B * B;
   constexpr int E = slot_words<TS>(BB);
-  constexpr int CH = E * 4 <= 40 ? 8 : 4;
+  constexpr int CH = E * 4 <= 20 ? 8 : 4;
   int cc[16];
 #pragma unroll
   for (int u = 0; u < 16; u++)
@@ -1129,7 +1131,7 @@ __device__ __forceinline__ void sell_madd_range(const int* __restrict__ col, con
 // BiCGSTAB omega = <s,AMs>/<AMs,AMs> uses w0 = s.  GATE: skip once the solver
 // has stopped (Krylov-internal products); the true-residual form runs always.
 template <int B, typename TS, typename T, bool DOTS, bool GATE>
-__global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __restrict__ sptr,
+__global__ void __launch_bounds__(kSpmvNT) k_spmv(int nslice, int nb, const int* __restrict__ sptr,
                                              const int* __restrict__ col, const TS* __restrict__ val,
                                              const TS* __restrict__ dval, const T* __restrict__ x,
                                              T* __restrict__ y, const T* __restrict__ res,
@@ -1138,7 +1140,7 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
                                              const KState* __restrict__ st) {
   constexpr int BB = B * B;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int s = xcd_block() * kWaves + wave;
+  const int s = xcd_block() * kSpmvWaves + wave;
   // the slice's slot range (scalar loads) is fetched in the same round trip
   // as the stop flag
   const int sl = s < nslice ? s : nslice - 1;
@@ -1174,7 +1176,7 @@ __global__ void __launch_bounds__(kNT) k_spmv(int nslice, int nb, const int* __r
       }
     }
   }
-  if constexpr (DOTS) write_partials<3>(dsum, partials, blockIdx.x, gridDim.x);
+  if constexpr (DOTS) write_partials<3, kSpmvNT>(dsum, partials, blockIdx.x, gridDim.x);
 }
 
 // t -= sum over slots k0 <= k < k1 (in that order) of block(k) * v[col(k)];

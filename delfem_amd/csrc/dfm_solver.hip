@@ -382,14 +382,15 @@ static int spmv(const Ops<TA, TF>& o, const T* x, T* y, const T* res, const T* w
                 double* partials, const KState* st) {
   const int nslice = o.planA->nslice;
   if constexpr (DOTS && GATE) {
-    if (g_ktimer.on && g_ktimer.used < kTimerMaxLaunches && grid_for(nslice, kWaves) > 0) {
+    if (g_ktimer.on && g_ktimer.used < kTimerMaxLaunches && grid_for(nslice, kSpmvWaves) > 0) {
       while (g_ktimer.ev.size() < 2 * (g_ktimer.used + 1)) {
         hipEvent_t e = nullptr;
         DFM_HIP(hipEventCreate(&e));
         g_ktimer.ev.push_back(e);
       }
       hipEvent_t* e = &g_ktimer.ev[2 * g_ktimer.used];
-      hipExtLaunchKernelGGL((k_spmv<B, TA, T, DOTS, GATE>), dim3(grid_for(nslice, kWaves)), dim3(kNT), 0, stream(),
+      hipExtLaunchKernelGGL((k_spmv<B, TA, T, DOTS, GATE>), dim3(grid_for(nslice, kSpmvWaves)), dim3(kSpmvNT), 0,
+                            stream(),
                             e[0], e[1], 0, nslice, (int)o.nb, (const int*)o.planA->slice_ptr.p,
                             (const int*)o.planA->col.p, (const TA*)o.A->val.p, (const TA*)o.A->dval.p, x, y, res, w0,
                             u2, v2, partials, st);
@@ -398,11 +399,11 @@ static int spmv(const Ops<TA, TF>& o, const T* x, T* y, const T* res, const T* w
       return DFM_OK;
     }
   }
-  DFM_LAUNCH((k_spmv<B, TA, T, DOTS, GATE>), grid_for(nslice, kWaves), nslice, o.nb, o.planA->slice_ptr.p,
-             o.planA->col.p, o.A->val.p, o.A->dval.p, x, y, res, w0, u2, v2, partials, st);
+  DFM_LAUNCH_NT((k_spmv<B, TA, T, DOTS, GATE>), grid_for(nslice, kSpmvWaves), kSpmvNT, nslice, o.nb,
+                o.planA->slice_ptr.p, o.planA->col.p, o.A->val.p, o.A->dval.p, x, y, res, w0, u2, v2, partials, st);
   return DFM_OK;
 }
-static int spmv_nparts(const Sell<double>& planA) { return grid_for(planA.nslice, kWaves); }
+static int spmv_nparts(const Sell<double>& planA) { return grid_for(planA.nslice, kSpmvWaves); }
 
 static int read_state(dfm_ilu_s* M) {
   DFM_HIP(hipMemcpyAsync(M->h_state, M->state.p, sizeof(KState), hipMemcpyDeviceToHost, stream()));
