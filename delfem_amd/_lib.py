@@ -186,6 +186,28 @@ _SIGS = {
     "dfm_zvec_inner": (C.c_int, [C.c_size_t, vp, vp, vp]),
     "dfm_zsolve_bicgstab": (C.c_int, [H, vp, vp, f64p, u32p]),
     "dfm_zsolve_bicgstabp": (C.c_int, [H, H, vp, vp, f64p, u32p]),
+    "dfm_fbcrs_create": (C.c_int, [C.c_uint, vp, C.POINTER(H)]),
+    "dfm_fbcrs_destroy": (C.c_int, [H]),
+    "dfm_fbcrs_add_pattern": (C.c_int, [H, C.c_uint, vp, vp]),
+    "dfm_fbcrs_info": (C.c_int, [H, u32p, u32p, u32p, u32p, u32p]),
+    "dfm_fbcrs_layout": (C.c_int, [H, vp, vp, vp, vp, vp]),
+    "dfm_fbcrs_set_zero": (C.c_int, [H]),
+    "dfm_fbcrs_set_values": (C.c_int, [H, vp, vp]),
+    "dfm_fbcrs_get_values": (C.c_int, [H, vp, vp]),
+    "dfm_fbcrs_device_values": (C.c_int, [H, C.POINTER(vp), C.POINTER(vp)]),
+    "dfm_fbcrs_mearge": (C.c_int, [H, C.c_uint, vp, vp, vp]),
+    "dfm_fbcrs_set_bc": (C.c_int, [H, vp]),
+    "dfm_fbcrs_matvec": (C.c_int, [H, C.c_double, vp, C.c_double, vp]),
+    "dfm_filu_create": (C.c_int, [H, C.c_int, C.POINTER(H)]),
+    "dfm_filu_destroy": (C.c_int, [H]),
+    "dfm_filu_info": (C.c_int, [H, u32p, u32p, i32p, i32p]),
+    "dfm_filu_get_pattern": (C.c_int, [H, vp, vp, vp, vp]),
+    "dfm_filu_set_value": (C.c_int, [H, H, i32p]),
+    "dfm_filu_get_values": (C.c_int, [H, vp, vp]),
+    "dfm_filu_forward": (C.c_int, [H, vp]),
+    "dfm_filu_backward": (C.c_int, [H, vp]),
+    "dfm_filu_solve_precond": (C.c_int, [H, vp]),
+    "dfm_fsolve": (C.c_int, [C.c_int, H, H, vp, vp, f64p, u32p]),
     "dfm_bench_kernel": (C.c_int, [H, H, C.c_int, C.c_int, C.c_int, f64p, f64p]),
     "dfm_bench_copy": (C.c_int, [C.c_size_t, C.c_int, f64p]),
     "dfm_bench_read": (C.c_int, [C.c_size_t, C.c_int, f64p]),

@@ -9,7 +9,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdelfem_amd.so")
 
-SOURCES = ["dfm_host.cpp", "dfm_api.hip", "dfm_solver.hip", "dfm_batch.hip", "dfm_complex.hip", "dfm_bmat.hip", "dfm_mseg.hip"]
+SOURCES = ["dfm_host.cpp", "dfm_api.hip", "dfm_solver.hip", "dfm_batch.hip", "dfm_complex.hip", "dfm_bmat.hip", "dfm_mseg.hip", "dfm_flex.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1130,3 +1130,185 @@ class ZHelmholtzSystem:
         _sync_torch()
         call("dfm_zvec_zero_bc", self.res.numel(), _ptr(self.flags), _dptr(self.res))
         return znorm2(self.res) ** 0.5
+
+
+class CMatDiaFlex_BlkCrs:
+    """MatVec::CMatDia_BlkCrs made with per-block lengths aLen
+    (src/matvec/matdia_blkcrs.cpp:70-104): block i has alen[i] dofs.  Values
+    are kept in the reference's layout (dofptr, valptr, diaptr)."""
+
+    def __init__(self, nblk, alen):
+        alen = np.ascontiguousarray(alen, dtype=np.uint32)
+        assert alen.size == nblk
+        h = C.c_void_p()
+        call("dfm_fbcrs_create", nblk, _ptr(alen), C.byref(h))
+        self.h, self.nblk, self.alen = h, nblk, alen
+        self.dofptr = np.zeros(nblk + 1, dtype=np.uint32)
+        self.dofptr[1:] = np.cumsum(alen)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_fbcrs_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def NBlkMatCol(self):
+        return self.nblk
+
+    def LenBlkCol(self, iblk=None):
+        """-1 (flex), or the length of block iblk (mat_blkcrs.h:82-92)."""
+        return -1 if iblk is None else int(self.alen[iblk])
+
+    def ndof(self):
+        return int(self.dofptr[-1])
+
+    def info(self):
+        v = [C.c_uint() for _ in range(5)]
+        call("dfm_fbcrs_info", self.h, *(C.byref(x) for x in v))
+        return dict(nblk=v[0].value, ndof=v[1].value, ncrs=v[2].value, nval=v[3].value, ndia=v[4].value)
+
+    def NCrs(self):
+        return self.info()["ncrs"]
+
+    def AddPattern(self, index, array):
+        index = np.ascontiguousarray(index, dtype=np.uint32)
+        array = np.ascontiguousarray(array, dtype=np.uint32)
+        if array.size == 0:
+            array = np.zeros(1, dtype=np.uint32)
+        call("dfm_fbcrs_add_pattern", self.h, index.size - 1, _ptr(index), _ptr(array))
+        return True
+
+    def layout(self):
+        """dict(dofptr, rowptr, col, valptr, diaptr)."""
+        i = self.info()
+        out = dict(dofptr=np.zeros(self.nblk + 1, dtype=np.uint32), rowptr=np.zeros(self.nblk + 1, dtype=np.uint32),
+                   col=np.zeros(max(1, i["ncrs"]), dtype=np.uint32), valptr=np.zeros(i["ncrs"] + 1, dtype=np.uint32),
+                   diaptr=np.zeros(self.nblk + 1, dtype=np.uint32))
+        call("dfm_fbcrs_layout", self.h, *(_ptr(out[k]) for k in ("dofptr", "rowptr", "col", "valptr", "diaptr")))
+        out["col"] = out["col"][:i["ncrs"]]
+        return out
+
+    def SetZero(self):
+        call("dfm_fbcrs_set_zero", self.h)
+        return True
+
+    def values(self):
+        i = self.info()
+        val, dia = np.zeros(max(1, i["nval"])), np.zeros(max(1, i["ndia"]))
+        call("dfm_fbcrs_get_values", self.h, _ptr(val), _ptr(dia))
+        return val[:i["nval"]], dia[:i["ndia"]]
+
+    def set_values(self, val, dia):
+        val = np.ascontiguousarray(val, dtype=np.float64).reshape(-1)
+        dia = np.ascontiguousarray(dia, dtype=np.float64).reshape(-1)
+        call("dfm_fbcrs_set_values", self.h, _ptr(val) if val.size else None, _ptr(dia) if dia.size else None)
+
+    def Mearge(self, blocks):
+        """Mearge of single blocks [(iblk, jblk, len_i x len_j values)], in
+        order (the flex branch takes one block per call, matdia_blkcrs.cpp:173-207)."""
+        if not blocks:
+            return True
+        ib = np.ascontiguousarray([b[0] for b in blocks], dtype=np.uint32)
+        jb = np.ascontiguousarray([b[1] for b in blocks], dtype=np.uint32)
+        em = np.ascontiguousarray(np.concatenate([np.ravel(b[2]) for b in blocks]), dtype=np.float64)
+        call("dfm_fbcrs_mearge", self.h, ib.size, _ptr(ib), _ptr(jb), _ptr(em))
+        return True
+
+    def SetBoundaryCondition(self, flags):
+        flags = np.ascontiguousarray(flags, dtype=np.int32).reshape(-1)
+        assert flags.size == self.ndof()
+        call("dfm_fbcrs_set_bc", self.h, _ptr(flags))
+        return True
+
+    def MatVec(self, alpha, x, beta, y):
+        _sync_torch()
+        call("dfm_fbcrs_matvec", self.h, float(alpha), _dptr(x), float(beta), _dptr(y))
+        return True
+
+
+class CMatDiaFracFlex_BlkCrs:
+    """MatVec::CMatDiaFrac_BlkCrs(lev, A) over a flex matrix
+    (src/matvec/matdiafrac_blkcrs.cpp:227-287, numeric :777-846, substitutions
+    :339-379 and :512-553), natural order; also the LsSol CPreconditioner_ILU
+    of a single flex segment."""
+
+    def __init__(self, lev, A):
+        h = C.c_void_p()
+        call("dfm_filu_create", A.h, int(lev), C.byref(h))
+        self.h, self.A, self.nblk = h, A, A.nblk
+        self.nsingular = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            try:
+                _lib.lib().dfm_filu_destroy(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    def info(self):
+        nc, nv, nl, nu = C.c_uint(), C.c_uint(), C.c_int(), C.c_int()
+        call("dfm_filu_info", self.h, C.byref(nc), C.byref(nv), C.byref(nl), C.byref(nu))
+        return dict(ncrs=nc.value, nval=nv.value, nlevel_lower=nl.value, nlevel_upper=nu.value)
+
+    def NCrs(self):
+        return self.info()["ncrs"]
+
+    def pattern(self):
+        i = self.info()
+        out = dict(rowptr=np.zeros(self.nblk + 1, dtype=np.uint32), col=np.zeros(max(1, i["ncrs"]), dtype=np.uint32),
+                   diaind=np.zeros(max(1, self.nblk), dtype=np.uint32),
+                   valptr=np.zeros(i["ncrs"] + 1, dtype=np.uint32))
+        call("dfm_filu_get_pattern", self.h, *(_ptr(out[k]) for k in ("rowptr", "col", "diaind", "valptr")))
+        out["col"] = out["col"][:i["ncrs"]]
+        out["diaind"] = out["diaind"][:self.nblk]
+        return out
+
+    def SetValue(self, A):
+        ns = C.c_int()
+        rc = call("dfm_filu_set_value", self.h, A.h, C.byref(ns), ok=(_lib.DFM_OK, _lib.DFM_SINGULAR))
+        self.nsingular = ns.value
+        return rc == _lib.DFM_OK
+
+    def values(self):
+        nv = self.info()["nval"]
+        nd = self.A.info()["ndia"]
+        val, dia = np.zeros(max(1, nv)), np.zeros(max(1, nd))
+        call("dfm_filu_get_values", self.h, _ptr(val), _ptr(dia))
+        return val[:nv], dia[:nd]
+
+    def ForwardSubstitution(self, vec):
+        _sync_torch()
+        call("dfm_filu_forward", self.h, _dptr(vec))
+        return True
+
+    def BackwardSubstitution(self, vec):
+        _sync_torch()
+        call("dfm_filu_backward", self.h, _dptr(vec))
+        return True
+
+    def SolvePrecond(self, vec):
+        _sync_torch()
+        call("dfm_filu_solve_precond", self.h, _dptr(vec))
+        return True
+
+
+_FX_KIND = {"pcg": 0, "pbicgstab": 1, "cg": 2, "bicgstab": 3}
+
+
+def solve_flex(kind, conv_ratio, iteration, A, prec, r, x):
+    """The LsSol solvers on one flex segment (src/ls/solver_ls_iter.cpp):
+    kind "pcg" / "pbicgstab" with prec (a CMatDiaFracFlex_BlkCrs holding
+    values), "cg" / "bicgstab" without (prec None).  r in: right-hand side,
+    out: final residual; x out: solution.  Returns (ok, conv_ratio,
+    iteration) with ok the reference's result (Solve_PCG false at max_iter,
+    the others true)."""
+    _sync_torch()
+    conv = C.c_double(conv_ratio)
+    it = C.c_uint(iteration)
+    rc = call("dfm_fsolve", _FX_KIND[kind], A.h, prec.h if prec is not None else None, _dptr(r), _dptr(x),
+              C.byref(conv), C.byref(it), ok=(_lib.DFM_OK, _lib.DFM_NOT_CONVERGED))
+    ok = rc == _lib.DFM_OK or kind != "pcg"
+    return ok, conv.value, it.value

@@ -466,6 +466,54 @@ int dfm_zsolve_bicgstab(dfm_zbcrs *A, double *r_dev, double *u_dev, double *conv
 int dfm_zsolve_bicgstabp(dfm_zbcrs *A, dfm_zilu *M, double *r_dev, double *u_dev,
                          double *conv_ratio, unsigned *iteration);
 
+/* ---- variable block sizes ("flex") ---------------------------------------
+ * MatVec::CMatDia_BlkCrs / CMatDiaFrac_BlkCrs made with per-block lengths aLen
+ * (blocks of 1..32 dofs; the reference's rigid bodies and constraints) and the
+ * LsSol solvers over one such segment (CLinearSystem::AddLinSysSeg(nnode, aLen),
+ * linearsystem.cpp:170-201).  Layout as the reference's: dofptr (m_DofPtrCol),
+ * the block pattern in CRS without the diagonal, valptr (m_ValPtr, len_i x len_j
+ * row-major blocks in CRS order), diaptr (m_DiaValPtr).  Vectors are flat device
+ * arrays of dofptr[nblk] values.  FP64, bitwise the reference's flex branches. */
+typedef struct dfm_fbcrs_s dfm_fbcrs;   /* CMatDia_BlkCrs with per-block sizes */
+typedef struct dfm_filu_s dfm_filu;     /* CMatDiaFrac_BlkCrs over a flex matrix */
+int dfm_fbcrs_create(unsigned nblk, const unsigned *alen, dfm_fbcrs **out); /* CMatDia_BlkCrs(nblk,aLen,nblk,aLen) matdia_blkcrs.cpp:70-104 */
+int dfm_fbcrs_destroy(dfm_fbcrs *A);
+int dfm_fbcrs_add_pattern(dfm_fbcrs *A, unsigned nrow, const unsigned *index,
+                          const unsigned *array);                         /* AddPattern(CIndexedArray) matdia_blkcrs.cpp:346-505; values zeroed */
+int dfm_fbcrs_info(const dfm_fbcrs *A, unsigned *nblk, unsigned *ndof, unsigned *ncrs,
+                   unsigned *nval, unsigned *ndia);                       /* NBlkMatCol / sizes (mat_blkcrs.h:83-157) */
+int dfm_fbcrs_layout(const dfm_fbcrs *A, unsigned *dofptr, unsigned *rowptr, unsigned *col,
+                     unsigned *valptr, unsigned *diaptr);                 /* m_DofPtrCol, pattern, m_ValPtr (mat_blkcrs.cpp:236-249), m_DiaValPtr */
+int dfm_fbcrs_set_zero(dfm_fbcrs *A);                                      /* SetZero mat_blkcrs.cpp:225-258, matdia_blkcrs.cpp:139-163 */
+int dfm_fbcrs_set_values(dfm_fbcrs *A, const double *val, const double *dia);
+int dfm_fbcrs_get_values(const dfm_fbcrs *A, double *val, double *dia);
+int dfm_fbcrs_device_values(dfm_fbcrs *A, double **val_dev, double **dia_dev);
+int dfm_fbcrs_mearge(dfm_fbcrs *A, unsigned n, const unsigned *iblk, const unsigned *jblk,
+                     const double *emat);                                 /* n calls of Mearge(1,&i,1,&j,...) matdia_blkcrs.cpp:173-207, in order */
+int dfm_fbcrs_set_bc(dfm_fbcrs *A, const int *flag);                       /* SetBoundaryCondition, flex branch matdia_blkcrs.cpp:268-306 */
+int dfm_fbcrs_matvec(dfm_fbcrs *A, double alpha, const double *x_dev, double beta,
+                     double *y_dev);                                      /* MatVec, flex branch matdia_blkcrs.cpp:931-960 */
+int dfm_filu_create(const dfm_fbcrs *A, int lev_fill, dfm_filu **out);     /* CMatDiaFrac_BlkCrs(lev,A) matdiafrac_blkcrs.cpp:227-287 */
+int dfm_filu_destroy(dfm_filu *M);
+int dfm_filu_info(const dfm_filu *M, unsigned *ncrs, unsigned *nval, int *nlevel_lower, int *nlevel_upper);
+int dfm_filu_get_pattern(const dfm_filu *M, unsigned *rowptr, unsigned *col, unsigned *diaind,
+                         unsigned *valptr);
+/* SetValue_Initialize + DoILUDecomp (flex branch matdiafrac_blkcrs.cpp:777-846):
+ * DFM_SINGULAR after more than 10 singular pivots, the factor then left as the
+ * reference leaves it (stopped in the row of the 11th). */
+int dfm_filu_set_value(dfm_filu *M, dfm_fbcrs *A, int *nsingular);
+int dfm_filu_get_values(const dfm_filu *M, double *val, double *dia);
+int dfm_filu_forward(dfm_filu *M, double *vec_dev);                        /* ForwardSubstitution matdiafrac_blkcrs.cpp:339-379 */
+int dfm_filu_backward(dfm_filu *M, double *vec_dev);                       /* BackwardSubstitution matdiafrac_blkcrs.cpp:512-553 */
+int dfm_filu_solve_precond(dfm_filu *M, double *vec_dev);                  /* SolvePrecond preconditioner.cpp:301-335 */
+/* LsSol solvers over one flex segment: kind 0 Solve_PCG (solver_ls_iter.cpp:100-172),
+ * 1 Solve_PBiCGSTAB (:291-408) with the factor M, 2 Solve_CG (:31-97), 3 Solve_BiCGSTAB
+ * (:177-286) without (M may be NULL).  r in: right-hand side, out: final residual; x out.
+ * DFM_OK when converged, DFM_NOT_CONVERGED at max_iter (the reference's Solve_PCG returns
+ * false there, the other three true). */
+int dfm_fsolve(int kind, dfm_fbcrs *A, dfm_filu *M, double *r_dev, double *x_dev,
+               double *conv_ratio, unsigned *iteration);
+
 /* ---- measurement (no reference counterpart) -------------------------------
  * Times one device kernel of a solve on its real operands: which = 0 the
  * sliced-ELL SpMV of A, 1 the forward sweep (all lower level sets), 2 the

@@ -666,10 +666,13 @@ typedef struct {
     const unsigned *frowptr, *fcol, *fdiaind; const double *fval, *fdia;
     const int *order; int *inv; double *pv;
     struct orc_ms_s *ms;   /* several segments (then the fields above are unused) */
+    struct orc_fx_s *fx;   /* one segment of variable block sizes (likewise) */
 } orc_sys;
 static void ms_matvec(struct orc_ms_s *m, double alpha, const double *x, double beta, double *y);
 static void ms_precond(struct orc_ms_s *m, double *z);
 static double ms_dot(struct orc_ms_s *m, const double *a, const double *c);
+static void fx_sys_matvec(struct orc_fx_s *f, double alpha, const double *x, double beta, double *y);
+static void fx_sys_precond(struct orc_fx_s *f, double *z);
 
 static double v_dot(size_t n, const double *a, const double *c)
 {
@@ -691,6 +694,7 @@ static void v_scal(size_t n, double alpha, double *x)
 static void precond(orc_sys *s, double *z)
 {
     if (s->ms) { ms_precond(s->ms, z); return; }
+    if (s->fx) { fx_sys_precond(s->fx, z); return; }
     const unsigned b = s->b;
     if (s->order) {
         for (unsigned i = 0; i < s->nb; i++)
@@ -711,7 +715,7 @@ static void sys_init(orc_sys *s, unsigned nb, unsigned b,
 {
     s->nb = nb; s->b = b; s->rowptr = rowptr; s->col = col; s->val = val; s->dia = dia;
     s->frowptr = frowptr; s->fcol = fcol; s->fdiaind = fdiaind; s->fval = fval; s->fdia = fdia;
-    s->order = order; s->inv = NULL; s->pv = NULL; s->ms = NULL;
+    s->order = order; s->inv = NULL; s->pv = NULL; s->ms = NULL; s->fx = NULL;
     if (order) {
         s->inv = (int *)malloc(sizeof(int) * (size_t)nb);
         for (unsigned i = 0; i < nb; i++) s->inv[order[i]] = (int)i;
@@ -722,6 +726,7 @@ static void sys_free(orc_sys *s) { free(s->inv); free(s->pv); }
 static void matvec_ls(orc_sys *s, double alpha, const double *x, double beta, double *y)
 {
     if (s->ms) { ms_matvec(s->ms, alpha, x, beta, y); return; }
+    if (s->fx) { fx_sys_matvec(s->fx, alpha, x, beta, y); return; }
     if (alpha == 0.0) { v_scal((size_t)s->nb * s->b, beta, y); return; }   /* :270-273 */
     orc_matvec(s->nb, s->b, s->rowptr, s->col, s->val, s->dia, alpha, x, beta, y);
 }
@@ -3022,5 +3027,306 @@ void orc_zemat_sommerfeld_b1(unsigned nline, const double *coord, const unsigned
             er[i].im = 0.0;
             for (int j = 0; j < 2; j++) er[i] = zsub(er[i], zmul(em[i * 2 + j], val[no[j]]));
         }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Variable block sizes ("flex"): CMatDia_BlkCrs / CMatDiaFrac_BlkCrs built
+ * with per-block lengths aLen (src/matvec/mat_blkcrs.cpp:63-92,
+ * src/matvec/matdia_blkcrs.cpp:70-104).  Layout, as the reference's:
+ * dofptr[i] (m_DofPtrCol) is the first dof of block i; the CRS pattern is
+ * the block pattern; valptr[q] (m_ValPtr, mat_blkcrs.cpp:236-249) is the
+ * offset of block q = (i, j), len_i x len_j row-major, offsets in CRS order;
+ * diaptr[i] (m_DiaValPtr, matdia_blkcrs.cpp:88-101) the offset of diagonal
+ * block i, len_i x len_i.  Vectors are flat (dofptr[nb] values). */
+#define FXLEN(i) (dofptr[(i) + 1] - dofptr[(i)])
+
+void orc_fx_layout(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                   unsigned *valptr, unsigned *diaptr)
+{
+    unsigned v = 0, d = 0;
+    for (unsigned i = 0; i < nb; i++) {
+        const unsigned li = FXLEN(i);
+        for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+            valptr[q] = v;
+            v += li * FXLEN(col[q]);
+        }
+        diaptr[i] = d;
+        d += li * li;
+    }
+    valptr[rowptr[nb]] = v;
+    diaptr[nb] = d;
+}
+
+/* CMatDia_BlkCrs::MatVec, flex branch (src/matvec/matdia_blkcrs.cpp:931-960):
+ * y_i = beta y_i, then alpha * a * x added term by term, CRS blocks in order
+ * and the diagonal block last. */
+void orc_fx_matvec(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                   const unsigned *valptr, const double *val, const unsigned *diaptr, const double *dia,
+                   double alpha, const double *x, double beta, double *y)
+{
+    for (unsigned i = 0; i < nb; i++) {
+        double *yi = y + dofptr[i];
+        const unsigned li = FXLEN(i);
+        for (unsigned r = 0; r < li; r++) yi[r] *= beta;
+        for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+            const unsigned j = col[q], lj = FXLEN(j);
+            if (j == i) continue;
+            const double *a = val + valptr[q], *xj = x + dofptr[j];
+            for (unsigned r = 0; r < li; r++)
+                for (unsigned c = 0; c < lj; c++) yi[r] += alpha * a[r * lj + c] * xj[c];
+        }
+        const double *a = dia + diaptr[i], *xi = x + dofptr[i];
+        for (unsigned r = 0; r < li; r++)
+            for (unsigned c = 0; c < li; c++) yi[r] += alpha * a[r * li + c] * xi[c];
+    }
+}
+
+/* CMatDia_BlkCrs::SetBoundaryCondition, flex branch (matdia_blkcrs.cpp:268-306):
+ * rows of fixed dofs zeroed with a unit diagonal, then columns of fixed dofs
+ * zeroed in the CRS blocks. */
+void orc_fx_set_bc(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                   const unsigned *valptr, double *val, const unsigned *diaptr, double *dia, const int *flag)
+{
+    for (unsigned i = 0; i < nb; i++) {
+        const unsigned li = FXLEN(i);
+        for (unsigned r = 0; r < li; r++) {
+            if (flag[dofptr[i] + r] == 0) continue;
+            double *d = dia + diaptr[i];
+            for (unsigned k = 0; k < li; k++) d[k * li + r] = 0.0;
+            for (unsigned c = 0; c < li; c++) d[r * li + c] = 0.0;
+            d[r * li + r] = 1.0;
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+                const unsigned lj = FXLEN(col[q]);
+                for (unsigned c = 0; c < lj; c++) val[valptr[q] + r * lj + c] = 0.0;
+            }
+        }
+    }
+    for (unsigned i = 0; i < nb; i++) {
+        const unsigned li = FXLEN(i);
+        for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+            const unsigned j = col[q], lj = FXLEN(j);
+            for (unsigned c = 0; c < lj; c++) {
+                if (flag[dofptr[j] + c] == 0) continue;
+                for (unsigned r = 0; r < li; r++) val[valptr[q] + r * lj + c] = 0.0;
+            }
+        }
+    }
+}
+
+/* CMatDia_BlkCrs::Mearge, flex branch (matdia_blkcrs.cpp:173-207): one
+ * block (i, j) of len_i x len_j values added; a block outside the pattern
+ * is an error there (assert) and is skipped here. */
+void orc_fx_mearge(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                   const unsigned *valptr, double *val, const unsigned *diaptr, double *dia,
+                   unsigned n, const unsigned *iblk, const unsigned *jblk, const double *emat)
+{
+    (void)nb;
+    size_t off = 0;
+    for (unsigned e = 0; e < n; e++) {
+        const unsigned i = iblk[e], j = jblk[e], bs = FXLEN(i) * FXLEN(j);
+        double *out = NULL;
+        if (i == j) out = dia + diaptr[i];
+        else
+            for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++)
+                if (col[q] == j) { out = val + valptr[q]; break; }
+        if (out)
+            for (unsigned t = 0; t < bs; t++) out[t] += emat[off + t];
+        off += bs;
+    }
+}
+
+/* CMatDiaFrac_BlkCrs::SetValue_Initialize (include/delfem/matvec/
+ * matdiafrac_blkcrs.h:90-95) for flex blocks: the factor zeroed, then the
+ * matrix's blocks copied into the factor pattern (CMat_BlkCrs::SetValue,
+ * src/matvec/mat_blkcrs.cpp:565-614) and its diagonal (matdia_blkcrs.cpp:687-701). */
+void orc_fx_ilu_setvalue(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                         const unsigned *valptr, const double *val, const unsigned *diaptr, const double *dia,
+                         const unsigned *frowptr, const unsigned *fcol, const unsigned *fvalptr,
+                         double *fval, double *fdia)
+{
+    memset(fval, 0, sizeof(double) * fvalptr[frowptr[nb]]);
+    int *pos = (int *)malloc(sizeof(int) * (size_t)(nb ? nb : 1));
+    for (unsigned i = 0; i < nb; i++) pos[i] = -1;
+    for (unsigned i = 0; i < nb; i++) {
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = (int)q;
+        for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+            const int f = pos[col[q]];
+            if (f < 0) continue;
+            const unsigned bs = FXLEN(i) * FXLEN(col[q]);
+            memcpy(fval + fvalptr[f], val + valptr[q], sizeof(double) * bs);
+        }
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = -1;
+    }
+    free(pos);
+    memcpy(fdia, dia, sizeof(double) * diaptr[nb]);
+}
+
+/* CalcInvMat (src/matvec/matdiafrac_blkcrs.cpp:107-136): Gauss-Jordan in
+ * place; returns 1 (stopping where it met it) on a pivot below 1e-30. */
+static int fx_inv(double *a, unsigned n)
+{
+    for (unsigned i = 0; i < n; i++) {
+        if (fabs(a[i * n + i]) < 1.0e-30) return 1;
+        double t = 1.0 / a[i * n + i];
+        a[i * n + i] = 1.0;
+        for (unsigned k = 0; k < n; k++) a[i * n + k] *= t;
+        for (unsigned j = 0; j < n; j++) {
+            if (j == i) continue;
+            t = a[j * n + i];
+            a[j * n + i] = 0.0;
+            for (unsigned k = 0; k < n; k++) a[j * n + k] -= t * a[i * n + k];
+        }
+    }
+    return 0;
+}
+
+/* CMatDiaFrac_BlkCrs::DoILUDecomp, flex branch (matdiafrac_blkcrs.cpp:777-846):
+ * row-wise IKJ with CalcSubMatPr (A_ij -= A_ik A_kj, k innermost per entry,
+ * :62-73), CalcInvMat of the diagonal block, CalcMatPr (A_ij = D_i^-1 A_ij,
+ * :45-60) over the U part.  A singular pivot is counted and the row goes on
+ * with the half-inverted block; the 11th stops the factorisation at once
+ * (returns 0 there, 1 otherwise). */
+int orc_fx_ilu_decomp(unsigned nb, const unsigned *dofptr, const unsigned *frowptr, const unsigned *fcol,
+                      const unsigned *fdiaind, const unsigned *fvalptr, double *fval, const unsigned *diaptr,
+                      double *fdia)
+{
+    unsigned maxlen = 0;
+    for (unsigned i = 0; i < nb; i++) if (FXLEN(i) > maxlen) maxlen = FXLEN(i);
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)(maxlen * maxlen + 1));
+    int *pos = (int *)malloc(sizeof(int) * (size_t)(nb ? nb : 1));
+    for (unsigned i = 0; i < nb; i++) pos[i] = -1;
+    int nsing = 0, ok = 1;
+    for (unsigned i = 0; i < nb && ok; i++) {
+        const unsigned li = FXLEN(i);
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = (int)q;
+        for (unsigned qk = frowptr[i]; qk < fdiaind[i]; qk++) {
+            const unsigned k = fcol[qk], lk = FXLEN(k);
+            const double *aik = fval + fvalptr[qk];
+            for (unsigned qj = fdiaind[k]; qj < frowptr[k + 1]; qj++) {
+                const unsigned j = fcol[qj], lj = FXLEN(j);
+                const double *akj = fval + fvalptr[qj];
+                double *aij;
+                if (j != i) {
+                    if (pos[j] < 0) continue;
+                    aij = fval + fvalptr[pos[j]];
+                } else {
+                    aij = fdia + diaptr[i];
+                }
+                for (unsigned r = 0; r < li; r++)
+                    for (unsigned c = 0; c < lj; c++)
+                        for (unsigned m = 0; m < lk; m++) aij[r * lj + c] -= aik[r * lk + m] * akj[m * lj + c];
+            }
+        }
+        double *dii = fdia + diaptr[i];
+        if (fx_inv(dii, li) == 1 && ++nsing > 10) {
+            ok = 0;
+            break;
+        }
+        for (unsigned q = fdiaind[i]; q < frowptr[i + 1]; q++) {
+            double *aij = fval + fvalptr[q];
+            const unsigned lj = FXLEN(fcol[q]);
+            for (unsigned r = 0; r < li; r++)
+                for (unsigned c = 0; c < lj; c++) {
+                    tmp[r * lj + c] = 0.0;
+                    for (unsigned m = 0; m < li; m++) tmp[r * lj + c] += dii[r * li + m] * aij[m * lj + c];
+                }
+            memcpy(aij, tmp, sizeof(double) * li * lj);
+        }
+        for (unsigned q = frowptr[i]; q < frowptr[i + 1]; q++) pos[fcol[q]] = -1;
+    }
+    free(pos);
+    free(tmp);
+    return ok;
+}
+
+/* ForwardSubstitution, flex branch (matdiafrac_blkcrs.cpp:339-379):
+ * t = v_i - sum L_ij v_j (entry by entry), v_i = D_i^-1 t. */
+void orc_fx_ilu_forward(unsigned nb, const unsigned *dofptr, const unsigned *frowptr, const unsigned *fcol,
+                        const unsigned *fdiaind, const unsigned *fvalptr, const double *fval,
+                        const unsigned *diaptr, const double *fdia, double *vec)
+{
+    double t[64];
+    for (unsigned i = 0; i < nb; i++) {
+        const unsigned li = FXLEN(i);
+        for (unsigned r = 0; r < li; r++) t[r] = vec[dofptr[i] + r];
+        for (unsigned q = frowptr[i]; q < fdiaind[i]; q++) {
+            const unsigned j = fcol[q], lj = FXLEN(j);
+            const double *a = fval + fvalptr[q];
+            for (unsigned r = 0; r < li; r++)
+                for (unsigned c = 0; c < lj; c++) t[r] -= a[r * lj + c] * vec[dofptr[j] + c];
+        }
+        const double *d = fdia + diaptr[i];
+        for (unsigned r = 0; r < li; r++) {
+            double s = 0.0;
+            for (unsigned c = 0; c < li; c++) s += d[r * li + c] * t[c];
+            vec[dofptr[i] + r] = s;
+        }
+    }
+}
+
+/* BackwardSubstitution, flex branch (matdiafrac_blkcrs.cpp:512-553):
+ * v_i -= sum U_ij v_j, rows last to first. */
+void orc_fx_ilu_backward(unsigned nb, const unsigned *dofptr, const unsigned *frowptr, const unsigned *fcol,
+                         const unsigned *fdiaind, const unsigned *fvalptr, const double *fval, double *vec)
+{
+    double t[64];
+    for (long ii = (long)nb - 1; ii >= 0; ii--) {
+        const unsigned i = (unsigned)ii, li = FXLEN(i);
+        for (unsigned r = 0; r < li; r++) t[r] = vec[dofptr[i] + r];
+        for (unsigned q = fdiaind[i]; q < frowptr[i + 1]; q++) {
+            const unsigned j = fcol[q], lj = FXLEN(j);
+            const double *a = fval + fvalptr[q];
+            for (unsigned r = 0; r < li; r++)
+                for (unsigned c = 0; c < lj; c++) t[r] -= a[r * lj + c] * vec[dofptr[j] + c];
+        }
+        for (unsigned r = 0; r < li; r++) vec[dofptr[i] + r] = t[r];
+    }
+}
+#undef FXLEN
+
+/* One LsSol::CLinearSystem segment of variable block sizes
+ * (CLinearSystem::AddLinSysSeg(nnode, aLen), src/ls/linearsystem.cpp:170-201)
+ * driven by the LsSol solvers: MATVEC over the flex matrix (alpha == 0
+ * scales only, linearsystem.cpp:270-273), SolvePrecond = forward then
+ * backward substitution in natural order (preconditioner.cpp:301-335), DOT
+ * and the vector updates flat (vector_blk.cpp:37-112).
+ * kind: 0 Solve_PCG, 1 Solve_PBiCGSTAB (solver_ls_iter.cpp:100-172, 291-408),
+ * 2 Solve_CG, 3 Solve_BiCGSTAB (:31-97, 177-286; the factor arguments unused
+ * and may be NULL). */
+struct orc_fx_s {
+    unsigned nb;
+    const unsigned *dofptr, *rowptr, *col, *valptr, *diaptr;
+    const double *val, *dia;
+    const unsigned *frowptr, *fcol, *fdiaind, *fvalptr;
+    const double *fval, *fdia;
+};
+static void fx_sys_matvec(struct orc_fx_s *f, double alpha, const double *x, double beta, double *y)
+{
+    if (alpha == 0.0) { v_scal(f->dofptr[f->nb], beta, y); return; }
+    orc_fx_matvec(f->nb, f->dofptr, f->rowptr, f->col, f->valptr, f->val, f->diaptr, f->dia, alpha, x, beta, y);
+}
+static void fx_sys_precond(struct orc_fx_s *f, double *z)
+{
+    orc_fx_ilu_forward(f->nb, f->dofptr, f->frowptr, f->fcol, f->fdiaind, f->fvalptr, f->fval, f->diaptr, f->fdia, z);
+    orc_fx_ilu_backward(f->nb, f->dofptr, f->frowptr, f->fcol, f->fdiaind, f->fvalptr, f->fval, z);
+}
+int orc_fx_solve(int kind, unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                 const unsigned *valptr, const double *val, const unsigned *diaptr, const double *dia,
+                 const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const unsigned *fvalptr,
+                 const double *fval, const double *fdia, double *r, double *x, double *conv_ratio,
+                 unsigned *iteration)
+{
+    struct orc_fx_s f = {nb, dofptr, rowptr, col, valptr, diaptr, val, dia, frowptr, fcol, fdiaind, fvalptr,
+                         fval, fdia};
+    orc_sys s;
+    memset(&s, 0, sizeof s);
+    s.fx = &f;
+    const size_t n = dofptr[nb];
+    switch (kind) {
+        case 0: return pcg_core(&s, n, r, x, conv_ratio, iteration);
+        case 1: return bicg_core(&s, n, r, x, conv_ratio, iteration);
+        case 2: return ls_cg_core(&s, n, r, x, conv_ratio, iteration);
+        default: return ls_bicg_core(&s, n, r, x, conv_ratio, iteration);
     }
 }

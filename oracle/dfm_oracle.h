@@ -207,4 +207,36 @@ void orc_zemat_helmholtz2d_p1(unsigned nelem, const double *coord, const unsigne
 void orc_zemat_sommerfeld_b1(unsigned nline, const double *coord, const unsigned *conn, double wave_length,
                              const double *value, double *emat, double *eres);
 
+
+/* Variable block sizes (flex): dofptr[nb+1] dof offsets, valptr[ncrs+1] and
+ * diaptr[nb+1] value offsets of the CRS and diagonal blocks (row-major
+ * len_i x len_j), as m_DofPtrCol / m_ValPtr / m_DiaValPtr of the reference. */
+void orc_fx_layout(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                   unsigned *valptr, unsigned *diaptr);
+void orc_fx_matvec(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                   const unsigned *valptr, const double *val, const unsigned *diaptr, const double *dia,
+                   double alpha, const double *x, double beta, double *y);
+void orc_fx_set_bc(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                   const unsigned *valptr, double *val, const unsigned *diaptr, double *dia, const int *flag);
+void orc_fx_mearge(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                   const unsigned *valptr, double *val, const unsigned *diaptr, double *dia,
+                   unsigned n, const unsigned *iblk, const unsigned *jblk, const double *emat);
+void orc_fx_ilu_setvalue(unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                         const unsigned *valptr, const double *val, const unsigned *diaptr, const double *dia,
+                         const unsigned *frowptr, const unsigned *fcol, const unsigned *fvalptr,
+                         double *fval, double *fdia);
+int orc_fx_ilu_decomp(unsigned nb, const unsigned *dofptr, const unsigned *frowptr, const unsigned *fcol,
+                      const unsigned *fdiaind, const unsigned *fvalptr, double *fval, const unsigned *diaptr,
+                      double *fdia);
+void orc_fx_ilu_forward(unsigned nb, const unsigned *dofptr, const unsigned *frowptr, const unsigned *fcol,
+                        const unsigned *fdiaind, const unsigned *fvalptr, const double *fval,
+                        const unsigned *diaptr, const double *fdia, double *vec);
+void orc_fx_ilu_backward(unsigned nb, const unsigned *dofptr, const unsigned *frowptr, const unsigned *fcol,
+                         const unsigned *fdiaind, const unsigned *fvalptr, const double *fval, double *vec);
+int orc_fx_solve(int kind, unsigned nb, const unsigned *dofptr, const unsigned *rowptr, const unsigned *col,
+                 const unsigned *valptr, const double *val, const unsigned *diaptr, const double *dia,
+                 const unsigned *frowptr, const unsigned *fcol, const unsigned *fdiaind, const unsigned *fvalptr,
+                 const double *fval, const double *fdia, double *r, double *x, double *conv_ratio,
+                 unsigned *iteration);
+
 #endif

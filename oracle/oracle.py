@@ -146,6 +146,18 @@ class Orc:
         L.orc_bmc_order.argtypes = [C.c_uint, vp, vp, C.c_uint, vp, vp]
         L.orc_level_sets.restype = C.c_int
         L.orc_level_sets.argtypes = [C.c_uint, vp, vp, vp, C.c_int, vp]
+        L.orc_fx_layout.argtypes = [C.c_uint, vp, vp, vp, vp, vp]
+        L.orc_fx_matvec.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, C.c_double, vp, C.c_double, vp]
+        L.orc_fx_set_bc.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_fx_mearge.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, C.c_uint, vp, vp, vp]
+        L.orc_fx_ilu_setvalue.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_fx_ilu_decomp.restype = C.c_int
+        L.orc_fx_ilu_decomp.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_fx_ilu_forward.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_fx_ilu_backward.argtypes = [C.c_uint, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_fx_solve.restype = C.c_int
+        L.orc_fx_solve.argtypes = [C.c_int, C.c_uint] + [vp] * 13 + [vp, vp, C.POINTER(C.c_double),
+                                                                   C.POINTER(C.c_uint)]
         self.L = L
 
     # -- patterns ---------------------------------------------------------
@@ -251,6 +263,95 @@ class Orc:
         ok = f(nb, b, _p(u32(rowptr)), _p(u32(col)), _p(f64(val)), _p(f64(dia)), _p(u32(F["rowptr"])),
                _p(u32(F["col"])), _p(u32(F["diaind"])), _p(f64(fval)), _p(f64(fdia)), _p(order), _p(r), _p(x),
                C.byref(conv), C.byref(it))
+        return bool(ok), conv.value, it.value, x, r
+
+    # -- variable block sizes (flex) -------------------------------------------
+    def fx_layout(self, dofptr, rowptr, col):
+        """(valptr, diaptr): value offsets of the CRS and diagonal blocks."""
+        dofptr, rowptr, col = u32(dofptr), u32(rowptr), u32(col)
+        nb = rowptr.size - 1
+        valptr = np.zeros(col.size + 1, dtype=np.uint32)
+        diaptr = np.zeros(nb + 1, dtype=np.uint32)
+        self.L.orc_fx_layout(nb, _p(dofptr), _p(rowptr), _p(col), _p(valptr), _p(diaptr))
+        return valptr, diaptr
+
+    def fx_mearge(self, M, blocks, val=None, dia=None):
+        """Mearge of blocks [(i, j, values)] in order into the flex matrix M
+        (dict dofptr, rowptr, col, valptr, diaptr); returns (val, dia)."""
+        val = np.zeros(int(M["valptr"][-1])) if val is None else f64(val).copy()
+        dia = np.zeros(int(M["diaptr"][-1])) if dia is None else f64(dia).copy()
+        ib = u32([b[0] for b in blocks])
+        jb = u32([b[1] for b in blocks])
+        em = f64(np.concatenate([np.ravel(b[2]) for b in blocks])) if blocks else np.zeros(1)
+        self.L.orc_fx_mearge(M["rowptr"].size - 1, _p(u32(M["dofptr"])), _p(u32(M["rowptr"])),
+                             _p(u32(M["col"])), _p(u32(M["valptr"])), _p(val), _p(u32(M["diaptr"])), _p(dia),
+                             ib.size, _p(ib), _p(jb), _p(em))
+        return val, dia
+
+    def fx_set_bc(self, M, val, dia, flag):
+        val, dia = f64(val).copy(), f64(dia).copy()
+        self.L.orc_fx_set_bc(M["rowptr"].size - 1, _p(u32(M["dofptr"])), _p(u32(M["rowptr"])), _p(u32(M["col"])),
+                             _p(u32(M["valptr"])), _p(val), _p(u32(M["diaptr"])), _p(dia), _p(i32(flag)))
+        return val, dia
+
+    def fx_matvec(self, M, val, dia, alpha, x, beta, y):
+        y = f64(y).copy()
+        self.L.orc_fx_matvec(M["rowptr"].size - 1, _p(u32(M["dofptr"])), _p(u32(M["rowptr"])), _p(u32(M["col"])),
+                             _p(u32(M["valptr"])), _p(f64(val)), _p(u32(M["diaptr"])), _p(f64(dia)), alpha,
+                             _p(f64(x)), beta, _p(y))
+        return y
+
+    def fx_factor(self, M, lev):
+        """The ILU(lev) pattern of the flex matrix with its value offsets."""
+        F = self.iluk_symbolic(M["rowptr"], M["col"], lev)
+        F["dofptr"] = u32(M["dofptr"])
+        F["valptr"], F["diaptr"] = self.fx_layout(M["dofptr"], F["rowptr"], F["col"])
+        return F
+
+    def fx_ilu_values(self, M, val, dia, F):
+        """SetValue_Initialize + DoILUDecomp (flex): (fval, fdia, ok)."""
+        nb = M["rowptr"].size - 1
+        fval = np.zeros(max(1, int(F["valptr"][-1])))
+        fdia = np.zeros(max(1, int(F["diaptr"][-1])))
+        self.L.orc_fx_ilu_setvalue(nb, _p(u32(M["dofptr"])), _p(u32(M["rowptr"])), _p(u32(M["col"])),
+                                   _p(u32(M["valptr"])), _p(f64(val)), _p(u32(M["diaptr"])), _p(f64(dia)),
+                                   _p(u32(F["rowptr"])), _p(u32(F["col"])), _p(u32(F["valptr"])), _p(fval),
+                                   _p(fdia))
+        ok = self.L.orc_fx_ilu_decomp(nb, _p(u32(F["dofptr"])), _p(u32(F["rowptr"])), _p(u32(F["col"])),
+                                      _p(u32(F["diaind"])), _p(u32(F["valptr"])), _p(fval), _p(u32(F["diaptr"])),
+                                      _p(fdia))
+        return fval, fdia, bool(ok)
+
+    def fx_forward(self, F, fval, fdia, vec):
+        vec = f64(vec).copy()
+        self.L.orc_fx_ilu_forward(F["rowptr"].size - 1, _p(u32(F["dofptr"])), _p(u32(F["rowptr"])),
+                                  _p(u32(F["col"])), _p(u32(F["diaind"])), _p(u32(F["valptr"])), _p(f64(fval)),
+                                  _p(u32(F["diaptr"])), _p(f64(fdia)), _p(vec))
+        return vec
+
+    def fx_backward(self, F, fval, vec):
+        vec = f64(vec).copy()
+        self.L.orc_fx_ilu_backward(F["rowptr"].size - 1, _p(u32(F["dofptr"])), _p(u32(F["rowptr"])),
+                                   _p(u32(F["col"])), _p(u32(F["diaind"])), _p(u32(F["valptr"])), _p(f64(fval)),
+                                   _p(vec))
+        return vec
+
+    def fx_solve(self, kind, M, val, dia, F, fval, fdia, rhs, tol, maxit):
+        """LsSol solvers on the flex system, kind 'pcg', 'pbicgstab', 'cg' or
+        'bicgstab': (ok, conv, iter, x, r_final)."""
+        k = {"pcg": 0, "pbicgstab": 1, "cg": 2, "bicgstab": 3}[kind]
+        r = f64(rhs).copy()
+        x = np.zeros(r.size)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        if F is None:
+            fr = fc = fd = fv = None
+        else:
+            fr, fc, fd, fv = u32(F["rowptr"]), u32(F["col"]), u32(F["diaind"]), u32(F["valptr"])
+        ok = self.L.orc_fx_solve(k, M["rowptr"].size - 1, _p(u32(M["dofptr"])), _p(u32(M["rowptr"])),
+                                 _p(u32(M["col"])), _p(u32(M["valptr"])), _p(f64(val)), _p(u32(M["diaptr"])),
+                                 _p(f64(dia)), _p(fr), _p(fc), _p(fd), _p(fv), _p(None if fval is None else f64(fval)),
+                                 _p(None if fdia is None else f64(fdia)), _p(r), _p(x), C.byref(conv), C.byref(it))
         return bool(ok), conv.value, it.value, x, r
 
     # -- complex twin (len 1; complex128 arrays) ------------------------------
@@ -586,6 +687,25 @@ class Ref:
             "ref_bmat_bc": (None, [vp, vp, C.c_int]),
             "ref_bmat_matvec": (None, [vp, C.c_double, vp, C.c_double, vp]),
             "ref_zsolve_bicgstab": (C.c_int, [vp, vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+            "ref_fx_create": (vp, [C.c_uint, vp, vp, vp]),
+            "ref_fx_free": (None, [vp]),
+            "ref_fx_merge": (None, [vp, C.c_uint, vp, vp, vp]),
+            "ref_fx_set_bc": (None, [vp, vp]),
+            "ref_fx_finalize": (C.c_double, [vp]),
+            "ref_fx_ncrs": (C.c_uint, [vp]),
+            "ref_fx_get_matrix": (None, [vp, vp, vp, vp, vp, vp, vp]),
+            "ref_fx_set_vector": (None, [vp, C.c_int, vp]),
+            "ref_fx_get_vector": (None, [vp, C.c_int, vp]),
+            "ref_fx_resize_tmp": (None, [vp, C.c_uint]),
+            "ref_fx_matvec": (None, [vp, C.c_double, C.c_int, C.c_double, C.c_int]),
+            "ref_fx_frac_create": (vp, [vp, C.c_int]),
+            "ref_fx_frac_free": (None, [vp]),
+            "ref_fx_frac_ncrs": (C.c_uint, [vp]),
+            "ref_fx_frac_set_value": (C.c_int, [vp, vp]),
+            "ref_fx_frac_get": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "ref_fx_frac_forward": (None, [vp, vp, vp]),
+            "ref_fx_frac_backward": (None, [vp, vp, vp]),
+            "ref_fx_solve": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
             "ref_emat_poisson2d_p1": (None, [C.c_uint, vp, vp, vp, C.c_double, vp, vp, vp]),
             "ref_emat_solid3d_p1": (None, [C.c_uint, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double,
                                            C.c_double, vp, vp, vp]),
@@ -1191,3 +1311,209 @@ def assemble_problem(orc, prob):
     val, dia = orc.set_bc(prob.nnode, prob.b, rowptr, col, val, dia, prob.flags)
     res = orc.zero_bc(prob.flags, res)
     return dict(rowptr=rowptr, col=col, val=val, dia=dia, res=res, emat=emat, eres=eres)
+
+
+def flex_problem(nb=40, maxlen=6, seed=0, nfix=3, density=3, symmetric=True, lens=None, zero_rows=(), window=0):
+    """A synthetic system of nb blocks of 1..maxlen dofs (rigid bodies and
+    constraints in the reference's rigid-field system are such blocks): a
+    random sparse symmetric block pattern (a path plus random links), block
+    contributions listed for Mearge (each off-diagonal block merged in two
+    parts and every diagonal block in two, so that Mearge accumulates),
+    Dirichlet flags on the dofs of nfix random blocks, and a right-hand side.
+    symmetric: off-diagonal block (j, i) = (i, j)^T and positive definite
+    diagonal dominance (for PCG); otherwise unrelated random blocks.
+    lens: the block lengths (instead of random ones); zero_rows: block rows
+    that receive no contribution (an exactly zero pivot block); window > 0:
+    random links only between blocks at most window apart (bounded ILU(k)
+    fill for large nb)."""
+    rng = np.random.default_rng(seed)
+    alen = rng.integers(1, maxlen + 1, size=nb).astype(np.uint32)
+    if lens is not None:
+        alen = np.ascontiguousarray(lens, dtype=np.uint32)
+    dofptr = np.zeros(nb + 1, dtype=np.uint32)
+    dofptr[1:] = np.cumsum(alen)
+    adj = [set() for _ in range(nb)]
+    for i in range(nb - 1):
+        adj[i].add(i + 1)
+        adj[i + 1].add(i)
+    for _ in range(density * nb):
+        i, j = (int(v) for v in rng.integers(0, nb, size=2))
+        if window > 0:
+            j = min(nb - 1, max(0, i + int(rng.integers(-window, window + 1))))
+        if i != j:
+            adj[i].add(j)
+            adj[j].add(i)
+    rowptr = np.zeros(nb + 1, dtype=np.uint32)
+    cols = []
+    for i in range(nb):
+        c = sorted(adj[i])
+        cols.extend(c)
+        rowptr[i + 1] = rowptr[i] + len(c)
+    col = np.array(cols, dtype=np.uint32)
+    blocks = []
+    off = {}
+    for i in range(nb):
+        for j in sorted(adj[i]):
+            if symmetric and j < i:
+                continue
+            off[(i, j)] = rng.uniform(-1.0, 1.0, size=(alen[i], alen[j]))
+            if not symmetric or j > i:
+                pass
+    rowsum = np.zeros(int(dofptr[-1]))
+    for (i, j), v in off.items():
+        rowsum[dofptr[i]:dofptr[i + 1]] += np.abs(v).sum(axis=1)
+        if symmetric:
+            rowsum[dofptr[j]:dofptr[j + 1]] += np.abs(v).sum(axis=0)
+    for i in range(nb):
+        n = int(alen[i])
+        m = rng.uniform(-0.5, 0.5, size=(n, n))
+        d = 0.5 * (m + m.T) if symmetric else m
+        d = d + np.diag(rowsum[dofptr[i]:dofptr[i + 1]] + n + 1.0)
+        h = rng.uniform(0.2, 0.8)
+        blocks.append((i, i, h * d))
+        blocks.append((i, i, (1.0 - h) * d))
+    for (i, j), v in sorted(off.items()):
+        h = rng.uniform(0.2, 0.8)
+        blocks.append((i, j, h * v))
+        blocks.append((i, j, (1.0 - h) * v))
+        if symmetric:
+            blocks.append((j, i, v.T.copy()))
+    order = rng.permutation(len(blocks))
+    zr = set(int(v) for v in zero_rows)
+    blocks = [blocks[k] for k in order if blocks[k][0] not in zr]
+    flags = np.zeros(int(dofptr[-1]), dtype=np.int32)
+    for i in rng.choice(nb, size=min(nfix, nb), replace=False):
+        flags[dofptr[i]:dofptr[i + 1]] = 1
+    rhs = rng.uniform(-1.0, 1.0, size=int(dofptr[-1]))
+    rhs[flags != 0] = 0.0
+    return dict(nb=nb, alen=alen, dofptr=dofptr, rowptr=rowptr, col=col, blocks=blocks, flags=flags, rhs=rhs)
+
+
+class RefFlex:
+    """One LsSol::CLinearSystem segment of per-block sizes in the reference."""
+
+    def __init__(self, ref, P):
+        self.ref, self.P = ref, P
+        self._alen, self._index = u32(P["alen"]), u32(P["rowptr"])
+        self._array = u32(P["col"]) if P["col"].size else np.zeros(1, dtype=np.uint32)
+        self.h = ref.L.ref_fx_create(P["nb"], _p(self._alen), _p(self._index), _p(self._array))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.ref.L.ref_fx_free(self.h)
+            self.h = None
+
+    def merge(self, blocks):
+        ib = u32([b[0] for b in blocks])
+        jb = u32([b[1] for b in blocks])
+        em = f64(np.concatenate([np.ravel(b[2]) for b in blocks]))
+        self.ref.L.ref_fx_merge(self.h, ib.size, _p(ib), _p(jb), _p(em))
+
+    def set_bc(self, flags):
+        self.ref.L.ref_fx_set_bc(self.h, _p(i32(flags)))
+
+    def finalize(self):
+        return self.ref.L.ref_fx_finalize(self.h)
+
+    def matrix(self):
+        nb = self.P["nb"]
+        ncrs = self.ref.L.ref_fx_ncrs(self.h)
+        rowptr = np.zeros(nb + 1, dtype=np.uint32)
+        col = np.zeros(max(1, ncrs), dtype=np.uint32)
+        valptr = np.zeros(ncrs + 1, dtype=np.uint32)
+        diaptr = np.zeros(nb + 1, dtype=np.uint32)
+        self.ref.L.ref_fx_get_matrix(self.h, _p(rowptr), _p(col), _p(valptr), _p(diaptr), None, None)
+        val = np.zeros(max(1, int(valptr[-1])))
+        dia = np.zeros(max(1, int(diaptr[-1])))
+        self.ref.L.ref_fx_get_matrix(self.h, None, None, None, None, _p(val), _p(dia))
+        return dict(rowptr=rowptr, col=col[:ncrs], valptr=valptr, diaptr=diaptr, val=val[:int(valptr[-1])],
+                    dia=dia[:int(diaptr[-1])])
+
+    def set_vector(self, iv, v):
+        self.ref.L.ref_fx_set_vector(self.h, iv, _p(f64(v)))
+
+    def get_vector(self, iv):
+        out = np.zeros(int(self.P["dofptr"][-1]))
+        self.ref.L.ref_fx_get_vector(self.h, iv, _p(out))
+        return out
+
+    def matvec(self, alpha, x, beta, y):
+        self.ref.L.ref_fx_resize_tmp(self.h, 2)
+        self.set_vector(0, x)
+        self.set_vector(1, y)
+        self.ref.L.ref_fx_matvec(self.h, alpha, 0, beta, 1)
+        return self.get_vector(1)
+
+    def factor(self, lev):
+        """(ok, F, fval, fdia) of CMatDiaFrac_BlkCrs(lev, A) after SetValue_Initialize + DoILUDecomp."""
+        q = self.ref.L.ref_fx_frac_create(self.h, lev)
+        try:
+            ok = self.ref.L.ref_fx_frac_set_value(q, self.h)
+            nb = self.P["nb"]
+            ncrs = self.ref.L.ref_fx_frac_ncrs(q)
+            F = dict(rowptr=np.zeros(nb + 1, dtype=np.uint32), col=np.zeros(max(1, ncrs), dtype=np.uint32),
+                     diaind=np.zeros(max(1, nb), dtype=np.uint32), valptr=np.zeros(ncrs + 1, dtype=np.uint32),
+                     diaptr=np.zeros(nb + 1, dtype=np.uint32))
+            self.ref.L.ref_fx_frac_get(q, self.h, _p(F["rowptr"]), _p(F["col"]), _p(F["diaind"]), _p(F["valptr"]),
+                                       _p(F["diaptr"]), None, None)
+            fval = np.zeros(max(1, int(F["valptr"][-1])))
+            fdia = np.zeros(max(1, int(F["diaptr"][-1])))
+            self.ref.L.ref_fx_frac_get(q, self.h, None, None, None, None, None, _p(fval), _p(fdia))
+            F["col"] = F["col"][:ncrs]
+            F["dofptr"] = u32(self.P["dofptr"])
+            return bool(ok), F, fval[:int(F["valptr"][-1])], fdia[:int(F["diaptr"][-1])]
+        finally:
+            self.ref.L.ref_fx_frac_free(q)
+
+    def precond(self, lev, vec):
+        """Forward then backward substitution with the ILU(lev) factor."""
+        q = self.ref.L.ref_fx_frac_create(self.h, lev)
+        try:
+            self.ref.L.ref_fx_frac_set_value(q, self.h)
+            v = f64(vec).copy()
+            self.ref.L.ref_fx_frac_forward(q, self.h, _p(v))
+            fwd = v.copy()
+            self.ref.L.ref_fx_frac_backward(q, self.h, _p(v))
+            return fwd, v
+        finally:
+            self.ref.L.ref_fx_frac_free(q)
+
+    def solve(self, kind, lev, rhs, tol, maxit):
+        """The update (-2) is zeroed first: Solve_PBiCGSTAB returns on a zero
+        right-hand side before it zeroes the update, which would otherwise
+        keep the previous solve's solution."""
+        k = {"pcg": 0, "pbicgstab": 1, "cg": 2, "bicgstab": 3}[kind]
+        self.set_vector(-2, np.zeros(int(self.P["dofptr"][-1])))
+        r = f64(rhs).copy()
+        x = np.zeros(r.size)
+        conv = C.c_double(tol)
+        it = C.c_uint(maxit)
+        ok = self.ref.L.ref_fx_solve(self.h, k, lev, _p(r), _p(x), C.byref(conv), C.byref(it))
+        return ok, conv.value, it.value, x, r
+
+
+def flex_assemble(orc, P):
+    """Oracle pipeline for a flex problem: layout, Mearge, Dirichlet rows and
+    columns; returns the matrix dict M with val and dia."""
+    M = dict(dofptr=u32(P["dofptr"]), rowptr=u32(P["rowptr"]), col=u32(P["col"]))
+    M["valptr"], M["diaptr"] = orc.fx_layout(M["dofptr"], M["rowptr"], M["col"])
+    val, dia = orc.fx_mearge(M, P["blocks"])
+    val, dia = orc.fx_set_bc(M, val, dia, P["flags"])
+    M["val"], M["dia"] = val, dia
+    return M
+
+
+def flex_dense(M, val, dia):
+    """The flex matrix as a dense array (dofptr-indexed)."""
+    d = M["dofptr"]
+    n = int(d[-1])
+    A = np.zeros((n, n))
+    nb = M["rowptr"].size - 1
+    for i in range(nb):
+        li = int(d[i + 1] - d[i])
+        A[d[i]:d[i + 1], d[i]:d[i + 1]] = np.asarray(dia[M["diaptr"][i]:M["diaptr"][i] + li * li]).reshape(li, li)
+        for q in range(int(M["rowptr"][i]), int(M["rowptr"][i + 1])):
+            j = int(M["col"][q])
+            lj = int(d[j + 1] - d[j])
+            A[d[i]:d[i + 1], d[j]:d[j + 1]] = np.asarray(val[M["valptr"][q]:M["valptr"][q] + li * lj]).reshape(li, lj)
+    return A

@@ -143,6 +143,46 @@ void copy_vec_out(const MatVec::CVector_Blk& v, unsigned nb, unsigned b, double*
     for (unsigned r = 0; r < b; r++) dst[(size_t)i * b + r] = v.GetValue(i, r);
 }
 
+// A system of one segment with per-block sizes (AddLinSysSeg(nnode, aLen),
+// src/ls/linearsystem.cpp:170-201).
+struct RefFx {
+  LsSol::CLinearSystem ls;
+  unsigned nb = 0;
+  std::vector<unsigned> dofptr;
+};
+// Flex factor (or, at level 0 with no decomposition, a value copy of the
+// matrix) with access to its pattern and value-offset arrays.
+class FxPeek : public MatVec::CMatDiaFrac_BlkCrs {
+ public:
+  FxPeek(int lev, const MatVec::CMatDia_BlkCrs& A) : MatVec::CMatDiaFrac_BlkCrs(lev, A) {}
+  const unsigned* RowPtr() const { return m_colInd_Blk; }
+  const unsigned* Col() const { return m_rowPtr_Blk; }
+  const unsigned* DiaInd() const { return m_DiaInd; }
+  const unsigned* ValPtr() const { return m_ValPtr; }
+  const unsigned* DiaValPtr() const { return m_DiaValPtr; }
+  const double* Val() const { return m_valCrs_Blk; }
+  const double* Dia() const { return m_valDia_Blk; }
+};
+void fx_vec_in(MatVec::CVector_Blk& v, const RefFx* s, const double* src) {
+  for (unsigned i = 0; i < s->nb; i++)
+    for (unsigned r = 0; r < s->dofptr[i + 1] - s->dofptr[i]; r++) v.SetValue(i, r, src[s->dofptr[i] + r]);
+}
+void fx_vec_out(const MatVec::CVector_Blk& v, const RefFx* s, double* dst) {
+  for (unsigned i = 0; i < s->nb; i++)
+    for (unsigned r = 0; r < s->dofptr[i + 1] - s->dofptr[i]; r++) dst[s->dofptr[i] + r] = v.GetValue(i, r);
+}
+// pattern, value offsets and values of a flex factor (or copy)
+void fx_peek_out(const FxPeek* f, unsigned nb, unsigned* rowptr, unsigned* col, unsigned* valptr,
+                 unsigned* diaptr, double* val, double* dia) {
+  const unsigned ncrs = f->NCrs();
+  if (rowptr) std::memcpy(rowptr, f->RowPtr(), sizeof(unsigned) * (nb + 1));
+  if (col && ncrs) std::memcpy(col, f->Col(), sizeof(unsigned) * ncrs);
+  if (valptr) std::memcpy(valptr, f->ValPtr(), sizeof(unsigned) * (ncrs + 1));
+  if (diaptr) std::memcpy(diaptr, f->DiaValPtr(), sizeof(unsigned) * (nb + 1));
+  if (val && f->ValPtr()[ncrs]) std::memcpy(val, f->Val(), sizeof(double) * f->ValPtr()[ncrs]);
+  if (dia) std::memcpy(dia, f->Dia(), sizeof(double) * f->DiaValPtr()[nb]);
+}
+
 }  // namespace
 
 extern "C" {
@@ -812,4 +852,118 @@ void ref_emat_solid3d_p1(unsigned nelem, const double* coord, const unsigned* co
   }
 }
 
+
+// ---- variable block sizes (flex) -------------------------------------------
+void* ref_fx_create(unsigned nb, const unsigned* alen, const unsigned* index, const unsigned* array) {
+  RefFx* s = new RefFx;
+  s->nb = nb;
+  s->dofptr.assign(nb + 1, 0);
+  for (unsigned i = 0; i < nb; i++) s->dofptr[i + 1] = s->dofptr[i] + alen[i];
+  Com::CIndexedArray crs;
+  crs.InitializeSize(nb);
+  for (unsigned i = 0; i <= nb; i++) crs.index[i] = index[i];
+  crs.array.assign(array, array + index[nb]);
+  s->ls.AddLinSysSeg(nb, std::vector<unsigned>(alen, alen + nb));
+  s->ls.AddMat_Dia(0, crs);
+  s->ls.InitializeMarge();
+  return s;
+}
+void ref_fx_free(void* p) { delete static_cast<RefFx*>(p); }
+// CMatDia_BlkCrs::Mearge of single blocks (i, j), in order
+void ref_fx_merge(void* p, unsigned n, const unsigned* iblk, const unsigned* jblk, const double* emat) {
+  RefFx* s = static_cast<RefFx*>(p);
+  MatVec::CMatDia_BlkCrs& A = s->ls.GetMatrix(0);
+  size_t off = 0;
+  for (unsigned e = 0; e < n; e++) {
+    const unsigned bs = (s->dofptr[iblk[e] + 1] - s->dofptr[iblk[e]]) * (s->dofptr[jblk[e] + 1] - s->dofptr[jblk[e]]);
+    A.Mearge(1, iblk + e, 1, jblk + e, bs, emat + off);
+    off += bs;
+  }
+}
+void ref_fx_set_bc(void* p, const int* flags) {
+  RefFx* s = static_cast<RefFx*>(p);
+  for (unsigned i = 0; i < s->nb; i++)
+    for (unsigned d = 0; d < s->dofptr[i + 1] - s->dofptr[i]; d++)
+      if (flags[s->dofptr[i] + d]) s->ls.GetBCFlag(0).SetBC(i, d);
+}
+double ref_fx_finalize(void* p) { return static_cast<RefFx*>(p)->ls.FinalizeMarge(); }
+unsigned ref_fx_ncrs(void* p) { return static_cast<RefFx*>(p)->ls.GetMatrix(0).NCrs(); }
+// the matrix through a level-0 flex copy (SetValue_Initialize copies the
+// values block by block into the same pattern)
+void ref_fx_get_matrix(void* p, unsigned* rowptr, unsigned* col, unsigned* valptr, unsigned* diaptr, double* val,
+                       double* dia) {
+  RefFx* s = static_cast<RefFx*>(p);
+  FxPeek f(0, s->ls.GetMatrix(0));
+  f.SetValue_Initialize(s->ls.GetMatrix(0));
+  fx_peek_out(&f, s->nb, rowptr, col, valptr, diaptr, val, dia);
+}
+void ref_fx_set_vector(void* p, int iv, const double* src) {
+  RefFx* s = static_cast<RefFx*>(p);
+  fx_vec_in(s->ls.GetVector(iv, 0), s, src);
+}
+void ref_fx_get_vector(void* p, int iv, double* dst) {
+  RefFx* s = static_cast<RefFx*>(p);
+  fx_vec_out(s->ls.GetVector(iv, 0), s, dst);
+}
+void ref_fx_resize_tmp(void* p, unsigned ntmp) { static_cast<RefFx*>(p)->ls.ReSizeTmpVecSolver(ntmp); }
+void ref_fx_matvec(void* p, double alpha, int iv1, double beta, int iv2) {
+  static_cast<RefFx*>(p)->ls.MATVEC(alpha, iv1, beta, iv2);
+}
+// CMatDiaFrac_BlkCrs(lev, A) over the flex matrix
+void* ref_fx_frac_create(void* p, int lev) {
+  RefFx* s = static_cast<RefFx*>(p);
+  return new FxPeek(lev, s->ls.GetMatrix(0));
+}
+void ref_fx_frac_free(void* q) { delete static_cast<FxPeek*>(q); }
+unsigned ref_fx_frac_ncrs(void* q) { return static_cast<FxPeek*>(q)->NCrs(); }
+int ref_fx_frac_set_value(void* q, void* p) {
+  FxPeek* f = static_cast<FxPeek*>(q);
+  f->SetValue_Initialize(static_cast<RefFx*>(p)->ls.GetMatrix(0));
+  return f->DoILUDecomp() ? 1 : 0;
+}
+void ref_fx_frac_get(void* q, void* p, unsigned* rowptr, unsigned* col, unsigned* diaind, unsigned* valptr,
+                     unsigned* diaptr, double* val, double* dia) {
+  FxPeek* f = static_cast<FxPeek*>(q);
+  const unsigned nb = static_cast<RefFx*>(p)->nb;
+  fx_peek_out(f, nb, rowptr, col, valptr, diaptr, val, dia);
+  if (diaind) std::memcpy(diaind, f->DiaInd(), sizeof(unsigned) * nb);
+}
+void ref_fx_frac_forward(void* q, void* p, double* vec) {
+  RefFx* s = static_cast<RefFx*>(p);
+  std::vector<unsigned> alen(s->nb);
+  for (unsigned i = 0; i < s->nb; i++) alen[i] = s->dofptr[i + 1] - s->dofptr[i];
+  MatVec::CVector_Blk w(s->nb, alen);
+  fx_vec_in(w, s, vec);
+  static_cast<FxPeek*>(q)->ForwardSubstitution(w);
+  fx_vec_out(w, s, vec);
+}
+void ref_fx_frac_backward(void* q, void* p, double* vec) {
+  RefFx* s = static_cast<RefFx*>(p);
+  std::vector<unsigned> alen(s->nb);
+  for (unsigned i = 0; i < s->nb; i++) alen[i] = s->dofptr[i + 1] - s->dofptr[i];
+  MatVec::CVector_Blk w(s->nb, alen);
+  fx_vec_in(w, s, vec);
+  static_cast<FxPeek*>(q)->BackwardSubstitution(w);
+  fx_vec_out(w, s, vec);
+}
+// LsSol solvers on the flex system: kind 0 Solve_PCG / 1 Solve_PBiCGSTAB with
+// CPreconditioner_ILU(ls, lev) (SetValue inside), 2 Solve_CG / 3
+// Solve_BiCGSTAB without.  The residual (-1) must hold the right-hand side;
+// r out: the final residual, x out: the update (-2).
+int ref_fx_solve(void* p, int kind, int lev, double* r, double* x, double* conv, unsigned* iter) {
+  RefFx* s = static_cast<RefFx*>(p);
+  fx_vec_in(s->ls.GetVector(-1, 0), s, r);
+  bool ok;
+  if (kind <= 1) {
+    LsSol::CPreconditioner_ILU prec(s->ls, lev);
+    if (!prec.SetValue(s->ls)) return -1;
+    LsSol::CLinearSystemPreconditioner lsp(s->ls, prec);
+    ok = kind == 0 ? LsSol::Solve_PCG(*conv, *iter, lsp) : LsSol::Solve_PBiCGSTAB(*conv, *iter, lsp);
+  } else {
+    ok = kind == 2 ? LsSol::Solve_CG(*conv, *iter, s->ls) : LsSol::Solve_BiCGSTAB(*conv, *iter, s->ls);
+  }
+  fx_vec_out(s->ls.GetVector(-1, 0), s, r);
+  fx_vec_out(s->ls.GetVector(-2, 0), s, x);
+  return ok ? 1 : 0;
+}
 }  // extern "C"
