@@ -52,7 +52,7 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
-SHIM_PROGRAMS = ["shim_test", "zshim_test", "msshim_test", "poisson_dropin"]  # tests/cpp/<name>.cpp -> tests/cpp/<name>
+SHIM_PROGRAMS = ["shim_test", "zshim_test", "msshim_test", "poisson_dropin", "flex_dropin"]  # tests/cpp/<name>.cpp -> tests/cpp/<name>
 
 
 def build_shim_test(verbose=False):

@@ -77,6 +77,15 @@ struct MsegDeleter {
 }  // namespace detail
 
 class CLinearSystem : public ILinearSystem_Sol {
+ private:
+  struct Seg {
+    unsigned nnode = 0, len = 0;
+    MatVec::DofPtr dofptr;  // per-block lengths, else NULL
+    size_t off = 0;
+    std::unique_ptr<CMatDia_BlkCrs> mat;
+    std::unique_ptr<CBCFlag> bc;
+    std::vector<std::unique_ptr<CMat_BlkCrs>> nd;
+  };
  public:
   CLinearSystem() = default;
   ~CLinearSystem() override = default;
@@ -95,9 +104,33 @@ class CLinearSystem : public ILinearSystem_Sol {
     g.off = total_;
     g.mat.reset(new CMatDia_BlkCrs(nnode, len));
     g.bc.reset(new CBCFlag(nnode, len));
+    return add_segment(std::move(g), (size_t)nnode * len);
+  }
+  // AddLinSysSeg(nnode, aLen) (linearsystem.cpp:170-201): a segment whose
+  // block i has aLen[i] dofs (the matrix, vectors and flags with per-block
+  // lengths).  Such a segment is the system's only one here: the
+  // off-diagonal blocks that would couple it to others are not provided.
+  int AddLinSysSeg(unsigned nnode, const std::vector<unsigned>& aLen) {
+    if (aLen.size() != nnode) throw Error("CLinearSystem::AddLinSysSeg: aLen needs one length per block", DFM_BAD_ARG);
+    Seg g;
+    g.nnode = nnode;
+    g.len = 0;
+    g.off = total_;
+    g.mat.reset(new CMatDia_BlkCrs(nnode, aLen));
+    g.bc.reset(new CBCFlag(nnode, aLen));
+    g.dofptr = g.mat->dofptr();
+    const size_t ndof = g.dofptr->back();
+    return add_segment(std::move(g), ndof);
+  }
+
+ private:
+  int add_segment(Seg&& g, size_t ndof) {
+    if (!segs_.empty() && (g.dofptr || segs_[0].dofptr))
+      throw Error("CLinearSystem::AddLinSysSeg: a segment of per-block lengths must be the only one",
+                  DFM_NOT_IMPLEMENTED);
     segs_.push_back(std::move(g));
     const size_t total_old = total_;
-    total_ += (size_t)nnode * len;
+    total_ += ndof;
     for (auto& s : segs_) s.nd.resize(segs_.size());
     grow(res_, total_old);
     grow(upd_, total_old);
@@ -105,6 +138,8 @@ class CLinearSystem : public ILinearSystem_Sol {
     structure_changed();
     return (int)segs_.size() - 1;
   }
+
+ public:
   unsigned GetNLinSysSeg() const { return (unsigned)segs_.size(); }
   unsigned GetBlkSizeLinSeg(unsigned ilss) const { return segs_.at(ilss).nnode; }
   // AddMat_Dia(ilss, crs) (linearsystem.cpp:419-448): the pattern of segment ilss's matrix
@@ -122,6 +157,9 @@ class CLinearSystem : public ILinearSystem_Sol {
   bool AddMat_NonDia(unsigned ilss, unsigned jlss, const std::vector<unsigned>& index,
                      const std::vector<unsigned>& array) {
     if (ilss >= segs_.size() || jlss >= segs_.size() || ilss == jlss) return false;
+    if (segs_[ilss].dofptr || segs_[jlss].dofptr)
+      throw Error("CLinearSystem::AddMat_NonDia: blocks of per-block lengths between segments are not provided",
+                  DFM_NOT_IMPLEMENTED);
     auto& m = segs_[ilss].nd[jlss];
     if (!m) m.reset(new CMat_BlkCrs(segs_[ilss].nnode, segs_[ilss].len, segs_[jlss].nnode, segs_[jlss].len));
     structure_changed();
@@ -247,13 +285,6 @@ class CLinearSystem : public ILinearSystem_Sol {
   std::vector<CVector_Blk*> m_Residual, m_Update;
 
  private:
-  struct Seg {
-    unsigned nnode = 0, len = 0;
-    size_t off = 0;
-    std::unique_ptr<CMatDia_BlkCrs> mat;
-    std::unique_ptr<CBCFlag> bc;
-    std::vector<std::unique_ptr<CMat_BlkCrs>> nd;
-  };
   struct Vec {
     std::unique_ptr<CVector_Blk> whole;
     std::vector<std::unique_ptr<CVector_Blk>> seg;  // views into whole's store
@@ -261,8 +292,11 @@ class CLinearSystem : public ILinearSystem_Sol {
   Vec make_vec() {
     Vec v;
     v.whole.reset(new CVector_Blk((unsigned)total_, 1));
-    for (auto& g : segs_) v.seg.emplace_back(new CVector_Blk(g.nnode, g.len, v.whole->store(), g.off));
+    for (auto& g : segs_) v.seg.emplace_back(seg_view(g, v.whole->store()));
     return v;
+  }
+  static CVector_Blk* seg_view(const Seg& g, const std::shared_ptr<MatVec::detail::VecStore>& st) {
+    return g.dofptr ? new CVector_Blk(g.nnode, g.dofptr, st, g.off) : new CVector_Blk(g.nnode, g.len, st, g.off);
   }
   // a vector id after a segment was added: a longer store, the old values
   // copied in, the old segment views re-pointed, a view for the new segment
@@ -277,7 +311,7 @@ class CLinearSystem : public ILinearSystem_Sol {
         v.seg[i]->rebind(nv.whole->store(), segs_[i].off);
         nv.seg.push_back(std::move(v.seg[i]));
       } else {
-        nv.seg.emplace_back(new CVector_Blk(segs_[i].nnode, segs_[i].len, nv.whole->store(), segs_[i].off));
+        nv.seg.emplace_back(seg_view(segs_[i], nv.whole->store()));
       }
     }
     v = std::move(nv);
@@ -382,7 +416,13 @@ class CPreconditioner_ILU : public CPreconditioner {
   void SetLinearSystem(const CLinearSystem& ls) override {
     release();
     const std::vector<int> alev = FillLevels(ls.GetNLinSysSeg());
-    if (ls.GetNLinSysSeg() == 1) {
+    if (ls.GetNLinSysSeg() == 1 && ls.GetMatrix(0).flex()) {
+      // per-block lengths: natural-order ILU(lev) (matdiafrac_blkcrs.cpp:227-287)
+      if (ordering_ != DFM_ORDER_NATURAL || !fill_blk_.empty())
+        throw Error("CPreconditioner_ILU: per-block lengths take neither orderings nor fill blocks",
+                    DFM_NOT_IMPLEMENTED);
+      check(dfm_filu_create(ls.GetMatrix(0).fhandle(), alev[0], &fh_), "CPreconditioner_ILU::SetLinearSystem");
+    } else if (ls.GetNLinSysSeg() == 1) {
       const dfm_bcrs* A = ls.GetMatrix(0).handle();
       if (!fill_blk_.empty() && ordering_ == DFM_ORDER_NATURAL) {
         check(dfm_ilu_create_fill(A, alev[0], fill_blk_.data(), (unsigned)fill_blk_.size(), &h_),
@@ -412,22 +452,29 @@ class CPreconditioner_ILU : public CPreconditioner {
       ls.mseg();  // the blocks' device values current
       return check(dfm_mseg_ilu_set_value(ms_, &nsing), "CPreconditioner_ILU::SetValue") == DFM_OK;
     }
+    if (fh_)
+      return check(dfm_filu_set_value(fh_, ls.GetMatrix(0).fhandle(), &nsing), "CPreconditioner_ILU::SetValue") ==
+             DFM_OK;
     return check(dfm_ilu_set_value(h_, ls.GetMatrix(0).handle(), &nsing), "CPreconditioner_ILU::SetValue") == DFM_OK;
   }
   // SolvePrecond (preconditioner.cpp:301-335)
   bool SolvePrecond(CLinearSystem& ls, unsigned iv) override {
     bound(ls, "SolvePrecond");
     if (ms_) return check(dfm_mseg_ilu_precond(ms_, ls.vec((int)iv).device_ptr()), "SolvePrecond") == DFM_OK;
+    if (fh_) return check(dfm_filu_solve_precond(fh_, ls.vec((int)iv).device_ptr()), "SolvePrecond") == DFM_OK;
     return check(dfm_ilu_solve_precond(h_, ls.vec((int)iv).device_ptr()), "SolvePrecond") == DFM_OK;
   }
   dfm_ilu* handle() const { return h_; }
   dfm_msilu* mseg_handle() const { return ms_; }
+  dfm_filu* flex_handle() const { return fh_; }
   // set up on this system, whose structure has not changed since
-  bool bound_to(const CLinearSystem& ls) const { return ls_ == &ls && gen_ == ls.structure() && (h_ || ms_); }
+  bool bound_to(const CLinearSystem& ls) const {
+    return ls_ == &ls && gen_ == ls.structure() && (h_ || ms_ || fh_);
+  }
 
  private:
   void bound(const CLinearSystem& ls, const char* what) const {
-    if (!(h_ || ms_)) throw Error(what, DFM_BAD_ARG);
+    if (!(h_ || ms_ || fh_)) throw Error(what, DFM_BAD_ARG);
     if (!bound_to(ls))
       throw Error("CPreconditioner_ILU: the linear system differs from, or changed since, SetLinearSystem",
                   DFM_BAD_ARG);
@@ -436,8 +483,10 @@ class CPreconditioner_ILU : public CPreconditioner {
   void release() {
     if (h_) dfm_ilu_destroy(h_);
     if (ms_) dfm_mseg_ilu_destroy(ms_);
+    if (fh_) dfm_filu_destroy(fh_);
     h_ = nullptr;
     ms_ = nullptr;
+    fh_ = nullptr;
     mseg_.reset();
     ls_ = nullptr;
   }
@@ -448,6 +497,7 @@ class CPreconditioner_ILU : public CPreconditioner {
   std::vector<unsigned> fill_blk_;
   dfm_ilu* h_ = nullptr;
   dfm_msilu* ms_ = nullptr;
+  dfm_filu* fh_ = nullptr;  // a segment of per-block lengths
   std::shared_ptr<dfm_mseg> mseg_;  // keeps the handle ms_ was built on alive
   const CLinearSystem* ls_ = nullptr;
   unsigned long long gen_ = 0;
@@ -488,7 +538,12 @@ inline CPreconditioner_ILU* device_ilu(ILinearSystemPreconditioner_Sol& lsp, CLi
 inline bool device_solve(bool bicg, double& conv_ratio, unsigned& iteration, CLinearSystem& ls,
                          CPreconditioner_ILU& prec, int precision) {
   int rc;
-  if (prec.mseg_handle()) {
+  if (prec.flex_handle()) {
+    if (precision != DFM_PREC_FP64) throw Error("per-block lengths: FP64 solvers only", DFM_NOT_IMPLEMENTED);
+    dfm_fbcrs* A = ls.GetMatrix(0).fhandle();
+    double* r = ls.GetResidual(0).device_ptr();
+    rc = dfm_fsolve(bicg ? 1 : 0, A, prec.flex_handle(), r, ls.GetUpdate(0).device_ptr(), &conv_ratio, &iteration);
+  } else if (prec.mseg_handle()) {
     if (precision != DFM_PREC_FP64) throw Error("several segments: FP64 solvers only", DFM_NOT_IMPLEMENTED);
     auto S = ls.mseg();
     double* r = ls.vec(-1).device_ptr();

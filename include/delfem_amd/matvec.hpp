@@ -25,8 +25,10 @@
 // DELFEM_AMD_NO_ALIASES is defined, namespace MatVec aliases it so that code
 // written against the reference compiles unchanged.  Failures of the library
 // (bad arguments, device errors) throw delfem_amd::Error; numerical outcomes
-// are returned as bool like the reference.  Variable ("flex") block sizes are
-// not provided.
+// are returned as bool like the reference.  Variable ("flex") block sizes
+// (the reference's aLen constructors) are provided for vectors, flags, the
+// diagonal matrix and its factor; a flex CMat_BlkCrs (off-diagonal blocks
+// between segments) is not.
 #pragma once
 
 #include <algorithm>
@@ -117,18 +119,39 @@ class VecStore {
 
 class CMatDia_BlkCrs;
 
+typedef std::shared_ptr<const std::vector<unsigned>> DofPtr;
+// first dof of each block (m_DofPtr) of per-block lengths aLen
+inline DofPtr make_dofptr(const std::vector<unsigned>& aLen) {
+  auto d = std::make_shared<std::vector<unsigned>>(aLen.size() + 1, 0);
+  for (size_t i = 0; i < aLen.size(); i++) (*d)[i + 1] = (*d)[i] + aLen[i];
+  return d;
+}
+
 class CVector_Blk {
  public:
   CVector_Blk(unsigned nblk, unsigned len)
       : nblk_(nblk), len_(len), st_(std::make_shared<detail::VecStore>((size_t)nblk * len)) {
     SetVectorZero();
   }
+  // per-block lengths (vector_blk.h:57-60, Initialize :84-99): Len() is -1
+  CVector_Blk(unsigned nblk, const std::vector<unsigned>& aLen) : CVector_Blk(nblk, make_dofptr(aLen)) {}
+  CVector_Blk(unsigned nblk, DofPtr dofptr)
+      : nblk_(nblk), len_(0), dofptr_(std::move(dofptr)), st_(std::make_shared<detail::VecStore>(dofptr_->back())) {
+    SetVectorZero();
+  }
   // a copy of the values (vector_blk.h:62-72)
-  CVector_Blk(const CVector_Blk& v) : CVector_Blk(v.nblk_, v.len_) { *this = v; }
+  CVector_Blk(const CVector_Blk& v)
+      : nblk_(v.nblk_), len_(v.len_), dofptr_(v.dofptr_), st_(std::make_shared<detail::VecStore>(v.size())) {
+    SetVectorZero();
+    *this = v;
+  }
   // a view of nblk*len entries of `st` from `off` (one segment of a
   // multi-segment system's vector)
   CVector_Blk(unsigned nblk, unsigned len, std::shared_ptr<detail::VecStore> st, size_t off)
       : nblk_(nblk), len_(len), st_(std::move(st)), off_(off) {}
+  // ... and of a segment of per-block lengths
+  CVector_Blk(unsigned nblk, DofPtr dofptr, std::shared_ptr<detail::VecStore> st, size_t off)
+      : nblk_(nblk), len_(0), dofptr_(std::move(dofptr)), st_(std::move(st)), off_(off) {}
   virtual ~CVector_Blk() = default;
   // operator= copies the values (vector_blk.cpp:62-71)
   CVector_Blk& operator=(const CVector_Blk& rhs) {
@@ -155,9 +178,10 @@ class CVector_Blk {
   }
   double GetSquaredVectorNorm() const { return (*this) * (*this); }
   unsigned NBlk() const { return nblk_; }
-  int Len() const { return (int)len_; }
-  unsigned Len(unsigned /*iblk*/) const { return len_; }
-  unsigned GetTotalDofSize() const { return nblk_ * len_; }
+  // the block length, -1 with per-block lengths (vector_blk.h:121-127)
+  int Len() const { return dofptr_ ? -1 : (int)len_; }
+  unsigned Len(unsigned iblk) const { return dofptr_ ? (*dofptr_)[iblk + 1] - (*dofptr_)[iblk] : len_; }
+  unsigned GetTotalDofSize() const { return (unsigned)size(); }
   friend double operator*(const CVector_Blk& a, const CVector_Blk& b) {
     double r = 0.0;
     const double* x = a.dread();
@@ -166,17 +190,15 @@ class CVector_Blk {
     return r;
   }
   // per-entry access on the host copy (vector_blk.h:135-181)
-  const double& GetValue(unsigned iblk, unsigned idofblk) const {
-    return st_->host_read()[off_ + (size_t)iblk * len_ + idofblk];
+  const double& GetValue(unsigned iblk, unsigned idofblk) const { return st_->host_read()[pos(iblk) + idofblk]; }
+  void SetValue(unsigned iblk, unsigned idofblk, double val) { st_->host_write()[pos(iblk) + idofblk] = val; }
+  void AddValue(unsigned iblk, unsigned idofblk, double val) { st_->host_write()[pos(iblk) + idofblk] += val; }
+  double* GetValuePtr(unsigned iblk) { return st_->host_write() + pos(iblk); }
+  const double* GetValuePtr(unsigned iblk) const { return st_->host_read() + pos(iblk); }
+  // a zero vector of the same block layout
+  static CVector_Blk like(const CVector_Blk& v) {
+    return v.dofptr_ ? CVector_Blk(v.nblk_, v.dofptr_) : CVector_Blk(v.nblk_, v.len_);
   }
-  void SetValue(unsigned iblk, unsigned idofblk, double val) {
-    st_->host_write()[off_ + (size_t)iblk * len_ + idofblk] = val;
-  }
-  void AddValue(unsigned iblk, unsigned idofblk, double val) {
-    st_->host_write()[off_ + (size_t)iblk * len_ + idofblk] += val;
-  }
-  double* GetValuePtr(unsigned iblk) { return st_->host_write() + off_ + (size_t)iblk * len_; }
-  const double* GetValuePtr(unsigned iblk) const { return st_->host_read() + off_ + (size_t)iblk * len_; }
   // host copies of the whole vector
   void SetValues(const double* host) {
     double* p = whole() ? st_->dev_overwrite() : st_->dev_write() + off_;
@@ -193,7 +215,9 @@ class CVector_Blk {
   // device memory of the values, current; the non-const form counts as a write
   double* device_ptr() { return dwrite(); }
   const double* device_ptr() const { return dread(); }
-  size_t size() const { return (size_t)nblk_ * len_; }
+  size_t size() const { return dofptr_ ? dofptr_->back() : (size_t)nblk_ * len_; }
+  // the first dof of each block with per-block lengths, else NULL
+  const DofPtr& dofptr() const { return dofptr_; }
   // re-point this vector at another store (the vector keeps its identity);
   // the caller has copied the values over
   void rebind(std::shared_ptr<detail::VecStore> st, size_t off) {
@@ -205,9 +229,11 @@ class CVector_Blk {
 
  private:
   bool whole() const { return off_ == 0 && size() == st_->size(); }
+  size_t pos(unsigned iblk) const { return off_ + (dofptr_ ? (size_t)(*dofptr_)[iblk] : (size_t)iblk * len_); }
   const double* dread() const { return st_->dev_read() + off_; }
   double* dwrite() const { return st_->dev_write() + off_; }
   unsigned nblk_, len_;
+  DofPtr dofptr_;
   std::shared_ptr<detail::VecStore> st_;
   size_t off_ = 0;
 };
@@ -215,13 +241,24 @@ class CVector_Blk {
 class CBCFlag {
  public:
   CBCFlag(unsigned nblk, unsigned len) : nblk_(nblk), len_(len), flag_((size_t)nblk * len, 0) {}
+  // per-block lengths (bcflag_blk.h:46-58)
+  CBCFlag(unsigned nblk, const std::vector<unsigned>& aLen)
+      : nblk_(nblk), len_(0), dofptr_(make_dofptr(aLen)), flag_(dofptr_->back(), 0) {}
   virtual ~CBCFlag() = default;
   void SetAllFlagZero() { std::fill(flag_.begin(), flag_.end(), 0); }
-  void SetBC(unsigned iblk, unsigned idof, int flag = 1) { flag_[(size_t)iblk * len_ + idof] = flag; }
-  int GetBCFlag(unsigned iblk, unsigned idof) const { return flag_[(size_t)iblk * len_ + idof]; }
+  // SetBC (bcflag_blk.h:153-162): false, and nothing set, with per-block
+  // lengths -- the reference refuses them (its flex flags stay zero)
+  bool SetBC(unsigned iblk, unsigned idof, int flag = 1) {
+    if (dofptr_) return false;
+    flag_[(size_t)iblk * len_ + idof] = flag;
+    return true;
+  }
+  int GetBCFlag(unsigned iblk, unsigned idof) const {
+    return flag_[(dofptr_ ? (size_t)(*dofptr_)[iblk] : (size_t)iblk * len_) + idof];
+  }
   unsigned NBlk() const { return nblk_; }
-  int LenBlk() const { return (int)len_; }
-  unsigned LenBlk(unsigned /*iblk*/) const { return len_; }
+  int LenBlk() const { return dofptr_ ? -1 : (int)len_; }
+  unsigned LenBlk(unsigned iblk) const { return dofptr_ ? (*dofptr_)[iblk + 1] - (*dofptr_)[iblk] : len_; }
   // CBCFlag::SetZeroToBCDof (bcflag_blk.h:102-129)
   bool SetZeroToBCDof(CVector_Blk& v) const {
     check(dfm_vec_zero_bc(v.size(), flag_.data(), v.device_ptr()), "SetZeroToBCDof");
@@ -231,6 +268,7 @@ class CBCFlag {
 
  private:
   unsigned nblk_, len_;
+  DofPtr dofptr_;
   std::vector<int> flag_;
 };
 
@@ -240,6 +278,7 @@ namespace detail {
 // back whole before the matrix is next used on the device.
 struct ValMirror {
   std::vector<unsigned> rowptr, col;  // pattern (host)
+  std::vector<unsigned> valptr, diaptr;  // per-block lengths: value offsets (m_ValPtr, m_DiaValPtr)
   std::vector<double> val, dia;
   bool valid = false;                 // val / dia hold the current values
   bool dirty = false;                 // ... and the device does not
@@ -252,22 +291,43 @@ class CMatDia_BlkCrs {
     check(dfm_bcrs_create(nblk, len, &h_), "CMatDia_BlkCrs");
     m_.rowptr.assign(nblk + 1, 0);
   }
+  // per-block lengths (Initialize(nblk, aLen), matdia_blkcrs.cpp:70-104):
+  // LenBlkCol() is -1, Mearge takes one block at a time
+  CMatDia_BlkCrs(unsigned nblk, const std::vector<unsigned>& aLen)
+      : nblk_(nblk), len_(0), dofptr_(make_dofptr(aLen)) {
+    if (aLen.size() != nblk) throw Error("CMatDia_BlkCrs: aLen needs one length per block", DFM_BAD_ARG);
+    check(dfm_fbcrs_create(nblk, aLen.data(), &fh_), "CMatDia_BlkCrs");
+    m_.rowptr.assign(nblk + 1, 0);
+    flex_layout();
+  }
   virtual ~CMatDia_BlkCrs() {
     if (h_) dfm_bcrs_destroy(h_);
+    if (fh_) dfm_fbcrs_destroy(fh_);
   }
   CMatDia_BlkCrs(const CMatDia_BlkCrs&) = delete;
   CMatDia_BlkCrs& operator=(const CMatDia_BlkCrs&) = delete;
   unsigned NBlkMatCol() const { return nblk_; }
   unsigned NBlkMatRow() const { return nblk_; }
-  int LenBlkCol() const { return (int)len_; }
-  int LenBlkRow() const { return (int)len_; }
+  int LenBlkCol() const { return fh_ ? -1 : (int)len_; }
+  int LenBlkRow() const { return fh_ ? -1 : (int)len_; }
+  unsigned LenBlkCol(unsigned iblk) const { return fh_ ? (*dofptr_)[iblk + 1] - (*dofptr_)[iblk] : len_; }
+  unsigned LenBlkRow(unsigned iblk) const { return LenBlkCol(iblk); }
   unsigned NCrs() const { return (unsigned)m_.col.size(); }
+  // with per-block lengths: the first dof of each block, else NULL
+  const DofPtr& dofptr() const { return dofptr_; }
   // AddPattern(const Com::CIndexedArray&) (matdia_blkcrs.cpp:346-505): the
   // diagonal is excluded, rows are merged with the existing pattern
   bool AddPattern(const Com::CIndexedArray& crs) { return AddPattern(crs.index, crs.array); }
   bool AddPattern(const std::vector<unsigned>& index, const std::vector<unsigned>& array) {
     flush();
     static const unsigned none = 0;
+    if (fh_) {
+      const bool ok = check(dfm_fbcrs_add_pattern(fh_, (unsigned)index.size() - 1, index.data(),
+                                                  array.empty() ? &none : array.data()),
+                            "AddPattern") == DFM_OK;
+      flex_layout();
+      return ok;
+    }
     const bool ok = check(dfm_bcrs_add_pattern(h_, (unsigned)index.size() - 1, index.data(),
                                                array.empty() ? &none : array.data()),
                           "AddPattern") == DFM_OK;
@@ -281,18 +341,20 @@ class CMatDia_BlkCrs {
   // SetZero (matdia_blkcrs.cpp:139-163): on the host copy, which the element
   // loop that follows fills
   bool SetZero() {
-    const size_t bb = (size_t)len_ * len_;
-    m_.val.assign(m_.col.size() * bb, 0.0);
-    m_.dia.assign((size_t)nblk_ * bb, 0.0);
+    m_.val.assign(nval(), 0.0);
+    m_.dia.assign(ndia(), 0.0);
     m_.valid = m_.dirty = true;
     return true;
   }
   // Mearge (matdia_blkcrs.cpp:165-258): element block (i, j) at
   // emat[(i*nblkel_row + j)*blksize]; position i == j goes to the diagonal
   // block of node blkel_col[i], other blocks to their CRS entry of row
-  // blkel_col[i], and blocks outside the pattern are dropped.
+  // blkel_col[i], and blocks outside the pattern are dropped.  With
+  // per-block lengths (:173-207) one block (i, j) of len_i x len_j values per
+  // call, which must be in the pattern.
   bool Mearge(unsigned nblkel_col, const unsigned* blkel_col, unsigned nblkel_row, const unsigned* blkel_row,
               unsigned blksize, const double* emat) {
+    if (fh_) return flex_mearge(nblkel_col, blkel_col, nblkel_row, blkel_row, blksize, emat);
     const size_t bb = (size_t)len_ * len_;
     if (blksize != bb) return false;
     for (unsigned i = 0; i < nblkel_col; i++)
@@ -319,36 +381,43 @@ class CMatDia_BlkCrs {
     }
     return true;
   }
-  // y = beta*y + alpha*A*x (matdia_blkcrs.cpp:921-1094)
+  // y = beta*y + alpha*A*x (matdia_blkcrs.cpp:921-1094; per-block lengths :931-960)
   bool MatVec(double alpha, const CVector_Blk& x, double beta, CVector_Blk& y) const {
+    if (fh_) {
+      dfm_fbcrs* h = fhandle();
+      const double* xp = x.device_ptr();
+      return check(dfm_fbcrs_matvec(h, alpha, xp, beta, y.device_ptr()), "MatVec") == DFM_OK;
+    }
     dfm_bcrs* h = handle();
     const double* xp = x.device_ptr();
     return check(dfm_bcrs_matvec(h, alpha, xp, beta, y.device_ptr()), "MatVec") == DFM_OK;
   }
   bool SetBoundaryCondition(const CBCFlag& flag) {
+    if (fh_) return check(dfm_fbcrs_set_bc(fhandle(), flag.data()), "SetBoundaryCondition") == DFM_OK;
     return check(dfm_bcrs_set_bc(handle(), flag.data()), "SetBoundaryCondition") == DFM_OK;
   }
   // the pattern and values of one row on the host (mat_blkcrs.h:150-173,
-  // matdia_blkcrs.h:102-111); the non-const forms count as writes
+  // matdia_blkcrs.h:102-111); the non-const forms count as writes.  With
+  // per-block lengths the values of a row are its blocks one after the other.
   const unsigned* GetPtrIndPSuP(unsigned ipoin, unsigned& npsup) const {
     npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
     return m_.col.data() + m_.rowptr[ipoin];
   }
   const double* GetPtrValPSuP(unsigned ipoin, unsigned& npsup) const {
     npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
-    return host_values(false).data() + (size_t)m_.rowptr[ipoin] * len_ * len_;
+    return host_values(false).data() + val_offset(m_.rowptr[ipoin]);
   }
   double* GetPtrValPSuP(unsigned ipoin, unsigned& npsup) {
     npsup = m_.rowptr[ipoin + 1] - m_.rowptr[ipoin];
-    return host_values(true).data() + (size_t)m_.rowptr[ipoin] * len_ * len_;
+    return host_values(true).data() + val_offset(m_.rowptr[ipoin]);
   }
   const double* GetPtrValDia(unsigned ipoin) const {
     host_values(false);
-    return m_.dia.data() + (size_t)ipoin * len_ * len_;
+    return m_.dia.data() + dia_offset(ipoin);
   }
   double* GetPtrValDia(unsigned ipoin) {
     host_values(true);
-    return m_.dia.data() + (size_t)ipoin * len_ * len_;
+    return m_.dia.data() + dia_offset(ipoin);
   }
   // host copies of the values (CRS order) and of the pattern
   void GetValues(std::vector<double>& val, std::vector<double>& dia) const {
@@ -359,20 +428,62 @@ class CMatDia_BlkCrs {
   const std::vector<unsigned>& RowPtr() const { return m_.rowptr; }
   const std::vector<unsigned>& Col() const { return m_.col; }
   // the C-ABI handle with the device values current; the caller may change
-  // them, so the host copy is dropped
+  // them, so the host copy is dropped.  handle(): fixed block length only;
+  // fhandle(): per-block lengths only.
   dfm_bcrs* handle() const {
+    if (fh_) throw Error("CMatDia_BlkCrs: per-block lengths have no fixed-length handle", DFM_NOT_IMPLEMENTED);
     flush();
     m_.valid = false;
     return h_;
   }
+  dfm_fbcrs* fhandle() const {
+    if (!fh_) throw Error("CMatDia_BlkCrs: a fixed block length has no per-block-length handle", DFM_BAD_ARG);
+    flush();
+    m_.valid = false;
+    return fh_;
+  }
+  bool flex() const { return fh_ != nullptr; }
 
  private:
+  size_t nval() const { return fh_ ? m_.valptr.back() : m_.col.size() * len_ * len_; }
+  size_t ndia() const { return fh_ ? m_.diaptr.back() : (size_t)nblk_ * len_ * len_; }
+  size_t val_offset(unsigned q) const { return fh_ ? m_.valptr[q] : (size_t)q * len_ * len_; }
+  size_t dia_offset(unsigned i) const { return fh_ ? m_.diaptr[i] : (size_t)i * len_ * len_; }
+  void flex_layout() {
+    unsigned nb = 0, ndof = 0, ncrs = 0, nv = 0, nd = 0;
+    check(dfm_fbcrs_info(fh_, &nb, &ndof, &ncrs, &nv, &nd), "CMatDia_BlkCrs");
+    m_.col.resize(ncrs);
+    m_.valptr.resize(ncrs + 1);
+    m_.diaptr.resize(nblk_ + 1);
+    check(dfm_fbcrs_layout(fh_, nullptr, m_.rowptr.data(), m_.col.data(), m_.valptr.data(), m_.diaptr.data()),
+          "CMatDia_BlkCrs");
+    m_.valid = m_.dirty = false;
+  }
+  bool flex_mearge(unsigned nblkel_col, const unsigned* blkel_col, unsigned nblkel_row, const unsigned* blkel_row,
+                   unsigned blksize, const double* emat) {
+    if (nblkel_col != 1 || nblkel_row != 1) return false;
+    const unsigned i = blkel_col[0], j = blkel_row[0];
+    if (i >= nblk_ || j >= nblk_ || blksize != LenBlkCol(i) * LenBlkCol(j)) return false;
+    host_values(true);
+    double* out;
+    if (i == j) {
+      out = m_.dia.data() + m_.diaptr[i];
+    } else {
+      const unsigned* c0 = m_.col.data() + m_.rowptr[i];
+      const unsigned* c1 = m_.col.data() + m_.rowptr[i + 1];
+      const unsigned* q = std::lower_bound(c0, c1, j);
+      if (q == c1 || *q != j) return false;
+      out = m_.val.data() + m_.valptr[q - m_.col.data()];
+    }
+    for (unsigned t = 0; t < blksize; t++) out[t] += emat[t];
+    return true;
+  }
   std::vector<double>& host_values(bool write) const {
     if (!m_.valid) {
-      const size_t bb = (size_t)len_ * len_;
-      m_.val.resize(m_.col.size() * bb);
-      m_.dia.resize((size_t)nblk_ * bb);
-      check(dfm_bcrs_get_values(h_, m_.val.data(), m_.dia.data()), "CMatDia_BlkCrs values");
+      m_.val.resize(nval());
+      m_.dia.resize(ndia());
+      if (fh_) check(dfm_fbcrs_get_values(fh_, m_.val.data(), m_.dia.data()), "CMatDia_BlkCrs values");
+      else check(dfm_bcrs_get_values(h_, m_.val.data(), m_.dia.data()), "CMatDia_BlkCrs values");
       m_.valid = true;
       m_.dirty = false;
     }
@@ -381,11 +492,14 @@ class CMatDia_BlkCrs {
   }
   void flush() const {
     if (!m_.dirty) return;
-    check(dfm_bcrs_set_values(h_, m_.val.data(), m_.dia.data()), "CMatDia_BlkCrs values");
+    if (fh_) check(dfm_fbcrs_set_values(fh_, m_.val.data(), m_.dia.data()), "CMatDia_BlkCrs values");
+    else check(dfm_bcrs_set_values(h_, m_.val.data(), m_.dia.data()), "CMatDia_BlkCrs values");
     m_.dirty = false;
   }
   unsigned nblk_, len_;
+  DofPtr dofptr_;
   dfm_bcrs* h_ = nullptr;
+  dfm_fbcrs* fh_ = nullptr;
   mutable detail::ValMirror m_;
 };
 
@@ -535,10 +649,15 @@ class CMatDiaFrac_BlkCrs : public CPrecond_Blk {
   // CMatDiaFrac_BlkCrs(A): ILU(0) pattern of A
   explicit CMatDiaFrac_BlkCrs(const CMatDia_BlkCrs& A) : CMatDiaFrac_BlkCrs(0, A) {}
   // CMatDiaFrac_BlkCrs(lev, A): symbolic ILU(lev) of A's pattern (lev = -1: unlimited)
+  // (with per-block lengths: the flex factor, matdiafrac_blkcrs.cpp:227-245)
   CMatDiaFrac_BlkCrs(int lev_fill, const CMatDia_BlkCrs& A) : src_(&A) {
+    if (A.flex()) {
+      check(dfm_filu_create(A.fhandle(), lev_fill, &fh_), "CMatDiaFrac_BlkCrs");
+      return;
+    }
     check(dfm_ilu_create(A.handle(), lev_fill, DFM_ORDER_NATURAL, nullptr, 0, &h_), "CMatDiaFrac_BlkCrs");
   }
-  // ordered variant, order[new] = old
+  // ordered variant, order[new] = old (fixed block length)
   CMatDiaFrac_BlkCrs(int lev_fill, const CMatDia_BlkCrs& A, const std::vector<int>& order)
       : src_(&A), ordering_(DFM_ORDER_GIVEN) {
     check(dfm_ilu_create(A.handle(), lev_fill, DFM_ORDER_GIVEN, order.data(), 0, &h_), "CMatDiaFrac_BlkCrs");
@@ -550,6 +669,7 @@ class CMatDiaFrac_BlkCrs : public CPrecond_Blk {
   }
   ~CMatDiaFrac_BlkCrs() override {
     if (h_) dfm_ilu_destroy(h_);
+    if (fh_) dfm_filu_destroy(fh_);
   }
   CMatDiaFrac_BlkCrs(const CMatDiaFrac_BlkCrs&) = delete;
   CMatDiaFrac_BlkCrs& operator=(const CMatDiaFrac_BlkCrs&) = delete;
@@ -558,7 +678,8 @@ class CMatDiaFrac_BlkCrs : public CPrecond_Blk {
   // which ignores DoILUDecomp's result there
   bool SetValue(const CMatDia_BlkCrs& A) {
     nsing_ = 0;
-    factored_ = check(dfm_ilu_set_value(h_, A.handle(), &nsing_), "SetValue") == DFM_OK;
+    if (fh_) factored_ = check(dfm_filu_set_value(fh_, A.fhandle(), &nsing_), "SetValue") == DFM_OK;
+    else factored_ = check(dfm_ilu_set_value(h_, A.handle(), &nsing_), "SetValue") == DFM_OK;
     return true;
   }
   // DoILUDecomp (matdiafrac_blkcrs.cpp:769-1102): the device factorises in
@@ -568,9 +689,11 @@ class CMatDiaFrac_BlkCrs : public CPrecond_Blk {
   // singular pivots met by the last factorisation
   int NSingular() const { return nsing_; }
   bool ForwardSubstitution(CVector_Blk& v) const {
+    if (fh_) return check(dfm_filu_forward(fh_, v.device_ptr()), "ForwardSubstitution") == DFM_OK;
     return check(dfm_ilu_forward(h_, v.device_ptr()), "ForwardSubstitution") == DFM_OK;
   }
   bool BackwardSubstitution(CVector_Blk& v) const {
+    if (fh_) return check(dfm_filu_backward(fh_, v.device_ptr()), "BackwardSubstitution") == DFM_OK;
     return check(dfm_ilu_backward(h_, v.device_ptr()), "BackwardSubstitution") == DFM_OK;
   }
   // Solve (matdiafrac_blkcrs.h:60-64): forward then backward substitution in
@@ -588,17 +711,20 @@ class CMatDiaFrac_BlkCrs : public CPrecond_Blk {
   const CMatDia_BlkCrs* source() const { return src_; }
   int ordering() const { return ordering_; }
   unsigned NCrs() const {
-    unsigned n = 0;
+    unsigned n = 0, nv = 0;
     int a, b, c;
-    check(dfm_ilu_info(h_, &n, &a, &b, &c), "NCrs");
+    if (fh_) check(dfm_filu_info(fh_, &n, &nv, &a, &b), "NCrs");
+    else check(dfm_ilu_info(h_, &n, &a, &b, &c), "NCrs");
     return n;
   }
   dfm_ilu* handle() const { return h_; }
+  dfm_filu* fhandle() const { return fh_; }
 
  private:
   const CMatDia_BlkCrs* src_ = nullptr;
   int ordering_ = DFM_ORDER_NATURAL;
   dfm_ilu* h_ = nullptr;
+  dfm_filu* fh_ = nullptr;
   int nsing_ = 0;
   bool factored_ = false;
 };
@@ -614,7 +740,7 @@ namespace Sol {
 namespace detail {
 inline const CMatDiaFrac_BlkCrs* device_factor(const CMatDia_BlkCrs& mat, const CPrecond_Blk& precond) {
   const auto* f = dynamic_cast<const CMatDiaFrac_BlkCrs*>(&precond);
-  return f && f->source() == &mat && f->ordering() == DFM_ORDER_NATURAL ? f : nullptr;
+  return f && !mat.flex() && f->source() == &mat && f->ordering() == DFM_ORDER_NATURAL ? f : nullptr;
 }
 }  // namespace detail
 
@@ -625,7 +751,6 @@ inline bool Solve_CG(double& conv_ratio, unsigned& iteration, const CMatDia_BlkC
                      CVector_Blk& u_vec) {
   const double tol = conv_ratio;
   const unsigned mx_iter = iteration;
-  const unsigned nblk = r_vec.NBlk(), len = r_vec.Len();
   u_vec.SetVectorZero();
   double rr = r_vec * r_vec;
   if (rr < 1.0e-30) {
@@ -634,7 +759,7 @@ inline bool Solve_CG(double& conv_ratio, unsigned& iteration, const CMatDia_BlkC
     return true;
   }
   const double inv_rr0 = 1.0 / rr;
-  CVector_Blk p_vec(nblk, len), Ap_vec(nblk, len);
+  CVector_Blk p_vec = CVector_Blk::like(r_vec), Ap_vec = CVector_Blk::like(r_vec);
   p_vec = r_vec;
   iteration = mx_iter;
   for (unsigned it = 1; it < mx_iter; it++) {
@@ -672,7 +797,6 @@ inline bool Solve_PCG(double& conv_ratio, unsigned& iteration, const CMatDia_Blk
     iteration = mx_iter;
     return false;
   }
-  const unsigned nblk = r_vec.NBlk(), len = r_vec.Len();
   u_vec.SetVectorZero();
   const double rr0 = r_vec * r_vec;
   if (rr0 < 1.0e-30) {
@@ -681,7 +805,7 @@ inline bool Solve_PCG(double& conv_ratio, unsigned& iteration, const CMatDia_Blk
     return true;
   }
   const double inv_rr0 = 1.0 / rr0;
-  CVector_Blk p_vec(nblk, len), z_vec(nblk, len);
+  CVector_Blk p_vec = CVector_Blk::like(r_vec), z_vec = CVector_Blk::like(r_vec);
   z_vec = r_vec;
   precond.SolvePrecond(mat_p, z_vec);
   p_vec = z_vec;
@@ -714,7 +838,6 @@ inline bool bicgstab_loop(double& conv_ratio, unsigned& iteration, const CMatDia
                           CVector_Blk& u_vec, const CPrecond_Blk* precond) {
   const double tol = conv_ratio;
   const unsigned mx_iter = iteration;
-  const unsigned nblk = r_vec.NBlk(), len = r_vec.Len();
   u_vec.SetVectorZero();
   const double rr0 = r_vec * r_vec;
   if (rr0 < 1.0e-30) {
@@ -723,11 +846,12 @@ inline bool bicgstab_loop(double& conv_ratio, unsigned& iteration, const CMatDia
     return true;
   }
   const double inv_rr0 = 1.0 / rr0;
-  CVector_Blk s_vec(nblk, len), p_vec(nblk, len), AMp_vec(nblk, len), AMs_vec(nblk, len), r0_vec(nblk, len);
+  CVector_Blk s_vec = CVector_Blk::like(r_vec), p_vec = CVector_Blk::like(r_vec);
+  CVector_Blk AMp_vec = CVector_Blk::like(r_vec), AMs_vec = CVector_Blk::like(r_vec), r0_vec = CVector_Blk::like(r_vec);
   std::unique_ptr<CVector_Blk> Mp_vec, Ms_vec;
   if (precond) {
-    Mp_vec.reset(new CVector_Blk(nblk, len));
-    Ms_vec.reset(new CVector_Blk(nblk, len));
+    Mp_vec.reset(new CVector_Blk(CVector_Blk::like(r_vec)));
+    Ms_vec.reset(new CVector_Blk(CVector_Blk::like(r_vec)));
   }
   r0_vec = r_vec;
   p_vec = r_vec;
