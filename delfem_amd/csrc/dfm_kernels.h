@@ -96,15 +96,52 @@ __device__ inline double strided_sum(const double* __restrict__ p, int n) {
   return acc;
 }
 
+// strided_sum of ND arrays p[d][0..n) at once: every array's loads of a
+// round are issued before any is added (one memory round trip per round for
+// all of them, not one per array); each array's sum is formed in the same
+// order as strided_sum's.
+template <int NT, int ND, int U = 8>
+__device__ inline void strided_sums(const double* const (&p)[ND], const int (&n)[ND], double (&acc)[ND]) {
+  int nmax = 0;
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    acc[d] = 0.0;
+    nmax = n[d] > nmax ? n[d] : nmax;
+  }
+  for (int base = (int)threadIdx.x; base < nmax; base += U * NT) {
+    double v[ND][U];
+#pragma unroll
+    for (int d = 0; d < ND; d++)
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int i = base + u * NT;
+        v[d][u] = i < n[d] ? p[d][i] : 0.0;
+      }
+#pragma unroll
+    for (int d = 0; d < ND; d++)
+#pragma unroll
+      for (int u = 0; u < U; u++) acc[d] += v[d][u];
+  }
+}
+
 template <int ND, int FIN>
 __global__ void __launch_bounds__(kFinNT) k_finalize(const double* __restrict__ partials, int nparts,
                                                     KState* __restrict__ st) {
   if (FIN != FIN_STORE && st->done) return;
   __shared__ double sh[kFinNT / 64];
   double s[3] = {0.0, 0.0, 0.0};
+  {
+    const double* p[ND];
+    int n[ND];
+    double t[ND];
 #pragma unroll
-  for (int d = 0; d < ND; d++) {
-    s[d] = block_sum_n<kFinNT>(strided_sum<kFinNT>(partials + (size_t)d * nparts, nparts), sh);
+    for (int d = 0; d < ND; d++) {
+      p[d] = partials + (size_t)d * nparts;
+      n[d] = nparts;
+    }
+    strided_sums<kFinNT, ND>(p, n, t);
+#pragma unroll
+    for (int d = 0; d < ND; d++) s[d] = block_sum_n<kFinNT>(t[d], sh);
   }
   if (threadIdx.x != 0) return;
   if constexpr (FIN == FIN_STORE) {
@@ -1876,8 +1913,15 @@ static __global__ void __launch_bounds__(kFinNT) k_finalize_rrz(const double* __
                                                         KState* __restrict__ st) {
   if (st->done) return;
   __shared__ double sh[kFinNT / 64];
-  const double rr = block_sum_n<kFinNT>(strided_sum<kFinNT>(prr, nrr), sh);
-  const double rz = block_sum_n<kFinNT>(strided_sum<kFinNT>(prz, nrz), sh);
+  double t[2];
+  {
+    // both partial arrays in the same load rounds
+    const double* p[2] = {prr, prz};
+    const int n[2] = {nrr, nrz};
+    strided_sums<kFinNT, 2>(p, n, t);
+  }
+  const double rr = block_sum_n<kFinNT>(t[0], sh);
+  const double rz = block_sum_n<kFinNT>(t[1], sh);
   if (threadIdx.x != 0) return;
   st->rr = rr;
   if (rr * st->inv_rr0 < st->tol2) {
