@@ -187,8 +187,8 @@ def summarize_trace(args):
     rows = []
     with open(args.summarize_trace, newline="") as f:
         for row in csv.DictReader(f):
-            rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]),
-                         re.sub(r"\(.*", "", row["Kernel_Name"]).strip()))
+            name = row["Kernel_Name"].replace("(anonymous namespace)::", "")
+            rows.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]), re.sub(r"\(.*", "", name).strip()))
     rows.sort()
     for t0, t1, name in rows:
         per.setdefault(name, []).append((t1 - t0) / 1e3)
