@@ -637,6 +637,13 @@ static int g_tile_rows = [] {
   return v >= 1 && v <= kTileMaxRows ? v : 64;
 }();
 
+// Phases found for factors without a colour ordering (DFM_DISCOVER_PHASES=0:
+// off, for comparisons).
+static bool g_discover_phases = [] {
+  const char* e = getenv("DFM_DISCOVER_PHASES");
+  return !e || atoi(e) != 0;
+}();
+
 // dfm_ilu_create; with given_row / given_col (natural order only) the factor
 // pattern is that one instead of the ILU(k) symbolic of A's: it must hold
 // A's pattern, and its extra entries start at zero
@@ -756,11 +763,16 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
     }
     DFM_TRY(M->d_rzmaskL.upload(mask));
   }
-  // phase-tiled sweep layouts, for factors under a colour ordering
+  // phase-tiled sweep layouts: phases are the colours of a colour ordering,
+  // else found in the factor's own numbering (a mesh the caller numbered by
+  // colours of tiles) where they beat the level sets
   TilePlan tpL, tpU;
-  const bool tiled = M->phase_ptr.size() >= 2 &&
-                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, M->phase_ptr, false, g_tile_rows, (int)b, tpL) &&
-                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, M->phase_ptr, true, g_tile_rows, (int)b, tpU);
+  std::vector<int> phases = M->phase_ptr;
+  if (phases.size() < 2 && g_discover_phases)
+    discover_phases(nb, M->frow, M->fcol, M->fdiaind, std::max(M->nlev_lower, M->nlev_upper), phases);
+  const bool tiled = phases.size() >= 2 &&
+                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, phases, false, g_tile_rows, (int)b, tpL) &&
+                     build_tile_plan(nb, M->frow, M->fcol, M->fdiaind, phases, true, g_tile_rows, (int)b, tpU);
   // permuted copy of A: rows in factor numbering, columns renumbered and sorted
   {
     std::vector<unsigned> beg(nb), end(nb), colv(A->col.size());

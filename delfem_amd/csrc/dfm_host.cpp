@@ -601,4 +601,96 @@ void build_tile_split(const TilePlan& tp, int b, TileSplit& ts) {
   }
 }
 
+
+// Phases for the tiled sweeps of a factor without a colour ordering.  Rows are
+// scanned in factor order, each joined by union-find to the rows of the
+// current phase it depends on (lower entries of the row and upper entries of
+// earlier rows that name it, so that one partition serves both sweeps); a new
+// phase starts at the row that would make a component exceed kTileMaxRows
+// rows or whose local level would reach the depth cap.  Caps are tried from
+// none down to 16 and the cheapest estimate kept: per phase a launch's fixed
+// cost and its deepest local level (from the measured tiled sweeps), against
+// one launch per level set.  Coloured and tile-numbered meshes give a few
+// phases; lexicographic numberings give long chains and keep the level sets.
+bool discover_phases(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
+                     const std::vector<unsigned>& fdiaind, int nlev, std::vector<int>& phase_ptr) {
+  constexpr double kPhaseUs = 3.0, kTileLevelUs = 0.45, kLevelSetUs = 5.0;
+  phase_ptr.clear();
+  if (nb < 2 || nlev < 8) return false;
+  // earlier rows each row depends on or is depended on by
+  std::vector<unsigned> aptr(nb + 1, 0);
+  for (unsigned i = 0; i < nb; i++) {
+    aptr[i + 1] += fdiaind[i] - frow[i];
+    for (unsigned q = fdiaind[i]; q < frow[i + 1]; q++) aptr[fcol[q] + 1]++;
+  }
+  for (unsigned i = 0; i < nb; i++) aptr[i + 1] += aptr[i];
+  std::vector<unsigned> adj(aptr[nb]), fillp(aptr.begin(), aptr.end() - 1);
+  for (unsigned i = 0; i < nb; i++) {
+    for (unsigned q = frow[i]; q < fdiaind[i]; q++) adj[fillp[i]++] = fcol[q];
+    for (unsigned q = fdiaind[i]; q < frow[i + 1]; q++) adj[fillp[fcol[q]]++] = i;
+  }
+  std::vector<int> parent(nb), size(nb), lvl(nb), mark(nb);
+  auto find = [&](int x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  double best = 0.5 * nlev * kLevelSetUs;  // tiles must at least halve the estimate
+  bool found = false;
+  std::vector<int> ptr;
+  for (int dcap : {0, 32, 24, 16}) {
+    for (unsigned i = 0; i < nb; i++) {
+      parent[i] = (int)i;
+      size[i] = 1;
+      mark[i] = -1;
+    }
+    ptr.assign(1, 0);
+    unsigned p0 = 0;
+    int depth = 0;
+    double cost = 0.0;
+    for (unsigned i = 0; i < nb && cost < best; i++) {
+      int tot = 1, l = 0;
+      for (unsigned q = aptr[i]; q < aptr[i + 1]; q++) {
+        const unsigned j = adj[q];
+        if (j < p0) continue;
+        l = std::max(l, lvl[j] + 1);
+        const int r = find((int)j);
+        if (mark[r] != (int)i) {
+          mark[r] = (int)i;
+          tot += size[r];
+        }
+      }
+      if (tot > kTileMaxRows || (dcap > 0 && l >= dcap)) {
+        cost += kPhaseUs + kTileLevelUs * depth;
+        ptr.push_back((int)i);
+        p0 = i;
+        depth = 0;
+        l = 0;
+      } else {
+        for (unsigned q = aptr[i]; q < aptr[i + 1]; q++) {
+          const unsigned j = adj[q];
+          if (j < p0) continue;
+          const int a = find((int)i), r = find((int)j);
+          if (a == r) continue;
+          const int big = size[a] >= size[r] ? a : r, small = big == a ? r : a;
+          parent[small] = big;
+          size[big] += size[small];
+        }
+      }
+      lvl[i] = l;
+      depth = std::max(depth, l + 1);
+    }
+    cost += kPhaseUs + kTileLevelUs * depth;
+    if (cost < best) {
+      best = cost;
+      ptr.push_back((int)nb);
+      phase_ptr = ptr;
+      found = true;
+    }
+  }
+  return found;
+}
+
 }  // namespace dfm

@@ -174,6 +174,11 @@ constexpr int kTileOverflowBytes = 48 * 1024;
 bool build_tile_plan(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
                      const std::vector<unsigned>& fdiaind, const std::vector<int>& phase_ptr, bool upper,
                      int pack_rows, int b, TilePlan& tp);
+// Phase boundaries (row offsets, first 0, last nb) for the tiled sweeps of a
+// factor without a colour ordering; false when level-set sweeps (nlev level
+// sets) are estimated to be as fast.
+bool discover_phases(unsigned nb, const std::vector<unsigned>& frow, const std::vector<unsigned>& fcol,
+                     const std::vector<unsigned>& fdiaind, int nlev, std::vector<int>& phase_ptr);
 
 // Sliced-ELL layout plan: rows grouped into "groups" (level sets, or one group
 // for a plain SpMV), each group cut into slices of kSlice rows.  For slice s

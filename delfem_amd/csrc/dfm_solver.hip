@@ -170,8 +170,8 @@ static int env_int(const char* name, int dflt) {
 }
 static int g_persist_mode = env_int("DFM_PERSIST", 1);      // dfm_set_sweep_mode
 static int g_persist_grid = env_int("DFM_PERSIST_GRID", 0);
-static bool use_tile(const TileSweep* ts, const SweepSync* ps, const KState* st) {
-  if (!ts || !st || ts->ntile == 0) return false;
+static bool use_tile(const TileSweep* ts, const SweepSync* ps, const KState*) {
+  if (!ts || ts->ntile == 0) return false;
   return g_persist_mode >= 3 || (g_persist_mode == 1 && ps && ps->ngroup >= kPersistMinLevels);
 }
 static bool use_persist(const SweepSync* ps, const KState* st, const TileSweep* ts = nullptr) {
@@ -316,18 +316,24 @@ static int sweep_upper(const Sell<TS>& U, const Sell<double>& plan, T* vec, cons
 static int lower_nparts(const Sell<double>& plan) { return upper_nparts(plan); }
 
 int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper) {
+  // level sets, or the phase tiles where the factor has them (bitwise the same)
+  const SweepSync *psL = &M->syncL, *psU = &M->syncU;
+  const TileSweep *tsL = &M->tileL, *tsU = &M->tileU;
   switch (M->b) {
     case 1:
-      if (lower) DFM_TRY((sweep_lower<1, double, double>(M->L64, M->L64, vec, vec, nullptr)));
-      if (upper) DFM_TRY((sweep_upper<1, double, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr)));
+      if (lower) DFM_TRY((sweep_lower<1, double, double>(M->L64, M->L64, vec, vec, nullptr, psL, tsL)));
+      if (upper)
+        DFM_TRY((sweep_upper<1, double, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, psU, tsU)));
       break;
     case 2:
-      if (lower) DFM_TRY((sweep_lower<2, double, double>(M->L64, M->L64, vec, vec, nullptr)));
-      if (upper) DFM_TRY((sweep_upper<2, double, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr)));
+      if (lower) DFM_TRY((sweep_lower<2, double, double>(M->L64, M->L64, vec, vec, nullptr, psL, tsL)));
+      if (upper)
+        DFM_TRY((sweep_upper<2, double, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, psU, tsU)));
       break;
     case 3:
-      if (lower) DFM_TRY((sweep_lower<3, double, double>(M->L64, M->L64, vec, vec, nullptr)));
-      if (upper) DFM_TRY((sweep_upper<3, double, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr)));
+      if (lower) DFM_TRY((sweep_lower<3, double, double>(M->L64, M->L64, vec, vec, nullptr, psL, tsL)));
+      if (upper)
+        DFM_TRY((sweep_upper<3, double, double, false>(M->U64, M->U64, vec, nullptr, nullptr, nullptr, psU, tsU)));
       break;
     default:
       set_error("block size %u: the device substitutions support 1x1, 2x2 and 3x3 blocks", M->b);

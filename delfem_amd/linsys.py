@@ -314,10 +314,11 @@ def Solve_PBiCGSTAB(conv_ratio, iteration, A, prec, r, x, precision="fp64", **op
 
 
 def set_sweep_mode(mode, grid=0):
-    """Level sets of the solvers' triangular sweeps: mode 0 one launch per
-    level; 1 (default) for factors of 8 or more levels one launch per colour
-    (dependent rows tiled into workgroups) where the ordering allows, else one
-    persistent launch; 2 persistent always; 3 per colour wherever possible;
+    """Level sets of the triangular sweeps: mode 0 one launch per level; 1
+    (default) for factors of 8 or more levels one launch per phase (dependent
+    rows tiled into workgroups; phases are the colours of a colour ordering,
+    or found in a natural-order factor's numbering), else in the solvers one
+    persistent launch; 2 persistent always; 3 per phase wherever possible;
     4 as 3 with one lane per block row of 2x2 / 3x3 blocks (3: one per tile
     row); grid = workgroups of a persistent launch (0: automatic)."""
     call("dfm_set_sweep_mode", int(mode), int(grid))

@@ -256,18 +256,23 @@ typedef struct {
  * opts may be NULL (FP64).  Returns DFM_OK (converged) or DFM_NOT_CONVERGED. */
 int dfm_solve_pcg(dfm_bcrs *A, dfm_ilu *M, double *r_dev, double *x_dev,
                   double *conv_ratio, unsigned *iteration, dfm_solve_opts *opts);
-/* How the solvers run the level sets of a triangular sweep (no reference
- * counterpart; a tuning knob): mode 0 one launch per level; 1 (default) for
- * factors of 8 or more levels one launch per colour of a multicolour or
- * block-multicolour ordering (tiles of dependent rows run level by level
- * inside one workgroup) where every tile fits 256 rows, else one persistent
- * launch; 2 persistent always; 3 per colour wherever possible (per level
- * elsewhere); 4 as 3 with one lane per block row of 2x2 / 3x3 blocks instead of
- * one per tile row (slower here; kept as an alternative).  grid = workgroups of a persistent launch (0: twice the slices
- * of the largest level, at least 64; at most 4096).  FP64 sweeps give the
- * same bits in every mode.  Process-wide; initial values from DFM_PERSIST /
- * DFM_PERSIST_GRID (DFM_TILE_ROWS, read at factor creation: rows a tile is
- * filled to, default 64). */
+/* How the solvers and the substitutions run the level sets of a triangular
+ * sweep (no reference counterpart; a tuning knob): mode 0 one launch per
+ * level; 1 (default) for factors of 8 or more levels one launch per phase
+ * (tiles of dependent rows run level by level inside one workgroup) where
+ * every tile fits 256 rows, else, in the solvers, one persistent launch.
+ * Phases are the colours of a multicolour or block-multicolour ordering; a
+ * factor in natural or given order has phases found in its own numbering
+ * when they are estimated to beat its level sets (a mesh numbered by colours
+ * of tiles, a small system: one 256-row tile per phase).  2 persistent
+ * always; 3 per phase wherever possible (per level elsewhere); 4 as 3 with
+ * one lane per block row of 2x2 / 3x3 blocks instead of one per tile row
+ * (slower here; kept as an alternative).  grid = workgroups of a persistent
+ * launch (0: twice the slices of the largest level, at least 64; at most
+ * 4096).  FP64 sweeps give the same bits in every mode.  Process-wide;
+ * initial values from DFM_PERSIST / DFM_PERSIST_GRID (read at factor
+ * creation: DFM_TILE_ROWS, rows a tile is filled to, default 64, and
+ * DFM_DISCOVER_PHASES=0, no phases found in natural or given order). */
 int dfm_set_sweep_mode(int mode, int grid);
 /* In-solve timing of the Krylov iteration's SpMV (no reference counterpart;
  * a measurement aid for the roofline figure).  on != 0 starts recording:

@@ -1,18 +1,19 @@
 // host_check.cpp -- the library's host-side integer setup (dfm_host.cpp)
 // built on its own, for runs under AddressSanitizer / UndefinedBehavior-
 // Sanitizer on the CPU: FE pattern, orderings, ILU(k) symbolic, level sets,
-// sliced-ELL plans and phase-tiled sweep plans, with the tile plans' layout
-// invariants checked here and the patterns written out for comparison with
-// the oracle.
+// sliced-ELL plans, phases found in a natural-order factor and phase-tiled
+// sweep plans, with the tile plans' layout invariants checked here and the
+// patterns written out for comparison with the oracle.
 //
 // usage: host_check <in> <out>
 //   in : int32 nnode, nelem, npe, lev, ordering (0 natural, 2 multicolour,
 //        3 block multicolour), block_nodes, pack_rows, block size b;
 //        uint32 conn[nelem*npe]
 //   out: int32 status (0 ok), nnz, nf, ncolour, nlev_lower, nlev_upper,
-//        tile threads L / U, tiles L / U; uint32 rowptr[nnode+1], col[nnz];
-//        int32 order[nnode]; uint32 frow[nnode+1], fcol[nf]; int32 flev[nf];
-//        int32 phase_ptr[ncolour+1]
+//        tile threads L / U, tiles L / U, nphase (phases found, natural
+//        order); uint32 rowptr[nnode+1], col[nnz]; int32 order[nnode];
+//        uint32 frow[nnode+1], fcol[nf]; int32 flev[nf];
+//        int32 phase_ptr[ncolour+1 or nphase+1]
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -258,7 +259,12 @@ int main(int argc, char** argv) {
     build_sell_plan(rows, gptr, beg, end, fcol, nullptr, plan);
     EXPECT(plan.col.size() == (size_t)plan.nk * kSlice, "sell plan size");
   }
-  int ntL = 0, ntU = 0, tiL = 0, tiU = 0;
+  int ntL = 0, ntU = 0, tiL = 0, tiU = 0, nphase = 0;
+  if (ordering == DFM_ORDER_NATURAL && discover_phases(nnode, frow, fcol, fdia, std::max(nlL, nlU), phase_ptr)) {
+    nphase = (int)phase_ptr.size() - 1;
+    EXPECT(nphase >= 1 && phase_ptr[0] == 0 && phase_ptr[nphase] == (int)nnode, "phase bounds");
+    for (int p = 0; p < nphase; p++) EXPECT(phase_ptr[p] < phase_ptr[p + 1], "empty phase %d", p);
+  }
   if (phase_ptr.size() >= 2) {
     TilePlan tL, tU;
     if (build_tile_plan(nnode, frow, fcol, fdia, phase_ptr, false, pack_rows, bsize, tL)) {
@@ -276,15 +282,16 @@ int main(int argc, char** argv) {
   }
   FILE* o = fopen(argv[2], "wb");
   if (!o) return 2;
-  const int out_hdr[10] = {g_fail ? 1 : 0, (int)col.size(), (int)fcol.size(), ncolour, nlL, nlU, ntL, ntU, tiL, tiU};
-  fwrite(out_hdr, sizeof(int), 10, o);
+  const int out_hdr[11] = {g_fail ? 1 : 0, (int)col.size(), (int)fcol.size(), ncolour, nlL, nlU, ntL, ntU, tiL, tiU,
+                           nphase};
+  fwrite(out_hdr, sizeof(int), 11, o);
   fwrite(rowptr.data(), sizeof(unsigned), rowptr.size(), o);
   fwrite(col.data(), sizeof(unsigned), col.size(), o);
   fwrite(order.data(), sizeof(int), order.size(), o);
   fwrite(frow.data(), sizeof(unsigned), frow.size(), o);
   fwrite(fcol.data(), sizeof(unsigned), fcol.size(), o);
   fwrite(flev.data(), sizeof(int), flev.size(), o);
-  if (ncolour) fwrite(phase_ptr.data(), sizeof(int), phase_ptr.size(), o);
+  if (ncolour || nphase) fwrite(phase_ptr.data(), sizeof(int), phase_ptr.size(), o);
   fclose(o);
   if (g_fail) fprintf(stderr, "%d checks failed\n", g_fail);
   return g_fail ? 1 : 0;

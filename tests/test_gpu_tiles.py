@@ -111,3 +111,63 @@ def test_kernel_timer_runs_tiled(system, gpu):
             ms, nbytes = C.c_double(), C.c_double()
             _lib.call("dfm_bench_kernel", s.A.h, M.h, which, 0, 5, C.byref(ms), C.byref(nbytes))
             assert ms.value > 0 and nbytes.value > 0
+
+
+# Natural-order factors whose phases are found in the factor's own numbering:
+# a mesh numbered by colours of geometric tiles, and a small lexicographic one
+# (whose phases are 256-row tiles, one workgroup each).
+FOUND = {
+    "oseen2d_32_tiles_ilu1": (lambda: meshgen.oseen2d(32, seed=3), True, 1),
+    "poisson2d_40_tiles_ilu2": (lambda: meshgen.poisson2d(40, seed=2), True, 2),
+    "solid3d_6_tiles_ilu0": (lambda: meshgen.solid3d(6, seed=2, sigma=0.5), True, 0),
+    "poisson2d_20_lex_ilu2": (lambda: meshgen.poisson2d(20, seed=1), False, 2),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FOUND))
+def test_found_phases(orc, gpu, name):
+    """The found phases' tile sweeps equal the oracle's substitutions bit for
+    bit; BiCGSTAB gives the same bits with tiles and with one launch per
+    level; PCG within one iteration and 1e-8 of the oracle."""
+    import torch
+    from oracle.oracle import assemble_problem
+    _case(f"found phases {name}")
+    make, tiles, lev = FOUND[name]
+    prob = make()
+    if tiles:
+        prob = meshgen.relabel(prob, meshgen.tile_colour_order(prob, 16))
+    ref = assemble_problem(orc, prob)
+    s = gpu.FEMSystem(prob)
+    s.assemble()
+    M = gpu.CPreconditioner_ILU(s.A, lev)
+    ti, info = M.tile_info(), M.info()
+    print(name, ti, info, flush=True)
+    assert ti["nphase"] >= 1 and ti["ntile_lower"] > 0 and ti["ntile_upper"] > 0
+    assert ti["nphase"] <= max(info["nlevel_lower"], info["nlevel_upper"]) // 4
+    assert M.SetValue(s.A)
+    F = orc.iluk_symbolic(ref["rowptr"], ref["col"], lev, None)
+    ov, od, ns = orc.ilu_values(prob.nnode, prob.b, ref["rowptr"], ref["col"], ref["val"], ref["dia"], F, None)
+    fval, fdia = M.values()
+    assert ns == 0 and np.array_equal(fval, ov) and np.array_equal(fdia, od)
+    v = np.random.default_rng(5).standard_normal(prob.ndof)
+    for mode in (1, 0):
+        gpu.set_sweep_mode(mode, 0)
+        vd = torch.from_numpy(v.copy()).cuda()
+        M.ForwardSubstitution(vd)
+        w = orc.forward(prob.nnode, prob.b, F, ov, od, v)
+        assert np.array_equal(vd.cpu().numpy(), w), mode
+        M.BackwardSubstitution(vd)
+        w = orc.backward(prob.nnode, prob.b, F, ov, od, w)
+        assert np.array_equal(vd.cpu().numpy(), w), mode
+    gpu.set_sweep_mode(0, 0)
+    base = _solve(gpu, s, M, "bicg", "fp64")
+    gpu.set_sweep_mode(1, 0)
+    got = _solve(gpu, s, M, "bicg", "fp64")
+    assert got[:3] == base[:3] and np.array_equal(got[3], base[3]) and np.array_equal(got[4], base[4])
+    if prob.kind != "oseen2d":
+        ok_o, conv_o, it_o, x_o, _ = orc.solve("pcg", prob.nnode, prob.b, ref["rowptr"], ref["col"], ref["val"],
+                                               ref["dia"], F, ov, od, ref["res"], 1e-10, 5000, None)
+        ok, conv, it, x, _ = _solve(gpu, s, M, "pcg", "fp64")
+        print(f"  pcg {it} iterations, oracle {it_o}", flush=True)
+        assert ok and ok_o and abs(it - it_o) <= 1
+        assert np.linalg.norm(x - x_o) <= 1e-8 * np.linalg.norm(x_o)

@@ -1,6 +1,7 @@
 """The library's host-side integer setup (delfem_amd/csrc/dfm_host.cpp: FE
 pattern, multicolour and block-multicolour orderings, ILU(k) symbolic, level
-sets, sliced-ELL plans, phase-tiled sweep plans) built with g++ under
+sets, sliced-ELL plans, phases found in natural-order factors, phase-tiled
+sweep plans) built with g++ under
 AddressSanitizer and UndefinedBehaviorSanitizer (tests/cpp/host_check.cpp),
 run on small meshes: no sanitizer report, the tile plans' layout invariants
 hold (every row once, in-phase dependencies at lower local levels inside the
@@ -48,15 +49,15 @@ def run(exe, tmp_path, prob, lev, ordering, block_nodes=16, pack_rows=64):
     r = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, env=env, timeout=300)
     assert r.returncode == 0 and "runtime error" not in r.stderr and "Sanitizer" not in r.stderr, r.stderr[-4000:]
     raw = np.fromfile(out, dtype=np.int32)
-    status, nnz, nf, ncol, nlL, nlU, ntL, ntU, tiL, tiU = (int(v) for v in raw[:10])
+    status, nnz, nf, ncol, nlL, nlU, ntL, ntU, tiL, tiU, nph = (int(v) for v in raw[:11])
     n = prob.nnode
-    parts, o = {}, 10
+    parts, o = {}, 11
     for name, size in (("rowptr", n + 1), ("col", nnz), ("order", n), ("frow", n + 1), ("fcol", nf), ("flev", nf),
-                       ("phase_ptr", ncol + 1 if ncol else 0)):
+                       ("phase_ptr", ncol + 1 if ncol else (nph + 1 if nph else 0))):
         parts[name] = raw[o:o + size]
         o += size
     assert o == raw.size
-    parts.update(status=status, ncolour=ncol, nlev=(nlL, nlU), nt=(ntL, ntU), ntile=(tiL, tiU))
+    parts.update(status=status, ncolour=ncol, nlev=(nlL, nlU), nt=(ntL, ntU), ntile=(tiL, tiU), nphase=nph)
     return parts
 
 
@@ -103,3 +104,34 @@ def test_host_setup_sanitized(host_check, tmp_path, orc, case):
         if pack == 1:
             # one component per tile: tiles = components >= rows / largest component
             assert got["ntile"][0] >= prob.nnode // got["nt"][0]
+
+
+# natural-order factors: a large lexicographic numbering keeps its level sets
+# (its phases would be long chains); a small one runs as a few 256-row tiles,
+# one workgroup each, instead of one launch per level; a mesh numbered by
+# colours of geometric tiles gives a few phases, far below the level count.
+# Wherever phases are found, both sweeps have valid tile plans.
+@pytest.mark.parametrize("name,make,lev,tiles", [
+    ("poisson2d_160_lex", lambda: meshgen.poisson2d(160, seed=1), 0, False),
+    ("poisson2d_20_lex", lambda: meshgen.poisson2d(20, seed=1), 2, True),
+    ("oseen2d_32_lex", lambda: meshgen.oseen2d(32, seed=3), 1, True),
+    ("oseen2d_32_tiles", lambda: meshgen.oseen2d(32, seed=3), 1, True),
+    ("poisson2d_40_tiles", lambda: meshgen.poisson2d(40, seed=2), 2, True),
+])
+def test_phases_found(host_check, tmp_path, orc, name, make, lev, tiles):
+    prob = make()
+    if name.endswith("tiles"):
+        prob = meshgen.relabel(prob, meshgen.tile_colour_order(prob, 16))
+    got = run(host_check, tmp_path, prob, lev, "natural", 16, 64)
+    print(name, got["nlev"], got["nphase"], got["phase_ptr"], got["ntile"], flush=True)
+    assert got["status"] == 0
+    rowptr, col = orc.fem_pattern(prob.nnode, prob.conn)
+    F = orc.iluk_symbolic(rowptr, col, lev, None)
+    assert np.array_equal(got["frow"], F["rowptr"]) and np.array_equal(got["fcol"], F["col"])
+    if not tiles:
+        assert got["nphase"] == 0 and got["ntile"] == (0, 0)
+        return
+    pp = got["phase_ptr"]
+    assert 1 <= got["nphase"] <= max(got["nlev"]) // 4
+    assert pp[0] == 0 and pp[-1] == prob.nnode and np.all(np.diff(pp) > 0)
+    assert got["ntile"][0] > 0 and got["ntile"][1] > 0
