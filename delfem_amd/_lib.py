@@ -150,6 +150,7 @@ _SIGS = {
     "dfm_mseg_ilu_precond": (C.c_int, [H, vp]),
     "dfm_mseg_solve_pcg": (C.c_int, [H, H, vp, vp, f64p, u32p]),
     "dfm_mseg_solve_pbicgstab": (C.c_int, [H, H, vp, vp, f64p, u32p]),
+    "dfm_mseg_set_device_loop": (C.c_int, [C.c_int]),
     "dfm_zbcrs_create": (C.c_int, [C.c_uint, C.c_uint, C.POINTER(H)]),
     "dfm_zbcrs_destroy": (C.c_int, [H]),
     "dfm_zbcrs_add_pattern": (C.c_int, [H, C.c_uint, vp, vp]),

@@ -807,6 +807,7 @@ int zrun_block(ZCtx& c, const std::string& key, int nblock, Body&& body) {
       DFM_HIP(hipGraphLaunch(g.second, s));
       return DFM_OK;
     }
+  DFM_TRY(evict_graph(c.graphs));
   hipGraph_t graph = nullptr;
   DFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = DFM_OK;

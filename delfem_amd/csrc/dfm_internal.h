@@ -25,6 +25,12 @@ struct SyncDefer {
   ~SyncDefer() { --g_sync_defer; }
 };
 int sync_stream();  // hipStreamSynchronize unless deferred
+std::string sweep_mode_key();  // the triangular sweeps' mode, as part of a graph's key
+// Graph caches (per factor) keep the iteration blocks of at most kMaxGraphs
+// operand sets; before a new capture the oldest goes, once the stream has
+// drained (a replay of it may still run).
+constexpr size_t kMaxGraphs = 8;
+int evict_graph(std::vector<std::pair<std::string, hipGraphExec_t>>& graphs);
 int ensure_init();
 // Value versions, unique across all matrices (real and complex).
 unsigned long long next_version();

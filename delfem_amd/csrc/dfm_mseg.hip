@@ -47,7 +47,12 @@ struct dfm_msilu_s {
   std::vector<std::vector<std::vector<unsigned>>> src_col;  // the blocks' patterns the factor was built on
   std::vector<std::vector<DevBuf<long long>>> src;  // factor entry -> block entry, -1 for fill
   std::vector<double*> work;                        // Krylov vectors
+  DevBuf<unsigned char> dstate;                     // device-driven loops: their scalars (MsState)
+  void* hstate = nullptr;                           // pinned copy, polled
+  std::vector<std::pair<std::string, hipGraphExec_t>> graphs;  // their iteration blocks, by operand set
   ~dfm_msilu_s() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    if (hstate) (void)hipHostFree(hstate);
     for (auto* m : dia)
       if (m) dfm_ilu_destroy(m);
     for (auto& row : F)
@@ -773,9 +778,6 @@ static int ms_solve_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double*
   *conv_ratio = std::sqrt(rr * inv_rr0);
   return DFM_NOT_CONVERGED;
 }
-int dfm_mseg_solve_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration) {
-  return deferred([&] { return ms_solve_pcg(S, M, r, x, conv_ratio, iteration); });
-}
 
 // LsSol::Solve_PBiCGSTAB (solver_ls_iter.cpp:291-408) on the segments.
 // Returns DFM_OK when converged, DFM_NOT_CONVERGED at max_iter (the
@@ -840,7 +842,360 @@ static int ms_solve_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, d
   }
   return DFM_NOT_CONVERGED;
 }
+// ---------------------------------------------------------------------------
+// Device-driven loops: the same operations in the same order, with the
+// scalars (alpha, beta, omega and the convergence test) formed on the device
+// from the segments' sums (added in segment order, as DOT does) and the host
+// only enqueueing: blocks of kMsPoll iterations, each replayed from a hipGraph
+// captured once per operand set, with the stop flag polled after each block.
+// Updates of x, r and p are skipped once the flag is set (converged, or
+// max_iter reached), so the iterations of a block after that change nothing
+// the caller sees: the results are bitwise those of the host-driven loops.
+namespace {
+
+struct MsState {
+  double inv_rr0, tol2, rz, alpha, beta, omega, r_r2, den2, rr, c_bo, one;
+  int done;  // 0 running, 1 converged, 2 stopped at max_iter
+  int iter, maxit, conv_iter;
+};
+constexpr int kMsPoll = 4;
+
+enum { MS_RZ0, MS_PAP, MS_PCG_RR, MS_RZNEW, MS_RR2, MS_DEN, MS_DEN2, MS_OMEGA, MS_BICG_RR, MS_BETA };
+
+// s = the segments' sums in segment order, then one step of the loop's scalar
+// arithmetic (written as the host loops write it)
+template <int OP>
+__global__ void k_ms_scalar(const KState* __restrict__ seg, int nseg, MsState* __restrict__ st) {
+  double s = 0.0;
+  for (int i = 0; i < nseg; i++) s += seg[i].d0;
+  if (st->done) return;
+  if constexpr (OP == MS_RZ0) {
+    st->rz = s;
+  } else if constexpr (OP == MS_PAP) {
+    st->alpha = st->rz / s;
+  } else if constexpr (OP == MS_PCG_RR) {
+    st->rr = s;
+    if (s * st->inv_rr0 < st->tol2) {
+      st->done = 1;
+      st->conv_iter = st->iter;
+    }
+    st->iter += 1;
+    if (!st->done && st->iter >= st->maxit) st->done = 2;
+  } else if constexpr (OP == MS_RZNEW) {
+    st->beta = s / st->rz;
+    st->rz = s;
+  } else if constexpr (OP == MS_RR2) {
+    st->r_r2 = s;
+  } else if constexpr (OP == MS_DEN) {
+    st->alpha = st->r_r2 / s;
+  } else if constexpr (OP == MS_DEN2) {
+    st->den2 = s;
+  } else if constexpr (OP == MS_OMEGA) {
+    st->omega = s / st->den2;
+  } else if constexpr (OP == MS_BICG_RR) {
+    st->rr = s;
+    if (s * st->inv_rr0 < st->tol2) {
+      st->done = 1;
+      st->conv_iter = st->iter;
+    }
+  } else if constexpr (OP == MS_BETA) {
+    st->beta = s * st->alpha / (st->r_r2 * st->omega);
+    st->c_bo = -st->beta * st->omega;
+    st->iter += 1;
+    if (st->iter >= st->maxit) st->done = 2;
+  }
+}
+// y += sign * a x; nothing when that factor is zero (CLinearSystem::AXPY,
+// linearsystem.cpp:302) or, with a stop flag, once it is set
+__global__ void k_ms_axpy(long n, const double* __restrict__ a, double sign, const double* __restrict__ x,
+                          double* __restrict__ y, const int* __restrict__ done) {
+  if (done && *done) return;
+  const double c = sign * *a;
+  if (c == 0.0) return;
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] += c * x[i];
+}
+__global__ void k_ms_scal(long n, const double* __restrict__ a, double* __restrict__ x,
+                          const int* __restrict__ done) {
+  if (*done) return;
+  const double c = *a;
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) x[i] *= c;
+}
+__global__ void k_ms_copy(long n, const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ done) {
+  if (done && *done) return;
+  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] = x[i];
+}
+
+struct MsLoop {
+  dfm_mseg_s* S;
+  dfm_msilu_s* M;
+  MsState* st;
+  size_t n;
+  int g;
+  // the segments' dot sums into dot_state, then the scalar step OP
+  template <int OP>
+  int dot(const double* x, const double* y) {
+    const unsigned ns = S->nseg;
+    int gmax = 1;
+    for (unsigned i = 0; i < ns; i++) gmax = std::max(gmax, grid_vec((long)(S->off[i + 1] - S->off[i])));
+    for (unsigned i = 0; i < ns; i++) {
+      const int gi = grid_vec((long)(S->off[i + 1] - S->off[i]));
+      double* part = S->dot_partials.p + (size_t)i * gmax;
+      DFM_MLAUNCH(k_dot<double>, gi, (long)(S->off[i + 1] - S->off[i]), x + S->off[i], y + S->off[i], part);
+      hipLaunchKernelGGL((k_finalize<1, FIN_STORE>), dim3(1), dim3(kFinNT), 0, stream(), part, gi,
+                         S->dot_state.p + i);
+      DFM_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL((k_ms_scalar<OP>), dim3(1), dim3(1), 0, stream(), S->dot_state.p, (int)ns, st);
+    DFM_HIP(hipGetLastError());
+    return DFM_OK;
+  }
+  int axpy(const double* a, double sign, const double* x, double* y) {
+    DFM_MLAUNCH(k_ms_axpy, g, (long)n, a, sign, x, y, &st->done);
+    return DFM_OK;
+  }
+  int scal(const double* a, double* x) {
+    DFM_MLAUNCH(k_ms_scal, g, (long)n, a, x, &st->done);
+    return DFM_OK;
+  }
+  int copy_gated(const double* x, double* y) {
+    DFM_MLAUNCH(k_ms_copy, g, (long)n, x, y, &st->done);
+    return DFM_OK;
+  }
+  int copy(const double* x, double* y) {  // a kernel, also inside graphs
+    DFM_MLAUNCH(k_ms_copy, g, (long)n, x, y, (const int*)nullptr);
+    return DFM_OK;
+  }
+  // the state's host copy after the work enqueued so far
+  int poll(MsState& h) {
+    DFM_HIP(hipMemcpyAsync(M->hstate, st, sizeof(MsState), hipMemcpyDeviceToHost, stream()));
+    DFM_HIP(hipStreamSynchronize(stream()));
+    std::memcpy(&h, M->hstate, sizeof(MsState));
+    return DFM_OK;
+  }
+};
+
+int ms_loop_init(dfm_mseg_s* S, dfm_msilu_s* M, double rr0, double tol, unsigned maxit, MsLoop& L) {
+  if (M->dstate.n < sizeof(MsState)) DFM_TRY(M->dstate.alloc(sizeof(MsState)));
+  if (!M->hstate) DFM_HIP(hipHostMalloc(&M->hstate, sizeof(MsState), hipHostMallocDefault));
+  MsState h{};
+  h.inv_rr0 = 1.0 / rr0;
+  h.tol2 = tol * tol;
+  h.one = 1.0;
+  h.maxit = (int)std::min<unsigned>(maxit, 0x7fffffffu);
+  std::memcpy(M->hstate, &h, sizeof(MsState));
+  L.S = S;
+  L.M = M;
+  L.st = reinterpret_cast<MsState*>(M->dstate.p);
+  L.n = S->off[S->nseg];
+  L.g = grid_vec((long)L.n);
+  DFM_HIP(hipMemcpyAsync(L.st, M->hstate, sizeof(MsState), hipMemcpyHostToDevice, stream()));
+  return DFM_OK;
+}
+
+// Operand set of a captured block: the vectors, the loop's buffers, every
+// block's and factor's storage it reads, and the sweep mode.
+std::string ms_graph_key(const char* tag, const MsLoop& L, std::initializer_list<const void*> ptrs) {
+  std::string k = std::string(tag) + ":" + std::to_string(kMsPoll) + ":" + sweep_mode_key();
+  char buf[24];
+  auto add = [&](const void* p) {
+    snprintf(buf, sizeof buf, ":%p", p);
+    k += buf;
+  };
+  for (const void* p : ptrs) add(p);
+  add(L.S->dot_partials.p);
+  add(L.S->dot_state.p);
+  add(L.st);
+  for (unsigned i = 0; i < L.S->nseg; i++) {
+    add(L.S->dia[i]);
+    add(L.M->dia[i]);
+    for (unsigned j = 0; j < L.S->nseg; j++) {
+      add(L.S->nd[i][j]);
+      add(L.M->F[i][j]);
+    }
+  }
+  return k;
+}
+// One block of kMsPoll iterations: replayed from the cached graph, captured
+// on first use, launched directly where the stream cannot be captured.
+template <class Body>
+int ms_run_block(dfm_msilu_s* M, const std::string& key, Body&& body) {
+  hipStream_t s = stream();
+  static const bool use_graphs = [] {
+    const char* e = getenv("DFM_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  static bool capture_ok = true;
+  if (s == nullptr || !use_graphs || !capture_ok) {
+    for (int i = 0; i < kMsPoll; i++) DFM_TRY(body());
+    return DFM_OK;
+  }
+  for (auto& g : M->graphs)
+    if (g.first == key) {
+      DFM_HIP(hipGraphLaunch(g.second, s));
+      return DFM_OK;
+    }
+  DFM_TRY(evict_graph(M->graphs));
+  hipGraph_t graph = nullptr;
+  DFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = DFM_OK;
+  for (int i = 0; i < kMsPoll && rc == DFM_OK; i++) rc = body();
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc != DFM_OK || e != hipSuccess) {
+    // nothing ran: launch directly from now on
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    capture_ok = false;
+    for (int i = 0; i < kMsPoll; i++) DFM_TRY(body());
+    return DFM_OK;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  DFM_HIP(e);
+  M->graphs.emplace_back(key, exec);
+  DFM_HIP(hipGraphLaunch(exec, s));
+  return DFM_OK;
+}
+
+// Solve_PCG as ms_solve_pcg
+int ms_dev_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration) {
+  DFM_CHECK(S && M && M->S == S && r && x && conv_ratio && iteration && r != x, DFM_BAD_ARG,
+            "bad arguments to dfm_mseg_solve_pcg");
+  const double tol = *conv_ratio;
+  const unsigned mx_iter = *iteration;
+  const size_t n = S->off[S->nseg];
+  DFM_TRY(msilu_work(M, 2));
+  double *p = M->work[0], *z = M->work[1];
+  DFM_TRY(dfm_vec_scal(n, 0.0, x));
+  double rr0 = 0.0;
+  DFM_TRY(seg_dot(S, r, r, &rr0));  // also sizes the dot buffers
+  if (rr0 < 1.0e-30) {
+    *conv_ratio = 0.0;
+    *iteration = 0;
+    return DFM_OK;
+  }
+  MsLoop L;
+  DFM_TRY(ms_loop_init(S, M, rr0, tol, mx_iter, L));
+  MsState* st = L.st;
+  DFM_TRY(L.copy(r, z));
+  DFM_TRY(dfm_mseg_ilu_precond(M, z));
+  DFM_TRY(L.copy(z, p));
+  DFM_TRY(L.dot<MS_RZ0>(r, z));
+  auto body = [&]() -> int {
+    DFM_TRY(dfm_mseg_matvec(S, 1.0, p, 0.0, z));
+    DFM_TRY(L.dot<MS_PAP>(p, z));
+    DFM_TRY(L.axpy(&st->alpha, -1.0, z, r));
+    DFM_TRY(L.axpy(&st->alpha, 1.0, p, x));
+    DFM_TRY(L.dot<MS_PCG_RR>(r, r));
+    DFM_TRY(L.copy(r, z));
+    DFM_TRY(dfm_mseg_ilu_precond(M, z));
+    DFM_TRY(L.dot<MS_RZNEW>(r, z));
+    DFM_TRY(L.scal(&st->beta, p));
+    return L.axpy(&st->one, 1.0, z, p);
+  };
+  const std::string key = ms_graph_key("pcg", L, {r, x, p, z});
+  MsState h{};
+  for (unsigned it = 0; it < mx_iter; it += kMsPoll) {
+    DFM_TRY(ms_run_block(M, key, body));
+    DFM_TRY(L.poll(h));
+    if (h.done) break;
+  }
+  if (h.done == 1) {
+    *conv_ratio = std::sqrt(h.rr * h.inv_rr0);
+    *iteration = (unsigned)h.conv_iter;
+    return DFM_OK;
+  }
+  // max_iter: the ratio of the final residual
+  double rr = 0.0;
+  DFM_TRY(seg_dot(S, r, r, &rr));
+  *conv_ratio = std::sqrt(rr * (1.0 / rr0));
+  return DFM_NOT_CONVERGED;
+}
+
+// Solve_PBiCGSTAB as ms_solve_pbicgstab
+int ms_dev_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration) {
+  DFM_CHECK(S && M && M->S == S && r && x && conv_ratio && iteration && r != x, DFM_BAD_ARG,
+            "bad arguments to dfm_mseg_solve_pbicgstab");
+  const double tol = *conv_ratio;
+  const unsigned max_iter = *iteration;
+  const size_t n = S->off[S->nseg];
+  DFM_TRY(msilu_work(M, 7));
+  double *vs = M->work[0], *Ms = M->work[1], *AMs = M->work[2], *p = M->work[3], *Mp = M->work[4],
+         *AMp = M->work[5], *r2 = M->work[6];
+  double rr0 = 0.0;
+  DFM_TRY(seg_dot(S, r, r, &rr0));
+  if (rr0 < 1.0e-60) {
+    *conv_ratio = std::sqrt(rr0);
+    *iteration = 0;
+    return DFM_OK;
+  }
+  MsLoop L;
+  DFM_TRY(ms_loop_init(S, M, rr0, tol, max_iter, L));
+  MsState* st = L.st;
+  DFM_TRY(dfm_vec_scal(n, 0.0, x));
+  DFM_TRY(L.copy(r, r2));
+  DFM_TRY(L.copy(r, p));
+  *iteration = max_iter;
+  // the loop's counter starts at 1 (it = 1 .. max_iter - 1)
+  MsState h{};
+  h.iter = 1;
+  DFM_HIP(hipMemcpyAsync(&st->iter, &h.iter, sizeof(int), hipMemcpyHostToDevice, stream()));
+  DFM_HIP(hipStreamSynchronize(stream()));
+  if (max_iter <= 1) return DFM_NOT_CONVERGED;
+  auto body = [&]() -> int {
+    DFM_TRY(L.copy(p, Mp));
+    DFM_TRY(dfm_mseg_ilu_precond(M, Mp));
+    DFM_TRY(L.dot<MS_RR2>(r, r2));
+    DFM_TRY(dfm_mseg_matvec(S, 1.0, Mp, 0.0, AMp));
+    DFM_TRY(L.dot<MS_DEN>(AMp, r2));
+    DFM_TRY(L.copy(r, vs));
+    DFM_TRY(L.axpy(&st->alpha, -1.0, AMp, vs));
+    DFM_TRY(L.copy(vs, Ms));
+    DFM_TRY(dfm_mseg_ilu_precond(M, Ms));
+    DFM_TRY(dfm_mseg_matvec(S, 1.0, Ms, 0.0, AMs));
+    DFM_TRY(L.dot<MS_DEN2>(AMs, AMs));
+    DFM_TRY(L.dot<MS_OMEGA>(vs, AMs));
+    DFM_TRY(L.axpy(&st->alpha, 1.0, Mp, x));
+    DFM_TRY(L.axpy(&st->omega, 1.0, Ms, x));
+    DFM_TRY(L.copy_gated(vs, r));
+    DFM_TRY(L.axpy(&st->omega, -1.0, AMs, r));
+    DFM_TRY(L.dot<MS_BICG_RR>(r, r));
+    DFM_TRY(L.dot<MS_BETA>(r, r2));
+    DFM_TRY(L.scal(&st->beta, p));
+    DFM_TRY(L.axpy(&st->one, 1.0, r, p));
+    return L.axpy(&st->c_bo, 1.0, AMp, p);
+  };
+  const std::string key = ms_graph_key("bicg", L, {r, x, vs, Ms, AMs, p, Mp, AMp, r2});
+  for (unsigned it = 1; it < max_iter; it += kMsPoll) {
+    DFM_TRY(ms_run_block(M, key, body));
+    DFM_TRY(L.poll(h));
+    if (h.done) break;
+  }
+  if (h.done == 1) {
+    *conv_ratio = std::sqrt(h.rr * h.inv_rr0);
+    *iteration = (unsigned)h.conv_iter;
+    return DFM_OK;
+  }
+  return DFM_NOT_CONVERGED;
+}
+
+bool g_ms_device_loop = [] {
+  const char* e = getenv("DFM_MSEG_LOOP");
+  return !(e && std::strcmp(e, "host") == 0);
+}();
+
+}  // namespace
+
+int dfm_mseg_solve_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio, unsigned* iteration) {
+  if (g_ms_device_loop) return deferred([&] { return ms_dev_pcg(S, M, r, x, conv_ratio, iteration); });
+  return deferred([&] { return ms_solve_pcg(S, M, r, x, conv_ratio, iteration); });
+}
 int dfm_mseg_solve_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_ratio,
                              unsigned* iteration) {
+  if (g_ms_device_loop) return deferred([&] { return ms_dev_pbicgstab(S, M, r, x, conv_ratio, iteration); });
   return deferred([&] { return ms_solve_pbicgstab(S, M, r, x, conv_ratio, iteration); });
+}
+int dfm_mseg_set_device_loop(int on) {
+  g_ms_device_loop = on != 0;
+  return DFM_OK;
 }

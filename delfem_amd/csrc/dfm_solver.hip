@@ -457,6 +457,7 @@ static int run_block(dfm_ilu_s* M, const std::string& key, int nblock, Body&& bo
     for (int i = 0; i < nblock; i++) DFM_TRY(body());
     return DFM_OK;
   }
+  DFM_TRY(evict_graph(M->graphs));
   hipGraph_t graph = nullptr;
   DFM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = DFM_OK;
@@ -479,6 +480,17 @@ static int run_block(dfm_ilu_s* M, const std::string& key, int nblock, Body&& bo
   DFM_HIP(hipGraphLaunch(exec, s));
   return DFM_OK;
 }
+
+int evict_graph(std::vector<std::pair<std::string, hipGraphExec_t>>& graphs) {
+  if (graphs.size() < kMaxGraphs) return DFM_OK;
+  DFM_HIP(hipStreamSynchronize(stream()));
+  (void)hipGraphExecDestroy(graphs.front().second);
+  graphs.erase(graphs.begin());
+  return DFM_OK;
+}
+
+// The sweep mode as a graph key part, for graphs captured outside this file.
+std::string sweep_mode_key() { return "p" + std::to_string(g_persist_mode) + "." + std::to_string(g_persist_grid); }
 
 static std::string graph_key(const std::string& tag, int B, int nblock, std::initializer_list<const void*> ptrs) {
   // the sweep mode is part of what a graph captured

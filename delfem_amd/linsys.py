@@ -313,6 +313,13 @@ def Solve_PBiCGSTAB(conv_ratio, iteration, A, prec, r, x, precision="fp64", **op
     return True, conv.value, it.value, info
 
 
+def set_mseg_device_loop(on):
+    """Multi-segment Solve_PCG / Solve_PBiCGSTAB device-driven (True, the
+    default: scalars on the device, the stop flag polled every four
+    iterations) or host-driven (one synchronisation per DOT); same bits."""
+    call("dfm_mseg_set_device_loop", 1 if on else 0)
+
+
 def set_sweep_mode(mode, grid=0):
     """Level sets of the triangular sweeps: mode 0 one launch per level; 1
     (default) for factors of 8 or more levels one launch per phase (dependent

@@ -382,6 +382,12 @@ int dfm_mseg_solve_pcg(dfm_mseg *S, dfm_msilu *M, double *r_dev, double *x_dev,
                        double *conv_ratio, unsigned *iteration);
 int dfm_mseg_solve_pbicgstab(dfm_mseg *S, dfm_msilu *M, double *r_dev, double *x_dev,
                              double *conv_ratio, unsigned *iteration);
+/* How the two solvers above run (no reference counterpart; a switch): on = 1
+ * (default) device-driven, the scalars formed on the device and the stop flag
+ * polled every four iterations; 0 host-driven, one synchronisation per DOT.
+ * Both give the same bits.  Process-wide; initial value from DFM_MSEG_LOOP
+ * ("host" for 0). */
+int dfm_mseg_set_device_loop(int on);
 
 /* ---- complex twin (scalar blocks, as the reference's complex classes) ----
  * Complex values are interleaved (re, im) doubles: the layout of

@@ -278,3 +278,73 @@ def test_segment_fill_blocks(orc, gpu, name, inputs):
     xd = torch.from_numpy(x.copy()).cuda()
     M.SolvePrecond(xd)
     assert np.array_equal(xd.cpu().numpy(), O.precond(x))
+
+
+LOOP_CASES = {
+    "solid3d_uv_w_pcg": (lambda: meshgen.solid3d(4, seed=7, sigma=0.5), [[0, 1], [2]], "pcg", 0, "natural"),
+    "solid3d_u_v_w_pcg_mc": (lambda: meshgen.solid3d(5, seed=3, sigma=0.5), [[0], [1], [2]], "pcg", 0,
+                             "multicolour"),
+    "oseen2d_uv_p_bicg": (lambda: meshgen.oseen2d(10, seed=7), [[0, 1], [2]], "bicg", 1, "natural"),
+    "oseen2d_uv_p_bicg_tiles": (lambda: meshgen.oseen2d(24, seed=5), [[0, 1], [2]], "bicg", 1, "tile_multicolour"),
+}
+
+
+@pytest.mark.parametrize("maxit", [3000, 9, 1, 0])
+@pytest.mark.parametrize("name", sorted(LOOP_CASES))
+def test_device_loop_bitwise(gpu, name, maxit):
+    """The device-driven loops (scalars on the device, the stop flag polled
+    every four iterations, updates skipped after it) return the host-driven
+    loops' results bit for bit: status, conv_ratio, iteration, x and r, also
+    when stopped by max_iter (iteration counts that are no multiple of the
+    polling interval) and for a zero right-hand side."""
+    import torch
+    make, groups, kind, lev, ordering = LOOP_CASES[name]
+    print(f"[case] {name} maxit {maxit}", flush=True)
+    S = gpu.MultiSegmentSystem(make(), groups, ordering=ordering)
+    S.assemble()
+    M = S.preconditioner(lev)
+    assert M.SetValue()
+    tol = 1e-10
+    for rhs_kind in ("res", "zero"):
+        rhs = S.res.clone() if rhs_kind == "res" else torch.zeros_like(S.res)
+        out = {}
+        try:
+            for dev in (False, True):
+                gpu.set_mseg_device_loop(dev)
+                r = rhs.clone()
+                x = torch.full_like(r, 7.0)
+                ok, conv, it = M.solve(kind, tol, maxit, r, x)
+                out[dev] = (ok, conv, it, x.cpu().numpy(), r.cpu().numpy())
+        finally:
+            gpu.set_mseg_device_loop(True)
+        print(f"  {rhs_kind}: ok {out[True][0]} conv {out[True][1]:.3e} it {out[True][2]}", flush=True)
+        assert out[True][:3] == out[False][:3]
+        assert np.array_equal(out[True][3], out[False][3]) and np.array_equal(out[True][4], out[False][4])
+        if rhs_kind == "res" and maxit == 3000:
+            assert out[True][0] and out[True][1] < tol
+
+
+def test_graph_cache_many_operands(gpu):
+    """More operand sets than the graph cache keeps (eight): every solve,
+    replayed or newly captured after an eviction, gives the same bits."""
+    import torch
+    print("[case] oseen2d_10 12 operand sets", flush=True)
+    S = gpu.MultiSegmentSystem(meshgen.oseen2d(10, seed=7), [[0, 1], [2]])
+    S.assemble()
+    M = S.preconditioner(1)
+    assert M.SetValue()
+    keep, first = [], None
+    for k in range(12):
+        r = S.res.clone()
+        x = torch.zeros_like(r)
+        keep.append((r, x))  # distinct addresses for every solve
+        ok, conv, it = M.solve("bicg", 1e-10, 3000, r, x)
+        got = (ok, conv, it, x.cpu().numpy())
+        if first is None:
+            first = got
+        assert got[:3] == first[:3] and np.array_equal(got[3], first[3]), k
+    r, x = keep[3]
+    r.copy_(S.res)
+    x.zero_()
+    assert M.solve("bicg", 1e-10, 3000, r, x)[:3] == first[:3]
+    assert np.array_equal(x.cpu().numpy(), first[3])
