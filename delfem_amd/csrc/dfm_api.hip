@@ -65,6 +65,13 @@ static inline int grid_vec(long n) { return (int)std::min<long>(std::max<long>(1
       DFM_HIP(hipGetLastError());                                                        \
     }                                                                                    \
   } while (0)
+#define DFM_LAUNCH_NT(kernel, grid, nt, ...)                                              \
+  do {                                                                                   \
+    if ((grid) > 0) {                                                                    \
+      hipLaunchKernelGGL((kernel), dim3(grid), dim3(nt), 0, ::dfm::stream(), __VA_ARGS__); \
+      DFM_HIP(hipGetLastError());                                                        \
+    }                                                                                    \
+  } while (0)
 
 }  // namespace dfm
 
@@ -742,6 +749,9 @@ int ilu_create_impl(const dfm_bcrs* A, int lev_fill, int ordering, const int* or
   group_rows(levL, M->nlev_lower, rowsL, gptrL);
   group_rows(levU, M->nlev_upper, rowsU, gptrU);
   M->lev_ptr_L = gptrL;
+  M->lev_maxrow_L.assign(M->nlev_lower, 0);
+  for (unsigned i = 0; i < nb; i++)
+    M->lev_maxrow_L[levL[i]] = std::max(M->lev_maxrow_L[levL[i]], (int)(M->frow[i + 1] - M->frow[i]));
   {
     std::vector<unsigned> begL(nb), endL(nb), begU(nb), endU(nb);
     for (unsigned i = 0; i < nb; i++) {
@@ -906,10 +916,34 @@ int dfm_ilu_set_value(dfm_ilu* M, dfm_bcrs* A, int* nsingular) {
 }  // extern "C"
 
 namespace {
-// one launch of the factor kernel over nrow rows listed at rows_dev
-int ilu_factor_launch(dfm_ilu_s* M, int nrow, const int* rows_dev, int partial) {
+// Rows of at most kIluWaveRow entries are factored one wave per row
+// (DFM_ILU_WAVE=0: one thread per row, for comparisons).
+bool g_ilu_wave = [] {
+  const char* e = getenv("DFM_ILU_WAVE");
+  return !(e && e[0] == '0');
+}();
+// one launch of the factor kernel over nrow rows listed at rows_dev, whose
+// longest row has maxrow entries (-1: not known)
+int ilu_factor_launch(dfm_ilu_s* M, int nrow, const int* rows_dev, int partial, int maxrow = -1) {
   const int b = (int)M->b;
   const int g = grid_for(nrow);
+  if (g_ilu_wave && b <= 3 && maxrow >= 0 && maxrow <= kIluWaveRow) {
+    const int gw = (nrow + kIluWaves - 1) / kIluWaves;
+    switch (b) {
+      case 1:
+        DFM_LAUNCH_NT(k_ilu_factor_w<1>, gw, 64 * kIluWaves, nrow, rows_dev, M->d_frow.p, M->d_fcol.p,
+                      M->d_fdiaind.p, M->d_fval.p, M->d_fdia.p, M->d_nsing.p, M->d_singrow.p, partial);
+        break;
+      case 2:
+        DFM_LAUNCH_NT(k_ilu_factor_w<2>, gw, 64 * kIluWaves, nrow, rows_dev, M->d_frow.p, M->d_fcol.p,
+                      M->d_fdiaind.p, M->d_fval.p, M->d_fdia.p, M->d_nsing.p, M->d_singrow.p, partial);
+        break;
+      default:
+        DFM_LAUNCH_NT(k_ilu_factor_w<3>, gw, 64 * kIluWaves, nrow, rows_dev, M->d_frow.p, M->d_fcol.p,
+                      M->d_fdiaind.p, M->d_fval.p, M->d_fdia.p, M->d_nsing.p, M->d_singrow.p, partial);
+    }
+    return DFM_OK;
+  }
   switch (b) {
     case 1:
       DFM_LAUNCH(k_ilu_factor<1>, g, nrow, rows_dev, M->d_frow.p, M->d_fcol.p, M->d_fdiaind.p, M->d_fval.p,
@@ -942,7 +976,7 @@ int dfm::ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular, ilu_res
   DFM_HIP(hipMemsetAsync(M->d_nsing.p, 0, sizeof(int), g_stream));
   for (int l = 0; l < M->nlev_lower; l++) {
     const int off = M->lev_ptr_L[l], nrow = M->lev_ptr_L[l + 1] - off;
-    DFM_TRY(ilu_factor_launch(M, nrow, M->d_rows_L.p + off, 0));
+    DFM_TRY(ilu_factor_launch(M, nrow, M->d_rows_L.p + off, 0, M->lev_maxrow_L[l]));
   }
   DFM_TRY(ilu_refresh_sell(M, A, M->have32, M->have16));
   int ns = 0;

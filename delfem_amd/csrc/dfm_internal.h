@@ -321,6 +321,7 @@ struct dfm_ilu_s {
   dfm::DevBuf<double> d_fval, d_fdia;
   dfm::DevBuf<int> d_rows_L;            // rows sorted by lower level
   std::vector<int> lev_ptr_L;
+  std::vector<int> lev_maxrow_L;         // longest factor row (entries) of each lower level
   dfm::DevBuf<int> d_nsing;
   dfm::DevBuf<int> d_singrow;           // per factor row: the pivot was singular
   dfm::DevBuf<int> d_fixrow;            // the row DoILUDecomp stopped in (one entry)

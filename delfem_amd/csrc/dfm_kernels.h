@@ -959,6 +959,156 @@ static __global__ void __launch_bounds__(kNT) k_ilu_factor(int nrow, const int* 
   }
 }
 
+// The same factorisation with one wave per row (rows of at most kIluWaveRow
+// entries): the row is staged in LDS, the updates of step k (one per entry
+// of U(k), each to a different entry of row i) are spread over the lanes and
+// the steps run in order, so every entry receives the same subtractions in
+// the same order as in k_ilu_factor (a wave's LDS accesses are performed in
+// program order); pivot and U scaling as there.
+constexpr int kIluWaveRow = 64;
+constexpr int kIluWaves = 4;
+template <int B>
+__device__ inline void ilu_invert_pivot(double (&d)[B * B], bool& singular) {
+  singular = false;
+  if constexpr (B == 1) {
+    if (fabs(d[0]) > 1.0e-30) d[0] = 1.0 / d[0];
+    else singular = true;
+  } else if constexpr (B == 2) {
+    const double det = d[0] * d[3] - d[1] * d[2];
+    if (fabs(det) > 1.0e-30) {
+      const double id = 1.0 / det;
+      const double d0 = d[0];
+      d[0] = id * d[3];
+      d[1] = -id * d[1];
+      d[2] = -id * d[2];
+      d[3] = id * d0;
+    } else {
+      singular = true;
+    }
+  } else {
+    const double det = d[0] * d[4] * d[8] + d[3] * d[7] * d[2] + d[6] * d[1] * d[5] - d[0] * d[7] * d[5] -
+                       d[6] * d[4] * d[2] - d[3] * d[1] * d[8];
+    if (fabs(det) > 1.0e-30) {
+      double s[9];
+#pragma unroll
+      for (int z = 0; z < 9; z++) s[z] = d[z];
+      const double id = 1.0 / det;
+      d[0] = id * (s[4] * s[8] - s[5] * s[7]);
+      d[1] = id * (s[2] * s[7] - s[1] * s[8]);
+      d[2] = id * (s[1] * s[5] - s[2] * s[4]);
+      d[3] = id * (s[5] * s[6] - s[3] * s[8]);
+      d[4] = id * (s[0] * s[8] - s[2] * s[6]);
+      d[5] = id * (s[2] * s[3] - s[0] * s[5]);
+      d[6] = id * (s[3] * s[7] - s[4] * s[6]);
+      d[7] = id * (s[1] * s[6] - s[0] * s[7]);
+      d[8] = id * (s[0] * s[4] - s[1] * s[3]);
+    } else {
+      singular = true;
+    }
+  }
+}
+template <int B>
+static __global__ void __launch_bounds__(64 * kIluWaves) k_ilu_factor_w(
+    int nrow, const int* __restrict__ rows, const int* __restrict__ frow, const int* __restrict__ fcol,
+    const int* __restrict__ fdiaind, double* __restrict__ fval, double* __restrict__ fdia, int* __restrict__ nsing,
+    int* __restrict__ singrow, int partial) {
+  constexpr int BB = B * B;
+  __shared__ double sv_all[kIluWaves][kIluWaveRow * BB + BB];
+  __shared__ int sc_all[kIluWaves][kIluWaveRow];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = blockIdx.x * kIluWaves + w;
+  if (t >= nrow) return;  // the whole wave
+  double* sv = sv_all[w];
+  double* sD = sv + kIluWaveRow * BB;
+  int* sc = sc_all[w];
+  const int i = __builtin_amdgcn_readfirstlane(rows[t]);
+  const int r0 = __builtin_amdgcn_readfirstlane(frow[i]), rd = __builtin_amdgcn_readfirstlane(fdiaind[i]),
+            r1 = __builtin_amdgcn_readfirstlane(frow[i + 1]);
+  const int n = r1 - r0;
+  for (int e = lane; e < n * BB; e += 64) sv[e] = fval[(size_t)r0 * BB + e];
+  for (int e = lane; e < n; e += 64) sc[e] = fcol[r0 + e];
+  if (lane < BB) sD[lane] = fdia[(size_t)i * BB + lane];
+  __builtin_amdgcn_wave_barrier();
+  for (int qk = r0; qk < rd; qk++) {
+    const int k = __builtin_amdgcn_readfirstlane(fcol[qk]);
+    double L[BB];
+#pragma unroll
+    for (int s2 = 0; s2 < BB; s2++) L[s2] = sv[(qk - r0) * BB + s2];
+    const int u0 = __builtin_amdgcn_readfirstlane(fdiaind[k]), u1 = __builtin_amdgcn_readfirstlane(frow[k + 1]);
+    for (int qj = u0 + lane; qj < u1; qj += 64) {
+      const int j = fcol[qj];
+      double* Aij;
+      if (j == i) {
+        Aij = sD;
+      } else {
+        int lo = qk + 1 - r0, hi = n;  // first column >= j in row i after entry qk
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (sc[mid] < j) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo >= n || sc[lo] != j) continue;
+        Aij = sv + lo * BB;
+      }
+      const double* U = fval + (size_t)qj * BB;
+      double u[BB];
+#pragma unroll
+      for (int s2 = 0; s2 < BB; s2++) u[s2] = U[s2];
+      if constexpr (B == 1) {
+        Aij[0] -= L[0] * u[0];
+      } else {
+#pragma unroll
+        for (int r = 0; r < B; r++)
+#pragma unroll
+          for (int c = 0; c < B; c++) {
+            double s2 = L[r * B] * u[c];
+#pragma unroll
+            for (int m = 1; m < B; m++) s2 = s2 + L[r * B + m] * u[m * B + c];
+            Aij[r * B + c] -= s2;
+          }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  double d[BB];
+#pragma unroll
+  for (int s2 = 0; s2 < BB; s2++) d[s2] = sD[s2];
+  bool singular;
+  ilu_invert_pivot<B>(d, singular);
+  if (lane == 0) {
+    if (singular) atomicAdd(nsing, 1);
+    singrow[i] = singular ? 1 : 0;
+  }
+  if (partial && singular) {
+    // the row as updated, pivot as the inversion attempt left it, U not scaled
+    for (int e = lane; e < n * BB; e += 64) fval[(size_t)r0 * BB + e] = sv[e];
+    if (lane < BB) fdia[(size_t)i * BB + lane] = sD[lane];
+    return;
+  }
+  if (lane < BB) fdia[(size_t)i * BB + lane] = d[lane];
+  for (int e = lane; e < (rd - r0) * BB; e += 64) fval[(size_t)r0 * BB + e] = sv[e];
+  for (int q = rd + lane; q < r1; q += 64) {
+    double u[BB], o[BB];
+#pragma unroll
+    for (int s2 = 0; s2 < BB; s2++) u[s2] = sv[(q - r0) * BB + s2];
+    if constexpr (B == 1) {
+      o[0] = u[0] * d[0];
+    } else {
+#pragma unroll
+      for (int r = 0; r < B; r++)
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+          double s2 = d[r * B] * u[c];
+#pragma unroll
+          for (int m = 1; m < B; m++) s2 = s2 + d[r * B + m] * u[m * B + c];
+          o[r * B + c] = s2;
+        }
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < BB; s2++) fval[(size_t)q * BB + s2] = o[s2];
+  }
+}
+
 // Generic block size (b > 3, b <= 8): CalcSubMatPr / CalcInvMat / CalcMatPr
 // (matdiafrac_blkcrs.cpp:45-136).
 static __global__ void __launch_bounds__(kNT) k_ilu_factor_gen(int nrow, int b, const int* __restrict__ rows,

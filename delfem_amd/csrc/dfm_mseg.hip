@@ -63,6 +63,8 @@ struct dfm_msilu_s {
   }
 };
 
+constexpr int kMaxSeg = 16;  // segments of one system
+
 #define DFM_MLAUNCH(kernel, grid, ...)                                                      \
   do {                                                                                      \
     if ((grid) > 0) {                                                                       \
@@ -176,6 +178,150 @@ __global__ void __launch_bounds__(kNT) k_ms_lowup_dia(int nrow, int row0, const 
     }
   }
 }
+
+// One wave per row for the four kernels above (rows of at most kMsWaveRow
+// entries, blocks of at most kMsWaveBlk values): the target row is staged in
+// LDS, the products of one step (one L entry: its partner row's entries,
+// each to a different target entry) spread over the lanes, the steps in
+// order, so every target entry receives the same subtractions in the same
+// order (a wave's LDS accesses are performed in program order).
+constexpr int kMsWaveRow = 64, kMsWaveBlk = 9, kMsWaves = 4;
+__device__ inline int lds_find(const int* sc, int n, int j) {
+  for (int q = 0; q < n; q++)
+    if (sc[q] == j) return q;
+  return -1;
+}
+struct MsRow {  // one wave's LDS: the row's blocks, its diagonal block, its columns
+  double v[kMsWaveRow * kMsWaveBlk + kMsWaveBlk];
+  int c[kMsWaveRow];
+};
+// stage n blocks of bs values from val[q0..] and their columns, and (dia) a bs block
+__device__ inline void ms_stage(MsRow& R, int lane, const double* val, const int* col, long q0, int n, int bs,
+                                const double* dia) {
+  for (int e = lane; e < n * bs; e += 64) R.v[e] = val[q0 * bs + e];
+  for (int e = lane; e < n; e += 64) R.c[e] = col[q0 + e];
+  if (dia && lane < bs) R.v[kMsWaveRow * kMsWaveBlk + lane] = dia[lane];
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ inline void ms_unstage(const MsRow& R, int lane, double* val, long q0, int n, int bs, double* dia) {
+  __builtin_amdgcn_wave_barrier();
+  for (int e = lane; e < n * bs; e += 64) val[q0 * bs + e] = R.v[e];
+  if (dia && lane < bs) dia[lane] = R.v[kMsWaveRow * kMsWaveBlk + lane];
+}
+#define MS_WAVE_ROW(nrow)                                    \
+  __shared__ MsRow rows_lds[kMsWaves];                       \
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63; \
+  const int t = blockIdx.x * kMsWaves + wv;                  \
+  if (t >= (nrow)) return;                                   \
+  MsRow& R = rows_lds[wv];
+
+__global__ void __launch_bounds__(64 * kMsWaves) k_ms_up_w(int nrow, const int* __restrict__ rows,
+                                                           const int* __restrict__ frow, const int* __restrict__ fcol,
+                                                           const int* __restrict__ fdiaind,
+                                                           const double* __restrict__ fval,
+                                                           const double* __restrict__ fdia, const int* __restrict__ rp,
+                                                           const int* __restrict__ cl, double* __restrict__ F, int bi,
+                                                           int bj) {
+  MS_WAVE_ROW(nrow)
+  const int r = __builtin_amdgcn_readfirstlane(rows[t]);
+  const int bs = bi * bj, bii = bi * bi;
+  const int q0 = __builtin_amdgcn_readfirstlane(rp[r]), n = __builtin_amdgcn_readfirstlane(rp[r + 1]) - q0;
+  ms_stage(R, lane, F, cl, q0, n, bs, nullptr);
+  const int i0 = __builtin_amdgcn_readfirstlane(frow[r]), i1 = __builtin_amdgcn_readfirstlane(fdiaind[r]);
+  for (int ik = i0; ik < i1; ik++) {
+    const int k = __builtin_amdgcn_readfirstlane(fcol[ik]);
+    const int k0 = __builtin_amdgcn_readfirstlane(rp[k]), k1 = __builtin_amdgcn_readfirstlane(rp[k + 1]);
+    for (int kj = k0 + lane; kj < k1; kj += 64) {
+      const int ij = lds_find(R.c, n, cl[kj]);
+      if (ij < 0) continue;
+      msub(R.v + ij * bs, fval + (long)ik * bii, F + (long)kj * bs, bi, bi, bj);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int q = lane; q < n; q += 64) mmul(R.v + q * bs, fdia + (long)r * bii, bi, bj);
+  ms_unstage(R, lane, F, q0, n, bs, nullptr);
+}
+__global__ void __launch_bounds__(64 * kMsWaves) k_ms_low_w(int nrow, const int* __restrict__ rp,
+                                                            const int* __restrict__ cl, double* __restrict__ F,
+                                                            const int* __restrict__ frow,
+                                                            const int* __restrict__ fcol,
+                                                            const int* __restrict__ fdiaind,
+                                                            const double* __restrict__ fval, int bi, int bj) {
+  MS_WAVE_ROW(nrow)
+  const int r = t;
+  const int bs = bi * bj, bjj = bj * bj;
+  const int q0 = __builtin_amdgcn_readfirstlane(rp[r]), n = __builtin_amdgcn_readfirstlane(rp[r + 1]) - q0;
+  ms_stage(R, lane, F, cl, q0, n, bs, nullptr);
+  for (int ik = 0; ik < n; ik++) {
+    const int k = R.c[ik];
+    const int k0 = __builtin_amdgcn_readfirstlane(fdiaind[k]), k1 = __builtin_amdgcn_readfirstlane(frow[k + 1]);
+    for (int kj = k0 + lane; kj < k1; kj += 64) {
+      const int ij = lds_find(R.c, n, fcol[kj]);
+      if (ij < 0) continue;
+      msub(R.v + ij * bs, R.v + ik * bs, fval + (long)kj * bjj, bi, bj, bj);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  ms_unstage(R, lane, F, q0, n, bs, nullptr);
+}
+__global__ void __launch_bounds__(64 * kMsWaves) k_ms_lowup_w(int nrow, const int* __restrict__ trp,
+                                                              const int* __restrict__ tcl, double* __restrict__ T,
+                                                              const int* __restrict__ lrp,
+                                                              const int* __restrict__ lcl,
+                                                              const double* __restrict__ Lv,
+                                                              const int* __restrict__ urp,
+                                                              const int* __restrict__ ucl,
+                                                              const double* __restrict__ Uv, int bi, int bk, int bj) {
+  MS_WAVE_ROW(nrow)
+  const int r = t;
+  const int bs = bi * bj;
+  const int q0 = __builtin_amdgcn_readfirstlane(trp[r]), n = __builtin_amdgcn_readfirstlane(trp[r + 1]) - q0;
+  ms_stage(R, lane, T, tcl, q0, n, bs, nullptr);
+  const int i0 = __builtin_amdgcn_readfirstlane(lrp[r]), i1 = __builtin_amdgcn_readfirstlane(lrp[r + 1]);
+  for (int ik = i0; ik < i1; ik++) {
+    const int k = __builtin_amdgcn_readfirstlane(lcl[ik]);
+    const int k0 = __builtin_amdgcn_readfirstlane(urp[k]), k1 = __builtin_amdgcn_readfirstlane(urp[k + 1]);
+    for (int kj = k0 + lane; kj < k1; kj += 64) {
+      const int ij = lds_find(R.c, n, ucl[kj]);
+      if (ij < 0) continue;
+      msub(R.v + ij * bs, Lv + (long)ik * bi * bk, Uv + (long)kj * bk * bj, bi, bk, bj);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  ms_unstage(R, lane, T, q0, n, bs, nullptr);
+}
+__global__ void __launch_bounds__(64 * kMsWaves) k_ms_lowup_dia_w(
+    int nrow, int row0, const int* __restrict__ frow, const int* __restrict__ fcol, double* __restrict__ fval,
+    double* __restrict__ fdia, const int* __restrict__ lrp, const int* __restrict__ lcl,
+    const double* __restrict__ Lv, const int* __restrict__ urp, const int* __restrict__ ucl,
+    const double* __restrict__ Uv, int bi, int bk) {
+  MS_WAVE_ROW(nrow - row0)
+  const int r = row0 + t;
+  const int bii = bi * bi;
+  const int q0 = __builtin_amdgcn_readfirstlane(frow[r]), n = __builtin_amdgcn_readfirstlane(frow[r + 1]) - q0;
+  ms_stage(R, lane, fval, fcol, q0, n, bii, fdia + (long)r * bii);
+  double* sD = R.v + kMsWaveRow * kMsWaveBlk;
+  const int i0 = __builtin_amdgcn_readfirstlane(lrp[r]), i1 = __builtin_amdgcn_readfirstlane(lrp[r + 1]);
+  for (int ik = i0; ik < i1; ik++) {
+    const int k = __builtin_amdgcn_readfirstlane(lcl[ik]);
+    const int k0 = __builtin_amdgcn_readfirstlane(urp[k]), k1 = __builtin_amdgcn_readfirstlane(urp[k + 1]);
+    const double* a = Lv + (long)ik * bi * bk;
+    for (int kj = k0 + lane; kj < k1; kj += 64) {
+      const int j = ucl[kj];
+      const double* c = Uv + (long)kj * bk * bi;
+      if (j != r) {
+        const int ij = lds_find(R.c, n, j);
+        if (ij < 0) continue;
+        msub(R.v + ij * bii, a, c, bi, bk, bi);
+      } else {
+        msub(sD, a, c, bi, bk, bi);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  ms_unstage(R, lane, fval, q0, n, bii, fdia + (long)r * bii);
+}
+#undef MS_WAVE_ROW
 
 // SetValue_Initialize of an off-diagonal factor block (CMat_BlkCrs::SetValue,
 // mat_blkcrs.cpp:557-694): the block's values where it has the entry, 0 on fill
@@ -399,7 +545,8 @@ int ls_axpy(size_t n, double alpha, const double* x, double* y) {
 }  // namespace
 
 int dfm_mseg_create(unsigned nseg, const unsigned* nblk, const unsigned* len, dfm_mseg** out) {
-  DFM_CHECK(out && nblk && len && nseg > 0 && nseg <= 16, DFM_BAD_ARG, "bad arguments to dfm_mseg_create");
+  DFM_CHECK(out && nblk && len && nseg > 0 && nseg <= (unsigned)kMaxSeg, DFM_BAD_ARG,
+            "bad arguments to dfm_mseg_create");
   DFM_TRY(ensure_init());
   std::unique_ptr<dfm_mseg_s> S(new dfm_mseg_s);
   S->nseg = nseg;
@@ -565,6 +712,22 @@ int dfm_mseg_ilu_destroy(dfm_msilu* M) {
   return DFM_OK;
 }
 namespace {
+// Longest row (entries) of a CRS pattern, and whether rows of that length
+// and blocks of bs values take the wave-per-row kernels (DFM_MS_WAVE=0: never,
+// for comparisons).
+bool g_ms_wave = [] {
+  const char* e = getenv("DFM_MS_WAVE");
+  return !(e && e[0] == '0');
+}();
+template <typename V>
+bool ms_wave_fits(const V& rowptr, int bs) {
+  if (!g_ms_wave || bs > kMsWaveBlk) return false;
+  for (size_t r = 0; r + 1 < rowptr.size(); r++)
+    if ((long)rowptr[r + 1] - (long)rowptr[r] > kMsWaveRow) return false;
+  return true;
+}
+inline int wgrid(long nrow) { return (int)std::max<long>(1, (nrow + kMsWaves - 1) / kMsWaves); }
+
 struct DiaPrep {
   dfm_msilu* M;
   unsigned i;
@@ -580,6 +743,15 @@ int dia_lowup(void* ctx, int row0) {
   for (unsigned k = 0; k < i; k++) {
     const dfm_bmat_s *lo = M->F[i][k], *up = M->F[k][i];
     if (!lo || !up) continue;
+    if ((long)S->nb[i] - row0 <= 0) continue;
+    if (ms_wave_fits(D->frow, (int)(S->b[i] * S->b[i]))) {
+      hipLaunchKernelGGL(k_ms_lowup_dia_w, dim3(wgrid((long)S->nb[i] - row0)), dim3(64 * kMsWaves), 0, stream(),
+                         (int)S->nb[i], row0, D->d_frow.p, D->d_fcol.p, D->d_fval.p, D->d_fdia.p, lo->d_rowptr.p,
+                         lo->col.empty() ? nullptr : lo->d_col.p, lo->d_val.p, up->d_rowptr.p,
+                         up->col.empty() ? nullptr : up->d_col.p, up->d_val.p, (int)S->b[i], (int)S->b[k]);
+      DFM_HIP(hipGetLastError());
+      continue;
+    }
     DFM_MLAUNCH(k_ms_lowup_dia, mgrid((long)S->nb[i] - row0), (int)S->nb[i], row0, D->d_frow.p, D->d_fcol.p,
                 D->d_fval.p, D->d_fdia.p, lo->d_rowptr.p, lo->col.empty() ? nullptr : lo->d_col.p, lo->d_val.p,
                 up->d_rowptr.p, up->col.empty() ? nullptr : up->d_col.p, up->d_val.p, (int)S->b[i], (int)S->b[k]);
@@ -630,12 +802,21 @@ int dfm_mseg_ilu_set_value(dfm_msilu* M, int* nsingular) {
       dfm_bmat_s* T = M->F[i][j];
       if (!T) continue;
       const int* tcl = T->col.empty() ? nullptr : T->d_col.p;
+      const bool wave = ms_wave_fits(T->rowptr, (int)(T->b * T->br));
+      const dim3 wblk(64 * kMsWaves);
       const unsigned kmax = std::min(i, j);
       for (unsigned k = 0; k < kmax + 1; k++) {
         if (k == i && k < j) {  // DoILUDecompUp with factor i, one launch per lower level
           const dfm_ilu_s* D = M->dia[i];
           for (int l = 0; l < D->nlev_lower; l++) {
             const int o = D->lev_ptr_L[l], nr = D->lev_ptr_L[l + 1] - o;
+            if (wave && T->b * T->b <= (unsigned)kMsWaveBlk) {
+              hipLaunchKernelGGL(k_ms_up_w, dim3(wgrid(nr)), wblk, 0, stream(), nr, D->d_rows_L.p + o, D->d_frow.p,
+                                 D->d_fcol.p, D->d_fdiaind.p, D->d_fval.p, D->d_fdia.p, T->d_rowptr.p, tcl,
+                                 T->d_val.p, (int)T->b, (int)T->br);
+              DFM_HIP(hipGetLastError());
+              continue;
+            }
             DFM_MLAUNCH(k_ms_up, mgrid(nr), nr, D->d_rows_L.p + o, D->d_frow.p, D->d_fcol.p, D->d_fdiaind.p,
                         D->d_fval.p, D->d_fdia.p, T->d_rowptr.p, tcl, T->d_val.p, (int)T->b, (int)T->br);
           }
@@ -643,12 +824,27 @@ int dfm_mseg_ilu_set_value(dfm_msilu* M, int* nsingular) {
         }
         if (k == j && k < i) {  // DoILUDecompLow with factor j
           const dfm_ilu_s* D = M->dia[j];
+          if (wave && T->br * T->br <= (unsigned)kMsWaveBlk) {
+            hipLaunchKernelGGL(k_ms_low_w, dim3(wgrid(T->nb)), wblk, 0, stream(), (int)T->nb, T->d_rowptr.p, tcl,
+                               T->d_val.p, D->d_frow.p, D->d_fcol.p, D->d_fdiaind.p, D->d_fval.p, (int)T->b,
+                               (int)T->br);
+            DFM_HIP(hipGetLastError());
+            continue;
+          }
           DFM_MLAUNCH(k_ms_low, mgrid(T->nb), (int)T->nb, T->d_rowptr.p, tcl, T->d_val.p, D->d_frow.p, D->d_fcol.p,
                       D->d_fdiaind.p, D->d_fval.p, (int)T->b, (int)T->br);
           continue;
         }
         const dfm_bmat_s *lo = M->F[i][k], *up = M->F[k][j];
         if (!lo || !up) continue;
+        if (wave) {
+          hipLaunchKernelGGL(k_ms_lowup_w, dim3(wgrid(T->nb)), wblk, 0, stream(), (int)T->nb, T->d_rowptr.p, tcl,
+                             T->d_val.p, lo->d_rowptr.p, lo->col.empty() ? nullptr : lo->d_col.p, lo->d_val.p,
+                             up->d_rowptr.p, up->col.empty() ? nullptr : up->d_col.p, up->d_val.p, (int)T->b,
+                             (int)lo->br, (int)T->br);
+          DFM_HIP(hipGetLastError());
+          continue;
+        }
         DFM_MLAUNCH(k_ms_lowup, mgrid(T->nb), (int)T->nb, T->d_rowptr.p, tcl, T->d_val.p, lo->d_rowptr.p,
                     lo->col.empty() ? nullptr : lo->d_col.p, lo->d_val.p, up->d_rowptr.p,
                     up->col.empty() ? nullptr : up->d_col.p, up->d_val.p, (int)T->b, (int)lo->br, (int)T->br);
@@ -854,76 +1050,186 @@ static int ms_solve_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, d
 namespace {
 
 struct MsState {
-  double inv_rr0, tol2, rz, alpha, beta, omega, r_r2, den2, rr, c_bo, one;
+  double inv_rr0, tol2, rz, alpha, beta, omega, r_r2, den2, rr, c_bo;
   int done;  // 0 running, 1 converged, 2 stopped at max_iter
   int iter, maxit, conv_iter;
 };
 constexpr int kMsPoll = 4;
 
-enum { MS_RZ0, MS_PAP, MS_PCG_RR, MS_RZNEW, MS_RR2, MS_DEN, MS_DEN2, MS_OMEGA, MS_BICG_RR, MS_BETA };
+enum { MS_RZ0, MS_PAP, MS_PCG_RR, MS_RZNEW, MS_RR2, MS_DEN, MS_DEN2_OMEGA, MS_RR_BETA };
 
-// s = the segments' sums in segment order, then one step of the loop's scalar
-// arithmetic (written as the host loops write it)
+// Segment table of the fused dots: segment i owns the blocks base[i] ..
+// base[i+1] - 1 (as many as dfm_vec_dot gives a vector of its length) and the
+// entries off[i] .. off[i+1] - 1.
+struct SegTab {
+  int nseg, G;
+  int base[kMaxSeg + 1];
+  long off[kMaxSeg + 1];
+};
+
+// One step of the loops' scalar arithmetic (written as the host loops write
+// it) on the sums s of one or two dots.
 template <int OP>
-__global__ void k_ms_scalar(const KState* __restrict__ seg, int nseg, MsState* __restrict__ st) {
-  double s = 0.0;
-  for (int i = 0; i < nseg; i++) s += seg[i].d0;
+__device__ void ms_step(const double* s, MsState* st) {
   if (st->done) return;
   if constexpr (OP == MS_RZ0) {
-    st->rz = s;
+    st->rz = s[0];
   } else if constexpr (OP == MS_PAP) {
-    st->alpha = st->rz / s;
+    st->alpha = st->rz / s[0];
   } else if constexpr (OP == MS_PCG_RR) {
-    st->rr = s;
-    if (s * st->inv_rr0 < st->tol2) {
+    st->rr = s[0];
+    if (s[0] * st->inv_rr0 < st->tol2) {
       st->done = 1;
       st->conv_iter = st->iter;
     }
     st->iter += 1;
     if (!st->done && st->iter >= st->maxit) st->done = 2;
   } else if constexpr (OP == MS_RZNEW) {
-    st->beta = s / st->rz;
-    st->rz = s;
+    st->beta = s[0] / st->rz;
+    st->rz = s[0];
   } else if constexpr (OP == MS_RR2) {
-    st->r_r2 = s;
+    st->r_r2 = s[0];
   } else if constexpr (OP == MS_DEN) {
-    st->alpha = st->r_r2 / s;
-  } else if constexpr (OP == MS_DEN2) {
-    st->den2 = s;
-  } else if constexpr (OP == MS_OMEGA) {
-    st->omega = s / st->den2;
-  } else if constexpr (OP == MS_BICG_RR) {
-    st->rr = s;
-    if (s * st->inv_rr0 < st->tol2) {
+    st->alpha = st->r_r2 / s[0];
+  } else if constexpr (OP == MS_DEN2_OMEGA) {  // s = (AMs, AMs), (vs, AMs)
+    st->den2 = s[0];
+    st->omega = s[1] / s[0];
+  } else if constexpr (OP == MS_RR_BETA) {  // s = (r, r), (r, r2)
+    st->rr = s[0];
+    if (s[0] * st->inv_rr0 < st->tol2) {
       st->done = 1;
       st->conv_iter = st->iter;
+      return;
     }
-  } else if constexpr (OP == MS_BETA) {
-    st->beta = s * st->alpha / (st->r_r2 * st->omega);
+    st->beta = s[1] * st->alpha / (st->r_r2 * st->omega);
     st->c_bo = -st->beta * st->omega;
     st->iter += 1;
     if (st->iter >= st->maxit) st->done = 2;
   }
 }
-// y += sign * a x; nothing when that factor is zero (CLinearSystem::AXPY,
-// linearsystem.cpp:302) or, with a stop flag, once it is set
-__global__ void k_ms_axpy(long n, const double* __restrict__ a, double sign, const double* __restrict__ x,
-                          double* __restrict__ y, const int* __restrict__ done) {
-  if (done && *done) return;
-  const double c = sign * *a;
-  if (c == 0.0) return;
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] += c * x[i];
+
+// ND dots <x_d, y_d> of every segment in one launch: block b of segment i
+// sums what block b - base[i] of dfm_vec_dot's kernel sums for that
+// segment's vectors, and writes its partial at d * G + b.
+template <int ND>
+__global__ void __launch_bounds__(kNT) k_ms_dot(SegTab t, const double* __restrict__ x0,
+                                                const double* __restrict__ y0, const double* __restrict__ x1,
+                                                const double* __restrict__ y1, double* __restrict__ partials) {
+  int i = 0;
+  while (i + 1 < t.nseg && (int)blockIdx.x >= t.base[i + 1]) i++;
+  const int lb = (int)blockIdx.x - t.base[i], g = t.base[i + 1] - t.base[i];
+  const long o = t.off[i], n = t.off[i + 1] - o;
+  double dsum[ND];
+#pragma unroll
+  for (int d = 0; d < ND; d++) dsum[d] = 0.0;
+  for (long k = (long)lb * kNT + threadIdx.x; k < n; k += (long)g * kNT) {
+    dsum[0] += x0[o + k] * y0[o + k];
+    if constexpr (ND > 1) dsum[1] += x1[o + k] * y1[o + k];
+  }
+  write_partials<ND>(dsum, partials, blockIdx.x, t.G);
 }
-__global__ void k_ms_scal(long n, const double* __restrict__ a, double* __restrict__ x,
-                          const int* __restrict__ done) {
-  if (*done) return;
-  const double c = *a;
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) x[i] *= c;
+// The segments' sums, each reduced as dfm_vec_dot's finalize reduces it, added
+// in segment order (DOT, linearsystem.cpp:335-364), then the scalar step OP.
+template <int ND, int OP>
+__global__ void __launch_bounds__(kFinNT) k_ms_dot_fin(SegTab t, const double* __restrict__ partials,
+                                                       MsState* __restrict__ st) {
+  __shared__ double sh[kFinNT / 64];
+  double tot[ND];
+#pragma unroll
+  for (int d = 0; d < ND; d++) tot[d] = 0.0;
+  for (int i = 0; i < t.nseg; i++) {
+    const double* p[ND];
+    int n[ND];
+    double a[ND];
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      p[d] = partials + (size_t)d * t.G + t.base[i];
+      n[d] = t.base[i + 1] - t.base[i];
+    }
+    strided_sums<kFinNT, ND>(p, n, a);
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      const double s = block_sum_n<kFinNT>(a[d], sh);
+      if (threadIdx.x == 0) tot[d] += s;
+    }
+  }
+  if (threadIdx.x == 0) ms_step<OP>(tot, st);
 }
-__global__ void k_ms_copy(long n, const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ done) {
-  if (done && *done) return;
-  for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < n; i += (long)gridDim.x * kNT) y[i] = x[i];
+
+// Fused vector updates, each the host loop's copies and AXPYs (an AXPY with a
+// zero factor is skipped, linearsystem.cpp:302) element by element in the
+// same order; nothing once the stop flag is set.
+#define MS_LOOP(n) for (long i = (long)blockIdx.x * kNT + threadIdx.x; i < (n); i += (long)gridDim.x * kNT)
+// PCG: r -= alpha Ap, x += alpha p, then z = r for the preconditioner
+__global__ void k_ms_pcg_xr(long n, const MsState* __restrict__ st, const double* __restrict__ p,
+                            double* __restrict__ z, double* __restrict__ r, double* __restrict__ x) {
+  if (st->done) return;
+  const double a = st->alpha, c = -a;
+  MS_LOOP(n) {
+    double ri = r[i];
+    if (c != 0.0) ri += c * z[i];
+    r[i] = ri;
+    if (a != 0.0) x[i] += a * p[i];
+    z[i] = ri;
+  }
 }
+// PCG: p = beta p + z
+__global__ void k_ms_pcg_p(long n, const MsState* __restrict__ st, const double* __restrict__ z,
+                           double* __restrict__ p) {
+  if (st->done) return;
+  const double b = st->beta;
+  MS_LOOP(n) {
+    double pi = p[i] * b;
+    pi += 1.0 * z[i];
+    p[i] = pi;
+  }
+}
+// BiCGSTAB: vs = r - alpha AMp, and Ms = vs for the preconditioner
+__global__ void k_ms_bicg_vs(long n, const MsState* __restrict__ st, const double* __restrict__ r,
+                             const double* __restrict__ AMp, double* __restrict__ vs, double* __restrict__ Ms) {
+  if (st->done) return;
+  const double c = -st->alpha;
+  MS_LOOP(n) {
+    double v = r[i];
+    if (c != 0.0) v += c * AMp[i];
+    vs[i] = v;
+    Ms[i] = v;
+  }
+}
+// BiCGSTAB: x += alpha Mp, x += omega Ms, r = vs - omega AMs
+__global__ void k_ms_bicg_xr(long n, const MsState* __restrict__ st, const double* __restrict__ Mp,
+                             const double* __restrict__ Ms, const double* __restrict__ vs,
+                             const double* __restrict__ AMs, double* __restrict__ x, double* __restrict__ r) {
+  if (st->done) return;
+  const double a = st->alpha, w = st->omega, c = -w;
+  MS_LOOP(n) {
+    double xi = x[i];
+    if (a != 0.0) xi += a * Mp[i];
+    if (w != 0.0) xi += w * Ms[i];
+    x[i] = xi;
+    double ri = vs[i];
+    if (c != 0.0) ri += c * AMs[i];
+    r[i] = ri;
+  }
+}
+// BiCGSTAB: p = beta p + r - beta omega AMp, and Mp = p for the next
+// iteration's preconditioner
+__global__ void k_ms_bicg_p(long n, const MsState* __restrict__ st, const double* __restrict__ r,
+                            const double* __restrict__ AMp, double* __restrict__ p, double* __restrict__ Mp) {
+  if (st->done) return;
+  const double b = st->beta, c = st->c_bo;
+  MS_LOOP(n) {
+    double pi = p[i] * b;
+    pi += 1.0 * r[i];
+    if (c != 0.0) pi += c * AMp[i];
+    p[i] = pi;
+    Mp[i] = pi;
+  }
+}
+__global__ void k_ms_copy(long n, const double* __restrict__ x, double* __restrict__ y) {
+  MS_LOOP(n) y[i] = x[i];
+}
+#undef MS_LOOP
 
 struct MsLoop {
   dfm_mseg_s* S;
@@ -931,38 +1237,17 @@ struct MsLoop {
   MsState* st;
   size_t n;
   int g;
-  // the segments' dot sums into dot_state, then the scalar step OP
-  template <int OP>
-  int dot(const double* x, const double* y) {
-    const unsigned ns = S->nseg;
-    int gmax = 1;
-    for (unsigned i = 0; i < ns; i++) gmax = std::max(gmax, grid_vec((long)(S->off[i + 1] - S->off[i])));
-    for (unsigned i = 0; i < ns; i++) {
-      const int gi = grid_vec((long)(S->off[i + 1] - S->off[i]));
-      double* part = S->dot_partials.p + (size_t)i * gmax;
-      DFM_MLAUNCH(k_dot<double>, gi, (long)(S->off[i + 1] - S->off[i]), x + S->off[i], y + S->off[i], part);
-      hipLaunchKernelGGL((k_finalize<1, FIN_STORE>), dim3(1), dim3(kFinNT), 0, stream(), part, gi,
-                         S->dot_state.p + i);
-      DFM_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL((k_ms_scalar<OP>), dim3(1), dim3(1), 0, stream(), S->dot_state.p, (int)ns, st);
+  SegTab tab;
+  // ND dots of all segments and the scalar step OP: two launches
+  template <int ND, int OP>
+  int dot(const double* x0, const double* y0, const double* x1 = nullptr, const double* y1 = nullptr) {
+    DFM_MLAUNCH((k_ms_dot<ND>), tab.G, tab, x0, y0, x1, y1, S->dot_partials.p);
+    hipLaunchKernelGGL((k_ms_dot_fin<ND, OP>), dim3(1), dim3(kFinNT), 0, stream(), tab, S->dot_partials.p, st);
     DFM_HIP(hipGetLastError());
     return DFM_OK;
   }
-  int axpy(const double* a, double sign, const double* x, double* y) {
-    DFM_MLAUNCH(k_ms_axpy, g, (long)n, a, sign, x, y, &st->done);
-    return DFM_OK;
-  }
-  int scal(const double* a, double* x) {
-    DFM_MLAUNCH(k_ms_scal, g, (long)n, a, x, &st->done);
-    return DFM_OK;
-  }
-  int copy_gated(const double* x, double* y) {
-    DFM_MLAUNCH(k_ms_copy, g, (long)n, x, y, &st->done);
-    return DFM_OK;
-  }
   int copy(const double* x, double* y) {  // a kernel, also inside graphs
-    DFM_MLAUNCH(k_ms_copy, g, (long)n, x, y, (const int*)nullptr);
+    DFM_MLAUNCH(k_ms_copy, g, (long)n, x, y);
     return DFM_OK;
   }
   // the state's host copy after the work enqueued so far
@@ -974,14 +1259,28 @@ struct MsLoop {
   }
 };
 
-int ms_loop_init(dfm_mseg_s* S, dfm_msilu_s* M, double rr0, double tol, unsigned maxit, MsLoop& L) {
+int ms_loop_init(dfm_mseg_s* S, dfm_msilu_s* M, double rr0, double tol, unsigned maxit, int iter0, MsLoop& L) {
   if (M->dstate.n < sizeof(MsState)) DFM_TRY(M->dstate.alloc(sizeof(MsState)));
   if (!M->hstate) DFM_HIP(hipHostMalloc(&M->hstate, sizeof(MsState), hipHostMallocDefault));
+  SegTab& t = L.tab;
+  t.nseg = (int)S->nseg;
+  t.base[0] = 0;
+  t.off[0] = 0;
+  for (unsigned i = 0; i < S->nseg; i++) {
+    t.base[i + 1] = t.base[i] + grid_vec((long)(S->off[i + 1] - S->off[i]));
+    t.off[i + 1] = (long)S->off[i + 1];
+  }
+  t.G = t.base[S->nseg];
+  // room for the fused dots and for seg_dot's layout (its largest grid per segment)
+  int gmax = 1;
+  for (unsigned i = 0; i < S->nseg; i++) gmax = std::max(gmax, t.base[i + 1] - t.base[i]);
+  const size_t need = std::max((size_t)2 * t.G, (size_t)gmax * S->nseg);
+  if (S->dot_partials.n < need) DFM_TRY(S->dot_partials.alloc(need));
   MsState h{};
   h.inv_rr0 = 1.0 / rr0;
   h.tol2 = tol * tol;
-  h.one = 1.0;
   h.maxit = (int)std::min<unsigned>(maxit, 0x7fffffffu);
+  h.iter = iter0;
   std::memcpy(M->hstate, &h, sizeof(MsState));
   L.S = S;
   L.M = M;
@@ -989,6 +1288,7 @@ int ms_loop_init(dfm_mseg_s* S, dfm_msilu_s* M, double rr0, double tol, unsigned
   L.n = S->off[S->nseg];
   L.g = grid_vec((long)L.n);
   DFM_HIP(hipMemcpyAsync(L.st, M->hstate, sizeof(MsState), hipMemcpyHostToDevice, stream()));
+  DFM_HIP(hipStreamSynchronize(stream()));  // the pinned copy is the poll buffer too
   return DFM_OK;
 }
 
@@ -1075,23 +1375,22 @@ int ms_dev_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_rat
     return DFM_OK;
   }
   MsLoop L;
-  DFM_TRY(ms_loop_init(S, M, rr0, tol, mx_iter, L));
+  DFM_TRY(ms_loop_init(S, M, rr0, tol, mx_iter, 0, L));
   MsState* st = L.st;
+  const long nn = (long)n;
   DFM_TRY(L.copy(r, z));
   DFM_TRY(dfm_mseg_ilu_precond(M, z));
   DFM_TRY(L.copy(z, p));
-  DFM_TRY(L.dot<MS_RZ0>(r, z));
+  DFM_TRY((L.dot<1, MS_RZ0>(r, z)));
   auto body = [&]() -> int {
     DFM_TRY(dfm_mseg_matvec(S, 1.0, p, 0.0, z));
-    DFM_TRY(L.dot<MS_PAP>(p, z));
-    DFM_TRY(L.axpy(&st->alpha, -1.0, z, r));
-    DFM_TRY(L.axpy(&st->alpha, 1.0, p, x));
-    DFM_TRY(L.dot<MS_PCG_RR>(r, r));
-    DFM_TRY(L.copy(r, z));
+    DFM_TRY((L.dot<1, MS_PAP>(p, z)));
+    DFM_MLAUNCH(k_ms_pcg_xr, L.g, nn, st, p, z, r, x);
+    DFM_TRY((L.dot<1, MS_PCG_RR>(r, r)));
     DFM_TRY(dfm_mseg_ilu_precond(M, z));
-    DFM_TRY(L.dot<MS_RZNEW>(r, z));
-    DFM_TRY(L.scal(&st->beta, p));
-    return L.axpy(&st->one, 1.0, z, p);
+    DFM_TRY((L.dot<1, MS_RZNEW>(r, z)));
+    DFM_MLAUNCH(k_ms_pcg_p, L.g, nn, st, z, p);
+    return DFM_OK;
   };
   const std::string key = ms_graph_key("pcg", L, {r, x, p, z});
   MsState h{};
@@ -1129,41 +1428,31 @@ int ms_dev_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* co
     *iteration = 0;
     return DFM_OK;
   }
+  // the loop's counter starts at 1 (it = 1 .. max_iter - 1)
   MsLoop L;
-  DFM_TRY(ms_loop_init(S, M, rr0, tol, max_iter, L));
+  DFM_TRY(ms_loop_init(S, M, rr0, tol, max_iter, 1, L));
   MsState* st = L.st;
+  const long nn = (long)n;
   DFM_TRY(dfm_vec_scal(n, 0.0, x));
   DFM_TRY(L.copy(r, r2));
   DFM_TRY(L.copy(r, p));
+  DFM_TRY(L.copy(p, Mp));
   *iteration = max_iter;
-  // the loop's counter starts at 1 (it = 1 .. max_iter - 1)
   MsState h{};
-  h.iter = 1;
-  DFM_HIP(hipMemcpyAsync(&st->iter, &h.iter, sizeof(int), hipMemcpyHostToDevice, stream()));
-  DFM_HIP(hipStreamSynchronize(stream()));
   if (max_iter <= 1) return DFM_NOT_CONVERGED;
   auto body = [&]() -> int {
-    DFM_TRY(L.copy(p, Mp));
     DFM_TRY(dfm_mseg_ilu_precond(M, Mp));
-    DFM_TRY(L.dot<MS_RR2>(r, r2));
+    DFM_TRY((L.dot<1, MS_RR2>(r, r2)));
     DFM_TRY(dfm_mseg_matvec(S, 1.0, Mp, 0.0, AMp));
-    DFM_TRY(L.dot<MS_DEN>(AMp, r2));
-    DFM_TRY(L.copy(r, vs));
-    DFM_TRY(L.axpy(&st->alpha, -1.0, AMp, vs));
-    DFM_TRY(L.copy(vs, Ms));
+    DFM_TRY((L.dot<1, MS_DEN>(AMp, r2)));
+    DFM_MLAUNCH(k_ms_bicg_vs, L.g, nn, st, r, AMp, vs, Ms);
     DFM_TRY(dfm_mseg_ilu_precond(M, Ms));
     DFM_TRY(dfm_mseg_matvec(S, 1.0, Ms, 0.0, AMs));
-    DFM_TRY(L.dot<MS_DEN2>(AMs, AMs));
-    DFM_TRY(L.dot<MS_OMEGA>(vs, AMs));
-    DFM_TRY(L.axpy(&st->alpha, 1.0, Mp, x));
-    DFM_TRY(L.axpy(&st->omega, 1.0, Ms, x));
-    DFM_TRY(L.copy_gated(vs, r));
-    DFM_TRY(L.axpy(&st->omega, -1.0, AMs, r));
-    DFM_TRY(L.dot<MS_BICG_RR>(r, r));
-    DFM_TRY(L.dot<MS_BETA>(r, r2));
-    DFM_TRY(L.scal(&st->beta, p));
-    DFM_TRY(L.axpy(&st->one, 1.0, r, p));
-    return L.axpy(&st->c_bo, 1.0, AMp, p);
+    DFM_TRY((L.dot<2, MS_DEN2_OMEGA>(AMs, AMs, vs, AMs)));
+    DFM_MLAUNCH(k_ms_bicg_xr, L.g, nn, st, Mp, Ms, vs, AMs, x, r);
+    DFM_TRY((L.dot<2, MS_RR_BETA>(r, r, r, r2)));
+    DFM_MLAUNCH(k_ms_bicg_p, L.g, nn, st, r, AMp, p, Mp);
+    return DFM_OK;
   };
   const std::string key = ms_graph_key("bicg", L, {r, x, vs, Ms, AMs, p, Mp, AMp, r2});
   for (unsigned it = 1; it < max_iter; it += kMsPoll) {
