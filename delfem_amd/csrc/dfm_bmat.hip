@@ -72,25 +72,64 @@ __global__ void __launch_bounds__(kNT) k_bmat_matvec(long ndof, int bc, int br, 
   if (t >= ndof) return;
   const long i = t / bc, id = t % bc;
   const long bs = (long)bc * br;
+  // the specialised forms take CH entries at a time: indices, values and x
+  // loaded before the first is used, then added one by one in row order
+  constexpr int CH = 8;
+  const int q0 = rowptr[i], q1 = rowptr[i + 1];
   if (bc == 1 && br == 1) {
     double acc = beta != 0.0 ? y[t] * beta : 0.0;
-    for (int q = rowptr[i]; q < rowptr[i + 1]; q++) acc += alpha * val[q] * x[col[q]];
+    for (int qb = q0; qb < q1; qb += CH) {
+      double v[CH], xv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int q = qb + u < q1 ? qb + u : q0;
+        v[u] = val[q];
+        xv[u] = x[col[q]];
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++)
+        if (qb + u < q1) acc += alpha * v[u] * xv[u];
+    }
     y[t] = acc;
   } else if (bc == 1 && (br == 2 || br == 3)) {
     double acc = y[t] * beta;
-    for (int q = rowptr[i]; q < rowptr[i + 1]; q++) {
-      const double* a = val + (long)q * br;
-      const double* xj = x + (long)col[q] * br;
-      double s = a[0] * xj[0] + a[1] * xj[1];
-      if (br == 3) s = s + a[2] * xj[2];
-      acc += alpha * s;
+    for (int qb = q0; qb < q1; qb += CH) {
+      double a[CH][3], xv[CH][3];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const long q = qb + u < q1 ? qb + u : q0;
+        const long c = col[q];
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+          a[u][e] = e < br ? val[q * br + e] : 0.0;
+          xv[u][e] = e < br ? x[c * br + e] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        if (qb + u >= q1) continue;
+        double s = a[u][0] * xv[u][0] + a[u][1] * xv[u][1];
+        if (br == 3) s = s + a[u][2] * xv[u][2];
+        acc += alpha * s;
+      }
     }
     y[t] = acc;
   } else if (br == 1 && bc > 1) {
     double acc = y[t] * beta;
-    for (int q = rowptr[i]; q < rowptr[i + 1]; q++) {
-      const double ajx = x[col[q]] * alpha;
-      acc += val[(long)q * bc + id] * ajx;
+    for (int qb = q0; qb < q1; qb += CH) {
+      double v[CH], xv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const long q = qb + u < q1 ? qb + u : q0;
+        v[u] = val[q * bc + id];
+        xv[u] = x[col[q]];
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        if (qb + u >= q1) continue;
+        const double ajx = xv[u] * alpha;
+        acc += v[u] * ajx;
+      }
     }
     y[t] = acc;
   } else {

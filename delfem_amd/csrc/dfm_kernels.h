@@ -391,13 +391,13 @@ static __global__ void __launch_bounds__(kNT) k_crs_matvec(int nb, const int* __
   const int i = blockIdx.x * kNT + threadIdx.x;
   if (i >= nb) return;
   constexpr int BB = B * B;
+  // entries taken CH at a time: their indices, blocks and x loaded before the
+  // first is used, then added one by one in row order
+  constexpr int CH = B == 3 ? 4 : 8;
   double acc[B];
 #pragma unroll
   for (int r = 0; r < B; r++) acc[r] = y[(size_t)i * B + r] * beta;
-  const int q0 = rowptr[i], q1 = rowptr[i + 1];
-  for (int q = q0; q <= q1; q++) {
-    const double* A = q < q1 ? val + (size_t)q * BB : dia + (size_t)i * BB;
-    const double* xj = x + (size_t)(q < q1 ? col[q] : i) * B;
+  auto add = [&](const double* A, const double* xj) {
     if constexpr (B == 1) {
       acc[0] += alpha * A[0] * xj[0];
     } else {
@@ -409,6 +409,32 @@ static __global__ void __launch_bounds__(kNT) k_crs_matvec(int nb, const int* __
         acc[r] += alpha * s;
       }
     }
+  };
+  const int q0 = rowptr[i], q1 = rowptr[i + 1];
+  for (int qb = q0; qb < q1; qb += CH) {
+    int c[CH];
+    double a[CH][BB], xv[CH][B];
+#pragma unroll
+    for (int u = 0; u < CH; u++) c[u] = qb + u < q1 ? col[qb + u] : i;
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const size_t q = qb + u < q1 ? (size_t)(qb + u) : (size_t)q0;
+#pragma unroll
+      for (int e = 0; e < BB; e++) a[u][e] = val[q * BB + e];
+#pragma unroll
+      for (int e = 0; e < B; e++) xv[u][e] = x[(size_t)c[u] * B + e];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; u++)
+      if (qb + u < q1) add(a[u], xv[u]);
+  }
+  {
+    double d[BB], xi[B];
+#pragma unroll
+    for (int e = 0; e < BB; e++) d[e] = dia[(size_t)i * BB + e];
+#pragma unroll
+    for (int e = 0; e < B; e++) xi[e] = x[(size_t)i * B + e];
+    add(d, xi);
   }
 #pragma unroll
   for (int r = 0; r < B; r++) y[(size_t)i * B + r] = acc[r];
