@@ -619,8 +619,8 @@ def bench_complex(cfg, args, torch, linsys, _lib):
 
 def bench_mseg(cfg, args, torch, linsys, _lib):
     """Multi-segment workload: every step = cross-segment ILU(k) SetValue +
-    Solve_PCG or Solve_PBiCGSTAB on the segments (host-driven loop over
-    device kernels)."""
+    Solve_PCG or Solve_PBiCGSTAB on the segments (device-driven loop: blocks
+    of iterations replayed from graphs, the stop flag polled between them)."""
     from delfem_amd import meshgen
     log(f"[bench] {args.config}: {cfg['workload']}")
     t0 = time.time()
@@ -666,7 +666,7 @@ def bench_mseg(cfg, args, torch, linsys, _lib):
         # the reference's natural numbering, scaled to its own iteration count
         cpu = cpu_baseline_mseg(prob, cfg, args, runs.get("natural", runs[dev_ordering])[3])
     out = {
-        "metric": f"ms per multi-segment {names} solve to {cfg['tol']:g} (host-driven loop)",
+        "metric": f"ms per multi-segment {names} solve to {cfg['tol']:g} (device-driven loop)",
         "value": round(ms, 3),
         "unit": "ms",
         "n_gpus": 1,
