@@ -98,6 +98,80 @@ unsigned long long dfm::next_version() {
   return ++counter;
 }
 
+bool dfm::mv_sell_on() {
+  static const bool on = [] {
+    const char* e = getenv("DFM_MV_SELL");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+int dfm::mv_sell_prepare(MvSell& m, unsigned nrow, const std::vector<unsigned>& rowptr,
+                         const std::vector<unsigned>& col, int bs, const double* dval, unsigned long long pattern_key,
+                         unsigned long long version) {
+  if (m.plan_key != pattern_key || m.nslice == 0) {
+    const int ns = (int)((nrow + 63) / 64);
+    std::vector<int> sptr(ns + 1, 0);
+    for (int s2 = 0; s2 < ns; s2++) {
+      unsigned w = 0;
+      for (unsigned i = (unsigned)s2 * 64; i < std::min(nrow, (unsigned)s2 * 64 + 64); i++)
+        w = std::max(w, rowptr[i + 1] - rowptr[i]);
+      sptr[s2 + 1] = sptr[s2] + (int)w;
+    }
+    const long nk = sptr[ns];
+    std::vector<int> cl((size_t)std::max<long>(1, nk) * 64, -1);
+    std::vector<long long> src(cl.size(), -1);
+    for (unsigned i = 0; i < nrow; i++) {
+      const int s2 = (int)(i / 64), l = (int)(i % 64);
+      for (unsigned q = rowptr[i]; q < rowptr[i + 1]; q++) {
+        const size_t k = (size_t)sptr[s2] + (q - rowptr[i]);
+        cl[k * 64 + l] = (int)col[q];
+        src[k * 64 + l] = q;
+      }
+    }
+    DFM_TRY(m.sptr.upload(sptr));
+    DFM_TRY(m.col.upload(cl));
+    DFM_TRY(m.src.upload(src));
+    DFM_TRY(m.val.alloc((size_t)std::max<long>(1, nk) * 64 * bs));
+    m.nslice = ns;
+    m.nk = nk;
+    m.plan_key = pattern_key;
+    m.version = 0;
+  }
+  if (m.version != version) {
+    const long n = m.nk * 64 * bs;
+    if (n) DFM_LAUNCH(k_mv_gather, grid_for(n), n, bs, m.src.p, dval, m.val.p);
+    m.version = version;
+  }
+  return DFM_OK;
+}
+bool dfm::mv_sell_launch(const MvSell& m, unsigned nrow, bool square, int bc, int br, const double* dia, double alpha,
+                         const double* x, double beta, double* y) {
+  const dim3 grid((unsigned)std::max(1, (m.nslice + 3) / 4)), blk(256);
+#define DFM_MVS(F, C, R) \
+  hipLaunchKernelGGL((k_sell_mv<F, C, R>), grid, blk, 0, stream(), (int)nrow, m.sptr.p, m.col.p, m.val.p, dia, alpha, x, beta, y)
+  if (square) {
+    switch (bc) {
+      case 1: DFM_MVS(MV_SQ, 1, 1); return true;
+      case 2: DFM_MVS(MV_SQ, 2, 2); return true;
+      case 3: DFM_MVS(MV_SQ, 3, 3); return true;
+      default: return false;
+    }
+  }
+  switch (bc * 4 + br) {
+    case 5: DFM_MVS(MV_SCALAR, 1, 1); return true;   // 1 x 1
+    case 6: DFM_MVS(MV_ROW, 1, 2); return true;      // 1 x 2
+    case 7: DFM_MVS(MV_ROW, 1, 3); return true;      // 1 x 3
+    case 9: DFM_MVS(MV_COL, 2, 1); return true;      // 2 x 1
+    case 13: DFM_MVS(MV_COL, 3, 1); return true;     // 3 x 1
+    case 10: DFM_MVS(MV_GEN, 2, 2); return true;
+    case 11: DFM_MVS(MV_GEN, 2, 3); return true;
+    case 14: DFM_MVS(MV_GEN, 3, 2); return true;
+    case 15: DFM_MVS(MV_GEN, 3, 3); return true;
+    default: return false;
+  }
+#undef DFM_MVS
+}
+
 extern "C" {
 
 const char* dfm_last_error(void) { return dfm::g_err.c_str(); }
@@ -401,6 +475,14 @@ int dfm_bcrs_set_bc(dfm_bcrs* A, const int* flag) {
 int dfm_bcrs_matvec(dfm_bcrs* A, double alpha, const double* x, double beta, double* y) {
   DFM_CHECK(A && x && y && x != y, DFM_BAD_ARG, "bad arguments to dfm_bcrs_matvec");
   const int nb = (int)A->nb, g = grid_for(nb);
+  if (nb > 0 && A->b <= 3 && mv_sell_on()) {
+    DFM_TRY(mv_sell_prepare(A->mv, A->nb, A->rowptr, A->col, (int)(A->b * A->b), A->d_val.p, A->pattern_hash,
+                            A->version));
+    if (mv_sell_launch(A->mv, A->nb, true, (int)A->b, (int)A->b, A->d_dia.p, alpha, x, beta, y)) {
+      DFM_HIP(hipGetLastError());
+      return sync_stream();
+    }
+  }
   switch (A->b) {
     case 1: DFM_LAUNCH(k_crs_matvec<1>, g, nb, A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, alpha, x, beta, y); break;
     case 2: DFM_LAUNCH(k_crs_matvec<2>, g, nb, A->d_rowptr.p, A->d_col.p, A->d_val.p, A->d_dia.p, alpha, x, beta, y); break;

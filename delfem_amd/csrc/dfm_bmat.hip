@@ -194,6 +194,8 @@ int dfm_bmat_create(unsigned nblk_col, unsigned len_col, unsigned nblk_row, unsi
   DFM_TRY(M->d_rowptr.upload(rp));
   DFM_TRY(M->d_val.alloc(1));
   DFM_HIP(hipStreamSynchronize(stream()));
+  M->version = next_version();
+  M->pattern_key = next_version();
   *out = M.release();
   return DFM_OK;
 }
@@ -238,6 +240,8 @@ int dfm_bmat_set_pattern_boundary(dfm_bmat* M, const dfm_bcrs* A, const int* fla
   DFM_HIP(hipMemsetAsync(M->d_val.p, 0, nval * sizeof(double), stream()));
   DFM_HIP(hipStreamSynchronize(stream()));
   M->src_hash = A->pattern_hash;
+  M->version = next_version();
+  M->pattern_key = next_version();
   return DFM_OK;
 }
 int dfm_bmat_info(const dfm_bmat* M, unsigned* nblk, unsigned* len, unsigned* ncrs) {
@@ -257,6 +261,7 @@ int dfm_bmat_set_zero(dfm_bmat* M) {
   DFM_CHECK(M, DFM_BAD_ARG, "NULL matrix");
   DFM_HIP(hipMemsetAsync(M->d_val.p, 0, M->d_val.n * sizeof(double), stream()));
   DFM_HIP(hipStreamSynchronize(stream()));
+  M->version = next_version();
   return DFM_OK;
 }
 int dfm_bmat_get_values(const dfm_bmat* M, double* val) {
@@ -271,6 +276,7 @@ int dfm_bmat_set_values(dfm_bmat* M, const double* val) {
   const size_t n = M->col.size() * M->b * M->br;
   if (n) DFM_HIP(hipMemcpyAsync(M->d_val.p, val, n * sizeof(double), hipMemcpyHostToDevice, stream()));
   DFM_HIP(hipStreamSynchronize(stream()));
+  M->version = next_version();
   return DFM_OK;
 }
 int dfm_bmat_mearge(dfm_bmat* M, unsigned nblkel, const unsigned* blkel, unsigned blksize, const double* emat) {
@@ -299,6 +305,7 @@ int dfm_bmat_mearge(dfm_bmat* M, unsigned nblkel, const unsigned* blkel, unsigne
                            hipMemcpyHostToDevice, stream()));
     DFM_HIP(hipStreamSynchronize(stream()));
   }
+  M->version = next_version();
   return DFM_OK;
 }
 int dfm_bmat_copy_from(dfm_bmat* M, const dfm_bcrs* A) {
@@ -310,6 +317,7 @@ int dfm_bmat_copy_from(dfm_bmat* M, const dfm_bcrs* A) {
   const long n = (long)M->col.size() * bb;
   if (n) DFM_BLAUNCH(k_bmat_copy, bgrid(n), n, bb, M->d_src.p, A->d_val.p, A->d_dia.p, M->d_val.p);
   DFM_HIP(hipStreamSynchronize(stream()));
+  M->version = next_version();
   return DFM_OK;
 }
 int dfm_bmat_set_bc_inverse_col(dfm_bmat* M, const int* flag) {
@@ -319,6 +327,7 @@ int dfm_bmat_set_bc_inverse_col(dfm_bmat* M, const int* flag) {
   const long ncrs = (long)M->col.size();
   if (ncrs) DFM_BLAUNCH(k_bmat_bc_inv_col, bgrid(ncrs), ncrs, (int)M->b, M->d_col.p, f.p, M->d_val.p);
   DFM_HIP(hipStreamSynchronize(stream()));
+  M->version = next_version();
   return DFM_OK;
 }
 int dfm_bmat_set_bc_row(dfm_bmat* M, const int* flag) {
@@ -328,11 +337,20 @@ int dfm_bmat_set_bc_row(dfm_bmat* M, const int* flag) {
   DFM_TRY(f.upload(flag, ndof));
   if (ndof) DFM_BLAUNCH(k_bmat_bc_row, bgrid(ndof), ndof, (int)M->b, (int)M->br, M->d_rowptr.p, f.p, M->d_val.p);
   DFM_HIP(hipStreamSynchronize(stream()));
+  M->version = next_version();
   return DFM_OK;
 }
 int dfm_bmat_matvec(dfm_bmat* M, double alpha, const double* x, double beta, double* y) {
   DFM_CHECK(M && x && y && x != y, DFM_BAD_ARG, "bad arguments to dfm_bmat_matvec");
   const long ndof = (long)M->nb * M->b;
+  if (ndof && !M->col.empty() && M->b <= 3 && M->br <= 3 && mv_sell_on()) {
+    DFM_TRY(mv_sell_prepare(M->mv, M->nb, M->rowptr, M->col, (int)(M->b * M->br), M->d_val.p, M->pattern_key,
+                            M->version));
+    if (mv_sell_launch(M->mv, M->nb, false, (int)M->b, (int)M->br, nullptr, alpha, x, beta, y)) {
+      DFM_HIP(hipGetLastError());
+      return sync_stream();
+    }
+  }
   if (ndof)
     DFM_BLAUNCH(k_bmat_matvec, bgrid(ndof), ndof, (int)M->b, (int)M->br, M->d_rowptr.p,
                 M->col.empty() ? nullptr : M->d_col.p, M->d_val.p, alpha, x, beta, y);
@@ -366,6 +384,8 @@ int dfm_bmat_add_pattern(dfm_bmat* M, unsigned nrow, const unsigned* index, cons
   DFM_HIP(hipMemsetAsync(M->d_val.p, 0, nval * sizeof(double), stream()));
   DFM_HIP(hipStreamSynchronize(stream()));
   M->src_hash = 0;
+  M->version = next_version();
+  M->pattern_key = next_version();
   return DFM_OK;
 }
 int dfm_bmat_set_bc_col(dfm_bmat* M, const int* flag) {
@@ -375,6 +395,7 @@ int dfm_bmat_set_bc_col(dfm_bmat* M, const int* flag) {
   const long ncrs = (long)M->col.size();
   if (ncrs) DFM_BLAUNCH(k_bmat_bc_col, bgrid(ncrs), ncrs, (int)M->b, (int)M->br, M->d_col.p, f.p, M->d_val.p);
   DFM_HIP(hipStreamSynchronize(stream()));
+  M->version = next_version();
   return DFM_OK;
 }
 int dfm_bmat_extract(dfm_bmat* M, const dfm_bcrs* A, const unsigned* dofs_row, const unsigned* dofs_col) {
@@ -410,6 +431,7 @@ int dfm_bmat_extract(dfm_bmat* M, const dfm_bcrs* A, const unsigned* dofs_row, c
     DFM_BLAUNCH(k_bmat_extract, bgrid(n), n, (int)M->b, (int)M->br, (int)A->b, d_src.p, dr.p, dc.p, A->d_val.p,
                 A->d_dia.p, M->d_val.p);
   DFM_HIP(hipStreamSynchronize(stream()));
+  M->version = next_version();
   return DFM_OK;
 }
 int dfm_bcrs_extract(dfm_bcrs* D, const dfm_bcrs* A, const unsigned* dofs) {

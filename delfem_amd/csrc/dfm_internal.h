@@ -278,6 +278,31 @@ int ilu_numeric_factor(dfm_ilu_s* M, dfm_bcrs_s* A, int* nsingular, ilu_restore_
                        void* restore_ctx = nullptr);
 int ilu_sweep64(dfm_ilu_s* M, double* vec, bool lower, bool upper);
 
+// Sliced-ELL copy of a CRS block matrix for its MatVec (built on first use,
+// its values refreshed whenever the matrix's value version differs from the
+// one copied): slices of 64 block rows, slot k of lane l = the row's k-th CRS
+// entry (column, -1 for padding after the row's last entry), the entry's bs
+// values at val[(k * bs + e) * 64 + l].  The MatVec kernel over it adds the
+// entries in CRS order as the one-thread-per-row kernels do, bit for bit.
+struct MvSell {
+  int nslice = 0;
+  long nk = 0;
+  DevBuf<int> sptr, col;
+  DevBuf<long long> src;
+  DevBuf<double> val;
+  unsigned long long plan_key = 0;  // pattern the plan was built for
+  unsigned long long version = 0;   // values copied
+};
+bool mv_sell_on();  // DFM_MV_SELL=0: the one-thread-per-row kernels (comparisons)
+// plan (if pattern_key changed) and values (if version changed) of nrow rows
+int mv_sell_prepare(MvSell& m, unsigned nrow, const std::vector<unsigned>& rowptr,
+                    const std::vector<unsigned>& col, int bs, const double* dval, unsigned long long pattern_key,
+                    unsigned long long version);
+// y = beta y + alpha A x over the copy; form as dfm_bcrs (square, diagonal
+// blocks dia) or dfm_bmat (rectangular bc x br); false when no kernel form fits
+bool mv_sell_launch(const MvSell& m, unsigned nrow, bool square, int bc, int br, const double* dia, double alpha,
+                    const double* x, double beta, double* y);
+
 }  // namespace dfm
 
 struct dfm_bcrs_s {
@@ -287,6 +312,7 @@ struct dfm_bcrs_s {
   dfm::DevBuf<double> d_val, d_dia;
   unsigned long long version = 0;       // unique across matrices, renewed whenever values change
   unsigned long long pattern_hash = 0;  // FNV-1a of (nb, rowptr, col), refreshed when the pattern changes
+  dfm::MvSell mv;                       // MatVec copy, on first use
 };
 
 struct dfm_asm_s {
@@ -305,6 +331,9 @@ struct dfm_bmat_s {
   dfm::DevBuf<double> d_val;            // ncrs * b * br, row-major blocks
   dfm::DevBuf<long long> d_src;         // per entry: CRS index in the source matrix, or -1-i for its diagonal block i
   unsigned long long src_hash = 0;      // pattern hash of the matrix the pattern was made from
+  unsigned long long version = 0;       // renewed whenever values change (next_version)
+  unsigned long long pattern_key = 0;   // renewed whenever the pattern changes
+  dfm::MvSell mv;                       // MatVec copy, on first use
 };
 
 struct dfm_ilu_s {

@@ -440,6 +440,101 @@ static __global__ void __launch_bounds__(kNT) k_crs_matvec(int nb, const int* __
   for (int r = 0; r < B; r++) y[(size_t)i * B + r] = acc[r];
 }
 
+// MatVec over the sliced-ELL copy (MvSell), one lane per block row, entries
+// in CRS order, eight (nine-value blocks: four) slots' columns, blocks and x
+// loaded before the first is added.  Forms (the sums of the one-thread-per-
+// row kernels, term for term): MV_SQ square blocks, diagonal block last
+// (CMatDia_BlkCrs::MatVec); rectangular CMat_BlkCrs::MatVec forms MV_SCALAR
+// (1x1, beta == 0 starts from 0), MV_ROW (one row dof: alpha (a . x)), MV_COL
+// (one column dof: a_id (x alpha)) and MV_GEN (alpha a_{id,jd} x_jd each).
+enum { MV_SQ, MV_SCALAR, MV_ROW, MV_COL, MV_GEN };
+static __global__ void k_mv_gather(long n, int bs, const long long* __restrict__ src, const double* __restrict__ aval,
+                                   double* __restrict__ val) {
+  const long t = (long)blockIdx.x * kNT + threadIdx.x;
+  if (t >= n) return;
+  const long kk = t / (bs * 64), rem = t % (bs * 64), e = rem / 64, l = rem % 64;
+  const long long q = src[kk * 64 + l];
+  val[t] = q >= 0 ? aval[q * bs + e] : 0.0;
+}
+template <int FORM, int BC, int BR>
+static __global__ void __launch_bounds__(256) k_sell_mv(int nrow, const int* __restrict__ sptr,
+                                                        const int* __restrict__ col, const double* __restrict__ val,
+                                                        const double* __restrict__ dia, double alpha,
+                                                        const double* __restrict__ x, double beta,
+                                                        double* __restrict__ y) {
+  constexpr int BS = BC * BR;
+  constexpr int CH = BS >= 9 ? 4 : 8;
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  const long i = (long)s * 64 + l;
+  if (i >= nrow) return;
+  const int k0 = sptr[s], k1 = sptr[s + 1];
+  double acc[BC];
+#pragma unroll
+  for (int r = 0; r < BC; r++) {
+    if constexpr (FORM == MV_SCALAR) acc[r] = beta != 0.0 ? y[i] * beta : 0.0;
+    else acc[r] = y[i * BC + r] * beta;
+  }
+  auto add = [&](const double (&a)[BS], const double (&xj)[BR]) {
+    if constexpr (FORM == MV_SQ && BC == 1) {
+      acc[0] += alpha * a[0] * xj[0];
+    } else if constexpr (FORM == MV_SQ) {
+#pragma unroll
+      for (int r = 0; r < BC; r++) {
+        double t = a[r * BC] * xj[0];
+#pragma unroll
+        for (int c = 1; c < BC; c++) t = t + a[r * BC + c] * xj[c];
+        acc[r] += alpha * t;
+      }
+    } else if constexpr (FORM == MV_SCALAR) {
+      acc[0] += alpha * a[0] * xj[0];
+    } else if constexpr (FORM == MV_ROW) {
+      double t = a[0] * xj[0] + a[1] * xj[1];
+      if constexpr (BR == 3) t = t + a[2] * xj[2];
+      acc[0] += alpha * t;
+    } else if constexpr (FORM == MV_COL) {
+      const double ajx = xj[0] * alpha;
+#pragma unroll
+      for (int r = 0; r < BC; r++) acc[r] += a[r] * ajx;
+    } else {
+#pragma unroll
+      for (int r = 0; r < BC; r++)
+#pragma unroll
+        for (int c = 0; c < BR; c++) acc[r] += alpha * a[r * BR + c] * xj[c];
+    }
+  };
+  for (int kb = k0; kb < k1; kb += CH) {
+    int c[CH];
+    double a[CH][BS], xv[CH][BR];
+#pragma unroll
+    for (int u = 0; u < CH; u++) c[u] = kb + u < k1 ? col[(size_t)(kb + u) * 64 + l] : -1;
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const size_t k = kb + u < k1 ? (size_t)(kb + u) : (size_t)k0;
+      const size_t cc = c[u] >= 0 ? (size_t)c[u] : 0;
+#pragma unroll
+      for (int e = 0; e < BS; e++) a[u][e] = val[(k * BS + e) * 64 + l];
+#pragma unroll
+      for (int e = 0; e < BR; e++) xv[u][e] = x[cc * BR + e];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; u++)
+      if (c[u] >= 0) add(a[u], xv[u]);
+  }
+  if constexpr (FORM == MV_SQ) {
+    double d[BS], xi[BR];
+#pragma unroll
+    for (int e = 0; e < BS; e++) d[e] = dia[i * BS + e];
+#pragma unroll
+    for (int e = 0; e < BR; e++) xi[e] = x[i * BR + e];
+    add(d, xi);
+  }
+#pragma unroll
+  for (int r = 0; r < BC; r++) {
+    if constexpr (FORM == MV_SCALAR) y[i] = acc[0];
+    else y[i * BC + r] = acc[r];
+  }
+}
+
 // Generic block size (b > 3): each product added separately (:1072-1085).
 static __global__ void __launch_bounds__(kNT) k_crs_matvec_gen(int nb, int b, const int* __restrict__ rowptr,
                                                        const int* __restrict__ col,

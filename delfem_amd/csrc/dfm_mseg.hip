@@ -771,6 +771,10 @@ int dfm_mseg_ilu_set_value(dfm_msilu* M, int* nsingular) {
   dfm_mseg_s* S = M->S;
   const unsigned n = S->nseg;
   int nsing_total = 0;
+  // every off-diagonal factor block gets new values below
+  for (auto& row : M->F)
+    for (dfm_bmat_s* F : row)
+      if (F) F->version = next_version();
   // SetValue_Initialize of every block first (preconditioner.cpp:231-251)
   for (unsigned i = 0; i < n; i++) {
     DFM_CHECK(S->dia[i]->pattern_hash == M->dia[i]->pattern_hash, DFM_BAD_ARG,
@@ -1292,6 +1296,25 @@ int ms_loop_init(dfm_mseg_s* S, dfm_msilu_s* M, double rr0, double tol, unsigned
   return DFM_OK;
 }
 
+// The MatVec copies of every block the loops multiply by, made current before
+// a block is captured (a capture may not allocate, and a refresh would be
+// replayed with it).
+int ms_prepare_matvecs(dfm_mseg_s* S, dfm_msilu_s* M) {
+  if (!mv_sell_on()) return DFM_OK;
+  for (unsigned i = 0; i < S->nseg; i++) {
+    dfm_bcrs_s* A = S->dia[i];
+    if (A->nb > 0 && A->b <= 3)
+      DFM_TRY(mv_sell_prepare(A->mv, A->nb, A->rowptr, A->col, (int)(A->b * A->b), A->d_val.p, A->pattern_hash,
+                              A->version));
+    for (unsigned j = 0; j < S->nseg; j++)
+      for (dfm_bmat_s* B : {S->nd[i][j], M->F[i][j]})
+        if (B && B->nb > 0 && !B->col.empty() && B->b <= 3 && B->br <= 3)
+          DFM_TRY(mv_sell_prepare(B->mv, B->nb, B->rowptr, B->col, (int)(B->b * B->br), B->d_val.p, B->pattern_key,
+                                  B->version));
+  }
+  return DFM_OK;
+}
+
 // Operand set of a captured block: the vectors, the loop's buffers, every
 // block's and factor's storage it reads, and the sweep mode.
 std::string ms_graph_key(const char* tag, const MsLoop& L, std::initializer_list<const void*> ptrs) {
@@ -1307,11 +1330,17 @@ std::string ms_graph_key(const char* tag, const MsLoop& L, std::initializer_list
   add(L.st);
   for (unsigned i = 0; i < L.S->nseg; i++) {
     add(L.S->dia[i]);
+    add(L.S->dia[i]->mv.val.p);
+    add(L.S->dia[i]->mv.col.p);
     add(L.M->dia[i]);
-    for (unsigned j = 0; j < L.S->nseg; j++) {
-      add(L.S->nd[i][j]);
-      add(L.M->F[i][j]);
-    }
+    for (unsigned j = 0; j < L.S->nseg; j++)
+      for (const dfm_bmat_s* B : {L.S->nd[i][j], L.M->F[i][j]}) {
+        add(B);
+        if (B) {
+          add(B->mv.val.p);
+          add(B->mv.col.p);
+        }
+      }
   }
   return k;
 }
@@ -1376,6 +1405,7 @@ int ms_dev_pcg(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* conv_rat
   }
   MsLoop L;
   DFM_TRY(ms_loop_init(S, M, rr0, tol, mx_iter, 0, L));
+  DFM_TRY(ms_prepare_matvecs(S, M));
   MsState* st = L.st;
   const long nn = (long)n;
   DFM_TRY(L.copy(r, z));
@@ -1431,6 +1461,7 @@ int ms_dev_pbicgstab(dfm_mseg* S, dfm_msilu* M, double* r, double* x, double* co
   // the loop's counter starts at 1 (it = 1 .. max_iter - 1)
   MsLoop L;
   DFM_TRY(ms_loop_init(S, M, rr0, tol, max_iter, 1, L));
+  DFM_TRY(ms_prepare_matvecs(S, M));
   MsState* st = L.st;
   const long nn = (long)n;
   DFM_TRY(dfm_vec_scal(n, 0.0, x));
