@@ -71,6 +71,9 @@ struct dfm_batch_s {
   dfm::DevBuf<char> d_items, d_teams;
   dfm::DevBuf<int> d_fail;
   int nitem = 0;
+  bool local = false;                    // teams queued per XCD (xq: queue offsets, singles from xq[8])
+  std::vector<int> xq;
+  int resident = 0;
   bool tuned = false;
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
   ~dfm_batch_s() {
@@ -98,8 +101,14 @@ struct Item {
   int sys, member, team, pad;
 };
 
+// Work counters: one per XCD for its teams, one for the single items, each
+// on its own 128-byte line.
+constexpr int kXcd = 8;
+constexpr int kCounterStride = 32;
 struct BatchArgs {
   int nitem;
+  int local;          // teams XCD-local: items[xq[x] .. xq[x+1]) for XCD x, singles from xq[kXcd]
+  int xq[kXcd + 1];
   const Item* items;
   unsigned* next;
   TeamState* teams;
@@ -151,6 +160,7 @@ __device__ inline double wg_sum(double v, double (*sh)[16], int& buf) {
 // in every member.
 struct Team {
   int K, m;
+  int local;    // all members on this XCD: barriers through its L2
   TeamState* ts;
   int* fail;
   unsigned nsync;
@@ -162,6 +172,32 @@ struct Team {
 
 __device__ inline void tsync(Team& t) {
   if (t.K == 1) {
+    __syncthreads();
+    return;
+  }
+  if (t.local) {
+    // Members on one XCD share its L2: once every wave's stores are
+    // acknowledged (vmcnt(0); the L1 writes through) they are in that L2, so
+    // the arrival needs no L2 write-back (measured: without it the team
+    // results are those of the agent-scope barrier); the departure
+    // invalidates with buffer_inv sc1 (sc0 alone left stale L1 lines here:
+    // wrong results).
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    t.nsync++;
+    if (threadIdx.x == 0 && !__hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      __hip_atomic_fetch_add(&t.ts->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned target = t.nsync * (unsigned)t.K;
+      unsigned spins = 0;
+      while (__hip_atomic_load(&t.ts->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kTeamSpinLimit) {
+          __hip_atomic_store(t.fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+      asm volatile("buffer_inv sc1" ::: "memory");
+    }
     __syncthreads();
     return;
   }
@@ -199,7 +235,8 @@ __device__ inline double tsum(Team& t, double v) {
   tsync(t);
   if (threadIdx.x == 0) {
     double a = 0.0;
-    for (int k = 0; k < t.K; k++) a += t.ts->part[t.tbuf][k];
+    for (int k = 0; k < t.K; k++)
+      a += __hip_atomic_load(&t.ts->part[t.tbuf][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *t.bc = a;
   }
   __syncthreads();
@@ -601,8 +638,26 @@ __global__ void __launch_bounds__(1024) k_batch_solve(BatchArgs g) {
   int buf = 0;
   for (;;) {
     if (threadIdx.x == 0) {
-      const unsigned t = atomicAdd(g.next, 1u);
-      s_item = t < (unsigned)g.nitem ? g.items[t] : Item{-1, 0, 1, 0};
+      Item item{-1, 0, 1, 0};
+      if (g.local) {
+        // this XCD's teams first, then the single items of every XCD
+        const int x = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcd - 1);  // HW_REG_XCC_ID
+        const unsigned nt = (unsigned)(g.xq[x + 1] - g.xq[x]);
+        unsigned* cx = g.next + x * kCounterStride;
+        unsigned t = nt;
+        if (__hip_atomic_load(cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nt) t = atomicAdd(cx, 1u);
+        if (t < nt) {
+          item = g.items[g.xq[x] + t];
+        } else {
+          const unsigned ns = (unsigned)(g.nitem - g.xq[kXcd]);
+          t = atomicAdd(g.next + kXcd * kCounterStride, 1u);
+          if (t < ns) item = g.items[g.xq[kXcd] + t];
+        }
+      } else {
+        const unsigned t = atomicAdd(g.next, 1u);
+        if (t < (unsigned)g.nitem) item = g.items[t];
+      }
+      s_item = item;
       s_t0 = wall_clock64();
     }
     __syncthreads();
@@ -611,7 +666,7 @@ __global__ void __launch_bounds__(1024) k_batch_solve(BatchArgs g) {
     __syncthreads();
     if (it.sys < 0) break;
     const int s = it.sys;
-    Team tm{it.team, it.member, g.teams + s, g.fail, 0u, 0, sh, buf, &s_bc};
+    Team tm{it.team, it.member, g.local, g.teams + s, g.fail, 0u, 0, sh, buf, &s_bc};
     const SysDesc d = g.sys[s];
     const int* cp = g.cptr + (size_t)s * (kMaxColourBatch + 1);
     const int nsing = wg_ilu0<W>(tm, d, cp, g.ecol, g.a64, g.f64, g.adia64, g.fdia64);
@@ -673,10 +728,8 @@ __global__ void __launch_bounds__(1024) k_batch_solve(BatchArgs g) {
 // Work counter, failure flag and every system's team counter back to zero.
 __global__ void k_batch_reset(unsigned* next, int* fail, TeamState* teams, int nsys) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) {
-    *next = 0u;
-    *fail = 0;
-  }
+  if (t <= kXcd) next[t * kCounterStride] = 0u;
+  if (t == 0) *fail = 0;
   if (t < nsys) teams[t].arrive = 0u;
 }
 __global__ void k_batch_gather(long long nslot, const long long* __restrict__ src, const double* __restrict__ val,
@@ -708,7 +761,18 @@ static int ncu_count();
 // the workgroups resident at once less a reserve; every other system is one
 // item.  Teams come first, so all their members are taken by resident
 // workgroups at the start of the launch and no member waits for a partner
-// that cannot start.
+// that cannot start.  XCD-local mode (workgroups go to the XCDs round-robin,
+// blockIdx % 8; DFM_BATCH_XCD=0 turns it off): each team is given to one XCD
+// (the one with the least work so far, by rows^1.5, that still has room for
+// it) and queued there; its members are then taken by that XCD's workgroups
+// and meet through its L2.  The single items stay in one queue for all.
+static bool batch_xcd_local() {
+  static const bool on = [] {
+    const char* e = getenv("DFM_BATCH_XCD");
+    return !(e && e[0] == '0');
+  }();
+  return on && ncu_count() % kXcd == 0;
+}
 template <int W, bool MIXED>
 static int build_items(dfm_batch_s* B) {
   int per_cu = 0;
@@ -716,26 +780,46 @@ static int build_items(dfm_batch_s* B) {
                                                    B->block, 0) != hipSuccess || per_cu < 1)
     per_cu = 1;
   const int resident = per_cu * ncu_count();
+  const bool local = batch_xcd_local();
   char key[96];
-  snprintf(key, sizeof key, "%d:%d:%d:%u:%d", W, (int)MIXED, B->block, B->team_min_rows, B->team_max);
+  snprintf(key, sizeof key, "%d:%d:%d:%u:%d:%d", W, (int)MIXED, B->block, B->team_min_rows, B->team_max, (int)local);
   if (B->items_key == key) return DFM_OK;
-  int budget = resident - std::max(16, resident / 4);
-  std::vector<Item> teams, singles;
+  std::vector<Item> singles;
+  std::vector<std::vector<Item>> xq(local ? kXcd : 1);
+  std::vector<int> budget(xq.size());
+  std::vector<double> cost(xq.size(), 0.0);
+  const int per_q = resident / (int)xq.size();
+  for (auto& bgt : budget) bgt = per_q - std::max(local ? 2 : 16, per_q / 4);
   for (int s : B->sched) {
     const int n = B->desc[s].n;
     int K = 1;
     if (B->team_max > 1 && (unsigned)n >= B->team_min_rows)
       K = std::min({B->team_max, kMaxTeam, std::max(2, n / (int)std::max(1u, B->team_min_rows / 4))});
-    if (K > 1 && K <= budget) {
-      budget -= K;
-      for (int m = 0; m < K; m++) teams.push_back(Item{s, m, K, 0});
+    int x = -1;
+    if (K > 1)
+      for (int c = 0; c < (int)xq.size(); c++)
+        if (budget[c] >= K && (x < 0 || cost[c] < cost[x])) x = c;
+    if (x >= 0) {
+      budget[x] -= K;
+      cost[x] += (double)n * std::sqrt((double)n);
+      for (int m = 0; m < K; m++) xq[x].push_back(Item{s, m, K, 0});
     } else {
       singles.push_back(Item{s, 0, 1, 0});
     }
   }
-  teams.insert(teams.end(), singles.begin(), singles.end());
-  B->nitem = (int)teams.size();
-  DFM_TRY(B->d_items.upload(reinterpret_cast<const char*>(teams.data()), teams.size() * sizeof(Item)));
+  std::vector<Item> items;
+  B->xq.assign(kXcd + 1, 0);
+  for (size_t c = 0; c < xq.size(); c++) {
+    items.insert(items.end(), xq[c].begin(), xq[c].end());
+    if (local) B->xq[c + 1] = (int)items.size();
+  }
+  if (!local)
+    for (int c = 1; c <= kXcd; c++) B->xq[c] = (int)items.size();
+  items.insert(items.end(), singles.begin(), singles.end());
+  B->nitem = (int)items.size();
+  B->local = local;
+  B->resident = resident;
+  DFM_TRY(B->d_items.upload(reinterpret_cast<const char*>(items.data()), items.size() * sizeof(Item)));
   DFM_HIP(hipStreamSynchronize(stream()));
   B->items_key = key;
   return DFM_OK;
@@ -743,6 +827,8 @@ static int build_items(dfm_batch_s* B) {
 
 static int batch_args(dfm_batch_s* B, bool mixed, double tol, unsigned maxit, BatchArgs& g) {
   g.nitem = B->nitem;
+  g.local = B->local ? 1 : 0;
+  for (int c = 0; c <= kXcd; c++) g.xq[c] = B->xq[c];
   g.items = reinterpret_cast<const Item*>(B->d_items.p);
   g.teams = reinterpret_cast<TeamState*>(B->d_teams.p);
   g.fail = B->d_fail.p;
@@ -791,7 +877,8 @@ static int batch_launch(dfm_batch_s* B, bool mixed, double tol, unsigned maxit) 
   else DFM_TRY(mixed ? (build_items<16, true>(B)) : (build_items<16, false>(B)));
   BatchArgs g;
   DFM_TRY(batch_args(B, mixed, tol, maxit, g));
-  const int grid = std::min<int>(B->nitem, ncu_count() * B->wpc);
+  // XCD-local teams need every XCD's workgroups, so the grid is not cut to the item count
+  const int grid = B->local ? ncu_count() * B->wpc : std::min<int>(B->nitem, ncu_count() * B->wpc);
   char key[200];
   snprintf(key, sizeof key, "%d:%d:%d:%.17g:%u:%p:%s", (int)mixed, B->block, B->wpc, tol, maxit, g.vec,
            B->items_key.c_str());
@@ -930,7 +1017,7 @@ int dfm_batch_create(const dfm_bcrs* A, unsigned nsys, const unsigned* sys_ptr, 
   DFM_TRY(B->d_perm.upload(B->perm));
   DFM_TRY(B->d_ell_col.upload(ecol));
   DFM_TRY(B->d_ell_src.upload(esrc));
-  DFM_TRY(B->d_next.alloc(1));
+  DFM_TRY(B->d_next.alloc((kXcd + 1) * kCounterStride));
   DFM_TRY(B->d_a64.alloc(nslot));
   DFM_TRY(B->d_f64.alloc(nslot));
   DFM_TRY(B->d_a32.alloc(nslot));
