@@ -81,8 +81,11 @@ int dfm_bcrs_device_values(dfm_bcrs *A, double **val_dev, double **dia_dev); /* 
 int dfm_bcrs_mearge(dfm_bcrs *A, unsigned nblkel, const unsigned *blkel,
                     unsigned blksize, const double *emat);                  /* Mearge (one element, host emat) matdia_blkcrs.cpp:165-258 */
 int dfm_bcrs_set_bc(dfm_bcrs *A, const int *flag);                          /* SetBoundaryCondition(CBCFlag) matdia_blkcrs.cpp:261-343 */
+/* MatVec matdia_blkcrs.cpp:921-1094, bitwise the reference's CRS order.  For b <= 3 the first call
+ * keeps a sliced-ELL copy of the values (about the matrix's size again, refreshed when they change;
+ * values written through dfm_bcrs_device_values pointers after a MatVec need that call again). */
 int dfm_bcrs_matvec(dfm_bcrs *A, double alpha, const double *x_dev, double beta,
-                    double *y_dev);                                         /* MatVec matdia_blkcrs.cpp:921-1094 */
+                    double *y_dev);
 
 /* ---- MatVec::CVector_Blk / LsSol::CLinearSystem BLAS-1 (linearsystem.cpp:250-415) */
 int dfm_vec_dot(size_t n, const double *x_dev, const double *y_dev, double *result); /* DOT / operator* vector_blk.cpp:37-47 */
@@ -315,8 +318,10 @@ int dfm_bmat_mearge(dfm_bmat *M, unsigned nblkel, const unsigned *blkel,
 int dfm_bmat_copy_from(dfm_bmat *M, const dfm_bcrs *A);
 int dfm_bmat_set_bc_inverse_col(dfm_bmat *M, const int *flag);             /* SetBoundaryConditionInverse_Colum mat_blkcrs.cpp:876-907 */
 int dfm_bmat_set_bc_row(dfm_bmat *M, const int *flag);                     /* SetBoundaryCondition_Row mat_blkcrs.cpp:910-952 */
+/* MatVec(..., isnt_trans=true) mat_blkcrs.cpp:955-1176, bitwise; blocks up to 3 x 3 through a
+ * sliced-ELL copy kept from the first call, as dfm_bcrs_matvec */
 int dfm_bmat_matvec(dfm_bmat *M, double alpha, const double *x_dev, double beta,
-                    double *y_dev);                                        /* MatVec(..., isnt_trans=true) mat_blkcrs.cpp:955-1176 */
+                    double *y_dev);
 /* General block CRS (off-diagonal blocks of multi-segment systems): rows of nblk_col blocks
  * of len_col dofs, columns of nblk_row blocks of len_row dofs (dfm_bmat_create takes any).
  * AddPattern(CIndexedArray) of a new matrix (mat_blkcrs.cpp:286-370): rows of column blocks. */
