@@ -810,7 +810,7 @@ class BatchSystem:
         return val[:nnz], dia, order
 
     # team axis: (rows from which a system gets a team, most workgroups per team)
-    TEAMS = ((16384, 4), (16384, 8), (8192, 4))
+    TEAMS = ((16384, 4), (16384, 8), (8192, 4), (8192, 3), (12288, 3))
 
     def autotune(self, rhs, x, tol, maxit, precision="mixed_f32", reps=2, shapes=None, teams=None):
         """Time each launch shape (threads per workgroup, workgroups per CU)
