@@ -197,7 +197,9 @@ int dfm_batch_set_launch(dfm_batch *B, int block, int wg_per_cu);
 /* Systems of at least min_rows rows are solved by a team of up to max_team
  * workgroups (row ranges split, team barriers between phases), as many as fit
  * in the workgroups resident at once; max_team 1 keeps one workgroup per
- * system.  Default: min_rows 16384, max_team 4. */
+ * system.  Each team is queued on one XCD and its barriers go through that
+ * XCD's L2 (DFM_BATCH_XCD=0: one queue, agent-scope barriers).  Default:
+ * min_rows 16384, max_team 4. */
 int dfm_batch_set_teams(dfm_batch *B, unsigned min_rows, int max_team);
 /* One step for every system: ILU(0) numeric, then PCG from x = 0 on rhs
  * (concatenated, A's numbering) to ||r|| < tol ||rhs||.  precision:
