@@ -779,10 +779,12 @@ static int build_items(dfm_batch_s* B) {
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_solve<W, MIXED>),
                                                    B->block, 0) != hipSuccess || per_cu < 1)
     per_cu = 1;
-  const int resident = per_cu * ncu_count();
+  // workgroups running at once: the launch has wg_per_cu per CU, occupancy permitting
+  const int resident = std::min(per_cu, std::max(1, B->wpc)) * ncu_count();
   const bool local = batch_xcd_local();
   char key[96];
-  snprintf(key, sizeof key, "%d:%d:%d:%u:%d:%d", W, (int)MIXED, B->block, B->team_min_rows, B->team_max, (int)local);
+  snprintf(key, sizeof key, "%d:%d:%d:%d:%u:%d:%d", W, (int)MIXED, B->block, B->wpc, B->team_min_rows, B->team_max,
+           (int)local);
   if (B->items_key == key) return DFM_OK;
   std::vector<Item> singles;
   std::vector<std::vector<Item>> xq(local ? kXcd : 1);
