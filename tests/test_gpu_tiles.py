@@ -124,29 +124,38 @@ FOUND = {
 }
 
 
+@pytest.mark.parametrize("numbering", ["mesh", "given"])
 @pytest.mark.parametrize("name", sorted(FOUND))
-def test_found_phases(orc, gpu, name):
+def test_found_phases(orc, gpu, name, numbering):
     """The found phases' tile sweeps equal the oracle's substitutions bit for
     bit; BiCGSTAB gives the same bits with tiles and with one launch per
-    level; PCG within one iteration and 1e-8 of the oracle."""
+    level; PCG within one iteration and 1e-8 of the oracle.  numbering
+    "mesh": the mesh is relabelled and the factor is in natural order;
+    "given": the mesh keeps its numbering and the tile numbering is the
+    factor's given order (phases found in factor numbering)."""
     import torch
     from oracle.oracle import assemble_problem
-    _case(f"found phases {name}")
+    _case(f"found phases {name} {numbering}")
     make, tiles, lev = FOUND[name]
     prob = make()
-    if tiles:
+    order = None
+    if tiles and numbering == "mesh":
         prob = meshgen.relabel(prob, meshgen.tile_colour_order(prob, 16))
+    elif tiles:
+        order = np.ascontiguousarray(meshgen.tile_colour_order(prob, 16), dtype=np.int32)
+    elif numbering == "given":
+        pytest.skip("a lexicographic mesh has no second numbering here")
     ref = assemble_problem(orc, prob)
     s = gpu.FEMSystem(prob)
     s.assemble()
-    M = gpu.CPreconditioner_ILU(s.A, lev)
+    M = gpu.CPreconditioner_ILU(s.A, lev, order=order)
     ti, info = M.tile_info(), M.info()
     print(name, ti, info, flush=True)
     assert ti["nphase"] >= 1 and ti["ntile_lower"] > 0 and ti["ntile_upper"] > 0
     assert ti["nphase"] <= max(info["nlevel_lower"], info["nlevel_upper"]) // 4
     assert M.SetValue(s.A)
-    F = orc.iluk_symbolic(ref["rowptr"], ref["col"], lev, None)
-    ov, od, ns = orc.ilu_values(prob.nnode, prob.b, ref["rowptr"], ref["col"], ref["val"], ref["dia"], F, None)
+    F = orc.iluk_symbolic(ref["rowptr"], ref["col"], lev, order)
+    ov, od, ns = orc.ilu_values(prob.nnode, prob.b, ref["rowptr"], ref["col"], ref["val"], ref["dia"], F, order)
     fval, fdia = M.values()
     assert ns == 0 and np.array_equal(fval, ov) and np.array_equal(fdia, od)
     v = np.random.default_rng(5).standard_normal(prob.ndof)
@@ -166,7 +175,7 @@ def test_found_phases(orc, gpu, name):
     assert got[:3] == base[:3] and np.array_equal(got[3], base[3]) and np.array_equal(got[4], base[4])
     if prob.kind != "oseen2d":
         ok_o, conv_o, it_o, x_o, _ = orc.solve("pcg", prob.nnode, prob.b, ref["rowptr"], ref["col"], ref["val"],
-                                               ref["dia"], F, ov, od, ref["res"], 1e-10, 5000, None)
+                                               ref["dia"], F, ov, od, ref["res"], 1e-10, 5000, order)
         ok, conv, it, x, _ = _solve(gpu, s, M, "pcg", "fp64")
         print(f"  pcg {it} iterations, oracle {it_o}", flush=True)
         assert ok and ok_o and abs(it - it_o) <= 1
