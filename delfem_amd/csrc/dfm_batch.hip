@@ -315,8 +315,11 @@ __device__ inline double ell_spmv(const SysDesc& d, int lo, int hi, const int* _
 // One substitution step over rows [lo, hi) of one colour.  Forward
 // (LOWER): z_i = Dinv_i (r_i - sum_{j<i} L_ij z_j); backward: z_i -= sum_{j>i}
 // U_ij z_j, returning the <r, z> partial of the finished rows.  Each sum in
-// ascending column order, as the reference's substitutions.
-template <int W, bool LOWER, typename V, typename T>
+// ascending column order, as the reference's substitutions.  With DOT a
+// forward step also returns the <r, z> partial of its rows: the backward step
+// of the last colour, whose rows have no later neighbours (the colouring is
+// proper), so the backward step changes nothing there.
+template <int W, bool LOWER, typename V, typename T, bool DOT = false>
 __device__ inline double ell_subst(const SysDesc& d, int lo, int hi, const int* __restrict__ ecol,
                                    const V* __restrict__ f, const V* __restrict__ fdia, const T* r, T* z) {
   constexpr int U = rows_per_pass<W, T>();
@@ -357,7 +360,9 @@ __device__ inline double ell_subst(const SysDesc& d, int lo, int hi, const int* 
         if (use) tt -= v[u][k];
       }
       if (LOWER) {
-        z[i] = di[u] * tt;
+        const T zi = di[u] * tt;
+        z[i] = zi;
+        if (DOT) part += (double)t[u] * (double)zi;
       } else {
         z[i] = tt;
         part += (double)r[i] * (double)tt;
@@ -368,22 +373,30 @@ __device__ inline double ell_subst(const SysDesc& d, int lo, int hi, const int* 
 }
 
 // z = (LU)^-1 r colour by colour (forward), then in reverse (backward);
-// returns <r, z> over the team
+// returns <r, z> over the team.  The last colour's forward step also takes
+// its <r, z> partial (its backward step, which subtracts nothing) with no
+// barrier between, as the same threads hold the same rows.  With c0 = 1 the
+// forward step of colour 0 (z_i = Dinv_i r_i: its rows have no earlier
+// neighbours) was made by xr_update; callers pass it only with two or more
+// colours.  The partial sums are added in the order of the colour-by-colour
+// backward sweep, so the results are bitwise those of separate steps.
 template <int W, typename V, typename T>
 __device__ inline double ell_precond(Team& tm, const SysDesc& d, const int* __restrict__ cp,
                                      const int* __restrict__ ecol, const V* __restrict__ f,
-                                     const V* __restrict__ fdia, const T* r, T* z) {
+                                     const V* __restrict__ fdia, const T* r, T* z, int c0 = 0) {
   int a, b;
-  for (int c = 0; c < d.ncol; c++) {
+  const int last = d.ncol - 1;
+  for (int c = c0; c < last; c++) {
     tshare(tm, cp[c], cp[c + 1], a, b);
     ell_subst<W, true, V, T>(d, a, b, ecol, f, fdia, r, z);
     tsync(tm);
   }
-  double part = 0.0;
-  for (int c = d.ncol - 1; c >= 0; c--) {
+  tshare(tm, cp[last], cp[last + 1], a, b);
+  double part = ell_subst<W, true, V, T, true>(d, a, b, ecol, f, fdia, r, z);
+  for (int c = last - 1; c >= 0; c--) {
+    tsync(tm);
     tshare(tm, cp[c], cp[c + 1], a, b);
     part += ell_subst<W, false, V, T>(d, a, b, ecol, f, fdia, r, z);
-    if (c > 0) tsync(tm);
   }
   return tsum(tm, part);
 }
@@ -410,14 +423,17 @@ __device__ inline void p_update(int a, int b, T beta, const T* z, T* p) {
   }
 }
 
-// x += alpha p; r -= alpha q over [a, b); returns the <r, r> partial
-template <int W, typename T>
-__device__ inline double xr_update(int a, int b, T alpha, const T* p, const T* q, T* x, T* r) {
+// x += alpha p; r -= alpha q over [a, b); returns the <r, r> partial.  With
+// fdia0 (the system's inverted pivots) the rows below n0 (colour 0) also get
+// z_i = Dinv_i r_i, the forward substitution step of colour 0, from the new r.
+template <int W, typename T, typename V = T>
+__device__ inline double xr_update(int a, int b, T alpha, const T* p, const T* q, T* x, T* r, int n0 = 0,
+                                   const V* __restrict__ fdia0 = nullptr, T* z = nullptr) {
   constexpr int U = rows_per_pass<W, T>();
   const T malpha = -alpha;
   double rr = 0.0;
   for (int i0 = a + (int)threadIdx.x; i0 < b; i0 += U * (int)blockDim.x) {
-    T xv[U], pv[U], rv[U], qv[U];
+    T xv[U], pv[U], rv[U], qv[U], dv[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int i = i0 + u * (int)blockDim.x;
@@ -426,6 +442,7 @@ __device__ inline double xr_update(int a, int b, T alpha, const T* p, const T* q
         pv[u] = p[i];
         rv[u] = r[i];
         qv[u] = q[i];
+        if (fdia0 && i < n0) dv[u] = (T)fdia0[i];
       }
     }
 #pragma unroll
@@ -436,6 +453,7 @@ __device__ inline double xr_update(int a, int b, T alpha, const T* p, const T* q
       const T rn = rv[u] + malpha * qv[u];
       r[i] = rn;
       rr += (double)rn * (double)rn;
+      if (fdia0 && i < n0) z[i] = dv[u] * rn;
     }
   }
   return rr;
@@ -459,6 +477,8 @@ __device__ inline int wg_pcg64(Team& tm, const SysDesc& d, const int* cp, const 
   rel2 = 0.0;
   if (rr0 < 1.0e-30) return 0;
   const double inv_rr0 = 1.0 / rr0;
+  // colour 0's forward step taken with the r update (two or more colours)
+  const bool z0 = d.ncol > 1;
   double rz = ell_precond<W, double, double>(tm, d, cp, ecol, f, fdia, r, z);
   for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) p[i] = z[i];
   tsync(tm);
@@ -466,10 +486,10 @@ __device__ inline int wg_pcg64(Team& tm, const SysDesc& d, const int* cp, const 
   for (int it = 0; it < maxit; it++) {
     const double pap = tsum(tm, ell_spmv<W, double, double>(d, ra, rb, ecol, a, adia, p, q));
     const double alpha = rz / pap;
-    const double rr = tsum(tm, xr_update<W, double>(ra, rb, alpha, p, q, x, r));
+    const double rr = tsum(tm, xr_update<W, double>(ra, rb, alpha, p, q, x, r, cp[1], z0 ? fdia + d.row0 : nullptr, z));
     rel2 = rr * inv_rr0;
     if (rel2 < tol2) return it;
-    const double rz_new = ell_precond<W, double, double>(tm, d, cp, ecol, f, fdia, r, z);
+    const double rz_new = ell_precond<W, double, double>(tm, d, cp, ecol, f, fdia, r, z, z0 ? 1 : 0);
     const double beta = rz_new / rz;
     rz = rz_new;
     p_update<W, double>(ra, rb, beta, z, p);
@@ -530,6 +550,7 @@ __device__ inline void wg_pcg_mixed(Team& tm, const SysDesc& d, const int* cp, c
     return;
   }
   const double delta2 = 0.01;
+  const bool z0 = d.ncol > 1;  // colour 0's forward step taken with the r update
   double rz = ell_precond<W, float, float>(tm, d, cp, ecol, g.f32, g.fdia32, r, z);
   for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) p[i] = z[i];
   tsync(tm);
@@ -539,8 +560,10 @@ __device__ inline void wg_pcg_mixed(Team& tm, const SysDesc& d, const int* cp, c
   for (; it < g.maxit && !done; it++) {
     const double pap = tsum(tm, ell_spmv<W, float, float>(d, ra, rb, ecol, g.a32, g.adia32, p, q));
     const float alpha = (float)(rz / pap);
-    const double rr = tsum(tm, xr_update<W, float>(ra, rb, alpha, p, q, xr, r));
+    const double rr =
+        tsum(tm, xr_update<W, float>(ra, rb, alpha, p, q, xr, r, cp[1], z0 ? g.fdia32 + d.row0 : nullptr, z));
     bool restart = false;
+    int c0 = z0 ? 1 : 0;
     if (rr < delta2 * rr_ref || rr < g.tol2 * bb) {
       for (int i = ra + (int)threadIdx.x; i < rb; i += blockDim.x) {
         x64[i] += (double)xr[i];
@@ -557,8 +580,9 @@ __device__ inline void wg_pcg_mixed(Team& tm, const SysDesc& d, const int* cp, c
       tsync(tm);
       restart = rr_true > 100.0 * rr;
       rr_ref = rr_true;
+      c0 = 0;  // r replaced: colour 0's forward step again
     }
-    const double rz_new = ell_precond<W, float, float>(tm, d, cp, ecol, g.f32, g.fdia32, r, z);
+    const double rz_new = ell_precond<W, float, float>(tm, d, cp, ecol, g.f32, g.fdia32, r, z, c0);
     const float beta = restart ? 0.0f : (float)(rz_new / rz);
     rz = rz_new;
     p_update<W, float>(ra, rb, beta, z, p);
@@ -970,6 +994,13 @@ int dfm_batch_create(const dfm_bcrs* A, unsigned nsys, const unsigned* sys_ptr, 
     const int nc = greedy_colour(n, lrow, lcol, colour);
     DFM_CHECK(nc <= kMaxColourBatch, DFM_NOT_IMPLEMENTED, "system %u needs %d colours (at most %d)", s, nc,
               kMaxColourBatch);
+    // the colour-by-colour sweeps need no two coupled rows in one colour (a
+    // structurally unsymmetric pattern could leave such a pair)
+    for (unsigned i = 0; i < n; i++)
+      for (unsigned q = lrow[i]; q < lrow[i + 1]; q++)
+        DFM_CHECK(lcol[q] == i || colour[lcol[q]] != colour[i], DFM_NOT_IMPLEMENTED,
+                  "system %u: rows %u and %u are coupled one way only (structurally unsymmetric pattern)", s, i,
+                  lcol[q]);
     colour_order(n, colour, nc, orders[s]);
     int* cp = B->cptr.data() + (size_t)s * (kMaxColourBatch + 1);
     for (unsigned i = 0; i < n; i++) cp[colour[i] + 1]++;
