@@ -267,6 +267,8 @@ __device__ inline void ell_cols(const int* __restrict__ ecol, const SysDesc& d, 
 // loads issued together before any arithmetic.
 template <int W, typename T = double>
 constexpr int rows_per_pass() { return W <= 8 ? 2 : 1; }
+// rows per thread per pass of the vector updates (no gathers)
+constexpr int kStreamRows = 4;
 
 // y = A x over rows [lo, hi) (ELL + diagonal); returns the <x, y> partial
 template <int W, typename V, typename T>
@@ -404,7 +406,7 @@ __device__ inline double ell_precond(Team& tm, const SysDesc& d, const int* __re
 // p = beta p + z over [a, b)
 template <int W, typename T>
 __device__ inline void p_update(int a, int b, T beta, const T* z, T* p) {
-  constexpr int U = rows_per_pass<W, T>();
+  constexpr int U = kStreamRows;
   for (int i0 = a + (int)threadIdx.x; i0 < b; i0 += U * (int)blockDim.x) {
     T pv[U], zv[U];
 #pragma unroll
@@ -429,7 +431,7 @@ __device__ inline void p_update(int a, int b, T beta, const T* z, T* p) {
 template <int W, typename T, typename V = T>
 __device__ inline double xr_update(int a, int b, T alpha, const T* p, const T* q, T* x, T* r, int n0 = 0,
                                    const V* __restrict__ fdia0 = nullptr, T* z = nullptr) {
-  constexpr int U = rows_per_pass<W, T>();
+  constexpr int U = kStreamRows;
   const T malpha = -alpha;
   double rr = 0.0;
   for (int i0 = a + (int)threadIdx.x; i0 < b; i0 += U * (int)blockDim.x) {
