@@ -158,6 +158,12 @@ def _mean_counter(path, kernel_substr, counter):
     return sum(vals.values()) / len(vals), len(vals)
 
 
+def _out_file(path):
+    """The summary file, or standard output when no --out is given."""
+    import contextlib
+    return open(path, "w") if path else contextlib.nullcontext(sys.stdout)
+
+
 def summarize_pmc(args):
     """HBM bytes per launch from a FETCH_SIZE pass and a WRITE_SIZE pass."""
     fetch_csv, write_csv = args.summarize_pmc
@@ -171,7 +177,7 @@ def summarize_pmc(args):
                fetch_bytes=round(fk * f_fetch), write_bytes=round(wk * f_write),
                fetch_counter=fk, write_counter=wk, copy_fetch_counter=fc, copy_write_counter=wc,
                fetch_scale=f_fetch, write_scale=f_write, dispatches=[nf, nw], copy_bytes=nbytes)
-    with open(args.out, "w") as f:
+    with _out_file(args.out) as f:
         f.write(f"# HBM traffic per launch of {args.kernel}\n")
         f.write(f"# {args.note}\n" if args.note else "")
         f.write("# FETCH_SIZE and WRITE_SIZE (rocprofv3 --pmc, one pass each) averaged over the kernel's\n")
@@ -201,7 +207,7 @@ def summarize_trace(args):
     for name in per:
         if args.kernel and name.endswith(args.kernel):
             key = name
-    with open(args.out, "w") as f:
+    with _out_file(args.out) as f:
         f.write(f"# {args.note}\n" if args.note else "")
         f.write("# columns: share of GPU time, calls, average duration (us), min / max duration (us), kernel\n")
         for ln in lines[:40]:
@@ -221,7 +227,8 @@ def summarize_trace(args):
                 f.write(f"# the last {len(t)} launches (bench.py's in-solve timing solve, launched directly with "
                         f"start / stop events): average {sum(t) / len(t):.2f} us; the {len(tw)} that did work "
                         f"average {sum(tw) / len(tw):.2f} us\n")
-    print(open(args.out).read())
+    if args.out:
+        print(open(args.out).read())
 
 
 def main():
