@@ -185,7 +185,10 @@ int dfm_ilu_solve_precond_noalloc(dfm_ilu *M, double *vec_dev);
  * A is one assembled 1x1-block matrix whose diagonal blocks, rows
  * [sys_ptr[s], sys_ptr[s+1]) (host array, nsys+1 entries), are the systems;
  * any coupling between systems is rejected.  Each system is ordered by a
- * greedy colouring of its graph. */
+ * greedy colouring of its graph; DFM_NOT_IMPLEMENTED for a system needing
+ * more colours than the kernel holds, a row of more than 16 off-diagonal
+ * entries, or a one-way coupling (a structurally unsymmetric pattern that
+ * leaves two coupled rows in one colour). */
 int dfm_batch_create(const dfm_bcrs *A, unsigned nsys, const unsigned *sys_ptr, dfm_batch **out);
 int dfm_batch_destroy(dfm_batch *B);
 int dfm_batch_info(const dfm_batch *B, unsigned *max_width, int *max_colours, int *block, int *wg_per_cu);

@@ -122,6 +122,23 @@ def test_batch_rejects_coupled_systems(gpu):
         gpu.BatchSystem(fem, np.array([0, 7, p.nnode], dtype=np.uint32))
 
 
+def test_batch_rejects_one_way_coupling(gpu):
+    # rows 0-1-2 a path, plus the entry (0, 2) without (2, 0): the greedy
+    # colouring leaves rows 0 and 2 in one colour, which the colour steps
+    # cannot sweep
+    A = gpu.CMatDia_BlkCrs(3, 1)
+    A.AddPattern(np.array([0, 2, 4, 5]), np.array([1, 2, 0, 2, 1]))
+    A.set_values(np.full(5, -1.0), np.full(3, 4.0))
+
+    class Owner:
+        pass
+
+    o = Owner()
+    o.A = A
+    with pytest.raises(gpu._lib.DfmError, match="one way"):
+        gpu.BatchSystem(o, np.array([0, 3], dtype=np.uint32))
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed_f32"])
 def test_batch_teams(batch, orc, prec):
     """Systems of at least 100 rows solved by teams of 2-4 workgroups (row
