@@ -170,11 +170,13 @@ struct Team {
   double* bc;   // LDS broadcast slot
 };
 
-__device__ inline void tsync(Team& t) {
-  if (t.K == 1) {
-    __syncthreads();
-    return;
-  }
+// Thread 0's part of a team barrier, after every wave has waited for its
+// own stores and a workgroup barrier: arrive on the system's counter, wait
+// until all K members arrived (relaxed polls, bounded: a timeout sets *fail
+// and the launch completes with the result marked invalid), and make the
+// members' stores visible to this workgroup.
+__device__ inline void team_meet(Team& t) {
+  if (__hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   if (t.local) {
     // Members on one XCD share its L2: once every wave's stores are
     // acknowledged (vmcnt(0); the L1 writes through) they are in that L2, so
@@ -183,57 +185,61 @@ __device__ inline void tsync(Team& t) {
     // invalidates with buffer_inv sc1 (sc0 alone left stale L1 lines here:
     // wrong results).
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    t.nsync++;
-    if (threadIdx.x == 0 && !__hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-      __hip_atomic_fetch_add(&t.ts->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned target = t.nsync * (unsigned)t.K;
-      unsigned spins = 0;
-      while (__hip_atomic_load(&t.ts->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > kTeamSpinLimit) {
-          __hip_atomic_store(t.fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-      }
-      asm volatile("buffer_inv sc1" ::: "memory");
+  } else {
+    // one agent-scope release -- an L2 write-back -- covers the stores of
+    // all waves, which have been acknowledged by L2
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  }
+  __hip_atomic_fetch_add(&t.ts->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned target = t.nsync * (unsigned)t.K;
+  unsigned spins = 0;
+  while (__hip_atomic_load(&t.ts->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > kTeamSpinLimit) {
+      __hip_atomic_store(t.fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
     }
+  }
+  // the acquire invalidates the CU's L1 (shared by all waves of the
+  // workgroup) and, across XCDs, the XCD's L2
+  if (t.local) asm volatile("buffer_inv sc1" ::: "memory");
+  else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Team barrier.  Every wave waits until L2 has acknowledged its own stores
+// (a workgroup-scope barrier does not wait for them on this device), then a
+// workgroup barrier, thread 0 meets the other members, and a second
+// workgroup barrier holds the other waves until it has.
+__device__ inline void tsync(Team& t) {
+  if (t.K == 1) {
     __syncthreads();
     return;
   }
-  // Every wave waits until L2 has acknowledged its own stores (a
-  // workgroup-scope barrier does not wait for them on this device), so that
-  // after the workgroup barrier one agent-scope release -- an L2 write-back --
-  // by wave 0 covers the stores of all waves.  Wave 0 then arrives, polls
-  // and makes the agent-scope acquire, which invalidates the CU's L1 (shared
-  // by all waves of the workgroup) and the XCD's L2 before the other waves
-  // pass the second workgroup barrier.
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   t.nsync++;
-  if (threadIdx.x == 0 && !__hip_atomic_load(t.fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(&t.ts->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned target = t.nsync * (unsigned)t.K;
-    unsigned spins = 0;
-    while (__hip_atomic_load(&t.ts->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > kTeamSpinLimit) {
-        __hip_atomic_store(t.fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
+  if (threadIdx.x == 0) team_meet(t);
   __syncthreads();
 }
 
+// Team reduction: the waves' sums to LDS in the first barrier of the team
+// barrier, thread 0 adds them in wave order, publishes the workgroup's sum in
+// its member slot, meets the other members and adds the K slots in member
+// order (identical in every member); the second barrier broadcasts it.
 __device__ inline double tsum(Team& t, double v) {
-  const double s = wg_sum(v, t.sh, t.buf);
-  if (t.K == 1) return s;
-  if (threadIdx.x == 0) t.ts->part[t.tbuf][t.m] = s;
-  tsync(t);
+  if (t.K == 1) return wg_sum(v, t.sh, t.buf);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) t.sh[t.buf][threadIdx.x >> 6] = v;
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  t.nsync++;
   if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < nw; i++) s += t.sh[t.buf][i];
+    t.ts->part[t.tbuf][t.m] = s;
+    team_meet(t);
     double a = 0.0;
     for (int k = 0; k < t.K; k++)
       a += __hip_atomic_load(&t.ts->part[t.tbuf][k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -241,6 +247,7 @@ __device__ inline double tsum(Team& t, double v) {
   }
   __syncthreads();
   const double r = *t.bc;
+  t.buf ^= 1;
   t.tbuf ^= 1;
   return r;
 }
